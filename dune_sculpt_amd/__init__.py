@@ -1,0 +1,448 @@
+"""MI355X sculpt brush engine -- Python host mirror of the C sculpt/PBVH interface.
+
+The product is libsculpt_gpu.so (HIP kernels for gfx950 behind the C ABI declared in
+include/sculpt_gpu.h).  This module only binds that library with ctypes: it mirrors
+the reference's PBVH operator interface (BKE_pbvh_build_mesh, BKE_pbvh_search_gather,
+BKE_pbvh_update_bounds, BKE_pbvh_update_normals ... kernel/intern/pbvh.c) and the brush
+entry points that replace the absent do_brush_action.  There is no CPU fallback: if the
+library is missing, importing the engine raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libsculpt_gpu.so"
+
+TOOL_DRAW, TOOL_SMOOTH, TOOL_INFLATE, TOOL_GRAB = 1, 2, 4, 5
+CURVE_CUSTOM, CURVE_SMOOTH, CURVE_SPHERE, CURVE_ROOT, CURVE_SHARP = 0, 1, 2, 3, 4
+CURVE_LIN, CURVE_POW4, CURVE_INVSQUARE, CURVE_CONSTANT, CURVE_SMOOTHER = 5, 6, 7, 8, 9
+BRUSH_DIR_IN = 1 << 9
+CM_TABLE = 256
+STROKE_GRAPH, STROKE_REFIT_EACH, STROKE_LOG = 1, 2, 4
+
+PBVH_Leaf = 1 << 0
+PBVH_UpdateNormals = 1 << 1
+PBVH_UpdateBB = 1 << 2
+PBVH_UpdateOriginalBB = 1 << 3
+PBVH_UpdateRedraw = 1 << 5
+PBVH_FullyHidden = 1 << 9
+ME_HIDE = 1 << 4
+
+TOOL_NAMES = {TOOL_DRAW: "draw", TOOL_SMOOTH: "smooth", TOOL_INFLATE: "inflate", TOOL_GRAB: "grab"}
+
+
+class MVert(C.Structure):  # types_meshdata.h:13-17
+    _fields_ = [("co", C.c_float * 3), ("flag", C.c_char), ("bweight", C.c_char), ("_pad", C.c_char * 2)]
+
+
+class MPoly(C.Structure):  # types_meshdata.h:50-57
+    _fields_ = [("loopstart", C.c_int), ("totloop", C.c_int), ("mat_nr", C.c_short), ("flag", C.c_char),
+                ("_pad", C.c_char)]
+
+
+class MLoop(C.Structure):  # types_meshdata.h:69-74
+    _fields_ = [("v", C.c_uint), ("e", C.c_uint)]
+
+
+class MLoopTri(C.Structure):  # types_meshdata.h:178-181
+    _fields_ = [("tri", C.c_uint * 3), ("poly", C.c_uint)]
+
+
+class SgDabParams(C.Structure):
+    _fields_ = [
+        ("sculpt_tool", C.c_int),
+        ("curve_preset", C.c_int),
+        ("flag", C.c_int),
+        ("smooth_iterations", C.c_int),
+        ("location", C.c_float * 3),
+        ("radius", C.c_float),
+        ("strength", C.c_float),
+        ("hardness", C.c_float),
+        ("normal_radius_factor", C.c_float),
+        ("grab_delta", C.c_float * 3),
+        ("curve_table", C.POINTER(C.c_float)),
+    ]
+
+
+SearchCB = C.CFUNCTYPE(C.c_bool, C.c_void_p, C.c_void_p)
+HitCB = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p)
+
+_lib = None
+
+# every symbol include/sculpt_gpu.h declares
+EXPORTS = [
+    "BKE_pbvh_new", "BKE_pbvh_free", "BKE_pbvh_build_mesh", "BKE_pbvh_search_gather",
+    "BKE_pbvh_search_callback", "BKE_pbvh_update_bounds", "BKE_pbvh_update_normals",
+    "BKE_pbvh_vert_coords_apply", "BKE_pbvh_vert_coords_alloc", "BKE_pbvh_node_mark_update",
+    "BKE_pbvh_node_mark_normals_update", "BKE_pbvh_node_mark_redraw", "BKE_pbvh_vert_mark_update",
+    "BKE_pbvh_node_get_BB", "BKE_pbvh_node_get_original_BB", "BKE_pbvh_node_num_verts",
+    "BKE_pbvh_node_get_verts", "BKE_pbvh_node_fully_hidden_get", "BKE_pbvh_node_add_proxy",
+    "BKE_pbvh_node_free_proxies", "BKE_pbvh_gather_proxies", "BKE_pbvh_node_frustum_contain_AABB",
+    "BKE_pbvh_node_frustum_exclude_AABB", "BKE_pbvh_get_verts", "DUNE_pbvh_get_vert_normals",
+    "DUNE_pbvh_respect_hide_set", "BKE_pbvh_type",
+    "sg_dab_params_default", "sg_pbvh_leaf_limit_set", "sg_dab_apply", "sg_dab_apply_stroke",
+    "sg_dab_stroke_log", "sg_sync", "sg_last_error", "sg_dab_last_affected", "sg_dab_last_stats",
+    "sg_dab_last_gathered", "sg_pbvh_set_stats", "sg_pbvh_set_grid", "sg_pbvh_read_positions",
+    "sg_pbvh_read_normals", "sg_pbvh_set_mask", "sg_pbvh_tree_info", "sg_pbvh_node_index",
+    "sg_pbvh_node_at", "sg_pbvh_prim_indices", "sg_mesh_icosphere_size", "sg_mesh_icosphere",
+    "sg_mesh_torus", "sg_mesh_noise_displace",
+]
+
+
+def lib():
+    """Load libsculpt_gpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() (no CPU fallback exists)")
+    L = C.CDLL(str(LIB_PATH))
+    vp, i32, f32p, i32p = C.c_void_p, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_int)
+    sig = {
+        "BKE_pbvh_new": (vp, []),
+        "BKE_pbvh_free": (None, [vp]),
+        "BKE_pbvh_build_mesh": (None, [vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, i32]),
+        "BKE_pbvh_search_gather": (None, [vp, SearchCB, vp, C.POINTER(C.POINTER(vp)), i32p]),
+        "BKE_pbvh_search_callback": (None, [vp, SearchCB, vp, HitCB, vp]),
+        "BKE_pbvh_update_bounds": (None, [vp, i32]),
+        "BKE_pbvh_update_normals": (None, [vp, vp]),
+        "BKE_pbvh_vert_coords_apply": (None, [vp, vp, i32]),
+        "BKE_pbvh_vert_coords_alloc": (vp, [vp]),
+        "BKE_pbvh_node_mark_update": (None, [vp]),
+        "BKE_pbvh_node_mark_normals_update": (None, [vp]),
+        "BKE_pbvh_node_mark_redraw": (None, [vp]),
+        "BKE_pbvh_vert_mark_update": (None, [vp, i32]),
+        "BKE_pbvh_node_get_BB": (None, [vp, f32p, f32p]),
+        "BKE_pbvh_node_get_original_BB": (None, [vp, f32p, f32p]),
+        "BKE_pbvh_node_num_verts": (None, [vp, vp, i32p, i32p]),
+        "BKE_pbvh_node_get_verts": (None, [vp, vp, C.POINTER(i32p), C.POINTER(vp)]),
+        "BKE_pbvh_node_fully_hidden_get": (C.c_bool, [vp]),
+        "BKE_pbvh_node_add_proxy": (vp, [vp, vp]),
+        "BKE_pbvh_node_free_proxies": (None, [vp]),
+        "BKE_pbvh_gather_proxies": (None, [vp, C.POINTER(C.POINTER(vp)), i32p]),
+        "BKE_pbvh_node_frustum_contain_AABB": (C.c_bool, [vp, vp]),
+        "BKE_pbvh_node_frustum_exclude_AABB": (C.c_bool, [vp, vp]),
+        "BKE_pbvh_get_verts": (vp, [vp]),
+        "DUNE_pbvh_get_vert_normals": (vp, [vp]),
+        "DUNE_pbvh_respect_hide_set": (None, [vp, C.c_bool]),
+        "BKE_pbvh_type": (i32, [vp]),
+        "sg_dab_params_default": (None, [C.POINTER(SgDabParams), i32]),
+        "sg_pbvh_leaf_limit_set": (i32, [vp, i32]),
+        "sg_dab_apply": (i32, [vp, C.POINTER(SgDabParams)]),
+        "sg_dab_apply_stroke": (i32, [vp, C.POINTER(SgDabParams), i32, i32]),
+        "sg_dab_stroke_log": (i32, [vp, C.POINTER(C.c_longlong), i32]),
+        "sg_sync": (i32, [vp]),
+        "sg_last_error": (C.c_char_p, []),
+        "sg_dab_last_affected": (i32, [vp, i32p, i32]),
+        "sg_dab_last_stats": (i32, [vp, C.POINTER(C.c_int64)]),
+        "sg_dab_last_gathered": (i32, [vp, i32p, i32]),
+        "sg_pbvh_set_stats": (i32, [vp, i32]),
+        "sg_pbvh_set_grid": (i32, [vp, i32]),
+        "sg_pbvh_read_positions": (i32, [vp, f32p]),
+        "sg_pbvh_read_normals": (i32, [vp, f32p]),
+        "sg_pbvh_set_mask": (i32, [vp, f32p]),
+        "sg_pbvh_tree_info": (i32, [vp, i32p]),
+        "sg_pbvh_node_index": (i32, [vp, vp]),
+        "sg_pbvh_node_at": (vp, [vp, i32]),
+        "sg_pbvh_prim_indices": (i32, [vp, i32p, i32]),
+        "sg_mesh_icosphere_size": (i32, [i32, i32p, i32p]),
+        "sg_mesh_icosphere": (i32, [i32, C.c_float, f32p, i32p]),
+        "sg_mesh_torus": (i32, [i32, i32, C.c_float, C.c_float, f32p, i32p]),
+        "sg_mesh_noise_displace": (i32, [f32p, f32p, i32, C.c_uint32, C.c_float, C.c_float, i32]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def last_error() -> str:
+    return (lib().sg_last_error() or b"").decode()
+
+
+def _check(rc, what):
+    if rc < 0:
+        raise RuntimeError(f"{what} failed ({rc}): {last_error()}")
+    return rc
+
+
+# ---------------------------------------------------------------------------
+# synthetic meshes (C generators in the library)
+# ---------------------------------------------------------------------------
+
+def icosphere(level: int, radius: float = 1.0):
+    L = lib()
+    V, T = C.c_int(), C.c_int()
+    _check(L.sg_mesh_icosphere_size(level, C.byref(V), C.byref(T)), "icosphere size")
+    co = np.empty((V.value, 3), np.float32)
+    tris = np.empty((T.value, 3), np.int32)
+    _check(L.sg_mesh_icosphere(level, radius, _fp(co), _ip(tris)), "icosphere")
+    return co, tris
+
+
+def torus(nu: int, nv: int, major_r: float = 1.0, minor_r: float = 0.3):
+    co = np.empty((nu * nv, 3), np.float32)
+    tris = np.empty((2 * nu * nv, 3), np.int32)
+    _check(lib().sg_mesh_torus(nu, nv, major_r, minor_r, _fp(co), _ip(tris)), "torus")
+    return co, tris
+
+
+def torus_normals(nu: int, nv: int):
+    """Analytic outward normals of torus(nu, nv) (direction for noise displacement)."""
+    u = 2 * np.pi * np.arange(nu)[:, None] / nu
+    v = 2 * np.pi * np.arange(nv)[None, :] / nv
+    n = np.stack([np.cos(v) * np.cos(u), np.cos(v) * np.sin(u), np.sin(v) + 0 * u], -1)
+    return n.reshape(-1, 3).astype(np.float32)
+
+
+def noise_displace(co, direction, seed: int, amplitude: float, frequency: float = 4.0, octaves: int = 4):
+    co = np.ascontiguousarray(co, np.float32)
+    direction = np.ascontiguousarray(direction, np.float32)
+    _check(lib().sg_mesh_noise_displace(_fp(co), _fp(direction), co.shape[0], seed, amplitude, frequency, octaves),
+           "noise")
+    return co
+
+
+def noisy_icosphere(level: int, seed: int, amplitude: float):
+    co, tris = icosphere(level)
+    if amplitude:
+        d = co / np.linalg.norm(co, axis=1, keepdims=True)
+        co = noise_displace(co, d.astype(np.float32), seed, amplitude)
+    return co, tris
+
+
+def noisy_torus(nu: int, nv: int, seed: int, amplitude: float, major_r=1.0, minor_r=0.3):
+    co, tris = torus(nu, nv, major_r, minor_r)
+    if amplitude:
+        co = noise_displace(co, torus_normals(nu, nv), seed, amplitude)
+    return co, tris
+
+
+# ---------------------------------------------------------------------------
+# strokes
+# ---------------------------------------------------------------------------
+
+def great_circle_centres(seed: int, radii, spacing_frac: float = 0.1, sphere_radius: float = 1.0):
+    """Dab centres along a seeded random great circle (axis uniform on S^2), arc step
+    spacing_frac * r_i between dab i-1 and dab i."""
+    rng = np.random.default_rng(seed)
+    axis = rng.normal(size=3)
+    axis /= np.linalg.norm(axis)
+    a = np.cross(axis, [1.0, 0.0, 0.0])
+    if np.linalg.norm(a) < 1e-6:
+        a = np.cross(axis, [0.0, 1.0, 0.0])
+    a /= np.linalg.norm(a)
+    b = np.cross(axis, a)
+    theta0 = rng.uniform(0, 2 * np.pi)
+    radii = np.asarray(radii, np.float64)
+    steps = spacing_frac * radii / sphere_radius
+    theta = theta0 + np.concatenate([[0.0], np.cumsum(steps[1:])])
+    pts = sphere_radius * (np.cos(theta)[:, None] * a + np.sin(theta)[:, None] * b)
+    return pts.astype(np.float32)
+
+
+def make_dab(tool=TOOL_DRAW, location=(0, 0, 1), radius=0.1, strength=0.5, falloff=CURVE_SMOOTH, hardness=0.0,
+             smooth_iterations=1, grab_delta=(0, 0, 0), flag=0, normal_radius_factor=0.5, curve_table=None):
+    d = SgDabParams()
+    d.sculpt_tool = tool
+    d.curve_preset = falloff
+    d.flag = flag
+    d.smooth_iterations = smooth_iterations
+    d.location[:] = [float(x) for x in location]
+    d.radius = radius
+    d.strength = strength
+    d.hardness = hardness
+    d.normal_radius_factor = normal_radius_factor
+    d.grab_delta[:] = [float(x) for x in grab_delta]
+    if curve_table is not None:
+        tab = np.ascontiguousarray(curve_table, np.float32)
+        assert tab.shape == (CM_TABLE + 1,)
+        d.curve_table = _fp(tab)
+        d._keep = tab  # keep the table alive with the struct
+    return d
+
+
+def stroke_config(tools, centres, radii, strengths, smooth_iterations=4, falloff=CURVE_SMOOTH):
+    """Dab list: tool i, centre i, radius i, strength i; Grab delta = next centre - centre."""
+    dabs = []
+    n = len(centres)
+    for i in range(n):
+        nxt = centres[i + 1] if i + 1 < n else centres[i]
+        dabs.append(make_dab(tool=int(tools[i]), location=centres[i], radius=float(radii[i]),
+                             strength=float(strengths[i]), falloff=falloff, smooth_iterations=smooth_iterations,
+                             grab_delta=(np.asarray(nxt) - np.asarray(centres[i]))))
+    return dabs
+
+
+# ---------------------------------------------------------------------------
+# PBVH handle
+# ---------------------------------------------------------------------------
+
+class PBVH:
+    """Owner of one GPU-resident PBVH (BKE_pbvh_new/build_mesh ... BKE_pbvh_free)."""
+
+    def __init__(self, co, tris, leaf_limit: int = 128, hidden=None, mat=None):
+        L = lib()
+        co = np.ascontiguousarray(co, np.float32)
+        tris = np.ascontiguousarray(tris, np.int32)
+        self.totvert = co.shape[0]
+        self.tottri = tris.shape[0]
+        self.h = L.BKE_pbvh_new()
+        if not self.h:
+            raise RuntimeError("BKE_pbvh_new failed: " + last_error())
+        _check(L.sg_pbvh_leaf_limit_set(self.h, leaf_limit), "leaf limit")
+        # reference mesh arrays: MVert, one MLoop per corner, one tri poly per face
+        self._mvert = (MVert * self.totvert)()
+        mv = np.frombuffer(self._mvert, dtype=np.uint8).reshape(self.totvert, 16)
+        mv[:, :12] = co.view(np.uint8).reshape(self.totvert, 12)
+        if hidden is not None:
+            mv[:, 12] = np.where(np.asarray(hidden, bool), ME_HIDE, 0).astype(np.uint8)
+        self._mloop = np.zeros((self.tottri * 3, 2), np.uint32)
+        self._mloop[:, 0] = tris.reshape(-1)
+        self._mpoly = np.zeros(self.tottri, dtype=[("loopstart", "<i4"), ("totloop", "<i4"), ("mat_nr", "<i2"),
+                                                   ("flag", "u1"), ("_pad", "u1")])
+        self._mpoly["loopstart"] = np.arange(self.tottri) * 3
+        self._mpoly["totloop"] = 3
+        if mat is not None:
+            self._mpoly["mat_nr"] = np.asarray(mat, np.int16)
+        # the PBVH takes ownership of looptri (freed with free()): allocate with malloc
+        libc = C.CDLL(None)
+        libc.malloc.restype = C.c_void_p
+        libc.malloc.argtypes = [C.c_size_t]
+        lt_ptr = libc.malloc(max(self.tottri, 1) * 16)
+        lt = np.frombuffer((C.c_uint32 * (max(self.tottri, 1) * 4)).from_address(lt_ptr), np.uint32).reshape(-1, 4)
+        lt[: self.tottri, :3] = np.arange(self.tottri * 3, dtype=np.uint32).reshape(-1, 3)
+        lt[: self.tottri, 3] = np.arange(self.tottri, dtype=np.uint32)
+        L.BKE_pbvh_build_mesh(self.h, None, self._mpoly.ctypes.data, self._mloop.ctypes.data,
+                              C.cast(self._mvert, C.c_void_p), self.totvert, None, None, None, lt_ptr, self.tottri)
+        err = last_error()
+        info = self.tree_info()
+        if info["nodes"] == 0 and self.tottri > 0:
+            raise RuntimeError("BKE_pbvh_build_mesh failed: " + err)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().BKE_pbvh_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- brush
+    def dab(self, d: SgDabParams):
+        _check(lib().sg_dab_apply(self.h, C.byref(d)), "sg_dab_apply")
+
+    def stroke(self, dabs, graph=True, refit_each=False, log=False):
+        arr = (SgDabParams * len(dabs))(*dabs)
+        flags = (STROKE_GRAPH if graph else 0) | (STROKE_REFIT_EACH if refit_each else 0) | (STROKE_LOG if log else 0)
+        _check(lib().sg_dab_apply_stroke(self.h, arr, len(dabs), flags), "sg_dab_apply_stroke")
+
+    def stroke_log(self, n):
+        out = np.zeros((n, 16), np.int64)
+        _check(lib().sg_dab_stroke_log(self.h, out.ctypes.data_as(C.POINTER(C.c_longlong)), n), "stroke log")
+        return out
+
+    def set_stats(self, on=True):
+        lib().sg_pbvh_set_stats(self.h, 1 if on else 0)
+
+    def set_grid(self, wg):
+        _check(lib().sg_pbvh_set_grid(self.h, wg), "grid")
+
+    def sync(self):
+        _check(lib().sg_sync(self.h), "sg_sync")
+
+    # -- readback
+    def positions(self):
+        out = np.empty((self.totvert, 3), np.float32)
+        _check(lib().sg_pbvh_read_positions(self.h, _fp(out)), "read positions")
+        return out
+
+    def normals(self):
+        out = np.empty((self.totvert, 3), np.float32)
+        _check(lib().sg_pbvh_read_normals(self.h, _fp(out)), "read normals")
+        return out
+
+    def set_mask(self, mask):
+        if mask is None:
+            _check(lib().sg_pbvh_set_mask(self.h, None), "mask")
+        else:
+            m = np.ascontiguousarray(mask, np.float32)
+            _check(lib().sg_pbvh_set_mask(self.h, _fp(m)), "mask")
+
+    def last_affected(self):
+        cap = self.totvert
+        out = np.empty(cap, np.int32)
+        n = _check(lib().sg_dab_last_affected(self.h, _ip(out), cap), "affected")
+        return out[:n].copy()
+
+    def last_gathered(self):
+        out = np.empty(max(self.tree_info()["leaves"], 1), np.int32)
+        n = _check(lib().sg_dab_last_gathered(self.h, _ip(out), out.size), "gathered")
+        return out[:n].copy()
+
+    def last_stats(self):
+        out = np.zeros(16, np.int64)
+        _check(lib().sg_dab_last_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_int64))), "stats")
+        return out
+
+    def tree_info(self):
+        out = np.zeros(4, np.int32)
+        lib().sg_pbvh_tree_info(self.h, _ip(out))
+        return {"nodes": int(out[0]), "leaves": int(out[1]), "cuts": int(out[2]), "sub_maxn": int(out[3])}
+
+    def node(self, index):
+        return lib().sg_pbvh_node_at(self.h, index)
+
+    def node_bb(self, index, original=False):
+        mn = np.zeros(3, np.float32)
+        mx = np.zeros(3, np.float32)
+        fn = lib().BKE_pbvh_node_get_original_BB if original else lib().BKE_pbvh_node_get_BB
+        fn(self.node(index), _fp(mn), _fp(mx))
+        return np.concatenate([mn, mx])
+
+    def node_verts(self, index):
+        L = lib()
+        node = self.node(index)
+        u, t = C.c_int(), C.c_int()
+        L.BKE_pbvh_node_num_verts(self.h, node, C.byref(u), C.byref(t))
+        ptr = C.POINTER(C.c_int)()
+        L.BKE_pbvh_node_get_verts(self.h, node, C.byref(ptr), None)
+        return np.ctypeslib.as_array(ptr, shape=(t.value,)).copy() if t.value else np.zeros(0, np.int32), u.value
+
+    def prim_indices(self):
+        out = np.empty(self.tottri, np.int32)
+        lib().sg_pbvh_prim_indices(self.h, _ip(out), self.tottri)
+        return out
+
+    def search_gather(self, callback=None):
+        """BKE_pbvh_search_gather with a Python callback(node_index) -> bool."""
+        L = lib()
+        if callback is None:
+            cb = SearchCB()
+        else:
+            cb = SearchCB(lambda node, data: bool(callback(L.sg_pbvh_node_index(self.h, node))))
+        arr = C.POINTER(C.c_void_p)()
+        tot = C.c_int()
+        L.BKE_pbvh_search_gather(self.h, cb, None, C.byref(arr), C.byref(tot))
+        res = [L.sg_pbvh_node_index(self.h, arr[i]) for i in range(tot.value)]
+        if tot.value:
+            C.CDLL(None).free(arr)
+        return res

@@ -1,0 +1,762 @@
+// C ABI: drop-in PBVH API (kernel/intern/pbvh.c) and the sg_* brush entry points.
+// All mesh state lives in HBM; the host keeps the tree shape, node flags and lazily
+// synchronised views (positions, normals, node bounds) for callers that read them.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cstdlib>
+#include <cstring>
+
+#include "sg_internal.h"
+#include "sg_kernels.h"
+
+namespace {
+
+bool ensure_stats_buffers(PBVH *p) {
+  SgDev &d = p->d;
+  if (d.stamp_v) return true;
+  const size_t V = (size_t)std::max(d.V, 1), T = (size_t)std::max(d.T, 1);
+  if (sg_check_hip(hipMalloc(&d.stamp_v, V * sizeof(int)), "hipMalloc")) return false;
+  if (sg_check_hip(hipMalloc(&d.stamp_v2, V * sizeof(int)), "hipMalloc")) return false;
+  if (sg_check_hip(hipMalloc(&d.stamp_t, T * sizeof(int)), "hipMalloc")) return false;
+  hipMemsetAsync(d.stamp_v, 0, V * sizeof(int), p->stream);
+  hipMemsetAsync(d.stamp_v2, 0, V * sizeof(int), p->stream);
+  hipMemsetAsync(d.stamp_t, 0, T * sizeof(int), p->stream);
+  return true;
+}
+
+int ensure_curves(PBVH *p, int n) {
+  SgDev &d = p->d;
+  if (n <= d.curve_cap) return 0;
+  if (d.curve) hipFree(d.curve);
+  d.curve = nullptr;
+  d.curve_cap = 0;
+  if (sg_check_hip(hipMalloc(&d.curve, (size_t)n * (SG_CM_TABLE + 1) * sizeof(float)), "hipMalloc")) return -1;
+  d.curve_cap = n;
+  return 0;
+}
+
+int ensure_log(PBVH *p, int n) {
+  SgDev &d = p->d;
+  if (n <= d.log_cap) return 0;
+  if (d.log) hipFree(d.log);
+  d.log = nullptr;
+  d.log_cap = 0;
+  if (sg_check_hip(hipMalloc(&d.log, (size_t)n * SG_CTR_COUNT * sizeof(long long)), "hipMalloc")) return -1;
+  d.log_cap = n;
+  return 0;
+}
+
+void invalidate_host_views(PBVH *p) {
+  p->host_bb_stale = true;
+  p->host_verts_stale = true;
+  p->host_normals_stale = true;
+}
+
+int sync_bbs(PBVH *p) {
+  if (!p->host_bb_stale || p->nodes.empty() || p->no_device) return 0;
+  if (p->nodes_dirty_deferred) {
+    if (sg_enqueue_flush(p)) return -1;
+    p->nodes_dirty_deferred = false;
+  }
+  const SgDev &d = p->d;
+  std::vector<float> lbb((size_t)d.L * 8), lobb((size_t)d.L * 8), nbb((size_t)d.N * 8), nobb((size_t)d.N * 8);
+  hipMemcpyAsync(lbb.data(), d.leaf_bb, lbb.size() * 4, hipMemcpyDeviceToHost, p->stream);
+  hipMemcpyAsync(lobb.data(), d.leaf_obb, lobb.size() * 4, hipMemcpyDeviceToHost, p->stream);
+  hipMemcpyAsync(nbb.data(), d.node_bb, nbb.size() * 4, hipMemcpyDeviceToHost, p->stream);
+  hipMemcpyAsync(nobb.data(), d.node_obb, nobb.size() * 4, hipMemcpyDeviceToHost, p->stream);
+  if (sg_check_hip(hipStreamSynchronize(p->stream), "sync bounds")) return -1;
+  for (PBVHNode &n : p->nodes) {
+    const float *src = (n.leaf >= 0) ? &lbb[8 * (size_t)n.leaf] : &nbb[8 * (size_t)n.index];
+    const float *osrc = (n.leaf >= 0) ? &lobb[8 * (size_t)n.leaf] : &nobb[8 * (size_t)n.index];
+    for (int k = 0; k < 3; k++) {
+      n.vb[k] = src[k];
+      n.vb[3 + k] = src[4 + k];
+      n.orig_vb[k] = osrc[k];
+      n.orig_vb[3 + k] = osrc[4 + k];
+    }
+  }
+  p->host_bb_stale = false;
+  return 0;
+}
+
+int read_aos(PBVH *p, int which, float *out) {
+  if (p->no_device) return sg_set_error("no HIP device");
+  const size_t n = 3 * (size_t)p->d.V;
+  float *tmp = nullptr;
+  if (sg_check_hip(hipMalloc(&tmp, std::max<size_t>(n, 1) * sizeof(float)), "hipMalloc")) return -1;
+  int rc = sg_enqueue_gather(p, which, tmp);
+  if (!rc) rc = sg_check_hip(hipMemcpyAsync(out, tmp, n * sizeof(float), hipMemcpyDeviceToHost, p->stream), "copy");
+  if (!rc) rc = sg_check_hip(hipStreamSynchronize(p->stream), "sync");
+  hipFree(tmp);
+  return rc;
+}
+
+int sync_verts(PBVH *p) {
+  if (!p->host_verts || !p->host_verts_stale || p->no_device) return 0;
+  std::vector<float> co(3 * (size_t)p->totvert);
+  if (read_aos(p, 0, co.data())) return -1;
+  for (int v = 0; v < p->totvert; v++) memcpy(p->host_verts[v].co, &co[3 * (size_t)v], 12);
+  p->host_verts_stale = false;
+  return 0;
+}
+
+int sync_normals(PBVH *p) {
+  if (!p->host_normals_stale) return 0;
+  p->host_normals.resize(3 * (size_t)p->totvert);
+  if (read_aos(p, 1, p->host_normals.data())) return -1;
+  p->host_normals_stale = false;
+  return 0;
+}
+
+// Upload the host view of the vertices of the given leaves (edits made through
+// BKE_pbvh_get_verts before the caller flagged the nodes).
+int upload_host_leaf_verts(PBVH *p, const std::vector<int> &leaves) {
+  if (!p->host_verts || p->host_verts_stale) return 0;
+  std::vector<int> idx;
+  for (int l : leaves) {
+    const PBVHNode &n = p->nodes[p->leaf_node[l]];
+    for (int k = 0; k < n.uniq_verts + n.face_verts; k++) idx.push_back(p->perm_new_of_old[n.vert_indices[k]]);
+  }
+  std::sort(idx.begin(), idx.end());
+  idx.erase(std::unique(idx.begin(), idx.end()), idx.end());
+  // contiguous runs -> few copies
+  size_t i = 0;
+  std::vector<float> x, y, z;
+  while (i < idx.size()) {
+    size_t j = i;
+    while (j + 1 < idx.size() && idx[j + 1] == idx[j] + 1) j++;
+    const int b = idx[i], n = (int)(j - i + 1);
+    x.resize(n);
+    y.resize(n);
+    z.resize(n);
+    for (int q = 0; q < n; q++) {
+      const float *co = p->host_verts[p->perm_old_of_new[b + q]].co;
+      x[q] = co[0];
+      y[q] = co[1];
+      z[q] = co[2];
+    }
+    hipMemcpy(p->d.px + b, x.data(), n * 4, hipMemcpyHostToDevice);
+    hipMemcpy(p->d.py + b, y.data(), n * 4, hipMemcpyHostToDevice);
+    hipMemcpy(p->d.pz + b, z.data(), n * 4, hipMemcpyHostToDevice);
+    i = j + 1;
+  }
+  return 0;
+}
+
+std::vector<int> flagged_leaves(PBVH *p, int flag) {
+  std::vector<int> out;
+  for (size_t l = 0; l < p->leaf_node.size(); l++) {
+    if (p->nodes[p->leaf_node[l]].flag & flag) out.push_back((int)l);
+  }
+  return out;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *sg_last_error(void) { return sg_last_error_str(); }
+
+PBVH *BKE_pbvh_new(void) {
+  PBVH *p = new PBVH();
+  p->respect_hide = true;  // pbvh.c:648
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    // No GPU: the handle can still build and expose the host tree (for inspection);
+    // every call that needs the device reports an error.
+    (void)hipGetLastError();
+    p->no_device = true;
+    return p;
+  }
+  if (sg_check_hip(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking), "hipStreamCreate")) {
+    delete p;
+    return nullptr;
+  }
+  return p;
+}
+
+void BKE_pbvh_free(PBVH *p) {
+  if (!p) return;
+  if (!p->no_device) hipStreamSynchronize(p->stream);
+  for (PBVHNode &n : p->nodes) {
+    if (n.flag & PBVH_Leaf) {
+      free(n.vert_indices);
+      free(n.face_vert_indices);
+      if (n.proxies) BKE_pbvh_node_free_proxies(&n);
+    }
+  }
+  if (!p->no_device) sg_free_device(p);
+  if (p->verts_owned) free(p->host_verts);
+  free(p->looptri);
+  if (!p->no_device) hipStreamDestroy(p->stream);
+  delete p;
+}
+
+int sg_pbvh_leaf_limit_set(PBVH *p, int leaf_limit) {
+  if (!p) return sg_set_error("null PBVH");
+  p->leaf_limit = leaf_limit > 0 ? leaf_limit : 10000;
+  return 0;
+}
+
+void DUNE_pbvh_respect_hide_set(PBVH *p, bool respect_hide) {
+  if (p) p->respect_hide = respect_hide;
+}
+
+PBVHType BKE_pbvh_type(const PBVH *p) { return p ? p->type : PBVH_FACES; }
+
+void BKE_pbvh_build_mesh(PBVH *p, Mesh *mesh, const MPoly *mpoly, const MLoop *mloop, MVert *verts,
+                         int totvert, CustomData *vdata, CustomData *ldata, CustomData *pdata,
+                         const MLoopTri *looptri, int looptri_num) {
+  (void)mesh;
+  (void)vdata;
+  (void)ldata;
+  (void)pdata;
+  if (!p || !mloop || !verts || totvert <= 0 || looptri_num < 0) {
+    sg_set_error("BKE_pbvh_build_mesh: invalid arguments");
+    return;
+  }
+  if (!p->nodes.empty()) {
+    for (PBVHNode &n : p->nodes) {
+      if (n.flag & PBVH_Leaf) {
+        free(n.vert_indices);
+        free(n.face_vert_indices);
+      }
+    }
+    if (!p->no_device) sg_free_device(p);
+  }
+  p->type = PBVH_FACES;
+  p->totvert = totvert;
+  p->totprim = looptri_num;
+  p->host_verts = verts;
+  p->verts_owned = false;
+  p->looptri = const_cast<MLoopTri *>(looptri);
+  p->tris.resize(3 * (size_t)looptri_num);
+  for (int t = 0; t < looptri_num; t++) {
+    for (int j = 0; j < 3; j++) p->tris[3 * (size_t)t + j] = (int)mloop[looptri[t].tri[j]].v;
+  }
+  p->tri_mat.clear();
+  if (mpoly && looptri_num > 0) {
+    std::vector<int> key(looptri_num);
+    bool uniform = true;
+    for (int t = 0; t < looptri_num; t++) {
+      const MPoly &mp = mpoly[looptri[t].poly];
+      key[t] = (int)(unsigned short)mp.mat_nr | ((mp.flag & ME_SMOOTH) << 16);  // face_materials_match
+      if (key[t] != key[0]) uniform = false;
+    }
+    if (!uniform) p->tri_mat.swap(key);
+  }
+  p->vhide.clear();
+  bool any_hidden = false;
+  for (int v = 0; v < totvert && !any_hidden; v++) any_hidden = (verts[v].flag & ME_HIDE) != 0;
+  if (any_hidden) {
+    p->vhide.resize(totvert);
+    for (int v = 0; v < totvert; v++) p->vhide[v] = (verts[v].flag & ME_HIDE) ? 1 : 0;
+  }
+  std::vector<float> co(3 * (size_t)totvert);
+  for (int v = 0; v < totvert; v++) memcpy(&co[3 * (size_t)v], verts[v].co, 12);
+  sg_build_tree(p, co.data(), p->tris.data(), looptri_num, totvert);
+  if (p->no_device) {
+    p->host_bb_stale = false;
+    p->host_verts_stale = false;
+    sg_set_error("no HIP device: tree built on the host only");
+    return;
+  }
+  p->build_ok = false;
+  sg_build_device(p, co.data());
+  if (p->cut.node.size() > 1024) {
+    sg_set_error("mesh too large for the two-level search (%zu cut subtrees)", p->cut.node.size());
+    return;
+  }
+  if ((size_t)p->d.sub_maxn * 24 > 65536 || (size_t)(p->d.C + p->d.top_n) * 24 > 65536) {
+    sg_set_error("refit tables exceed LDS");
+    return;
+  }
+  p->build_ok = true;
+  sg_enqueue_full_normals(p, 0);
+  hipStreamSynchronize(p->stream);
+  p->host_vmark.assign(totvert, 0);
+  invalidate_host_views(p);
+  p->host_verts_stale = false;  // the caller's array is the source
+}
+
+void sg_dab_params_default(SgDabParams *d, int tool) {
+  memset(d, 0, sizeof(*d));
+  d->sculpt_tool = tool;
+  d->curve_preset = SG_CURVE_SMOOTH;
+  d->smooth_iterations = 1;
+  d->radius = 0.1f;
+  d->strength = 0.5f;
+  d->hardness = 0.0f;
+  d->normal_radius_factor = 0.5f;
+}
+
+static int check_dab(PBVH *p, const SgDabParams *dab) {
+  if (!p || !dab) return sg_set_error("null argument");
+  if (p->no_device) return sg_set_error("no HIP device");
+  if (p->nodes.empty() || !p->build_ok) return sg_set_error("PBVH has no usable mesh");
+  const int t = dab->sculpt_tool;
+  if (t != SG_TOOL_DRAW && t != SG_TOOL_SMOOTH && t != SG_TOOL_INFLATE && t != SG_TOOL_GRAB)
+    return sg_set_error("unsupported sculpt_tool %d", t);
+  if (!(dab->radius > 0.0f)) return sg_set_error("radius must be > 0");
+  if (dab->curve_preset < 0 || dab->curve_preset > SG_CURVE_SMOOTHER) return sg_set_error("bad curve_preset");
+  if (t == SG_TOOL_SMOOTH && dab->smooth_iterations > 64) return sg_set_error("smooth_iterations > 64");
+  return 0;
+}
+
+static bool g_stats = false;
+
+int sg_pbvh_set_stats(PBVH *p, int on) {
+  (void)p;
+  g_stats = on != 0;
+  return 0;
+}
+
+int sg_pbvh_set_grid(PBVH *p, int workgroups) {
+  if (!p || workgroups < 1 || workgroups > 65536) return sg_set_error("bad grid");
+  p->grid_wg = workgroups;
+  return 0;
+}
+
+int sg_dab_apply(PBVH *p, const SgDabParams *dab) {
+  if (int rc = check_dab(p, dab)) return rc;
+  const float *curve = nullptr;
+  if (dab->curve_preset == SG_CURVE_CUSTOM && dab->curve_table) {
+    if (ensure_curves(p, 1)) return -1;
+    hipMemcpyAsync(p->d.curve, dab->curve_table, (SG_CM_TABLE + 1) * sizeof(float), hipMemcpyHostToDevice,
+                   p->stream);
+    curve = p->d.curve;
+  }
+  if (g_stats && !ensure_stats_buffers(p)) return -1;
+  SgDabK k = sg_make_dab(dab, g_stats ? 1 : 0);
+  const int stamp = (++p->stamp) * 16;
+  int rc = sg_enqueue_dab(p, k, curve, nullptr, 0, stamp);
+  if (!rc) rc = sg_enqueue_flush(p);
+  long long ctr[SG_CTR_COUNT];
+  if (!rc) rc = sg_check_hip(hipMemcpyAsync(ctr, p->d.ctr, sizeof(ctr), hipMemcpyDeviceToHost, p->stream), "copy");
+  if (!rc) rc = sg_check_hip(hipStreamSynchronize(p->stream), "dab");
+  if (rc) return rc;
+  for (int i = 0; i < 8; i++) p->last_stats[i] = 0;
+  p->last_tool = dab->sculpt_tool;
+  p->dab_count++;
+  p->nodes_dirty_deferred = false;
+  invalidate_host_views(p);
+  memcpy(p->last_ctr, ctr, sizeof(ctr));
+  return 0;
+}
+
+int sg_dab_apply_stroke(PBVH *p, const SgDabParams *dabs, int n, int flags) {
+  if (!p || (!dabs && n > 0) || n < 0) return sg_set_error("bad arguments");
+  if (p->no_device) return sg_set_error("no HIP device");
+  for (int i = 0; i < n; i++) {
+    if (int rc = check_dab(p, &dabs[i])) return rc;
+  }
+  if (n == 0) return 0;
+  const bool use_graph = (flags & 1) != 0;
+  const bool flush_each = (flags & 2) != 0;
+  const bool log = (flags & 4) != 0;
+  int ncustom = 0;
+  for (int i = 0; i < n; i++) ncustom += (dabs[i].curve_preset == SG_CURVE_CUSTOM && dabs[i].curve_table) ? 1 : 0;
+  if (ncustom && ensure_curves(p, ncustom)) return -1;
+  if (log && ensure_log(p, n)) return -1;
+  if (g_stats && !ensure_stats_buffers(p)) return -1;
+  {
+    int ci = 0;
+    for (int i = 0; i < n; i++) {
+      if (dabs[i].curve_preset == SG_CURVE_CUSTOM && dabs[i].curve_table) {
+        hipMemcpyAsync(p->d.curve + (size_t)ci * (SG_CM_TABLE + 1), dabs[i].curve_table,
+                       (SG_CM_TABLE + 1) * sizeof(float), hipMemcpyHostToDevice, p->stream);
+        ci++;
+      }
+    }
+  }
+  auto enqueue_all = [&]() -> int {
+    int ci = 0;
+    for (int i = 0; i < n; i++) {
+      const float *curve = nullptr;
+      if (dabs[i].curve_preset == SG_CURVE_CUSTOM && dabs[i].curve_table) curve = p->d.curve + (size_t)(ci++) * (SG_CM_TABLE + 1);
+      SgDabK k = sg_make_dab(&dabs[i], g_stats ? 1 : 0);
+      const int stamp = (++p->stamp) * 16;
+      if (int rc = sg_enqueue_dab(p, k, curve, log ? p->d.log : nullptr, i, stamp)) return rc;
+      if (flush_each && sg_enqueue_flush(p)) return -1;
+    }
+    if (!flush_each && sg_enqueue_flush(p)) return -1;
+    return 0;
+  };
+  int rc = 0;
+  if (use_graph) {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    rc = sg_check_hip(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    if (!rc) {
+      const int erc = enqueue_all();
+      const hipError_t e = hipStreamEndCapture(p->stream, &graph);
+      rc = erc ? erc : sg_check_hip(e, "end capture");
+    }
+    if (!rc) rc = sg_check_hip(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "instantiate");
+    if (!rc) rc = sg_check_hip(hipGraphLaunch(exec, p->stream), "graph launch");
+    if (!rc) rc = sg_check_hip(hipStreamSynchronize(p->stream), "stroke");
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+  } else {
+    rc = enqueue_all();
+    if (!rc) rc = sg_check_hip(hipStreamSynchronize(p->stream), "stroke");
+  }
+  if (rc) return rc;
+  long long ctr[SG_CTR_COUNT];
+  hipMemcpy(ctr, p->d.ctr, sizeof(ctr), hipMemcpyDeviceToHost);
+  memcpy(p->last_ctr, ctr, sizeof(ctr));
+  if (log) hipMemcpy(p->d.log + (size_t)(n - 1) * SG_CTR_COUNT, ctr, sizeof(ctr), hipMemcpyHostToDevice);
+  p->last_tool = dabs[n - 1].sculpt_tool;
+  p->dab_count += n;
+  p->nodes_dirty_deferred = false;
+  invalidate_host_views(p);
+  return 0;
+}
+
+int sg_dab_stroke_log(PBVH *p, long long *out, int n) {
+  if (!p || !p->d.log || n > p->d.log_cap) return sg_set_error("no stroke log");
+  return sg_check_hip(hipMemcpy(out, p->d.log, (size_t)n * SG_CTR_COUNT * sizeof(long long), hipMemcpyDeviceToHost),
+                      "copy log");
+}
+
+int sg_sync(PBVH *p) {
+  if (!p) return sg_set_error("null PBVH");
+  if (p->no_device) return sg_set_error("no HIP device");
+  return sg_check_hip(hipStreamSynchronize(p->stream), "sync");
+}
+
+int sg_dab_last_affected(PBVH *p, int *out, int cap) {
+  if (!p) return sg_set_error("null PBVH");
+  if (p->no_device) return sg_set_error("no HIP device");
+  const int na = (int)p->last_ctr[SG_CTR_NAFF];
+  std::vector<int> tmp(std::max(na, 1));
+  if (na > 0 && sg_check_hip(hipMemcpy(tmp.data(), p->d.affected, na * sizeof(int), hipMemcpyDeviceToHost), "copy"))
+    return -1;
+  for (int i = 0; i < na; i++) tmp[i] = p->perm_old_of_new[tmp[i]];
+  std::sort(tmp.begin(), tmp.begin() + na);
+  for (int i = 0; i < na && i < cap; i++) out[i] = tmp[i];
+  return na;
+}
+
+int sg_dab_last_stats(PBVH *p, int64_t out[16]) {
+  if (!p) return sg_set_error("null PBVH");
+  for (int i = 0; i < 16; i++) out[i] = p->last_ctr[i];
+  return 0;
+}
+
+int sg_dab_last_gathered(PBVH *p, int *out, int cap) {
+  if (!p) return sg_set_error("null PBVH");
+  if (p->no_device) return sg_set_error("no HIP device");
+  const int ng = (int)p->last_ctr[SG_CTR_NGATHER];
+  std::vector<int> tmp(std::max(ng, 1));
+  if (ng > 0 && sg_check_hip(hipMemcpy(tmp.data(), p->d.gathered, ng * sizeof(int), hipMemcpyDeviceToHost), "copy"))
+    return -1;
+  for (int i = 0; i < ng && i < cap; i++) out[i] = p->leaf_node[tmp[i]];
+  return ng;
+}
+
+int sg_pbvh_read_positions(PBVH *p, float *out) {
+  if (!p) return sg_set_error("null PBVH");
+  return read_aos(p, 0, out);
+}
+
+int sg_pbvh_read_normals(PBVH *p, float *out) {
+  if (!p) return sg_set_error("null PBVH");
+  return read_aos(p, 1, out);
+}
+
+int sg_pbvh_set_mask(PBVH *p, const float *mask) {
+  if (!p) return sg_set_error("null PBVH");
+  if (p->no_device) return sg_set_error("no HIP device");
+  SgDev &d = p->d;
+  if (!mask) {
+    if (d.mask) hipFree(d.mask);
+    d.mask = nullptr;
+    return 0;
+  }
+  if (!d.mask && sg_check_hip(hipMalloc(&d.mask, (size_t)std::max(d.V, 1) * sizeof(float)), "hipMalloc")) return -1;
+  std::vector<float> m(d.V);
+  for (int i = 0; i < d.V; i++) m[i] = mask[p->perm_old_of_new[i]];
+  return sg_check_hip(hipMemcpy(d.mask, m.data(), (size_t)d.V * sizeof(float), hipMemcpyHostToDevice), "copy");
+}
+
+int sg_pbvh_tree_info(PBVH *p, int out[4]) {
+  if (!p) return sg_set_error("null PBVH");
+  out[0] = (int)p->nodes.size();
+  out[1] = (int)p->leaf_node.size();
+  out[2] = (int)p->cut.node.size();
+  out[3] = p->d.sub_maxn;
+  return 0;
+}
+
+int sg_pbvh_node_index(PBVH *p, PBVHNode *node) {
+  if (!p || !node) return -1;
+  return node->index;
+}
+
+PBVHNode *sg_pbvh_node_at(PBVH *p, int index) {
+  if (!p || index < 0 || index >= (int)p->nodes.size()) return nullptr;
+  return &p->nodes[index];
+}
+
+int sg_pbvh_prim_indices(PBVH *p, int *out, int cap) {
+  if (!p) return sg_set_error("null PBVH");
+  const int n = (int)p->prim_indices.size();
+  for (int i = 0; i < n && i < cap; i++) out[i] = p->prim_indices[i];
+  return n;
+}
+
+// ---- search ----------------------------------------------------------------
+
+// pbvh_iter_next order (pbvh.c:746-787): children pushed right then left.
+static void host_search(PBVH *p, BKE_pbvh_SearchCallback scb, void *data, std::vector<PBVHNode *> &out) {
+  out.clear();
+  if (p->nodes.empty()) return;
+  sync_bbs(p);
+  std::vector<int> st{0};
+  while (!st.empty()) {
+    const int ni = st.back();
+    st.pop_back();
+    PBVHNode *n = &p->nodes[ni];
+    if (scb && !scb(n, data)) continue;
+    if (n->flag & PBVH_Leaf) {
+      out.push_back(n);
+      continue;
+    }
+    st.push_back(n->children_offset + 1);
+    st.push_back(n->children_offset);
+  }
+}
+
+void BKE_pbvh_search_gather(PBVH *p, BKE_pbvh_SearchCallback scb, void *search_data, PBVHNode ***r_array,
+                            int *r_tot) {
+  *r_array = nullptr;
+  *r_tot = 0;
+  if (!p) return;
+  std::vector<PBVHNode *> v;
+  host_search(p, scb, search_data, v);
+  if (v.empty()) return;  // NULL when nothing found (pbvh.c:842-845)
+  PBVHNode **arr = (PBVHNode **)malloc(sizeof(PBVHNode *) * v.size());
+  memcpy(arr, v.data(), sizeof(PBVHNode *) * v.size());
+  *r_array = arr;
+  *r_tot = (int)v.size();
+}
+
+void BKE_pbvh_search_callback(PBVH *p, BKE_pbvh_SearchCallback scb, void *search_data, BKE_pbvh_HitCallback hcb,
+                              void *hit_data) {
+  if (!p) return;
+  std::vector<PBVHNode *> v;
+  host_search(p, scb, search_data, v);
+  for (PBVHNode *n : v) hcb(n, hit_data);
+}
+
+// ---- marking ---------------------------------------------------------------
+
+void BKE_pbvh_node_mark_update(PBVHNode *node) {  // pbvh.c:1723-1727
+  node->flag |= PBVH_UpdateNormals | PBVH_UpdateBB | PBVH_UpdateOriginalBB | PBVH_UpdateDrawBuffers | PBVH_UpdateRedraw;
+}
+void BKE_pbvh_node_mark_normals_update(PBVHNode *node) { node->flag |= PBVH_UpdateNormals; }
+void BKE_pbvh_node_mark_redraw(PBVHNode *node) { node->flag |= PBVH_UpdateDrawBuffers | PBVH_UpdateRedraw; }
+void BKE_pbvh_vert_mark_update(PBVH *p, int index) {
+  if (p && index >= 0 && index < p->totvert) p->host_vmark[index] = 1;
+}
+
+// ---- updates ---------------------------------------------------------------
+
+void BKE_pbvh_update_bounds(PBVH *p, int flag) {  // pbvh.c:1401-1421
+  if (!p || p->nodes.empty() || p->no_device) return;
+  if (!(flag & (PBVH_UpdateBB | PBVH_UpdateOriginalBB))) {
+    for (PBVHNode &n : p->nodes) n.flag &= ~(flag & PBVH_UpdateRedraw);
+    return;
+  }
+  std::vector<int> leaves = flagged_leaves(p, flag & (PBVH_UpdateBB | PBVH_UpdateOriginalBB));
+  if (!leaves.empty()) {
+    upload_host_leaf_verts(p, leaves);
+    int *dl = nullptr;
+    if (sg_check_hip(hipMalloc(&dl, leaves.size() * sizeof(int)), "hipMalloc")) return;
+    hipMemcpyAsync(dl, leaves.data(), leaves.size() * sizeof(int), hipMemcpyHostToDevice, p->stream);
+    sg_enqueue_refit_list(p, dl, (int)leaves.size());
+    sg_enqueue_flush(p);
+    hipStreamSynchronize(p->stream);
+    hipFree(dl);
+    p->host_bb_stale = true;
+  }
+  for (int l : leaves) p->nodes[p->leaf_node[l]].flag &= ~(PBVH_UpdateBB | PBVH_UpdateOriginalBB | PBVH_UpdateRedraw);
+}
+
+void BKE_pbvh_update_normals(PBVH *p, SubdivCCG *subdiv_ccg) {  // pbvh.c:4559-4587
+  (void)subdiv_ccg;
+  if (!p || p->nodes.empty() || p->no_device) return;
+  std::vector<int> leaves = flagged_leaves(p, PBVH_UpdateNormals);
+  if (leaves.empty()) return;
+  upload_host_leaf_verts(p, leaves);
+  std::vector<uint8_t> marks(p->d.V, 0);
+  for (int v = 0; v < p->totvert; v++) marks[p->perm_new_of_old[v]] = p->host_vmark[v];
+  hipMemcpyAsync(p->d.vmark, marks.data(), marks.size(), hipMemcpyHostToDevice, p->stream);
+  int *dl = nullptr;
+  if (sg_check_hip(hipMalloc(&dl, leaves.size() * sizeof(int)), "hipMalloc")) return;
+  hipMemcpyAsync(dl, leaves.data(), leaves.size() * sizeof(int), hipMemcpyHostToDevice, p->stream);
+  sg_enqueue_normals_list(p, dl, (int)leaves.size());
+  hipStreamSynchronize(p->stream);
+  hipFree(dl);
+  // marks of uniq verts of the processed leaves are consumed (pbvh.c:1084-1087)
+  for (int l : leaves) {
+    const PBVHNode &n = p->nodes[p->leaf_node[l]];
+    for (int k = 0; k < n.uniq_verts; k++) p->host_vmark[n.vert_indices[k]] = 0;
+    p->nodes[p->leaf_node[l]].flag &= ~PBVH_UpdateNormals;
+  }
+  p->host_normals_stale = true;
+}
+
+void BKE_pbvh_vert_coords_apply(PBVH *p, const float (*vertCos)[3], int totvert) {  // pbvh.c:4707-4749
+  if (!p || p->nodes.empty() || p->no_device) return;
+  if (totvert != p->totvert) {
+    sg_set_error("PBVH: Given deforming vcos number does not match PBVH vertex number!");
+    return;
+  }
+  if (!p->verts_owned && p->host_verts) {  // duplicate on first deform (pbvh.c:4714-4726)
+    sync_verts(p);
+    MVert *dup = (MVert *)malloc(sizeof(MVert) * (size_t)totvert);
+    memcpy(dup, p->host_verts, sizeof(MVert) * (size_t)totvert);
+    p->host_verts = dup;
+    p->verts_owned = true;
+  }
+  float *src = nullptr;
+  if (sg_check_hip(hipMalloc(&src, 3 * (size_t)totvert * sizeof(float)), "hipMalloc")) return;
+  hipMemcpyAsync(src, vertCos, 3 * (size_t)totvert * sizeof(float), hipMemcpyHostToDevice, p->stream);
+  sg_enqueue_coords_apply(p, src);
+  sg_enqueue_full_normals(p, 1);
+  sg_enqueue_clear_marks(p);
+  sg_enqueue_flush(p);
+  hipStreamSynchronize(p->stream);
+  hipFree(src);
+  for (PBVHNode &n : p->nodes) {
+    BKE_pbvh_node_mark_update(&n);
+    n.flag &= ~(PBVH_UpdateBB | PBVH_UpdateOriginalBB | PBVH_UpdateNormals);
+  }
+  invalidate_host_views(p);
+}
+
+float (*BKE_pbvh_vert_coords_alloc(PBVH *p))[3] {
+  if (!p || p->totvert == 0) return nullptr;
+  float(*out)[3] = (float(*)[3])calloc((size_t)p->totvert, sizeof(float[3]));
+  if (read_aos(p, 0, &out[0][0])) {
+    free(out);
+    return nullptr;
+  }
+  return out;
+}
+
+// ---- node access -------------------------------------------------------------
+
+void BKE_pbvh_node_get_BB(PBVHNode *node, float bb_min[3], float bb_max[3]) {
+  sync_bbs(node->owner);
+  memcpy(bb_min, node->vb, 12);
+  memcpy(bb_max, node->vb + 3, 12);
+}
+
+void BKE_pbvh_node_get_original_BB(PBVHNode *node, float bb_min[3], float bb_max[3]) {
+  sync_bbs(node->owner);
+  memcpy(bb_min, node->orig_vb, 12);
+  memcpy(bb_max, node->orig_vb + 3, 12);
+}
+
+void BKE_pbvh_node_num_verts(PBVH *p, PBVHNode *node, int *r_uniquevert, int *r_totvert) {
+  (void)p;
+  if (r_totvert) *r_totvert = node->uniq_verts + node->face_verts;
+  if (r_uniquevert) *r_uniquevert = node->uniq_verts;
+}
+
+void BKE_pbvh_node_get_verts(PBVH *p, PBVHNode *node, const int **r_vert_indices, MVert **r_verts) {
+  if (r_vert_indices) *r_vert_indices = node->vert_indices;
+  if (r_verts) *r_verts = BKE_pbvh_get_verts(p);
+}
+
+bool BKE_pbvh_node_fully_hidden_get(PBVHNode *node) {
+  return (node->flag & PBVH_Leaf) && (node->flag & PBVH_FullyHidden);
+}
+
+MVert *BKE_pbvh_get_verts(const PBVH *cp) {
+  PBVH *p = const_cast<PBVH *>(cp);
+  if (!p) return nullptr;
+  sync_verts(p);
+  return p->host_verts;
+}
+
+const float (*DUNE_pbvh_get_vert_normals(const PBVH *cp))[3] {
+  PBVH *p = const_cast<PBVH *>(cp);
+  if (!p) return nullptr;
+  sync_normals(p);
+  return (const float(*)[3])p->host_normals.data();
+}
+
+// ---- proxies (pbvh.c:4757-4818) ----------------------------------------------
+
+PBVHProxyNode *BKE_pbvh_node_add_proxy(PBVH *p, PBVHNode *node) {
+  const int index = node->proxy_count;
+  node->proxy_count++;
+  node->proxies = (PBVHProxyNode *)realloc(node->proxies, node->proxy_count * sizeof(PBVHProxyNode));
+  int totverts = 0;
+  BKE_pbvh_node_num_verts(p, node, &totverts, nullptr);
+  node->proxies[index].co = (float(*)[3])calloc((size_t)std::max(totverts, 1), sizeof(float[3]));
+  return node->proxies + index;
+}
+
+void BKE_pbvh_node_free_proxies(PBVHNode *node) {
+  for (int i = 0; i < node->proxy_count; i++) {
+    free(node->proxies[i].co);
+    node->proxies[i].co = nullptr;
+  }
+  free(node->proxies);
+  node->proxies = nullptr;
+  node->proxy_count = 0;
+}
+
+void BKE_pbvh_gather_proxies(PBVH *p, PBVHNode ***r_array, int *r_tot) {
+  std::vector<PBVHNode *> v;
+  for (PBVHNode &n : p->nodes) {
+    if (n.proxy_count > 0) v.push_back(&n);
+  }
+  *r_array = nullptr;
+  *r_tot = 0;
+  if (v.empty()) return;
+  *r_array = (PBVHNode **)malloc(sizeof(PBVHNode *) * v.size());
+  memcpy(*r_array, v.data(), sizeof(PBVHNode *) * v.size());
+  *r_tot = (int)v.size();
+}
+
+// ---- frustum (pbvh.c:4507-4557) -----------------------------------------------
+
+static int frustum_aabb(const float *bmin, const float *bmax, const PBVHFrustumPlanes *f) {
+  int ret = 0;  // 0 inside, 1 intersect, 2 outside
+  for (int i = 0; i < f->num_planes; i++) {
+    float vmin[3], vmax[3];
+    for (int a = 0; a < 3; a++) {
+      if (f->planes[i][a] < 0) {
+        vmin[a] = bmin[a];
+        vmax[a] = bmax[a];
+      } else {
+        vmin[a] = bmax[a];
+        vmax[a] = bmin[a];
+      }
+    }
+    const float *pl = f->planes[i];
+    if (pl[0] * vmin[0] + pl[1] * vmin[1] + pl[2] * vmin[2] + pl[3] < 0) return 2;
+    if (pl[0] * vmax[0] + pl[1] * vmax[1] + pl[2] * vmax[2] + pl[3] <= 0) ret = 1;
+  }
+  return ret;
+}
+
+bool BKE_pbvh_node_frustum_contain_AABB(PBVHNode *node, void *data) {
+  sync_bbs(node->owner);
+  return frustum_aabb(node->vb, node->vb + 3, (const PBVHFrustumPlanes *)data) != 2;
+}
+
+bool BKE_pbvh_node_frustum_exclude_AABB(PBVHNode *node, void *data) {
+  sync_bbs(node->owner);
+  return frustum_aabb(node->vb, node->vb + 3, (const PBVHFrustumPlanes *)data) != 0;
+}
+
+}  // extern "C"

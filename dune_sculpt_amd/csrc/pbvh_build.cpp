@@ -1,0 +1,637 @@
+// Host-side PBVH build and the leaf-ordered SoA layout uploaded to HBM.
+//
+// The tree is built exactly as the reference builds it (kernel/intern/pbvh.c:534-596
+// BKE_pbvh_build_mesh -> :509 pbvh_build -> :454 build_sub -> :152 partition_indices /
+// :172 partition_indices_material -> :391 build_leaf -> :256 build_mesh_leaf_node), so
+// node numbering, leaf membership and the uniq/shared vertex split are identical.  The
+// recursion is replaced by an explicit stack that visits nodes in the same order.
+//
+// On top of the tree this file derives the GPU layout:
+//  * vertices renumbered leaf by leaf in DFS (left->right) order, uniq vertices of a
+//    leaf contiguous, so a leaf's vertex reads are one coalesced range;
+//  * triangles in prim_indices order (each leaf's tris contiguous);
+//  * vertex->tri and vertex->vertex CSR in the new numbering, each list ordered by
+//    ascending ORIGINAL tri / vertex id (the order the normal and smooth sums use);
+//  * a cut of the tree into subtrees of <= K leaves for the two-level sphere search,
+//    with height-ordered tables for the deferred refit of internal nodes.
+#include <algorithm>
+#include <cfloat>
+#include <cstdio>
+#include <cstring>
+#include <thread>
+
+#include "sg_internal.h"
+
+namespace {
+
+struct BBC {
+  float bmin[3], bmax[3], bcentroid[3];
+};
+
+inline float fmin_ref(float a, float b) { return (a < b) ? a : b; }  // min_ff
+inline float fmax_ref(float a, float b) { return (a > b) ? a : b; }  // max_ff
+
+inline void bb_reset(float *bb) {
+  bb[0] = bb[1] = bb[2] = FLT_MAX;
+  bb[3] = bb[4] = bb[5] = -FLT_MAX;
+}
+inline void bb_expand(float *bb, const float *co) {
+  for (int i = 0; i < 3; i++) {
+    bb[i] = fmin_ref(bb[i], co[i]);
+    bb[3 + i] = fmax_ref(bb[3 + i], co[i]);
+  }
+}
+inline void bb_expand_bb(float *bb, const float *b2) {
+  for (int i = 0; i < 3; i++) {
+    bb[i] = fmin_ref(bb[i], b2[i]);
+    bb[3 + i] = fmax_ref(bb[3 + i], b2[3 + i]);
+  }
+}
+inline int bb_widest_axis(const float *bb) {  // BB_widest_axis pbvh.c:78-96
+  float d[3];
+  for (int i = 0; i < 3; i++) d[i] = bb[3 + i] - bb[i];
+  if (d[0] > d[1]) return (d[0] > d[2]) ? 0 : 2;
+  return (d[1] > d[2]) ? 1 : 2;
+}
+
+template <class F>
+void parallel_for(int n, F f) {
+  int nt = (int)std::thread::hardware_concurrency();
+  if (nt > 16) nt = 16;
+  if (nt < 1 || n < 4096) {
+    for (int i = 0; i < n; i++) f(i);
+    return;
+  }
+  std::vector<std::thread> th;
+  const int chunk = (n + nt - 1) / nt;
+  for (int t = 0; t < nt; t++) {
+    const int b = t * chunk, e = std::min(n, b + chunk);
+    if (b >= e) break;
+    th.emplace_back([=, &f]() {
+      for (int i = b; i < e; i++) f(i);
+    });
+  }
+  for (auto &x : th) x.join();
+}
+
+struct Builder {
+  PBVH *p;
+  const float *co;
+  const int *tris;
+  std::vector<BBC> bbc;
+  std::vector<uint8_t> vert_bitmap;
+  std::vector<int> map_local;
+  std::vector<int> touched;
+
+  bool mat_match(int t1, int t2) const {
+    if (p->tri_mat.empty()) return true;
+    return p->tri_mat[t1] == p->tri_mat[t2];
+  }
+
+  void update_vb(PBVHNode &n, int offset, int count) {  // pbvh.c:322-329
+    bb_reset(n.vb);
+    for (int i = offset + count - 1; i >= offset; i--) {
+      bb_expand_bb(n.vb, bbc[p->prim_indices[i]].bmin);
+    }
+    memcpy(n.orig_vb, n.vb, sizeof(n.vb));
+  }
+
+  int partition(int lo, int hi, int axis, float mid) {  // pbvh.c:152-170
+    int *prim = p->prim_indices.data();
+    int i = lo, j = hi;
+    for (;;) {
+      while (bbc[prim[i]].bcentroid[axis] < mid) i++;
+      while (mid < bbc[prim[j]].bcentroid[axis]) j--;
+      if (!(i < j)) return i;
+      std::swap(prim[i], prim[j]);
+      i++;
+    }
+  }
+
+  int partition_material(int lo, int hi) {  // pbvh.c:172-212
+    int *ind = p->prim_indices.data();
+    const int first = ind[lo];
+    int i = lo, j = hi;
+    for (;;) {
+      while (mat_match(first, ind[i])) i++;
+      while (!mat_match(first, ind[j])) j--;
+      if (!(i < j)) return i;
+      std::swap(ind[i], ind[j]);
+      i++;
+    }
+  }
+
+  bool needs_material_split(int offset, int count) const {  // pbvh.c:409-440
+    if (count <= 1 || p->tri_mat.empty()) return false;
+    const int first = p->prim_indices[offset];
+    for (int i = offset + count - 1; i > offset; i--) {
+      if (!mat_match(first, p->prim_indices[i])) return true;
+    }
+    return false;
+  }
+
+  // build_mesh_leaf_node pbvh.c:256-320 with map_insert_vert :230-254.  The GHash
+  // value is the insertion counter, so vert_indices is in first-insertion order.
+  void build_leaf(int ni, int offset, int count) {
+    PBVHNode &n = p->nodes[ni];
+    n.flag |= PBVH_Leaf;
+    n.children_offset = -1;
+    n.prim_off = offset;
+    n.totprim = count;
+    update_vb(n, offset, count);
+    n.uniq_verts = n.face_verts = 0;
+    n.face_vert_indices = (int(*)[3])malloc(sizeof(int[3]) * (size_t)std::max(count, 1));
+    touched.clear();
+    bool has_visible = !p->respect_hide;
+    for (int i = 0; i < count; i++) {
+      const int t = p->prim_indices[offset + i];
+      bool hidden = false;
+      for (int j = 0; j < 3; j++) {
+        const int v = tris[3 * t + j];
+        int val = map_local[v];
+        if (val == INT32_MIN) {
+          if (!vert_bitmap[v]) {
+            vert_bitmap[v] = 1;
+            val = n.uniq_verts++;
+          } else {
+            val = ~n.face_verts;
+            n.face_verts++;
+          }
+          map_local[v] = val;
+          touched.push_back(v);
+        }
+        n.face_vert_indices[i][j] = val;
+        if (!p->vhide.empty() && p->vhide[v]) hidden = true;
+      }
+      if (!hidden) has_visible = true;
+    }
+    const int tot = n.uniq_verts + n.face_verts;
+    n.vert_indices = (int *)calloc((size_t)std::max(tot, 1), sizeof(int));
+    for (int v : touched) {
+      int ndx = map_local[v];
+      if (ndx < 0) ndx = -ndx + n.uniq_verts - 1;
+      n.vert_indices[ndx] = v;
+      map_local[v] = INT32_MIN;
+    }
+    for (int i = 0; i < count; i++) {
+      for (int j = 0; j < 3; j++) {
+        if (n.face_vert_indices[i][j] < 0) {
+          n.face_vert_indices[i][j] = -n.face_vert_indices[i][j] + n.uniq_verts - 1;
+        }
+      }
+    }
+    n.flag |= PBVH_RebuildDrawBuffers | PBVH_UpdateDrawBuffers | PBVH_UpdateRedraw;
+    if (!has_visible) n.flag |= PBVH_FullyHidden;
+  }
+
+  void new_node() {
+    PBVHNode n;
+    memset(&n, 0, sizeof(n));
+    n.children_offset = -1;
+    n.leaf = -1;
+    n.owner = p;
+    p->nodes.push_back(n);
+  }
+
+  void build(int T) {
+    struct Task {
+      int node, offset, count;
+      bool root;
+    };
+    p->prim_indices.resize(T);
+    for (int i = 0; i < T; i++) p->prim_indices[i] = i;
+    // root centroid bounds, ascending prim order (pbvh.c:572-586)
+    float cb_root[6];
+    bb_reset(cb_root);
+    for (int i = 0; i < T; i++) bb_expand(cb_root, bbc[i].bcentroid);
+
+    p->nodes.clear();
+    p->nodes.reserve((size_t)std::max(2 * (T / std::max(p->leaf_limit / 2, 1)) + 16, 16));
+    new_node();
+    std::vector<Task> stack;
+    stack.push_back({0, 0, T, true});
+    while (!stack.empty()) {
+      Task tk = stack.back();
+      stack.pop_back();
+      const bool below = tk.count <= p->leaf_limit;
+      if (below && !needs_material_split(tk.offset, tk.count)) {
+        build_leaf(tk.node, tk.offset, tk.count);
+        continue;
+      }
+      const int c = (int)p->nodes.size();
+      new_node();
+      new_node();
+      p->nodes[tk.node].children_offset = c;
+      update_vb(p->nodes[tk.node], tk.offset, tk.count);
+      int end;
+      if (!below) {
+        float cbb[6];
+        const float *cb = cb_root;
+        if (!tk.root) {
+          bb_reset(cbb);
+          for (int i = tk.offset + tk.count - 1; i >= tk.offset; i--) {
+            bb_expand(cbb, bbc[p->prim_indices[i]].bcentroid);
+          }
+          cb = cbb;
+        }
+        const int axis = bb_widest_axis(cb);
+        end = partition(tk.offset, tk.offset + tk.count - 1, axis,
+                        (cb[3 + axis] + cb[axis]) * 0.5f);
+      } else {
+        end = partition_material(tk.offset, tk.offset + tk.count - 1);
+      }
+      // right pushed first so the left subtree is built (and numbered) first
+      stack.push_back({c + 1, end, tk.offset + tk.count - end, false});
+      stack.push_back({c, tk.offset, end - tk.offset, false});
+    }
+    for (size_t i = 0; i < p->nodes.size(); i++) p->nodes[i].index = (int)i;
+  }
+};
+
+}  // namespace
+
+void sg_build_tree(PBVH *p, const float *co, const int *tris, int T, int V) {
+  Builder b;
+  b.p = p;
+  b.co = co;
+  b.tris = tris;
+  b.bbc.resize((size_t)T);
+  parallel_for(T, [&](int i) {  // per-prim BB + centroid (pbvh.c:572-586, :98-103)
+    BBC &x = b.bbc[i];
+    bb_reset(x.bmin);
+    for (int j = 0; j < 3; j++) bb_expand(x.bmin, co + 3 * (size_t)tris[3 * (size_t)i + j]);
+    for (int k = 0; k < 3; k++) x.bcentroid[k] = (x.bmin[k] + x.bmax[k]) * 0.5f;
+  });
+  b.vert_bitmap.assign((size_t)V, 0);
+  b.map_local.assign((size_t)V, INT32_MIN);
+  if (T > 0) b.build(T);
+
+  // DFS leaf order (pbvh_iter_next pushes right then left: leaves come out left->right)
+  p->leaf_node.clear();
+  if (!p->nodes.empty()) {
+    std::vector<int> st{0};
+    while (!st.empty()) {
+      const int ni = st.back();
+      st.pop_back();
+      PBVHNode &n = p->nodes[ni];
+      if (n.flag & PBVH_Leaf) {
+        n.leaf = (int)p->leaf_node.size();
+        p->leaf_node.push_back(ni);
+      } else {
+        st.push_back(n.children_offset + 1);
+        st.push_back(n.children_offset);
+      }
+    }
+  }
+  // vertex renumbering: uniq verts leaf by leaf, then unreferenced verts
+  p->perm_new_of_old.assign((size_t)V, -1);
+  p->perm_old_of_new.clear();
+  p->perm_old_of_new.reserve((size_t)V);
+  for (int ni : p->leaf_node) {
+    const PBVHNode &n = p->nodes[ni];
+    for (int k = 0; k < n.uniq_verts; k++) {
+      const int v = n.vert_indices[k];
+      p->perm_new_of_old[v] = (int)p->perm_old_of_new.size();
+      p->perm_old_of_new.push_back(v);
+    }
+  }
+  for (int v = 0; v < V; v++) {
+    if (p->perm_new_of_old[v] < 0) {
+      p->perm_new_of_old[v] = (int)p->perm_old_of_new.size();
+      p->perm_old_of_new.push_back(v);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Device layout.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+template <class T>
+int upload(T **dst, const T *src, size_t n, hipStream_t s) {
+  if (n == 0) n = 1;
+  if (sg_check_hip(hipMalloc((void **)dst, n * sizeof(T)), "hipMalloc")) return -1;
+  if (src) {
+    if (sg_check_hip(hipMemcpyAsync(*dst, src, n * sizeof(T), hipMemcpyHostToDevice, s),
+                     "hipMemcpyAsync"))
+      return -1;
+  }
+  return 0;
+}
+
+template <class T>
+int alloc_fill(T **dst, size_t n, int byte, hipStream_t s) {
+  if (n == 0) n = 1;
+  if (sg_check_hip(hipMalloc((void **)dst, n * sizeof(T)), "hipMalloc")) return -1;
+  return sg_check_hip(hipMemsetAsync(*dst, byte, n * sizeof(T), s), "hipMemsetAsync");
+}
+
+}  // namespace
+
+void sg_build_device(PBVH *p, const float *co) {
+  SgDev &d = p->d;
+  hipStream_t s = p->stream;
+  const int V = p->totvert, T = p->totprim;
+  const int L = (int)p->leaf_node.size();
+  const int N = (int)p->nodes.size();
+  d.V = V;
+  d.T = T;
+  d.L = L;
+  d.N = N;
+
+  // positions SoA in the new numbering
+  std::vector<float> x(V), y(V), z(V);
+  parallel_for(V, [&](int i) {
+    const int o = p->perm_old_of_new[i];
+    x[i] = co[3 * (size_t)o];
+    y[i] = co[3 * (size_t)o + 1];
+    z[i] = co[3 * (size_t)o + 2];
+  });
+  upload(&d.px, x.data(), V, s);
+  upload(&d.py, y.data(), V, s);
+  upload(&d.pz, z.data(), V, s);
+  alloc_fill(&d.nx, V, 0, s);
+  alloc_fill(&d.ny, V, 0, s);
+  alloc_fill(&d.nz, V, 0, s);
+
+  // triangles in prim_indices order
+  std::vector<int> tri_new_of_old(T);
+  for (int i = 0; i < T; i++) tri_new_of_old[p->prim_indices[i]] = i;
+  std::vector<int4> tri(T);
+  parallel_for(T, [&](int i) {
+    const int o = p->prim_indices[i];
+    tri[i] = make_int4(p->perm_new_of_old[p->tris[3 * (size_t)o]],
+                       p->perm_new_of_old[p->tris[3 * (size_t)o + 1]],
+                       p->perm_new_of_old[p->tris[3 * (size_t)o + 2]], o);
+  });
+  upload(&d.tri, tri.data(), T, s);
+
+  // vertex -> tri CSR (ascending original tri id), entries tri | corner << 30
+  std::vector<int> vt_off(V + 1, 0), vt_idx(3 * (size_t)T);
+  for (size_t k = 0; k < 3 * (size_t)T; k++) vt_off[p->perm_new_of_old[p->tris[k]] + 1]++;
+  for (int v = 0; v < V; v++) vt_off[v + 1] += vt_off[v];
+  {
+    std::vector<int> fill(vt_off.begin(), vt_off.end() - 1);
+    for (int t = 0; t < T; t++) {
+      for (int j = 0; j < 3; j++) {
+        const int v = p->perm_new_of_old[p->tris[3 * (size_t)t + j]];
+        vt_idx[fill[v]++] = tri_new_of_old[t] | (j << 30);
+      }
+    }
+  }
+  // vertex -> vertex CSR: distinct edge neighbours, ascending original id
+  std::vector<int> vv_cnt(V, 0);
+  std::vector<std::vector<int>> tmp_tls;
+  auto neighbours = [&](int v, std::vector<int> &buf) {
+    buf.clear();
+    for (int q = vt_off[v]; q < vt_off[v + 1]; q++) {
+      const int t = vt_idx[q] & 0x3fffffff;
+      const int4 c = tri[t];
+      const int cs[3] = {c.x, c.y, c.z};
+      for (int j = 0; j < 3; j++) {
+        if (cs[j] != v) buf.push_back(p->perm_old_of_new[cs[j]]);
+      }
+    }
+    std::sort(buf.begin(), buf.end());
+    buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
+  };
+  {
+    const int nt = 16;
+    std::vector<std::thread> th;
+    const int chunk = (V + nt - 1) / nt;
+    for (int t = 0; t < nt; t++) {
+      th.emplace_back([&, t]() {
+        std::vector<int> buf;
+        for (int v = t * chunk; v < std::min(V, (t + 1) * chunk); v++) {
+          neighbours(v, buf);
+          vv_cnt[v] = (int)buf.size();
+        }
+      });
+    }
+    for (auto &h : th) h.join();
+  }
+  std::vector<int> vv_off(V + 1, 0);
+  for (int v = 0; v < V; v++) vv_off[v + 1] = vv_off[v] + vv_cnt[v];
+  std::vector<int> vv_idx((size_t)vv_off[V] + 1);
+  {
+    const int nt = 16;
+    std::vector<std::thread> th;
+    const int chunk = (V + nt - 1) / nt;
+    for (int t = 0; t < nt; t++) {
+      th.emplace_back([&, t]() {
+        std::vector<int> buf;
+        for (int v = t * chunk; v < std::min(V, (t + 1) * chunk); v++) {
+          neighbours(v, buf);
+          for (size_t k = 0; k < buf.size(); k++) vv_idx[vv_off[v] + k] = p->perm_new_of_old[buf[k]];
+        }
+      });
+    }
+    for (auto &h : th) h.join();
+  }
+  upload(&d.vt_off, vt_off.data(), V + 1, s);
+  upload(&d.vt_idx, vt_idx.data(), 3 * (size_t)T, s);
+  upload(&d.vv_off, vv_off.data(), V + 1, s);
+  upload(&d.vv_idx, vv_idx.data(), (size_t)vv_off[V] + 1, s);
+
+  // leaves
+  std::vector<int> leaf_vbeg(L + 1, 0), leaf_sh_off(L + 1, 0), leaf_flags(L, 0);
+  std::vector<int> leaf_sh;
+  std::vector<float> leaf_bb((size_t)L * 8, 0.0f), leaf_obb((size_t)L * 8, 0.0f);
+  for (int l = 0; l < L; l++) {
+    const PBVHNode &n = p->nodes[p->leaf_node[l]];
+    leaf_vbeg[l + 1] = leaf_vbeg[l] + n.uniq_verts;
+    for (int k = 0; k < n.face_verts; k++) leaf_sh.push_back(p->perm_new_of_old[n.vert_indices[n.uniq_verts + k]]);
+    leaf_sh_off[l + 1] = (int)leaf_sh.size();
+    leaf_flags[l] = (n.flag & PBVH_FullyHidden) ? 1 : 0;
+    for (int k = 0; k < 3; k++) {
+      leaf_bb[8 * (size_t)l + k] = n.vb[k];
+      leaf_bb[8 * (size_t)l + 4 + k] = n.vb[3 + k];
+      leaf_obb[8 * (size_t)l + k] = n.orig_vb[k];
+      leaf_obb[8 * (size_t)l + 4 + k] = n.orig_vb[3 + k];
+    }
+  }
+  upload(&d.leaf_vbeg, leaf_vbeg.data(), L + 1, s);
+  upload(&d.leaf_sh_off, leaf_sh_off.data(), L + 1, s);
+  upload(&d.leaf_sh, leaf_sh.data(), leaf_sh.size(), s);
+  upload(&d.leaf_flags, leaf_flags.data(), L, s);
+  upload(&d.leaf_bb, leaf_bb.data(), (size_t)L * 8, s);
+  upload(&d.leaf_obb, leaf_obb.data(), (size_t)L * 8, s);
+
+  // nodes
+  std::vector<float> node_bb((size_t)N * 8, 0.0f), node_obb((size_t)N * 8, 0.0f);
+  for (int i = 0; i < N; i++) {
+    const PBVHNode &n = p->nodes[i];
+    for (int k = 0; k < 3; k++) {
+      node_bb[8 * (size_t)i + k] = n.vb[k];
+      node_bb[8 * (size_t)i + 4 + k] = n.vb[3 + k];
+      node_obb[8 * (size_t)i + k] = n.orig_vb[k];
+      node_obb[8 * (size_t)i + 4 + k] = n.orig_vb[3 + k];
+    }
+  }
+  upload(&d.node_bb, node_bb.data(), (size_t)N * 8, s);
+  upload(&d.node_obb, node_obb.data(), (size_t)N * 8, s);
+
+  // leaf ranges and heights per node (children have larger indices than parents)
+  std::vector<int> lbeg(N, 0), lend(N, 0), height(N, 0);
+  for (int i = N - 1; i >= 0; i--) {
+    const PBVHNode &n = p->nodes[i];
+    if (n.flag & PBVH_Leaf) {
+      lbeg[i] = n.leaf;
+      lend[i] = n.leaf + 1;
+      height[i] = 0;
+    } else {
+      const int c = n.children_offset;
+      lbeg[i] = lbeg[c];
+      lend[i] = lend[c + 1];
+      height[i] = 1 + std::max(height[c], height[c + 1]);
+    }
+  }
+  // cut: maximal subtrees with <= K leaves, K the smallest power of two >= 64 that
+  // keeps the cut at <= 1024 subtrees (search cost C + passing*K; LDS tables)
+  int K = 64;
+  std::vector<int> cut_nodes;
+  for (;;) {
+    cut_nodes.clear();
+    if (N > 0) {
+      std::vector<int> st{0};
+      while (!st.empty()) {
+        const int ni = st.back();
+        st.pop_back();
+        if (lend[ni] - lbeg[ni] <= K) {
+          cut_nodes.push_back(ni);
+        } else {
+          st.push_back(p->nodes[ni].children_offset + 1);
+          st.push_back(p->nodes[ni].children_offset);
+        }
+      }
+    }
+    if ((int)cut_nodes.size() <= 1024 || K >= 1024) break;
+    K *= 2;
+  }
+  const int C = (int)cut_nodes.size();
+  d.C = C;
+  p->cut.node = cut_nodes;
+  p->cut.lbeg.resize(C);
+  p->cut.lend.resize(C);
+  std::vector<int> leaf_cut(L, 0);
+  for (int c = 0; c < C; c++) {
+    p->cut.lbeg[c] = lbeg[cut_nodes[c]];
+    p->cut.lend[c] = lend[cut_nodes[c]];
+    for (int l = p->cut.lbeg[c]; l < p->cut.lend[c]; l++) leaf_cut[l] = c;
+  }
+  upload(&d.cut_node, cut_nodes.data(), C, s);
+  upload(&d.cut_lbeg, p->cut.lbeg.data(), C, s);
+  upload(&d.cut_lend, p->cut.lend.data(), C, s);
+  upload(&d.leaf_cut, leaf_cut.data(), L, s);
+
+  // deferred refit tables.  Local numbering inside a cut: its leaves 0..nl-1 in DFS
+  // order, then its internal nodes by ascending height.
+  std::vector<int> sub_off(C + 1, 0);
+  std::vector<int4> sub_tab;
+  int sub_maxn = 1;
+  {
+    std::vector<int> local(N, -1);
+    for (int c = 0; c < C; c++) {
+      const int root = cut_nodes[c];
+      std::vector<int> inner;
+      std::vector<int> st{root};
+      while (!st.empty()) {
+        const int ni = st.back();
+        st.pop_back();
+        const PBVHNode &n = p->nodes[ni];
+        if (n.flag & PBVH_Leaf) {
+          local[ni] = n.leaf - p->cut.lbeg[c];
+        } else {
+          inner.push_back(ni);
+          st.push_back(n.children_offset);
+          st.push_back(n.children_offset + 1);
+        }
+      }
+      std::stable_sort(inner.begin(), inner.end(), [&](int a, int b) { return height[a] < height[b]; });
+      const int nl = p->cut.lend[c] - p->cut.lbeg[c];
+      for (size_t k = 0; k < inner.size(); k++) local[inner[k]] = nl + (int)k;
+      for (int ni : inner) {
+        const int ch = p->nodes[ni].children_offset;
+        sub_tab.push_back(make_int4(ni, local[ch], local[ch + 1], height[ni]));
+      }
+      sub_off[c + 1] = (int)sub_tab.size();
+      sub_maxn = std::max(sub_maxn, nl + (int)inner.size());
+    }
+    // top tree: strict ancestors of cut roots; local 0..C-1 = cut roots
+    std::vector<int> top_local(N, -1);
+    for (int c = 0; c < C; c++) top_local[cut_nodes[c]] = c;
+    std::vector<int> top;
+    if (N > 0 && top_local[0] < 0) {
+      std::vector<int> st{0};
+      while (!st.empty()) {
+        const int ni = st.back();
+        st.pop_back();
+        if (top_local[ni] >= 0) continue;
+        top.push_back(ni);
+        st.push_back(p->nodes[ni].children_offset);
+        st.push_back(p->nodes[ni].children_offset + 1);
+      }
+    }
+    // height within the top tree (cut roots = 0)
+    std::vector<int> th(N, 0);
+    std::vector<int> order = top;
+    std::sort(order.begin(), order.end(), [](int a, int b) { return a > b; });  // children first
+    for (int ni : order) {
+      const int ch = p->nodes[ni].children_offset;
+      const int h0 = top_local[ch] >= 0 && top_local[ch] < C ? 0 : th[ch];
+      const int h1 = top_local[ch + 1] >= 0 && top_local[ch + 1] < C ? 0 : th[ch + 1];
+      th[ni] = 1 + std::max(h0, h1);
+    }
+    std::stable_sort(top.begin(), top.end(), [&](int a, int b) { return th[a] < th[b]; });
+    for (size_t k = 0; k < top.size(); k++) top_local[top[k]] = C + (int)k;
+    std::vector<int4> top_tab;
+    int maxh = 0;
+    for (int ni : top) {
+      const int ch = p->nodes[ni].children_offset;
+      top_tab.push_back(make_int4(ni, top_local[ch], top_local[ch + 1], th[ni]));
+      maxh = std::max(maxh, th[ni]);
+    }
+    d.top_n = (int)top.size();
+    d.top_maxh = maxh;
+    d.sub_maxn = sub_maxn;
+    upload(&d.sub_off, sub_off.data(), C + 1, s);
+    upload(&d.sub_tab, sub_tab.data(), sub_tab.size(), s);
+    upload(&d.top_tab, top_tab.data(), top_tab.size(), s);
+  }
+  alloc_fill(&d.cut_dirty, C, 0, s);
+  alloc_fill(&d.prev_cuts, C, 0, s);
+
+  if (!p->vhide.empty()) {
+    std::vector<int> vh(V);
+    for (int i = 0; i < V; i++) vh[i] = p->vhide[p->perm_old_of_new[i]];
+    upload(&d.vhide, vh.data(), V, s);
+  }
+  alloc_fill(&d.vmark, V, 0, s);
+  // scratch
+  alloc_fill(&d.gathered, L, 0, s);
+  alloc_fill(&d.affected, V, 0, s);
+  alloc_fill(&d.aff_fade, V, 0, s);
+  alloc_fill(&d.slot, V, 0xff, s);
+  alloc_fill(&d.sbuf[0], 3 * (size_t)V, 0, s);
+  alloc_fill(&d.sbuf[1], 3 * (size_t)V, 0, s);
+  alloc_fill(&d.ctr, SG_CTR_COUNT, 0, s);
+  alloc_fill(&d.err, 4, 0, s);
+  upload(&d.vperm_old, p->perm_old_of_new.data(), V, s);
+  hipStreamSynchronize(s);
+}
+
+void sg_free_device(PBVH *p) {
+  SgDev &d = p->d;
+  void *ptrs[] = {d.px, d.py, d.pz, d.nx, d.ny, d.nz, d.mask, d.tri, d.vt_off, d.vt_idx,
+                  d.vv_off, d.vv_idx, d.leaf_vbeg, d.leaf_sh_off, d.leaf_sh, d.leaf_flags,
+                  d.leaf_cut, d.leaf_bb, d.leaf_obb, d.node_bb, d.node_obb, d.cut_node,
+                  d.cut_lbeg, d.cut_lend, d.sub_off, d.sub_tab, d.top_tab_n, d.top_tab,
+                  d.cut_dirty, d.vhide, d.vmark, d.gathered, d.affected, d.aff_fade, d.slot,
+                  d.sbuf[0], d.sbuf[1], d.prev_cuts, d.ctr, d.err, d.vperm_old};
+  for (void *q : ptrs) {
+    if (q) hipFree(q);
+  }
+  d = SgDev();
+}

@@ -1,0 +1,142 @@
+// Internal structures of the sculpt engine (host side + device buffer handles).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/sculpt_gpu.h"
+
+// Cut of the tree used by the two-level sphere search: maximal subtrees holding at
+// most cut_leaf_cap leaves.  Leaves are numbered in DFS (left->right) order, so every
+// cut subtree owns a contiguous leaf range.
+struct SgCut {
+  std::vector<int> node;       // node id of each cut subtree root (DFS order)
+  std::vector<int> lbeg, lend; // leaf range [lbeg, lend)
+};
+
+// Host mirror of one tree node (the PBVHNode handle handed to callers).
+struct PBVHNode {
+  float vb[6];      // bmin xyz, bmax xyz
+  float orig_vb[6];
+  int children_offset;  // -1 for leaves
+  int flag;
+  int index;            // slot in nodes[]
+  int leaf;             // DFS leaf number or -1
+  int prim_off, totprim;
+  int uniq_verts, face_verts;
+  int *vert_indices;    // original vertex ids: uniq first, then shared
+  int (*face_vert_indices)[3];
+  PBVHNode *self_check;
+  int proxy_count;
+  PBVHProxyNode *proxies;
+  struct PBVH *owner;
+};
+
+// Device-side mesh and tree (all HBM, SoA, leaf-ordered vertex numbering).
+struct SgDev {
+  int V = 0, T = 0, L = 0, N = 0, C = 0;
+  float *px = nullptr, *py = nullptr, *pz = nullptr; // positions [V]
+  float *nx = nullptr, *ny = nullptr, *nz = nullptr; // normals [V]
+  float *mask = nullptr;                             // [V] or null
+  int4 *tri = nullptr;          // [T] corners (new vertex ids), .w = original tri id
+  int *vt_off = nullptr, *vt_idx = nullptr; // vertex -> (tri | corner << 30), asc. orig tri
+  int *vv_off = nullptr, *vv_idx = nullptr; // vertex -> edge neighbours, asc. orig id
+  int *leaf_vbeg = nullptr;     // [L+1] uniq vertex range of each leaf
+  int *leaf_sh_off = nullptr;   // [L+1]
+  int *leaf_sh = nullptr;       // shared (face) vertices of each leaf
+  int *leaf_flags = nullptr;    // [L] bit0 fully hidden
+  int *leaf_cut = nullptr;      // [L] cut index of each leaf
+  float *leaf_bb = nullptr;     // [L*8] vb   (min xyz, 0, max xyz, 0)
+  float *leaf_obb = nullptr;    // [L*8] orig_vb
+  float *node_bb = nullptr;     // [N*8] internal nodes (and cut leaves)
+  float *node_obb = nullptr;
+  int *cut_node = nullptr, *cut_lbeg = nullptr, *cut_lend = nullptr; // [C]
+  // deferred refit tables: per cut, its internal nodes by height (int4: node id,
+  // local child 0, local child 1, height) and the top tree above the cut
+  int *sub_off = nullptr;       // [C+1]
+  int4 *sub_tab = nullptr;
+  int *top_tab_n = nullptr;
+  int4 *top_tab = nullptr;
+  int top_n = 0, top_maxh = 0, sub_maxn = 0;
+  int *cut_dirty = nullptr;     // [C]
+  int *vhide = nullptr;         // [V] or null
+  uint8_t *vmark = nullptr;     // [V] host-driven normal update marks
+  // per-dab scratch
+  int *gathered = nullptr;      // [L] gathered leaves (ascending)
+  int *affected = nullptr;      // [V]
+  float *aff_fade = nullptr;    // [V]
+  int *slot = nullptr;          // [V] -1 unless affected by the running Smooth dab
+  float *sbuf[2] = {nullptr, nullptr}; // [3V] each, compact Jacobi iterates (SoA x|y|z)
+  int *prev_cuts = nullptr;     // [C] cuts refit by the previous dab (+ count in ctr)
+  long long *ctr = nullptr;     // counters, see SG_CTR_*
+  int *err = nullptr;           // device error word
+  int *vperm_old = nullptr;     // [V] new -> old id (device copy)
+  int *stamp_v = nullptr, *stamp_v2 = nullptr, *stamp_t = nullptr; // stats mode only
+  float *curve = nullptr;       // custom falloff tables for queued dabs
+  int curve_cap = 0;
+  long long *log = nullptr;     // per-dab counter log for strokes
+  int log_cap = 0;
+};
+
+enum {
+  SG_CTR_NGATHER = 0,
+  SG_CTR_NAFF = 1,
+  SG_CTR_AREA_X = 2,
+  SG_CTR_AREA_Y = 3,
+  SG_CTR_AREA_Z = 4,
+  SG_CTR_NPREV = 5,
+  SG_CTR_TESTS = 6,
+  SG_CTR_NCAND = 7,
+  SG_CTR_NAREA = 8,
+  SG_CTR_NBR = 9,
+  SG_CTR_TRIS = 10,
+  SG_CTR_COUNT = 16,
+};
+
+struct PBVH {
+  PBVHType type = PBVH_FACES;
+  int leaf_limit = 10000;
+  bool respect_hide = true;
+  bool no_device = false;            // no HIP device: host tree only
+  bool build_ok = false;             // device layout built and within kernel limits
+  int totvert = 0, totprim = 0;
+  std::vector<PBVHNode> nodes;
+  std::vector<int> prim_indices;
+  std::vector<int> leaf_node;        // DFS leaf -> node
+  std::vector<int> perm_new_of_old;  // vertex renumbering
+  std::vector<int> perm_old_of_new;
+  std::vector<int> tris;             // original corner vertex ids [T*3]
+  std::vector<int> tri_mat;          // material key per tri (empty = one material)
+  std::vector<uint8_t> vhide;
+  SgCut cut;
+  MVert *host_verts = nullptr;       // caller's array (borrowed; duplicated on deform)
+  bool verts_owned = false;
+  MLoopTri *looptri = nullptr;       // owned (freed at BKE_pbvh_free)
+  std::vector<float> host_normals;   // [V*3] original order view
+  std::vector<uint8_t> host_vmark;   // BKE_pbvh_vert_mark_update marks
+  bool host_bb_stale = false;
+  bool host_verts_stale = false;
+  bool host_normals_stale = false;
+  bool nodes_dirty_deferred = false; // internal-node refit pending on the device
+  hipStream_t stream = nullptr;
+  SgDev d;
+  // last dab
+  int last_tool = 0;
+  int64_t last_stats[8] = {0};
+  long long last_ctr[16] = {0};
+  int dab_count = 0;
+  int grid_wg = 256;                 // workgroups of the per-dab kernels (tunable)
+  int stamp = 0;                     // stats-mode stamp generator
+};
+
+// host build (pbvh_build.cpp)
+void sg_build_tree(PBVH *p, const float *co, const int *tris, int T, int V);
+void sg_build_device(PBVH *p, const float *co);
+void sg_free_device(PBVH *p);
+
+// errors (kernels.hip)
+int sg_set_error(const char *fmt, ...);
+int sg_check_hip(hipError_t e, const char *what);

@@ -1,0 +1,57 @@
+// Kernel-side views and launch entry points shared by the host API and the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "sg_internal.h"
+
+enum {
+  SG_CTR_R1X = 11,    // distinct non-affected 1-ring vertices read (stats mode)
+  SG_CTR_FA = 12,     // distinct tris incident to the affected set (stats mode)
+  SG_CTR_VFA = 13,    // distinct vertices of those tris (stats mode)
+  SG_CTR_SHARED = 14, // shared vertices read by the leaf refit
+};
+
+// Plain-pointer view of the device mesh passed to kernels by value.
+struct SgDevView {
+  int V, T, L, N, C, respect_hide;
+  float *px, *py, *pz, *nx, *ny, *nz, *mask;
+  const int4 *tri;
+  const int *vt_off, *vt_idx, *vv_off, *vv_idx;
+  const int *leaf_vbeg, *leaf_sh_off, *leaf_sh, *leaf_flags, *leaf_cut;
+  float *leaf_bb, *leaf_obb, *node_bb, *node_obb;
+  const int *cut_node, *cut_lbeg, *cut_lend, *sub_off;
+  const int4 *sub_tab, *top_tab;
+  int *cut_dirty;
+  const int *vhide;
+  uint8_t *vmark;
+  int *gathered, *affected;
+  float *aff_fade;
+  int *slot;
+  float *sbuf0, *sbuf1;
+  int *prev_cuts;
+  long long *ctr;
+  const int *vperm_old;
+  int *stamp_v, *stamp_v2, *stamp_t;
+};
+
+// Per-dab constants (spec S1-S9), computed on the host in the same FP32 order as the
+// CPU statement of the spec.
+struct SgDabK {
+  float cx, cy, cz, r, r2, nr2, bstr, hardness;
+  float gx, gy, gz;
+  int tool, falloff, iters, use_orig, stats;
+};
+
+SgDevView sg_view(PBVH *p);
+SgDabK sg_make_dab(const SgDabParams *dab, int stats);
+int sg_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long long *log, int log_index,
+                   int stamp);
+int sg_enqueue_flush(PBVH *p);
+int sg_enqueue_full_normals(PBVH *p, int mode);
+int sg_enqueue_refit_list(PBVH *p, const int *leaves_dev, int n);
+int sg_enqueue_normals_list(PBVH *p, const int *leaves_dev, int n);
+int sg_enqueue_coords_apply(PBVH *p, const float *src_dev);
+int sg_enqueue_clear_marks(PBVH *p);
+int sg_enqueue_gather(PBVH *p, int which, float *out_dev);
+const char *sg_last_error_str();

@@ -1,0 +1,251 @@
+/*
+ * sculpt_gpu.h -- C ABI of the MI355X sculpt brush engine (libsculpt_gpu.so).
+ *
+ * Two groups of entry points:
+ *
+ *  1. Drop-in replacements for the reference's PBVH C API (PBVH_FACES only).  The
+ *     reference's public header (BKE_pbvh.h / DUNE_pbvh.h / KERNEL_pbvh.h) is absent,
+ *     so each signature is taken from its definition in kernel/intern/pbvh.c and its
+ *     call sites (paint.c:1676-1771, :2084-2191; editors/sculpt_paint/paint_hide.c).
+ *     Each declaration cites the reference function it replaces.  Mesh data stays in
+ *     HBM (SoA, leaf-ordered); host views are synchronised lazily.
+ *
+ *  2. New sg_* entry points replacing the reference's absent brush action
+ *     (do_brush_action / SCULPT_OT_brush_stroke, named at paint_curve.c:671):
+ *     one dab or a whole stroke of dabs on the GPU, plus batched multi-mesh steps.
+ *     These return int status codes (0 = ok, < 0 = error, text in sg_last_error()).
+ *
+ * Struct layouts of MVert/MPoly/MLoop/MLoopTri match types/types_meshdata.h:13-17,
+ * :50-57, :69-74, :178-181 so reference mesh arrays can be passed as they are.
+ * Threading: calls from one host thread per PBVH; work is queued on the PBVH's own
+ * HIP stream and every drop-in call returns with the stream drained.
+ */
+#ifndef SCULPT_GPU_H
+#define SCULPT_GPU_H
+
+#include <stdbool.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- mesh data layouts (types/types_meshdata.h) ---------------------------- */
+typedef struct MVert { /* types_meshdata.h:13-17 */
+  float co[3];
+  char flag, bweight;
+  char _pad[2];
+} MVert;
+enum { ME_HIDE = (1 << 4) }; /* types_meshdata.h:20-24 */
+
+typedef struct MPoly { /* types_meshdata.h:50-57 */
+  int loopstart;
+  int totloop;
+  short mat_nr;
+  char flag, _pad;
+} MPoly;
+enum { ME_SMOOTH = (1 << 0) }; /* types_meshdata.h:60-64 */
+
+typedef struct MLoop { /* types_meshdata.h:69-74 */
+  unsigned int v;
+  unsigned int e;
+} MLoop;
+
+typedef struct MLoopTri { /* types_meshdata.h:178-181 */
+  unsigned int tri[3];
+  unsigned int poly;
+} MLoopTri;
+
+/* Opaque reference types accepted (and ignored) by the drop-in signatures. */
+typedef struct Mesh Mesh;
+typedef struct CustomData CustomData;
+typedef struct SubdivCCG SubdivCCG;
+
+typedef struct PBVH PBVH;
+typedef struct PBVHNode PBVHNode;
+
+/* PBVHType / PBVHNodeFlags are used but never defined by the reference
+ * (pbvh.c:550, :114, :1003, :1215, :1725); the names follow those uses. */
+typedef enum { PBVH_FACES = 0, PBVH_GRIDS = 1, PBVH_BMESH = 2 } PBVHType;
+typedef enum {
+  PBVH_Leaf = 1 << 0,
+  PBVH_UpdateNormals = 1 << 1,
+  PBVH_UpdateBB = 1 << 2,
+  PBVH_UpdateOriginalBB = 1 << 3,
+  PBVH_UpdateDrawBuffers = 1 << 4,
+  PBVH_UpdateRedraw = 1 << 5,
+  PBVH_UpdateMask = 1 << 6,
+  PBVH_UpdateVisibility = 1 << 7,
+  PBVH_RebuildDrawBuffers = 1 << 8,
+  PBVH_FullyHidden = 1 << 9,
+  PBVH_FullyMasked = 1 << 10,
+  PBVH_FullyUnmasked = 1 << 11,
+  PBVH_UpdateColor = 1 << 12,
+} PBVHNodeFlags;
+
+typedef struct PBVHProxyNode { /* used at pbvh.c:4757-4789 */
+  float (*co)[3];
+} PBVHProxyNode;
+
+typedef struct PBVHFrustumPlanes { /* used at pbvh.c:4507-4557 */
+  float (*planes)[4];
+  int num_planes;
+} PBVHFrustumPlanes;
+
+typedef bool (*BKE_pbvh_SearchCallback)(PBVHNode *node, void *data); /* pbvh.c:769 */
+typedef void (*BKE_pbvh_HitCallback)(PBVHNode *node, void *data);    /* pbvh.c:864 */
+
+/* ---- 1. drop-in PBVH API ---------------------------------------------------- */
+
+/* pbvh.c:645 BKE_pbvh_new */
+PBVH *BKE_pbvh_new(void);
+/* pbvh.c:652 BKE_pbvh_free (also frees the looptri array it took over, pbvh.c:687) */
+void BKE_pbvh_free(PBVH *pbvh);
+/* pbvh.c:534 BKE_pbvh_build_mesh.  Builds the tree on the host exactly as the
+ * reference does, then uploads a leaf-ordered SoA copy of the mesh to HBM and
+ * computes the angle-weighted vertex normals on the GPU (mesh_normals.c:225-299).
+ * Takes ownership of looptri (freed with free() by BKE_pbvh_free, as pbvh.c:687). */
+void BKE_pbvh_build_mesh(PBVH *pbvh, Mesh *mesh, const MPoly *mpoly, const MLoop *mloop,
+                         MVert *verts, int totvert, CustomData *vdata, CustomData *ldata,
+                         CustomData *pdata, const MLoopTri *looptri, int looptri_num);
+/* pbvh.c:818 BKE_pbvh_search_gather: result array is malloc'ed, caller frees. */
+void BKE_pbvh_search_gather(PBVH *pbvh, BKE_pbvh_SearchCallback scb, void *search_data,
+                            PBVHNode ***r_array, int *r_tot);
+/* pbvh.c:851 BKE_pbvh_search_callback */
+void BKE_pbvh_search_callback(PBVH *pbvh, BKE_pbvh_SearchCallback scb, void *search_data,
+                              BKE_pbvh_HitCallback hcb, void *hit_data);
+/* pbvh.c:1401 BKE_pbvh_update_bounds: refits flagged leaves and their ancestors on
+ * the GPU (host-side edits made through BKE_pbvh_get_verts or proxies are uploaded
+ * for the flagged leaves first). */
+void BKE_pbvh_update_bounds(PBVH *pbvh, int flag);
+/* pbvh.c:4559 BKE_pbvh_update_normals: marked vertices of UpdateNormals leaves get
+ * normalize(sum of incident unit tri normals) -- deterministic CSR gather on GPU. */
+void BKE_pbvh_update_normals(PBVH *pbvh, SubdivCCG *subdiv_ccg);
+/* pbvh.c:4707 BKE_pbvh_vert_coords_apply */
+void BKE_pbvh_vert_coords_apply(PBVH *pbvh, const float (*vertCos)[3], int totvert);
+/* pbvh.c:4689 BKE_pbvh_vert_coords_alloc (malloc'ed, caller frees) */
+float (*BKE_pbvh_vert_coords_alloc(PBVH *pbvh))[3];
+/* pbvh.c:1723 / :1811 / :1753 marking */
+void BKE_pbvh_node_mark_update(PBVHNode *node);
+void BKE_pbvh_node_mark_normals_update(PBVHNode *node);
+void BKE_pbvh_node_mark_redraw(PBVHNode *node);
+void BKE_pbvh_vert_mark_update(PBVH *pbvh, int index);
+/* pbvh.c:3830-3840 node bounds (synchronised from HBM on first read after a change) */
+void BKE_pbvh_node_get_BB(PBVHNode *node, float bb_min[3], float bb_max[3]);
+void BKE_pbvh_node_get_original_BB(PBVHNode *node, float bb_min[3], float bb_max[3]);
+/* pbvh.c:1831 / :1817 node vertex access (original vertex indices; uniq first) */
+void BKE_pbvh_node_num_verts(PBVH *pbvh, PBVHNode *node, int *r_uniquevert, int *r_totvert);
+void BKE_pbvh_node_get_verts(PBVH *pbvh, PBVHNode *node, const int **r_vert_indices,
+                             MVert **r_verts);
+/* pbvh.c:1761 / :1766 */
+bool BKE_pbvh_node_fully_hidden_get(PBVHNode *node);
+/* pbvh.c:4757 / :4778 / :4791 proxies */
+PBVHProxyNode *BKE_pbvh_node_add_proxy(PBVH *pbvh, PBVHNode *node);
+void BKE_pbvh_node_free_proxies(PBVHNode *node);
+void BKE_pbvh_gather_proxies(PBVH *pbvh, PBVHNode ***r_array, int *r_tot);
+/* pbvh.c:4539-4557 frustum tests (search callbacks taking PBVHFrustumPlanes*) */
+bool BKE_pbvh_node_frustum_contain_AABB(PBVHNode *node, void *frustum);
+bool BKE_pbvh_node_frustum_exclude_AABB(PBVHNode *node, void *frustum);
+/* pbvh.c:4961 / :4967 host views, synchronised from HBM when called */
+MVert *BKE_pbvh_get_verts(const PBVH *pbvh);
+const float (*DUNE_pbvh_get_vert_normals(const PBVH *pbvh))[3];
+/* pbvh.c:4983 */
+void DUNE_pbvh_respect_hide_set(PBVH *pbvh, bool respect_hide);
+/* pbvh.c:4953 PBVHType accessor (BKE_pbvh_type is called throughout paint.c) */
+PBVHType BKE_pbvh_type(const PBVH *pbvh);
+
+/* ---- 2. brush engine entry points (new; replace the absent do_brush_action) ---- */
+
+/* Tools (types_brush_enums.h:410-415) and falloff presets (:176-187). */
+enum { SG_TOOL_DRAW = 1, SG_TOOL_SMOOTH = 2, SG_TOOL_INFLATE = 4, SG_TOOL_GRAB = 5 };
+enum {
+  SG_CURVE_CUSTOM = 0, SG_CURVE_SMOOTH = 1, SG_CURVE_SPHERE = 2, SG_CURVE_ROOT = 3,
+  SG_CURVE_SHARP = 4, SG_CURVE_LIN = 5, SG_CURVE_POW4 = 6, SG_CURVE_INVSQUARE = 7,
+  SG_CURVE_CONSTANT = 8, SG_CURVE_SMOOTHER = 9,
+};
+#define SG_BRUSH_DIR_IN (1 << 9)    /* BRUSH_DIR_IN, types_brush_enums.h:347 */
+#define SG_CM_TABLE 256             /* CM_TABLE, types_color.h:12 */
+
+/* One dab.  Field names mirror the Brush RNA properties (api_brush.c:2281-2880) and
+ * OperatorStrokeElement (api_brush.c:3526-3576) they come from. */
+typedef struct SgDabParams {
+  int sculpt_tool;            /* SG_TOOL_* (api_brush.c:2281) */
+  int curve_preset;           /* SG_CURVE_* (api_brush.c:2364) */
+  int flag;                   /* SG_BRUSH_DIR_IN */
+  int smooth_iterations;      /* Laplacian Smooth(k); 0 -> 1 */
+  float location[3];          /* dab centre (OperatorStrokeElement.location) */
+  float radius;               /* world-space radius (unprojected_radius, :2463) */
+  float strength;             /* Brush.alpha (:2536) */
+  float hardness;             /* (:2420) */
+  float normal_radius_factor; /* area-normal sphere for Draw (:2714 area), default 0.5 */
+  float grab_delta[3];        /* Grab translation for this dab */
+  const float *curve_table;   /* SG_CM_TABLE+1 samples over [0,1] for SG_CURVE_CUSTOM */
+} SgDabParams;
+
+/* Fills defaults (types_brush_defaults.h:8-90: SMOOTH falloff, hardness 0,
+ * normal_radius_factor 0.5, strength 0.5). */
+void sg_dab_params_default(SgDabParams *p, int tool);
+
+/* Set the leaf size (triangles per leaf) used by the next BKE_pbvh_build_mesh.
+ * The reference hard-codes LEAF_LIMIT 10000 (pbvh.c:32); 0 restores that. */
+int sg_pbvh_leaf_limit_set(PBVH *pbvh, int leaf_limit);
+
+/* Apply one dab: search -> falloff/strength -> displacement -> normals -> leaf and
+ * node bounds refit, all on the GPU, returning when done. */
+int sg_dab_apply(PBVH *pbvh, const SgDabParams *dab);
+/* Apply n dabs in order on the PBVH's stream without host synchronisation, then
+ * refit every dirty internal node and drain.  flags: SG_STROKE_GRAPH captures the
+ * whole stroke into one hipGraph; SG_STROKE_REFIT_EACH refits internal nodes after
+ * every dab (leaf and search-level bounds are always refit per dab);
+ * SG_STROKE_LOG keeps per-dab counters for sg_dab_stroke_log. */
+enum { SG_STROKE_GRAPH = 1, SG_STROKE_REFIT_EACH = 2, SG_STROKE_LOG = 4 };
+int sg_dab_apply_stroke(PBVH *pbvh, const SgDabParams *dabs, int n, int flags);
+/* Per-dab counters (16 per dab, layout of sg_dab_last_stats) of the last stroke. */
+int sg_dab_stroke_log(PBVH *pbvh, long long *out, int n);
+/* Block until queued work is done; returns the first recorded device error. */
+int sg_sync(PBVH *pbvh);
+const char *sg_last_error(void);
+
+/* Affected vertices (original indices, ascending) of the most recent dab. */
+int sg_dab_last_affected(PBVH *pbvh, int *out, int cap);
+/* Counters of the most recent dab:
+ * [0] gathered leaves [1] affected verts [2..4] area-normal sums (fixed point)
+ * [5] cut subtrees touched [6] boxes tested; with statistics on (sg_pbvh_set_stats):
+ * [7] candidate verts [8] area-normal verts [9] smooth neighbour reads (steps >= 2
+ * plus step 1) [10] normal tri reads [11] distinct non-affected 1-ring verts (last
+ * Jacobi steps) [12] distinct tris incident to A [13] distinct verts of those tris
+ * [14] shared verts read by the leaf refit. */
+int sg_dab_last_stats(PBVH *pbvh, int64_t out[16]);
+/* Gathered leaves (node indices, left->right) of the most recent dab. */
+int sg_dab_last_gathered(PBVH *pbvh, int *out, int cap);
+/* Exact byte-accounting counters in the brush kernels (adds atomics; off by default). */
+int sg_pbvh_set_stats(PBVH *pbvh, int on);
+/* Workgroups launched by the per-dab kernels (autotuned by the host layer). */
+int sg_pbvh_set_grid(PBVH *pbvh, int workgroups);
+
+/* Copies of the device state in original vertex order (AoS float[3]). */
+int sg_pbvh_read_positions(PBVH *pbvh, float *out);
+int sg_pbvh_read_normals(PBVH *pbvh, float *out);
+/* Per-vertex paint mask (CD_PAINT_MASK); NULL clears it. */
+int sg_pbvh_set_mask(PBVH *pbvh, const float *mask);
+/* Tree shape for inspection: node count, leaf count, cut count. */
+int sg_pbvh_tree_info(PBVH *pbvh, int out[4]);
+/* Node index of a handle (its slot in the reference's nodes array order). */
+int sg_pbvh_node_index(PBVH *pbvh, PBVHNode *node);
+PBVHNode *sg_pbvh_node_at(PBVH *pbvh, int index);
+/* prim_indices array (looptri order of the leaves) */
+int sg_pbvh_prim_indices(PBVH *pbvh, int *out, int cap);
+
+/* ---- synthetic meshes and strokes (deterministic; for tests and benchmarks) ---- */
+int sg_mesh_icosphere_size(int level, int *r_totvert, int *r_tottri);
+int sg_mesh_icosphere(int level, float radius, float *co, int *tris);
+int sg_mesh_torus(int nu, int nv, float major_r, float minor_r, float *co, int *tris);
+/* co += dir * amplitude * fbm(co * frequency); value noise (lattice hash, trilinear
+ * with smoothstep), octaves with lacunarity 2 and gain 0.5, normalised to [-1,1]. */
+int sg_mesh_noise_displace(float *co, const float *dir, int totvert, uint32_t seed,
+                           float amplitude, float frequency, int octaves);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SCULPT_GPU_H */

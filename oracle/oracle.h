@@ -1,0 +1,125 @@
+/*
+ * CPU oracle for the sculpt brush hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * This library is the checker, never the product.  Only tests/, the smoke()
+ * entry and bench.py's cpu_baseline leg may load it.  The product path
+ * (libsculpt_gpu.so) never links, loads or calls anything here.
+ *
+ * It restates, in plain C (FP32 with no FMA contraction, like the reference's
+ * SSE2-only build, CMakeLists.txt:703-768), the reference's present PBVH code
+ * and the project's written spec for the brush code the reference lacks:
+ *
+ *   build      kernel/intern/pbvh.c:152-214, :256-320, :322-329, :391-407,
+ *              :454-532, :534-596
+ *   search     kernel/intern/pbvh.c:704-787, :818-849
+ *   normals    kernel/intern/pbvh.c:994-1118 (summed in vertex->tri CSR order
+ *              instead of float atomics, see orc_dab), math_geom.cc:31-46,
+ *              math_vector_inline.c:1165-1180
+ *   initial    kernel/intern/mesh_normals.c:225-299 (angle weighted)
+ *   refit      kernel/intern/pbvh.c:108-128, :1206-1242, :1369-1421
+ *   brushes    project spec S1-S11 (the reference's sculpt.c/brush.c are absent)
+ *
+ * Parity status of each function is stated in its own comment in the .c files.
+ * An FP64 replay runs beside the FP32 state (same affected sets, every other
+ * arithmetic step in double) so error against exact-ish arithmetic can be
+ * reported.
+ */
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Node flags: names follow the reference's PBVHNodeFlags uses (pbvh.c:114, :1003,
+ * :1215, :1725); values are this project's (the enum is absent upstream). */
+enum {
+  ORC_Leaf = 1 << 0,
+  ORC_UpdateNormals = 1 << 1,
+  ORC_UpdateBB = 1 << 2,
+  ORC_UpdateOriginalBB = 1 << 3,
+  ORC_UpdateDrawBuffers = 1 << 4,
+  ORC_UpdateRedraw = 1 << 5,
+  ORC_FullyHidden = 1 << 10,
+};
+
+/* Tools (types_brush_enums.h:410-415) and falloff presets (:176-187). */
+enum { ORC_TOOL_DRAW = 1, ORC_TOOL_SMOOTH = 2, ORC_TOOL_INFLATE = 4, ORC_TOOL_GRAB = 5 };
+enum {
+  ORC_CURVE_CUSTOM = 0, ORC_CURVE_SMOOTH = 1, ORC_CURVE_SPHERE = 2, ORC_CURVE_ROOT = 3,
+  ORC_CURVE_SHARP = 4, ORC_CURVE_LIN = 5, ORC_CURVE_POW4 = 6, ORC_CURVE_INVSQUARE = 7,
+  ORC_CURVE_CONSTANT = 8, ORC_CURVE_SMOOTHER = 9,
+};
+#define ORC_BRUSH_DIR_IN (1 << 9) /* types_brush_enums.h:347 */
+#define ORC_CM_TABLE 256          /* types_color.h:12 */
+
+typedef struct OrcDab {
+  int tool;
+  int falloff;
+  int flag;          /* ORC_BRUSH_DIR_IN */
+  int smooth_iters;  /* Smooth(k) */
+  float center[3];
+  float radius;
+  float strength;    /* Brush.alpha */
+  float hardness;
+  float normal_radius_factor;
+  float grab_delta[3];
+  const float *curve_table; /* ORC_CM_TABLE+1 samples of y over x in [0,1] (CUSTOM) */
+} OrcDab;
+
+typedef struct OrcPBVH OrcPBVH;
+
+/* co: totvert*3 floats (MVert.co order); tris: tottri*3 vertex ids (looptri corners
+ * resolved through MLoop.v).  tri_mat (may be NULL): per-tri material key
+ * (mat_nr | smooth<<16) used by the material split.  vhide (may be NULL): ME_HIDE. */
+OrcPBVH *orc_build(const float *co, int totvert, const int *tris, int tottri, const int *tri_mat,
+                   const uint8_t *vhide, int leaf_limit, int respect_hide, int with_fp64);
+void orc_free(OrcPBVH *o);
+
+int orc_totnode(const OrcPBVH *o);
+int orc_totleaf(const OrcPBVH *o);
+/* Per node: flag, children_offset, prim offset, totprim, uniq_verts, face_verts. */
+void orc_node_info(const OrcPBVH *o, int node, int out[6]);
+void orc_node_bb(const OrcPBVH *o, int node, float vb[6], float orig_vb[6]);
+/* Copies uniq+face verts of a leaf; returns count. */
+int orc_node_verts(const OrcPBVH *o, int node, int *out, int cap);
+/* prim_indices array (tottri). */
+void orc_prim_indices(const OrcPBVH *o, int *out);
+/* Leaves in DFS left->right order (node ids). */
+int orc_leaves_dfs(const OrcPBVH *o, int *out, int cap);
+
+void orc_get_positions(const OrcPBVH *o, float *out);
+void orc_get_normals(const OrcPBVH *o, float *out);
+void orc_get_positions64(const OrcPBVH *o, double *out);
+void orc_get_normals64(const OrcPBVH *o, double *out);
+void orc_set_mask(OrcPBVH *o, const float *mask); /* per-vertex paint mask or NULL */
+/* Replace the vertex normals (e.g. to start two runs from identical state). */
+void orc_set_normals(OrcPBVH *o, const float *no);
+
+/* Sphere search (spec S1, pbvh.c:818-849 traversal).  Returns leaf node ids in
+ * traversal (left->right) order. */
+int orc_search_sphere(const OrcPBVH *o, const float center[3], float radius, int use_orig,
+                      int *out, int cap);
+
+/* One dab: search, affected set, displacement, normals, refit.  Writes the affected
+ * vertex ids (ascending) when out != NULL.  Returns |A|. */
+int orc_dab(OrcPBVH *o, const OrcDab *dab, int *out_affected, int cap);
+
+/* Falloff value as the brush computes it (spec S3), for known-answer tests. */
+float orc_falloff(const OrcDab *dab, float d2);
+
+/* Reference-style whole-mesh pieces, exposed for tests. */
+void orc_tri_normal(const float v1[3], const float v2[3], const float v3[3], float out[3]);
+void orc_calc_initial_normals(const float *co, int totvert, const int *tris, int tottri,
+                              float *out);
+
+/* Threads used by the OpenMP loops (0 = runtime default). */
+void orc_set_threads(int n);
+int orc_get_threads(void);
+
+/* Stats of the last dab for the byte model (gathered leaves etc.). */
+void orc_last_stats(const OrcPBVH *o, int64_t out[8]);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,234 @@
+"""ctypes binding of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this
+module.  It is the checker: the product (dune_sculpt_amd / libsculpt_gpu.so) never
+imports, links or calls it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "liboracle.so"
+
+TOOL_DRAW, TOOL_SMOOTH, TOOL_INFLATE, TOOL_GRAB = 1, 2, 4, 5
+
+
+class OrcDab(C.Structure):
+    _fields_ = [
+        ("tool", C.c_int), ("falloff", C.c_int), ("flag", C.c_int), ("smooth_iters", C.c_int),
+        ("center", C.c_float * 3), ("radius", C.c_float), ("strength", C.c_float), ("hardness", C.c_float),
+        ("normal_radius_factor", C.c_float), ("grab_delta", C.c_float * 3), ("curve_table", C.POINTER(C.c_float)),
+    ]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        build()
+    L = C.CDLL(str(LIB_PATH))
+    vp, i32, f32p, i32p = C.c_void_p, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_int)
+    sig = {
+        "orc_build": (vp, [f32p, i32, i32p, i32, i32p, C.POINTER(C.c_uint8), i32, i32, i32]),
+        "orc_free": (None, [vp]),
+        "orc_totnode": (i32, [vp]),
+        "orc_totleaf": (i32, [vp]),
+        "orc_node_info": (None, [vp, i32, i32p]),
+        "orc_node_bb": (None, [vp, i32, f32p, f32p]),
+        "orc_node_verts": (i32, [vp, i32, i32p, i32]),
+        "orc_prim_indices": (None, [vp, i32p]),
+        "orc_leaves_dfs": (i32, [vp, i32p, i32]),
+        "orc_get_positions": (None, [vp, f32p]),
+        "orc_get_normals": (None, [vp, f32p]),
+        "orc_get_positions64": (None, [vp, C.POINTER(C.c_double)]),
+        "orc_get_normals64": (None, [vp, C.POINTER(C.c_double)]),
+        "orc_set_mask": (None, [vp, f32p]),
+        "orc_set_normals": (None, [vp, f32p]),
+        "orc_search_sphere": (i32, [vp, f32p, C.c_float, i32, i32p, i32]),
+        "orc_dab": (i32, [vp, C.POINTER(OrcDab), i32p, i32]),
+        "orc_falloff": (C.c_float, [C.POINTER(OrcDab), C.c_float]),
+        "orc_tri_normal": (None, [f32p, f32p, f32p, f32p]),
+        "orc_calc_initial_normals": (None, [f32p, i32, i32p, i32, f32p]),
+        "orc_set_threads": (None, [i32]),
+        "orc_get_threads": (i32, []),
+        "orc_last_stats": (None, [vp, C.POINTER(C.c_int64)]),
+    }
+    for k, (r, a) in sig.items():
+        f = getattr(L, k)
+        f.restype = r
+        f.argtypes = a
+    _lib = L
+    return L
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def to_orc(d) -> OrcDab:
+    """Convert an SgDabParams-like object (same field meaning) into an OrcDab."""
+    o = OrcDab()
+    o.tool = d.sculpt_tool
+    o.falloff = d.curve_preset
+    o.flag = d.flag
+    o.smooth_iters = d.smooth_iterations
+    o.center[:] = list(d.location)
+    o.radius = d.radius
+    o.strength = d.strength
+    o.hardness = d.hardness
+    o.normal_radius_factor = d.normal_radius_factor
+    o.grab_delta[:] = list(d.grab_delta)
+    if d.curve_table:
+        o.curve_table = d.curve_table
+    return o
+
+
+class Oracle:
+    def __init__(self, co, tris, leaf_limit=128, fp64=True, hidden=None, mat=None, respect_hide=True):
+        L = lib()
+        co = np.ascontiguousarray(co, np.float32)
+        tris = np.ascontiguousarray(tris, np.int32)
+        self.totvert, self.tottri = co.shape[0], tris.shape[0]
+        hp = None
+        if hidden is not None:
+            self._hide = np.ascontiguousarray(np.asarray(hidden, np.uint8))
+            hp = self._hide.ctypes.data_as(C.POINTER(C.c_uint8))
+        mp = None
+        if mat is not None:
+            self._mat = np.ascontiguousarray(np.asarray(mat, np.int32))
+            mp = _ip(self._mat)
+        self.h = L.orc_build(_fp(co), self.totvert, _ip(tris), self.tottri, mp, hp, leaf_limit,
+                             1 if respect_hide else 0, 1 if fp64 else 0)
+
+    def close(self):
+        if self.h:
+            lib().orc_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def dab(self, d):
+        o = to_orc(d)
+        out = np.empty(self.totvert, np.int32)
+        n = lib().orc_dab(self.h, C.byref(o), _ip(out), self.totvert)
+        return out[:n].copy()
+
+    def positions(self):
+        out = np.empty((self.totvert, 3), np.float32)
+        lib().orc_get_positions(self.h, _fp(out))
+        return out
+
+    def normals(self):
+        out = np.empty((self.totvert, 3), np.float32)
+        lib().orc_get_normals(self.h, _fp(out))
+        return out
+
+    def positions64(self):
+        out = np.empty((self.totvert, 3), np.float64)
+        lib().orc_get_positions64(self.h, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def normals64(self):
+        out = np.empty((self.totvert, 3), np.float64)
+        lib().orc_get_normals64(self.h, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def set_normals(self, no):
+        self._no = np.ascontiguousarray(no, np.float32)
+        lib().orc_set_normals(self.h, _fp(self._no))
+
+    def set_mask(self, mask):
+        if mask is None:
+            lib().orc_set_mask(self.h, None)
+        else:
+            self._mask = np.ascontiguousarray(mask, np.float32)
+            lib().orc_set_mask(self.h, _fp(self._mask))
+
+    def totnode(self):
+        return lib().orc_totnode(self.h)
+
+    def totleaf(self):
+        return lib().orc_totleaf(self.h)
+
+    def node_info(self, i):
+        out = np.zeros(6, np.int32)
+        lib().orc_node_info(self.h, i, _ip(out))
+        return out
+
+    def node_bb(self, i):
+        vb = np.zeros(6, np.float32)
+        ob = np.zeros(6, np.float32)
+        lib().orc_node_bb(self.h, i, _fp(vb), _fp(ob))
+        return vb, ob
+
+    def node_verts(self, i):
+        cap = 4096
+        out = np.empty(cap, np.int32)
+        n = lib().orc_node_verts(self.h, i, _ip(out), cap)
+        return out[:n].copy()
+
+    def prim_indices(self):
+        out = np.empty(self.tottri, np.int32)
+        lib().orc_prim_indices(self.h, _ip(out))
+        return out
+
+    def leaves_dfs(self):
+        out = np.empty(self.tottri + 1, np.int32)
+        n = lib().orc_leaves_dfs(self.h, _ip(out), out.size)
+        return out[:n].copy()
+
+    def search_sphere(self, center, radius, use_orig=False):
+        c = np.asarray(center, np.float32)
+        out = np.empty(self.tottri + 1, np.int32)
+        n = lib().orc_search_sphere(self.h, _fp(c), radius, 1 if use_orig else 0, _ip(out), out.size)
+        return out[:n].copy()
+
+    def last_stats(self):
+        out = np.zeros(8, np.int64)
+        lib().orc_last_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_int64)))
+        return out
+
+
+def falloff(d, d2):
+    o = to_orc(d)
+    return lib().orc_falloff(C.byref(o), d2)
+
+
+def tri_normal(a, b, c):
+    a, b, c = (np.ascontiguousarray(x, np.float32) for x in (a, b, c))
+    out = np.zeros(3, np.float32)
+    lib().orc_tri_normal(_fp(a), _fp(b), _fp(c), _fp(out))
+    return out
+
+
+def initial_normals(co, tris):
+    co = np.ascontiguousarray(co, np.float32)
+    tris = np.ascontiguousarray(tris, np.int32)
+    out = np.zeros_like(co)
+    lib().orc_calc_initial_normals(_fp(co), co.shape[0], _ip(tris), tris.shape[0], _fp(out))
+    return out
+
+
+def set_threads(n):
+    lib().orc_set_threads(n)
