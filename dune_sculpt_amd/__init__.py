@@ -90,8 +90,13 @@ EXPORTS = [
     "sg_dab_last_gathered", "sg_pbvh_set_stats", "sg_pbvh_set_grid", "sg_pbvh_read_positions",
     "sg_pbvh_read_normals", "sg_pbvh_set_mask", "sg_pbvh_tree_info", "sg_pbvh_node_index",
     "sg_pbvh_node_at", "sg_pbvh_prim_indices", "sg_mesh_icosphere_size", "sg_mesh_icosphere",
-    "sg_mesh_torus", "sg_mesh_noise_displace",
+    "sg_mesh_torus", "sg_mesh_noise_displace", "sg_pbvh_set_profile", "sg_pbvh_profile_read",
+    "sg_stroke_prepare", "sg_stroke_launch", "sg_stroke_free", "sg_pbvh_set_kernel_grids",
+    "sg_pbvh_get_kernel_grids", "sg_pbvh_snapshot_save", "sg_pbvh_snapshot_restore",
+    "sg_pbvh_autotune",
 ]
+
+KERNEL_NAMES = ["k_search", "k_area", "k_brush", "k_smooth_iter", "k_post", "k_flush_sub", "k_flush_top"]
 
 
 def lib():
@@ -154,6 +159,16 @@ def lib():
         "sg_mesh_icosphere": (i32, [i32, C.c_float, f32p, i32p]),
         "sg_mesh_torus": (i32, [i32, i32, C.c_float, C.c_float, f32p, i32p]),
         "sg_mesh_noise_displace": (i32, [f32p, f32p, i32, C.c_uint32, C.c_float, C.c_float, i32]),
+        "sg_pbvh_set_profile": (i32, [vp, i32]),
+        "sg_stroke_prepare": (vp, [vp, C.POINTER(SgDabParams), i32, i32]),
+        "sg_pbvh_set_kernel_grids": (i32, [vp, i32, i32, i32]),
+        "sg_pbvh_get_kernel_grids": (i32, [vp, i32p]),
+        "sg_pbvh_snapshot_save": (i32, [vp]),
+        "sg_pbvh_autotune": (i32, [vp, C.POINTER(SgDabParams), i32, C.POINTER(C.c_double)]),
+        "sg_pbvh_snapshot_restore": (i32, [vp]),
+        "sg_stroke_launch": (i32, [vp, i32]),
+        "sg_stroke_free": (None, [vp]),
+        "sg_pbvh_profile_read": (i32, [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong), i32, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -293,6 +308,30 @@ def stroke_config(tools, centres, radii, strengths, smooth_iterations=4, falloff
 # PBVH handle
 # ---------------------------------------------------------------------------
 
+class PreparedStroke:
+    """A dab list captured into a hipGraph; freed before its PBVH."""
+
+    def __init__(self, h, owner):
+        self.h = h
+        self.owner = owner
+
+    def launch(self, sync=True):
+        if not self.h:
+            raise RuntimeError("stroke already freed")
+        _check(lib().sg_stroke_launch(self.h, 1 if sync else 0), "sg_stroke_launch")
+
+    def close(self):
+        if self.h and getattr(self.owner, "h", None):
+            lib().sg_stroke_free(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class PBVH:
     """Owner of one GPU-resident PBVH (BKE_pbvh_new/build_mesh ... BKE_pbvh_free)."""
 
@@ -302,6 +341,7 @@ class PBVH:
         tris = np.ascontiguousarray(tris, np.int32)
         self.totvert = co.shape[0]
         self.tottri = tris.shape[0]
+        self._strokes = []
         self.h = L.BKE_pbvh_new()
         if not self.h:
             raise RuntimeError("BKE_pbvh_new failed: " + last_error())
@@ -336,6 +376,9 @@ class PBVH:
             raise RuntimeError("BKE_pbvh_build_mesh failed: " + err)
 
     def close(self):
+        for st in getattr(self, "_strokes", []):
+            st.close()
+        self._strokes = []
         if getattr(self, "h", None):
             lib().BKE_pbvh_free(self.h)
             self.h = None
@@ -355,6 +398,17 @@ class PBVH:
         flags = (STROKE_GRAPH if graph else 0) | (STROKE_REFIT_EACH if refit_each else 0) | (STROKE_LOG if log else 0)
         _check(lib().sg_dab_apply_stroke(self.h, arr, len(dabs), flags), "sg_dab_apply_stroke")
 
+    def prepare(self, dabs, refit_each=False, log=False):
+        """Capture the dab list into a hipGraph; returns a PreparedStroke."""
+        arr = (SgDabParams * len(dabs))(*dabs)
+        flags = (STROKE_REFIT_EACH if refit_each else 0) | (STROKE_LOG if log else 0)
+        h = lib().sg_stroke_prepare(self.h, arr, len(dabs), flags)
+        if not h:
+            raise RuntimeError("sg_stroke_prepare failed: " + last_error())
+        st = PreparedStroke(h, self)
+        self._strokes.append(st)
+        return st
+
     def stroke_log(self, n):
         out = np.zeros((n, 16), np.int64)
         _check(lib().sg_dab_stroke_log(self.h, out.ctypes.data_as(C.POINTER(C.c_longlong)), n), "stroke log")
@@ -362,6 +416,39 @@ class PBVH:
 
     def set_stats(self, on=True):
         lib().sg_pbvh_set_stats(self.h, 1 if on else 0)
+
+    def set_profile(self, on=True):
+        _check(lib().sg_pbvh_set_profile(self.h, 1 if on else 0), "profile")
+
+    def profile_read(self, reset=True):
+        ms = np.zeros(16, np.float64)
+        cnt = np.zeros(16, np.int64)
+        n = _check(lib().sg_pbvh_profile_read(self.h, ms.ctypes.data_as(C.POINTER(C.c_double)),
+                                              cnt.ctypes.data_as(C.POINTER(C.c_longlong)), 16, 1 if reset else 0),
+                   "profile read")
+        return {KERNEL_NAMES[k]: (float(ms[k]), int(cnt[k])) for k in range(n)}
+
+    def set_kernel_grids(self, brush=0, post=0, smooth=0):
+        _check(lib().sg_pbvh_set_kernel_grids(self.h, brush, post, smooth), "grids")
+
+    def kernel_grids(self):
+        out = np.zeros(4, np.int32)
+        lib().sg_pbvh_get_kernel_grids(self.h, _ip(out))
+        return {"common": int(out[0]), "brush": int(out[1]), "post": int(out[2]), "smooth": int(out[3])}
+
+    def autotune(self, dabs):
+        """Pick per-kernel workgroup counts on a sample of dabs; returns (before, after) ms."""
+        arr = (SgDabParams * len(dabs))(*dabs)
+        out = np.zeros(2, np.float64)
+        _check(lib().sg_pbvh_autotune(self.h, arr, len(dabs), out.ctypes.data_as(C.POINTER(C.c_double))),
+               "autotune")
+        return float(out[0]), float(out[1])
+
+    def snapshot(self):
+        _check(lib().sg_pbvh_snapshot_save(self.h), "snapshot")
+
+    def restore(self):
+        _check(lib().sg_pbvh_snapshot_restore(self.h), "restore")
 
     def set_grid(self, wg):
         _check(lib().sg_pbvh_set_grid(self.h, wg), "grid")

@@ -20,6 +20,8 @@ bool ensure_stats_buffers(PBVH *p) {
   if (sg_check_hip(hipMalloc(&d.stamp_v, V * sizeof(int)), "hipMalloc")) return false;
   if (sg_check_hip(hipMalloc(&d.stamp_v2, V * sizeof(int)), "hipMalloc")) return false;
   if (sg_check_hip(hipMalloc(&d.stamp_t, T * sizeof(int)), "hipMalloc")) return false;
+  if (sg_check_hip(hipMalloc(&d.stamp_a, V * sizeof(int)), "hipMalloc")) return false;
+  hipMemsetAsync(d.stamp_a, 0, V * sizeof(int), p->stream);
   hipMemsetAsync(d.stamp_v, 0, V * sizeof(int), p->stream);
   hipMemsetAsync(d.stamp_v2, 0, V * sizeof(int), p->stream);
   hipMemsetAsync(d.stamp_t, 0, T * sizeof(int), p->stream);
@@ -140,6 +142,9 @@ int upload_host_leaf_verts(PBVH *p, const std::vector<int> &leaves) {
     hipMemcpy(p->d.px + b, x.data(), n * 4, hipMemcpyHostToDevice);
     hipMemcpy(p->d.py + b, y.data(), n * 4, hipMemcpyHostToDevice);
     hipMemcpy(p->d.pz + b, z.data(), n * 4, hipMemcpyHostToDevice);
+    hipMemcpy(p->d.qx + b, x.data(), n * 4, hipMemcpyHostToDevice);
+    hipMemcpy(p->d.qy + b, y.data(), n * 4, hipMemcpyHostToDevice);
+    hipMemcpy(p->d.qz + b, z.data(), n * 4, hipMemcpyHostToDevice);
     i = j + 1;
   }
   return 0;
@@ -188,6 +193,7 @@ void BKE_pbvh_free(PBVH *p) {
     }
   }
   if (!p->no_device) sg_free_device(p);
+  for (void *q : p->snap) hipFree(q);
   if (p->verts_owned) free(p->host_verts);
   free(p->looptri);
   if (!p->no_device) hipStreamDestroy(p->stream);
@@ -316,7 +322,70 @@ int sg_pbvh_set_stats(PBVH *p, int on) {
 int sg_pbvh_set_grid(PBVH *p, int workgroups) {
   if (!p || workgroups < 1 || workgroups > 65536) return sg_set_error("bad grid");
   p->grid_wg = workgroups;
+  p->grid_brush = p->grid_post = p->grid_smooth = 0;
   return 0;
+}
+
+int sg_pbvh_set_kernel_grids(PBVH *p, int brush, int post, int smooth) {
+  if (!p) return sg_set_error("null PBVH");
+  for (int g : {brush, post, smooth})
+    if (g < 0 || g > 65536) return sg_set_error("bad grid");
+  p->grid_brush = brush;
+  p->grid_post = post;
+  p->grid_smooth = smooth;
+  return 0;
+}
+
+int sg_pbvh_get_kernel_grids(PBVH *p, int out[4]) {
+  if (!p) return sg_set_error("null PBVH");
+  out[0] = p->grid_wg;
+  out[1] = p->grid_brush;
+  out[2] = p->grid_post;
+  out[3] = p->grid_smooth;
+  return 0;
+}
+
+// ---- checkpoint: device copy of the mutable state (positions, shadow copy,
+// normals, leaf / subtree / node boxes).  Restore returns the mesh to it exactly.
+
+namespace {
+struct SnapItem {
+  void *dev;
+  size_t bytes;
+};
+std::vector<SnapItem> snap_items(PBVH *p) {
+  SgDev &d = p->d;
+  const size_t V = (size_t)d.V, L = (size_t)d.L, N = (size_t)d.N, C = (size_t)d.C;
+  return {{d.px, V * 4},       {d.py, V * 4},       {d.pz, V * 4},           {d.qx, V * 4},
+          {d.qy, V * 4},       {d.qz, V * 4},       {d.nx, V * 4},           {d.ny, V * 4},
+          {d.nz, V * 4},       {d.leaf_bb, L * 32}, {d.leaf_obb, L * 32},    {d.node_bb, N * 32},
+          {d.node_obb, N * 32}, {d.cut_bb, C * 32}, {d.cut_obb, C * 32}};
+}
+}  // namespace
+
+int sg_pbvh_snapshot_save(PBVH *p) {
+  if (!p || p->no_device || !p->build_ok) return sg_set_error("no usable PBVH");
+  auto items = snap_items(p);
+  if (p->snap.size() != items.size()) {
+    for (void *q : p->snap) hipFree(q);
+    p->snap.assign(items.size(), nullptr);
+    for (size_t i = 0; i < items.size(); i++)
+      if (sg_check_hip(hipMalloc(&p->snap[i], std::max<size_t>(items[i].bytes, 4)), "hipMalloc")) return -1;
+  }
+  if (p->nodes_dirty_deferred) sg_enqueue_flush(p);
+  for (size_t i = 0; i < items.size(); i++)
+    hipMemcpyAsync(p->snap[i], items[i].dev, items[i].bytes, hipMemcpyDeviceToDevice, p->stream);
+  p->snap_valid = true;
+  return sg_check_hip(hipStreamSynchronize(p->stream), "snapshot");
+}
+
+int sg_pbvh_snapshot_restore(PBVH *p) {
+  if (!p || !p->snap_valid) return sg_set_error("no snapshot");
+  auto items = snap_items(p);
+  for (size_t i = 0; i < items.size(); i++)
+    hipMemcpyAsync(items[i].dev, p->snap[i], items[i].bytes, hipMemcpyDeviceToDevice, p->stream);
+  invalidate_host_views(p);
+  return sg_check_hip(hipStreamSynchronize(p->stream), "restore");
 }
 
 int sg_dab_apply(PBVH *p, const SgDabParams *dab) {
@@ -330,8 +399,9 @@ int sg_dab_apply(PBVH *p, const SgDabParams *dab) {
   }
   if (g_stats && !ensure_stats_buffers(p)) return -1;
   SgDabK k = sg_make_dab(dab, g_stats ? 1 : 0);
-  const int stamp = (++p->stamp) * 16;
-  int rc = sg_enqueue_dab(p, k, curve, nullptr, 0, stamp);
+  const int stamp = (++p->stamp) * 128;
+  int rc = sg_enqueue_clear_ctr(p);
+  if (!rc) rc = sg_enqueue_dab(p, k, curve, nullptr, 0, stamp, 0);
   if (!rc) rc = sg_enqueue_flush(p);
   long long ctr[SG_CTR_COUNT];
   if (!rc) rc = sg_check_hip(hipMemcpyAsync(ctr, p->d.ctr, sizeof(ctr), hipMemcpyDeviceToHost, p->stream), "copy");
@@ -373,18 +443,21 @@ int sg_dab_apply_stroke(PBVH *p, const SgDabParams *dabs, int n, int flags) {
   }
   auto enqueue_all = [&]() -> int {
     int ci = 0;
+    if (sg_enqueue_clear_ctr(p)) return -1;
     for (int i = 0; i < n; i++) {
       const float *curve = nullptr;
       if (dabs[i].curve_preset == SG_CURVE_CUSTOM && dabs[i].curve_table) curve = p->d.curve + (size_t)(ci++) * (SG_CM_TABLE + 1);
       SgDabK k = sg_make_dab(&dabs[i], g_stats ? 1 : 0);
-      const int stamp = (++p->stamp) * 16;
-      if (int rc = sg_enqueue_dab(p, k, curve, log ? p->d.log : nullptr, i, stamp)) return rc;
+      const int stamp = (++p->stamp) * 128;
+      if (int rc = sg_enqueue_dab(p, k, curve, log ? p->d.log : nullptr, i, stamp, i)) return rc;
       if (flush_each && sg_enqueue_flush(p)) return -1;
     }
     if (!flush_each && sg_enqueue_flush(p)) return -1;
+    if (log && sg_enqueue_log_row(p, p->d.log, n - 1)) return -1;
     return 0;
   };
   int rc = 0;
+  if (use_graph && p->profile) return sg_set_error("per-kernel profiling needs stream launches (no graph)");
   if (use_graph) {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -405,14 +478,203 @@ int sg_dab_apply_stroke(PBVH *p, const SgDabParams *dabs, int n, int flags) {
   }
   if (rc) return rc;
   long long ctr[SG_CTR_COUNT];
-  hipMemcpy(ctr, p->d.ctr, sizeof(ctr), hipMemcpyDeviceToHost);
+  hipMemcpy(ctr, p->d.ctr + (size_t)((n - 1) & 1) * SG_CTR_COUNT, sizeof(ctr), hipMemcpyDeviceToHost);
   memcpy(p->last_ctr, ctr, sizeof(ctr));
-  if (log) hipMemcpy(p->d.log + (size_t)(n - 1) * SG_CTR_COUNT, ctr, sizeof(ctr), hipMemcpyHostToDevice);
   p->last_tool = dabs[n - 1].sculpt_tool;
   p->dab_count += n;
   p->nodes_dirty_deferred = false;
   invalidate_host_views(p);
   return 0;
+}
+
+// ---- prepared strokes: one hipGraph per dab list, replayed on demand ----------
+
+struct SgStroke {
+  PBVH *p = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  float *curves = nullptr;
+  int n = 0;
+  bool profiled = false;        // captured with event nodes (pool events 2i, 2i+1)
+  std::vector<int> ev_kid;
+};
+
+SgStroke *sg_stroke_prepare(PBVH *p, const SgDabParams *dabs, int n, int flags) {
+  if (!p || n <= 0 || !dabs) {
+    sg_set_error("bad arguments");
+    return nullptr;
+  }
+  if (p->no_device) {
+    sg_set_error("no HIP device");
+    return nullptr;
+  }
+  for (int i = 0; i < n; i++) {
+    if (check_dab(p, &dabs[i])) return nullptr;
+  }
+  const bool flush_each = (flags & 2) != 0;
+  const bool log = (flags & 4) != 0;
+  if (log && ensure_log(p, n)) return nullptr;
+  SgStroke *s = new SgStroke();
+  s->p = p;
+  s->n = n;
+  int ncustom = 0;
+  for (int i = 0; i < n; i++) ncustom += (dabs[i].curve_preset == SG_CURVE_CUSTOM && dabs[i].curve_table) ? 1 : 0;
+  if (ncustom) {
+    if (sg_check_hip(hipMalloc(&s->curves, (size_t)ncustom * (SG_CM_TABLE + 1) * sizeof(float)), "hipMalloc")) {
+      delete s;
+      return nullptr;
+    }
+    int ci = 0;
+    for (int i = 0; i < n; i++) {
+      if (dabs[i].curve_preset == SG_CURVE_CUSTOM && dabs[i].curve_table)
+        hipMemcpy(s->curves + (size_t)(ci++) * (SG_CM_TABLE + 1), dabs[i].curve_table,
+                  (SG_CM_TABLE + 1) * sizeof(float), hipMemcpyHostToDevice);
+    }
+  }
+  // with profiling on, event nodes around every kernel node are captured too; their
+  // timings are collected after each launch (sg_pbvh_profile_read)
+  p->capturing = true;
+  int rc = sg_check_hip(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+  int erc = 0;
+  if (!rc) {
+    int ci = 0;
+    erc = sg_enqueue_clear_ctr(p);
+    for (int i = 0; i < n && !erc; i++) {
+      const float *curve = nullptr;
+      if (dabs[i].curve_preset == SG_CURVE_CUSTOM && dabs[i].curve_table)
+        curve = s->curves + (size_t)(ci++) * (SG_CM_TABLE + 1);
+      SgDabK k = sg_make_dab(&dabs[i], 0);
+      erc = sg_enqueue_dab(p, k, curve, log ? p->d.log : nullptr, i, 0, i);
+      if (!erc && flush_each) erc = sg_enqueue_flush(p);
+    }
+    if (!erc && !flush_each) erc = sg_enqueue_flush(p);
+    if (!erc && log) erc = sg_enqueue_log_row(p, p->d.log, n - 1);
+    const hipError_t e = hipStreamEndCapture(p->stream, &s->graph);
+    rc = erc ? erc : sg_check_hip(e, "end capture");
+  }
+  p->capturing = false;
+  s->profiled = p->profile;
+  s->ev_kid = p->ev_kid;
+  p->ev_kid.clear();
+  p->ev_used = 0;
+  if (!rc) rc = sg_check_hip(hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0), "instantiate");
+  if (rc) {
+    if (s->graph) hipGraphDestroy(s->graph);
+    if (s->curves) hipFree(s->curves);
+    delete s;
+    return nullptr;
+  }
+  return s;
+}
+
+int sg_stroke_launch(SgStroke *s, int sync) {
+  if (!s) return sg_set_error("null stroke");
+  PBVH *p = s->p;
+  int rc = sg_check_hip(hipGraphLaunch(s->exec, p->stream), "graph launch");
+  if (!rc && (sync || s->profiled)) rc = sg_check_hip(hipStreamSynchronize(p->stream), "stroke");
+  if (!rc && s->profiled) {
+    for (size_t i = 0; i < s->ev_kid.size(); i++) {
+      float ms = 0.0f;
+      hipEventElapsedTime(&ms, p->ev_pool[2 * i], p->ev_pool[2 * i + 1]);
+      p->prof_ms[s->ev_kid[i]] += ms;
+      p->prof_n[s->ev_kid[i]] += 1;
+    }
+  }
+  p->dab_count += s->n;
+  invalidate_host_views(p);
+  return rc;
+}
+
+void sg_stroke_free(SgStroke *s) {
+  if (!s) return;
+  hipStreamSynchronize(s->p->stream);
+  if (s->exec) hipGraphExecDestroy(s->exec);
+  if (s->graph) hipGraphDestroy(s->graph);
+  if (s->curves) hipFree(s->curves);
+  delete s;
+}
+
+// ---- autotuner: workgroup counts of the per-dab kernels --------------------------
+// Times a sample stroke (captured graph, launched from the same checkpoint each
+// trial) for each candidate grid of one kernel at a time (coordinate descent over
+// brush, post, smooth step), keeps the fastest, and restores the mesh afterwards.
+static double time_stroke(PBVH *p, const SgDabParams *dabs, int n, int reps) {
+  SgStroke *st = sg_stroke_prepare(p, dabs, n, 0);
+  if (!st) return -1.0;
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  double best = 1e30;
+  for (int r = 0; r < reps + 1; r++) {
+    sg_pbvh_snapshot_restore(p);
+    hipEventRecord(e0, p->stream);
+    sg_stroke_launch(st, 0);
+    hipEventRecord(e1, p->stream);
+    hipEventSynchronize(e1);
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, e0, e1);
+    if (r > 0 && ms < best) best = ms;  // first launch warms up
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  sg_stroke_free(st);
+  return best;
+}
+
+int sg_pbvh_autotune(PBVH *p, const SgDabParams *dabs, int n, double *out_ms) {
+  if (!p || !dabs || n <= 0) return sg_set_error("bad arguments");
+  if (int rc = sg_pbvh_snapshot_save(p)) return rc;
+  const int cand[] = {32, 64, 128, 192, 256, 384, 512};
+  int *slots[3] = {&p->grid_brush, &p->grid_post, &p->grid_smooth};
+  for (int *sl : slots) *sl = p->grid_wg;
+  double best = time_stroke(p, dabs, n, 2);
+  if (best < 0) return -1;
+  const double start = best;
+  for (int pass = 0; pass < 2; pass++) {
+    for (int *sl : slots) {
+      const int keep = *sl;
+      int choice = keep;
+      for (int g : cand) {
+        if (g == keep) continue;
+        *sl = g;
+        const double t = time_stroke(p, dabs, n, 2);
+        if (t > 0 && t < best * 0.98) {
+          best = t;
+          choice = g;
+        }
+      }
+      *sl = choice;
+    }
+  }
+  sg_pbvh_snapshot_restore(p);
+  if (out_ms) {
+    out_ms[0] = start;
+    out_ms[1] = best;
+  }
+  return 0;
+}
+
+int sg_pbvh_set_profile(PBVH *p, int on) {
+  if (!p || p->no_device) return sg_set_error("no device");
+  if (p->profile && !on) sg_profile_collect(p);
+  p->profile = on != 0;
+  return 0;
+}
+
+int sg_pbvh_profile_read(PBVH *p, double *ms, long long *count, int n, int reset) {
+  if (!p || p->no_device) return sg_set_error("no device");
+  if (sg_profile_collect(p)) return -1;
+  for (int i = 0; i < n && i < 16; i++) {
+    ms[i] = p->prof_ms[i];
+    count[i] = p->prof_n[i];
+  }
+  if (reset) {
+    for (int i = 0; i < 16; i++) {
+      p->prof_ms[i] = 0.0;
+      p->prof_n[i] = 0;
+    }
+  }
+  return SG_K_COUNT;
 }
 
 int sg_dab_stroke_log(PBVH *p, long long *out, int n) {
@@ -432,7 +694,7 @@ int sg_dab_last_affected(PBVH *p, int *out, int cap) {
   if (p->no_device) return sg_set_error("no HIP device");
   const int na = (int)p->last_ctr[SG_CTR_NAFF];
   std::vector<int> tmp(std::max(na, 1));
-  if (na > 0 && sg_check_hip(hipMemcpy(tmp.data(), p->d.affected, na * sizeof(int), hipMemcpyDeviceToHost), "copy"))
+  if (na > 0 && sg_check_hip(hipMemcpy(tmp.data(), p->d.aff_v, na * sizeof(int), hipMemcpyDeviceToHost), "copy"))
     return -1;
   for (int i = 0; i < na; i++) tmp[i] = p->perm_old_of_new[tmp[i]];
   std::sort(tmp.begin(), tmp.begin() + na);
@@ -453,6 +715,7 @@ int sg_dab_last_gathered(PBVH *p, int *out, int cap) {
   std::vector<int> tmp(std::max(ng, 1));
   if (ng > 0 && sg_check_hip(hipMemcpy(tmp.data(), p->d.gathered, ng * sizeof(int), hipMemcpyDeviceToHost), "copy"))
     return -1;
+  std::sort(tmp.begin(), tmp.begin() + ng);  // DFS leaf order = the reference's gather order
   for (int i = 0; i < ng && i < cap; i++) out[i] = p->leaf_node[tmp[i]];
   return ng;
 }

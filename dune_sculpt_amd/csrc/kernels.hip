@@ -9,6 +9,7 @@
 // Layout (see pbvh_build.cpp): vertices SoA in leaf-major order, each leaf's uniq
 // vertices one contiguous range; tris int4 in leaf order; vertex->tri and
 // vertex->vertex CSR; bounding boxes as 8 floats (min xyz, pad, max xyz, pad).
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -233,178 +234,145 @@ __device__ __forceinline__ float sg_falloff(const SgDabK &k, const float *curve,
 __device__ __forceinline__ unsigned long long as_u64(long long x) { return (unsigned long long)x; }
 
 // ---------------------------------------------------------------------------
-// K1: search.  One workgroup of 1024 threads.
-//  phase 0  refresh the boxes of cut subtrees the previous dab touched (ordered
-//           reduction of their leaf boxes; these are the only boxes the search needs)
-//  phase A  test every cut subtree box, ordered compaction of the passing ones
-//  phase B  test the leaves of passing subtrees, ordered compaction -> gathered[]
-// The gathered list is in ascending (DFS) leaf order, as BKE_pbvh_search_gather
-// returns it.
+// K1: search.  One wave per cut subtree (all waves independent, no block barrier):
+//   level 1  the subtree's box, leaf range, flags and last-touch stamp
+//   level 2  if the previous dab touched the subtree, its box is rebuilt from its
+//            leaf boxes (ordered reduction, loads issued together) and written back
+//   level 3  when the subtree passes, up to 8 leaves per lane are tested at once
+//   level 4  passing leaves: their vertex ranges load while one atomic per wave
+//            reserves their slots in the gathered list
+// The gathered list is unordered (consumers never depend on its order; the API sorts
+// it into the reference's left->right order when asked).  The first wave also logs
+// the previous dab's counters and clears the next dab's counter set.
 // ---------------------------------------------------------------------------
-#define SEARCH_THREADS 1024
-#define SEARCH_WAVES (SEARCH_THREADS / SG_WAVE)
-#define MAX_CUT 1024
+#define SEARCH_UNROLL 8
 
-__global__ __launch_bounds__(SEARCH_THREADS) void k_search(SgDevView d, SgDabK k, long long *log,
-                                                           int log_index) {
-  __shared__ int s_list[MAX_CUT];
-  __shared__ int s_pre[MAX_CUT + 1];
-  __shared__ int s_has[MAX_CUT];
-  __shared__ int s_wave[SEARCH_WAVES + 1];
-  __shared__ int s_total;
-  const int tid = threadIdx.x, lane = lane_id(), wave = tid / SG_WAVE;
-
-  if (tid == 0 && log && log_index > 0) {
-    for (int i = 0; i < SG_CTR_COUNT; i++) log[(size_t)(log_index - 1) * SG_CTR_COUNT + i] = d.ctr[i];
-  }
-
-  // phase 0: one wave per previously touched cut
-  const int nprev = (int)d.ctr[SG_CTR_NPREV];
-  for (int pc = wave; pc < nprev; pc += SEARCH_WAVES) {
-    const int c = d.prev_cuts[pc];
-    const int lb = d.cut_lbeg[c], le = d.cut_lend[c];
-    const int nl = le - lb;
-    const int per = (nl + SG_WAVE - 1) / SG_WAVE;
-    BB6 acc, oacc;
-    bb_reset(acc);
-    bb_reset(oacc);
-    for (int q = 0; q < per; q++) {
-      const int l = lb + lane * per + q;
-      if (l < le) {
-        bb_add_bb(acc, bb_load(d.leaf_bb + 8 * (size_t)l));
-        bb_add_bb(oacc, bb_load(d.leaf_obb + 8 * (size_t)l));
+__device__ __forceinline__ void leaves_box(const float *bbs, const float *obbs, int lb, int le, BB6 &nb,
+                                           BB6 &ob) {
+  // ordered BB_expand_with_bb over leaf boxes [lb, le) (current and original): lane l
+  // folds a contiguous chunk of `per` leaves (all loads issued together), then the
+  // wave combines lanes in order
+  const int lane = lane_id();
+  const int per = (le - lb + SG_WAVE - 1) / SG_WAVE;
+  bb_reset(nb);
+  bb_reset(ob);
+  for (int q0 = 0; q0 < per; q0 += SEARCH_UNROLL) {
+    BB6 t[SEARCH_UNROLL], u[SEARCH_UNROLL];
+#pragma unroll
+    for (int j = 0; j < SEARCH_UNROLL; j++) {
+      const int l = lb + lane * per + q0 + j;
+      if (q0 + j < per && l < le) {
+        t[j] = bb_load(bbs + 8 * (size_t)l);
+        u[j] = bb_load(obbs + 8 * (size_t)l);
+      } else {
+        bb_reset(t[j]);
+        bb_reset(u[j]);
       }
     }
-    acc = bb_wave_reduce(acc);
-    oacc = bb_wave_reduce(oacc);
-    if (lane == 0) {
-      bb_store(d.node_bb + 8 * (size_t)d.cut_node[c], acc);
-      bb_store(d.node_obb + 8 * (size_t)d.cut_node[c], oacc);
+#pragma unroll
+    for (int j = 0; j < SEARCH_UNROLL; j++) {
+      bb_add_bb(nb, t[j]);
+      bb_add_bb(ob, u[j]);
     }
   }
-  __syncthreads();
+  nb = bb_wave_reduce(nb);
+  ob = bb_wave_reduce(ob);
+}
 
-  // phase A
-  const float *cut_bbs = k.use_orig ? d.node_obb : d.node_bb;
-  const int C = d.C;
-  bool pass = false;
-  int c = tid;
-  if (c < C) {
-    const int nd = d.cut_node[c];
-    pass = sphere_box(bb_load(cut_bbs + 8 * (size_t)nd), k.cx, k.cy, k.cz, k.r2);
-    const int lb = d.cut_lbeg[c];
-    if (pass && d.cut_lend[c] - lb == 1 && (d.leaf_flags[lb] & 1)) pass = false;
+__global__ __launch_bounds__(256) void k_search(SgDevView d, SgDabK k, long long *log, int log_index,
+                                                int serial) {
+  const int lane = lane_id();
+  const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
+  const int nw = gridDim.x * blockDim.x / SG_WAVE;
+  if (gw == 0 && lane < SG_CTR_COUNT) {
+    const long long prev = d.ctr_next[lane];  // the set the previous dab used
+    if (log && log_index > 0) log[(size_t)(log_index - 1) * SG_CTR_COUNT + lane] = prev;
+    d.ctr_next[lane] = 0;
   }
-  {
-    const unsigned long long m = __ballot(pass);
-    if (lane == 0) s_wave[wave] = __popcll(m);
-    __syncthreads();
-    if (tid == 0) {
-      int acc = 0;
-      for (int w = 0; w < SEARCH_WAVES; w++) {
-        const int t = s_wave[w];
-        s_wave[w] = acc;
-        acc += t;
-      }
-      s_total = acc;
-    }
-    __syncthreads();
-    if (pass) {
-      const int pos = s_wave[wave] + mask_prefix(m);
-      s_list[pos] = c;
-      s_pre[pos] = d.cut_lend[c] - d.cut_lbeg[c];
-      s_has[pos] = 0;
-    }
-    __syncthreads();
-  }
-  const int npass = s_total;
-  // exclusive scan of leaf counts (npass <= 1024): simple two-level scan
-  {
-    int v = (tid < npass) ? s_pre[tid] : 0;
-    // inclusive wave scan
-    for (int off = 1; off < SG_WAVE; off <<= 1) {
-      const int o = __shfl_up(v, off);
-      if (lane >= off) v += o;
-    }
-    if (lane == SG_WAVE - 1) s_wave[wave] = v;
-    __syncthreads();
-    if (tid == 0) {
-      int acc = 0;
-      for (int w = 0; w < SEARCH_WAVES; w++) {
-        const int t = s_wave[w];
-        s_wave[w] = acc;
-        acc += t;
-      }
-      s_wave[SEARCH_WAVES] = acc;
-    }
-    __syncthreads();
-    const int incl = v + s_wave[wave];
-    const int cnt = (tid < npass) ? s_pre[tid] : 0;
-    __syncthreads();
-    if (tid < npass) s_pre[tid] = incl - cnt;
-    if (tid == 0) s_pre[npass] = s_wave[SEARCH_WAVES];
-    __syncthreads();
-  }
-  const int total = s_pre[npass];
+  const float *cut_bbs = k.use_orig ? d.cut_obb : d.cut_bb;
   const float *leaf_bbs = k.use_orig ? d.leaf_obb : d.leaf_bb;
-  int gbase = 0;
-  for (int base = 0; base < total; base += SEARCH_THREADS) {
-    const int idx = base + tid;
-    bool lp = false;
-    int leaf = 0;
-    if (idx < total) {
-      int lo = 0, hi = npass - 1;  // last j with s_pre[j] <= idx
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (s_pre[mid] <= idx)
-          lo = mid;
-        else
-          hi = mid - 1;
+  long long ntests = 0;
+  for (int c = gw; c < d.C; c += nw) {
+    const int touch = d.cut_touch[c];
+    BB6 cb = bb_load(cut_bbs + 8 * (size_t)c);
+    const int lb = d.cut_lbeg[c], le = d.cut_lend[c];
+    const int cfl = d.cut_flags[c];
+    if (touch == serial - 1) {
+      BB6 nb, ob;
+      leaves_box(d.leaf_bb, d.leaf_obb, lb, le, nb, ob);
+      if (lane == 0) {
+        bb_store(d.cut_bb + 8 * (size_t)c, nb);
+        bb_store(d.cut_obb + 8 * (size_t)c, ob);
       }
-      leaf = d.cut_lbeg[s_list[lo]] + (idx - s_pre[lo]);
-      lp = !(d.leaf_flags[leaf] & 1) &&
-           sphere_box(bb_load(leaf_bbs + 8 * (size_t)leaf), k.cx, k.cy, k.cz, k.r2);
-      if (lp) s_has[lo] = 1;
+      cb = k.use_orig ? ob : nb;
+      cb.mn[0] = __shfl(cb.mn[0], 0);
+      cb.mn[1] = __shfl(cb.mn[1], 0);
+      cb.mn[2] = __shfl(cb.mn[2], 0);
+      cb.mx[0] = __shfl(cb.mx[0], 0);
+      cb.mx[1] = __shfl(cb.mx[1], 0);
+      cb.mx[2] = __shfl(cb.mx[2], 0);
     }
-    const unsigned long long m = __ballot(lp);
-    if (lane == 0) s_wave[wave] = __popcll(m);
-    __syncthreads();
-    if (tid == 0) {
-      int acc = 0;
-      for (int w = 0; w < SEARCH_WAVES; w++) {
-        const int t = s_wave[w];
-        s_wave[w] = acc;
-        acc += t;
+    ntests++;
+    if ((cfl & 1) || !sphere_box(cb, k.cx, k.cy, k.cz, k.r2)) continue;
+    const int nl = le - lb;
+    ntests += nl;
+    for (int q0 = 0; q0 < nl; q0 += SG_WAVE * SEARCH_UNROLL) {
+      bool pass[SEARCH_UNROLL];
+#pragma unroll
+      for (int j = 0; j < SEARCH_UNROLL; j++) {
+        const int l = lb + q0 + j * SG_WAVE + lane;
+        pass[j] = false;
+        if (l < le) {
+          const BB6 b = bb_load(leaf_bbs + 8 * (size_t)l);
+          const int fl = d.leaf_flags[l];
+          pass[j] = !(fl & 1) && sphere_box(b, k.cx, k.cy, k.cz, k.r2);
+        }
       }
-      s_wave[SEARCH_WAVES] = acc;
+      unsigned long long m[SEARCH_UNROLL];
+      int cnt = 0;
+#pragma unroll
+      for (int j = 0; j < SEARCH_UNROLL; j++) {
+        m[j] = __ballot(pass[j]);
+        cnt += __popcll(m[j]);
+      }
+      if (cnt == 0) continue;
+      int wbase = 0;
+      if (lane == 0) wbase = (int)atomicAdd((unsigned long long *)&d.ctr[SG_CTR_NGATHER], (unsigned long long)cnt);
+      int2 vr[SEARCH_UNROLL], sr[SEARCH_UNROLL];
+#pragma unroll
+      for (int j = 0; j < SEARCH_UNROLL; j++) {
+        const int l = lb + q0 + j * SG_WAVE + lane;
+        if (pass[j]) {
+          vr[j] = make_int2(d.leaf_vbeg[l], d.leaf_vbeg[l + 1]);
+          sr[j] = make_int2(d.leaf_sh_off[l], d.leaf_sh_off[l + 1]);
+        }
+      }
+      wbase = __shfl(wbase, 0);
+      int off = wbase;
+#pragma unroll
+      for (int j = 0; j < SEARCH_UNROLL; j++) {
+        if (pass[j]) {
+          const int pos = off + mask_prefix(m[j]);
+          d.gathered[pos] = lb + q0 + j * SG_WAVE + lane;
+          d.gl_v[pos] = vr[j];
+          d.gl_s[pos] = sr[j];
+        }
+        off += __popcll(m[j]);
+      }
+      if (lane == 0) {
+        d.cut_touch[c] = serial;
+        d.cut_dirty[c] = 1;
+      }
     }
-    __syncthreads();
-    if (lp) d.gathered[gbase + s_wave[wave] + mask_prefix(m)] = leaf;
-    gbase += s_wave[SEARCH_WAVES];
-    __syncthreads();
   }
-  // cuts with gathered leaves: dirty for the deferred refit, refreshed by the next dab
-  if (tid == 0) {
-    int n = 0;
-    for (int j = 0; j < npass; j++) {
-      if (s_has[j]) {
-        d.prev_cuts[n++] = s_list[j];
-        d.cut_dirty[s_list[j]] = 1;
-      }
-    }
-    d.ctr[SG_CTR_NPREV] = n;
-    d.ctr[SG_CTR_NGATHER] = gbase;
-    d.ctr[SG_CTR_NAFF] = 0;
-    d.ctr[SG_CTR_AREA_X] = 0;
-    d.ctr[SG_CTR_AREA_Y] = 0;
-    d.ctr[SG_CTR_AREA_Z] = 0;
-    d.ctr[SG_CTR_TESTS] = C + total;
-    d.ctr[SG_CTR_NCAND] = 0;
-    d.ctr[SG_CTR_NAREA] = 0;
-    d.ctr[SG_CTR_NBR] = 0;
-    d.ctr[SG_CTR_TRIS] = 0;
-    for (int i = 11; i < SG_CTR_COUNT; i++) d.ctr[i] = 0;
+  if (k.stats) {
+    if (lane == 0) atomicAdd((unsigned long long *)&d.ctr[SG_CTR_TESTS], (unsigned long long)ntests);
   }
+}
+
+// zero both counter sets at the start of a stroke
+__global__ void k_clear_ctr(long long *ctr) {
+  if (threadIdx.x < 2 * SG_CTR_COUNT) ctr[threadIdx.x] = 0;
 }
 
 // wave-wide sums
@@ -421,27 +389,30 @@ __device__ __forceinline__ int wave_sum_i(int v) {
 // K2 (Draw only): area normal.  Sum of the normals of uniq verts of gathered leaves
 // within normal_radius_factor * r, each component rounded to a multiple of 2^-32
 // and accumulated in int64 -- exact, so the atomic order cannot change the bits.
+// Positions and normals are fetched in the same load level.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_area(SgDevView d, SgDabK k) {
   const int lane = lane_id();
   const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
   const int nw = gridDim.x * blockDim.x / SG_WAVE;
+  const int2 vr0 = gw < d.L ? d.gl_v[gw] : make_int2(0, 0);
   const int ng = (int)d.ctr[SG_CTR_NGATHER];
   long long sx = 0, sy = 0, sz = 0;
   int na = 0;
   for (int g = gw; g < ng; g += nw) {
-    const int leaf = d.gathered[g];
-    const int vb = d.leaf_vbeg[leaf], ve = d.leaf_vbeg[leaf + 1];
-    for (int v = vb + lane; v < ve; v += SG_WAVE) {
+    const int2 vr = (g == gw) ? vr0 : d.gl_v[g];
+    for (int v = vr.x + lane; v < vr.y; v += SG_WAVE) {
+      const float px = d.px[v], py = d.py[v], pz = d.pz[v];
+      const float nx = d.nx[v], ny = d.ny[v], nz = d.nz[v];
       if (d.vhide && d.respect_hide && d.vhide[v]) continue;
-      const float dx = k.cx - d.px[v];
-      const float dy = k.cy - d.py[v];
-      const float dz = k.cz - d.pz[v];
+      const float dx = k.cx - px;
+      const float dy = k.cy - py;
+      const float dz = k.cz - pz;
       const float d2 = dx * dx + dy * dy + dz * dz;
       if (d2 <= k.nr2) {
-        sx += __double2ll_rn((double)d.nx[v] * 4294967296.0);
-        sy += __double2ll_rn((double)d.ny[v] * 4294967296.0);
-        sz += __double2ll_rn((double)d.nz[v] * 4294967296.0);
+        sx += __double2ll_rn((double)nx * 4294967296.0);
+        sy += __double2ll_rn((double)ny * 4294967296.0);
+        sz += __double2ll_rn((double)nz * 4294967296.0);
         na++;
       }
     }
@@ -458,19 +429,142 @@ __global__ __launch_bounds__(256) void k_area(SgDevView d, SgDabK k) {
   }
 }
 
+// Sum of the (previous-iterate) positions of v's edge neighbours in CSR order, with
+// the neighbour ids and then their positions fetched eight at a time.
+__device__ __forceinline__ void nbr_sum(const int *__restrict__ vv_idx, int b, int e, const float *__restrict__ sx_,
+                                        const float *__restrict__ sy_, const float *__restrict__ sz_, float &sx,
+                                        float &sy, float &sz) {
+  sx = 0.0f;
+  sy = 0.0f;
+  sz = 0.0f;
+  for (int q0 = b; q0 < e; q0 += 8) {
+    int u[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) u[j] = (q0 + j < e) ? vv_idx[q0 + j] : -1;
+    float x[8], y[8], z[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      if (u[j] >= 0) {
+        x[j] = sx_[u[j]];
+        y[j] = sy_[u[j]];
+        z[j] = sz_[u[j]];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      if (u[j] >= 0) {
+        sx = sx + x[j];
+        sy = sy + y[j];
+        sz = sz + z[j];
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // K3: brush.  One wave per gathered leaf (grid-stride); lanes over the leaf's uniq
 // vertices (one coalesced range).  Spec S2 vertex test, S3/S4 factor, S6-S9
 // displacement.  Affected vertices are compacted with ballot + prefix popcount and
-// one atomic per wave.  For Smooth it also records the factor and slot and runs the
-// first Jacobi step (neighbours are read from the untouched positions).
+// one atomic per wave into records (vertex, its vertex->vertex and vertex->tri CSR
+// ranges) so the later kernels skip those lookups.  Positions are written to both
+// the position array and its shadow copy.  Smooth runs its first Jacobi step here,
+// reading the untouched positions and writing the shadow copy.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_brush(SgDevView d, SgDabK k, const float *curve) {
+// Per-vertex inputs of the brush, loaded together for two vertices per lane.
+struct BrushIn {
+  float x, y, z, d2, nx, ny, nz, mk;
+  int2 vt, vv;
+  bool aff;
+};
+
+__device__ __forceinline__ BrushIn brush_load(const SgDevView &d, const SgDabK &k, int v, int vend, bool smooth) {
+  BrushIn in;
+  in.aff = false;
+  in.x = in.y = in.z = in.d2 = in.nx = in.ny = in.nz = in.mk = 0.0f;
+  in.vt = make_int2(0, 0);
+  in.vv = make_int2(0, 0);
+  if (v < vend) {
+    in.x = d.px[v];
+    in.y = d.py[v];
+    in.z = d.pz[v];
+    in.vt = make_int2(d.vt_off[v], d.vt_off[v + 1]);
+    if (smooth) in.vv = make_int2(d.vv_off[v], d.vv_off[v + 1]);
+    if (k.tool == SG_TOOL_INFLATE) {
+      in.nx = d.nx[v];
+      in.ny = d.ny[v];
+      in.nz = d.nz[v];
+    }
+    if (d.mask) in.mk = d.mask[v];
+    const float dx = k.cx - in.x;
+    const float dy = k.cy - in.y;
+    const float dz = k.cz - in.z;
+    in.d2 = dx * dx + dy * dy + dz * dz;
+    in.aff = in.d2 <= k.r2;
+    if (d.vhide && d.respect_hide && d.vhide[v]) in.aff = false;
+  }
+  return in;
+}
+
+// Spec S3/S4 factor and S6-S9 displacement of one affected vertex; returns the Smooth
+// factor (0 for other tools).  Smooth step 1 reads the untouched positions and writes
+// the shadow copy; the other tools write both copies.
+__device__ __forceinline__ float brush_apply(const SgDevView &d, const SgDabK &k, const float *curve, int v,
+                                             const BrushIn &in, float ox, float oy, float oz, long long &nbr) {
+  const float f = sg_falloff(k, curve, sqrtf(in.d2)) * (1.0f - in.mk);
+  float rx = in.x, ry = in.y, rz = in.z, fade = 0.0f;
+  if (k.tool == SG_TOOL_DRAW) {
+    rx = in.x + ox * f;
+    ry = in.y + oy * f;
+    rz = in.z + oz * f;
+  } else if (k.tool == SG_TOOL_INFLATE) {
+    const float s = (k.bstr * f) * k.r;
+    rx = in.x + in.nx * s;
+    ry = in.y + in.ny * s;
+    rz = in.z + in.nz * s;
+  } else if (k.tool == SG_TOOL_GRAB) {
+    fade = k.bstr * f;
+    rx = in.x + k.gx * fade;
+    ry = in.y + k.gy * fade;
+    rz = in.z + k.gz * fade;
+  } else {
+    fade = k.bstr * f;
+    float sx, sy, sz;
+    nbr_sum(d.vv_idx, in.vv.x, in.vv.y, d.px, d.py, d.pz, sx, sy, sz);
+    nbr += in.vv.y - in.vv.x;
+    if (in.vv.y > in.vv.x) {
+      const float inv = 1.0f / (float)(in.vv.y - in.vv.x);
+      const float avx = sx * inv, avy = sy * inv, avz = sz * inv;
+      rx = in.x + (avx - in.x) * fade;
+      ry = in.y + (avy - in.y) * fade;
+      rz = in.z + (avz - in.z) * fade;
+    }
+  }
+  if (k.tool != SG_TOOL_SMOOTH) {
+    d.px[v] = rx;
+    d.py[v] = ry;
+    d.pz[v] = rz;
+  }
+  d.qx[v] = rx;
+  d.qy[v] = ry;
+  d.qz[v] = rz;
+  return fade;
+}
+
+// ---------------------------------------------------------------------------
+// K3: brush.  One wave per gathered leaf (grid-stride); two vertices per lane, so a
+// leaf of up to 128 uniq vertices is one load level, one ballot pair and one atomic.
+// Spec S2 vertex test, S3/S4 factor, S6-S9 displacement.  Affected vertices are
+// compacted with ballot + prefix popcount into records (vertex, its vertex->tri and
+// vertex->vertex CSR ranges) so the later kernels skip those lookups.  The slot
+// reservation is in flight while the displacement is computed.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_brush(SgDevView d, SgDabK k, const float *curve, int stamp) {
   const int lane = lane_id();
   const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
   const int nw = gridDim.x * blockDim.x / SG_WAVE;
+  const int2 vr0 = gw < d.L ? d.gl_v[gw] : make_int2(0, 0);
   const int ng = (int)d.ctr[SG_CTR_NGATHER];
-  const int V = d.V;
+  const bool smooth = k.tool == SG_TOOL_SMOOTH;
   // Draw offset: (normalize(area) * r) * bstrength
   float ox = 0.0f, oy = 0.0f, oz = 0.0f;
   if (k.tool == SG_TOOL_DRAW) {
@@ -485,77 +579,48 @@ __global__ __launch_bounds__(256) void k_brush(SgDevView d, SgDabK k, const floa
   int ncand = 0;
   long long nbr = 0;
   for (int g = gw; g < ng; g += nw) {
-    const int leaf = d.gathered[g];
-    const int vb = d.leaf_vbeg[leaf], ve = d.leaf_vbeg[leaf + 1];
-    ncand += ve - vb;
-    for (int base = vb; base < ve; base += SG_WAVE) {
-      const int v = base + lane;
-      bool aff = false;
-      float x = 0.0f, y = 0.0f, z = 0.0f, d2 = 0.0f;
-      if (v < ve) {
-        x = d.px[v];
-        y = d.py[v];
-        z = d.pz[v];
-        const float dx = k.cx - x;
-        const float dy = k.cy - y;
-        const float dz = k.cz - z;
-        d2 = dx * dx + dy * dy + dz * dz;
-        aff = d2 <= k.r2;
-        if (d.vhide && d.respect_hide && d.vhide[v]) aff = false;
-      }
-      const unsigned long long m = __ballot(aff);
-      if (m == 0ull) continue;
+    const int2 vr = (g == gw) ? vr0 : d.gl_v[g];
+    ncand += vr.y - vr.x;
+    for (int base = vr.x; base < vr.y; base += 2 * SG_WAVE) {
+      const int v0 = base + lane, v1 = base + SG_WAVE + lane;
+      const BrushIn in0 = brush_load(d, k, v0, vr.y, smooth);
+      const BrushIn in1 = brush_load(d, k, v1, vr.y, smooth);
+      const unsigned long long m0 = __ballot(in0.aff);
+      const unsigned long long m1 = __ballot(in1.aff);
+      const int c0 = __popcll(m0), cnt = c0 + __popcll(m1);
+      if (cnt == 0) continue;
       int wbase = 0;
-      if (lane == 0) wbase = (int)atomicAdd((unsigned long long *)&d.ctr[SG_CTR_NAFF], (unsigned long long)__popcll(m));
+      if (lane == 0) wbase = (int)atomicAdd((unsigned long long *)&d.ctr[SG_CTR_NAFF], (unsigned long long)cnt);
+      float fade0 = 0.0f, fade1 = 0.0f;
+      if (in0.aff) fade0 = brush_apply(d, k, curve, v0, in0, ox, oy, oz, nbr);
+      if (in1.aff) fade1 = brush_apply(d, k, curve, v1, in1, ox, oy, oz, nbr);
+      if (k.stats) {
+        if (in0.aff) d.stamp_a[v0] = stamp;
+        if (in1.aff) d.stamp_a[v1] = stamp;
+      }
       wbase = __shfl(wbase, 0);
-      if (!aff) continue;
-      const int pos = wbase + mask_prefix(m);
-      d.affected[pos] = v;
-      const float mk = d.mask ? d.mask[v] : 0.0f;
-      const float f = sg_falloff(k, curve, sqrtf(d2)) * (1.0f - mk);
-      if (k.tool == SG_TOOL_DRAW) {
-        d.px[v] = x + ox * f;
-        d.py[v] = y + oy * f;
-        d.pz[v] = z + oz * f;
-      } else if (k.tool == SG_TOOL_INFLATE) {
-        const float s = (k.bstr * f) * k.r;
-        d.px[v] = x + d.nx[v] * s;
-        d.py[v] = y + d.ny[v] * s;
-        d.pz[v] = z + d.nz[v] * s;
-      } else if (k.tool == SG_TOOL_GRAB) {
-        const float fade = k.bstr * f;
-        d.px[v] = x + k.gx * fade;
-        d.py[v] = y + k.gy * fade;
-        d.pz[v] = z + k.gz * fade;
-      } else {  // SMOOTH: factor, slot, first Jacobi step into sbuf[0]
-        const float fade = k.bstr * f;
-        d.aff_fade[pos] = fade;
-        d.slot[v] = pos;
-        const int b = d.vv_off[v], e = d.vv_off[v + 1];
-        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-        for (int q = b; q < e; q++) {
-          const int u = d.vv_idx[q];
-          sx = sx + d.px[u];
-          sy = sy + d.py[u];
-          sz = sz + d.pz[u];
+      if (in0.aff) {
+        const int pos = wbase + mask_prefix(m0);
+        d.aff_v[pos] = v0;
+        d.aff_vt[pos] = in0.vt;
+        if (smooth) {
+          d.aff_vv[pos] = in0.vv;
+          d.aff_fade[pos] = fade0;
         }
-        nbr += e - b;
-        float nxv = x, nyv = y, nzv = z;
-        if (e > b) {
-          const float inv = 1.0f / (float)(e - b);
-          const float avx = sx * inv, avy = sy * inv, avz = sz * inv;
-          nxv = x + (avx - x) * fade;
-          nyv = y + (avy - y) * fade;
-          nzv = z + (avz - z) * fade;
+      }
+      if (in1.aff) {
+        const int pos = wbase + c0 + mask_prefix(m1);
+        d.aff_v[pos] = v1;
+        d.aff_vt[pos] = in1.vt;
+        if (smooth) {
+          d.aff_vv[pos] = in1.vv;
+          d.aff_fade[pos] = fade1;
         }
-        float *s0 = d.sbuf0;
-        s0[pos] = nxv;
-        s0[(size_t)V + pos] = nyv;
-        s0[2 * (size_t)V + pos] = nzv;
       }
     }
   }
   if (k.stats) {
+    ncand = (lane == 0) ? ncand : 0;
     ncand = wave_sum_i(ncand);
     nbr = wave_sum_ll(nbr);
     if (lane == 0) {
@@ -566,56 +631,48 @@ __global__ __launch_bounds__(256) void k_brush(SgDevView d, SgDabK k, const floa
 }
 
 // ---------------------------------------------------------------------------
-// K4: Jacobi step it >= 2 of Smooth(k) (spec S8).  Thread per affected vertex; a
-// neighbour's previous iterate comes from the compact buffer when it is affected
-// (slot >= 0) and from the untouched positions otherwise.  The last step writes the
-// positions directly (no reader needs the old value of an affected vertex).
+// K4: Jacobi step it >= 2 of Smooth(k) (spec S8).  Thread per affected vertex.  Odd
+// steps read the positions and write the shadow copy, even steps the reverse; a
+// vertex outside the affected set has the same value in both, so neighbours are read
+// straight from the source array with no lookup.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_smooth_iter(SgDevView d, int src_buf, int last, int stats,
+__global__ __launch_bounds__(256) void k_smooth_iter(SgDevView d, int from_shadow, int stats, int stamp_dab,
                                                      int stamp) {
+  const int nthreads = gridDim.x * blockDim.x;
+  const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const float *sx_ = from_shadow ? d.qx : d.px;
+  const float *sy_ = from_shadow ? d.qy : d.py;
+  const float *sz_ = from_shadow ? d.qz : d.pz;
+  float *dx_ = from_shadow ? d.px : d.qx;
+  float *dy_ = from_shadow ? d.py : d.qy;
+  float *dz_ = from_shadow ? d.pz : d.qz;
   const int na = (int)d.ctr[SG_CTR_NAFF];
-  const int V = d.V;
-  const float *S = src_buf ? d.sbuf1 : d.sbuf0;
-  float *D = src_buf ? d.sbuf0 : d.sbuf1;
   long long nbr = 0, r1 = 0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
-    const int v = d.affected[i];
-    const float x = S[i], y = S[(size_t)V + i], z = S[2 * (size_t)V + i];
-    const int b = d.vv_off[v], e = d.vv_off[v + 1];
-    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    for (int q = b; q < e; q++) {
-      const int u = d.vv_idx[q];
-      const int su = d.slot[u];
-      if (su >= 0) {
-        sx = sx + S[su];
-        sy = sy + S[(size_t)V + su];
-        sz = sz + S[2 * (size_t)V + su];
-      } else {
-        sx = sx + d.px[u];
-        sy = sy + d.py[u];
-        sz = sz + d.pz[u];
-        if (stats && atomicExch(&d.stamp_v[u], stamp) != stamp) r1++;
+  for (int i = i0; i < na; i += nthreads) {
+    const int v = d.aff_v[i];
+    const int2 vv = d.aff_vv[i];
+    const float fade = d.aff_fade[i];
+    const float x = sx_[v], y = sy_[v], z = sz_[v];
+    float sx, sy, sz;
+    nbr_sum(d.vv_idx, vv.x, vv.y, sx_, sy_, sz_, sx, sy, sz);
+    nbr += vv.y - vv.x;
+    if (stats) {
+      for (int q = vv.x; q < vv.y; q++) {
+        const int u = d.vv_idx[q];
+        if (d.stamp_a[u] != stamp_dab && atomicExch(&d.stamp_v[u], stamp) != stamp) r1++;
       }
     }
-    nbr += e - b;
-    const float fade = d.aff_fade[i];
-    float nxv = x, nyv = y, nzv = z;
-    if (e > b) {
-      const float inv = 1.0f / (float)(e - b);
+    float rx = x, ry = y, rz = z;
+    if (vv.y > vv.x) {
+      const float inv = 1.0f / (float)(vv.y - vv.x);
       const float avx = sx * inv, avy = sy * inv, avz = sz * inv;
-      nxv = x + (avx - x) * fade;
-      nyv = y + (avy - y) * fade;
-      nzv = z + (avz - z) * fade;
+      rx = x + (avx - x) * fade;
+      ry = y + (avy - y) * fade;
+      rz = z + (avz - z) * fade;
     }
-    if (last) {
-      d.px[v] = nxv;
-      d.py[v] = nyv;
-      d.pz[v] = nzv;
-    } else {
-      D[i] = nxv;
-      D[(size_t)V + i] = nyv;
-      D[2 * (size_t)V + i] = nzv;
-    }
+    dx_[v] = rx;
+    dy_[v] = ry;
+    dz_[v] = rz;
   }
   if (stats) {
     nbr = wave_sum_ll(nbr);
@@ -627,39 +684,71 @@ __global__ __launch_bounds__(256) void k_smooth_iter(SgDevView d, int src_buf, i
   }
 }
 
-// Smooth(1): copy the single iterate back.
-__global__ __launch_bounds__(256) void k_smooth_copy(SgDevView d) {
+// Copy the final Jacobi iterate from the shadow copy (odd k) or to it (even k), so
+// both arrays agree again.
+__global__ __launch_bounds__(256) void k_smooth_sync(SgDevView d, int from_shadow) {
   const int na = (int)d.ctr[SG_CTR_NAFF];
-  const int V = d.V;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
-    const int v = d.affected[i];
-    d.px[v] = d.sbuf0[i];
-    d.py[v] = d.sbuf0[(size_t)V + i];
-    d.pz[v] = d.sbuf0[2 * (size_t)V + i];
+    const int v = d.aff_v[i];
+    if (from_shadow) {
+      d.px[v] = d.qx[v];
+      d.py[v] = d.qy[v];
+      d.pz[v] = d.qz[v];
+    } else {
+      d.qx[v] = d.px[v];
+      d.qy[v] = d.py[v];
+      d.qz[v] = d.pz[v];
+    }
   }
 }
 
 // Normal of vertex v (spec S10): normalize(sum over incident tris, in CSR order, of
-// normal_tri_v3).  Deterministic gather, no atomics.
-__device__ __forceinline__ void vertex_normal(const SgDevView &d, int v, int stats, int stamp,
+// normal_tri_v3).  Deterministic gather, no atomics.  The incident tris' corner ids
+// are stored flattened in CSR order (vt4), so a batch of eight tris needs one load
+// level for the corners and one for their positions.
+__device__ __forceinline__ void vertex_normal(const SgDevView &d, int v, int b, int e, int stats, int stamp,
                                               long long &ntri, long long &nfa, long long &nvfa) {
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  const int b = d.vt_off[v], e = d.vt_off[v + 1];
-  for (int q = b; q < e; q++) {
-    const int t = d.vt_idx[q] & 0x3fffffff;
-    const int4 c = d.tri[t];
-    float fx, fy, fz;
-    sg_tri_normal(d.px[c.x], d.py[c.x], d.pz[c.x], d.px[c.y], d.py[c.y], d.pz[c.y], d.px[c.z],
-                  d.py[c.z], d.pz[c.z], fx, fy, fz);
-    ax = ax + fx;
-    ay = ay + fy;
-    az = az + fz;
+  for (int q0 = b; q0 < e; q0 += 8) {
+    int4 c[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) c[j] = (q0 + j < e) ? d.vt4[q0 + j] : make_int4(-1, -1, -1, -1);
+    float P[8][9];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      if (c[j].x >= 0) {
+        P[j][0] = d.px[c[j].x];
+        P[j][1] = d.py[c[j].x];
+        P[j][2] = d.pz[c[j].x];
+        P[j][3] = d.px[c[j].y];
+        P[j][4] = d.py[c[j].y];
+        P[j][5] = d.pz[c[j].y];
+        P[j][6] = d.px[c[j].z];
+        P[j][7] = d.py[c[j].z];
+        P[j][8] = d.pz[c[j].z];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      if (c[j].x >= 0) {
+        float fx, fy, fz;
+        sg_tri_normal(P[j][0], P[j][1], P[j][2], P[j][3], P[j][4], P[j][5], P[j][6], P[j][7], P[j][8], fx, fy,
+                      fz);
+        ax = ax + fx;
+        ay = ay + fy;
+        az = az + fz;
+      }
+    }
     if (stats) {
-      if (atomicExch(&d.stamp_t[t], stamp) != stamp) {
-        nfa++;
-        if (atomicExch(&d.stamp_v2[c.x], stamp) != stamp) nvfa++;
-        if (atomicExch(&d.stamp_v2[c.y], stamp) != stamp) nvfa++;
-        if (atomicExch(&d.stamp_v2[c.z], stamp) != stamp) nvfa++;
+      for (int j = 0; j < 8; j++) {
+        if (c[j].x < 0) continue;
+        const int t = c[j].w & 0x3fffffff;
+        if (atomicExch(&d.stamp_t[t], stamp) != stamp) {
+          nfa++;
+          if (atomicExch(&d.stamp_v2[c[j].x], stamp) != stamp) nvfa++;
+          if (atomicExch(&d.stamp_v2[c[j].y], stamp) != stamp) nvfa++;
+          if (atomicExch(&d.stamp_v2[c[j].z], stamp) != stamp) nvfa++;
+        }
       }
     }
   }
@@ -673,45 +762,76 @@ __device__ __forceinline__ void vertex_normal(const SgDevView &d, int v, int sta
 // Leaf bounds over ALL its vertices, uniq then shared (update_node_vb pbvh.c:108-128,
 // BB_expand in vert_indices order).  Each lane folds a contiguous chunk, then the
 // wave combines lanes in order.  Writes vb and orig_vb (UpdateOriginalBB).
-__device__ __forceinline__ void leaf_refit(const SgDevView &d, int leaf, long long &nsh) {
-  const int vb = d.leaf_vbeg[leaf], ve = d.leaf_vbeg[leaf + 1];
-  const int sb = d.leaf_sh_off[leaf], se = d.leaf_sh_off[leaf + 1];
-  const int nu = ve - vb, ns = se - sb, tot = nu + ns;
+__device__ __forceinline__ void leaf_refit(const SgDevView &d, int leaf, int2 vr, int2 sr, long long &nsh) {
+  const int nu = vr.y - vr.x, ns = sr.y - sr.x, tot = nu + ns;
   const int per = (tot + SG_WAVE - 1) / SG_WAVE;
   const int lane = lane_id();
   BB6 acc;
   bb_reset(acc);
-  for (int q = 0; q < per; q++) {
-    const int i = lane * per + q;
-    if (i < tot) {
-      const int v = (i < nu) ? vb + i : d.leaf_sh[sb + i - nu];
-      bb_add_point(acc, d.px[v], d.py[v], d.pz[v]);
+  for (int q0 = 0; q0 < per; q0 += 8) {
+    int vid[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int i = lane * per + q0 + j;
+      vid[j] = -1;
+      if (q0 + j < per && i < tot) vid[j] = (i < nu) ? vr.x + i : d.leaf_sh[sr.x + i - nu];
+    }
+    float x[8], y[8], z[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      if (vid[j] >= 0) {
+        x[j] = d.px[vid[j]];
+        y[j] = d.py[vid[j]];
+        z[j] = d.pz[vid[j]];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      if (vid[j] >= 0) bb_add_point(acc, x[j], y[j], z[j]);
     }
   }
   acc = bb_wave_reduce(acc);
   if (lane == 0) {
     bb_store(d.leaf_bb + 8 * (size_t)leaf, acc);
     bb_store(d.leaf_obb + 8 * (size_t)leaf, acc);
+    nsh += ns;
   }
-  nsh += ns;
 }
 
 // ---------------------------------------------------------------------------
 // K5: post.  Normals of the affected vertices and bounds of the gathered leaves (all
 // gathered leaves are marked, BKE_pbvh_node_mark_update).  Both only read positions.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_post(SgDevView d, int smooth, int stats, int stamp) {
+__global__ __launch_bounds__(256) void k_post(SgDevView d, int stats, int stamp, int sync_q) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nthreads = gridDim.x * blockDim.x;
+  const int gw = tid / SG_WAVE;
+  const int nw = nthreads / SG_WAVE;
+  // speculative first-record loads alongside the counters
+  const int v0 = tid < d.V ? d.aff_v[tid] : 0;
+  const int2 vt0 = tid < d.V ? d.aff_vt[tid] : make_int2(0, 0);
+  const int leaf0 = gw < d.L ? d.gathered[gw] : 0;
+  const int2 vr0 = gw < d.L ? d.gl_v[gw] : make_int2(0, 0);
+  const int2 sr0 = gw < d.L ? d.gl_s[gw] : make_int2(0, 0);
   const int na = (int)d.ctr[SG_CTR_NAFF];
   const int ng = (int)d.ctr[SG_CTR_NGATHER];
   long long ntri = 0, nfa = 0, nvfa = 0, nsh = 0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
-    const int v = d.affected[i];
-    vertex_normal(d, v, stats, stamp, ntri, nfa, nvfa);
-    if (smooth) d.slot[v] = -1;
+  for (int i = tid; i < na; i += nthreads) {
+    const int v = (i == tid) ? v0 : d.aff_v[i];
+    const int2 vt = (i == tid) ? vt0 : d.aff_vt[i];
+    if (sync_q) {
+      d.qx[v] = d.px[v];
+      d.qy[v] = d.py[v];
+      d.qz[v] = d.pz[v];
+    }
+    vertex_normal(d, v, vt.x, vt.y, stats, stamp, ntri, nfa, nvfa);
   }
-  const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
-  const int nw = gridDim.x * blockDim.x / SG_WAVE;
-  for (int g = gw; g < ng; g += nw) leaf_refit(d, d.gathered[g], nsh);
+  for (int g = gw; g < ng; g += nw) {
+    if (g == gw)
+      leaf_refit(d, leaf0, vr0, sr0, nsh);
+    else
+      leaf_refit(d, d.gathered[g], d.gl_v[g], d.gl_s[g], nsh);
+  }
   if (stats) {
     ntri = wave_sum_ll(ntri);
     nfa = wave_sum_ll(nfa);
@@ -720,7 +840,7 @@ __global__ __launch_bounds__(256) void k_post(SgDevView d, int smooth, int stats
       atomicAdd((unsigned long long *)&d.ctr[SG_CTR_TRIS], as_u64(ntri));
       atomicAdd((unsigned long long *)&d.ctr[SG_CTR_FA], as_u64(nfa));
       atomicAdd((unsigned long long *)&d.ctr[SG_CTR_VFA], as_u64(nvfa));
-      atomicAdd((unsigned long long *)&d.ctr[SG_CTR_SHARED], as_u64(nsh / SG_WAVE));
+      atomicAdd((unsigned long long *)&d.ctr[SG_CTR_SHARED], as_u64(nsh));
     }
   }
 }
@@ -772,11 +892,17 @@ __global__ __launch_bounds__(FLUSH_THREADS) void k_flush_sub(SgDevView d, int ma
     const int nd = d.sub_tab[e].x;
     bb_store(d.node_bb + 8 * (size_t)nd, b);
     bb_store(d.node_obb + 8 * (size_t)nd, b);
+    if (e == ee - 1) {  // the last entry is the subtree root
+      bb_store(d.cut_bb + 8 * (size_t)c, b);
+      bb_store(d.cut_obb + 8 * (size_t)c, b);
+    }
   }
-  if (threadIdx.x == 0 && ee == eb) {  // cut root is a leaf: mirror its box
+  if (threadIdx.x == 0 && ee == eb) {  // subtree root is a leaf: mirror its box
     const BB6 b = bb_load(d.leaf_bb + 8 * (size_t)lb);
     bb_store(d.node_bb + 8 * (size_t)d.cut_node[c], b);
     bb_store(d.node_obb + 8 * (size_t)d.cut_node[c], b);
+    bb_store(d.cut_bb + 8 * (size_t)c, b);
+    bb_store(d.cut_obb + 8 * (size_t)c, b);
   }
 }
 
@@ -784,7 +910,7 @@ __global__ __launch_bounds__(1024) void k_flush_top(SgDevView d, int top_n, int 
   extern __shared__ float s_bb[];  // [(C + top_n) * 6]
   const int C = d.C;
   for (int i = threadIdx.x; i < C; i += blockDim.x) {
-    const BB6 b = bb_load(d.node_bb + 8 * (size_t)d.cut_node[i]);
+    const BB6 b = bb_load(d.cut_bb + 8 * (size_t)i);
     for (int q = 0; q < 3; q++) {
       s_bb[6 * i + q] = b.mn[q];
       s_bb[6 * i + 3 + q] = b.mx[q];
@@ -829,14 +955,50 @@ __global__ __launch_bounds__(1024) void k_flush_top(SgDevView d, int top_n, int 
 // Host-driven updates: refit a list of leaves; normals of marked verts of a list of
 // leaves (BKE_pbvh_update_bounds / BKE_pbvh_update_normals with caller flags).
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ void leaf_refit_id(const SgDevView &d, int leaf, long long &nsh) {
+  leaf_refit(d, leaf, make_int2(d.leaf_vbeg[leaf], d.leaf_vbeg[leaf + 1]),
+             make_int2(d.leaf_sh_off[leaf], d.leaf_sh_off[leaf + 1]), nsh);
+}
+
+// cut subtree box from its leaves (ordered), for the host-driven refits
+__device__ __forceinline__ void cut_refresh(const SgDevView &d, int c) {
+  const int lb = d.cut_lbeg[c], le = d.cut_lend[c];
+  const int per = (le - lb + SG_WAVE - 1) / SG_WAVE;
+  BB6 acc, oacc;
+  bb_reset(acc);
+  bb_reset(oacc);
+  for (int q = 0; q < per; q++) {
+    const int l = lb + lane_id() * per + q;
+    if (l < le) {
+      bb_add_bb(acc, bb_load(d.leaf_bb + 8 * (size_t)l));
+      bb_add_bb(oacc, bb_load(d.leaf_obb + 8 * (size_t)l));
+    }
+  }
+  acc = bb_wave_reduce(acc);
+  oacc = bb_wave_reduce(oacc);
+  if (lane_id() == 0) {
+    bb_store(d.cut_bb + 8 * (size_t)c, acc);
+    bb_store(d.cut_obb + 8 * (size_t)c, oacc);
+  }
+}
+
 __global__ __launch_bounds__(256) void k_refit_list(SgDevView d, const int *leaves, int n) {
   const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
   const int nw = gridDim.x * blockDim.x / SG_WAVE;
   long long nsh = 0;
   for (int g = gw; g < n; g += nw) {
     const int leaf = leaves[g];
-    leaf_refit(d, leaf, nsh);
+    leaf_refit_id(d, leaf, nsh);
     if (lane_id() == 0) d.cut_dirty[d.leaf_cut[leaf]] = 1;
+  }
+}
+
+// after a host-driven leaf refit: every dirty subtree's search box from its leaves
+__global__ __launch_bounds__(256) void k_cut_refresh(SgDevView d) {
+  const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
+  const int nw = gridDim.x * blockDim.x / SG_WAVE;
+  for (int c = gw; c < d.C; c += nw) {
+    if (d.cut_dirty[c]) cut_refresh(d, c);
   }
 }
 
@@ -849,7 +1011,7 @@ __global__ __launch_bounds__(256) void k_normals_list(SgDevView d, const int *le
     const int vb = d.leaf_vbeg[leaf], ve = d.leaf_vbeg[leaf + 1];
     for (int v = vb + lane_id(); v < ve; v += SG_WAVE) {
       if (!d.vmark[v]) continue;
-      vertex_normal(d, v, 0, 0, a, b, c);
+      vertex_normal(d, v, d.vt_off[v], d.vt_off[v + 1], 0, 0, a, b, c);
       d.vmark[v] = 0;
     }
   }
@@ -872,10 +1034,8 @@ __global__ __launch_bounds__(256) void k_full_normals(SgDevView d, int mode) {
   for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
     for (int q = d.vt_off[v]; q < d.vt_off[v + 1]; q++) {
-      const int e = d.vt_idx[q];
-      const int t = e & 0x3fffffff;
-      const int ci = (int)((unsigned)e >> 30);
-      const int4 c = d.tri[t];
+      const int4 c = d.vt4[q];
+      const int ci = (int)((unsigned)c.w >> 30);
       const int id[3] = {c.x, c.y, c.z};
       float P[3][3];
       for (int j = 0; j < 3; j++) {
@@ -939,6 +1099,9 @@ __global__ __launch_bounds__(256) void k_coords_apply(SgDevView d, const float *
       d.px[v] = x;
       d.py[v] = y;
       d.pz[v] = z;
+      d.qx[v] = x;
+      d.qy[v] = y;
+      d.qz[v] = z;
       d.vmark[v] = 1;
     }
   }
@@ -948,13 +1111,25 @@ __global__ __launch_bounds__(256) void k_refit_all(SgDevView d) {
   const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
   const int nw = gridDim.x * blockDim.x / SG_WAVE;
   long long nsh = 0;
-  for (int l = gw; l < d.L; l += nw) leaf_refit(d, l, nsh);
+  for (int l = gw; l < d.L; l += nw) leaf_refit_id(d, l, nsh);
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < d.C; c += gridDim.x * blockDim.x)
     d.cut_dirty[c] = 1;
 }
 
 __global__ void k_clear_marks(SgDevView d) {
   for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < d.V; v += gridDim.x * blockDim.x) d.vmark[v] = 0;
+}
+
+// copy the counters of the last dab of a stroke into its log row
+__global__ void k_log_row(const long long *ctr, long long *log, int row) {
+  const int i = threadIdx.x;
+  if (i < SG_CTR_COUNT) log[(size_t)row * SG_CTR_COUNT + i] = ctr[i];
+}
+
+int sg_enqueue_log_row(PBVH *p, long long *log, int row) {
+  hipLaunchKernelGGL(k_log_row, dim3(1), dim3(64), 0, p->stream, p->d.ctr + (size_t)(row & 1) * SG_CTR_COUNT, log,
+                     row);
+  return sg_check_hip(hipGetLastError(), "log launch");
 }
 
 // gather a device array in original vertex order (AoS float3)
@@ -983,13 +1158,15 @@ SgDevView sg_view(PBVH *p) {
   v.px = d.px;
   v.py = d.py;
   v.pz = d.pz;
+  v.qx = d.qx;
+  v.qy = d.qy;
+  v.qz = d.qz;
   v.nx = d.nx;
   v.ny = d.ny;
   v.nz = d.nz;
   v.mask = d.mask;
-  v.tri = d.tri;
+  v.vt4 = d.vt4;
   v.vt_off = d.vt_off;
-  v.vt_idx = d.vt_idx;
   v.vv_off = d.vv_off;
   v.vv_idx = d.vv_idx;
   v.leaf_vbeg = d.leaf_vbeg;
@@ -1001,9 +1178,12 @@ SgDevView sg_view(PBVH *p) {
   v.leaf_obb = d.leaf_obb;
   v.node_bb = d.node_bb;
   v.node_obb = d.node_obb;
+  v.cut_bb = d.cut_bb;
+  v.cut_obb = d.cut_obb;
   v.cut_node = d.cut_node;
   v.cut_lbeg = d.cut_lbeg;
   v.cut_lend = d.cut_lend;
+  v.cut_flags = d.cut_flags;
   v.sub_off = d.sub_off;
   v.sub_tab = d.sub_tab;
   v.top_tab = d.top_tab;
@@ -1011,17 +1191,21 @@ SgDevView sg_view(PBVH *p) {
   v.vhide = d.vhide;
   v.vmark = d.vmark;
   v.gathered = d.gathered;
-  v.affected = d.affected;
+  v.gl_v = d.gl_v;
+  v.gl_s = d.gl_s;
+  v.aff_v = d.aff_v;
+  v.aff_vt = d.aff_vt;
+  v.aff_vv = d.aff_vv;
   v.aff_fade = d.aff_fade;
-  v.slot = d.slot;
-  v.sbuf0 = d.sbuf[0];
-  v.sbuf1 = d.sbuf[1];
   v.prev_cuts = d.prev_cuts;
   v.ctr = d.ctr;
+  v.ctr_next = d.ctr + SG_CTR_COUNT;
+  v.cut_touch = d.cut_touch;
   v.vperm_old = d.vperm_old;
   v.stamp_v = d.stamp_v;
   v.stamp_v2 = d.stamp_v2;
   v.stamp_t = d.stamp_t;
+  v.stamp_a = d.stamp_a;
   return v;
 }
 
@@ -1058,28 +1242,108 @@ SgDabK sg_make_dab(const SgDabParams *dab, int stats) {
   return k;
 }
 
+// Event pair around one launch when per-kernel profiling is on.
+// Per-kernel timing: with profiling on, every launch goes through
+// hipExtLaunchKernelGGL with a start/stop event pair that the dispatch packet itself
+// timestamps, so the measured span is the kernel's execution on this stream.
+// While a stroke is being captured into a graph, profiling records plain event
+// nodes around each kernel node instead.
+struct ProfScope {
+  PBVH *p;
+  int kid;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  bool graph = false;
+  ProfScope(PBVH *pp, int k) : p(pp), kid(k) {
+    if (!p->profile) return;
+    while (p->ev_pool.size() < p->ev_used + 2) {
+      hipEvent_t e;
+      hipEventCreate(&e);
+      p->ev_pool.push_back(e);
+    }
+    e0 = p->ev_pool[p->ev_used];
+    e1 = p->ev_pool[p->ev_used + 1];
+    p->ev_used += 2;
+    p->ev_kid.push_back(kid);
+    graph = p->capturing;
+    if (graph) hipEventRecord(e0, p->stream);
+  }
+  ~ProfScope() {
+    if (graph) hipEventRecord(e1, p->stream);
+  }
+};
+
+#define SG_LAUNCH(ps, kern, grid, block, shm, stream, ...)                                   \
+  do {                                                                                      \
+    if ((ps).e0 && !(ps).graph)                                                             \
+      hipExtLaunchKernelGGL(kern, grid, block, shm, stream, (ps).e0, (ps).e1, 0, __VA_ARGS__); \
+    else                                                                                    \
+      hipLaunchKernelGGL(kern, grid, block, shm, stream, __VA_ARGS__);                      \
+  } while (0)
+
+int sg_profile_collect(PBVH *p) {
+  if (p->ev_kid.empty()) return 0;
+  if (sg_check_hip(hipStreamSynchronize(p->stream), "profile sync")) return -1;
+  for (size_t i = 0; i < p->ev_kid.size(); i++) {
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, p->ev_pool[2 * i], p->ev_pool[2 * i + 1]);
+    p->prof_ms[p->ev_kid[i]] += ms;
+    p->prof_n[p->ev_kid[i]] += 1;
+  }
+  p->ev_kid.clear();
+  p->ev_used = 0;
+  return 0;
+}
+
+int sg_enqueue_clear_ctr(PBVH *p) {
+  hipLaunchKernelGGL(k_clear_ctr, dim3(1), dim3(64), 0, p->stream, p->d.ctr);
+  return sg_check_hip(hipGetLastError(), "clear launch");
+}
+
+// One dab; `serial` is the dab's position in its stroke (selects the counter set and
+// the "touched by the previous dab" test of the search).
 int sg_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long long *log, int log_index,
-                   int stamp) {
-  const SgDevView v = sg_view(p);
+                   int stamp, int serial) {
+  SgDevView v = sg_view(p);
+  v.ctr = p->d.ctr + (size_t)(serial & 1) * SG_CTR_COUNT;
+  v.ctr_next = p->d.ctr + (size_t)((serial + 1) & 1) * SG_CTR_COUNT;
   hipStream_t s = p->stream;
   const int G = p->grid_wg;
-  hipLaunchKernelGGL(k_search, dim3(1), dim3(SEARCH_THREADS), 0, s, v, k, log, log_index);
-  if (k.tool == SG_TOOL_DRAW) hipLaunchKernelGGL(k_area, dim3(G), dim3(256), 0, s, v, k);
-  hipLaunchKernelGGL(k_brush, dim3(G), dim3(256), 0, s, v, k, curve_dev);
+  const int GB = p->grid_brush > 0 ? p->grid_brush : G;
+  const int GP = p->grid_post > 0 ? p->grid_post : G;
+  const int GS = p->grid_smooth > 0 ? p->grid_smooth : G;
+  {
+    ProfScope ps(p, SG_K_SEARCH);
+    const int sg = (p->d.C + 3) / 4;
+    SG_LAUNCH(ps, k_search, dim3(sg > 0 ? sg : 1), dim3(256), 0, s, v, k, log, log_index, serial);
+  }
+  if (k.tool == SG_TOOL_DRAW) {
+    ProfScope ps(p, SG_K_AREA);
+    SG_LAUNCH(ps, k_area, dim3(GB), dim3(256), 0, s, v, k);
+  }
+  {
+    ProfScope ps(p, SG_K_BRUSH);
+    SG_LAUNCH(ps, k_brush, dim3(GB), dim3(256), 0, s, v, k, curve_dev, stamp);
+  }
   if (k.tool == SG_TOOL_SMOOTH) {
-    if (k.iters == 1) {
-      hipLaunchKernelGGL(k_smooth_copy, dim3(G), dim3(256), 0, s, v);
-    } else {
-      int src = 0;
-      for (int it = 2; it <= k.iters; it++) {
-        hipLaunchKernelGGL(k_smooth_iter, dim3(G), dim3(256), 0, s, v, src, it == k.iters ? 1 : 0,
-                           k.stats, stamp);
-        src ^= 1;
-      }
+    // step 1 (in k_brush) wrote the shadow copy; step it reads the shadow copy when it
+    // is even and writes the positions, and the reverse when it is odd
+    for (int it = 2; it <= k.iters; it++) {
+      ProfScope ps(p, SG_K_SMOOTH);
+      SG_LAUNCH(ps, k_smooth_iter, dim3(GS), dim3(256), 0, s, v, (it % 2 == 0) ? 1 : 0, k.stats, stamp, stamp + it);
+    }
+    // odd k ends in the shadow copy: copy it back before the normals read positions;
+    // even k ends in the positions and k_post refreshes the shadow copy
+    if (k.iters % 2 == 1) {
+      ProfScope ps(p, SG_K_SMOOTH);
+      SG_LAUNCH(ps, k_smooth_sync, dim3(GS), dim3(256), 0, s, v, 1);
     }
   }
-  hipLaunchKernelGGL(k_post, dim3(G), dim3(256), 0, s, v, k.tool == SG_TOOL_SMOOTH ? 1 : 0, k.stats,
-                     stamp);
+  {
+    ProfScope ps(p, SG_K_POST);
+    SG_LAUNCH(ps, k_post, dim3(GP), dim3(256), 0, s, v, k.stats, stamp,
+              (k.tool == SG_TOOL_SMOOTH && k.iters % 2 == 0) ? 1 : 0);
+  }
+  if (p->profile && !p->capturing && p->ev_kid.size() > 4096) sg_profile_collect(p);
   return sg_check_hip(hipGetLastError(), "dab launch");
 }
 
@@ -1087,9 +1351,15 @@ int sg_enqueue_flush(PBVH *p) {
   const SgDevView v = sg_view(p);
   hipStream_t s = p->stream;
   const size_t sub_lds = (size_t)p->d.sub_maxn * 6 * sizeof(float);
-  hipLaunchKernelGGL(k_flush_sub, dim3(p->d.C), dim3(FLUSH_THREADS), sub_lds, s, v, p->d.sub_maxn);
+  {
+    ProfScope ps(p, SG_K_FLUSH_SUB);
+    SG_LAUNCH(ps, k_flush_sub, dim3(p->d.C), dim3(FLUSH_THREADS), sub_lds, s, v, p->d.sub_maxn);
+  }
   const size_t top_lds = (size_t)(p->d.C + p->d.top_n) * 6 * sizeof(float);
-  hipLaunchKernelGGL(k_flush_top, dim3(1), dim3(1024), top_lds, s, v, p->d.top_n, 1);
+  {
+    ProfScope ps(p, SG_K_FLUSH_TOP);
+    SG_LAUNCH(ps, k_flush_top, dim3(1), dim3(1024), top_lds, s, v, p->d.top_n, 1);
+  }
   return sg_check_hip(hipGetLastError(), "flush launch");
 }
 
@@ -1103,6 +1373,7 @@ int sg_enqueue_full_normals(PBVH *p, int mode) {
 int sg_enqueue_refit_list(PBVH *p, const int *leaves_dev, int n) {
   const SgDevView v = sg_view(p);
   if (n > 0) hipLaunchKernelGGL(k_refit_list, dim3(p->grid_wg), dim3(256), 0, p->stream, v, leaves_dev, n);
+  hipLaunchKernelGGL(k_cut_refresh, dim3(p->grid_wg), dim3(256), 0, p->stream, v);
   return sg_check_hip(hipGetLastError(), "refit launch");
 }
 
@@ -1117,6 +1388,7 @@ int sg_enqueue_coords_apply(PBVH *p, const float *src_dev) {
   const int blocks = (p->d.V + 255) / 256;
   hipLaunchKernelGGL(k_coords_apply, dim3(blocks > 0 ? blocks : 1), dim3(256), 0, p->stream, v, src_dev);
   hipLaunchKernelGGL(k_refit_all, dim3(p->grid_wg), dim3(256), 0, p->stream, v);
+  hipLaunchKernelGGL(k_cut_refresh, dim3(p->grid_wg), dim3(256), 0, p->stream, v);
   return sg_check_hip(hipGetLastError(), "coords launch");
 }
 

@@ -352,6 +352,9 @@ void sg_build_device(PBVH *p, const float *co) {
   upload(&d.px, x.data(), V, s);
   upload(&d.py, y.data(), V, s);
   upload(&d.pz, z.data(), V, s);
+  upload(&d.qx, x.data(), V, s);
+  upload(&d.qy, y.data(), V, s);
+  upload(&d.qz, z.data(), V, s);
   alloc_fill(&d.nx, V, 0, s);
   alloc_fill(&d.ny, V, 0, s);
   alloc_fill(&d.nz, V, 0, s);
@@ -366,7 +369,6 @@ void sg_build_device(PBVH *p, const float *co) {
                        p->perm_new_of_old[p->tris[3 * (size_t)o + 1]],
                        p->perm_new_of_old[p->tris[3 * (size_t)o + 2]], o);
   });
-  upload(&d.tri, tri.data(), T, s);
 
   // vertex -> tri CSR (ascending original tri id), entries tri | corner << 30
   std::vector<int> vt_off(V + 1, 0), vt_idx(3 * (size_t)T);
@@ -430,8 +432,18 @@ void sg_build_device(PBVH *p, const float *co) {
     }
     for (auto &h : th) h.join();
   }
+  {
+    // flattened incidences: the corners of each incident tri, in CSR order
+    std::vector<int4> vt4(3 * (size_t)T);
+    parallel_for((int)vt4.size(), [&](int q) {
+      const int e = vt_idx[q];
+      const int4 c = tri[e & 0x3fffffff];
+      vt4[q] = make_int4(c.x, c.y, c.z, e);
+    });
+    upload(&d.vt4, vt4.data(), vt4.size(), s);
+    hipStreamSynchronize(s);
+  }
   upload(&d.vt_off, vt_off.data(), V + 1, s);
-  upload(&d.vt_idx, vt_idx.data(), 3 * (size_t)T, s);
   upload(&d.vv_off, vv_off.data(), V + 1, s);
   upload(&d.vv_idx, vv_idx.data(), (size_t)vv_off[V] + 1, s);
 
@@ -522,6 +534,22 @@ void sg_build_device(PBVH *p, const float *co) {
     for (int l = p->cut.lbeg[c]; l < p->cut.lend[c]; l++) leaf_cut[l] = c;
   }
   upload(&d.cut_node, cut_nodes.data(), C, s);
+  {
+    std::vector<int> cflags(C, 0);
+    std::vector<float> cbb((size_t)C * 8, 0.0f);
+    for (int c = 0; c < C; c++) {
+      const PBVHNode &n = p->nodes[cut_nodes[c]];
+      if ((n.flag & PBVH_Leaf) && (n.flag & PBVH_FullyHidden)) cflags[c] = 1;
+      for (int k = 0; k < 3; k++) {
+        cbb[8 * (size_t)c + k] = n.vb[k];
+        cbb[8 * (size_t)c + 4 + k] = n.vb[3 + k];
+      }
+    }
+    upload(&d.cut_flags, cflags.data(), C, s);
+    upload(&d.cut_bb, cbb.data(), cbb.size(), s);
+    upload(&d.cut_obb, cbb.data(), cbb.size(), s);
+    hipStreamSynchronize(s);
+  }
   upload(&d.cut_lbeg, p->cut.lbeg.data(), C, s);
   upload(&d.cut_lend, p->cut.lend.data(), C, s);
   upload(&d.leaf_cut, leaf_cut.data(), L, s);
@@ -601,6 +629,7 @@ void sg_build_device(PBVH *p, const float *co) {
     upload(&d.top_tab, top_tab.data(), top_tab.size(), s);
   }
   alloc_fill(&d.cut_dirty, C, 0, s);
+  alloc_fill(&d.cut_touch, C, 0x80, s);  // INT_MIN-ish: never "touched by the previous dab"
   alloc_fill(&d.prev_cuts, C, 0, s);
 
   if (!p->vhide.empty()) {
@@ -611,12 +640,13 @@ void sg_build_device(PBVH *p, const float *co) {
   alloc_fill(&d.vmark, V, 0, s);
   // scratch
   alloc_fill(&d.gathered, L, 0, s);
-  alloc_fill(&d.affected, V, 0, s);
+  alloc_fill(&d.gl_v, L, 0, s);
+  alloc_fill(&d.gl_s, L, 0, s);
+  alloc_fill(&d.aff_v, V, 0, s);
+  alloc_fill(&d.aff_vt, V, 0, s);
+  alloc_fill(&d.aff_vv, V, 0, s);
   alloc_fill(&d.aff_fade, V, 0, s);
-  alloc_fill(&d.slot, V, 0xff, s);
-  alloc_fill(&d.sbuf[0], 3 * (size_t)V, 0, s);
-  alloc_fill(&d.sbuf[1], 3 * (size_t)V, 0, s);
-  alloc_fill(&d.ctr, SG_CTR_COUNT, 0, s);
+  alloc_fill(&d.ctr, 2 * SG_CTR_COUNT, 0, s);
   alloc_fill(&d.err, 4, 0, s);
   upload(&d.vperm_old, p->perm_old_of_new.data(), V, s);
   hipStreamSynchronize(s);
@@ -624,12 +654,14 @@ void sg_build_device(PBVH *p, const float *co) {
 
 void sg_free_device(PBVH *p) {
   SgDev &d = p->d;
-  void *ptrs[] = {d.px, d.py, d.pz, d.nx, d.ny, d.nz, d.mask, d.tri, d.vt_off, d.vt_idx,
-                  d.vv_off, d.vv_idx, d.leaf_vbeg, d.leaf_sh_off, d.leaf_sh, d.leaf_flags,
-                  d.leaf_cut, d.leaf_bb, d.leaf_obb, d.node_bb, d.node_obb, d.cut_node,
-                  d.cut_lbeg, d.cut_lend, d.sub_off, d.sub_tab, d.top_tab_n, d.top_tab,
-                  d.cut_dirty, d.vhide, d.vmark, d.gathered, d.affected, d.aff_fade, d.slot,
-                  d.sbuf[0], d.sbuf[1], d.prev_cuts, d.ctr, d.err, d.vperm_old};
+  void *ptrs[] = {d.px, d.py, d.pz, d.qx, d.qy, d.qz, d.nx, d.ny, d.nz, d.mask, d.vt4,
+                  d.vt_off, d.vv_off, d.vv_idx, d.leaf_vbeg, d.leaf_sh_off, d.leaf_sh,
+                  d.leaf_flags, d.leaf_cut, d.leaf_bb, d.leaf_obb, d.node_bb, d.node_obb,
+                  d.cut_bb, d.cut_obb, d.cut_flags, d.cut_node, d.cut_lbeg, d.cut_lend,
+                  d.sub_off, d.sub_tab, d.top_tab_n, d.top_tab, d.cut_dirty, d.vhide, d.vmark,
+                  d.gathered, d.gl_v, d.gl_s, d.aff_v, d.aff_vt, d.aff_vv, d.aff_fade,
+                  d.prev_cuts, d.cut_touch, d.ctr, d.err, d.vperm_old, d.stamp_v, d.stamp_v2, d.stamp_t,
+                  d.stamp_a, d.curve, d.log};
   for (void *q : ptrs) {
     if (q) hipFree(q);
   }

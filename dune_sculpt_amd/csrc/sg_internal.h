@@ -39,10 +39,11 @@ struct PBVHNode {
 struct SgDev {
   int V = 0, T = 0, L = 0, N = 0, C = 0;
   float *px = nullptr, *py = nullptr, *pz = nullptr; // positions [V]
+  float *qx = nullptr, *qy = nullptr, *qz = nullptr; // shadow copy (== positions between dabs)
   float *nx = nullptr, *ny = nullptr, *nz = nullptr; // normals [V]
   float *mask = nullptr;                             // [V] or null
-  int4 *tri = nullptr;          // [T] corners (new vertex ids), .w = original tri id
-  int *vt_off = nullptr, *vt_idx = nullptr; // vertex -> (tri | corner << 30), asc. orig tri
+  int *vt_off = nullptr;        // [V+1] vertex -> incident tris, ascending original tri id
+  int4 *vt4 = nullptr;          // [3T] per incidence: tri corners (new ids), .w = tri | corner << 30
   int *vv_off = nullptr, *vv_idx = nullptr; // vertex -> edge neighbours, asc. orig id
   int *leaf_vbeg = nullptr;     // [L+1] uniq vertex range of each leaf
   int *leaf_sh_off = nullptr;   // [L+1]
@@ -54,6 +55,8 @@ struct SgDev {
   float *node_bb = nullptr;     // [N*8] internal nodes (and cut leaves)
   float *node_obb = nullptr;
   int *cut_node = nullptr, *cut_lbeg = nullptr, *cut_lend = nullptr; // [C]
+  int *cut_flags = nullptr;     // [C] bit0: single hidden leaf
+  float *cut_bb = nullptr, *cut_obb = nullptr; // [C*8] search boxes of the cut subtrees
   // deferred refit tables: per cut, its internal nodes by height (int4: node id,
   // local child 0, local child 1, height) and the top tree above the cut
   int *sub_off = nullptr;       // [C+1]
@@ -62,19 +65,20 @@ struct SgDev {
   int4 *top_tab = nullptr;
   int top_n = 0, top_maxh = 0, sub_maxn = 0;
   int *cut_dirty = nullptr;     // [C]
+  int *cut_touch = nullptr;     // [C] serial of the last dab that gathered a leaf of it
   int *vhide = nullptr;         // [V] or null
   uint8_t *vmark = nullptr;     // [V] host-driven normal update marks
   // per-dab scratch
   int *gathered = nullptr;      // [L] gathered leaves (ascending)
-  int *affected = nullptr;      // [V]
-  float *aff_fade = nullptr;    // [V]
-  int *slot = nullptr;          // [V] -1 unless affected by the running Smooth dab
-  float *sbuf[2] = {nullptr, nullptr}; // [3V] each, compact Jacobi iterates (SoA x|y|z)
+  int2 *gl_v = nullptr, *gl_s = nullptr; // [L] their uniq / shared vertex ranges
+  int *aff_v = nullptr;         // [V] affected vertices of the running dab
+  int2 *aff_vt = nullptr, *aff_vv = nullptr; // [V] their vertex->tri / vertex->vertex ranges
+  float *aff_fade = nullptr;    // [V] Smooth factor
   int *prev_cuts = nullptr;     // [C] cuts refit by the previous dab (+ count in ctr)
-  long long *ctr = nullptr;     // counters, see SG_CTR_*
+  long long *ctr = nullptr;     // two counter sets (even / odd dabs), see SG_CTR_*
   int *err = nullptr;           // device error word
   int *vperm_old = nullptr;     // [V] new -> old id (device copy)
-  int *stamp_v = nullptr, *stamp_v2 = nullptr, *stamp_t = nullptr; // stats mode only
+  int *stamp_v = nullptr, *stamp_v2 = nullptr, *stamp_t = nullptr, *stamp_a = nullptr; // stats mode
   float *curve = nullptr;       // custom falloff tables for queued dabs
   int curve_cap = 0;
   long long *log = nullptr;     // per-dab counter log for strokes
@@ -129,6 +133,18 @@ struct PBVH {
   long long last_ctr[16] = {0};
   int dab_count = 0;
   int grid_wg = 256;                 // workgroups of the per-dab kernels (tunable)
+  int grid_brush = 0, grid_post = 0, grid_smooth = 0;  // per-kernel overrides (autotuner)
+  // device snapshot (checkpoint) of the mutable mesh state
+  std::vector<void *> snap;
+  bool snap_valid = false;
+  // per-kernel event timing (sg_pbvh_set_profile)
+  bool profile = false;
+  bool capturing = false;            // a stroke is being captured (profiling uses event nodes)
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::vector<int> ev_kid;           // kernel id per recorded pair (events 2i, 2i+1)
+  double prof_ms[16] = {0};
+  long long prof_n[16] = {0};
   int stamp = 0;                     // stats-mode stamp generator
 };
 

@@ -15,24 +15,26 @@ enum {
 // Plain-pointer view of the device mesh passed to kernels by value.
 struct SgDevView {
   int V, T, L, N, C, respect_hide;
-  float *px, *py, *pz, *nx, *ny, *nz, *mask;
-  const int4 *tri;
-  const int *vt_off, *vt_idx, *vv_off, *vv_idx;
+  float *px, *py, *pz, *qx, *qy, *qz, *nx, *ny, *nz, *mask;
+  const int4 *vt4;
+  const int *vt_off, *vv_off, *vv_idx;
   const int *leaf_vbeg, *leaf_sh_off, *leaf_sh, *leaf_flags, *leaf_cut;
-  float *leaf_bb, *leaf_obb, *node_bb, *node_obb;
-  const int *cut_node, *cut_lbeg, *cut_lend, *sub_off;
+  float *leaf_bb, *leaf_obb, *node_bb, *node_obb, *cut_bb, *cut_obb;
+  const int *cut_node, *cut_lbeg, *cut_lend, *cut_flags, *sub_off;
   const int4 *sub_tab, *top_tab;
   int *cut_dirty;
   const int *vhide;
   uint8_t *vmark;
-  int *gathered, *affected;
+  int *gathered;
+  int2 *gl_v, *gl_s;
+  int *aff_v;
+  int2 *aff_vt, *aff_vv;
   float *aff_fade;
-  int *slot;
-  float *sbuf0, *sbuf1;
   int *prev_cuts;
-  long long *ctr;
+  int *cut_touch;
+  long long *ctr, *ctr_next;
   const int *vperm_old;
-  int *stamp_v, *stamp_v2, *stamp_t;
+  int *stamp_v, *stamp_v2, *stamp_t, *stamp_a;
 };
 
 // Per-dab constants (spec S1-S9), computed on the host in the same FP32 order as the
@@ -43,11 +45,18 @@ struct SgDabK {
   int tool, falloff, iters, use_orig, stats;
 };
 
+// kernel ids for the per-kernel profile
+enum { SG_K_SEARCH = 0, SG_K_AREA = 1, SG_K_BRUSH = 2, SG_K_SMOOTH = 3, SG_K_POST = 4, SG_K_FLUSH_SUB = 5,
+       SG_K_FLUSH_TOP = 6, SG_K_COUNT = 7 };
+
 SgDevView sg_view(PBVH *p);
+int sg_profile_collect(PBVH *p);
 SgDabK sg_make_dab(const SgDabParams *dab, int stats);
 int sg_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long long *log, int log_index,
-                   int stamp);
+                   int stamp, int serial);
+int sg_enqueue_clear_ctr(PBVH *p);
 int sg_enqueue_flush(PBVH *p);
+int sg_enqueue_log_row(PBVH *p, long long *log, int row);
 int sg_enqueue_full_normals(PBVH *p, int mode);
 int sg_enqueue_refit_list(PBVH *p, const int *leaves_dev, int n);
 int sg_enqueue_normals_list(PBVH *p, const int *leaves_dev, int n);

@@ -200,6 +200,13 @@ int sg_dab_apply(PBVH *pbvh, const SgDabParams *dab);
  * SG_STROKE_LOG keeps per-dab counters for sg_dab_stroke_log. */
 enum { SG_STROKE_GRAPH = 1, SG_STROKE_REFIT_EACH = 2, SG_STROKE_LOG = 4 };
 int sg_dab_apply_stroke(PBVH *pbvh, const SgDabParams *dabs, int n, int flags);
+/* A stroke captured once into a hipGraph (all its dabs + the final refit) and
+ * launched any number of times on the PBVH's stream (each launch applies the dabs
+ * again to the current mesh).  sync != 0 waits for completion. */
+typedef struct SgStroke SgStroke;
+SgStroke *sg_stroke_prepare(PBVH *pbvh, const SgDabParams *dabs, int n, int flags);
+int sg_stroke_launch(SgStroke *stroke, int sync);
+void sg_stroke_free(SgStroke *stroke);
 /* Per-dab counters (16 per dab, layout of sg_dab_last_stats) of the last stroke. */
 int sg_dab_stroke_log(PBVH *pbvh, long long *out, int n);
 /* Block until queued work is done; returns the first recorded device error. */
@@ -222,6 +229,24 @@ int sg_dab_last_gathered(PBVH *pbvh, int *out, int cap);
 int sg_pbvh_set_stats(PBVH *pbvh, int on);
 /* Workgroups launched by the per-dab kernels (autotuned by the host layer). */
 int sg_pbvh_set_grid(PBVH *pbvh, int workgroups);
+/* Workgroups of the brush / post / smooth-step kernels (0 = the common grid). */
+int sg_pbvh_set_kernel_grids(PBVH *pbvh, int brush, int post, int smooth);
+int sg_pbvh_get_kernel_grids(PBVH *pbvh, int out[4]);
+/* Autotuner: times a sample of dabs from the current mesh state (restored after each
+ * trial) for candidate workgroup counts of each per-dab kernel and keeps the fastest.
+ * out_ms (may be NULL) receives {time before, time after} of the sample in ms. */
+int sg_pbvh_autotune(PBVH *pbvh, const SgDabParams *dabs, int n, double *out_ms);
+/* Checkpoint of the mutable device state (positions, normals, all node bounds):
+ * save copies it in HBM, restore puts the mesh back exactly (undo / stroke cancel /
+ * replaying a stroke on the same starting mesh). */
+int sg_pbvh_snapshot_save(PBVH *pbvh);
+int sg_pbvh_snapshot_restore(PBVH *pbvh);
+/* Per-kernel timing with HIP events on the PBVH's stream (stream launches only).
+ * sg_pbvh_profile_read returns the number of kernel kinds and fills total ms and
+ * launch counts for: 0 search, 1 area normal, 2 brush, 3 smooth step, 4 post
+ * (normals + leaf bounds), 5 subtree refit, 6 top refit. */
+int sg_pbvh_set_profile(PBVH *pbvh, int on);
+int sg_pbvh_profile_read(PBVH *pbvh, double *ms, long long *count, int n, int reset);
 
 /* Copies of the device state in original vertex order (AoS float[3]). */
 int sg_pbvh_read_positions(PBVH *pbvh, float *out);

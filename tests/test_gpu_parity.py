@@ -123,3 +123,61 @@ def test_initial_normals_match_oracle(sg, po):
     p = sg.PBVH(co, tris, leaf_limit=128)
     want = po.initial_normals(co, tris)
     assert rel_err(p.normals(), want) <= REL_TOL
+
+
+def test_prepared_stroke_replay(sg, po):
+    """sg_stroke_prepare captures once; each launch applies the dabs again."""
+    co, tris = sg.noisy_icosphere(5, seed=21, amplitude=0.02)
+    n = 32
+    rng = np.random.default_rng(22)
+    radii = rng.uniform(0.05, 0.15, n).astype(np.float32)
+    centres = sg.great_circle_centres(23, radii)
+    dabs = sg.stroke_config([1, 4, 2, 5] * (n // 4), centres, radii, rng.uniform(0.2, 1.0, n), smooth_iterations=4)
+    p = sg.PBVH(co, tris, leaf_limit=128)
+    o = po.Oracle(co, tris, leaf_limit=128)
+    o.set_normals(p.normals())
+    st = p.prepare(dabs, log=True)
+    want_aff = []
+    for rep in range(2):
+        st.launch()
+        for d in dabs:
+            want_aff.append(len(o.dab(d)))
+    logrows = p.stroke_log(n)
+    assert list(logrows[:, 1]) == want_aff[n:]
+    assert rel_err(p.positions(), o.positions()) <= REL_TOL
+    assert rel_err(p.normals(), o.normals()) <= REL_TOL
+    for i in range(o.totnode()):
+        assert np.array_equal(p.node_bb(i).view(np.uint32), o.node_bb(i)[0].view(np.uint32)), i
+    st.close()
+    p.close()
+
+
+def test_snapshot_restore_and_autotune_keep_results(sg):
+    """Checkpoint restore is exact; the autotuner changes launch shapes, never results."""
+    co, tris = sg.noisy_icosphere(5, seed=31, amplitude=0.02)
+    n = 40
+    rng = np.random.default_rng(32)
+    radii = rng.uniform(0.05, 0.15, n).astype(np.float32)
+    centres = sg.great_circle_centres(33, radii)
+    dabs = sg.stroke_config([1, 4, 2, 5] * (n // 4), centres, radii, rng.uniform(0.2, 1.0, n), smooth_iterations=4)
+    p = sg.PBVH(co, tris, leaf_limit=128)
+    p0, n0 = p.positions(), p.normals()
+    bb0 = [p.node_bb(i) for i in range(p.tree_info()["nodes"])]
+    p.snapshot()
+    p.stroke(dabs)
+    ref_pos, ref_nor = p.positions(), p.normals()
+    p.restore()
+    assert np.array_equal(p.positions().view(np.uint32), p0.view(np.uint32))
+    assert np.array_equal(p.normals().view(np.uint32), n0.view(np.uint32))
+    for i, b in enumerate(bb0):
+        assert np.array_equal(p.node_bb(i).view(np.uint32), b.view(np.uint32))
+    before, after = p.autotune(dabs[:20])
+    assert after <= before * 1.05
+    assert np.array_equal(p.positions().view(np.uint32), p0.view(np.uint32))  # tuner restores
+    for grids in [(32, 64, 128), (512, 32, 64)]:
+        p.restore()
+        p.set_kernel_grids(*grids)
+        p.stroke(dabs)
+        assert np.array_equal(p.positions().view(np.uint32), ref_pos.view(np.uint32))
+        assert np.array_equal(p.normals().view(np.uint32), ref_nor.view(np.uint32))
+    p.close()
