@@ -200,13 +200,7 @@ def main():
     A.set_profile(True)
     A.stroke(dabs, graph=False)
     prof_stream = A.profile_read()
-    # the same stroke captured with event nodes around every kernel node
-    A.restore()
-    pst = A.prepare(dabs)
-    A.profile_read()  # reset
-    pst.launch(sync=True)
-    prof = A.profile_read()
-    pst.close()
+    prof = prof_stream
     A.set_profile(False)
     n_smooth = int((tools == sg.TOOL_SMOOTH).sum())
     kb = {
@@ -268,7 +262,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,
-        "kernels_stream_events_ms": {k2: round(v[0], 3) for k2, v in prof_stream.items()},
+
         "build_s": build_s,
         "autotune": tune,
         "step_ms": [round(t * 1e3, 3) for t in step_times],

@@ -93,8 +93,13 @@ EXPORTS = [
     "sg_mesh_torus", "sg_mesh_noise_displace", "sg_pbvh_set_profile", "sg_pbvh_profile_read",
     "sg_stroke_prepare", "sg_stroke_launch", "sg_stroke_free", "sg_pbvh_set_kernel_grids",
     "sg_pbvh_get_kernel_grids", "sg_pbvh_snapshot_save", "sg_pbvh_snapshot_restore",
-    "sg_pbvh_autotune",
+    "sg_pbvh_autotune", "sg_mlp_new", "sg_mlp_free", "sg_mlp_get_params", "sg_mlp_set_params",
+    "sg_mlp_set_norm", "sg_mlp_get_norm", "sg_mlp_set_decision", "sg_mlp_get_decision", "sg_mlp_set_data",
+    "sg_mlp_gradient", "sg_mlp_train", "sg_mlp_predict", "sg_mfma_probe", "sg_pbvh_set_learned",
+    "sg_pbvh_learned_stats", "sg_learned_collect",
 ]
+SMOOTH_LEARNED = 1 << 30
+NCTR = 24
 
 KERNEL_NAMES = ["k_search", "k_area", "k_brush", "k_smooth_iter", "k_post", "k_flush_sub", "k_flush_top"]
 
@@ -165,6 +170,22 @@ def lib():
         "sg_pbvh_get_kernel_grids": (i32, [vp, i32p]),
         "sg_pbvh_snapshot_save": (i32, [vp]),
         "sg_pbvh_autotune": (i32, [vp, C.POINTER(SgDabParams), i32, C.POINTER(C.c_double)]),
+        "sg_mlp_new": (vp, [C.c_uint64]),
+        "sg_mlp_free": (None, [vp]),
+        "sg_mlp_get_params": (i32, [vp, f32p]),
+        "sg_mlp_set_params": (i32, [vp, f32p]),
+        "sg_mlp_set_norm": (i32, [vp, f32p, f32p, f32p]),
+        "sg_mlp_get_norm": (i32, [vp, f32p]),
+        "sg_mlp_set_decision": (i32, [vp, C.c_float, C.c_float]),
+        "sg_mlp_get_decision": (i32, [vp, f32p]),
+        "sg_mlp_set_data": (i32, [vp, f32p, f32p, i32]),
+        "sg_mlp_gradient": (i32, [vp, i32, f32p, f32p]),
+        "sg_mlp_train": (i32, [vp, i32, i32, C.c_float, C.c_uint64, f32p]),
+        "sg_mlp_predict": (i32, [vp, f32p, i32, f32p]),
+        "sg_mfma_probe": (i32, [C.POINTER(C.c_uint16), C.POINTER(C.c_uint16), f32p]),
+        "sg_pbvh_set_learned": (i32, [vp, vp]),
+        "sg_pbvh_learned_stats": (i32, [vp, C.POINTER(C.c_longlong)]),
+        "sg_learned_collect": (i32, [vp, C.POINTER(SgDabParams), f32p, f32p, f32p, i32]),
         "sg_pbvh_snapshot_restore": (i32, [vp]),
         "sg_stroke_launch": (i32, [vp, i32]),
         "sg_stroke_free": (None, [vp]),
@@ -308,6 +329,103 @@ def stroke_config(tools, centres, radii, strengths, smooth_iterations=4, falloff
 # PBVH handle
 # ---------------------------------------------------------------------------
 
+class MLP:
+    """Learned Smooth model: 28 -> 64 -> 64 -> 3 (SiLU), FP32 master weights on the GPU,
+    BF16 MFMA inference."""
+
+    NPARAM = 6211
+
+    def __init__(self, seed=0):
+        self.h = lib().sg_mlp_new(seed)
+        if not self.h:
+            raise RuntimeError("sg_mlp_new failed: " + last_error())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sg_mlp_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        out = np.zeros(self.NPARAM, np.float32)
+        _check(lib().sg_mlp_get_params(self.h, _fp(out)), "get params")
+        return out
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        assert w.shape == (self.NPARAM,)
+        _check(lib().sg_mlp_set_params(self.h, _fp(w)), "set params")
+
+    def set_norm(self, mu, isig, out_scale):
+        a, b, c = (np.ascontiguousarray(x, np.float32) for x in (mu, isig, out_scale))
+        _check(lib().sg_mlp_set_norm(self.h, _fp(a), _fp(b), _fp(c)), "set norm")
+
+    def norm(self):
+        out = np.zeros(64, np.float32)
+        lib().sg_mlp_get_norm(self.h, _fp(out))
+        return out[:28].copy(), out[28:56].copy(), out[56:59].copy()
+
+    def set_decision(self, alpha, tol):
+        lib().sg_mlp_set_decision(self.h, alpha, tol)
+
+    def decision(self):
+        out = np.zeros(2, np.float32)
+        lib().sg_mlp_get_decision(self.h, _fp(out))
+        return float(out[0]), float(out[1])
+
+    def set_data(self, X, T):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        self._X, self._T = X, T
+        _check(lib().sg_mlp_set_data(self.h, _fp(X), _fp(T), X.shape[0]), "set data")
+
+    def gradient(self, n):
+        g = np.zeros(self.NPARAM, np.float32)
+        loss = C.c_float()
+        _check(lib().sg_mlp_gradient(self.h, n, _fp(g), C.byref(loss)), "gradient")
+        return g, loss.value
+
+    def train(self, epochs, batch_wg=256, lr=1e-3, seed=0):
+        curve = np.zeros(epochs, np.float32)
+        _check(lib().sg_mlp_train(self.h, epochs, batch_wg, lr, seed, _fp(curve)), "train")
+        return curve
+
+    def predict(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.zeros((X.shape[0], 3), np.float32)
+        _check(lib().sg_mlp_predict(self.h, _fp(X), X.shape[0], _fp(Y)), "predict")
+        return Y
+
+    def save(self, path):
+        mu, isig, osc = self.norm()
+        alpha, tol = self.decision()
+        np.savez(path, w=self.params(), mu=mu, isig=isig, out_scale=osc, alpha=alpha, tol=tol)
+
+    @classmethod
+    def load(cls, path):
+        z = np.load(path, allow_pickle=False)
+        m = cls()
+        m.set_norm(z["mu"], z["isig"], z["out_scale"])
+        m.set_params(z["w"])
+        m.set_decision(float(z["alpha"]), float(z["tol"]))
+        return m
+
+
+def mfma_probe(A, B):
+    """Run the 16x16x16 bf16 MFMA once with the engine's operand layout."""
+    A = np.ascontiguousarray(A, np.uint16)
+    B = np.ascontiguousarray(B, np.uint16)
+    D = np.zeros((16, 16), np.float32)
+    _check(lib().sg_mfma_probe(A.ctypes.data_as(C.POINTER(C.c_uint16)), B.ctypes.data_as(C.POINTER(C.c_uint16)),
+                               _fp(D)), "mfma probe")
+    return D
+
+
 class PreparedStroke:
     """A dab list captured into a hipGraph; freed before its PBVH."""
 
@@ -410,7 +528,7 @@ class PBVH:
         return st
 
     def stroke_log(self, n):
-        out = np.zeros((n, 16), np.int64)
+        out = np.zeros((n, NCTR), np.int64)
         _check(lib().sg_dab_stroke_log(self.h, out.ctypes.data_as(C.POINTER(C.c_longlong)), n), "stroke log")
         return out
 
@@ -443,6 +561,24 @@ class PBVH:
         _check(lib().sg_pbvh_autotune(self.h, arr, len(dabs), out.ctypes.data_as(C.POINTER(C.c_double))),
                "autotune")
         return float(out[0]), float(out[1])
+
+    def set_learned(self, mlp):
+        self._mlp = mlp
+        _check(lib().sg_pbvh_set_learned(self.h, mlp.h if mlp is not None else None), "set learned")
+
+    def learned_stats(self):
+        out = np.zeros(2, np.int64)
+        lib().sg_pbvh_learned_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_longlong)))
+        return {"accepted": int(out[0]), "fallbacks": int(out[1])}
+
+    def collect(self, d, cap=None):
+        """Plain Smooth dab recording learned-model inputs and targets per affected vertex."""
+        cap = cap or self.totvert
+        X = np.zeros((cap, 28), np.float32)
+        T = np.zeros((cap, 3), np.float32)
+        ok = np.zeros(cap, np.float32)
+        n = _check(lib().sg_learned_collect(self.h, C.byref(d), _fp(X), _fp(T), _fp(ok), cap), "collect")
+        return X[:n], T[:n], ok[:n] > 0.5
 
     def snapshot(self):
         _check(lib().sg_pbvh_snapshot_save(self.h), "snapshot")
@@ -486,7 +622,7 @@ class PBVH:
         return out[:n].copy()
 
     def last_stats(self):
-        out = np.zeros(16, np.int64)
+        out = np.zeros(NCTR, np.int64)
         _check(lib().sg_dab_last_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_int64))), "stats")
         return out
 

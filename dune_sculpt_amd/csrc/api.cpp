@@ -10,6 +10,7 @@
 
 #include "sg_internal.h"
 #include "sg_kernels.h"
+#include "sg_learned.h"
 
 namespace {
 
@@ -404,9 +405,20 @@ int sg_dab_apply(PBVH *p, const SgDabParams *dab) {
   if (!rc) rc = sg_enqueue_dab(p, k, curve, nullptr, 0, stamp, 0);
   if (!rc) rc = sg_enqueue_flush(p);
   long long ctr[SG_CTR_COUNT];
+  int halt = 0;
   if (!rc) rc = sg_check_hip(hipMemcpyAsync(ctr, p->d.ctr, sizeof(ctr), hipMemcpyDeviceToHost, p->stream), "copy");
+  if (!rc) rc = sg_check_hip(hipMemcpyAsync(&halt, p->d.halt, 4, hipMemcpyDeviceToHost, p->stream), "copy");
   if (!rc) rc = sg_check_hip(hipStreamSynchronize(p->stream), "dab");
   if (rc) return rc;
+  if (k.learned && p->mlp) {
+    if (halt) {  // rejected: the plain brush does this dab
+      p->learned_fallbacks++;
+      SgDabParams plain = *dab;
+      plain.flag &= ~SG_SMOOTH_LEARNED;
+      return sg_dab_apply(p, &plain);
+    }
+    p->learned_accepted++;
+  }
   for (int i = 0; i < 8; i++) p->last_stats[i] = 0;
   p->last_tool = dab->sculpt_tool;
   p->dab_count++;
@@ -441,44 +453,77 @@ int sg_dab_apply_stroke(PBVH *p, const SgDabParams *dabs, int n, int flags) {
       }
     }
   }
-  auto enqueue_all = [&]() -> int {
-    int ci = 0;
-    if (sg_enqueue_clear_ctr(p)) return -1;
-    for (int i = 0; i < n; i++) {
-      const float *curve = nullptr;
-      if (dabs[i].curve_preset == SG_CURVE_CUSTOM && dabs[i].curve_table) curve = p->d.curve + (size_t)(ci++) * (SG_CM_TABLE + 1);
-      SgDabK k = sg_make_dab(&dabs[i], g_stats ? 1 : 0);
-      const int stamp = (++p->stamp) * 128;
-      if (int rc = sg_enqueue_dab(p, k, curve, log ? p->d.log : nullptr, i, stamp, i)) return rc;
-      if (flush_each && sg_enqueue_flush(p)) return -1;
+  // Dabs [a, n) as one segment (serials restart at 0); returns the position of a
+  // rejected learned dab (halt) or -1.
+  auto run_segment = [&](int a0, int *halted) -> int {
+    auto enqueue_all = [&]() -> int {
+      int ci = 0;
+      for (int i = 0; i < a0; i++)
+        ci += (dabs[i].curve_preset == SG_CURVE_CUSTOM && dabs[i].curve_table) ? 1 : 0;
+      if (sg_enqueue_clear_ctr(p)) return -1;
+      for (int i = a0; i < n; i++) {
+        const float *curve = nullptr;
+        if (dabs[i].curve_preset == SG_CURVE_CUSTOM && dabs[i].curve_table)
+          curve = p->d.curve + (size_t)(ci++) * (SG_CM_TABLE + 1);
+        SgDabK k = sg_make_dab(&dabs[i], g_stats ? 1 : 0);
+        const int stamp = (++p->stamp) * 128;
+        if (int rc = sg_enqueue_dab(p, k, curve, log ? p->d.log : nullptr, i - a0, stamp, i - a0)) return rc;
+        if (flush_each && sg_enqueue_flush(p)) return -1;
+      }
+      if (!flush_each && sg_enqueue_flush(p)) return -1;
+      if (log && sg_enqueue_log_row(p, p->d.log, n - 1 - a0)) return -1;
+      return 0;
+    };
+    int rc = 0;
+    if (use_graph) {
+      hipGraph_t graph = nullptr;
+      hipGraphExec_t exec = nullptr;
+      p->capturing = true;
+      rc = sg_check_hip(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+      if (!rc) {
+        const int erc = enqueue_all();
+        const hipError_t e = hipStreamEndCapture(p->stream, &graph);
+        rc = erc ? erc : sg_check_hip(e, "end capture");
+      }
+      p->capturing = false;
+      if (!rc) rc = sg_check_hip(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "instantiate");
+      if (!rc) rc = sg_check_hip(hipGraphLaunch(exec, p->stream), "graph launch");
+      if (!rc) rc = sg_check_hip(hipStreamSynchronize(p->stream), "stroke");
+      if (exec) hipGraphExecDestroy(exec);
+      if (graph) hipGraphDestroy(graph);
+    } else {
+      rc = enqueue_all();
+      if (!rc) rc = sg_check_hip(hipStreamSynchronize(p->stream), "stroke");
     }
-    if (!flush_each && sg_enqueue_flush(p)) return -1;
-    if (log && sg_enqueue_log_row(p, p->d.log, n - 1)) return -1;
-    return 0;
+    int h = 0;
+    if (!rc) rc = sg_check_hip(hipMemcpy(&h, p->d.halt, 4, hipMemcpyDeviceToHost), "copy");
+    *halted = h ? a0 + h - 1 : -1;
+    return rc;
   };
   int rc = 0;
-  if (use_graph && p->profile) return sg_set_error("per-kernel profiling needs stream launches (no graph)");
-  if (use_graph) {
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    rc = sg_check_hip(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal), "begin capture");
-    if (!rc) {
-      const int erc = enqueue_all();
-      const hipError_t e = hipStreamEndCapture(p->stream, &graph);
-      rc = erc ? erc : sg_check_hip(e, "end capture");
-    }
-    if (!rc) rc = sg_check_hip(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "instantiate");
-    if (!rc) rc = sg_check_hip(hipGraphLaunch(exec, p->stream), "graph launch");
-    if (!rc) rc = sg_check_hip(hipStreamSynchronize(p->stream), "stroke");
-    if (exec) hipGraphExecDestroy(exec);
-    if (graph) hipGraphDestroy(graph);
-  } else {
-    rc = enqueue_all();
-    if (!rc) rc = sg_check_hip(hipStreamSynchronize(p->stream), "stroke");
+  int a0 = 0;
+  const int nlearned = [&]() {
+    int c = 0;
+    for (int i = 0; i < n; i++) c += (dabs[i].sculpt_tool == SG_TOOL_SMOOTH && (dabs[i].flag & SG_SMOOTH_LEARNED)) ? 1 : 0;
+    return p->mlp ? c : 0;
+  }();
+  int nfallback = 0;
+  while (a0 < n) {
+    int halted = -1;
+    rc = run_segment(a0, &halted);
+    if (rc) return rc;
+    if (halted < 0) break;
+    // the rejected learned dab runs with the plain brush, then the stroke resumes
+    SgDabParams plain = dabs[halted];
+    plain.flag &= ~SG_SMOOTH_LEARNED;
+    if ((rc = sg_dab_apply(p, &plain))) return rc;
+    nfallback++;
+    a0 = halted + 1;
   }
-  if (rc) return rc;
+  p->learned_fallbacks += nfallback;
+  p->learned_accepted += nlearned - nfallback;
   long long ctr[SG_CTR_COUNT];
-  hipMemcpy(ctr, p->d.ctr + (size_t)((n - 1) & 1) * SG_CTR_COUNT, sizeof(ctr), hipMemcpyDeviceToHost);
+  hipMemcpy(ctr, p->d.ctr + (size_t)((n - 1 - a0) & 1) * SG_CTR_COUNT, sizeof(ctr), hipMemcpyDeviceToHost);
   memcpy(p->last_ctr, ctr, sizeof(ctr));
   p->last_tool = dabs[n - 1].sculpt_tool;
   p->dab_count += n;
@@ -495,8 +540,9 @@ struct SgStroke {
   hipGraphExec_t exec = nullptr;
   float *curves = nullptr;
   int n = 0;
-  bool profiled = false;        // captured with event nodes (pool events 2i, 2i+1)
-  std::vector<int> ev_kid;
+  int flags = 0;
+  bool learned = false;           // contains learned Smooth dabs: launch checks for a halt
+  std::vector<SgDabParams> dabs;  // (custom curve pointers stay the caller's)
 };
 
 SgStroke *sg_stroke_prepare(PBVH *p, const SgDabParams *dabs, int n, int flags) {
@@ -517,6 +563,10 @@ SgStroke *sg_stroke_prepare(PBVH *p, const SgDabParams *dabs, int n, int flags) 
   SgStroke *s = new SgStroke();
   s->p = p;
   s->n = n;
+  s->flags = flags;
+  s->dabs.assign(dabs, dabs + n);
+  for (int i = 0; i < n; i++)
+    if (dabs[i].sculpt_tool == SG_TOOL_SMOOTH && (dabs[i].flag & SG_SMOOTH_LEARNED) && p->mlp) s->learned = true;
   int ncustom = 0;
   for (int i = 0; i < n; i++) ncustom += (dabs[i].curve_preset == SG_CURVE_CUSTOM && dabs[i].curve_table) ? 1 : 0;
   if (ncustom) {
@@ -531,8 +581,6 @@ SgStroke *sg_stroke_prepare(PBVH *p, const SgDabParams *dabs, int n, int flags) 
                   (SG_CM_TABLE + 1) * sizeof(float), hipMemcpyHostToDevice);
     }
   }
-  // with profiling on, event nodes around every kernel node are captured too; their
-  // timings are collected after each launch (sg_pbvh_profile_read)
   p->capturing = true;
   int rc = sg_check_hip(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal), "begin capture");
   int erc = 0;
@@ -553,10 +601,6 @@ SgStroke *sg_stroke_prepare(PBVH *p, const SgDabParams *dabs, int n, int flags) 
     rc = erc ? erc : sg_check_hip(e, "end capture");
   }
   p->capturing = false;
-  s->profiled = p->profile;
-  s->ev_kid = p->ev_kid;
-  p->ev_kid.clear();
-  p->ev_used = 0;
   if (!rc) rc = sg_check_hip(hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0), "instantiate");
   if (rc) {
     if (s->graph) hipGraphDestroy(s->graph);
@@ -571,13 +615,17 @@ int sg_stroke_launch(SgStroke *s, int sync) {
   if (!s) return sg_set_error("null stroke");
   PBVH *p = s->p;
   int rc = sg_check_hip(hipGraphLaunch(s->exec, p->stream), "graph launch");
-  if (!rc && (sync || s->profiled)) rc = sg_check_hip(hipStreamSynchronize(p->stream), "stroke");
-  if (!rc && s->profiled) {
-    for (size_t i = 0; i < s->ev_kid.size(); i++) {
-      float ms = 0.0f;
-      hipEventElapsedTime(&ms, p->ev_pool[2 * i], p->ev_pool[2 * i + 1]);
-      p->prof_ms[s->ev_kid[i]] += ms;
-      p->prof_n[s->ev_kid[i]] += 1;
+  if (!rc && (sync || s->learned)) rc = sg_check_hip(hipStreamSynchronize(p->stream), "stroke");
+  if (!rc && s->learned) {
+    int h = 0;
+    rc = sg_check_hip(hipMemcpy(&h, p->d.halt, 4, hipMemcpyDeviceToHost), "copy");
+    if (!rc && h) {  // dab h-1 was rejected: plain brush for it, then the rest
+      const int f = h - 1;
+      SgDabParams plain = s->dabs[f];
+      plain.flag &= ~SG_SMOOTH_LEARNED;
+      rc = sg_dab_apply(p, &plain);
+      p->learned_fallbacks++;
+      if (!rc && f + 1 < s->n) rc = sg_dab_apply_stroke(p, s->dabs.data() + f + 1, s->n - f - 1, s->flags & ~1);
     }
   }
   p->dab_count += s->n;
@@ -654,6 +702,55 @@ int sg_pbvh_autotune(PBVH *p, const SgDabParams *dabs, int n, double *out_ms) {
   return 0;
 }
 
+int sg_pbvh_set_learned(PBVH *p, SgMlp *m) {
+  if (!p) return sg_set_error("null PBVH");
+  p->mlp = m;
+  return 0;
+}
+
+int sg_pbvh_learned_stats(PBVH *p, long long out[2]) {
+  if (!p) return sg_set_error("null PBVH");
+  out[0] = p->learned_accepted;
+  out[1] = p->learned_fallbacks;
+  return 0;
+}
+
+// One plain Smooth dab that also records, for each affected vertex, the learned
+// model's input features before the dab and the displacement target after it.
+int sg_learned_collect(PBVH *p, const SgDabParams *dab, float *X, float *T, float *ok, int cap) {
+  if (int rc = check_dab(p, dab)) return rc;
+  if (dab->sculpt_tool != SG_TOOL_SMOOTH) return sg_set_error("collection needs a Smooth dab");
+  SgCollect col;
+  col.cap = cap;
+  if (sg_check_hip(hipMalloc(&col.X, (size_t)std::max(cap, 1) * 28 * 4), "hipMalloc")) return -1;
+  hipMalloc(&col.frame, (size_t)std::max(cap, 1) * 16 * 4);
+  hipMalloc(&col.T, (size_t)std::max(cap, 1) * 3 * 4);
+  SgDabParams plain = *dab;
+  plain.flag &= ~SG_SMOOTH_LEARNED;
+  SgDabK k = sg_make_dab(&plain, 0);
+  int rc = sg_enqueue_clear_ctr(p);
+  if (!rc) rc = sg_enqueue_dab(p, k, nullptr, nullptr, 0, 0, 0, &col);
+  if (!rc) rc = sg_enqueue_flush(p);
+  long long ctr[SG_CTR_COUNT];
+  if (!rc) rc = sg_check_hip(hipMemcpyAsync(ctr, p->d.ctr, sizeof(ctr), hipMemcpyDeviceToHost, p->stream), "copy");
+  if (!rc) rc = sg_check_hip(hipStreamSynchronize(p->stream), "collect");
+  int na = 0;
+  if (!rc) {
+    na = (int)std::min<long long>(ctr[SG_CTR_NAFF], cap);
+    std::vector<float> fr((size_t)std::max(na, 1) * 16);
+    hipMemcpy(X, col.X, (size_t)na * 28 * 4, hipMemcpyDeviceToHost);
+    hipMemcpy(T, col.T, (size_t)na * 3 * 4, hipMemcpyDeviceToHost);
+    hipMemcpy(fr.data(), col.frame, (size_t)na * 16 * 4, hipMemcpyDeviceToHost);
+    for (int i = 0; i < na; i++) ok[i] = fr[(size_t)i * 16 + 13];
+    memcpy(p->last_ctr, ctr, sizeof(ctr));
+  }
+  hipFree(col.X);
+  hipFree(col.frame);
+  hipFree(col.T);
+  invalidate_host_views(p);
+  return rc ? rc : na;
+}
+
 int sg_pbvh_set_profile(PBVH *p, int on) {
   if (!p || p->no_device) return sg_set_error("no device");
   if (p->profile && !on) sg_profile_collect(p);
@@ -702,9 +799,9 @@ int sg_dab_last_affected(PBVH *p, int *out, int cap) {
   return na;
 }
 
-int sg_dab_last_stats(PBVH *p, int64_t out[16]) {
+int sg_dab_last_stats(PBVH *p, int64_t out[24]) {
   if (!p) return sg_set_error("null PBVH");
-  for (int i = 0; i < 16; i++) out[i] = p->last_ctr[i];
+  for (int i = 0; i < 24; i++) out[i] = p->last_ctr[i];
   return 0;
 }
 

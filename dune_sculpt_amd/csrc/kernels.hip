@@ -20,8 +20,8 @@
 
 #include "sg_internal.h"
 #include "sg_kernels.h"
-
-#define SG_WAVE 64
+#include "sg_device.h"
+#include "sg_learned.h"
 
 // ---------------------------------------------------------------------------
 // error reporting
@@ -42,196 +42,6 @@ int sg_check_hip(hipError_t e, const char *what) {
   sg_set_error("%s: %s", what, hipGetErrorString(e));
   return -(int)e - 1000;
 }
-
-// ---------------------------------------------------------------------------
-// device helpers
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ float sg_min(float a, float b) { return (a < b) ? a : b; }  // min_ff
-__device__ __forceinline__ float sg_max(float a, float b) { return (a > b) ? a : b; }  // max_ff
-
-__device__ __forceinline__ int lane_id() { return __lane_id(); }
-
-// prefix popcount of the lanes below this one in a 64-bit mask
-__device__ __forceinline__ int mask_prefix(unsigned long long m) {
-  return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-}
-
-// normalize_v3 (math_vector_inline.c:1165-1180): zero below 1e-35, else multiply by
-// the rounded reciprocal of the correctly rounded sqrt.
-__device__ __forceinline__ float sg_normalize(float &x, float &y, float &z) {
-  float d = x * x + y * y + z * z;
-  if (d > 1.0e-35f) {
-    d = sqrtf(d);
-    const float inv = 1.0f / d;
-    x = x * inv;
-    y = y * inv;
-    z = z * inv;
-  } else {
-    x = y = z = 0.0f;
-    d = 0.0f;
-  }
-  return d;
-}
-
-// normal_tri_v3 (math_geom.cc:31-46)
-__device__ __forceinline__ void sg_tri_normal(float ax, float ay, float az, float bx, float by,
-                                              float bz, float cx, float cy, float cz, float &nx,
-                                              float &ny, float &nz) {
-  const float n1x = ax - bx, n2x = bx - cx;
-  const float n1y = ay - by, n2y = by - cy;
-  const float n1z = az - bz, n2z = bz - cz;
-  nx = n1y * n2z - n1z * n2y;
-  ny = n1z * n2x - n1x * n2z;
-  nz = n1x * n2y - n1y * n2x;
-  sg_normalize(nx, ny, nz);
-}
-
-// Ordered bounding-box combine: `b` covers later elements than `a`.  Equal to
-// BB_expand_with_bb(a, b) (pbvh.c:66-72), which keeps the later value on ties.
-struct BB6 {
-  float mn[3], mx[3];
-};
-__device__ __forceinline__ void bb_reset(BB6 &b) {
-  b.mn[0] = b.mn[1] = b.mn[2] = FLT_MAX;
-  b.mx[0] = b.mx[1] = b.mx[2] = -FLT_MAX;
-}
-__device__ __forceinline__ void bb_add_point(BB6 &b, float x, float y, float z) {
-  b.mn[0] = sg_min(b.mn[0], x);
-  b.mn[1] = sg_min(b.mn[1], y);
-  b.mn[2] = sg_min(b.mn[2], z);
-  b.mx[0] = sg_max(b.mx[0], x);
-  b.mx[1] = sg_max(b.mx[1], y);
-  b.mx[2] = sg_max(b.mx[2], z);
-}
-__device__ __forceinline__ void bb_add_bb(BB6 &a, const BB6 &b) {
-  for (int i = 0; i < 3; i++) {
-    a.mn[i] = sg_min(a.mn[i], b.mn[i]);
-    a.mx[i] = sg_max(a.mx[i], b.mx[i]);
-  }
-}
-__device__ __forceinline__ BB6 bb_load(const float *p) {
-  const float4 a = *(const float4 *)p;
-  const float4 b = *(const float4 *)(p + 4);
-  BB6 r;
-  r.mn[0] = a.x;
-  r.mn[1] = a.y;
-  r.mn[2] = a.z;
-  r.mx[0] = b.x;
-  r.mx[1] = b.y;
-  r.mx[2] = b.z;
-  return r;
-}
-__device__ __forceinline__ void bb_store(float *p, const BB6 &b) {
-  *(float4 *)p = make_float4(b.mn[0], b.mn[1], b.mn[2], 0.0f);
-  *(float4 *)(p + 4) = make_float4(b.mx[0], b.mx[1], b.mx[2], 0.0f);
-}
-// wave-ordered reduction: lane i's box covers elements before lane i+1's.
-__device__ __forceinline__ BB6 bb_wave_reduce(BB6 b) {
-  for (int off = 1; off < SG_WAVE; off <<= 1) {
-    BB6 o;
-    for (int i = 0; i < 3; i++) {
-      o.mn[i] = __shfl_down(b.mn[i], off);
-      o.mx[i] = __shfl_down(b.mx[i], off);
-    }
-    // lane l combines with lane l+off (later elements); lanes whose partner is out
-    // of range keep their own value (their result is never used)
-    if (lane_id() + off < SG_WAVE && (lane_id() & (2 * off - 1)) == 0) bb_add_bb(b, o);
-  }
-  return b;  // lane 0 holds the ordered combination
-}
-
-// Spec S1 sphere-vs-box test: nearest point of the box, strict '<' against r*r.
-__device__ __forceinline__ bool sphere_box(const BB6 &b, float cx, float cy, float cz, float r2) {
-  const float c[3] = {cx, cy, cz};
-  float t[3];
-  for (int i = 0; i < 3; i++) {
-    float nearest;
-    if (b.mn[i] > c[i])
-      nearest = b.mn[i];
-    else if (b.mx[i] < c[i])
-      nearest = b.mx[i];
-    else
-      nearest = c[i];
-    t[i] = c[i] - nearest;
-  }
-  return (t[0] * t[0] + t[1] * t[1] + t[2] * t[2]) < r2;
-}
-
-// Spec S3 falloff: hardness remap, then the preset curve, clamped to [0,1].
-__device__ __forceinline__ float sg_falloff(const SgDabK &k, const float *curve, float dist) {
-  const float r = k.r, h = k.hardness;
-  {
-    float pr = dist / r;
-    if (pr < h) {
-      dist = 0.0f;
-    } else if (h == 1.0f) {
-      dist = r;
-    } else {
-      pr = (pr - h) / (1.0f - h);
-      dist = pr * r;
-    }
-  }
-  if (dist >= r) return 0.0f;
-  float p = dist / r;
-  p = 1.0f - p;
-  float s;
-  switch (k.falloff) {
-    case SG_CURVE_CUSTOM:
-      if (curve) {
-        const float value = 1.0f - p;
-        float fi = value * (float)SG_CM_TABLE;
-        const int i = (int)fi;
-        if (fi < 0.0f || fi > (float)SG_CM_TABLE) {
-          s = fi < 0.0f ? curve[0] : curve[SG_CM_TABLE];
-        } else if (i < 0) {
-          s = curve[0];
-        } else if (i >= SG_CM_TABLE) {
-          s = curve[SG_CM_TABLE];
-        } else {
-          fi = fi - (float)i;
-          s = (1.0f - fi) * curve[i] + (fi)*curve[i + 1];
-        }
-      } else {
-        s = 1.0f;
-      }
-      break;
-    case SG_CURVE_SHARP:
-      s = p * p;
-      break;
-    case SG_CURVE_SMOOTHER:
-      s = ((p * p) * p) * (6.0f * p * p - 15.0f * p + 10.0f);
-      break;
-    case SG_CURVE_ROOT:
-      s = sqrtf(p);
-      break;
-    case SG_CURVE_LIN:
-      s = p;
-      break;
-    case SG_CURVE_CONSTANT:
-      s = 1.0f;
-      break;
-    case SG_CURVE_SPHERE:
-      s = sqrtf(2.0f * p - p * p);
-      break;
-    case SG_CURVE_POW4:
-      s = p * p * p * p;
-      break;
-    case SG_CURVE_INVSQUARE:
-      s = p * (2.0f - p);
-      break;
-    case SG_CURVE_SMOOTH:
-    default:
-      s = 3.0f * p * p - 2.0f * p * p * p;
-      break;
-  }
-  if (s < 0.0f)
-    s = 0.0f;
-  else if (s > 1.0f)
-    s = 1.0f;
-  return s;
-}
-
-__device__ __forceinline__ unsigned long long as_u64(long long x) { return (unsigned long long)x; }
 
 // ---------------------------------------------------------------------------
 // K1: search.  One wave per cut subtree (all waves independent, no block barrier):
@@ -281,6 +91,7 @@ __device__ __forceinline__ void leaves_box(const float *bbs, const float *obbs, 
 
 __global__ __launch_bounds__(256) void k_search(SgDevView d, SgDabK k, long long *log, int log_index,
                                                 int serial) {
+  if (d.halt[0]) return;
   const int lane = lane_id();
   const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
   const int nw = gridDim.x * blockDim.x / SG_WAVE;
@@ -370,19 +181,10 @@ __global__ __launch_bounds__(256) void k_search(SgDevView d, SgDabK k, long long
   }
 }
 
-// zero both counter sets at the start of a stroke
-__global__ void k_clear_ctr(long long *ctr) {
+// zero both counter sets and the halt flag at the start of a stroke
+__global__ void k_clear_ctr(long long *ctr, int *halt) {
   if (threadIdx.x < 2 * SG_CTR_COUNT) ctr[threadIdx.x] = 0;
-}
-
-// wave-wide sums
-__device__ __forceinline__ long long wave_sum_ll(long long v) {
-  for (int off = SG_WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  return v;
-}
-__device__ __forceinline__ int wave_sum_i(int v) {
-  for (int off = SG_WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  return v;
+  if (threadIdx.x == 0) halt[0] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -392,6 +194,7 @@ __device__ __forceinline__ int wave_sum_i(int v) {
 // Positions and normals are fetched in the same load level.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_area(SgDevView d, SgDabK k) {
+  if (d.halt[0]) return;
   const int lane = lane_id();
   const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
   const int nw = gridDim.x * blockDim.x / SG_WAVE;
@@ -429,127 +232,6 @@ __global__ __launch_bounds__(256) void k_area(SgDevView d, SgDabK k) {
   }
 }
 
-// Sum of the (previous-iterate) positions of v's edge neighbours in CSR order, with
-// the neighbour ids and then their positions fetched eight at a time.
-__device__ __forceinline__ void nbr_sum(const int *__restrict__ vv_idx, int b, int e, const float *__restrict__ sx_,
-                                        const float *__restrict__ sy_, const float *__restrict__ sz_, float &sx,
-                                        float &sy, float &sz) {
-  sx = 0.0f;
-  sy = 0.0f;
-  sz = 0.0f;
-  for (int q0 = b; q0 < e; q0 += 8) {
-    int u[8];
-#pragma unroll
-    for (int j = 0; j < 8; j++) u[j] = (q0 + j < e) ? vv_idx[q0 + j] : -1;
-    float x[8], y[8], z[8];
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-      if (u[j] >= 0) {
-        x[j] = sx_[u[j]];
-        y[j] = sy_[u[j]];
-        z[j] = sz_[u[j]];
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-      if (u[j] >= 0) {
-        sx = sx + x[j];
-        sy = sy + y[j];
-        sz = sz + z[j];
-      }
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// K3: brush.  One wave per gathered leaf (grid-stride); lanes over the leaf's uniq
-// vertices (one coalesced range).  Spec S2 vertex test, S3/S4 factor, S6-S9
-// displacement.  Affected vertices are compacted with ballot + prefix popcount and
-// one atomic per wave into records (vertex, its vertex->vertex and vertex->tri CSR
-// ranges) so the later kernels skip those lookups.  Positions are written to both
-// the position array and its shadow copy.  Smooth runs its first Jacobi step here,
-// reading the untouched positions and writing the shadow copy.
-// ---------------------------------------------------------------------------
-// Per-vertex inputs of the brush, loaded together for two vertices per lane.
-struct BrushIn {
-  float x, y, z, d2, nx, ny, nz, mk;
-  int2 vt, vv;
-  bool aff;
-};
-
-__device__ __forceinline__ BrushIn brush_load(const SgDevView &d, const SgDabK &k, int v, int vend, bool smooth) {
-  BrushIn in;
-  in.aff = false;
-  in.x = in.y = in.z = in.d2 = in.nx = in.ny = in.nz = in.mk = 0.0f;
-  in.vt = make_int2(0, 0);
-  in.vv = make_int2(0, 0);
-  if (v < vend) {
-    in.x = d.px[v];
-    in.y = d.py[v];
-    in.z = d.pz[v];
-    in.vt = make_int2(d.vt_off[v], d.vt_off[v + 1]);
-    if (smooth) in.vv = make_int2(d.vv_off[v], d.vv_off[v + 1]);
-    if (k.tool == SG_TOOL_INFLATE) {
-      in.nx = d.nx[v];
-      in.ny = d.ny[v];
-      in.nz = d.nz[v];
-    }
-    if (d.mask) in.mk = d.mask[v];
-    const float dx = k.cx - in.x;
-    const float dy = k.cy - in.y;
-    const float dz = k.cz - in.z;
-    in.d2 = dx * dx + dy * dy + dz * dz;
-    in.aff = in.d2 <= k.r2;
-    if (d.vhide && d.respect_hide && d.vhide[v]) in.aff = false;
-  }
-  return in;
-}
-
-// Spec S3/S4 factor and S6-S9 displacement of one affected vertex; returns the Smooth
-// factor (0 for other tools).  Smooth step 1 reads the untouched positions and writes
-// the shadow copy; the other tools write both copies.
-__device__ __forceinline__ float brush_apply(const SgDevView &d, const SgDabK &k, const float *curve, int v,
-                                             const BrushIn &in, float ox, float oy, float oz, long long &nbr) {
-  const float f = sg_falloff(k, curve, sqrtf(in.d2)) * (1.0f - in.mk);
-  float rx = in.x, ry = in.y, rz = in.z, fade = 0.0f;
-  if (k.tool == SG_TOOL_DRAW) {
-    rx = in.x + ox * f;
-    ry = in.y + oy * f;
-    rz = in.z + oz * f;
-  } else if (k.tool == SG_TOOL_INFLATE) {
-    const float s = (k.bstr * f) * k.r;
-    rx = in.x + in.nx * s;
-    ry = in.y + in.ny * s;
-    rz = in.z + in.nz * s;
-  } else if (k.tool == SG_TOOL_GRAB) {
-    fade = k.bstr * f;
-    rx = in.x + k.gx * fade;
-    ry = in.y + k.gy * fade;
-    rz = in.z + k.gz * fade;
-  } else {
-    fade = k.bstr * f;
-    float sx, sy, sz;
-    nbr_sum(d.vv_idx, in.vv.x, in.vv.y, d.px, d.py, d.pz, sx, sy, sz);
-    nbr += in.vv.y - in.vv.x;
-    if (in.vv.y > in.vv.x) {
-      const float inv = 1.0f / (float)(in.vv.y - in.vv.x);
-      const float avx = sx * inv, avy = sy * inv, avz = sz * inv;
-      rx = in.x + (avx - in.x) * fade;
-      ry = in.y + (avy - in.y) * fade;
-      rz = in.z + (avz - in.z) * fade;
-    }
-  }
-  if (k.tool != SG_TOOL_SMOOTH) {
-    d.px[v] = rx;
-    d.py[v] = ry;
-    d.pz[v] = rz;
-  }
-  d.qx[v] = rx;
-  d.qy[v] = ry;
-  d.qz[v] = rz;
-  return fade;
-}
-
 // ---------------------------------------------------------------------------
 // K3: brush.  One wave per gathered leaf (grid-stride); two vertices per lane, so a
 // leaf of up to 128 uniq vertices is one load level, one ballot pair and one atomic.
@@ -559,6 +241,7 @@ __device__ __forceinline__ float brush_apply(const SgDevView &d, const SgDabK &k
 // reservation is in flight while the displacement is computed.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_brush(SgDevView d, SgDabK k, const float *curve, int stamp) {
+  if (d.halt[0]) return;
   const int lane = lane_id();
   const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
   const int nw = gridDim.x * blockDim.x / SG_WAVE;
@@ -638,6 +321,7 @@ __global__ __launch_bounds__(256) void k_brush(SgDevView d, SgDabK k, const floa
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_smooth_iter(SgDevView d, int from_shadow, int stats, int stamp_dab,
                                                      int stamp) {
+  if (d.halt[0]) return;
   const int nthreads = gridDim.x * blockDim.x;
   const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
   const float *sx_ = from_shadow ? d.qx : d.px;
@@ -687,6 +371,7 @@ __global__ __launch_bounds__(256) void k_smooth_iter(SgDevView d, int from_shado
 // Copy the final Jacobi iterate from the shadow copy (odd k) or to it (even k), so
 // both arrays agree again.
 __global__ __launch_bounds__(256) void k_smooth_sync(SgDevView d, int from_shadow) {
+  if (d.halt[0]) return;
   const int na = (int)d.ctr[SG_CTR_NAFF];
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
     const int v = d.aff_v[i];
@@ -702,107 +387,12 @@ __global__ __launch_bounds__(256) void k_smooth_sync(SgDevView d, int from_shado
   }
 }
 
-// Normal of vertex v (spec S10): normalize(sum over incident tris, in CSR order, of
-// normal_tri_v3).  Deterministic gather, no atomics.  The incident tris' corner ids
-// are stored flattened in CSR order (vt4), so a batch of eight tris needs one load
-// level for the corners and one for their positions.
-__device__ __forceinline__ void vertex_normal(const SgDevView &d, int v, int b, int e, int stats, int stamp,
-                                              long long &ntri, long long &nfa, long long &nvfa) {
-  float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  for (int q0 = b; q0 < e; q0 += 8) {
-    int4 c[8];
-#pragma unroll
-    for (int j = 0; j < 8; j++) c[j] = (q0 + j < e) ? d.vt4[q0 + j] : make_int4(-1, -1, -1, -1);
-    float P[8][9];
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-      if (c[j].x >= 0) {
-        P[j][0] = d.px[c[j].x];
-        P[j][1] = d.py[c[j].x];
-        P[j][2] = d.pz[c[j].x];
-        P[j][3] = d.px[c[j].y];
-        P[j][4] = d.py[c[j].y];
-        P[j][5] = d.pz[c[j].y];
-        P[j][6] = d.px[c[j].z];
-        P[j][7] = d.py[c[j].z];
-        P[j][8] = d.pz[c[j].z];
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-      if (c[j].x >= 0) {
-        float fx, fy, fz;
-        sg_tri_normal(P[j][0], P[j][1], P[j][2], P[j][3], P[j][4], P[j][5], P[j][6], P[j][7], P[j][8], fx, fy,
-                      fz);
-        ax = ax + fx;
-        ay = ay + fy;
-        az = az + fz;
-      }
-    }
-    if (stats) {
-      for (int j = 0; j < 8; j++) {
-        if (c[j].x < 0) continue;
-        const int t = c[j].w & 0x3fffffff;
-        if (atomicExch(&d.stamp_t[t], stamp) != stamp) {
-          nfa++;
-          if (atomicExch(&d.stamp_v2[c[j].x], stamp) != stamp) nvfa++;
-          if (atomicExch(&d.stamp_v2[c[j].y], stamp) != stamp) nvfa++;
-          if (atomicExch(&d.stamp_v2[c[j].z], stamp) != stamp) nvfa++;
-        }
-      }
-    }
-  }
-  ntri += e - b;
-  sg_normalize(ax, ay, az);
-  d.nx[v] = ax;
-  d.ny[v] = ay;
-  d.nz[v] = az;
-}
-
-// Leaf bounds over ALL its vertices, uniq then shared (update_node_vb pbvh.c:108-128,
-// BB_expand in vert_indices order).  Each lane folds a contiguous chunk, then the
-// wave combines lanes in order.  Writes vb and orig_vb (UpdateOriginalBB).
-__device__ __forceinline__ void leaf_refit(const SgDevView &d, int leaf, int2 vr, int2 sr, long long &nsh) {
-  const int nu = vr.y - vr.x, ns = sr.y - sr.x, tot = nu + ns;
-  const int per = (tot + SG_WAVE - 1) / SG_WAVE;
-  const int lane = lane_id();
-  BB6 acc;
-  bb_reset(acc);
-  for (int q0 = 0; q0 < per; q0 += 8) {
-    int vid[8];
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-      const int i = lane * per + q0 + j;
-      vid[j] = -1;
-      if (q0 + j < per && i < tot) vid[j] = (i < nu) ? vr.x + i : d.leaf_sh[sr.x + i - nu];
-    }
-    float x[8], y[8], z[8];
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-      if (vid[j] >= 0) {
-        x[j] = d.px[vid[j]];
-        y[j] = d.py[vid[j]];
-        z[j] = d.pz[vid[j]];
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-      if (vid[j] >= 0) bb_add_point(acc, x[j], y[j], z[j]);
-    }
-  }
-  acc = bb_wave_reduce(acc);
-  if (lane == 0) {
-    bb_store(d.leaf_bb + 8 * (size_t)leaf, acc);
-    bb_store(d.leaf_obb + 8 * (size_t)leaf, acc);
-    nsh += ns;
-  }
-}
-
 // ---------------------------------------------------------------------------
 // K5: post.  Normals of the affected vertices and bounds of the gathered leaves (all
 // gathered leaves are marked, BKE_pbvh_node_mark_update).  Both only read positions.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_post(SgDevView d, int stats, int stamp, int sync_q) {
+  if (d.halt[0]) return;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int nthreads = gridDim.x * blockDim.x;
   const int gw = tid / SG_WAVE;
@@ -1201,6 +791,7 @@ SgDevView sg_view(PBVH *p) {
   v.ctr = d.ctr;
   v.ctr_next = d.ctr + SG_CTR_COUNT;
   v.cut_touch = d.cut_touch;
+  v.halt = d.halt;
   v.vperm_old = d.vperm_old;
   v.stamp_v = d.stamp_v;
   v.stamp_v2 = d.stamp_v2;
@@ -1227,6 +818,7 @@ SgDabK sg_make_dab(const SgDabParams *dab, int stats) {
   k.gz = dab->grab_delta[2];
   k.iters = dab->smooth_iterations > 0 ? dab->smooth_iterations : 1;
   k.use_orig = (k.tool == SG_TOOL_GRAB) ? 1 : 0;
+  k.learned = (k.tool == SG_TOOL_SMOOTH && (dab->flag & SG_SMOOTH_LEARNED)) ? 1 : 0;
   k.stats = stats;
   // Spec S5 bstrength
   const float s = dab->strength;
@@ -1246,35 +838,29 @@ SgDabK sg_make_dab(const SgDabParams *dab, int stats) {
 // Per-kernel timing: with profiling on, every launch goes through
 // hipExtLaunchKernelGGL with a start/stop event pair that the dispatch packet itself
 // timestamps, so the measured span is the kernel's execution on this stream.
-// While a stroke is being captured into a graph, profiling records plain event
-// nodes around each kernel node instead.
+// Launches captured into a graph are not profiled.
 struct ProfScope {
   PBVH *p;
   int kid;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  bool graph = false;
   ProfScope(PBVH *pp, int k) : p(pp), kid(k) {
-    if (!p->profile) return;
+    if (!p->profile || p->capturing) return;
     while (p->ev_pool.size() < p->ev_used + 2) {
       hipEvent_t e;
-      hipEventCreate(&e);
+      // no system-scope fence: the events only time kernels on this stream
+      hipEventCreateWithFlags(&e, hipEventDisableSystemFence);
       p->ev_pool.push_back(e);
     }
     e0 = p->ev_pool[p->ev_used];
     e1 = p->ev_pool[p->ev_used + 1];
     p->ev_used += 2;
     p->ev_kid.push_back(kid);
-    graph = p->capturing;
-    if (graph) hipEventRecord(e0, p->stream);
-  }
-  ~ProfScope() {
-    if (graph) hipEventRecord(e1, p->stream);
   }
 };
 
 #define SG_LAUNCH(ps, kern, grid, block, shm, stream, ...)                                   \
   do {                                                                                      \
-    if ((ps).e0 && !(ps).graph)                                                             \
+    if ((ps).e0)                                                                            \
       hipExtLaunchKernelGGL(kern, grid, block, shm, stream, (ps).e0, (ps).e1, 0, __VA_ARGS__); \
     else                                                                                    \
       hipLaunchKernelGGL(kern, grid, block, shm, stream, __VA_ARGS__);                      \
@@ -1295,14 +881,14 @@ int sg_profile_collect(PBVH *p) {
 }
 
 int sg_enqueue_clear_ctr(PBVH *p) {
-  hipLaunchKernelGGL(k_clear_ctr, dim3(1), dim3(64), 0, p->stream, p->d.ctr);
+  hipLaunchKernelGGL(k_clear_ctr, dim3(1), dim3(64), 0, p->stream, p->d.ctr, p->d.halt);
   return sg_check_hip(hipGetLastError(), "clear launch");
 }
 
 // One dab; `serial` is the dab's position in its stroke (selects the counter set and
 // the "touched by the previous dab" test of the search).
 int sg_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long long *log, int log_index,
-                   int stamp, int serial) {
+                   int stamp, int serial, const SgCollect *col) {
   SgDevView v = sg_view(p);
   v.ctr = p->d.ctr + (size_t)(serial & 1) * SG_CTR_COUNT;
   v.ctr_next = p->d.ctr + (size_t)((serial + 1) & 1) * SG_CTR_COUNT;
@@ -1316,6 +902,17 @@ int sg_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long long *
     const int sg = (p->d.C + 3) / 4;
     SG_LAUNCH(ps, k_search, dim3(sg > 0 ? sg : 1), dim3(256), 0, s, v, k, log, log_index, serial);
   }
+  if (k.learned && p->mlp) {  // learned Smooth: brush + MLP, then decide / commit
+    {
+      ProfScope ps(p, SG_K_BRUSH);
+      sg_enqueue_learned_brush(p, v, k, curve_dev, p->mlp);
+    }
+    {
+      ProfScope ps(p, SG_K_POST);
+      sg_enqueue_learned_post(p, v, p->mlp, serial);
+    }
+    return sg_check_hip(hipGetLastError(), "dab launch");
+  }
   if (k.tool == SG_TOOL_DRAW) {
     ProfScope ps(p, SG_K_AREA);
     SG_LAUNCH(ps, k_area, dim3(GB), dim3(256), 0, s, v, k);
@@ -1324,6 +921,7 @@ int sg_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long long *
     ProfScope ps(p, SG_K_BRUSH);
     SG_LAUNCH(ps, k_brush, dim3(GB), dim3(256), 0, s, v, k, curve_dev, stamp);
   }
+  if (col) sg_enqueue_collect(p, v, col->X, col->frame, col->cap);
   if (k.tool == SG_TOOL_SMOOTH) {
     // step 1 (in k_brush) wrote the shadow copy; step it reads the shadow copy when it
     // is even and writes the positions, and the reverse when it is odd
@@ -1343,6 +941,7 @@ int sg_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long long *
     SG_LAUNCH(ps, k_post, dim3(GP), dim3(256), 0, s, v, k.stats, stamp,
               (k.tool == SG_TOOL_SMOOTH && k.iters % 2 == 0) ? 1 : 0);
   }
+  if (col) sg_enqueue_target(p, v, col->frame, col->T, col->cap);
   if (p->profile && !p->capturing && p->ev_kid.size() > 4096) sg_profile_collect(p);
   return sg_check_hip(hipGetLastError(), "dab launch");
 }

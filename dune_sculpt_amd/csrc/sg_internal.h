@@ -66,6 +66,7 @@ struct SgDev {
   int top_n = 0, top_maxh = 0, sub_maxn = 0;
   int *cut_dirty = nullptr;     // [C]
   int *cut_touch = nullptr;     // [C] serial of the last dab that gathered a leaf of it
+  int *halt = nullptr;          // [4] set by a rejected learned dab: its serial + 1
   int *vhide = nullptr;         // [V] or null
   uint8_t *vmark = nullptr;     // [V] host-driven normal update marks
   // per-dab scratch
@@ -97,7 +98,7 @@ enum {
   SG_CTR_NAREA = 8,
   SG_CTR_NBR = 9,
   SG_CTR_TRIS = 10,
-  SG_CTR_COUNT = 16,
+  SG_CTR_COUNT = 24,
 };
 
 struct PBVH {
@@ -130,7 +131,9 @@ struct PBVH {
   // last dab
   int last_tool = 0;
   int64_t last_stats[8] = {0};
-  long long last_ctr[16] = {0};
+  long long last_ctr[24] = {0};
+  struct SgMlp *mlp = nullptr;       // learned Smooth model (borrowed)
+  long long learned_accepted = 0, learned_fallbacks = 0;
   int dab_count = 0;
   int grid_wg = 256;                 // workgroups of the per-dab kernels (tunable)
   int grid_brush = 0, grid_post = 0, grid_smooth = 0;  // per-kernel overrides (autotuner)

@@ -10,6 +10,15 @@ enum {
   SG_CTR_FA = 12,     // distinct tris incident to the affected set (stats mode)
   SG_CTR_VFA = 13,    // distinct vertices of those tris (stats mode)
   SG_CTR_SHARED = 14, // shared vertices read by the leaf refit
+  SG_CTR_LERR = 15,   // learned Smooth: sum (|d|/r)^2 in 2^-40 fixed point
+  SG_CTR_LOOD = 16,   // learned Smooth: vertices outside the trained range
+  SG_CTR_LACC = 17,   // learned Smooth: 1 when the dab was accepted
+};
+
+// pair collection for the learned Smooth (device buffers)
+struct SgCollect {
+  float *X, *frame, *T;
+  int cap;
 };
 
 // Plain-pointer view of the device mesh passed to kernels by value.
@@ -32,6 +41,7 @@ struct SgDevView {
   float *aff_fade;
   int *prev_cuts;
   int *cut_touch;
+  int *halt;
   long long *ctr, *ctr_next;
   const int *vperm_old;
   int *stamp_v, *stamp_v2, *stamp_t, *stamp_a;
@@ -42,7 +52,7 @@ struct SgDevView {
 struct SgDabK {
   float cx, cy, cz, r, r2, nr2, bstr, hardness;
   float gx, gy, gz;
-  int tool, falloff, iters, use_orig, stats;
+  int tool, falloff, iters, use_orig, stats, learned;
 };
 
 // kernel ids for the per-kernel profile
@@ -53,7 +63,7 @@ SgDevView sg_view(PBVH *p);
 int sg_profile_collect(PBVH *p);
 SgDabK sg_make_dab(const SgDabParams *dab, int stats);
 int sg_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long long *log, int log_index,
-                   int stamp, int serial);
+                   int stamp, int serial, const SgCollect *col = nullptr);
 int sg_enqueue_clear_ctr(PBVH *p);
 int sg_enqueue_flush(PBVH *p);
 int sg_enqueue_log_row(PBVH *p, long long *log, int row);

@@ -164,6 +164,7 @@ enum {
   SG_CURVE_CONSTANT = 8, SG_CURVE_SMOOTHER = 9,
 };
 #define SG_BRUSH_DIR_IN (1 << 9)    /* BRUSH_DIR_IN, types_brush_enums.h:347 */
+#define SG_SMOOTH_LEARNED (1 << 30) /* Smooth through the attached learned model */
 #define SG_CM_TABLE 256             /* CM_TABLE, types_color.h:12 */
 
 /* One dab.  Field names mirror the Brush RNA properties (api_brush.c:2281-2880) and
@@ -207,7 +208,7 @@ typedef struct SgStroke SgStroke;
 SgStroke *sg_stroke_prepare(PBVH *pbvh, const SgDabParams *dabs, int n, int flags);
 int sg_stroke_launch(SgStroke *stroke, int sync);
 void sg_stroke_free(SgStroke *stroke);
-/* Per-dab counters (16 per dab, layout of sg_dab_last_stats) of the last stroke. */
+/* Per-dab counters (24 per dab, layout of sg_dab_last_stats) of the last stroke. */
 int sg_dab_stroke_log(PBVH *pbvh, long long *out, int n);
 /* Block until queued work is done; returns the first recorded device error. */
 int sg_sync(PBVH *pbvh);
@@ -221,8 +222,9 @@ int sg_dab_last_affected(PBVH *pbvh, int *out, int cap);
  * [7] candidate verts [8] area-normal verts [9] smooth neighbour reads (steps >= 2
  * plus step 1) [10] normal tri reads [11] distinct non-affected 1-ring verts (last
  * Jacobi steps) [12] distinct tris incident to A [13] distinct verts of those tris
- * [14] shared verts read by the leaf refit. */
-int sg_dab_last_stats(PBVH *pbvh, int64_t out[16]);
+ * [14] shared verts read by the leaf refit; learned Smooth: [15] sum (|d|/r)^2 in 2^-40
+ * units, [16] out-of-range vertices, [17] 1 when accepted. */
+int sg_dab_last_stats(PBVH *pbvh, int64_t out[24]);
 /* Gathered leaves (node indices, left->right) of the most recent dab. */
 int sg_dab_last_gathered(PBVH *pbvh, int *out, int cap);
 /* Exact byte-accounting counters in the brush kernels (adds atomics; off by default). */
@@ -260,6 +262,32 @@ int sg_pbvh_node_index(PBVH *pbvh, PBVHNode *node);
 PBVHNode *sg_pbvh_node_at(PBVH *pbvh, int index);
 /* prim_indices array (looptri order of the leaves) */
 int sg_pbvh_prim_indices(PBVH *pbvh, int *out, int cap);
+
+/* ---- learned Smooth (MLP over 1-ring features; BF16 MFMA inference) ----------
+ * A Smooth dab with SG_SMOOTH_LEARNED in SgDabParams.flag goes through the model
+ * attached to the PBVH; when its error estimate (alpha * RMS(|d|)/r) exceeds the
+ * tolerance, or a vertex is outside the trained range, the dab is redone with the
+ * plain brush (sg_dab_apply*, sg_stroke_launch handle this). */
+typedef struct SgMlp SgMlp;
+SgMlp *sg_mlp_new(uint64_t seed);
+void sg_mlp_free(SgMlp *mlp);
+int sg_mlp_get_params(SgMlp *mlp, float *out);           /* 6,211 FP32 master weights */
+int sg_mlp_set_params(SgMlp *mlp, const float *w);
+int sg_mlp_set_norm(SgMlp *mlp, const float *in_mu, const float *in_inv_sigma, const float *out_scale);
+int sg_mlp_get_norm(SgMlp *mlp, float *out64);
+int sg_mlp_set_decision(SgMlp *mlp, float alpha, float tol);
+int sg_mlp_get_decision(SgMlp *mlp, float *out2);
+int sg_mlp_set_data(SgMlp *mlp, const float *X, const float *T, int n);  /* rows of 28 / 3 */
+int sg_mlp_gradient(SgMlp *mlp, int n, float *grad_out, float *loss_out);
+int sg_mlp_train(SgMlp *mlp, int epochs, int batch_wg, float lr, uint64_t seed, float *loss_curve);
+int sg_mlp_predict(SgMlp *mlp, const float *X, int n, float *Y);
+int sg_mfma_probe(const uint16_t *A, const uint16_t *B, float *D);  /* 16x16 bf16 layout check */
+int sg_pbvh_set_learned(PBVH *pbvh, SgMlp *mlp);
+int sg_pbvh_learned_stats(PBVH *pbvh, long long out[2]);  /* accepted, fallbacks */
+/* A plain Smooth dab recording features (X[i][28]) before and targets (T[i][3]) after
+ * it for each affected vertex; ok[i] = 0 for vertices outside the model's range.
+ * Returns the number of rows. */
+int sg_learned_collect(PBVH *pbvh, const SgDabParams *dab, float *X, float *T, float *ok, int cap);
 
 /* ---- synthetic meshes and strokes (deterministic; for tests and benchmarks) ---- */
 int sg_mesh_icosphere_size(int level, int *r_totvert, int *r_tottri);

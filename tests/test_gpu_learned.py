@@ -1,0 +1,122 @@
+"""GPU tests of the learned Smooth operator: MFMA operand layout, training gradients
+against the FP64 oracle, BF16 inference against its emulation, training on pairs
+from the plain brush (with the gain over the methods without the network), the
+plain-brush fallback (forced by tolerance 0 and by corrupted weights), determinism."""
+import numpy as np
+import pytest
+
+import mlp_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+
+def test_mfma_layout(sg):
+    rng = np.random.default_rng(1)
+    A = mo.bf16(rng.normal(size=(16, 16)).astype(np.float32))
+    B = mo.bf16(rng.normal(size=(16, 16)).astype(np.float32))
+    D = sg.mfma_probe(A.view(np.uint32) >> 16, B.view(np.uint32) >> 16)
+    want = A.astype(np.float64) @ B.astype(np.float64)
+    assert np.abs(D - want).max() < 1e-4 * np.abs(want).max()
+
+
+def _data(n, seed=0):
+    rng = np.random.default_rng(seed)
+    X = rng.normal(size=(n, 28)).astype(np.float32)
+    T = (np.tanh(X[:, :3]) * 0.5 + 0.1 * X[:, 3:6] * X[:, 25:26]).astype(np.float32)
+    return X, T
+
+
+def test_gradient_matches_oracle(sg):
+    X, T = _data(1000)
+    m = sg.MLP(seed=3)
+    mu, isig, osc = X.mean(0), 1 / X.std(0), T.std(0)
+    m.set_norm(mu, isig, osc)
+    m.set_data(X, T)
+    g, loss = m.gradient(1000)
+    w = m.params().astype(np.float64)
+    Xn, Tn = mo.normalize(X, T, (mu.astype(np.float64), isig.astype(np.float64), osc.astype(np.float64)))
+    lo, go = mo.loss_and_grad(w, Xn, Tn)
+    assert abs(loss - lo) <= 1e-4 * lo
+    assert np.abs(g - go).max() <= 2e-4 * np.abs(go).max()
+
+
+def test_predict_matches_bf16_emulation(sg):
+    X, T = _data(777, seed=2)
+    m = sg.MLP(seed=5)
+    mu, isig, osc = X.mean(0), 1 / X.std(0), T.std(0)
+    m.set_norm(mu, isig, osc)
+    Y = m.predict(X)
+    Xn = (X - mu) * isig
+    want = mo.predict_bf16(m.params(), Xn) * osc
+    assert np.abs(Y - want).max() <= 2e-2 * np.abs(want).max()
+
+
+def test_training_gains_over_methods_without_network(sg):
+    from dune_sculpt_amd import learned as L
+
+    X, T = L.collect_pairs(60000, mesh_seeds=(201,), level=7, amplitude=0.02, radius=(0.15, 0.3), seed=1)
+    Xv, Tv = L.collect_pairs(15000, mesh_seeds=(202,), level=7, amplitude=0.02, radius=(0.15, 0.3), seed=2)
+    m, info = L.train_model(X, T, Xv, Tv, epochs=30, batch_wg=64, lr=2e-3)
+    print(info["val_rel_rms"], info["baselines_rel_rms"], info["loss_curve"][:3], info["loss_curve"][-3:])
+    assert info["loss_curve"][-1] < 0.5 * info["loss_curve"][0]
+    for name, b in info["baselines_rel_rms"].items():
+        assert info["val_rel_rms"] < b, (name, info)
+
+
+def _small_setup(sg, seed=301):
+    from dune_sculpt_amd import learned as L
+
+    X, T = L.collect_pairs(40000, mesh_seeds=(seed,), level=7, amplitude=0.02, radius=(0.15, 0.3), seed=3)
+    m, info = L.train_model(X, T, X[:5000], T[:5000], epochs=15, batch_wg=64, lr=2e-3)
+    co, tris = sg.noisy_icosphere(7, seed=seed + 50, amplitude=0.02)
+    n = 24
+    radii = np.full(n, 0.2, np.float32)
+    c = L.random_surface_centres(n, np.random.default_rng(4), seed + 50, 0.02)
+    dabs = [sg.make_dab(tool=sg.TOOL_SMOOTH, location=ci, radius=0.2, strength=0.6, smooth_iterations=4) for ci in c]
+    return L, m, co, tris, dabs
+
+
+def test_learned_dabs_fallback_and_determinism(sg):
+    L, m, co, tris, dabs = _small_setup(sg)
+    ld = L.learned_dabs(dabs)
+    # plain reference
+    pp = sg.PBVH(co, tris, leaf_limit=128)
+    pp.stroke(dabs)
+    plain = pp.positions()
+    # tolerance 0: every learned dab is rejected and redone by the plain brush -> bitwise equal
+    alpha, tol = m.decision()
+    m.set_decision(alpha, 0.0)
+    p = sg.PBVH(co, tris, leaf_limit=128)
+    p.set_learned(m)
+    p.stroke(ld)
+    assert np.array_equal(p.positions().view(np.uint32), plain.view(np.uint32))
+    st = p.learned_stats()
+    assert st["fallbacks"] == len(dabs) and st["accepted"] == 0
+    p.close()
+    # corrupted weights (NaN): rejected, plain result
+    m2 = sg.MLP(seed=9)
+    w = m.params()
+    w[5] = np.nan
+    m2.set_norm(*m.norm())
+    m2.set_params(w)
+    m2.set_decision(alpha, 1.0)
+    p = sg.PBVH(co, tris, leaf_limit=128)
+    p.set_learned(m2)
+    p.stroke(ld)
+    assert np.array_equal(p.positions().view(np.uint32), plain.view(np.uint32))
+    p.close()
+    # accepting everything: runs the network; deterministic run to run
+    m.set_decision(alpha, 1.0)
+    outs = []
+    for _ in range(2):
+        p = sg.PBVH(co, tris, leaf_limit=128)
+        p.set_learned(m)
+        p.stroke(ld)
+        outs.append(p.positions())
+        assert p.learned_stats()["accepted"] == len(dabs)
+        p.close()
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+    assert not np.array_equal(outs[0].view(np.uint32), plain.view(np.uint32))
+    # the learned stroke stays close to the plain stroke
+    moved = np.abs(plain - co).max()
+    assert np.abs(outs[0] - plain).max() < 0.5 * moved
