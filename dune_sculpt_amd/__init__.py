@@ -573,7 +573,7 @@ class PBVH:
 
     def collect(self, d, cap=None):
         """Plain Smooth dab recording learned-model inputs and targets per affected vertex."""
-        cap = cap or self.totvert
+        cap = cap or min(self.totvert, 1 << 18)
         X = np.zeros((cap, 28), np.float32)
         T = np.zeros((cap, 3), np.float32)
         ok = np.zeros(cap, np.float32)

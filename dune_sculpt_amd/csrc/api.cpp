@@ -522,9 +522,11 @@ int sg_dab_apply_stroke(PBVH *p, const SgDabParams *dabs, int n, int flags) {
   }
   p->learned_fallbacks += nfallback;
   p->learned_accepted += nlearned - nfallback;
-  long long ctr[SG_CTR_COUNT];
-  hipMemcpy(ctr, p->d.ctr + (size_t)((n - 1 - a0) & 1) * SG_CTR_COUNT, sizeof(ctr), hipMemcpyDeviceToHost);
-  memcpy(p->last_ctr, ctr, sizeof(ctr));
+  if (a0 < n) {  // otherwise the last dab was a plain fallback and set last_ctr itself
+    long long ctr[SG_CTR_COUNT];
+    hipMemcpy(ctr, p->d.ctr + (size_t)((n - 1 - a0) & 1) * SG_CTR_COUNT, sizeof(ctr), hipMemcpyDeviceToHost);
+    memcpy(p->last_ctr, ctr, sizeof(ctr));
+  }
   p->last_tool = dabs[n - 1].sculpt_tool;
   p->dab_count += n;
   p->nodes_dirty_deferred = false;

@@ -232,7 +232,7 @@ __global__ __launch_bounds__(256) void k_learned_feat(SgDevView d, float *X, flo
     fr[11] = y;
     fr[12] = z;
     fr[13] = ok ? 1.0f : 0.0f;
-    fr[14] = (float)v;
+    fr[14] = __int_as_float(v);  // index bits, exact for any size
   }
 }
 
@@ -241,7 +241,7 @@ __global__ __launch_bounds__(256) void k_learned_target(SgDevView d, const float
   const int n = (int)(na < cap ? na : cap);  // rows the feature kernel wrote
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const float *fr = frame + (size_t)i * 16;
-    const int v = (int)fr[14];
+    const int v = __float_as_int(fr[14]);
     const float dx = d.px[v] - fr[10], dy = d.py[v] - fr[11], dz = d.pz[v] - fr[12];
     const float ih = fr[9] > 0.0f ? 1.0f / fr[9] : 0.0f;
     T[(size_t)i * 3 + 0] = (fr[0] * dx + fr[1] * dy + fr[2] * dz) * ih;
@@ -340,7 +340,7 @@ __global__ __launch_bounds__(64 * LRN_WAVES) void k_brush_learned(SgDevView d, S
         if (!ok || !(rel2 < 1.0f)) {  // out of range, NaN or a displacement beyond r
           nood++;
         } else {
-          esum += __double2ll_rn((double)rel2 * 1099511627776.0);  // 2^40
+          esum += __double2ll_rn((double)rel2 * 4294967296.0);  // 2^32: rel2 < 1, no overflow below 2^31 vertices
         }
       }
       wbase = __shfl(wbase, 0);
@@ -368,7 +368,7 @@ __global__ __launch_bounds__(256) void k_post_learned(SgDevView d, SgDevView dq,
   const long long es = d.ctr[SG_CTR_LERR];
   const long long nood = d.ctr[SG_CTR_LOOD];
   // error estimate: alpha * RMS(|d| / r), alpha = held-out RMS error / RMS displacement
-  const double rms = na > 0 ? sqrt((double)es / 1099511627776.0 / (double)na) : 0.0;
+  const double rms = na > 0 ? sqrt((double)es / 4294967296.0 / (double)na) : 0.0;
   const bool accept = nood == 0 && (double)alpha * rms <= (double)tol;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int nthreads = gridDim.x * blockDim.x;
@@ -483,6 +483,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(const float *w, const float *
     for (int s = 0; s < TB; s++) g += sZ2[s][tid];
     G[SG_OFF_B2 + tid] = g;
   }
+  __syncthreads();  // dW2 above still reads A1
   // dZ1 in place
   for (int q = tid; q < TB * 64; q += 256) {
     const int s = q / 64, i = q % 64;
