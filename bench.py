@@ -32,6 +32,9 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBPS = 8000.0
+# HBM bytes per dab from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (summed over the
+# kernels of the stroke, scaled by the copy-kernel calibration); None until measured.
+TRAFFIC_BYTES_PER_DAB = None
 METRIC = "vertex-updates/s + HBM GB/s; learned vs plain Smooth dabs/s at ≤1e-3 RMS"
 
 
@@ -83,6 +86,64 @@ def dab_bytes(sg, tool, k, c):
     return out
 
 
+def run_learned(sg, A, W, args):
+    """Learned Smooth on the same mesh: 500 Smooth(4) dabs, r = 0.05, s ~ U[0.2, 1.0]
+    along a seeded great circle; plain vs learned strokes timed from the pristine mesh
+    (HIP events around each stroke; rejected learned dabs are redone by the plain brush
+    inside the timed region), then every dab judged against the plain brush."""
+    from dune_sculpt_amd import learned as L
+
+    if not L.MODEL_PATH.exists():
+        return {"skipped": f"no trained model at {L.MODEL_PATH.name}"}
+    m = sg.MLP.load(L.MODEL_PATH)
+    alpha, tol = m.decision()
+    n = args.learned_dabs
+    rng = np.random.default_rng(5)
+    radii = np.full(n, 0.05, np.float32)
+    strengths = rng.uniform(0.2, 1.0, n).astype(np.float32)
+    centres = surface_centres(sg, sg.great_circle_centres(6, radii))
+    plain = sg.stroke_config(np.full(n, sg.TOOL_SMOOTH), centres, radii, strengths, smooth_iterations=4)
+    learned = L.learned_dabs(plain)
+    A.set_learned(m)
+    out = {"dabs": n, "radius": 0.05, "k": 4, "model": L.MODEL_PATH.name, "alpha": alpha, "tol": tol}
+    K = args.learned_steps
+    for name, dabs in (("plain", plain), ("learned", learned)):
+        prep = A.prepare(dabs)
+        st0 = A.learned_stats()
+        times = []
+        for s in range(W + K):
+            A.restore()
+            A.sync()
+            A.timer_start()
+            prep.launch(sync=False)
+            ms = A.timer_stop()
+            if s >= W:
+                times.append(ms)
+        st1 = A.learned_stats()
+        prep.close()
+        ms = float(np.mean(times))
+        out[name] = {"ms_per_stroke": ms, "dabs_per_s": n / (ms * 1e-3), "us_per_dab": ms * 1e3 / n}
+        if name == "learned":
+            out[name]["accepted_per_stroke"] = (st1["accepted"] - st0["accepted"]) / (W + K)
+            out[name]["fallbacks_per_stroke"] = (st1["fallbacks"] - st0["fallbacks"]) / (W + K)
+    out["speedup"] = out["learned"]["dabs_per_s"] / out["plain"]["dabs_per_s"]
+    if not args.no_judge:
+        A.restore()
+        t = time.perf_counter()
+        errs, disp, acc = L.judge(A, plain, m)
+        out["judge"] = {"dabs": int(len(errs)), "seconds": round(time.perf_counter() - t, 2),
+                        "rms_err_rel_r_mean": float(errs.mean()), "rms_err_rel_r_median": float(np.median(errs)),
+                        "rms_err_rel_r_max": float(errs.max()),
+                        "frac_dabs_within_1e-3": float((errs <= 1e-3).mean()),
+                        "rms_plain_disp_rel_r_mean": float(disp.mean()),
+                        "accepted_frac": float(acc.mean()),
+                        "rms_err_rel_r_accepted_max": float(errs[acc].max()) if acc.any() else None}
+    A.set_learned(None)
+    m.close()
+    log(f"learned smooth: {out}")
+    return out
+
+
 def cpu_baseline(sg, co, tris, dabs, budget_s, threads):
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle as po
@@ -117,6 +178,10 @@ def main():
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--tune-dabs", type=int, default=200)
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--learned-dabs", type=int, default=500)
+    ap.add_argument("--learned-steps", type=int, default=3)
+    ap.add_argument("--no-judge", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -159,16 +224,19 @@ def main():
         else:
             A.stroke(dabs, graph=False, log=True)
 
-    step_times = []
+    step_times, event_ms = [], []
     for s in range(W + K):
         A.restore()  # untimed: back to the pristine mesh
         A.sync()
         t0 = time.perf_counter()
+        A.timer_start()
         step()
-        A.sync()
+        ms = A.timer_stop()  # HIP events on the engine's stream around the whole stroke
         if s >= W:
             step_times.append(time.perf_counter() - t0)
+            event_ms.append(ms)
     elapsed = sum(step_times)
+    event_s = sum(event_ms) / 1e3
     logrows = A.stroke_log(args.dabs)
     log(f"timed {K} strokes in {elapsed:.3f}s ({elapsed / K * 1e3:.2f} ms/stroke, "
         f"{elapsed / K / args.dabs * 1e6:.2f} us/dab)")
@@ -194,36 +262,21 @@ def main():
     vups = timed_aff / elapsed
     gbps = timed_bytes / elapsed / 1e9
 
-    # per-kernel times: one more stroke from the same mesh, each launch bracketed by
-    # start/stop events that its dispatch packet timestamps (hipExtLaunchKernelGGL)
-    A.restore()
-    A.set_profile(True)
-    A.stroke(dabs, graph=False)
-    prof_stream = A.profile_read()
-    prof = prof_stream
-    A.set_profile(False)
-    n_smooth = int((tools == sg.TOOL_SMOOTH).sum())
-    kb = {
-        "k_search": kern_bytes["search"],
-        "k_area": kern_bytes.get("area", 0),
-        "k_brush": kern_bytes["brush"],
-        "k_smooth_iter": kern_bytes.get("smooth_step", 0) * 3,
-        "k_post": kern_bytes["post"],
-    }
-    kernels = {}
-    for name, (ms, cnt) in prof.items():
-        if cnt:
-            avg_us = ms / cnt * 1e3
-            kernels[name] = {"launches": cnt, "total_ms": round(ms, 3), "avg_us": round(avg_us, 3)}
-            if name in kb and kb[name]:
-                kernels[name]["alg_bytes_per_launch"] = kb[name] / cnt
-                kernels[name]["achieved_gbps"] = kb[name] / (ms * 1e-3) / 1e9
-    dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
-    dk = kernels[dom]
-    achieved = dk.get("achieved_gbps", 0.0)
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 5), "traffic": None,
-                "alg_bytes_per_launch": dk.get("alg_bytes_per_launch"), "avg_launch_us": dk["avg_us"]}
+    # Roofline of the dab step: the stroke graph is one dependent chain of small kernels
+    # (k_search -> [k_area] -> k_brush -> [k_smooth_iter x3] -> k_post per dab), so
+    # its unit is the dab: algorithmic bytes per dab over event time per dab.
+    us_per_dab_ev = event_s / (K * args.dabs) * 1e6
+    bytes_per_dab = stroke_bytes / args.dabs
+    achieved = bytes_per_dab / (us_per_dab_ev * 1e-6) / 1e9
+    roofline = {"bound": "hbm", "kernel": "dab step (stroke hipGraph, all kernels of one dab)",
+                "achieved": round(achieved, 3), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBPS, 6), "traffic": TRAFFIC_BYTES_PER_DAB,
+                "alg_bytes_per_launch": round(bytes_per_dab, 1), "avg_launch_us": round(us_per_dab_ev, 3)}
+    kernel_bytes_per_dab = {k: round(v / args.dabs, 1) for k, v in kern_bytes.items()}
+
+    c3 = None
+    if not args.no_learned:
+        c3 = run_learned(sg, A, W, args)
 
     cpu = None
     if not args.no_cpu:
@@ -261,7 +314,9 @@ def main():
         "alg_bytes_per_dab": timed_bytes / (K * args.dabs),
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "kernels": kernels,
+        "event_s": event_s,
+        "alg_bytes_per_dab_by_kernel": kernel_bytes_per_dab,
+        "learned_smooth": c3,
 
         "build_s": build_s,
         "autotune": tune,

@@ -96,7 +96,8 @@ EXPORTS = [
     "sg_pbvh_autotune", "sg_mlp_new", "sg_mlp_free", "sg_mlp_get_params", "sg_mlp_set_params",
     "sg_mlp_set_norm", "sg_mlp_get_norm", "sg_mlp_set_decision", "sg_mlp_get_decision", "sg_mlp_set_data",
     "sg_mlp_gradient", "sg_mlp_train", "sg_mlp_predict", "sg_mfma_probe", "sg_pbvh_set_learned",
-    "sg_pbvh_learned_stats", "sg_learned_collect",
+    "sg_pbvh_learned_stats", "sg_learned_collect", "sg_pbvh_timer_start", "sg_pbvh_timer_stop",
+    "sg_learned_judge",
 ]
 SMOOTH_LEARNED = 1 << 30
 NCTR = 24
@@ -186,6 +187,9 @@ def lib():
         "sg_pbvh_set_learned": (i32, [vp, vp]),
         "sg_pbvh_learned_stats": (i32, [vp, C.POINTER(C.c_longlong)]),
         "sg_learned_collect": (i32, [vp, C.POINTER(SgDabParams), f32p, f32p, f32p, i32]),
+        "sg_learned_judge": (i32, [vp, C.POINTER(SgDabParams), C.POINTER(C.c_double)]),
+        "sg_pbvh_timer_start": (i32, [vp]),
+        "sg_pbvh_timer_stop": (i32, [vp, f32p]),
         "sg_pbvh_snapshot_restore": (i32, [vp]),
         "sg_stroke_launch": (i32, [vp, i32]),
         "sg_stroke_free": (None, [vp]),
@@ -534,6 +538,23 @@ class PBVH:
 
     def set_stats(self, on=True):
         lib().sg_pbvh_set_stats(self.h, 1 if on else 0)
+
+    def judge(self, d):
+        """Learned vs plain Smooth dab from the current state (the plain result stays
+        applied): (RMS error / r, RMS plain displacement / r, accepted, affected)."""
+        out = (C.c_double * 4)()
+        _check(lib().sg_learned_judge(self.h, C.byref(d), out), "judge")
+        return float(out[0]), float(out[1]), bool(out[2]), int(out[3])
+
+    def timer_start(self):
+        """Record a HIP event on the engine's stream (start of a timed region)."""
+        _check(lib().sg_pbvh_timer_start(self.h), "timer start")
+
+    def timer_stop(self):
+        """Record the closing event, wait for it; milliseconds since timer_start."""
+        ms = C.c_float()
+        _check(lib().sg_pbvh_timer_stop(self.h, C.byref(ms)), "timer stop")
+        return ms.value
 
     def set_profile(self, on=True):
         _check(lib().sg_pbvh_set_profile(self.h, 1 if on else 0), "profile")

@@ -194,9 +194,14 @@ void BKE_pbvh_free(PBVH *p) {
     }
   }
   if (!p->no_device) sg_free_device(p);
-  for (void *q : p->snap) hipFree(q);
+  for (int sl = 0; sl < 2; sl++)
+    for (void *q : p->snap[sl]) hipFree(q);
+  if (p->judge_buf) hipFree(p->judge_buf);
+  if (p->judge_acc) hipFree(p->judge_acc);
   if (p->verts_owned) free(p->host_verts);
   free(p->looptri);
+  for (int i = 0; i < 2; i++)
+    if (p->timer[i]) hipEventDestroy(p->timer[i]);
   if (!p->no_device) hipStreamDestroy(p->stream);
   delete p;
 }
@@ -364,30 +369,35 @@ std::vector<SnapItem> snap_items(PBVH *p) {
 }
 }  // namespace
 
-int sg_pbvh_snapshot_save(PBVH *p) {
+static int snap_save(PBVH *p, int slot) {
   if (!p || p->no_device || !p->build_ok) return sg_set_error("no usable PBVH");
   auto items = snap_items(p);
-  if (p->snap.size() != items.size()) {
-    for (void *q : p->snap) hipFree(q);
-    p->snap.assign(items.size(), nullptr);
+  std::vector<void *> &sn = p->snap[slot];
+  if (sn.size() != items.size()) {
+    for (void *q : sn) hipFree(q);
+    sn.assign(items.size(), nullptr);
     for (size_t i = 0; i < items.size(); i++)
-      if (sg_check_hip(hipMalloc(&p->snap[i], std::max<size_t>(items[i].bytes, 4)), "hipMalloc")) return -1;
+      if (sg_check_hip(hipMalloc(&sn[i], std::max<size_t>(items[i].bytes, 4)), "hipMalloc")) return -1;
   }
   if (p->nodes_dirty_deferred) sg_enqueue_flush(p);
   for (size_t i = 0; i < items.size(); i++)
-    hipMemcpyAsync(p->snap[i], items[i].dev, items[i].bytes, hipMemcpyDeviceToDevice, p->stream);
-  p->snap_valid = true;
+    hipMemcpyAsync(sn[i], items[i].dev, items[i].bytes, hipMemcpyDeviceToDevice, p->stream);
+  p->snap_valid[slot] = true;
   return sg_check_hip(hipStreamSynchronize(p->stream), "snapshot");
 }
 
-int sg_pbvh_snapshot_restore(PBVH *p) {
-  if (!p || !p->snap_valid) return sg_set_error("no snapshot");
+static int snap_restore(PBVH *p, int slot) {
+  if (!p || !p->snap_valid[slot]) return sg_set_error("no snapshot");
   auto items = snap_items(p);
   for (size_t i = 0; i < items.size(); i++)
-    hipMemcpyAsync(items[i].dev, p->snap[i], items[i].bytes, hipMemcpyDeviceToDevice, p->stream);
+    hipMemcpyAsync(items[i].dev, p->snap[slot][i], items[i].bytes, hipMemcpyDeviceToDevice, p->stream);
   invalidate_host_views(p);
   return sg_check_hip(hipStreamSynchronize(p->stream), "restore");
 }
+
+int sg_pbvh_snapshot_save(PBVH *p) { return snap_save(p, 0); }
+
+int sg_pbvh_snapshot_restore(PBVH *p) { return snap_restore(p, 0); }
 
 int sg_dab_apply(PBVH *p, const SgDabParams *dab) {
   if (int rc = check_dab(p, dab)) return rc;
@@ -621,6 +631,11 @@ int sg_stroke_launch(SgStroke *s, int sync) {
   if (!rc && s->learned) {
     int h = 0;
     rc = sg_check_hip(hipMemcpy(&h, p->d.halt, 4, hipMemcpyDeviceToHost), "copy");
+    const int ran = h ? h - 1 : s->n;  // dabs the graph completed
+    if (p->mlp)
+      for (int i = 0; i < ran; i++)
+        p->learned_accepted +=
+            (s->dabs[i].sculpt_tool == SG_TOOL_SMOOTH && (s->dabs[i].flag & SG_SMOOTH_LEARNED)) ? 1 : 0;
     if (!rc && h) {  // dab h-1 was rejected: plain brush for it, then the rest
       const int f = h - 1;
       SgDabParams plain = s->dabs[f];
@@ -656,7 +671,7 @@ static double time_stroke(PBVH *p, const SgDabParams *dabs, int n, int reps) {
   hipEventCreate(&e1);
   double best = 1e30;
   for (int r = 0; r < reps + 1; r++) {
-    sg_pbvh_snapshot_restore(p);
+    snap_restore(p, 1);
     hipEventRecord(e0, p->stream);
     sg_stroke_launch(st, 0);
     hipEventRecord(e1, p->stream);
@@ -673,7 +688,7 @@ static double time_stroke(PBVH *p, const SgDabParams *dabs, int n, int reps) {
 
 int sg_pbvh_autotune(PBVH *p, const SgDabParams *dabs, int n, double *out_ms) {
   if (!p || !dabs || n <= 0) return sg_set_error("bad arguments");
-  if (int rc = sg_pbvh_snapshot_save(p)) return rc;
+  if (int rc = snap_save(p, 1)) return rc;
   const int cand[] = {32, 64, 128, 192, 256, 384, 512};
   int *slots[3] = {&p->grid_brush, &p->grid_post, &p->grid_smooth};
   for (int *sl : slots) *sl = p->grid_wg;
@@ -696,11 +711,51 @@ int sg_pbvh_autotune(PBVH *p, const SgDabParams *dabs, int n, double *out_ms) {
       *sl = choice;
     }
   }
-  sg_pbvh_snapshot_restore(p);
+  snap_restore(p, 1);
   if (out_ms) {
     out_ms[0] = start;
     out_ms[1] = best;
   }
+  return 0;
+}
+
+// The plain brush as the judge of a learned Smooth dab: from the current state run the
+// learned dab and keep its positions, return to the state, run the plain dab (which
+// stays applied) and compare the two on the affected vertices.
+int sg_learned_judge(PBVH *p, const SgDabParams *dab, double out[4]) {
+  if (int rc = check_dab(p, dab)) return rc;
+  if (!p->mlp || dab->sculpt_tool != SG_TOOL_SMOOTH) return sg_set_error("judge needs a model and a Smooth dab");
+  const size_t V = (size_t)p->d.V;
+  if (!p->judge_buf && sg_check_hip(hipMalloc(&p->judge_buf, std::max<size_t>(V, 1) * 12), "hipMalloc")) return -1;
+  if (!p->judge_acc && sg_check_hip(hipMalloc(&p->judge_acc, 16), "hipMalloc")) return -1;
+  if (int rc = snap_save(p, 1)) return rc;
+  SgDabParams ld = *dab;
+  ld.flag |= SG_SMOOTH_LEARNED;
+  const long long acc0 = p->learned_accepted;
+  if (int rc = sg_dab_apply(p, &ld)) return rc;
+  const bool accepted = p->learned_accepted > acc0;
+  hipMemcpyAsync(p->judge_buf, p->d.px, V * 4, hipMemcpyDeviceToDevice, p->stream);
+  hipMemcpyAsync(p->judge_buf + V, p->d.py, V * 4, hipMemcpyDeviceToDevice, p->stream);
+  hipMemcpyAsync(p->judge_buf + 2 * V, p->d.pz, V * 4, hipMemcpyDeviceToDevice, p->stream);
+  if (int rc = snap_restore(p, 1)) return rc;
+  SgDabParams pd = *dab;
+  pd.flag &= ~SG_SMOOTH_LEARNED;
+  if (int rc = sg_dab_apply(p, &pd)) return rc;
+  const int na = (int)p->last_ctr[SG_CTR_NAFF];
+  double acc[2] = {0.0, 0.0};
+  if (na > 0) {
+    // snapshot slot 1 holds the positions before the dab (items 0..2)
+    int rc = sg_enqueue_judge(p, na, p->judge_buf, (const float *)p->snap[1][0], (const float *)p->snap[1][1],
+                              (const float *)p->snap[1][2], p->judge_acc);
+    if (!rc) rc = sg_check_hip(hipMemcpyAsync(acc, p->judge_acc, 16, hipMemcpyDeviceToHost, p->stream), "copy");
+    if (!rc) rc = sg_check_hip(hipStreamSynchronize(p->stream), "judge");
+    if (rc) return rc;
+  }
+  const double r = dab->radius;
+  out[0] = na > 0 ? sqrt(acc[0] / na) / r : 0.0;
+  out[1] = na > 0 ? sqrt(acc[1] / na) / r : 0.0;
+  out[2] = accepted ? 1.0 : 0.0;
+  out[3] = (double)na;
   return 0;
 }
 
@@ -774,6 +829,20 @@ int sg_pbvh_profile_read(PBVH *p, double *ms, long long *count, int n, int reset
     }
   }
   return SG_K_COUNT;
+}
+
+int sg_pbvh_timer_start(PBVH *p) {
+  if (!p || p->no_device) return sg_set_error("no device");
+  for (int i = 0; i < 2; i++)
+    if (!p->timer[i] && sg_check_hip(hipEventCreate(&p->timer[i]), "event create")) return -1;
+  return sg_check_hip(hipEventRecord(p->timer[0], p->stream), "event record");
+}
+
+int sg_pbvh_timer_stop(PBVH *p, float *ms) {
+  if (!p || p->no_device || !p->timer[0]) return sg_set_error("timer not started");
+  if (sg_check_hip(hipEventRecord(p->timer[1], p->stream), "event record")) return -1;
+  if (sg_check_hip(hipEventSynchronize(p->timer[1]), "event sync")) return -1;
+  return sg_check_hip(hipEventElapsedTime(ms, p->timer[0], p->timer[1]), "event elapsed");
 }
 
 int sg_dab_stroke_log(PBVH *p, long long *out, int n) {

@@ -11,6 +11,7 @@
 // plain Smooth(k), normalised per component.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <random>
@@ -784,6 +785,39 @@ int sg_enqueue_learned_post(PBVH *p, const SgDevView &v, SgMlp *m, int serial) {
   const int G = p->grid_post > 0 ? p->grid_post : p->grid_wg;
   hipLaunchKernelGGL(k_post_learned, dim3(G), dim3(256), 0, p->stream, v, q, m->tol, m->alpha, serial);
   return sg_check_hip(hipGetLastError(), "learned post launch");
+}
+
+// Squared distances on the affected vertices of the last dab: learned result vs the
+// plain result (acc[0]) and the plain result vs the positions before the dab (acc[1]).
+__global__ __launch_bounds__(256) void k_judge(const int *aff_v, int na, const float *px, const float *py,
+                                              const float *pz, const float *lrn, int V, const float *bx,
+                                              const float *by, const float *bz, double *acc) {
+  double e = 0.0, m = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
+    const int v = aff_v[i];
+    const double x = px[v], y = py[v], z = pz[v];
+    const double ex = lrn[v] - x, ey = lrn[V + v] - y, ez = lrn[2 * V + v] - z;
+    const double mx = x - bx[v], my = y - by[v], mz = z - bz[v];
+    e += ex * ex + ey * ey + ez * ez;
+    m += mx * mx + my * my + mz * mz;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    e += __shfl_xor(e, off);
+    m += __shfl_xor(m, off);
+  }
+  if (__lane_id() == 0) {
+    atomicAdd(&acc[0], e);
+    atomicAdd(&acc[1], m);
+  }
+}
+
+int sg_enqueue_judge(PBVH *p, int na, const float *lrn, const float *bx, const float *by, const float *bz,
+                     double *acc) {
+  hipMemsetAsync(acc, 0, 16, p->stream);
+  const int blocks = std::min((na + 255) / 256, 1024);
+  hipLaunchKernelGGL(k_judge, dim3(blocks), dim3(256), 0, p->stream, p->d.aff_v, na, p->d.px, p->d.py, p->d.pz, lrn,
+                     p->d.V, bx, by, bz, acc);
+  return sg_check_hip(hipGetLastError(), "judge launch");
 }
 
 int sg_enqueue_collect(PBVH *p, const SgDevView &v, float *X, float *frame, int cap) {

@@ -138,8 +138,11 @@ struct PBVH {
   int grid_wg = 256;                 // workgroups of the per-dab kernels (tunable)
   int grid_brush = 0, grid_post = 0, grid_smooth = 0;  // per-kernel overrides (autotuner)
   // device snapshot (checkpoint) of the mutable mesh state
-  std::vector<void *> snap;
-  bool snap_valid = false;
+  // slot 0: the caller's (sg_pbvh_snapshot_save/restore); slot 1: internal (autotuner, judge)
+  std::vector<void *> snap[2];
+  bool snap_valid[2] = {false, false};
+  float *judge_buf = nullptr;        // [3V] positions of the learned result (sg_learned_judge)
+  double *judge_acc = nullptr;       // [2] device sums
   // per-kernel event timing (sg_pbvh_set_profile)
   bool profile = false;
   bool capturing = false;            // a stroke is being captured (profiling uses event nodes)
@@ -149,6 +152,7 @@ struct PBVH {
   double prof_ms[16] = {0};
   long long prof_n[16] = {0};
   int stamp = 0;                     // stats-mode stamp generator
+  hipEvent_t timer[2] = {nullptr, nullptr};  // sg_pbvh_timer_start/stop
 };
 
 // host build (pbvh_build.cpp)

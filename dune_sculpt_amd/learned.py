@@ -111,25 +111,17 @@ def learned_dabs(dabs):
     return out
 
 
-def judge(pbvh, dabs, mlp, every=1):
+def judge(pbvh, dabs, mlp):
     """Per-dab RMS displacement error of the learned dab against the plain brush from
     the same state, relative to the radius (the plain result is kept, so the mesh
-    follows the plain stroke).  Returns the list of RMS/r."""
-    out = []
+    follows the plain stroke).  Returns (errors, plain displacements, accepted flags),
+    each relative to r."""
+    errs, disp, acc = [], [], []
     pbvh.set_learned(mlp)
-    for i, d in enumerate(dabs):
-        if i % every:
-            pbvh.dab(d)
-            continue
-        pbvh.snapshot()
-        ld = learned_dabs([d])[0]
-        pbvh.dab(ld)
-        pl = pbvh.positions()
-        aff = pbvh.last_affected()
-        pbvh.restore()
-        pbvh.dab(d)
-        pp = pbvh.positions()
-        if len(aff):
-            err = np.sqrt(((pl[aff].astype(np.float64) - pp[aff]) ** 2).sum(1).mean()) / d.radius
-            out.append(float(err))
-    return out
+    for d in dabs:
+        e, m, a, n = pbvh.judge(d)
+        if n:
+            errs.append(e)
+            disp.append(m)
+            acc.append(a)
+    return np.array(errs), np.array(disp), np.array(acc, bool)

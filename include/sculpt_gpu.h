@@ -249,6 +249,10 @@ int sg_pbvh_snapshot_restore(PBVH *pbvh);
  * (normals + leaf bounds), 5 subtree refit, 6 top refit. */
 int sg_pbvh_set_profile(PBVH *pbvh, int on);
 int sg_pbvh_profile_read(PBVH *pbvh, double *ms, long long *count, int n, int reset);
+/* Wall time of everything enqueued on the PBVH's stream between start and stop (a HIP
+ * event pair on that stream; stop waits for the second event). */
+int sg_pbvh_timer_start(PBVH *pbvh);
+int sg_pbvh_timer_stop(PBVH *pbvh, float *ms);
 
 /* Copies of the device state in original vertex order (AoS float[3]). */
 int sg_pbvh_read_positions(PBVH *pbvh, float *out);
@@ -284,6 +288,12 @@ int sg_mlp_predict(SgMlp *mlp, const float *X, int n, float *Y);
 int sg_mfma_probe(const uint16_t *A, const uint16_t *B, float *D);  /* 16x16 bf16 layout check */
 int sg_pbvh_set_learned(PBVH *pbvh, SgMlp *mlp);
 int sg_pbvh_learned_stats(PBVH *pbvh, long long out[2]);  /* accepted, fallbacks */
+/* The plain brush as the judge of a learned Smooth dab, from the current state: runs
+ * the learned dab, returns to the state, runs the plain dab (which stays applied).
+ * out = {RMS |learned - plain| / r, RMS |plain - before| / r over the affected
+ * vertices, 1 if the learned dab was accepted (0: it fell back to the plain brush),
+ * affected count}. */
+int sg_learned_judge(PBVH *pbvh, const SgDabParams *dab, double out[4]);
 /* A plain Smooth dab recording features (X[i][28]) before and targets (T[i][3]) after
  * it for each affected vertex; ok[i] = 0 for vertices outside the model's range.
  * Returns the number of rows. */
