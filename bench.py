@@ -108,19 +108,23 @@ def run_learned(sg, A, W, args):
     out = {"dabs": n, "radius": 0.05, "k": 4, "model": L.MODEL_PATH.name, "alpha": alpha, "tol": tol}
     K = args.learned_steps
     for name, dabs in (("plain", plain), ("learned", learned)):
-        prep = A.prepare(dabs)
+        prep = A.prepare(dabs) if args.mode == "graph" else None
         st0 = A.learned_stats()
         times = []
         for s in range(W + K):
             A.restore()
             A.sync()
             A.timer_start()
-            prep.launch(sync=False)
+            if prep is not None:
+                prep.launch(sync=False)
+            else:
+                A.stroke(dabs, graph=False)
             ms = A.timer_stop()
             if s >= W:
                 times.append(ms)
         st1 = A.learned_stats()
-        prep.close()
+        if prep is not None:
+            prep.close()
         ms = float(np.mean(times))
         out[name] = {"ms_per_stroke": ms, "dabs_per_s": n / (ms * 1e-3), "us_per_dab": ms * 1e3 / n}
         if name == "learned":
@@ -182,6 +186,8 @@ def main():
     ap.add_argument("--learned-dabs", type=int, default=500)
     ap.add_argument("--learned-steps", type=int, default=3)
     ap.add_argument("--no-judge", action="store_true")
+    ap.add_argument("--only-learned", action="store_true",
+                    help="run only the learned Smooth section (profiling); prints its JSON")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -206,6 +212,10 @@ def main():
 
     W, K = args.warmup, args.steps
     A.snapshot()  # the pristine mesh: every step applies the stroke to it
+    if args.only_learned:
+        print(json.dumps({"learned_smooth": run_learned(sg, A, W, args)}))
+        A.close()
+        return
     tune = None
     if not args.no_tune:
         t = time.perf_counter()
