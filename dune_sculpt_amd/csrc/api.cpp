@@ -411,6 +411,7 @@ int sg_dab_apply(PBVH *p, const SgDabParams *dab) {
   if (g_stats && !ensure_stats_buffers(p)) return -1;
   SgDabK k = sg_make_dab(dab, g_stats ? 1 : 0);
   const int stamp = (++p->stamp) * 128;
+  p->halt_live = k.learned && p->mlp;
   int rc = sg_enqueue_clear_ctr(p);
   if (!rc) rc = sg_enqueue_dab(p, k, curve, nullptr, 0, stamp, 0);
   if (!rc) rc = sg_enqueue_flush(p);
@@ -518,6 +519,7 @@ int sg_dab_apply_stroke(PBVH *p, const SgDabParams *dabs, int n, int flags) {
     return p->mlp ? c : 0;
   }();
   int nfallback = 0;
+  p->halt_live = nlearned > 0;
   while (a0 < n) {
     int halted = -1;
     rc = run_segment(a0, &halted);
@@ -529,6 +531,7 @@ int sg_dab_apply_stroke(PBVH *p, const SgDabParams *dabs, int n, int flags) {
     if ((rc = sg_dab_apply(p, &plain))) return rc;
     nfallback++;
     a0 = halted + 1;
+    p->halt_live = true;  // sg_dab_apply cleared it for the plain dab
   }
   p->learned_fallbacks += nfallback;
   p->learned_accepted += nlearned - nfallback;
@@ -593,6 +596,7 @@ SgStroke *sg_stroke_prepare(PBVH *p, const SgDabParams *dabs, int n, int flags) 
                   (SG_CM_TABLE + 1) * sizeof(float), hipMemcpyHostToDevice);
     }
   }
+  p->halt_live = s->learned;
   p->capturing = true;
   int rc = sg_check_hip(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal), "begin capture");
   int erc = 0;
@@ -785,6 +789,7 @@ int sg_learned_collect(PBVH *p, const SgDabParams *dab, float *X, float *T, floa
   SgDabParams plain = *dab;
   plain.flag &= ~SG_SMOOTH_LEARNED;
   SgDabK k = sg_make_dab(&plain, 0);
+  p->halt_live = false;
   int rc = sg_enqueue_clear_ctr(p);
   if (!rc) rc = sg_enqueue_dab(p, k, nullptr, nullptr, 0, 0, 0, &col);
   if (!rc) rc = sg_enqueue_flush(p);

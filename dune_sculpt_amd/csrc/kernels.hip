@@ -91,7 +91,7 @@ __device__ __forceinline__ void leaves_box(const float *bbs, const float *obbs, 
 
 __global__ __launch_bounds__(256) void k_search(SgDevView d, SgDabK k, long long *log, int log_index,
                                                 int serial) {
-  if (d.halt[0]) return;
+  if (d.halt && d.halt[0]) return;
   const int lane = lane_id();
   const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
   const int nw = gridDim.x * blockDim.x / SG_WAVE;
@@ -194,7 +194,7 @@ __global__ void k_clear_ctr(long long *ctr, int *halt) {
 // Positions and normals are fetched in the same load level.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_area(SgDevView d, SgDabK k) {
-  if (d.halt[0]) return;
+  if (d.halt && d.halt[0]) return;
   const int lane = lane_id();
   const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
   const int nw = gridDim.x * blockDim.x / SG_WAVE;
@@ -241,7 +241,7 @@ __global__ __launch_bounds__(256) void k_area(SgDevView d, SgDabK k) {
 // reservation is in flight while the displacement is computed.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_brush(SgDevView d, SgDabK k, const float *curve, int stamp) {
-  if (d.halt[0]) return;
+  if (d.halt && d.halt[0]) return;
   const int lane = lane_id();
   const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
   const int nw = gridDim.x * blockDim.x / SG_WAVE;
@@ -321,7 +321,7 @@ __global__ __launch_bounds__(256) void k_brush(SgDevView d, SgDabK k, const floa
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_smooth_iter(SgDevView d, int from_shadow, int stats, int stamp_dab,
                                                      int stamp) {
-  if (d.halt[0]) return;
+  if (d.halt && d.halt[0]) return;
   const int nthreads = gridDim.x * blockDim.x;
   const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
   const float *sx_ = from_shadow ? d.qx : d.px;
@@ -371,7 +371,7 @@ __global__ __launch_bounds__(256) void k_smooth_iter(SgDevView d, int from_shado
 // Copy the final Jacobi iterate from the shadow copy (odd k) or to it (even k), so
 // both arrays agree again.
 __global__ __launch_bounds__(256) void k_smooth_sync(SgDevView d, int from_shadow) {
-  if (d.halt[0]) return;
+  if (d.halt && d.halt[0]) return;
   const int na = (int)d.ctr[SG_CTR_NAFF];
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
     const int v = d.aff_v[i];
@@ -392,7 +392,7 @@ __global__ __launch_bounds__(256) void k_smooth_sync(SgDevView d, int from_shado
 // gathered leaves are marked, BKE_pbvh_node_mark_update).  Both only read positions.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_post(SgDevView d, int stats, int stamp, int sync_q) {
-  if (d.halt[0]) return;
+  if (d.halt && d.halt[0]) return;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int nthreads = gridDim.x * blockDim.x;
   const int gw = tid / SG_WAVE;
@@ -791,7 +791,7 @@ SgDevView sg_view(PBVH *p) {
   v.ctr = d.ctr;
   v.ctr_next = d.ctr + SG_CTR_COUNT;
   v.cut_touch = d.cut_touch;
-  v.halt = d.halt;
+  v.halt = p->halt_live ? d.halt : nullptr;  // no flag test in strokes without learned dabs
   v.vperm_old = d.vperm_old;
   v.stamp_v = d.stamp_v;
   v.stamp_v2 = d.stamp_v2;

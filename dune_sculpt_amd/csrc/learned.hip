@@ -35,9 +35,12 @@ __host__ __device__ inline uint16_t sg_f2bf(float f) {
 }
 __device__ inline float sg_bf2f(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
 
-__device__ inline float sg_silu(float z) { return z / (1.0f + __expf(-z)); }
+// SiLU with the hardware exp and reciprocal (about 1 ulp each; the network runs in
+// BF16 for inference and its FP32 gradients are checked to 2e-4 relative)
+__device__ inline float sg_sigmoid(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ inline float sg_silu(float z) { return z * sg_sigmoid(z); }
 __device__ inline float sg_silu_grad(float z) {
-  const float s = 1.0f / (1.0f + __expf(-z));
+  const float s = sg_sigmoid(z);
   return s * (1.0f + z * (1.0f - s));
 }
 
@@ -67,41 +70,62 @@ __global__ void k_mfma_probe(const uint16_t *A, const uint16_t *B, float *D) {
 }
 
 // Inference weights in LDS: W1t[64][32], W2t[64][64], W3t[16][64] (output-major, so a
-// B fragment is one 8-byte read), biases b1[64], b2[64], b3[16].
-struct MlpLds {
+// B fragment is one 8-byte read), biases b1[64], b2[64], b3[16].  Same layout as the
+// packed device copy, so the block loads it with 16-byte vector loads.
+struct __align__(16) MlpLds {
   uint16_t w1t[64 * 32];
   uint16_t w2t[64 * 64];
   uint16_t w3t[16 * 64];
   float b1[64], b2[64], b3[16];
 };
+#define MLP_W_VEC ((64 * 32 + 64 * 64 + 16 * 64) * 2 / 16)  // 896 uint4
+#define MLP_B_VEC (144 / 4)                                 // 36 float4
 
+// Every load is issued before the first LDS store (a fixed, unrolled count per thread),
+// so the fill costs one memory round trip.
+template <int NT>
 __device__ inline void mlp_lds_load(MlpLds &s, const SgMlpView &m) {
-  const int t = threadIdx.x, nt = blockDim.x;
-  for (int i = t; i < 64 * 32; i += nt) s.w1t[i] = m.wb[i];
-  for (int i = t; i < 64 * 64; i += nt) s.w2t[i] = m.wb[64 * 32 + i];
-  for (int i = t; i < 16 * 64; i += nt) s.w3t[i] = m.wb[64 * 32 + 64 * 64 + i];
-  for (int i = t; i < 64; i += nt) {
-    s.b1[i] = m.bias[i];
-    s.b2[i] = m.bias[64 + i];
+  const int t = threadIdx.x;
+  constexpr int NW = (MLP_W_VEC + NT - 1) / NT;
+  const uint4 *src = (const uint4 *)m.wb;
+  uint4 *dst = (uint4 *)s.w1t;
+  uint4 r[NW];
+#pragma unroll
+  for (int j = 0; j < NW; j++) {
+    const int i = t + j * NT;
+    r[j] = i < MLP_W_VEC ? src[i] : make_uint4(0u, 0u, 0u, 0u);
   }
-  for (int i = t; i < 16; i += nt) s.b3[i] = m.bias[128 + i];
+  const float4 b = t < MLP_B_VEC ? ((const float4 *)m.bias)[t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+  for (int j = 0; j < NW; j++) {
+    const int i = t + j * NT;
+    if (i < MLP_W_VEC) dst[i] = r[j];
+  }
+  if (t < MLP_B_VEC) ((float4 *)s.b1)[t] = b;
 }
 
+// Activation staging rows are padded to 72 bf16 (144 B) so the 8-byte MFMA fragment
+// reads of 16 consecutive rows fall in distinct LDS banks.
+#define MLP_ST 72
+
 // One wave evaluates the MLP for 64 rows (one per lane): x[28] already normalised;
-// bufA / bufB are this wave's 64x64 bf16 staging buffers, sOut 64x4 floats.
+// bufA / bufB are this wave's 64 x MLP_ST bf16 staging buffers, sOut 64x4 floats.
 __device__ inline void mlp_wave(const MlpLds &s, uint16_t *bufA, uint16_t *bufB, float *sOut, const float *x,
                                 float *y) {
   const int L = __lane_id();
   const int r = L & 15, kq = (L >> 4) * 4;
-  for (int k = 0; k < 28; k++) bufA[L * 64 + k] = sg_f2bf(x[k]);
-  for (int k = 28; k < 32; k++) bufA[L * 64 + k] = 0;
+  uint32_t *rowA = (uint32_t *)&bufA[L * MLP_ST];
+#pragma unroll
+  for (int k = 0; k < 14; k++) rowA[k] = (uint32_t)sg_f2bf(x[2 * k]) | ((uint32_t)sg_f2bf(x[2 * k + 1]) << 16);
+  rowA[14] = 0u;
+  rowA[15] = 0u;
   wave_lds_sync();
   // layer 1: [64 x 32] x [32 x 64]
   for (int mt = 0; mt < 4; mt++) {
     sg_float4 acc[4];
     for (int nt = 0; nt < 4; nt++) acc[nt] = sg_float4{0.0f, 0.0f, 0.0f, 0.0f};
     for (int kt = 0; kt < 2; kt++) {
-      const sg_short4 a = *(const sg_short4 *)&bufA[(mt * 16 + r) * 64 + kt * 16 + kq];
+      const sg_short4 a = *(const sg_short4 *)&bufA[(mt * 16 + r) * MLP_ST + kt * 16 + kq];
       for (int nt = 0; nt < 4; nt++) {
         const sg_short4 b = *(const sg_short4 *)&s.w1t[(nt * 16 + r) * 32 + kt * 16 + kq];
         acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nt], 0, 0, 0);
@@ -110,7 +134,7 @@ __device__ inline void mlp_wave(const MlpLds &s, uint16_t *bufA, uint16_t *bufB,
     for (int nt = 0; nt < 4; nt++) {
       for (int i = 0; i < 4; i++) {
         const float z = acc[nt][i] + s.b1[nt * 16 + r];
-        bufB[(mt * 16 + kq + i) * 64 + nt * 16 + r] = sg_f2bf(sg_silu(z));
+        bufB[(mt * 16 + kq + i) * MLP_ST + nt * 16 + r] = sg_f2bf(sg_silu(z));
       }
     }
   }
@@ -120,7 +144,7 @@ __device__ inline void mlp_wave(const MlpLds &s, uint16_t *bufA, uint16_t *bufB,
     sg_float4 acc[4];
     for (int nt = 0; nt < 4; nt++) acc[nt] = sg_float4{0.0f, 0.0f, 0.0f, 0.0f};
     for (int kt = 0; kt < 4; kt++) {
-      const sg_short4 a = *(const sg_short4 *)&bufB[(mt * 16 + r) * 64 + kt * 16 + kq];
+      const sg_short4 a = *(const sg_short4 *)&bufB[(mt * 16 + r) * MLP_ST + kt * 16 + kq];
       for (int nt = 0; nt < 4; nt++) {
         const sg_short4 b = *(const sg_short4 *)&s.w2t[(nt * 16 + r) * 64 + kt * 16 + kq];
         acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nt], 0, 0, 0);
@@ -129,7 +153,7 @@ __device__ inline void mlp_wave(const MlpLds &s, uint16_t *bufA, uint16_t *bufB,
     for (int nt = 0; nt < 4; nt++) {
       for (int i = 0; i < 4; i++) {
         const float z = acc[nt][i] + s.b2[nt * 16 + r];
-        bufA[(mt * 16 + kq + i) * 64 + nt * 16 + r] = sg_f2bf(sg_silu(z));
+        bufA[(mt * 16 + kq + i) * MLP_ST + nt * 16 + r] = sg_f2bf(sg_silu(z));
       }
     }
   }
@@ -138,7 +162,7 @@ __device__ inline void mlp_wave(const MlpLds &s, uint16_t *bufA, uint16_t *bufB,
   for (int mt = 0; mt < 4; mt++) {
     sg_float4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
     for (int kt = 0; kt < 4; kt++) {
-      const sg_short4 a = *(const sg_short4 *)&bufA[(mt * 16 + r) * 64 + kt * 16 + kq];
+      const sg_short4 a = *(const sg_short4 *)&bufA[(mt * 16 + r) * MLP_ST + kt * 16 + kq];
       const sg_short4 b = *(const sg_short4 *)&s.w3t[r * 64 + kt * 16 + kq];
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
     }
@@ -257,10 +281,10 @@ __global__ __launch_bounds__(256) void k_learned_target(SgDevView d, const float
 #define LRN_WAVES 2
 __global__ __launch_bounds__(64 * LRN_WAVES) void k_mlp_predict(SgMlpView m, const float *X, int n, float *Y) {
   __shared__ __align__(16) MlpLds s;
-  __shared__ __align__(16) uint16_t bufA[LRN_WAVES][64 * 64];
-  __shared__ __align__(16) uint16_t bufB[LRN_WAVES][64 * 64];
+  __shared__ __align__(16) uint16_t bufA[LRN_WAVES][64 * MLP_ST];
+  __shared__ __align__(16) uint16_t bufB[LRN_WAVES][64 * MLP_ST];
   __shared__ __align__(16) float sOut[LRN_WAVES][64 * 4];
-  mlp_lds_load(s, m);
+  mlp_lds_load<64 * LRN_WAVES>(s, m);
   __syncthreads();
   const int w = threadIdx.x / 64, L = __lane_id();
   const int nw = gridDim.x * LRN_WAVES;
@@ -291,21 +315,24 @@ __global__ __launch_bounds__(64 * LRN_WAVES) void k_mlp_predict(SgMlpView m, con
 __global__ __launch_bounds__(64 * LRN_WAVES) void k_brush_learned(SgDevView d, SgDabK k, const float *curve,
                                                                    SgMlpView m) {
   __shared__ __align__(16) MlpLds s;
-  __shared__ __align__(16) uint16_t bufA[LRN_WAVES][64 * 64];
-  __shared__ __align__(16) uint16_t bufB[LRN_WAVES][64 * 64];
+  __shared__ __align__(16) uint16_t bufA[LRN_WAVES][64 * MLP_ST];
+  __shared__ __align__(16) uint16_t bufB[LRN_WAVES][64 * MLP_ST];
   __shared__ __align__(16) float sOut[LRN_WAVES][64 * 4];
-  if (d.halt[0]) return;
-  mlp_lds_load(s, m);
+  if (d.halt && d.halt[0]) return;
+  // work items = (gathered leaf, half): a wave takes every other 64-vertex chunk of a
+  // leaf, so a leaf's chunks run on two waves at once
+  const int nitems = 2 * (int)d.ctr[SG_CTR_NGATHER];
+  if (blockIdx.x * LRN_WAVES >= nitems) return;  // whole block idle: skip the weight fill
+  mlp_lds_load<64 * LRN_WAVES>(s, m);
   __syncthreads();
   const int w = threadIdx.x / 64, lane = __lane_id();
   const int gw = blockIdx.x * LRN_WAVES + w;
   const int nw = gridDim.x * LRN_WAVES;
-  const int ng = (int)d.ctr[SG_CTR_NGATHER];
   long long esum = 0;
   int nood = 0;
-  for (int g = gw; g < ng; g += nw) {
-    const int2 vr = d.gl_v[g];
-    for (int base = vr.x; base < vr.y; base += SG_WAVE) {
+  for (int it = gw; it < nitems; it += nw) {
+    const int2 vr = d.gl_v[it >> 1];
+    for (int base = vr.x + (it & 1) * SG_WAVE; base < vr.y; base += 2 * SG_WAVE) {
       const int v = base + lane;
       const BrushIn in = brush_load(d, k, v, vr.y, true);
       float nx = 0.0f, ny = 0.0f, nz = 0.0f;
@@ -363,7 +390,7 @@ __global__ __launch_bounds__(64 * LRN_WAVES) void k_brush_learned(SgDevView d, S
 
 __global__ __launch_bounds__(256) void k_post_learned(SgDevView d, SgDevView dq, float tol, float alpha,
                                                       int serial) {
-  if (d.halt[0]) return;
+  if (d.halt && d.halt[0]) return;
   const int na = (int)d.ctr[SG_CTR_NAFF];
   const int ng = (int)d.ctr[SG_CTR_NGATHER];
   const long long es = d.ctr[SG_CTR_LERR];
@@ -380,7 +407,7 @@ __global__ __launch_bounds__(256) void k_post_learned(SgDevView d, SgDevView dq,
       d.qy[v] = d.py[v];
       d.qz[v] = d.pz[v];
     }
-    if (tid == 0) d.halt[0] = serial + 1;
+    if (tid == 0 && d.halt) d.halt[0] = serial + 1;
     return;
   }
   long long a = 0, b = 0, c = 0, nsh = 0;

@@ -153,6 +153,7 @@ struct PBVH {
   long long prof_n[16] = {0};
   int stamp = 0;                     // stats-mode stamp generator
   hipEvent_t timer[2] = {nullptr, nullptr};  // sg_pbvh_timer_start/stop
+  bool halt_live = false;            // the work being enqueued contains learned dabs (kernels test the halt flag)
 };
 
 // host build (pbvh_build.cpp)
