@@ -97,7 +97,11 @@ EXPORTS = [
     "sg_mlp_set_norm", "sg_mlp_get_norm", "sg_mlp_set_decision", "sg_mlp_get_decision", "sg_mlp_set_data",
     "sg_mlp_gradient", "sg_mlp_train", "sg_mlp_predict", "sg_mfma_probe", "sg_pbvh_set_learned",
     "sg_pbvh_learned_stats", "sg_learned_collect", "sg_pbvh_timer_start", "sg_pbvh_timer_stop",
-    "sg_learned_judge",
+    "sg_learned_judge", "sg_batch_new", "sg_batch_free", "sg_batch_info", "sg_batch_load", "sg_batch_launch",
+    "sg_batch_rewind", "sg_batch_run", "sg_batch_sync", "sg_batch_totals", "sg_batch_read_positions",
+    "sg_batch_read_normals", "sg_batch_snapshot_save", "sg_batch_snapshot_restore", "sg_batch_set_launch",
+    "sg_batch_get_launch", "sg_batch_autotune", "sg_batch_set_learned", "sg_batch_timer_start",
+    "sg_batch_timer_stop",
 ]
 SMOOTH_LEARNED = 1 << 30
 NCTR = 24
@@ -188,6 +192,25 @@ def lib():
         "sg_pbvh_learned_stats": (i32, [vp, C.POINTER(C.c_longlong)]),
         "sg_learned_collect": (i32, [vp, C.POINTER(SgDabParams), f32p, f32p, f32p, i32]),
         "sg_learned_judge": (i32, [vp, C.POINTER(SgDabParams), C.POINTER(C.c_double)]),
+        "sg_batch_new": (vp, [i32, C.POINTER(f32p), C.POINTER(i32p), i32p, i32p, i32]),
+        "sg_batch_free": (None, [vp]),
+        "sg_batch_info": (i32, [vp, C.POINTER(C.c_longlong), i32]),
+        "sg_batch_load": (i32, [vp, C.POINTER(SgDabParams), i32]),
+        "sg_batch_launch": (i32, [vp, i32, i32]),
+        "sg_batch_rewind": (i32, [vp]),
+        "sg_batch_run": (i32, [vp, C.POINTER(SgDabParams), i32, i32]),
+        "sg_batch_sync": (i32, [vp]),
+        "sg_batch_totals": (i32, [vp, C.POINTER(C.c_longlong), i32]),
+        "sg_batch_read_positions": (i32, [vp, i32, f32p]),
+        "sg_batch_read_normals": (i32, [vp, i32, f32p]),
+        "sg_batch_snapshot_save": (i32, [vp]),
+        "sg_batch_snapshot_restore": (i32, [vp]),
+        "sg_batch_set_launch": (i32, [vp, i32p]),
+        "sg_batch_get_launch": (i32, [vp, i32p]),
+        "sg_batch_autotune": (i32, [vp, C.POINTER(SgDabParams), i32, C.POINTER(C.c_double)]),
+        "sg_batch_set_learned": (i32, [vp, vp]),
+        "sg_batch_timer_start": (i32, [vp]),
+        "sg_batch_timer_stop": (i32, [vp, f32p]),
         "sg_pbvh_timer_start": (i32, [vp]),
         "sg_pbvh_timer_stop": (i32, [vp, f32p]),
         "sg_pbvh_snapshot_restore": (i32, [vp]),
@@ -690,3 +713,130 @@ class PBVH:
         if tot.value:
             C.CDLL(None).free(arr)
         return res
+
+
+# ---------------------------------------------------------------------------
+# many meshes, one dab per mesh per step
+# ---------------------------------------------------------------------------
+
+BATCH_TOTALS = ["gathered", "affected", "tests", "candidates", "deg_t", "smooth_nbr_reads", "shared", "area_verts",
+                "learned_accepted", "learned_rejected", "steps"]
+BATCH_LAUNCH = ["search", "brush", "smooth", "post", "learned", "brush_threads", "brush_vpl"]
+
+
+class Batch:
+    """Owner of an SgBatch: many meshes in one device arena, stepped together (one dab
+    per mesh per step, one captured hipGraph launch per step)."""
+
+    def __init__(self, meshes, leaf_limit: int = 128):
+        L = lib()
+        self.M = len(meshes)
+        cos = [np.ascontiguousarray(c, np.float32) for c, _ in meshes]
+        trs = [np.ascontiguousarray(t, np.int32) for _, t in meshes]
+        self.nverts = [c.shape[0] for c in cos]
+        co_p = (C.POINTER(C.c_float) * self.M)(*[_fp(c) for c in cos])
+        tr_p = (C.POINTER(C.c_int) * self.M)(*[_ip(t) for t in trs])
+        nv = (C.c_int * self.M)(*self.nverts)
+        nt = (C.c_int * self.M)(*[t.shape[0] for t in trs])
+        self.h = L.sg_batch_new(self.M, co_p, tr_p, nv, nt, leaf_limit)
+        if not self.h:
+            raise RuntimeError("sg_batch_new failed: " + last_error())
+        self._mlp = None
+        self._rows = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sg_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        out = (C.c_longlong * (6 + self.M))()
+        _check(lib().sg_batch_info(self.h, out, 6 + self.M), "batch info")
+        return {"meshes": out[0], "verts": out[1], "tris": out[2], "leaves": out[3], "cuts": out[4],
+                "vbase": [out[5 + i] for i in range(self.M + 1)]}
+
+    def _table(self, table):
+        steps = len(table)
+        arr = (SgDabParams * (steps * self.M))()
+        for s, row in enumerate(table):
+            assert len(row) == self.M
+            for m, d in enumerate(row):
+                if d is not None:
+                    arr[s * self.M + m] = d
+        return arr, steps
+
+    def load(self, table):
+        """table[step][mesh] = SgDabParams or None (no dab)."""
+        arr, steps = self._table(table)
+        _check(lib().sg_batch_load(self.h, arr, steps), "sg_batch_load")
+        self._rows = steps
+
+    def launch(self, steps=None, graph=True, sync=True):
+        steps = self._rows if steps is None else steps
+        flags = (0 if graph else 1) | (0 if sync else 2)
+        _check(lib().sg_batch_launch(self.h, steps, flags), "sg_batch_launch")
+
+    def rewind(self):
+        _check(lib().sg_batch_rewind(self.h), "rewind")
+
+    def run(self, table, graph=True):
+        self.load(table)
+        self.launch(graph=graph)
+
+    def sync(self):
+        _check(lib().sg_batch_sync(self.h), "sync")
+
+    def totals(self, reset=False):
+        out = (C.c_longlong * 16)()
+        _check(lib().sg_batch_totals(self.h, out, 1 if reset else 0), "totals")
+        return {k: int(out[i]) for i, k in enumerate(BATCH_TOTALS)}
+
+    def positions(self, m):
+        out = np.empty((self.nverts[m], 3), np.float32)
+        _check(lib().sg_batch_read_positions(self.h, m, _fp(out)), "read positions")
+        return out
+
+    def normals(self, m):
+        out = np.empty((self.nverts[m], 3), np.float32)
+        _check(lib().sg_batch_read_normals(self.h, m, _fp(out)), "read normals")
+        return out
+
+    def snapshot(self):
+        _check(lib().sg_batch_snapshot_save(self.h), "snapshot")
+
+    def restore(self):
+        _check(lib().sg_batch_snapshot_restore(self.h), "restore")
+
+    def set_launch(self, **kw):
+        g = (C.c_int * 7)(*[int(kw.get(k, 0)) for k in BATCH_LAUNCH])
+        _check(lib().sg_batch_set_launch(self.h, g), "set launch")
+
+    def launch_shape(self):
+        g = (C.c_int * 7)()
+        _check(lib().sg_batch_get_launch(self.h, g), "get launch")
+        return {k: int(g[i]) for i, k in enumerate(BATCH_LAUNCH)}
+
+    def autotune(self, table):
+        arr, steps = self._table(table)
+        out = (C.c_double * 2)()
+        _check(lib().sg_batch_autotune(self.h, arr, steps, out), "batch autotune")
+        self._rows = 0  # the tuner loaded its own sample
+        return float(out[0]), float(out[1])
+
+    def set_learned(self, mlp):
+        self._mlp = mlp
+        _check(lib().sg_batch_set_learned(self.h, mlp.h if mlp is not None else None), "set learned")
+
+    def timer_start(self):
+        _check(lib().sg_batch_timer_start(self.h), "timer start")
+
+    def timer_stop(self):
+        ms = C.c_float()
+        _check(lib().sg_batch_timer_stop(self.h, C.byref(ms)), "timer stop")
+        return ms.value

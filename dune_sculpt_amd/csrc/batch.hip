@@ -1,0 +1,1008 @@
+// Many independent meshes stepped together: each step applies one dab to every mesh
+// (or none, tool 0), and every kernel of the step covers all meshes at once, so the
+// per-launch latency that bounds a single serial stroke is paid once per step for the
+// whole batch instead of once per dab.
+//
+// Arithmetic is the single-mesh engine's (the same device helpers, same FP32 order):
+// each mesh of a batch ends bit-identical to the same dabs applied to that mesh alone.
+//
+// Step (one captured hipGraph, launched once per step; every kernel reads the step's
+// table row and serial from device memory, so the same graph serves every step):
+//   k_bsearch       wave per search cut of every mesh -> gathered leaves (+ mesh)
+//   k_barea         Draw meshes: area normal per mesh (exact fixed-point sums)
+//   k_bbrush        wave per gathered leaf: displacement + affected records (+ mesh)
+//   k_bbrush_learned / k_bdecide   learned Smooth meshes (learned.hip)
+//   k_bsmooth_iter  Jacobi steps 2..k of Smooth records
+//   k_bsmooth_sync  odd k: final iterate back to the positions
+//   k_bpost         normals of affected vertices, bounds of gathered leaves
+//   k_bstep_end     totals, decisions, counters cleared, step advanced
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "sg_batch.h"
+#include "sg_device.h"
+#include "sg_learned.h"
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int sb_uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+__global__ __launch_bounds__(256) void k_bsearch(SgBatchView b) {
+  const SgDevView &d = b.d;
+  const int lane = lane_id();
+  const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
+  const int nw = gridDim.x * blockDim.x / SG_WAVE;
+  const int serial = b.step[0], row = b.step[1];
+  long long ntests = 0;
+  for (int c = gw; c < d.C; c += nw) {
+    const int mesh = sb_uniform(b.cut_mesh[c]);
+    const SgDabK k = b.tab[(size_t)row * b.M + mesh];
+    const int touch = d.cut_touch[c];
+    const int lb = d.cut_lbeg[c], le = d.cut_lend[c];
+    if (k.tool == 0) {  // no dab for this mesh: still rebuild a box the previous step left stale
+      if (touch == serial - 1) {
+        BB6 nb, ob;
+        leaves_box(d.leaf_bb, d.leaf_obb, lb, le, nb, ob);
+        if (lane == 0) {
+          bb_store(d.cut_bb + 8 * (size_t)c, nb);
+          bb_store(d.cut_obb + 8 * (size_t)c, ob);
+        }
+      }
+      continue;
+    }
+    BB6 cb = bb_load((k.use_orig ? d.cut_obb : d.cut_bb) + 8 * (size_t)c);
+    const int cfl = d.cut_flags[c];
+    if (touch == serial - 1) {  // leaves of this cut moved in the previous step
+      BB6 nb, ob;
+      leaves_box(d.leaf_bb, d.leaf_obb, lb, le, nb, ob);
+      if (lane == 0) {
+        bb_store(d.cut_bb + 8 * (size_t)c, nb);
+        bb_store(d.cut_obb + 8 * (size_t)c, ob);
+      }
+      cb = k.use_orig ? ob : nb;
+      cb.mn[0] = __shfl(cb.mn[0], 0);
+      cb.mn[1] = __shfl(cb.mn[1], 0);
+      cb.mn[2] = __shfl(cb.mn[2], 0);
+      cb.mx[0] = __shfl(cb.mx[0], 0);
+      cb.mx[1] = __shfl(cb.mx[1], 0);
+      cb.mx[2] = __shfl(cb.mx[2], 0);
+    }
+    ntests++;
+    if ((cfl & 1) || !sphere_box(cb, k.cx, k.cy, k.cz, k.r2)) continue;
+    const float *leaf_bbs = k.use_orig ? d.leaf_obb : d.leaf_bb;
+    const int nl = le - lb;
+    ntests += nl;
+    for (int q0 = 0; q0 < nl; q0 += SG_WAVE * SEARCH_UNROLL) {
+      bool pass[SEARCH_UNROLL];
+#pragma unroll
+      for (int j = 0; j < SEARCH_UNROLL; j++) {
+        const int l = lb + q0 + j * SG_WAVE + lane;
+        pass[j] = false;
+        if (l < le) {
+          const BB6 bx = bb_load(leaf_bbs + 8 * (size_t)l);
+          pass[j] = !(d.leaf_flags[l] & 1) && sphere_box(bx, k.cx, k.cy, k.cz, k.r2);
+        }
+      }
+      unsigned long long m[SEARCH_UNROLL];
+      int cnt = 0;
+#pragma unroll
+      for (int j = 0; j < SEARCH_UNROLL; j++) {
+        m[j] = __ballot(pass[j]);
+        cnt += __popcll(m[j]);
+      }
+      if (cnt == 0) continue;
+      int wbase = 0;
+      if (lane == 0) wbase = (int)atomicAdd((unsigned long long *)&b.ctr[SB_NGATHER], (unsigned long long)cnt);
+      int2 vr[SEARCH_UNROLL], sr[SEARCH_UNROLL];
+#pragma unroll
+      for (int j = 0; j < SEARCH_UNROLL; j++) {
+        const int l = lb + q0 + j * SG_WAVE + lane;
+        if (pass[j]) {
+          vr[j] = b.leaf_vr[l];
+          sr[j] = b.leaf_shr[l];
+        }
+      }
+      wbase = __shfl(wbase, 0);
+      int off = wbase;
+#pragma unroll
+      for (int j = 0; j < SEARCH_UNROLL; j++) {
+        if (pass[j]) {
+          const int pos = off + mask_prefix(m[j]);
+          d.gathered[pos] = lb + q0 + j * SG_WAVE + lane;
+          d.gl_v[pos] = vr[j];
+          d.gl_s[pos] = sr[j];
+          b.gl_mesh[pos] = mesh;
+        }
+        off += __popcll(m[j]);
+      }
+      if (lane == 0) d.cut_touch[c] = serial;
+    }
+  }
+  if (lane == 0 && ntests) atomicAdd((unsigned long long *)&b.ctr[SB_TESTS], (unsigned long long)ntests);
+}
+
+// Draw meshes: area normal from the uniq vertices of gathered leaves within
+// normal_radius_factor * r (2^-32 fixed point, exact in any order).
+__global__ __launch_bounds__(256) void k_barea(SgBatchView b) {
+  const SgDevView &d = b.d;
+  const int lane = lane_id();
+  const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
+  const int nw = gridDim.x * blockDim.x / SG_WAVE;
+  const int row = b.step[1];
+  const int ng = (int)b.ctr[SB_NGATHER];
+  int ntot = 0;
+  for (int g = gw; g < ng; g += nw) {
+    const int mesh = sb_uniform(b.gl_mesh[g]);
+    const SgDabK k = b.tab[(size_t)row * b.M + mesh];
+    if (k.tool != SG_TOOL_DRAW) continue;
+    const int2 vr = d.gl_v[g];
+    long long sx = 0, sy = 0, sz = 0;
+    int na = 0;
+    for (int v = vr.x + lane; v < vr.y; v += SG_WAVE) {
+      const float px = d.px[v], py = d.py[v], pz = d.pz[v];
+      const float nx = d.nx[v], ny = d.ny[v], nz = d.nz[v];
+      if (d.vhide && d.respect_hide && d.vhide[v]) continue;
+      const float dx = k.cx - px;
+      const float dy = k.cy - py;
+      const float dz = k.cz - pz;
+      const float d2 = dx * dx + dy * dy + dz * dz;
+      if (d2 <= k.nr2) {
+        sx += __double2ll_rn((double)nx * 4294967296.0);
+        sy += __double2ll_rn((double)ny * 4294967296.0);
+        sz += __double2ll_rn((double)nz * 4294967296.0);
+        na++;
+      }
+    }
+    sx = wave_sum_ll(sx);
+    sy = wave_sum_ll(sy);
+    sz = wave_sum_ll(sz);
+    na = wave_sum_i(na);
+    if (lane == 0 && na) {
+      long long *mc = b.ctr + SB_COUNT + SB_PM * mesh;
+      atomicAdd((unsigned long long *)&mc[SBM_AREA_X], as_u64(sx));
+      atomicAdd((unsigned long long *)&mc[SBM_AREA_Y], as_u64(sy));
+      atomicAdd((unsigned long long *)&mc[SBM_AREA_Z], as_u64(sz));
+    }
+    ntot += na;
+  }
+  if (lane == 0 && ntot) atomicAdd((unsigned long long *)&b.ctr[SB_NAREA], (unsigned long long)ntot);
+}
+
+// Brush: wave per gathered leaf, VPL vertices per lane (a VPL*64-vertex tile per load
+// level).  Learned Smooth meshes are left to the learned brush.
+template <int VPL>
+__global__ __launch_bounds__(256) void k_bbrush(SgBatchView b) {
+  const SgDevView &d = b.d;
+  const int lane = lane_id();
+  const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
+  const int nw = gridDim.x * blockDim.x / SG_WAVE;
+  const int row = b.step[1];
+  const int ng = (int)b.ctr[SB_NGATHER];
+  long long ncand = 0, degt = 0, deg = 0;
+  for (int g = gw; g < ng; g += nw) {
+    const int mesh = sb_uniform(b.gl_mesh[g]);
+    const SgDabK k = b.tab[(size_t)row * b.M + mesh];
+    if (k.learned) continue;
+    const bool smooth = k.tool == SG_TOOL_SMOOTH;
+    float ox = 0.0f, oy = 0.0f, oz = 0.0f;
+    if (k.tool == SG_TOOL_DRAW) {  // offset (normalize(area) * r) * bstrength
+      const long long *mc = b.ctr + SB_COUNT + SB_PM * mesh;
+      float ax = (float)((double)mc[SBM_AREA_X] * (1.0 / 4294967296.0));
+      float ay = (float)((double)mc[SBM_AREA_Y] * (1.0 / 4294967296.0));
+      float az = (float)((double)mc[SBM_AREA_Z] * (1.0 / 4294967296.0));
+      sg_normalize(ax, ay, az);
+      ox = (ax * k.r) * k.bstr;
+      oy = (ay * k.r) * k.bstr;
+      oz = (az * k.r) * k.bstr;
+    }
+    const int2 vr = d.gl_v[g];
+    ncand += vr.y - vr.x;
+    for (int base = vr.x; base < vr.y; base += VPL * SG_WAVE) {
+      BrushIn in[VPL];
+      unsigned long long m[VPL];
+      int cnt = 0;
+#pragma unroll
+      for (int j = 0; j < VPL; j++) {
+        in[j] = brush_load(d, k, base + j * SG_WAVE + lane, vr.y, smooth);
+        m[j] = __ballot(in[j].aff);
+        cnt += __popcll(m[j]);
+      }
+      if (cnt == 0) continue;
+      int wbase = 0;
+      if (lane == 0) wbase = (int)atomicAdd((unsigned long long *)&b.ctr[SB_NAFF], (unsigned long long)cnt);
+      float fade[VPL];
+      long long nbr = 0;
+#pragma unroll
+      for (int j = 0; j < VPL; j++) {
+        fade[j] = 0.0f;
+        if (in[j].aff) fade[j] = brush_apply(d, k, nullptr, base + j * SG_WAVE + lane, in[j], ox, oy, oz, nbr);
+      }
+      wbase = __shfl(wbase, 0);
+      int off = wbase;
+#pragma unroll
+      for (int j = 0; j < VPL; j++) {
+        if (in[j].aff) {
+          const int pos = off + mask_prefix(m[j]);
+          d.aff_v[pos] = base + j * SG_WAVE + lane;
+          d.aff_vt[pos] = in[j].vt;
+          d.aff_vv[pos] = in[j].vv;
+          d.aff_fade[pos] = fade[j];
+          b.aff_mesh[pos] = mesh;
+          degt += in[j].vt.y - in[j].vt.x;
+        }
+        off += __popcll(m[j]);
+      }
+      if (smooth) deg += nbr * k.iters;
+    }
+  }
+  degt = wave_sum_ll(degt);
+  deg = wave_sum_ll(deg);
+  if (lane == 0) {
+    if (ncand) atomicAdd((unsigned long long *)&b.ctr[SB_CAND], as_u64(ncand));
+    if (degt) atomicAdd((unsigned long long *)&b.ctr[SB_DEGT], as_u64(degt));
+    if (deg) atomicAdd((unsigned long long *)&b.ctr[SB_DEG], as_u64(deg));
+  }
+}
+
+// Records of meshes running the plain Smooth this step: plain Smooth dabs, and
+// learned Smooth dabs that were rejected.
+__device__ __forceinline__ bool sb_plain_smooth(const SgBatchView &b, const SgDabK *kp, int mesh) {
+  return kp->tool == SG_TOOL_SMOOTH && (!kp->learned || sb_learned_rejected(b, mesh));
+}
+
+// Jacobi step it >= 2 (even steps read the shadow copy and write the positions,
+// odd steps the reverse).
+__global__ __launch_bounds__(256) void k_bsmooth_iter(SgBatchView b, int it) {
+  const SgDevView &d = b.d;
+  const int row = b.step[1];
+  const int na = (int)b.ctr[SB_NAFF];
+  const bool from_shadow = (it % 2) == 0;
+  const float *sx_ = from_shadow ? d.qx : d.px;
+  const float *sy_ = from_shadow ? d.qy : d.py;
+  const float *sz_ = from_shadow ? d.qz : d.pz;
+  float *dx_ = from_shadow ? d.px : d.qx;
+  float *dy_ = from_shadow ? d.py : d.qy;
+  float *dz_ = from_shadow ? d.pz : d.qz;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
+    const int mesh = b.aff_mesh[i];
+    const SgDabK *kp = &b.tab[(size_t)row * b.M + mesh];
+    if (it > kp->iters || !sb_plain_smooth(b, kp, mesh)) continue;
+    const int v = d.aff_v[i];
+    const int2 vv = d.aff_vv[i];
+    const float fade = d.aff_fade[i];
+    const float x = sx_[v], y = sy_[v], z = sz_[v];
+    float sx, sy, sz;
+    nbr_sum(d.vv_idx, vv.x, vv.y, sx_, sy_, sz_, sx, sy, sz);
+    float rx = x, ry = y, rz = z;
+    if (vv.y > vv.x) {
+      const float inv = 1.0f / (float)(vv.y - vv.x);
+      const float avx = sx * inv, avy = sy * inv, avz = sz * inv;
+      rx = x + (avx - x) * fade;
+      ry = y + (avy - y) * fade;
+      rz = z + (avz - z) * fade;
+    }
+    dx_[v] = rx;
+    dy_[v] = ry;
+    dz_[v] = rz;
+  }
+}
+
+// Odd k ends in the shadow copy: copy it to the positions.
+__global__ __launch_bounds__(256) void k_bsmooth_sync(SgBatchView b) {
+  const SgDevView &d = b.d;
+  const int row = b.step[1];
+  const int na = (int)b.ctr[SB_NAFF];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
+    const int mesh = b.aff_mesh[i];
+    const SgDabK *kp = &b.tab[(size_t)row * b.M + mesh];
+    if ((kp->iters & 1) == 0 || !sb_plain_smooth(b, kp, mesh)) continue;
+    const int v = d.aff_v[i];
+    d.px[v] = d.qx[v];
+    d.py[v] = d.qy[v];
+    d.pz[v] = d.qz[v];
+  }
+}
+
+// Normals of the affected vertices (even-k Smooth first refreshes the shadow copy),
+// then bounds of the gathered leaves.
+__global__ __launch_bounds__(256) void k_bpost(SgBatchView b) {
+  const SgDevView &d = b.d;
+  const int row = b.step[1];
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nthreads = gridDim.x * blockDim.x;
+  const int na = (int)b.ctr[SB_NAFF];
+  const int ng = (int)b.ctr[SB_NGATHER];
+  long long ntri = 0, nfa = 0, nvfa = 0, nsh = 0;
+  for (int i = tid; i < na; i += nthreads) {
+    const int mesh = b.aff_mesh[i];
+    const SgDabK *kp = &b.tab[(size_t)row * b.M + mesh];
+    const int v = d.aff_v[i];
+    const int2 vt = d.aff_vt[i];
+    if ((kp->iters & 1) == 0 && sb_plain_smooth(b, kp, mesh)) {
+      d.qx[v] = d.px[v];
+      d.qy[v] = d.py[v];
+      d.qz[v] = d.pz[v];
+    }
+    vertex_normal(d, v, vt.x, vt.y, 0, 0, ntri, nfa, nvfa);
+  }
+  const int gw = tid / SG_WAVE, nw = nthreads / SG_WAVE;
+  for (int g = gw; g < ng; g += nw) leaf_refit(d, d.gathered[g], d.gl_v[g], d.gl_s[g], nsh);
+  if (lane_id() == 0 && nsh) atomicAdd((unsigned long long *)&b.ctr[SB_SHARED], as_u64(nsh));
+}
+
+// End of step: learned decisions counted, counters added to the totals and cleared,
+// serial and table row advanced.  One block.
+__global__ __launch_bounds__(256) void k_bstep_end(SgBatchView b) {
+  __shared__ int s_acc, s_rej;
+  const int t = threadIdx.x;
+  if (t == 0) {
+    s_acc = 0;
+    s_rej = 0;
+  }
+  __syncthreads();
+  const int row = b.step[1];
+  int acc = 0, rej = 0;
+  for (int m = t; m < b.M; m += blockDim.x) {
+    const SgDabK *kp = &b.tab[(size_t)row * b.M + m];
+    if (kp->tool == SG_TOOL_SMOOTH && kp->learned) {
+      if (sb_learned_rejected(b, m))
+        rej++;
+      else
+        acc++;
+    }
+  }
+  if (acc) atomicAdd(&s_acc, acc);
+  if (rej) atomicAdd(&s_rej, rej);
+  long long c = 0;
+  if (t < SB_COUNT) c = b.ctr[t];
+  __syncthreads();  // every read of this step's counters is done
+  if (t < SB_COUNT) b.tot[t] += c;
+  const int n = SB_COUNT + SB_PM * b.M;
+  for (int i = t; i < n; i += blockDim.x) b.ctr[i] = 0;
+  if (t == 0) {
+    b.tot[SBT_LACC] += s_acc;
+    b.tot[SBT_LREJ] += s_rej;
+    b.tot[SBT_STEPS] += 1;
+    b.step[0] += 1;
+    b.step[1] += 1;
+  }
+}
+
+// Boxes of the cuts the last step touched (so boxes are exact between runs).
+__global__ __launch_bounds__(256) void k_bcut_refresh(SgBatchView b) {
+  const SgDevView &d = b.d;
+  const int lane = lane_id();
+  const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
+  const int nw = gridDim.x * blockDim.x / SG_WAVE;
+  const int last = b.step[0] - 1;
+  for (int c = gw; c < d.C; c += nw) {
+    if (d.cut_touch[c] != last) continue;
+    BB6 nb, ob;
+    leaves_box(d.leaf_bb, d.leaf_obb, d.cut_lbeg[c], d.cut_lend[c], nb, ob);
+    if (lane == 0) {
+      bb_store(d.cut_bb + 8 * (size_t)c, nb);
+      bb_store(d.cut_obb + 8 * (size_t)c, ob);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+namespace {
+
+// device array grown by appending host chunks (capacity doubles)
+template <class T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0, cap = 0;
+  int append(const T *src, size_t k, hipStream_t s) {
+    if (n + k > cap) {
+      size_t nc = std::max<size_t>(std::max<size_t>(cap * 2, n + k), 1024);
+      T *q = nullptr;
+      if (sg_check_hip(hipMalloc((void **)&q, nc * sizeof(T)), "hipMalloc")) return -1;
+      if (n) hipMemcpyAsync(q, p, n * sizeof(T), hipMemcpyDeviceToDevice, s);
+      hipStreamSynchronize(s);
+      if (p) hipFree(p);
+      p = q;
+      cap = nc;
+    }
+    if (k) hipMemcpyAsync(p + n, src, k * sizeof(T), hipMemcpyHostToDevice, s);
+    n += k;
+    return sg_check_hip(hipStreamSynchronize(s), "append");  // src is a temporary
+  }
+};
+
+template <class T>
+int dev_fill(T **dst, size_t n, int byte, hipStream_t s) {
+  if (n == 0) n = 1;
+  if (sg_check_hip(hipMalloc((void **)dst, n * sizeof(T)), "hipMalloc")) return -1;
+  return sg_check_hip(hipMemsetAsync(*dst, byte, n * sizeof(T), s), "hipMemsetAsync");
+}
+
+struct SnapItem {
+  void *dev;
+  size_t bytes;
+};
+std::vector<SnapItem> batch_snap_items(SgBatch *b) {
+  SgDev &d = b->d;
+  const size_t V = (size_t)d.V, L = (size_t)d.L, C = (size_t)d.C;
+  return {{d.px, V * 4},       {d.py, V * 4},          {d.pz, V * 4},       {d.qx, V * 4},
+          {d.qy, V * 4},       {d.qz, V * 4},          {d.nx, V * 4},       {d.ny, V * 4},
+          {d.nz, V * 4},       {d.leaf_bb, L * 32},    {d.leaf_obb, L * 32}, {d.cut_bb, C * 32},
+          {d.cut_obb, C * 32}};
+}
+
+void batch_free_device(SgBatch *b) {
+  SgDev &d = b->d;
+  void *ptrs[] = {d.px,       d.py,       d.pz,       d.qx,         d.qy,      d.qz,        d.nx,
+                  d.ny,       d.nz,       d.vt4,      d.vt_off,     d.vv_off,  d.vv_idx,    d.leaf_sh,
+                  d.leaf_flags, d.leaf_bb, d.leaf_obb, d.cut_bb,    d.cut_obb, d.cut_flags, d.cut_lbeg,
+                  d.cut_lend, d.cut_touch, d.gathered, d.gl_v,      d.gl_s,    d.aff_v,     d.aff_vt,
+                  d.aff_vv,   d.aff_fade, d.vperm_old, b->leaf_vr,  b->leaf_shr, b->cut_mesh, b->gl_mesh,
+                  b->aff_mesh, b->ctr,    b->tot,     b->step,      b->tab,    b->gather_buf};
+  for (void *q : ptrs)
+    if (q) hipFree(q);
+  for (void *q : b->snap)
+    if (q) hipFree(q);
+}
+
+int batch_check(SgBatch *b) {
+  if (!b) return sg_set_error("null batch");
+  return 0;
+}
+
+}  // namespace
+
+SgBatchView sg_batch_view(SgBatch *b) {
+  SgBatchView v;
+  v.d = sg_view_of(b->d, true, false);
+  v.M = b->M;
+  v.leaf_vr = b->leaf_vr;
+  v.leaf_shr = b->leaf_shr;
+  v.cut_mesh = b->cut_mesh;
+  v.gl_mesh = b->gl_mesh;
+  v.aff_mesh = b->aff_mesh;
+  v.ctr = b->ctr;
+  v.tot = b->tot;
+  v.step = b->step;
+  v.tab = b->tab;
+  v.alpha = b->mlp ? b->mlp->alpha : 0.0f;
+  v.tol = b->mlp ? b->mlp->tol : 0.0f;
+  return v;
+}
+
+int sg_batch_enqueue_step(SgBatch *b, int maxk, bool learned) {
+  const SgBatchView v = sg_batch_view(b);
+  hipStream_t s = b->stream;
+  const int sgrid = std::max(1, std::min(b->g_search, (b->d.C + 3) / 4));
+  hipLaunchKernelGGL(k_bsearch, dim3(sgrid), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_barea, dim3(b->g_brush), dim3(256), 0, s, v);
+  if (b->brush_vpl == 2)
+    hipLaunchKernelGGL(k_bbrush<2>, dim3(b->g_brush), dim3(b->brush_threads), 0, s, v);
+  else
+    hipLaunchKernelGGL(k_bbrush<1>, dim3(b->g_brush), dim3(b->brush_threads), 0, s, v);
+  if (learned) {
+    if (sg_batch_enqueue_learned_brush(b, v)) return -1;
+    if (sg_batch_enqueue_decide(b, v)) return -1;
+  }
+  for (int it = 2; it <= maxk; it++) hipLaunchKernelGGL(k_bsmooth_iter, dim3(b->g_smooth), dim3(256), 0, s, v, it);
+  if (maxk >= 1) hipLaunchKernelGGL(k_bsmooth_sync, dim3(b->g_smooth), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_bpost, dim3(b->g_post), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_bstep_end, dim3(1), dim3(256), 0, s, v);
+  return sg_check_hip(hipGetLastError(), "batch step launch");
+}
+
+static int batch_cut_refresh(SgBatch *b) {
+  const SgBatchView v = sg_batch_view(b);
+  const int grid = std::max(1, std::min(1024, (b->d.C + 3) / 4));
+  hipLaunchKernelGGL(k_bcut_refresh, dim3(grid), dim3(256), 0, b->stream, v);
+  return sg_check_hip(hipGetLastError(), "cut refresh launch");
+}
+
+extern "C" {
+
+SgBatch *sg_batch_new(int n_meshes, const float *const *co, const int *const *tris, const int *totvert,
+                      const int *tottri, int leaf_limit) {
+  if (n_meshes <= 0 || !co || !tris || !totvert || !tottri) {
+    sg_set_error("sg_batch_new: bad arguments");
+    return nullptr;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    (void)hipGetLastError();
+    sg_set_error("no HIP device");
+    return nullptr;
+  }
+  long long Vt = 0, Tt = 0;
+  for (int m = 0; m < n_meshes; m++) {
+    if (totvert[m] <= 0 || tottri[m] <= 0 || !co[m] || !tris[m]) {
+      sg_set_error("sg_batch_new: mesh %d is empty", m);
+      return nullptr;
+    }
+    Vt += totvert[m];
+    Tt += tottri[m];
+  }
+  if (Vt >= (1ll << 31) || 3 * Tt >= (1ll << 31) || Tt >= (1ll << 30)) {
+    sg_set_error("sg_batch_new: batch too large for 32-bit ids");
+    return nullptr;
+  }
+  SgBatch *b = new SgBatch();
+  b->M = n_meshes;
+  hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  hipStream_t s = b->stream;
+  DevBuf<float> px, py, pz;
+  DevBuf<int4> vt4;
+  DevBuf<int> vt_off, vv_off, vv_idx, leaf_sh, leaf_flags, cut_lbeg, cut_lend, cut_flags, cut_mesh, vperm;
+  DevBuf<float> leaf_bb, leaf_obb, cut_bb;
+  DevBuf<int2> leaf_vr, leaf_shr;
+  b->vbase.assign(1, 0);
+  b->lbase.assign(1, 0);
+  b->cbase.assign(1, 0);
+  b->tbase.assign(1, 0);
+  long long vvbase = 0, shbase = 0;
+  int rc = 0;
+  for (int m = 0; m < n_meshes && !rc; m++) {
+    const int V = totvert[m], T = tottri[m];
+    for (long long q = 0; q < 3ll * T; q++) {
+      if (tris[m][q] < 0 || tris[m][q] >= V) {
+        sg_set_error("sg_batch_new: mesh %d has a corner out of range", m);
+        rc = -1;
+        break;
+      }
+    }
+    if (rc) break;
+    PBVH *p = new PBVH();
+    p->no_device = true;
+    p->leaf_limit = leaf_limit > 0 ? leaf_limit : 10000;
+    p->totvert = V;
+    p->totprim = T;
+    p->tris.assign(tris[m], tris[m] + 3 * (size_t)T);
+    sg_build_tree(p, co[m], p->tris.data(), T, V);
+    SgLayout h;
+    sg_build_layout(p, co[m], h);
+    std::vector<int> lbeg, lend, height, leaf_cut;
+    sg_build_cut(p, lbeg, lend, height, leaf_cut);
+    const long long vb = b->vbase.back(), tb = b->tbase.back(), lb = b->lbase.back();
+    const int L = h.L, C = (int)p->cut.node.size();
+    // vertices
+    rc |= px.append(h.x.data(), V, s);
+    rc |= py.append(h.y.data(), V, s);
+    rc |= pz.append(h.z.data(), V, s);
+    for (int4 &e : h.vt4) {
+      e.x += (int)vb;
+      e.y += (int)vb;
+      e.z += (int)vb;
+      e.w = (int)(((unsigned)e.w & 0xC0000000u) | (unsigned)((e.w & 0x3fffffff) + (int)tb));
+    }
+    rc |= vt4.append(h.vt4.data(), h.vt4.size(), s);
+    for (int v = 0; v < V; v++) h.vt_off[v] += (int)(3 * tb);
+    rc |= vt_off.append(h.vt_off.data(), V, s);
+    const int nvv = h.vv_off[V];
+    for (int v = 0; v < V; v++) h.vv_off[v] += (int)vvbase;
+    rc |= vv_off.append(h.vv_off.data(), V, s);
+    for (int q = 0; q < nvv; q++) h.vv_idx[q] += (int)vb;
+    rc |= vv_idx.append(h.vv_idx.data(), nvv, s);
+    // leaves
+    std::vector<int2> vr(L), sr(L);
+    for (int l = 0; l < L; l++) {
+      vr[l] = make_int2(h.leaf_vbeg[l] + (int)vb, h.leaf_vbeg[l + 1] + (int)vb);
+      sr[l] = make_int2(h.leaf_sh_off[l] + (int)shbase, h.leaf_sh_off[l + 1] + (int)shbase);
+    }
+    rc |= leaf_vr.append(vr.data(), L, s);
+    rc |= leaf_shr.append(sr.data(), L, s);
+    for (int &x : h.leaf_sh) x += (int)vb;
+    rc |= leaf_sh.append(h.leaf_sh.data(), h.leaf_sh.size(), s);
+    rc |= leaf_flags.append(h.leaf_flags.data(), L, s);
+    rc |= leaf_bb.append(h.leaf_bb.data(), h.leaf_bb.size(), s);
+    rc |= leaf_obb.append(h.leaf_obb.data(), h.leaf_obb.size(), s);
+    // cuts
+    std::vector<int> cl(C), ce(C), cf(C, 0), cm(C, m);
+    std::vector<float> cbb((size_t)C * 8, 0.0f);
+    for (int c = 0; c < C; c++) {
+      cl[c] = p->cut.lbeg[c] + (int)lb;
+      ce[c] = p->cut.lend[c] + (int)lb;
+      const PBVHNode &n = p->nodes[p->cut.node[c]];
+      if ((n.flag & PBVH_Leaf) && (n.flag & PBVH_FullyHidden)) cf[c] = 1;
+      for (int k = 0; k < 3; k++) {
+        cbb[8 * (size_t)c + k] = n.vb[k];
+        cbb[8 * (size_t)c + 4 + k] = n.vb[3 + k];
+      }
+    }
+    rc |= cut_lbeg.append(cl.data(), C, s);
+    rc |= cut_lend.append(ce.data(), C, s);
+    rc |= cut_flags.append(cf.data(), C, s);
+    rc |= cut_mesh.append(cm.data(), C, s);
+    rc |= cut_bb.append(cbb.data(), cbb.size(), s);
+    rc |= vperm.append(p->perm_old_of_new.data(), V, s);
+    vvbase += nvv;
+    shbase += (long long)h.leaf_sh.size();
+    b->vbase.push_back(vb + V);
+    b->tbase.push_back(tb + T);
+    b->lbase.push_back(lb + L);
+    b->cbase.push_back(b->cbase.back() + C);
+    BKE_pbvh_free(p);
+  }
+  if (!rc) {
+    const int sentinel[2] = {(int)(3 * b->tbase.back()), (int)vvbase};
+    rc |= vt_off.append(&sentinel[0], 1, s);
+    rc |= vv_off.append(&sentinel[1], 1, s);
+    const int pad = 0;
+    rc |= vv_idx.append(&pad, 1, s);
+    rc |= leaf_sh.append(&pad, 1, s);
+  }
+  SgDev &d = b->d;
+  d.px = px.p;
+  d.py = py.p;
+  d.pz = pz.p;
+  d.vt4 = vt4.p;
+  d.vt_off = vt_off.p;
+  d.vv_off = vv_off.p;
+  d.vv_idx = vv_idx.p;
+  d.leaf_sh = leaf_sh.p;
+  d.leaf_flags = leaf_flags.p;
+  d.leaf_bb = leaf_bb.p;
+  d.leaf_obb = leaf_obb.p;
+  d.cut_lbeg = cut_lbeg.p;
+  d.cut_lend = cut_lend.p;
+  d.cut_flags = cut_flags.p;
+  d.cut_bb = cut_bb.p;
+  d.vperm_old = vperm.p;
+  b->leaf_vr = leaf_vr.p;
+  b->leaf_shr = leaf_shr.p;
+  b->cut_mesh = cut_mesh.p;
+  if (rc) {
+    batch_free_device(b);
+    hipStreamDestroy(b->stream);
+    delete b;
+    return nullptr;
+  }
+  const size_t V = (size_t)b->vbase.back(), L = (size_t)b->lbase.back(), C = (size_t)b->cbase.back();
+  d.V = (int)V;
+  d.T = (int)b->tbase.back();
+  d.L = (int)L;
+  d.C = (int)C;
+  rc |= dev_fill(&d.qx, V, 0, s);
+  rc |= dev_fill(&d.qy, V, 0, s);
+  rc |= dev_fill(&d.qz, V, 0, s);
+  if (!rc) {
+    hipMemcpyAsync(d.qx, d.px, V * 4, hipMemcpyDeviceToDevice, s);
+    hipMemcpyAsync(d.qy, d.py, V * 4, hipMemcpyDeviceToDevice, s);
+    hipMemcpyAsync(d.qz, d.pz, V * 4, hipMemcpyDeviceToDevice, s);
+  }
+  rc |= dev_fill(&d.nx, V, 0, s);
+  rc |= dev_fill(&d.ny, V, 0, s);
+  rc |= dev_fill(&d.nz, V, 0, s);
+  rc |= dev_fill(&d.cut_obb, C * 8, 0, s);
+  if (!rc) hipMemcpyAsync(d.cut_obb, d.cut_bb, C * 32, hipMemcpyDeviceToDevice, s);
+  rc |= dev_fill(&d.cut_touch, C, 0x80, s);  // never "touched by the previous step"
+  rc |= dev_fill(&d.gathered, L, 0, s);
+  rc |= dev_fill(&d.gl_v, L, 0, s);
+  rc |= dev_fill(&d.gl_s, L, 0, s);
+  rc |= dev_fill(&b->gl_mesh, L, 0, s);
+  rc |= dev_fill(&d.aff_v, V, 0, s);
+  rc |= dev_fill(&d.aff_vt, V, 0, s);
+  rc |= dev_fill(&d.aff_vv, V, 0, s);
+  rc |= dev_fill(&d.aff_fade, V, 0, s);
+  rc |= dev_fill(&b->aff_mesh, V, 0, s);
+  rc |= dev_fill(&b->ctr, SB_COUNT + SB_PM * (size_t)b->M, 0, s);
+  rc |= dev_fill(&b->tot, SBT_COUNT, 0, s);
+  rc |= dev_fill(&b->step, 2, 0, s);
+  if (!rc) {
+    const int st[2] = {1, 0};  // serial starts at 1: the fill value of cut_touch never equals serial - 1
+    rc = sg_check_hip(hipStreamSynchronize(s), "batch build");
+    if (!rc) rc = sg_check_hip(hipMemcpy(b->step, st, sizeof(st), hipMemcpyHostToDevice), "copy");
+    if (!rc) rc = sg_enqueue_full_normals_view(sg_view_of(d, true, false), s, 0);
+  }
+  if (!rc) rc = sg_check_hip(hipStreamSynchronize(s), "batch build");
+  if (rc) {
+    batch_free_device(b);
+    hipStreamDestroy(b->stream);
+    delete b;
+    return nullptr;
+  }
+  return b;
+}
+
+void sg_batch_free(SgBatch *b) {
+  if (!b) return;
+  hipStreamSynchronize(b->stream);
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  for (int i = 0; i < 2; i++)
+    if (b->timer[i]) hipEventDestroy(b->timer[i]);
+  batch_free_device(b);
+  hipStreamDestroy(b->stream);
+  delete b;
+}
+
+// out: meshes, vertices, tris, leaves, cuts, then the vertex and leaf offset of mesh i
+// for i < n_meshes when out has room (out_n entries).
+int sg_batch_info(SgBatch *b, long long *out, int out_n) {
+  if (batch_check(b)) return -1;
+  const long long head[5] = {b->M, b->vbase.back(), b->tbase.back(), b->lbase.back(), b->cbase.back()};
+  for (int i = 0; i < 5 && i < out_n; i++) out[i] = head[i];
+  for (int m = 0; m <= b->M && 5 + m < out_n; m++) out[5 + m] = b->vbase[m];
+  return 0;
+}
+
+// Upload a [steps][n_meshes] dab table (tool 0 = no dab for that mesh in that step)
+// and capture the step graph for it; sg_batch_launch then runs it.
+int sg_batch_load(SgBatch *b, const SgDabParams *dabs, int steps) {
+  if (batch_check(b)) return -1;
+  if (steps <= 0 || !dabs) return sg_set_error("sg_batch_load: bad arguments");
+  const size_t rows = (size_t)steps, M = (size_t)b->M;
+  std::vector<SgDabK> tab(rows * M);
+  int maxk = 0;
+  bool learned = false;
+  for (size_t i = 0; i < rows * M; i++) {
+    const SgDabParams &p = dabs[i];
+    if (p.sculpt_tool == 0) {
+      memset(&tab[i], 0, sizeof(SgDabK));
+      continue;
+    }
+    if (p.sculpt_tool != SG_TOOL_DRAW && p.sculpt_tool != SG_TOOL_SMOOTH && p.sculpt_tool != SG_TOOL_INFLATE &&
+        p.sculpt_tool != SG_TOOL_GRAB)
+      return sg_set_error("sg_batch_load: unsupported tool %d", p.sculpt_tool);
+    if (p.curve_preset == SG_CURVE_CUSTOM) return sg_set_error("sg_batch_load: custom falloff curves are not batched");
+    if (!(p.radius > 0.0f)) return sg_set_error("sg_batch_load: radius must be positive");
+    tab[i] = sg_make_dab(&p, 0);
+    if (!b->mlp) tab[i].learned = 0;
+    if (tab[i].tool == SG_TOOL_SMOOTH) maxk = std::max(maxk, tab[i].iters);
+    learned = learned || tab[i].learned;
+  }
+  // nothing in flight may still read the previous table
+  if (int rc = sg_check_hip(hipStreamSynchronize(b->stream), "batch sync")) return rc;
+  if (rows > b->tab_cap) {
+    if (b->exec) {
+      hipGraphExecDestroy(b->exec);
+      b->exec = nullptr;
+    }
+    if (b->tab) hipFree(b->tab);
+    b->tab = nullptr;
+    b->tab_cap = 0;
+    if (sg_check_hip(hipMalloc(&b->tab, rows * M * sizeof(SgDabK)), "hipMalloc")) return -1;
+    b->tab_cap = rows;
+  }
+  if (int rc = sg_check_hip(hipMemcpy(b->tab, tab.data(), rows * M * sizeof(SgDabK), hipMemcpyHostToDevice), "copy"))
+    return rc;
+  b->rows = (int)rows;
+  b->row_next = 0;
+  b->maxk = maxk;
+  b->learned = learned;
+  const int cfg[] = {maxk, learned ? 1 : 0, b->mlp ? 1 : 0, b->g_search, b->g_brush, b->g_smooth,
+                     b->g_post, b->g_learned, b->brush_threads, b->brush_vpl};
+  long long key = 1469598103934665603ll;
+  for (int c : cfg) key = (key ^ (long long)c) * 1099511628211ll;
+  if (b->exec && b->exec_key == key) return 0;
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  b->exec = nullptr;
+  hipGraph_t graph = nullptr;
+  int rc = sg_check_hip(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+  if (!rc) {
+    const int erc = sg_batch_enqueue_step(b, maxk, learned);
+    const hipError_t e = hipStreamEndCapture(b->stream, &graph);
+    rc = erc ? erc : sg_check_hip(e, "end capture");
+  }
+  if (!rc) rc = sg_check_hip(hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0), "instantiate");
+  if (graph) hipGraphDestroy(graph);
+  if (!rc) b->exec_key = key;
+  return rc;
+}
+
+// Run the next `steps` rows of the loaded table: one graph launch per step (flags bit
+// 0 clear: the step's kernels are enqueued one by one instead), then the boxes of the
+// cuts touched last are rebuilt.  flags bit 1: return without waiting.
+int sg_batch_launch(SgBatch *b, int steps, int flags) {
+  if (batch_check(b)) return -1;
+  if (!b->exec || steps <= 0 || b->row_next + steps > b->rows) return sg_set_error("sg_batch_launch: no such rows");
+  int rc = 0;
+  if (b->row_next == 0) {
+    static const int zero = 0;
+    rc = sg_check_hip(hipMemcpyAsync(b->step + 1, &zero, 4, hipMemcpyHostToDevice, b->stream), "copy");
+  }
+  for (int s = 0; s < steps && !rc; s++) {
+    if (flags & 1)
+      rc = sg_batch_enqueue_step(b, b->maxk, b->learned);
+    else
+      rc = sg_check_hip(hipGraphLaunch(b->exec, b->stream), "graph launch");
+  }
+  b->row_next += steps;
+  if (!rc) rc = batch_cut_refresh(b);
+  if (!rc && !(flags & 2)) rc = sg_check_hip(hipStreamSynchronize(b->stream), "batch run");
+  return rc;
+}
+
+// Restart the loaded table at row 0 (e.g. after a snapshot restore).
+int sg_batch_rewind(SgBatch *b) {
+  if (batch_check(b)) return -1;
+  b->row_next = 0;
+  return 0;
+}
+
+int sg_batch_run(SgBatch *b, const SgDabParams *dabs, int steps, int flags) {
+  if (int rc = sg_batch_load(b, dabs, steps)) return rc;
+  return sg_batch_launch(b, steps, flags);
+}
+
+int sg_batch_sync(SgBatch *b) {
+  if (batch_check(b)) return -1;
+  return sg_check_hip(hipStreamSynchronize(b->stream), "batch sync");
+}
+
+// totals since the last reset: [0..7] gathered leaves, affected vertices, box tests,
+// candidate vertices, vertex->tri degrees of affected vertices, Smooth neighbour reads,
+// shared vertices refit, area-normal vertices; [8] learned dabs accepted, [9] rejected
+// (redone plain), [10] steps.
+int sg_batch_totals(SgBatch *b, long long *out16, int reset) {
+  if (batch_check(b)) return -1;
+  int rc = sg_check_hip(hipMemcpyAsync(out16, b->tot, SBT_COUNT * 8, hipMemcpyDeviceToHost, b->stream), "copy");
+  if (!rc && reset) rc = sg_check_hip(hipMemsetAsync(b->tot, 0, SBT_COUNT * 8, b->stream), "memset");
+  if (!rc) rc = sg_check_hip(hipStreamSynchronize(b->stream), "totals");
+  return rc;
+}
+
+static int batch_read(SgBatch *b, int mesh, int which, float *out) {
+  if (batch_check(b)) return -1;
+  if (mesh < 0 || mesh >= b->M || !out) return sg_set_error("sg_batch_read: bad mesh");
+  const long long vb = b->vbase[mesh];
+  const int V = (int)(b->vbase[mesh + 1] - vb);
+  if ((size_t)V * 3 > b->gather_cap) {
+    if (b->gather_buf) hipFree(b->gather_buf);
+    b->gather_buf = nullptr;
+    if (sg_check_hip(hipMalloc(&b->gather_buf, (size_t)V * 12), "hipMalloc")) return -1;
+    b->gather_cap = (size_t)V * 3;
+  }
+  const SgDev &d = b->d;
+  const float *x = which ? d.nx : d.px, *y = which ? d.ny : d.py, *z = which ? d.nz : d.pz;
+  int rc = sg_enqueue_gather_raw(b->stream, x + vb, y + vb, z + vb, d.vperm_old + vb, V, b->gather_buf);
+  if (!rc) rc = sg_check_hip(hipMemcpyAsync(out, b->gather_buf, (size_t)V * 12, hipMemcpyDeviceToHost, b->stream),
+                             "copy");
+  if (!rc) rc = sg_check_hip(hipStreamSynchronize(b->stream), "read");
+  return rc;
+}
+
+int sg_batch_read_positions(SgBatch *b, int mesh, float *out) { return batch_read(b, mesh, 0, out); }
+int sg_batch_read_normals(SgBatch *b, int mesh, float *out) { return batch_read(b, mesh, 1, out); }
+
+int sg_batch_snapshot_save(SgBatch *b) {
+  if (batch_check(b)) return -1;
+  auto items = batch_snap_items(b);
+  if (b->snap.size() != items.size()) {
+    b->snap.assign(items.size(), nullptr);
+    for (size_t i = 0; i < items.size(); i++)
+      if (sg_check_hip(hipMalloc(&b->snap[i], std::max<size_t>(items[i].bytes, 4)), "hipMalloc")) return -1;
+  }
+  for (size_t i = 0; i < items.size(); i++)
+    hipMemcpyAsync(b->snap[i], items[i].dev, items[i].bytes, hipMemcpyDeviceToDevice, b->stream);
+  b->snap_valid = true;
+  return sg_check_hip(hipStreamSynchronize(b->stream), "snapshot");
+}
+
+int sg_batch_snapshot_restore(SgBatch *b) {
+  if (batch_check(b)) return -1;
+  if (!b->snap_valid) return sg_set_error("no batch snapshot");
+  auto items = batch_snap_items(b);
+  for (size_t i = 0; i < items.size(); i++)
+    hipMemcpyAsync(items[i].dev, b->snap[i], items[i].bytes, hipMemcpyDeviceToDevice, b->stream);
+  return sg_check_hip(hipStreamSynchronize(b->stream), "restore");
+}
+
+// g = {search, brush, smooth, post, learned brush workgroups, brush threads, brush
+// vertices per lane}; entries <= 0 keep the current value
+int sg_batch_set_launch(SgBatch *b, const int *g7) {
+  if (batch_check(b)) return -1;
+  int *dst[7] = {&b->g_search, &b->g_brush, &b->g_smooth, &b->g_post, &b->g_learned, &b->brush_threads,
+                 &b->brush_vpl};
+  for (int i = 0; i < 7; i++)
+    if (g7[i] > 0) *dst[i] = g7[i];
+  if (b->brush_threads != 64 && b->brush_threads != 128 && b->brush_threads != 256) b->brush_threads = 256;
+  if (b->brush_vpl != 1 && b->brush_vpl != 2) b->brush_vpl = 1;
+  return 0;
+}
+
+int sg_batch_get_launch(SgBatch *b, int *g7) {
+  if (batch_check(b)) return -1;
+  const int src[7] = {b->g_search, b->g_brush, b->g_smooth, b->g_post, b->g_learned, b->brush_threads,
+                      b->brush_vpl};
+  for (int i = 0; i < 7; i++) g7[i] = src[i];
+  return 0;
+}
+
+int sg_batch_timer_start(SgBatch *b) {
+  if (batch_check(b)) return -1;
+  for (int i = 0; i < 2; i++)
+    if (!b->timer[i] && sg_check_hip(hipEventCreate(&b->timer[i]), "event create")) return -1;
+  return sg_check_hip(hipEventRecord(b->timer[0], b->stream), "event record");
+}
+
+int sg_batch_timer_stop(SgBatch *b, float *ms) {
+  if (batch_check(b) || !b->timer[0]) return sg_set_error("timer not started");
+  if (sg_check_hip(hipEventRecord(b->timer[1], b->stream), "event record")) return -1;
+  if (sg_check_hip(hipEventSynchronize(b->timer[1]), "event sync")) return -1;
+  return sg_check_hip(hipEventElapsedTime(ms, b->timer[0], b->timer[1]), "event elapsed");
+}
+
+int sg_batch_set_learned(SgBatch *b, SgMlp *m) {
+  if (batch_check(b)) return -1;
+  b->mlp = m;
+  if (b->exec) {
+    hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
+  return 0;
+}
+
+// Autotuner: coordinate descent over the launch shape, each candidate timed on the
+// same sample of steps from the same checkpoint (graph launches); keeps the fastest
+// and returns the mesh state to the checkpoint.  out_ms = {before, after}.
+int sg_batch_autotune(SgBatch *b, const SgDabParams *dabs, int steps, double *out_ms) {
+  if (batch_check(b)) return -1;
+  std::vector<void *> saved = b->snap;
+  const bool had = b->snap_valid;
+  b->snap.clear();
+  if (int rc = sg_batch_snapshot_save(b)) return rc;
+  long long tot0[SBT_COUNT];
+  if (int rc = sg_batch_totals(b, tot0, 0)) return rc;
+  auto time_once = [&]() -> double {
+    if (sg_batch_load(b, dabs, steps)) return -1.0;  // captures the graph for this shape
+    double best = 1e30;
+    for (int r = 0; r < 3; r++) {
+      if (sg_batch_snapshot_restore(b) || sg_batch_rewind(b)) return -1.0;
+      float ms = 0.0f;
+      if (sg_batch_timer_start(b)) return -1.0;
+      if (sg_batch_launch(b, steps, 2)) return -1.0;
+      if (sg_batch_timer_stop(b, &ms)) return -1.0;
+      if (r > 0 && ms < best) best = ms;  // the first launch warms up
+    }
+    return best;
+  };
+  double best = time_once();
+  if (best < 0) return -1;
+  const double start = best;
+  struct Knob {
+    int *v;
+    std::vector<int> cand;
+  };
+  std::vector<Knob> knobs = {{&b->g_brush, {256, 512, 1024, 2048, 4096}},
+                             {&b->brush_threads, {64, 128, 256}},
+                             {&b->brush_vpl, {1, 2}},
+                             {&b->g_smooth, {256, 512, 1024, 2048, 4096}},
+                             {&b->g_post, {256, 512, 1024, 2048, 4096}},
+                             {&b->g_search, {256, 512, 1024, 2048}}};
+  if (b->mlp) knobs.push_back({&b->g_learned, {256, 512, 1024, 2048}});
+  for (int pass = 0; pass < 2; pass++) {
+    for (Knob &k : knobs) {
+      const int keep = *k.v;
+      int choice = keep;
+      for (int c : k.cand) {
+        if (c == keep) continue;
+        *k.v = c;
+        const double t = time_once();
+        if (t > 0 && t < best * 0.98) {
+          best = t;
+          choice = c;
+        }
+      }
+      *k.v = choice;
+    }
+  }
+  int rc = sg_batch_snapshot_restore(b);
+  // put the caller's checkpoint back and the totals as they were
+  for (void *q : b->snap)
+    if (q) hipFree(q);
+  b->snap = saved;
+  b->snap_valid = had;
+  if (!rc) rc = sg_check_hip(hipMemcpy(b->tot, tot0, sizeof(tot0), hipMemcpyHostToDevice), "copy");
+  if (out_ms) {
+    out_ms[0] = start;
+    out_ms[1] = best;
+  }
+  return rc;
+}
+
+}  // extern "C"

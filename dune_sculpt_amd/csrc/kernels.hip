@@ -55,40 +55,6 @@ int sg_check_hip(hipError_t e, const char *what) {
 // it into the reference's left->right order when asked).  The first wave also logs
 // the previous dab's counters and clears the next dab's counter set.
 // ---------------------------------------------------------------------------
-#define SEARCH_UNROLL 8
-
-__device__ __forceinline__ void leaves_box(const float *bbs, const float *obbs, int lb, int le, BB6 &nb,
-                                           BB6 &ob) {
-  // ordered BB_expand_with_bb over leaf boxes [lb, le) (current and original): lane l
-  // folds a contiguous chunk of `per` leaves (all loads issued together), then the
-  // wave combines lanes in order
-  const int lane = lane_id();
-  const int per = (le - lb + SG_WAVE - 1) / SG_WAVE;
-  bb_reset(nb);
-  bb_reset(ob);
-  for (int q0 = 0; q0 < per; q0 += SEARCH_UNROLL) {
-    BB6 t[SEARCH_UNROLL], u[SEARCH_UNROLL];
-#pragma unroll
-    for (int j = 0; j < SEARCH_UNROLL; j++) {
-      const int l = lb + lane * per + q0 + j;
-      if (q0 + j < per && l < le) {
-        t[j] = bb_load(bbs + 8 * (size_t)l);
-        u[j] = bb_load(obbs + 8 * (size_t)l);
-      } else {
-        bb_reset(t[j]);
-        bb_reset(u[j]);
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < SEARCH_UNROLL; j++) {
-      bb_add_bb(nb, t[j]);
-      bb_add_bb(ob, u[j]);
-    }
-  }
-  nb = bb_wave_reduce(nb);
-  ob = bb_wave_reduce(ob);
-}
-
 __global__ __launch_bounds__(256) void k_search(SgDevView d, SgDabK k, long long *log, int log_index,
                                                 int serial) {
   if (d.halt && d.halt[0]) return;
@@ -736,15 +702,16 @@ __global__ __launch_bounds__(256) void k_gather_aos(const float *x, const float 
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
-SgDevView sg_view(PBVH *p) {
-  SgDev &d = p->d;
+SgDevView sg_view(PBVH *p) { return sg_view_of(p->d, p->respect_hide, p->halt_live); }
+
+SgDevView sg_view_of(SgDev &d, bool respect_hide, bool halt_live) {
   SgDevView v;
   v.V = d.V;
   v.T = d.T;
   v.L = d.L;
   v.N = d.N;
   v.C = d.C;
-  v.respect_hide = p->respect_hide ? 1 : 0;
+  v.respect_hide = respect_hide ? 1 : 0;
   v.px = d.px;
   v.py = d.py;
   v.pz = d.pz;
@@ -791,7 +758,7 @@ SgDevView sg_view(PBVH *p) {
   v.ctr = d.ctr;
   v.ctr_next = d.ctr + SG_CTR_COUNT;
   v.cut_touch = d.cut_touch;
-  v.halt = p->halt_live ? d.halt : nullptr;  // no flag test in strokes without learned dabs
+  v.halt = halt_live ? d.halt : nullptr;  // no flag test in strokes without learned dabs
   v.vperm_old = d.vperm_old;
   v.stamp_v = d.stamp_v;
   v.stamp_v2 = d.stamp_v2;
@@ -967,6 +934,19 @@ int sg_enqueue_full_normals(PBVH *p, int mode) {
   const int blocks = (p->d.V + 255) / 256;
   hipLaunchKernelGGL(k_full_normals, dim3(blocks > 0 ? blocks : 1), dim3(256), 0, p->stream, v, mode);
   return sg_check_hip(hipGetLastError(), "normals launch");
+}
+
+int sg_enqueue_full_normals_view(const SgDevView &v, hipStream_t s, int mode) {
+  const int blocks = (v.V + 255) / 256;
+  hipLaunchKernelGGL(k_full_normals, dim3(blocks > 0 ? blocks : 1), dim3(256), 0, s, v, mode);
+  return sg_check_hip(hipGetLastError(), "normals launch");
+}
+
+int sg_enqueue_gather_raw(hipStream_t s, const float *x, const float *y, const float *z, const int *perm_old, int V,
+                          float *out_dev) {
+  const int blocks = (V + 255) / 256;
+  hipLaunchKernelGGL(k_gather_aos, dim3(blocks > 0 ? blocks : 1), dim3(256), 0, s, x, y, z, perm_old, V, out_dev);
+  return sg_check_hip(hipGetLastError(), "gather launch");
 }
 
 int sg_enqueue_refit_list(PBVH *p, const int *leaves_dev, int n) {

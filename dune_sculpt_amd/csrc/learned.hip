@@ -17,6 +17,7 @@
 #include <random>
 #include <vector>
 
+#include "sg_batch.h"
 #include "sg_device.h"
 #include "sg_internal.h"
 #include "sg_kernels.h"
@@ -312,6 +313,42 @@ __global__ __launch_bounds__(64 * LRN_WAVES) void k_mlp_predict(SgMlpView m, con
 // shadow copy and raise the halt flag so the rest of the stroke stops and the host
 // runs this dab with the plain brush.
 // ---------------------------------------------------------------------------
+// Evaluate one 64-vertex chunk of a learned dab (every lane of the wave calls it; the
+// MLP runs on all 64 rows): factor as the plain Smooth brush, features -> MLP ->
+// predicted position written to the shadow copy; accumulates the error estimate
+// sum((|d| / r)^2) in 2^-32 fixed point and counts vertices outside the trained range.
+__device__ inline float learned_eval(const SgDevView &d, const SgDabK &k, const float *curve, const SgMlpView &m,
+                                     const MlpLds &s, uint16_t *bufA, uint16_t *bufB, float *sOut, int v,
+                                     const BrushIn &in, float nx, float ny, float nz, long long &esum, int &nood) {
+  float fade = 0.0f, feat[28], R[9], h = 0.0f;
+  bool ok = true;
+  if (in.aff) {
+    fade = k.bstr * (sg_falloff(k, curve, sqrtf(in.d2)) * (1.0f - in.mk));
+    ok = learned_features(d, in.vv, in.x, in.y, in.z, nx, ny, nz, fade, feat, R, h);
+  } else {
+    for (int q = 0; q < 28; q++) feat[q] = 0.0f;
+  }
+  float xn[28], y[3];
+  for (int q = 0; q < 28; q++) xn[q] = in.aff ? (feat[q] - m.norm[q]) * m.norm[28 + q] : 0.0f;
+  mlp_wave(s, bufA, bufB, sOut, xn, y);
+  if (in.aff) {
+    const float o0 = y[0] * m.norm[56] * h, o1 = y[1] * m.norm[57] * h, o2 = y[2] * m.norm[58] * h;
+    const float dx = R[0] * o0 + R[3] * o1 + R[6] * o2;
+    const float dy = R[1] * o0 + R[4] * o1 + R[7] * o2;
+    const float dz = R[2] * o0 + R[5] * o1 + R[8] * o2;
+    d.qx[v] = in.x + dx;
+    d.qy[v] = in.y + dy;
+    d.qz[v] = in.z + dz;
+    const float rel2 = (dx * dx + dy * dy + dz * dz) / k.r2;
+    if (!ok || !(rel2 < 1.0f)) {  // out of range, NaN or a displacement beyond r
+      nood++;
+    } else {
+      esum += __double2ll_rn((double)rel2 * 4294967296.0);  // 2^32: rel2 < 1, no overflow below 2^31 vertices
+    }
+  }
+  return fade;
+}
+
 __global__ __launch_bounds__(64 * LRN_WAVES) void k_brush_learned(SgDevView d, SgDabK k, const float *curve,
                                                                    SgMlpView m) {
   __shared__ __align__(16) MlpLds s;
@@ -345,32 +382,7 @@ __global__ __launch_bounds__(64 * LRN_WAVES) void k_brush_learned(SgDevView d, S
       if (mk == 0ull) continue;
       int wbase = 0;
       if (lane == 0) wbase = (int)atomicAdd((unsigned long long *)&d.ctr[SG_CTR_NAFF], (unsigned long long)__popcll(mk));
-      float fade = 0.0f, feat[28], R[9], h = 0.0f;
-      bool ok = true;
-      if (in.aff) {
-        fade = k.bstr * (sg_falloff(k, curve, sqrtf(in.d2)) * (1.0f - in.mk));
-        ok = learned_features(d, in.vv, in.x, in.y, in.z, nx, ny, nz, fade, feat, R, h);
-      } else {
-        for (int q = 0; q < 28; q++) feat[q] = 0.0f;
-      }
-      float xn[28], y[3];
-      for (int q = 0; q < 28; q++) xn[q] = in.aff ? (feat[q] - m.norm[q]) * m.norm[28 + q] : 0.0f;
-      mlp_wave(s, bufA[w], bufB[w], sOut[w], xn, y);
-      if (in.aff) {
-        const float o0 = y[0] * m.norm[56] * h, o1 = y[1] * m.norm[57] * h, o2 = y[2] * m.norm[58] * h;
-        const float dx = R[0] * o0 + R[3] * o1 + R[6] * o2;
-        const float dy = R[1] * o0 + R[4] * o1 + R[7] * o2;
-        const float dz = R[2] * o0 + R[5] * o1 + R[8] * o2;
-        d.qx[v] = in.x + dx;
-        d.qy[v] = in.y + dy;
-        d.qz[v] = in.z + dz;
-        const float rel2 = (dx * dx + dy * dy + dz * dz) / k.r2;
-        if (!ok || !(rel2 < 1.0f)) {  // out of range, NaN or a displacement beyond r
-          nood++;
-        } else {
-          esum += __double2ll_rn((double)rel2 * 4294967296.0);  // 2^32: rel2 < 1, no overflow below 2^31 vertices
-        }
-      }
+      learned_eval(d, k, curve, m, s, bufA[w], bufB[w], sOut[w], v, in, nx, ny, nz, esum, nood);
       wbase = __shfl(wbase, 0);
       if (in.aff) {
         const int pos = wbase + mask_prefix(mk);
@@ -385,6 +397,110 @@ __global__ __launch_bounds__(64 * LRN_WAVES) void k_brush_learned(SgDevView d, S
   if (lane == 0 && (esum || nood)) {
     atomicAdd((unsigned long long *)&d.ctr[SG_CTR_LERR], (unsigned long long)esum);
     atomicAdd((unsigned long long *)&d.ctr[SG_CTR_LOOD], (unsigned long long)nood);
+  }
+}
+
+// Batched learned brush: items (gathered leaf of a learned-Smooth mesh, half) of every
+// mesh; per-mesh error sums feed the per-mesh decision.
+__global__ __launch_bounds__(64 * LRN_WAVES) void k_bbrush_learned(SgBatchView b, SgMlpView m) {
+  __shared__ __align__(16) MlpLds s;
+  __shared__ __align__(16) uint16_t bufA[LRN_WAVES][64 * MLP_ST];
+  __shared__ __align__(16) uint16_t bufB[LRN_WAVES][64 * MLP_ST];
+  __shared__ __align__(16) float sOut[LRN_WAVES][64 * 4];
+  const SgDevView &d = b.d;
+  const int nitems = 2 * (int)b.ctr[SB_NGATHER];
+  if (blockIdx.x * LRN_WAVES >= nitems) return;
+  mlp_lds_load<64 * LRN_WAVES>(s, m);
+  __syncthreads();
+  const int w = threadIdx.x / 64, lane = __lane_id();
+  const int gw = blockIdx.x * LRN_WAVES + w;
+  const int nw = gridDim.x * LRN_WAVES;
+  const int row = b.step[1];
+  long long degt = 0, ncand = 0;
+  for (int it = gw; it < nitems; it += nw) {
+    const int g = it >> 1;
+    const int mesh = __builtin_amdgcn_readfirstlane(b.gl_mesh[g]);
+    const SgDabK k = b.tab[(size_t)row * b.M + mesh];
+    if (!k.learned) continue;
+    const int2 vr = d.gl_v[g];
+    if ((it & 1) == 0) ncand += vr.y - vr.x;
+    long long esum = 0;
+    int nood = 0, naff = 0;
+    for (int base = vr.x + (it & 1) * SG_WAVE; base < vr.y; base += 2 * SG_WAVE) {
+      const int v = base + lane;
+      const BrushIn in = brush_load(d, k, v, vr.y, true);
+      float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+      if (v < vr.y) {
+        nx = d.nx[v];
+        ny = d.ny[v];
+        nz = d.nz[v];
+      }
+      const unsigned long long mk = __ballot(in.aff);
+      if (mk == 0ull) continue;
+      int wbase = 0;
+      if (lane == 0) wbase = (int)atomicAdd((unsigned long long *)&b.ctr[SB_NAFF], (unsigned long long)__popcll(mk));
+      const float fade = learned_eval(d, k, nullptr, m, s, bufA[w], bufB[w], sOut[w], v, in, nx, ny, nz, esum, nood);
+      wbase = __shfl(wbase, 0);
+      naff += __popcll(mk);
+      if (in.aff) {
+        const int pos = wbase + mask_prefix(mk);
+        d.aff_v[pos] = v;
+        d.aff_vt[pos] = in.vt;
+        d.aff_vv[pos] = in.vv;
+        d.aff_fade[pos] = fade;
+        b.aff_mesh[pos] = mesh;
+        degt += in.vt.y - in.vt.x;
+      }
+    }
+    esum = wave_sum_ll(esum);
+    nood = wave_sum_i(nood);
+    if (lane == 0 && naff) {
+      long long *mc = b.ctr + SB_COUNT + SB_PM * mesh;
+      atomicAdd((unsigned long long *)&mc[SBM_LERR], (unsigned long long)esum);
+      atomicAdd((unsigned long long *)&mc[SBM_LOOD], (unsigned long long)nood);
+      atomicAdd((unsigned long long *)&mc[SBM_LNAFF], (unsigned long long)naff);
+    }
+  }
+  degt = wave_sum_ll(degt);
+  if (lane == 0) {
+    if (degt) atomicAdd((unsigned long long *)&b.ctr[SB_DEGT], (unsigned long long)degt);
+    if (ncand) atomicAdd((unsigned long long *)&b.ctr[SB_CAND], (unsigned long long)ncand);
+  }
+}
+
+// Batched decision, per record of a learned-Smooth mesh: accepted -> the prediction is
+// committed to the positions; rejected -> the shadow copy gets the plain brush's first
+// Jacobi step from the untouched positions (steps 2..k follow in the plain kernels).
+__global__ __launch_bounds__(256) void k_bdecide(SgBatchView b) {
+  const SgDevView &d = b.d;
+  const int row = b.step[1];
+  const int na = (int)b.ctr[SB_NAFF];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
+    const int mesh = b.aff_mesh[i];
+    if (!b.tab[(size_t)row * b.M + mesh].learned) continue;
+    const int v = d.aff_v[i];
+    if (!sb_learned_rejected(b, mesh)) {
+      d.px[v] = d.qx[v];
+      d.py[v] = d.qy[v];
+      d.pz[v] = d.qz[v];
+      continue;
+    }
+    const int2 vv = d.aff_vv[i];
+    const float fade = d.aff_fade[i];
+    const float x = d.px[v], y = d.py[v], z = d.pz[v];
+    float sx, sy, sz;
+    nbr_sum(d.vv_idx, vv.x, vv.y, d.px, d.py, d.pz, sx, sy, sz);
+    float rx = x, ry = y, rz = z;
+    if (vv.y > vv.x) {
+      const float inv = 1.0f / (float)(vv.y - vv.x);
+      const float avx = sx * inv, avy = sy * inv, avz = sz * inv;
+      rx = x + (avx - x) * fade;
+      ry = y + (avy - y) * fade;
+      rz = z + (avz - z) * fade;
+    }
+    d.qx[v] = rx;
+    d.qy[v] = ry;
+    d.qz[v] = rz;
   }
 }
 
@@ -845,6 +961,17 @@ int sg_enqueue_judge(PBVH *p, int na, const float *lrn, const float *bx, const f
   hipLaunchKernelGGL(k_judge, dim3(blocks), dim3(256), 0, p->stream, p->d.aff_v, na, p->d.px, p->d.py, p->d.pz, lrn,
                      p->d.V, bx, by, bz, acc);
   return sg_check_hip(hipGetLastError(), "judge launch");
+}
+
+int sg_batch_enqueue_learned_brush(SgBatch *b, const SgBatchView &v) {
+  hipLaunchKernelGGL(k_bbrush_learned, dim3(b->g_learned), dim3(64 * LRN_WAVES), 0, b->stream, v,
+                     sg_mlp_view(b->mlp));
+  return sg_check_hip(hipGetLastError(), "batch learned brush launch");
+}
+
+int sg_batch_enqueue_decide(SgBatch *b, const SgBatchView &v) {
+  hipLaunchKernelGGL(k_bdecide, dim3(b->g_smooth), dim3(256), 0, b->stream, v);
+  return sg_check_hip(hipGetLastError(), "batch decide launch");
 }
 
 int sg_enqueue_collect(PBVH *p, const SgDevView &v, float *X, float *frame, int cap) {

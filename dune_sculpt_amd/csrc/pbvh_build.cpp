@@ -330,34 +330,26 @@ int alloc_fill(T **dst, size_t n, int byte, hipStream_t s) {
 
 }  // namespace
 
-void sg_build_device(PBVH *p, const float *co) {
-  SgDev &d = p->d;
-  hipStream_t s = p->stream;
+void sg_build_layout(PBVH *p, const float *co, SgLayout &h) {
   const int V = p->totvert, T = p->totprim;
   const int L = (int)p->leaf_node.size();
   const int N = (int)p->nodes.size();
-  d.V = V;
-  d.T = T;
-  d.L = L;
-  d.N = N;
+  h.V = V;
+  h.T = T;
+  h.L = L;
+  h.N = N;
 
   // positions SoA in the new numbering
-  std::vector<float> x(V), y(V), z(V);
+  std::vector<float> &x = h.x, &y = h.y, &z = h.z;
+  x.resize(V);
+  y.resize(V);
+  z.resize(V);
   parallel_for(V, [&](int i) {
     const int o = p->perm_old_of_new[i];
     x[i] = co[3 * (size_t)o];
     y[i] = co[3 * (size_t)o + 1];
     z[i] = co[3 * (size_t)o + 2];
   });
-  upload(&d.px, x.data(), V, s);
-  upload(&d.py, y.data(), V, s);
-  upload(&d.pz, z.data(), V, s);
-  upload(&d.qx, x.data(), V, s);
-  upload(&d.qy, y.data(), V, s);
-  upload(&d.qz, z.data(), V, s);
-  alloc_fill(&d.nx, V, 0, s);
-  alloc_fill(&d.ny, V, 0, s);
-  alloc_fill(&d.nz, V, 0, s);
 
   // triangles in prim_indices order
   std::vector<int> tri_new_of_old(T);
@@ -371,7 +363,8 @@ void sg_build_device(PBVH *p, const float *co) {
   });
 
   // vertex -> tri CSR (ascending original tri id), entries tri | corner << 30
-  std::vector<int> vt_off(V + 1, 0), vt_idx(3 * (size_t)T);
+  std::vector<int> &vt_off = h.vt_off, vt_idx(3 * (size_t)T);
+  vt_off.assign(V + 1, 0);
   for (size_t k = 0; k < 3 * (size_t)T; k++) vt_off[p->perm_new_of_old[p->tris[k]] + 1]++;
   for (int v = 0; v < V; v++) vt_off[v + 1] += vt_off[v];
   {
@@ -414,9 +407,10 @@ void sg_build_device(PBVH *p, const float *co) {
     }
     for (auto &h : th) h.join();
   }
-  std::vector<int> vv_off(V + 1, 0);
+  std::vector<int> &vv_off = h.vv_off, &vv_idx = h.vv_idx;
+  vv_off.assign(V + 1, 0);
   for (int v = 0; v < V; v++) vv_off[v + 1] = vv_off[v] + vv_cnt[v];
-  std::vector<int> vv_idx((size_t)vv_off[V] + 1);
+  vv_idx.assign((size_t)vv_off[V] + 1, 0);
   {
     const int nt = 16;
     std::vector<std::thread> th;
@@ -434,23 +428,25 @@ void sg_build_device(PBVH *p, const float *co) {
   }
   {
     // flattened incidences: the corners of each incident tri, in CSR order
-    std::vector<int4> vt4(3 * (size_t)T);
+    std::vector<int4> &vt4 = h.vt4;
+    vt4.resize(3 * (size_t)T);
     parallel_for((int)vt4.size(), [&](int q) {
       const int e = vt_idx[q];
       const int4 c = tri[e & 0x3fffffff];
       vt4[q] = make_int4(c.x, c.y, c.z, e);
     });
-    upload(&d.vt4, vt4.data(), vt4.size(), s);
-    hipStreamSynchronize(s);
   }
-  upload(&d.vt_off, vt_off.data(), V + 1, s);
-  upload(&d.vv_off, vv_off.data(), V + 1, s);
-  upload(&d.vv_idx, vv_idx.data(), (size_t)vv_off[V] + 1, s);
 
   // leaves
-  std::vector<int> leaf_vbeg(L + 1, 0), leaf_sh_off(L + 1, 0), leaf_flags(L, 0);
-  std::vector<int> leaf_sh;
-  std::vector<float> leaf_bb((size_t)L * 8, 0.0f), leaf_obb((size_t)L * 8, 0.0f);
+  std::vector<int> &leaf_vbeg = h.leaf_vbeg, &leaf_sh_off = h.leaf_sh_off, &leaf_flags = h.leaf_flags;
+  std::vector<int> &leaf_sh = h.leaf_sh;
+  std::vector<float> &leaf_bb = h.leaf_bb, &leaf_obb = h.leaf_obb;
+  leaf_vbeg.assign(L + 1, 0);
+  leaf_sh_off.assign(L + 1, 0);
+  leaf_flags.assign(L, 0);
+  leaf_sh.clear();
+  leaf_bb.assign((size_t)L * 8, 0.0f);
+  leaf_obb.assign((size_t)L * 8, 0.0f);
   for (int l = 0; l < L; l++) {
     const PBVHNode &n = p->nodes[p->leaf_node[l]];
     leaf_vbeg[l + 1] = leaf_vbeg[l] + n.uniq_verts;
@@ -464,29 +460,19 @@ void sg_build_device(PBVH *p, const float *co) {
       leaf_obb[8 * (size_t)l + 4 + k] = n.orig_vb[3 + k];
     }
   }
-  upload(&d.leaf_vbeg, leaf_vbeg.data(), L + 1, s);
-  upload(&d.leaf_sh_off, leaf_sh_off.data(), L + 1, s);
-  upload(&d.leaf_sh, leaf_sh.data(), leaf_sh.size(), s);
-  upload(&d.leaf_flags, leaf_flags.data(), L, s);
-  upload(&d.leaf_bb, leaf_bb.data(), (size_t)L * 8, s);
-  upload(&d.leaf_obb, leaf_obb.data(), (size_t)L * 8, s);
+}
 
-  // nodes
-  std::vector<float> node_bb((size_t)N * 8, 0.0f), node_obb((size_t)N * 8, 0.0f);
-  for (int i = 0; i < N; i++) {
-    const PBVHNode &n = p->nodes[i];
-    for (int k = 0; k < 3; k++) {
-      node_bb[8 * (size_t)i + k] = n.vb[k];
-      node_bb[8 * (size_t)i + 4 + k] = n.vb[3 + k];
-      node_obb[8 * (size_t)i + k] = n.orig_vb[k];
-      node_obb[8 * (size_t)i + 4 + k] = n.orig_vb[3 + k];
-    }
-  }
-  upload(&d.node_bb, node_bb.data(), (size_t)N * 8, s);
-  upload(&d.node_obb, node_obb.data(), (size_t)N * 8, s);
-
-  // leaf ranges and heights per node (children have larger indices than parents)
-  std::vector<int> lbeg(N, 0), lend(N, 0), height(N, 0);
+// Leaf ranges and heights per node (children have larger indices than parents) and
+// the search cut: maximal subtrees with <= K leaves, K the smallest power of two >= 64
+// that keeps the cut at <= 1024 subtrees (search cost C + passing*K; LDS tables).
+// Fills p->cut and leaf_cut[L].
+void sg_build_cut(PBVH *p, std::vector<int> &lbeg, std::vector<int> &lend, std::vector<int> &height,
+                  std::vector<int> &leaf_cut) {
+  const int N = (int)p->nodes.size();
+  const int L = (int)p->leaf_node.size();
+  lbeg.assign(N, 0);
+  lend.assign(N, 0);
+  height.assign(N, 0);
   for (int i = N - 1; i >= 0; i--) {
     const PBVHNode &n = p->nodes[i];
     if (n.flag & PBVH_Leaf) {
@@ -500,8 +486,6 @@ void sg_build_device(PBVH *p, const float *co) {
       height[i] = 1 + std::max(height[c], height[c + 1]);
     }
   }
-  // cut: maximal subtrees with <= K leaves, K the smallest power of two >= 64 that
-  // keeps the cut at <= 1024 subtrees (search cost C + passing*K; LDS tables)
   int K = 64;
   std::vector<int> cut_nodes;
   for (;;) {
@@ -523,16 +507,68 @@ void sg_build_device(PBVH *p, const float *co) {
     K *= 2;
   }
   const int C = (int)cut_nodes.size();
-  d.C = C;
   p->cut.node = cut_nodes;
   p->cut.lbeg.resize(C);
   p->cut.lend.resize(C);
-  std::vector<int> leaf_cut(L, 0);
+  leaf_cut.assign(L, 0);
   for (int c = 0; c < C; c++) {
     p->cut.lbeg[c] = lbeg[cut_nodes[c]];
     p->cut.lend[c] = lend[cut_nodes[c]];
     for (int l = p->cut.lbeg[c]; l < p->cut.lend[c]; l++) leaf_cut[l] = c;
   }
+}
+
+void sg_build_device(PBVH *p, const float *co) {
+  SgDev &d = p->d;
+  hipStream_t s = p->stream;
+  SgLayout h;
+  sg_build_layout(p, co, h);
+  const int V = h.V, T = h.T, L = h.L, N = h.N;
+  (void)T;
+  d.V = V;
+  d.T = h.T;
+  d.L = L;
+  d.N = N;
+  upload(&d.px, h.x.data(), V, s);
+  upload(&d.py, h.y.data(), V, s);
+  upload(&d.pz, h.z.data(), V, s);
+  upload(&d.qx, h.x.data(), V, s);
+  upload(&d.qy, h.y.data(), V, s);
+  upload(&d.qz, h.z.data(), V, s);
+  alloc_fill(&d.nx, V, 0, s);
+  alloc_fill(&d.ny, V, 0, s);
+  alloc_fill(&d.nz, V, 0, s);
+  upload(&d.vt4, h.vt4.data(), h.vt4.size(), s);
+  upload(&d.vt_off, h.vt_off.data(), V + 1, s);
+  upload(&d.vv_off, h.vv_off.data(), V + 1, s);
+  upload(&d.vv_idx, h.vv_idx.data(), h.vv_idx.size(), s);
+  upload(&d.leaf_vbeg, h.leaf_vbeg.data(), L + 1, s);
+  upload(&d.leaf_sh_off, h.leaf_sh_off.data(), L + 1, s);
+  upload(&d.leaf_sh, h.leaf_sh.data(), h.leaf_sh.size(), s);
+  upload(&d.leaf_flags, h.leaf_flags.data(), L, s);
+  upload(&d.leaf_bb, h.leaf_bb.data(), (size_t)L * 8, s);
+  upload(&d.leaf_obb, h.leaf_obb.data(), (size_t)L * 8, s);
+  hipStreamSynchronize(s);  // the layout's host arrays are freed on return
+
+  // nodes
+  std::vector<float> node_bb((size_t)N * 8, 0.0f), node_obb((size_t)N * 8, 0.0f);
+  for (int i = 0; i < N; i++) {
+    const PBVHNode &n = p->nodes[i];
+    for (int k = 0; k < 3; k++) {
+      node_bb[8 * (size_t)i + k] = n.vb[k];
+      node_bb[8 * (size_t)i + 4 + k] = n.vb[3 + k];
+      node_obb[8 * (size_t)i + k] = n.orig_vb[k];
+      node_obb[8 * (size_t)i + 4 + k] = n.orig_vb[3 + k];
+    }
+  }
+  upload(&d.node_bb, node_bb.data(), (size_t)N * 8, s);
+  upload(&d.node_obb, node_obb.data(), (size_t)N * 8, s);
+
+  std::vector<int> lbeg, lend, height, leaf_cut;
+  sg_build_cut(p, lbeg, lend, height, leaf_cut);
+  const std::vector<int> &cut_nodes = p->cut.node;
+  const int C = (int)cut_nodes.size();
+  d.C = C;
   upload(&d.cut_node, cut_nodes.data(), C, s);
   {
     std::vector<int> cflags(C, 0);

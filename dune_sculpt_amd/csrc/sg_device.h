@@ -125,6 +125,42 @@ __device__ __forceinline__ bool sphere_box(const BB6 &b, float cx, float cy, flo
   return (t[0] * t[0] + t[1] * t[1] + t[2] * t[2]) < r2;
 }
 
+// Subtree box of a search cut from its leaf boxes (used by the search's refresh).
+#define SEARCH_UNROLL 8
+
+__device__ __forceinline__ void leaves_box(const float *bbs, const float *obbs, int lb, int le, BB6 &nb,
+                                           BB6 &ob) {
+  // ordered BB_expand_with_bb over leaf boxes [lb, le) (current and original): lane l
+  // folds a contiguous chunk of `per` leaves (all loads issued together), then the
+  // wave combines lanes in order
+  const int lane = lane_id();
+  const int per = (le - lb + SG_WAVE - 1) / SG_WAVE;
+  bb_reset(nb);
+  bb_reset(ob);
+  for (int q0 = 0; q0 < per; q0 += SEARCH_UNROLL) {
+    BB6 t[SEARCH_UNROLL], u[SEARCH_UNROLL];
+#pragma unroll
+    for (int j = 0; j < SEARCH_UNROLL; j++) {
+      const int l = lb + lane * per + q0 + j;
+      if (q0 + j < per && l < le) {
+        t[j] = bb_load(bbs + 8 * (size_t)l);
+        u[j] = bb_load(obbs + 8 * (size_t)l);
+      } else {
+        bb_reset(t[j]);
+        bb_reset(u[j]);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < SEARCH_UNROLL; j++) {
+      bb_add_bb(nb, t[j]);
+      bb_add_bb(ob, u[j]);
+    }
+  }
+  nb = bb_wave_reduce(nb);
+  ob = bb_wave_reduce(ob);
+}
+
+
 // Spec S3 falloff: hardness remap, then the preset curve, clamped to [0,1].
 __device__ __forceinline__ float sg_falloff(const SgDabK &k, const float *curve, float dist) {
   const float r = k.r, h = k.hardness;

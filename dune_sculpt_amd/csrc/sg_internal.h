@@ -156,8 +156,21 @@ struct PBVH {
   bool halt_live = false;            // the work being enqueued contains learned dabs (kernels test the halt flag)
 };
 
+// Host copies of the leaf-ordered device arrays of one mesh (pbvh_build.cpp).
+struct SgLayout {
+  int V = 0, T = 0, L = 0, N = 0;
+  std::vector<float> x, y, z;                     // positions, new numbering
+  std::vector<int4> vt4;                          // [3T]
+  std::vector<int> vt_off, vv_off, vv_idx;        // [V+1], [V+1], neighbours
+  std::vector<int> leaf_vbeg, leaf_sh_off, leaf_sh, leaf_flags;
+  std::vector<float> leaf_bb, leaf_obb;           // [L*8]
+};
+
 // host build (pbvh_build.cpp)
 void sg_build_tree(PBVH *p, const float *co, const int *tris, int T, int V);
+void sg_build_layout(PBVH *p, const float *co, SgLayout &h);
+void sg_build_cut(PBVH *p, std::vector<int> &lbeg, std::vector<int> &lend, std::vector<int> &height,
+                  std::vector<int> &leaf_cut);
 void sg_build_device(PBVH *p, const float *co);
 void sg_free_device(PBVH *p);
 

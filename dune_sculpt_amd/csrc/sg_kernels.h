@@ -60,6 +60,10 @@ enum { SG_K_SEARCH = 0, SG_K_AREA = 1, SG_K_BRUSH = 2, SG_K_SMOOTH = 3, SG_K_POS
        SG_K_FLUSH_TOP = 6, SG_K_COUNT = 7 };
 
 SgDevView sg_view(PBVH *p);
+SgDevView sg_view_of(SgDev &d, bool respect_hide, bool halt_live);
+int sg_enqueue_full_normals_view(const SgDevView &v, hipStream_t s, int mode);
+int sg_enqueue_gather_raw(hipStream_t s, const float *x, const float *y, const float *z, const int *perm_old, int V,
+                          float *out_dev);
 int sg_profile_collect(PBVH *p);
 SgDabK sg_make_dab(const SgDabParams *dab, int stats);
 int sg_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long long *log, int log_index,

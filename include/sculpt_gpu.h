@@ -299,6 +299,46 @@ int sg_learned_judge(PBVH *pbvh, const SgDabParams *dab, double out[4]);
  * Returns the number of rows. */
 int sg_learned_collect(PBVH *pbvh, const SgDabParams *dab, float *X, float *T, float *ok, int cap);
 
+/* ---- many meshes, one dab per mesh per step (no counterpart in the reference) ----
+ * The meshes are concatenated into one device arena and every kernel of a step covers
+ * all of them; one captured hipGraph is launched per step.  Each mesh ends
+ * bit-identical to the same dabs applied to it alone with sg_dab_apply_stroke.
+ * Tools: Draw, Smooth (plain or learned), Inflate, Grab; preset falloff curves only. */
+typedef struct SgBatch SgBatch;
+SgBatch *sg_batch_new(int n_meshes, const float *const *co, const int *const *tris, const int *totvert,
+                      const int *tottri, int leaf_limit);
+void sg_batch_free(SgBatch *batch);
+/* out[0..4] = meshes, vertices, tris, leaves, search cuts; then the first vertex of
+ * each mesh in the arena (n_meshes + 1 entries) while out_n allows. */
+int sg_batch_info(SgBatch *batch, long long *out, int out_n);
+/* dabs[step * n_meshes + mesh], sculpt_tool 0 = no dab for that mesh in that step.
+ * load uploads the table and captures the step graph; launch runs the next `steps`
+ * rows (flags bit 0: enqueue kernels instead of the graph, bit 1: do not wait);
+ * rewind restarts at row 0; run = load + launch of all rows. */
+int sg_batch_load(SgBatch *batch, const SgDabParams *dabs, int steps);
+int sg_batch_launch(SgBatch *batch, int steps, int flags);
+int sg_batch_rewind(SgBatch *batch);
+int sg_batch_run(SgBatch *batch, const SgDabParams *dabs, int steps, int flags);
+int sg_batch_sync(SgBatch *batch);
+/* Totals since the last reset: gathered leaves, affected vertices, box tests,
+ * candidate vertices, vertex->tri degrees of affected vertices, Smooth neighbour
+ * reads, shared vertices refit, area-normal vertices, learned dabs accepted,
+ * rejected (redone by the plain brush in the same step), steps. */
+int sg_batch_totals(SgBatch *batch, long long out[16], int reset);
+int sg_batch_read_positions(SgBatch *batch, int mesh, float *out);
+int sg_batch_read_normals(SgBatch *batch, int mesh, float *out);
+int sg_batch_snapshot_save(SgBatch *batch);
+int sg_batch_snapshot_restore(SgBatch *batch);
+/* launch shape {search, brush, smooth, post, learned-brush workgroups, brush threads
+ * per block (64/128/256), brush vertices per lane (1/2)}; <= 0 keeps a value */
+int sg_batch_set_launch(SgBatch *batch, const int g[7]);
+int sg_batch_get_launch(SgBatch *batch, int g[7]);
+/* Autotuner over the launch shape on a sample of steps (restored afterwards). */
+int sg_batch_autotune(SgBatch *batch, const SgDabParams *dabs, int steps, double out_ms[2]);
+int sg_batch_set_learned(SgBatch *batch, SgMlp *mlp);
+int sg_batch_timer_start(SgBatch *batch);
+int sg_batch_timer_stop(SgBatch *batch, float *ms);
+
 /* ---- synthetic meshes and strokes (deterministic; for tests and benchmarks) ---- */
 int sg_mesh_icosphere_size(int level, int *r_totvert, int *r_tottri);
 int sg_mesh_icosphere(int level, float radius, float *co, int *tris);
