@@ -148,6 +148,170 @@ def run_learned(sg, A, W, args):
     return out
 
 
+def _path_centres(sg, spec, n, radii, rng):
+    """Dab centres along a seeded path on the mesh's noisy surface, arc step 0.1 r: a
+    great circle on an icosphere, a ring at a seeded tube angle on a torus (major /
+    minor radius 1.0 / 0.3).  Each point of the smooth surface is displaced along its
+    normal by the mesh's own seeded noise field, as the mesh vertices were."""
+    kind, seed, amplitude = spec
+    if kind == "sphere":
+        axis = rng.normal(size=3)
+        axis /= np.linalg.norm(axis)
+        e1 = np.cross(axis, [1.0, 0.0, 0.0] if abs(axis[0]) < 0.9 else [0.0, 1.0, 0.0])
+        e1 /= np.linalg.norm(e1)
+        e2 = np.cross(axis, e1)
+        t = np.cumsum(0.1 * radii) - 0.1 * radii[0] + rng.uniform(0, 2 * np.pi)
+        pts = np.cos(t)[:, None] * e1 + np.sin(t)[:, None] * e2
+        nrm = pts
+    else:
+        v0 = rng.uniform(0, 2 * np.pi)
+        ring = 1.0 + 0.3 * np.cos(v0)
+        u = np.cumsum(0.1 * radii / ring) + rng.uniform(0, 2 * np.pi)
+        pts = np.stack([ring * np.cos(u), ring * np.sin(u), np.full(n, 0.3 * np.sin(v0))], -1)
+        nrm = np.stack([np.cos(v0) * np.cos(u), np.cos(v0) * np.sin(u), np.full(n, np.sin(v0))], -1)
+    return sg.noise_displace(pts.astype(np.float32), nrm.astype(np.float32), seed, amplitude)
+
+
+def batch_meshes(sg, n, seed, vmin, vmax, amplitude, log_uniform=True, levels=(4, 8)):
+    """n seeded meshes with vertex counts drawn from [vmin, vmax] (log-uniform or
+    uniform), alternating between the nearest icosphere level (within `levels`) and a
+    torus with n_u / n_v = 10 / 3 (near-equilateral triangles)."""
+    rng = np.random.default_rng(seed)
+    meshes, specs = [], []
+    for i in range(n):
+        V = float(np.exp(rng.uniform(np.log(vmin), np.log(vmax)))) if log_uniform else float(rng.uniform(vmin, vmax))
+        if i % 2 == 0:
+            level = int(np.clip(np.round(np.log(V / 10.0) / np.log(4.0)), levels[0], levels[1]))
+            meshes.append(sg.noisy_icosphere(level, seed=1000 + i, amplitude=amplitude))
+            specs.append(("sphere", 1000 + i, amplitude))
+        else:
+            nv = max(3, int(round(np.sqrt(V * 3.0 / 10.0))))
+            nu = max(3, int(round(V / nv)))
+            meshes.append(sg.noisy_torus(nu, nv, seed=1000 + i, amplitude=amplitude))
+            specs.append(("torus", 1000 + i, amplitude))
+    return meshes, specs
+
+
+def batch_table(sg, meshes, specs, steps, seed, tools, rmin, rmax, learned_smooth=False):
+    """table[step][mesh]: mesh m's dab `step` cycles `tools` (offset by m), r ~ U[rmin,
+    rmax], s ~ U[0.2, 1.0], Smooth with k = 4, along the mesh's path."""
+    rng = np.random.default_rng(seed)
+    table = [[None] * len(meshes) for _ in range(steps)]
+    for m, spec in enumerate(specs):
+        radii = rng.uniform(rmin, rmax, steps)
+        strengths = rng.uniform(0.2, 1.0, steps)
+        centres = _path_centres(sg, spec, steps, radii, rng)
+        for s in range(steps):
+            tool = tools[(s + m) % len(tools)]
+            flag = sg.SMOOTH_LEARNED if (learned_smooth and tool == sg.TOOL_SMOOTH) else 0
+            table[s][m] = sg.make_dab(tool=tool, location=centres[s], radius=float(radii[s]),
+                                      strength=float(strengths[s]), smooth_iterations=4, flag=flag)
+    return table
+
+
+def batch_bytes(t):
+    """Lower bound on the HBM bytes of the batch's dab work from its exact totals (SoA
+    FP32 12 B per vec3, int32 topology, 16 B per flattened incidence): box tests 24 B,
+    candidate positions 12 B, affected writes 12 B, Draw area normals 12 B, Smooth k
+    passes over the affected records (12 B read + 12 B write + 4 B per neighbour index;
+    neighbour positions counted only as the record itself, a lower bound), learned
+    records (one neighbour pass + write), normals (16 B per incidence + 12 B read + 12 B
+    write per affected vertex), refit (12 B per shared vertex + 24 B per leaf)."""
+    b = 24 * t["tests"] + 12 * t["candidates"] + 12 * t["affected"] + 12 * t["area_verts"]
+    b += 24 * t["smooth_records_k"] + 4 * t["smooth_nbr_reads"]
+    b += 24 * t["learned_records"]
+    b += 16 * t["deg_t"] + 24 * t["affected"]
+    b += 12 * t["shared"] + 24 * t["gathered"]
+    return b
+
+
+def run_batch(sg, name, meshes, table, W, K, tune_steps, learned_model=None, plain_table=None):
+    """Time the [steps][mesh] table on one batch from a pristine snapshot: W warmup + K
+    timed runs, HIP events around each run (one graph launch per step)."""
+    t = time.perf_counter()
+    B = sg.Batch(meshes, leaf_limit=128)
+    build_s = time.perf_counter() - t
+    info = B.info()
+    log(f"{name}: batch of {info['meshes']} meshes, {info['verts']} V, {info['leaves']} leaves, "
+        f"{info['cuts']} cuts, built in {build_s:.1f}s")
+    B.snapshot()
+    out = {"meshes": info["meshes"], "verts": info["verts"], "tris": info["tris"], "leaves": info["leaves"],
+           "cuts": info["cuts"], "steps": len(table), "build_s": round(build_s, 2)}
+    if learned_model is not None:
+        B.set_learned(learned_model)
+    if tune_steps:
+        t = time.perf_counter()
+        before, after = B.autotune(table[:tune_steps])
+        out["autotune"] = {"sample_steps": tune_steps, "before_ms": round(before, 3), "after_ms": round(after, 3),
+                           "launch": B.launch_shape(), "seconds": round(time.perf_counter() - t, 2)}
+        log(f"{name}: autotune {out['autotune']}")
+
+    def timed(tab):
+        B.load(tab)
+        times = []
+        tot = None
+        for s in range(W + K):
+            B.restore()
+            B.rewind()
+            B.totals(reset=True)
+            B.timer_start()
+            B.launch(sync=False)
+            ms = B.timer_stop()
+            if s >= W:
+                times.append(ms)
+                tot = B.totals()
+        ms = float(np.mean(times))
+        n_dabs = sum(1 for row in tab for d in row if d is not None)
+        r = {"ms_per_run": ms, "us_per_step": ms * 1e3 / len(tab), "dabs_per_s": n_dabs / (ms * 1e-3),
+             "vertex_updates_per_s": tot["affected"] / (ms * 1e-3), "affected_per_run": tot["affected"],
+             "alg_bytes_per_run": batch_bytes(tot), "totals": tot}
+        r["hbm_gbps"] = r["alg_bytes_per_run"] / (ms * 1e-3) / 1e9
+        return r
+
+    out["run"] = timed(table)
+    if plain_table is not None:
+        out["plain"] = timed(plain_table)
+        out["learned_speedup_dabs_per_s"] = out["run"]["dabs_per_s"] / out["plain"]["dabs_per_s"]
+    B.close()
+    log(f"{name}: {json.dumps(out)[:600]}")
+    return out
+
+
+def run_batch_workloads(sg, args, W):
+    """Many meshes: 256 meshes of 5k-500k vertices (log-uniform; icospheres and tori,
+    noise 0.01), 64 steps of Draw / Smooth(4) with C2's dab distribution; and 64
+    meshes of 1M-4M vertices, 128 steps cycling Draw / Inflate / learned Smooth(4),
+    r ~ U[0.02, 0.08], timed against the same table with plain Smooth(4)."""
+    res = {}
+    t = time.perf_counter()
+    meshes, kinds = batch_meshes(sg, args.batch_meshes, seed=11, vmin=5000, vmax=500000, amplitude=0.01)
+    table = batch_table(sg, meshes, kinds, args.batch_steps, seed=12, tools=(sg.TOOL_DRAW, sg.TOOL_SMOOTH),
+                        rmin=0.02, rmax=0.1)
+    log(f"c4 meshes and strokes in {time.perf_counter() - t:.1f}s")
+    res["c4"] = run_batch(sg, "c4", meshes, table, W, 3, tune_steps=8)
+    res["c4"]["workload"] = (f"{len(meshes)} meshes, V log-uniform [5k, 500k] (icospheres / tori 1.0/0.3, noise "
+                             f"0.01), {args.batch_steps} Draw/Smooth(4) dabs each, r~U[0.02,0.1], s~U[0.2,1.0]")
+    del meshes, table
+    if not args.no_c5:
+        from dune_sculpt_amd import learned as L
+
+        mlp = sg.MLP.load(L.MODEL_PATH) if L.MODEL_PATH.exists() else None
+        t = time.perf_counter()
+        meshes, kinds = batch_meshes(sg, args.c5_meshes, seed=21, vmin=1e6, vmax=4e6, amplitude=0.02,
+                                     log_uniform=False, levels=(9, 9))
+        tools = (sg.TOOL_DRAW, sg.TOOL_INFLATE, sg.TOOL_SMOOTH)
+        tl = batch_table(sg, meshes, kinds, args.c5_steps, seed=22, tools=tools, rmin=0.02, rmax=0.08,
+                         learned_smooth=mlp is not None)
+        tp = batch_table(sg, meshes, kinds, args.c5_steps, seed=22, tools=tools, rmin=0.02, rmax=0.08)
+        log(f"c5 meshes and strokes in {time.perf_counter() - t:.1f}s")
+        res["c5"] = run_batch(sg, "c5", meshes, tl, W, 2, tune_steps=8, learned_model=mlp, plain_table=tp)
+        res["c5"]["workload"] = (f"{len(meshes)} meshes of 1M-4M V (icosphere-9 / tori, noise 0.02), "
+                                 f"{args.c5_steps} dabs each cycling Draw/Inflate/learned Smooth(4), r~U[0.02,0.08]")
+        if mlp is not None:
+            mlp.close()
+    return res
+
+
 def cpu_baseline(sg, co, tris, dabs, budget_s, threads):
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle as po
@@ -188,6 +352,13 @@ def main():
     ap.add_argument("--no-judge", action="store_true")
     ap.add_argument("--only-learned", action="store_true",
                     help="run only the learned Smooth section (profiling); prints its JSON")
+    ap.add_argument("--no-batch", action="store_true", help="skip the many-mesh workloads")
+    ap.add_argument("--only-batch", action="store_true", help="run only the many-mesh workloads")
+    ap.add_argument("--batch-meshes", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=64)
+    ap.add_argument("--no-c5", action="store_true", help="skip the 64 x 1-4M-vertex learned batch")
+    ap.add_argument("--c5-meshes", type=int, default=64)
+    ap.add_argument("--c5-steps", type=int, default=128)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -196,6 +367,10 @@ def main():
         sys.exit(2)
 
     import dune_sculpt_amd as sg
+
+    if args.only_batch:
+        print(json.dumps({"batch": run_batch_workloads(sg, args, args.warmup)}))
+        return
 
     t = time.perf_counter()
     co, tris = sg.noisy_icosphere(args.level, seed=1, amplitude=0.02)
@@ -287,6 +462,9 @@ def main():
     c3 = None
     if not args.no_learned:
         c3 = run_learned(sg, A, W, args)
+    batch = None
+    if not args.no_batch:
+        batch = run_batch_workloads(sg, args, W)
 
     cpu = None
     if not args.no_cpu:
@@ -327,6 +505,7 @@ def main():
         "event_s": event_s,
         "alg_bytes_per_dab_by_kernel": kernel_bytes_per_dab,
         "learned_smooth": c3,
+        "batch": batch,
 
         "build_s": build_s,
         "autotune": tune,

@@ -720,7 +720,7 @@ class PBVH:
 # ---------------------------------------------------------------------------
 
 BATCH_TOTALS = ["gathered", "affected", "tests", "candidates", "deg_t", "smooth_nbr_reads", "shared", "area_verts",
-                "learned_accepted", "learned_rejected", "steps"]
+                "smooth_records_k", "learned_records", "", "", "learned_accepted", "learned_rejected", "steps"]
 BATCH_LAUNCH = ["search", "brush", "smooth", "post", "learned", "brush_threads", "brush_vpl"]
 
 
@@ -795,7 +795,7 @@ class Batch:
     def totals(self, reset=False):
         out = (C.c_longlong * 16)()
         _check(lib().sg_batch_totals(self.h, out, 1 if reset else 0), "totals")
-        return {k: int(out[i]) for i, k in enumerate(BATCH_TOTALS)}
+        return {k: int(out[i]) for i, k in enumerate(BATCH_TOTALS) if k}
 
     def positions(self, m):
         out = np.empty((self.nverts[m], 3), np.float32)

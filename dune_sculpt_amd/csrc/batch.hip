@@ -317,22 +317,31 @@ __global__ __launch_bounds__(256) void k_bpost(SgBatchView b) {
   const int nthreads = gridDim.x * blockDim.x;
   const int na = (int)b.ctr[SB_NAFF];
   const int ng = (int)b.ctr[SB_NGATHER];
-  long long ntri = 0, nfa = 0, nvfa = 0, nsh = 0;
+  long long ntri = 0, nfa = 0, nvfa = 0, nsh = 0, ak = 0, alrn = 0;
   for (int i = tid; i < na; i += nthreads) {
     const int mesh = b.aff_mesh[i];
     const SgDabK *kp = &b.tab[(size_t)row * b.M + mesh];
     const int v = d.aff_v[i];
     const int2 vt = d.aff_vt[i];
-    if ((kp->iters & 1) == 0 && sb_plain_smooth(b, kp, mesh)) {
+    const bool plain = sb_plain_smooth(b, kp, mesh);
+    if ((kp->iters & 1) == 0 && plain) {
       d.qx[v] = d.px[v];
       d.qy[v] = d.py[v];
       d.qz[v] = d.pz[v];
     }
+    if (plain) ak += kp->iters;
+    if (kp->learned) alrn++;
     vertex_normal(d, v, vt.x, vt.y, 0, 0, ntri, nfa, nvfa);
   }
   const int gw = tid / SG_WAVE, nw = nthreads / SG_WAVE;
   for (int g = gw; g < ng; g += nw) leaf_refit(d, d.gathered[g], d.gl_v[g], d.gl_s[g], nsh);
-  if (lane_id() == 0 && nsh) atomicAdd((unsigned long long *)&b.ctr[SB_SHARED], as_u64(nsh));
+  ak = wave_sum_ll(ak);
+  alrn = wave_sum_ll(alrn);
+  if (lane_id() == 0) {
+    if (nsh) atomicAdd((unsigned long long *)&b.ctr[SB_SHARED], as_u64(nsh));
+    if (ak) atomicAdd((unsigned long long *)&b.ctr[SB_AK], as_u64(ak));
+    if (alrn) atomicAdd((unsigned long long *)&b.ctr[SB_ALRN], as_u64(alrn));
+  }
 }
 
 // End of step: learned decisions counted, counters added to the totals and cleared,
@@ -834,10 +843,10 @@ int sg_batch_sync(SgBatch *b) {
   return sg_check_hip(hipStreamSynchronize(b->stream), "batch sync");
 }
 
-// totals since the last reset: [0..7] gathered leaves, affected vertices, box tests,
+// totals since the last reset: [0..9] gathered leaves, affected vertices, box tests,
 // candidate vertices, vertex->tri degrees of affected vertices, Smooth neighbour reads,
-// shared vertices refit, area-normal vertices; [8] learned dabs accepted, [9] rejected
-// (redone plain), [10] steps.
+// shared vertices refit, area-normal vertices, plain Smooth records times k, learned
+// Smooth records; [12] learned dabs accepted, [13] rejected (redone plain), [14] steps.
 int sg_batch_totals(SgBatch *b, long long *out16, int reset) {
   if (batch_check(b)) return -1;
   int rc = sg_check_hip(hipMemcpyAsync(out16, b->tot, SBT_COUNT * 8, hipMemcpyDeviceToHost, b->stream), "copy");

@@ -24,7 +24,9 @@ enum {
   SB_DEG = 5,       // sum of vertex->vertex degrees of Smooth records, times k
   SB_SHARED = 6,    // shared vertices read by the leaf refit
   SB_NAREA = 7,     // vertices inside the Draw area-normal sphere
-  SB_COUNT = 8,
+  SB_AK = 8,        // plain Smooth records times their iteration count
+  SB_ALRN = 9,      // learned Smooth records
+  SB_COUNT = 12,
 };
 // per-mesh counters, SB_PM entries per mesh after the global ones
 enum {
@@ -36,7 +38,7 @@ enum {
 };
 // running totals
 // running totals: entries 0..SB_COUNT-1 accumulate the step counters of the same index
-enum { SBT_LACC = 8, SBT_LREJ = 9, SBT_STEPS = 10, SBT_COUNT = 16 };
+enum { SBT_LACC = 12, SBT_LREJ = 13, SBT_STEPS = 14, SBT_COUNT = 16 };
 
 // kernel-side view of a batch
 struct SgBatchView {

@@ -320,10 +320,11 @@ int sg_batch_launch(SgBatch *batch, int steps, int flags);
 int sg_batch_rewind(SgBatch *batch);
 int sg_batch_run(SgBatch *batch, const SgDabParams *dabs, int steps, int flags);
 int sg_batch_sync(SgBatch *batch);
-/* Totals since the last reset: gathered leaves, affected vertices, box tests,
- * candidate vertices, vertex->tri degrees of affected vertices, Smooth neighbour
- * reads, shared vertices refit, area-normal vertices, learned dabs accepted,
- * rejected (redone by the plain brush in the same step), steps. */
+/* Totals since the last reset: [0] gathered leaves, [1] affected vertices, [2] box
+ * tests, [3] candidate vertices, [4] vertex->tri degrees of affected vertices, [5]
+ * Smooth neighbour reads, [6] shared vertices refit, [7] area-normal vertices, [8]
+ * plain Smooth records times k, [9] learned Smooth records, [12] learned dabs
+ * accepted, [13] rejected (redone by the plain brush in the same step), [14] steps. */
 int sg_batch_totals(SgBatch *batch, long long out[16], int reset);
 int sg_batch_read_positions(SgBatch *batch, int mesh, float *out);
 int sg_batch_read_normals(SgBatch *batch, int mesh, float *out);
