@@ -353,6 +353,7 @@ def main():
     ap.add_argument("--only-learned", action="store_true",
                     help="run only the learned Smooth section (profiling); prints its JSON")
     ap.add_argument("--no-batch", action="store_true", help="skip the many-mesh workloads")
+    ap.add_argument("--copy-mb", type=int, default=1024, help="size of the measured HBM copy (0: skip)")
     ap.add_argument("--only-batch", action="store_true", help="run only the many-mesh workloads")
     ap.add_argument("--batch-meshes", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=64)
@@ -428,20 +429,23 @@ def main():
 
     # exact byte accounting: the same stroke from the same mesh with the kernels'
     # statistics counters on (untimed); every timed step repeats it exactly
-    B = sg.PBVH(co, tris, leaf_limit=128)
-    B.set_stats(not args.no_stats)
-    B.stroke(dabs, graph=False, log=True)
-    rows = B.stroke_log(args.dabs)
-    B.close()
-    per_stroke_aff = int(rows[:, 1].sum())
-    assert int(logrows[:, 1].sum()) == per_stroke_aff, "stats replay diverged from the timed run"
     kern_bytes = {}
     stroke_bytes = 0
-    for i in range(args.dabs):
-        bb = dab_bytes(sg, int(tools[i]), 4, rows[i])
-        stroke_bytes += bb["total"]
-        for key, val in bb.items():
-            kern_bytes[key] = kern_bytes.get(key, 0) + val
+    if args.no_stats:  # profiling runs: no replay (its launches would join the profile)
+        per_stroke_aff = int(logrows[:, 1].sum())
+    else:
+        B = sg.PBVH(co, tris, leaf_limit=128)
+        B.set_stats(True)
+        B.stroke(dabs, graph=False, log=True)
+        rows = B.stroke_log(args.dabs)
+        B.close()
+        per_stroke_aff = int(rows[:, 1].sum())
+        assert int(logrows[:, 1].sum()) == per_stroke_aff, "stats replay diverged from the timed run"
+        for i in range(args.dabs):
+            bb = dab_bytes(sg, int(tools[i]), 4, rows[i])
+            stroke_bytes += bb["total"]
+            for key, val in bb.items():
+                kern_bytes[key] = kern_bytes.get(key, 0) + val
     timed_aff = per_stroke_aff * K
     timed_bytes = stroke_bytes * K
     vups = timed_aff / elapsed
@@ -458,6 +462,12 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBPS, 6), "traffic": TRAFFIC_BYTES_PER_DAB,
                 "alg_bytes_per_launch": round(bytes_per_dab, 1), "avg_launch_us": round(us_per_dab_ev, 3)}
     kernel_bytes_per_dab = {k: round(v / args.dabs, 1) for k, v in kern_bytes.items()}
+
+    copy = None
+    if args.copy_mb:
+        ms, gbps_copy = sg.hbm_copy_probe(args.copy_mb << 20)
+        copy = {"bytes": args.copy_mb << 20, "ms": ms, "gbps_read_plus_write": gbps_copy}
+        log(f"HBM copy probe: {copy}")
 
     c3 = None
     if not args.no_learned:
@@ -504,6 +514,7 @@ def main():
         "cpu_baseline": cpu,
         "event_s": event_s,
         "alg_bytes_per_dab_by_kernel": kernel_bytes_per_dab,
+        "hbm_copy": copy,
         "learned_smooth": c3,
         "batch": batch,
 

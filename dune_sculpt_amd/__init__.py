@@ -101,7 +101,7 @@ EXPORTS = [
     "sg_batch_rewind", "sg_batch_run", "sg_batch_sync", "sg_batch_totals", "sg_batch_read_positions",
     "sg_batch_read_normals", "sg_batch_snapshot_save", "sg_batch_snapshot_restore", "sg_batch_set_launch",
     "sg_batch_get_launch", "sg_batch_autotune", "sg_batch_set_learned", "sg_batch_timer_start",
-    "sg_batch_timer_stop",
+    "sg_batch_timer_stop", "sg_hbm_copy_probe",
 ]
 SMOOTH_LEARNED = 1 << 30
 NCTR = 24
@@ -192,6 +192,7 @@ def lib():
         "sg_pbvh_learned_stats": (i32, [vp, C.POINTER(C.c_longlong)]),
         "sg_learned_collect": (i32, [vp, C.POINTER(SgDabParams), f32p, f32p, f32p, i32]),
         "sg_learned_judge": (i32, [vp, C.POINTER(SgDabParams), C.POINTER(C.c_double)]),
+        "sg_hbm_copy_probe": (i32, [C.c_longlong, i32, f32p]),
         "sg_batch_new": (vp, [i32, C.POINTER(f32p), C.POINTER(i32p), i32p, i32p, i32]),
         "sg_batch_free": (None, [vp]),
         "sg_batch_info": (i32, [vp, C.POINTER(C.c_longlong), i32]),
@@ -441,6 +442,14 @@ class MLP:
         m.set_params(z["w"])
         m.set_decision(float(z["alpha"]), float(z["tol"]))
         return m
+
+
+def hbm_copy_probe(nbytes, reps=5):
+    """Best time (ms) of a device-to-device copy of nbytes; returns (ms, GB/s counting
+    read + write)."""
+    ms = C.c_float()
+    _check(lib().sg_hbm_copy_probe(int(nbytes), reps, C.byref(ms)), "copy probe")
+    return ms.value, 2 * nbytes / (ms.value * 1e-3) / 1e9
 
 
 def mfma_probe(A, B):

@@ -836,6 +836,39 @@ int sg_pbvh_profile_read(PBVH *p, double *ms, long long *count, int n, int reset
   return SG_K_COUNT;
 }
 
+// Device-to-device copy of `bytes` between two fresh buffers, best of `reps` copies
+// timed with HIP events: the GPU's measured copy bandwidth, and a kernel of known
+// traffic (bytes read, bytes written) for calibrating the hardware byte counters.
+int sg_hbm_copy_probe(long long bytes, int reps, float *ms_out) {
+  if (bytes <= 0 || reps <= 0 || !ms_out) return sg_set_error("bad arguments");
+  void *a = nullptr, *b = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = sg_check_hip(hipMalloc(&a, (size_t)bytes), "hipMalloc");
+  if (!rc) rc = sg_check_hip(hipMalloc(&b, (size_t)bytes), "hipMalloc");
+  if (!rc) rc = sg_check_hip(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "stream");
+  if (!rc) rc = sg_check_hip(hipEventCreate(&e0), "event");
+  if (!rc) rc = sg_check_hip(hipEventCreate(&e1), "event");
+  if (!rc) rc = sg_check_hip(hipMemsetAsync(a, 1, (size_t)bytes, s), "memset");
+  float best = 1e30f;
+  for (int r = 0; r < reps && !rc; r++) {
+    hipEventRecord(e0, s);
+    rc = sg_check_hip(hipMemcpyAsync(b, a, (size_t)bytes, hipMemcpyDeviceToDevice, s), "copy");
+    hipEventRecord(e1, s);
+    if (!rc) rc = sg_check_hip(hipEventSynchronize(e1), "copy sync");
+    float ms = 0.0f;
+    if (!rc) hipEventElapsedTime(&ms, e0, e1);
+    if (ms < best) best = ms;
+  }
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (s) hipStreamDestroy(s);
+  if (a) hipFree(a);
+  if (b) hipFree(b);
+  *ms_out = best;
+  return rc;
+}
+
 int sg_pbvh_timer_start(PBVH *p) {
   if (!p || p->no_device) return sg_set_error("no device");
   for (int i = 0; i < 2; i++)

@@ -252,6 +252,9 @@ int sg_pbvh_profile_read(PBVH *pbvh, double *ms, long long *count, int n, int re
 /* Wall time of everything enqueued on the PBVH's stream between start and stop (a HIP
  * event pair on that stream; stop waits for the second event). */
 int sg_pbvh_timer_start(PBVH *pbvh);
+/* Device-to-device copy of `bytes` (best of `reps`, HIP events): the measured copy
+ * bandwidth, and a kernel of known traffic for calibrating hardware byte counters. */
+int sg_hbm_copy_probe(long long bytes, int reps, float *ms);
 int sg_pbvh_timer_stop(PBVH *pbvh, float *ms);
 
 /* Copies of the device state in original vertex order (AoS float[3]). */
