@@ -32,9 +32,11 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBPS = 8000.0
-# HBM bytes per dab from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (summed over the
-# kernels of the stroke, scaled by the copy-kernel calibration); None until measured.
-TRAFFIC_BYTES_PER_DAB = None
+# HBM bytes per dab of the 1000-dab stroke on the 10.5M-vertex mesh, from two
+# rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) summed over the dab kernels, FETCH_SIZE
+# scaled by 2.00 and WRITE_SIZE by 1.00 (what they read for a 1 GiB device copy);
+# profiles/dab_step_c2.md holds the commands and the per-kernel figures.
+TRAFFIC_BYTES_PER_DAB = 5_857_807
 METRIC = "vertex-updates/s + HBM GB/s; learned vs plain Smooth dabs/s at ≤1e-3 RMS"
 
 
