@@ -32,6 +32,16 @@
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int sb_uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
+// Search: one lane per cut for the cut-level work (64 cuts of consecutive meshes per
+// wave), then the wave takes the passing cuts one at a time for the leaf tests.
+//   level 1  each lane: its cut's mesh, range, flags, touch stamp and both boxes
+//   level 2  each lane: its mesh's dab centre / radius for this step
+//   rebuild  cuts whose leaves moved in the previous step get their boxes rebuilt from
+//            the leaf boxes, one cut at a time by the whole wave (also for meshes
+//            without a dab this step)
+//   leaves   per passing cut: up to 128 leaf tests per load level, one atomic per
+//            level reserves the gathered slots
+#define BSEARCH_UNROLL 2
 __global__ __launch_bounds__(256) void k_bsearch(SgBatchView b) {
   const SgDevView &d = b.d;
   const int lane = lane_id();
@@ -39,90 +49,118 @@ __global__ __launch_bounds__(256) void k_bsearch(SgBatchView b) {
   const int nw = gridDim.x * blockDim.x / SG_WAVE;
   const int serial = b.step[0], row = b.step[1];
   long long ntests = 0;
-  for (int c = gw; c < d.C; c += nw) {
-    const int mesh = sb_uniform(b.cut_mesh[c]);
-    const SgDabK k = b.tab[(size_t)row * b.M + mesh];
-    const int touch = d.cut_touch[c];
-    const int lb = d.cut_lbeg[c], le = d.cut_lend[c];
-    if (k.tool == 0) {  // no dab for this mesh: still rebuild a box the previous step left stale
-      if (touch == serial - 1) {
-        BB6 nb, ob;
-        leaves_box(d.leaf_bb, d.leaf_obb, lb, le, nb, ob);
-        if (lane == 0) {
-          bb_store(d.cut_bb + 8 * (size_t)c, nb);
-          bb_store(d.cut_obb + 8 * (size_t)c, ob);
-        }
-      }
-      continue;
+  for (int c0 = gw * SG_WAVE; c0 < d.C; c0 += nw * SG_WAVE) {
+    const int c = c0 + lane;
+    const bool valid = c < d.C;
+    int mesh = 0, touch = 0, lb = 0, le = 0, cfl = 0;
+    BB6 nbox, obox;
+    bb_reset(nbox);
+    bb_reset(obox);
+    if (valid) {
+      mesh = b.cut_mesh[c];
+      touch = d.cut_touch[c];
+      lb = d.cut_lbeg[c];
+      le = d.cut_lend[c];
+      cfl = d.cut_flags[c];
+      nbox = bb_load(d.cut_bb + 8 * (size_t)c);
+      obox = bb_load(d.cut_obb + 8 * (size_t)c);
     }
-    BB6 cb = bb_load((k.use_orig ? d.cut_obb : d.cut_bb) + 8 * (size_t)c);
-    const int cfl = d.cut_flags[c];
-    if (touch == serial - 1) {  // leaves of this cut moved in the previous step
+    const SgDabK *kp = &b.tab[(size_t)row * b.M + mesh];
+    int tool = 0, use_orig = 0;
+    float cx = 0.0f, cy = 0.0f, cz = 0.0f, r2 = 0.0f;
+    if (valid) {
+      tool = kp->tool;
+      use_orig = kp->use_orig;
+      cx = kp->cx;
+      cy = kp->cy;
+      cz = kp->cz;
+      r2 = kp->r2;
+    }
+    unsigned long long rm = __ballot(valid && touch == serial - 1);
+    while (rm) {
+      const int j = __ffsll((long long)rm) - 1;
+      rm &= rm - 1;
       BB6 nb, ob;
-      leaves_box(d.leaf_bb, d.leaf_obb, lb, le, nb, ob);
-      if (lane == 0) {
-        bb_store(d.cut_bb + 8 * (size_t)c, nb);
-        bb_store(d.cut_obb + 8 * (size_t)c, ob);
+      leaves_box(d.leaf_bb, d.leaf_obb, __shfl(lb, j), __shfl(le, j), nb, ob);
+      const float v[12] = {__shfl(nb.mn[0], 0), __shfl(nb.mn[1], 0), __shfl(nb.mn[2], 0),
+                           __shfl(nb.mx[0], 0), __shfl(nb.mx[1], 0), __shfl(nb.mx[2], 0),
+                           __shfl(ob.mn[0], 0), __shfl(ob.mn[1], 0), __shfl(ob.mn[2], 0),
+                           __shfl(ob.mx[0], 0), __shfl(ob.mx[1], 0), __shfl(ob.mx[2], 0)};
+      if (lane == j) {
+        for (int q = 0; q < 3; q++) {
+          nbox.mn[q] = v[q];
+          nbox.mx[q] = v[3 + q];
+          obox.mn[q] = v[6 + q];
+          obox.mx[q] = v[9 + q];
+        }
+        bb_store(d.cut_bb + 8 * (size_t)c, nbox);
+        bb_store(d.cut_obb + 8 * (size_t)c, obox);
       }
-      cb = k.use_orig ? ob : nb;
-      cb.mn[0] = __shfl(cb.mn[0], 0);
-      cb.mn[1] = __shfl(cb.mn[1], 0);
-      cb.mn[2] = __shfl(cb.mn[2], 0);
-      cb.mx[0] = __shfl(cb.mx[0], 0);
-      cb.mx[1] = __shfl(cb.mx[1], 0);
-      cb.mx[2] = __shfl(cb.mx[2], 0);
     }
-    ntests++;
-    if ((cfl & 1) || !sphere_box(cb, k.cx, k.cy, k.cz, k.r2)) continue;
-    const float *leaf_bbs = k.use_orig ? d.leaf_obb : d.leaf_bb;
-    const int nl = le - lb;
-    ntests += nl;
-    for (int q0 = 0; q0 < nl; q0 += SG_WAVE * SEARCH_UNROLL) {
-      bool pass[SEARCH_UNROLL];
+    const bool active = valid && tool != 0;
+    const bool pass = active && !(cfl & 1) && sphere_box(use_orig ? obox : nbox, cx, cy, cz, r2);
+    ntests += active ? 1 : 0;
+    unsigned long long pm = __ballot(pass);
+    while (pm) {
+      const int j = __ffsll((long long)pm) - 1;
+      pm &= pm - 1;
+      const int cj = c0 + j;
+      const int lbj = __shfl(lb, j), lej = __shfl(le, j), meshj = __shfl(mesh, j);
+      const int origj = __shfl(use_orig, j);
+      const float jx = __shfl(cx, j), jy = __shfl(cy, j), jz = __shfl(cz, j), jr2 = __shfl(r2, j);
+      const float *leaf_bbs = origj ? d.leaf_obb : d.leaf_bb;
+      const int nl = lej - lbj;
+      if (lane == 0) ntests += nl;
+      bool any = false;
+      for (int q0 = 0; q0 < nl; q0 += SG_WAVE * BSEARCH_UNROLL) {
+        bool lp[BSEARCH_UNROLL];
 #pragma unroll
-      for (int j = 0; j < SEARCH_UNROLL; j++) {
-        const int l = lb + q0 + j * SG_WAVE + lane;
-        pass[j] = false;
-        if (l < le) {
-          const BB6 bx = bb_load(leaf_bbs + 8 * (size_t)l);
-          pass[j] = !(d.leaf_flags[l] & 1) && sphere_box(bx, k.cx, k.cy, k.cz, k.r2);
+        for (int u = 0; u < BSEARCH_UNROLL; u++) {
+          const int l = lbj + q0 + u * SG_WAVE + lane;
+          lp[u] = false;
+          if (l < lej) {
+            const BB6 bx = bb_load(leaf_bbs + 8 * (size_t)l);
+            lp[u] = !(d.leaf_flags[l] & 1) && sphere_box(bx, jx, jy, jz, jr2);
+          }
+        }
+        unsigned long long m[BSEARCH_UNROLL];
+        int cnt = 0;
+#pragma unroll
+        for (int u = 0; u < BSEARCH_UNROLL; u++) {
+          m[u] = __ballot(lp[u]);
+          cnt += __popcll(m[u]);
+        }
+        if (cnt == 0) continue;
+        any = true;
+        int wbase = 0;
+        if (lane == 0) wbase = (int)atomicAdd((unsigned long long *)&b.ctr[SB_NGATHER], (unsigned long long)cnt);
+        int2 vr[BSEARCH_UNROLL], sr[BSEARCH_UNROLL];
+#pragma unroll
+        for (int u = 0; u < BSEARCH_UNROLL; u++) {
+          const int l = lbj + q0 + u * SG_WAVE + lane;
+          if (lp[u]) {
+            vr[u] = b.leaf_vr[l];
+            sr[u] = b.leaf_shr[l];
+          }
+        }
+        wbase = __shfl(wbase, 0);
+        int off = wbase;
+#pragma unroll
+        for (int u = 0; u < BSEARCH_UNROLL; u++) {
+          if (lp[u]) {
+            const int pos = off + mask_prefix(m[u]);
+            d.gathered[pos] = lbj + q0 + u * SG_WAVE + lane;
+            d.gl_v[pos] = vr[u];
+            d.gl_s[pos] = sr[u];
+            b.gl_mesh[pos] = meshj;
+          }
+          off += __popcll(m[u]);
         }
       }
-      unsigned long long m[SEARCH_UNROLL];
-      int cnt = 0;
-#pragma unroll
-      for (int j = 0; j < SEARCH_UNROLL; j++) {
-        m[j] = __ballot(pass[j]);
-        cnt += __popcll(m[j]);
-      }
-      if (cnt == 0) continue;
-      int wbase = 0;
-      if (lane == 0) wbase = (int)atomicAdd((unsigned long long *)&b.ctr[SB_NGATHER], (unsigned long long)cnt);
-      int2 vr[SEARCH_UNROLL], sr[SEARCH_UNROLL];
-#pragma unroll
-      for (int j = 0; j < SEARCH_UNROLL; j++) {
-        const int l = lb + q0 + j * SG_WAVE + lane;
-        if (pass[j]) {
-          vr[j] = b.leaf_vr[l];
-          sr[j] = b.leaf_shr[l];
-        }
-      }
-      wbase = __shfl(wbase, 0);
-      int off = wbase;
-#pragma unroll
-      for (int j = 0; j < SEARCH_UNROLL; j++) {
-        if (pass[j]) {
-          const int pos = off + mask_prefix(m[j]);
-          d.gathered[pos] = lb + q0 + j * SG_WAVE + lane;
-          d.gl_v[pos] = vr[j];
-          d.gl_s[pos] = sr[j];
-          b.gl_mesh[pos] = mesh;
-        }
-        off += __popcll(m[j]);
-      }
-      if (lane == 0) d.cut_touch[c] = serial;
+      if (any && lane == 0) d.cut_touch[cj] = serial;
     }
   }
+  ntests = wave_sum_ll(ntests);
   if (lane == 0 && ntests) atomicAdd((unsigned long long *)&b.ctr[SB_TESTS], (unsigned long long)ntests);
 }
 
@@ -501,7 +539,7 @@ int sg_batch_enqueue_step(SgBatch *b, int maxk, bool learned) {
     if (sg_batch_enqueue_decide(b, v)) return -1;
   }
   for (int it = 2; it <= maxk; it++) hipLaunchKernelGGL(k_bsmooth_iter, dim3(b->g_smooth), dim3(256), 0, s, v, it);
-  if (maxk >= 1) hipLaunchKernelGGL(k_bsmooth_sync, dim3(b->g_smooth), dim3(256), 0, s, v);
+  if (b->odd_k) hipLaunchKernelGGL(k_bsmooth_sync, dim3(b->g_smooth), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_bpost, dim3(b->g_post), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_bstep_end, dim3(1), dim3(256), 0, s, v);
   return sg_check_hip(hipGetLastError(), "batch step launch");
@@ -747,7 +785,7 @@ int sg_batch_load(SgBatch *b, const SgDabParams *dabs, int steps) {
   const size_t rows = (size_t)steps, M = (size_t)b->M;
   std::vector<SgDabK> tab(rows * M);
   int maxk = 0;
-  bool learned = false;
+  bool learned = false, odd_k = false;
   for (size_t i = 0; i < rows * M; i++) {
     const SgDabParams &p = dabs[i];
     if (p.sculpt_tool == 0) {
@@ -761,7 +799,10 @@ int sg_batch_load(SgBatch *b, const SgDabParams *dabs, int steps) {
     if (!(p.radius > 0.0f)) return sg_set_error("sg_batch_load: radius must be positive");
     tab[i] = sg_make_dab(&p, 0);
     if (!b->mlp) tab[i].learned = 0;
-    if (tab[i].tool == SG_TOOL_SMOOTH) maxk = std::max(maxk, tab[i].iters);
+    if (tab[i].tool == SG_TOOL_SMOOTH) {
+      maxk = std::max(maxk, tab[i].iters);
+      odd_k = odd_k || (tab[i].iters & 1);
+    }
     learned = learned || tab[i].learned;
   }
   // nothing in flight may still read the previous table
@@ -783,7 +824,8 @@ int sg_batch_load(SgBatch *b, const SgDabParams *dabs, int steps) {
   b->row_next = 0;
   b->maxk = maxk;
   b->learned = learned;
-  const int cfg[] = {maxk, learned ? 1 : 0, b->mlp ? 1 : 0, b->g_search, b->g_brush, b->g_smooth,
+  b->odd_k = odd_k;
+  const int cfg[] = {maxk, learned ? 1 : 0, odd_k ? 1 : 0, b->mlp ? 1 : 0, b->g_search, b->g_brush, b->g_smooth,
                      b->g_post, b->g_learned, b->brush_threads, b->brush_vpl};
   long long key = 1469598103934665603ll;
   for (int c : cfg) key = (key ^ (long long)c) * 1099511628211ll;
@@ -960,7 +1002,7 @@ int sg_batch_autotune(SgBatch *b, const SgDabParams *dabs, int steps, double *ou
   auto time_once = [&]() -> double {
     if (sg_batch_load(b, dabs, steps)) return -1.0;  // captures the graph for this shape
     double best = 1e30;
-    for (int r = 0; r < 3; r++) {
+    for (int r = 0; r < 5; r++) {
       if (sg_batch_snapshot_restore(b) || sg_batch_rewind(b)) return -1.0;
       float ms = 0.0f;
       if (sg_batch_timer_start(b)) return -1.0;
@@ -992,7 +1034,7 @@ int sg_batch_autotune(SgBatch *b, const SgDabParams *dabs, int steps, double *ou
         if (c == keep) continue;
         *k.v = c;
         const double t = time_once();
-        if (t > 0 && t < best * 0.98) {
+        if (t > 0 && t < best * 0.97) {
           best = t;
           choice = c;
         }

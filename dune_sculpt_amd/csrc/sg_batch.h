@@ -69,6 +69,7 @@ struct SgBatch {
   int rows = 0, row_next = 0;  // rows loaded, next row to run
   int maxk = 0;             // largest Smooth iteration count of the loaded table
   bool learned = false;     // the loaded table has learned Smooth dabs
+  bool odd_k = false;       // ... Smooth dabs with an odd iteration count
   std::vector<long long> vbase, lbase, cbase, tbase;  // [M+1]
   struct SgMlp *mlp = nullptr;
   // launch shape (autotuned): workgroups of search / brush / smooth+post / learned brush,
