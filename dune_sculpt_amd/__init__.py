@@ -84,7 +84,7 @@ EXPORTS = [
     "BKE_pbvh_node_get_verts", "BKE_pbvh_node_fully_hidden_get", "BKE_pbvh_node_add_proxy",
     "BKE_pbvh_node_free_proxies", "BKE_pbvh_gather_proxies", "BKE_pbvh_node_frustum_contain_AABB",
     "BKE_pbvh_node_frustum_exclude_AABB", "BKE_pbvh_get_verts", "DUNE_pbvh_get_vert_normals",
-    "DUNE_pbvh_respect_hide_set", "BKE_pbvh_type",
+    "DUNE_pbvh_respect_hide_set", "BKE_pbvh_respect_hide_set", "BKE_pbvh_type",
     "sg_dab_params_default", "sg_pbvh_leaf_limit_set", "sg_dab_apply", "sg_dab_apply_stroke",
     "sg_dab_stroke_log", "sg_sync", "sg_last_error", "sg_dab_last_affected", "sg_dab_last_stats",
     "sg_dab_last_gathered", "sg_pbvh_set_stats", "sg_pbvh_set_grid", "sg_pbvh_read_positions",

@@ -216,6 +216,8 @@ void DUNE_pbvh_respect_hide_set(PBVH *p, bool respect_hide) {
   if (p) p->respect_hide = respect_hide;
 }
 
+void BKE_pbvh_respect_hide_set(PBVH *p, bool respect_hide) { DUNE_pbvh_respect_hide_set(p, respect_hide); }
+
 PBVHType BKE_pbvh_type(const PBVH *p) { return p ? p->type : PBVH_FACES; }
 
 void BKE_pbvh_build_mesh(PBVH *p, Mesh *mesh, const MPoly *mpoly, const MLoop *mloop, MVert *verts,

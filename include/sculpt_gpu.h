@@ -151,6 +151,8 @@ MVert *BKE_pbvh_get_verts(const PBVH *pbvh);
 const float (*DUNE_pbvh_get_vert_normals(const PBVH *pbvh))[3];
 /* pbvh.c:4983 */
 void DUNE_pbvh_respect_hide_set(PBVH *pbvh, bool respect_hide);
+/* paint.c:2089 calls the setter by this name; same function. */
+void BKE_pbvh_respect_hide_set(PBVH *pbvh, bool respect_hide);
 /* pbvh.c:4953 PBVHType accessor (BKE_pbvh_type is called throughout paint.c) */
 PBVHType BKE_pbvh_type(const PBVH *pbvh);
 
