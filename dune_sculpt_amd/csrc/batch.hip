@@ -39,9 +39,47 @@ __device__ __forceinline__ int sb_uniform(int x) { return __builtin_amdgcn_readf
 //   rebuild  cuts whose leaves moved in the previous step get their boxes rebuilt from
 //            the leaf boxes, one cut at a time by the whole wave (also for meshes
 //            without a dab this step)
-//   leaves   per passing cut: up to 128 leaf tests per load level, one atomic per
-//            level reserves the gathered slots
+//   leaves   per passing cut, up to 128 leaf tests per load level; the wave counts its
+//            passing leaves over all its cuts first, reserves their slots with one
+//            atomic, then tests again (boxes now cached) and writes them
 #define BSEARCH_UNROLL 2
+
+struct BCut {
+  int c, lb, le, mesh, orig;
+  float cx, cy, cz, r2;
+};
+
+__device__ __forceinline__ BCut bcut_of(int j, int c0, int lb, int le, int mesh, int use_orig, float cx, float cy,
+                                        float cz, float r2) {
+  BCut q;
+  q.c = c0 + j;
+  q.lb = __shfl(lb, j);
+  q.le = __shfl(le, j);
+  q.mesh = __shfl(mesh, j);
+  q.orig = __shfl(use_orig, j);
+  q.cx = __shfl(cx, j);
+  q.cy = __shfl(cy, j);
+  q.cz = __shfl(cz, j);
+  q.r2 = __shfl(r2, j);
+  return q;
+}
+
+// ballots of the leaf tests of cut q, leaves [lb + q0, lb + q0 + 128)
+__device__ __forceinline__ void bleaf_tests(const SgDevView &d, const BCut &q, int q0, unsigned long long *m) {
+  const int lane = lane_id();
+  const float *leaf_bbs = q.orig ? d.leaf_obb : d.leaf_bb;
+#pragma unroll
+  for (int u = 0; u < BSEARCH_UNROLL; u++) {
+    const int l = q.lb + q0 + u * SG_WAVE + lane;
+    bool p = false;
+    if (l < q.le) {
+      const BB6 bx = bb_load(leaf_bbs + 8 * (size_t)l);
+      p = !(d.leaf_flags[l] & 1) && sphere_box(bx, q.cx, q.cy, q.cz, q.r2);
+    }
+    m[u] = __ballot(p);
+  }
+}
+
 __global__ __launch_bounds__(256) void k_bsearch(SgBatchView b) {
   const SgDevView &d = b.d;
   const int lane = lane_id();
@@ -100,68 +138,58 @@ __global__ __launch_bounds__(256) void k_bsearch(SgBatchView b) {
     const bool active = valid && tool != 0;
     const bool pass = active && !(cfl & 1) && sphere_box(use_orig ? obox : nbox, cx, cy, cz, r2);
     ntests += active ? 1 : 0;
-    unsigned long long pm = __ballot(pass);
-    while (pm) {
+    const unsigned long long pm0 = __ballot(pass);
+    if (pm0 == 0ull) continue;
+    // pass 1: count the passing leaves of every passing cut
+    int total = 0;
+    for (unsigned long long pm = pm0; pm;) {
       const int j = __ffsll((long long)pm) - 1;
       pm &= pm - 1;
-      const int cj = c0 + j;
-      const int lbj = __shfl(lb, j), lej = __shfl(le, j), meshj = __shfl(mesh, j);
-      const int origj = __shfl(use_orig, j);
-      const float jx = __shfl(cx, j), jy = __shfl(cy, j), jz = __shfl(cz, j), jr2 = __shfl(r2, j);
-      const float *leaf_bbs = origj ? d.leaf_obb : d.leaf_bb;
-      const int nl = lej - lbj;
-      if (lane == 0) ntests += nl;
-      bool any = false;
-      for (int q0 = 0; q0 < nl; q0 += SG_WAVE * BSEARCH_UNROLL) {
-        bool lp[BSEARCH_UNROLL];
-#pragma unroll
-        for (int u = 0; u < BSEARCH_UNROLL; u++) {
-          const int l = lbj + q0 + u * SG_WAVE + lane;
-          lp[u] = false;
-          if (l < lej) {
-            const BB6 bx = bb_load(leaf_bbs + 8 * (size_t)l);
-            lp[u] = !(d.leaf_flags[l] & 1) && sphere_box(bx, jx, jy, jz, jr2);
-          }
-        }
+      const BCut q = bcut_of(j, c0, lb, le, mesh, use_orig, cx, cy, cz, r2);
+      if (lane == 0) ntests += q.le - q.lb;
+      for (int q0 = 0; q0 < q.le - q.lb; q0 += SG_WAVE * BSEARCH_UNROLL) {
         unsigned long long m[BSEARCH_UNROLL];
-        int cnt = 0;
+        bleaf_tests(d, q, q0, m);
+#pragma unroll
+        for (int u = 0; u < BSEARCH_UNROLL; u++) total += __popcll(m[u]);
+      }
+    }
+    if (total == 0) continue;
+    int off = 0;
+    if (lane == 0) off = (int)atomicAdd((unsigned long long *)&b.ctr[SB_NGATHER], (unsigned long long)total);
+    off = __shfl(off, 0);
+    // pass 2: the same tests, now writing the reserved slots
+    for (unsigned long long pm = pm0; pm;) {
+      const int j = __ffsll((long long)pm) - 1;
+      pm &= pm - 1;
+      const BCut q = bcut_of(j, c0, lb, le, mesh, use_orig, cx, cy, cz, r2);
+      bool any = false;
+      for (int q0 = 0; q0 < q.le - q.lb; q0 += SG_WAVE * BSEARCH_UNROLL) {
+        unsigned long long m[BSEARCH_UNROLL];
+        bleaf_tests(d, q, q0, m);
 #pragma unroll
         for (int u = 0; u < BSEARCH_UNROLL; u++) {
-          m[u] = __ballot(lp[u]);
-          cnt += __popcll(m[u]);
-        }
-        if (cnt == 0) continue;
-        any = true;
-        int wbase = 0;
-        if (lane == 0) wbase = (int)atomicAdd((unsigned long long *)&b.ctr[SB_NGATHER], (unsigned long long)cnt);
-        int2 vr[BSEARCH_UNROLL], sr[BSEARCH_UNROLL];
-#pragma unroll
-        for (int u = 0; u < BSEARCH_UNROLL; u++) {
-          const int l = lbj + q0 + u * SG_WAVE + lane;
-          if (lp[u]) {
-            vr[u] = b.leaf_vr[l];
-            sr[u] = b.leaf_shr[l];
-          }
-        }
-        wbase = __shfl(wbase, 0);
-        int off = wbase;
-#pragma unroll
-        for (int u = 0; u < BSEARCH_UNROLL; u++) {
-          if (lp[u]) {
+          const int l = q.lb + q0 + u * SG_WAVE + lane;
+          if ((m[u] >> lane) & 1ull) {
             const int pos = off + mask_prefix(m[u]);
-            d.gathered[pos] = lbj + q0 + u * SG_WAVE + lane;
-            d.gl_v[pos] = vr[u];
-            d.gl_s[pos] = sr[u];
-            b.gl_mesh[pos] = meshj;
+            d.gathered[pos] = l;
+            d.gl_v[pos] = b.leaf_vr[l];
+            d.gl_s[pos] = b.leaf_shr[l];
+            b.gl_mesh[pos] = q.mesh;
           }
           off += __popcll(m[u]);
+          any = any || m[u] != 0ull;
         }
       }
-      if (any && lane == 0) d.cut_touch[cj] = serial;
+      if (any && lane == 0) d.cut_touch[q.c] = serial;
     }
   }
   ntests = wave_sum_ll(ntests);
   if (lane == 0 && ntests) atomicAdd((unsigned long long *)&b.ctr[SB_TESTS], (unsigned long long)ntests);
+}
+
+__device__ __forceinline__ long long *sb_mesh_ctr(const SgBatchView &b, int mesh) {
+  return b.ctr + SB_COUNT + SB_PM * (size_t)mesh;
 }
 
 // Draw meshes: area normal from the uniq vertices of gathered leaves within
@@ -173,7 +201,6 @@ __global__ __launch_bounds__(256) void k_barea(SgBatchView b) {
   const int nw = gridDim.x * blockDim.x / SG_WAVE;
   const int row = b.step[1];
   const int ng = (int)b.ctr[SB_NGATHER];
-  int ntot = 0;
   for (int g = gw; g < ng; g += nw) {
     const int mesh = sb_uniform(b.gl_mesh[g]);
     const SgDabK k = b.tab[(size_t)row * b.M + mesh];
@@ -201,27 +228,27 @@ __global__ __launch_bounds__(256) void k_barea(SgBatchView b) {
     sz = wave_sum_ll(sz);
     na = wave_sum_i(na);
     if (lane == 0 && na) {
-      long long *mc = b.ctr + SB_COUNT + SB_PM * mesh;
+      long long *mc = sb_mesh_ctr(b, mesh);
       atomicAdd((unsigned long long *)&mc[SBM_AREA_X], as_u64(sx));
       atomicAdd((unsigned long long *)&mc[SBM_AREA_Y], as_u64(sy));
       atomicAdd((unsigned long long *)&mc[SBM_AREA_Z], as_u64(sz));
+      atomicAdd((unsigned long long *)&mc[SBM_NAREA], (unsigned long long)na);
     }
-    ntot += na;
   }
-  if (lane == 0 && ntot) atomicAdd((unsigned long long *)&b.ctr[SB_NAREA], (unsigned long long)ntot);
 }
 
 // Brush: wave per gathered leaf, VPL vertices per lane (a VPL*64-vertex tile per load
-// level).  Learned Smooth meshes are left to the learned brush.
+// level).  Affected vertices get the step's stamp (and Smooth factor); the per-leaf
+// counts go to the mesh's counters.  Learned Smooth meshes are left to the learned
+// brush.
 template <int VPL>
 __global__ __launch_bounds__(256) void k_bbrush(SgBatchView b) {
   const SgDevView &d = b.d;
   const int lane = lane_id();
   const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
   const int nw = gridDim.x * blockDim.x / SG_WAVE;
-  const int row = b.step[1];
+  const int serial = b.step[0], row = b.step[1];
   const int ng = (int)b.ctr[SB_NGATHER];
-  long long ncand = 0, degt = 0, deg = 0;
   for (int g = gw; g < ng; g += nw) {
     const int mesh = sb_uniform(b.gl_mesh[g]);
     const SgDabK k = b.tab[(size_t)row * b.M + mesh];
@@ -229,7 +256,7 @@ __global__ __launch_bounds__(256) void k_bbrush(SgBatchView b) {
     const bool smooth = k.tool == SG_TOOL_SMOOTH;
     float ox = 0.0f, oy = 0.0f, oz = 0.0f;
     if (k.tool == SG_TOOL_DRAW) {  // offset (normalize(area) * r) * bstrength
-      const long long *mc = b.ctr + SB_COUNT + SB_PM * mesh;
+      const long long *mc = sb_mesh_ctr(b, mesh);
       float ax = (float)((double)mc[SBM_AREA_X] * (1.0 / 4294967296.0));
       float ay = (float)((double)mc[SBM_AREA_Y] * (1.0 / 4294967296.0));
       float az = (float)((double)mc[SBM_AREA_Z] * (1.0 / 4294967296.0));
@@ -239,66 +266,55 @@ __global__ __launch_bounds__(256) void k_bbrush(SgBatchView b) {
       oz = (az * k.r) * k.bstr;
     }
     const int2 vr = d.gl_v[g];
-    ncand += vr.y - vr.x;
+    int naff = 0;
+    long long degt = 0, nbr = 0;
     for (int base = vr.x; base < vr.y; base += VPL * SG_WAVE) {
       BrushIn in[VPL];
-      unsigned long long m[VPL];
-      int cnt = 0;
 #pragma unroll
-      for (int j = 0; j < VPL; j++) {
-        in[j] = brush_load(d, k, base + j * SG_WAVE + lane, vr.y, smooth);
-        m[j] = __ballot(in[j].aff);
-        cnt += __popcll(m[j]);
-      }
-      if (cnt == 0) continue;
-      int wbase = 0;
-      if (lane == 0) wbase = (int)atomicAdd((unsigned long long *)&b.ctr[SB_NAFF], (unsigned long long)cnt);
-      float fade[VPL];
-      long long nbr = 0;
-#pragma unroll
-      for (int j = 0; j < VPL; j++) {
-        fade[j] = 0.0f;
-        if (in[j].aff) fade[j] = brush_apply(d, k, nullptr, base + j * SG_WAVE + lane, in[j], ox, oy, oz, nbr);
-      }
-      wbase = __shfl(wbase, 0);
-      int off = wbase;
+      for (int j = 0; j < VPL; j++) in[j] = brush_load(d, k, base + j * SG_WAVE + lane, vr.y, smooth);
 #pragma unroll
       for (int j = 0; j < VPL; j++) {
         if (in[j].aff) {
-          const int pos = off + mask_prefix(m[j]);
-          d.aff_v[pos] = base + j * SG_WAVE + lane;
-          d.aff_vt[pos] = in[j].vt;
-          d.aff_vv[pos] = in[j].vv;
-          d.aff_fade[pos] = fade[j];
-          b.aff_mesh[pos] = mesh;
+          const int v = base + j * SG_WAVE + lane;
+          const float fade = brush_apply(d, k, nullptr, v, in[j], ox, oy, oz, nbr);
+          b.vstamp[v] = serial;
+          if (smooth) b.vfade[v] = fade;
+          naff++;
           degt += in[j].vt.y - in[j].vt.x;
         }
-        off += __popcll(m[j]);
       }
-      if (smooth) deg += nbr * k.iters;
     }
-  }
-  degt = wave_sum_ll(degt);
-  deg = wave_sum_ll(deg);
-  if (lane == 0) {
-    if (ncand) atomicAdd((unsigned long long *)&b.ctr[SB_CAND], as_u64(ncand));
-    if (degt) atomicAdd((unsigned long long *)&b.ctr[SB_DEGT], as_u64(degt));
-    if (deg) atomicAdd((unsigned long long *)&b.ctr[SB_DEG], as_u64(deg));
+    naff = wave_sum_i(naff);
+    degt = wave_sum_ll(degt);
+    nbr = wave_sum_ll(nbr);
+    if (lane == 0) {
+      long long *mc = sb_mesh_ctr(b, mesh);
+      atomicAdd((unsigned long long *)&mc[SBM_CAND], (unsigned long long)(vr.y - vr.x));
+      if (naff) {
+        atomicAdd((unsigned long long *)&mc[SBM_NAFF], (unsigned long long)naff);
+        atomicAdd((unsigned long long *)&mc[SBM_DEGT], as_u64(degt));
+        if (nbr) atomicAdd((unsigned long long *)&mc[SBM_DEG], as_u64(nbr));
+      }
+    }
   }
 }
 
-// Records of meshes running the plain Smooth this step: plain Smooth dabs, and
-// learned Smooth dabs that were rejected.
-__device__ __forceinline__ bool sb_plain_smooth(const SgBatchView &b, const SgDabK *kp, int mesh) {
-  return kp->tool == SG_TOOL_SMOOTH && (!kp->learned || sb_learned_rejected(b, mesh));
+// Meshes running the plain Smooth this step: plain Smooth dabs, and learned Smooth
+// dabs that were rejected.
+__device__ __forceinline__ bool sb_plain_smooth(const SgBatchView &b, const SgDabK &k, int mesh) {
+  return k.tool == SG_TOOL_SMOOTH && (!k.learned || sb_learned_rejected(b, mesh));
 }
 
 // Jacobi step it >= 2 (even steps read the shadow copy and write the positions,
-// odd steps the reverse).
+// odd steps the reverse): wave per gathered leaf, lanes over its vertices, work on
+// those stamped by this step.
 __global__ __launch_bounds__(256) void k_bsmooth_iter(SgBatchView b, int it) {
   const SgDevView &d = b.d;
-  const int row = b.step[1];
-  const int na = (int)b.ctr[SB_NAFF];
+  const int lane = lane_id();
+  const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
+  const int nw = gridDim.x * blockDim.x / SG_WAVE;
+  const int serial = b.step[0], row = b.step[1];
+  const int ng = (int)b.ctr[SB_NGATHER];
   const bool from_shadow = (it % 2) == 0;
   const float *sx_ = from_shadow ? d.qx : d.px;
   const float *sy_ = from_shadow ? d.qy : d.py;
@@ -306,115 +322,138 @@ __global__ __launch_bounds__(256) void k_bsmooth_iter(SgBatchView b, int it) {
   float *dx_ = from_shadow ? d.px : d.qx;
   float *dy_ = from_shadow ? d.py : d.qy;
   float *dz_ = from_shadow ? d.pz : d.qz;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
-    const int mesh = b.aff_mesh[i];
-    const SgDabK *kp = &b.tab[(size_t)row * b.M + mesh];
-    if (it > kp->iters || !sb_plain_smooth(b, kp, mesh)) continue;
-    const int v = d.aff_v[i];
-    const int2 vv = d.aff_vv[i];
-    const float fade = d.aff_fade[i];
-    const float x = sx_[v], y = sy_[v], z = sz_[v];
-    float sx, sy, sz;
-    nbr_sum(d.vv_idx, vv.x, vv.y, sx_, sy_, sz_, sx, sy, sz);
-    float rx = x, ry = y, rz = z;
-    if (vv.y > vv.x) {
-      const float inv = 1.0f / (float)(vv.y - vv.x);
-      const float avx = sx * inv, avy = sy * inv, avz = sz * inv;
-      rx = x + (avx - x) * fade;
-      ry = y + (avy - y) * fade;
-      rz = z + (avz - z) * fade;
+  for (int g = gw; g < ng; g += nw) {
+    const int mesh = sb_uniform(b.gl_mesh[g]);
+    const SgDabK k = b.tab[(size_t)row * b.M + mesh];
+    if (it > k.iters || !sb_plain_smooth(b, k, mesh)) continue;
+    const int2 vr = d.gl_v[g];
+    for (int v = vr.x + lane; v < vr.y; v += SG_WAVE) {
+      if (b.vstamp[v] != serial) continue;
+      const int2 vv = make_int2(d.vv_off[v], d.vv_off[v + 1]);
+      const float fade = b.vfade[v];
+      const float x = sx_[v], y = sy_[v], z = sz_[v];
+      float sx, sy, sz;
+      nbr_sum(d.vv_idx, vv.x, vv.y, sx_, sy_, sz_, sx, sy, sz);
+      float rx = x, ry = y, rz = z;
+      if (vv.y > vv.x) {
+        const float inv = 1.0f / (float)(vv.y - vv.x);
+        const float avx = sx * inv, avy = sy * inv, avz = sz * inv;
+        rx = x + (avx - x) * fade;
+        ry = y + (avy - y) * fade;
+        rz = z + (avz - z) * fade;
+      }
+      dx_[v] = rx;
+      dy_[v] = ry;
+      dz_[v] = rz;
     }
-    dx_[v] = rx;
-    dy_[v] = ry;
-    dz_[v] = rz;
   }
 }
 
 // Odd k ends in the shadow copy: copy it to the positions.
 __global__ __launch_bounds__(256) void k_bsmooth_sync(SgBatchView b) {
   const SgDevView &d = b.d;
-  const int row = b.step[1];
-  const int na = (int)b.ctr[SB_NAFF];
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
-    const int mesh = b.aff_mesh[i];
-    const SgDabK *kp = &b.tab[(size_t)row * b.M + mesh];
-    if ((kp->iters & 1) == 0 || !sb_plain_smooth(b, kp, mesh)) continue;
-    const int v = d.aff_v[i];
-    d.px[v] = d.qx[v];
-    d.py[v] = d.qy[v];
-    d.pz[v] = d.qz[v];
+  const int lane = lane_id();
+  const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
+  const int nw = gridDim.x * blockDim.x / SG_WAVE;
+  const int serial = b.step[0], row = b.step[1];
+  const int ng = (int)b.ctr[SB_NGATHER];
+  for (int g = gw; g < ng; g += nw) {
+    const int mesh = sb_uniform(b.gl_mesh[g]);
+    const SgDabK k = b.tab[(size_t)row * b.M + mesh];
+    if ((k.iters & 1) == 0 || !sb_plain_smooth(b, k, mesh)) continue;
+    const int2 vr = d.gl_v[g];
+    for (int v = vr.x + lane; v < vr.y; v += SG_WAVE) {
+      if (b.vstamp[v] != serial) continue;
+      d.px[v] = d.qx[v];
+      d.py[v] = d.qy[v];
+      d.pz[v] = d.qz[v];
+    }
   }
 }
 
-// Normals of the affected vertices (even-k Smooth first refreshes the shadow copy),
-// then bounds of the gathered leaves.
+// Post: items (gathered leaf, part): part 0 = normals of the leaf's stamped vertices
+// (even-k plain Smooth first refreshes the shadow copy), part 1 = the leaf's bounds.
 __global__ __launch_bounds__(256) void k_bpost(SgBatchView b) {
   const SgDevView &d = b.d;
-  const int row = b.step[1];
-  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nthreads = gridDim.x * blockDim.x;
-  const int na = (int)b.ctr[SB_NAFF];
+  const int lane = lane_id();
+  const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
+  const int nw = gridDim.x * blockDim.x / SG_WAVE;
+  const int serial = b.step[0], row = b.step[1];
   const int ng = (int)b.ctr[SB_NGATHER];
-  long long ntri = 0, nfa = 0, nvfa = 0, nsh = 0, ak = 0, alrn = 0;
-  for (int i = tid; i < na; i += nthreads) {
-    const int mesh = b.aff_mesh[i];
-    const SgDabK *kp = &b.tab[(size_t)row * b.M + mesh];
-    const int v = d.aff_v[i];
-    const int2 vt = d.aff_vt[i];
-    const bool plain = sb_plain_smooth(b, kp, mesh);
-    if ((kp->iters & 1) == 0 && plain) {
-      d.qx[v] = d.px[v];
-      d.qy[v] = d.py[v];
-      d.qz[v] = d.pz[v];
+  long long ntri = 0, nfa = 0, nvfa = 0;
+  for (int it = gw; it < 2 * ng; it += nw) {
+    const int g = it >> 1;
+    const int mesh = sb_uniform(b.gl_mesh[g]);
+    if (it & 1) {
+      long long nsh = 0;
+      leaf_refit(d, d.gathered[g], d.gl_v[g], d.gl_s[g], nsh);
+      if (lane == 0) atomicAdd((unsigned long long *)&sb_mesh_ctr(b, mesh)[SBM_SHARED], as_u64(nsh));
+      continue;
     }
-    if (plain) ak += kp->iters;
-    if (kp->learned) alrn++;
-    vertex_normal(d, v, vt.x, vt.y, 0, 0, ntri, nfa, nvfa);
-  }
-  const int gw = tid / SG_WAVE, nw = nthreads / SG_WAVE;
-  for (int g = gw; g < ng; g += nw) leaf_refit(d, d.gathered[g], d.gl_v[g], d.gl_s[g], nsh);
-  ak = wave_sum_ll(ak);
-  alrn = wave_sum_ll(alrn);
-  if (lane_id() == 0) {
-    if (nsh) atomicAdd((unsigned long long *)&b.ctr[SB_SHARED], as_u64(nsh));
-    if (ak) atomicAdd((unsigned long long *)&b.ctr[SB_AK], as_u64(ak));
-    if (alrn) atomicAdd((unsigned long long *)&b.ctr[SB_ALRN], as_u64(alrn));
+    const SgDabK k = b.tab[(size_t)row * b.M + mesh];
+    const bool sync_q = (k.iters & 1) == 0 && sb_plain_smooth(b, k, mesh);
+    const int2 vr = d.gl_v[g];
+    for (int v = vr.x + lane; v < vr.y; v += SG_WAVE) {
+      if (b.vstamp[v] != serial) continue;
+      if (sync_q) {
+        d.qx[v] = d.px[v];
+        d.qy[v] = d.py[v];
+        d.qz[v] = d.pz[v];
+      }
+      vertex_normal(d, v, d.vt_off[v], d.vt_off[v + 1], 0, 0, ntri, nfa, nvfa);
+    }
   }
 }
 
-// End of step: learned decisions counted, counters added to the totals and cleared,
-// serial and table row advanced.  One block.
+// End of step: per-mesh counters summed into the totals (with the learned decisions),
+// every counter cleared, serial and table row advanced.  One block.
 __global__ __launch_bounds__(256) void k_bstep_end(SgBatchView b) {
-  __shared__ int s_acc, s_rej;
+  __shared__ unsigned long long s_sum[SBT_COUNT];
   const int t = threadIdx.x;
-  if (t == 0) {
-    s_acc = 0;
-    s_rej = 0;
-  }
+  if (t < SBT_COUNT) s_sum[t] = 0ull;
   __syncthreads();
   const int row = b.step[1];
-  int acc = 0, rej = 0;
+  long long aff = 0, cand = 0, degt = 0, nbr = 0, sh = 0, narea = 0, ak = 0, alrn = 0, acc = 0, rej = 0;
   for (int m = t; m < b.M; m += blockDim.x) {
-    const SgDabK *kp = &b.tab[(size_t)row * b.M + m];
-    if (kp->tool == SG_TOOL_SMOOTH && kp->learned) {
-      if (sb_learned_rejected(b, m))
-        rej++;
-      else
-        acc++;
+    const SgDabK k = b.tab[(size_t)row * b.M + m];
+    const long long *mc = sb_mesh_ctr(b, m);
+    const long long na = mc[SBM_NAFF];
+    aff += na;
+    cand += mc[SBM_CAND];
+    degt += mc[SBM_DEGT];
+    sh += mc[SBM_SHARED];
+    narea += mc[SBM_NAREA];
+    if (k.tool == SG_TOOL_SMOOTH) {
+      const bool plain = sb_plain_smooth(b, k, m);
+      if (plain) {
+        ak += na * k.iters;
+        nbr += mc[SBM_DEG] * k.iters;
+      }
+      if (k.learned) {
+        alrn += na;
+        if (plain)
+          rej++;
+        else
+          acc++;
+      }
     }
   }
-  if (acc) atomicAdd(&s_acc, acc);
-  if (rej) atomicAdd(&s_rej, rej);
-  long long c = 0;
-  if (t < SB_COUNT) c = b.ctr[t];
+  const long long vals[10] = {aff, cand, degt, nbr, sh, narea, ak, alrn, acc, rej};
+  const int slot[10] = {SBT_AFF, SBT_CAND, SBT_DEGT, SBT_NBR, SBT_SHARED, SBT_NAREA, SBT_AK, SBT_ALRN, SBT_LACC,
+                        SBT_LREJ};
+  for (int i = 0; i < 10; i++)
+    if (vals[i]) atomicAdd(&s_sum[slot[i]], (unsigned long long)vals[i]);
   __syncthreads();  // every read of this step's counters is done
-  if (t < SB_COUNT) b.tot[t] += c;
+  if (t == 0) {
+    s_sum[SBT_GATHER] += (unsigned long long)b.ctr[SB_NGATHER];
+    s_sum[SBT_TESTS] += (unsigned long long)b.ctr[SB_TESTS];
+    s_sum[SBT_STEPS] += 1ull;
+  }
+  __syncthreads();
+  if (t < SBT_COUNT) b.tot[t] += (long long)s_sum[t];
   const int n = SB_COUNT + SB_PM * b.M;
   for (int i = t; i < n; i += blockDim.x) b.ctr[i] = 0;
   if (t == 0) {
-    b.tot[SBT_LACC] += s_acc;
-    b.tot[SBT_LREJ] += s_rej;
-    b.tot[SBT_STEPS] += 1;
     b.step[0] += 1;
     b.step[1] += 1;
   }
@@ -490,9 +529,9 @@ void batch_free_device(SgBatch *b) {
   void *ptrs[] = {d.px,       d.py,       d.pz,       d.qx,         d.qy,      d.qz,        d.nx,
                   d.ny,       d.nz,       d.vt4,      d.vt_off,     d.vv_off,  d.vv_idx,    d.leaf_sh,
                   d.leaf_flags, d.leaf_bb, d.leaf_obb, d.cut_bb,    d.cut_obb, d.cut_flags, d.cut_lbeg,
-                  d.cut_lend, d.cut_touch, d.gathered, d.gl_v,      d.gl_s,    d.aff_v,     d.aff_vt,
-                  d.aff_vv,   d.aff_fade, d.vperm_old, b->leaf_vr,  b->leaf_shr, b->cut_mesh, b->gl_mesh,
-                  b->aff_mesh, b->ctr,    b->tot,     b->step,      b->tab,    b->gather_buf};
+                  d.cut_lend, d.cut_touch, d.gathered, d.gl_v,      d.gl_s,    d.vperm_old, b->leaf_vr,
+                  b->leaf_shr, b->cut_mesh, b->gl_mesh, b->vstamp, b->vfade, b->ctr,    b->tot,
+                  b->step,    b->tab,     b->gather_buf};
   for (void *q : ptrs)
     if (q) hipFree(q);
   for (void *q : b->snap)
@@ -514,7 +553,8 @@ SgBatchView sg_batch_view(SgBatch *b) {
   v.leaf_shr = b->leaf_shr;
   v.cut_mesh = b->cut_mesh;
   v.gl_mesh = b->gl_mesh;
-  v.aff_mesh = b->aff_mesh;
+  v.vstamp = b->vstamp;
+  v.vfade = b->vfade;
   v.ctr = b->ctr;
   v.tot = b->tot;
   v.step = b->step;
@@ -732,11 +772,8 @@ SgBatch *sg_batch_new(int n_meshes, const float *const *co, const int *const *tr
   rc |= dev_fill(&d.gl_v, L, 0, s);
   rc |= dev_fill(&d.gl_s, L, 0, s);
   rc |= dev_fill(&b->gl_mesh, L, 0, s);
-  rc |= dev_fill(&d.aff_v, V, 0, s);
-  rc |= dev_fill(&d.aff_vt, V, 0, s);
-  rc |= dev_fill(&d.aff_vv, V, 0, s);
-  rc |= dev_fill(&d.aff_fade, V, 0, s);
-  rc |= dev_fill(&b->aff_mesh, V, 0, s);
+  rc |= dev_fill(&b->vstamp, V, 0xFF, s);  // -1: never a step's serial (serials start at 1)
+  rc |= dev_fill(&b->vfade, V, 0, s);
   rc |= dev_fill(&b->ctr, SB_COUNT + SB_PM * (size_t)b->M, 0, s);
   rc |= dev_fill(&b->tot, SBT_COUNT, 0, s);
   rc |= dev_fill(&b->step, 2, 0, s);

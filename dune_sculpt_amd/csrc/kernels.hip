@@ -66,6 +66,7 @@ __global__ __launch_bounds__(256) void k_search(SgDevView d, SgDabK k, long long
     if (log && log_index > 0) log[(size_t)(log_index - 1) * SG_CTR_COUNT + lane] = prev;
     d.ctr_next[lane] = 0;
   }
+  if (gw == 0 && lane == 0 && d.seq) d.seq[0] += 1;  // this dab's stamp for the later kernels
   const float *cut_bbs = k.use_orig ? d.cut_obb : d.cut_bb;
   const float *leaf_bbs = k.use_orig ? d.leaf_obb : d.leaf_bb;
   long long ntests = 0;
@@ -759,6 +760,8 @@ SgDevView sg_view_of(SgDev &d, bool respect_hide, bool halt_live) {
   v.ctr_next = d.ctr + SG_CTR_COUNT;
   v.cut_touch = d.cut_touch;
   v.halt = halt_live ? d.halt : nullptr;  // no flag test in strokes without learned dabs
+  v.seq = d.seq;
+  v.vstamp = d.vstamp;
   v.vperm_old = d.vperm_old;
   v.stamp_v = d.stamp_v;
   v.stamp_v2 = d.stamp_v2;

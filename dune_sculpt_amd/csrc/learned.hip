@@ -109,72 +109,84 @@ __device__ inline void mlp_lds_load(MlpLds &s, const SgMlpView &m) {
 // reads of 16 consecutive rows fall in distinct LDS banks.
 #define MLP_ST 72
 
-// One wave evaluates the MLP for 64 rows (one per lane): x[28] already normalised;
-// bufA / bufB are this wave's 64 x MLP_ST bf16 staging buffers, sOut 64x4 floats.
-__device__ inline void mlp_wave(const MlpLds &s, uint16_t *bufA, uint16_t *bufB, float *sOut, const float *x,
-                                float *y) {
+// One wave evaluates the MLP for 16 rows: lane r < 16 supplies row r's normalised
+// inputs x[28] and receives y[3]; all 64 lanes run the MFMAs (28 of them: 8 + 16 + 4),
+// each lane 16 SiLUs per hidden layer.  bufA / bufB are this wave's 16 x MLP_ST bf16
+// staging rows, sOut 16x4 floats.  A short chain per wave: the dab kernels are latency
+// bound, so more waves with 16 rows each beat fewer waves with 64.
+__device__ inline void mlp_wave16(const MlpLds &s, uint16_t *bufA, uint16_t *bufB, float *sOut, const float *x,
+                                  float *y) {
   const int L = __lane_id();
   const int r = L & 15, kq = (L >> 4) * 4;
-  uint32_t *rowA = (uint32_t *)&bufA[L * MLP_ST];
+  if (L < 16) {
+    uint32_t *rowA = (uint32_t *)&bufA[L * MLP_ST];
 #pragma unroll
-  for (int k = 0; k < 14; k++) rowA[k] = (uint32_t)sg_f2bf(x[2 * k]) | ((uint32_t)sg_f2bf(x[2 * k + 1]) << 16);
-  rowA[14] = 0u;
-  rowA[15] = 0u;
+    for (int k = 0; k < 14; k++) rowA[k] = (uint32_t)sg_f2bf(x[2 * k]) | ((uint32_t)sg_f2bf(x[2 * k + 1]) << 16);
+    rowA[14] = 0u;
+    rowA[15] = 0u;
+  }
   wave_lds_sync();
-  // layer 1: [64 x 32] x [32 x 64]
-  for (int mt = 0; mt < 4; mt++) {
-    sg_float4 acc[4];
-    for (int nt = 0; nt < 4; nt++) acc[nt] = sg_float4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int kt = 0; kt < 2; kt++) {
-      const sg_short4 a = *(const sg_short4 *)&bufA[(mt * 16 + r) * MLP_ST + kt * 16 + kq];
-      for (int nt = 0; nt < 4; nt++) {
-        const sg_short4 b = *(const sg_short4 *)&s.w1t[(nt * 16 + r) * 32 + kt * 16 + kq];
-        acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nt], 0, 0, 0);
-      }
-    }
+  // layer 1: [16 x 32] x [32 x 64]
+  sg_float4 acc[4];
+#pragma unroll
+  for (int nt = 0; nt < 4; nt++) acc[nt] = sg_float4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int kt = 0; kt < 2; kt++) {
+    const sg_short4 a = *(const sg_short4 *)&bufA[r * MLP_ST + kt * 16 + kq];
+#pragma unroll
     for (int nt = 0; nt < 4; nt++) {
-      for (int i = 0; i < 4; i++) {
-        const float z = acc[nt][i] + s.b1[nt * 16 + r];
-        bufB[(mt * 16 + kq + i) * MLP_ST + nt * 16 + r] = sg_f2bf(sg_silu(z));
-      }
+      const sg_short4 bb = *(const sg_short4 *)&s.w1t[(nt * 16 + r) * 32 + kt * 16 + kq];
+      acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bb, acc[nt], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int nt = 0; nt < 4; nt++) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const float z = acc[nt][i] + s.b1[nt * 16 + r];
+      bufB[(kq + i) * MLP_ST + nt * 16 + r] = sg_f2bf(sg_silu(z));
     }
   }
   wave_lds_sync();
-  // layer 2: [64 x 64] x [64 x 64]
-  for (int mt = 0; mt < 4; mt++) {
-    sg_float4 acc[4];
-    for (int nt = 0; nt < 4; nt++) acc[nt] = sg_float4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int kt = 0; kt < 4; kt++) {
-      const sg_short4 a = *(const sg_short4 *)&bufB[(mt * 16 + r) * MLP_ST + kt * 16 + kq];
-      for (int nt = 0; nt < 4; nt++) {
-        const sg_short4 b = *(const sg_short4 *)&s.w2t[(nt * 16 + r) * 64 + kt * 16 + kq];
-        acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nt], 0, 0, 0);
-      }
-    }
+  // layer 2: [16 x 64] x [64 x 64]
+#pragma unroll
+  for (int nt = 0; nt < 4; nt++) acc[nt] = sg_float4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int kt = 0; kt < 4; kt++) {
+    const sg_short4 a = *(const sg_short4 *)&bufB[r * MLP_ST + kt * 16 + kq];
+#pragma unroll
     for (int nt = 0; nt < 4; nt++) {
-      for (int i = 0; i < 4; i++) {
-        const float z = acc[nt][i] + s.b2[nt * 16 + r];
-        bufA[(mt * 16 + kq + i) * MLP_ST + nt * 16 + r] = sg_f2bf(sg_silu(z));
-      }
+      const sg_short4 bb = *(const sg_short4 *)&s.w2t[(nt * 16 + r) * 64 + kt * 16 + kq];
+      acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bb, acc[nt], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int nt = 0; nt < 4; nt++) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const float z = acc[nt][i] + s.b2[nt * 16 + r];
+      bufA[(kq + i) * MLP_ST + nt * 16 + r] = sg_f2bf(sg_silu(z));
     }
   }
   wave_lds_sync();
-  // layer 3: [64 x 64] x [64 x 16] (3 used)
-  for (int mt = 0; mt < 4; mt++) {
-    sg_float4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int kt = 0; kt < 4; kt++) {
-      const sg_short4 a = *(const sg_short4 *)&bufA[(mt * 16 + r) * MLP_ST + kt * 16 + kq];
-      const sg_short4 b = *(const sg_short4 *)&s.w3t[r * 64 + kt * 16 + kq];
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-    }
-    if (r < 4) {
-      for (int i = 0; i < 4; i++) sOut[(mt * 16 + kq + i) * 4 + r] = acc[i] + s.b3[r];
-    }
+  // layer 3: [16 x 64] x [64 x 16] (3 used)
+  sg_float4 o = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int kt = 0; kt < 4; kt++) {
+    const sg_short4 a = *(const sg_short4 *)&bufA[r * MLP_ST + kt * 16 + kq];
+    const sg_short4 bb = *(const sg_short4 *)&s.w3t[r * 64 + kt * 16 + kq];
+    o = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bb, o, 0, 0, 0);
+  }
+  if (r < 4) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) sOut[(kq + i) * 4 + r] = o[i] + s.b3[r];
   }
   wave_lds_sync();
-  y[0] = sOut[L * 4 + 0];
-  y[1] = sOut[L * 4 + 1];
-  y[2] = sOut[L * 4 + 2];
+  if (L < 16) {
+    y[0] = sOut[L * 4 + 0];
+    y[1] = sOut[L * 4 + 1];
+    y[2] = sOut[L * 4 + 2];
+  }
   wave_lds_sync();
 }
 
@@ -277,46 +289,51 @@ __global__ __launch_bounds__(256) void k_learned_target(SgDevView d, const float
 }
 
 // ---------------------------------------------------------------------------
-// Inference on a flat batch (validation, tests): wave per 64 rows.
+// Inference on a flat batch (validation, tests): the dab path's MLP, 16 rows per wave.
 // ---------------------------------------------------------------------------
-#define LRN_WAVES 2
-__global__ __launch_bounds__(64 * LRN_WAVES) void k_mlp_predict(SgMlpView m, const float *X, int n, float *Y) {
-  __shared__ __align__(16) MlpLds s;
-  __shared__ __align__(16) uint16_t bufA[LRN_WAVES][64 * MLP_ST];
-  __shared__ __align__(16) uint16_t bufB[LRN_WAVES][64 * MLP_ST];
-  __shared__ __align__(16) float sOut[LRN_WAVES][64 * 4];
-  mlp_lds_load<64 * LRN_WAVES>(s, m);
+#define LRN_WAVES 4
+#define LRN_THREADS (64 * LRN_WAVES)
+#define LRN_ROWS 16
+__global__ __launch_bounds__(LRN_THREADS) void k_mlp_predict(SgMlpView m, const float *X, int n, float *Y) {
+  __shared__ MlpLds s;
+  __shared__ __align__(16) uint16_t bufA[LRN_WAVES][LRN_ROWS * MLP_ST];
+  __shared__ __align__(16) uint16_t bufB[LRN_WAVES][LRN_ROWS * MLP_ST];
+  __shared__ __align__(16) float sOut[LRN_WAVES][LRN_ROWS * 4];
+  mlp_lds_load<LRN_THREADS>(s, m);
   __syncthreads();
   const int w = threadIdx.x / 64, L = __lane_id();
   const int nw = gridDim.x * LRN_WAVES;
-  for (int base = (blockIdx.x * LRN_WAVES + w) * 64; base < n; base += nw * 64) {
+  for (int base = (blockIdx.x * LRN_WAVES + w) * LRN_ROWS; base < n; base += nw * LRN_ROWS) {
     const int row = base + L;
-    float x[28], y[3];
+    float x[28], y[3] = {0.0f, 0.0f, 0.0f};
     for (int k = 0; k < 28; k++) {
-      const float raw = row < n ? X[(size_t)row * 28 + k] : 0.0f;
+      const float raw = (L < LRN_ROWS && row < n) ? X[(size_t)row * 28 + k] : 0.0f;
       x[k] = (raw - m.norm[k]) * m.norm[28 + k];
     }
-    mlp_wave(s, bufA[w], bufB[w], sOut[w], x, y);
-    if (row < n) {
+    mlp_wave16(s, bufA[w], bufB[w], sOut[w], x, y);
+    if (L < LRN_ROWS && row < n) {
       for (int o = 0; o < 3; o++) Y[(size_t)row * 3 + o] = y[o] * m.norm[56 + o];
     }
   }
 }
 
 // ---------------------------------------------------------------------------
-// Learned dab.  k_brush_learned: one wave per gathered leaf, one vertex per lane:
-// sphere test and factor as the plain Smooth brush, then features -> MLP (MFMA) ->
-// predicted position written to the shadow copy; accumulates the error estimate
-// sum((|d|/r)^2) in fixed point and counts out-of-range vertices.
+// Learned dab.  k_brush_learned: work items (gathered leaf, slot), the wave of slot t
+// takes the leaf's 16-vertex tiles t, t+4, ...: sphere test and factor as the plain
+// Smooth brush, then features -> MLP (MFMA, 16 rows) -> predicted position written to
+// the shadow copy; affected vertices get the dab's stamp.  The error estimate
+// sum((|d|/r)^2) (2^-32 fixed point), the out-of-range count and the affected count
+// are reduced per block and added once per block.
 // k_post_learned: accept (error estimate and range checks within tolerance): normals
 // and leaf bounds from the shadow copy, commit it to the positions; reject: undo the
 // shadow copy and raise the halt flag so the rest of the stroke stops and the host
-// runs this dab with the plain brush.
+// runs this dab with the plain brush.  Both walk the gathered leaves' vertex ranges
+// and test the stamp (no per-vertex list, so no slot reservations).
 // ---------------------------------------------------------------------------
-// Evaluate one 64-vertex chunk of a learned dab (every lane of the wave calls it; the
-// MLP runs on all 64 rows): factor as the plain Smooth brush, features -> MLP ->
-// predicted position written to the shadow copy; accumulates the error estimate
-// sum((|d| / r)^2) in 2^-32 fixed point and counts vertices outside the trained range.
+#define LRN_SLOTS 4  // waves per gathered leaf
+
+// Evaluate one 16-vertex tile of a learned dab (every lane of the wave calls it; lanes
+// < 16 own a vertex).  Returns the Smooth factor of the lane's vertex.
 __device__ inline float learned_eval(const SgDevView &d, const SgDabK &k, const float *curve, const SgMlpView &m,
                                      const MlpLds &s, uint16_t *bufA, uint16_t *bufB, float *sOut, int v,
                                      const BrushIn &in, float nx, float ny, float nz, long long &esum, int &nood) {
@@ -328,9 +345,9 @@ __device__ inline float learned_eval(const SgDevView &d, const SgDabK &k, const 
   } else {
     for (int q = 0; q < 28; q++) feat[q] = 0.0f;
   }
-  float xn[28], y[3];
+  float xn[28], y[3] = {0.0f, 0.0f, 0.0f};
   for (int q = 0; q < 28; q++) xn[q] = in.aff ? (feat[q] - m.norm[q]) * m.norm[28 + q] : 0.0f;
-  mlp_wave(s, bufA, bufB, sOut, xn, y);
+  mlp_wave16(s, bufA, bufB, sOut, xn, y);
   if (in.aff) {
     const float o0 = y[0] * m.norm[56] * h, o1 = y[1] * m.norm[57] * h, o2 = y[2] * m.norm[58] * h;
     const float dx = R[0] * o0 + R[3] * o1 + R[6] * o2;
@@ -349,164 +366,179 @@ __device__ inline float learned_eval(const SgDevView &d, const SgDabK &k, const 
   return fade;
 }
 
-__global__ __launch_bounds__(64 * LRN_WAVES) void k_brush_learned(SgDevView d, SgDabK k, const float *curve,
-                                                                   SgMlpView m) {
-  __shared__ __align__(16) MlpLds s;
-  __shared__ __align__(16) uint16_t bufA[LRN_WAVES][64 * MLP_ST];
-  __shared__ __align__(16) uint16_t bufB[LRN_WAVES][64 * MLP_ST];
-  __shared__ __align__(16) float sOut[LRN_WAVES][64 * 4];
+// Loads of one tile: lanes < 16 take vertices base + lane (< end), the rest none.
+__device__ __forceinline__ BrushIn learned_tile_load(const SgDevView &d, const SgDabK &k, int base, int end, int lane,
+                                                     float &nx, float &ny, float &nz) {
+  const int v = base + lane;
+  const int vend = min(end, base + LRN_ROWS);
+  const BrushIn in = brush_load(d, k, v, vend, true);
+  nx = ny = nz = 0.0f;
+  if (v < vend) {
+    nx = d.nx[v];
+    ny = d.ny[v];
+    nz = d.nz[v];
+  }
+  return in;
+}
+
+__global__ __launch_bounds__(LRN_THREADS) void k_brush_learned(SgDevView d, SgDabK k, const float *curve,
+                                                               SgMlpView m) {
+  __shared__ MlpLds s;
+  __shared__ __align__(16) uint16_t bufA[LRN_WAVES][LRN_ROWS * MLP_ST];
+  __shared__ __align__(16) uint16_t bufB[LRN_WAVES][LRN_ROWS * MLP_ST];
+  __shared__ __align__(16) float sOut[LRN_WAVES][LRN_ROWS * 4];
+  __shared__ unsigned long long s_acc[3];
   if (d.halt && d.halt[0]) return;
-  // work items = (gathered leaf, half): a wave takes every other 64-vertex chunk of a
-  // leaf, so a leaf's chunks run on two waves at once
-  const int nitems = 2 * (int)d.ctr[SG_CTR_NGATHER];
+  const int nitems = LRN_SLOTS * (int)d.ctr[SG_CTR_NGATHER];
   if (blockIdx.x * LRN_WAVES >= nitems) return;  // whole block idle: skip the weight fill
-  mlp_lds_load<64 * LRN_WAVES>(s, m);
+  mlp_lds_load<LRN_THREADS>(s, m);
+  if (threadIdx.x < 3) s_acc[threadIdx.x] = 0ull;
   __syncthreads();
+  const int stamp = d.seq[0];
   const int w = threadIdx.x / 64, lane = __lane_id();
   const int gw = blockIdx.x * LRN_WAVES + w;
   const int nw = gridDim.x * LRN_WAVES;
   long long esum = 0;
-  int nood = 0;
+  int nood = 0, naff = 0;
   for (int it = gw; it < nitems; it += nw) {
-    const int2 vr = d.gl_v[it >> 1];
-    for (int base = vr.x + (it & 1) * SG_WAVE; base < vr.y; base += 2 * SG_WAVE) {
+    const int2 vr = d.gl_v[it / LRN_SLOTS];
+    for (int base = vr.x + (it % LRN_SLOTS) * LRN_ROWS; base < vr.y; base += LRN_SLOTS * LRN_ROWS) {
+      float nx, ny, nz;
+      const BrushIn in = learned_tile_load(d, k, base, vr.y, lane, nx, ny, nz);
+      if (__ballot(in.aff) == 0ull) continue;
       const int v = base + lane;
-      const BrushIn in = brush_load(d, k, v, vr.y, true);
-      float nx = 0.0f, ny = 0.0f, nz = 0.0f;
-      if (v < vr.y) {
-        nx = d.nx[v];
-        ny = d.ny[v];
-        nz = d.nz[v];
-      }
-      const unsigned long long mk = __ballot(in.aff);
-      if (mk == 0ull) continue;
-      int wbase = 0;
-      if (lane == 0) wbase = (int)atomicAdd((unsigned long long *)&d.ctr[SG_CTR_NAFF], (unsigned long long)__popcll(mk));
       learned_eval(d, k, curve, m, s, bufA[w], bufB[w], sOut[w], v, in, nx, ny, nz, esum, nood);
-      wbase = __shfl(wbase, 0);
       if (in.aff) {
-        const int pos = wbase + mask_prefix(mk);
-        d.aff_v[pos] = v;
-        d.aff_vt[pos] = in.vt;
-        d.aff_vv[pos] = in.vv;
+        d.vstamp[v] = stamp;
+        naff++;
       }
     }
   }
   esum = wave_sum_ll(esum);
   nood = wave_sum_i(nood);
-  if (lane == 0 && (esum || nood)) {
-    atomicAdd((unsigned long long *)&d.ctr[SG_CTR_LERR], (unsigned long long)esum);
-    atomicAdd((unsigned long long *)&d.ctr[SG_CTR_LOOD], (unsigned long long)nood);
+  naff = wave_sum_i(naff);
+  if (lane == 0 && naff) {
+    atomicAdd(&s_acc[0], (unsigned long long)esum);
+    atomicAdd(&s_acc[1], (unsigned long long)nood);
+    atomicAdd(&s_acc[2], (unsigned long long)naff);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && s_acc[2]) {
+    atomicAdd((unsigned long long *)&d.ctr[SG_CTR_LERR], s_acc[0]);
+    atomicAdd((unsigned long long *)&d.ctr[SG_CTR_LOOD], s_acc[1]);
+    atomicAdd((unsigned long long *)&d.ctr[SG_CTR_NAFF], s_acc[2]);
   }
 }
 
-// Batched learned brush: items (gathered leaf of a learned-Smooth mesh, half) of every
-// mesh; per-mesh error sums feed the per-mesh decision.
-__global__ __launch_bounds__(64 * LRN_WAVES) void k_bbrush_learned(SgBatchView b, SgMlpView m) {
-  __shared__ __align__(16) MlpLds s;
-  __shared__ __align__(16) uint16_t bufA[LRN_WAVES][64 * MLP_ST];
-  __shared__ __align__(16) uint16_t bufB[LRN_WAVES][64 * MLP_ST];
-  __shared__ __align__(16) float sOut[LRN_WAVES][64 * 4];
+// Batched learned brush: items (gathered leaf of a learned-Smooth mesh, slot) of every
+// mesh; per-mesh error / range / affected counts feed the per-mesh decision.
+__global__ __launch_bounds__(LRN_THREADS) void k_bbrush_learned(SgBatchView b, SgMlpView m) {
+  __shared__ MlpLds s;
+  __shared__ __align__(16) uint16_t bufA[LRN_WAVES][LRN_ROWS * MLP_ST];
+  __shared__ __align__(16) uint16_t bufB[LRN_WAVES][LRN_ROWS * MLP_ST];
+  __shared__ __align__(16) float sOut[LRN_WAVES][LRN_ROWS * 4];
   const SgDevView &d = b.d;
-  const int nitems = 2 * (int)b.ctr[SB_NGATHER];
+  const int nitems = LRN_SLOTS * (int)b.ctr[SB_NGATHER];
   if (blockIdx.x * LRN_WAVES >= nitems) return;
-  mlp_lds_load<64 * LRN_WAVES>(s, m);
+  mlp_lds_load<LRN_THREADS>(s, m);
   __syncthreads();
   const int w = threadIdx.x / 64, lane = __lane_id();
   const int gw = blockIdx.x * LRN_WAVES + w;
   const int nw = gridDim.x * LRN_WAVES;
-  const int row = b.step[1];
-  long long degt = 0, ncand = 0;
+  const int serial = b.step[0], row = b.step[1];
   for (int it = gw; it < nitems; it += nw) {
-    const int g = it >> 1;
+    const int g = it / LRN_SLOTS;
     const int mesh = __builtin_amdgcn_readfirstlane(b.gl_mesh[g]);
     const SgDabK k = b.tab[(size_t)row * b.M + mesh];
     if (!k.learned) continue;
     const int2 vr = d.gl_v[g];
-    if ((it & 1) == 0) ncand += vr.y - vr.x;
-    long long esum = 0;
+    long long esum = 0, degt = 0;
     int nood = 0, naff = 0;
-    for (int base = vr.x + (it & 1) * SG_WAVE; base < vr.y; base += 2 * SG_WAVE) {
+    for (int base = vr.x + (it % LRN_SLOTS) * LRN_ROWS; base < vr.y; base += LRN_SLOTS * LRN_ROWS) {
+      float nx, ny, nz;
+      const BrushIn in = learned_tile_load(d, k, base, vr.y, lane, nx, ny, nz);
+      if (__ballot(in.aff) == 0ull) continue;
       const int v = base + lane;
-      const BrushIn in = brush_load(d, k, v, vr.y, true);
-      float nx = 0.0f, ny = 0.0f, nz = 0.0f;
-      if (v < vr.y) {
-        nx = d.nx[v];
-        ny = d.ny[v];
-        nz = d.nz[v];
-      }
-      const unsigned long long mk = __ballot(in.aff);
-      if (mk == 0ull) continue;
-      int wbase = 0;
-      if (lane == 0) wbase = (int)atomicAdd((unsigned long long *)&b.ctr[SB_NAFF], (unsigned long long)__popcll(mk));
       const float fade = learned_eval(d, k, nullptr, m, s, bufA[w], bufB[w], sOut[w], v, in, nx, ny, nz, esum, nood);
-      wbase = __shfl(wbase, 0);
-      naff += __popcll(mk);
       if (in.aff) {
-        const int pos = wbase + mask_prefix(mk);
-        d.aff_v[pos] = v;
-        d.aff_vt[pos] = in.vt;
-        d.aff_vv[pos] = in.vv;
-        d.aff_fade[pos] = fade;
-        b.aff_mesh[pos] = mesh;
+        b.vstamp[v] = serial;
+        b.vfade[v] = fade;
+        naff++;
         degt += in.vt.y - in.vt.x;
       }
     }
     esum = wave_sum_ll(esum);
     nood = wave_sum_i(nood);
-    if (lane == 0 && naff) {
-      long long *mc = b.ctr + SB_COUNT + SB_PM * mesh;
-      atomicAdd((unsigned long long *)&mc[SBM_LERR], (unsigned long long)esum);
-      atomicAdd((unsigned long long *)&mc[SBM_LOOD], (unsigned long long)nood);
-      atomicAdd((unsigned long long *)&mc[SBM_LNAFF], (unsigned long long)naff);
+    naff = wave_sum_i(naff);
+    degt = wave_sum_ll(degt);
+    if (lane == 0) {
+      long long *mc = b.ctr + SB_COUNT + SB_PM * (size_t)mesh;
+      if (it % LRN_SLOTS == 0) atomicAdd((unsigned long long *)&mc[SBM_CAND], (unsigned long long)(vr.y - vr.x));
+      if (naff) {
+        atomicAdd((unsigned long long *)&mc[SBM_LERR], (unsigned long long)esum);
+        atomicAdd((unsigned long long *)&mc[SBM_LOOD], (unsigned long long)nood);
+        atomicAdd((unsigned long long *)&mc[SBM_LNAFF], (unsigned long long)naff);
+        atomicAdd((unsigned long long *)&mc[SBM_NAFF], (unsigned long long)naff);
+        atomicAdd((unsigned long long *)&mc[SBM_DEGT], (unsigned long long)degt);
+      }
     }
-  }
-  degt = wave_sum_ll(degt);
-  if (lane == 0) {
-    if (degt) atomicAdd((unsigned long long *)&b.ctr[SB_DEGT], (unsigned long long)degt);
-    if (ncand) atomicAdd((unsigned long long *)&b.ctr[SB_CAND], (unsigned long long)ncand);
   }
 }
 
-// Batched decision, per record of a learned-Smooth mesh: accepted -> the prediction is
-// committed to the positions; rejected -> the shadow copy gets the plain brush's first
-// Jacobi step from the untouched positions (steps 2..k follow in the plain kernels).
+// Batched decision, per stamped vertex of a learned-Smooth mesh (wave per gathered
+// leaf): accepted -> the prediction is committed to the positions; rejected -> the
+// shadow copy gets the plain brush's first Jacobi step from the untouched positions
+// (steps 2..k follow in the plain kernels).
 __global__ __launch_bounds__(256) void k_bdecide(SgBatchView b) {
   const SgDevView &d = b.d;
-  const int row = b.step[1];
-  const int na = (int)b.ctr[SB_NAFF];
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
-    const int mesh = b.aff_mesh[i];
+  const int lane = __lane_id();
+  const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
+  const int nw = gridDim.x * blockDim.x / SG_WAVE;
+  const int serial = b.step[0], row = b.step[1];
+  const int ng = (int)b.ctr[SB_NGATHER];
+  for (int g = gw; g < ng; g += nw) {
+    const int mesh = __builtin_amdgcn_readfirstlane(b.gl_mesh[g]);
     if (!b.tab[(size_t)row * b.M + mesh].learned) continue;
-    const int v = d.aff_v[i];
-    if (!sb_learned_rejected(b, mesh)) {
-      d.px[v] = d.qx[v];
-      d.py[v] = d.qy[v];
-      d.pz[v] = d.qz[v];
-      continue;
+    const bool rejected = sb_learned_rejected(b, mesh);
+    const int2 vr = d.gl_v[g];
+    for (int v = vr.x + lane; v < vr.y; v += SG_WAVE) {
+      if (b.vstamp[v] != serial) continue;
+      if (!rejected) {
+        d.px[v] = d.qx[v];
+        d.py[v] = d.qy[v];
+        d.pz[v] = d.qz[v];
+        continue;
+      }
+      const int2 vv = make_int2(d.vv_off[v], d.vv_off[v + 1]);
+      const float fade = b.vfade[v];
+      const float x = d.px[v], y = d.py[v], z = d.pz[v];
+      float sx, sy, sz;
+      nbr_sum(d.vv_idx, vv.x, vv.y, d.px, d.py, d.pz, sx, sy, sz);
+      float rx = x, ry = y, rz = z;
+      if (vv.y > vv.x) {
+        const float inv = 1.0f / (float)(vv.y - vv.x);
+        const float avx = sx * inv, avy = sy * inv, avz = sz * inv;
+        rx = x + (avx - x) * fade;
+        ry = y + (avy - y) * fade;
+        rz = z + (avz - z) * fade;
+      }
+      d.qx[v] = rx;
+      d.qy[v] = ry;
+      d.qz[v] = rz;
     }
-    const int2 vv = d.aff_vv[i];
-    const float fade = d.aff_fade[i];
-    const float x = d.px[v], y = d.py[v], z = d.pz[v];
-    float sx, sy, sz;
-    nbr_sum(d.vv_idx, vv.x, vv.y, d.px, d.py, d.pz, sx, sy, sz);
-    float rx = x, ry = y, rz = z;
-    if (vv.y > vv.x) {
-      const float inv = 1.0f / (float)(vv.y - vv.x);
-      const float avx = sx * inv, avy = sy * inv, avz = sz * inv;
-      rx = x + (avx - x) * fade;
-      ry = y + (avy - y) * fade;
-      rz = z + (avz - z) * fade;
-    }
-    d.qx[v] = rx;
-    d.qy[v] = ry;
-    d.qz[v] = rz;
   }
 }
 
+// Items (gathered leaf, part): part 0 = the leaf's stamped vertices (accept: normals
+// from the shadow copy, then commit; reject: restore the shadow copy), part 1 = the
+// leaf's bounds from the shadow copy (accept only).
 __global__ __launch_bounds__(256) void k_post_learned(SgDevView d, SgDevView dq, float tol, float alpha,
                                                       int serial) {
   if (d.halt && d.halt[0]) return;
+  const int lane = __lane_id();
+  const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
+  const int nw = gridDim.x * blockDim.x / SG_WAVE;
+  const int stamp = d.seq[0];
   const int na = (int)d.ctr[SG_CTR_NAFF];
   const int ng = (int)d.ctr[SG_CTR_NGATHER];
   const long long es = d.ctr[SG_CTR_LERR];
@@ -514,30 +546,34 @@ __global__ __launch_bounds__(256) void k_post_learned(SgDevView d, SgDevView dq,
   // error estimate: alpha * RMS(|d| / r), alpha = held-out RMS error / RMS displacement
   const double rms = na > 0 ? sqrt((double)es / 4294967296.0 / (double)na) : 0.0;
   const bool accept = nood == 0 && (double)alpha * rms <= (double)tol;
-  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nthreads = gridDim.x * blockDim.x;
-  if (!accept) {
-    for (int i = tid; i < na; i += nthreads) {
-      const int v = d.aff_v[i];
-      d.qx[v] = d.px[v];
-      d.qy[v] = d.py[v];
-      d.qz[v] = d.pz[v];
-    }
-    if (tid == 0 && d.halt) d.halt[0] = serial + 1;
-    return;
-  }
   long long a = 0, b = 0, c = 0, nsh = 0;
-  for (int i = tid; i < na; i += nthreads) {
-    const int v = d.aff_v[i];
-    const int2 vt = d.aff_vt[i];
-    vertex_normal(dq, v, vt.x, vt.y, 0, 0, a, b, c);  // corners read from the shadow copy
-    d.px[v] = d.qx[v];
-    d.py[v] = d.qy[v];
-    d.pz[v] = d.qz[v];
+  for (int it = gw; it < 2 * ng; it += nw) {
+    const int g = it >> 1;
+    if (it & 1) {
+      if (accept) leaf_refit(dq, d.gathered[g], d.gl_v[g], d.gl_s[g], nsh);
+      continue;
+    }
+    const int2 vr = d.gl_v[g];
+    for (int v = vr.x + lane; v < vr.y; v += SG_WAVE) {
+      if (d.vstamp[v] != stamp) continue;
+      if (accept) {
+        vertex_normal(dq, v, d.vt_off[v], d.vt_off[v + 1], 0, 0, a, b, c);  // corners from the shadow copy
+        d.px[v] = d.qx[v];
+        d.py[v] = d.qy[v];
+        d.pz[v] = d.qz[v];
+      } else {
+        d.qx[v] = d.px[v];
+        d.qy[v] = d.py[v];
+        d.qz[v] = d.pz[v];
+      }
+    }
   }
-  const int gw = tid / SG_WAVE, nw = nthreads / SG_WAVE;
-  for (int g = gw; g < ng; g += nw) leaf_refit(dq, d.gathered[g], d.gl_v[g], d.gl_s[g], nsh);
-  if (tid == 0) d.ctr[SG_CTR_LACC] = 1;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (accept)
+      d.ctr[SG_CTR_LACC] = 1;
+    else if (d.halt)
+      d.halt[0] = serial + 1;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -884,8 +920,8 @@ int sg_mlp_predict(SgMlp *m, const float *X, int n, float *Y) {
   hipMalloc(&dX, (size_t)n * 28 * 4);
   hipMalloc(&dY, (size_t)n * 3 * 4);
   hipMemcpyAsync(dX, X, (size_t)n * 28 * 4, hipMemcpyHostToDevice, m->stream);
-  const int blocks = (n + 64 * LRN_WAVES - 1) / (64 * LRN_WAVES);
-  hipLaunchKernelGGL(k_mlp_predict, dim3(blocks < 2048 ? blocks : 2048), dim3(64 * LRN_WAVES), 0, m->stream,
+  const int blocks = (n + LRN_ROWS * LRN_WAVES - 1) / (LRN_ROWS * LRN_WAVES);
+  hipLaunchKernelGGL(k_mlp_predict, dim3(blocks < 4096 ? blocks : 4096), dim3(LRN_THREADS), 0, m->stream,
                      sg_mlp_view(m), dX, n, dY);
   hipMemcpyAsync(Y, dY, (size_t)n * 3 * 4, hipMemcpyDeviceToHost, m->stream);
   int rc = sg_check_hip(hipStreamSynchronize(m->stream), "predict");
@@ -916,7 +952,7 @@ int sg_mfma_probe(const uint16_t *A, const uint16_t *B, float *D) {
 // ---- launch helpers used by the dab path (api.cpp / kernels.hip) ----
 int sg_enqueue_learned_brush(PBVH *p, const SgDevView &v, const SgDabK &k, const float *curve, SgMlp *m) {
   const int G = p->grid_brush > 0 ? p->grid_brush : p->grid_wg;
-  hipLaunchKernelGGL(k_brush_learned, dim3(G), dim3(64 * LRN_WAVES), 0, p->stream, v, k, curve, sg_mlp_view(m));
+  hipLaunchKernelGGL(k_brush_learned, dim3(G), dim3(LRN_THREADS), 0, p->stream, v, k, curve, sg_mlp_view(m));
   return sg_check_hip(hipGetLastError(), "learned brush launch");
 }
 
@@ -964,7 +1000,7 @@ int sg_enqueue_judge(PBVH *p, int na, const float *lrn, const float *bx, const f
 }
 
 int sg_batch_enqueue_learned_brush(SgBatch *b, const SgBatchView &v) {
-  hipLaunchKernelGGL(k_bbrush_learned, dim3(b->g_learned), dim3(64 * LRN_WAVES), 0, b->stream, v,
+  hipLaunchKernelGGL(k_bbrush_learned, dim3(b->g_learned), dim3(LRN_THREADS), 0, b->stream, v,
                      sg_mlp_view(b->mlp));
   return sg_check_hip(hipGetLastError(), "batch learned brush launch");
 }

@@ -684,6 +684,8 @@ void sg_build_device(PBVH *p, const float *co) {
   alloc_fill(&d.aff_fade, V, 0, s);
   alloc_fill(&d.ctr, 2 * SG_CTR_COUNT, 0, s);
   alloc_fill(&d.halt, 4, 0, s);
+  alloc_fill(&d.seq, 1, 0, s);
+  alloc_fill(&d.vstamp, V, 0xFF, s);  // -1: never a dab's stamp (stamps start at 1)
   alloc_fill(&d.err, 4, 0, s);
   upload(&d.vperm_old, p->perm_old_of_new.data(), V, s);
   hipStreamSynchronize(s);
@@ -697,7 +699,7 @@ void sg_free_device(PBVH *p) {
                   d.cut_bb, d.cut_obb, d.cut_flags, d.cut_node, d.cut_lbeg, d.cut_lend,
                   d.sub_off, d.sub_tab, d.top_tab_n, d.top_tab, d.cut_dirty, d.vhide, d.vmark,
                   d.gathered, d.gl_v, d.gl_s, d.aff_v, d.aff_vt, d.aff_vv, d.aff_fade,
-                  d.prev_cuts, d.cut_touch, d.halt, d.ctr, d.err, d.vperm_old, d.stamp_v, d.stamp_v2, d.stamp_t,
+                  d.prev_cuts, d.cut_touch, d.halt, d.seq, d.vstamp, d.ctr, d.err, d.vperm_old, d.stamp_v, d.stamp_v2, d.stamp_t,
                   d.stamp_a, d.curve, d.log};
   for (void *q : ptrs) {
     if (q) hipFree(q);

@@ -5,6 +5,9 @@
 // arena, so a batch is a forest of PBVHs over one disconnected mesh.  Each kernel of a
 // step covers all meshes at once: work items (cuts, gathered leaves, affected records)
 // carry their mesh, whose dab constants come from a [step][mesh] table in HBM.
+// Affected vertices are not listed: the brush stamps each with the step's serial and
+// later kernels walk the gathered leaves' vertex ranges testing the stamp, so no
+// kernel reserves list slots with atomics.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -14,19 +17,13 @@
 #include "sg_internal.h"
 #include "sg_kernels.h"
 
-// step counters (one set, cleared by the step-end kernel)
+// Step counters (one set, cleared by the step-end kernel).  Counters that every wave
+// would bump are kept per mesh instead: returning atomics on one address serialise
+// across the XCDs (~12 ns each), per-mesh addresses spread them out.
 enum {
-  SB_NGATHER = 0,   // gathered leaves (all meshes)
-  SB_NAFF = 1,      // affected records
-  SB_TESTS = 2,     // cut + leaf box tests
-  SB_CAND = 3,      // uniq vertices of gathered leaves
-  SB_DEGT = 4,      // sum of vertex->tri degrees of affected vertices
-  SB_DEG = 5,       // sum of vertex->vertex degrees of Smooth records, times k
-  SB_SHARED = 6,    // shared vertices read by the leaf refit
-  SB_NAREA = 7,     // vertices inside the Draw area-normal sphere
-  SB_AK = 8,        // plain Smooth records times their iteration count
-  SB_ALRN = 9,      // learned Smooth records
-  SB_COUNT = 12,
+  SB_NGATHER = 0,   // gathered leaves (all meshes; one reservation per search wave)
+  SB_TESTS = 1,     // cut + leaf box tests (one add per search wave)
+  SB_COUNT = 4,
 };
 // per-mesh counters, SB_PM entries per mesh after the global ones
 enum {
@@ -34,11 +31,19 @@ enum {
   SBM_LERR = 3,     // learned Smooth: sum (|d|/r)^2, 2^-32 fixed point
   SBM_LOOD = 4,     // learned Smooth: vertices outside the trained range
   SBM_LNAFF = 5,    // learned Smooth: affected vertices
-  SB_PM = 8,
+  SBM_NAFF = 6,     // affected vertices
+  SBM_CAND = 7,     // uniq vertices of gathered leaves
+  SBM_DEGT = 8,     // vertex->tri degrees of affected vertices
+  SBM_DEG = 9,      // vertex->vertex degrees of Smooth affected vertices (one step)
+  SBM_SHARED = 10,  // shared vertices read by the leaf refit
+  SBM_NAREA = 11,   // vertices inside the Draw area-normal sphere
+  SB_PM = 16,       // 128 bytes per mesh
 };
-// running totals
-// running totals: entries 0..SB_COUNT-1 accumulate the step counters of the same index
-enum { SBT_LACC = 12, SBT_LREJ = 13, SBT_STEPS = 14, SBT_COUNT = 16 };
+// running totals (step-end kernel)
+enum {
+  SBT_GATHER = 0, SBT_AFF = 1, SBT_TESTS = 2, SBT_CAND = 3, SBT_DEGT = 4, SBT_NBR = 5, SBT_SHARED = 6,
+  SBT_NAREA = 7, SBT_AK = 8, SBT_ALRN = 9, SBT_LACC = 12, SBT_LREJ = 13, SBT_STEPS = 14, SBT_COUNT = 16,
+};
 
 // kernel-side view of a batch
 struct SgBatchView {
@@ -48,7 +53,8 @@ struct SgBatchView {
   const int2 *leaf_shr;     // [L] shared vertex range (into leaf_sh)
   const int *cut_mesh;      // [C]
   int *gl_mesh;             // [L] mesh of each gathered entry
-  int *aff_mesh;            // [V] mesh of each affected record
+  int *vstamp;              // [V] serial of the last step that affected the vertex
+  float *vfade;             // [V] Smooth factor of that step
   long long *ctr;           // [SB_COUNT + SB_PM * M]
   long long *tot;           // [SBT_COUNT]
   int *step;                // [2]: serial (monotonic, for the search's touch test), table row
@@ -61,7 +67,8 @@ struct SgBatch {
   hipStream_t stream = nullptr;
   SgDev d;
   int2 *leaf_vr = nullptr, *leaf_shr = nullptr;
-  int *cut_mesh = nullptr, *gl_mesh = nullptr, *aff_mesh = nullptr;
+  int *cut_mesh = nullptr, *gl_mesh = nullptr, *vstamp = nullptr;
+  float *vfade = nullptr;
   long long *ctr = nullptr, *tot = nullptr;
   int *step = nullptr;
   SgDabK *tab = nullptr;

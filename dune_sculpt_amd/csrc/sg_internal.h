@@ -67,6 +67,8 @@ struct SgDev {
   int *cut_dirty = nullptr;     // [C]
   int *cut_touch = nullptr;     // [C] serial of the last dab that gathered a leaf of it
   int *halt = nullptr;          // [4] set by a rejected learned dab: its serial + 1
+  int *seq = nullptr;           // [1] dab counter (each dab's search adds one): the learned dab's stamp
+  int *vstamp = nullptr;        // [V] stamp of the last learned dab that affected the vertex
   int *vhide = nullptr;         // [V] or null
   uint8_t *vmark = nullptr;     // [V] host-driven normal update marks
   // per-dab scratch

@@ -42,6 +42,7 @@ struct SgDevView {
   int *prev_cuts;
   int *cut_touch;
   int *halt;
+  int *seq, *vstamp;
   long long *ctr, *ctr_next;
   const int *vperm_old;
   int *stamp_v, *stamp_v2, *stamp_t, *stamp_a;
