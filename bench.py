@@ -285,15 +285,16 @@ def run_batch_workloads(sg, args, W):
     meshes of 1M-4M vertices, 128 steps cycling Draw / Inflate / learned Smooth(4),
     r ~ U[0.02, 0.08], timed against the same table with plain Smooth(4)."""
     res = {}
-    t = time.perf_counter()
-    meshes, kinds = batch_meshes(sg, args.batch_meshes, seed=11, vmin=5000, vmax=500000, amplitude=0.01)
-    table = batch_table(sg, meshes, kinds, args.batch_steps, seed=12, tools=(sg.TOOL_DRAW, sg.TOOL_SMOOTH),
-                        rmin=0.02, rmax=0.1)
-    log(f"c4 meshes and strokes in {time.perf_counter() - t:.1f}s")
-    res["c4"] = run_batch(sg, "c4", meshes, table, W, 3, tune_steps=8)
-    res["c4"]["workload"] = (f"{len(meshes)} meshes, V log-uniform [5k, 500k] (icospheres / tori 1.0/0.3, noise "
-                             f"0.01), {args.batch_steps} Draw/Smooth(4) dabs each, r~U[0.02,0.1], s~U[0.2,1.0]")
-    del meshes, table
+    if not args.no_c4:
+        t = time.perf_counter()
+        meshes, kinds = batch_meshes(sg, args.batch_meshes, seed=11, vmin=5000, vmax=500000, amplitude=0.01)
+        table = batch_table(sg, meshes, kinds, args.batch_steps, seed=12, tools=(sg.TOOL_DRAW, sg.TOOL_SMOOTH),
+                            rmin=0.02, rmax=0.1)
+        log(f"c4 meshes and strokes in {time.perf_counter() - t:.1f}s")
+        res["c4"] = run_batch(sg, "c4", meshes, table, W, 3, tune_steps=0 if args.no_tune else 8)
+        res["c4"]["workload"] = (f"{len(meshes)} meshes, V log-uniform [5k, 500k] (icospheres / tori 1.0/0.3, noise "
+                                 f"0.01), {args.batch_steps} Draw/Smooth(4) dabs each, r~U[0.02,0.1], s~U[0.2,1.0]")
+        del meshes, table
     if not args.no_c5:
         from dune_sculpt_amd import learned as L
 
@@ -306,7 +307,8 @@ def run_batch_workloads(sg, args, W):
                          learned_smooth=mlp is not None)
         tp = batch_table(sg, meshes, kinds, args.c5_steps, seed=22, tools=tools, rmin=0.02, rmax=0.08)
         log(f"c5 meshes and strokes in {time.perf_counter() - t:.1f}s")
-        res["c5"] = run_batch(sg, "c5", meshes, tl, W, 2, tune_steps=8, learned_model=mlp, plain_table=tp)
+        res["c5"] = run_batch(sg, "c5", meshes, tl, W, 2, tune_steps=0 if args.no_tune else 8, learned_model=mlp,
+                              plain_table=tp)
         res["c5"]["workload"] = (f"{len(meshes)} meshes of 1M-4M V (icosphere-9 / tori, noise 0.02), "
                                  f"{args.c5_steps} dabs each cycling Draw/Inflate/learned Smooth(4), r~U[0.02,0.08]")
         if mlp is not None:
@@ -359,6 +361,7 @@ def main():
     ap.add_argument("--only-batch", action="store_true", help="run only the many-mesh workloads")
     ap.add_argument("--batch-meshes", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=64)
+    ap.add_argument("--no-c4", action="store_true", help="skip the 256-mesh batch")
     ap.add_argument("--no-c5", action="store_true", help="skip the 64 x 1-4M-vertex learned batch")
     ap.add_argument("--c5-meshes", type=int, default=64)
     ap.add_argument("--c5-steps", type=int, default=128)
@@ -488,6 +491,10 @@ def main():
                "kind": "port",
                "sample": f"first {r['dabs']} dabs of the same stroke on the same icosphere-10 mesh "
                          f"({r['seconds']:.1f}s, C oracle -O2 -ffp-contract=off, OpenMP over gathered leaves)"}
+        r1 = cpu_baseline(sg, co, tris, dabs, args.cpu_budget / 2, 1)
+        cpu["one_thread"] = {"value": r1["affected"] / r1["seconds"], "cores": r1["threads"],
+                             "sample": f"first {r1['dabs']} dabs ({r1['seconds']:.1f}s)"}
+        log(f"CPU baseline: {cpu}")
     A.close()
 
     line = {

@@ -32,159 +32,111 @@
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int sb_uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
-// Search: one lane per cut for the cut-level work (64 cuts of consecutive meshes per
-// wave), then the wave takes the passing cuts one at a time for the leaf tests.
-//   level 1  each lane: its cut's mesh, range, flags, touch stamp and both boxes
-//   level 2  each lane: its mesh's dab centre / radius for this step
-//   rebuild  cuts whose leaves moved in the previous step get their boxes rebuilt from
-//            the leaf boxes, one cut at a time by the whole wave (also for meshes
-//            without a dab this step)
-//   leaves   per passing cut, up to 128 leaf tests per load level; the wave counts its
-//            passing leaves over all its cuts first, reserves their slots with one
-//            atomic, then tests again (boxes now cached) and writes them
+// Search, two kernels:
+//   k_bsearch_cuts    one lane per cut (64 cuts of consecutive meshes per wave): its
+//                     mesh's dab, the sphere against the cut's box; passing cuts are
+//                     listed (one slot reservation per wave)
+//   k_bsearch_leaves  one wave per listed cut: up to 128 leaf tests per load level,
+//                     passing leaves appended to the gathered list
+// Cut boxes are exact at search time: the step-end kernel rebuilds the box of every
+// listed cut from its leaf boxes after the step's refit.
 #define BSEARCH_UNROLL 2
 
-struct BCut {
-  int c, lb, le, mesh, orig;
-  float cx, cy, cz, r2;
-};
-
-__device__ __forceinline__ BCut bcut_of(int j, int c0, int lb, int le, int mesh, int use_orig, float cx, float cy,
-                                        float cz, float r2) {
-  BCut q;
-  q.c = c0 + j;
-  q.lb = __shfl(lb, j);
-  q.le = __shfl(le, j);
-  q.mesh = __shfl(mesh, j);
-  q.orig = __shfl(use_orig, j);
-  q.cx = __shfl(cx, j);
-  q.cy = __shfl(cy, j);
-  q.cz = __shfl(cz, j);
-  q.r2 = __shfl(r2, j);
-  return q;
-}
-
-// ballots of the leaf tests of cut q, leaves [lb + q0, lb + q0 + 128)
-__device__ __forceinline__ void bleaf_tests(const SgDevView &d, const BCut &q, int q0, unsigned long long *m) {
-  const int lane = lane_id();
-  const float *leaf_bbs = q.orig ? d.leaf_obb : d.leaf_bb;
-#pragma unroll
-  for (int u = 0; u < BSEARCH_UNROLL; u++) {
-    const int l = q.lb + q0 + u * SG_WAVE + lane;
-    bool p = false;
-    if (l < q.le) {
-      const BB6 bx = bb_load(leaf_bbs + 8 * (size_t)l);
-      p = !(d.leaf_flags[l] & 1) && sphere_box(bx, q.cx, q.cy, q.cz, q.r2);
-    }
-    m[u] = __ballot(p);
-  }
-}
-
-__global__ __launch_bounds__(256) void k_bsearch(SgBatchView b) {
+__global__ __launch_bounds__(256) void k_bsearch_cuts(SgBatchView b) {
   const SgDevView &d = b.d;
   const int lane = lane_id();
   const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
   const int nw = gridDim.x * blockDim.x / SG_WAVE;
-  const int serial = b.step[0], row = b.step[1];
+  const int row = b.step[1];
   long long ntests = 0;
   for (int c0 = gw * SG_WAVE; c0 < d.C; c0 += nw * SG_WAVE) {
     const int c = c0 + lane;
     const bool valid = c < d.C;
-    int mesh = 0, touch = 0, lb = 0, le = 0, cfl = 0;
+    int mesh = 0, cfl = 0;
     BB6 nbox, obox;
     bb_reset(nbox);
     bb_reset(obox);
     if (valid) {
       mesh = b.cut_mesh[c];
-      touch = d.cut_touch[c];
-      lb = d.cut_lbeg[c];
-      le = d.cut_lend[c];
       cfl = d.cut_flags[c];
       nbox = bb_load(d.cut_bb + 8 * (size_t)c);
       obox = bb_load(d.cut_obb + 8 * (size_t)c);
     }
-    const SgDabK *kp = &b.tab[(size_t)row * b.M + mesh];
-    int tool = 0, use_orig = 0;
-    float cx = 0.0f, cy = 0.0f, cz = 0.0f, r2 = 0.0f;
+    bool pass = false;
     if (valid) {
-      tool = kp->tool;
-      use_orig = kp->use_orig;
-      cx = kp->cx;
-      cy = kp->cy;
-      cz = kp->cz;
-      r2 = kp->r2;
+      const SgDabK *kp = &b.tab[(size_t)row * b.M + mesh];
+      const int tool = kp->tool;
+      const float cx = kp->cx, cy = kp->cy, cz = kp->cz, r2 = kp->r2;
+      pass = tool != 0 && !(cfl & 1) && sphere_box(kp->use_orig ? obox : nbox, cx, cy, cz, r2);
+      ntests += tool != 0 ? 1 : 0;
     }
-    unsigned long long rm = __ballot(valid && touch == serial - 1);
-    while (rm) {
-      const int j = __ffsll((long long)rm) - 1;
-      rm &= rm - 1;
-      BB6 nb, ob;
-      leaves_box(d.leaf_bb, d.leaf_obb, __shfl(lb, j), __shfl(le, j), nb, ob);
-      const float v[12] = {__shfl(nb.mn[0], 0), __shfl(nb.mn[1], 0), __shfl(nb.mn[2], 0),
-                           __shfl(nb.mx[0], 0), __shfl(nb.mx[1], 0), __shfl(nb.mx[2], 0),
-                           __shfl(ob.mn[0], 0), __shfl(ob.mn[1], 0), __shfl(ob.mn[2], 0),
-                           __shfl(ob.mx[0], 0), __shfl(ob.mx[1], 0), __shfl(ob.mx[2], 0)};
-      if (lane == j) {
-        for (int q = 0; q < 3; q++) {
-          nbox.mn[q] = v[q];
-          nbox.mx[q] = v[3 + q];
-          obox.mn[q] = v[6 + q];
-          obox.mx[q] = v[9 + q];
-        }
-        bb_store(d.cut_bb + 8 * (size_t)c, nbox);
-        bb_store(d.cut_obb + 8 * (size_t)c, obox);
-      }
-    }
-    const bool active = valid && tool != 0;
-    const bool pass = active && !(cfl & 1) && sphere_box(use_orig ? obox : nbox, cx, cy, cz, r2);
-    ntests += active ? 1 : 0;
-    const unsigned long long pm0 = __ballot(pass);
-    if (pm0 == 0ull) continue;
-    // pass 1: count the passing leaves of every passing cut
-    int total = 0;
-    for (unsigned long long pm = pm0; pm;) {
-      const int j = __ffsll((long long)pm) - 1;
-      pm &= pm - 1;
-      const BCut q = bcut_of(j, c0, lb, le, mesh, use_orig, cx, cy, cz, r2);
-      if (lane == 0) ntests += q.le - q.lb;
-      for (int q0 = 0; q0 < q.le - q.lb; q0 += SG_WAVE * BSEARCH_UNROLL) {
-        unsigned long long m[BSEARCH_UNROLL];
-        bleaf_tests(d, q, q0, m);
-#pragma unroll
-        for (int u = 0; u < BSEARCH_UNROLL; u++) total += __popcll(m[u]);
-      }
-    }
-    if (total == 0) continue;
-    int off = 0;
-    if (lane == 0) off = (int)atomicAdd((unsigned long long *)&b.ctr[SB_NGATHER], (unsigned long long)total);
-    off = __shfl(off, 0);
-    // pass 2: the same tests, now writing the reserved slots
-    for (unsigned long long pm = pm0; pm;) {
-      const int j = __ffsll((long long)pm) - 1;
-      pm &= pm - 1;
-      const BCut q = bcut_of(j, c0, lb, le, mesh, use_orig, cx, cy, cz, r2);
-      bool any = false;
-      for (int q0 = 0; q0 < q.le - q.lb; q0 += SG_WAVE * BSEARCH_UNROLL) {
-        unsigned long long m[BSEARCH_UNROLL];
-        bleaf_tests(d, q, q0, m);
-#pragma unroll
-        for (int u = 0; u < BSEARCH_UNROLL; u++) {
-          const int l = q.lb + q0 + u * SG_WAVE + lane;
-          if ((m[u] >> lane) & 1ull) {
-            const int pos = off + mask_prefix(m[u]);
-            d.gathered[pos] = l;
-            d.gl_v[pos] = b.leaf_vr[l];
-            d.gl_s[pos] = b.leaf_shr[l];
-            b.gl_mesh[pos] = q.mesh;
-          }
-          off += __popcll(m[u]);
-          any = any || m[u] != 0ull;
-        }
-      }
-      if (any && lane == 0) d.cut_touch[q.c] = serial;
-    }
+    const unsigned long long m = __ballot(pass);
+    if (m == 0ull) continue;
+    int base = 0;
+    if (lane == 0) base = (int)atomicAdd((unsigned long long *)&b.ctr[SB_NPC], (unsigned long long)__popcll(m));
+    base = __shfl(base, 0);
+    if (pass) b.pc_list[base + mask_prefix(m)] = c;
   }
   ntests = wave_sum_ll(ntests);
+  if (lane == 0 && ntests) atomicAdd((unsigned long long *)&b.ctr[SB_TESTS], (unsigned long long)ntests);
+}
+
+__global__ __launch_bounds__(256) void k_bsearch_leaves(SgBatchView b) {
+  const SgDevView &d = b.d;
+  const int lane = lane_id();
+  const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
+  const int nw = gridDim.x * blockDim.x / SG_WAVE;
+  const int row = b.step[1];
+  const int npc = (int)b.ctr[SB_NPC];
+  long long ntests = 0;
+  for (int i = gw; i < npc; i += nw) {
+    const int c = b.pc_list[i];
+    const int mesh = __builtin_amdgcn_readfirstlane(b.cut_mesh[c]);
+    const int lb = d.cut_lbeg[c], le = d.cut_lend[c];
+    const SgDabK k = b.tab[(size_t)row * b.M + mesh];
+    const float *leaf_bbs = k.use_orig ? d.leaf_obb : d.leaf_bb;
+    ntests += le - lb;
+    for (int q0 = 0; q0 < le - lb; q0 += SG_WAVE * BSEARCH_UNROLL) {
+      unsigned long long m[BSEARCH_UNROLL];
+      int cnt = 0;
+#pragma unroll
+      for (int u = 0; u < BSEARCH_UNROLL; u++) {
+        const int l = lb + q0 + u * SG_WAVE + lane;
+        bool p = false;
+        if (l < le) {
+          const BB6 bx = bb_load(leaf_bbs + 8 * (size_t)l);
+          p = !(d.leaf_flags[l] & 1) && sphere_box(bx, k.cx, k.cy, k.cz, k.r2);
+        }
+        m[u] = __ballot(p);
+        cnt += __popcll(m[u]);
+      }
+      if (cnt == 0) continue;
+      int off = 0;
+      if (lane == 0) off = (int)atomicAdd((unsigned long long *)&b.ctr[SB_NGATHER], (unsigned long long)cnt);
+      int2 vr[BSEARCH_UNROLL], sr[BSEARCH_UNROLL];
+#pragma unroll
+      for (int u = 0; u < BSEARCH_UNROLL; u++) {
+        const int l = lb + q0 + u * SG_WAVE + lane;
+        if ((m[u] >> lane) & 1ull) {
+          vr[u] = b.leaf_vr[l];
+          sr[u] = b.leaf_shr[l];
+        }
+      }
+      off = __shfl(off, 0);
+#pragma unroll
+      for (int u = 0; u < BSEARCH_UNROLL; u++) {
+        const int l = lb + q0 + u * SG_WAVE + lane;
+        if ((m[u] >> lane) & 1ull) {
+          const int pos = off + mask_prefix(m[u]);
+          d.gathered[pos] = l;
+          d.gl_v[pos] = vr[u];
+          d.gl_s[pos] = sr[u];
+          b.gl_mesh[pos] = mesh;
+        }
+        off += __popcll(m[u]);
+      }
+    }
+  }
   if (lane == 0 && ntests) atomicAdd((unsigned long long *)&b.ctr[SB_TESTS], (unsigned long long)ntests);
 }
 
@@ -405,44 +357,64 @@ __global__ __launch_bounds__(256) void k_bpost(SgBatchView b) {
   }
 }
 
-// End of step: per-mesh counters summed into the totals (with the learned decisions),
-// every counter cleared, serial and table row advanced.  One block.
+// End of step.  Every block rebuilds the boxes of its share of this step's listed cuts
+// from their (refit) leaf boxes; the last block to finish then sums the per-mesh
+// counters into the totals (with the learned decisions), clears every counter and
+// advances serial and table row.
 __global__ __launch_bounds__(256) void k_bstep_end(SgBatchView b) {
   __shared__ unsigned long long s_sum[SBT_COUNT];
-  const int t = threadIdx.x;
+  __shared__ int s_last;
+  const SgDevView &d = b.d;
+  const int t = threadIdx.x, lane = lane_id();
+  const int gw = (blockIdx.x * blockDim.x + t) / SG_WAVE;
+  const int nw = gridDim.x * blockDim.x / SG_WAVE;
+  const int npc = (int)b.ctr[SB_NPC];
+  for (int i = gw; i < npc; i += nw) {
+    const int c = b.pc_list[i];
+    BB6 nb, ob;
+    leaves_box(d.leaf_bb, d.leaf_obb, d.cut_lbeg[c], d.cut_lend[c], nb, ob);
+    if (lane == 0) {
+      bb_store(d.cut_bb + 8 * (size_t)c, nb);
+      bb_store(d.cut_obb + 8 * (size_t)c, ob);
+    }
+  }
+  __threadfence();
+  __syncthreads();
+  if (t == 0) s_last = (atomicAdd(b.done, 1u) == gridDim.x - 1) ? 1 : 0;
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
   if (t < SBT_COUNT) s_sum[t] = 0ull;
   __syncthreads();
   const int row = b.step[1];
-  long long aff = 0, cand = 0, degt = 0, nbr = 0, sh = 0, narea = 0, ak = 0, alrn = 0, acc = 0, rej = 0;
+  long long v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int m = t; m < b.M; m += blockDim.x) {
-    const SgDabK k = b.tab[(size_t)row * b.M + m];
+    const SgDabK *kp = &b.tab[(size_t)row * b.M + m];
     const long long *mc = sb_mesh_ctr(b, m);
     const long long na = mc[SBM_NAFF];
-    aff += na;
-    cand += mc[SBM_CAND];
-    degt += mc[SBM_DEGT];
-    sh += mc[SBM_SHARED];
-    narea += mc[SBM_NAREA];
-    if (k.tool == SG_TOOL_SMOOTH) {
-      const bool plain = sb_plain_smooth(b, k, m);
+    v[0] += na;
+    v[1] += mc[SBM_CAND];
+    v[2] += mc[SBM_DEGT];
+    v[4] += mc[SBM_SHARED];
+    v[5] += mc[SBM_NAREA];
+    if (kp->tool == SG_TOOL_SMOOTH) {
+      const bool plain = !kp->learned || sb_learned_rejected(b, m);
       if (plain) {
-        ak += na * k.iters;
-        nbr += mc[SBM_DEG] * k.iters;
+        v[3] += mc[SBM_DEG] * kp->iters;
+        v[6] += na * kp->iters;
       }
-      if (k.learned) {
-        alrn += na;
-        if (plain)
-          rej++;
-        else
-          acc++;
+      if (kp->learned) {
+        v[7] += na;
+        v[plain ? 9 : 8] += 1;
       }
     }
   }
-  const long long vals[10] = {aff, cand, degt, nbr, sh, narea, ak, alrn, acc, rej};
   const int slot[10] = {SBT_AFF, SBT_CAND, SBT_DEGT, SBT_NBR, SBT_SHARED, SBT_NAREA, SBT_AK, SBT_ALRN, SBT_LACC,
                         SBT_LREJ};
-  for (int i = 0; i < 10; i++)
-    if (vals[i]) atomicAdd(&s_sum[slot[i]], (unsigned long long)vals[i]);
+  for (int i = 0; i < 10; i++) {
+    const long long w = wave_sum_ll(v[i]);
+    if (lane == 0 && w) atomicAdd(&s_sum[slot[i]], (unsigned long long)w);
+  }
   __syncthreads();  // every read of this step's counters is done
   if (t == 0) {
     s_sum[SBT_GATHER] += (unsigned long long)b.ctr[SB_NGATHER];
@@ -454,26 +426,9 @@ __global__ __launch_bounds__(256) void k_bstep_end(SgBatchView b) {
   const int n = SB_COUNT + SB_PM * b.M;
   for (int i = t; i < n; i += blockDim.x) b.ctr[i] = 0;
   if (t == 0) {
+    b.done[0] = 0u;
     b.step[0] += 1;
     b.step[1] += 1;
-  }
-}
-
-// Boxes of the cuts the last step touched (so boxes are exact between runs).
-__global__ __launch_bounds__(256) void k_bcut_refresh(SgBatchView b) {
-  const SgDevView &d = b.d;
-  const int lane = lane_id();
-  const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
-  const int nw = gridDim.x * blockDim.x / SG_WAVE;
-  const int last = b.step[0] - 1;
-  for (int c = gw; c < d.C; c += nw) {
-    if (d.cut_touch[c] != last) continue;
-    BB6 nb, ob;
-    leaves_box(d.leaf_bb, d.leaf_obb, d.cut_lbeg[c], d.cut_lend[c], nb, ob);
-    if (lane == 0) {
-      bb_store(d.cut_bb + 8 * (size_t)c, nb);
-      bb_store(d.cut_obb + 8 * (size_t)c, ob);
-    }
   }
 }
 
@@ -529,7 +484,7 @@ void batch_free_device(SgBatch *b) {
   void *ptrs[] = {d.px,       d.py,       d.pz,       d.qx,         d.qy,      d.qz,        d.nx,
                   d.ny,       d.nz,       d.vt4,      d.vt_off,     d.vv_off,  d.vv_idx,    d.leaf_sh,
                   d.leaf_flags, d.leaf_bb, d.leaf_obb, d.cut_bb,    d.cut_obb, d.cut_flags, d.cut_lbeg,
-                  d.cut_lend, d.cut_touch, d.gathered, d.gl_v,      d.gl_s,    d.vperm_old, b->leaf_vr,
+                  d.cut_lend, b->pc_list, b->done, d.gathered, d.gl_v,      d.gl_s,    d.vperm_old, b->leaf_vr,
                   b->leaf_shr, b->cut_mesh, b->gl_mesh, b->vstamp, b->vfade, b->ctr,    b->tot,
                   b->step,    b->tab,     b->gather_buf};
   for (void *q : ptrs)
@@ -555,6 +510,8 @@ SgBatchView sg_batch_view(SgBatch *b) {
   v.gl_mesh = b->gl_mesh;
   v.vstamp = b->vstamp;
   v.vfade = b->vfade;
+  v.pc_list = b->pc_list;
+  v.done = b->done;
   v.ctr = b->ctr;
   v.tot = b->tot;
   v.step = b->step;
@@ -567,8 +524,9 @@ SgBatchView sg_batch_view(SgBatch *b) {
 int sg_batch_enqueue_step(SgBatch *b, int maxk, bool learned) {
   const SgBatchView v = sg_batch_view(b);
   hipStream_t s = b->stream;
-  const int sgrid = std::max(1, std::min(b->g_search, (b->d.C + 3) / 4));
-  hipLaunchKernelGGL(k_bsearch, dim3(sgrid), dim3(256), 0, s, v);
+  const int sgrid = std::max(1, (b->d.C + 255) / 256);  // one lane per cut
+  hipLaunchKernelGGL(k_bsearch_cuts, dim3(sgrid), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_bsearch_leaves, dim3(b->g_search), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_barea, dim3(b->g_brush), dim3(256), 0, s, v);
   if (b->brush_vpl == 2)
     hipLaunchKernelGGL(k_bbrush<2>, dim3(b->g_brush), dim3(b->brush_threads), 0, s, v);
@@ -581,16 +539,10 @@ int sg_batch_enqueue_step(SgBatch *b, int maxk, bool learned) {
   for (int it = 2; it <= maxk; it++) hipLaunchKernelGGL(k_bsmooth_iter, dim3(b->g_smooth), dim3(256), 0, s, v, it);
   if (b->odd_k) hipLaunchKernelGGL(k_bsmooth_sync, dim3(b->g_smooth), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_bpost, dim3(b->g_post), dim3(256), 0, s, v);
-  hipLaunchKernelGGL(k_bstep_end, dim3(1), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_bstep_end, dim3(b->g_end), dim3(256), 0, s, v);
   return sg_check_hip(hipGetLastError(), "batch step launch");
 }
 
-static int batch_cut_refresh(SgBatch *b) {
-  const SgBatchView v = sg_batch_view(b);
-  const int grid = std::max(1, std::min(1024, (b->d.C + 3) / 4));
-  hipLaunchKernelGGL(k_bcut_refresh, dim3(grid), dim3(256), 0, b->stream, v);
-  return sg_check_hip(hipGetLastError(), "cut refresh launch");
-}
 
 extern "C" {
 
@@ -767,7 +719,8 @@ SgBatch *sg_batch_new(int n_meshes, const float *const *co, const int *const *tr
   rc |= dev_fill(&d.nz, V, 0, s);
   rc |= dev_fill(&d.cut_obb, C * 8, 0, s);
   if (!rc) hipMemcpyAsync(d.cut_obb, d.cut_bb, C * 32, hipMemcpyDeviceToDevice, s);
-  rc |= dev_fill(&d.cut_touch, C, 0x80, s);  // never "touched by the previous step"
+  rc |= dev_fill(&b->pc_list, C, 0, s);
+  rc |= dev_fill(&b->done, 1, 0, s);
   rc |= dev_fill(&d.gathered, L, 0, s);
   rc |= dev_fill(&d.gl_v, L, 0, s);
   rc |= dev_fill(&d.gl_s, L, 0, s);
@@ -778,7 +731,7 @@ SgBatch *sg_batch_new(int n_meshes, const float *const *co, const int *const *tr
   rc |= dev_fill(&b->tot, SBT_COUNT, 0, s);
   rc |= dev_fill(&b->step, 2, 0, s);
   if (!rc) {
-    const int st[2] = {1, 0};  // serial starts at 1: the fill value of cut_touch never equals serial - 1
+    const int st[2] = {1, 0};  // serial starts at 1: the fill value of vstamp (-1) is never a serial
     rc = sg_check_hip(hipStreamSynchronize(s), "batch build");
     if (!rc) rc = sg_check_hip(hipMemcpy(b->step, st, sizeof(st), hipMemcpyHostToDevice), "copy");
     if (!rc) rc = sg_enqueue_full_normals_view(sg_view_of(d, true, false), s, 0);
@@ -900,7 +853,6 @@ int sg_batch_launch(SgBatch *b, int steps, int flags) {
       rc = sg_check_hip(hipGraphLaunch(b->exec, b->stream), "graph launch");
   }
   b->row_next += steps;
-  if (!rc) rc = batch_cut_refresh(b);
   if (!rc && !(flags & 2)) rc = sg_check_hip(hipStreamSynchronize(b->stream), "batch run");
   return rc;
 }

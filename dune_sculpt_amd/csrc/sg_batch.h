@@ -21,8 +21,9 @@
 // would bump are kept per mesh instead: returning atomics on one address serialise
 // across the XCDs (~12 ns each), per-mesh addresses spread them out.
 enum {
-  SB_NGATHER = 0,   // gathered leaves (all meshes; one reservation per search wave)
+  SB_NGATHER = 0,   // gathered leaves (all meshes)
   SB_TESTS = 1,     // cut + leaf box tests (one add per search wave)
+  SB_NPC = 2,       // listed (passing) cuts
   SB_COUNT = 4,
 };
 // per-mesh counters, SB_PM entries per mesh after the global ones
@@ -55,6 +56,8 @@ struct SgBatchView {
   int *gl_mesh;             // [L] mesh of each gathered entry
   int *vstamp;              // [V] serial of the last step that affected the vertex
   float *vfade;             // [V] Smooth factor of that step
+  int *pc_list;             // [C] cuts whose box passed this step's sphere test
+  unsigned *done;           // [1] blocks of the step-end kernel that finished
   long long *ctr;           // [SB_COUNT + SB_PM * M]
   long long *tot;           // [SBT_COUNT]
   int *step;                // [2]: serial (monotonic, for the search's touch test), table row
@@ -67,8 +70,9 @@ struct SgBatch {
   hipStream_t stream = nullptr;
   SgDev d;
   int2 *leaf_vr = nullptr, *leaf_shr = nullptr;
-  int *cut_mesh = nullptr, *gl_mesh = nullptr, *vstamp = nullptr;
+  int *cut_mesh = nullptr, *gl_mesh = nullptr, *vstamp = nullptr, *pc_list = nullptr;
   float *vfade = nullptr;
+  unsigned *done = nullptr;
   long long *ctr = nullptr, *tot = nullptr;
   int *step = nullptr;
   SgDabK *tab = nullptr;
@@ -81,7 +85,7 @@ struct SgBatch {
   struct SgMlp *mlp = nullptr;
   // launch shape (autotuned): workgroups of search / brush / smooth+post / learned brush,
   // threads per block of the brush, vertices per lane of the brush
-  int g_search = 1024, g_brush = 1024, g_smooth = 1024, g_post = 1024, g_learned = 512;
+  int g_search = 1024, g_brush = 1024, g_smooth = 1024, g_post = 1024, g_learned = 512, g_end = 256;
   int brush_threads = 256, brush_vpl = 1;
   // captured step graph and the configuration it was captured for
   hipGraphExec_t exec = nullptr;
