@@ -63,11 +63,17 @@ __global__ __launch_bounds__(256) void k_bsearch_cuts(SgBatchView b) {
       obox = bb_load(d.cut_obb + 8 * (size_t)c);
     }
     bool pass = false;
+    int use_orig = 0;
+    float cx = 0.0f, cy = 0.0f, cz = 0.0f, r2 = 0.0f;
     if (valid) {
       const SgDabK *kp = &b.tab[(size_t)row * b.M + mesh];
       const int tool = kp->tool;
-      const float cx = kp->cx, cy = kp->cy, cz = kp->cz, r2 = kp->r2;
-      pass = tool != 0 && !(cfl & 1) && sphere_box(kp->use_orig ? obox : nbox, cx, cy, cz, r2);
+      use_orig = kp->use_orig;
+      cx = kp->cx;
+      cy = kp->cy;
+      cz = kp->cz;
+      r2 = kp->r2;
+      pass = tool != 0 && !(cfl & 1) && sphere_box(use_orig ? obox : nbox, cx, cy, cz, r2);
       ntests += tool != 0 ? 1 : 0;
     }
     const unsigned long long m = __ballot(pass);
@@ -75,68 +81,69 @@ __global__ __launch_bounds__(256) void k_bsearch_cuts(SgBatchView b) {
     int base = 0;
     if (lane == 0) base = (int)atomicAdd((unsigned long long *)&b.ctr[SB_NPC], (unsigned long long)__popcll(m));
     base = __shfl(base, 0);
-    if (pass) b.pc_list[base + mask_prefix(m)] = c;
+    if (pass) {  // everything the leaf pass needs, so it starts with one load
+      const int pos = base + mask_prefix(m);
+      b.pc_list[pos] = make_int4(c, d.cut_lbeg[c], d.cut_lend[c], mesh | (use_orig << 30));
+      b.pc_sph[pos] = make_float4(cx, cy, cz, r2);
+    }
   }
   ntests = wave_sum_ll(ntests);
   if (lane == 0 && ntests) atomicAdd((unsigned long long *)&b.ctr[SB_TESTS], (unsigned long long)ntests);
 }
 
+#define BLEAF_MAX 1024  // leaves per cut (the cut construction caps it)
 __global__ __launch_bounds__(256) void k_bsearch_leaves(SgBatchView b) {
+  __shared__ int s_leaf[4][BLEAF_MAX];
+  __shared__ int s_cnt[4];
+  __shared__ int s_base;
   const SgDevView &d = b.d;
-  const int lane = lane_id();
+  const int lane = lane_id(), w = threadIdx.x / SG_WAVE;
   const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
   const int nw = gridDim.x * blockDim.x / SG_WAVE;
-  const int row = b.step[1];
   const int npc = (int)b.ctr[SB_NPC];
   long long ntests = 0;
-  for (int i = gw; i < npc; i += nw) {
-    const int c = b.pc_list[i];
-    const int mesh = __builtin_amdgcn_readfirstlane(b.cut_mesh[c]);
-    const int lb = d.cut_lbeg[c], le = d.cut_lend[c];
-    const SgDabK k = b.tab[(size_t)row * b.M + mesh];
-    const float *leaf_bbs = k.use_orig ? d.leaf_obb : d.leaf_bb;
-    ntests += le - lb;
-    for (int q0 = 0; q0 < le - lb; q0 += SG_WAVE * BSEARCH_UNROLL) {
-      unsigned long long m[BSEARCH_UNROLL];
-      int cnt = 0;
-#pragma unroll
-      for (int u = 0; u < BSEARCH_UNROLL; u++) {
-        const int l = lb + q0 + u * SG_WAVE + lane;
+  // every wave of the block runs every round (the block reserves its slots together)
+  for (int r0 = blockIdx.x * 4; r0 < npc; r0 += nw) {
+    const int i = r0 + w;
+    int cnt = 0, mesh = 0;
+    if (i < npc) {
+      const int4 e = b.pc_list[i];
+      const float4 sp = b.pc_sph[i];
+      mesh = e.w & 0x3fffffff;
+      const float *leaf_bbs = (e.w >> 30) ? d.leaf_obb : d.leaf_bb;
+      const int lb = e.y, le = e.z;
+      ntests += le - lb;
+      for (int q0 = 0; q0 < le - lb; q0 += SG_WAVE) {
+        const int l = lb + q0 + lane;
         bool p = false;
         if (l < le) {
           const BB6 bx = bb_load(leaf_bbs + 8 * (size_t)l);
-          p = !(d.leaf_flags[l] & 1) && sphere_box(bx, k.cx, k.cy, k.cz, k.r2);
+          p = !(d.leaf_flags[l] & 1) && sphere_box(bx, sp.x, sp.y, sp.z, sp.w);
         }
-        m[u] = __ballot(p);
-        cnt += __popcll(m[u]);
-      }
-      if (cnt == 0) continue;
-      int off = 0;
-      if (lane == 0) off = (int)atomicAdd((unsigned long long *)&b.ctr[SB_NGATHER], (unsigned long long)cnt);
-      int2 vr[BSEARCH_UNROLL], sr[BSEARCH_UNROLL];
-#pragma unroll
-      for (int u = 0; u < BSEARCH_UNROLL; u++) {
-        const int l = lb + q0 + u * SG_WAVE + lane;
-        if ((m[u] >> lane) & 1ull) {
-          vr[u] = b.leaf_vr[l];
-          sr[u] = b.leaf_shr[l];
-        }
-      }
-      off = __shfl(off, 0);
-#pragma unroll
-      for (int u = 0; u < BSEARCH_UNROLL; u++) {
-        const int l = lb + q0 + u * SG_WAVE + lane;
-        if ((m[u] >> lane) & 1ull) {
-          const int pos = off + mask_prefix(m[u]);
-          d.gathered[pos] = l;
-          d.gl_v[pos] = vr[u];
-          d.gl_s[pos] = sr[u];
-          b.gl_mesh[pos] = mesh;
-        }
-        off += __popcll(m[u]);
+        const unsigned long long m = __ballot(p);
+        if (p) s_leaf[w][cnt + mask_prefix(m)] = l;
+        cnt += __popcll(m);
       }
     }
+    if (lane == 0) s_cnt[w] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int tot = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+      s_base = tot ? (int)atomicAdd((unsigned long long *)&b.ctr[SB_NGATHER], (unsigned long long)tot) : 0;
+    }
+    __syncthreads();
+    int off = s_base;
+    for (int q = 0; q < w; q++) off += s_cnt[q];
+    for (int j = lane; j < cnt; j += SG_WAVE) {
+      const int l = s_leaf[w][j];
+      d.gathered[off + j] = l;
+      d.gl_v[off + j] = b.leaf_vr[l];
+      d.gl_s[off + j] = b.leaf_shr[l];
+      b.gl_mesh[off + j] = mesh;
+    }
+    __syncthreads();  // s_leaf / s_cnt are reused by the next round
   }
+  ntests = wave_sum_ll(ntests);
   if (lane == 0 && ntests) atomicAdd((unsigned long long *)&b.ctr[SB_TESTS], (unsigned long long)ntests);
 }
 
@@ -370,20 +377,22 @@ __global__ __launch_bounds__(256) void k_bstep_end(SgBatchView b) {
   const int nw = gridDim.x * blockDim.x / SG_WAVE;
   const int npc = (int)b.ctr[SB_NPC];
   for (int i = gw; i < npc; i += nw) {
-    const int c = b.pc_list[i];
+    const int4 e = b.pc_list[i];
+    const int c = e.x;
     BB6 nb, ob;
-    leaves_box(d.leaf_bb, d.leaf_obb, d.cut_lbeg[c], d.cut_lend[c], nb, ob);
+    leaves_box(d.leaf_bb, d.leaf_obb, e.y, e.z, nb, ob);
     if (lane == 0) {
       bb_store(d.cut_bb + 8 * (size_t)c, nb);
       bb_store(d.cut_obb + 8 * (size_t)c, ob);
     }
   }
-  __threadfence();
+  // The last block reads only what earlier kernels wrote, and clears counters the other
+  // blocks read before their increment, so no fence is needed (a device-scope fence
+  // writes the XCD's L2 back, which costs more than this whole kernel).
   __syncthreads();
   if (t == 0) s_last = (atomicAdd(b.done, 1u) == gridDim.x - 1) ? 1 : 0;
   __syncthreads();
   if (!s_last) return;
-  __threadfence();
   if (t < SBT_COUNT) s_sum[t] = 0ull;
   __syncthreads();
   const int row = b.step[1];
@@ -484,7 +493,7 @@ void batch_free_device(SgBatch *b) {
   void *ptrs[] = {d.px,       d.py,       d.pz,       d.qx,         d.qy,      d.qz,        d.nx,
                   d.ny,       d.nz,       d.vt4,      d.vt_off,     d.vv_off,  d.vv_idx,    d.leaf_sh,
                   d.leaf_flags, d.leaf_bb, d.leaf_obb, d.cut_bb,    d.cut_obb, d.cut_flags, d.cut_lbeg,
-                  d.cut_lend, b->pc_list, b->done, d.gathered, d.gl_v,      d.gl_s,    d.vperm_old, b->leaf_vr,
+                  d.cut_lend, b->pc_list, b->pc_sph, b->done, d.gathered, d.gl_v,      d.gl_s,    d.vperm_old, b->leaf_vr,
                   b->leaf_shr, b->cut_mesh, b->gl_mesh, b->vstamp, b->vfade, b->ctr,    b->tot,
                   b->step,    b->tab,     b->gather_buf};
   for (void *q : ptrs)
@@ -511,6 +520,7 @@ SgBatchView sg_batch_view(SgBatch *b) {
   v.vstamp = b->vstamp;
   v.vfade = b->vfade;
   v.pc_list = b->pc_list;
+  v.pc_sph = b->pc_sph;
   v.done = b->done;
   v.ctr = b->ctr;
   v.tot = b->tot;
@@ -644,6 +654,10 @@ SgBatch *sg_batch_new(int n_meshes, const float *const *co, const int *const *tr
     std::vector<int> cl(C), ce(C), cf(C, 0), cm(C, m);
     std::vector<float> cbb((size_t)C * 8, 0.0f);
     for (int c = 0; c < C; c++) {
+      if (p->cut.lend[c] - p->cut.lbeg[c] > BLEAF_MAX) {  // the cut construction caps it; checked for the LDS list
+        sg_set_error("sg_batch_new: mesh %d has a search cut of %d leaves", m, p->cut.lend[c] - p->cut.lbeg[c]);
+        rc = -1;
+      }
       cl[c] = p->cut.lbeg[c] + (int)lb;
       ce[c] = p->cut.lend[c] + (int)lb;
       const PBVHNode &n = p->nodes[p->cut.node[c]];
@@ -720,6 +734,7 @@ SgBatch *sg_batch_new(int n_meshes, const float *const *co, const int *const *tr
   rc |= dev_fill(&d.cut_obb, C * 8, 0, s);
   if (!rc) hipMemcpyAsync(d.cut_obb, d.cut_bb, C * 32, hipMemcpyDeviceToDevice, s);
   rc |= dev_fill(&b->pc_list, C, 0, s);
+  rc |= dev_fill(&b->pc_sph, C, 0, s);
   rc |= dev_fill(&b->done, 1, 0, s);
   rc |= dev_fill(&d.gathered, L, 0, s);
   rc |= dev_fill(&d.gl_v, L, 0, s);

@@ -56,7 +56,8 @@ struct SgBatchView {
   int *gl_mesh;             // [L] mesh of each gathered entry
   int *vstamp;              // [V] serial of the last step that affected the vertex
   float *vfade;             // [V] Smooth factor of that step
-  int *pc_list;             // [C] cuts whose box passed this step's sphere test
+  int4 *pc_list;            // [C] cuts whose box passed this step's sphere test: cut, leaf range, mesh | orig << 30
+  float4 *pc_sph;           // [C] their dab sphere (centre, r^2)
   unsigned *done;           // [1] blocks of the step-end kernel that finished
   long long *ctr;           // [SB_COUNT + SB_PM * M]
   long long *tot;           // [SBT_COUNT]
@@ -70,7 +71,9 @@ struct SgBatch {
   hipStream_t stream = nullptr;
   SgDev d;
   int2 *leaf_vr = nullptr, *leaf_shr = nullptr;
-  int *cut_mesh = nullptr, *gl_mesh = nullptr, *vstamp = nullptr, *pc_list = nullptr;
+  int *cut_mesh = nullptr, *gl_mesh = nullptr, *vstamp = nullptr;
+  int4 *pc_list = nullptr;
+  float4 *pc_sph = nullptr;
   float *vfade = nullptr;
   unsigned *done = nullptr;
   long long *ctr = nullptr, *tot = nullptr;
