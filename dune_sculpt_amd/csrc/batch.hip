@@ -143,7 +143,6 @@ __global__ __launch_bounds__(256) void k_bsearch_leaves(SgBatchView b) {
     }
     __syncthreads();  // s_leaf / s_cnt are reused by the next round
   }
-  ntests = wave_sum_ll(ntests);
   if (lane == 0 && ntests) atomicAdd((unsigned long long *)&b.ctr[SB_TESTS], (unsigned long long)ntests);
 }
 
@@ -283,13 +282,20 @@ __global__ __launch_bounds__(256) void k_bsmooth_iter(SgBatchView b, int it) {
   float *dz_ = from_shadow ? d.pz : d.qz;
   for (int g = gw; g < ng; g += nw) {
     const int mesh = sb_uniform(b.gl_mesh[g]);
+    const int2 vr = d.gl_v[g];
+    // the first tile's per-vertex loads go out with the dab constants
+    const int v0 = vr.x + lane;
+    const bool in0 = v0 < vr.y;
+    const int st0 = in0 ? b.vstamp[v0] : 0;
+    const int2 vv0 = in0 ? make_int2(d.vv_off[v0], d.vv_off[v0 + 1]) : make_int2(0, 0);
+    const float fade0 = in0 ? b.vfade[v0] : 0.0f;
     const SgDabK k = b.tab[(size_t)row * b.M + mesh];
     if (it > k.iters || !sb_plain_smooth(b, k, mesh)) continue;
-    const int2 vr = d.gl_v[g];
-    for (int v = vr.x + lane; v < vr.y; v += SG_WAVE) {
-      if (b.vstamp[v] != serial) continue;
-      const int2 vv = make_int2(d.vv_off[v], d.vv_off[v + 1]);
-      const float fade = b.vfade[v];
+    for (int v = v0; v < vr.y; v += SG_WAVE) {
+      const bool first = v == v0;
+      if ((first ? st0 : b.vstamp[v]) != serial) continue;
+      const int2 vv = first ? vv0 : make_int2(d.vv_off[v], d.vv_off[v + 1]);
+      const float fade = first ? fade0 : b.vfade[v];
       const float x = sx_[v], y = sy_[v], z = sz_[v];
       float sx, sy, sz;
       nbr_sum(d.vv_idx, vv.x, vv.y, sx_, sy_, sz_, sx, sy, sz);
@@ -349,17 +355,23 @@ __global__ __launch_bounds__(256) void k_bpost(SgBatchView b) {
       if (lane == 0) atomicAdd((unsigned long long *)&sb_mesh_ctr(b, mesh)[SBM_SHARED], as_u64(nsh));
       continue;
     }
+    const int2 vr = d.gl_v[g];
+    const int v0 = vr.x + lane;
+    const bool in0 = v0 < vr.y;
+    const int st0 = in0 ? b.vstamp[v0] : 0;
+    const int2 vt0 = in0 ? make_int2(d.vt_off[v0], d.vt_off[v0 + 1]) : make_int2(0, 0);
     const SgDabK k = b.tab[(size_t)row * b.M + mesh];
     const bool sync_q = (k.iters & 1) == 0 && sb_plain_smooth(b, k, mesh);
-    const int2 vr = d.gl_v[g];
-    for (int v = vr.x + lane; v < vr.y; v += SG_WAVE) {
-      if (b.vstamp[v] != serial) continue;
+    for (int v = v0; v < vr.y; v += SG_WAVE) {
+      const bool first = v == v0;
+      if ((first ? st0 : b.vstamp[v]) != serial) continue;
       if (sync_q) {
         d.qx[v] = d.px[v];
         d.qy[v] = d.py[v];
         d.qz[v] = d.pz[v];
       }
-      vertex_normal(d, v, d.vt_off[v], d.vt_off[v + 1], 0, 0, ntri, nfa, nvfa);
+      const int2 vt = first ? vt0 : make_int2(d.vt_off[v], d.vt_off[v + 1]);
+      vertex_normal(d, v, vt.x, vt.y, 0, 0, ntri, nfa, nvfa);
     }
   }
 }
