@@ -21,11 +21,11 @@ def n_bit_diff(a, b):
 
 
 def run_pair(sg, po, co, tris, dabs, leaf_limit=128, check_each=True, check_bb=True, mask=None,
-             same_start_normals=True):
+             same_start_normals=True, hidden=None, mat=None):
     """Apply dabs one by one on both; assert affected sets and gathered leaves are
     identical per dab and positions/normals agree.  Returns a report dict."""
-    p = sg.PBVH(co, tris, leaf_limit=leaf_limit)
-    o = po.Oracle(co, tris, leaf_limit=leaf_limit, fp64=True)
+    p = sg.PBVH(co, tris, leaf_limit=leaf_limit, hidden=hidden, mat=mat)
+    o = po.Oracle(co, tris, leaf_limit=leaf_limit, fp64=True, hidden=hidden, mat=mat)
     if mask is not None:
         p.set_mask(mask)
         o.set_mask(mask)
