@@ -68,12 +68,12 @@ __global__ __launch_bounds__(256) void k_bsearch_cuts(SgBatchView b) {
     if (valid) {
       const SgDabK *kp = &b.tab[(size_t)row * b.M + mesh];
       const int tool = kp->tool;
-      use_orig = kp->use_orig;
+      use_orig = kp->use_orig | (kp->learned << 1);
       cx = kp->cx;
       cy = kp->cy;
       cz = kp->cz;
       r2 = kp->r2;
-      pass = tool != 0 && !(cfl & 1) && sphere_box(use_orig ? obox : nbox, cx, cy, cz, r2);
+      pass = tool != 0 && !(cfl & 1) && sphere_box((use_orig & 1) ? obox : nbox, cx, cy, cz, r2);
       ntests += tool != 0 ? 1 : 0;
     }
     const unsigned long long m = __ballot(pass);
@@ -83,7 +83,7 @@ __global__ __launch_bounds__(256) void k_bsearch_cuts(SgBatchView b) {
     base = __shfl(base, 0);
     if (pass) {  // everything the leaf pass needs, so it starts with one load
       const int pos = base + mask_prefix(m);
-      b.pc_list[pos] = make_int4(c, d.cut_lbeg[c], d.cut_lend[c], mesh | (use_orig << 30));
+      b.pc_list[pos] = make_int4(c, d.cut_lbeg[c], d.cut_lend[c], mesh | (use_orig << 29));  // bit 29 orig, 30 learned
       b.pc_sph[pos] = make_float4(cx, cy, cz, r2);
     }
   }
@@ -94,8 +94,8 @@ __global__ __launch_bounds__(256) void k_bsearch_cuts(SgBatchView b) {
 #define BLEAF_MAX 1024  // leaves per cut (the cut construction caps it)
 __global__ __launch_bounds__(256) void k_bsearch_leaves(SgBatchView b) {
   __shared__ int s_leaf[4][BLEAF_MAX];
-  __shared__ int s_cnt[4];
-  __shared__ int s_base;
+  __shared__ int s_cnt[4], s_lrn[4];
+  __shared__ int s_base, s_lbase;
   const SgDevView &d = b.d;
   const int lane = lane_id(), w = threadIdx.x / SG_WAVE;
   const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
@@ -106,11 +106,13 @@ __global__ __launch_bounds__(256) void k_bsearch_leaves(SgBatchView b) {
   for (int r0 = blockIdx.x * 4; r0 < npc; r0 += nw) {
     const int i = r0 + w;
     int cnt = 0, mesh = 0;
+    bool lrn = false;
     if (i < npc) {
       const int4 e = b.pc_list[i];
       const float4 sp = b.pc_sph[i];
-      mesh = e.w & 0x3fffffff;
-      const float *leaf_bbs = (e.w >> 30) ? d.leaf_obb : d.leaf_bb;
+      mesh = e.w & 0x1fffffff;
+      lrn = (e.w >> 30) & 1;
+      const float *leaf_bbs = ((e.w >> 29) & 1) ? d.leaf_obb : d.leaf_bb;
       const int lb = e.y, le = e.z;
       ntests += le - lb;
       for (int q0 = 0; q0 < le - lb; q0 += SG_WAVE) {
@@ -125,21 +127,30 @@ __global__ __launch_bounds__(256) void k_bsearch_leaves(SgBatchView b) {
         cnt += __popcll(m);
       }
     }
-    if (lane == 0) s_cnt[w] = cnt;
+    if (lane == 0) {
+      s_cnt[w] = cnt;
+      s_lrn[w] = lrn ? cnt : 0;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
       const int tot = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+      const int totl = s_lrn[0] + s_lrn[1] + s_lrn[2] + s_lrn[3];
       s_base = tot ? (int)atomicAdd((unsigned long long *)&b.ctr[SB_NGATHER], (unsigned long long)tot) : 0;
+      s_lbase = totl ? (int)atomicAdd((unsigned long long *)&b.ctr[SB_NGL], (unsigned long long)totl) : 0;
     }
     __syncthreads();
-    int off = s_base;
-    for (int q = 0; q < w; q++) off += s_cnt[q];
+    int off = s_base, loff = s_lbase;
+    for (int q = 0; q < w; q++) {
+      off += s_cnt[q];
+      loff += s_lrn[q];
+    }
     for (int j = lane; j < cnt; j += SG_WAVE) {
       const int l = s_leaf[w][j];
       d.gathered[off + j] = l;
       d.gl_v[off + j] = b.leaf_vr[l];
       d.gl_s[off + j] = b.leaf_shr[l];
       b.gl_mesh[off + j] = mesh;
+      if (lrn) b.gl_lrn[loff + j] = off + j;  // the learned brush's own list
     }
     __syncthreads();  // s_leaf / s_cnt are reused by the next round
   }
@@ -505,7 +516,7 @@ void batch_free_device(SgBatch *b) {
   void *ptrs[] = {d.px,       d.py,       d.pz,       d.qx,         d.qy,      d.qz,        d.nx,
                   d.ny,       d.nz,       d.vt4,      d.vt_off,     d.vv_off,  d.vv_idx,    d.leaf_sh,
                   d.leaf_flags, d.leaf_bb, d.leaf_obb, d.cut_bb,    d.cut_obb, d.cut_flags, d.cut_lbeg,
-                  d.cut_lend, b->pc_list, b->pc_sph, b->done, d.gathered, d.gl_v,      d.gl_s,    d.vperm_old, b->leaf_vr,
+                  d.cut_lend, b->pc_list, b->pc_sph, b->gl_lrn, b->done, d.gathered, d.gl_v,      d.gl_s,    d.vperm_old, b->leaf_vr,
                   b->leaf_shr, b->cut_mesh, b->gl_mesh, b->vstamp, b->vfade, b->ctr,    b->tot,
                   b->step,    b->tab,     b->gather_buf};
   for (void *q : ptrs)
@@ -533,6 +544,7 @@ SgBatchView sg_batch_view(SgBatch *b) {
   v.vfade = b->vfade;
   v.pc_list = b->pc_list;
   v.pc_sph = b->pc_sph;
+  v.gl_lrn = b->gl_lrn;
   v.done = b->done;
   v.ctr = b->ctr;
   v.tot = b->tot;
@@ -747,6 +759,7 @@ SgBatch *sg_batch_new(int n_meshes, const float *const *co, const int *const *tr
   if (!rc) hipMemcpyAsync(d.cut_obb, d.cut_bb, C * 32, hipMemcpyDeviceToDevice, s);
   rc |= dev_fill(&b->pc_list, C, 0, s);
   rc |= dev_fill(&b->pc_sph, C, 0, s);
+  rc |= dev_fill(&b->gl_lrn, L, 0, s);
   rc |= dev_fill(&b->done, 1, 0, s);
   rc |= dev_fill(&d.gathered, L, 0, s);
   rc |= dev_fill(&d.gl_v, L, 0, s);

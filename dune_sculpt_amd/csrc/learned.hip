@@ -438,7 +438,7 @@ __global__ __launch_bounds__(LRN_THREADS) void k_bbrush_learned(SgBatchView b, S
   __shared__ __align__(16) uint16_t bufB[LRN_WAVES][LRN_ROWS * MLP_ST];
   __shared__ __align__(16) float sOut[LRN_WAVES][LRN_ROWS * 4];
   const SgDevView &d = b.d;
-  const int nitems = LRN_SLOTS * (int)b.ctr[SB_NGATHER];
+  const int nitems = LRN_SLOTS * (int)b.ctr[SB_NGL];  // leaves of learned meshes only
   if (blockIdx.x * LRN_WAVES >= nitems) return;
   mlp_lds_load<LRN_THREADS>(s, m);
   __syncthreads();
@@ -447,10 +447,9 @@ __global__ __launch_bounds__(LRN_THREADS) void k_bbrush_learned(SgBatchView b, S
   const int nw = gridDim.x * LRN_WAVES;
   const int serial = b.step[0], row = b.step[1];
   for (int it = gw; it < nitems; it += nw) {
-    const int g = it / LRN_SLOTS;
+    const int g = b.gl_lrn[it / LRN_SLOTS];
     const int mesh = __builtin_amdgcn_readfirstlane(b.gl_mesh[g]);
     const SgDabK k = b.tab[(size_t)row * b.M + mesh];
-    if (!k.learned) continue;
     const int2 vr = d.gl_v[g];
     long long esum = 0, degt = 0;
     int nood = 0, naff = 0;

@@ -24,6 +24,7 @@ enum {
   SB_NGATHER = 0,   // gathered leaves (all meshes)
   SB_TESTS = 1,     // cut + leaf box tests (one add per search wave)
   SB_NPC = 2,       // listed (passing) cuts
+  SB_NGL = 3,       // gathered leaves of learned-Smooth meshes (the learned brush's list)
   SB_COUNT = 4,
 };
 // per-mesh counters, SB_PM entries per mesh after the global ones
@@ -58,6 +59,7 @@ struct SgBatchView {
   float *vfade;             // [V] Smooth factor of that step
   int4 *pc_list;            // [C] cuts whose box passed this step's sphere test: cut, leaf range, mesh | orig << 30
   float4 *pc_sph;           // [C] their dab sphere (centre, r^2)
+  int *gl_lrn;              // [L] positions in the gathered list of leaves of learned-Smooth meshes
   unsigned *done;           // [1] blocks of the step-end kernel that finished
   long long *ctr;           // [SB_COUNT + SB_PM * M]
   long long *tot;           // [SBT_COUNT]
@@ -74,6 +76,7 @@ struct SgBatch {
   int *cut_mesh = nullptr, *gl_mesh = nullptr, *vstamp = nullptr;
   int4 *pc_list = nullptr;
   float4 *pc_sph = nullptr;
+  int *gl_lrn = nullptr;
   float *vfade = nullptr;
   unsigned *done = nullptr;
   long long *ctr = nullptr, *tot = nullptr;
@@ -88,7 +91,7 @@ struct SgBatch {
   struct SgMlp *mlp = nullptr;
   // launch shape (autotuned): workgroups of search / brush / smooth+post / learned brush,
   // threads per block of the brush, vertices per lane of the brush
-  int g_search = 1024, g_brush = 1024, g_smooth = 1024, g_post = 1024, g_learned = 512, g_end = 256;
+  int g_search = 1024, g_brush = 1024, g_smooth = 1024, g_post = 1024, g_learned = 1024, g_end = 256;
   int brush_threads = 256, brush_vpl = 1;
   // captured step graph and the configuration it was captured for
   hipGraphExec_t exec = nullptr;
