@@ -434,6 +434,7 @@ int sg_dab_apply(PBVH *p, const SgDabParams *dab) {
   }
   for (int i = 0; i < 8; i++) p->last_stats[i] = 0;
   p->last_tool = dab->sculpt_tool;
+  p->last_learned = k.learned && p->mlp;
   p->dab_count++;
   p->nodes_dirty_deferred = false;
   invalidate_host_views(p);
@@ -541,9 +542,10 @@ int sg_dab_apply_stroke(PBVH *p, const SgDabParams *dabs, int n, int flags) {
     long long ctr[SG_CTR_COUNT];
     hipMemcpy(ctr, p->d.ctr + (size_t)((n - 1 - a0) & 1) * SG_CTR_COUNT, sizeof(ctr), hipMemcpyDeviceToHost);
     memcpy(p->last_ctr, ctr, sizeof(ctr));
+    p->last_learned = p->mlp && dabs[n - 1].sculpt_tool == SG_TOOL_SMOOTH && (dabs[n - 1].flag & SG_SMOOTH_LEARNED);
+    p->last_tool = dabs[n - 1].sculpt_tool;
   }
-  p->last_tool = dabs[n - 1].sculpt_tool;
-  p->dab_count += n;
+  p->dab_count += n - nfallback;  // the fallbacks counted themselves
   p->nodes_dirty_deferred = false;
   invalidate_host_views(p);
   return 0;
@@ -646,12 +648,23 @@ int sg_stroke_launch(SgStroke *s, int sync) {
       const int f = h - 1;
       SgDabParams plain = s->dabs[f];
       plain.flag &= ~SG_SMOOTH_LEARNED;
+      p->dab_count += f;  // the graph's dabs before it; the calls below count themselves
       rc = sg_dab_apply(p, &plain);
       p->learned_fallbacks++;
       if (!rc && f + 1 < s->n) rc = sg_dab_apply_stroke(p, s->dabs.data() + f + 1, s->n - f - 1, s->flags & ~1);
+      invalidate_host_views(p);
+      return rc;
     }
   }
   p->dab_count += s->n;
+  if (!rc && (sync || s->learned)) {  // the stream is idle: record the last dab's counters
+    const SgDabParams &last = s->dabs[s->n - 1];
+    rc = sg_check_hip(hipMemcpy(p->last_ctr, p->d.ctr + (size_t)((s->n - 1) & 1) * SG_CTR_COUNT,
+                                SG_CTR_COUNT * sizeof(long long), hipMemcpyDeviceToHost),
+                      "copy");
+    p->last_tool = last.sculpt_tool;
+    p->last_learned = s->learned && last.sculpt_tool == SG_TOOL_SMOOTH && (last.flag & SG_SMOOTH_LEARNED);
+  }
   invalidate_host_views(p);
   return rc;
 }
@@ -900,10 +913,31 @@ int sg_sync(PBVH *p) {
 int sg_dab_last_affected(PBVH *p, int *out, int cap) {
   if (!p) return sg_set_error("null PBVH");
   if (p->no_device) return sg_set_error("no HIP device");
-  const int na = (int)p->last_ctr[SG_CTR_NAFF];
+  int na = (int)p->last_ctr[SG_CTR_NAFF];
   std::vector<int> tmp(std::max(na, 1));
-  if (na > 0 && sg_check_hip(hipMemcpy(tmp.data(), p->d.aff_v, na * sizeof(int), hipMemcpyDeviceToHost), "copy"))
+  if (p->last_learned) {
+    // an accepted learned dab stamps its vertices instead of listing them: the
+    // stamped vertices of the gathered leaves' ranges
+    const int ng = (int)p->last_ctr[SG_CTR_NGATHER];
+    std::vector<int2> gv(std::max(ng, 1));
+    std::vector<int> stamp((size_t)std::max(p->d.V, 1));
+    int seq = 0;
+    if (ng > 0 && sg_check_hip(hipMemcpy(gv.data(), p->d.gl_v, ng * sizeof(int2), hipMemcpyDeviceToHost), "copy"))
+      return -1;
+    if (sg_check_hip(hipMemcpy(stamp.data(), p->d.vstamp, (size_t)p->d.V * sizeof(int), hipMemcpyDeviceToHost),
+                     "copy") ||
+        sg_check_hip(hipMemcpy(&seq, p->d.seq, sizeof(int), hipMemcpyDeviceToHost), "copy"))
+      return -1;
+    tmp.clear();
+    for (int g = 0; g < ng; g++)
+      for (int v = gv[g].x; v < gv[g].y; v++)
+        if (stamp[v] == seq) tmp.push_back(v);
+    na = (int)tmp.size();
+    tmp.resize(std::max(na, 1));
+  } else if (na > 0 &&
+             sg_check_hip(hipMemcpy(tmp.data(), p->d.aff_v, na * sizeof(int), hipMemcpyDeviceToHost), "copy")) {
     return -1;
+  }
   for (int i = 0; i < na; i++) tmp[i] = p->perm_old_of_new[tmp[i]];
   std::sort(tmp.begin(), tmp.begin() + na);
   for (int i = 0; i < na && i < cap; i++) out[i] = tmp[i];

@@ -136,6 +136,7 @@ struct PBVH {
   long long last_ctr[24] = {0};
   struct SgMlp *mlp = nullptr;       // learned Smooth model (borrowed)
   long long learned_accepted = 0, learned_fallbacks = 0;
+  bool last_learned = false;         // the last dab was an accepted learned dab (affected = stamped vertices)
   int dab_count = 0;
   int grid_wg = 256;                 // workgroups of the per-dab kernels (tunable)
   int grid_brush = 0, grid_post = 0, grid_smooth = 0;  // per-kernel overrides (autotuner)

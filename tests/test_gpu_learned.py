@@ -150,3 +150,24 @@ def test_judge_against_plain_brush(sg):
     assert not acc and e == 0.0 and n > 0
     p.close()
     q.close()
+
+
+def test_learned_dab_reports_its_affected_set(sg):
+    """An accepted learned dab stamps its vertices instead of listing them; the API
+    still returns the affected set, which equals the plain brush's from the same state."""
+    L, m, co, tris, dabs = _small_setup(sg)
+    alpha, _ = m.decision()
+    m.set_decision(alpha, 1.0)
+    ld = L.learned_dabs(dabs[:3])
+    for d, l in zip(dabs[:3], ld):
+        p = sg.PBVH(co, tris, leaf_limit=128)
+        q = sg.PBVH(co, tris, leaf_limit=128)
+        p.set_learned(m)
+        p.dab(l)
+        q.dab(d)
+        assert p.learned_stats()["accepted"] == 1
+        a, b = p.last_affected(), q.last_affected()
+        assert len(a) > 0 and np.array_equal(a, b)
+        assert np.array_equal(p.last_gathered(), q.last_gathered())
+        p.close()
+        q.close()
