@@ -8,14 +8,15 @@
 //
 // Step (one captured hipGraph, launched once per step; every kernel reads the step's
 // table row and serial from device memory, so the same graph serves every step):
-//   k_bsearch       wave per search cut of every mesh -> gathered leaves (+ mesh)
-//   k_barea         Draw meshes: area normal per mesh (exact fixed-point sums)
-//   k_bbrush        wave per gathered leaf: displacement + affected records (+ mesh)
+//   k_bsearch_cuts   lane per search cut of every mesh -> passing cuts
+//   k_bsearch_leaves wave per passing cut -> gathered leaves (+ mesh)
+//   k_barea          Draw meshes: area normal per mesh (exact fixed-point sums)
+//   k_bbrush         wave per gathered leaf: displacement, affected vertices stamped
 //   k_bbrush_learned / k_bdecide   learned Smooth meshes (learned.hip)
-//   k_bsmooth_iter  Jacobi steps 2..k of Smooth records
-//   k_bsmooth_sync  odd k: final iterate back to the positions
-//   k_bpost         normals of affected vertices, bounds of gathered leaves
-//   k_bstep_end     totals, decisions, counters cleared, step advanced
+//   k_bsmooth_iter   Jacobi steps 2..k of the stamped vertices of Smooth meshes
+//   k_bsmooth_sync   odd k: final iterate back to the positions
+//   k_bpost          normals of stamped vertices, bounds of gathered leaves
+//   k_bstep_end      cut boxes rebuilt; totals, decisions, counters cleared, step advanced
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -916,8 +917,8 @@ int sg_batch_sync(SgBatch *b) {
 
 // totals since the last reset: [0..9] gathered leaves, affected vertices, box tests,
 // candidate vertices, vertex->tri degrees of affected vertices, Smooth neighbour reads,
-// shared vertices refit, area-normal vertices, plain Smooth records times k, learned
-// Smooth records; [12] learned dabs accepted, [13] rejected (redone plain), [14] steps.
+// shared vertices refit, area-normal vertices, plain Smooth vertices times k, learned
+// Smooth vertices; [12] learned dabs accepted, [13] rejected (redone plain), [14] steps.
 int sg_batch_totals(SgBatch *b, long long *out16, int reset) {
   if (batch_check(b)) return -1;
   int rc = sg_check_hip(hipMemcpyAsync(out16, b->tot, SBT_COUNT * 8, hipMemcpyDeviceToHost, b->stream), "copy");

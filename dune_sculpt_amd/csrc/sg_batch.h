@@ -3,8 +3,8 @@
 // The meshes are concatenated into one device arena: every per-vertex, per-tri,
 // per-leaf and per-cut array of the single-mesh layout, with indices offset to the
 // arena, so a batch is a forest of PBVHs over one disconnected mesh.  Each kernel of a
-// step covers all meshes at once: work items (cuts, gathered leaves, affected records)
-// carry their mesh, whose dab constants come from a [step][mesh] table in HBM.
+// step covers all meshes at once: work items (cuts, gathered leaves) carry their
+// mesh, whose dab constants come from a [step][mesh] table in HBM.
 // Affected vertices are not listed: the brush stamps each with the step's serial and
 // later kernels walk the gathered leaves' vertex ranges testing the stamp, so no
 // kernel reserves list slots with atomics.

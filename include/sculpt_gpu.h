@@ -328,7 +328,7 @@ int sg_batch_sync(SgBatch *batch);
 /* Totals since the last reset: [0] gathered leaves, [1] affected vertices, [2] box
  * tests, [3] candidate vertices, [4] vertex->tri degrees of affected vertices, [5]
  * Smooth neighbour reads, [6] shared vertices refit, [7] area-normal vertices, [8]
- * plain Smooth records times k, [9] learned Smooth records, [12] learned dabs
+ * plain Smooth vertices times k, [9] learned Smooth vertices, [12] learned dabs
  * accepted, [13] rejected (redone by the plain brush in the same step), [14] steps. */
 int sg_batch_totals(SgBatch *batch, long long out[16], int reset);
 int sg_batch_read_positions(SgBatch *batch, int mesh, float *out);
