@@ -140,12 +140,10 @@ int upload_host_leaf_verts(PBVH *p, const std::vector<int> &leaves) {
       y[q] = co[1];
       z[q] = co[2];
     }
-    hipMemcpy(p->d.px + b, x.data(), n * 4, hipMemcpyHostToDevice);
-    hipMemcpy(p->d.py + b, y.data(), n * 4, hipMemcpyHostToDevice);
-    hipMemcpy(p->d.pz + b, z.data(), n * 4, hipMemcpyHostToDevice);
-    hipMemcpy(p->d.qx + b, x.data(), n * 4, hipMemcpyHostToDevice);
-    hipMemcpy(p->d.qy + b, y.data(), n * 4, hipMemcpyHostToDevice);
-    hipMemcpy(p->d.qz + b, z.data(), n * 4, hipMemcpyHostToDevice);
+    float *dst[6] = {p->d.px, p->d.py, p->d.pz, p->d.qx, p->d.qy, p->d.qz};
+    const float *src[6] = {x.data(), y.data(), z.data(), x.data(), y.data(), z.data()};
+    for (int a = 0; a < 6; a++)
+      if (int rc = sg_check_hip(hipMemcpy(dst[a] + b, src[a], n * 4, hipMemcpyHostToDevice), "copy")) return rc;
     i = j + 1;
   }
   return 0;
@@ -796,11 +794,18 @@ int sg_pbvh_learned_stats(PBVH *p, long long out[2]) {
 int sg_learned_collect(PBVH *p, const SgDabParams *dab, float *X, float *T, float *ok, int cap) {
   if (int rc = check_dab(p, dab)) return rc;
   if (dab->sculpt_tool != SG_TOOL_SMOOTH) return sg_set_error("collection needs a Smooth dab");
+  if (cap <= 0 || !X || !T || !ok) return sg_set_error("bad arguments");
   SgCollect col;
   col.cap = cap;
-  if (sg_check_hip(hipMalloc(&col.X, (size_t)std::max(cap, 1) * 28 * 4), "hipMalloc")) return -1;
-  hipMalloc(&col.frame, (size_t)std::max(cap, 1) * 16 * 4);
-  hipMalloc(&col.T, (size_t)std::max(cap, 1) * 3 * 4);
+  col.X = col.frame = col.T = nullptr;
+  int arc = sg_check_hip(hipMalloc(&col.X, (size_t)cap * 28 * 4), "hipMalloc");
+  if (!arc) arc = sg_check_hip(hipMalloc(&col.frame, (size_t)cap * 16 * 4), "hipMalloc");
+  if (!arc) arc = sg_check_hip(hipMalloc(&col.T, (size_t)cap * 3 * 4), "hipMalloc");
+  if (arc) {
+    if (col.X) hipFree(col.X);
+    if (col.frame) hipFree(col.frame);
+    return arc;
+  }
   SgDabParams plain = *dab;
   plain.flag &= ~SG_SMOOTH_LEARNED;
   SgDabK k = sg_make_dab(&plain, 0);
@@ -815,11 +820,14 @@ int sg_learned_collect(PBVH *p, const SgDabParams *dab, float *X, float *T, floa
   if (!rc) {
     na = (int)std::min<long long>(ctr[SG_CTR_NAFF], cap);
     std::vector<float> fr((size_t)std::max(na, 1) * 16);
-    hipMemcpy(X, col.X, (size_t)na * 28 * 4, hipMemcpyDeviceToHost);
-    hipMemcpy(T, col.T, (size_t)na * 3 * 4, hipMemcpyDeviceToHost);
-    hipMemcpy(fr.data(), col.frame, (size_t)na * 16 * 4, hipMemcpyDeviceToHost);
+    rc = sg_check_hip(hipMemcpy(X, col.X, (size_t)na * 28 * 4, hipMemcpyDeviceToHost), "copy");
+    if (!rc) rc = sg_check_hip(hipMemcpy(T, col.T, (size_t)na * 3 * 4, hipMemcpyDeviceToHost), "copy");
+    if (!rc) rc = sg_check_hip(hipMemcpy(fr.data(), col.frame, (size_t)na * 16 * 4, hipMemcpyDeviceToHost), "copy");
     for (int i = 0; i < na; i++) ok[i] = fr[(size_t)i * 16 + 13];
     memcpy(p->last_ctr, ctr, sizeof(ctr));
+    p->last_tool = SG_TOOL_SMOOTH;
+    p->last_learned = false;
+    p->dab_count++;
   }
   hipFree(col.X);
   hipFree(col.frame);

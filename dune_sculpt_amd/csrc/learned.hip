@@ -735,8 +735,8 @@ static int mlp_export(SgMlp *m) {
     bias[64 + j] = w[SG_OFF_B2 + j];
   }
   for (int o = 0; o < 3; o++) bias[128 + o] = w[SG_OFF_B3 + o];
-  hipMemcpy(m->wb, wb.data(), wb.size() * 2, hipMemcpyHostToDevice);
-  hipMemcpy(m->bias, bias.data(), bias.size() * 4, hipMemcpyHostToDevice);
+  if (sg_check_hip(hipMemcpy(m->wb, wb.data(), wb.size() * 2, hipMemcpyHostToDevice), "copy")) return -1;
+  if (sg_check_hip(hipMemcpy(m->bias, bias.data(), bias.size() * 4, hipMemcpyHostToDevice), "copy")) return -1;
   return sg_check_hip(hipMemcpy(m->norm, m->host_norm, sizeof(m->host_norm), hipMemcpyHostToDevice), "copy");
 }
 
@@ -750,12 +750,17 @@ SgMlp *sg_mlp_new(uint64_t seed) {
     return nullptr;
   }
   SgMlp *m = new SgMlp();
-  hipMalloc(&m->w, SG_MLP_NPARAM * 4);
-  hipMalloc(&m->mo, SG_MLP_NPARAM * 4);
-  hipMalloc(&m->ve, SG_MLP_NPARAM * 4);
-  hipMalloc(&m->wb, (64 * 32 + 64 * 64 + 16 * 64) * 2);
-  hipMalloc(&m->bias, 144 * 4);
-  hipMalloc(&m->norm, 64 * 4);
+  int rc = sg_check_hip(hipMalloc(&m->w, SG_MLP_NPARAM * 4), "hipMalloc");
+  if (!rc) rc = sg_check_hip(hipMalloc(&m->mo, SG_MLP_NPARAM * 4), "hipMalloc");
+  if (!rc) rc = sg_check_hip(hipMalloc(&m->ve, SG_MLP_NPARAM * 4), "hipMalloc");
+  if (!rc) rc = sg_check_hip(hipMalloc(&m->wb, (64 * 32 + 64 * 64 + 16 * 64) * 2), "hipMalloc");
+  if (!rc) rc = sg_check_hip(hipMalloc(&m->bias, 144 * 4), "hipMalloc");
+  if (!rc) rc = sg_check_hip(hipMalloc(&m->norm, 64 * 4), "hipMalloc");
+  if (!rc) rc = sg_check_hip(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking), "stream");
+  if (rc) {
+    sg_mlp_free(m);
+    return nullptr;
+  }
   hipMemset(m->mo, 0, SG_MLP_NPARAM * 4);
   hipMemset(m->ve, 0, SG_MLP_NPARAM * 4);
   // He-style init from a seeded generator
@@ -765,21 +770,23 @@ SgMlp *sg_mlp_new(uint64_t seed) {
   for (int i = 0; i < 28 * 64; i++) w[SG_OFF_W1 + i] = nd(rng) * sqrtf(2.0f / 28.0f);
   for (int i = 0; i < 64 * 64; i++) w[SG_OFF_W2 + i] = nd(rng) * sqrtf(2.0f / 64.0f);
   for (int i = 0; i < 64 * 3; i++) w[SG_OFF_W3 + i] = nd(rng) * sqrtf(1.0f / 64.0f);
-  hipMemcpy(m->w, w.data(), SG_MLP_NPARAM * 4, hipMemcpyHostToDevice);
   for (int i = 0; i < 64; i++) m->host_norm[i] = 0.0f;
   for (int i = 0; i < 28; i++) m->host_norm[28 + i] = 1.0f;
   for (int o = 0; o < 3; o++) m->host_norm[56 + o] = 1.0f;
-  hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
-  mlp_export(m);
+  if (sg_check_hip(hipMemcpy(m->w, w.data(), SG_MLP_NPARAM * 4, hipMemcpyHostToDevice), "copy") || mlp_export(m)) {
+    sg_mlp_free(m);
+    return nullptr;
+  }
   return m;
 }
 
 void sg_mlp_free(SgMlp *m) {
   if (!m) return;
+  if (m->stream) hipStreamSynchronize(m->stream);
   void *ptrs[] = {m->w, m->mo, m->ve, m->wb, m->bias, m->norm, m->grad_part, m->loss_part, m->dX, m->dT, m->perm};
   for (void *q : ptrs)
     if (q) hipFree(q);
-  hipStreamDestroy(m->stream);
+  if (m->stream) hipStreamDestroy(m->stream);
   delete m;
 }
 
@@ -828,8 +835,8 @@ int sg_mlp_set_data(SgMlp *m, const float *X, const float *T, int n) {
   if (sg_check_hip(hipMalloc(&m->dX, (size_t)n * 28 * 4), "hipMalloc")) return -1;
   if (sg_check_hip(hipMalloc(&m->dT, (size_t)n * 3 * 4), "hipMalloc")) return -1;
   if (sg_check_hip(hipMalloc(&m->perm, (size_t)n * 4), "hipMalloc")) return -1;
-  hipMemcpy(m->dX, X, (size_t)n * 28 * 4, hipMemcpyHostToDevice);
-  hipMemcpy(m->dT, T, (size_t)n * 3 * 4, hipMemcpyHostToDevice);
+  if (sg_check_hip(hipMemcpy(m->dX, X, (size_t)n * 28 * 4, hipMemcpyHostToDevice), "copy")) return -1;
+  if (sg_check_hip(hipMemcpy(m->dT, T, (size_t)n * 3 * 4, hipMemcpyHostToDevice), "copy")) return -1;
   m->n = n;
   return 0;
 }
@@ -852,9 +859,9 @@ int sg_mlp_gradient(SgMlp *m, int n, float *grad_out, float *loss_out) {
   if (ensure_parts(m, nwg)) return -1;
   std::vector<int> id(m->n);
   for (int i = 0; i < m->n; i++) id[i] = i;
-  hipMemcpy(m->perm, id.data(), (size_t)m->n * 4, hipMemcpyHostToDevice);
+  if (sg_check_hip(hipMemcpy(m->perm, id.data(), (size_t)m->n * 4, hipMemcpyHostToDevice), "copy")) return -1;
   float *dg = nullptr;
-  hipMalloc(&dg, SG_MLP_NPARAM * 4);
+  if (sg_check_hip(hipMalloc(&dg, SG_MLP_NPARAM * 4), "hipMalloc")) return -1;
   hipLaunchKernelGGL(k_mlp_train, dim3(nwg), dim3(256), 0, m->stream, m->w, m->dX, m->dT, m->perm, n, 0,
                      1.0f / (float)(n * 3), m->norm, m->grad_part, m->loss_part);
   hipLaunchKernelGGL(k_mlp_adam, dim3((SG_MLP_NPARAM + 255) / 256), dim3(256), 0, m->stream, m->w, m->mo, m->ve,
@@ -916,8 +923,11 @@ int sg_mlp_train(SgMlp *m, int epochs, int batch_wg, float lr, uint64_t seed, fl
 int sg_mlp_predict(SgMlp *m, const float *X, int n, float *Y) {
   if (!m || n <= 0) return sg_set_error("bad arguments");
   float *dX = nullptr, *dY = nullptr;
-  hipMalloc(&dX, (size_t)n * 28 * 4);
-  hipMalloc(&dY, (size_t)n * 3 * 4);
+  if (sg_check_hip(hipMalloc(&dX, (size_t)n * 28 * 4), "hipMalloc")) return -1;
+  if (sg_check_hip(hipMalloc(&dY, (size_t)n * 3 * 4), "hipMalloc")) {
+    hipFree(dX);
+    return -1;
+  }
   hipMemcpyAsync(dX, X, (size_t)n * 28 * 4, hipMemcpyHostToDevice, m->stream);
   const int blocks = (n + LRN_ROWS * LRN_WAVES - 1) / (LRN_ROWS * LRN_WAVES);
   hipLaunchKernelGGL(k_mlp_predict, dim3(blocks < 4096 ? blocks : 4096), dim3(LRN_THREADS), 0, m->stream,
@@ -930,16 +940,17 @@ int sg_mlp_predict(SgMlp *m, const float *X, int n, float *Y) {
 }
 
 int sg_mfma_probe(const uint16_t *A, const uint16_t *B, float *D) {
-  uint16_t *dA, *dB;
-  float *dD;
-  hipMalloc(&dA, 512);
-  hipMalloc(&dB, 512);
-  hipMalloc(&dD, 1024);
-  hipMemcpy(dA, A, 512, hipMemcpyHostToDevice);
-  hipMemcpy(dB, B, 512, hipMemcpyHostToDevice);
-  hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, 0, dA, dB, dD);
-  hipMemcpy(D, dD, 1024, hipMemcpyDeviceToHost);
-  int rc = sg_check_hip(hipDeviceSynchronize(), "probe");
+  uint16_t *dA = nullptr, *dB = nullptr;
+  float *dD = nullptr;
+  int rc = sg_check_hip(hipMalloc(&dA, 512), "hipMalloc");
+  if (!rc) rc = sg_check_hip(hipMalloc(&dB, 512), "hipMalloc");
+  if (!rc) rc = sg_check_hip(hipMalloc(&dD, 1024), "hipMalloc");
+  if (!rc) rc = sg_check_hip(hipMemcpy(dA, A, 512, hipMemcpyHostToDevice), "copy");
+  if (!rc) rc = sg_check_hip(hipMemcpy(dB, B, 512, hipMemcpyHostToDevice), "copy");
+  if (!rc) {
+    hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, 0, dA, dB, dD);
+    rc = sg_check_hip(hipMemcpy(D, dD, 1024, hipMemcpyDeviceToHost), "probe");
+  }
   hipFree(dA);
   hipFree(dB);
   hipFree(dD);
