@@ -200,12 +200,6 @@ void BKE_pbvh_free(PBVH *p) {
   free(p->looptri);
   for (int i = 0; i < 2; i++)
     if (p->timer[i]) hipEventDestroy(p->timer[i]);
-  if (p->ev_fork) hipEventDestroy(p->ev_fork);
-  if (p->ev_join) hipEventDestroy(p->ev_join);
-  if (p->stream2) {
-    hipStreamSynchronize(p->stream2);
-    hipStreamDestroy(p->stream2);
-  }
   if (!p->no_device) hipStreamDestroy(p->stream);
   delete p;
 }
@@ -544,7 +538,7 @@ int sg_dab_apply_stroke(PBVH *p, const SgDabParams *dabs, int n, int flags) {
   p->learned_accepted += nlearned - nfallback;
   if (a0 < n) {  // otherwise the last dab was a plain fallback and set last_ctr itself
     long long ctr[SG_CTR_COUNT];
-    hipMemcpy(ctr, p->d.ctr + (size_t)((n - 1 - a0) % SG_CTR_SETS) * SG_CTR_COUNT, sizeof(ctr), hipMemcpyDeviceToHost);
+    hipMemcpy(ctr, p->d.ctr + (size_t)((n - 1 - a0) & 1) * SG_CTR_COUNT, sizeof(ctr), hipMemcpyDeviceToHost);
     memcpy(p->last_ctr, ctr, sizeof(ctr));
     p->last_learned = p->mlp && dabs[n - 1].sculpt_tool == SG_TOOL_SMOOTH && (dabs[n - 1].flag & SG_SMOOTH_LEARNED);
     p->last_tool = dabs[n - 1].sculpt_tool;
@@ -663,7 +657,7 @@ int sg_stroke_launch(SgStroke *s, int sync) {
   p->dab_count += s->n;
   if (!rc && (sync || s->learned)) {  // the stream is idle: record the last dab's counters
     const SgDabParams &last = s->dabs[s->n - 1];
-    rc = sg_check_hip(hipMemcpy(p->last_ctr, p->d.ctr + (size_t)((s->n - 1) % SG_CTR_SETS) * SG_CTR_COUNT,
+    rc = sg_check_hip(hipMemcpy(p->last_ctr, p->d.ctr + (size_t)((s->n - 1) & 1) * SG_CTR_COUNT,
                                 SG_CTR_COUNT * sizeof(long long), hipMemcpyDeviceToHost),
                       "copy");
     p->last_tool = last.sculpt_tool;

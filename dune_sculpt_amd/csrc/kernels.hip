@@ -62,9 +62,9 @@ __global__ __launch_bounds__(256) void k_search(SgDevView d, SgDabK k, long long
   const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
   const int nw = gridDim.x * blockDim.x / SG_WAVE;
   if (gw == 0 && lane < SG_CTR_COUNT) {
-    const long long prev = d.ctr_prev[lane];  // the set the previous dab used
+    const long long prev = d.ctr_next[lane];  // the set the previous dab used
     if (log && log_index > 0) log[(size_t)(log_index - 1) * SG_CTR_COUNT + lane] = prev;
-    d.ctr_next[lane] = 0;  // the next dab's set (three sets: the previous dab's normals may still run)
+    d.ctr_next[lane] = 0;
   }
   if (gw == 0 && lane == 0 && d.seq) d.seq[0] += 1;  // this dab's stamp for the later kernels
   const float *cut_bbs = k.use_orig ? d.cut_obb : d.cut_bb;
@@ -148,9 +148,9 @@ __global__ __launch_bounds__(256) void k_search(SgDevView d, SgDabK k, long long
   }
 }
 
-// zero every counter set and the halt flag at the start of a stroke
+// zero both counter sets and the halt flag at the start of a stroke
 __global__ void k_clear_ctr(long long *ctr, int *halt) {
-  for (int i = threadIdx.x; i < SG_CTR_SETS * SG_CTR_COUNT; i += blockDim.x) ctr[i] = 0;
+  if (threadIdx.x < 2 * SG_CTR_COUNT) ctr[threadIdx.x] = 0;
   if (threadIdx.x == 0) halt[0] = 0;
 }
 
@@ -358,24 +358,20 @@ __global__ __launch_bounds__(256) void k_smooth_sync(SgDevView d, int from_shado
 // K5: post.  Normals of the affected vertices and bounds of the gathered leaves (all
 // gathered leaves are marked, BKE_pbvh_node_mark_update).  Both only read positions.
 // ---------------------------------------------------------------------------
-// parts: bit 0 = normals (and the even-k shadow refresh), bit 1 = leaf bounds.  The
-// two can run as separate launches: the refit on the dab stream (the next dab's
-// search needs it), the normals on the side stream alongside that search.
-__global__ __launch_bounds__(256) void k_post(SgDevView d, int stats, int stamp, int sync_q, int parts) {
+__global__ __launch_bounds__(256) void k_post(SgDevView d, int stats, int stamp, int sync_q) {
   if (d.halt && d.halt[0]) return;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int nthreads = gridDim.x * blockDim.x;
   const int gw = tid / SG_WAVE;
   const int nw = nthreads / SG_WAVE;
   // speculative first-record loads alongside the counters
-  const bool pn = (parts & 1) != 0, pr = (parts & 2) != 0;
-  const int v0 = pn && tid < d.V ? d.aff_v[tid] : 0;
-  const int2 vt0 = pn && tid < d.V ? d.aff_vt[tid] : make_int2(0, 0);
-  const int leaf0 = pr && gw < d.L ? d.gathered[gw] : 0;
-  const int2 vr0 = pr && gw < d.L ? d.gl_v[gw] : make_int2(0, 0);
-  const int2 sr0 = pr && gw < d.L ? d.gl_s[gw] : make_int2(0, 0);
-  const int na = pn ? (int)d.ctr[SG_CTR_NAFF] : 0;
-  const int ng = pr ? (int)d.ctr[SG_CTR_NGATHER] : 0;
+  const int v0 = tid < d.V ? d.aff_v[tid] : 0;
+  const int2 vt0 = tid < d.V ? d.aff_vt[tid] : make_int2(0, 0);
+  const int leaf0 = gw < d.L ? d.gathered[gw] : 0;
+  const int2 vr0 = gw < d.L ? d.gl_v[gw] : make_int2(0, 0);
+  const int2 sr0 = gw < d.L ? d.gl_s[gw] : make_int2(0, 0);
+  const int na = (int)d.ctr[SG_CTR_NAFF];
+  const int ng = (int)d.ctr[SG_CTR_NGATHER];
   long long ntri = 0, nfa = 0, nvfa = 0, nsh = 0;
   for (int i = tid; i < na; i += nthreads) {
     const int v = (i == tid) ? v0 : d.aff_v[i];
@@ -688,7 +684,7 @@ __global__ void k_log_row(const long long *ctr, long long *log, int row) {
 }
 
 int sg_enqueue_log_row(PBVH *p, long long *log, int row) {
-  hipLaunchKernelGGL(k_log_row, dim3(1), dim3(64), 0, p->stream, p->d.ctr + (size_t)(row % SG_CTR_SETS) * SG_CTR_COUNT, log,
+  hipLaunchKernelGGL(k_log_row, dim3(1), dim3(64), 0, p->stream, p->d.ctr + (size_t)(row & 1) * SG_CTR_COUNT, log,
                      row);
   return sg_check_hip(hipGetLastError(), "log launch");
 }
@@ -762,7 +758,6 @@ SgDevView sg_view_of(SgDev &d, bool respect_hide, bool halt_live) {
   v.prev_cuts = d.prev_cuts;
   v.ctr = d.ctr;
   v.ctr_next = d.ctr + SG_CTR_COUNT;
-  v.ctr_prev = d.ctr + 2 * SG_CTR_COUNT;
   v.cut_touch = d.cut_touch;
   v.halt = halt_live ? d.halt : nullptr;  // no flag test in strokes without learned dabs
   v.seq = d.seq;
@@ -865,9 +860,8 @@ int sg_enqueue_clear_ctr(PBVH *p) {
 int sg_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long long *log, int log_index,
                    int stamp, int serial, const SgCollect *col) {
   SgDevView v = sg_view(p);
-  v.ctr = p->d.ctr + (size_t)(serial % SG_CTR_SETS) * SG_CTR_COUNT;
-  v.ctr_next = p->d.ctr + (size_t)((serial + 1) % SG_CTR_SETS) * SG_CTR_COUNT;
-  v.ctr_prev = p->d.ctr + (size_t)((serial + SG_CTR_SETS - 1) % SG_CTR_SETS) * SG_CTR_COUNT;
+  v.ctr = p->d.ctr + (size_t)(serial & 1) * SG_CTR_COUNT;
+  v.ctr_next = p->d.ctr + (size_t)((serial + 1) & 1) * SG_CTR_COUNT;
   hipStream_t s = p->stream;
   const int G = p->grid_wg;
   const int GB = p->grid_brush > 0 ? p->grid_brush : G;
@@ -878,7 +872,6 @@ int sg_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long long *
     const int sg = (p->d.C + 3) / 4;
     SG_LAUNCH(ps, k_search, dim3(sg > 0 ? sg : 1), dim3(256), 0, s, v, k, log, log_index, serial);
   }
-  sg_join_side(p);  // the previous dab's normals are read (area, Inflate) and its vertices move from here
   if (k.learned && p->mlp) {  // learned Smooth: brush + MLP, then decide / commit
     {
       ProfScope ps(p, SG_K_BRUSH);
@@ -913,39 +906,17 @@ int sg_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long long *
       SG_LAUNCH(ps, k_smooth_sync, dim3(GS), dim3(256), 0, s, v, 1);
     }
   }
-  const int sync_q = (k.tool == SG_TOOL_SMOOTH && k.iters % 2 == 0) ? 1 : 0;
-  if (!k.stats && !col && !p->profile && p->overlap_normals && sg_ensure_side(p) == 0) {
-    // normals on the side stream, overlapping the refit and the next dab's search
-    hipEventRecord(p->ev_fork, s);
-    hipStreamWaitEvent(p->stream2, p->ev_fork, 0);
-    hipLaunchKernelGGL(k_post, dim3(GP), dim3(256), 0, p->stream2, v, 0, stamp, sync_q, 1);
-    hipEventRecord(p->ev_join, p->stream2);
-    p->join_pending = true;
-    hipLaunchKernelGGL(k_post, dim3(GP), dim3(256), 0, s, v, 0, stamp, 0, 2);
-  } else {
+  {
     ProfScope ps(p, SG_K_POST);
-    SG_LAUNCH(ps, k_post, dim3(GP), dim3(256), 0, s, v, k.stats, stamp, sync_q, 3);
+    SG_LAUNCH(ps, k_post, dim3(GP), dim3(256), 0, s, v, k.stats, stamp,
+              (k.tool == SG_TOOL_SMOOTH && k.iters % 2 == 0) ? 1 : 0);
   }
   if (col) sg_enqueue_target(p, v, col->frame, col->T, col->cap);
   if (p->profile && !p->capturing && p->ev_kid.size() > 4096) sg_profile_collect(p);
   return sg_check_hip(hipGetLastError(), "dab launch");
 }
 
-int sg_ensure_side(PBVH *p) {
-  if (p->stream2) return 0;
-  if (sg_check_hip(hipStreamCreateWithFlags(&p->stream2, hipStreamNonBlocking), "side stream")) return -1;
-  if (sg_check_hip(hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming), "event")) return -1;
-  return sg_check_hip(hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming), "event");
-}
-
-void sg_join_side(PBVH *p) {
-  if (!p->join_pending) return;
-  hipStreamWaitEvent(p->stream, p->ev_join, 0);
-  p->join_pending = false;
-}
-
 int sg_enqueue_flush(PBVH *p) {
-  sg_join_side(p);  // a stroke (and a captured graph) ends with the side stream joined
   const SgDevView v = sg_view(p);
   hipStream_t s = p->stream;
   const size_t sub_lds = (size_t)p->d.sub_maxn * 6 * sizeof(float);

@@ -682,7 +682,7 @@ void sg_build_device(PBVH *p, const float *co) {
   alloc_fill(&d.aff_vt, V, 0, s);
   alloc_fill(&d.aff_vv, V, 0, s);
   alloc_fill(&d.aff_fade, V, 0, s);
-  alloc_fill(&d.ctr, SG_CTR_SETS * SG_CTR_COUNT, 0, s);
+  alloc_fill(&d.ctr, 2 * SG_CTR_COUNT, 0, s);
   alloc_fill(&d.halt, 4, 0, s);
   alloc_fill(&d.seq, 1, 0, s);
   alloc_fill(&d.vstamp, V, 0xFF, s);  // -1: never a dab's stamp (stamps start at 1)

@@ -101,7 +101,6 @@ enum {
   SG_CTR_NBR = 9,
   SG_CTR_TRIS = 10,
   SG_CTR_COUNT = 24,
-  SG_CTR_SETS = 3,  // counter sets in rotation (dab serial mod 3)
 };
 
 struct PBVH {
@@ -157,11 +156,6 @@ struct PBVH {
   long long prof_n[16] = {0};
   int stamp = 0;                     // stats-mode stamp generator
   hipEvent_t timer[2] = {nullptr, nullptr};  // sg_pbvh_timer_start/stop
-  // side stream: a dab's normals overlap its refit and the next dab's search
-  hipStream_t stream2 = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  bool join_pending = false;
-  bool overlap_normals = true;
   bool halt_live = false;            // the work being enqueued contains learned dabs (kernels test the halt flag)
 };
 

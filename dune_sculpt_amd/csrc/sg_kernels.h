@@ -43,7 +43,7 @@ struct SgDevView {
   int *cut_touch;
   int *halt;
   int *seq, *vstamp;
-  long long *ctr, *ctr_next, *ctr_prev;
+  long long *ctr, *ctr_next;
   const int *vperm_old;
   int *stamp_v, *stamp_v2, *stamp_t, *stamp_a;
 };
@@ -71,8 +71,6 @@ int sg_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long long *
                    int stamp, int serial, const SgCollect *col = nullptr);
 int sg_enqueue_clear_ctr(PBVH *p);
 int sg_enqueue_flush(PBVH *p);
-int sg_ensure_side(PBVH *p);
-void sg_join_side(PBVH *p);
 int sg_enqueue_log_row(PBVH *p, long long *log, int row);
 int sg_enqueue_full_normals(PBVH *p, int mode);
 int sg_enqueue_refit_list(PBVH *p, const int *leaves_dev, int n);
