@@ -69,6 +69,19 @@ class SgDabParams(C.Structure):
     ]
 
 
+class SgRay(C.Structure):
+    _fields_ = [("start", C.c_float * 3), ("normal", C.c_float * 3), ("depth", C.c_float), ("reserved", C.c_int)]
+
+
+class SgRayHit(C.Structure):
+    _fields_ = [("hit", C.c_int), ("depth", C.c_float), ("location", C.c_float * 3), ("face_normal", C.c_float * 3),
+                ("active_vertex", C.c_int), ("active_face", C.c_int), ("looptri", C.c_int), ("node", C.c_int),
+                ("status", C.c_int), ("reserved", C.c_int)]
+
+
+RAY_OVERFLOW = -2
+FLT_MAX = float(np.finfo(np.float32).max)
+
 SearchCB = C.CFUNCTYPE(C.c_bool, C.c_void_p, C.c_void_p)
 HitCB = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p)
 
@@ -101,7 +114,7 @@ EXPORTS = [
     "sg_batch_rewind", "sg_batch_run", "sg_batch_sync", "sg_batch_totals", "sg_batch_read_positions",
     "sg_batch_read_normals", "sg_batch_snapshot_save", "sg_batch_snapshot_restore", "sg_batch_set_launch",
     "sg_batch_get_launch", "sg_batch_autotune", "sg_batch_set_learned", "sg_batch_timer_start",
-    "sg_batch_timer_stop", "sg_hbm_copy_probe",
+    "sg_batch_timer_stop", "sg_hbm_copy_probe", "sg_raycast", "sg_raycast_device", "sg_raycast_stats",
 ]
 SMOOTH_LEARNED = 1 << 30
 NCTR = 24
@@ -218,6 +231,9 @@ def lib():
         "sg_stroke_launch": (i32, [vp, i32]),
         "sg_stroke_free": (None, [vp]),
         "sg_pbvh_profile_read": (i32, [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong), i32, i32]),
+        "sg_raycast": (i32, [vp, C.POINTER(SgRay), i32, i32, C.POINTER(SgRayHit)]),
+        "sg_raycast_device": (i32, [vp, vp, i32, i32, vp]),
+        "sg_raycast_stats": (i32, [vp, C.POINTER(SgRay), i32, i32, C.POINTER(C.c_longlong)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -708,6 +724,21 @@ class PBVH:
         lib().sg_pbvh_prim_indices(self.h, _ip(out), self.tottri)
         return out
 
+    def raycast(self, starts, normals, depths=None, original=False):
+        """Rays (n x 3 starts, n x 3 directions, optional initial depths) -> structured
+        array of hits (sg_raycast)."""
+        rays = make_rays(starts, normals, depths)
+        hits = (SgRayHit * len(rays))()
+        _check(lib().sg_raycast(self.h, rays, len(rays), 1 if original else 0, hits), "sg_raycast")
+        return hits_to_numpy(hits)
+
+    def raycast_stats(self, starts, normals, depths=None, original=False):
+        """Work counters of a ray batch: boxes tested, leaves scanned, tris tested."""
+        rays = make_rays(starts, normals, depths)
+        out = (C.c_longlong * 3)()
+        _check(lib().sg_raycast_stats(self.h, rays, len(rays), 1 if original else 0, out), "sg_raycast_stats")
+        return {"boxes": int(out[0]), "leaves": int(out[1]), "tris": int(out[2])}
+
     def search_gather(self, callback=None):
         """BKE_pbvh_search_gather with a Python callback(node_index) -> bool."""
         L = lib()
@@ -722,6 +753,29 @@ class PBVH:
         if tot.value:
             C.CDLL(None).free(arr)
         return res
+
+
+def make_rays(starts, normals, depths=None):
+    starts = np.asarray(starts, np.float32).reshape(-1, 3)
+    normals = np.asarray(normals, np.float32).reshape(-1, 3)
+    n = starts.shape[0]
+    dep = np.full(n, FLT_MAX, np.float32) if depths is None else np.broadcast_to(np.asarray(depths, np.float32), (n,))
+    rays = (SgRay * n)()
+    a = np.frombuffer(rays, np.float32).reshape(n, 8)
+    a[:, 0:3] = starts
+    a[:, 3:6] = normals
+    a[:, 6] = dep
+    a[:, 7] = 0.0
+    return rays
+
+
+RAY_HIT_DTYPE = np.dtype([("hit", "<i4"), ("depth", "<f4"), ("location", "<f4", 3), ("face_normal", "<f4", 3),
+                          ("active_vertex", "<i4"), ("active_face", "<i4"), ("looptri", "<i4"), ("node", "<i4"),
+                          ("status", "<i4"), ("reserved", "<i4")])
+
+
+def hits_to_numpy(hits):
+    return np.frombuffer(hits, RAY_HIT_DTYPE).copy()
 
 
 # ---------------------------------------------------------------------------

@@ -700,7 +700,8 @@ void sg_free_device(PBVH *p) {
                   d.sub_off, d.sub_tab, d.top_tab_n, d.top_tab, d.cut_dirty, d.vhide, d.vmark,
                   d.gathered, d.gl_v, d.gl_s, d.aff_v, d.aff_vt, d.aff_vv, d.aff_fade,
                   d.prev_cuts, d.cut_touch, d.halt, d.seq, d.vstamp, d.ctr, d.err, d.vperm_old, d.stamp_v, d.stamp_v2, d.stamp_t,
-                  d.stamp_a, d.curve, d.log};
+                  d.stamp_a, d.curve, d.log, d.ltri, d.leaf_toff, d.tri_poly, d.node_parent, d.leaf_node,
+                  d.ray_in, d.ray_out, d.place, d.place_rays};
   for (void *q : ptrs) {
     if (q) hipFree(q);
   }

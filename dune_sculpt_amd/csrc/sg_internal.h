@@ -86,6 +86,16 @@ struct SgDev {
   int curve_cap = 0;
   long long *log = nullptr;     // per-dab counter log for strokes
   int log_cap = 0;
+  // ray cast (built on first use): tris in leaf order (new corner ids, .w = looptri |
+  // hidden << 31), each leaf's range, looptri -> poly, tree links
+  int4 *ltri = nullptr;
+  int *leaf_toff = nullptr, *tri_poly = nullptr, *node_parent = nullptr, *leaf_node = nullptr;
+  SgRay *ray_in = nullptr;      // staging for host-side ray lists
+  SgRayHit *ray_out = nullptr;
+  int ray_cap = 0;
+  float *place = nullptr;       // [8 per dab] placements of a ray-placed stroke
+  SgRay *place_rays = nullptr;  // the stroke's rays
+  int place_cap = 0;
 };
 
 enum {
@@ -176,6 +186,12 @@ void sg_build_cut(PBVH *p, std::vector<int> &lbeg, std::vector<int> &lend, std::
                   std::vector<int> &leaf_cut);
 void sg_build_device(PBVH *p, const float *co);
 void sg_free_device(PBVH *p);
+
+// ray cast (ray.hip)
+int sg_ray_ensure(PBVH *p);
+int sg_enqueue_raycast(PBVH *p, const SgRay *rays_dev, int n, int original, int stale, SgRayHit *out_dev,
+                       float *place_dev, int grab_pair, long long *stats_dev);
+bool sg_ray_host_needs_refit(const SgRay *rays, int n);
 
 // errors (kernels.hip)
 int sg_set_error(const char *fmt, ...);

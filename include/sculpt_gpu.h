@@ -272,6 +272,41 @@ PBVHNode *sg_pbvh_node_at(PBVH *pbvh, int index);
 /* prim_indices array (looptri order of the leaves) */
 int sg_pbvh_prim_indices(PBVH *pbvh, int *out, int cap);
 
+/* ---- ray cast (stroke placement) ----------------------------------------------
+ * One ray = BKE_pbvh_raycast (pbvh.c:3915-3928) with the sculpt operator's hit
+ * callback (sculpt.c is absent: leaves visited in ascending box-entry tmin, each
+ * tested while its tmin is below the running bound, FLT_MAX then the hit depth) and
+ * BKE_pbvh_node_raycast -> pbvh_faces_node_raycast (:4041-4107, :4203-4260) on the
+ * current positions; the hit point is ray_start + ray_normal * depth (the stroke's
+ * location, SCULPT_stroke_get_location in the absent sculpt.c).  Bit-identical to
+ * that sequential visit. */
+typedef struct SgRay {
+  float start[3];  /* object space */
+  float normal[3]; /* direction (unit in the reference's callers) */
+  float depth;     /* initial depth: only hits strictly nearer count (FLT_MAX: none) */
+  int reserved;
+} SgRay;
+typedef struct SgRayHit {
+  int hit;              /* a triangle was hit */
+  float depth;          /* ray parameter of the hit (the initial depth when none) */
+  float location[3];    /* start + normal * depth */
+  float face_normal[3]; /* normal_tri_v3 of the hit triangle (pbvh.c:4073) */
+  int active_vertex;    /* its corner nearest the hit (pbvh.c:4076-4101), mesh index */
+  int active_face;      /* MLoopTri.poly of the hit triangle */
+  int looptri;          /* looptri index of the hit triangle */
+  int node;             /* PBVH node index of its leaf */
+  int status;           /* 0, or SG_RAY_OVERFLOW (more than 1024 leaves hit: no result) */
+  int reserved;
+} SgRayHit;
+enum { SG_RAY_OVERFLOW = -2 };
+/* Rays from host memory, results to host memory; original != 0 tests orig_vb
+ * (BKE_pbvh_raycast's `original`). */
+int sg_raycast(PBVH *pbvh, const SgRay *rays, int n, int original, SgRayHit *hits);
+/* The same with device pointers, queued on the PBVH's stream (sg_sync waits). */
+int sg_raycast_device(PBVH *pbvh, const SgRay *rays_dev, int n, int original, SgRayHit *hits_dev);
+/* Work counters of a ray batch: [0] boxes tested [1] leaves scanned [2] tris tested. */
+int sg_raycast_stats(PBVH *pbvh, const SgRay *rays, int n, int original, long long out[3]);
+
 /* ---- learned Smooth (MLP over 1-ring features; BF16 MFMA inference) ----------
  * A Smooth dab with SG_SMOOTH_LEARNED in SgDabParams.flag goes through the model
  * attached to the PBVH; when its error estimate (alpha * RMS(|d|)/r) exceeds the

@@ -6,6 +6,7 @@
  * written (the reference build is SSE2-only and never fuses, CMakeLists.txt:703-768).
  */
 #include "oracle.h"
+#include "oracle_int.h"
 
 #include <float.h>
 #include <math.h>
@@ -99,12 +100,6 @@ static void tri_normal64(const double v1[3], const double v2[3], const double v3
 /* Bounding boxes: pbvh.c:55-105.                                            */
 /* ------------------------------------------------------------------------- */
 
-typedef struct {
-  float bmin[3], bmax[3];
-} OBB;
-typedef struct {
-  float bmin[3], bmax[3], bcentroid[3];
-} OBBC;
 
 static inline void bb_reset(OBB *bb)
 {
@@ -141,33 +136,6 @@ static int bb_widest_axis(const OBB *bb)
 /* Tree.                                                                     */
 /* ------------------------------------------------------------------------- */
 
-typedef struct {
-  OBB vb, orig_vb;
-  int children_offset;
-  int prim_off, totprim;
-  int *vert_indices;
-  int uniq_verts, face_verts;
-  int (*face_vert_indices)[3];
-  int flag;
-} ONode;
-
-struct OrcPBVH {
-  int totvert, tottri, leaf_limit, respect_hide, with_fp64;
-  float *co, *no, *mask;
-  double *co64, *no64;
-  int *tris, *tri_mat;
-  uint8_t *vhide;
-  ONode *nodes;
-  int totnode, node_mem;
-  int *prim_indices;
-  uint8_t *vert_bitmap;
-  /* vertex -> incident tris (ascending tri id) and vertex -> edge neighbours
-   * (ascending vertex id): the 1-ring sources (mesh_mapping.c:182-295). */
-  int *vt_off, *vt_idx, *vv_off, *vv_idx;
-  /* build scratch */
-  int *map_local, *map_touched;
-  int64_t stats[8];
-};
 
 static int g_threads = 0;
 void orc_set_threads(int n) { g_threads = n; }
@@ -1047,6 +1015,77 @@ static int cmp_int(const void *pa, const void *pb)
 
 static inline float mask_of(const OrcPBVH *o, int v) { return o->mask ? o->mask[v] : 0.0f; }
 
+/* BKE_pbvh_update_bounds(UpdateBB|UpdateOriginalBB|UpdateRedraw): pbvh.c:1401-1421 ->
+ * :1206-1242 -> update_node_vb :108-128 -> flush :1369-1399. */
+static void update_bounds(OrcPBVH *o, int nthreads)
+{
+  int *upd;
+  int flag = ORC_UpdateBB | ORC_UpdateOriginalBB | ORC_UpdateRedraw;
+  const int nupd = search_gather(o, update_cb, &flag, &upd, NULL);
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
+  for (int li = 0; li < nupd; li++) {
+    ONode *n = &o->nodes[upd[li]];
+    if (n->flag & ORC_UpdateBB) {
+      OBB vb;
+      bb_reset(&vb);
+      const int tot = n->uniq_verts + n->face_verts;
+      for (int k = 0; k < tot; k++) {
+        bb_expand(&vb, o->co + 3 * n->vert_indices[k]);
+      }
+      n->vb = vb;
+    }
+    if (n->flag & ORC_UpdateOriginalBB) {
+      n->orig_vb = n->vb;
+    }
+    n->flag &= ~ORC_UpdateRedraw;
+  }
+  free(upd);
+  /* pbvh_flush_bb: serial recursion over the whole tree (explicit stack,
+   * children before parents). */
+  int *st = (int *)malloc(sizeof(int) * 2 * (size_t)(o->totnode + 2));
+  int *upd_of = (int *)calloc((size_t)o->totnode, sizeof(int));
+  int ssz = 0;
+  st[ssz++] = 0;
+  st[ssz++] = 0;
+  while (ssz) {
+    const int stage = st[--ssz];
+    const int ni = st[--ssz];
+    ONode *n = &o->nodes[ni];
+    if (n->flag & ORC_Leaf) {
+      int u = 0;
+      u |= n->flag & ORC_UpdateBB;
+      n->flag &= ~ORC_UpdateBB;
+      u |= n->flag & ORC_UpdateOriginalBB;
+      n->flag &= ~ORC_UpdateOriginalBB;
+      upd_of[ni] = u;
+      continue;
+    }
+    if (stage == 0) {
+      st[ssz++] = ni;
+      st[ssz++] = 1;
+      st[ssz++] = n->children_offset + 1;
+      st[ssz++] = 0;
+      st[ssz++] = n->children_offset;
+      st[ssz++] = 0;
+      continue;
+    }
+    const int u = upd_of[n->children_offset] | upd_of[n->children_offset + 1];
+    if (u & ORC_UpdateBB) {
+      OBB vb;
+      bb_reset(&vb);
+      bb_expand_bb(&vb, &o->nodes[n->children_offset].vb);
+      bb_expand_bb(&vb, &o->nodes[n->children_offset + 1].vb);
+      n->vb = vb;
+    }
+    if (u & ORC_UpdateOriginalBB) {
+      n->orig_vb = n->vb;
+    }
+    upd_of[ni] = u;
+  }
+  free(st);
+  free(upd_of);
+}
+
 int orc_dab(OrcPBVH *o, const OrcDab *dab, int *out_affected, int cap)
 {
   memset(o->stats, 0, sizeof(o->stats));
@@ -1389,75 +1428,8 @@ int orc_dab(OrcPBVH *o, const OrcDab *dab, int *out_affected, int cap)
     free(upd);
   }
 
-  /* Bounds: BKE_pbvh_update_bounds(UpdateBB|UpdateOriginalBB|UpdateRedraw),
-   * pbvh.c:1401-1421 -> :1206-1242 -> update_node_vb :108-128 -> flush :1369-1399. */
-  {
-    int *upd;
-    int flag = ORC_UpdateBB | ORC_UpdateOriginalBB | ORC_UpdateRedraw;
-    const int nupd = search_gather(o, update_cb, &flag, &upd, NULL);
-#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
-    for (int li = 0; li < nupd; li++) {
-      ONode *n = &o->nodes[upd[li]];
-      if (n->flag & ORC_UpdateBB) {
-        OBB vb;
-        bb_reset(&vb);
-        const int tot = n->uniq_verts + n->face_verts;
-        for (int k = 0; k < tot; k++) {
-          bb_expand(&vb, o->co + 3 * n->vert_indices[k]);
-        }
-        n->vb = vb;
-      }
-      if (n->flag & ORC_UpdateOriginalBB) {
-        n->orig_vb = n->vb;
-      }
-      n->flag &= ~ORC_UpdateRedraw;
-    }
-    free(upd);
-    /* pbvh_flush_bb: serial recursion over the whole tree (explicit stack,
-     * children before parents). */
-    int *st = (int *)malloc(sizeof(int) * 2 * (size_t)(o->totnode + 2));
-    int *upd_of = (int *)calloc((size_t)o->totnode, sizeof(int));
-    int ssz = 0;
-    st[ssz++] = 0;
-    st[ssz++] = 0;
-    while (ssz) {
-      const int stage = st[--ssz];
-      const int ni = st[--ssz];
-      ONode *n = &o->nodes[ni];
-      if (n->flag & ORC_Leaf) {
-        int u = 0;
-        u |= n->flag & ORC_UpdateBB;
-        n->flag &= ~ORC_UpdateBB;
-        u |= n->flag & ORC_UpdateOriginalBB;
-        n->flag &= ~ORC_UpdateOriginalBB;
-        upd_of[ni] = u;
-        continue;
-      }
-      if (stage == 0) {
-        st[ssz++] = ni;
-        st[ssz++] = 1;
-        st[ssz++] = n->children_offset + 1;
-        st[ssz++] = 0;
-        st[ssz++] = n->children_offset;
-        st[ssz++] = 0;
-        continue;
-      }
-      const int u = upd_of[n->children_offset] | upd_of[n->children_offset + 1];
-      if (u & ORC_UpdateBB) {
-        OBB vb;
-        bb_reset(&vb);
-        bb_expand_bb(&vb, &o->nodes[n->children_offset].vb);
-        bb_expand_bb(&vb, &o->nodes[n->children_offset + 1].vb);
-        n->vb = vb;
-      }
-      if (u & ORC_UpdateOriginalBB) {
-        n->orig_vb = n->vb;
-      }
-      upd_of[ni] = u;
-    }
-    free(st);
-    free(upd_of);
-  }
+  /* Bounds: BKE_pbvh_update_bounds(UpdateBB|UpdateOriginalBB|UpdateRedraw). */
+  update_bounds(o, nthreads);
 
   /* output */
   if (out_affected) {
@@ -1493,4 +1465,22 @@ int orc_leaves_dfs(const OrcPBVH *o, int *out, int cap)
   }
   free(arr);
   return tot;
+}
+
+/* Replace every position (e.g. with another engine's result) and refit all node
+ * bounds as a dab does (orig_vb follows vb). */
+void orc_set_positions(OrcPBVH *o, const float *co)
+{
+  memcpy(o->co, co, sizeof(float) * 3 * (size_t)o->totvert);
+  if (o->co64) {
+    for (size_t i = 0; i < 3 * (size_t)o->totvert; i++) {
+      o->co64[i] = co[i];
+    }
+  }
+  for (int i = 0; i < o->totnode; i++) {
+    if (o->nodes[i].flag & ORC_Leaf) {
+      o->nodes[i].flag |= ORC_UpdateBB | ORC_UpdateOriginalBB | ORC_UpdateRedraw;
+    }
+  }
+  update_bounds(o, orc_get_threads());
 }

@@ -93,6 +93,9 @@ void orc_get_normals(const OrcPBVH *o, float *out);
 void orc_get_positions64(const OrcPBVH *o, double *out);
 void orc_get_normals64(const OrcPBVH *o, double *out);
 void orc_set_mask(OrcPBVH *o, const float *mask); /* per-vertex paint mask or NULL */
+/* Replace every position and refit all node bounds (orig_vb follows vb), e.g. to
+ * continue from another engine's state. */
+void orc_set_positions(OrcPBVH *o, const float *co);
 /* Replace the vertex normals (e.g. to start two runs from identical state). */
 void orc_set_normals(OrcPBVH *o, const float *no);
 
@@ -112,6 +115,43 @@ float orc_falloff(const OrcDab *dab, float d2);
 void orc_tri_normal(const float v1[3], const float v2[3], const float v3[3], float out[3]);
 void orc_calc_initial_normals(const float *co, int totvert, const int *tris, int tottri,
                               float *out);
+
+/* Ray cast (ray.c).  One ray against the current positions; depth0 bounds the hit
+ * (hits at depth >= depth0 are ignored).  Returns hit.  stats (may be NULL): node
+ * box tests, leaves passing the traversal, leaves ray-tested, tris tested. */
+typedef struct OrcRayHit {
+  int hit;
+  float depth;
+  float location[3];
+  float face_normal[3];
+  int tri;           /* looptri index of the hit tri */
+  int active_vertex; /* corner of that tri nearest the hit point */
+  int leaf;          /* node id of its leaf */
+} OrcRayHit;
+int orc_raycast(const OrcPBVH *o, const float start[3], const float dir[3], float depth0,
+                int original, OrcRayHit *out, int64_t stats[4]);
+
+typedef struct OrcRayNearest {
+  int hit;
+  float depth;
+  float dist_sq;
+  float location[3];
+  int tri;
+  int leaf;
+} OrcRayNearest;
+int orc_find_nearest_to_ray(const OrcPBVH *o, const float start[3], const float dir[3],
+                            int original, OrcRayNearest *out);
+/* BKE_pbvh_raycast_project_ray_root; returns 1 when start/end were moved. */
+int orc_project_ray_root(const OrcPBVH *o, int original, float start[3], float end[3], float dir[3]);
+/* The pieces, for known-answer tests. */
+int orc_isect_ray_tri(const float start[3], const float dir[3], const float v0[3], const float v1[3],
+                      const float v2[3], float *lambda);
+int orc_isect_ray_box(const float start[3], const float dir[3], const float bmin[3],
+                      const float bmax[3], float *tmin);
+float orc_dist_sq_ray_box(const float start[3], const float dir[3], const float bmin[3],
+                          const float bmax[3], float *depth);
+float orc_dist_sq_ray_tri(const float start[3], const float dir[3], const float v0[3],
+                          const float v1[3], const float v2[3], float *depth);
 
 /* Threads used by the OpenMP loops (0 = runtime default). */
 void orc_set_threads(int n);

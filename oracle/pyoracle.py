@@ -26,6 +26,16 @@ class OrcDab(C.Structure):
     ]
 
 
+class OrcRayHit(C.Structure):
+    _fields_ = [("hit", C.c_int), ("depth", C.c_float), ("location", C.c_float * 3),
+                ("face_normal", C.c_float * 3), ("tri", C.c_int), ("active_vertex", C.c_int), ("leaf", C.c_int)]
+
+
+class OrcRayNearest(C.Structure):
+    _fields_ = [("hit", C.c_int), ("depth", C.c_float), ("dist_sq", C.c_float), ("location", C.c_float * 3),
+                ("tri", C.c_int), ("leaf", C.c_int)]
+
+
 _lib = None
 
 
@@ -57,6 +67,7 @@ def lib():
         "orc_get_normals64": (None, [vp, C.POINTER(C.c_double)]),
         "orc_set_mask": (None, [vp, f32p]),
         "orc_set_normals": (None, [vp, f32p]),
+        "orc_set_positions": (None, [vp, f32p]),
         "orc_search_sphere": (i32, [vp, f32p, C.c_float, i32, i32p, i32]),
         "orc_dab": (i32, [vp, C.POINTER(OrcDab), i32p, i32]),
         "orc_falloff": (C.c_float, [C.POINTER(OrcDab), C.c_float]),
@@ -65,6 +76,13 @@ def lib():
         "orc_set_threads": (None, [i32]),
         "orc_get_threads": (i32, []),
         "orc_last_stats": (None, [vp, C.POINTER(C.c_int64)]),
+        "orc_raycast": (i32, [vp, f32p, f32p, C.c_float, i32, C.POINTER(OrcRayHit), C.POINTER(C.c_int64)]),
+        "orc_find_nearest_to_ray": (i32, [vp, f32p, f32p, i32, C.POINTER(OrcRayNearest)]),
+        "orc_project_ray_root": (i32, [vp, i32, f32p, f32p, f32p]),
+        "orc_isect_ray_tri": (i32, [f32p, f32p, f32p, f32p, f32p, f32p]),
+        "orc_isect_ray_box": (i32, [f32p, f32p, f32p, f32p, f32p]),
+        "orc_dist_sq_ray_box": (C.c_float, [f32p, f32p, f32p, f32p, f32p]),
+        "orc_dist_sq_ray_tri": (C.c_float, [f32p, f32p, f32p, f32p, f32p, f32p]),
     }
     for k, (r, a) in sig.items():
         f = getattr(L, k)
@@ -158,6 +176,10 @@ class Oracle:
         self._no = np.ascontiguousarray(no, np.float32)
         lib().orc_set_normals(self.h, _fp(self._no))
 
+    def set_positions(self, co):
+        c = np.ascontiguousarray(co, np.float32)
+        lib().orc_set_positions(self.h, _fp(c))
+
     def set_mask(self, mask):
         if mask is None:
             lib().orc_set_mask(self.h, None)
@@ -204,6 +226,30 @@ class Oracle:
         n = lib().orc_search_sphere(self.h, _fp(c), radius, 1 if use_orig else 0, _ip(out), out.size)
         return out[:n].copy()
 
+    def raycast(self, start, direction, depth0=3.4028234663852886e38, original=False, stats=None):
+        """One ray (ray.c): returns an OrcRayHit; stats (int64[4], optional) gets node
+        box tests, leaves passing, leaves ray-tested, tris tested."""
+        a = np.ascontiguousarray(start, np.float32)
+        b = np.ascontiguousarray(direction, np.float32)
+        h = OrcRayHit()
+        sp = None if stats is None else stats.ctypes.data_as(C.POINTER(C.c_int64))
+        lib().orc_raycast(self.h, _fp(a), _fp(b), depth0, 1 if original else 0, C.byref(h), sp)
+        return h
+
+    def nearest_to_ray(self, start, direction, original=False):
+        a = np.ascontiguousarray(start, np.float32)
+        b = np.ascontiguousarray(direction, np.float32)
+        r = OrcRayNearest()
+        lib().orc_find_nearest_to_ray(self.h, _fp(a), _fp(b), 1 if original else 0, C.byref(r))
+        return r
+
+    def project_ray_root(self, start, end, direction, original=False):
+        a = np.array(start, np.float32)
+        e = np.array(end, np.float32)
+        d = np.array(direction, np.float32)
+        moved = lib().orc_project_ray_root(self.h, 1 if original else 0, _fp(a), _fp(e), _fp(d))
+        return bool(moved), a, e
+
     def last_stats(self):
         out = np.zeros(8, np.int64)
         lib().orc_last_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_int64)))
@@ -232,3 +278,33 @@ def initial_normals(co, tris):
 
 def set_threads(n):
     lib().orc_set_threads(n)
+
+
+def _f3(v):
+    return np.ascontiguousarray(v, np.float32)
+
+
+def isect_ray_tri(start, direction, v0, v1, v2):
+    lam = np.zeros(1, np.float32)
+    ok = lib().orc_isect_ray_tri(_fp(_f3(start)), _fp(_f3(direction)), _fp(_f3(v0)), _fp(_f3(v1)),
+                                 _fp(_f3(v2)), _fp(lam))
+    return bool(ok), float(lam[0])
+
+
+def isect_ray_box(start, direction, bmin, bmax):
+    t = np.zeros(1, np.float32)
+    ok = lib().orc_isect_ray_box(_fp(_f3(start)), _fp(_f3(direction)), _fp(_f3(bmin)), _fp(_f3(bmax)), _fp(t))
+    return bool(ok), float(t[0])
+
+
+def dist_sq_ray_box(start, direction, bmin, bmax):
+    dep = np.zeros(1, np.float32)
+    d = lib().orc_dist_sq_ray_box(_fp(_f3(start)), _fp(_f3(direction)), _fp(_f3(bmin)), _fp(_f3(bmax)), _fp(dep))
+    return float(d), float(dep[0])
+
+
+def dist_sq_ray_tri(start, direction, v0, v1, v2):
+    dep = np.zeros(1, np.float32)
+    d = lib().orc_dist_sq_ray_tri(_fp(_f3(start)), _fp(_f3(direction)), _fp(_f3(v0)), _fp(_f3(v1)),
+                                  _fp(_f3(v2)), _fp(dep))
+    return float(d), float(dep[0])
