@@ -316,12 +316,146 @@ def run_batch_workloads(sg, args, W):
     return res
 
 
-def cpu_baseline(sg, co, tris, dabs, budget_s, threads):
+def camera_rays(n_side, eye_dist=3.0, seed=7):
+    """A pinhole camera eye_dist from the origin looking at it; one ray per pixel of an
+    n_side x n_side image whose field of view just covers a unit sphere (the image
+    corners miss).  Returns starts and unit directions (FP32)."""
+    rng = np.random.default_rng(seed)
+    fwd = rng.normal(size=3)
+    fwd /= np.linalg.norm(fwd)
+    right = np.cross(fwd, [0.0, 0.0, 1.0])
+    right /= np.linalg.norm(right)
+    up = np.cross(right, fwd)
+    half = np.tan(np.arcsin(1.05 / eye_dist))
+    u = (np.arange(n_side) + 0.5) / n_side * 2.0 - 1.0
+    uu, vv = np.meshgrid(u * half, u * half, indexing="ij")
+    d = fwd[None, :] + uu.reshape(-1, 1) * right[None, :] + vv.reshape(-1, 1) * up[None, :]
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    starts = np.repeat((-fwd * eye_dist)[None, :], d.shape[0], axis=0)
+    return starts.astype(np.float32), d.astype(np.float32)
+
+
+def _hip():
+    import ctypes as C
+
+    return C.CDLL("libamdhip64.so")
+
+
+def run_raycast(sg, A, W, K, n_side):
+    """Ray cast throughput on the mesh as it is: one camera image of n_side^2 rays,
+    rays and hits resident in HBM, HIP events around each batch on the engine's
+    stream.  Algorithmic bytes per ray of this kernel's algorithm: 24 per box tested
+    (search-cut and leaf boxes), 52 per triangle tested (corner ids 16 + three corner
+    positions 36), 32 per ray read and 56 per hit written, counted exactly by an
+    untimed pass with the work counters on."""
+    import ctypes as C
+
+    starts, dirs = camera_rays(n_side)
+    n = starts.shape[0]
+    rays = sg.make_rays(starts, dirs)
+    hip = _hip()
+    dr, dh = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(dr), C.c_size_t(C.sizeof(rays))) or \
+            hip.hipMalloc(C.byref(dh), C.c_size_t(n * C.sizeof(sg.SgRayHit))):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemcpy(dr, rays, C.c_size_t(C.sizeof(rays)), 1)
+    L = sg.lib()
+    times = []
+    for s in range(W + K):
+        A.sync()
+        A.timer_start()
+        if L.sg_raycast_device(A.h, dr, n, 0, dh) != 0:
+            raise RuntimeError(sg.last_error())
+        ms = A.timer_stop()
+        if s >= W:
+            times.append(ms)
+    out = (sg.SgRayHit * n)()
+    hip.hipMemcpy(out, dh, C.c_size_t(C.sizeof(out)), 2)
+    hits = sg.hits_to_numpy(out)
+    hip.hipFree(dr)
+    hip.hipFree(dh)
+    st = A.raycast_stats(starts, dirs)
+    ms = float(np.mean(times))
+    alg = 24 * st["boxes"] + 52 * st["tris"] + (32 + 56) * n
+    gbps = alg / (ms * 1e-3) / 1e9
+    res = {"rays": n, "image": f"{n_side}x{n_side} pinhole camera at distance 3",
+           "hit_frac": float(hits["hit"].mean()), "ms_per_batch": ms, "rays_per_s": n / (ms * 1e-3),
+           "us_per_ray_amortized": ms * 1e3 / n, "boxes_per_ray": st["boxes"] / n, "tris_per_ray": st["tris"] / n,
+           "leaves_scanned_per_ray": st["leaves"] / n,
+           "roofline": {"bound": "hbm", "kernel": "k_raycast (one workgroup per ray)", "achieved": round(gbps, 3),
+                        "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(gbps / HBM_PEAK_GBPS, 6),
+                        "alg_bytes_per_launch": alg, "avg_launch_us": round(ms * 1e3, 3)}}
+    log(f"raycast: {res}")
+    return res, starts, dirs
+
+
+def placed_rays(sg, centres, seed=11, eye=3.0, jitter=0.05):
+    """Stroke input as a camera would give it: for each dab, a ray from eye units out
+    along the centre's direction (plus jitter) aimed at the centre."""
+    rng = np.random.default_rng(seed)
+    c = np.asarray(centres, np.float64)
+    u = c / np.linalg.norm(c, axis=1, keepdims=True)
+    starts = u * eye + rng.normal(size=c.shape) * jitter
+    d = c - starts
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return starts.astype(np.float32), d.astype(np.float32)
+
+
+def run_placed(sg, A, dabs, W, K):
+    """The headline stroke with every dab placed by a GPU ray cast inside the stroke
+    graph (dab i's centre = where ray i hits the mesh the previous dabs left) against
+    the same dabs at fixed centres; both captured once and launched per step from the
+    pristine mesh."""
+    centres = np.array([list(d.location) for d in dabs], np.float32)
+    starts, dirs = placed_rays(sg, centres)
+    res = {}
+    for name in ("fixed", "placed"):
+        prep = A.prepare(dabs) if name == "fixed" else A.prepare_placed(dabs, starts, dirs)
+        times = []
+        for s in range(W + K):
+            A.restore()
+            A.sync()
+            A.timer_start()
+            prep.launch(sync=False)
+            ms = A.timer_stop()
+            if s >= W:
+                times.append(ms)
+        if name == "placed":
+            pl = A.placements(len(dabs))
+            res["hit_frac"] = float(pl[:, 6].mean())
+        prep.close()
+        ms = float(np.mean(times))
+        res[name] = {"ms_per_stroke": ms, "us_per_dab": ms * 1e3 / len(dabs)}
+    res["placement_us_per_dab"] = res["placed"]["us_per_dab"] - res["fixed"]["us_per_dab"]
+    log(f"placed stroke: {res}")
+    return res
+
+
+def cpu_raycast(o, starts, dirs, budget_s):
+    """The oracle's sequential ray cast (the reference's traversal and callback) on the
+    same camera rays, one host thread, until the budget is spent."""
+    n = 0
+    t0 = time.perf_counter()
+    for i in range(len(starts)):
+        o.raycast(starts[i], dirs[i])
+        n += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "rays/s", "cores": 1, "kind": "port",
+            "sample": f"first {n} rays of the same camera image ({dt:.1f}s, C oracle, one thread)"}
+
+
+def cpu_baseline(sg, co, tris, dabs, budget_s, threads, before=None):
+    """The oracle's dab loop on the same mesh; `before(oracle)` runs first on the fresh
+    mesh (the ray cast sample shares the build)."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle as po
 
     po.set_threads(threads)
     o = po.Oracle(co, tris, leaf_limit=128, fp64=False)
+    if before is not None:
+        before(o)
     used = po.lib().orc_get_threads()
     n = 0
     aff = 0
@@ -365,6 +499,9 @@ def main():
     ap.add_argument("--no-c5", action="store_true", help="skip the 64 x 1-4M-vertex learned batch")
     ap.add_argument("--c5-meshes", type=int, default=64)
     ap.add_argument("--c5-steps", type=int, default=128)
+    ap.add_argument("--no-ray", action="store_true", help="skip the ray cast and placed-stroke measurements")
+    ap.add_argument("--ray-side", type=int, default=128, help="camera image side (rays per batch = side^2)")
+    ap.add_argument("--only-ray", action="store_true", help="run only the ray cast measurements")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -393,6 +530,11 @@ def main():
 
     W, K = args.warmup, args.steps
     A.snapshot()  # the pristine mesh: every step applies the stroke to it
+    if args.only_ray:
+        ray, _, _ = run_raycast(sg, A, W, max(K, 5), args.ray_side)
+        print(json.dumps({"raycast": ray, "placed_stroke": run_placed(sg, A, dabs, W, K)}))
+        A.close()
+        return
     if args.only_learned:
         print(json.dumps({"learned_smooth": run_learned(sg, A, W, args)}))
         A.close()
@@ -475,6 +617,13 @@ def main():
         copy = {"bytes": args.copy_mb << 20, "ms": ms, "gbps_read_plus_write": gbps_copy}
         log(f"HBM copy probe: {copy}")
 
+    ray = placed = None
+    ray_starts = ray_dirs = None
+    if not args.no_ray:
+        A.restore()
+        ray, ray_starts, ray_dirs = run_raycast(sg, A, W, max(K, 5), args.ray_side)
+        placed = run_placed(sg, A, dabs, W, K)
+
     c3 = None
     if not args.no_learned:
         c3 = run_learned(sg, A, W, args)
@@ -486,7 +635,12 @@ def main():
     if not args.no_cpu:
         threads = os.cpu_count() or 1
         log(f"CPU baseline: oracle, {threads} threads, budget {args.cpu_budget}s")
-        r = cpu_baseline(sg, co, tris, dabs, args.cpu_budget, threads)
+        ray_cpu = {}
+        before = None
+        if ray is not None:
+            def before(o):
+                ray_cpu.update(cpu_raycast(o, ray_starts, ray_dirs, args.cpu_budget / 4))
+        r = cpu_baseline(sg, co, tris, dabs, args.cpu_budget, threads, before)
         cpu = {"value": r["affected"] / r["seconds"], "unit": "vertex-updates/s", "cores": r["threads"],
                "kind": "port",
                "sample": f"first {r['dabs']} dabs of the same stroke on the same icosphere-10 mesh "
@@ -495,6 +649,9 @@ def main():
         cpu["one_thread"] = {"value": r1["affected"] / r1["seconds"], "cores": r1["threads"],
                              "sample": f"first {r1['dabs']} dabs ({r1['seconds']:.1f}s)"}
         log(f"CPU baseline: {cpu}")
+        if ray is not None:
+            ray["cpu_baseline"] = ray_cpu
+            log(f"CPU ray cast: {ray_cpu}")
     A.close()
 
     line = {
@@ -527,6 +684,8 @@ def main():
         "hbm_copy": copy,
         "learned_smooth": c3,
         "batch": batch,
+        "raycast": ray,
+        "placed_stroke": placed,
 
         "build_s": build_s,
         "autotune": tune,

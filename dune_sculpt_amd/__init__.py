@@ -115,6 +115,7 @@ EXPORTS = [
     "sg_batch_read_normals", "sg_batch_snapshot_save", "sg_batch_snapshot_restore", "sg_batch_set_launch",
     "sg_batch_get_launch", "sg_batch_autotune", "sg_batch_set_learned", "sg_batch_timer_start",
     "sg_batch_timer_stop", "sg_hbm_copy_probe", "sg_raycast", "sg_raycast_device", "sg_raycast_stats",
+    "sg_dab_apply_stroke_placed", "sg_stroke_prepare_placed", "sg_stroke_placements",
 ]
 SMOOTH_LEARNED = 1 << 30
 NCTR = 24
@@ -234,6 +235,9 @@ def lib():
         "sg_raycast": (i32, [vp, C.POINTER(SgRay), i32, i32, C.POINTER(SgRayHit)]),
         "sg_raycast_device": (i32, [vp, vp, i32, i32, vp]),
         "sg_raycast_stats": (i32, [vp, C.POINTER(SgRay), i32, i32, C.POINTER(C.c_longlong)]),
+        "sg_dab_apply_stroke_placed": (i32, [vp, C.POINTER(SgDabParams), C.POINTER(SgRay), i32, i32]),
+        "sg_stroke_prepare_placed": (vp, [vp, C.POINTER(SgDabParams), C.POINTER(SgRay), i32, i32]),
+        "sg_stroke_placements": (i32, [vp, f32p, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -578,6 +582,30 @@ class PBVH:
         st = PreparedStroke(h, self)
         self._strokes.append(st)
         return st
+
+    def stroke_placed(self, dabs, starts, normals, depths=None, graph=True, refit_each=False, log=False):
+        """Dab i centred where ray i hits the mesh the previous dabs left (cast on the GPU)."""
+        arr = (SgDabParams * len(dabs))(*dabs)
+        rays = make_rays(starts, normals, depths)
+        flags = (STROKE_GRAPH if graph else 0) | (STROKE_REFIT_EACH if refit_each else 0) | (STROKE_LOG if log else 0)
+        _check(lib().sg_dab_apply_stroke_placed(self.h, arr, rays, len(dabs), flags), "sg_dab_apply_stroke_placed")
+
+    def prepare_placed(self, dabs, starts, normals, depths=None, refit_each=False, log=False):
+        arr = (SgDabParams * len(dabs))(*dabs)
+        rays = make_rays(starts, normals, depths)
+        flags = (STROKE_REFIT_EACH if refit_each else 0) | (STROKE_LOG if log else 0)
+        h = lib().sg_stroke_prepare_placed(self.h, arr, rays, len(dabs), flags)
+        if not h:
+            raise RuntimeError("sg_stroke_prepare_placed failed: " + last_error())
+        st = PreparedStroke(h, self)
+        self._strokes.append(st)
+        return st
+
+    def placements(self, n):
+        """(n, 8) placements of the last placed stroke: centre, Grab translation, hit, 0."""
+        out = np.zeros((n, 8), np.float32)
+        m = _check(lib().sg_stroke_placements(self.h, _fp(out), n), "placements")
+        return out[:m]
 
     def stroke_log(self, n):
         out = np.zeros((n, NCTR), np.int64)

@@ -440,13 +440,75 @@ int sg_dab_apply(PBVH *p, const SgDabParams *dab) {
   return 0;
 }
 
-int sg_dab_apply_stroke(PBVH *p, const SgDabParams *dabs, int n, int flags) {
+// Device buffers of a ray-placed stroke: its rays and one 8-float placement slot per
+// dab (cx cy cz gx gy gz hit 0).
+static int ensure_place(float **place, SgRay **rays, int *cap, int n) {
+  if (n <= *cap) return 0;
+  if (*place) hipFree(*place);
+  if (*rays) hipFree(*rays);
+  *place = nullptr;
+  *rays = nullptr;
+  *cap = 0;
+  if (sg_check_hip(hipMalloc(place, (size_t)n * 8 * sizeof(float)), "hipMalloc")) return -1;
+  if (sg_check_hip(hipMalloc(rays, ((size_t)n + 1) * sizeof(SgRay)), "hipMalloc")) return -1;
+  *cap = n;
+  return 0;
+}
+
+// Dab i's placement: a ray cast on the mesh as the previous dab left it (serial - 1 is
+// the dab whose touched search boxes are not refreshed yet); Grab also casts the next
+// dab's ray for its translation.  Rays along a coordinate plane need current internal
+// boxes, so the deferred refit runs first for them.
+static int enqueue_place(PBVH *p, const SgDabParams *dabs, const SgRay *rays_host, const SgRay *rays_dev,
+                         float *place_dev, int i, int n, int serial) {
+  const bool grab = dabs[i].sculpt_tool == SG_TOOL_GRAB && i + 1 < n;
+  if (sg_ray_host_needs_refit(rays_host + i, grab ? 2 : 1) && sg_enqueue_flush(p)) return -1;
+  return sg_enqueue_raycast(p, rays_dev + i, grab ? 2 : 1, 0, serial - 1, nullptr, place_dev + 8 * (size_t)i,
+                            grab ? 1 : 0, nullptr);
+}
+
+static int check_rays(const SgRay *rays, int n) {
+  for (int i = 0; i < n; i++) {
+    const float *d = rays[i].normal;
+    if (!(d[0] == d[0] && d[1] == d[1] && d[2] == d[2]) || (d[0] == 0.0f && d[1] == 0.0f && d[2] == 0.0f))
+      return sg_set_error("ray %d has no direction", i);
+  }
+  return 0;
+}
+
+// A rejected learned dab of a placed stroke is redone by the plain brush at the centre
+// its ray cast found (or not at all when the ray missed).
+static int plain_fallback(PBVH *p, const SgDabParams &dab, const float *place_dev, int i) {
+  SgDabParams plain = dab;
+  plain.flag &= ~SG_SMOOTH_LEARNED;
+  if (place_dev) {
+    float pl[8];
+    if (int rc = sg_check_hip(hipMemcpy(pl, place_dev + 8 * (size_t)i, sizeof(pl), hipMemcpyDeviceToHost), "copy"))
+      return rc;
+    if (pl[6] == 0.0f) return 0;
+    memcpy(plain.location, pl, 3 * sizeof(float));
+    memcpy(plain.grab_delta, pl + 3, 3 * sizeof(float));
+  }
+  return sg_dab_apply(p, &plain);
+}
+
+static int stroke_apply(PBVH *p, const SgDabParams *dabs, const SgRay *rays, int n, int flags) {
   if (!p || (!dabs && n > 0) || n < 0) return sg_set_error("bad arguments");
   if (p->no_device) return sg_set_error("no HIP device");
   for (int i = 0; i < n; i++) {
     if (int rc = check_dab(p, &dabs[i])) return rc;
   }
+  if (rays && check_rays(rays, n)) return -1;
   if (n == 0) return 0;
+  if (rays) {
+    if (sg_ray_ensure(p)) return -1;  // builds its tables with synchronous copies: not under capture
+    if (ensure_place(&p->d.place, &p->d.place_rays, &p->d.place_cap, n)) return -1;
+    if (sg_check_hip(hipMemcpyAsync(p->d.place_rays, rays, (size_t)n * sizeof(SgRay), hipMemcpyHostToDevice, p->stream),
+                     "copy"))
+      return -1;
+    p->last_place = p->d.place;
+    p->last_place_n = n;
+  }
   const bool use_graph = (flags & 1) != 0;
   const bool flush_each = (flags & 2) != 0;
   const bool log = (flags & 4) != 0;
@@ -479,7 +541,13 @@ int sg_dab_apply_stroke(PBVH *p, const SgDabParams *dabs, int n, int flags) {
           curve = p->d.curve + (size_t)(ci++) * (SG_CM_TABLE + 1);
         SgDabK k = sg_make_dab(&dabs[i], g_stats ? 1 : 0);
         const int stamp = (++p->stamp) * 128;
-        if (int rc = sg_enqueue_dab(p, k, curve, log ? p->d.log : nullptr, i - a0, stamp, i - a0)) return rc;
+        const float *place = nullptr;
+        if (rays) {
+          if (int rc = enqueue_place(p, dabs, rays, p->d.place_rays, p->d.place, i, n, i - a0)) return rc;
+          place = p->d.place + 8 * (size_t)i;
+        }
+        if (int rc = sg_enqueue_dab(p, k, curve, log ? p->d.log : nullptr, i - a0, stamp, i - a0, nullptr, place))
+          return rc;
         if (flush_each && sg_enqueue_flush(p)) return -1;
       }
       if (!flush_each && sg_enqueue_flush(p)) return -1;
@@ -527,9 +595,7 @@ int sg_dab_apply_stroke(PBVH *p, const SgDabParams *dabs, int n, int flags) {
     if (rc) return rc;
     if (halted < 0) break;
     // the rejected learned dab runs with the plain brush, then the stroke resumes
-    SgDabParams plain = dabs[halted];
-    plain.flag &= ~SG_SMOOTH_LEARNED;
-    if ((rc = sg_dab_apply(p, &plain))) return rc;
+    if ((rc = plain_fallback(p, dabs[halted], rays ? p->d.place : nullptr, halted))) return rc;
     nfallback++;
     a0 = halted + 1;
     p->halt_live = true;  // sg_dab_apply cleared it for the plain dab
@@ -549,6 +615,25 @@ int sg_dab_apply_stroke(PBVH *p, const SgDabParams *dabs, int n, int flags) {
   return 0;
 }
 
+int sg_dab_apply_stroke(PBVH *p, const SgDabParams *dabs, int n, int flags) {
+  return stroke_apply(p, dabs, nullptr, n, flags);
+}
+
+int sg_dab_apply_stroke_placed(PBVH *p, const SgDabParams *dabs, const SgRay *rays, int n, int flags) {
+  if (n > 0 && !rays) return sg_set_error("bad arguments");
+  return stroke_apply(p, dabs, rays, n, flags);
+}
+
+int sg_stroke_placements(PBVH *p, float *out, int n) {
+  if (!p || !out || n < 0) return sg_set_error("bad arguments");
+  if (!p->last_place) return sg_set_error("no placed stroke has run");
+  const int m = n < p->last_place_n ? n : p->last_place_n;
+  if (int rc = sg_check_hip(hipStreamSynchronize(p->stream), "sync")) return rc;
+  if (int rc = sg_check_hip(hipMemcpy(out, p->last_place, (size_t)m * 8 * sizeof(float), hipMemcpyDeviceToHost), "copy"))
+    return rc;
+  return m;
+}
+
 // ---- prepared strokes: one hipGraph per dab list, replayed on demand ----------
 
 struct SgStroke {
@@ -560,13 +645,27 @@ struct SgStroke {
   int flags = 0;
   bool learned = false;           // contains learned Smooth dabs: launch checks for a halt
   std::vector<SgDabParams> dabs;  // (custom curve pointers stay the caller's)
+  std::vector<SgRay> rays;        // ray-placed stroke: one ray per dab
+  float *place = nullptr;         // its placement slots and device rays
+  SgRay *place_rays = nullptr;
+  int place_cap = 0;
 };
 
-SgStroke *sg_stroke_prepare(PBVH *p, const SgDabParams *dabs, int n, int flags) {
+static void stroke_release(SgStroke *s) {
+  if (s->exec) hipGraphExecDestroy(s->exec);
+  if (s->graph) hipGraphDestroy(s->graph);
+  if (s->curves) hipFree(s->curves);
+  if (s->place) hipFree(s->place);
+  if (s->place_rays) hipFree(s->place_rays);
+  delete s;
+}
+
+static SgStroke *stroke_prepare(PBVH *p, const SgDabParams *dabs, const SgRay *rays, int n, int flags) {
   if (!p || n <= 0 || !dabs) {
     sg_set_error("bad arguments");
     return nullptr;
   }
+  if (rays && (check_rays(rays, n) || sg_ray_ensure(p))) return nullptr;
   if (p->no_device) {
     sg_set_error("no HIP device");
     return nullptr;
@@ -582,6 +681,14 @@ SgStroke *sg_stroke_prepare(PBVH *p, const SgDabParams *dabs, int n, int flags) 
   s->n = n;
   s->flags = flags;
   s->dabs.assign(dabs, dabs + n);
+  if (rays) {
+    s->rays.assign(rays, rays + n);
+    if (ensure_place(&s->place, &s->place_rays, &s->place_cap, n) ||
+        sg_check_hip(hipMemcpy(s->place_rays, rays, (size_t)n * sizeof(SgRay), hipMemcpyHostToDevice), "copy")) {
+      stroke_release(s);
+      return nullptr;
+    }
+  }
   for (int i = 0; i < n; i++)
     if (dabs[i].sculpt_tool == SG_TOOL_SMOOTH && (dabs[i].flag & SG_SMOOTH_LEARNED) && p->mlp) s->learned = true;
   int ncustom = 0;
@@ -610,7 +717,12 @@ SgStroke *sg_stroke_prepare(PBVH *p, const SgDabParams *dabs, int n, int flags) 
       if (dabs[i].curve_preset == SG_CURVE_CUSTOM && dabs[i].curve_table)
         curve = s->curves + (size_t)(ci++) * (SG_CM_TABLE + 1);
       SgDabK k = sg_make_dab(&dabs[i], 0);
-      erc = sg_enqueue_dab(p, k, curve, log ? p->d.log : nullptr, i, 0, i);
+      const float *place = nullptr;
+      if (rays) {
+        erc = enqueue_place(p, dabs, rays, s->place_rays, s->place, i, n, i);
+        place = s->place + 8 * (size_t)i;
+      }
+      if (!erc) erc = sg_enqueue_dab(p, k, curve, log ? p->d.log : nullptr, i, 0, i, nullptr, place);
       if (!erc && flush_each) erc = sg_enqueue_flush(p);
     }
     if (!erc && !flush_each) erc = sg_enqueue_flush(p);
@@ -621,17 +733,31 @@ SgStroke *sg_stroke_prepare(PBVH *p, const SgDabParams *dabs, int n, int flags) 
   p->capturing = false;
   if (!rc) rc = sg_check_hip(hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0), "instantiate");
   if (rc) {
-    if (s->graph) hipGraphDestroy(s->graph);
-    if (s->curves) hipFree(s->curves);
-    delete s;
+    stroke_release(s);
     return nullptr;
   }
   return s;
 }
 
+SgStroke *sg_stroke_prepare(PBVH *p, const SgDabParams *dabs, int n, int flags) {
+  return stroke_prepare(p, dabs, nullptr, n, flags);
+}
+
+SgStroke *sg_stroke_prepare_placed(PBVH *p, const SgDabParams *dabs, const SgRay *rays, int n, int flags) {
+  if (!rays) {
+    sg_set_error("bad arguments");
+    return nullptr;
+  }
+  return stroke_prepare(p, dabs, rays, n, flags);
+}
+
 int sg_stroke_launch(SgStroke *s, int sync) {
   if (!s) return sg_set_error("null stroke");
   PBVH *p = s->p;
+  if (s->place) {
+    p->last_place = s->place;
+    p->last_place_n = s->n;
+  }
   int rc = sg_check_hip(hipGraphLaunch(s->exec, p->stream), "graph launch");
   if (!rc && (sync || s->learned)) rc = sg_check_hip(hipStreamSynchronize(p->stream), "stroke");
   if (!rc && s->learned) {
@@ -644,12 +770,12 @@ int sg_stroke_launch(SgStroke *s, int sync) {
             (s->dabs[i].sculpt_tool == SG_TOOL_SMOOTH && (s->dabs[i].flag & SG_SMOOTH_LEARNED)) ? 1 : 0;
     if (!rc && h) {  // dab h-1 was rejected: plain brush for it, then the rest
       const int f = h - 1;
-      SgDabParams plain = s->dabs[f];
-      plain.flag &= ~SG_SMOOTH_LEARNED;
       p->dab_count += f;  // the graph's dabs before it; the calls below count themselves
-      rc = sg_dab_apply(p, &plain);
+      rc = plain_fallback(p, s->dabs[f], s->place, f);
       p->learned_fallbacks++;
-      if (!rc && f + 1 < s->n) rc = sg_dab_apply_stroke(p, s->dabs.data() + f + 1, s->n - f - 1, s->flags & ~1);
+      if (!rc && f + 1 < s->n)
+        rc = stroke_apply(p, s->dabs.data() + f + 1, s->rays.empty() ? nullptr : s->rays.data() + f + 1,
+                          s->n - f - 1, s->flags & ~1);
       invalidate_host_views(p);
       return rc;
     }
@@ -670,10 +796,11 @@ int sg_stroke_launch(SgStroke *s, int sync) {
 void sg_stroke_free(SgStroke *s) {
   if (!s) return;
   hipStreamSynchronize(s->p->stream);
-  if (s->exec) hipGraphExecDestroy(s->exec);
-  if (s->graph) hipGraphDestroy(s->graph);
-  if (s->curves) hipFree(s->curves);
-  delete s;
+  if (s->p->last_place == s->place) {
+    s->p->last_place = nullptr;
+    s->p->last_place_n = 0;
+  }
+  stroke_release(s);
 }
 
 // ---- autotuner: workgroup counts of the per-dab kernels --------------------------

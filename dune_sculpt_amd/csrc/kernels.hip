@@ -56,8 +56,9 @@ int sg_check_hip(hipError_t e, const char *what) {
 // the previous dab's counters and clears the next dab's counter set.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_search(SgDevView d, SgDabK k, long long *log, int log_index,
-                                                int serial) {
+                                                int serial, const float *place) {
   if (d.halt && d.halt[0]) return;
+  sg_place(k, place);
   const int lane = lane_id();
   const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
   const int nw = gridDim.x * blockDim.x / SG_WAVE;
@@ -160,8 +161,9 @@ __global__ void k_clear_ctr(long long *ctr, int *halt) {
 // and accumulated in int64 -- exact, so the atomic order cannot change the bits.
 // Positions and normals are fetched in the same load level.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_area(SgDevView d, SgDabK k) {
+__global__ __launch_bounds__(256) void k_area(SgDevView d, SgDabK k, const float *place) {
   if (d.halt && d.halt[0]) return;
+  sg_place(k, place);
   const int lane = lane_id();
   const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
   const int nw = gridDim.x * blockDim.x / SG_WAVE;
@@ -207,8 +209,10 @@ __global__ __launch_bounds__(256) void k_area(SgDevView d, SgDabK k) {
 // vertex->vertex CSR ranges) so the later kernels skip those lookups.  The slot
 // reservation is in flight while the displacement is computed.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_brush(SgDevView d, SgDabK k, const float *curve, int stamp) {
+__global__ __launch_bounds__(256) void k_brush(SgDevView d, SgDabK k, const float *curve, int stamp,
+                                               const float *place) {
   if (d.halt && d.halt[0]) return;
+  sg_place(k, place);
   const int lane = lane_id();
   const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
   const int nw = gridDim.x * blockDim.x / SG_WAVE;
@@ -858,7 +862,7 @@ int sg_enqueue_clear_ctr(PBVH *p) {
 // One dab; `serial` is the dab's position in its stroke (selects the counter set and
 // the "touched by the previous dab" test of the search).
 int sg_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long long *log, int log_index,
-                   int stamp, int serial, const SgCollect *col) {
+                   int stamp, int serial, const SgCollect *col, const float *place) {
   SgDevView v = sg_view(p);
   v.ctr = p->d.ctr + (size_t)(serial & 1) * SG_CTR_COUNT;
   v.ctr_next = p->d.ctr + (size_t)((serial + 1) & 1) * SG_CTR_COUNT;
@@ -870,12 +874,12 @@ int sg_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long long *
   {
     ProfScope ps(p, SG_K_SEARCH);
     const int sg = (p->d.C + 3) / 4;
-    SG_LAUNCH(ps, k_search, dim3(sg > 0 ? sg : 1), dim3(256), 0, s, v, k, log, log_index, serial);
+    SG_LAUNCH(ps, k_search, dim3(sg > 0 ? sg : 1), dim3(256), 0, s, v, k, log, log_index, serial, place);
   }
   if (k.learned && p->mlp) {  // learned Smooth: brush + MLP, then decide / commit
     {
       ProfScope ps(p, SG_K_BRUSH);
-      sg_enqueue_learned_brush(p, v, k, curve_dev, p->mlp);
+      sg_enqueue_learned_brush(p, v, k, curve_dev, p->mlp, place);
     }
     {
       ProfScope ps(p, SG_K_POST);
@@ -885,11 +889,11 @@ int sg_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long long *
   }
   if (k.tool == SG_TOOL_DRAW) {
     ProfScope ps(p, SG_K_AREA);
-    SG_LAUNCH(ps, k_area, dim3(GB), dim3(256), 0, s, v, k);
+    SG_LAUNCH(ps, k_area, dim3(GB), dim3(256), 0, s, v, k, place);
   }
   {
     ProfScope ps(p, SG_K_BRUSH);
-    SG_LAUNCH(ps, k_brush, dim3(GB), dim3(256), 0, s, v, k, curve_dev, stamp);
+    SG_LAUNCH(ps, k_brush, dim3(GB), dim3(256), 0, s, v, k, curve_dev, stamp, place);
   }
   if (col) sg_enqueue_collect(p, v, col->X, col->frame, col->cap);
   if (k.tool == SG_TOOL_SMOOTH) {

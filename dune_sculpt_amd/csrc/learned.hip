@@ -382,13 +382,14 @@ __device__ __forceinline__ BrushIn learned_tile_load(const SgDevView &d, const S
 }
 
 __global__ __launch_bounds__(LRN_THREADS) void k_brush_learned(SgDevView d, SgDabK k, const float *curve,
-                                                               SgMlpView m) {
+                                                               SgMlpView m, const float *place) {
   __shared__ MlpLds s;
   __shared__ __align__(16) uint16_t bufA[LRN_WAVES][LRN_ROWS * MLP_ST];
   __shared__ __align__(16) uint16_t bufB[LRN_WAVES][LRN_ROWS * MLP_ST];
   __shared__ __align__(16) float sOut[LRN_WAVES][LRN_ROWS * 4];
   __shared__ unsigned long long s_acc[3];
   if (d.halt && d.halt[0]) return;
+  sg_place(k, place);
   const int nitems = LRN_SLOTS * (int)d.ctr[SG_CTR_NGATHER];
   if (blockIdx.x * LRN_WAVES >= nitems) return;  // whole block idle: skip the weight fill
   mlp_lds_load<LRN_THREADS>(s, m);
@@ -960,9 +961,11 @@ int sg_mfma_probe(const uint16_t *A, const uint16_t *B, float *D) {
 }  // extern "C"
 
 // ---- launch helpers used by the dab path (api.cpp / kernels.hip) ----
-int sg_enqueue_learned_brush(PBVH *p, const SgDevView &v, const SgDabK &k, const float *curve, SgMlp *m) {
+int sg_enqueue_learned_brush(PBVH *p, const SgDevView &v, const SgDabK &k, const float *curve, SgMlp *m,
+                             const float *place) {
   const int G = p->grid_brush > 0 ? p->grid_brush : p->grid_wg;
-  hipLaunchKernelGGL(k_brush_learned, dim3(G), dim3(LRN_THREADS), 0, p->stream, v, k, curve, sg_mlp_view(m));
+  hipLaunchKernelGGL(k_brush_learned, dim3(G), dim3(LRN_THREADS), 0, p->stream, v, k, curve, sg_mlp_view(m),
+                     place);
   return sg_check_hip(hipGetLastError(), "learned brush launch");
 }
 

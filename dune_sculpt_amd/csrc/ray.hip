@@ -8,7 +8,8 @@
 // ray-tested only while its tmin is below the running bound (FLT_MAX, then the hit
 // depth), and a triangle replaces the hit only when strictly nearer (:3951-3966).
 //
-// How it runs here, one 1024-thread workgroup per ray:
+// How it runs here, one workgroup per ray (256 threads per ray of a batch, 1024 for the
+// single ray that places a dab):
 //   1. the search cut's subtree boxes (stale ones, touched by the previous dab and not
 //      yet refreshed, are kept: their leaves are tested anyway)
 //   2. the leaves of the passing subtrees, flattened over the workgroup
@@ -36,12 +37,10 @@
 #include "sg_kernels.h"
 #include "sg_device.h"
 
-#define RAY_THREADS 1024
-#define RAY_WAVES (RAY_THREADS / SG_WAVE)
+// Workgroup sizes: a batch of rays runs 256 threads per ray (several rays in flight per
+// CU hide each ray's dependent load levels); a single placement ray runs 1024.
 #define RAY_LEAF_UNROLL 2
-#define RAY_ROUND (RAY_THREADS * RAY_LEAF_UNROLL)  // leaves tested per round
-#define RAY_CAND RAY_ROUND                         // passing leaves held before their scan
-#define RAY_HITS 1024                              // leaves with a hit, per ray
+#define RAY_HITS 1024  // leaves with a hit, per ray (both sizes: the same limit)
 
 struct SgRayView {
   const int4 *ltri;        // [T] tris in leaf order: new corner ids, .w = looptri | hidden << 31
@@ -167,11 +166,13 @@ __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long k)
   return k;
 }
 
+template <int NT>
 struct RayShared {
+  static constexpr int kCand = NT * RAY_LEAF_UNROLL;  // one round of leaves
   int cut[1024];
   int pre[1025];
-  int cand_leaf[RAY_CAND];
-  float cand_t[RAY_CAND];
+  int cand_leaf[kCand];
+  float cand_t[kCand];
   int hit_leaf[RAY_HITS];
   float hit_t[RAY_HITS];
   unsigned long long hit_key[RAY_HITS];
@@ -182,11 +183,12 @@ struct RayShared {
 
 // Scan the listed candidate leaves' tris (a wave per leaf); leaves with a hit below
 // the depth bound go to the hit list.
-__device__ __forceinline__ void scan_candidates(const SgDevView &d, const SgRayView &rv, const RayPre &r, RayShared &S, int ncand,
-                                long long &ntris) {
+template <int NT>
+__device__ __forceinline__ void scan_candidates(const SgDevView &d, const SgRayView &rv, const RayPre &r,
+                                                RayShared<NT> &S, int ncand, long long &ntris) {
   const int lane = lane_id();
   const int wave = threadIdx.x / SG_WAVE;
-  for (int ci = wave; ci < ncand; ci += RAY_WAVES) {
+  for (int ci = wave; ci < ncand; ci += NT / SG_WAVE) {
     const int l = S.cand_leaf[ci];
     const int tb = rv.toff[l], te = rv.toff[l + 1];
     unsigned long long best = ~0ull;
@@ -244,8 +246,9 @@ __device__ __forceinline__ void scan_candidates(const SgDevView &d, const SgRayV
 }
 
 // One ray, the whole workgroup; lane 0 of wave 0 leaves the result in S.res[slot].
-__device__ __forceinline__ void ray_one(const SgDevView &d, const SgRayView &rv, const SgRay *ray, int original, int stale,
-                        RayShared &S, int slot, long long *stats) {
+template <int NT>
+__device__ __forceinline__ void ray_one(const SgDevView &d, const SgRayView &rv, const SgRay *ray, int original,
+                                        int stale, RayShared<NT> &S, int slot, long long *stats) {
   const int tid = threadIdx.x;
   const int lane = lane_id();
   const int wave = tid / SG_WAVE;
@@ -263,7 +266,7 @@ __device__ __forceinline__ void ray_one(const SgDevView &d, const SgRayView &rv,
   long long ntests = 0, ntris = 0, ncands = 0;
 
   // 1. search-cut subtree boxes
-  for (int c = tid; c < d.C; c += RAY_THREADS) {
+  for (int c = tid; c < d.C; c += NT) {
     if (d.cut_flags[c] & 1) continue;  // a single fully hidden leaf: never hit
     bool pass = d.cut_touch[c] == stale;
     if (!pass) {
@@ -302,13 +305,13 @@ __device__ __forceinline__ void ray_one(const SgDevView &d, const SgRayView &rv,
   const int total = S.pre[nc];
 
   // 2. + 3. leaves in rounds, each round's passing leaves scanned at its end
-  for (int g0 = 0; g0 < total; g0 += RAY_ROUND) {
+  for (int g0 = 0; g0 < total; g0 += NT * RAY_LEAF_UNROLL) {
     int leaf[RAY_LEAF_UNROLL];
     BB6 b[RAY_LEAF_UNROLL];
     int fl[RAY_LEAF_UNROLL];
 #pragma unroll
     for (int j = 0; j < RAY_LEAF_UNROLL; j++) {
-      const int g = g0 + j * RAY_THREADS + tid;
+      const int g = g0 + j * NT + tid;
       leaf[j] = -1;
       if (g < total) {
         int lo = 0, hi = nc - 1;  // last k with pre[k] <= g
@@ -446,10 +449,11 @@ __device__ __forceinline__ void ray_one(const SgDevView &d, const SgRayView &rv,
 // Rays in a grid-stride loop (hits written per ray), or one dab placement: ray 0 gives
 // the dab centre and, for Grab, ray 1's hit minus ray 0's gives the translation.
 // place[8]: cx cy cz gx gy gz hit 0.
-__global__ __launch_bounds__(RAY_THREADS) void k_raycast(SgDevView d, SgRayView rv, const SgRay *rays, int n,
-                                                         int original, int stale, SgRayHit *out, float *place,
-                                                         int grab_pair, long long *stats) {
-  __shared__ RayShared S;
+template <int NT>
+__global__ __launch_bounds__(NT) void k_raycast(SgDevView d, SgRayView rv, const SgRay *rays, int n, int original,
+                                                int stale, SgRayHit *out, float *place, int grab_pair,
+                                                long long *stats) {
+  __shared__ RayShared<NT> S;
   if (place) {
     if (blockIdx.x != 0) return;
     ray_one(d, rv, rays, original, stale, S, 0, stats);
@@ -558,9 +562,12 @@ int sg_enqueue_raycast(PBVH *p, const SgRay *rays_dev, int n, int original, int 
   if (int rc = sg_ray_ensure(p)) return rc;
   const SgDevView v = sg_view(p);
   const SgRayView rv = ray_view(p);
-  const int grid = place_dev ? 1 : std::max(1, std::min(n, 65536));
-  hipLaunchKernelGGL(k_raycast, dim3(grid), dim3(RAY_THREADS), 0, p->stream, v, rv, rays_dev, n, original, stale,
-                     out_dev, place_dev, grab_pair, stats_dev);
+  if (place_dev)
+    hipLaunchKernelGGL(k_raycast<1024>, dim3(1), dim3(1024), 0, p->stream, v, rv, rays_dev, n, original, stale,
+                       out_dev, place_dev, grab_pair, stats_dev);
+  else
+    hipLaunchKernelGGL(k_raycast<256>, dim3(std::max(1, std::min(n, 65536))), dim3(256), 0, p->stream, v, rv,
+                       rays_dev, n, original, stale, out_dev, place_dev, grab_pair, stats_dev);
   return sg_check_hip(hipGetLastError(), "raycast launch");
 }
 

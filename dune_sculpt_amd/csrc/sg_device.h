@@ -108,6 +108,25 @@ __device__ __forceinline__ BB6 bb_wave_reduce(BB6 b) {
   return b;  // lane 0 holds the ordered combination
 }
 
+// A dab placed by a ray cast on the GPU reads its centre (and Grab translation) from
+// the placement slot the ray cast wrote: cx cy cz gx gy gz hit 0.  A ray that missed
+// leaves the dab empty (no box or vertex passes a negative squared radius).
+__device__ __forceinline__ void sg_place(SgDabK &k, const float *place) {
+  if (!place) return;
+  const float4 a = *(const float4 *)place;
+  const float4 b = *(const float4 *)(place + 4);
+  k.cx = a.x;
+  k.cy = a.y;
+  k.cz = a.z;
+  k.gx = a.w;
+  k.gy = b.x;
+  k.gz = b.y;
+  if (b.z == 0.0f) {
+    k.r2 = -1.0f;
+    k.nr2 = -1.0f;
+  }
+}
+
 // Spec S1 sphere-vs-box test: nearest point of the box, strict '<' against r*r.
 __device__ __forceinline__ bool sphere_box(const BB6 &b, float cx, float cy, float cz, float r2) {
   const float c[3] = {cx, cy, cz};

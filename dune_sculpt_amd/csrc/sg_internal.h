@@ -167,6 +167,8 @@ struct PBVH {
   int stamp = 0;                     // stats-mode stamp generator
   hipEvent_t timer[2] = {nullptr, nullptr};  // sg_pbvh_timer_start/stop
   bool halt_live = false;            // the work being enqueued contains learned dabs (kernels test the halt flag)
+  const float *last_place = nullptr; // placement slots of the last ray-placed stroke
+  int last_place_n = 0;
 };
 
 // Host copies of the leaf-ordered device arrays of one mesh (pbvh_build.cpp).

@@ -68,7 +68,7 @@ int sg_enqueue_gather_raw(hipStream_t s, const float *x, const float *y, const f
 int sg_profile_collect(PBVH *p);
 SgDabK sg_make_dab(const SgDabParams *dab, int stats);
 int sg_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long long *log, int log_index,
-                   int stamp, int serial, const SgCollect *col = nullptr);
+                   int stamp, int serial, const SgCollect *col = nullptr, const float *place = nullptr);
 int sg_enqueue_clear_ctr(PBVH *p);
 int sg_enqueue_flush(PBVH *p);
 int sg_enqueue_log_row(PBVH *p, long long *log, int row);

@@ -39,7 +39,8 @@ struct SgMlp {
 };
 
 SgMlpView sg_mlp_view(SgMlp *m);
-int sg_enqueue_learned_brush(PBVH *p, const SgDevView &v, const SgDabK &k, const float *curve, SgMlp *m);
+int sg_enqueue_learned_brush(PBVH *p, const SgDevView &v, const SgDabK &k, const float *curve, SgMlp *m,
+                             const float *place = nullptr);
 int sg_enqueue_learned_post(PBVH *p, const SgDevView &v, SgMlp *m, int serial);
 int sg_enqueue_collect(PBVH *p, const SgDevView &v, float *X, float *frame, int cap);
 int sg_enqueue_judge(PBVH *p, int na, const float *lrn, const float *bx, const float *by, const float *bz,

@@ -306,6 +306,17 @@ int sg_raycast(PBVH *pbvh, const SgRay *rays, int n, int original, SgRayHit *hit
 int sg_raycast_device(PBVH *pbvh, const SgRay *rays_dev, int n, int original, SgRayHit *hits_dev);
 /* Work counters of a ray batch: [0] boxes tested [1] leaves scanned [2] tris tested. */
 int sg_raycast_stats(PBVH *pbvh, const SgRay *rays, int n, int original, long long out[3]);
+/* A stroke whose dab centres come from rays cast on the GPU, each on the mesh as the
+ * previous dab left it (the stroke location of each OperatorStrokeElement,
+ * api_brush.c:3526-3576, found by SCULPT_stroke_get_location in the absent sculpt.c):
+ * dab i's location is ray i's hit, and a dab whose ray misses does nothing; a Grab
+ * dab's translation is ray i+1's hit minus ray i's (zero for the last dab or a miss).
+ * Other SgDabParams fields and flags as for sg_dab_apply_stroke / sg_stroke_prepare. */
+int sg_dab_apply_stroke_placed(PBVH *pbvh, const SgDabParams *dabs, const SgRay *rays, int n, int flags);
+SgStroke *sg_stroke_prepare_placed(PBVH *pbvh, const SgDabParams *dabs, const SgRay *rays, int n, int flags);
+/* Placements of the last ray-placed stroke, 8 floats per dab: centre xyz, Grab
+ * translation xyz, 1 when the ray hit (else 0), 0.  Returns the number of dabs. */
+int sg_stroke_placements(PBVH *pbvh, float *out, int n);
 
 /* ---- learned Smooth (MLP over 1-ring features; BF16 MFMA inference) ----------
  * A Smooth dab with SG_SMOOTH_LEARNED in SgDabParams.flag goes through the model

@@ -181,3 +181,93 @@ def test_raycast_device_pointers(sg, po):
     hip.hipFree(dr)
     hip.hipFree(dh)
     p.close()
+
+
+def _camera_rays(sg, n, seed, radii, miss_every=9):
+    """A stroke seen from a camera: rays from 2.5 units out aimed at points of a great
+    circle on the unit sphere (every miss_every-th aimed past the mesh)."""
+    rng = np.random.default_rng(seed)
+    centres = sg.great_circle_centres(seed, radii)
+    starts, dirs = [], []
+    for i, c in enumerate(centres):
+        c = np.asarray(c, np.float64)
+        s = 2.5 * c / np.linalg.norm(c) + rng.normal(size=3) * 0.2
+        aim = c * (1.8 if (miss_every and i % miss_every == miss_every - 1) else 1.0)
+        if miss_every and i % miss_every == miss_every - 1:
+            aim = aim + np.cross(c, [0.0, 0.0, 1.0]) * 2.0
+        starts.append(s)
+        dirs.append(_unit(aim - s))
+    return np.array(starts, F32), np.array(dirs, F32)
+
+
+def _oracle_placed(sg, po, o, dabs, starts, dirs):
+    """The oracle's stroke: ray i cast on its current state gives dab i's centre (and
+    ray i+1 the Grab translation); a miss skips the dab.  Returns the placements."""
+    out = np.zeros((len(dabs), 8), F32)
+    for i, d in enumerate(dabs):
+        h = o.raycast(starts[i], dirs[i])
+        g = np.zeros(3, F32)
+        if d.sculpt_tool == sg.TOOL_GRAB and i + 1 < len(dabs):
+            h2 = o.raycast(starts[i + 1], dirs[i + 1])
+            if h.hit and h2.hit:
+                g = np.array(h2.location, F32) - np.array(h.location, F32)
+        if h.hit:
+            out[i, :3] = np.array(h.location, F32)
+            out[i, 6] = 1.0
+        out[i, 3:6] = g
+        if h.hit:
+            q = sg.make_dab(tool=d.sculpt_tool, location=out[i, :3], radius=d.radius, strength=d.strength,
+                            smooth_iterations=d.smooth_iterations, grab_delta=g)
+            o.dab(q)
+    return out
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_placed_stroke_matches_oracle(sg, po, graph):
+    from parity_util import REL_TOL, n_bit_diff, rel_err
+
+    co, tris = sg.noisy_icosphere(5, seed=8, amplitude=0.02)
+    p = sg.PBVH(co, tris, leaf_limit=32)
+    o = po.Oracle(co, tris, leaf_limit=32)
+    o.set_normals(p.normals())
+    n = 36
+    rng = np.random.default_rng(6)
+    radii = rng.uniform(0.06, 0.15, n).astype(F32)
+    starts, dirs = _camera_rays(sg, n, 6, radii)
+    tools = [1, 4, 2, 5]
+    dabs = [sg.make_dab(tool=tools[i % 4], location=(0, 0, 0), radius=float(radii[i]),
+                        strength=float(rng.uniform(0.3, 1.0)), smooth_iterations=3) for i in range(n)]
+    print(f"[ray] placed stroke of {n} dabs, graph={graph}", flush=True)
+    p.stroke_placed(dabs, starts, dirs, graph=graph)
+    got = p.placements(n)
+    want = _oracle_placed(sg, po, o, dabs, starts, dirs)
+    assert want[:, 6].sum() >= n - n // 9 - 1 and want[:, 6].sum() < n  # hits and misses both present
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    pp, op = p.positions(), o.positions()
+    print("position bits differing:", n_bit_diff(pp, op))
+    assert rel_err(pp, op) <= REL_TOL and rel_err(p.normals(), o.normals()) <= REL_TOL
+    p.close()
+    o.close()
+
+
+def test_prepared_placed_stroke_replays(sg, po):
+    """A captured placed stroke launched twice from a checkpoint gives the same bits as
+    the direct call; the placements follow the mesh it runs on."""
+    co, tris = sg.noisy_icosphere(4, seed=9, amplitude=0.02)
+    p = sg.PBVH(co, tris, leaf_limit=64)
+    n = 20
+    radii = np.full(n, 0.12, F32)
+    starts, dirs = _camera_rays(sg, n, 3, radii, miss_every=0)
+    dabs = [sg.make_dab(tool=[1, 2, 4, 5][i % 4], location=(0, 0, 0), radius=0.12, strength=0.6) for i in range(n)]
+    p.snapshot()
+    p.stroke_placed(dabs, starts, dirs)
+    ref_pos, ref_pl = p.positions(), p.placements(n)
+    st = p.prepare_placed(dabs, starts, dirs)
+    for _ in range(2):
+        p.restore()
+        st.launch()
+        assert np.array_equal(p.positions().view(np.uint32), ref_pos.view(np.uint32))
+        assert np.array_equal(p.placements(n).view(np.uint32), ref_pl.view(np.uint32))
+    st.launch()  # on the already sculpted mesh: same rays, new surface, new centres
+    assert not np.array_equal(p.placements(n), ref_pl)
+    p.close()
