@@ -84,6 +84,12 @@ FLT_MAX = float(np.finfo(np.float32).max)
 
 SearchCB = C.CFUNCTYPE(C.c_bool, C.c_void_p, C.c_void_p)
 HitCB = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p)
+OccludedCB = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p, C.POINTER(C.c_float))
+
+
+class IsectRayPrecalc(C.Structure):
+    _fields_ = [("kx", C.c_int), ("ky", C.c_int), ("kz", C.c_int), ("sx", C.c_float), ("sy", C.c_float),
+                ("sz", C.c_float)]
 
 _lib = None
 
@@ -115,7 +121,9 @@ EXPORTS = [
     "sg_batch_read_normals", "sg_batch_snapshot_save", "sg_batch_snapshot_restore", "sg_batch_set_launch",
     "sg_batch_get_launch", "sg_batch_autotune", "sg_batch_set_learned", "sg_batch_timer_start",
     "sg_batch_timer_stop", "sg_hbm_copy_probe", "sg_raycast", "sg_raycast_device", "sg_raycast_stats",
-    "sg_dab_apply_stroke_placed", "sg_stroke_prepare_placed", "sg_stroke_placements",
+    "sg_dab_apply_stroke_placed", "sg_stroke_prepare_placed", "sg_stroke_placements", "BKE_pbvh_raycast",
+    "BKE_pbvh_find_nearest_to_ray", "BKE_pbvh_node_get_tmin", "BKE_pbvh_node_raycast",
+    "BKE_pbvh_node_find_nearest_to_ray", "BKE_pbvh_raycast_project_ray_root", "sg_isect_ray_tri_precalc",
 ]
 SMOOTH_LEARNED = 1 << 30
 NCTR = 24
@@ -238,6 +246,14 @@ def lib():
         "sg_dab_apply_stroke_placed": (i32, [vp, C.POINTER(SgDabParams), C.POINTER(SgRay), i32, i32]),
         "sg_stroke_prepare_placed": (vp, [vp, C.POINTER(SgDabParams), C.POINTER(SgRay), i32, i32]),
         "sg_stroke_placements": (i32, [vp, f32p, i32]),
+        "BKE_pbvh_raycast": (None, [vp, OccludedCB, vp, f32p, f32p, C.c_bool]),
+        "BKE_pbvh_find_nearest_to_ray": (None, [vp, OccludedCB, vp, f32p, f32p, C.c_bool]),
+        "BKE_pbvh_node_get_tmin": (C.c_float, [vp]),
+        "BKE_pbvh_node_raycast": (C.c_bool, [vp, vp, vp, C.c_bool, f32p, f32p, C.POINTER(IsectRayPrecalc), f32p,
+                                             i32p, i32p, f32p]),
+        "BKE_pbvh_node_find_nearest_to_ray": (C.c_bool, [vp, vp, vp, C.c_bool, f32p, f32p, f32p, f32p]),
+        "BKE_pbvh_raycast_project_ray_root": (None, [vp, C.c_bool, f32p, f32p, f32p]),
+        "sg_isect_ray_tri_precalc": (None, [C.POINTER(IsectRayPrecalc), f32p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -766,6 +782,74 @@ class PBVH:
         out = (C.c_longlong * 3)()
         _check(lib().sg_raycast_stats(self.h, rays, len(rays), 1 if original else 0, out), "sg_raycast_stats")
         return {"boxes": int(out[0]), "leaves": int(out[1]), "tris": int(out[2])}
+
+    def occluded_order(self, start, direction, mode=0, original=False):
+        """BKE_pbvh_raycast (mode 0) / BKE_pbvh_find_nearest_to_ray (mode 1) with a
+        recording callback: [(node index, tmin)] in visiting order."""
+        L = lib()
+        seen = []
+
+        def rec(node, data, tmin):
+            seen.append((L.sg_pbvh_node_index(self.h, node), L.BKE_pbvh_node_get_tmin(node)))
+
+        cb = OccludedCB(rec)
+        a = np.ascontiguousarray(start, np.float32)
+        b = np.ascontiguousarray(direction, np.float32)
+        fn = L.BKE_pbvh_raycast if mode == 0 else L.BKE_pbvh_find_nearest_to_ray
+        fn(self.h, cb, None, _fp(a), _fp(b), bool(original))
+        return seen
+
+    def raycast_callbacks(self, start, direction, depth=None, original=False, origco=False):
+        """The stroke-location ray cast through the reference's callback API: the hit
+        callback tests a leaf while its tmin is below the bound and lowers the bound to
+        the hit depth.  origco=True passes the leaf's current positions as original
+        coordinates (same result).  Returns (hit, depth, active vertex, face, normal)."""
+        L = lib()
+        a = np.ascontiguousarray(start, np.float32)
+        b = np.ascontiguousarray(direction, np.float32)
+        pre = IsectRayPrecalc()
+        L.sg_isect_ray_tri_precalc(C.byref(pre), _fp(b))
+        st = {"hit": False, "depth": C.c_float(FLT_MAX if depth is None else depth), "av": C.c_int(-1),
+              "af": C.c_int(-1), "fn": (C.c_float * 3)()}
+        pos = self.positions() if origco else None
+
+        def hit_cb(node, data, tmin):
+            if L.BKE_pbvh_node_get_tmin(node) < tmin[0]:
+                oc = None
+                if origco:
+                    verts, _ = self.node_verts(L.sg_pbvh_node_index(self.h, node))
+                    buf = np.ascontiguousarray(pos[verts], np.float32)
+                    st.setdefault("bufs", []).append(buf)
+                    oc = buf.ctypes.data
+                if L.BKE_pbvh_node_raycast(self.h, node, oc, False, _fp(a), _fp(b), C.byref(pre),
+                                           C.byref(st["depth"]), C.byref(st["av"]), C.byref(st["af"]), st["fn"]):
+                    st["hit"] = True
+                    tmin[0] = st["depth"].value
+
+        L.BKE_pbvh_raycast(self.h, OccludedCB(hit_cb), None, _fp(a), _fp(b), bool(original))
+        return st["hit"], st["depth"].value, st["av"].value, st["af"].value, np.array(st["fn"][:], np.float32)
+
+    def nearest_callbacks(self, start, direction, original=False):
+        """BKE_pbvh_find_nearest_to_ray with the nearest callback: (hit, depth, dist_sq)."""
+        L = lib()
+        a = np.ascontiguousarray(start, np.float32)
+        b = np.ascontiguousarray(direction, np.float32)
+        st = {"hit": False, "depth": C.c_float(FLT_MAX), "dsq": C.c_float(FLT_MAX)}
+
+        def near_cb(node, data, tmin):
+            if L.BKE_pbvh_node_get_tmin(node) < tmin[0]:
+                if L.BKE_pbvh_node_find_nearest_to_ray(self.h, node, None, False, _fp(a), _fp(b),
+                                                       C.byref(st["depth"]), C.byref(st["dsq"])):
+                    st["hit"] = True
+                    tmin[0] = st["dsq"].value
+
+        L.BKE_pbvh_find_nearest_to_ray(self.h, OccludedCB(near_cb), None, _fp(a), _fp(b), bool(original))
+        return st["hit"], st["depth"].value, st["dsq"].value
+
+    def project_ray_root(self, start, end, direction, original=False):
+        a, e, d = (np.array(x, np.float32) for x in (start, end, direction))
+        lib().BKE_pbvh_raycast_project_ray_root(self.h, bool(original), _fp(a), _fp(e), _fp(d))
+        return a, e
 
     def search_gather(self, callback=None):
         """BKE_pbvh_search_gather with a Python callback(node_index) -> bool."""

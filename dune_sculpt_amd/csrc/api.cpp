@@ -159,6 +159,8 @@ std::vector<int> flagged_leaves(PBVH *p, int flag) {
 
 }  // namespace
 
+int sg_sync_host_bbs(PBVH *p) { return sync_bbs(p); }
+
 extern "C" {
 
 const char *sg_last_error(void) { return sg_last_error_str(); }

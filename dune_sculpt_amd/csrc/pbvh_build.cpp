@@ -701,7 +701,7 @@ void sg_free_device(PBVH *p) {
                   d.gathered, d.gl_v, d.gl_s, d.aff_v, d.aff_vt, d.aff_vv, d.aff_fade,
                   d.prev_cuts, d.cut_touch, d.halt, d.seq, d.vstamp, d.ctr, d.err, d.vperm_old, d.stamp_v, d.stamp_v2, d.stamp_t,
                   d.stamp_a, d.curve, d.log, d.ltri, d.leaf_toff, d.tri_poly, d.node_parent, d.leaf_node,
-                  d.ray_in, d.ray_out, d.place, d.place_rays};
+                  d.ray_in, d.ray_out, d.place, d.place_rays, d.rl_list, d.rl_args};
   for (void *q : ptrs) {
     if (q) hipFree(q);
   }

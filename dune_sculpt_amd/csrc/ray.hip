@@ -482,6 +482,325 @@ __global__ __launch_bounds__(NT) void k_raycast(SgDevView d, SgRayView rv, const
 }
 
 // ---------------------------------------------------------------------------
+// The reference's callback API: which leaves the occluded search visits, and the leaf
+// tests its callbacks run.
+// ---------------------------------------------------------------------------
+
+// dist_squared_ray_to_aabb_v3_precalc / dist_squared_ray_to_aabb_v3 (math_geom.cc:643-744)
+// and dist_squared_ray_to_seg_v3 with isect_ray_line_v3 (:574-606, :1997-2021).
+struct RayDistPre {
+  float o[3], dir[3], inv[3];
+};
+__device__ __forceinline__ RayDistPre ray_dist_pre(const float o[3], const float dir[3]) {
+  RayDistPre p;
+  for (int i = 0; i < 3; i++) {
+    p.o[i] = o[i];
+    p.dir[i] = dir[i];
+    p.inv[i] = (dir[i] != 0.0f) ? (1.0f / dir[i]) : FLT_MAX;
+  }
+  return p;
+}
+
+__device__ __forceinline__ float dot3(const float a[3], const float b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+__device__ __forceinline__ float ray_seg_dist_sq(const float o[3], const float dir[3], const float v0[3],
+                                                 const float v1[3], float &r_depth) {
+  float a[3], t[3], n[3], pt[3];
+  for (int i = 0; i < 3; i++) {
+    a[i] = v1[i] - v0[i];
+    t[i] = v0[i] - o[i];
+  }
+  n[0] = a[1] * dir[2] - a[2] * dir[1];
+  n[1] = a[2] * dir[0] - a[0] * dir[2];
+  n[2] = a[0] * dir[1] - a[1] * dir[0];
+  const float nlen = dot3(n, n);
+  bool on_v0 = true;
+  float lambda = 0.0f;
+  if (nlen != 0.0f) {
+    float c[3], cray[3];
+    for (int i = 0; i < 3; i++) c[i] = n[i] - t[i];
+    cray[0] = c[1] * dir[2] - c[2] * dir[1];
+    cray[1] = c[2] * dir[0] - c[0] * dir[2];
+    cray[2] = c[0] * dir[1] - c[1] * dir[0];
+    lambda = dot3(cray, n) / nlen;
+    on_v0 = lambda <= 0.0f;
+  }
+  if (on_v0) {
+    for (int i = 0; i < 3; i++) pt[i] = v0[i];
+  } else if (lambda >= 1.0f) {
+    for (int i = 0; i < 3; i++) pt[i] = v1[i];
+  } else {
+    const float s = 1.0f - lambda;
+    for (int i = 0; i < 3; i++) pt[i] = s * v0[i] + lambda * v1[i];
+  }
+  float dv[3];
+  for (int i = 0; i < 3; i++) dv[i] = pt[i] - o[i];
+  const float depth = dot3(dv, dir);
+  r_depth = depth;
+  return dot3(dv, dv) - depth * depth;
+}
+
+__device__ __forceinline__ float ray_box_dist_sq(const RayDistPre &d, const BB6 &b, float &r_depth) {
+  float lo[3], hi[3];
+  for (int i = 0; i < 3; i++) {
+    const bool neg = d.dir[i] < 0.0f;
+    lo[i] = neg ? b.mx[i] : b.mn[i];
+    hi[i] = neg ? b.mn[i] : b.mx[i];
+  }
+  float tmin[3], tmax[3];
+  for (int i = 0; i < 3; i++) {
+    tmin[i] = (lo[i] - d.o[i]) * d.inv[i];
+    tmax[i] = (hi[i] - d.o[i]) * d.inv[i];
+  }
+  float va[3], vb[3], rtmin, rtmax;
+  int main_axis;
+  // the fixed-index writes below keep va/vb in registers
+  va[0] = va[1] = va[2] = vb[0] = vb[1] = vb[2] = 0.0f;
+  if ((tmax[0] <= tmax[1]) && (tmax[0] <= tmax[2])) {
+    rtmax = tmax[0];
+    va[0] = vb[0] = hi[0];
+    main_axis = 3;
+  } else if ((tmax[1] <= tmax[0]) && (tmax[1] <= tmax[2])) {
+    rtmax = tmax[1];
+    va[1] = vb[1] = hi[1];
+    main_axis = 2;
+  } else {
+    rtmax = tmax[2];
+    va[2] = vb[2] = hi[2];
+    main_axis = 1;
+  }
+  if ((tmin[0] >= tmin[1]) && (tmin[0] >= tmin[2])) {
+    rtmin = tmin[0];
+    va[0] = vb[0] = lo[0];
+    main_axis -= 3;
+  } else if ((tmin[1] >= tmin[0]) && (tmin[1] >= tmin[2])) {
+    rtmin = tmin[1];
+    va[1] = vb[1] = lo[1];
+    main_axis -= 1;
+  } else {
+    rtmin = tmin[2];
+    va[2] = vb[2] = lo[2];
+    main_axis -= 2;
+  }
+  if (main_axis < 0) main_axis += 3;
+  if (rtmin <= rtmax) {
+    float dv[3];
+    for (int i = 0; i < 3; i++) dv[i] = hi[i] - d.o[i];
+    r_depth = dot3(dv, d.dir);
+    return 0.0f;
+  }
+  const float dm = comp(d.dir[0], d.dir[1], d.dir[2], main_axis);
+  const float lm = comp(lo[0], lo[1], lo[2], main_axis), hm = comp(hi[0], hi[1], hi[2], main_axis);
+  const float am = dm >= 0.0f ? lm : hm, bm = dm >= 0.0f ? hm : lm;
+  va[0] = main_axis == 0 ? am : va[0];
+  va[1] = main_axis == 1 ? am : va[1];
+  va[2] = main_axis == 2 ? am : va[2];
+  vb[0] = main_axis == 0 ? bm : vb[0];
+  vb[1] = main_axis == 1 ? bm : vb[1];
+  vb[2] = main_axis == 2 ? bm : vb[2];
+  return ray_seg_dist_sq(d.o, d.dir, va, vb, r_depth);
+}
+
+// dist_squared_ray_to_tri_v3_fast (pbvh.c:3968-3990): edges (2,0), (0,1), (1,2); the
+// first always initialises the result.
+__device__ __forceinline__ float ray_tri_edges_dist_sq(const float o[3], const float dir[3], const float c[9],
+                                                       float &r_depth) {
+  float best = FLT_MAX;
+  for (int i = 0, j = 2; i < 3; j = i++) {
+    float dt = FLT_MAX;
+    const float dsq = ray_seg_dist_sq(o, dir, c + 3 * i, c + 3 * j, dt);
+    if (dsq < best || i == 0) {
+      r_depth = dt;
+      best = dsq;
+    }
+  }
+  return best;
+}
+
+// Leaves the occluded search passes for one ray (pbvh_iter_next_occluded, pbvh.c:789-816):
+// mode 0 isect_ray_aabb_v3 (ray_aabb_intersect, :3895-3913), mode 1 depth > 0 of
+// dist_squared_ray_to_aabb_v3 (nearest_to_ray_aabb_dist_sq, :4316-4338).  A leaf is listed
+// with its tmin when it and every ancestor pass; the ancestors are walked whenever the
+// leaf test alone does not imply them (mode 1 always, mode 0 for a non-finite 1/dir).
+// One workgroup; the list is unordered.
+__global__ __launch_bounds__(1024) void k_ray_leaves(SgDevView d, SgRayView rv, const SgRay *ray, int mode,
+                                                     int original, int2 *out, int *count) {
+  __shared__ int s_cut[1024];
+  __shared__ int s_nc;
+  const int tid = threadIdx.x;
+  if (tid == 0) s_nc = 0;
+  __syncthreads();
+  const RayPre r = ray_pre(ray);
+  const RayDistPre dp = ray_dist_pre(r.o, r.dir);
+  const float *cut_bbs = original ? d.cut_obb : d.cut_bb;
+  const float *leaf_bbs = original ? d.leaf_obb : d.leaf_bb;
+  const float *node_bbs = original ? d.node_obb : d.node_bb;
+  auto test = [&](const BB6 &b, float &t) -> bool {
+    if (mode == 0) return ray_box(r, b, t);
+    float depth;
+    t = ray_box_dist_sq(dp, b, depth);
+    return depth > 0.0f;
+  };
+  const bool walk = mode == 1 || r.slow;
+  for (int c = tid; c < d.C; c += 1024) {
+    float t;
+    if (test(bb_load(cut_bbs + 8 * (size_t)c), t)) s_cut[atomicAdd(&s_nc, 1)] = c;
+  }
+  __syncthreads();
+  const int nc = s_nc;
+  for (int k = 0; k < nc; k++) {
+    const int c = s_cut[k];
+    const int lb = d.cut_lbeg[c], le = d.cut_lend[c];
+    for (int l = lb + tid; l < le; l += 1024) {
+      float t;
+      bool pass = test(bb_load(leaf_bbs + 8 * (size_t)l), t);
+      if (pass && walk) {
+        for (int nd = rv.node_parent[rv.leaf_node[l]]; nd >= 0 && pass; nd = rv.node_parent[nd]) {
+          float ta;
+          pass = test(bb_load(node_bbs + 8 * (size_t)nd), ta);
+        }
+      }
+      if (pass) out[atomicAdd(count, 1)] = make_int2(l, __float_as_int(t));
+    }
+  }
+}
+
+// One leaf's triangles against one ray, as the callbacks' node tests do.  mode 0
+// (pbvh_faces_node_raycast, pbvh.c:4041-4107): the first nearest watertight hit below
+// *depth updates depth, face normal, active vertex and face.  mode 1
+// (pbvh_faces_node_nearest_to_ray, :4353-4392): the first smallest edge distance below
+// *dist_sq updates dist_sq and depth.  With oco (node-local original coordinates,
+// AoS) and ofv (the node's face_vert_indices) the corners come from there.
+// args[14]: start xyz, dir xyz, kx ky kz (int bits), sx sy sz, depth, dist_sq.
+// res[0..8]: depth, dist_sq, hit, normal xyz, active vertex, poly, looptri (bits).
+__global__ __launch_bounds__(64) void k_node_ray(SgDevView d, SgRayView rv, const float *args, int mode, int leaf,
+                                                 const float *oco, const int *ofv, float *res) {
+  const int lane = lane_id();
+  RayPre r;
+  for (int i = 0; i < 3; i++) {
+    r.o[i] = args[i];
+    r.dir[i] = args[3 + i];
+  }
+  r.kx = __float_as_int(args[6]);
+  r.ky = __float_as_int(args[7]);
+  r.kz = __float_as_int(args[8]);
+  r.sx = args[9];
+  r.sy = args[10];
+  r.sz = args[11];
+  const float depth_in = args[12], dsq_in = args[13];
+  const int tb = rv.toff[leaf], te = rv.toff[leaf + 1];
+  unsigned long long best = ~0ull;
+  float best_depth = 0.0f;
+  for (int t = tb + lane; t < te; t += SG_WAVE) {
+    const int4 tr = rv.ltri[t];
+    if (d.respect_hide && tr.w < 0) continue;
+    float c[9];
+    if (oco) {
+      const int q = t - tb;
+      for (int j = 0; j < 3; j++) {
+        const int lv = ofv[3 * q + j];
+        c[3 * j] = oco[3 * lv];
+        c[3 * j + 1] = oco[3 * lv + 1];
+        c[3 * j + 2] = oco[3 * lv + 2];
+      }
+    } else {
+      const int cv[3] = {tr.x, tr.y, tr.z};
+      for (int j = 0; j < 3; j++) {
+        c[3 * j] = d.px[cv[j]];
+        c[3 * j + 1] = d.py[cv[j]];
+        c[3 * j + 2] = d.pz[cv[j]];
+      }
+    }
+    if (mode == 0) {
+      float lam;
+      if (ray_tri(r, c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7], c[8], lam)) {
+        const unsigned long long k = hit_key(lam, t);
+        best = k < best ? k : best;
+      }
+    } else {
+      float dep;
+      const float dsq = ray_tri_edges_dist_sq(r.o, r.dir, c, dep);
+      // squared distances can be negative by rounding: order them as signed floats
+      unsigned u = __float_as_uint(dsq);
+      u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+      const unsigned long long k = ((unsigned long long)u << 32) | (unsigned)t;
+      if (k < best) {
+        best = k;
+        best_depth = dep;
+      }
+    }
+  }
+  const unsigned long long wb = wave_min_u64(best);
+  const unsigned long long mine = __ballot(best == wb && wb != ~0ull);
+  if (wb == ~0ull) {
+    if (lane == 0) res[2] = 0.0f;
+    return;
+  }
+  const int owner = __ffsll((long long)mine) - 1;
+  const float dep_owner = __shfl(best_depth, owner);
+  if (lane != 0) return;
+  const int t = (int)(unsigned)wb;
+  const int4 tr = rv.ltri[t];
+  float c[9];
+  int cv[3] = {tr.x, tr.y, tr.z};
+  if (oco) {
+    const int q = t - tb;
+    for (int j = 0; j < 3; j++) {
+      const int lv = ofv[3 * q + j];
+      c[3 * j] = oco[3 * lv];
+      c[3 * j + 1] = oco[3 * lv + 1];
+      c[3 * j + 2] = oco[3 * lv + 2];
+    }
+  } else {
+    for (int j = 0; j < 3; j++) {
+      c[3 * j] = d.px[cv[j]];
+      c[3 * j + 1] = d.py[cv[j]];
+      c[3 * j + 2] = d.pz[cv[j]];
+    }
+  }
+  if (mode == 1) {
+    unsigned u = (unsigned)(wb >> 32);
+    u = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+    const float dsq = __uint_as_float(u);
+    if (!(dsq < dsq_in)) {
+      res[2] = 0.0f;
+      return;
+    }
+    res[0] = dep_owner;
+    res[1] = dsq;
+    res[2] = 1.0f;
+    return;
+  }
+  float lam = 0.0f;
+  ray_tri(r, c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7], c[8], lam);
+  if (!(lam < depth_in)) {
+    res[2] = 0.0f;
+    return;
+  }
+  res[0] = lam;
+  res[2] = 1.0f;
+  sg_tri_normal(c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7], c[8], res[3], res[4], res[5]);
+  float loc[3];
+  for (int k = 0; k < 3; k++) loc[k] = r.o[k] + r.dir[k] * lam;
+  float bx = 0.0f, by = 0.0f, bz = 0.0f;
+  int bv = cv[0];
+  for (int j = 0; j < 3; j++) {
+    const float dj0 = c[3 * j] - loc[0], dj1 = c[3 * j + 1] - loc[1], dj2 = c[3 * j + 2] - loc[2];
+    const float db0 = bx - loc[0], db1 = by - loc[1], db2 = bz - loc[2];
+    if (j == 0 || (dj0 * dj0 + dj1 * dj1 + dj2 * dj2) < (db0 * db0 + db1 * db1 + db2 * db2)) {
+      bx = c[3 * j];
+      by = c[3 * j + 1];
+      bz = c[3 * j + 2];
+      bv = cv[j];
+    }
+  }
+  const int looptri = tr.w & 0x7fffffff;
+  res[6] = __int_as_float(d.vperm_old[bv]);
+  res[7] = __int_as_float(rv.tri_poly[looptri]);
+  res[8] = __int_as_float(looptri);
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 
@@ -640,3 +959,255 @@ int sg_raycast_stats(PBVH *p, const SgRay *rays, int n, int original, long long 
   if (st) hipFree(st);
   return rc;
 }
+
+// ---------------------------------------------------------------------------
+// The reference's callback-driven ray API (host side of the drop-ins)
+// ---------------------------------------------------------------------------
+
+static inline float h_bits_f(int i) {
+  float f;
+  memcpy(&f, &i, 4);
+  return f;
+}
+static inline int h_f_bits(float f) {
+  int i;
+  memcpy(&i, &f, 4);
+  return i;
+}
+
+static int ensure_ray_scratch(PBVH *p) {
+  SgDev &d = p->d;
+  if (d.rl_list) return 0;
+  if (sg_check_hip(hipMalloc(&d.rl_list, (size_t)std::max(d.L, 1) * sizeof(int2)), "hipMalloc")) return -1;
+  if (sg_check_hip(hipMalloc(&d.rl_args, 64 * sizeof(float)), "hipMalloc")) return -1;
+  return 0;
+}
+
+// The occluded search's visiting order over the listed leaves: they enter
+// node_tree_insert (pbvh.c:878-896) in left->right order; a leaf goes left of a node
+// iff its tmin is smaller, so ties and NaN go right.  The in-order result is the
+// stable sort by tmin for the non-NaN leaves, and a NaN leaf sits right after every
+// leaf that entered before it and every later leaf whose tmin is below the largest
+// non-NaN tmin present when it entered: sorting by (tmin, or that largest tmin for a
+// NaN, then entry order) gives exactly the tree's order without building it.
+static void occluded_order(std::vector<int2> &v) {
+  std::sort(v.begin(), v.end(), [](const int2 &a, const int2 &b) { return a.x < b.x; });
+  std::vector<float> key(v.size());
+  float run = -INFINITY;
+  for (size_t i = 0; i < v.size(); i++) {
+    const float t = h_bits_f(v[i].y);
+    if (t != t) {
+      key[i] = run;
+    } else {
+      key[i] = t;
+      run = std::max(run, t);
+    }
+  }
+  std::vector<int> idx(v.size());
+  for (size_t i = 0; i < idx.size(); i++) idx[i] = (int)i;
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return key[a] < key[b]; });
+  std::vector<int2> out(v.size());
+  for (size_t i = 0; i < idx.size(); i++) out[i] = v[idx[i]];
+  v.swap(out);
+}
+
+static int ray_leaves(PBVH *p, const float start[3], const float dir[3], int mode, bool original,
+                      std::vector<int2> &out) {
+  out.clear();
+  if (int rc = sg_ray_ensure(p)) return rc;
+  if (int rc = ensure_ray_buffers(p, 1)) return rc;
+  if (int rc = ensure_ray_scratch(p)) return rc;
+  if (p->nodes_dirty_deferred) {
+    if (int rc = sg_enqueue_flush(p)) return rc;
+    p->nodes_dirty_deferred = false;
+  }
+  SgRay r;
+  memcpy(r.start, start, sizeof(r.start));
+  memcpy(r.normal, dir, sizeof(r.normal));
+  r.depth = FLT_MAX;
+  r.reserved = 0;
+  int *cnt = (int *)(p->d.rl_args + 32);
+  int rc = sg_check_hip(hipMemcpyAsync(p->d.ray_in, &r, sizeof(r), hipMemcpyHostToDevice, p->stream), "copy");
+  if (!rc) rc = sg_check_hip(hipMemsetAsync(cnt, 0, sizeof(int), p->stream), "memset");
+  if (!rc) {
+    hipLaunchKernelGGL(k_ray_leaves, dim3(1), dim3(1024), 0, p->stream, sg_view(p), ray_view(p), p->d.ray_in, mode,
+                       original ? 1 : 0, p->d.rl_list, cnt);
+    rc = sg_check_hip(hipGetLastError(), "ray leaves launch");
+  }
+  int n = 0;
+  if (!rc) rc = sg_check_hip(hipMemcpyAsync(&n, cnt, sizeof(int), hipMemcpyDeviceToHost, p->stream), "copy");
+  if (!rc) rc = sg_check_hip(hipStreamSynchronize(p->stream), "ray leaves");
+  if (rc) return rc;
+  out.resize(n);
+  if (n) rc = sg_check_hip(hipMemcpy(out.data(), p->d.rl_list, (size_t)n * sizeof(int2), hipMemcpyDeviceToHost), "copy");
+  if (!rc) occluded_order(out);
+  return rc;
+}
+
+static void occluded_visit(PBVH *p, const float start[3], const float dir[3], int mode, bool original,
+                           BKE_pbvh_HitOccludedCallback cb, void *data) {
+  if (!p || !cb || p->nodes.empty() || p->no_device || !p->build_ok) return;
+  std::vector<int2> v;
+  if (ray_leaves(p, start, dir, mode, original, v)) return;
+  for (const int2 &e : v) p->nodes[p->leaf_node[e.x]].tmin = h_bits_f(e.y);
+  float tmin = FLT_MAX;  // traverse_tree's bound (pbvh.c:966-968)
+  for (const int2 &e : v) cb(&p->nodes[p->leaf_node[e.x]], data, &tmin);
+}
+
+// One leaf test on the GPU (k_node_ray); res as documented there.
+static bool node_ray(PBVH *p, PBVHNode *node, float (*origco)[3], const float start[3], const float dir[3],
+                     const IsectRayPrecalc *pre, int mode, float depth, float dist_sq, float res[9]) {
+  if (!p || !node || node->leaf < 0 || p->no_device || !p->build_ok) return false;
+  if (node->flag & PBVH_FullyHidden) return false;
+  if (sg_ray_ensure(p) || ensure_ray_scratch(p)) return false;
+  float args[14];
+  memcpy(args, start, 3 * sizeof(float));
+  memcpy(args + 3, dir, 3 * sizeof(float));
+  IsectRayPrecalc own;
+  if (!pre) {
+    sg_isect_ray_tri_precalc(&own, dir);
+    pre = &own;
+  }
+  memcpy(args + 6, &pre->kx, 4);
+  memcpy(args + 7, &pre->ky, 4);
+  memcpy(args + 8, &pre->kz, 4);
+  args[9] = pre->sx;
+  args[10] = pre->sy;
+  args[11] = pre->sz;
+  args[12] = depth;
+  args[13] = dist_sq;
+  float *oco = nullptr;
+  int *ofv = nullptr;
+  int rc = sg_check_hip(hipMemcpyAsync(p->d.rl_args, args, sizeof(args), hipMemcpyHostToDevice, p->stream), "copy");
+  if (!rc && origco) {
+    const int nv = node->uniq_verts + node->face_verts;
+    rc = sg_check_hip(hipMalloc(&oco, (size_t)std::max(nv, 1) * 3 * sizeof(float)), "hipMalloc");
+    if (!rc) rc = sg_check_hip(hipMalloc(&ofv, (size_t)std::max(node->totprim, 1) * 3 * sizeof(int)), "hipMalloc");
+    if (!rc)
+      rc = sg_check_hip(hipMemcpyAsync(oco, origco, (size_t)nv * 3 * sizeof(float), hipMemcpyHostToDevice, p->stream),
+                        "copy");
+    if (!rc)
+      rc = sg_check_hip(hipMemcpyAsync(ofv, node->face_vert_indices, (size_t)node->totprim * 3 * sizeof(int),
+                                       hipMemcpyHostToDevice, p->stream),
+                        "copy");
+  }
+  float *dres = p->d.rl_args + 16;
+  if (!rc) {
+    hipLaunchKernelGGL(k_node_ray, dim3(1), dim3(64), 0, p->stream, sg_view(p), ray_view(p), p->d.rl_args, mode,
+                       node->leaf, oco, ofv, dres);
+    rc = sg_check_hip(hipGetLastError(), "node ray launch");
+  }
+  if (!rc) rc = sg_check_hip(hipMemcpyAsync(res, dres, 9 * sizeof(float), hipMemcpyDeviceToHost, p->stream), "copy");
+  if (!rc) rc = sg_check_hip(hipStreamSynchronize(p->stream), "node ray");
+  if (oco) hipFree(oco);
+  if (ofv) hipFree(ofv);
+  return !rc && res[2] != 0.0f;
+}
+
+extern "C" {
+
+void sg_isect_ray_tri_precalc(IsectRayPrecalc *out, const float dir[3]) {
+  if (!out || !dir) return;
+  const float x = fabsf(dir[0]), y = fabsf(dir[1]), z = fabsf(dir[2]);
+  const int kz = (x > y) ? ((x > z) ? 0 : 2) : ((y > z) ? 1 : 2);
+  int kx = (kz != 2) ? (kz + 1) : 0;
+  int ky = (kx != 2) ? (kx + 1) : 0;
+  if (dir[kz] < 0.0f) std::swap(kx, ky);
+  const float inv_z = 1.0f / dir[kz];
+  out->sx = dir[kx] * inv_z;
+  out->sy = dir[ky] * inv_z;
+  out->sz = inv_z;
+  out->kx = kx;
+  out->ky = ky;
+  out->kz = kz;
+}
+
+void BKE_pbvh_raycast(PBVH *p, BKE_pbvh_HitOccludedCallback cb, void *data, const float ray_start[3],
+                      const float ray_normal[3], bool original) {
+  occluded_visit(p, ray_start, ray_normal, 0, original, cb, data);
+}
+
+void BKE_pbvh_find_nearest_to_ray(PBVH *p, BKE_pbvh_SearchNearestCallback cb, void *data, const float ray_start[3],
+                                  const float ray_normal[3], bool original) {
+  occluded_visit(p, ray_start, ray_normal, 1, original, cb, data);
+}
+
+float BKE_pbvh_node_get_tmin(PBVHNode *node) { return node ? node->tmin : 0.0f; }
+
+bool BKE_pbvh_node_raycast(PBVH *p, PBVHNode *node, float (*origco)[3], bool use_origco, const float ray_start[3],
+                           const float ray_normal[3], IsectRayPrecalc *isect_precalc, float *depth,
+                           int *active_vertex_index, int *active_face_grid_index, float *face_normal) {
+  (void)use_origco;  // dyntopo only
+  if (!depth) return false;
+  float res[9];
+  if (!node_ray(p, node, origco, ray_start, ray_normal, isect_precalc, 0, *depth, FLT_MAX, res)) return false;
+  *depth = res[0];
+  if (face_normal) memcpy(face_normal, res + 3, 3 * sizeof(float));
+  if (active_vertex_index) {
+    *active_vertex_index = h_f_bits(res[6]);
+    if (active_face_grid_index) *active_face_grid_index = h_f_bits(res[7]);
+  }
+  return true;
+}
+
+bool BKE_pbvh_node_find_nearest_to_ray(PBVH *p, PBVHNode *node, float (*origco)[3], bool use_origco,
+                                       const float ray_start[3], const float ray_normal[3], float *depth,
+                                       float *dist_sq) {
+  (void)use_origco;
+  if (!depth || !dist_sq) return false;
+  float res[9];
+  if (!node_ray(p, node, origco, ray_start, ray_normal, nullptr, 1, *depth, *dist_sq, res)) return false;
+  *depth = res[0];
+  *dist_sq = res[1];
+  return true;
+}
+
+// BKE_pbvh_raycast_project_ray_root (pbvh.c:4262-4307): clip the segment to the root box
+// grown by 1e-3 (absolute) and 0.1% (relative); a few FP32 operations on one box.
+void BKE_pbvh_raycast_project_ray_root(PBVH *p, bool original, float ray_start[3], float ray_end[3],
+                                       float ray_normal[3]) {
+  if (!p || p->nodes.empty()) return;
+  sg_sync_host_bbs(p);
+  const float *vb = original ? p->nodes[0].orig_vb : p->nodes[0].vb;
+  const float offset = 1.0f + 1e-3f;
+  float bmin[3], bmax[3];
+  for (int i = 0; i < 3; i++) {
+    const float center = 0.5f * (vb[3 + i] + vb[i]);
+    float diff = vb[3 + i] - center;
+    diff = diff + 1e-3f;
+    bmax[i] = center + diff * offset;
+    bmin[i] = center + diff * -offset;
+  }
+  auto slab = [&](const float o[3], const float dir[3], float *tmin_out) -> bool {
+    float inv[3];
+    int sg[3];
+    for (int i = 0; i < 3; i++) {
+      inv[i] = 1.0f / dir[i];
+      sg[i] = inv[i] < 0.0f;
+    }
+    const float *bb[2] = {bmin, bmax};
+    float tmin = (bb[sg[0]][0] - o[0]) * inv[0];
+    float tmax = (bb[1 - sg[0]][0] - o[0]) * inv[0];
+    const float tymin = (bb[sg[1]][1] - o[1]) * inv[1];
+    const float tymax = (bb[1 - sg[1]][1] - o[1]) * inv[1];
+    if ((tmin > tymax) || (tymin > tmax)) return false;
+    if (tymin > tmin) tmin = tymin;
+    if (tymax < tmax) tmax = tymax;
+    const float tzmin = (bb[sg[2]][2] - o[2]) * inv[2];
+    const float tzmax = (bb[1 - sg[2]][2] - o[2]) * inv[2];
+    if ((tmin > tzmax) || (tzmin > tmax)) return false;
+    if (tzmin > tmin) tmin = tzmin;
+    *tmin_out = tmin;
+    return true;
+  };
+  float t_start, t_end, inv_dir[3];
+  if (!slab(ray_start, ray_normal, &t_start)) return;
+  for (int i = 0; i < 3; i++) inv_dir[i] = ray_normal[i] * -1.0f;
+  if (!slab(ray_end, inv_dir, &t_end)) return;
+  for (int i = 0; i < 3; i++) {
+    ray_start[i] = ray_start[i] + ray_normal[i] * t_start;
+    ray_end[i] = ray_end[i] + inv_dir[i] * t_end;
+  }
+}
+
+}  // extern "C"

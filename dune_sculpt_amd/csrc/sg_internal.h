@@ -30,6 +30,7 @@ struct PBVHNode {
   int *vert_indices;    // original vertex ids: uniq first, then shared
   int (*face_vert_indices)[3];
   PBVHNode *self_check;
+  float tmin;           // set by the occluded searches (BKE_pbvh_node_get_tmin)
   int proxy_count;
   PBVHProxyNode *proxies;
   struct PBVH *owner;
@@ -93,6 +94,8 @@ struct SgDev {
   SgRay *ray_in = nullptr;      // staging for host-side ray lists
   SgRayHit *ray_out = nullptr;
   int ray_cap = 0;
+  int2 *rl_list = nullptr;      // [L] leaves an occluded search passes (callback API)
+  float *rl_args = nullptr;     // [64] one ray's arguments, results and list count
   float *place = nullptr;       // [8 per dab] placements of a ray-placed stroke
   SgRay *place_rays = nullptr;  // the stroke's rays
   int place_cap = 0;
@@ -191,6 +194,7 @@ void sg_free_device(PBVH *p);
 
 // ray cast (ray.hip)
 int sg_ray_ensure(PBVH *p);
+int sg_sync_host_bbs(PBVH *p);  // api.cpp: host node bounds from HBM
 int sg_enqueue_raycast(PBVH *p, const SgRay *rays_dev, int n, int original, int stale, SgRayHit *out_dev,
                        float *place_dev, int grab_pair, long long *stats_dev);
 bool sg_ray_host_needs_refit(const SgRay *rays, int n);

@@ -94,6 +94,15 @@ typedef struct PBVHFrustumPlanes { /* used at pbvh.c:4507-4557 */
 
 typedef bool (*BKE_pbvh_SearchCallback)(PBVHNode *node, void *data); /* pbvh.c:769 */
 typedef void (*BKE_pbvh_HitCallback)(PBVHNode *node, void *data);    /* pbvh.c:864 */
+/* the occluded searches' hit callbacks, called with traverse_tree's bound (pbvh.c:898-912) */
+typedef void (*BKE_pbvh_HitOccludedCallback)(PBVHNode *node, void *data, float *tmin);
+typedef void (*BKE_pbvh_SearchNearestCallback)(PBVHNode *node, void *data, float *tmin);
+/* Watertight ray/triangle precalc; fields as math_geom.cc:1755-1855 uses them (the lib
+ * header is absent). */
+typedef struct IsectRayPrecalc {
+  int kx, ky, kz;
+  float sx, sy, sz;
+} IsectRayPrecalc;
 
 /* ---- 1. drop-in PBVH API ---------------------------------------------------- */
 
@@ -155,6 +164,32 @@ void DUNE_pbvh_respect_hide_set(PBVH *pbvh, bool respect_hide);
 void BKE_pbvh_respect_hide_set(PBVH *pbvh, bool respect_hide);
 /* pbvh.c:4953 PBVHType accessor (BKE_pbvh_type is called throughout paint.c) */
 PBVHType BKE_pbvh_type(const PBVH *pbvh);
+/* pbvh.c:3915 BKE_pbvh_raycast / :4339 BKE_pbvh_find_nearest_to_ray: the leaves the
+ * occluded search passes (every ancestor's box too) are found on the GPU, then cb is
+ * called for each in the reference's order (ascending tmin, ties left->right, the
+ * node_tree_insert placement of NaN) with the bound starting at FLT_MAX. */
+void BKE_pbvh_raycast(PBVH *pbvh, BKE_pbvh_HitOccludedCallback cb, void *data, const float ray_start[3],
+                      const float ray_normal[3], bool original);
+void BKE_pbvh_find_nearest_to_ray(PBVH *pbvh, BKE_pbvh_SearchNearestCallback cb, void *data,
+                                  const float ray_start[3], const float ray_normal[3], bool original);
+/* pbvh.c:929 BKE_pbvh_node_get_tmin: the tmin the last occluded search gave the node */
+float BKE_pbvh_node_get_tmin(PBVHNode *node);
+/* pbvh.c:4203 BKE_pbvh_node_raycast / :4463 BKE_pbvh_node_find_nearest_to_ray: one
+ * leaf's triangles on the GPU (origco: the node's original coordinates, indexed by its
+ * face_vert_indices, as the reference; use_origco is for dyntopo and ignored). */
+bool BKE_pbvh_node_raycast(PBVH *pbvh, PBVHNode *node, float (*origco)[3], bool use_origco,
+                           const float ray_start[3], const float ray_normal[3],
+                           IsectRayPrecalc *isect_precalc, float *depth, int *active_vertex_index,
+                           int *active_face_grid_index, float *face_normal);
+bool BKE_pbvh_node_find_nearest_to_ray(PBVH *pbvh, PBVHNode *node, float (*origco)[3],
+                                       bool use_origco, const float ray_start[3],
+                                       const float ray_normal[3], float *depth, float *dist_sq);
+/* pbvh.c:4262 BKE_pbvh_raycast_project_ray_root */
+void BKE_pbvh_raycast_project_ray_root(PBVH *pbvh, bool original, float ray_start[3],
+                                       float ray_end[3], float ray_normal[3]);
+/* isect_ray_tri_watertight_v3_precalc (math_geom.cc:1755-1780) under this library's
+ * prefix, for callers without the reference's math library. */
+void sg_isect_ray_tri_precalc(IsectRayPrecalc *out, const float ray_direction[3]);
 
 /* ---- 2. brush engine entry points (new; replace the absent do_brush_action) ---- */
 

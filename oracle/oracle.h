@@ -141,6 +141,10 @@ typedef struct OrcRayNearest {
 } OrcRayNearest;
 int orc_find_nearest_to_ray(const OrcPBVH *o, const float start[3], const float dir[3],
                             int original, OrcRayNearest *out);
+/* Leaves in the occluded search's visiting order with their tmin (mode 0 ray cast,
+ * 1 nearest to ray); returns the count. */
+int orc_occluded_order(const OrcPBVH *o, const float start[3], const float dir[3], int original,
+                       int mode, int *nodes, float *tmin, int cap);
 /* BKE_pbvh_raycast_project_ray_root; returns 1 when start/end were moved. */
 int orc_project_ray_root(const OrcPBVH *o, int original, float start[3], float end[3], float dir[3]);
 /* The pieces, for known-answer tests. */

@@ -79,6 +79,7 @@ def lib():
         "orc_raycast": (i32, [vp, f32p, f32p, C.c_float, i32, C.POINTER(OrcRayHit), C.POINTER(C.c_int64)]),
         "orc_find_nearest_to_ray": (i32, [vp, f32p, f32p, i32, C.POINTER(OrcRayNearest)]),
         "orc_project_ray_root": (i32, [vp, i32, f32p, f32p, f32p]),
+        "orc_occluded_order": (i32, [vp, f32p, f32p, i32, i32, i32p, f32p, i32]),
         "orc_isect_ray_tri": (i32, [f32p, f32p, f32p, f32p, f32p, f32p]),
         "orc_isect_ray_box": (i32, [f32p, f32p, f32p, f32p, f32p]),
         "orc_dist_sq_ray_box": (C.c_float, [f32p, f32p, f32p, f32p, f32p]),
@@ -242,6 +243,16 @@ class Oracle:
         r = OrcRayNearest()
         lib().orc_find_nearest_to_ray(self.h, _fp(a), _fp(b), 1 if original else 0, C.byref(r))
         return r
+
+    def occluded_order(self, start, direction, mode=0, original=False):
+        """(node ids, tmins) in the occluded search's visiting order (0 ray cast, 1 nearest)."""
+        a = np.ascontiguousarray(start, np.float32)
+        b = np.ascontiguousarray(direction, np.float32)
+        cap = self.totleaf()
+        nodes = np.zeros(cap, np.int32)
+        tm = np.zeros(cap, np.float32)
+        n = lib().orc_occluded_order(self.h, _fp(a), _fp(b), 1 if original else 0, mode, _ip(nodes), _fp(tm), cap)
+        return nodes[:n].copy(), tm[:n].copy()
 
     def project_ray_root(self, start, end, direction, original=False):
         a = np.array(start, np.float32)

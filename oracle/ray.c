@@ -608,6 +608,30 @@ int orc_find_nearest_to_ray(const OrcPBVH *o, const float start[3], const float 
   return r->hit;
 }
 
+/* --------------------------------------------------- visit order (tests) */
+
+int orc_occluded_order(const OrcPBVH *o, const float start[3], const float dir[3], int original,
+                       int mode, int *nodes, float *tmin, int cap)
+{
+  CastSearch cs;
+  NearSearch ns;
+  raybox_precalc(&cs.box, start, dir);
+  cs.original = original;
+  rayboxdist_precalc(&ns.box, start, dir);
+  ns.original = original;
+  int *leaf;
+  float *lt;
+  const int tot = mode == 0 ? occluded_order(o, cast_scb, &cs, &leaf, &lt, NULL)
+                            : occluded_order(o, near_scb, &ns, &leaf, &lt, NULL);
+  for (int i = 0; i < tot && i < cap; i++) {
+    nodes[i] = leaf[i];
+    tmin[i] = lt[i];
+  }
+  free(leaf);
+  free(lt);
+  return tot;
+}
+
 /* ------------------------------------------------ root projection, pieces */
 
 int orc_project_ray_root(const OrcPBVH *o, int original, float start[3], float end[3], float dir[3])

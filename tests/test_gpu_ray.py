@@ -271,3 +271,61 @@ def test_prepared_placed_stroke_replays(sg, po):
     st.launch()  # on the already sculpted mesh: same rays, new surface, new centres
     assert not np.array_equal(p.placements(n), ref_pl)
     p.close()
+
+
+def _same_tmin(a, b):
+    a, b = np.float32(a), np.float32(b)
+    if np.isnan(a) or np.isnan(b):
+        return bool(np.isnan(a) and np.isnan(b))  # NaN payloads differ between x86 and the GPU
+    return a.view(np.uint32) == b.view(np.uint32)
+
+
+def test_callback_api_matches_oracle(sg, po):
+    """BKE_pbvh_raycast / BKE_pbvh_find_nearest_to_ray with the sculpt operator's
+    callbacks (BKE_pbvh_node_get_tmin, BKE_pbvh_node_raycast,
+    BKE_pbvh_node_find_nearest_to_ray): the visiting order and tmins, and the results,
+    equal the oracle's; the ray cast also equals sg_raycast.  Rays along an axis whose
+    start lies on leaf box faces give NaN tmins (placed as node_tree_insert places them)."""
+    co, tris = sg.noisy_icosphere(4, seed=3, amplitude=0.02)
+    p = sg.PBVH(co, tris, leaf_limit=16)
+    o = po.Oracle(co, tris, leaf_limit=16)
+    starts, dirs, _ = _rays(23, 48)
+    rng = np.random.default_rng(1)
+    extra_s, extra_d = [], []
+    for v in rng.choice(len(co), 16, replace=False):
+        extra_s.append(np.array([co[v, 0], co[v, 1], 3.0], F32))
+        extra_d.append(np.array([0.0, 0.0, -1.0], F32))
+    starts = np.concatenate([starts, np.array(extra_s)])
+    dirs = np.concatenate([dirs, np.array(extra_d)])
+    batch = p.raycast(starts, dirs)
+    n_nan = 0
+    print("[ray] callback API", flush=True)
+    for i in range(len(starts)):
+        s, d = starts[i], dirs[i]
+        for mode in (0, 1):
+            got = p.occluded_order(s, d, mode=mode)
+            wn, wt = o.occluded_order(s, d, mode=mode)
+            assert [g[0] for g in got] == list(wn), (i, mode)
+            assert all(_same_tmin(g[1], t) for g, t in zip(got, wt)), (i, mode)
+            n_nan += int(np.isnan(wt).sum())
+        hit, depth, av, af, fn = p.raycast_callbacks(s, d)
+        w = o.raycast(s, d)
+        assert hit == bool(w.hit) and np.float32(depth).view(np.uint32) == np.float32(w.depth).view(np.uint32), i
+        assert hit == bool(batch[i]["hit"]) and np.float32(depth) == batch[i]["depth"]
+        if hit:
+            assert av == w.active_vertex and af == w.tri and av == batch[i]["active_vertex"]
+            assert np.array_equal(fn.view(np.uint32), np.array(w.face_normal, F32).view(np.uint32))
+            assert p.raycast_callbacks(s, d, origco=True)[:4] == (hit, depth, av, af)
+        nh, ndep, ndsq = p.nearest_callbacks(s, d)
+        wnear = o.nearest_to_ray(s, d)
+        assert nh == bool(wnear.hit), i
+        if nh:
+            assert np.float32(ndsq).view(np.uint32) == np.float32(wnear.dist_sq).view(np.uint32), i
+            assert np.float32(ndep).view(np.uint32) == np.float32(wnear.depth).view(np.uint32), i
+    assert n_nan > 0
+    for s, e, d in (([0.1, 0.2, 5.0], [0.1, 0.2, -5.0], [0, 0, -1.0]), ([3.0, 0.1, 0.2], [-3.0, 0.1, 0.2], [-1.0, 0, 0])):
+        gs, ge = p.project_ray_root(s, e, d)
+        moved, ws, we = o.project_ray_root(s, e, d)
+        assert moved and np.array_equal(gs, ws) and np.array_equal(ge, we)
+    p.close()
+    o.close()
