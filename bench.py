@@ -401,7 +401,7 @@ def placed_rays(sg, centres, seed=11, eye=3.0, jitter=0.05):
     return starts.astype(np.float32), d.astype(np.float32)
 
 
-def run_placed(sg, A, dabs, W, K):
+def run_placed(sg, A, dabs, W, K, only_placed=False):
     """The headline stroke with every dab placed by a GPU ray cast inside the stroke
     graph (dab i's centre = where ray i hits the mesh the previous dabs left) against
     the same dabs at fixed centres; both captured once and launched per step from the
@@ -409,7 +409,7 @@ def run_placed(sg, A, dabs, W, K):
     centres = np.array([list(d.location) for d in dabs], np.float32)
     starts, dirs = placed_rays(sg, centres)
     res = {}
-    for name in ("fixed", "placed"):
+    for name in (("placed",) if only_placed else ("fixed", "placed")):
         prep = A.prepare(dabs) if name == "fixed" else A.prepare_placed(dabs, starts, dirs)
         times = []
         for s in range(W + K):
@@ -426,7 +426,8 @@ def run_placed(sg, A, dabs, W, K):
         prep.close()
         ms = float(np.mean(times))
         res[name] = {"ms_per_stroke": ms, "us_per_dab": ms * 1e3 / len(dabs)}
-    res["placement_us_per_dab"] = res["placed"]["us_per_dab"] - res["fixed"]["us_per_dab"]
+    if "fixed" in res:
+        res["placement_us_per_dab"] = res["placed"]["us_per_dab"] - res["fixed"]["us_per_dab"]
     log(f"placed stroke: {res}")
     return res
 
@@ -502,6 +503,9 @@ def main():
     ap.add_argument("--no-ray", action="store_true", help="skip the ray cast and placed-stroke measurements")
     ap.add_argument("--ray-side", type=int, default=128, help="camera image side (rays per batch = side^2)")
     ap.add_argument("--only-ray", action="store_true", help="run only the ray cast measurements")
+    ap.add_argument("--no-placed", action="store_true", help="skip the placed-stroke measurement")
+    ap.add_argument("--placed-only", action="store_true",
+                    help="placed stroke without the fixed-centre reference stroke (one captured graph)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -531,8 +535,14 @@ def main():
     W, K = args.warmup, args.steps
     A.snapshot()  # the pristine mesh: every step applies the stroke to it
     if args.only_ray:
-        ray, _, _ = run_raycast(sg, A, W, max(K, 5), args.ray_side)
-        print(json.dumps({"raycast": ray, "placed_stroke": run_placed(sg, A, dabs, W, K)}))
+        out = {}
+        if args.copy_mb:
+            ms, gbps_copy = sg.hbm_copy_probe(args.copy_mb << 20)
+            out["hbm_copy"] = {"bytes": args.copy_mb << 20, "ms": ms, "gbps_read_plus_write": gbps_copy}
+        out["raycast"], _, _ = run_raycast(sg, A, W, max(K, 5), args.ray_side)
+        if not args.no_placed:
+            out["placed_stroke"] = run_placed(sg, A, dabs, W, K, args.placed_only)
+        print(json.dumps(out))
         A.close()
         return
     if args.only_learned:
@@ -622,7 +632,7 @@ def main():
     if not args.no_ray:
         A.restore()
         ray, ray_starts, ray_dirs = run_raycast(sg, A, W, max(K, 5), args.ray_side)
-        placed = run_placed(sg, A, dabs, W, K)
+        placed = None if args.no_placed else run_placed(sg, A, dabs, W, K)
 
     c3 = None
     if not args.no_learned:

@@ -40,7 +40,7 @@
 // Workgroup sizes: a batch of rays runs 256 threads per ray (several rays in flight per
 // CU hide each ray's dependent load levels); a single placement ray runs 1024.
 #define RAY_LEAF_UNROLL 2
-#define RAY_HITS 1024  // leaves with a hit, per ray (both sizes: the same limit)
+#define RAY_HITS 512  // leaves with a hit, per ray (both sizes: the same limit)
 
 struct SgRayView {
   const int4 *ltri;        // [T] tris in leaf order: new corner ids, .w = looptri | hidden << 31
@@ -168,21 +168,26 @@ __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long k)
 
 template <int NT>
 struct RayShared {
-  static constexpr int kCand = NT * RAY_LEAF_UNROLL;  // one round of leaves
-  int cut[1024];
+  static constexpr int kRound = NT * RAY_LEAF_UNROLL;  // leaves tested per round
+  static constexpr int kCand = kRound + 256;            // passing leaves held before their scan
+  int cut_lb[1024];                                     // passing subtrees: first leaf, leaf count
+  int cut_n[1024];
   int pre[1025];
-  int cand_leaf[kCand];
+  int cand_leaf[kCand];  // passing leaves of the round: leaf, tmin, triangle range
   float cand_t[kCand];
-  int hit_leaf[RAY_HITS];
+  int cand_tb[kCand];
+  int cand_te[kCand];
+  int hit_leaf[RAY_HITS];  // leaves with a hit: leaf, tmin, key, exact depth of their hit
   float hit_t[RAY_HITS];
   unsigned long long hit_key[RAY_HITS];
+  float hit_lam[RAY_HITS];
   int order[RAY_HITS];
   int nc, ncand, nhit, status;
   SgRayHit res[2];
 };
 
 // Scan the listed candidate leaves' tris (a wave per leaf); leaves with a hit below
-// the depth bound go to the hit list.
+// the depth bound go to the hit list with the exact depth of that hit.
 template <int NT>
 __device__ __forceinline__ void scan_candidates(const SgDevView &d, const SgRayView &rv, const RayPre &r,
                                                 RayShared<NT> &S, int ncand, long long &ntris) {
@@ -190,8 +195,9 @@ __device__ __forceinline__ void scan_candidates(const SgDevView &d, const SgRayV
   const int wave = threadIdx.x / SG_WAVE;
   for (int ci = wave; ci < ncand; ci += NT / SG_WAVE) {
     const int l = S.cand_leaf[ci];
-    const int tb = rv.toff[l], te = rv.toff[l + 1];
+    const int tb = S.cand_tb[ci], te = S.cand_te[ci];
     unsigned long long best = ~0ull;
+    float best_lam = 0.0f;
     for (int t0 = tb; t0 < te; t0 += 2 * SG_WAVE) {
       int4 tr[2];
       bool ok[2];
@@ -226,18 +232,22 @@ __device__ __forceinline__ void scan_candidates(const SgDevView &d, const SgRayV
           float lam;
           if (ray_tri(r, c[j][0], c[j][1], c[j][2], c[j][3], c[j][4], c[j][5], c[j][6], c[j][7], c[j][8], lam)) {
             const unsigned long long k = hit_key(lam, t0 + j * SG_WAVE + lane);
-            best = k < best ? k : best;
+            if (k < best) {
+              best = k;
+              best_lam = lam;
+            }
           }
         }
       }
     }
-    best = wave_min_u64(best);
-    if (lane == 0 && best != ~0ull && key_depth(best) < r.depth0) {
+    const unsigned long long wb = wave_min_u64(best);
+    if (wb != ~0ull && key_depth(wb) < r.depth0 && best == wb) {  // the one lane holding the minimum
       const int h = atomicAdd(&S.nhit, 1);
       if (h < RAY_HITS) {
         S.hit_leaf[h] = l;
         S.hit_t[h] = S.cand_t[ci];
-        S.hit_key[h] = best;
+        S.hit_key[h] = wb;
+        S.hit_lam[h] = best_lam;
       } else {
         S.status = SG_RAY_OVERFLOW;
       }
@@ -246,9 +256,12 @@ __device__ __forceinline__ void scan_candidates(const SgDevView &d, const SgRayV
 }
 
 // One ray, the whole workgroup; lane 0 of wave 0 leaves the result in S.res[slot].
+// Global load levels: subtree boxes and ranges; leaf boxes and triangle ranges;
+// triangle ids; corner positions; then, unless only the hit point is wanted, the
+// winner's ids and positions again for the face normal, corner and poly.
 template <int NT>
 __device__ __forceinline__ void ray_one(const SgDevView &d, const SgRayView &rv, const SgRay *ray, int original,
-                                        int stale, RayShared<NT> &S, int slot, long long *stats) {
+                                        int stale, RayShared<NT> &S, int slot, long long *stats, bool point_only) {
   const int tid = threadIdx.x;
   const int lane = lane_id();
   const int wave = tid / SG_WAVE;
@@ -265,16 +278,22 @@ __device__ __forceinline__ void ray_one(const SgDevView &d, const SgRayView &rv,
   const float *node_bbs = original ? d.node_obb : d.node_bb;
   long long ntests = 0, ntris = 0, ncands = 0;
 
-  // 1. search-cut subtree boxes
+  // 1. search-cut subtree boxes, with their leaf ranges
   for (int c = tid; c < d.C; c += NT) {
-    if (d.cut_flags[c] & 1) continue;  // a single fully hidden leaf: never hit
-    bool pass = d.cut_touch[c] == stale;
-    if (!pass) {
-      float t;
-      pass = ray_box(r, bb_load(cut_bbs + 8 * (size_t)c), t);
-    }
+    const int cfl = d.cut_flags[c];
+    const int touch = d.cut_touch[c];
+    const BB6 cb = bb_load(cut_bbs + 8 * (size_t)c);
+    const int lb = d.cut_lbeg[c], le = d.cut_lend[c];
+    if (cfl & 1) continue;  // a single fully hidden leaf: never hit
+    float t;
+    // a box not refreshed since the previous dab moved its leaves counts as passing
+    const bool pass = touch == stale || ray_box(r, cb, t);
     ntests++;
-    if (pass) S.cut[atomicAdd(&S.nc, 1)] = c;
+    if (pass) {
+      const int k = atomicAdd(&S.nc, 1);
+      S.cut_lb[k] = lb;
+      S.cut_n[k] = le - lb;
+    }
   }
   __syncthreads();
   const int nc = S.nc;
@@ -284,7 +303,7 @@ __device__ __forceinline__ void ray_one(const SgDevView &d, const SgRayView &rv,
     int sum = 0;
     for (int q = 0; q < per; q++) {
       const int k = lane * per + q;
-      if (k < nc) sum += d.cut_lend[S.cut[k]] - d.cut_lbeg[S.cut[k]];
+      if (k < nc) sum += S.cut_n[k];
     }
     int incl = sum;
     for (int off = 1; off < SG_WAVE; off <<= 1) {
@@ -296,7 +315,7 @@ __device__ __forceinline__ void ray_one(const SgDevView &d, const SgRayView &rv,
       const int k = lane * per + q;
       if (k < nc) {
         S.pre[k] = run;
-        run += d.cut_lend[S.cut[k]] - d.cut_lbeg[S.cut[k]];
+        run += S.cut_n[k];
       }
     }
     if (lane == SG_WAVE - 1) S.pre[nc] = incl;
@@ -304,11 +323,11 @@ __device__ __forceinline__ void ray_one(const SgDevView &d, const SgRayView &rv,
   __syncthreads();
   const int total = S.pre[nc];
 
-  // 2. + 3. leaves in rounds, each round's passing leaves scanned at its end
-  for (int g0 = 0; g0 < total; g0 += NT * RAY_LEAF_UNROLL) {
-    int leaf[RAY_LEAF_UNROLL];
+  // 2. leaves in rounds; 3. the passing leaves' triangles, once at the end (or
+  // whenever the next round could overflow the list)
+  for (int g0 = 0; g0 < total; g0 += RayShared<NT>::kRound) {
+    int leaf[RAY_LEAF_UNROLL], fl[RAY_LEAF_UNROLL], tb[RAY_LEAF_UNROLL], te[RAY_LEAF_UNROLL];
     BB6 b[RAY_LEAF_UNROLL];
-    int fl[RAY_LEAF_UNROLL];
 #pragma unroll
     for (int j = 0; j < RAY_LEAF_UNROLL; j++) {
       const int g = g0 + j * NT + tid;
@@ -322,9 +341,11 @@ __device__ __forceinline__ void ray_one(const SgDevView &d, const SgRayView &rv,
           else
             hi = mid - 1;
         }
-        leaf[j] = d.cut_lbeg[S.cut[lo]] + (g - S.pre[lo]);
+        leaf[j] = S.cut_lb[lo] + (g - S.pre[lo]);
         b[j] = bb_load(leaf_bbs + 8 * (size_t)leaf[j]);
         fl[j] = d.leaf_flags[leaf[j]];
+        tb[j] = rv.toff[leaf[j]];
+        te[j] = rv.toff[leaf[j] + 1];
       }
     }
 #pragma unroll
@@ -346,14 +367,24 @@ __device__ __forceinline__ void ray_one(const SgDevView &d, const SgRayView &rv,
         const int q = atomicAdd(&S.ncand, 1);
         S.cand_leaf[q] = leaf[j];
         S.cand_t[q] = t;
+        S.cand_tb[q] = tb[j];
+        S.cand_te[q] = te[j];
       }
     }
     __syncthreads();
     const int ncand = S.ncand;
+    if (g0 + RayShared<NT>::kRound < total && ncand + RayShared<NT>::kRound > RayShared<NT>::kCand) {
+      ncands += (tid == 0) ? ncand : 0;
+      scan_candidates(d, rv, r, S, ncand, ntris);
+      __syncthreads();
+      if (tid == 0) S.ncand = 0;
+      __syncthreads();
+    }
+  }
+  {
+    const int ncand = S.ncand;
     ncands += (tid == 0) ? ncand : 0;
     scan_candidates(d, rv, r, S, ncand, ntris);
-    __syncthreads();
-    if (tid == 0) S.ncand = 0;
     __syncthreads();
   }
 
@@ -395,40 +426,40 @@ __device__ __forceinline__ void ray_one(const SgDevView &d, const SgRayView &rv,
         }
       }
       if (win >= 0 && h.status == 0) {
-        // the winning triangle again: exact depth (sign of zero included), normal,
-        // nearest corner (pbvh.c:4076-4101)
-        const int pos = (int)(unsigned)S.hit_key[win];
-        const int4 tr = rv.ltri[pos];
-        const int cv[3] = {tr.x, tr.y, tr.z};
-        float co[3][3];
-        for (int j = 0; j < 3; j++) {
-          co[j][0] = d.px[cv[j]];
-          co[j][1] = d.py[cv[j]];
-          co[j][2] = d.pz[cv[j]];
-        }
-        float lam = 0.0f;
-        ray_tri(r, co[0][0], co[0][1], co[0][2], co[1][0], co[1][1], co[1][2], co[2][0], co[2][1], co[2][2], lam);
+        const float lam = S.hit_lam[win];  // the exact depth (sign of zero included)
         h.hit = 1;
         h.depth = lam;
         for (int k = 0; k < 3; k++) h.location[k] = r.o[k] + r.dir[k] * lam;
-        sg_tri_normal(co[0][0], co[0][1], co[0][2], co[1][0], co[1][1], co[1][2], co[2][0], co[2][1], co[2][2],
-                      h.face_normal[0], h.face_normal[1], h.face_normal[2]);
-        float best[3] = {0.0f, 0.0f, 0.0f};
-        int bv = cv[0];
-        for (int j = 0; j < 3; j++) {
-          const float dj0 = co[j][0] - h.location[0], dj1 = co[j][1] - h.location[1], dj2 = co[j][2] - h.location[2];
-          const float db0 = best[0] - h.location[0], db1 = best[1] - h.location[1], db2 = best[2] - h.location[2];
-          if (j == 0 || (dj0 * dj0 + dj1 * dj1 + dj2 * dj2) < (db0 * db0 + db1 * db1 + db2 * db2)) {
-            best[0] = co[j][0];
-            best[1] = co[j][1];
-            best[2] = co[j][2];
-            bv = cv[j];
+        if (!point_only) {
+          // the winning triangle's normal, nearest corner (pbvh.c:4073-4101) and ids
+          const int pos = (int)(unsigned)S.hit_key[win];
+          const int4 tr = rv.ltri[pos];
+          const int cv[3] = {tr.x, tr.y, tr.z};
+          float co[3][3];
+          for (int j = 0; j < 3; j++) {
+            co[j][0] = d.px[cv[j]];
+            co[j][1] = d.py[cv[j]];
+            co[j][2] = d.pz[cv[j]];
           }
+          sg_tri_normal(co[0][0], co[0][1], co[0][2], co[1][0], co[1][1], co[1][2], co[2][0], co[2][1], co[2][2],
+                        h.face_normal[0], h.face_normal[1], h.face_normal[2]);
+          float best[3] = {0.0f, 0.0f, 0.0f};
+          int bv = cv[0];
+          for (int j = 0; j < 3; j++) {
+            const float dj0 = co[j][0] - h.location[0], dj1 = co[j][1] - h.location[1], dj2 = co[j][2] - h.location[2];
+            const float db0 = best[0] - h.location[0], db1 = best[1] - h.location[1], db2 = best[2] - h.location[2];
+            if (j == 0 || (dj0 * dj0 + dj1 * dj1 + dj2 * dj2) < (db0 * db0 + db1 * db1 + db2 * db2)) {
+              best[0] = co[j][0];
+              best[1] = co[j][1];
+              best[2] = co[j][2];
+              bv = cv[j];
+            }
+          }
+          h.active_vertex = d.vperm_old[bv];
+          h.looptri = tr.w & 0x7fffffff;
+          h.active_face = rv.tri_poly[h.looptri];
+          h.node = rv.leaf_node[S.hit_leaf[win]];
         }
-        h.active_vertex = d.vperm_old[bv];
-        h.looptri = tr.w & 0x7fffffff;
-        h.active_face = rv.tri_poly[h.looptri];
-        h.node = rv.leaf_node[S.hit_leaf[win]];
       }
     }
   }
@@ -450,14 +481,14 @@ __device__ __forceinline__ void ray_one(const SgDevView &d, const SgRayView &rv,
 // the dab centre and, for Grab, ray 1's hit minus ray 0's gives the translation.
 // place[8]: cx cy cz gx gy gz hit 0.
 template <int NT>
-__global__ __launch_bounds__(NT) void k_raycast(SgDevView d, SgRayView rv, const SgRay *rays, int n, int original,
+__attribute__((amdgpu_waves_per_eu(4, 8))) __global__ __launch_bounds__(NT) void k_raycast(SgDevView d, SgRayView rv, const SgRay *rays, int n, int original,
                                                 int stale, SgRayHit *out, float *place, int grab_pair,
                                                 long long *stats) {
   __shared__ RayShared<NT> S;
   if (place) {
     if (blockIdx.x != 0) return;
-    ray_one(d, rv, rays, original, stale, S, 0, stats);
-    if (grab_pair) ray_one(d, rv, rays + 1, original, stale, S, 1, stats);
+    ray_one(d, rv, rays, original, stale, S, 0, stats, true);
+    if (grab_pair) ray_one(d, rv, rays + 1, original, stale, S, 1, stats, true);
     if (threadIdx.x == 0) {
       const SgRayHit &a = S.res[0];
       float g[3] = {0.0f, 0.0f, 0.0f};
@@ -467,15 +498,11 @@ __global__ __launch_bounds__(NT) void k_raycast(SgDevView d, SgRayView rv, const
       const bool hit = a.hit && a.status == 0;
       *(float4 *)place = make_float4(a.location[0], a.location[1], a.location[2], g[0]);
       *(float4 *)(place + 4) = make_float4(g[1], g[2], hit ? 1.0f : 0.0f, 0.0f);
-      if (out) {
-        out[0] = a;
-        if (grab_pair) out[1] = S.res[1];
-      }
     }
     return;
   }
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
-    ray_one(d, rv, rays + i, original, stale, S, 0, stats);
+    ray_one(d, rv, rays + i, original, stale, S, 0, stats, false);
     if (threadIdx.x == 0) out[i] = S.res[0];
     __syncthreads();
   }
@@ -861,7 +888,7 @@ int sg_ray_ensure(PBVH *p) {
   if (rc) {
     void *q[] = {d.ltri, d.leaf_toff, d.tri_poly, d.node_parent, d.leaf_node};
     for (void *x : q)
-      if (x) hipFree(x);
+      if (x) (void)hipFree(x);
     d.ltri = nullptr;
     d.leaf_toff = d.tri_poly = d.node_parent = d.leaf_node = nullptr;
   }
@@ -899,8 +926,8 @@ bool sg_ray_host_needs_refit(const SgRay *rays, int n) {
 static int ensure_ray_buffers(PBVH *p, int n) {
   SgDev &d = p->d;
   if (n <= d.ray_cap) return 0;
-  if (d.ray_in) hipFree(d.ray_in);
-  if (d.ray_out) hipFree(d.ray_out);
+  if (d.ray_in) (void)hipFree(d.ray_in);
+  if (d.ray_out) (void)hipFree(d.ray_out);
   d.ray_in = nullptr;
   d.ray_out = nullptr;
   d.ray_cap = 0;
@@ -956,7 +983,7 @@ int sg_raycast_stats(PBVH *p, const SgRay *rays, int n, int original, long long 
   if (!rc) rc = sg_enqueue_raycast(p, p->d.ray_in, n, original ? 1 : 0, -1, p->d.ray_out, nullptr, 0, st);
   if (!rc) rc = sg_check_hip(hipMemcpyAsync(out, st, 3 * sizeof(long long), hipMemcpyDeviceToHost, p->stream), "copy");
   if (!rc) rc = sg_check_hip(hipStreamSynchronize(p->stream), "raycast");
-  if (st) hipFree(st);
+  if (st) (void)hipFree(st);
   return rc;
 }
 
@@ -1099,8 +1126,8 @@ static bool node_ray(PBVH *p, PBVHNode *node, float (*origco)[3], const float st
   }
   if (!rc) rc = sg_check_hip(hipMemcpyAsync(res, dres, 9 * sizeof(float), hipMemcpyDeviceToHost, p->stream), "copy");
   if (!rc) rc = sg_check_hip(hipStreamSynchronize(p->stream), "node ray");
-  if (oco) hipFree(oco);
-  if (ofv) hipFree(ofv);
+  if (oco) (void)hipFree(oco);
+  if (ofv) (void)hipFree(ofv);
   return !rc && res[2] != 0.0f;
 }
 

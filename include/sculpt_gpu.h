@@ -330,7 +330,7 @@ typedef struct SgRayHit {
   int active_face;      /* MLoopTri.poly of the hit triangle */
   int looptri;          /* looptri index of the hit triangle */
   int node;             /* PBVH node index of its leaf */
-  int status;           /* 0, or SG_RAY_OVERFLOW (more than 1024 leaves hit: no result) */
+  int status;           /* 0, or SG_RAY_OVERFLOW (more than 512 leaves hit: no result) */
   int reserved;
 } SgRayHit;
 enum { SG_RAY_OVERFLOW = -2 };
