@@ -432,6 +432,43 @@ def run_placed(sg, A, dabs, W, K, only_placed=False):
     return res
 
 
+def run_draw(sg, A, dabs, W, K):
+    """Draw buffers of the whole mesh in the reference's 36-byte vertex layout, built in
+    HBM (one face set): a full refill timed with HIP events, then the incremental
+    refill of the leaves one stroke gathered.  Algorithmic bytes of a full refill: 108
+    written per visible tri, 28 read per tri (corner ids, slot, polygon, face set) and
+    24 per vertex (position, normal) once."""
+    A.restore()
+    P = A.tottri
+    A.draw_build(np.ones(P, np.int32))
+    info = A.draw_info()
+    times = []
+    for s in range(W + K):
+        A.sync()
+        A.timer_start()
+        sg.lib().sg_pbvh_draw_buffers_update(A.h, 1)
+        ms = A.timer_stop()
+        if s >= W:
+            times.append(ms)
+    full_ms = float(np.mean(times))
+    alg = info["visible_tris"] * (108 + 28) + A.totvert * 24
+    prep = A.prepare(dabs)
+    prep.launch(sync=True)
+    A.timer_start()
+    sg.lib().sg_pbvh_draw_buffers_update(A.h, 0)
+    inc_ms = A.timer_stop()
+    prep.close()
+    gbps = alg / (full_ms * 1e-3) / 1e9
+    res = {"visible_tris": info["visible_tris"], "vbo_bytes": info["bytes"], "full_refill_ms": full_ms,
+           "incremental_refill_ms_after_stroke": inc_ms,
+           "roofline": {"bound": "hbm", "kernel": "k_draw_fill (full refill)", "achieved": round(gbps, 3),
+                        "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(gbps / HBM_PEAK_GBPS, 6),
+                        "alg_bytes_per_launch": alg, "avg_launch_us": round(full_ms * 1e3, 3)}}
+    sg.lib().sg_pbvh_draw_buffers_free(A.h)
+    log(f"draw buffers: {res}")
+    return res
+
+
 def cpu_raycast(o, starts, dirs, budget_s):
     """The oracle's sequential ray cast (the reference's traversal and callback) on the
     same camera rays, one host thread, until the budget is spent."""
@@ -504,6 +541,7 @@ def main():
     ap.add_argument("--ray-side", type=int, default=128, help="camera image side (rays per batch = side^2)")
     ap.add_argument("--only-ray", action="store_true", help="run only the ray cast measurements")
     ap.add_argument("--no-placed", action="store_true", help="skip the placed-stroke measurement")
+    ap.add_argument("--no-draw", action="store_true", help="skip the draw-buffer measurement")
     ap.add_argument("--placed-only", action="store_true",
                     help="placed stroke without the fixed-centre reference stroke (one captured graph)")
     args = ap.parse_args()
@@ -542,6 +580,8 @@ def main():
         out["raycast"], _, _ = run_raycast(sg, A, W, max(K, 5), args.ray_side)
         if not args.no_placed:
             out["placed_stroke"] = run_placed(sg, A, dabs, W, K, args.placed_only)
+        if not args.no_draw:
+            out["draw_buffers"] = run_draw(sg, A, dabs, W, max(K, 3))
         print(json.dumps(out))
         A.close()
         return
@@ -633,6 +673,7 @@ def main():
         A.restore()
         ray, ray_starts, ray_dirs = run_raycast(sg, A, W, max(K, 5), args.ray_side)
         placed = None if args.no_placed else run_placed(sg, A, dabs, W, K)
+    draw = None if (args.no_ray or args.no_draw) else run_draw(sg, A, dabs, W, max(K, 3))
 
     c3 = None
     if not args.no_learned:
@@ -696,6 +737,7 @@ def main():
         "batch": batch,
         "raycast": ray,
         "placed_stroke": placed,
+        "draw_buffers": draw,
 
         "build_s": build_s,
         "autotune": tune,

@@ -32,6 +32,7 @@ PBVH_UpdateOriginalBB = 1 << 3
 PBVH_UpdateRedraw = 1 << 5
 PBVH_FullyHidden = 1 << 9
 ME_HIDE = 1 << 4
+ME_SMOOTH = 1 << 0
 
 TOOL_NAMES = {TOOL_DRAW: "draw", TOOL_SMOOTH: "smooth", TOOL_INFLATE: "inflate", TOOL_GRAB: "grab"}
 
@@ -124,6 +125,8 @@ EXPORTS = [
     "sg_dab_apply_stroke_placed", "sg_stroke_prepare_placed", "sg_stroke_placements", "BKE_pbvh_raycast",
     "BKE_pbvh_find_nearest_to_ray", "BKE_pbvh_node_get_tmin", "BKE_pbvh_node_raycast",
     "BKE_pbvh_node_find_nearest_to_ray", "BKE_pbvh_raycast_project_ray_root", "sg_isect_ray_tri_precalc",
+    "sg_pbvh_draw_buffers_build", "sg_pbvh_draw_buffers_update", "sg_pbvh_draw_buffers_info",
+    "sg_pbvh_draw_buffer_node", "sg_pbvh_draw_buffer_read", "sg_pbvh_draw_buffers_free",
 ]
 SMOOTH_LEARNED = 1 << 30
 NCTR = 24
@@ -254,6 +257,12 @@ def lib():
         "BKE_pbvh_node_find_nearest_to_ray": (C.c_bool, [vp, vp, vp, C.c_bool, f32p, f32p, f32p, f32p]),
         "BKE_pbvh_raycast_project_ray_root": (None, [vp, C.c_bool, f32p, f32p, f32p]),
         "sg_isect_ray_tri_precalc": (None, [C.POINTER(IsectRayPrecalc), f32p]),
+        "sg_pbvh_draw_buffers_build": (i32, [vp, i32p, i32, i32, i32]),
+        "sg_pbvh_draw_buffers_update": (i32, [vp, i32]),
+        "sg_pbvh_draw_buffers_info": (i32, [vp, C.POINTER(C.c_void_p), C.POINTER(C.c_longlong), i32p]),
+        "sg_pbvh_draw_buffer_node": (i32, [vp, vp, C.POINTER(C.c_longlong), i32p, i32p, i32p]),
+        "sg_pbvh_draw_buffer_read": (C.c_longlong, [vp, vp, vp, C.c_longlong]),
+        "sg_pbvh_draw_buffers_free": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -525,9 +534,13 @@ class PreparedStroke:
 class PBVH:
     """Owner of one GPU-resident PBVH (BKE_pbvh_new/build_mesh ... BKE_pbvh_free)."""
 
-    def __init__(self, co, tris, leaf_limit: int = 128, hidden=None, mat=None):
+    def __init__(self, co, tris, leaf_limit: int = 128, hidden=None, mat=None, polys=None):
+        """polys (optional): (poly_off, poly_verts, smooth flags, mat_nr) -- the mesh's
+        polygons, fan-triangulated into looptris; then `tris` is ignored."""
         L = lib()
         co = np.ascontiguousarray(co, np.float32)
+        if polys is not None:
+            return self._init_polys(co, polys, leaf_limit, hidden)
         tris = np.ascontiguousarray(tris, np.int32)
         self.totvert = co.shape[0]
         self.tottri = tris.shape[0]
@@ -564,6 +577,72 @@ class PBVH:
         info = self.tree_info()
         if info["nodes"] == 0 and self.tottri > 0:
             raise RuntimeError("BKE_pbvh_build_mesh failed: " + err)
+
+    def _init_polys(self, co, polys, leaf_limit, hidden):
+        L = lib()
+        off, pv, smooth, mat = (np.asarray(x) for x in polys)
+        off = off.astype(np.int64)
+        P = len(off) - 1
+        self.totvert = co.shape[0]
+        tri_poly, tri_loops = fan_looptris(off)
+        self.tottri = len(tri_poly)
+        self.tri_poly = tri_poly
+        self._strokes = []
+        self.h = L.BKE_pbvh_new()
+        if not self.h:
+            raise RuntimeError("BKE_pbvh_new failed: " + last_error())
+        _check(L.sg_pbvh_leaf_limit_set(self.h, leaf_limit), "leaf limit")
+        self._mvert = (MVert * self.totvert)()
+        mv = np.frombuffer(self._mvert, dtype=np.uint8).reshape(self.totvert, 16)
+        mv[:, :12] = co.view(np.uint8).reshape(self.totvert, 12)
+        if hidden is not None:
+            mv[:, 12] = np.where(np.asarray(hidden, bool), ME_HIDE, 0).astype(np.uint8)
+        self._mloop = np.zeros((len(pv), 2), np.uint32)
+        self._mloop[:, 0] = np.asarray(pv, np.uint32)
+        self._mpoly = np.zeros(P, dtype=[("loopstart", "<i4"), ("totloop", "<i4"), ("mat_nr", "<i2"),
+                                          ("flag", "u1"), ("_pad", "u1")])
+        self._mpoly["loopstart"] = off[:-1]
+        self._mpoly["totloop"] = np.diff(off)
+        self._mpoly["mat_nr"] = np.asarray(mat, np.int16)
+        self._mpoly["flag"] = np.where(np.asarray(smooth, bool), ME_SMOOTH, 0).astype(np.uint8)
+        libc = C.CDLL(None)
+        libc.malloc.restype = C.c_void_p
+        libc.malloc.argtypes = [C.c_size_t]
+        lt_ptr = libc.malloc(max(self.tottri, 1) * 16)
+        lt = np.frombuffer((C.c_uint32 * (max(self.tottri, 1) * 4)).from_address(lt_ptr), np.uint32).reshape(-1, 4)
+        lt[: self.tottri, :3] = tri_loops
+        lt[: self.tottri, 3] = tri_poly
+        L.BKE_pbvh_build_mesh(self.h, None, self._mpoly.ctypes.data, self._mloop.ctypes.data,
+                              C.cast(self._mvert, C.c_void_p), self.totvert, None, None, None, lt_ptr, self.tottri)
+        self.tris = np.asarray(pv, np.int32)[tri_loops]
+        if self.tree_info()["nodes"] == 0 and self.tottri > 0:
+            raise RuntimeError("BKE_pbvh_build_mesh failed: " + last_error())
+
+    # -- draw buffers
+    def draw_build(self, face_sets, seed=0, default=1):
+        fs = np.ascontiguousarray(face_sets, np.int32)
+        _check(lib().sg_pbvh_draw_buffers_build(self.h, _ip(fs), len(fs), seed, default), "draw build")
+
+    def draw_update(self, all=False):
+        _check(lib().sg_pbvh_draw_buffers_update(self.h, 1 if all else 0), "draw update")
+
+    def draw_info(self):
+        ptr, nb, nt = C.c_void_p(), C.c_longlong(), C.c_int()
+        _check(lib().sg_pbvh_draw_buffers_info(self.h, C.byref(ptr), C.byref(nb), C.byref(nt)), "draw info")
+        return {"bytes": nb.value, "visible_tris": nt.value}
+
+    def draw_leaf(self, index):
+        """(records as a (verts, 36) uint8 array, smooth, material) of one leaf."""
+        L = lib()
+        node = self.node(index)
+        off, nv, sm, mt = C.c_longlong(), C.c_int(), C.c_int(), C.c_int()
+        _check(L.sg_pbvh_draw_buffer_node(self.h, node, C.byref(off), C.byref(nv), C.byref(sm), C.byref(mt)),
+               "draw node")
+        out = np.zeros((nv.value, 36), np.uint8)
+        n = L.sg_pbvh_draw_buffer_read(self.h, node, out.ctypes.data, out.nbytes)
+        if n < 0:
+            raise RuntimeError("draw read failed: " + last_error())
+        return out, sm.value, mt.value
 
     def close(self):
         for st in getattr(self, "_strokes", []):
@@ -865,6 +944,19 @@ class PBVH:
         if tot.value:
             C.CDLL(None).free(arr)
         return res
+
+
+def fan_looptris(poly_off):
+    """Fan triangulation of polygons given by corner offsets: (tri -> poly, the three
+    loop indices of each tri)."""
+    off = np.asarray(poly_off, np.int64)
+    n = np.diff(off)
+    tri_poly = np.repeat(np.arange(len(n)), np.maximum(n - 2, 0)).astype(np.uint32)
+    loops = []
+    for q in range(len(n)):
+        for k in range(1, n[q] - 1):
+            loops.append((off[q], off[q] + k, off[q] + k + 1))
+    return tri_poly, np.array(loops, np.uint32).reshape(-1, 3)
 
 
 def make_rays(starts, normals, depths=None):

@@ -250,6 +250,39 @@ void BKE_pbvh_build_mesh(PBVH *p, Mesh *mesh, const MPoly *mpoly, const MLoop *m
   for (int t = 0; t < looptri_num; t++) {
     for (int j = 0; j < 3; j++) p->tris[3 * (size_t)t + j] = (int)mloop[looptri[t].tri[j]].v;
   }
+  // polygons as the draw buffers need them (corner lists, smooth flag, material);
+  // without mpoly each looptri is its own flat polygon
+  p->poly_off.clear();
+  p->poly_verts.clear();
+  p->poly_smooth.clear();
+  p->poly_mat.clear();
+  {
+    int npoly = 0;
+    for (int t = 0; t < looptri_num; t++) npoly = std::max(npoly, (int)looptri[t].poly + 1);
+    p->poly_off.assign(npoly + 1, 0);
+    p->poly_smooth.assign(npoly, 0);
+    p->poly_mat.assign(npoly, 0);
+    if (mpoly) {
+      for (int q = 0; q < npoly; q++) {
+        p->poly_off[q + 1] = p->poly_off[q] + mpoly[q].totloop;
+        p->poly_smooth[q] = (mpoly[q].flag & ME_SMOOTH) ? 1 : 0;
+        p->poly_mat[q] = mpoly[q].mat_nr;
+      }
+      p->poly_verts.resize(p->poly_off[npoly]);
+      for (int q = 0; q < npoly; q++)
+        for (int k = 0; k < mpoly[q].totloop; k++)
+          p->poly_verts[p->poly_off[q] + k] = (int)mloop[mpoly[q].loopstart + k].v;
+    } else {
+      std::vector<int> first(npoly, -1);
+      for (int t = 0; t < looptri_num; t++)
+        if (first[looptri[t].poly] < 0) first[looptri[t].poly] = t;
+      for (int q = 0; q < npoly; q++) p->poly_off[q + 1] = p->poly_off[q] + (first[q] >= 0 ? 3 : 0);
+      p->poly_verts.resize(p->poly_off[npoly]);
+      for (int q = 0; q < npoly; q++)
+        if (first[q] >= 0)
+          for (int k = 0; k < 3; k++) p->poly_verts[p->poly_off[q] + k] = p->tris[3 * (size_t)first[q] + k];
+    }
+  }
   p->tri_mat.clear();
   if (mpoly && looptri_num > 0) {
     std::vector<int> key(looptri_num);

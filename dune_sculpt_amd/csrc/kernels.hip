@@ -135,6 +135,7 @@ __global__ __launch_bounds__(256) void k_search(SgDevView d, SgDabK k, long long
           d.gathered[pos] = lb + q0 + j * SG_WAVE + lane;
           d.gl_v[pos] = vr[j];
           d.gl_s[pos] = sr[j];
+          if (d.draw_dirty) d.draw_dirty[lb + q0 + j * SG_WAVE + lane] = 1;  // UpdateDrawBuffers
         }
         off += __popcll(m[j]);
       }
@@ -753,6 +754,7 @@ SgDevView sg_view_of(SgDev &d, bool respect_hide, bool halt_live) {
   v.vhide = d.vhide;
   v.vmark = d.vmark;
   v.gathered = d.gathered;
+  v.draw_dirty = d.draw_dirty;
   v.gl_v = d.gl_v;
   v.gl_s = d.gl_s;
   v.aff_v = d.aff_v;

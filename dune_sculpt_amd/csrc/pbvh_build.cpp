@@ -676,6 +676,7 @@ void sg_build_device(PBVH *p, const float *co) {
   alloc_fill(&d.vmark, V, 0, s);
   // scratch
   alloc_fill(&d.gathered, L, 0, s);
+  alloc_fill(&d.draw_dirty, L, 0, s);  // leaves the dabs gathered since the last draw refill
   alloc_fill(&d.gl_v, L, 0, s);
   alloc_fill(&d.gl_s, L, 0, s);
   alloc_fill(&d.aff_v, V, 0, s);
@@ -701,7 +702,9 @@ void sg_free_device(PBVH *p) {
                   d.gathered, d.gl_v, d.gl_s, d.aff_v, d.aff_vt, d.aff_vv, d.aff_fade,
                   d.prev_cuts, d.cut_touch, d.halt, d.seq, d.vstamp, d.ctr, d.err, d.vperm_old, d.stamp_v, d.stamp_v2, d.stamp_t,
                   d.stamp_a, d.curve, d.log, d.ltri, d.leaf_toff, d.tri_poly, d.node_parent, d.leaf_node,
-                  d.ray_in, d.ray_out, d.place, d.place_rays, d.rl_list, d.rl_args};
+                  d.ray_in, d.ray_out, d.place, d.place_rays, d.rl_list, d.rl_args,
+                  d.draw_dirty, d.draw_tri, d.draw_off, d.draw_leaf, d.draw_fset, d.draw_poly_off,
+                  d.draw_poly_v, d.draw_vbo};
   for (void *q : ptrs) {
     if (q) hipFree(q);
   }

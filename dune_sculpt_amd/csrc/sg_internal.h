@@ -96,6 +96,16 @@ struct SgDev {
   int ray_cap = 0;
   int2 *rl_list = nullptr;      // [L] leaves an occluded search passes (callback API)
   float *rl_args = nullptr;     // [64] one ray's arguments, results and list count
+  // draw buffers (built on request): visible tris per leaf, their vertex records
+  int *draw_dirty = nullptr;    // [L] leaves whose draw buffer is stale (set by the search; lives with the PBVH)
+  int *draw_tri = nullptr;      // [visible tris] position in ltri, leaf after leaf
+  int *draw_off = nullptr;      // [L+1] each leaf's first visible tri
+  int *draw_leaf = nullptr;     // [L] bit0 smooth (first face), bits 8+ material
+  int *draw_fset = nullptr;     // [polys] face sets (copy)
+  int *draw_poly_off = nullptr, *draw_poly_v = nullptr;  // polygon corners (new ids)
+  uint8_t *draw_vbo = nullptr;  // 36 bytes per vertex, 3 per visible tri
+  long long draw_bytes = 0;
+  int draw_ntri = 0, draw_seed = 0, draw_fset_default = 0;
   float *place = nullptr;       // [8 per dab] placements of a ray-placed stroke
   SgRay *place_rays = nullptr;  // the stroke's rays
   int place_cap = 0;
@@ -130,6 +140,9 @@ struct PBVH {
   std::vector<int> perm_old_of_new;
   std::vector<int> tris;             // original corner vertex ids [T*3]
   std::vector<int> tri_mat;          // material key per tri (empty = one material)
+  std::vector<int> poly_off, poly_verts;  // polygons: corner lists (original vertex ids)
+  std::vector<uint8_t> poly_smooth;       // ME_SMOOTH per polygon
+  std::vector<int> poly_mat;              // mat_nr per polygon
   std::vector<uint8_t> vhide;
   SgCut cut;
   MVert *host_verts = nullptr;       // caller's array (borrowed; duplicated on deform)

@@ -35,6 +35,7 @@ struct SgDevView {
   const int *vhide;
   uint8_t *vmark;
   int *gathered;
+  int *draw_dirty;
   int2 *gl_v, *gl_s;
   int *aff_v;
   int2 *aff_vt, *aff_vv;

@@ -353,6 +353,25 @@ SgStroke *sg_stroke_prepare_placed(PBVH *pbvh, const SgDabParams *dabs, const Sg
  * translation xyz, 1 when the ray hit (else 0), 0.  Returns the number of dabs. */
 int sg_stroke_placements(PBVH *pbvh, float *out, int n);
 
+/* ---- draw buffers (the draw code's per-leaf vertex data, kept in HBM) -----------
+ * What GPU_pbvh_mesh_buffers_build / gpu_pbvh_mesh_buffers_update
+ * (gpu/intern/gpu_buffers.c:174-350, driven by pbvh_update_draw_buffers,
+ * pbvh.c:1251-1367) write for every leaf, in the same vertex layout (36-byte stride:
+ * pos float3 @0, nor short3 @16, msk uchar @22, ac ushort4 @24 (zero), fset uchar3 @32),
+ * written on the GPU into one device arena, leaf after leaf, so a renderer on the same
+ * GPU reads it in place.  face_sets: one int per polygon (CD_SCULPT_FACE_SETS; a tri
+ * is drawn when no corner is hidden and its face set is > 0, as the reference).  */
+int sg_pbvh_draw_buffers_build(PBVH *pbvh, const int *face_sets, int npoly, int seed, int face_set_default);
+/* Refill leaves the dabs gathered since the last update and leaves marked
+ * PBVH_UpdateDrawBuffers / PBVH_RebuildDrawBuffers on the host (all != 0: every leaf). */
+int sg_pbvh_draw_buffers_update(PBVH *pbvh, int all);
+int sg_pbvh_draw_buffers_info(PBVH *pbvh, void **vbo_dev, long long *bytes, int *visible_tris);
+/* One leaf's records in the arena; smooth and material as the reference's buffers hold them. */
+int sg_pbvh_draw_buffer_node(PBVH *pbvh, PBVHNode *node, long long *byte_offset, int *vert_count,
+                             int *smooth, int *material);
+long long sg_pbvh_draw_buffer_read(PBVH *pbvh, PBVHNode *node, void *out, long long cap);
+void sg_pbvh_draw_buffers_free(PBVH *pbvh);
+
 /* ---- learned Smooth (MLP over 1-ring features; BF16 MFMA inference) ----------
  * A Smooth dab with SG_SMOOTH_LEARNED in SgDabParams.flag goes through the model
  * attached to the PBVH; when its error estimate (alpha * RMS(|d|)/r) exceeds the

@@ -157,6 +157,24 @@ float orc_dist_sq_ray_box(const float start[3], const float dir[3], const float 
 float orc_dist_sq_ray_tri(const float start[3], const float dir[3], const float v0[3],
                           const float v1[3], const float v2[3], float *depth);
 
+/* Draw buffer of one leaf (draw.c): the visible tris' vertices in the reference's
+ * 36-byte vertex layout.  Polygons: tri -> poly, poly corner lists (CSR, vertex ids),
+ * smooth flag and material per poly.  mask may be NULL; face_sets per poly (NULL:
+ * nothing is visible, as in the reference).  Returns the vertex count (-1 if cap_verts
+ * is too small). */
+typedef struct OrcPolys {
+  const int *tri_poly;
+  const int *poly_off;
+  const int *poly_verts;
+  const uint8_t *poly_smooth;
+  const int *poly_mat;
+} OrcPolys;
+int orc_draw_leaf(const OrcPBVH *o, int node, const OrcPolys *polys, const float *mask,
+                  const int *face_sets, int seed, int fset_default, unsigned char *out, int cap_verts,
+                  int *r_smooth, int *r_material);
+/* BKE_paint_face_set_overlay_color_get (rgb) */
+void orc_face_set_color(int face_set, int seed, unsigned char out[3]);
+
 /* Threads used by the OpenMP loops (0 = runtime default). */
 void orc_set_threads(int n);
 int orc_get_threads(void);

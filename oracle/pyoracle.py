@@ -36,6 +36,12 @@ class OrcRayNearest(C.Structure):
                 ("tri", C.c_int), ("leaf", C.c_int)]
 
 
+class OrcPolys(C.Structure):
+    _fields_ = [("tri_poly", C.POINTER(C.c_int)), ("poly_off", C.POINTER(C.c_int)),
+                ("poly_verts", C.POINTER(C.c_int)), ("poly_smooth", C.POINTER(C.c_uint8)),
+                ("poly_mat", C.POINTER(C.c_int))]
+
+
 _lib = None
 
 
@@ -80,6 +86,9 @@ def lib():
         "orc_find_nearest_to_ray": (i32, [vp, f32p, f32p, i32, C.POINTER(OrcRayNearest)]),
         "orc_project_ray_root": (i32, [vp, i32, f32p, f32p, f32p]),
         "orc_occluded_order": (i32, [vp, f32p, f32p, i32, i32, i32p, f32p, i32]),
+        "orc_draw_leaf": (i32, [vp, i32, C.POINTER(OrcPolys), f32p, i32p, i32, i32, C.POINTER(C.c_uint8), i32,
+                                i32p, i32p]),
+        "orc_face_set_color": (None, [i32, i32, C.POINTER(C.c_uint8)]),
         "orc_isect_ray_tri": (i32, [f32p, f32p, f32p, f32p, f32p, f32p]),
         "orc_isect_ray_box": (i32, [f32p, f32p, f32p, f32p, f32p]),
         "orc_dist_sq_ray_box": (C.c_float, [f32p, f32p, f32p, f32p, f32p]),
@@ -254,6 +263,23 @@ class Oracle:
         n = lib().orc_occluded_order(self.h, _fp(a), _fp(b), 1 if original else 0, mode, _ip(nodes), _fp(tm), cap)
         return nodes[:n].copy(), tm[:n].copy()
 
+    def draw_leaf(self, node, tri_poly, poly_off, poly_verts, poly_smooth, poly_mat, face_sets, mask=None, seed=0,
+                  default=1):
+        """The reference's draw records of one leaf: ((verts, 36) uint8, smooth, material)."""
+        keep = [np.ascontiguousarray(x, np.int32) for x in (tri_poly, poly_off, poly_verts, poly_mat, face_sets)]
+        sm = np.ascontiguousarray(poly_smooth, np.uint8)
+        pl = OrcPolys(_ip(keep[0]), _ip(keep[1]), _ip(keep[2]), sm.ctypes.data_as(C.POINTER(C.c_uint8)), _ip(keep[3]))
+        cap = 3 * self.tottri
+        out = np.zeros((cap, 36), np.uint8)
+        r_sm, r_mat = C.c_int(), C.c_int()
+        mp = None if mask is None else _fp(np.ascontiguousarray(mask, np.float32))
+        mkeep = None if mask is None else np.ascontiguousarray(mask, np.float32)
+        if mkeep is not None:
+            mp = _fp(mkeep)
+        n = lib().orc_draw_leaf(self.h, node, C.byref(pl), mp, _ip(keep[4]), seed, default,
+                                out.ctypes.data_as(C.POINTER(C.c_uint8)), cap, C.byref(r_sm), C.byref(r_mat))
+        return out[:max(n, 0)].copy(), r_sm.value, r_mat.value
+
     def project_ray_root(self, start, end, direction, original=False):
         a = np.array(start, np.float32)
         e = np.array(end, np.float32)
@@ -319,3 +345,9 @@ def dist_sq_ray_tri(start, direction, v0, v1, v2):
     d = lib().orc_dist_sq_ray_tri(_fp(_f3(start)), _fp(_f3(direction)), _fp(_f3(v0)), _fp(_f3(v1)),
                                   _fp(_f3(v2)), _fp(dep))
     return float(d), float(dep[0])
+
+
+def face_set_color(fs, seed):
+    out = (C.c_uint8 * 3)()
+    lib().orc_face_set_color(fs, seed, out)
+    return bytes(out)
