@@ -484,6 +484,28 @@ def cpu_raycast(o, starts, dirs, budget_s):
             "sample": f"first {n} rays of the same camera image ({dt:.1f}s, C oracle, one thread)"}
 
 
+def cpu_draw(o, tris, budget_s):
+    """The oracle's per-leaf draw fill (the reference's gpu_pbvh_mesh_buffers_update
+    restated) on the same mesh, one host thread, leaf after leaf until the budget."""
+    T = len(tris)
+    tri_poly = np.arange(T, dtype=np.int32)
+    off = np.arange(0, 3 * T + 1, 3, dtype=np.int32)
+    pv = np.ascontiguousarray(tris.reshape(-1), np.int32)
+    sm = np.zeros(T, np.uint8)
+    mt = np.zeros(T, np.int32)
+    fs = np.ones(T, np.int32)
+    n = 0
+    t0 = time.perf_counter()
+    for leaf in o.leaves_dfs():
+        recs, _, _ = o.draw_leaf(int(leaf), tri_poly, off, pv, sm, mt, fs)
+        n += len(recs) // 3
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "tris/s", "cores": 1, "kind": "port",
+            "sample": f"{n} tris of the same mesh's leaves ({dt:.1f}s, C oracle, one thread, Python per leaf)"}
+
+
 def cpu_baseline(sg, co, tris, dabs, budget_s, threads, before=None):
     """The oracle's dab loop on the same mesh; `before(oracle)` runs first on the fresh
     mesh (the ray cast sample shares the build)."""
@@ -686,11 +708,14 @@ def main():
     if not args.no_cpu:
         threads = os.cpu_count() or 1
         log(f"CPU baseline: oracle, {threads} threads, budget {args.cpu_budget}s")
-        ray_cpu = {}
+        ray_cpu, draw_cpu = {}, {}
         before = None
-        if ray is not None:
+        if ray is not None or draw is not None:
             def before(o):
-                ray_cpu.update(cpu_raycast(o, ray_starts, ray_dirs, args.cpu_budget / 4))
+                if ray is not None:
+                    ray_cpu.update(cpu_raycast(o, ray_starts, ray_dirs, args.cpu_budget / 4))
+                if draw is not None:
+                    draw_cpu.update(cpu_draw(o, tris, args.cpu_budget / 4))
         r = cpu_baseline(sg, co, tris, dabs, args.cpu_budget, threads, before)
         cpu = {"value": r["affected"] / r["seconds"], "unit": "vertex-updates/s", "cores": r["threads"],
                "kind": "port",
@@ -703,6 +728,9 @@ def main():
         if ray is not None:
             ray["cpu_baseline"] = ray_cpu
             log(f"CPU ray cast: {ray_cpu}")
+        if draw is not None:
+            draw["cpu_baseline"] = draw_cpu
+            log(f"CPU draw fill: {draw_cpu}")
     A.close()
 
     line = {

@@ -1231,9 +1231,13 @@ void BKE_pbvh_search_callback(PBVH *p, BKE_pbvh_SearchCallback scb, void *search
 
 void BKE_pbvh_node_mark_update(PBVHNode *node) {  // pbvh.c:1723-1727
   node->flag |= PBVH_UpdateNormals | PBVH_UpdateBB | PBVH_UpdateOriginalBB | PBVH_UpdateDrawBuffers | PBVH_UpdateRedraw;
+  if (node->owner) node->owner->draw_marks_pending = true;
 }
 void BKE_pbvh_node_mark_normals_update(PBVHNode *node) { node->flag |= PBVH_UpdateNormals; }
-void BKE_pbvh_node_mark_redraw(PBVHNode *node) { node->flag |= PBVH_UpdateDrawBuffers | PBVH_UpdateRedraw; }
+void BKE_pbvh_node_mark_redraw(PBVHNode *node) {
+  node->flag |= PBVH_UpdateDrawBuffers | PBVH_UpdateRedraw;
+  if (node->owner) node->owner->draw_marks_pending = true;
+}
 void BKE_pbvh_vert_mark_update(PBVH *p, int index) {
   if (p && index >= 0 && index < p->totvert) p->host_vmark[index] = 1;
 }

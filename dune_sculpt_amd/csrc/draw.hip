@@ -89,8 +89,9 @@ __global__ __launch_bounds__(1024) void k_draw_scan(const int *count, int L, int
   if (tid == 1023) off[L] = s_part[1023];
 }
 
-// per leaf: the visible tris' ltri positions, in prim order
-__global__ __launch_bounds__(256) void k_draw_compact(SgDrawView w, int L) {
+// per leaf: the visible tris' ltri positions, in prim order; bit 31 when the tri's
+// polygon is the same triangle in the same corner order (tri_is_poly[looptri])
+__global__ __launch_bounds__(256) void k_draw_compact(SgDrawView w, int L, const uint8_t *tri_is_poly) {
   const int lane = lane_id();
   const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
   const int nw = gridDim.x * blockDim.x / SG_WAVE;
@@ -99,9 +100,11 @@ __global__ __launch_bounds__(256) void k_draw_compact(SgDrawView w, int L) {
     int base = w.doff[l];
     for (int t0 = tb; t0 < te; t0 += SG_WAVE) {
       const int t = t0 + lane;
-      const bool vis = t < te && draw_visible(w, w.ltri[t]);
+      int4 tr = make_int4(0, 0, 0, -1);
+      if (t < te) tr = w.ltri[t];
+      const bool vis = t < te && draw_visible(w, tr);
       const unsigned long long m = __ballot(vis);
-      if (vis) w.dtri[base + mask_prefix(m)] = t;
+      if (vis) w.dtri[base + mask_prefix(m)] = t | (tri_is_poly[tr.w & 0x7fffffff] ? (int)0x80000000u : 0);
       base += __popcll(m);
     }
   }
@@ -208,65 +211,86 @@ __device__ unsigned face_set_rgb(int fs, int seed, int fdefault) {
   return r | (g << 8) | (bl << 16);
 }
 
-// Fill the records of stale leaves (all = every leaf): wave per leaf, lane per tri; the
-// leaf's tris are written as 9 dwords per vertex, 27 per tri.
-__global__ __launch_bounds__(256) void k_draw_fill(SgDevView d, SgDrawView w, int L, int all) {
+// Fill the records of stale leaves (all = every leaf): wave per leaf, lane per tri in
+// chunks of 64 tris; each lane builds its tri's 27 dwords in LDS (stride 27, odd: no
+// bank conflicts), then the wave stores the chunk's 64 x 108 bytes contiguously.
+// A draw_tri entry with bit 31 set is a tri whose polygon is that same triangle in the
+// same corner order: its face normal comes from the corners already loaded.
+#define DRAW_WAVES 4
+__global__ __launch_bounds__(64 * DRAW_WAVES) void k_draw_fill(SgDevView d, SgDrawView w, int L, int all) {
+  __shared__ uint32_t s_rec[DRAW_WAVES][64 * 27];
   const int lane = lane_id();
+  const int wid = threadIdx.x / SG_WAVE;
+  uint32_t *rec = s_rec[wid];
   const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
   const int nw = gridDim.x * blockDim.x / SG_WAVE;
   for (int l = gw; l < L; l += nw) {
     if (!all && !w.dirty[l]) continue;
     const int sb = w.doff[l], se = w.doff[l + 1];
     const bool smooth = (w.dleaf[l] & 1) != 0;
-    for (int s = sb + lane; s < se; s += SG_WAVE) {
-      const int4 tr = w.ltri[w.dtri[s]];
-      const int poly = w.tri_poly[tr.w & 0x7fffffff];
-      const int cv[3] = {tr.x, tr.y, tr.z};
-      float co[3][3];
-      for (int j = 0; j < 3; j++) {
-        co[j][0] = d.px[cv[j]];
-        co[j][1] = d.py[cv[j]];
-        co[j][2] = d.pz[cv[j]];
-      }
-      short no[3][3];
-      if (smooth) {
+    for (int s0 = sb; s0 < se; s0 += SG_WAVE) {
+      const int s = s0 + lane;
+      if (s < se) {
+        const int e = w.dtri[s];
+        const int4 tr = w.ltri[e & 0x7fffffff];
+        const int poly = w.tri_poly[tr.w & 0x7fffffff];
+        const int cv[3] = {tr.x, tr.y, tr.z};
+        float co[3][3];
         for (int j = 0; j < 3; j++) {
-          no[j][0] = to_short(d.nx[cv[j]]);
-          no[j][1] = to_short(d.ny[cv[j]]);
-          no[j][2] = to_short(d.nz[cv[j]]);
+          co[j][0] = d.px[cv[j]];
+          co[j][1] = d.py[cv[j]];
+          co[j][2] = d.pz[cv[j]];
         }
-      } else {
-        float fx, fy, fz;
-        poly_normal(d, w, poly, fx, fy, fz);
-        for (int j = 0; j < 3; j++) {
-          no[j][0] = to_short(fx);
-          no[j][1] = to_short(fy);
-          no[j][2] = to_short(fz);
-        }
-      }
-      unsigned char msk[3] = {0, 0, 0};
-      if (d.mask) {
+        short no[3][3];
         if (smooth) {
-          for (int j = 0; j < 3; j++) msk[j] = (unsigned char)(d.mask[cv[j]] * 255);
+          for (int j = 0; j < 3; j++) {
+            no[j][0] = to_short(d.nx[cv[j]]);
+            no[j][1] = to_short(d.ny[cv[j]]);
+            no[j][2] = to_short(d.nz[cv[j]]);
+          }
         } else {
-          const float fm = (d.mask[cv[0]] + d.mask[cv[1]] + d.mask[cv[2]]) / 3.0f;
-          msk[0] = msk[1] = msk[2] = (unsigned char)(fm * 255);
+          float fx, fy, fz;
+          if (e < 0)
+            sg_tri_normal(co[0][0], co[0][1], co[0][2], co[1][0], co[1][1], co[1][2], co[2][0], co[2][1], co[2][2],
+                          fx, fy, fz);
+          else
+            poly_normal(d, w, poly, fx, fy, fz);
+          for (int j = 0; j < 3; j++) {
+            no[j][0] = to_short(fx);
+            no[j][1] = to_short(fy);
+            no[j][2] = to_short(fz);
+          }
+        }
+        unsigned char msk[3] = {0, 0, 0};
+        if (d.mask) {
+          if (smooth) {
+            for (int j = 0; j < 3; j++) msk[j] = (unsigned char)(d.mask[cv[j]] * 255);
+          } else {
+            const float fm = (d.mask[cv[0]] + d.mask[cv[1]] + d.mask[cv[2]]) / 3.0f;
+            msk[0] = msk[1] = msk[2] = (unsigned char)(fm * 255);
+          }
+        }
+        const unsigned rgb = face_set_rgb(abs(w.fset[poly]), w.seed, w.fset_default);
+        uint32_t *r = rec + 27 * lane;
+        for (int j = 0; j < 3; j++, r += 9) {
+          r[0] = __float_as_uint(co[j][0]);
+          r[1] = __float_as_uint(co[j][1]);
+          r[2] = __float_as_uint(co[j][2]);
+          r[3] = 0u;
+          r[4] = (uint32_t)(uint16_t)no[j][0] | ((uint32_t)(uint16_t)no[j][1] << 16);
+          r[5] = (uint32_t)(uint16_t)no[j][2] | ((uint32_t)msk[j] << 16);
+          r[6] = 0u;
+          r[7] = 0u;
+          r[8] = rgb;
         }
       }
-      const unsigned rgb = face_set_rgb(abs(w.fset[poly]), w.seed, w.fset_default);
-      uint32_t *out = (uint32_t *)(w.vbo + (size_t)s * 3 * DRAW_STRIDE);
-      for (int j = 0; j < 3; j++) {
-        uint32_t *r = out + 9 * j;
-        r[0] = __float_as_uint(co[j][0]);
-        r[1] = __float_as_uint(co[j][1]);
-        r[2] = __float_as_uint(co[j][2]);
-        r[3] = 0u;
-        r[4] = (uint32_t)(uint16_t)no[j][0] | ((uint32_t)(uint16_t)no[j][1] << 16);
-        r[5] = (uint32_t)(uint16_t)no[j][2] | ((uint32_t)msk[j] << 16);
-        r[6] = 0u;
-        r[7] = 0u;
-        r[8] = rgb;
-      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+      const int nd = min(SG_WAVE, se - s0) * 27;
+      uint32_t *out = (uint32_t *)(w.vbo + (size_t)s0 * 3 * DRAW_STRIDE);
+      for (int k = lane; k < nd; k += SG_WAVE) out[k] = rec[k];
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
     }
     if (!all && lane == 0) w.dirty[l] = 0;
   }
@@ -323,12 +347,24 @@ int sg_pbvh_draw_buffers_build(PBVH *p, const int *face_sets, int npoly, int see
   for (size_t k = 0; k < pv.size(); k++) pv[k] = p->perm_new_of_old[p->poly_verts[k]];
   std::vector<uint8_t> sm(std::max(P, 1), 0);
   for (int q = 0; q < P; q++) sm[q] = p->poly_smooth[q];
+  // a looptri whose polygon is that triangle, corners in the same order
+  std::vector<uint8_t> tip((size_t)std::max(p->totprim, 1), 0);
+  for (int t = 0; t < p->totprim; t++) {
+    const int q = p->looptri ? (int)p->looptri[t].poly : t;
+    const int b = p->poly_off[q];
+    tip[t] = (p->poly_off[q + 1] - b == 3 && p->poly_verts[b] == p->tris[3 * (size_t)t] &&
+              p->poly_verts[b + 1] == p->tris[3 * (size_t)t + 1] && p->poly_verts[b + 2] == p->tris[3 * (size_t)t + 2])
+                 ? 1
+                 : 0;
+  }
+  uint8_t *dtp = nullptr;
   uint8_t *dsm = nullptr;
   int *count = nullptr;
   int rc = dev_upload(&d.draw_fset, face_sets, (size_t)P);
   if (!rc) rc = dev_upload(&d.draw_poly_off, p->poly_off.data(), p->poly_off.size());
   if (!rc) rc = dev_upload(&d.draw_poly_v, pv.data(), pv.size());
   if (!rc) rc = dev_upload(&dsm, sm.data(), sm.size());
+  if (!rc) rc = dev_upload(&dtp, tip.data(), tip.size());
   if (!rc) rc = sg_check_hip(hipMalloc(&d.draw_off, ((size_t)L + 1) * sizeof(int)), "hipMalloc");
   if (!rc) rc = sg_check_hip(hipMalloc(&d.draw_leaf, (size_t)std::max(L, 1) * sizeof(int)), "hipMalloc");
   if (!rc) rc = sg_check_hip(hipMalloc(&count, (size_t)std::max(L, 1) * sizeof(int)), "hipMalloc");
@@ -353,13 +389,18 @@ int sg_pbvh_draw_buffers_build(PBVH *p, const int *face_sets, int npoly, int see
   if (!rc) {
     SgDrawView w = draw_view(p);
     hipLaunchKernelGGL(k_draw_compact, dim3(std::max(1, std::min((L + 3) / 4, 4096))), dim3(256), 0, p->stream, w,
-                       L);
-    hipLaunchKernelGGL(k_draw_fill, dim3(std::max(1, std::min((L + 3) / 4, 8192))), dim3(256), 0, p->stream,
-                       sg_view(p), w, L, 1);
+                       L, dtp);
+    hipLaunchKernelGGL(k_draw_fill, dim3(std::max(1, std::min((L + DRAW_WAVES - 1) / DRAW_WAVES, 8192))),
+                       dim3(64 * DRAW_WAVES), 0, p->stream, sg_view(p), w, L, 1);
     rc = sg_check_hip(hipGetLastError(), "draw fill launch");
     if (!rc) rc = sg_check_hip(hipStreamSynchronize(p->stream), "draw build");
   }
   if (dsm) (void)hipFree(dsm);
+  if (dtp) (void)hipFree(dtp);
+  if (!rc) {  // the full fill consumed every node's draw marks
+    for (PBVHNode &n : p->nodes) n.flag &= ~(PBVH_UpdateDrawBuffers | PBVH_RebuildDrawBuffers);
+    p->draw_marks_pending = false;
+  }
   if (count) (void)hipFree(count);
   if (rc) sg_pbvh_draw_buffers_free(p);
   return rc;
@@ -372,11 +413,14 @@ int sg_pbvh_draw_buffers_update(PBVH *p, int all) {
   if (!p) return sg_set_error("bad arguments");
   if (!p->d.draw_vbo) return sg_set_error("draw buffers not built");
   std::vector<int> marked;
-  for (PBVHNode &n : p->nodes) {
-    if (n.leaf >= 0 && (n.flag & (PBVH_UpdateDrawBuffers | PBVH_RebuildDrawBuffers))) {
-      marked.push_back(n.leaf);
-      n.flag &= ~(PBVH_UpdateDrawBuffers | PBVH_RebuildDrawBuffers);
+  if (p->draw_marks_pending) {
+    for (PBVHNode &n : p->nodes) {
+      if (n.leaf >= 0 && (n.flag & (PBVH_UpdateDrawBuffers | PBVH_RebuildDrawBuffers))) {
+        marked.push_back(n.leaf);
+        n.flag &= ~(PBVH_UpdateDrawBuffers | PBVH_RebuildDrawBuffers);
+      }
     }
+    p->draw_marks_pending = false;
   }
   int rc = 0;
   int *dl = nullptr;
@@ -397,8 +441,8 @@ int sg_pbvh_draw_buffers_update(PBVH *p, int all) {
     }
   }
   const int L = p->d.L;
-  hipLaunchKernelGGL(k_draw_fill, dim3(std::max(1, std::min((L + 3) / 4, 8192))), dim3(256), 0, p->stream, sg_view(p),
-                     draw_view(p), L, all ? 1 : 0);
+  hipLaunchKernelGGL(k_draw_fill, dim3(std::max(1, std::min((L + DRAW_WAVES - 1) / DRAW_WAVES, 8192))),
+                     dim3(64 * DRAW_WAVES), 0, p->stream, sg_view(p), draw_view(p), L, all ? 1 : 0);
   rc = sg_check_hip(hipGetLastError(), "draw fill launch");
   if (!rc && all) rc = sg_check_hip(hipMemsetAsync(p->d.draw_dirty, 0, (size_t)L * sizeof(int), p->stream), "memset");
   if (!rc) rc = sg_check_hip(hipStreamSynchronize(p->stream), "draw update");

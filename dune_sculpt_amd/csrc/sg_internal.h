@@ -183,6 +183,7 @@ struct PBVH {
   int stamp = 0;                     // stats-mode stamp generator
   hipEvent_t timer[2] = {nullptr, nullptr};  // sg_pbvh_timer_start/stop
   bool halt_live = false;            // the work being enqueued contains learned dabs (kernels test the halt flag)
+  bool draw_marks_pending = false;   // some node carries a host-side draw-buffer mark
   const float *last_place = nullptr; // placement slots of the last ray-placed stroke
   int last_place_n = 0;
 };

@@ -41,7 +41,9 @@ def test_draw_buffers_quads_flat_smooth_hidden_mask_face_sets(sg, po):
     hidden = rng.random(len(co)) < 0.02
     mask = rng.random(len(co)).astype(F32)
     p = sg.PBVH(co, None, leaf_limit=32, hidden=hidden, polys=(off, pv, smooth, mat))
-    o = po.Oracle(co, p.tris, leaf_limit=32, hidden=hidden.astype(np.uint8))
+    # the tree splits leaves by material and smooth flag (the reference's material key)
+    key = (mat | (smooth.astype(np.int32) << 16))[p.tri_poly]
+    o = po.Oracle(co, p.tris, leaf_limit=32, hidden=hidden.astype(np.uint8), mat=key)
     o.set_normals(p.normals())
     p.set_mask(mask)
     print("[draw] quad torus build", flush=True)
