@@ -401,7 +401,7 @@ def placed_rays(sg, centres, seed=11, eye=3.0, jitter=0.05):
     return starts.astype(np.float32), d.astype(np.float32)
 
 
-def run_placed(sg, A, dabs, W, K, only_placed=False):
+def run_placed(sg, A, dabs, W, K, only_placed=False, stream=False):
     """The headline stroke with every dab placed by a GPU ray cast inside the stroke
     graph (dab i's centre = where ray i hits the mesh the previous dabs left) against
     the same dabs at fixed centres; both captured once and launched per step from the
@@ -410,20 +410,28 @@ def run_placed(sg, A, dabs, W, K, only_placed=False):
     starts, dirs = placed_rays(sg, centres)
     res = {}
     for name in (("placed",) if only_placed else ("fixed", "placed")):
-        prep = A.prepare(dabs) if name == "fixed" else A.prepare_placed(dabs, starts, dirs)
+        prep = None
+        if not stream:
+            prep = A.prepare(dabs) if name == "fixed" else A.prepare_placed(dabs, starts, dirs)
         times = []
         for s in range(W + K):
             A.restore()
             A.sync()
             A.timer_start()
-            prep.launch(sync=False)
+            if prep is not None:
+                prep.launch(sync=False)
+            elif name == "fixed":
+                A.stroke(dabs, graph=False)
+            else:
+                A.stroke_placed(dabs, starts, dirs, graph=False)
             ms = A.timer_stop()
             if s >= W:
                 times.append(ms)
         if name == "placed":
             pl = A.placements(len(dabs))
             res["hit_frac"] = float(pl[:, 6].mean())
-        prep.close()
+        if prep is not None:
+            prep.close()
         ms = float(np.mean(times))
         res[name] = {"ms_per_stroke": ms, "us_per_dab": ms * 1e3 / len(dabs)}
     if "fixed" in res:
@@ -452,12 +460,11 @@ def run_draw(sg, A, dabs, W, K):
             times.append(ms)
     full_ms = float(np.mean(times))
     alg = info["visible_tris"] * (108 + 28) + A.totvert * 24
-    prep = A.prepare(dabs)
-    prep.launch(sync=True)
+    A.stroke(dabs, graph=False)  # the dabs mark the leaves they gather
+    A.sync()
     A.timer_start()
     sg.lib().sg_pbvh_draw_buffers_update(A.h, 0)
     inc_ms = A.timer_stop()
-    prep.close()
     gbps = alg / (full_ms * 1e-3) / 1e9
     res = {"visible_tris": info["visible_tris"], "vbo_bytes": info["bytes"], "full_refill_ms": full_ms,
            "incremental_refill_ms_after_stroke": inc_ms,
@@ -601,7 +608,7 @@ def main():
             out["hbm_copy"] = {"bytes": args.copy_mb << 20, "ms": ms, "gbps_read_plus_write": gbps_copy}
         out["raycast"], _, _ = run_raycast(sg, A, W, max(K, 5), args.ray_side)
         if not args.no_placed:
-            out["placed_stroke"] = run_placed(sg, A, dabs, W, K, args.placed_only)
+            out["placed_stroke"] = run_placed(sg, A, dabs, W, K, args.placed_only, args.mode == "stream")
         if not args.no_draw:
             out["draw_buffers"] = run_draw(sg, A, dabs, W, max(K, 3))
         print(json.dumps(out))
