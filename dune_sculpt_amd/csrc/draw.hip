@@ -16,9 +16,10 @@
 // 39-84).  Mask bytes follow :246-265; face-set colours BKE_paint_face_set_overlay_color_get
 // (kernel/intern/paint.c:2219-2235).
 //
-// Which leaves are refilled: the dab search marks every gathered leaf in a per-leaf
-// array that lives as long as the PBVH (the brush marks those nodes UpdateDrawBuffers),
-// and host-side marks add theirs.
+// Which leaves are refilled: those of every search subtree a dab gathered leaves from
+// (the brush marks its gathered nodes UpdateDrawBuffers; the search marks their subtree,
+// one store per subtree, in an array that lives as long as the PBVH), plus those of
+// subtrees holding host-marked nodes.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -38,7 +39,8 @@ struct SgDrawView {
   int *dtri;   // visible tris' ltri positions
   int *dleaf;  // bit0: the leaf draws smooth
   const int *doff, *fset, *poly_off, *poly_v;
-  int *dirty;
+  const int *leaf_cut;  // leaf -> search subtree
+  int *dirty;           // per subtree
   uint8_t *vbo;
   int seed, fset_default;
 };
@@ -225,7 +227,7 @@ __global__ __launch_bounds__(64 * DRAW_WAVES) void k_draw_fill(SgDevView d, SgDr
   const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
   const int nw = gridDim.x * blockDim.x / SG_WAVE;
   for (int l = gw; l < L; l += nw) {
-    if (!all && !w.dirty[l]) continue;
+    if (!all && !w.dirty[w.leaf_cut[l]]) continue;
     const int sb = w.doff[l], se = w.doff[l + 1];
     const bool smooth = (w.dleaf[l] & 1) != 0;
     for (int s0 = sb; s0 < se; s0 += SG_WAVE) {
@@ -292,12 +294,11 @@ __global__ __launch_bounds__(64 * DRAW_WAVES) void k_draw_fill(SgDevView d, SgDr
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
     }
-    if (!all && lane == 0) w.dirty[l] = 0;
   }
 }
 
-__global__ void k_draw_mark(int *dirty, const int *leaves, int n) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dirty[leaves[i]] = 1;
+__global__ void k_draw_mark(int *dirty, const int *leaf_cut, const int *leaves, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dirty[leaf_cut[leaves[i]]] = 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -317,6 +318,7 @@ static SgDrawView draw_view(PBVH *p) {
   w.poly_off = d.draw_poly_off;
   w.poly_v = d.draw_poly_v;
   w.dirty = d.draw_dirty;
+  w.leaf_cut = d.leaf_cut;
   w.vbo = d.draw_vbo;
   w.seed = d.draw_seed;
   w.fset_default = d.draw_fset_default;
@@ -368,7 +370,7 @@ int sg_pbvh_draw_buffers_build(PBVH *p, const int *face_sets, int npoly, int see
   if (!rc) rc = sg_check_hip(hipMalloc(&d.draw_off, ((size_t)L + 1) * sizeof(int)), "hipMalloc");
   if (!rc) rc = sg_check_hip(hipMalloc(&d.draw_leaf, (size_t)std::max(L, 1) * sizeof(int)), "hipMalloc");
   if (!rc) rc = sg_check_hip(hipMalloc(&count, (size_t)std::max(L, 1) * sizeof(int)), "hipMalloc");
-  if (!rc) rc = sg_check_hip(hipMemset(d.draw_dirty, 0, (size_t)std::max(L, 1) * sizeof(int)), "memset");
+  if (!rc) rc = sg_check_hip(hipMemset(d.draw_dirty, 0, (size_t)std::max(d.C, 1) * sizeof(int)), "memset");
   d.draw_seed = seed;
   d.draw_fset_default = fset_default;
   int ntri = 0;
@@ -432,7 +434,7 @@ int sg_pbvh_draw_buffers_update(PBVH *p, int all) {
     if (!rc) {
       const int n = (int)marked.size();
       hipLaunchKernelGGL(k_draw_mark, dim3(std::max(1, std::min((n + 255) / 256, 1024))), dim3(256), 0, p->stream,
-                         p->d.draw_dirty, dl, n);
+                         p->d.draw_dirty, p->d.leaf_cut, dl, n);
       rc = sg_check_hip(hipGetLastError(), "draw mark launch");
     }
     if (rc) {
@@ -444,7 +446,7 @@ int sg_pbvh_draw_buffers_update(PBVH *p, int all) {
   hipLaunchKernelGGL(k_draw_fill, dim3(std::max(1, std::min((L + DRAW_WAVES - 1) / DRAW_WAVES, 8192))),
                      dim3(64 * DRAW_WAVES), 0, p->stream, sg_view(p), draw_view(p), L, all ? 1 : 0);
   rc = sg_check_hip(hipGetLastError(), "draw fill launch");
-  if (!rc && all) rc = sg_check_hip(hipMemsetAsync(p->d.draw_dirty, 0, (size_t)L * sizeof(int), p->stream), "memset");
+  if (!rc) rc = sg_check_hip(hipMemsetAsync(p->d.draw_dirty, 0, (size_t)p->d.C * sizeof(int), p->stream), "memset");
   if (!rc) rc = sg_check_hip(hipStreamSynchronize(p->stream), "draw update");
   if (dl) (void)hipFree(dl);
   return rc;

@@ -135,13 +135,13 @@ __global__ __launch_bounds__(256) void k_search(SgDevView d, SgDabK k, long long
           d.gathered[pos] = lb + q0 + j * SG_WAVE + lane;
           d.gl_v[pos] = vr[j];
           d.gl_s[pos] = sr[j];
-          if (d.draw_dirty) d.draw_dirty[lb + q0 + j * SG_WAVE + lane] = 1;  // UpdateDrawBuffers
         }
         off += __popcll(m[j]);
       }
       if (lane == 0) {
         d.cut_touch[c] = serial;
         d.cut_dirty[c] = 1;
+        d.draw_dirty[c] = 1;  // the subtree's leaves need new draw records (UpdateDrawBuffers)
       }
     }
   }

@@ -665,6 +665,7 @@ void sg_build_device(PBVH *p, const float *co) {
     upload(&d.top_tab, top_tab.data(), top_tab.size(), s);
   }
   alloc_fill(&d.cut_dirty, C, 0, s);
+  alloc_fill(&d.draw_dirty, C, 0, s);  // subtrees the dabs touched since the last draw refill
   alloc_fill(&d.cut_touch, C, 0x80, s);  // INT_MIN-ish: never "touched by the previous dab"
   alloc_fill(&d.prev_cuts, C, 0, s);
 
@@ -676,7 +677,7 @@ void sg_build_device(PBVH *p, const float *co) {
   alloc_fill(&d.vmark, V, 0, s);
   // scratch
   alloc_fill(&d.gathered, L, 0, s);
-  alloc_fill(&d.draw_dirty, L, 0, s);  // leaves the dabs gathered since the last draw refill
+
   alloc_fill(&d.gl_v, L, 0, s);
   alloc_fill(&d.gl_s, L, 0, s);
   alloc_fill(&d.aff_v, V, 0, s);

@@ -97,7 +97,7 @@ struct SgDev {
   int2 *rl_list = nullptr;      // [L] leaves an occluded search passes (callback API)
   float *rl_args = nullptr;     // [64] one ray's arguments, results and list count
   // draw buffers (built on request): visible tris per leaf, their vertex records
-  int *draw_dirty = nullptr;    // [L] leaves whose draw buffer is stale (set by the search; lives with the PBVH)
+  int *draw_dirty = nullptr;    // [C] subtrees whose leaves' draw records are stale (set by the search; lives with the PBVH)
   int *draw_tri = nullptr;      // [visible tris] position in ltri, leaf after leaf
   int *draw_off = nullptr;      // [L+1] each leaf's first visible tri
   int *draw_leaf = nullptr;     // [L] bit0 smooth (first face), bits 8+ material
