@@ -476,6 +476,49 @@ def run_draw(sg, A, dabs, W, K):
     return res
 
 
+def run_undo(sg, A, launch, base_ms, n_dabs, W, K):
+    """Undo steps of the headline stroke: the stroke timed with a step open (every dab
+    saves the leaves it gathers the first time it writes them), then undo and redo of
+    the step.  An undo or redo swaps each saved vertex's position, normal and record
+    (read and write of position, normal and record, write of the shadow copy: 108 bytes
+    per vertex) and refits the touched subtrees."""
+    rec_ms = []
+    info = None
+    for s in range(W + K):
+        A.restore()
+        A.undo_begin()
+        A.sync()
+        A.timer_start()
+        launch()
+        ms = A.timer_stop()
+        A.undo_end()
+        info = A.undo_info()
+        if s >= W:
+            rec_ms.append(ms)
+        if s < W + K - 1:
+            A.undo_clear()
+    undo_ms, redo_ms = [], []
+    for _ in range(max(K, 3)):
+        A.sync()
+        A.timer_start()
+        assert A.undo()
+        undo_ms.append(A.timer_stop())
+        A.timer_start()
+        assert A.redo()
+        redo_ms.append(A.timer_stop())
+    A.undo_clear()
+    n_v = (info["bytes"] - 80 * info["leaves"]) // 24
+    rec = float(np.mean(rec_ms))
+    um, rm = float(np.mean(undo_ms)), float(np.mean(redo_ms))
+    res = {"step_leaves": info["leaves"], "step_vertices": int(n_v), "step_bytes": info["bytes"],
+           "stroke_ms_recording": rec, "stroke_ms_plain": base_ms,
+           "recording_us_per_dab": (rec - base_ms) * 1e3 / n_dabs,
+           "undo_ms": um, "redo_ms": rm,
+           "undo_gbps": round(108 * n_v / (um * 1e-3) / 1e9, 3)}
+    log(f"undo: {res}")
+    return res
+
+
 def cpu_raycast(o, starts, dirs, budget_s):
     """The oracle's sequential ray cast (the reference's traversal and callback) on the
     same camera rays, one host thread, until the budget is spent."""
@@ -571,6 +614,7 @@ def main():
     ap.add_argument("--only-ray", action="store_true", help="run only the ray cast measurements")
     ap.add_argument("--no-placed", action="store_true", help="skip the placed-stroke measurement")
     ap.add_argument("--no-draw", action="store_true", help="skip the draw-buffer measurement")
+    ap.add_argument("--no-undo", action="store_true", help="skip the undo-step measurement")
     ap.add_argument("--placed-only", action="store_true",
                     help="placed stroke without the fixed-centre reference stroke (one captured graph)")
     args = ap.parse_args()
@@ -690,6 +734,8 @@ def main():
                 "alg_bytes_per_launch": round(bytes_per_dab, 1), "avg_launch_us": round(us_per_dab_ev, 3)}
     kernel_bytes_per_dab = {k: round(v / args.dabs, 1) for k, v in kern_bytes.items()}
 
+    undo = None if args.no_undo else run_undo(sg, A, step, float(np.mean(event_ms)), args.dabs, W, K)
+
     copy = None
     if args.copy_mb:
         ms, gbps_copy = sg.hbm_copy_probe(args.copy_mb << 20)
@@ -773,6 +819,7 @@ def main():
         "raycast": ray,
         "placed_stroke": placed,
         "draw_buffers": draw,
+        "undo": undo,
 
         "build_s": build_s,
         "autotune": tune,

@@ -127,6 +127,7 @@ EXPORTS = [
     "BKE_pbvh_node_find_nearest_to_ray", "BKE_pbvh_raycast_project_ray_root", "sg_isect_ray_tri_precalc",
     "sg_pbvh_draw_buffers_build", "sg_pbvh_draw_buffers_update", "sg_pbvh_draw_buffers_info",
     "sg_pbvh_draw_buffer_node", "sg_pbvh_draw_buffer_read", "sg_pbvh_draw_buffers_free",
+    "sg_undo_begin", "sg_undo_push_node", "sg_undo_end", "sg_undo", "sg_redo", "sg_undo_info", "sg_undo_clear",
 ]
 SMOOTH_LEARNED = 1 << 30
 NCTR = 24
@@ -263,6 +264,13 @@ def lib():
         "sg_pbvh_draw_buffer_node": (i32, [vp, vp, C.POINTER(C.c_longlong), i32p, i32p, i32p]),
         "sg_pbvh_draw_buffer_read": (C.c_longlong, [vp, vp, vp, C.c_longlong]),
         "sg_pbvh_draw_buffers_free": (None, [vp]),
+        "sg_undo_begin": (i32, [vp, C.c_longlong]),
+        "sg_undo_push_node": (i32, [vp, vp]),
+        "sg_undo_end": (i32, [vp]),
+        "sg_undo": (i32, [vp]),
+        "sg_redo": (i32, [vp]),
+        "sg_undo_info": (i32, [vp, C.POINTER(C.c_longlong)]),
+        "sg_undo_clear": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -643,6 +651,31 @@ class PBVH:
         if n < 0:
             raise RuntimeError("draw read failed: " + last_error())
         return out, sm.value, mt.value
+
+    # -- undo steps
+    def undo_begin(self, max_verts=0):
+        _check(lib().sg_undo_begin(self.h, int(max_verts)), "undo begin")
+
+    def undo_push_node(self, index):
+        _check(lib().sg_undo_push_node(self.h, self.node(index)), "undo push")
+
+    def undo_end(self):
+        _check(lib().sg_undo_end(self.h), "undo end")
+
+    def undo(self):
+        """True after undoing a step, False when there is none."""
+        return _check(lib().sg_undo(self.h), "undo") == 0
+
+    def redo(self):
+        return _check(lib().sg_redo(self.h), "redo") == 0
+
+    def undo_info(self):
+        out = (C.c_longlong * 4)()
+        _check(lib().sg_undo_info(self.h, out), "undo info")
+        return {"steps": out[0], "applied": out[1], "bytes": out[2], "leaves": out[3]}
+
+    def undo_clear(self):
+        lib().sg_undo_clear(self.h)
 
     def close(self):
         for st in getattr(self, "_strokes", []):

@@ -236,6 +236,7 @@ __global__ __launch_bounds__(256) void k_brush(SgDevView d, SgDabK k, const floa
   for (int g = gw; g < ng; g += nw) {
     const int2 vr = (g == gw) ? vr0 : d.gl_v[g];
     ncand += vr.y - vr.x;
+    undo_save_leaf(d, d.gathered[g], vr);  // before this wave writes the leaf
     for (int base = vr.x; base < vr.y; base += 2 * SG_WAVE) {
       const int v0 = base + lane, v1 = base + SG_WAVE + lane;
       const BrushIn in0 = brush_load(d, k, v0, vr.y, smooth);
@@ -755,6 +756,7 @@ SgDevView sg_view_of(SgDev &d, bool respect_hide, bool halt_live) {
   v.vmark = d.vmark;
   v.gathered = d.gathered;
   v.draw_dirty = d.draw_dirty;
+  v.undo = d.undo;
   v.gl_v = d.gl_v;
   v.gl_s = d.gl_s;
   v.aff_v = d.aff_v;

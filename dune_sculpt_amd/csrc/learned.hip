@@ -403,6 +403,9 @@ __global__ __launch_bounds__(LRN_THREADS) void k_brush_learned(SgDevView d, SgDa
   int nood = 0, naff = 0;
   for (int it = gw; it < nitems; it += nw) {
     const int2 vr = d.gl_v[it / LRN_SLOTS];
+    // the learned brush writes the positions and leaves the shadow copy alone: the
+    // leaf's first slot saves it for undo from there, whatever the other slots write
+    if (it % LRN_SLOTS == 0) undo_save_leaf(d, d.gathered[it / LRN_SLOTS], vr);
     for (int base = vr.x + (it % LRN_SLOTS) * LRN_ROWS; base < vr.y; base += LRN_SLOTS * LRN_ROWS) {
       float nx, ny, nz;
       const BrushIn in = learned_tile_load(d, k, base, vr.y, lane, nx, ny, nz);

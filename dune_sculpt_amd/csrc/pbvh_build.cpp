@@ -21,6 +21,7 @@
 #include <thread>
 
 #include "sg_internal.h"
+#include "sg_kernels.h"
 
 namespace {
 
@@ -666,6 +667,8 @@ void sg_build_device(PBVH *p, const float *co) {
   }
   alloc_fill(&d.cut_dirty, C, 0, s);
   alloc_fill(&d.draw_dirty, C, 0, s);  // subtrees the dabs touched since the last draw refill
+  alloc_fill(&d.undo_stamp, L, 0xFF, s);  // -1: saved by no undo step
+  alloc_fill(&d.undo, 1, 0, s);          // undo recording off
   alloc_fill(&d.cut_touch, C, 0x80, s);  // INT_MIN-ish: never "touched by the previous dab"
   alloc_fill(&d.prev_cuts, C, 0, s);
 
@@ -694,6 +697,7 @@ void sg_build_device(PBVH *p, const float *co) {
 }
 
 void sg_free_device(PBVH *p) {
+  sg_undo_release(p);
   SgDev &d = p->d;
   void *ptrs[] = {d.px, d.py, d.pz, d.qx, d.qy, d.qz, d.nx, d.ny, d.nz, d.mask, d.vt4,
                   d.vt_off, d.vv_off, d.vv_idx, d.leaf_vbeg, d.leaf_sh_off, d.leaf_sh,
@@ -705,7 +709,7 @@ void sg_free_device(PBVH *p) {
                   d.stamp_a, d.curve, d.log, d.ltri, d.leaf_toff, d.tri_poly, d.node_parent, d.leaf_node,
                   d.ray_in, d.ray_out, d.place, d.place_rays, d.rl_list, d.rl_args,
                   d.draw_dirty, d.draw_tri, d.draw_off, d.draw_leaf, d.draw_fset, d.draw_poly_off,
-                  d.draw_poly_v, d.draw_vbo};
+                  d.draw_poly_v, d.draw_vbo, d.undo, d.undo_stamp};
   for (void *q : ptrs) {
     if (q) hipFree(q);
   }

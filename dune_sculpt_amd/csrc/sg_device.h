@@ -127,6 +127,44 @@ __device__ __forceinline__ void sg_place(SgDabK &k, const float *place) {
   }
 }
 
+// Undo recording: save a leaf the running dab is about to write, once per step (one
+// wave; the leaf's uniq vertices are the contiguous range vr).  Reads the shadow copy,
+// which holds the positions from before the dab in every brush kernel.
+__device__ __forceinline__ void undo_save_leaf(const SgDevView &d, int leaf, int2 vr) {
+  SgUndoCtl *u = d.undo;
+  if (!u || !u->active) return;
+  const int lane = lane_id();
+  const int n = vr.y - vr.x;
+  int e = -1;
+  unsigned long long vb = 0;
+  if (lane == 0 && u->stamp[leaf] != u->step) {
+    e = atomicAdd(&u->n_e, 1);
+    vb = atomicAdd(&u->n_v, (unsigned long long)n);
+    if (e >= u->cap_e || vb + (unsigned long long)n > u->cap_v) {
+      u->overflow = 1;
+      e = -1;
+    } else {
+      u->tab[e] = make_int4(leaf, n, (int)(unsigned)vb, (int)(unsigned)(vb >> 32));
+      u->stamp[leaf] = u->step;
+    }
+  }
+  e = __shfl(e, 0);
+  if (e < 0) return;
+  vb = ((unsigned long long)(unsigned)__shfl((int)(unsigned)(vb >> 32), 0) << 32) |
+       (unsigned)__shfl((int)(unsigned)vb, 0);
+  for (int i = lane; i < n; i += SG_WAVE) {
+    const int v = vr.x + i;
+    const size_t o = 3 * (size_t)(vb + i);
+    u->pos[o] = d.qx[v];
+    u->pos[o + 1] = d.qy[v];
+    u->pos[o + 2] = d.qz[v];
+    u->nor[o] = d.nx[v];
+    u->nor[o + 1] = d.ny[v];
+    u->nor[o + 2] = d.nz[v];
+  }
+  if (lane < 16) u->bbs[16 * (size_t)e + lane] = lane < 8 ? d.leaf_bb[8 * (size_t)leaf + lane] : d.leaf_obb[8 * (size_t)leaf + lane - 8];
+}
+
 // Spec S1 sphere-vs-box test: nearest point of the box, strict '<' against r*r.
 __device__ __forceinline__ bool sphere_box(const BB6 &b, float cx, float cy, float cz, float r2) {
   const float c[3] = {cx, cy, cz};

@@ -37,6 +37,25 @@ struct PBVHNode {
 };
 
 // Device-side mesh and tree (all HBM, SoA, leaf-ordered vertex numbering).
+struct SgUndoCtl;  // device control block of the undo recording (sg_kernels.h)
+// One recorded undo step (device arrays, exact size) and the recording arena.
+struct SgUndoStep {
+  float *pos = nullptr, *nor = nullptr, *bbs = nullptr;
+  int4 *tab = nullptr;
+  int n_e = 0;
+  long long n_v = 0;
+};
+struct SgUndoState {
+  std::vector<SgUndoStep> steps;
+  int cursor = 0;               // steps [0, cursor) are applied
+  bool recording = false;
+  int step_id = 0;
+  float *rec_pos = nullptr, *rec_nor = nullptr, *rec_bbs = nullptr;
+  int4 *rec_tab = nullptr;
+  long long rec_cap_v = 0;
+  int rec_cap_e = 0;
+};
+
 struct SgDev {
   int V = 0, T = 0, L = 0, N = 0, C = 0;
   float *px = nullptr, *py = nullptr, *pz = nullptr; // positions [V]
@@ -106,6 +125,8 @@ struct SgDev {
   uint8_t *draw_vbo = nullptr;  // 36 bytes per vertex, 3 per visible tri
   long long draw_bytes = 0;
   int draw_ntri = 0, draw_seed = 0, draw_fset_default = 0;
+  SgUndoCtl *undo = nullptr;    // undo recording control (lives with the PBVH)
+  int *undo_stamp = nullptr;    // [L]
   float *place = nullptr;       // [8 per dab] placements of a ray-placed stroke
   SgRay *place_rays = nullptr;  // the stroke's rays
   int place_cap = 0;
@@ -184,6 +205,7 @@ struct PBVH {
   hipEvent_t timer[2] = {nullptr, nullptr};  // sg_pbvh_timer_start/stop
   bool halt_live = false;            // the work being enqueued contains learned dabs (kernels test the halt flag)
   bool draw_marks_pending = false;   // some node carries a host-side draw-buffer mark
+  SgUndoState undo;                  // undo steps recorded on the GPU (undo.hip)
   const float *last_place = nullptr; // placement slots of the last ray-placed stroke
   int last_place_n = 0;
 };
@@ -205,6 +227,9 @@ void sg_build_cut(PBVH *p, std::vector<int> &lbeg, std::vector<int> &lend, std::
                   std::vector<int> &leaf_cut);
 void sg_build_device(PBVH *p, const float *co);
 void sg_free_device(PBVH *p);
+
+// undo (undo.hip)
+void sg_undo_release(PBVH *p);  // every step and the arena
 
 // ray cast (ray.hip)
 int sg_ray_ensure(PBVH *p);

@@ -21,6 +21,22 @@ struct SgCollect {
   int cap;
 };
 
+// Undo recording (undo.hip).  Lives as long as the PBVH so captured strokes can read it
+// at run time: while `active`, the first brush kernel that writes a leaf in the open
+// step copies the leaf's uniq positions and normals (from the shadow copy, equal to
+// the positions before the dab) and its two boxes into the recording arena.
+struct SgUndoCtl {
+  int active, step, overflow, n_e;  // n_e: entries used
+  unsigned long long n_v;           // vertices used
+  unsigned long long cap_v;
+  int cap_e, pad;
+  int *stamp;   // [L] step that saved the leaf
+  float *pos;   // [cap_v * 3]
+  float *nor;   // [cap_v * 3]
+  int4 *tab;    // [cap_e]: leaf, count, vertex offset low, high
+  float *bbs;   // [cap_e * 16]: vb, orig_vb (8 floats each)
+};
+
 // Plain-pointer view of the device mesh passed to kernels by value.
 struct SgDevView {
   int V, T, L, N, C, respect_hide;
@@ -36,6 +52,7 @@ struct SgDevView {
   uint8_t *vmark;
   int *gathered;
   int *draw_dirty;
+  SgUndoCtl *undo;
   int2 *gl_v, *gl_s;
   int *aff_v;
   int2 *aff_vt, *aff_vv;

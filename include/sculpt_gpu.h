@@ -372,6 +372,26 @@ int sg_pbvh_draw_buffer_node(PBVH *pbvh, PBVHNode *node, long long *byte_offset,
 long long sg_pbvh_draw_buffer_read(PBVH *pbvh, PBVHNode *node, void *out, long long cap);
 void sg_pbvh_draw_buffers_free(PBVH *pbvh);
 
+/* ---- undo steps (snapshots of the leaves an operation touches, kept in HBM) ------
+ * Replaces SCULPT_undo_push_begin / SCULPT_undo_push_node / SCULPT_undo_push_end as
+ * the reference calls them around an operation (editors/sculpt_paint/paint_hide.c:
+ * 350-376 begin/end, :78/:126/:255 node; sculpt_undo.c itself is absent): between
+ * begin and end every dab saves each leaf it gathers, once, with its positions,
+ * normals and box as they were before the step; push_node saves a leaf the host is
+ * about to change.  max_verts bounds the saved vertices of one step (<= 0: the whole
+ * mesh); a step that would exceed it is dropped at end with an error.  sg_undo /
+ * sg_redo return 0 after applying a step, 1 when there is none; they mark the touched
+ * subtrees for the bounds refit and the draw-buffer refill.  Beginning a step drops
+ * the steps undone so far.  info: steps held, steps applied, device bytes held,
+ * leaves in the newest step. */
+int sg_undo_begin(PBVH *pbvh, long long max_verts);
+int sg_undo_push_node(PBVH *pbvh, PBVHNode *node);
+int sg_undo_end(PBVH *pbvh);
+int sg_undo(PBVH *pbvh);
+int sg_redo(PBVH *pbvh);
+int sg_undo_info(PBVH *pbvh, long long out[4]);
+void sg_undo_clear(PBVH *pbvh);
+
 /* ---- learned Smooth (MLP over 1-ring features; BF16 MFMA inference) ----------
  * A Smooth dab with SG_SMOOTH_LEARNED in SgDabParams.flag goes through the model
  * attached to the PBVH; when its error estimate (alpha * RMS(|d|)/r) exceeds the

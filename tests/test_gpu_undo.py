@@ -1,0 +1,243 @@
+"""GPU undo steps (sg_undo_*): snapshots of the leaves an operation touches.
+
+Bar: bit-exact.  Undoing a step gives back the positions, normals and every node box
+(current and original) the mesh had before the step; redoing it gives back the state
+after the step.  The leaves a step saves are the union of the leaves the oracle's
+sphere search gathers for the step's dabs (the leaves the reference would push with
+SCULPT_undo_push_node), and the step's size follows from their unique vertices."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+F32 = np.float32
+
+
+def _state(p):
+    n = p.tree_info()["nodes"]
+    bb = np.array([p.node_bb(i) for i in range(n)], F32)
+    obb = np.array([p.node_bb(i, original=True) for i in range(n)], F32)
+    return p.positions(), p.normals(), bb, obb
+
+
+def _same(a, b):
+    return all(np.array_equal(np.ascontiguousarray(x, F32).view(np.uint32),
+                              np.ascontiguousarray(y, F32).view(np.uint32)) for x, y in zip(a, b))
+
+
+def _stroke_dabs(sg, seed, n=16, r=0.15, tools=(1, 4, 2, 5)):
+    radii = np.full(n, r, F32)
+    centres = sg.great_circle_centres(seed, radii)
+    rng = np.random.default_rng(seed)
+    dabs = []
+    for i in range(n):
+        t = tools[i % len(tools)]
+        g = tuple(rng.normal(0, 0.01, 3)) if t == sg.TOOL_GRAB else (0, 0, 0)
+        dabs.append(sg.make_dab(tool=t, location=centres[i], radius=r, strength=0.6, smooth_iterations=3,
+                                grab_delta=g))
+    return dabs
+
+
+def test_undo_redo_stroke_bit_exact_and_saved_leaves(sg, po):
+    from parity_util import REL_TOL, rel_err
+
+    co, tris = sg.noisy_icosphere(5, seed=21, amplitude=0.02)
+    p = sg.PBVH(co, tris, leaf_limit=64)
+    o = po.Oracle(co, tris, leaf_limit=64)
+    o.set_normals(p.normals())
+    dabs = _stroke_dabs(sg, 3)
+    # the leaves the stroke gathers, from the oracle run dab by dab
+    want = set()
+    for d in dabs:
+        want |= set(o.search_sphere(list(d.location), d.radius, use_orig=(d.sculpt_tool == sg.TOOL_GRAB)).tolist())
+        o.dab(d)
+    s0 = _state(p)
+    print("[undo] stroke of", len(dabs), "dabs in one step", flush=True)
+    p.undo_begin()
+    p.stroke(dabs)
+    p.undo_end()
+    s1 = _state(p)
+    assert rel_err(s1[0], o.positions()) <= REL_TOL and rel_err(s1[1], o.normals()) <= REL_TOL
+    info = p.undo_info()
+    n_v = sum(p.node_verts(int(l))[1] for l in want)
+    assert info["steps"] == 1 and info["applied"] == 1 and info["leaves"] == len(want)
+    assert info["bytes"] == 24 * n_v + 80 * len(want)
+    assert p.undo()
+    assert _same(_state(p), s0)
+    assert not p.undo()  # nothing left
+    assert p.redo()
+    assert _same(_state(p), s1)
+    assert not p.redo()
+    # undo, then a dab outside any step: the mesh continues from the undone state
+    assert p.undo()
+    p.dab(dabs[0])
+    q = sg.PBVH(co, tris, leaf_limit=64)
+    q.dab(dabs[0])
+    assert _same(_state(p), _state(q))
+    q.close()
+    p.close()
+    o.close()
+
+
+def test_undo_several_steps_and_dropped_redo(sg):
+    co, tris = sg.noisy_icosphere(5, seed=22, amplitude=0.02)
+    p = sg.PBVH(co, tris, leaf_limit=32)
+    states = [_state(p)]
+    for k in range(3):
+        print("[undo] step", k, flush=True)
+        p.undo_begin()
+        p.stroke(_stroke_dabs(sg, 10 + k, n=12))
+        p.undo_end()
+        states.append(_state(p))
+    assert p.undo_info()["steps"] == 3
+    for k in (2, 1, 0):
+        assert p.undo()
+        assert _same(_state(p), states[k]), k
+    assert not p.undo()
+    for k in (1, 2):
+        assert p.redo()
+        assert _same(_state(p), states[k]), k
+    # a new step drops the step still undone
+    p.undo_begin()
+    p.stroke(_stroke_dabs(sg, 40, n=8))
+    p.undo_end()
+    s_new = _state(p)
+    info = p.undo_info()
+    assert info["steps"] == 3 and info["applied"] == 3
+    assert not p.redo()
+    assert p.undo()
+    assert _same(_state(p), states[2])
+    assert p.redo()
+    assert _same(_state(p), s_new)
+    p.undo_clear()
+    assert p.undo_info()["steps"] == 0 and not p.undo()
+    p.close()
+
+
+def test_undo_overflow_drops_step(sg):
+    co, tris = sg.noisy_icosphere(4, seed=23, amplitude=0.02)
+    p = sg.PBVH(co, tris, leaf_limit=64)
+    s0 = _state(p)
+    p.undo_begin(max_verts=50)  # far less than one leaf set of a stroke
+    p.stroke(_stroke_dabs(sg, 5, n=8))
+    with pytest.raises(RuntimeError, match="overflow"):
+        p.undo_end()
+    assert p.undo_info()["steps"] == 0
+    s1 = _state(p)
+    # a later step with room records normally
+    p.undo_begin()
+    p.stroke(_stroke_dabs(sg, 6, n=8))
+    p.undo_end()
+    assert p.undo()
+    assert _same(_state(p), s1)
+    assert not _same(s1, s0)
+    p.close()
+
+
+def test_undo_placed_stroke_in_graph_and_host_push(sg, po):
+    """A captured placed stroke records its leaves at run time (the step opened after the
+    graph was built), and a host edit is saved by pushing nodes explicitly."""
+    co, tris = sg.noisy_icosphere(5, seed=24, amplitude=0.02)
+    p = sg.PBVH(co, tris, leaf_limit=64)
+    n = 16
+    rng = np.random.default_rng(7)
+    ang = np.linspace(0, 1.0, n)
+    starts = np.stack([np.cos(ang) * 3, np.sin(ang) * 3, 0.2 * np.ones(n)], 1).astype(F32)
+    dirs = (-starts / np.linalg.norm(starts, axis=1, keepdims=True)).astype(F32)
+    dabs = [sg.make_dab(tool=[1, 2, 4, 5][i % 4], location=(0, 0, 0), radius=0.12,
+                        strength=float(rng.uniform(0.3, 0.9))) for i in range(n)]
+    s0 = _state(p)
+    st = p.prepare_placed(dabs, starts, dirs)
+    p.undo_begin()
+    print("[undo] prepared placed stroke", flush=True)
+    st.launch()
+    p.undo_end()
+    s1 = _state(p)
+    assert not _same(s1, s0)
+    assert p.undo()
+    assert _same(_state(p), s0)
+    assert p.redo()
+    assert _same(_state(p), s1)
+    # host edit of every vertex (BKE_pbvh_vert_coords_apply): push every leaf first
+    o = po.Oracle(co, tris, leaf_limit=64)
+    leaves = o.leaves_dfs()
+    o.close()
+    p.undo_begin()
+    with pytest.raises(RuntimeError, match="not a leaf"):
+        p.undo_push_node(0)  # the root
+    for i in leaves:
+        p.undo_push_node(int(i))
+    moved = np.ascontiguousarray(s1[0] * F32(1.01), F32)
+    print("[undo] host coordinate edit", flush=True)
+    sg.lib().BKE_pbvh_vert_coords_apply(p.h, moved.ctypes.data, len(moved))
+    assert np.array_equal(p.positions(), moved)
+    p.undo_end()
+    assert p.undo_info()["leaves"] == len(leaves)
+    assert p.undo()
+    assert _same(_state(p), s1)
+    p.close()
+
+
+def test_undo_refreshes_draw_buffers_and_ray_cast(sg, po):
+    co, tris = sg.noisy_icosphere(5, seed=25, amplitude=0.02)
+    P = len(tris)
+    off = np.arange(0, 3 * P + 1, 3)
+    p = sg.PBVH(co, None, leaf_limit=64,
+                polys=(off, tris.reshape(-1).astype(np.int32), np.ones(P, np.uint8), np.zeros(P, np.int32)))
+    p.draw_build(np.ones(P, np.int32))
+    o = po.Oracle(co, p.tris, leaf_limit=64)
+    leaves = [int(i) for i in o.leaves_dfs()]
+    o.close()
+
+    def records():
+        return [p.draw_leaf(i)[0] for i in leaves]
+
+    rng = np.random.default_rng(2)
+    d = rng.normal(size=(256, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    starts, dirs = (3 * d).astype(F32), (-d).astype(F32)
+    r0, h0 = records(), p.raycast(starts, dirs)
+    p.undo_begin()
+    print("[undo] stroke with draw buffers", flush=True)
+    p.stroke(_stroke_dabs(sg, 8, n=20))
+    p.undo_end()
+    p.draw_update()
+    r1, h1 = records(), p.raycast(starts, dirs)
+    assert any(not np.array_equal(a, b) for a, b in zip(r0, r1))
+    assert p.undo()
+    p.draw_update()
+    assert all(np.array_equal(a, b) for a, b in zip(records(), r0))
+    assert p.raycast(starts, dirs).tobytes() == h0.tobytes()
+    assert p.redo()
+    p.draw_update()
+    assert all(np.array_equal(a, b) for a, b in zip(records(), r1))
+    assert p.raycast(starts, dirs).tobytes() == h1.tobytes()
+    p.close()
+
+
+def test_undo_learned_dabs(sg):
+    """Learned Smooth dabs save their leaves in the learned brush kernel."""
+    co, tris = sg.noisy_icosphere(5, seed=26, amplitude=0.02)
+    m = sg.MLP(seed=9)
+    alpha, _ = m.decision()
+    m.set_decision(alpha, 1e30)  # accept every learned dab
+    p = sg.PBVH(co, tris, leaf_limit=64)
+    p.set_learned(m)
+    radii = np.full(10, 0.15, F32)
+    c = sg.great_circle_centres(12, radii)
+    dabs = [sg.make_dab(tool=sg.TOOL_SMOOTH, location=ci, radius=0.15, strength=0.6, smooth_iterations=2,
+                        flag=sg.SMOOTH_LEARNED) for ci in c]
+    s0 = _state(p)
+    p.undo_begin()
+    print("[undo] learned dabs", flush=True)
+    p.stroke(dabs)
+    p.undo_end()
+    assert p.learned_stats()["accepted"] > 0
+    s1 = _state(p)
+    assert not _same(s1, s0)
+    assert p.undo()
+    assert _same(_state(p), s0)
+    assert p.redo()
+    assert _same(_state(p), s1)
+    p.close()
+    m.close()
