@@ -1117,6 +1117,7 @@ int sg_dab_last_affected(PBVH *p, int *out, int cap) {
 int sg_dab_last_stats(PBVH *p, int64_t out[24]) {
   if (!p) return sg_set_error("null PBVH");
   for (int i = 0; i < 24; i++) out[i] = p->last_ctr[i];
+  out[SG_CTR_NGATHER] &= 0xffffffffll;  // the high half flags an open undo step
   return 0;
 }
 

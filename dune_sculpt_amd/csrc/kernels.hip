@@ -64,10 +64,13 @@ __global__ __launch_bounds__(256) void k_search(SgDevView d, SgDabK k, long long
   const int nw = gridDim.x * blockDim.x / SG_WAVE;
   if (gw == 0 && lane < SG_CTR_COUNT) {
     const long long prev = d.ctr_next[lane];  // the set the previous dab used
-    if (log && log_index > 0) log[(size_t)(log_index - 1) * SG_CTR_COUNT + lane] = prev;
+    if (log && log_index > 0)
+      log[(size_t)(log_index - 1) * SG_CTR_COUNT + lane] = lane == SG_CTR_NGATHER ? (prev & 0xffffffffll) : prev;
     d.ctr_next[lane] = 0;
   }
   if (gw == 0 && lane == 0 && d.seq) d.seq[0] += 1;  // this dab's stamp for the later kernels
+  if (gw == 0 && lane == 0 && undo_active(d))
+    atomicAdd((unsigned long long *)&d.ctr[SG_CTR_NGATHER], SG_NGATHER_UNDO);
   const float *cut_bbs = k.use_orig ? d.cut_obb : d.cut_bb;
   const float *leaf_bbs = k.use_orig ? d.leaf_obb : d.leaf_bb;
   long long ntests = 0;
@@ -218,7 +221,8 @@ __global__ __launch_bounds__(256) void k_brush(SgDevView d, SgDabK k, const floa
   const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
   const int nw = gridDim.x * blockDim.x / SG_WAVE;
   const int2 vr0 = gw < d.L ? d.gl_v[gw] : make_int2(0, 0);
-  const int ng = (int)d.ctr[SG_CTR_NGATHER];
+  const long long ng_ctr = d.ctr[SG_CTR_NGATHER];
+  const int ng = (int)ng_ctr;
   const bool smooth = k.tool == SG_TOOL_SMOOTH;
   // Draw offset: (normalize(area) * r) * bstrength
   float ox = 0.0f, oy = 0.0f, oz = 0.0f;
@@ -233,10 +237,16 @@ __global__ __launch_bounds__(256) void k_brush(SgDevView d, SgDabK k, const floa
   }
   int ncand = 0;
   long long nbr = 0;
+  if (ng_ctr & SG_NGATHER_UNDO) {
+    // an undo step is open: save this wave's leaves before it writes them (a leaf's
+    // positions and shadow copy are written only by the wave that owns it, so both
+    // still hold the values before the dab); a loop of its own keeps the brush loop's
+    // registers as they are without it
+    for (int g = gw; g < ng; g += nw) undo_save_leaf(d, d.gathered[g], (g == gw) ? vr0 : d.gl_v[g], d.qx, d.qy, d.qz);
+  }
   for (int g = gw; g < ng; g += nw) {
     const int2 vr = (g == gw) ? vr0 : d.gl_v[g];
     ncand += vr.y - vr.x;
-    undo_save_leaf(d, d.gathered[g], vr);  // before this wave writes the leaf
     for (int base = vr.x; base < vr.y; base += 2 * SG_WAVE) {
       const int v0 = base + lane, v1 = base + SG_WAVE + lane;
       const BrushIn in0 = brush_load(d, k, v0, vr.y, smooth);
@@ -686,7 +696,7 @@ __global__ void k_clear_marks(SgDevView d) {
 // copy the counters of the last dab of a stroke into its log row
 __global__ void k_log_row(const long long *ctr, long long *log, int row) {
   const int i = threadIdx.x;
-  if (i < SG_CTR_COUNT) log[(size_t)row * SG_CTR_COUNT + i] = ctr[i];
+  if (i < SG_CTR_COUNT) log[(size_t)row * SG_CTR_COUNT + i] = i == SG_CTR_NGATHER ? (ctr[i] & 0xffffffffll) : ctr[i];
 }
 
 int sg_enqueue_log_row(PBVH *p, long long *log, int row) {

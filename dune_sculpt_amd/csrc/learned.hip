@@ -390,7 +390,8 @@ __global__ __launch_bounds__(LRN_THREADS) void k_brush_learned(SgDevView d, SgDa
   __shared__ unsigned long long s_acc[3];
   if (d.halt && d.halt[0]) return;
   sg_place(k, place);
-  const int nitems = LRN_SLOTS * (int)d.ctr[SG_CTR_NGATHER];
+  const long long ng_ctr = d.ctr[SG_CTR_NGATHER];
+  const int nitems = LRN_SLOTS * (int)ng_ctr;
   if (blockIdx.x * LRN_WAVES >= nitems) return;  // whole block idle: skip the weight fill
   mlp_lds_load<LRN_THREADS>(s, m);
   if (threadIdx.x < 3) s_acc[threadIdx.x] = 0ull;
@@ -401,11 +402,16 @@ __global__ __launch_bounds__(LRN_THREADS) void k_brush_learned(SgDevView d, SgDa
   const int nw = gridDim.x * LRN_WAVES;
   long long esum = 0;
   int nood = 0, naff = 0;
+  if (ng_ctr & SG_NGATHER_UNDO) {
+    // an undo step is open: the learned brush writes the shadow copy and leaves the
+    // positions alone (k_post_learned copies or redoes them), so each leaf is saved
+    // from the positions by the wave of its first slot, whatever the other slots write
+    for (int it = gw; it < nitems; it += nw)
+      if (it % LRN_SLOTS == 0)
+        undo_save_leaf(d, d.gathered[it / LRN_SLOTS], d.gl_v[it / LRN_SLOTS], d.px, d.py, d.pz);
+  }
   for (int it = gw; it < nitems; it += nw) {
     const int2 vr = d.gl_v[it / LRN_SLOTS];
-    // the learned brush writes the positions and leaves the shadow copy alone: the
-    // leaf's first slot saves it for undo from there, whatever the other slots write
-    if (it % LRN_SLOTS == 0) undo_save_leaf(d, d.gathered[it / LRN_SLOTS], vr);
     for (int base = vr.x + (it % LRN_SLOTS) * LRN_ROWS; base < vr.y; base += LRN_SLOTS * LRN_ROWS) {
       float nx, ny, nz;
       const BrushIn in = learned_tile_load(d, k, base, vr.y, lane, nx, ny, nz);

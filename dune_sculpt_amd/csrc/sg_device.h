@@ -128,11 +128,15 @@ __device__ __forceinline__ void sg_place(SgDabK &k, const float *place) {
 }
 
 // Undo recording: save a leaf the running dab is about to write, once per step (one
-// wave; the leaf's uniq vertices are the contiguous range vr).  Reads the shadow copy,
-// which holds the positions from before the dab in every brush kernel.
-__device__ __forceinline__ void undo_save_leaf(const SgDevView &d, int leaf, int2 vr) {
+// wave; the leaf's uniq vertices are the contiguous range vr): positions, normals and
+// the leaf's two boxes, as they were before the dab.
+__device__ __forceinline__ bool undo_active(const SgDevView &d) { return d.undo && d.undo->active; }
+
+// Caller has seen undo_active(d).  (sx, sy, sz): the position arrays that still hold
+// the leaf's positions before the dab.
+__device__ __forceinline__ void undo_save_leaf(const SgDevView &d, int leaf, int2 vr, const float *sx,
+                                               const float *sy, const float *sz) {
   SgUndoCtl *u = d.undo;
-  if (!u || !u->active) return;
   const int lane = lane_id();
   const int n = vr.y - vr.x;
   int e = -1;
@@ -155,9 +159,9 @@ __device__ __forceinline__ void undo_save_leaf(const SgDevView &d, int leaf, int
   for (int i = lane; i < n; i += SG_WAVE) {
     const int v = vr.x + i;
     const size_t o = 3 * (size_t)(vb + i);
-    u->pos[o] = d.qx[v];
-    u->pos[o + 1] = d.qy[v];
-    u->pos[o + 2] = d.qz[v];
+    u->pos[o] = sx[v];
+    u->pos[o + 1] = sy[v];
+    u->pos[o + 2] = sz[v];
     u->nor[o] = d.nx[v];
     u->nor[o + 1] = d.ny[v];
     u->nor[o + 2] = d.nz[v];

@@ -15,6 +15,11 @@ enum {
   SG_CTR_LACC = 17,   // learned Smooth: 1 when the dab was accepted
 };
 
+// High half of the gathered-leaf counter: an undo step is open.  k_search sets it for
+// the brush kernels, which load that counter anyway (a load of its own would add a
+// dependent memory round trip to their start); the count is the low half.
+#define SG_NGATHER_UNDO (1ull << 32)
+
 // pair collection for the learned Smooth (device buffers)
 struct SgCollect {
   float *X, *frame, *T;

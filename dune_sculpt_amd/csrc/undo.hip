@@ -72,7 +72,9 @@ __global__ __launch_bounds__(256) void k_undo_swap(SgDevView d, const int4 *tab,
 }
 
 // Host-side push of one leaf (SCULPT_undo_push_node for host edits).
-__global__ void k_undo_push_leaf(SgDevView d, int leaf, int2 vr) { undo_save_leaf(d, leaf, vr); }
+__global__ void k_undo_push_leaf(SgDevView d, int leaf, int2 vr) {
+  if (undo_active(d)) undo_save_leaf(d, leaf, vr, d.px, d.py, d.pz);
+}
 
 // ---------------------------------------------------------------------------
 

@@ -114,6 +114,30 @@ def test_undo_several_steps_and_dropped_redo(sg):
     p.close()
 
 
+def test_counters_and_log_unchanged_while_recording(sg):
+    """The open-step flag the dabs carry for the brush kernels does not leak into the
+    gathered-leaf counts the host reads (per-dab log and last-dab statistics)."""
+    co, tris = sg.noisy_icosphere(4, seed=27, amplitude=0.02)
+    dabs = _stroke_dabs(sg, 4, n=8)
+    p = sg.PBVH(co, tris, leaf_limit=64)
+    q = sg.PBVH(co, tris, leaf_limit=64)
+    p.undo_begin()
+    p.stroke(dabs, log=True)
+    lp, sp = p.stroke_log(len(dabs)), p.last_stats()
+    p.undo_end()
+    q.stroke(dabs, log=True)
+    lq, sq = q.stroke_log(len(dabs)), q.last_stats()
+    assert lp[:, 0].max() > 0 and lp[:, 0].max() < 1 << 32
+    assert np.array_equal(lp, lq) and np.array_equal(sp, sq)
+    assert _same(_state(p), _state(q))
+    p.undo_begin()
+    p.dab(dabs[0])
+    assert p.last_stats()[0] == len(p.last_gathered()) > 0
+    p.undo_end()
+    p.close()
+    q.close()
+
+
 def test_undo_overflow_drops_step(sg):
     co, tris = sg.noisy_icosphere(4, seed=23, amplitude=0.02)
     p = sg.PBVH(co, tris, leaf_limit=64)
