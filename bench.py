@@ -615,6 +615,7 @@ def main():
     ap.add_argument("--no-placed", action="store_true", help="skip the placed-stroke measurement")
     ap.add_argument("--no-draw", action="store_true", help="skip the draw-buffer measurement")
     ap.add_argument("--no-undo", action="store_true", help="skip the undo-step measurement")
+    ap.add_argument("--only-undo", action="store_true", help="run only the undo-step measurement")
     ap.add_argument("--placed-only", action="store_true",
                     help="placed stroke without the fixed-centre reference stroke (one captured graph)")
     args = ap.parse_args()
@@ -655,6 +656,32 @@ def main():
             out["placed_stroke"] = run_placed(sg, A, dabs, W, K, args.placed_only, args.mode == "stream")
         if not args.no_draw:
             out["draw_buffers"] = run_draw(sg, A, dabs, W, max(K, 3))
+        print(json.dumps(out))
+        A.close()
+        return
+    if args.only_undo:
+        out = {}
+        if args.copy_mb:
+            ms, gbps_copy = sg.hbm_copy_probe(args.copy_mb << 20)
+            out["hbm_copy"] = {"bytes": args.copy_mb << 20, "ms": ms, "gbps_read_plus_write": gbps_copy}
+        prepared = A.prepare(dabs) if args.mode == "graph" else None
+
+        def launch():
+            if prepared is not None:
+                prepared.launch(sync=False)
+            else:
+                A.stroke(dabs, graph=False)
+
+        base = []
+        for s in range(W + K):
+            A.restore()
+            A.sync()
+            A.timer_start()
+            launch()
+            ms = A.timer_stop()
+            if s >= W:
+                base.append(ms)
+        out["undo"] = run_undo(sg, A, launch, float(np.mean(base)), args.dabs, W, K)
         print(json.dumps(out))
         A.close()
         return

@@ -128,6 +128,7 @@ EXPORTS = [
     "sg_pbvh_draw_buffers_build", "sg_pbvh_draw_buffers_update", "sg_pbvh_draw_buffers_info",
     "sg_pbvh_draw_buffer_node", "sg_pbvh_draw_buffer_read", "sg_pbvh_draw_buffers_free",
     "sg_undo_begin", "sg_undo_push_node", "sg_undo_end", "sg_undo", "sg_redo", "sg_undo_info", "sg_undo_clear",
+    "sg_undo_cancel", "sg_undo_original", "sg_pbvh_combine_proxies",
 ]
 SMOOTH_LEARNED = 1 << 30
 NCTR = 24
@@ -271,6 +272,9 @@ def lib():
         "sg_redo": (i32, [vp]),
         "sg_undo_info": (i32, [vp, C.POINTER(C.c_longlong)]),
         "sg_undo_clear": (None, [vp]),
+        "sg_undo_cancel": (i32, [vp]),
+        "sg_undo_original": (i32, [vp, vp, vp]),
+        "sg_pbvh_combine_proxies": (i32, [vp, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -676,6 +680,34 @@ class PBVH:
 
     def undo_clear(self):
         lib().sg_undo_clear(self.h)
+
+    def undo_cancel(self):
+        _check(lib().sg_undo_cancel(self.h), "undo cancel")
+
+    def undo_original(self):
+        """(positions, normals) at the begin of the open undo step (current ones when
+        none is open), in the caller's vertex order."""
+        co = np.empty((self.totvert, 3), np.float32)
+        no = np.empty((self.totvert, 3), np.float32)
+        _check(lib().sg_undo_original(self.h, co.ctypes.data, no.ctypes.data), "undo original")
+        return co, no
+
+    # -- proxies
+    def add_proxy(self, index):
+        """BKE_pbvh_node_add_proxy on node `index`: a zeroed (uniq_verts, 3) float32
+        array backed by the proxy's host buffer (write displacements into it)."""
+        node = self.node(index)
+        px = lib().BKE_pbvh_node_add_proxy(self.h, node)
+        if not px:
+            raise RuntimeError("add proxy failed: " + last_error())
+        uniq = self.node_verts(index)[1]
+        co_ptr = C.cast(px, C.POINTER(C.POINTER(C.c_float))).contents
+        return np.ctypeslib.as_array(co_ptr, shape=(max(uniq, 1), 3))[:uniq]
+
+    def combine_proxies(self, use_orco=False):
+        """sg_pbvh_combine_proxies: folds every node's proxies into the mesh on the GPU;
+        returns the number of leaves changed."""
+        return _check(lib().sg_pbvh_combine_proxies(self.h, 1 if use_orco else 0), "combine proxies")
 
     def close(self):
         for st in getattr(self, "_strokes", []):

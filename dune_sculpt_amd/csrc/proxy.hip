@@ -1,0 +1,152 @@
+// Proxies combined on the GPU.
+//
+// A proxy is a per-leaf buffer of displacements, one float3 per uniq vertex of the
+// leaf, zeroed when added (BKE_pbvh_node_add_proxy, kernel/intern/pbvh.c:4757-4776),
+// which a brush written against the reference's API fills on the host.  The sculpt
+// code that folds them into the mesh (sculpt_combine_proxies) is absent from the
+// reference; Blender's published version does, for every node BKE_pbvh_gather_proxies
+// returns (pbvh.c:4791-4818), for each uniq vertex: val = original coordinate (when the
+// tool works from the original mesh) or the current one, val += each proxy in order
+// (add_v3_v3), then SCULPT_clip (a plain copy without mirror clipping), the vertex is
+// flagged for the normal update, and the node's proxies are freed
+// (BKE_pbvh_node_free_proxies, :4778-4789).  Here the proxies of all nodes go to HBM in
+// one copy and one kernel writes the vertices, marks them for normals and saves each
+// leaf for the open undo step first; the normals, leaf boxes and search boxes follow
+// on the GPU.  The original coordinates are those of the open undo step (the state at
+// sg_undo_begin): the reference reads them from its undo nodes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "sg_internal.h"
+#include "sg_kernels.h"
+#include "sg_device.h"
+
+// One wave per leaf holding proxies.  tab: leaf, proxy count, float offset low, high
+// of the leaf's packed proxies ([count][uniq][3]).
+__global__ __launch_bounds__(256) void k_combine_proxies(SgDevView d, const int4 *tab, const float *prox, int n,
+                                                         int use_orco) {
+  const int lane = lane_id();
+  const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
+  const int nw = gridDim.x * blockDim.x / SG_WAVE;
+  for (int e = gw; e < n; e += nw) {
+    const int4 t = tab[e];
+    const int leaf = t.x, np = t.y;
+    const size_t off = ((size_t)(unsigned)t.w << 32) | (unsigned)t.z;
+    const int2 vr = make_int2(d.leaf_vbeg[leaf], d.leaf_vbeg[leaf + 1]);
+    const int nu = vr.y - vr.x;
+    // original coordinates: the step's record when an earlier kernel of the open step
+    // saved this leaf; otherwise the leaf is unchanged since the step began
+    const float *orig = nullptr;
+    const bool rec = undo_active(d);
+    if (rec && use_orco) {
+      const SgUndoCtl *u = d.undo;
+      if (u->stamp[leaf] == u->step) {
+        const int4 r = u->tab[u->slot[leaf]];
+        orig = u->pos + 3 * (((size_t)(unsigned)r.w << 32) | (unsigned)r.z);
+      }
+    }
+    if (rec) undo_save_leaf(d, leaf, vr, d.px, d.py, d.pz);  // before this wave writes the leaf
+    for (int i = lane; i < nu; i += SG_WAVE) {
+      const int v = vr.x + i;
+      float x, y, z;
+      if (orig) {
+        x = orig[3 * (size_t)i];
+        y = orig[3 * (size_t)i + 1];
+        z = orig[3 * (size_t)i + 2];
+      } else {
+        x = d.px[v];
+        y = d.py[v];
+        z = d.pz[v];
+      }
+      for (int q = 0; q < np; q++) {
+        const float *pq = prox + off + 3 * ((size_t)q * nu + i);
+        x = x + pq[0];
+        y = y + pq[1];
+        z = z + pq[2];
+      }
+      d.px[v] = x;
+      d.py[v] = y;
+      d.pz[v] = z;
+      d.qx[v] = x;
+      d.qy[v] = y;
+      d.qz[v] = z;
+      d.vmark[v] = 1;  // ME_VERT_PBVH_UPDATE: the normal update takes it
+    }
+    if (lane == 0) d.draw_dirty[d.leaf_cut[leaf]] = 1;
+  }
+}
+
+int sg_enqueue_normals_list(PBVH *p, const int *leaves_dev, int n);
+int sg_enqueue_refit_list(PBVH *p, const int *leaves_dev, int n);
+
+extern "C" {
+
+// Folds every node's proxies into the mesh and frees them.  use_orco != 0 starts from
+// the original coordinates (those of the open undo step), else from the current ones.
+// Returns the number of leaves changed.
+int sg_pbvh_combine_proxies(PBVH *p, int use_orco) {
+  if (!p || p->no_device || !p->build_ok) return sg_set_error("no device mesh");
+  std::vector<int> leaves;
+  std::vector<int4> tab;
+  size_t total = 0;
+  for (PBVHNode &n : p->nodes) {
+    if (n.proxy_count <= 0) continue;
+    if (n.leaf < 0) return sg_set_error("a proxy on an internal node");
+    const size_t cnt = (size_t)n.proxy_count * n.uniq_verts * 3;
+    tab.push_back(make_int4(n.leaf, n.proxy_count, (int)(unsigned)total, (int)(unsigned)(total >> 32)));
+    leaves.push_back(n.leaf);
+    total += cnt;
+  }
+  if (leaves.empty()) return 0;
+  std::vector<float> packed(std::max<size_t>(total, 1));
+  size_t w = 0;
+  for (PBVHNode &n : p->nodes) {
+    if (n.proxy_count <= 0) continue;
+    for (int q = 0; q < n.proxy_count; q++) {
+      memcpy(packed.data() + w, n.proxies[q].co, (size_t)n.uniq_verts * 3 * sizeof(float));
+      w += (size_t)n.uniq_verts * 3;
+    }
+  }
+  float *dprox = nullptr;
+  int4 *dtab = nullptr;
+  int *dleaves = nullptr;
+  const int nl = (int)leaves.size();
+  int rc = sg_check_hip(hipMalloc(&dprox, packed.size() * sizeof(float)), "hipMalloc");
+  if (!rc) rc = sg_check_hip(hipMalloc(&dtab, tab.size() * sizeof(int4)), "hipMalloc");
+  if (!rc) rc = sg_check_hip(hipMalloc(&dleaves, leaves.size() * sizeof(int)), "hipMalloc");
+  if (!rc)
+    rc = sg_check_hip(hipMemcpyAsync(dprox, packed.data(), packed.size() * sizeof(float), hipMemcpyHostToDevice,
+                                     p->stream),
+                      "copy");
+  if (!rc)
+    rc = sg_check_hip(hipMemcpyAsync(dtab, tab.data(), tab.size() * sizeof(int4), hipMemcpyHostToDevice, p->stream),
+                      "copy");
+  if (!rc)
+    rc = sg_check_hip(
+        hipMemcpyAsync(dleaves, leaves.data(), leaves.size() * sizeof(int), hipMemcpyHostToDevice, p->stream), "copy");
+  if (!rc) {
+    const int blocks = std::max(1, std::min((nl + 3) / 4, 8192));
+    hipLaunchKernelGGL(k_combine_proxies, dim3(blocks), dim3(256), 0, p->stream, sg_view(p), dtab, dprox, nl,
+                       use_orco ? 1 : 0);
+    rc = sg_check_hip(hipGetLastError(), "combine launch");
+  }
+  // normals of the marked vertices, then leaf boxes and search boxes, then the tree
+  if (!rc) rc = sg_enqueue_normals_list(p, dleaves, nl);
+  if (!rc) rc = sg_enqueue_refit_list(p, dleaves, nl);
+  if (!rc) rc = sg_enqueue_flush(p);
+  if (!rc) rc = sg_check_hip(hipStreamSynchronize(p->stream), "combine proxies");
+  if (dprox) (void)hipFree(dprox);
+  if (dtab) (void)hipFree(dtab);
+  if (dleaves) (void)hipFree(dleaves);
+  if (rc) return rc;
+  p->nodes_dirty_deferred = false;
+  p->host_bb_stale = p->host_verts_stale = p->host_normals_stale = true;
+  for (PBVHNode &n : p->nodes)
+    if (n.proxy_count > 0) BKE_pbvh_node_free_proxies(&n);
+  return nl;
+}
+
+}  // extern "C"

@@ -149,6 +149,7 @@ __device__ __forceinline__ void undo_save_leaf(const SgDevView &d, int leaf, int
       e = -1;
     } else {
       u->tab[e] = make_int4(leaf, n, (int)(unsigned)vb, (int)(unsigned)(vb >> 32));
+      u->slot[leaf] = e;
       u->stamp[leaf] = u->step;
     }
   }

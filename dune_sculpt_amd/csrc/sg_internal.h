@@ -127,6 +127,7 @@ struct SgDev {
   int draw_ntri = 0, draw_seed = 0, draw_fset_default = 0;
   SgUndoCtl *undo = nullptr;    // undo recording control (lives with the PBVH)
   int *undo_stamp = nullptr;    // [L]
+  int *undo_slot = nullptr;     // [L]
   float *place = nullptr;       // [8 per dab] placements of a ray-placed stroke
   SgRay *place_rays = nullptr;  // the stroke's rays
   int place_cap = 0;

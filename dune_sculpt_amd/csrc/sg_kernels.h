@@ -27,15 +27,16 @@ struct SgCollect {
 };
 
 // Undo recording (undo.hip).  Lives as long as the PBVH so captured strokes can read it
-// at run time: while `active`, the first brush kernel that writes a leaf in the open
-// step copies the leaf's uniq positions and normals (from the shadow copy, equal to
-// the positions before the dab) and its two boxes into the recording arena.
+// at run time: while `active`, the first kernel that writes a leaf in the open step
+// copies the leaf's uniq positions and normals as they were before the step, and its
+// two boxes, into the recording arena.
 struct SgUndoCtl {
   int active, step, overflow, n_e;  // n_e: entries used
   unsigned long long n_v;           // vertices used
   unsigned long long cap_v;
   int cap_e, pad;
   int *stamp;   // [L] step that saved the leaf
+  int *slot;    // [L] its entry in tab (valid when stamp == step)
   float *pos;   // [cap_v * 3]
   float *nor;   // [cap_v * 3]
   int4 *tab;    // [cap_e]: leaf, count, vertex offset low, high

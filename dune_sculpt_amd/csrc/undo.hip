@@ -71,6 +71,34 @@ __global__ __launch_bounds__(256) void k_undo_swap(SgDevView d, const int4 *tab,
   }
 }
 
+// Original data of the open step: the saved leaves' positions and normals over the
+// current ones (copied in first), leaf-major SoA.
+__global__ __launch_bounds__(256) void k_undo_orig(SgDevView d, const int4 *tab, const float *pos, const float *nor,
+                                                   int n_e, float *ox, float *oy, float *oz, float *onx, float *ony,
+                                                   float *onz) {
+  const int lane = lane_id();
+  const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
+  const int nw = gridDim.x * blockDim.x / SG_WAVE;
+  for (int e = gw; e < n_e; e += nw) {
+    const int4 t = tab[e];
+    const unsigned long long vb = ((unsigned long long)(unsigned)t.w << 32) | (unsigned)t.z;
+    const int v0 = d.leaf_vbeg[t.x];
+    for (int i = lane; i < t.y; i += SG_WAVE) {
+      const size_t o = 3 * (size_t)(vb + i);
+      if (ox) {
+        ox[v0 + i] = pos[o];
+        oy[v0 + i] = pos[o + 1];
+        oz[v0 + i] = pos[o + 2];
+      }
+      if (onx) {
+        onx[v0 + i] = nor[o];
+        ony[v0 + i] = nor[o + 1];
+        onz[v0 + i] = nor[o + 2];
+      }
+    }
+  }
+}
+
 // Host-side push of one leaf (SCULPT_undo_push_node for host edits).
 __global__ void k_undo_push_leaf(SgDevView d, int leaf, int2 vr) {
   if (undo_active(d)) undo_save_leaf(d, leaf, vr, d.px, d.py, d.pz);
@@ -113,6 +141,7 @@ static int write_ctl(PBVH *p, bool active, int step) {
   c.cap_v = (unsigned long long)u.rec_cap_v;
   c.cap_e = u.rec_cap_e;
   c.stamp = p->d.undo_stamp;
+  c.slot = p->d.undo_slot;
   c.pos = u.rec_pos;
   c.nor = u.rec_nor;
   c.tab = u.rec_tab;
@@ -219,6 +248,72 @@ int sg_undo_end(PBVH *p) {
   u.steps.push_back(s);
   u.cursor = (int)u.steps.size();
   return 0;
+}
+
+// Cancels the open step: its leaves go back to their state at sg_undo_begin and the
+// step is dropped (a cancelled stroke).  Fails, changing nothing, when the arena
+// overflowed (some leaves were not saved).
+int sg_undo_cancel(PBVH *p) {
+  if (!p) return sg_set_error("bad arguments");
+  SgUndoState &u = p->undo;
+  if (!u.recording) return sg_set_error("no undo step is open");
+  SgUndoCtl c;
+  int rc = sg_check_hip(hipMemcpyAsync(&c, p->d.undo, sizeof(c), hipMemcpyDeviceToHost, p->stream), "copy");
+  if (!rc) rc = write_ctl(p, false, u.step_id);
+  u.recording = false;
+  if (rc) return rc;
+  if (c.overflow) return sg_set_error("undo step overflowed its arena: cannot cancel");
+  SgUndoStep s;  // the arena itself, not owned
+  s.pos = u.rec_pos;
+  s.nor = u.rec_nor;
+  s.bbs = u.rec_bbs;
+  s.tab = u.rec_tab;
+  s.n_e = c.n_e;
+  s.n_v = (long long)c.n_v;
+  return swap_step(p, s);
+}
+
+// Positions and normals at the begin of the open step (the current ones when no step
+// is open), AoS in the caller's vertex order; either output may be null.  The original
+// coordinates the reference reads from its undo nodes (SculptOrigVertData) for Grab,
+// anchored strokes and BKE_pbvh_node_raycast's origco.
+int sg_undo_original(PBVH *p, float *co, float *no) {
+  if (!p || p->no_device || !p->build_ok) return sg_set_error("no device mesh");
+  SgDev &d = p->d;
+  const size_t V = (size_t)d.V;
+  SgUndoCtl c;
+  memset(&c, 0, sizeof(c));
+  if (p->undo.recording) {
+    if (int rc = sg_check_hip(hipMemcpyAsync(&c, d.undo, sizeof(c), hipMemcpyDeviceToHost, p->stream), "copy"))
+      return rc;
+    if (int rc = sg_check_hip(hipStreamSynchronize(p->stream), "undo original")) return rc;
+    if (c.overflow) return sg_set_error("undo step overflowed its arena: original data incomplete");
+  }
+  float *soa = nullptr, *aos = nullptr;
+  int rc = sg_check_hip(hipMalloc(&soa, V * 6 * sizeof(float)), "hipMalloc");
+  if (!rc) rc = sg_check_hip(hipMalloc(&aos, V * 3 * sizeof(float)), "hipMalloc");
+  float *o[6] = {soa, soa + V, soa + 2 * V, soa + 3 * V, soa + 4 * V, soa + 5 * V};
+  const float *cur[6] = {d.px, d.py, d.pz, d.nx, d.ny, d.nz};
+  for (int k = 0; k < 6 && !rc; k++)
+    rc = sg_check_hip(hipMemcpyAsync(o[k], cur[k], V * sizeof(float), hipMemcpyDeviceToDevice, p->stream), "copy");
+  if (!rc && c.n_e > 0) {
+    const int blocks = std::max(1, std::min((c.n_e + 3) / 4, 8192));
+    hipLaunchKernelGGL(k_undo_orig, dim3(blocks), dim3(256), 0, p->stream, sg_view(p), p->undo.rec_tab,
+                       p->undo.rec_pos, p->undo.rec_nor, c.n_e, o[0], o[1], o[2], o[3], o[4], o[5]);
+    rc = sg_check_hip(hipGetLastError(), "undo original launch");
+  }
+  for (int k = 0; k < 2 && !rc; k++) {
+    float *out = k == 0 ? co : no;
+    if (!out) continue;
+    rc = sg_enqueue_gather_raw(p->stream, o[3 * k], o[3 * k + 1], o[3 * k + 2], d.vperm_old, d.V, aos);
+    if (!rc)
+      rc = sg_check_hip(hipMemcpyAsync(out, aos, V * 3 * sizeof(float), hipMemcpyDeviceToHost, p->stream), "copy");
+    if (!rc) rc = sg_check_hip(hipStreamSynchronize(p->stream), "undo original");
+  }
+  if (!rc) rc = sg_check_hip(hipStreamSynchronize(p->stream), "undo original");
+  if (soa) (void)hipFree(soa);
+  if (aos) (void)hipFree(aos);
+  return rc;
 }
 
 // Returns 0 after undoing a step, 1 when there is nothing to undo.

@@ -124,8 +124,8 @@ void BKE_pbvh_search_gather(PBVH *pbvh, BKE_pbvh_SearchCallback scb, void *searc
 void BKE_pbvh_search_callback(PBVH *pbvh, BKE_pbvh_SearchCallback scb, void *search_data,
                               BKE_pbvh_HitCallback hcb, void *hit_data);
 /* pbvh.c:1401 BKE_pbvh_update_bounds: refits flagged leaves and their ancestors on
- * the GPU (host-side edits made through BKE_pbvh_get_verts or proxies are uploaded
- * for the flagged leaves first). */
+ * the GPU (host-side edits made through BKE_pbvh_get_verts are uploaded for the
+ * flagged leaves first; proxies are folded in by sg_pbvh_combine_proxies). */
 void BKE_pbvh_update_bounds(PBVH *pbvh, int flag);
 /* pbvh.c:4559 BKE_pbvh_update_normals: marked vertices of UpdateNormals leaves get
  * normalize(sum of incident unit tri normals) -- deterministic CSR gather on GPU. */
@@ -391,6 +391,23 @@ int sg_undo(PBVH *pbvh);
 int sg_redo(PBVH *pbvh);
 int sg_undo_info(PBVH *pbvh, long long out[4]);
 void sg_undo_clear(PBVH *pbvh);
+/* Cancel of the open step (a cancelled stroke): its leaves return to their state at
+ * begin and the step is dropped. */
+int sg_undo_cancel(PBVH *pbvh);
+/* Original data: positions and normals at the begin of the open step (current ones when
+ * none is open), float[3] per vertex in the caller's order; either may be NULL.  What
+ * the reference reads from its undo nodes (SculptOrigVertData: Grab, anchored strokes,
+ * BKE_pbvh_node_raycast's origco). */
+int sg_undo_original(PBVH *pbvh, float *co, float *no);
+
+/* ---- proxies combined on the GPU -------------------------------------------------
+ * Folds the proxies added with BKE_pbvh_node_add_proxy (host buffers, one float[3] per
+ * uniq vertex, in the order of BKE_pbvh_node_get_verts) into the mesh on the GPU:
+ * vertex = (original coordinate when use_orco, else current) + each proxy in order;
+ * then its normal, the leaf boxes and the tree; frees the proxies (the sculpt code's
+ * combine step over BKE_pbvh_gather_proxies, pbvh.c:4791-4818; sculpt.c is absent).
+ * Original coordinates are those of the open undo step.  Returns the leaves changed. */
+int sg_pbvh_combine_proxies(PBVH *pbvh, int use_orco);
 
 /* ---- learned Smooth (MLP over 1-ring features; BF16 MFMA inference) ----------
  * A Smooth dab with SG_SMOOTH_LEARNED in SgDabParams.flag goes through the model

@@ -1086,6 +1086,56 @@ static void update_bounds(OrcPBVH *o, int nthreads)
   free(upd_of);
 }
 
+/* Normals of the flagged vertices of UpdateNormals leaves (pbvh_faces_update_normals
+ * pbvh.c:1093-1118 structure; each vertex sums its incident tris in vertex->tri CSR
+ * order, spec S10).  Returns the tri reads. */
+static int64_t update_normals(OrcPBVH *o, int nthreads)
+{
+  const int fp64 = o->with_fp64;
+  int *upd;
+  int flag = ORC_UpdateNormals;
+  const int nupd = search_gather(o, update_cb, &flag, &upd, NULL);
+  int64_t tri_reads = 0;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads) reduction(+ : tri_reads)
+  for (int li = 0; li < nupd; li++) {
+    ONode *n = &o->nodes[upd[li]];
+    for (int k = 0; k < n->uniq_verts; k++) {
+      const int v = n->vert_indices[k];
+      if (!o->vert_bitmap[v]) {
+        continue;
+      }
+      float acc[3] = {0.0f, 0.0f, 0.0f};
+      double acc64[3] = {0.0, 0.0, 0.0};
+      tri_reads += o->vt_off[v + 1] - o->vt_off[v];
+      for (int q = o->vt_off[v]; q < o->vt_off[v + 1]; q++) {
+        const int *t = o->tris + 3 * o->vt_idx[q];
+        float fn[3];
+        orc_tri_normal(o->co + 3 * t[0], o->co + 3 * t[1], o->co + 3 * t[2], fn);
+        acc[0] = acc[0] + fn[0];
+        acc[1] = acc[1] + fn[1];
+        acc[2] = acc[2] + fn[2];
+        if (fp64) {
+          double fn64[3];
+          tri_normal64(o->co64 + 3 * t[0], o->co64 + 3 * t[1], o->co64 + 3 * t[2], fn64);
+          acc64[0] += fn64[0];
+          acc64[1] += fn64[1];
+          acc64[2] += fn64[2];
+        }
+      }
+      o_normalize(acc);
+      memcpy(o->no + 3 * v, acc, sizeof(acc));
+      if (fp64) {
+        o_normalize64(acc64);
+        memcpy(o->no64 + 3 * v, acc64, sizeof(acc64));
+      }
+      o->vert_bitmap[v] = 0;
+    }
+    n->flag &= ~ORC_UpdateNormals;
+  }
+  free(upd);
+  return tri_reads;
+}
+
 int orc_dab(OrcPBVH *o, const OrcDab *dab, int *out_affected, int cap)
 {
   memset(o->stats, 0, sizeof(o->stats));
@@ -1383,50 +1433,7 @@ int orc_dab(OrcPBVH *o, const OrcDab *dab, int *out_affected, int cap)
 
   /* Normals (pbvh_faces_update_normals pbvh.c:1093-1118 structure; the accumulate
    * step sums every incident tri in vertex->tri CSR order, spec S10). */
-  {
-    int *upd;
-    int flag = ORC_UpdateNormals;
-    const int nupd = search_gather(o, update_cb, &flag, &upd, NULL);
-    int64_t tri_reads = 0;
-#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads) reduction(+ : tri_reads)
-    for (int li = 0; li < nupd; li++) {
-      ONode *n = &o->nodes[upd[li]];
-      for (int k = 0; k < n->uniq_verts; k++) {
-        const int v = n->vert_indices[k];
-        if (!o->vert_bitmap[v]) {
-          continue;
-        }
-        float acc[3] = {0.0f, 0.0f, 0.0f};
-        double acc64[3] = {0.0, 0.0, 0.0};
-        tri_reads += o->vt_off[v + 1] - o->vt_off[v];
-        for (int q = o->vt_off[v]; q < o->vt_off[v + 1]; q++) {
-          const int *t = o->tris + 3 * o->vt_idx[q];
-          float fn[3];
-          orc_tri_normal(o->co + 3 * t[0], o->co + 3 * t[1], o->co + 3 * t[2], fn);
-          acc[0] = acc[0] + fn[0];
-          acc[1] = acc[1] + fn[1];
-          acc[2] = acc[2] + fn[2];
-          if (fp64) {
-            double fn64[3];
-            tri_normal64(o->co64 + 3 * t[0], o->co64 + 3 * t[1], o->co64 + 3 * t[2], fn64);
-            acc64[0] += fn64[0];
-            acc64[1] += fn64[1];
-            acc64[2] += fn64[2];
-          }
-        }
-        o_normalize(acc);
-        memcpy(o->no + 3 * v, acc, sizeof(acc));
-        if (fp64) {
-          o_normalize64(acc64);
-          memcpy(o->no64 + 3 * v, acc64, sizeof(acc64));
-        }
-        o->vert_bitmap[v] = 0;
-      }
-      n->flag &= ~ORC_UpdateNormals;
-    }
-    o->stats[6] = tri_reads;
-    free(upd);
-  }
+  o->stats[6] = update_normals(o, nthreads);
 
   /* Bounds: BKE_pbvh_update_bounds(UpdateBB|UpdateOriginalBB|UpdateRedraw). */
   update_bounds(o, nthreads);
@@ -1482,5 +1489,47 @@ void orc_set_positions(OrcPBVH *o, const float *co)
       o->nodes[i].flag |= ORC_UpdateBB | ORC_UpdateOriginalBB | ORC_UpdateRedraw;
     }
   }
+  update_bounds(o, orc_get_threads());
+}
+
+/* The sculpt code's proxy combine (sculpt_combine_proxies, absent from the reference;
+ * Blender's published version) over the nodes BKE_pbvh_gather_proxies returns
+ * (pbvh.c:4791-4818), in node order: every uniq vertex i of a node becomes
+ * base + proxy_0[i] + proxy_1[i] + ... (add_v3_v3, component by component, left to
+ * right), base = orco[v] (the original coordinate) when orco is given, else co[v];
+ * SCULPT_clip without mirror clipping copies it; the vertex is flagged
+ * (ME_VERT_PBVH_UPDATE) and the node marked as the brush marks it
+ * (BKE_pbvh_node_mark_update, pbvh.c:1723-1727); then the normal and bounds updates.
+ * prox: the nodes' proxies packed [nprox[j]][uniq_verts][3], node after node. */
+void orc_combine_proxies(OrcPBVH *o, const int *nodes, int n, const int *nprox, const float *prox,
+                         const float *orco)
+{
+  size_t off = 0;
+  for (int j = 0; j < n; j++) {
+    ONode *node = &o->nodes[nodes[j]];
+    const int nu = node->uniq_verts;
+    for (int i = 0; i < nu; i++) {
+      const int v = node->vert_indices[i];
+      const float *b = orco ? orco + 3 * (size_t)v : o->co + 3 * (size_t)v;
+      float val[3] = {b[0], b[1], b[2]};
+      for (int q = 0; q < nprox[j]; q++) {
+        const float *d = prox + off + 3 * ((size_t)q * nu + i);
+        val[0] = val[0] + d[0];
+        val[1] = val[1] + d[1];
+        val[2] = val[2] + d[2];
+      }
+      memcpy(o->co + 3 * (size_t)v, val, sizeof(val));
+      if (o->co64) {
+        o->co64[3 * (size_t)v] = val[0];
+        o->co64[3 * (size_t)v + 1] = val[1];
+        o->co64[3 * (size_t)v + 2] = val[2];
+      }
+      o->vert_bitmap[v] = 1;
+    }
+    off += (size_t)nprox[j] * nu * 3;
+    node->flag |= ORC_UpdateNormals | ORC_UpdateBB | ORC_UpdateOriginalBB | ORC_UpdateDrawBuffers |
+                  ORC_UpdateRedraw;
+  }
+  update_normals(o, orc_get_threads());
   update_bounds(o, orc_get_threads());
 }

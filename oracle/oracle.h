@@ -98,6 +98,11 @@ void orc_set_mask(OrcPBVH *o, const float *mask); /* per-vertex paint mask or NU
 void orc_set_positions(OrcPBVH *o, const float *co);
 /* Replace the vertex normals (e.g. to start two runs from identical state). */
 void orc_set_normals(OrcPBVH *o, const float *no);
+/* Proxy combine over the given nodes (node order): uniq vertex i of node j becomes
+ * (orco or co)[v] + its proxies in order; prox packed [nprox[j]][uniq][3] node after
+ * node; then normals of those vertices and the bounds update. */
+void orc_combine_proxies(OrcPBVH *o, const int *nodes, int n, const int *nprox, const float *prox,
+                         const float *orco);
 
 /* Sphere search (spec S1, pbvh.c:818-849 traversal).  Returns leaf node ids in
  * traversal (left->right) order. */

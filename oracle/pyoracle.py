@@ -74,6 +74,7 @@ def lib():
         "orc_set_mask": (None, [vp, f32p]),
         "orc_set_normals": (None, [vp, f32p]),
         "orc_set_positions": (None, [vp, f32p]),
+        "orc_combine_proxies": (None, [vp, i32p, i32, i32p, f32p, f32p]),
         "orc_search_sphere": (i32, [vp, f32p, C.c_float, i32, i32p, i32]),
         "orc_dab": (i32, [vp, C.POINTER(OrcDab), i32p, i32]),
         "orc_falloff": (C.c_float, [C.POINTER(OrcDab), C.c_float]),
@@ -189,6 +190,15 @@ class Oracle:
     def set_positions(self, co):
         c = np.ascontiguousarray(co, np.float32)
         lib().orc_set_positions(self.h, _fp(c))
+
+    def combine_proxies(self, nodes, proxies, orco=None):
+        """proxies[j]: list of (uniq_verts, 3) arrays for nodes[j] (added order)."""
+        nd = np.ascontiguousarray(nodes, np.int32)
+        npx = np.array([len(pl) for pl in proxies], np.int32)
+        flat = [np.ascontiguousarray(a, np.float32).reshape(-1) for pl in proxies for a in pl]
+        pk = np.concatenate(flat) if flat else np.zeros(1, np.float32)
+        oc = None if orco is None else np.ascontiguousarray(orco, np.float32)
+        lib().orc_combine_proxies(self.h, _ip(nd), len(nd), _ip(npx), _fp(pk), _fp(oc) if oc is not None else None)
 
     def set_mask(self, mask):
         if mask is None:

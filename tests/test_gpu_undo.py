@@ -265,3 +265,80 @@ def test_undo_learned_dabs(sg):
     assert _same(_state(p), s1)
     p.close()
     m.close()
+
+
+def test_undo_cancel_and_original_data(sg):
+    co, tris = sg.noisy_icosphere(5, seed=28, amplitude=0.02)
+    p = sg.PBVH(co, tris, leaf_limit=64)
+    s0 = _state(p)
+    assert _same(p.undo_original(), s0[:2])  # no open step: the current state
+    p.undo_begin()
+    print("[undo] stroke then cancel", flush=True)
+    p.stroke(_stroke_dabs(sg, 14, n=12))
+    assert not _same(_state(p), s0)
+    assert _same(p.undo_original(), s0[:2])  # the state at begin
+    p.undo_cancel()
+    assert _same(_state(p), s0)
+    assert p.undo_info()["steps"] == 0
+    # a cancel keeps the steps before it
+    p.undo_begin()
+    p.stroke(_stroke_dabs(sg, 15, n=12))
+    p.undo_end()
+    s2 = _state(p)
+    p.undo_begin()
+    p.stroke(_stroke_dabs(sg, 16, n=12))
+    p.undo_cancel()
+    assert _same(_state(p), s2) and p.undo_info()["steps"] == 1
+    assert p.undo()
+    assert _same(_state(p), s0)
+    with pytest.raises(RuntimeError, match="no undo step is open"):
+        p.undo_cancel()
+    p.close()
+
+
+def test_combine_proxies_matches_oracle(sg, po):
+    """Proxies on 40 leaves (1-3 each) folded in on the GPU, first from the original
+    coordinates of the open step, then from the current ones: positions, normals and
+    every node box equal the oracle's; undoing the step undoes the stroke and the
+    combine."""
+    from parity_util import n_bit_diff
+
+    co, tris = sg.noisy_icosphere(5, seed=29, amplitude=0.02)
+    p = sg.PBVH(co, tris, leaf_limit=64)
+    o = po.Oracle(co, tris, leaf_limit=64, fp64=False)
+    s0 = _state(p)
+    rng = np.random.default_rng(5)
+    leaves = o.leaves_dfs()
+    assert p.combine_proxies() == 0
+    p.undo_begin()
+    p.stroke(_stroke_dabs(sg, 17, n=10))
+    o.set_positions(p.positions())
+    o.set_normals(p.normals())
+    for use_orco in (True, False):
+        nodes = sorted(int(x) for x in rng.choice(leaves, 40, replace=False))
+        props = []
+        for nd in nodes:
+            lst = []
+            for _ in range(int(rng.integers(1, 4))):
+                a = p.add_proxy(nd)
+                assert a.shape == (p.node_verts(nd)[1], 3) and not a.any()
+                a[:] = rng.normal(0, 0.01, a.shape)
+                lst.append(a.copy())
+            props.append(lst)
+        print(f"[undo] combine proxies use_orco={use_orco}", flush=True)
+        assert p.combine_proxies(use_orco=use_orco) == len(nodes)
+        o.combine_proxies(nodes, props, orco=s0[0] if use_orco else None)
+        pp, op = p.positions(), o.positions()
+        assert n_bit_diff(pp, op) == 0
+        assert n_bit_diff(p.normals(), o.normals()) == 0
+        bad = 0
+        for i in range(o.totnode()):
+            vb, ob = o.node_bb(i)
+            bad += n_bit_diff(p.node_bb(i), vb) != 0 or n_bit_diff(p.node_bb(i, original=True), ob) != 0
+        assert bad == 0
+        assert p.combine_proxies() == 0  # the proxies were freed
+    p.undo_end()
+    assert p.undo()
+    assert _same(_state(p), s0)
+    p.close()
+    o.close()
