@@ -519,6 +519,32 @@ def run_undo(sg, A, launch, base_ms, n_dabs, W, K):
     return res
 
 
+def run_proxies(sg, A, reps):
+    """Proxies combined on the GPU: one proxy on every tenth leaf (~10% of the mesh),
+    filled on the host, then sg_pbvh_combine_proxies (pack + host-to-device copy +
+    combine kernel + normals + leaf and tree refit), timed by wall clock."""
+    A.restore()
+    leaves = A.search_gather()[::10]
+    nverts = sum(A.node_verts(l)[1] for l in leaves)
+    times = []
+    for r in range(reps + 1):
+        for l in leaves:
+            A.add_proxy(l)[:] = 1e-4
+        A.sync()
+        t0 = time.perf_counter()
+        n = A.combine_proxies()
+        dt = time.perf_counter() - t0
+        assert n == len(leaves)
+        if r:
+            times.append(dt)
+    ms = float(np.mean(times)) * 1e3
+    res = {"leaves": len(leaves), "vertices": int(nverts), "combine_ms": ms,
+           "vertices_per_s": nverts / (ms * 1e-3), "host_bytes_uploaded": 12 * int(nverts)}
+    log(f"proxies: {res}")
+    A.restore()
+    return res
+
+
 def cpu_raycast(o, starts, dirs, budget_s):
     """The oracle's sequential ray cast (the reference's traversal and callback) on the
     same camera rays, one host thread, until the budget is spent."""
@@ -682,6 +708,7 @@ def main():
             if s >= W:
                 base.append(ms)
         out["undo"] = run_undo(sg, A, launch, float(np.mean(base)), args.dabs, W, K)
+        out["proxies"] = run_proxies(sg, A, max(K, 2))
         print(json.dumps(out))
         A.close()
         return
@@ -762,6 +789,7 @@ def main():
     kernel_bytes_per_dab = {k: round(v / args.dabs, 1) for k, v in kern_bytes.items()}
 
     undo = None if args.no_undo else run_undo(sg, A, step, float(np.mean(event_ms)), args.dabs, W, K)
+    proxies = None if args.no_undo else run_proxies(sg, A, max(K, 2))
 
     copy = None
     if args.copy_mb:
@@ -847,6 +875,7 @@ def main():
         "placed_stroke": placed,
         "draw_buffers": draw,
         "undo": undo,
+        "proxies": proxies,
 
         "build_s": build_s,
         "autotune": tune,
