@@ -1376,6 +1376,7 @@ const float (*DUNE_pbvh_get_vert_normals(const PBVH *cp))[3] {
 
 PBVHProxyNode *BKE_pbvh_node_add_proxy(PBVH *p, PBVHNode *node) {
   const int index = node->proxy_count;
+  if (index == 0) p->proxy_nodes.push_back((int)(node - p->nodes.data()));  // the combine's worklist
   node->proxy_count++;
   node->proxies = (PBVHProxyNode *)realloc(node->proxies, node->proxy_count * sizeof(PBVHProxyNode));
   int totverts = 0;
@@ -1395,9 +1396,13 @@ void BKE_pbvh_node_free_proxies(PBVHNode *node) {
 }
 
 void BKE_pbvh_gather_proxies(PBVH *p, PBVHNode ***r_array, int *r_tot) {
+  // the nodes holding proxies in node order (the worklist add_proxy keeps, instead of
+  // a walk over every node)
   std::vector<PBVHNode *> v;
-  for (PBVHNode &n : p->nodes) {
-    if (n.proxy_count > 0) v.push_back(&n);
+  std::sort(p->proxy_nodes.begin(), p->proxy_nodes.end());
+  p->proxy_nodes.erase(std::unique(p->proxy_nodes.begin(), p->proxy_nodes.end()), p->proxy_nodes.end());
+  for (int i : p->proxy_nodes) {
+    if (p->nodes[i].proxy_count > 0) v.push_back(&p->nodes[i]);
   }
   *r_array = nullptr;
   *r_tot = 0;

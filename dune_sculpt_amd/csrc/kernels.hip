@@ -425,8 +425,25 @@ __global__ __launch_bounds__(256) void k_post(SgDevView d, int stats, int stamp,
 // over the tree above the cut, with the cut roots as its leaves.
 // ---------------------------------------------------------------------------
 #define FLUSH_THREADS 256
+#define FLUSH_MAXH 256
+
+// Where each height's run starts in a table sorted by height (.w): hs[h] = first entry
+// of height >= h for h = 1..FLUSH_MAXH, n past the end.  Every entry compares its
+// height with its predecessor's, so the runs are found in one parallel pass instead of
+// a serial scan per height.  Block-wide; ends with a barrier.
+__device__ __forceinline__ void height_starts(const int4 *tab, int n, int *hs) {
+  for (int h = threadIdx.x; h <= FLUSH_MAXH; h += blockDim.x) hs[h] = n;
+  __syncthreads();
+  for (int e = threadIdx.x; e < n; e += blockDim.x) {
+    const int w = min(tab[e].w, FLUSH_MAXH), pw = e > 0 ? min(tab[e - 1].w, FLUSH_MAXH) : 0;
+    for (int h = pw + 1; h <= w; h++) hs[h] = e;  // one writer per h: the run boundary
+  }
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(FLUSH_THREADS) void k_flush_sub(SgDevView d, int maxn) {
   extern __shared__ float s_bb[];  // [maxn * 6]
+  __shared__ int s_hs[FLUSH_MAXH + 1];
   const int c = blockIdx.x;
   if (!d.cut_dirty[c]) return;
   const int lb = d.cut_lbeg[c], nl = d.cut_lend[c] - lb;
@@ -439,12 +456,9 @@ __global__ __launch_bounds__(FLUSH_THREADS) void k_flush_sub(SgDevView d, int ma
   }
   __syncthreads();
   const int eb = d.sub_off[c], ee = d.sub_off[c + 1];
-  int h = 1;
-  for (int done = eb; done < ee && h < 256; h++) {
-    int nxt = done;
-    // entries are sorted by height: [done, nxt) is this height
-    while (nxt < ee && d.sub_tab[nxt].w == h) nxt++;
-    for (int e = done + threadIdx.x; e < nxt; e += blockDim.x) {
+  height_starts(d.sub_tab + eb, ee - eb, s_hs);  // entries are sorted by height
+  for (int h = 1; h < FLUSH_MAXH && s_hs[h] < ee - eb; h++) {
+    for (int e = eb + s_hs[h] + threadIdx.x; e < eb + s_hs[h + 1]; e += blockDim.x) {
       const int4 t = d.sub_tab[e];
       const int li = nl + (e - eb);
       for (int q = 0; q < 3; q++) {
@@ -453,7 +467,6 @@ __global__ __launch_bounds__(FLUSH_THREADS) void k_flush_sub(SgDevView d, int ma
       }
     }
     __syncthreads();
-    done = nxt;
   }
   for (int e = eb + threadIdx.x; e < ee; e += blockDim.x) {
     const int li = nl + (e - eb);
@@ -481,6 +494,7 @@ __global__ __launch_bounds__(FLUSH_THREADS) void k_flush_sub(SgDevView d, int ma
 
 __global__ __launch_bounds__(1024) void k_flush_top(SgDevView d, int top_n, int clear) {
   extern __shared__ float s_bb[];  // [(C + top_n) * 6]
+  __shared__ int s_hs[FLUSH_MAXH + 1];
   const int C = d.C;
   for (int i = threadIdx.x; i < C; i += blockDim.x) {
     const BB6 b = bb_load(d.cut_bb + 8 * (size_t)i);
@@ -489,12 +503,9 @@ __global__ __launch_bounds__(1024) void k_flush_top(SgDevView d, int top_n, int 
       s_bb[6 * i + 3 + q] = b.mx[q];
     }
   }
-  __syncthreads();
-  int h = 1;
-  for (int done = 0; done < top_n && h < 256; h++) {
-    int nxt = done;
-    while (nxt < top_n && d.top_tab[nxt].w == h) nxt++;
-    for (int e = done + threadIdx.x; e < nxt; e += blockDim.x) {
+  height_starts(d.top_tab, top_n, s_hs);  // also the barrier after the staging
+  for (int h = 1; h < FLUSH_MAXH && s_hs[h] < top_n; h++) {
+    for (int e = s_hs[h] + threadIdx.x; e < s_hs[h + 1]; e += blockDim.x) {
       const int4 t = d.top_tab[e];
       const int li = C + e;
       for (int q = 0; q < 3; q++) {
@@ -503,7 +514,6 @@ __global__ __launch_bounds__(1024) void k_flush_top(SgDevView d, int top_n, int 
       }
     }
     __syncthreads();
-    done = nxt;
   }
   for (int e = threadIdx.x; e < top_n; e += blockDim.x) {
     const int li = C + e;

@@ -207,6 +207,7 @@ struct Builder {
     for (int i = 0; i < T; i++) bb_expand(cb_root, bbc[i].bcentroid);
 
     p->nodes.clear();
+    p->proxy_nodes.clear();
     p->nodes.reserve((size_t)std::max(2 * (T / std::max(p->leaf_limit / 2, 1)) + 16, 16));
     new_node();
     std::vector<Task> stack;
@@ -710,9 +711,11 @@ void sg_free_device(PBVH *p) {
                   d.stamp_a, d.curve, d.log, d.ltri, d.leaf_toff, d.tri_poly, d.node_parent, d.leaf_node,
                   d.ray_in, d.ray_out, d.place, d.place_rays, d.rl_list, d.rl_args,
                   d.draw_dirty, d.draw_tri, d.draw_off, d.draw_leaf, d.draw_fset, d.draw_poly_off,
-                  d.draw_poly_v, d.draw_vbo, d.undo, d.undo_stamp, d.undo_slot};
+                  d.draw_poly_v, d.draw_vbo, d.undo, d.undo_stamp, d.undo_slot,
+                  d.prox_dev, d.prox_tab, d.prox_leaves};
   for (void *q : ptrs) {
     if (q) hipFree(q);
   }
+  if (d.prox_host) (void)hipHostFree(d.prox_host);
   d = SgDev();
 }

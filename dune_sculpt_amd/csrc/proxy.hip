@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -92,8 +93,17 @@ int sg_pbvh_combine_proxies(PBVH *p, int use_orco) {
   std::vector<int> leaves;
   std::vector<int4> tab;
   size_t total = 0;
-  for (PBVHNode &n : p->nodes) {
-    if (n.proxy_count <= 0) continue;
+  // nodes holding proxies, in node order (BKE_pbvh_gather_proxies)
+  std::vector<PBVHNode *> nodes;
+  {
+    PBVHNode **arr = nullptr;
+    int tot = 0;
+    BKE_pbvh_gather_proxies(p, &arr, &tot);
+    nodes.assign(arr, arr + tot);
+    free(arr);
+  }
+  for (PBVHNode *np : nodes) {
+    const PBVHNode &n = *np;
     if (n.leaf < 0) return sg_set_error("a proxy on an internal node");
     const size_t cnt = (size_t)n.proxy_count * n.uniq_verts * 3;
     tab.push_back(make_int4(n.leaf, n.proxy_count, (int)(unsigned)total, (int)(unsigned)(total >> 32)));
@@ -101,51 +111,67 @@ int sg_pbvh_combine_proxies(PBVH *p, int use_orco) {
     total += cnt;
   }
   if (leaves.empty()) return 0;
-  std::vector<float> packed(std::max<size_t>(total, 1));
+  SgDev &d = p->d;
+  const int nl = (int)leaves.size();
+  const size_t need = std::max<size_t>(total, 1);
+  int rc = 0;
+  if (need > d.prox_cap) {  // grow-only staging: pinned host pack and its device copy
+    if (d.prox_host) (void)hipHostFree(d.prox_host);
+    if (d.prox_dev) (void)hipFree(d.prox_dev);
+    d.prox_host = d.prox_dev = nullptr;
+    d.prox_cap = 0;
+    const size_t cap = need + need / 2;
+    rc = sg_check_hip(hipHostMalloc(&d.prox_host, cap * sizeof(float), hipHostMallocDefault), "hipHostMalloc");
+    if (!rc) rc = sg_check_hip(hipMalloc(&d.prox_dev, cap * sizeof(float)), "hipMalloc");
+    if (rc) return rc;
+    d.prox_cap = cap;
+  }
+  if (nl > d.prox_tab_cap) {
+    if (d.prox_tab) (void)hipFree(d.prox_tab);
+    if (d.prox_leaves) (void)hipFree(d.prox_leaves);
+    d.prox_tab = nullptr;
+    d.prox_leaves = nullptr;
+    d.prox_tab_cap = 0;
+    const int cap = nl + nl / 2;
+    rc = sg_check_hip(hipMalloc(&d.prox_tab, (size_t)cap * sizeof(int4)), "hipMalloc");
+    if (!rc) rc = sg_check_hip(hipMalloc(&d.prox_leaves, (size_t)cap * sizeof(int)), "hipMalloc");
+    if (rc) return rc;
+    d.prox_tab_cap = cap;
+  }
   size_t w = 0;
-  for (PBVHNode &n : p->nodes) {
-    if (n.proxy_count <= 0) continue;
+  for (PBVHNode *np : nodes) {
+    const PBVHNode &n = *np;
     for (int q = 0; q < n.proxy_count; q++) {
-      memcpy(packed.data() + w, n.proxies[q].co, (size_t)n.uniq_verts * 3 * sizeof(float));
+      memcpy(d.prox_host + w, n.proxies[q].co, (size_t)n.uniq_verts * 3 * sizeof(float));
       w += (size_t)n.uniq_verts * 3;
     }
   }
-  float *dprox = nullptr;
-  int4 *dtab = nullptr;
-  int *dleaves = nullptr;
-  const int nl = (int)leaves.size();
-  int rc = sg_check_hip(hipMalloc(&dprox, packed.size() * sizeof(float)), "hipMalloc");
-  if (!rc) rc = sg_check_hip(hipMalloc(&dtab, tab.size() * sizeof(int4)), "hipMalloc");
-  if (!rc) rc = sg_check_hip(hipMalloc(&dleaves, leaves.size() * sizeof(int)), "hipMalloc");
+  rc = sg_check_hip(hipMemcpyAsync(d.prox_dev, d.prox_host, need * sizeof(float), hipMemcpyHostToDevice, p->stream),
+                    "copy");
   if (!rc)
-    rc = sg_check_hip(hipMemcpyAsync(dprox, packed.data(), packed.size() * sizeof(float), hipMemcpyHostToDevice,
+    rc = sg_check_hip(hipMemcpyAsync(d.prox_tab, tab.data(), tab.size() * sizeof(int4), hipMemcpyHostToDevice,
                                      p->stream),
                       "copy");
   if (!rc)
-    rc = sg_check_hip(hipMemcpyAsync(dtab, tab.data(), tab.size() * sizeof(int4), hipMemcpyHostToDevice, p->stream),
+    rc = sg_check_hip(hipMemcpyAsync(d.prox_leaves, leaves.data(), leaves.size() * sizeof(int),
+                                     hipMemcpyHostToDevice, p->stream),
                       "copy");
-  if (!rc)
-    rc = sg_check_hip(
-        hipMemcpyAsync(dleaves, leaves.data(), leaves.size() * sizeof(int), hipMemcpyHostToDevice, p->stream), "copy");
   if (!rc) {
     const int blocks = std::max(1, std::min((nl + 3) / 4, 8192));
-    hipLaunchKernelGGL(k_combine_proxies, dim3(blocks), dim3(256), 0, p->stream, sg_view(p), dtab, dprox, nl,
-                       use_orco ? 1 : 0);
+    hipLaunchKernelGGL(k_combine_proxies, dim3(blocks), dim3(256), 0, p->stream, sg_view(p), d.prox_tab, d.prox_dev,
+                       nl, use_orco ? 1 : 0);
     rc = sg_check_hip(hipGetLastError(), "combine launch");
   }
   // normals of the marked vertices, then leaf boxes and search boxes, then the tree
-  if (!rc) rc = sg_enqueue_normals_list(p, dleaves, nl);
-  if (!rc) rc = sg_enqueue_refit_list(p, dleaves, nl);
+  if (!rc) rc = sg_enqueue_normals_list(p, d.prox_leaves, nl);
+  if (!rc) rc = sg_enqueue_refit_list(p, d.prox_leaves, nl);
   if (!rc) rc = sg_enqueue_flush(p);
   if (!rc) rc = sg_check_hip(hipStreamSynchronize(p->stream), "combine proxies");
-  if (dprox) (void)hipFree(dprox);
-  if (dtab) (void)hipFree(dtab);
-  if (dleaves) (void)hipFree(dleaves);
   if (rc) return rc;
   p->nodes_dirty_deferred = false;
   p->host_bb_stale = p->host_verts_stale = p->host_normals_stale = true;
-  for (PBVHNode &n : p->nodes)
-    if (n.proxy_count > 0) BKE_pbvh_node_free_proxies(&n);
+  for (PBVHNode *np : nodes) BKE_pbvh_node_free_proxies(np);
+  p->proxy_nodes.clear();
   return nl;
 }
 

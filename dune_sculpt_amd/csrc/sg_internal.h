@@ -131,6 +131,13 @@ struct SgDev {
   float *place = nullptr;       // [8 per dab] placements of a ray-placed stroke
   SgRay *place_rays = nullptr;  // the stroke's rays
   int place_cap = 0;
+  // proxy combine staging (grow-only): pinned host pack, device copy, leaf table
+  float *prox_host = nullptr;   // pinned
+  float *prox_dev = nullptr;
+  size_t prox_cap = 0;          // floats in both
+  int4 *prox_tab = nullptr;
+  int *prox_leaves = nullptr;
+  int prox_tab_cap = 0;
 };
 
 enum {
@@ -206,6 +213,7 @@ struct PBVH {
   hipEvent_t timer[2] = {nullptr, nullptr};  // sg_pbvh_timer_start/stop
   bool halt_live = false;            // the work being enqueued contains learned dabs (kernels test the halt flag)
   bool draw_marks_pending = false;   // some node carries a host-side draw-buffer mark
+  std::vector<int> proxy_nodes;      // nodes given a proxy since the last combine (may repeat)
   SgUndoState undo;                  // undo steps recorded on the GPU (undo.hip)
   const float *last_place = nullptr; // placement slots of the last ray-placed stroke
   int last_place_n = 0;
