@@ -131,6 +131,7 @@ EXPORTS = [
     "sg_undo_cancel", "sg_undo_original", "sg_pbvh_combine_proxies",
 ]
 SMOOTH_LEARNED = 1 << 30
+DAB_ORIGINAL = 1 << 28  # Grab from the original coordinates (SG_DAB_ORIGINAL)
 NCTR = 24
 
 KERNEL_NAMES = ["k_search", "k_area", "k_brush", "k_smooth_iter", "k_post", "k_flush_sub", "k_flush_top"]

@@ -296,6 +296,83 @@ __global__ __launch_bounds__(256) void k_brush(SgDevView d, SgDabK k, const floa
   }
 }
 
+// Grab from the original coordinates (SG_DAB_ORIGINAL).  The reference's Grab tests and
+// displaces each vertex from its original coordinate (SculptOrigVertData, the undo
+// node's copy) into a proxy, and the proxy combine then writes original + proxy to
+// every visible uniq vertex of the node (use_orco), so vertices outside the sphere go
+// back to their original positions.  A wave per gathered leaf; the original
+// coordinates are the open undo step's record of the leaf (saved here first when this
+// dab is the step's first to touch it), or the current positions when no step is
+// open.  Every written vertex joins the affected list, so its normal is recomputed.
+__global__ __launch_bounds__(256) void k_brush_orig(SgDevView d, SgDabK k, const float *curve, int stamp,
+                                                    const float *place) {
+  if (d.halt && d.halt[0]) return;
+  sg_place(k, place);
+  const int lane = lane_id();
+  const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
+  const int nw = gridDim.x * blockDim.x / SG_WAVE;
+  const long long ng_ctr = d.ctr[SG_CTR_NGATHER];
+  const int ng = (int)ng_ctr;
+  const bool rec = (ng_ctr & SG_NGATHER_UNDO) != 0;
+  for (int g = gw; g < ng; g += nw) {
+    const int leaf = d.gathered[g];
+    const int2 vr = d.gl_v[g];
+    const float *orig = nullptr;
+    if (rec) {
+      const SgUndoCtl *u = d.undo;
+      long long vb;
+      if (u->stamp[leaf] == u->step) {  // saved by an earlier kernel of the step
+        const int4 r = u->tab[u->slot[leaf]];
+        vb = (long long)(((unsigned long long)(unsigned)r.w << 32) | (unsigned)r.z);
+      } else {
+        vb = undo_save_leaf(d, leaf, vr, d.px, d.py, d.pz);  // -1: arena full
+      }
+      if (vb >= 0) orig = u->pos + 3 * (size_t)vb;  // read back lane by lane: same lane wrote it
+    }
+    for (int base = vr.x; base < vr.y; base += SG_WAVE) {
+      const int v = base + lane;
+      const bool vis = v < vr.y && !(d.vhide && d.respect_hide && d.vhide[v]);
+      const unsigned long long m = __ballot(vis);
+      if (m == 0ull) continue;
+      int wbase = 0;
+      if (lane == 0) wbase = (int)atomicAdd((unsigned long long *)&d.ctr[SG_CTR_NAFF], (unsigned long long)__popcll(m));
+      wbase = __shfl(wbase, 0);
+      if (!vis) continue;
+      float ox, oy, oz;
+      if (orig) {
+        const size_t i = 3 * (size_t)(v - vr.x);
+        ox = orig[i];
+        oy = orig[i + 1];
+        oz = orig[i + 2];
+      } else {
+        ox = d.px[v];
+        oy = d.py[v];
+        oz = d.pz[v];
+      }
+      const float dx = k.cx - ox, dy = k.cy - oy, dz = k.cz - oz;
+      const float d2 = dx * dx + dy * dy + dz * dz;
+      float rx = ox, ry = oy, rz = oz;
+      if (d2 <= k.r2) {
+        const float f = sg_falloff(k, curve, sqrtf(d2)) * (1.0f - (d.mask ? d.mask[v] : 0.0f));
+        const float fade = k.bstr * f;
+        rx = ox + k.gx * fade;
+        ry = oy + k.gy * fade;
+        rz = oz + k.gz * fade;
+      }
+      d.px[v] = rx;
+      d.py[v] = ry;
+      d.pz[v] = rz;
+      d.qx[v] = rx;
+      d.qy[v] = ry;
+      d.qz[v] = rz;
+      const int pos = wbase + mask_prefix(m);
+      d.aff_v[pos] = v;
+      d.aff_vt[pos] = make_int2(d.vt_off[v], d.vt_off[v + 1]);
+      if (k.stats) d.stamp_a[v] = stamp;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // K4: Jacobi step it >= 2 of Smooth(k) (spec S8).  Thread per affected vertex.  Odd
 // steps read the positions and write the shadow copy, even steps the reverse; a
@@ -816,6 +893,7 @@ SgDabK sg_make_dab(const SgDabParams *dab, int stats) {
   k.gz = dab->grab_delta[2];
   k.iters = dab->smooth_iterations > 0 ? dab->smooth_iterations : 1;
   k.use_orig = (k.tool == SG_TOOL_GRAB) ? 1 : 0;
+  k.orig_co = (k.tool == SG_TOOL_GRAB && (dab->flag & SG_DAB_ORIGINAL)) ? 1 : 0;
   k.learned = (k.tool == SG_TOOL_SMOOTH && (dab->flag & SG_SMOOTH_LEARNED)) ? 1 : 0;
   k.stats = stats;
   // Spec S5 bstrength
@@ -915,7 +993,10 @@ int sg_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long long *
     ProfScope ps(p, SG_K_AREA);
     SG_LAUNCH(ps, k_area, dim3(GB), dim3(256), 0, s, v, k, place);
   }
-  {
+  if (k.orig_co) {
+    ProfScope ps(p, SG_K_BRUSH);
+    SG_LAUNCH(ps, k_brush_orig, dim3(GB), dim3(256), 0, s, v, k, curve_dev, stamp, place);
+  } else {
     ProfScope ps(p, SG_K_BRUSH);
     SG_LAUNCH(ps, k_brush, dim3(GB), dim3(256), 0, s, v, k, curve_dev, stamp, place);
   }

@@ -133,9 +133,10 @@ __device__ __forceinline__ void sg_place(SgDabK &k, const float *place) {
 __device__ __forceinline__ bool undo_active(const SgDevView &d) { return d.undo && d.undo->active; }
 
 // Caller has seen undo_active(d).  (sx, sy, sz): the position arrays that still hold
-// the leaf's positions before the dab.
-__device__ __forceinline__ void undo_save_leaf(const SgDevView &d, int leaf, int2 vr, const float *sx,
-                                               const float *sy, const float *sz) {
+// the leaf's positions before the dab.  Returns the record's first vertex in the arena
+// when this call saved the leaf, -1 when it was saved before or the arena is full.
+__device__ __forceinline__ long long undo_save_leaf(const SgDevView &d, int leaf, int2 vr, const float *sx,
+                                                    const float *sy, const float *sz) {
   SgUndoCtl *u = d.undo;
   const int lane = lane_id();
   const int n = vr.y - vr.x;
@@ -154,7 +155,7 @@ __device__ __forceinline__ void undo_save_leaf(const SgDevView &d, int leaf, int
     }
   }
   e = __shfl(e, 0);
-  if (e < 0) return;
+  if (e < 0) return -1;
   vb = ((unsigned long long)(unsigned)__shfl((int)(unsigned)(vb >> 32), 0) << 32) |
        (unsigned)__shfl((int)(unsigned)vb, 0);
   for (int i = lane; i < n; i += SG_WAVE) {
@@ -168,6 +169,7 @@ __device__ __forceinline__ void undo_save_leaf(const SgDevView &d, int leaf, int
     u->nor[o + 2] = d.nz[v];
   }
   if (lane < 16) u->bbs[16 * (size_t)e + lane] = lane < 8 ? d.leaf_bb[8 * (size_t)leaf + lane] : d.leaf_obb[8 * (size_t)leaf + lane - 8];
+  return (long long)vb;
 }
 
 // Spec S1 sphere-vs-box test: nearest point of the box, strict '<' against r*r.

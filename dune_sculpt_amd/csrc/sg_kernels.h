@@ -78,6 +78,7 @@ struct SgDabK {
   float cx, cy, cz, r, r2, nr2, bstr, hardness;
   float gx, gy, gz;
   int tool, falloff, iters, use_orig, stats, learned;
+  int orig_co;  // Grab from the original coordinates (SG_DAB_ORIGINAL)
 };
 
 // kernel ids for the per-kernel profile

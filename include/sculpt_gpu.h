@@ -202,6 +202,13 @@ enum {
 };
 #define SG_BRUSH_DIR_IN (1 << 9)    /* BRUSH_DIR_IN, types_brush_enums.h:347 */
 #define SG_SMOOTH_LEARNED (1 << 30) /* Smooth through the attached learned model */
+/* Grab from the original coordinates (single-mesh entry points): each uniq vertex of a
+ * gathered leaf is tested and displaced from its coordinate at sg_undo_begin (the
+ * current one when no undo step is open) and every one of them becomes original +
+ * displacement, zero outside the sphere -- the reference's Grab over SculptOrigVertData
+ * followed by the proxy combine from the original coordinates.  All written vertices
+ * are the dab's affected set. */
+#define SG_DAB_ORIGINAL (1 << 28)
 #define SG_CM_TABLE 256             /* CM_TABLE, types_color.h:12 */
 
 /* One dab.  Field names mirror the Brush RNA properties (api_brush.c:2281-2880) and

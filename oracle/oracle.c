@@ -1463,6 +1463,79 @@ int orc_dab(OrcPBVH *o, const OrcDab *dab, int *out_affected, int cap)
   return naff;
 }
 
+/* Grab from the original coordinates: the Grab brush task over SculptOrigVertData
+ * (sculpt.c, absent from the reference; Blender's published do_grab_brush_task_cb_ex:
+ * test and falloff on the original coordinate, proxy = grab_delta * fade) followed by
+ * the proxy combine with use_orco (every visible uniq vertex of a gathered node becomes
+ * original + proxy, zero outside the sphere).  Search on orig_vb as the Grab dab; the
+ * node and vertex marks, normals and bounds as orc_dab.  orco: totvert*3 original
+ * coordinates.  out_affected: every written vertex, ascending. */
+int orc_dab_grab_orig(OrcPBVH *o, const OrcDab *dab, const float *orco, int *out_affected, int cap)
+{
+  memset(o->stats, 0, sizeof(o->stats));
+  if (!o->nodes || o->totnode == 0 || dab->tool != ORC_TOOL_GRAB) {
+    return 0;
+  }
+  const float r = dab->radius;
+  const float r2 = r * r;
+  const float *c = dab->center;
+  const float bstr = bstrength32(dab);
+  SphereData sd;
+  memcpy(sd.center, c, sizeof(sd.center));
+  sd.radius_squared = r2;
+  sd.original = 1;
+  int *leaves;
+  int64_t ntests = 0;
+  const int nleaf = search_gather(o, sphere_cb, &sd, &leaves, &ntests);
+  o->stats[0] = nleaf;
+  o->stats[1] = ntests;
+  int naff = 0, w = 0;
+  for (int li = 0; li < nleaf; li++) {
+    ONode *n = &o->nodes[leaves[li]];
+    n->flag |= ORC_UpdateNormals | ORC_UpdateBB | ORC_UpdateOriginalBB | ORC_UpdateDrawBuffers |
+               ORC_UpdateRedraw;
+    for (int k = 0; k < n->uniq_verts; k++) {
+      const int v = n->vert_indices[k];
+      if (o->respect_hide && o->vhide && o->vhide[v]) {
+        continue; /* the vertex iterator skips hidden vertices */
+      }
+      const float *oc = orco + 3 * (size_t)v;
+      float d[3];
+      d[0] = c[0] - oc[0];
+      d[1] = c[1] - oc[1];
+      d[2] = c[2] - oc[2];
+      const float d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      float val[3] = {oc[0], oc[1], oc[2]};
+      if (d2 <= r2) {
+        const float f = falloff32(dab, sqrtf(d2)) * (1.0f - mask_of(o, v));
+        const float fade = bstr * f;
+        for (int q = 0; q < 3; q++) {
+          val[q] = oc[q] + dab->grab_delta[q] * fade;
+        }
+      }
+      memcpy(o->co + 3 * (size_t)v, val, sizeof(val));
+      if (o->co64) {
+        for (int q = 0; q < 3; q++) {
+          o->co64[3 * (size_t)v + q] = val[q];
+        }
+      }
+      o->vert_bitmap[v] = 1;
+      naff++;
+      if (out_affected && w < cap) {
+        out_affected[w++] = v;
+      }
+    }
+  }
+  o->stats[5] = naff;
+  o->stats[6] = update_normals(o, orc_get_threads());
+  update_bounds(o, orc_get_threads());
+  if (out_affected) {
+    qsort(out_affected, (size_t)w, sizeof(int), cmp_int);
+  }
+  free(leaves);
+  return naff;
+}
+
 int orc_leaves_dfs(const OrcPBVH *o, int *out, int cap)
 {
   int *arr;

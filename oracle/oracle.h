@@ -98,6 +98,10 @@ void orc_set_mask(OrcPBVH *o, const float *mask); /* per-vertex paint mask or NU
 void orc_set_positions(OrcPBVH *o, const float *co);
 /* Replace the vertex normals (e.g. to start two runs from identical state). */
 void orc_set_normals(OrcPBVH *o, const float *no);
+/* Grab from the original coordinates orco (totvert*3): tested and displaced from orco,
+ * every visible uniq vertex of a gathered leaf written (orco + displacement, zero
+ * outside the sphere) and returned as affected (ascending). */
+int orc_dab_grab_orig(OrcPBVH *o, const OrcDab *dab, const float *orco, int *out_affected, int cap);
 /* Proxy combine over the given nodes (node order): uniq vertex i of node j becomes
  * (orco or co)[v] + its proxies in order; prox packed [nprox[j]][uniq][3] node after
  * node; then normals of those vertices and the bounds update. */

@@ -75,6 +75,7 @@ def lib():
         "orc_set_normals": (None, [vp, f32p]),
         "orc_set_positions": (None, [vp, f32p]),
         "orc_combine_proxies": (None, [vp, i32p, i32, i32p, f32p, f32p]),
+        "orc_dab_grab_orig": (i32, [vp, C.POINTER(OrcDab), f32p, i32p, i32]),
         "orc_search_sphere": (i32, [vp, f32p, C.c_float, i32, i32p, i32]),
         "orc_dab": (i32, [vp, C.POINTER(OrcDab), i32p, i32]),
         "orc_falloff": (C.c_float, [C.POINTER(OrcDab), C.c_float]),
@@ -190,6 +191,14 @@ class Oracle:
     def set_positions(self, co):
         c = np.ascontiguousarray(co, np.float32)
         lib().orc_set_positions(self.h, _fp(c))
+
+    def dab_grab_orig(self, d, orco):
+        """Grab from the original coordinates orco; returns the written vertices."""
+        od = to_orc(d)
+        oc = np.ascontiguousarray(orco, np.float32)
+        out = np.empty(self.totvert + 1, np.int32)
+        n = lib().orc_dab_grab_orig(self.h, C.byref(od), _fp(oc), _ip(out), out.size)
+        return out[:n].copy()
 
     def combine_proxies(self, nodes, proxies, orco=None):
         """proxies[j]: list of (uniq_verts, 3) arrays for nodes[j] (added order)."""

@@ -342,3 +342,50 @@ def test_combine_proxies_matches_oracle(sg, po):
     assert _same(_state(p), s0)
     p.close()
     o.close()
+
+
+@pytest.mark.parametrize("in_step", [True, False])
+def test_grab_from_original_coordinates(sg, po, in_step):
+    """SG_DAB_ORIGINAL Grab dabs at an anchored centre with growing deltas: inside an
+    undo step they work from the step's original coordinates, outside one from the
+    positions before each dab; every dab's written set, gathered leaves and positions
+    equal the oracle's, and at the end the normals and every node box too."""
+    from parity_util import n_bit_diff
+
+    co, tris = sg.noisy_icosphere(5, seed=31, amplitude=0.02)
+    p = sg.PBVH(co, tris, leaf_limit=64)
+    o = po.Oracle(co, tris, leaf_limit=64, fp64=False)
+    o.set_normals(p.normals())
+    c = co[777].copy()
+    dabs = [sg.make_dab(tool=sg.TOOL_GRAB, location=c, radius=0.25, strength=0.8,
+                        grab_delta=(0.004 * k, -0.003 * k, 0.002 * k), flag=sg.DAB_ORIGINAL) for k in range(1, 9)]
+    s0 = _state(p)
+    if in_step:
+        p.undo_begin()
+    print(f"[undo] original-coordinate grab, in_step={in_step}", flush=True)
+    for i, d in enumerate(dabs):
+        orco = s0[0] if in_step else o.positions()
+        want_leaves = o.search_sphere(list(d.location), d.radius, use_orig=True)
+        want = o.dab_grab_orig(d, orco)
+        p.dab(d)
+        assert np.array_equal(p.last_gathered(), want_leaves), i
+        assert np.array_equal(p.last_affected(), want), i
+        assert n_bit_diff(p.positions(), o.positions()) == 0, i
+    assert n_bit_diff(p.normals(), o.normals()) == 0
+    bad = 0
+    for i in range(o.totnode()):
+        vb, ob = o.node_bb(i)
+        bad += n_bit_diff(p.node_bb(i), vb) != 0 or n_bit_diff(p.node_bb(i, original=True), ob) != 0
+    assert bad == 0
+    if in_step:
+        # the same dabs as one captured stroke inside a new step give the same bits
+        end = _state(p)
+        p.undo_end()
+        assert p.undo()
+        assert _same(_state(p), s0)
+        p.undo_begin()
+        p.stroke(dabs)
+        p.undo_end()
+        assert _same(_state(p), end)
+    p.close()
+    o.close()
