@@ -128,7 +128,7 @@ EXPORTS = [
     "sg_pbvh_draw_buffers_build", "sg_pbvh_draw_buffers_update", "sg_pbvh_draw_buffers_info",
     "sg_pbvh_draw_buffer_node", "sg_pbvh_draw_buffer_read", "sg_pbvh_draw_buffers_free",
     "sg_undo_begin", "sg_undo_push_node", "sg_undo_end", "sg_undo", "sg_redo", "sg_undo_info", "sg_undo_clear",
-    "sg_undo_cancel", "sg_undo_original", "sg_pbvh_combine_proxies",
+    "sg_undo_cancel", "sg_undo_original", "sg_pbvh_combine_proxies", "sg_undo_restore_original",
 ]
 SMOOTH_LEARNED = 1 << 30
 DAB_ORIGINAL = 1 << 28  # Grab from the original coordinates (SG_DAB_ORIGINAL)
@@ -274,6 +274,7 @@ def lib():
         "sg_undo_info": (i32, [vp, C.POINTER(C.c_longlong)]),
         "sg_undo_clear": (None, [vp]),
         "sg_undo_cancel": (i32, [vp]),
+        "sg_undo_restore_original": (i32, [vp]),
         "sg_undo_original": (i32, [vp, vp, vp]),
         "sg_pbvh_combine_proxies": (i32, [vp, i32]),
     }
@@ -684,6 +685,10 @@ class PBVH:
 
     def undo_cancel(self):
         _check(lib().sg_undo_cancel(self.h), "undo cancel")
+
+    def undo_restore_original(self):
+        """The open step's leaves back to their state at begin; the step stays open."""
+        _check(lib().sg_undo_restore_original(self.h), "undo restore original")
 
     def undo_original(self):
         """(positions, normals) at the begin of the open undo step (current ones when

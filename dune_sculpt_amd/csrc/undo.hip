@@ -22,8 +22,9 @@
 #include "sg_device.h"
 
 // One wave per record: swap positions (and the shadow copy), normals and the leaf box.
+// keep != 0 copies the record into the mesh and leaves the record as it is.
 __global__ __launch_bounds__(256) void k_undo_swap(SgDevView d, const int4 *tab, float *pos, float *nor, float *bbs,
-                                                   int n_e) {
+                                                   int n_e, int keep) {
   const int lane = lane_id();
   const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
   const int nw = gridDim.x * blockDim.x / SG_WAVE;
@@ -37,12 +38,14 @@ __global__ __launch_bounds__(256) void k_undo_swap(SgDevView d, const int4 *tab,
       const size_t o = 3 * (size_t)(vb + i);
       const float sx = pos[o], sy = pos[o + 1], sz = pos[o + 2];
       const float tx = nor[o], ty = nor[o + 1], tz = nor[o + 2];
-      pos[o] = d.px[v];
-      pos[o + 1] = d.py[v];
-      pos[o + 2] = d.pz[v];
-      nor[o] = d.nx[v];
-      nor[o + 1] = d.ny[v];
-      nor[o + 2] = d.nz[v];
+      if (!keep) {
+        pos[o] = d.px[v];
+        pos[o + 1] = d.py[v];
+        pos[o + 2] = d.pz[v];
+        nor[o] = d.nx[v];
+        nor[o + 1] = d.ny[v];
+        nor[o + 2] = d.nz[v];
+      }
       d.px[v] = sx;
       d.py[v] = sy;
       d.pz[v] = sz;
@@ -59,7 +62,7 @@ __global__ __launch_bounds__(256) void k_undo_swap(SgDevView d, const int4 *tab,
       // which update_node_vb turns into orig_vb = vb, pbvh.c:1221-1223)
       float *cur = d.leaf_bb + 8 * (size_t)leaf + lane;
       const float saved = bbs[16 * (size_t)e + lane];
-      bbs[16 * (size_t)e + lane] = *cur;
+      if (!keep) bbs[16 * (size_t)e + lane] = *cur;
       *cur = saved;
       d.leaf_obb[8 * (size_t)leaf + lane] = saved;
     }
@@ -151,11 +154,11 @@ static int write_ctl(PBVH *p, bool active, int step) {
   return sg_check_hip(hipStreamSynchronize(p->stream), "undo control");
 }
 
-static int swap_step(PBVH *p, SgUndoStep &s) {
+static int swap_step(PBVH *p, SgUndoStep &s, int keep = 0) {
   if (s.n_e > 0) {
     const int blocks = std::max(1, std::min((s.n_e + 3) / 4, 8192));
     hipLaunchKernelGGL(k_undo_swap, dim3(blocks), dim3(256), 0, p->stream, sg_view(p), s.tab, s.pos, s.nor, s.bbs,
-                       s.n_e);
+                       s.n_e, keep);
     if (int rc = sg_check_hip(hipGetLastError(), "undo launch")) return rc;
   }
   if (int rc = sg_enqueue_flush(p)) return rc;  // internal nodes and subtree boxes from the leaf boxes
@@ -271,6 +274,29 @@ int sg_undo_cancel(PBVH *p) {
   s.n_e = c.n_e;
   s.n_v = (long long)c.n_v;
   return swap_step(p, s);
+}
+
+// Puts the open step's saved leaves back to their state at sg_undo_begin and keeps the
+// step open with its records: what the sculpt code (absent from the reference) does
+// before each dab of an anchored stroke, restoring the original mesh from its undo
+// nodes; a later dab of the step saves nothing again.
+int sg_undo_restore_original(PBVH *p) {
+  if (!p) return sg_set_error("bad arguments");
+  SgUndoState &u = p->undo;
+  if (!u.recording) return sg_set_error("no undo step is open");
+  SgUndoCtl c;
+  int rc = sg_check_hip(hipMemcpyAsync(&c, p->d.undo, sizeof(c), hipMemcpyDeviceToHost, p->stream), "copy");
+  if (!rc) rc = sg_check_hip(hipStreamSynchronize(p->stream), "undo restore");
+  if (rc) return rc;
+  if (c.overflow) return sg_set_error("undo step overflowed its arena: original data incomplete");
+  SgUndoStep s;  // the arena itself, not owned
+  s.pos = u.rec_pos;
+  s.nor = u.rec_nor;
+  s.bbs = u.rec_bbs;
+  s.tab = u.rec_tab;
+  s.n_e = c.n_e;
+  s.n_v = (long long)c.n_v;
+  return swap_step(p, s, 1);
 }
 
 // Positions and normals at the begin of the open step (the current ones when no step

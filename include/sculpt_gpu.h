@@ -406,6 +406,10 @@ int sg_undo_cancel(PBVH *pbvh);
  * the reference reads from its undo nodes (SculptOrigVertData: Grab, anchored strokes,
  * BKE_pbvh_node_raycast's origco). */
 int sg_undo_original(PBVH *pbvh, float *co, float *no);
+/* The open step's saved leaves back to their state at begin, the step staying open
+ * (the sculpt code restores the original mesh from its undo nodes before each dab of
+ * an anchored stroke; sculpt.c is absent from the reference). */
+int sg_undo_restore_original(PBVH *pbvh);
 
 /* ---- proxies combined on the GPU -------------------------------------------------
  * Folds the proxies added with BKE_pbvh_node_add_proxy (host buffers, one float[3] per

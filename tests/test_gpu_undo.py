@@ -389,3 +389,32 @@ def test_grab_from_original_coordinates(sg, po, in_step):
         assert _same(_state(p), end)
     p.close()
     o.close()
+
+
+def test_restore_original_for_anchored_strokes(sg):
+    """Inside a step, restoring the original mesh gives back the begin state bit for
+    bit and keeps the step open; a dab after it equals the same dab on a fresh mesh,
+    and undoing the step returns to the begin state."""
+    co, tris = sg.noisy_icosphere(5, seed=32, amplitude=0.02)
+    p = sg.PBVH(co, tris, leaf_limit=64)
+    s0 = _state(p)
+    c = co[2024].copy()
+    grow = [sg.make_dab(tool=sg.TOOL_DRAW, location=c, radius=0.05 * k, strength=0.7) for k in range(1, 5)]
+    p.undo_begin()
+    print("[undo] anchored stroke", flush=True)
+    for d in grow[:-1]:
+        p.dab(d)
+        p.undo_restore_original()
+        assert _same(_state(p), s0)
+    p.dab(grow[-1])
+    p.undo_end()
+    q = sg.PBVH(co, tris, leaf_limit=64)
+    q.dab(grow[-1])
+    assert _same(_state(p), _state(q))
+    assert p.undo_info()["steps"] == 1
+    assert p.undo()
+    assert _same(_state(p), s0)
+    with pytest.raises(RuntimeError, match="no undo step is open"):
+        p.undo_restore_original()
+    q.close()
+    p.close()
