@@ -457,12 +457,16 @@ __global__ __launch_bounds__(256) void k_post(SgDevView d, int stats, int stamp,
   const int nthreads = gridDim.x * blockDim.x;
   const int gw = tid / SG_WAVE;
   const int nw = nthreads / SG_WAVE;
+  // The normals take the affected records from the first thread on, the leaf refits
+  // the gathered leaves from the last wave back, so the two independent chains run in
+  // different waves side by side instead of one after the other in the first waves.
+  const int rw = nw - 1 - gw;
   // speculative first-record loads alongside the counters
   const int v0 = tid < d.V ? d.aff_v[tid] : 0;
   const int2 vt0 = tid < d.V ? d.aff_vt[tid] : make_int2(0, 0);
-  const int leaf0 = gw < d.L ? d.gathered[gw] : 0;
-  const int2 vr0 = gw < d.L ? d.gl_v[gw] : make_int2(0, 0);
-  const int2 sr0 = gw < d.L ? d.gl_s[gw] : make_int2(0, 0);
+  const int leaf0 = rw < d.L ? d.gathered[rw] : 0;
+  const int2 vr0 = rw < d.L ? d.gl_v[rw] : make_int2(0, 0);
+  const int2 sr0 = rw < d.L ? d.gl_s[rw] : make_int2(0, 0);
   const int na = (int)d.ctr[SG_CTR_NAFF];
   const int ng = (int)d.ctr[SG_CTR_NGATHER];
   long long ntri = 0, nfa = 0, nvfa = 0, nsh = 0;
@@ -476,8 +480,8 @@ __global__ __launch_bounds__(256) void k_post(SgDevView d, int stats, int stamp,
     }
     vertex_normal(d, v, vt.x, vt.y, stats, stamp, ntri, nfa, nvfa);
   }
-  for (int g = gw; g < ng; g += nw) {
-    if (g == gw)
+  for (int g = rw; g < ng; g += nw) {
+    if (g == rw)
       leaf_refit(d, leaf0, vr0, sr0, nsh);
     else
       leaf_refit(d, d.gathered[g], d.gl_v[g], d.gl_s[g], nsh);

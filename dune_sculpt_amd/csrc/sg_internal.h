@@ -54,6 +54,7 @@ struct SgUndoState {
   int4 *rec_tab = nullptr;
   long long rec_cap_v = 0;
   int rec_cap_e = 0;
+  long long limit_v = 0;        // the open step's max_verts
 };
 
 struct SgDev {

@@ -141,7 +141,7 @@ static int write_ctl(PBVH *p, bool active, int step) {
   memset(&c, 0, sizeof(c));
   c.active = active ? 1 : 0;
   c.step = step;
-  c.cap_v = (unsigned long long)u.rec_cap_v;
+  c.cap_v = (unsigned long long)std::min(u.rec_cap_v, u.limit_v);  // the open step's limit
   c.cap_e = u.rec_cap_e;
   c.stamp = p->d.undo_stamp;
   c.slot = p->d.undo_slot;
@@ -187,9 +187,10 @@ int sg_undo_begin(PBVH *p, long long max_verts) {
       free_arena(u);
       return -1;
     }
-    u.rec_cap_v = max_verts;
+    u.rec_cap_v = max_verts;  // grow-only: a larger arena serves smaller limits
     u.rec_cap_e = cap_e;
   }
+  u.limit_v = max_verts;
   u.step_id++;
   if (int rc = write_ctl(p, true, u.step_id)) return rc;
   u.recording = true;
@@ -218,7 +219,7 @@ int sg_undo_end(PBVH *p) {
   if (!rc) rc = write_ctl(p, false, u.step_id);
   u.recording = false;
   if (rc) return rc;
-  if (c.overflow) return sg_set_error("undo step overflowed its arena (%lld vertices): step dropped", u.rec_cap_v);
+  if (c.overflow) return sg_set_error("undo step overflowed its arena (%lld vertices): step dropped", u.limit_v);
   SgUndoStep s;
   s.n_e = c.n_e;
   s.n_v = (long long)c.n_v;

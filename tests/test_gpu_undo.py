@@ -155,6 +155,11 @@ def test_undo_overflow_drops_step(sg):
     assert p.undo()
     assert _same(_state(p), s1)
     assert not _same(s1, s0)
+    # the limit holds for the step that asks for it, whatever arena earlier steps left
+    p.undo_begin(max_verts=50)
+    p.stroke(_stroke_dabs(sg, 7, n=8))
+    with pytest.raises(RuntimeError, match="overflow"):
+        p.undo_end()
     p.close()
 
 
