@@ -62,14 +62,17 @@ __global__ __launch_bounds__(256) void k_search(SgDevView d, SgDabK k, long long
   const int lane = lane_id();
   const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
   const int nw = gridDim.x * blockDim.x / SG_WAVE;
-  if (gw == 0 && lane < SG_CTR_COUNT) {
+  // bookkeeping in the last wave, which has the fewest or no subtrees to test, so its
+  // loads stay off the searching waves' chains
+  const int bw = nw - 1;
+  if (gw == bw && lane < SG_CTR_COUNT) {
     const long long prev = d.ctr_next[lane];  // the set the previous dab used
     if (log && log_index > 0)
       log[(size_t)(log_index - 1) * SG_CTR_COUNT + lane] = lane == SG_CTR_NGATHER ? (prev & 0xffffffffll) : prev;
     d.ctr_next[lane] = 0;
   }
-  if (gw == 0 && lane == 0 && d.seq) d.seq[0] += 1;  // this dab's stamp for the later kernels
-  if (gw == 0 && lane == 0 && undo_active(d))
+  if (gw == bw && lane == 0 && d.seq) d.seq[0] += 1;  // this dab's stamp for the later kernels
+  if (gw == bw && lane == 0 && undo_active(d))
     atomicAdd((unsigned long long *)&d.ctr[SG_CTR_NGATHER], SG_NGATHER_UNDO);
   const float *cut_bbs = k.use_orig ? d.cut_obb : d.cut_bb;
   const float *leaf_bbs = k.use_orig ? d.leaf_obb : d.leaf_bb;
@@ -197,11 +200,27 @@ __global__ __launch_bounds__(256) void k_area(SgDevView d, SgDabK k, const float
   sy = wave_sum_ll(sy);
   sz = wave_sum_ll(sz);
   na = wave_sum_i(na);
-  if (lane == 0 && na) {
-    atomicAdd((unsigned long long *)&d.ctr[SG_CTR_AREA_X], as_u64(sx));
-    atomicAdd((unsigned long long *)&d.ctr[SG_CTR_AREA_Y], as_u64(sy));
-    atomicAdd((unsigned long long *)&d.ctr[SG_CTR_AREA_Z], as_u64(sz));
-    atomicAdd((unsigned long long *)&d.ctr[SG_CTR_NAREA], (unsigned long long)na);
+  // one set of atomics per workgroup (the four sums of its waves added in LDS first):
+  // every wave's atomics go to the same four counters
+  __shared__ long long s_part[4][4];
+  const int w = threadIdx.x / SG_WAVE;
+  if (lane == 0) {
+    s_part[w][0] = sx;
+    s_part[w][1] = sy;
+    s_part[w][2] = sz;
+    s_part[w][3] = na;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long t[4] = {0, 0, 0, 0};
+    for (int i = 0; i < (int)(blockDim.x / SG_WAVE); i++)
+      for (int q = 0; q < 4; q++) t[q] += s_part[i][q];
+    if (t[3]) {
+      atomicAdd((unsigned long long *)&d.ctr[SG_CTR_AREA_X], as_u64(t[0]));
+      atomicAdd((unsigned long long *)&d.ctr[SG_CTR_AREA_Y], as_u64(t[1]));
+      atomicAdd((unsigned long long *)&d.ctr[SG_CTR_AREA_Z], as_u64(t[2]));
+      atomicAdd((unsigned long long *)&d.ctr[SG_CTR_NAREA], (unsigned long long)t[3]);
+    }
   }
 }
 
