@@ -877,6 +877,7 @@ SgDevView sg_view_of(SgDev &d, bool respect_hide, bool halt_live) {
   v.gathered = d.gathered;
   v.draw_dirty = d.draw_dirty;
   v.undo = d.undo;
+  v.place_pair = d.place_pair;
   v.gl_v = d.gl_v;
   v.gl_s = d.gl_s;
   v.aff_v = d.aff_v;

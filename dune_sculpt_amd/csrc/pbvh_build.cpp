@@ -671,6 +671,7 @@ void sg_build_device(PBVH *p, const float *co) {
   alloc_fill(&d.undo_stamp, L, 0xFF, s);  // -1: saved by no undo step
   alloc_fill(&d.undo_slot, L, 0, s);
   alloc_fill(&d.undo, 1, 0, s);          // undo recording off
+  alloc_fill(&d.place_pair, 16, 0, s);   // count 0: no ray of a Grab placement has arrived
   alloc_fill(&d.cut_touch, C, 0x80, s);  // INT_MIN-ish: never "touched by the previous dab"
   alloc_fill(&d.prev_cuts, C, 0, s);
 
@@ -711,7 +712,7 @@ void sg_free_device(PBVH *p) {
                   d.stamp_a, d.curve, d.log, d.ltri, d.leaf_toff, d.tri_poly, d.node_parent, d.leaf_node,
                   d.ray_in, d.ray_out, d.place, d.place_rays, d.rl_list, d.rl_args,
                   d.draw_dirty, d.draw_tri, d.draw_off, d.draw_leaf, d.draw_fset, d.draw_poly_off,
-                  d.draw_poly_v, d.draw_vbo, d.undo, d.undo_stamp, d.undo_slot,
+                  d.draw_poly_v, d.draw_vbo, d.undo, d.undo_stamp, d.undo_slot, d.place_pair,
                   d.prox_dev, d.prox_tab, d.prox_leaves};
   for (void *q : ptrs) {
     if (q) hipFree(q);

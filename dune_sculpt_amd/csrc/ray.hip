@@ -485,19 +485,54 @@ __attribute__((amdgpu_waves_per_eu(4, 8))) __global__ __launch_bounds__(NT) void
                                                 int stale, SgRayHit *out, float *place, int grab_pair,
                                                 long long *stats) {
   __shared__ RayShared<NT> S;
+  if (place && grab_pair) {
+    // A Grab placement's two rays run in two workgroups side by side.  Each leaves its
+    // hit point, hit flag and status in d.place_pair with agent-scope atomics and counts
+    // itself in; the second to arrive writes the placement and resets the count.  No
+    // block waits for the other.
+    if (blockIdx.x > 1) return;
+    const int b = blockIdx.x;
+    ray_one(d, rv, rays + b, original, stale, S, 0, stats, true);
+    __shared__ int s_last;
+    if (threadIdx.x == 0) {
+      const SgRayHit &h = S.res[0];
+      int *pp = d.place_pair + 1 + 5 * b;
+      for (int k = 0; k < 3; k++)
+        __hip_atomic_store(pp + k, __float_as_int(h.location[k]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(pp + 3, h.hit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(pp + 4, h.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s_last = __hip_atomic_fetch_add(d.place_pair, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == 1;
+    }
+    __syncthreads();
+    if (s_last && threadIdx.x == 0) {
+      float loc[2][3];
+      int hit[2], status[2];
+      for (int q = 0; q < 2; q++) {
+        const int *pp = d.place_pair + 1 + 5 * q;
+        for (int k = 0; k < 3; k++)
+          loc[q][k] = __int_as_float(__hip_atomic_load(pp + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        hit[q] = __hip_atomic_load(pp + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        status[q] = __hip_atomic_load(pp + 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      float g[3] = {0.0f, 0.0f, 0.0f};
+      if (hit[0] && hit[1]) {
+        for (int k = 0; k < 3; k++) g[k] = loc[1][k] - loc[0][k];
+      }
+      const bool ok = hit[0] && status[0] == 0;
+      *(float4 *)place = make_float4(loc[0][0], loc[0][1], loc[0][2], g[0]);
+      *(float4 *)(place + 4) = make_float4(g[1], g[2], ok ? 1.0f : 0.0f, 0.0f);
+      __hip_atomic_store(d.place_pair, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // for the next dab
+    }
+    return;
+  }
   if (place) {
     if (blockIdx.x != 0) return;
     ray_one(d, rv, rays, original, stale, S, 0, stats, true);
-    if (grab_pair) ray_one(d, rv, rays + 1, original, stale, S, 1, stats, true);
     if (threadIdx.x == 0) {
       const SgRayHit &a = S.res[0];
-      float g[3] = {0.0f, 0.0f, 0.0f};
-      if (grab_pair && a.hit && S.res[1].hit) {
-        for (int k = 0; k < 3; k++) g[k] = S.res[1].location[k] - a.location[k];
-      }
       const bool hit = a.hit && a.status == 0;
-      *(float4 *)place = make_float4(a.location[0], a.location[1], a.location[2], g[0]);
-      *(float4 *)(place + 4) = make_float4(g[1], g[2], hit ? 1.0f : 0.0f, 0.0f);
+      *(float4 *)place = make_float4(a.location[0], a.location[1], a.location[2], 0.0f);
+      *(float4 *)(place + 4) = make_float4(0.0f, 0.0f, hit ? 1.0f : 0.0f, 0.0f);
     }
     return;
   }
@@ -908,9 +943,9 @@ int sg_enqueue_raycast(PBVH *p, const SgRay *rays_dev, int n, int original, int 
   if (int rc = sg_ray_ensure(p)) return rc;
   const SgDevView v = sg_view(p);
   const SgRayView rv = ray_view(p);
-  if (place_dev)
-    hipLaunchKernelGGL(k_raycast<1024>, dim3(1), dim3(1024), 0, p->stream, v, rv, rays_dev, n, original, stale,
-                       out_dev, place_dev, grab_pair, stats_dev);
+  if (place_dev)  // one workgroup per ray of the placement
+    hipLaunchKernelGGL(k_raycast<1024>, dim3(grab_pair ? 2 : 1), dim3(1024), 0, p->stream, v, rv, rays_dev, n,
+                       original, stale, out_dev, place_dev, grab_pair, stats_dev);
   else
     hipLaunchKernelGGL(k_raycast<256>, dim3(std::max(1, std::min(n, 65536))), dim3(256), 0, p->stream, v, rv,
                        rays_dev, n, original, stale, out_dev, place_dev, grab_pair, stats_dev);

@@ -130,6 +130,7 @@ struct SgDev {
   int *undo_stamp = nullptr;    // [L]
   int *undo_slot = nullptr;     // [L]
   float *place = nullptr;       // [8 per dab] placements of a ray-placed stroke
+  int *place_pair = nullptr;    // [16] a Grab placement's two rays meet here (count, then results)
   SgRay *place_rays = nullptr;  // the stroke's rays
   int place_cap = 0;
   // proxy combine staging (grow-only): pinned host pack, device copy, leaf table

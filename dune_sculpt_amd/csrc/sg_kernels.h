@@ -59,6 +59,7 @@ struct SgDevView {
   int *gathered;
   int *draw_dirty;
   SgUndoCtl *undo;
+  int *place_pair;  // handoff of a Grab placement's two rays (ray.hip)
   int2 *gl_v, *gl_s;
   int *aff_v;
   int2 *aff_vt, *aff_vv;
