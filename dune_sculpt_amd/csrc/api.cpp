@@ -1376,7 +1376,8 @@ const float (*DUNE_pbvh_get_vert_normals(const PBVH *cp))[3] {
 
 PBVHProxyNode *BKE_pbvh_node_add_proxy(PBVH *p, PBVHNode *node) {
   const int index = node->proxy_count;
-  if (index == 0) p->proxy_nodes.push_back((int)(node - p->nodes.data()));  // the combine's worklist
+  if (index == 0 && node >= p->nodes.data() && node < p->nodes.data() + p->nodes.size())
+    p->proxy_nodes.push_back((int)(node - p->nodes.data()));  // the combine's worklist
   node->proxy_count++;
   node->proxies = (PBVHProxyNode *)realloc(node->proxies, node->proxy_count * sizeof(PBVHProxyNode));
   int totverts = 0;

@@ -198,7 +198,9 @@ int sg_undo_begin(PBVH *p, long long max_verts) {
 }
 
 int sg_undo_push_node(PBVH *p, PBVHNode *node) {
-  if (!p || !node || node->leaf < 0) return sg_set_error("not a leaf");
+  if (!p || !node || node < p->nodes.data() || node >= p->nodes.data() + p->nodes.size())
+    return sg_set_error("not a node of this PBVH");
+  if (node->leaf < 0 || node->leaf >= p->d.L) return sg_set_error("not a leaf");
   if (!p->undo.recording) return sg_set_error("no undo step is open");
   int2 vr;
   if (int rc = sg_check_hip(hipMemcpy(&vr, p->d.leaf_vbeg + node->leaf, sizeof(int2), hipMemcpyDeviceToHost), "copy"))

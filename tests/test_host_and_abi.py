@@ -125,3 +125,30 @@ def test_stroke_spacing(sg):
     step = np.linalg.norm(np.diff(np.float64(c), axis=0), axis=1)
     assert np.allclose(step, 2 * np.sin(0.01 / 2), rtol=1e-5)
     assert np.allclose(np.linalg.norm(c, axis=1), 1.0, atol=1e-6)
+
+
+def test_host_proxies_worklist(sg, po):
+    """BKE_pbvh_node_add_proxy / _gather_proxies / _free_proxies on the host tree: zeroed
+    buffers of one float[3] per uniq vertex, gathered in node order whatever order they
+    were added in (the combine's worklist), none after a free."""
+    co, tris = sg.noisy_icosphere(3, seed=2, amplitude=0.02)
+    p = _host_tree(sg, co, tris, 16)
+    o = po.Oracle(co, tris, leaf_limit=16)
+    leaves = [int(x) for x in o.leaves_dfs()]
+    picks = [leaves[5], leaves[1], leaves[9], leaves[1]]  # leaf 1 twice: two proxies
+    for i in picks:
+        a = p.add_proxy(i)
+        assert a.shape == (o.node_info(i)[4], 3) and not a.any()
+        a[:] = 1.0
+    L = sg.lib()
+    arr = sg.C.POINTER(sg.C.c_void_p)()
+    tot = sg.C.c_int()
+    L.BKE_pbvh_gather_proxies(p.h, sg.C.byref(arr), sg.C.byref(tot))
+    got = [L.sg_pbvh_node_index(p.h, arr[k]) for k in range(tot.value)]
+    sg.C.CDLL(None).free(arr)
+    assert got == sorted(set(picks))
+    for i in set(picks):
+        L.BKE_pbvh_node_free_proxies(p.node(i))
+    L.BKE_pbvh_gather_proxies(p.h, sg.C.byref(arr), sg.C.byref(tot))
+    assert tot.value == 0
+    p.close()
