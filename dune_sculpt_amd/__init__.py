@@ -18,7 +18,8 @@ import numpy as np
 _PKG = Path(__file__).resolve().parent
 LIB_PATH = _PKG / "libsculpt_gpu.so"
 
-TOOL_DRAW, TOOL_SMOOTH, TOOL_INFLATE, TOOL_GRAB = 1, 2, 4, 5
+TOOL_DRAW, TOOL_SMOOTH, TOOL_INFLATE, TOOL_GRAB, TOOL_ELASTIC_DEFORM = 1, 2, 4, 5, 21
+ELASTIC_GRAB, ELASTIC_GRAB_BISCALE, ELASTIC_GRAB_TRISCALE, ELASTIC_SCALE, ELASTIC_TWIST = range(5)
 CURVE_CUSTOM, CURVE_SMOOTH, CURVE_SPHERE, CURVE_ROOT, CURVE_SHARP = 0, 1, 2, 3, 4
 CURVE_LIN, CURVE_POW4, CURVE_INVSQUARE, CURVE_CONSTANT, CURVE_SMOOTHER = 5, 6, 7, 8, 9
 BRUSH_DIR_IN = 1 << 9
@@ -34,7 +35,8 @@ PBVH_FullyHidden = 1 << 9
 ME_HIDE = 1 << 4
 ME_SMOOTH = 1 << 0
 
-TOOL_NAMES = {TOOL_DRAW: "draw", TOOL_SMOOTH: "smooth", TOOL_INFLATE: "inflate", TOOL_GRAB: "grab"}
+TOOL_NAMES = {TOOL_DRAW: "draw", TOOL_SMOOTH: "smooth", TOOL_INFLATE: "inflate", TOOL_GRAB: "grab",
+              TOOL_ELASTIC_DEFORM: "elastic_deform"}
 
 
 class MVert(C.Structure):  # types_meshdata.h:13-17
@@ -67,6 +69,9 @@ class SgDabParams(C.Structure):
         ("normal_radius_factor", C.c_float),
         ("grab_delta", C.c_float * 3),
         ("curve_table", C.POINTER(C.c_float)),
+        ("elastic_deform_type", C.c_int),
+        ("elastic_deform_volume_preservation", C.c_float),
+        ("surface_normal", C.c_float * 3),
     ]
 
 
@@ -380,8 +385,12 @@ def great_circle_centres(seed: int, radii, spacing_frac: float = 0.1, sphere_rad
 
 
 def make_dab(tool=TOOL_DRAW, location=(0, 0, 1), radius=0.1, strength=0.5, falloff=CURVE_SMOOTH, hardness=0.0,
-             smooth_iterations=1, grab_delta=(0, 0, 0), flag=0, normal_radius_factor=0.5, curve_table=None):
+             smooth_iterations=1, grab_delta=(0, 0, 0), flag=0, normal_radius_factor=0.5, curve_table=None,
+             elastic_type=0, volume_preservation=0.4, surface_normal=(0, 0, 1)):
     d = SgDabParams()
+    d.elastic_deform_type = elastic_type
+    d.elastic_deform_volume_preservation = volume_preservation
+    d.surface_normal[:] = [float(x) for x in surface_normal]
     d.sculpt_tool = tool
     d.curve_preset = falloff
     d.flag = flag

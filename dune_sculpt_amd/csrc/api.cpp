@@ -337,6 +337,8 @@ void sg_dab_params_default(SgDabParams *d, int tool) {
   d->strength = 0.5f;
   d->hardness = 0.0f;
   d->normal_radius_factor = 0.5f;
+  d->elastic_deform_volume_preservation = 0.4f;
+  d->surface_normal[2] = 1.0f;
 }
 
 static int check_dab(PBVH *p, const SgDabParams *dab) {
@@ -344,8 +346,14 @@ static int check_dab(PBVH *p, const SgDabParams *dab) {
   if (p->no_device) return sg_set_error("no HIP device");
   if (p->nodes.empty() || !p->build_ok) return sg_set_error("PBVH has no usable mesh");
   const int t = dab->sculpt_tool;
-  if (t != SG_TOOL_DRAW && t != SG_TOOL_SMOOTH && t != SG_TOOL_INFLATE && t != SG_TOOL_GRAB)
+  if (t != SG_TOOL_DRAW && t != SG_TOOL_SMOOTH && t != SG_TOOL_INFLATE && t != SG_TOOL_GRAB &&
+      t != SG_TOOL_ELASTIC_DEFORM)
     return sg_set_error("unsupported sculpt_tool %d", t);
+  if (t == SG_TOOL_ELASTIC_DEFORM &&
+      (dab->elastic_deform_type < SG_ELASTIC_GRAB || dab->elastic_deform_type > SG_ELASTIC_TWIST))
+    return sg_set_error("bad elastic_deform_type");
+  if (t == SG_TOOL_ELASTIC_DEFORM && (dab->flag & (SG_SMOOTH_LEARNED | SG_DAB_ORIGINAL)))
+    return sg_set_error("flags not valid for elastic deform");
   if (!(dab->radius > 0.0f)) return sg_set_error("radius must be > 0");
   if (dab->curve_preset < 0 || dab->curve_preset > SG_CURVE_SMOOTHER) return sg_set_error("bad curve_preset");
   if (t == SG_TOOL_SMOOTH && dab->smooth_iterations > 64) return sg_set_error("smooth_iterations > 64");
@@ -1085,7 +1093,19 @@ int sg_dab_last_affected(PBVH *p, int *out, int cap) {
   if (p->no_device) return sg_set_error("no HIP device");
   int na = (int)p->last_ctr[SG_CTR_NAFF];
   std::vector<int> tmp(std::max(na, 1));
-  if (p->last_learned) {
+  if (p->last_tool == SG_TOOL_ELASTIC_DEFORM) {
+    // an elastic deform writes every visible uniq vertex (it keeps no list)
+    tmp.clear();
+    for (const PBVHNode &n : p->nodes) {
+      if (n.leaf < 0) continue;
+      for (int k = 0; k < n.uniq_verts; k++) {
+        const int v = n.vert_indices[k];
+        if (!(p->respect_hide && !p->vhide.empty() && p->vhide[v])) tmp.push_back(p->perm_new_of_old[v]);
+      }
+    }
+    na = (int)tmp.size();
+    tmp.resize(std::max(na, 1));
+  } else if (p->last_learned) {
     // an accepted learned dab stamps its vertices instead of listing them: the
     // stamped vertices of the gathered leaves' ranges
     const int ng = (int)p->last_ctr[SG_CTR_NGATHER];

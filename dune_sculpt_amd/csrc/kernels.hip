@@ -98,7 +98,7 @@ __global__ __launch_bounds__(256) void k_search(SgDevView d, SgDabK k, long long
       cb.mx[2] = __shfl(cb.mx[2], 0);
     }
     ntests++;
-    if ((cfl & 1) || !sphere_box(cb, k.cx, k.cy, k.cz, k.r2)) continue;
+    if ((cfl & k.hide_skip) || !sphere_box(cb, k.cx, k.cy, k.cz, k.r2)) continue;
     const int nl = le - lb;
     ntests += nl;
     for (int q0 = 0; q0 < nl; q0 += SG_WAVE * SEARCH_UNROLL) {
@@ -110,7 +110,7 @@ __global__ __launch_bounds__(256) void k_search(SgDevView d, SgDabK k, long long
         if (l < le) {
           const BB6 b = bb_load(leaf_bbs + 8 * (size_t)l);
           const int fl = d.leaf_flags[l];
-          pass[j] = !(fl & 1) && sphere_box(b, k.cx, k.cy, k.cz, k.r2);
+          pass[j] = !(fl & k.hide_skip) && sphere_box(b, k.cx, k.cy, k.cz, k.r2);
         }
       }
       unsigned long long m[SEARCH_UNROLL];
@@ -389,6 +389,197 @@ __global__ __launch_bounds__(256) void k_brush_orig(SgDevView d, SgDabK k, const
       d.aff_vt[pos] = make_int2(d.vt_off[v], d.vt_off[v + 1]);
       if (k.stats) d.stamp_a[v] = stamp;
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Elastic deform (SCULPT_TOOL_ELASTIC_DEFORM).  The Kelvinlet displacements restate
+// kernel/intern/kelvinlet.c (BKE_kelvinlet_init_params :6-21, init_kelvinlet_grab
+// :23-45, grab / bi-scale / tri-scale :47-109, the integration :114-138, scale :141-155
+// and twist :167-182) operation for operation.  The brush task is absent from the
+// reference; it follows Blender's published one: every node, each visible uniq vertex
+// displaced from its original coordinate, the grab modes scaled by bstrength * 20, the
+// result scaled by 1 - mask and combined as original + displacement.
+// ---------------------------------------------------------------------------
+struct Kelvinlet {
+  float a, b, c, f, rs[3];
+};
+
+__device__ __forceinline__ Kelvinlet kelvinlet_params(float radius, float force, float shear_modulus,
+                                                     float poisson_ratio) {
+  Kelvinlet p;
+  p.a = 1.0f / (4.0f * (float)M_PI * shear_modulus);
+  p.b = p.a / (4.0f * (1.0f - poisson_ratio));
+  p.c = 2.0f * (3.0f * p.a - 2.0f * p.b);
+  p.f = force;
+  p.rs[0] = radius * 1.0f;
+  p.rs[1] = p.rs[0] * 2.0f;
+  p.rs[2] = p.rs[1] * 2.0f;
+  return p;
+}
+
+__device__ __forceinline__ float kl_p2(float x) { return x * x; }
+__device__ __forceinline__ float kl_p3(float x) { return kl_p2(x) * x; }
+__device__ __forceinline__ float kl_p5(float x) { return kl_p2(kl_p2(x)) * x; }
+
+// Formula (6) for the first n radii
+__device__ __forceinline__ void kelvinlet_grab_terms(const Kelvinlet &p, float radius, int n, float kv[3]) {
+  for (int i = 0; i < n; i++) {
+    const float re = sqrtf(kl_p2(radius) + kl_p2(p.rs[i]));
+    kv[i] = ((p.a - p.b) / re) + ((p.b * kl_p2(radius)) / kl_p3(re)) + ((p.a * kl_p2(p.rs[i])) / (2.0f * kl_p3(re)));
+  }
+}
+
+// grab, bi-scale or tri-scale: the displacement factor along brush_delta
+__device__ __forceinline__ float kelvinlet_grab_fade(const Kelvinlet &p, int mode, float radius) {
+  float kv[3];
+  if (mode == SG_ELASTIC_GRAB) {
+    kelvinlet_grab_terms(p, radius, 1, kv);
+    return kv[0] * p.c;
+  }
+  if (mode == SG_ELASTIC_GRAB_BISCALE) {
+    kelvinlet_grab_terms(p, radius, 2, kv);
+    const float u = kv[0] - kv[1];
+    return u * p.c / ((1.0f / p.rs[0]) - (1.0f / p.rs[1]));
+  }
+  kelvinlet_grab_terms(p, radius, 3, kv);
+  const float w0 = 1.0f;
+  const float w1 = -((kl_p2(p.rs[2]) - kl_p2(p.rs[0])) / (kl_p2(p.rs[2]) - kl_p2(p.rs[1])));
+  const float w2 = ((kl_p2(p.rs[1]) - kl_p2(p.rs[0])) / (kl_p2(p.rs[2]) - kl_p2(p.rs[1])));
+  const float u = w0 * kv[0] + w1 * kv[1] + w2 * kv[2];
+  return u * p.c / (w0 / p.rs[0] + w1 / p.rs[1] + w2 / p.rs[2]);
+}
+
+// Formula (16) (scale) or (15) (twist) at vertex co
+__device__ __forceinline__ void kelvinlet_scale_twist(const Kelvinlet &p, bool twist, const float co[3],
+                                                      const float loc[3], const float nor[3], float disp[3]) {
+  const float rv[3] = {co[0] - loc[0], co[1] - loc[1], co[2] - loc[2]};
+  const float radius = sqrtf(rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2]);
+  const float re = sqrtf(kl_p2(radius) + kl_p2(p.rs[0]));
+  const float lead = twist ? -p.a : (2.0f * p.b - p.a);
+  const float u = lead * ((1.0f / kl_p3(re))) + ((3.0f * kl_p2(p.rs[0])) / (2.0f * kl_p5(re)));
+  const float fade = u * p.c;
+  const float s = fade * p.f;
+  if (twist) {
+    const float q[3] = {nor[1] * rv[2] - nor[2] * rv[1], nor[2] * rv[0] - nor[0] * rv[2],
+                        nor[0] * rv[1] - nor[1] * rv[0]};
+    disp[0] = q[0] * s;
+    disp[1] = q[1] * s;
+    disp[2] = q[2] * s;
+  } else {
+    disp[0] = rv[0] * s;
+    disp[1] = rv[1] * s;
+    disp[2] = rv[2] * s;
+  }
+}
+
+// the four-stage integration of scale / twist (the fourth stage offsets by -location
+// as the reference does)
+__device__ __forceinline__ void kelvinlet_integrate(const Kelvinlet &p, bool twist, const float co[3],
+                                                    const float loc[3], const float nor[3], float r[3]) {
+  float k0[3], k1[3], k2[3], k3[3], it[3];
+  kelvinlet_scale_twist(p, twist, co, loc, nor, k0);
+  for (int i = 0; i < 3; i++) it[i] = co[i] + k0[i] * 0.5f;
+  kelvinlet_scale_twist(p, twist, it, loc, nor, k1);
+  for (int i = 0; i < 3; i++) it[i] = co[i] + k1[i] * 0.5f;
+  kelvinlet_scale_twist(p, twist, it, loc, nor, k2);
+  for (int i = 0; i < 3; i++) it[i] = (co[i] + k2[i]) - loc[i];
+  kelvinlet_scale_twist(p, twist, it, loc, nor, k3);
+  const float sixth = 1.0f / 6.0f;
+  for (int i = 0; i < 3; i++) {
+    float v = k0[i];
+    v += k1[i] * 2.0f;
+    v += k2[i] * 2.0f;
+    v += k3[i];
+    r[i] = v * sixth;
+  }
+}
+
+// A wave per leaf over every leaf.  Original coordinates as k_brush_orig: the open undo
+// step's record (saved here first), else the current positions.
+__global__ __launch_bounds__(256) void k_elastic(SgDevView d, SgDabK k, const float *place) {
+  if (d.halt && d.halt[0]) return;
+  sg_place(k, place);
+  const int lane = lane_id();
+  const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
+  const int nw = gridDim.x * blockDim.x / SG_WAVE;
+  const bool rec = undo_active(d);
+  const Kelvinlet prm = kelvinlet_params(k.r, k.eforce, 1.0f, k.epois);
+  const float loc[3] = {k.cx, k.cy, k.cz}, nor[3] = {k.enx, k.eny, k.enz};
+  const float delta[3] = {k.gx, k.gy, k.gz};
+  const bool grab = k.etype <= SG_ELASTIC_GRAB_TRISCALE;
+  const float gscale = k.bstr * 20.0f;
+  long long nvis = 0;
+  for (int leaf = gw; leaf < d.L; leaf += nw) {
+    const int2 vr = make_int2(d.leaf_vbeg[leaf], d.leaf_vbeg[leaf + 1]);
+    const float *orig = nullptr;
+    if (rec) {
+      const SgUndoCtl *u = d.undo;
+      long long vb;
+      if (u->stamp[leaf] == u->step) {
+        const int4 r = u->tab[u->slot[leaf]];
+        vb = (long long)(((unsigned long long)(unsigned)r.w << 32) | (unsigned)r.z);
+      } else {
+        vb = undo_save_leaf(d, leaf, vr, d.px, d.py, d.pz);
+      }
+      if (vb >= 0) orig = u->pos + 3 * (size_t)vb;  // same lane wrote each record it reads
+    }
+    for (int v = vr.x + lane; v < vr.y; v += SG_WAVE) {
+      if (d.vhide && d.respect_hide && d.vhide[v]) continue;
+      nvis++;
+      float co[3];
+      if (orig) {
+        const size_t i = 3 * (size_t)(v - vr.x);
+        co[0] = orig[i];
+        co[1] = orig[i + 1];
+        co[2] = orig[i + 2];
+      } else {
+        co[0] = d.px[v];
+        co[1] = d.py[v];
+        co[2] = d.pz[v];
+      }
+      float disp[3];
+      if (grab) {
+        const float e[3] = {co[0] - loc[0], co[1] - loc[1], co[2] - loc[2]};
+        const float radius = sqrtf(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+        const float fade = kelvinlet_grab_fade(prm, k.etype, radius);
+        for (int i = 0; i < 3; i++) disp[i] = delta[i] * fade;
+        for (int i = 0; i < 3; i++) disp[i] = disp[i] * gscale;
+      } else {
+        kelvinlet_integrate(prm, k.etype == SG_ELASTIC_TWIST, co, loc, nor, disp);
+      }
+      if (d.mask) {
+        const float keep = 1.0f - d.mask[v];
+        for (int i = 0; i < 3; i++) disp[i] = disp[i] * keep;
+      }
+      const float rx = co[0] + disp[0], ry = co[1] + disp[1], rz = co[2] + disp[2];
+      d.px[v] = rx;
+      d.py[v] = ry;
+      d.pz[v] = rz;
+      d.qx[v] = rx;
+      d.qy[v] = ry;
+      d.qz[v] = rz;
+    }
+  }
+  // written vertices: one atomic per workgroup
+  nvis = wave_sum_ll(nvis);
+  __shared__ long long s_n[4];
+  if (lane == 0) s_n[threadIdx.x / SG_WAVE] = nvis;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long t = 0;
+    for (int i = 0; i < (int)(blockDim.x / SG_WAVE); i++) t += s_n[i];
+    if (t) atomicAdd((unsigned long long *)&d.ctr[SG_CTR_NAFF], (unsigned long long)t);
+  }
+}
+
+// Normals of every visible uniq vertex (the combine flags them all), spec S10.
+__global__ __launch_bounds__(256) void k_elastic_normals(SgDevView d) {
+  const int nu = d.leaf_vbeg[d.L];
+  long long a = 0, b = 0, c = 0;
+  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nu; v += gridDim.x * blockDim.x) {
+    if (d.vhide && d.respect_hide && d.vhide[v]) continue;
+    vertex_normal(d, v, d.vt_off[v], d.vt_off[v + 1], 0, 0, a, b, c);
   }
 }
 
@@ -917,6 +1108,13 @@ SgDabK sg_make_dab(const SgDabParams *dab, int stats) {
   k.gz = dab->grab_delta[2];
   k.iters = dab->smooth_iterations > 0 ? dab->smooth_iterations : 1;
   k.use_orig = (k.tool == SG_TOOL_GRAB) ? 1 : 0;
+  k.hide_skip = 1;
+  k.etype = dab->elastic_deform_type;
+  k.epois = dab->elastic_deform_volume_preservation;
+  k.enx = dab->surface_normal[0];
+  k.eny = dab->surface_normal[1];
+  k.enz = dab->surface_normal[2];
+  k.eforce = 0.0f;
   k.orig_co = (k.tool == SG_TOOL_GRAB && (dab->flag & SG_DAB_ORIGINAL)) ? 1 : 0;
   k.learned = (k.tool == SG_TOOL_SMOOTH && (dab->flag & SG_SMOOTH_LEARNED)) ? 1 : 0;
   k.stats = stats;
@@ -929,7 +1127,18 @@ SgDabK sg_make_dab(const SgDabParams *dab, int stats) {
     case SG_TOOL_INFLATE: k.bstr = 0.25f * alpha * flip; break;
     case SG_TOOL_SMOOTH: k.bstr = s < 0.0f ? 0.0f : (s > 1.0f ? 1.0f : s); break;
     case SG_TOOL_GRAB: k.bstr = s; break;
+    case SG_TOOL_ELASTIC_DEFORM: k.bstr = s; break;
     default: k.bstr = 0.0f; break;
+  }
+  if (k.tool == SG_TOOL_ELASTIC_DEFORM) {
+    // Every leaf (the brush needs all nodes: a Kelvinlet has no finite support), the
+    // fully hidden ones too (the node search without a callback returns them).  The
+    // force: |grab_delta| * direction * bstrength, the direction -1 with DIR_IN (the
+    // sign the stroke's mouse motion gives).
+    k.r2 = __builtin_inff();
+    k.hide_skip = 0;
+    const float gl = sqrtf(k.gx * k.gx + k.gy * k.gy + k.gz * k.gz);
+    k.eforce = gl * flip * k.bstr;
   }
   return k;
 }
@@ -1001,6 +1210,32 @@ int sg_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long long *
     ProfScope ps(p, SG_K_SEARCH);
     const int sg = (p->d.C + 3) / 4;
     SG_LAUNCH(ps, k_search, dim3(sg > 0 ? sg : 1), dim3(256), 0, s, v, k, log, log_index, serial, place);
+  }
+  if (k.tool == SG_TOOL_ELASTIC_DEFORM) {  // every leaf; then all normals, leaf boxes and the tree
+    {
+      ProfScope ps(p, SG_K_BRUSH);
+      SG_LAUNCH(ps, k_elastic, dim3(GB), dim3(256), 0, s, v, k, place);
+    }
+    {
+      ProfScope ps(p, SG_K_POST);
+      const int nb = std::max(1, std::min((p->d.V + 255) / 256, 8192));
+      SG_LAUNCH(ps, k_elastic_normals, dim3(nb), dim3(256), 0, s, v);
+    }
+    {
+      ProfScope ps(p, SG_K_POST);
+      SG_LAUNCH(ps, k_refit_all, dim3(GP), dim3(256), 0, s, v);
+    }
+    {
+      ProfScope ps(p, SG_K_FLUSH_SUB);
+      const size_t sub_lds = (size_t)p->d.sub_maxn * 6 * sizeof(float);
+      SG_LAUNCH(ps, k_flush_sub, dim3(p->d.C), dim3(FLUSH_THREADS), sub_lds, s, v, p->d.sub_maxn);
+    }
+    {
+      ProfScope ps(p, SG_K_FLUSH_TOP);
+      const size_t top_lds = (size_t)(p->d.C + p->d.top_n) * 6 * sizeof(float);
+      SG_LAUNCH(ps, k_flush_top, dim3(1), dim3(1024), top_lds, s, v, p->d.top_n, 1);
+    }
+    return sg_check_hip(hipGetLastError(), "dab launch");
   }
   if (k.learned && p->mlp) {  // learned Smooth: brush + MLP, then decide / commit
     {

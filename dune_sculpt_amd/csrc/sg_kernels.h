@@ -80,6 +80,9 @@ struct SgDabK {
   float gx, gy, gz;
   int tool, falloff, iters, use_orig, stats, learned;
   int orig_co;  // Grab from the original coordinates (SG_DAB_ORIGINAL)
+  int hide_skip;  // 1: the search skips fully hidden subtrees and leaves; 0: every leaf (elastic deform)
+  int etype;      // elastic deform mode
+  float epois, enx, eny, enz, eforce;  // Poisson ratio, sculpt normal, force (Kelvinlet parameters)
 };
 
 // kernel ids for the per-kernel profile

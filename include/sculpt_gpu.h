@@ -194,7 +194,12 @@ void sg_isect_ray_tri_precalc(IsectRayPrecalc *out, const float ray_direction[3]
 /* ---- 2. brush engine entry points (new; replace the absent do_brush_action) ---- */
 
 /* Tools (types_brush_enums.h:410-415) and falloff presets (:176-187). */
-enum { SG_TOOL_DRAW = 1, SG_TOOL_SMOOTH = 2, SG_TOOL_INFLATE = 4, SG_TOOL_GRAB = 5 };
+enum { SG_TOOL_DRAW = 1, SG_TOOL_SMOOTH = 2, SG_TOOL_INFLATE = 4, SG_TOOL_GRAB = 5, SG_TOOL_ELASTIC_DEFORM = 21 };
+/* Elastic deform modes (eBrushElasticDeformType, types_brush_enums.h:195-199). */
+enum {
+  SG_ELASTIC_GRAB = 0, SG_ELASTIC_GRAB_BISCALE = 1, SG_ELASTIC_GRAB_TRISCALE = 2, SG_ELASTIC_SCALE = 3,
+  SG_ELASTIC_TWIST = 4,
+};
 enum {
   SG_CURVE_CUSTOM = 0, SG_CURVE_SMOOTH = 1, SG_CURVE_SPHERE = 2, SG_CURVE_ROOT = 3,
   SG_CURVE_SHARP = 4, SG_CURVE_LIN = 5, SG_CURVE_POW4 = 6, SG_CURVE_INVSQUARE = 7,
@@ -225,6 +230,12 @@ typedef struct SgDabParams {
   float normal_radius_factor; /* area-normal sphere for Draw (:2714 area), default 0.5 */
   float grab_delta[3];        /* Grab translation for this dab */
   const float *curve_table;   /* SG_CM_TABLE+1 samples over [0,1] for SG_CURVE_CUSTOM */
+  /* Elastic deform (single-mesh entry points): mode (elastic_deform_type, :2375),
+   * Poisson ratio (elastic_deform_volume_preservation, :2722, range 0..0.9) and the
+   * stroke's sculpt normal, which Scale and Twist use. */
+  int elastic_deform_type;
+  float elastic_deform_volume_preservation;
+  float surface_normal[3];
 } SgDabParams;
 
 /* Fills defaults (types_brush_defaults.h:8-90: SMOOTH falloff, hardness 0,

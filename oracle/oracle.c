@@ -993,9 +993,11 @@ static float bstrength32(const OrcDab *dab)
     case ORC_TOOL_INFLATE: return 0.25f * alpha * flip;
     case ORC_TOOL_SMOOTH: return s < 0.0f ? 0.0f : (s > 1.0f ? 1.0f : s);
     case ORC_TOOL_GRAB: return s;
+    case ORC_TOOL_ELASTIC: return s;
   }
   return 0.0f;
 }
+float orc_bstrength(const OrcDab *dab) { return bstrength32(dab); }
 
 /* Exact order-independent sum for the Draw area normal (spec S6): each FP32
  * component is rounded to a multiple of 2^-32 and summed in int64, so any
@@ -1605,4 +1607,12 @@ void orc_combine_proxies(OrcPBVH *o, const int *nodes, int n, const int *nprox, 
   }
   update_normals(o, orc_get_threads());
   update_bounds(o, orc_get_threads());
+}
+
+/* The normal and bounds updates a dab ends with, for the other oracle files. */
+int64_t orc_update_normals_bounds(OrcPBVH *o)
+{
+  const int64_t t = update_normals(o, orc_get_threads());
+  update_bounds(o, orc_get_threads());
+  return t;
 }

@@ -44,7 +44,7 @@ enum {
 };
 
 /* Tools (types_brush_enums.h:410-415) and falloff presets (:176-187). */
-enum { ORC_TOOL_DRAW = 1, ORC_TOOL_SMOOTH = 2, ORC_TOOL_INFLATE = 4, ORC_TOOL_GRAB = 5 };
+enum { ORC_TOOL_DRAW = 1, ORC_TOOL_SMOOTH = 2, ORC_TOOL_INFLATE = 4, ORC_TOOL_GRAB = 5, ORC_TOOL_ELASTIC = 21 };
 enum {
   ORC_CURVE_CUSTOM = 0, ORC_CURVE_SMOOTH = 1, ORC_CURVE_SPHERE = 2, ORC_CURVE_ROOT = 3,
   ORC_CURVE_SHARP = 4, ORC_CURVE_LIN = 5, ORC_CURVE_POW4 = 6, ORC_CURVE_INVSQUARE = 7,
@@ -65,6 +65,9 @@ typedef struct OrcDab {
   float normal_radius_factor;
   float grab_delta[3];
   const float *curve_table; /* ORC_CM_TABLE+1 samples of y over x in [0,1] (CUSTOM) */
+  int elastic_type;         /* eBrushElasticDeformType */
+  float poisson_ratio;      /* elastic_deform_volume_preservation */
+  float surface_normal[3];  /* the stroke's sculpt normal (Scale, Twist) */
 } OrcDab;
 
 typedef struct OrcPBVH OrcPBVH;
@@ -102,6 +105,10 @@ void orc_set_normals(OrcPBVH *o, const float *no);
  * every visible uniq vertex of a gathered leaf written (orco + displacement, zero
  * outside the sphere) and returned as affected (ascending). */
 int orc_dab_grab_orig(OrcPBVH *o, const OrcDab *dab, const float *orco, int *out_affected, int cap);
+/* Elastic deform (tool 21) from the original coordinates orco (totvert*3): every leaf,
+ * each visible uniq vertex = orco + Kelvinlet displacement; written vertices returned
+ * (ascending). */
+int orc_dab_elastic(OrcPBVH *o, const OrcDab *dab, const float *orco, int *out_affected, int cap);
 /* Proxy combine over the given nodes (node order): uniq vertex i of node j becomes
  * (orco or co)[v] + its proxies in order; prox packed [nprox[j]][uniq][3] node after
  * node; then normals of those vertices and the bounds update. */

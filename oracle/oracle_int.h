@@ -44,3 +44,5 @@ struct OrcPBVH {
   int *map_local, *map_touched;
   int64_t stats[8];
 };
+
+int64_t orc_update_normals_bounds(OrcPBVH *o);

@@ -23,6 +23,7 @@ class OrcDab(C.Structure):
         ("tool", C.c_int), ("falloff", C.c_int), ("flag", C.c_int), ("smooth_iters", C.c_int),
         ("center", C.c_float * 3), ("radius", C.c_float), ("strength", C.c_float), ("hardness", C.c_float),
         ("normal_radius_factor", C.c_float), ("grab_delta", C.c_float * 3), ("curve_table", C.POINTER(C.c_float)),
+        ("elastic_type", C.c_int), ("poisson_ratio", C.c_float), ("surface_normal", C.c_float * 3),
     ]
 
 
@@ -76,6 +77,8 @@ def lib():
         "orc_set_positions": (None, [vp, f32p]),
         "orc_combine_proxies": (None, [vp, i32p, i32, i32p, f32p, f32p]),
         "orc_dab_grab_orig": (i32, [vp, C.POINTER(OrcDab), f32p, i32p, i32]),
+        "orc_dab_elastic": (i32, [vp, C.POINTER(OrcDab), f32p, i32p, i32]),
+        "orc_kelvinlet": (None, [C.POINTER(OrcDab), f32p, f32p, f32p]),
         "orc_search_sphere": (i32, [vp, f32p, C.c_float, i32, i32p, i32]),
         "orc_dab": (i32, [vp, C.POINTER(OrcDab), i32p, i32]),
         "orc_falloff": (C.c_float, [C.POINTER(OrcDab), C.c_float]),
@@ -127,6 +130,9 @@ def to_orc(d) -> OrcDab:
     o.grab_delta[:] = list(d.grab_delta)
     if d.curve_table:
         o.curve_table = d.curve_table
+    o.elastic_type = getattr(d, "elastic_deform_type", 0)
+    o.poisson_ratio = getattr(d, "elastic_deform_volume_preservation", 0.0)
+    o.surface_normal[:] = list(getattr(d, "surface_normal", (0.0, 0.0, 1.0)))
     return o
 
 
@@ -198,6 +204,14 @@ class Oracle:
         oc = np.ascontiguousarray(orco, np.float32)
         out = np.empty(self.totvert + 1, np.int32)
         n = lib().orc_dab_grab_orig(self.h, C.byref(od), _fp(oc), _ip(out), out.size)
+        return out[:n].copy()
+
+    def dab_elastic(self, d, orco):
+        """Elastic deform from the original coordinates orco; returns the written vertices."""
+        od = to_orc(d)
+        oc = np.ascontiguousarray(orco, np.float32)
+        out = np.empty(self.totvert + 1, np.int32)
+        n = lib().orc_dab_elastic(self.h, C.byref(od), _fp(oc), _ip(out), out.size)
         return out[:n].copy()
 
     def combine_proxies(self, nodes, proxies, orco=None):
@@ -364,6 +378,17 @@ def dist_sq_ray_tri(start, direction, v0, v1, v2):
     d = lib().orc_dist_sq_ray_tri(_fp(_f3(start)), _fp(_f3(direction)), _fp(_f3(v0)), _fp(_f3(v1)),
                                   _fp(_f3(v2)), _fp(dep))
     return float(d), float(dep[0])
+
+
+def kelvinlet(d, co, bstrength):
+    """Displacement of one original coordinate co by the elastic deform dab d (before the
+    mask), with the dab's bstrength given."""
+    od = to_orc(d)
+    c = np.ascontiguousarray(co, np.float32)
+    b = np.array([bstrength], np.float32)
+    out = np.zeros(3, np.float32)
+    lib().orc_kelvinlet(C.byref(od), _fp(c), _fp(b), _fp(out))
+    return out
 
 
 def face_set_color(fs, seed):
