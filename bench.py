@@ -545,6 +545,48 @@ def run_proxies(sg, A, reps):
     return res
 
 
+def run_elastic(sg, A, W, K):
+    """Elastic deform dabs on the whole mesh (every leaf; each visible vertex moved by a
+    Kelvinlet of the dab), one per mode, from the current coordinates.  k_elastic is
+    timed by the engine's per-kernel events; its algorithmic bytes: 36 per vertex (read
+    the position, write it and its shadow copy) and 8 per leaf (its vertex range)."""
+    A.restore()
+    info = A.tree_info()
+    nbytes = 36 * A.totvert + 8 * info["leaves"]
+    res = {"vertices": A.totvert, "alg_bytes_per_launch": nbytes, "modes": {}}
+    names = ["grab", "grab_biscale", "grab_triscale", "scale", "twist"]
+    co = A.positions()
+    for mode in range(5):
+        d = sg.make_dab(tool=sg.TOOL_ELASTIC_DEFORM, location=co[12345], radius=0.1, strength=0.5,
+                        grab_delta=(0.001, 0.0, 0.0), elastic_type=mode, volume_preservation=0.4,
+                        surface_normal=(0.0, 0.0, 1.0))
+        for _ in range(W):
+            A.dab(d)
+        A.set_profile(True)
+        A.profile_read(reset=True)
+        walls = []
+        for _ in range(K):
+            A.sync()
+            A.timer_start()
+            A.dab(d)
+            walls.append(A.timer_stop())
+        prof = A.profile_read(reset=True)
+        A.set_profile(False)
+        kms, kn = prof["k_brush"]
+        us = kms / max(kn, 1) * 1e3
+        gbps = nbytes / (us * 1e-6) / 1e9
+        res["modes"][names[mode]] = {"dab_ms": float(np.mean(walls)), "k_elastic_us": round(us, 2),
+                                     "gbps": round(gbps, 1)}
+    best = max(res["modes"].values(), key=lambda m: m["gbps"])
+    res["roofline"] = {"bound": "hbm", "kernel": "k_elastic (grab)", "achieved": res["modes"]["grab"]["gbps"],
+                       "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                       "frac": round(res["modes"]["grab"]["gbps"] / HBM_PEAK_GBPS, 4),
+                       "avg_launch_us": res["modes"]["grab"]["k_elastic_us"], "best_mode_gbps": best["gbps"]}
+    log(f"elastic: {res}")
+    A.restore()
+    return res
+
+
 def cpu_raycast(o, starts, dirs, budget_s):
     """The oracle's sequential ray cast (the reference's traversal and callback) on the
     same camera rays, one host thread, until the budget is spent."""
@@ -642,6 +684,8 @@ def main():
     ap.add_argument("--no-draw", action="store_true", help="skip the draw-buffer measurement")
     ap.add_argument("--no-undo", action="store_true", help="skip the undo-step measurement")
     ap.add_argument("--only-undo", action="store_true", help="run only the undo-step measurement")
+    ap.add_argument("--no-elastic", action="store_true", help="skip the elastic deform measurement")
+    ap.add_argument("--only-elastic", action="store_true", help="run only the elastic deform measurement")
     ap.add_argument("--placed-only", action="store_true",
                     help="placed stroke without the fixed-centre reference stroke (one captured graph)")
     args = ap.parse_args()
@@ -683,6 +727,10 @@ def main():
         if not args.no_draw:
             out["draw_buffers"] = run_draw(sg, A, dabs, W, max(K, 3))
         print(json.dumps(out))
+        A.close()
+        return
+    if args.only_elastic:
+        print(json.dumps({"elastic_deform": run_elastic(sg, A, W, max(K, 3))}))
         A.close()
         return
     if args.only_undo:
@@ -790,6 +838,7 @@ def main():
 
     undo = None if args.no_undo else run_undo(sg, A, step, float(np.mean(event_ms)), args.dabs, W, K)
     proxies = None if args.no_undo else run_proxies(sg, A, max(K, 2))
+    elastic = None if args.no_elastic else run_elastic(sg, A, W, max(K, 3))
 
     copy = None
     if args.copy_mb:
@@ -876,6 +925,7 @@ def main():
         "draw_buffers": draw,
         "undo": undo,
         "proxies": proxies,
+        "elastic_deform": elastic,
 
         "build_s": build_s,
         "autotune": tune,

@@ -497,7 +497,7 @@ __device__ __forceinline__ void kelvinlet_integrate(const Kelvinlet &p, bool twi
 
 // A wave per leaf over every leaf.  Original coordinates as k_brush_orig: the open undo
 // step's record (saved here first), else the current positions.
-__global__ __launch_bounds__(256) void k_elastic(SgDevView d, SgDabK k, const float *place) {
+__global__ __launch_bounds__(256) void k_elastic(SgDevView d, SgDabK k, const float *place, long long nwritten) {
   if (d.halt && d.halt[0]) return;
   sg_place(k, place);
   const int lane = lane_id();
@@ -509,7 +509,8 @@ __global__ __launch_bounds__(256) void k_elastic(SgDevView d, SgDabK k, const fl
   const float delta[3] = {k.gx, k.gy, k.gz};
   const bool grab = k.etype <= SG_ELASTIC_GRAB_TRISCALE;
   const float gscale = k.bstr * 20.0f;
-  long long nvis = 0;
+  // the written count is known on the host (every visible uniq vertex)
+  if (blockIdx.x == 0 && threadIdx.x == 0) d.ctr[SG_CTR_NAFF] = nwritten;
   for (int leaf = gw; leaf < d.L; leaf += nw) {
     const int2 vr = make_int2(d.leaf_vbeg[leaf], d.leaf_vbeg[leaf + 1]);
     const float *orig = nullptr;
@@ -526,7 +527,6 @@ __global__ __launch_bounds__(256) void k_elastic(SgDevView d, SgDabK k, const fl
     }
     for (int v = vr.x + lane; v < vr.y; v += SG_WAVE) {
       if (d.vhide && d.respect_hide && d.vhide[v]) continue;
-      nvis++;
       float co[3];
       if (orig) {
         const size_t i = 3 * (size_t)(v - vr.x);
@@ -560,16 +560,6 @@ __global__ __launch_bounds__(256) void k_elastic(SgDevView d, SgDabK k, const fl
       d.qy[v] = ry;
       d.qz[v] = rz;
     }
-  }
-  // written vertices: one atomic per workgroup
-  nvis = wave_sum_ll(nvis);
-  __shared__ long long s_n[4];
-  if (lane == 0) s_n[threadIdx.x / SG_WAVE] = nvis;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    long long t = 0;
-    for (int i = 0; i < (int)(blockDim.x / SG_WAVE); i++) t += s_n[i];
-    if (t) atomicAdd((unsigned long long *)&d.ctr[SG_CTR_NAFF], (unsigned long long)t);
   }
 }
 
@@ -1212,18 +1202,22 @@ int sg_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long long *
     SG_LAUNCH(ps, k_search, dim3(sg > 0 ? sg : 1), dim3(256), 0, s, v, k, log, log_index, serial, place);
   }
   if (k.tool == SG_TOOL_ELASTIC_DEFORM) {  // every leaf; then all normals, leaf boxes and the tree
+    // whole-mesh passes: a wave per leaf, a thread per vertex (enough waves in flight
+    // to cover the memory latency, rather than the per-dab kernels' fixed grid)
+    const int GL = std::max(1, (p->d.L + 3) / 4);
     {
       ProfScope ps(p, SG_K_BRUSH);
-      SG_LAUNCH(ps, k_elastic, dim3(GB), dim3(256), 0, s, v, k, place);
+      const long long nw = p->respect_hide ? p->d.nuniq_visible : p->d.nuniq;
+      SG_LAUNCH(ps, k_elastic, dim3(GL), dim3(256), 0, s, v, k, place, nw);
     }
     {
       ProfScope ps(p, SG_K_POST);
-      const int nb = std::max(1, std::min((p->d.V + 255) / 256, 8192));
+      const int nb = std::max(1, (p->d.V + 255) / 256);
       SG_LAUNCH(ps, k_elastic_normals, dim3(nb), dim3(256), 0, s, v);
     }
     {
       ProfScope ps(p, SG_K_POST);
-      SG_LAUNCH(ps, k_refit_all, dim3(GP), dim3(256), 0, s, v);
+      SG_LAUNCH(ps, k_refit_all, dim3(GL), dim3(256), 0, s, v);
     }
     {
       ProfScope ps(p, SG_K_FLUSH_SUB);
@@ -1337,7 +1331,7 @@ int sg_enqueue_coords_apply(PBVH *p, const float *src_dev) {
   const SgDevView v = sg_view(p);
   const int blocks = (p->d.V + 255) / 256;
   hipLaunchKernelGGL(k_coords_apply, dim3(blocks > 0 ? blocks : 1), dim3(256), 0, p->stream, v, src_dev);
-  hipLaunchKernelGGL(k_refit_all, dim3(p->grid_wg), dim3(256), 0, p->stream, v);
+  hipLaunchKernelGGL(k_refit_all, dim3(std::max(1, (p->d.L + 3) / 4)), dim3(256), 0, p->stream, v);  // a wave per leaf
   hipLaunchKernelGGL(k_cut_refresh, dim3(p->grid_wg), dim3(256), 0, p->stream, v);
   return sg_check_hip(hipGetLastError(), "coords launch");
 }

@@ -675,9 +675,11 @@ void sg_build_device(PBVH *p, const float *co) {
   alloc_fill(&d.cut_touch, C, 0x80, s);  // INT_MIN-ish: never "touched by the previous dab"
   alloc_fill(&d.prev_cuts, C, 0, s);
 
+  d.nuniq = d.nuniq_visible = h.leaf_vbeg[L];
   if (!p->vhide.empty()) {
     std::vector<int> vh(V);
     for (int i = 0; i < V; i++) vh[i] = p->vhide[p->perm_old_of_new[i]];
+    for (int i = 0; i < h.leaf_vbeg[L]; i++) d.nuniq_visible -= vh[i] ? 1 : 0;
     upload(&d.vhide, vh.data(), V, s);
   }
   alloc_fill(&d.vmark, V, 0, s);
