@@ -929,10 +929,11 @@ int sg_learned_judge(PBVH *p, const SgDabParams *dab, double out[4]) {
   pd.flag &= ~SG_SMOOTH_LEARNED;
   if (int rc = sg_dab_apply(p, &pd)) return rc;
   const int na = (int)p->last_ctr[SG_CTR_NAFF];
+  const int nslot = (int)p->last_ctr[SG_CTR_NGATHER] << p->d.aff_shift;  // the list's slots
   double acc[2] = {0.0, 0.0};
   if (na > 0) {
     // snapshot slot 1 holds the positions before the dab (items 0..2)
-    int rc = sg_enqueue_judge(p, na, p->judge_buf, (const float *)p->snap[1][0], (const float *)p->snap[1][1],
+    int rc = sg_enqueue_judge(p, nslot, p->judge_buf, (const float *)p->snap[1][0], (const float *)p->snap[1][1],
                               (const float *)p->snap[1][2], p->judge_acc);
     if (!rc) rc = sg_check_hip(hipMemcpyAsync(acc, p->judge_acc, 16, hipMemcpyDeviceToHost, p->stream), "copy");
     if (!rc) rc = sg_check_hip(hipStreamSynchronize(p->stream), "judge");
@@ -988,7 +989,8 @@ int sg_learned_collect(PBVH *p, const SgDabParams *dab, float *X, float *T, floa
   if (!rc) rc = sg_check_hip(hipStreamSynchronize(p->stream), "collect");
   int na = 0;
   if (!rc) {
-    na = (int)std::min<long long>(ctr[SG_CTR_NAFF], cap);
+    // one row per slot of the affected list (rows without a record are marked not valid)
+    na = (int)std::min<long long>((long long)(int)ctr[SG_CTR_NGATHER] << p->d.aff_shift, cap);
     std::vector<float> fr((size_t)std::max(na, 1) * 16);
     rc = sg_check_hip(hipMemcpy(X, col.X, (size_t)na * 28 * 4, hipMemcpyDeviceToHost), "copy");
     if (!rc) rc = sg_check_hip(hipMemcpy(T, col.T, (size_t)na * 3 * 4, hipMemcpyDeviceToHost), "copy");
@@ -1124,9 +1126,20 @@ int sg_dab_last_affected(PBVH *p, int *out, int cap) {
         if (stamp[v] == seq) tmp.push_back(v);
     na = (int)tmp.size();
     tmp.resize(std::max(na, 1));
-  } else if (na > 0 &&
-             sg_check_hip(hipMemcpy(tmp.data(), p->d.aff_v, na * sizeof(int), hipMemcpyDeviceToHost), "copy")) {
-    return -1;
+  } else {
+    // the list: gathered leaf g's records in slots (g << aff_shift) + [0, aff_cnt[g])
+    const int ng = (int)p->last_ctr[SG_CTR_NGATHER];
+    const size_t nslot = (size_t)ng << p->d.aff_shift;
+    std::vector<int> cnt(std::max(ng, 1)), lst(std::max<size_t>(nslot, 1));
+    if (ng > 0 &&
+        (sg_check_hip(hipMemcpy(cnt.data(), p->d.aff_cnt, ng * sizeof(int), hipMemcpyDeviceToHost), "copy") ||
+         sg_check_hip(hipMemcpy(lst.data(), p->d.aff_v, nslot * sizeof(int), hipMemcpyDeviceToHost), "copy")))
+      return -1;
+    tmp.clear();
+    for (int g = 0; g < ng; g++)
+      for (int i = 0; i < cnt[g]; i++) tmp.push_back(lst[((size_t)g << p->d.aff_shift) + i]);
+    na = (int)tmp.size();
+    tmp.resize(std::max(na, 1));
   }
   for (int i = 0; i < na; i++) tmp[i] = p->perm_old_of_new[tmp[i]];
   std::sort(tmp.begin(), tmp.begin() + na);

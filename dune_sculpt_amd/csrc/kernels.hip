@@ -266,6 +266,8 @@ __global__ __launch_bounds__(256) void k_brush(SgDevView d, SgDabK k, const floa
   for (int g = gw; g < ng; g += nw) {
     const int2 vr = (g == gw) ? vr0 : d.gl_v[g];
     ncand += vr.y - vr.x;
+    // the leaf's affected records go to its own slots: no reservation to wait for
+    int used = 0;
     for (int base = vr.x; base < vr.y; base += 2 * SG_WAVE) {
       const int v0 = base + lane, v1 = base + SG_WAVE + lane;
       const BrushIn in0 = brush_load(d, k, v0, vr.y, smooth);
@@ -274,8 +276,8 @@ __global__ __launch_bounds__(256) void k_brush(SgDevView d, SgDabK k, const floa
       const unsigned long long m1 = __ballot(in1.aff);
       const int c0 = __popcll(m0), cnt = c0 + __popcll(m1);
       if (cnt == 0) continue;
-      int wbase = 0;
-      if (lane == 0) wbase = (int)atomicAdd((unsigned long long *)&d.ctr[SG_CTR_NAFF], (unsigned long long)cnt);
+      const int wbase = (g << d.aff_shift) + used;
+      used += cnt;
       float fade0 = 0.0f, fade1 = 0.0f;
       if (in0.aff) fade0 = brush_apply(d, k, curve, v0, in0, ox, oy, oz, nbr);
       if (in1.aff) fade1 = brush_apply(d, k, curve, v1, in1, ox, oy, oz, nbr);
@@ -283,7 +285,6 @@ __global__ __launch_bounds__(256) void k_brush(SgDevView d, SgDabK k, const floa
         if (in0.aff) d.stamp_a[v0] = stamp;
         if (in1.aff) d.stamp_a[v1] = stamp;
       }
-      wbase = __shfl(wbase, 0);
       if (in0.aff) {
         const int pos = wbase + mask_prefix(m0);
         d.aff_v[pos] = v0;
@@ -303,6 +304,7 @@ __global__ __launch_bounds__(256) void k_brush(SgDevView d, SgDabK k, const floa
         }
       }
     }
+    if (lane == 0) d.aff_cnt[g] = used;
   }
   if (k.stats) {
     ncand = (lane == 0) ? ncand : 0;
@@ -348,14 +350,14 @@ __global__ __launch_bounds__(256) void k_brush_orig(SgDevView d, SgDabK k, const
       }
       if (vb >= 0) orig = u->pos + 3 * (size_t)vb;  // read back lane by lane: same lane wrote it
     }
+    int used = 0;
     for (int base = vr.x; base < vr.y; base += SG_WAVE) {
       const int v = base + lane;
       const bool vis = v < vr.y && !(d.vhide && d.respect_hide && d.vhide[v]);
       const unsigned long long m = __ballot(vis);
       if (m == 0ull) continue;
-      int wbase = 0;
-      if (lane == 0) wbase = (int)atomicAdd((unsigned long long *)&d.ctr[SG_CTR_NAFF], (unsigned long long)__popcll(m));
-      wbase = __shfl(wbase, 0);
+      const int wbase = (g << d.aff_shift) + used;
+      used += __popcll(m);
       if (!vis) continue;
       float ox, oy, oz;
       if (orig) {
@@ -389,6 +391,7 @@ __global__ __launch_bounds__(256) void k_brush_orig(SgDevView d, SgDabK k, const
       d.aff_vt[pos] = make_int2(d.vt_off[v], d.vt_off[v + 1]);
       if (k.stats) d.stamp_a[v] = stamp;
     }
+    if (lane == 0) d.aff_cnt[g] = used;
   }
 }
 
@@ -590,10 +593,11 @@ __global__ __launch_bounds__(256) void k_smooth_iter(SgDevView d, int from_shado
   float *dx_ = from_shadow ? d.px : d.qx;
   float *dy_ = from_shadow ? d.py : d.qy;
   float *dz_ = from_shadow ? d.pz : d.qz;
-  const int na = (int)d.ctr[SG_CTR_NAFF];
+  const int na = (int)d.ctr[SG_CTR_NGATHER] << d.aff_shift;  // slots of the gathered leaves
   long long nbr = 0, r1 = 0;
   for (int i = i0; i < na; i += nthreads) {
-    const int v = d.aff_v[i];
+    const int v = aff_slot_vertex(d, i, d.aff_cnt[i >> d.aff_shift]);
+    if (v < 0) continue;
     const int2 vv = d.aff_vv[i];
     const float fade = d.aff_fade[i];
     const float x = sx_[v], y = sy_[v], z = sz_[v];
@@ -632,9 +636,10 @@ __global__ __launch_bounds__(256) void k_smooth_iter(SgDevView d, int from_shado
 // both arrays agree again.
 __global__ __launch_bounds__(256) void k_smooth_sync(SgDevView d, int from_shadow) {
   if (d.halt && d.halt[0]) return;
-  const int na = (int)d.ctr[SG_CTR_NAFF];
+  const int na = (int)d.ctr[SG_CTR_NGATHER] << d.aff_shift;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
-    const int v = d.aff_v[i];
+    const int v = aff_slot_vertex(d, i, d.aff_cnt[i >> d.aff_shift]);
+    if (v < 0) continue;
     if (from_shadow) {
       d.px[v] = d.qx[v];
       d.py[v] = d.qy[v];
@@ -662,16 +667,28 @@ __global__ __launch_bounds__(256) void k_post(SgDevView d, int stats, int stamp,
   // different waves side by side instead of one after the other in the first waves.
   const int rw = nw - 1 - gw;
   // speculative first-record loads alongside the counters
-  const int v0 = tid < d.V ? d.aff_v[tid] : 0;
-  const int2 vt0 = tid < d.V ? d.aff_vt[tid] : make_int2(0, 0);
+  const int nslot = d.L << d.aff_shift;
+  const int v0 = tid < nslot ? d.aff_v[tid] : 0;
+  const int2 vt0 = tid < nslot ? d.aff_vt[tid] : make_int2(0, 0);
+  const int c0 = (tid >> d.aff_shift) < d.L ? d.aff_cnt[tid >> d.aff_shift] : 0;
   const int leaf0 = rw < d.L ? d.gathered[rw] : 0;
   const int2 vr0 = rw < d.L ? d.gl_v[rw] : make_int2(0, 0);
   const int2 sr0 = rw < d.L ? d.gl_s[rw] : make_int2(0, 0);
-  const int na = (int)d.ctr[SG_CTR_NAFF];
   const int ng = (int)d.ctr[SG_CTR_NGATHER];
+  const int na = ng << d.aff_shift;  // slots of the gathered leaves
   long long ntri = 0, nfa = 0, nvfa = 0, nsh = 0;
+  // the dab's affected count for the host and the logs: a wave halfway through the
+  // grid (between the normals' waves and the refits') sums the leaves' counts
+  if (gw == nw / 2) {
+    int s = 0;
+    for (int g = lane_id(); g < ng; g += SG_WAVE) s += d.aff_cnt[g];
+    s = wave_sum_i(s);
+    if (lane_id() == 0) d.ctr[SG_CTR_NAFF] = s;
+  }
   for (int i = tid; i < na; i += nthreads) {
-    const int v = (i == tid) ? v0 : d.aff_v[i];
+    const int v = (i == tid) ? ((i & ((1 << d.aff_shift) - 1)) < c0 ? v0 : -1)
+                             : aff_slot_vertex(d, i, d.aff_cnt[i >> d.aff_shift]);
+    if (v < 0) continue;
     const int2 vt = (i == tid) ? vt0 : d.aff_vt[i];
     if (sync_q) {
       d.qx[v] = d.px[v];
@@ -1065,6 +1082,8 @@ SgDevView sg_view_of(SgDev &d, bool respect_hide, bool halt_live) {
   v.aff_vt = d.aff_vt;
   v.aff_vv = d.aff_vv;
   v.aff_fade = d.aff_fade;
+  v.aff_cnt = d.aff_cnt;
+  v.aff_shift = d.aff_shift;
   v.prev_cuts = d.prev_cuts;
   v.ctr = d.ctr;
   v.ctr_next = d.ctr + SG_CTR_COUNT;

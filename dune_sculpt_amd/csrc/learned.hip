@@ -254,9 +254,18 @@ __device__ inline bool learned_features(const SgDevView &d, int2 vv, float x, fl
 // displacement in their local frames.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_learned_feat(SgDevView d, float *X, float *frame, int cap) {
-  const int na = (int)d.ctr[SG_CTR_NAFF];
+  // one row per slot of the affected list; a slot without a record gives a row marked
+  // not valid (vertex -1)
+  const int na = (int)d.ctr[SG_CTR_NGATHER] << d.aff_shift;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < na && i < cap; i += gridDim.x * blockDim.x) {
-    const int v = d.aff_v[i];
+    const int v = aff_slot_vertex(d, i, d.aff_cnt[i >> d.aff_shift]);
+    if (v < 0) {
+      for (int k = 0; k < 28; k++) X[(size_t)i * 28 + k] = 0.0f;
+      float *fr = frame + (size_t)i * 16;
+      for (int k = 0; k < 16; k++) fr[k] = 0.0f;
+      fr[14] = __int_as_float(-1);
+      continue;
+    }
     const int2 vv = d.aff_vv[i];
     const float fade = d.aff_fade[i];
     const float x = d.px[v], y = d.py[v], z = d.pz[v];
@@ -275,11 +284,15 @@ __global__ __launch_bounds__(256) void k_learned_feat(SgDevView d, float *X, flo
 }
 
 __global__ __launch_bounds__(256) void k_learned_target(SgDevView d, const float *frame, float *T, int cap) {
-  const long long na = d.ctr[SG_CTR_NAFF];
+  const long long na = (long long)(int)d.ctr[SG_CTR_NGATHER] << d.aff_shift;
   const int n = (int)(na < cap ? na : cap);  // rows the feature kernel wrote
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const float *fr = frame + (size_t)i * 16;
     const int v = __float_as_int(fr[14]);
+    if (v < 0) {  // a slot without a record
+      T[(size_t)i * 3 + 0] = T[(size_t)i * 3 + 1] = T[(size_t)i * 3 + 2] = 0.0f;
+      continue;
+    }
     const float dx = d.px[v] - fr[10], dy = d.py[v] - fr[11], dz = d.pz[v] - fr[12];
     const float ih = fr[9] > 0.0f ? 1.0f / fr[9] : 0.0f;
     T[(size_t)i * 3 + 0] = (fr[0] * dx + fr[1] * dy + fr[2] * dz) * ih;
@@ -990,12 +1003,13 @@ int sg_enqueue_learned_post(PBVH *p, const SgDevView &v, SgMlp *m, int serial) {
 
 // Squared distances on the affected vertices of the last dab: learned result vs the
 // plain result (acc[0]) and the plain result vs the positions before the dab (acc[1]).
-__global__ __launch_bounds__(256) void k_judge(const int *aff_v, int na, const float *px, const float *py,
+__global__ __launch_bounds__(256) void k_judge(SgDevView d, int na, const float *px, const float *py,
                                               const float *pz, const float *lrn, int V, const float *bx,
                                               const float *by, const float *bz, double *acc) {
   double e = 0.0, m = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
-    const int v = aff_v[i];
+    const int v = aff_slot_vertex(d, i, d.aff_cnt[i >> d.aff_shift]);
+    if (v < 0) continue;
     const double x = px[v], y = py[v], z = pz[v];
     const double ex = lrn[v] - x, ey = lrn[V + v] - y, ez = lrn[2 * V + v] - z;
     const double mx = x - bx[v], my = y - by[v], mz = z - bz[v];
@@ -1016,7 +1030,7 @@ int sg_enqueue_judge(PBVH *p, int na, const float *lrn, const float *bx, const f
                      double *acc) {
   hipMemsetAsync(acc, 0, 16, p->stream);
   const int blocks = std::min((na + 255) / 256, 1024);
-  hipLaunchKernelGGL(k_judge, dim3(blocks), dim3(256), 0, p->stream, p->d.aff_v, na, p->d.px, p->d.py, p->d.pz, lrn,
+  hipLaunchKernelGGL(k_judge, dim3(blocks), dim3(256), 0, p->stream, sg_view(p), na, p->d.px, p->d.py, p->d.pz, lrn,
                      p->d.V, bx, by, bz, acc);
   return sg_check_hip(hipGetLastError(), "judge launch");
 }

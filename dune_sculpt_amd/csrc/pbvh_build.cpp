@@ -688,10 +688,18 @@ void sg_build_device(PBVH *p, const float *co) {
 
   alloc_fill(&d.gl_v, L, 0, s);
   alloc_fill(&d.gl_s, L, 0, s);
-  alloc_fill(&d.aff_v, V, 0, s);
-  alloc_fill(&d.aff_vt, V, 0, s);
-  alloc_fill(&d.aff_vv, V, 0, s);
-  alloc_fill(&d.aff_fade, V, 0, s);
+  {
+    int maxu = 1;
+    for (int l = 0; l < L; l++) maxu = std::max(maxu, h.leaf_vbeg[l + 1] - h.leaf_vbeg[l]);
+    d.aff_shift = 6;
+    while ((1 << d.aff_shift) < maxu) d.aff_shift++;
+  }
+  const size_t na = std::max<size_t>((size_t)L << d.aff_shift, 1);
+  alloc_fill(&d.aff_v, na, 0, s);
+  alloc_fill(&d.aff_vt, na, 0, s);
+  alloc_fill(&d.aff_vv, na, 0, s);
+  alloc_fill(&d.aff_fade, na, 0, s);
+  alloc_fill(&d.aff_cnt, std::max(L, 1), 0, s);
   alloc_fill(&d.ctr, 2 * SG_CTR_COUNT, 0, s);
   alloc_fill(&d.halt, 4, 0, s);
   alloc_fill(&d.seq, 1, 0, s);
@@ -709,7 +717,7 @@ void sg_free_device(PBVH *p) {
                   d.leaf_flags, d.leaf_cut, d.leaf_bb, d.leaf_obb, d.node_bb, d.node_obb,
                   d.cut_bb, d.cut_obb, d.cut_flags, d.cut_node, d.cut_lbeg, d.cut_lend,
                   d.sub_off, d.sub_tab, d.top_tab_n, d.top_tab, d.cut_dirty, d.vhide, d.vmark,
-                  d.gathered, d.gl_v, d.gl_s, d.aff_v, d.aff_vt, d.aff_vv, d.aff_fade,
+                  d.gathered, d.gl_v, d.gl_s, d.aff_v, d.aff_vt, d.aff_vv, d.aff_fade, d.aff_cnt,
                   d.prev_cuts, d.cut_touch, d.halt, d.seq, d.vstamp, d.ctr, d.err, d.vperm_old, d.stamp_v, d.stamp_v2, d.stamp_t,
                   d.stamp_a, d.curve, d.log, d.ltri, d.leaf_toff, d.tri_poly, d.node_parent, d.leaf_node,
                   d.ray_in, d.ray_out, d.place, d.place_rays, d.rl_list, d.rl_args,

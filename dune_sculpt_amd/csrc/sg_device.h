@@ -127,6 +127,12 @@ __device__ __forceinline__ void sg_place(SgDabK &k, const float *place) {
   }
 }
 
+// Affected-list slot t: its vertex when slot t holds a record, else -1 (gathered leaf
+// t >> aff_shift owns the slots below its count).
+__device__ __forceinline__ int aff_slot_vertex(const SgDevView &d, int t, int cnt) {
+  return (t & ((1 << d.aff_shift) - 1)) < cnt ? d.aff_v[t] : -1;
+}
+
 // Undo recording: save a leaf the running dab is about to write, once per step (one
 // wave; the leaf's uniq vertices are the contiguous range vr): positions, normals and
 // the leaf's two boxes, as they were before the dab.

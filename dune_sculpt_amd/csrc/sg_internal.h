@@ -96,9 +96,13 @@ struct SgDev {
   // per-dab scratch
   int *gathered = nullptr;      // [L] gathered leaves (ascending)
   int2 *gl_v = nullptr, *gl_s = nullptr; // [L] their uniq / shared vertex ranges
-  int *aff_v = nullptr;         // [V] affected vertices of the running dab
-  int2 *aff_vt = nullptr, *aff_vv = nullptr; // [V] their vertex->tri / vertex->vertex ranges
-  float *aff_fade = nullptr;    // [V] Smooth factor
+  // affected vertices of the running dab: gathered leaf g owns the slots
+  // [g << aff_shift, (g << aff_shift) + aff_cnt[g]) (2^aff_shift >= its uniq vertices)
+  int *aff_v = nullptr;         // [L << aff_shift]
+  int2 *aff_vt = nullptr, *aff_vv = nullptr; // their vertex->tri / vertex->vertex ranges
+  float *aff_fade = nullptr;    // Smooth factor
+  int *aff_cnt = nullptr;       // [L] affected vertices per gathered leaf
+  int aff_shift = 6;
   int *prev_cuts = nullptr;     // [C] cuts refit by the previous dab (+ count in ctr)
   long long *ctr = nullptr;     // two counter sets (even / odd dabs), see SG_CTR_*
   int *err = nullptr;           // device error word

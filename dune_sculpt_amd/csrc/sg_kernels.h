@@ -61,9 +61,11 @@ struct SgDevView {
   SgUndoCtl *undo;
   int *place_pair;  // handoff of a Grab placement's two rays (ray.hip)
   int2 *gl_v, *gl_s;
-  int *aff_v;
+  int *aff_v;  // gathered leaf g: slots (g << aff_shift) + [0, aff_cnt[g])
   int2 *aff_vt, *aff_vv;
   float *aff_fade;
+  int *aff_cnt;
+  int aff_shift;
   int *prev_cuts;
   int *cut_touch;
   int *halt;
