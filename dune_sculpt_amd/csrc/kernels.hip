@@ -226,11 +226,11 @@ __global__ __launch_bounds__(256) void k_area(SgDevView d, SgDabK k, const float
 
 // ---------------------------------------------------------------------------
 // K3: brush.  One wave per gathered leaf (grid-stride); two vertices per lane, so a
-// leaf of up to 128 uniq vertices is one load level, one ballot pair and one atomic.
-// Spec S2 vertex test, S3/S4 factor, S6-S9 displacement.  Affected vertices are
-// compacted with ballot + prefix popcount into records (vertex, its vertex->tri and
-// vertex->vertex CSR ranges) so the later kernels skip those lookups.  The slot
-// reservation is in flight while the displacement is computed.
+// leaf of up to 128 uniq vertices is one load level and one ballot pair.  Spec S2
+// vertex test, S3/S4 factor, S6-S9 displacement.  Affected vertices are compacted with
+// ballot + prefix popcount into records (vertex, its vertex->tri and vertex->vertex CSR
+// ranges) in the gathered leaf's own slots of the list, with the leaf's count beside
+// them, so the later kernels skip those lookups and no wave waits on a shared counter.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_brush(SgDevView d, SgDabK k, const float *curve, int stamp,
                                                const float *place) {
