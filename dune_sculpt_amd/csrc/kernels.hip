@@ -76,33 +76,29 @@ __global__ __launch_bounds__(256) void k_search(SgDevView d, SgDabK k, long long
     atomicAdd((unsigned long long *)&d.ctr[SG_CTR_NGATHER], SG_NGATHER_UNDO);
   const float *cut_bbs = k.use_orig ? d.cut_obb : d.cut_bb;
   const float *leaf_bbs = k.use_orig ? d.leaf_obb : d.leaf_bb;
+  const float *leaf_alt = k.use_orig ? d.leaf_bb : d.leaf_obb;
   long long ntests = 0;
   for (int c = gw; c < d.C; c += nw) {
     const int touch = d.cut_touch[c];
-    BB6 cb = bb_load(cut_bbs + 8 * (size_t)c);
+    const BB6 cb = bb_load(cut_bbs + 8 * (size_t)c);
     const int lb = d.cut_lbeg[c], le = d.cut_lend[c];
     const int cfl = d.cut_flags[c];
-    if (touch == serial - 1) {
-      BB6 nb, ob;
-      leaves_box(d.leaf_bb, d.leaf_obb, lb, le, nb, ob);
-      if (lane == 0) {
-        bb_store(d.cut_bb + 8 * (size_t)c, nb);
-        bb_store(d.cut_obb + 8 * (size_t)c, ob);
-      }
-      cb = k.use_orig ? ob : nb;
-      cb.mn[0] = __shfl(cb.mn[0], 0);
-      cb.mn[1] = __shfl(cb.mn[1], 0);
-      cb.mn[2] = __shfl(cb.mn[2], 0);
-      cb.mx[0] = __shfl(cb.mx[0], 0);
-      cb.mx[1] = __shfl(cb.mx[1], 0);
-      cb.mx[2] = __shfl(cb.mx[2], 0);
-    }
+    // The previous dab changed this subtree, so its box is stale: every leaf is tested
+    // and the leaf boxes are folded into a fresh subtree box on the way.  The subtree
+    // box only prunes (a leaf box lies inside it, so a leaf it rejects fails its own
+    // test too, and a fully hidden subtree has only hidden leaves): the gathered set
+    // is the same as with a refreshed box, one dependent load sooner.
+    const bool refresh = touch == serial - 1;
     ntests++;
-    if ((cfl & k.hide_skip) || !sphere_box(cb, k.cx, k.cy, k.cz, k.r2)) continue;
+    if (!refresh && ((cfl & k.hide_skip) || !sphere_box(cb, k.cx, k.cy, k.cz, k.r2))) continue;
     const int nl = le - lb;
     ntests += nl;
+    BB6 tb, ab;  // folds of the tested and the other leaf boxes (refresh only)
+    bb_reset(tb);
+    bb_reset(ab);
     for (int q0 = 0; q0 < nl; q0 += SG_WAVE * SEARCH_UNROLL) {
       bool pass[SEARCH_UNROLL];
+      int2 vr[SEARCH_UNROLL], sr[SEARCH_UNROLL];
 #pragma unroll
       for (int j = 0; j < SEARCH_UNROLL; j++) {
         const int l = lb + q0 + j * SG_WAVE + lane;
@@ -110,6 +106,14 @@ __global__ __launch_bounds__(256) void k_search(SgDevView d, SgDabK k, long long
         if (l < le) {
           const BB6 b = bb_load(leaf_bbs + 8 * (size_t)l);
           const int fl = d.leaf_flags[l];
+          // the leaf's vertex ranges are loaded with its box, ahead of the test, so
+          // the writes after the slot reservation wait on nothing else
+          vr[j] = make_int2(d.leaf_vbeg[l], d.leaf_vbeg[l + 1]);
+          sr[j] = make_int2(d.leaf_sh_off[l], d.leaf_sh_off[l + 1]);
+          if (refresh) {
+            bb_add_bb(tb, b);
+            bb_add_bb(ab, bb_load(leaf_alt + 8 * (size_t)l));
+          }
           pass[j] = !(fl & k.hide_skip) && sphere_box(b, k.cx, k.cy, k.cz, k.r2);
         }
       }
@@ -123,15 +127,6 @@ __global__ __launch_bounds__(256) void k_search(SgDevView d, SgDabK k, long long
       if (cnt == 0) continue;
       int wbase = 0;
       if (lane == 0) wbase = (int)atomicAdd((unsigned long long *)&d.ctr[SG_CTR_NGATHER], (unsigned long long)cnt);
-      int2 vr[SEARCH_UNROLL], sr[SEARCH_UNROLL];
-#pragma unroll
-      for (int j = 0; j < SEARCH_UNROLL; j++) {
-        const int l = lb + q0 + j * SG_WAVE + lane;
-        if (pass[j]) {
-          vr[j] = make_int2(d.leaf_vbeg[l], d.leaf_vbeg[l + 1]);
-          sr[j] = make_int2(d.leaf_sh_off[l], d.leaf_sh_off[l + 1]);
-        }
-      }
       wbase = __shfl(wbase, 0);
       int off = wbase;
 #pragma unroll
@@ -148,6 +143,14 @@ __global__ __launch_bounds__(256) void k_search(SgDevView d, SgDabK k, long long
         d.cut_touch[c] = serial;
         d.cut_dirty[c] = 1;
         d.draw_dirty[c] = 1;  // the subtree's leaves need new draw records (UpdateDrawBuffers)
+      }
+    }
+    if (refresh) {
+      tb = bb_wave_reduce(tb);
+      ab = bb_wave_reduce(ab);
+      if (lane == 0) {
+        bb_store(d.cut_bb + 8 * (size_t)c, k.use_orig ? ab : tb);
+        bb_store(d.cut_obb + 8 * (size_t)c, k.use_orig ? tb : ab);
       }
     }
   }
