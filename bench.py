@@ -36,7 +36,7 @@ HBM_PEAK_GBPS = 8000.0
 # rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) summed over the dab kernels, FETCH_SIZE
 # scaled by 2.00 and WRITE_SIZE by 1.00 (what they read for a 1 GiB device copy);
 # profiles/dab_step_c2.md holds the commands and the per-kernel figures.
-TRAFFIC_BYTES_PER_DAB = 5_893_580
+TRAFFIC_BYTES_PER_DAB = 5_867_877
 METRIC = "vertex-updates/s + HBM GB/s; learned vs plain Smooth dabs/s at ≤1e-3 RMS"
 
 

@@ -193,6 +193,8 @@ void BKE_pbvh_free(PBVH *p) {
       if (n.proxies) BKE_pbvh_node_free_proxies(&n);
     }
   }
+  for (SgProxChunk &c : p->prox_chunks) (void)hipHostFree(c.host);
+  p->prox_chunks.clear();
   if (!p->no_device) sg_free_device(p);
   for (int sl = 0; sl < 2; sl++)
     for (void *q : p->snap[sl]) hipFree(q);
@@ -236,8 +238,10 @@ void BKE_pbvh_build_mesh(PBVH *p, Mesh *mesh, const MPoly *mpoly, const MLoop *m
       if (n.flag & PBVH_Leaf) {
         free(n.vert_indices);
         free(n.face_vert_indices);
+        if (n.proxies) BKE_pbvh_node_free_proxies(&n);
       }
     }
+    for (SgProxChunk &c : p->prox_chunks) c.used = 0;
     if (!p->no_device) sg_free_device(p);
   }
   p->type = PBVH_FACES;
@@ -1407,21 +1411,53 @@ const float (*DUNE_pbvh_get_vert_normals(const PBVH *cp))[3] {
 
 // ---- proxies (pbvh.c:4757-4818) ----------------------------------------------
 
+// A zeroed block of nf floats for a proxy of a node in p->nodes: carved from the
+// pinned proxy arena (a new 16 MiB chunk when the last one is full), so the combine
+// uploads it without a host pack; from the heap when pinned memory is unavailable.
+static float *proxy_alloc(PBVH *p, size_t nf, bool in_tree) {
+  const size_t nfa = (nf + 15) & ~(size_t)15;  // 64-byte blocks
+  if (in_tree && !p->no_device) {
+    if (p->prox_chunks.empty() || p->prox_chunks.back().cap - p->prox_chunks.back().used < nfa) {
+      const size_t cap = std::max<size_t>(nfa, (size_t)1 << 22);
+      void *h = nullptr;
+      if (hipHostMalloc(&h, cap * sizeof(float), hipHostMallocDefault) == hipSuccess)
+        p->prox_chunks.push_back({(float *)h, cap, 0});
+      else
+        (void)hipGetLastError();
+    }
+    if (!p->prox_chunks.empty() && p->prox_chunks.back().cap - p->prox_chunks.back().used >= nfa) {
+      SgProxChunk &c = p->prox_chunks.back();
+      float *r = c.host + c.used;
+      c.used += nfa;
+      memset(r, 0, nf * sizeof(float));
+      return r;
+    }
+  }
+  return (float *)calloc(nf, sizeof(float));
+}
+
+bool sg_proxy_in_arena(const PBVH *p, const void *ptr) {
+  for (const SgProxChunk &c : p->prox_chunks)
+    if (ptr >= (const void *)c.host && ptr < (const void *)(c.host + c.cap)) return true;
+  return false;
+}
+
 PBVHProxyNode *BKE_pbvh_node_add_proxy(PBVH *p, PBVHNode *node) {
   const int index = node->proxy_count;
-  if (index == 0 && node >= p->nodes.data() && node < p->nodes.data() + p->nodes.size())
-    p->proxy_nodes.push_back((int)(node - p->nodes.data()));  // the combine's worklist
+  const bool in_tree = node >= p->nodes.data() && node < p->nodes.data() + p->nodes.size();
+  if (index == 0 && in_tree) p->proxy_nodes.push_back((int)(node - p->nodes.data()));  // the combine's worklist
   node->proxy_count++;
   node->proxies = (PBVHProxyNode *)realloc(node->proxies, node->proxy_count * sizeof(PBVHProxyNode));
   int totverts = 0;
   BKE_pbvh_node_num_verts(p, node, &totverts, nullptr);
-  node->proxies[index].co = (float(*)[3])calloc((size_t)std::max(totverts, 1), sizeof(float[3]));
+  node->proxies[index].co = (float(*)[3])proxy_alloc(p, 3 * (size_t)std::max(totverts, 1), in_tree);
   return node->proxies + index;
 }
 
 void BKE_pbvh_node_free_proxies(PBVHNode *node) {
   for (int i = 0; i < node->proxy_count; i++) {
-    free(node->proxies[i].co);
+    // arena blocks are reclaimed together once no proxy is left (the combine)
+    if (!(node->owner && sg_proxy_in_arena(node->owner, node->proxies[i].co))) free(node->proxies[i].co);
     node->proxies[i].co = nullptr;
   }
   free(node->proxies);

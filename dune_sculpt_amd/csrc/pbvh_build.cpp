@@ -723,7 +723,7 @@ void sg_free_device(PBVH *p) {
                   d.ray_in, d.ray_out, d.place, d.place_rays, d.rl_list, d.rl_args,
                   d.draw_dirty, d.draw_tri, d.draw_off, d.draw_leaf, d.draw_fset, d.draw_poly_off,
                   d.draw_poly_v, d.draw_vbo, d.undo, d.undo_stamp, d.undo_slot, d.place_pair,
-                  d.prox_dev, d.prox_tab, d.prox_leaves};
+                  d.prox_dev, d.prox_tab, d.prox_leaves, d.prox_off};
   for (void *q : ptrs) {
     if (q) hipFree(q);
   }

@@ -9,11 +9,13 @@
 // tool works from the original mesh) or the current one, val += each proxy in order
 // (add_v3_v3), then SCULPT_clip (a plain copy without mirror clipping), the vertex is
 // flagged for the normal update, and the node's proxies are freed
-// (BKE_pbvh_node_free_proxies, :4778-4789).  Here the proxies of all nodes go to HBM in
-// one copy and one kernel writes the vertices, marks them for normals and saves each
-// leaf for the open undo step first; the normals, leaf boxes and search boxes follow
-// on the GPU.  The original coordinates are those of the open undo step (the state at
-// sg_undo_begin): the reference reads them from its undo nodes.
+// (BKE_pbvh_node_free_proxies, :4778-4789).  Here the proxies live in pinned host
+// chunks (BKE_pbvh_node_add_proxy carves them there), so the caller's fill is already
+// in memory the copy engine reads: each chunk goes to HBM in one copy, and one kernel
+// writes the vertices, marks them for normals and saves each leaf for the open undo
+// step first; the normals, leaf boxes and search boxes follow on the GPU.  The original
+// coordinates are those of the open undo step (the state at sg_undo_begin): the
+// reference reads them from its undo nodes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,17 +27,16 @@
 #include "sg_kernels.h"
 #include "sg_device.h"
 
-// One wave per leaf holding proxies.  tab: leaf, proxy count, float offset low, high
-// of the leaf's packed proxies ([count][uniq][3]).
-__global__ __launch_bounds__(256) void k_combine_proxies(SgDevView d, const int4 *tab, const float *prox, int n,
-                                                         int use_orco) {
+// One wave per leaf holding proxies.  tab: leaf, proxy count, index of its first
+// proxy in poff; poff: each proxy's float offset in prox ([uniq][3] per proxy).
+__global__ __launch_bounds__(256) void k_combine_proxies(SgDevView d, const int4 *tab, const long long *poff,
+                                                         const float *prox, int n, int use_orco) {
   const int lane = lane_id();
   const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
   const int nw = gridDim.x * blockDim.x / SG_WAVE;
   for (int e = gw; e < n; e += nw) {
     const int4 t = tab[e];
-    const int leaf = t.x, np = t.y;
-    const size_t off = ((size_t)(unsigned)t.w << 32) | (unsigned)t.z;
+    const int leaf = t.x, np = t.y, pi = t.z;
     const int2 vr = make_int2(d.leaf_vbeg[leaf], d.leaf_vbeg[leaf + 1]);
     const int nu = vr.y - vr.x;
     // original coordinates: the step's record when an earlier kernel of the open step
@@ -63,7 +64,7 @@ __global__ __launch_bounds__(256) void k_combine_proxies(SgDevView d, const int4
         z = d.pz[v];
       }
       for (int q = 0; q < np; q++) {
-        const float *pq = prox + off + 3 * ((size_t)q * nu + i);
+        const float *pq = prox + poff[pi + q] + 3 * (size_t)i;
         x = x + pq[0];
         y = y + pq[1];
         z = z + pq[2];
@@ -82,6 +83,7 @@ __global__ __launch_bounds__(256) void k_combine_proxies(SgDevView d, const int4
 
 int sg_enqueue_normals_list(PBVH *p, const int *leaves_dev, int n);
 int sg_enqueue_refit_list(PBVH *p, const int *leaves_dev, int n);
+bool sg_proxy_in_arena(const PBVH *p, const void *ptr);
 
 extern "C" {
 
@@ -90,9 +92,6 @@ extern "C" {
 // Returns the number of leaves changed.
 int sg_pbvh_combine_proxies(PBVH *p, int use_orco) {
   if (!p || p->no_device || !p->build_ok) return sg_set_error("no device mesh");
-  std::vector<int> leaves;
-  std::vector<int4> tab;
-  size_t total = 0;
   // nodes holding proxies, in node order (BKE_pbvh_gather_proxies)
   std::vector<PBVHNode *> nodes;
   {
@@ -102,29 +101,65 @@ int sg_pbvh_combine_proxies(PBVH *p, int use_orco) {
     nodes.assign(arr, arr + tot);
     free(arr);
   }
+  if (nodes.empty()) return 0;
+  for (PBVHNode *np : nodes)
+    if (np->leaf < 0) return sg_set_error("a proxy on an internal node");
+  // device layout: the used part of each arena chunk, in chunk order, then the proxies
+  // that came from the heap, packed
+  std::vector<size_t> cbase(p->prox_chunks.size());
+  size_t total = 0;
+  for (size_t c = 0; c < p->prox_chunks.size(); c++) {
+    cbase[c] = total;
+    total += p->prox_chunks[c].used;
+  }
+  const size_t arena_total = total;
+  std::vector<int> leaves;
+  std::vector<int4> tab;
+  std::vector<long long> poff;
+  std::vector<const float *> heap_src;
+  std::vector<size_t> heap_len;
   for (PBVHNode *np : nodes) {
     const PBVHNode &n = *np;
-    if (n.leaf < 0) return sg_set_error("a proxy on an internal node");
-    const size_t cnt = (size_t)n.proxy_count * n.uniq_verts * 3;
-    tab.push_back(make_int4(n.leaf, n.proxy_count, (int)(unsigned)total, (int)(unsigned)(total >> 32)));
+    tab.push_back(make_int4(n.leaf, n.proxy_count, (int)poff.size(), 0));
     leaves.push_back(n.leaf);
-    total += cnt;
+    const size_t len = (size_t)n.uniq_verts * 3;
+    for (int q = 0; q < n.proxy_count; q++) {
+      const float *co = &n.proxies[q].co[0][0];
+      long long off = -1;
+      for (size_t c = 0; c < p->prox_chunks.size() && off < 0; c++) {
+        const SgProxChunk &ch = p->prox_chunks[c];
+        if (co >= ch.host && co < ch.host + ch.cap) off = (long long)(cbase[c] + (size_t)(co - ch.host));
+      }
+      if (off < 0) {  // a heap block: packed after the chunks
+        off = (long long)total;
+        heap_src.push_back(co);
+        heap_len.push_back(len);
+        total += len;
+      }
+      poff.push_back(off);
+    }
   }
-  if (leaves.empty()) return 0;
   SgDev &d = p->d;
-  const int nl = (int)leaves.size();
-  const size_t need = std::max<size_t>(total, 1);
+  const int nl = (int)leaves.size(), npx = (int)poff.size();
+  const size_t need = std::max<size_t>(total, 1), heap_need = total - arena_total;
   int rc = 0;
-  if (need > d.prox_cap) {  // grow-only staging: pinned host pack and its device copy
-    if (d.prox_host) (void)hipHostFree(d.prox_host);
+  if (need > d.prox_cap) {  // grow-only device copy of the proxies
     if (d.prox_dev) (void)hipFree(d.prox_dev);
-    d.prox_host = d.prox_dev = nullptr;
+    d.prox_dev = nullptr;
     d.prox_cap = 0;
     const size_t cap = need + need / 2;
-    rc = sg_check_hip(hipHostMalloc(&d.prox_host, cap * sizeof(float), hipHostMallocDefault), "hipHostMalloc");
-    if (!rc) rc = sg_check_hip(hipMalloc(&d.prox_dev, cap * sizeof(float)), "hipMalloc");
+    rc = sg_check_hip(hipMalloc(&d.prox_dev, cap * sizeof(float)), "hipMalloc");
     if (rc) return rc;
     d.prox_cap = cap;
+  }
+  if (heap_need > d.prox_host_cap) {  // grow-only pinned pack of the heap proxies
+    if (d.prox_host) (void)hipHostFree(d.prox_host);
+    d.prox_host = nullptr;
+    d.prox_host_cap = 0;
+    const size_t cap = heap_need + heap_need / 2;
+    rc = sg_check_hip(hipHostMalloc(&d.prox_host, cap * sizeof(float), hipHostMallocDefault), "hipHostMalloc");
+    if (rc) return rc;
+    d.prox_host_cap = cap;
   }
   if (nl > d.prox_tab_cap) {
     if (d.prox_tab) (void)hipFree(d.prox_tab);
@@ -138,19 +173,39 @@ int sg_pbvh_combine_proxies(PBVH *p, int use_orco) {
     if (rc) return rc;
     d.prox_tab_cap = cap;
   }
-  size_t w = 0;
-  for (PBVHNode *np : nodes) {
-    const PBVHNode &n = *np;
-    for (int q = 0; q < n.proxy_count; q++) {
-      memcpy(d.prox_host + w, n.proxies[q].co, (size_t)n.uniq_verts * 3 * sizeof(float));
-      w += (size_t)n.uniq_verts * 3;
-    }
+  if (npx > d.prox_off_cap) {
+    if (d.prox_off) (void)hipFree(d.prox_off);
+    d.prox_off = nullptr;
+    d.prox_off_cap = 0;
+    const int cap = npx + npx / 2;
+    rc = sg_check_hip(hipMalloc(&d.prox_off, (size_t)cap * sizeof(long long)), "hipMalloc");
+    if (rc) return rc;
+    d.prox_off_cap = cap;
   }
-  rc = sg_check_hip(hipMemcpyAsync(d.prox_dev, d.prox_host, need * sizeof(float), hipMemcpyHostToDevice, p->stream),
-                    "copy");
+  for (size_t c = 0; c < p->prox_chunks.size() && !rc; c++) {
+    const SgProxChunk &ch = p->prox_chunks[c];
+    if (ch.used)
+      rc = sg_check_hip(hipMemcpyAsync(d.prox_dev + cbase[c], ch.host, ch.used * sizeof(float),
+                                       hipMemcpyHostToDevice, p->stream),
+                        "copy");
+  }
+  if (!rc && heap_need) {
+    size_t w = 0;
+    for (size_t h = 0; h < heap_src.size(); h++) {
+      memcpy(d.prox_host + w, heap_src[h], heap_len[h] * sizeof(float));
+      w += heap_len[h];
+    }
+    rc = sg_check_hip(hipMemcpyAsync(d.prox_dev + arena_total, d.prox_host, heap_need * sizeof(float),
+                                     hipMemcpyHostToDevice, p->stream),
+                      "copy");
+  }
   if (!rc)
     rc = sg_check_hip(hipMemcpyAsync(d.prox_tab, tab.data(), tab.size() * sizeof(int4), hipMemcpyHostToDevice,
                                      p->stream),
+                      "copy");
+  if (!rc)
+    rc = sg_check_hip(hipMemcpyAsync(d.prox_off, poff.data(), poff.size() * sizeof(long long),
+                                     hipMemcpyHostToDevice, p->stream),
                       "copy");
   if (!rc)
     rc = sg_check_hip(hipMemcpyAsync(d.prox_leaves, leaves.data(), leaves.size() * sizeof(int),
@@ -158,20 +213,23 @@ int sg_pbvh_combine_proxies(PBVH *p, int use_orco) {
                       "copy");
   if (!rc) {
     const int blocks = std::max(1, std::min((nl + 3) / 4, 8192));
-    hipLaunchKernelGGL(k_combine_proxies, dim3(blocks), dim3(256), 0, p->stream, sg_view(p), d.prox_tab, d.prox_dev,
-                       nl, use_orco ? 1 : 0);
+    hipLaunchKernelGGL(k_combine_proxies, dim3(blocks), dim3(256), 0, p->stream, sg_view(p), d.prox_tab, d.prox_off,
+                       d.prox_dev, nl, use_orco ? 1 : 0);
     rc = sg_check_hip(hipGetLastError(), "combine launch");
   }
   // normals of the marked vertices, then leaf boxes and search boxes, then the tree
   if (!rc) rc = sg_enqueue_normals_list(p, d.prox_leaves, nl);
   if (!rc) rc = sg_enqueue_refit_list(p, d.prox_leaves, nl);
   if (!rc) rc = sg_enqueue_flush(p);
+  // the copies have read the arena once the stream is idle
   if (!rc) rc = sg_check_hip(hipStreamSynchronize(p->stream), "combine proxies");
   if (rc) return rc;
   p->nodes_dirty_deferred = false;
   p->host_bb_stale = p->host_verts_stale = p->host_normals_stale = true;
   for (PBVHNode *np : nodes) BKE_pbvh_node_free_proxies(np);
   p->proxy_nodes.clear();
+  // every proxy of the tree was gathered and is freed: the arena starts over
+  for (SgProxChunk &c : p->prox_chunks) c.used = 0;
   return nl;
 }
 

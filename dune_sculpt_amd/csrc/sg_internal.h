@@ -138,13 +138,24 @@ struct SgDev {
   int *place_pair = nullptr;    // [16] a Grab placement's two rays meet here (count, then results)
   SgRay *place_rays = nullptr;  // the stroke's rays
   int place_cap = 0;
-  // proxy combine staging (grow-only): pinned host pack, device copy, leaf table
+  // proxy combine staging (grow-only): pinned host pack of heap-allocated proxies,
+  // device copy of all proxies, leaf table, per-proxy offsets
   float *prox_host = nullptr;   // pinned
+  size_t prox_host_cap = 0;     // floats
   float *prox_dev = nullptr;
-  size_t prox_cap = 0;          // floats in both
+  size_t prox_cap = 0;          // floats
   int4 *prox_tab = nullptr;
   int *prox_leaves = nullptr;
   int prox_tab_cap = 0;
+  long long *prox_off = nullptr;  // float offset of each proxy in prox_dev
+  int prox_off_cap = 0;
+};
+
+// Pinned host chunk the proxies are carved from: the caller fills them in place and
+// the combine uploads each chunk's used part in one copy.
+struct SgProxChunk {
+  float *host;
+  size_t cap, used;  // floats
 };
 
 enum {
@@ -221,6 +232,7 @@ struct PBVH {
   bool halt_live = false;            // the work being enqueued contains learned dabs (kernels test the halt flag)
   bool draw_marks_pending = false;   // some node carries a host-side draw-buffer mark
   std::vector<int> proxy_nodes;      // nodes given a proxy since the last combine (may repeat)
+  std::vector<SgProxChunk> prox_chunks;  // proxy arena (pinned; reset once no proxy is left)
   SgUndoState undo;                  // undo steps recorded on the GPU (undo.hip)
   const float *last_place = nullptr; // placement slots of the last ray-placed stroke
   int last_place_n = 0;

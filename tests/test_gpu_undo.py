@@ -349,6 +349,37 @@ def test_combine_proxies_matches_oracle(sg, po):
     o.close()
 
 
+def test_combine_proxies_across_arena_chunks(sg, po):
+    """Proxies are carved from 16 MiB pinned chunks: 9 proxies on every leaf of a 164k-
+    vertex mesh (1.5 M proxy vertices, two chunks), added round-robin over the leaves so
+    each leaf's proxies are scattered over both chunks, fold in proxy order, bit-equal
+    to the oracle's; a second, smaller combine reuses the reset arena."""
+    from parity_util import n_bit_diff
+
+    co, tris = sg.noisy_icosphere(7, seed=31, amplitude=0.02)
+    p = sg.PBVH(co, tris, leaf_limit=64)
+    o = po.Oracle(co, tris, leaf_limit=64, fp64=False)
+    leaves = [int(x) for x in o.leaves_dfs()]
+    rng = np.random.default_rng(8)
+    for rnd, (nprox, nodes) in enumerate(((9, leaves), (2, leaves[::7]))):
+        props = {nd: [] for nd in nodes}
+        for _ in range(nprox):
+            for nd in nodes:
+                a = p.add_proxy(nd)
+                assert not a.any()
+                a[:] = rng.normal(0, 1e-3, a.shape)
+                props[nd].append(a.copy())
+        print(f"[undo] combine across arena chunks round {rnd}: {len(nodes)} leaves x {nprox}", flush=True)
+        assert p.combine_proxies() == len(nodes)
+        o.combine_proxies(nodes, [props[nd] for nd in nodes])
+        assert n_bit_diff(p.positions(), o.positions()) == 0
+        assert n_bit_diff(p.normals(), o.normals()) == 0
+        vb, _ = o.node_bb(0)
+        assert n_bit_diff(p.node_bb(0), vb) == 0
+    p.close()
+    o.close()
+
+
 @pytest.mark.parametrize("in_step", [True, False])
 def test_grab_from_original_coordinates(sg, po, in_step):
     """SG_DAB_ORIGINAL Grab dabs at an anchored centre with growing deltas: inside an
