@@ -148,7 +148,9 @@ void BKE_pbvh_node_get_verts(PBVH *pbvh, PBVHNode *node, const int **r_vert_indi
                              MVert **r_verts);
 /* pbvh.c:1761 / :1766 */
 bool BKE_pbvh_node_fully_hidden_get(PBVHNode *node);
-/* pbvh.c:4757 / :4778 / :4791 proxies */
+/* pbvh.c:4757 / :4778 / :4791 proxies.  add_proxy returns a zeroed buffer carved from
+ * pinned host memory the PBVH owns (uploaded by sg_pbvh_combine_proxies without a
+ * copy on the host); release it only through BKE_pbvh_node_free_proxies. */
 PBVHProxyNode *BKE_pbvh_node_add_proxy(PBVH *pbvh, PBVHNode *node);
 void BKE_pbvh_node_free_proxies(PBVHNode *node);
 void BKE_pbvh_gather_proxies(PBVH *pbvh, PBVHNode ***r_array, int *r_tot);
