@@ -343,7 +343,8 @@ __global__ __launch_bounds__(LRN_THREADS) void k_mlp_predict(SgMlpView m, const 
 // runs this dab with the plain brush.  Both walk the gathered leaves' vertex ranges
 // and test the stamp (no per-vertex list, so no slot reservations).
 // ---------------------------------------------------------------------------
-#define LRN_SLOTS 4  // waves per gathered leaf
+#define LRN_SLOTS 4   // waves per gathered leaf (batch)
+#define LRN_DSLOTS 8  // waves per gathered leaf (single mesh): one 16-row tile each for leaves up to 128 uniq vertices
 
 // Evaluate one 16-vertex tile of a learned dab (every lane of the wave calls it; lanes
 // < 16 own a vertex).  Returns the Smooth factor of the lane's vertex.
@@ -403,16 +404,24 @@ __global__ __launch_bounds__(LRN_THREADS) void k_brush_learned(SgDevView d, SgDa
   __shared__ unsigned long long s_acc[3];
   if (d.halt && d.halt[0]) return;
   sg_place(k, place);
+  const int w = threadIdx.x / 64, lane = __lane_id();
+  const int gw = blockIdx.x * LRN_WAVES + w;
+  const int nw = gridDim.x * LRN_WAVES;
+  // the wave's first leaf range, loaded beside the counter (speculatively: any index
+  // below L is inside the gathered-leaf list; the loop uses it only when it is live)
+  const int2 vr0 = gw / LRN_DSLOTS < d.L ? d.gl_v[gw / LRN_DSLOTS] : make_int2(0, 0);
   const long long ng_ctr = d.ctr[SG_CTR_NGATHER];
-  const int nitems = LRN_SLOTS * (int)ng_ctr;
+  const int nitems = LRN_DSLOTS * (int)ng_ctr;
   if (blockIdx.x * LRN_WAVES >= nitems) return;  // whole block idle: skip the weight fill
+  // the first tile's loads go out before the weight fill, so both share one round trip
+  const int base0 = vr0.x + (gw % LRN_DSLOTS) * LRN_ROWS;
+  float nx0 = 0.0f, ny0 = 0.0f, nz0 = 0.0f;
+  BrushIn in0{};
+  if (gw < nitems && base0 < vr0.y) in0 = learned_tile_load(d, k, base0, vr0.y, lane, nx0, ny0, nz0);
   mlp_lds_load<LRN_THREADS>(s, m);
   if (threadIdx.x < 3) s_acc[threadIdx.x] = 0ull;
   __syncthreads();
   const int stamp = d.seq[0];
-  const int w = threadIdx.x / 64, lane = __lane_id();
-  const int gw = blockIdx.x * LRN_WAVES + w;
-  const int nw = gridDim.x * LRN_WAVES;
   long long esum = 0;
   int nood = 0, naff = 0;
   if (ng_ctr & SG_NGATHER_UNDO) {
@@ -420,14 +429,16 @@ __global__ __launch_bounds__(LRN_THREADS) void k_brush_learned(SgDevView d, SgDa
     // positions alone (k_post_learned copies or redoes them), so each leaf is saved
     // from the positions by the wave of its first slot, whatever the other slots write
     for (int it = gw; it < nitems; it += nw)
-      if (it % LRN_SLOTS == 0)
-        undo_save_leaf(d, d.gathered[it / LRN_SLOTS], d.gl_v[it / LRN_SLOTS], d.px, d.py, d.pz);
+      if (it % LRN_DSLOTS == 0)
+        undo_save_leaf(d, d.gathered[it / LRN_DSLOTS], d.gl_v[it / LRN_DSLOTS], d.px, d.py, d.pz);
   }
   for (int it = gw; it < nitems; it += nw) {
-    const int2 vr = d.gl_v[it / LRN_SLOTS];
-    for (int base = vr.x + (it % LRN_SLOTS) * LRN_ROWS; base < vr.y; base += LRN_SLOTS * LRN_ROWS) {
-      float nx, ny, nz;
-      const BrushIn in = learned_tile_load(d, k, base, vr.y, lane, nx, ny, nz);
+    int2 vr = vr0;
+    if (it != gw) vr = d.gl_v[it / LRN_DSLOTS];
+    for (int base = vr.x + (it % LRN_DSLOTS) * LRN_ROWS; base < vr.y; base += LRN_DSLOTS * LRN_ROWS) {
+      float nx = nx0, ny = ny0, nz = nz0;
+      BrushIn in = in0;
+      if (base != base0 || it != gw) in = learned_tile_load(d, k, base, vr.y, lane, nx, ny, nz);
       if (__ballot(in.aff) == 0ull) continue;
       const int v = base + lane;
       learned_eval(d, k, curve, m, s, bufA[w], bufB[w], sOut[w], v, in, nx, ny, nz, esum, nood);
@@ -985,7 +996,9 @@ int sg_mfma_probe(const uint16_t *A, const uint16_t *B, float *D) {
 // ---- launch helpers used by the dab path (api.cpp / kernels.hip) ----
 int sg_enqueue_learned_brush(PBVH *p, const SgDevView &v, const SgDabK &k, const float *curve, SgMlp *m,
                              const float *place) {
-  const int G = p->grid_brush > 0 ? p->grid_brush : p->grid_wg;
+  // at least 512 workgroups (2048 waves): a wave per (leaf, slot) for up to 256 gathered
+  // leaves, so no wave runs two tiles' load chains back to back
+  const int G = std::max(512, p->grid_brush > 0 ? p->grid_brush : p->grid_wg);
   hipLaunchKernelGGL(k_brush_learned, dim3(G), dim3(LRN_THREADS), 0, p->stream, v, k, curve, sg_mlp_view(m),
                      place);
   return sg_check_hip(hipGetLastError(), "learned brush launch");
