@@ -571,26 +571,44 @@ __global__ __launch_bounds__(256) void k_post_learned(SgDevView d, SgDevView dq,
   const int lane = __lane_id();
   const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
   const int nw = gridDim.x * blockDim.x / SG_WAVE;
+  // Work items: three per gathered leaf, vertices [0, 64) and [64, 128) of its uniq
+  // range (then every 128 on), and its box refit, so a wave runs one 64-vertex chain.
+  // The first item's leaf range, and then its vertices' stamps and incidence ranges,
+  // are loaded beside the counters (speculatively: any index below L is inside the
+  // gathered-leaf list and any vertex of a listed range inside the mesh).
+  const int g0 = gw / 3, part0 = gw % 3;
+  const int2 vr0 = g0 < d.L ? d.gl_v[g0] : make_int2(0, 0);
   const int stamp = d.seq[0];
   const int na = (int)d.ctr[SG_CTR_NAFF];
   const int ng = (int)d.ctr[SG_CTR_NGATHER];
   const long long es = d.ctr[SG_CTR_LERR];
   const long long nood = d.ctr[SG_CTR_LOOD];
+  const int v0 = vr0.x + part0 * SG_WAVE + lane;
+  const bool pre = part0 < 2 && v0 < vr0.y;
+  const int st0 = pre ? d.vstamp[v0] : 0;
+  int2 vt0 = make_int2(0, 0);
+  if (pre) vt0 = make_int2(d.vt_off[v0], d.vt_off[v0 + 1]);
   // error estimate: alpha * RMS(|d| / r), alpha = held-out RMS error / RMS displacement
   const double rms = na > 0 ? sqrt((double)es / 4294967296.0 / (double)na) : 0.0;
   const bool accept = nood == 0 && (double)alpha * rms <= (double)tol;
   long long a = 0, b = 0, c = 0, nsh = 0;
-  for (int it = gw; it < 2 * ng; it += nw) {
-    const int g = it >> 1;
-    if (it & 1) {
+  for (int it = gw; it < 3 * ng; it += nw) {
+    const int g = it / 3, part = it % 3;
+    if (part == 2) {
       if (accept) leaf_refit(dq, d.gathered[g], d.gl_v[g], d.gl_s[g], nsh);
       continue;
     }
-    const int2 vr = d.gl_v[g];
-    for (int v = vr.x + lane; v < vr.y; v += SG_WAVE) {
-      if (d.vstamp[v] != stamp) continue;
+    int2 vr = vr0;
+    if (it != gw) vr = d.gl_v[g];
+    for (int v = vr.x + part * SG_WAVE + lane; v < vr.y; v += 2 * SG_WAVE) {
+      const bool first = it == gw && v == v0;
+      int st = st0;
+      if (!first) st = d.vstamp[v];
+      if (st != stamp) continue;
       if (accept) {
-        vertex_normal(dq, v, d.vt_off[v], d.vt_off[v + 1], 0, 0, a, b, c);  // corners from the shadow copy
+        int2 vt = vt0;
+        if (!first) vt = make_int2(d.vt_off[v], d.vt_off[v + 1]);
+        vertex_normal(dq, v, vt.x, vt.y, 0, 0, a, b, c);  // corners from the shadow copy
         d.px[v] = d.qx[v];
         d.py[v] = d.qy[v];
         d.pz[v] = d.qz[v];
