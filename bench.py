@@ -12,8 +12,9 @@ refit runs at the end of every stroke.  The stroke is captured once into a hipGr
 and launched per step (the mesh keeps evolving from step to step).
 
 value = vertex-updates/s = sum over timed dabs of |affected set| / wall time.
-Algorithmic HBM bytes per dab follow the byte model in DESIGN.md (counted exactly by
-an untimed replay of the same strokes with the kernels' statistics counters on).
+Algorithmic HBM bytes per dab follow the byte model in `dab_bytes` below (FP32 12 B
+per vec3, int32 topology, each item counted once per dab), counted exactly by an
+untimed replay of the same strokes with the kernels' statistics counters on.
 The CPU baseline is the C oracle (OpenMP over gathered leaves) on a bounded sample of
 the same stroke on the same mesh.
 """
