@@ -146,6 +146,24 @@ __device__ __forceinline__ long long undo_save_leaf(const SgDevView &d, int leaf
   SgUndoCtl *u = d.undo;
   const int lane = lane_id();
   const int n = vr.y - vr.x;
+  // the leaf's first 128 vertices and its two boxes are loaded while lane 0 checks the
+  // stamp and reserves the record, so the copy below only stores them
+  float c0[2][6];
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    const int i = lane + j * SG_WAVE;
+    if (i < n) {
+      const int v = vr.x + i;
+      c0[j][0] = sx[v];
+      c0[j][1] = sy[v];
+      c0[j][2] = sz[v];
+      c0[j][3] = d.nx[v];
+      c0[j][4] = d.ny[v];
+      c0[j][5] = d.nz[v];
+    }
+  }
+  const float bbv = lane < 16 ? (lane < 8 ? d.leaf_bb[8 * (size_t)leaf + lane] : d.leaf_obb[8 * (size_t)leaf + lane - 8])
+                              : 0.0f;
   int e = -1;
   unsigned long long vb = 0;
   if (lane == 0 && u->stamp[leaf] != u->step) {
@@ -164,7 +182,20 @@ __device__ __forceinline__ long long undo_save_leaf(const SgDevView &d, int leaf
   if (e < 0) return -1;
   vb = ((unsigned long long)(unsigned)__shfl((int)(unsigned)(vb >> 32), 0) << 32) |
        (unsigned)__shfl((int)(unsigned)vb, 0);
-  for (int i = lane; i < n; i += SG_WAVE) {
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    const int i = lane + j * SG_WAVE;
+    if (i < n) {
+      const size_t o = 3 * (size_t)(vb + i);
+      u->pos[o] = c0[j][0];
+      u->pos[o + 1] = c0[j][1];
+      u->pos[o + 2] = c0[j][2];
+      u->nor[o] = c0[j][3];
+      u->nor[o + 1] = c0[j][4];
+      u->nor[o + 2] = c0[j][5];
+    }
+  }
+  for (int i = lane + 2 * SG_WAVE; i < n; i += SG_WAVE) {
     const int v = vr.x + i;
     const size_t o = 3 * (size_t)(vb + i);
     u->pos[o] = sx[v];
@@ -174,7 +205,7 @@ __device__ __forceinline__ long long undo_save_leaf(const SgDevView &d, int leaf
     u->nor[o + 1] = d.ny[v];
     u->nor[o + 2] = d.nz[v];
   }
-  if (lane < 16) u->bbs[16 * (size_t)e + lane] = lane < 8 ? d.leaf_bb[8 * (size_t)leaf + lane] : d.leaf_obb[8 * (size_t)leaf + lane - 8];
+  if (lane < 16) u->bbs[16 * (size_t)e + lane] = bbv;
   return (long long)vb;
 }
 
