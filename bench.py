@@ -9,7 +9,8 @@ seeded great circle (spacing 10% of the radius), PBVH leaves of <= 128 tris.  Ea
 runs the whole hot path on the GPU: sphere search, falloff and strength, displacement,
 normals of the affected vertices, leaf and search-level bounds; the internal-node
 refit runs at the end of every stroke.  The stroke is captured once into a hipGraph
-and launched per step (the mesh keeps evolving from step to step).
+and launched per step; every step starts from the same pristine mesh (a device
+snapshot restored, untimed, before each step).
 
 value = vertex-updates/s = sum over timed dabs of |affected set| / wall time.
 Algorithmic HBM bytes per dab follow the byte model in `dab_bytes` below (FP32 12 B
@@ -32,6 +33,8 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
+from dune_sculpt_amd.workloads import batch_meshes, batch_table, plain_stroke, smooth_stroke  # noqa: E402
+
 HBM_PEAK_GBPS = 8000.0
 # HBM bytes per dab of the 1000-dab stroke on the 10.5M-vertex mesh, from two
 # rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) summed over the dab kernels, FETCH_SIZE
@@ -43,23 +46,6 @@ METRIC = "vertex-updates/s + HBM GB/s; learned vs plain Smooth dabs/s at ≤1e-3
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
-
-
-def surface_centres(sg, centres, mesh_seed=1, amplitude=0.02):
-    """Project great-circle points onto the noisy surface: the mesh was displaced
-    radially by the same seeded noise field, so the surface point above a unit
-    direction u is u displaced by that field."""
-    u = centres / np.linalg.norm(centres, axis=1, keepdims=True)
-    return sg.noise_displace(u.astype(np.float32), u.astype(np.float32), mesh_seed, amplitude)
-
-
-def make_c2_stroke(sg, n_dabs=1000, seed=3):
-    rng = np.random.default_rng(seed)
-    radii = rng.uniform(0.02, 0.1, n_dabs).astype(np.float32)
-    strengths = rng.uniform(0.2, 1.0, n_dabs).astype(np.float32)
-    centres = surface_centres(sg, sg.great_circle_centres(seed + 1, radii))
-    tools = np.array([sg.TOOL_DRAW, sg.TOOL_INFLATE, sg.TOOL_SMOOTH, sg.TOOL_GRAB] * (n_dabs // 4 + 1))[:n_dabs]
-    return sg.stroke_config(tools, centres, radii, strengths, smooth_iterations=4), tools
 
 
 def dab_bytes(sg, tool, k, c):
@@ -101,11 +87,7 @@ def run_learned(sg, A, W, args):
     m = sg.MLP.load(L.MODEL_PATH)
     alpha, tol = m.decision()
     n = args.learned_dabs
-    rng = np.random.default_rng(5)
-    radii = np.full(n, 0.05, np.float32)
-    strengths = rng.uniform(0.2, 1.0, n).astype(np.float32)
-    centres = surface_centres(sg, sg.great_circle_centres(6, radii))
-    plain = sg.stroke_config(np.full(n, sg.TOOL_SMOOTH), centres, radii, strengths, smooth_iterations=4)
+    plain = smooth_stroke(sg, n)
     learned = L.learned_dabs(plain)
     A.set_learned(m)
     out = {"dabs": n, "radius": 0.05, "k": 4, "model": L.MODEL_PATH.name, "alpha": alpha, "tol": tol}
@@ -149,67 +131,6 @@ def run_learned(sg, A, W, args):
     m.close()
     log(f"learned smooth: {out}")
     return out
-
-
-def _path_centres(sg, spec, n, radii, rng):
-    """Dab centres along a seeded path on the mesh's noisy surface, arc step 0.1 r: a
-    great circle on an icosphere, a ring at a seeded tube angle on a torus (major /
-    minor radius 1.0 / 0.3).  Each point of the smooth surface is displaced along its
-    normal by the mesh's own seeded noise field, as the mesh vertices were."""
-    kind, seed, amplitude = spec
-    if kind == "sphere":
-        axis = rng.normal(size=3)
-        axis /= np.linalg.norm(axis)
-        e1 = np.cross(axis, [1.0, 0.0, 0.0] if abs(axis[0]) < 0.9 else [0.0, 1.0, 0.0])
-        e1 /= np.linalg.norm(e1)
-        e2 = np.cross(axis, e1)
-        t = np.cumsum(0.1 * radii) - 0.1 * radii[0] + rng.uniform(0, 2 * np.pi)
-        pts = np.cos(t)[:, None] * e1 + np.sin(t)[:, None] * e2
-        nrm = pts
-    else:
-        v0 = rng.uniform(0, 2 * np.pi)
-        ring = 1.0 + 0.3 * np.cos(v0)
-        u = np.cumsum(0.1 * radii / ring) + rng.uniform(0, 2 * np.pi)
-        pts = np.stack([ring * np.cos(u), ring * np.sin(u), np.full(n, 0.3 * np.sin(v0))], -1)
-        nrm = np.stack([np.cos(v0) * np.cos(u), np.cos(v0) * np.sin(u), np.full(n, np.sin(v0))], -1)
-    return sg.noise_displace(pts.astype(np.float32), nrm.astype(np.float32), seed, amplitude)
-
-
-def batch_meshes(sg, n, seed, vmin, vmax, amplitude, log_uniform=True, levels=(4, 8)):
-    """n seeded meshes with vertex counts drawn from [vmin, vmax] (log-uniform or
-    uniform), alternating between the nearest icosphere level (within `levels`) and a
-    torus with n_u / n_v = 10 / 3 (near-equilateral triangles)."""
-    rng = np.random.default_rng(seed)
-    meshes, specs = [], []
-    for i in range(n):
-        V = float(np.exp(rng.uniform(np.log(vmin), np.log(vmax)))) if log_uniform else float(rng.uniform(vmin, vmax))
-        if i % 2 == 0:
-            level = int(np.clip(np.round(np.log(V / 10.0) / np.log(4.0)), levels[0], levels[1]))
-            meshes.append(sg.noisy_icosphere(level, seed=1000 + i, amplitude=amplitude))
-            specs.append(("sphere", 1000 + i, amplitude))
-        else:
-            nv = max(3, int(round(np.sqrt(V * 3.0 / 10.0))))
-            nu = max(3, int(round(V / nv)))
-            meshes.append(sg.noisy_torus(nu, nv, seed=1000 + i, amplitude=amplitude))
-            specs.append(("torus", 1000 + i, amplitude))
-    return meshes, specs
-
-
-def batch_table(sg, meshes, specs, steps, seed, tools, rmin, rmax, learned_smooth=False):
-    """table[step][mesh]: mesh m's dab `step` cycles `tools` (offset by m), r ~ U[rmin,
-    rmax], s ~ U[0.2, 1.0], Smooth with k = 4, along the mesh's path."""
-    rng = np.random.default_rng(seed)
-    table = [[None] * len(meshes) for _ in range(steps)]
-    for m, spec in enumerate(specs):
-        radii = rng.uniform(rmin, rmax, steps)
-        strengths = rng.uniform(0.2, 1.0, steps)
-        centres = _path_centres(sg, spec, steps, radii, rng)
-        for s in range(steps):
-            tool = tools[(s + m) % len(tools)]
-            flag = sg.SMOOTH_LEARNED if (learned_smooth and tool == sg.TOOL_SMOOTH) else 0
-            table[s][m] = sg.make_dab(tool=tool, location=centres[s], radius=float(radii[s]),
-                                      strength=float(strengths[s]), smooth_iterations=4, flag=flag)
-    return table
 
 
 def batch_bytes(t):
@@ -705,7 +626,7 @@ def main():
     t = time.perf_counter()
     co, tris = sg.noisy_icosphere(args.level, seed=1, amplitude=0.02)
     log(f"mesh icosphere-{args.level}: {co.shape[0]} verts {tris.shape[0]} tris in {time.perf_counter() - t:.1f}s")
-    dabs, tools = make_c2_stroke(sg, args.dabs)
+    dabs, tools = plain_stroke(sg, args.dabs)
 
     t = time.perf_counter()
     A = sg.PBVH(co, tris, leaf_limit=128)

@@ -287,6 +287,22 @@ void BKE_pbvh_build_mesh(PBVH *p, Mesh *mesh, const MPoly *mpoly, const MLoop *m
           for (int k = 0; k < 3; k++) p->poly_verts[p->poly_off[q] + k] = p->tris[3 * (size_t)first[q] + k];
     }
   }
+  // Polygon normal rules are needed when some looptri is not exactly its polygon's
+  // three corners in loop order (quads, n-gons); a triangle mesh keeps the tri paths,
+  // which give the same bits there.
+  p->poly_mode = false;
+  if (mpoly) {
+    for (int t = 0; t < looptri_num && !p->poly_mode; t++) {
+      const int q = (int)looptri[t].poly;
+      const int b = p->poly_off[q];
+      if (p->poly_off[q + 1] - b != 3) {
+        p->poly_mode = true;
+        break;
+      }
+      for (int j = 0; j < 3; j++)
+        if (p->tris[3 * (size_t)t + j] != p->poly_verts[b + j]) p->poly_mode = true;
+    }
+  }
   p->tri_mat.clear();
   if (mpoly && looptri_num > 0) {
     std::vector<int> key(looptri_num);

@@ -19,6 +19,8 @@
 //   k_bstep_end      cut boxes rebuilt; totals, decisions, counters cleared, step advanced
 #include <hip/hip_runtime.h>
 
+#include <thread>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -621,27 +623,55 @@ SgBatch *sg_batch_new(int n_meshes, const float *const *co, const int *const *tr
   b->tbase.assign(1, 0);
   long long vvbase = 0, shbase = 0;
   int rc = 0;
-  for (int m = 0; m < n_meshes && !rc; m++) {
-    const int V = totvert[m], T = tottri[m];
-    for (long long q = 0; q < 3ll * T; q++) {
-      if (tris[m][q] < 0 || tris[m][q] >= V) {
-        sg_set_error("sg_batch_new: mesh %d has a corner out of range", m);
+  // Host builds (tree, leaf-major layout, search cut) of a wave of meshes run on
+  // threads side by side; the wave's meshes are then appended to the arena in order.
+  struct MeshHost {
+    PBVH *p = nullptr;
+    SgLayout h;
+    bool bad = false;
+  };
+  const int wave = 8;
+  std::vector<MeshHost> hosts(wave);
+  for (int w0 = 0; w0 < n_meshes && !rc; w0 += wave) {
+    const int nw = std::min(wave, n_meshes - w0);
+    std::vector<std::thread> th;
+    for (int j = 0; j < nw; j++) {
+      th.emplace_back([&, j]() {
+        const int m = w0 + j;
+        MeshHost &mh = hosts[j];
+        const int V = totvert[m], T = tottri[m];
+        mh.bad = false;
+        for (long long q = 0; q < 3ll * T; q++) {
+          if (tris[m][q] < 0 || tris[m][q] >= V) {
+            mh.bad = true;
+            return;
+          }
+        }
+        PBVH *p = new PBVH();
+        p->no_device = true;
+        p->leaf_limit = leaf_limit > 0 ? leaf_limit : 10000;
+        p->totvert = V;
+        p->totprim = T;
+        p->tris.assign(tris[m], tris[m] + 3 * (size_t)T);
+        sg_build_tree(p, co[m], p->tris.data(), T, V);
+        sg_build_layout(p, co[m], mh.h);
+        std::vector<int> lbeg, lend, height, leaf_cut;
+        sg_build_cut(p, lbeg, lend, height, leaf_cut);
+        mh.p = p;
+      });
+    }
+    for (auto &t : th) t.join();
+    for (int j = 0; j < nw && !rc; j++) {
+      if (hosts[j].bad) {
+        sg_set_error("sg_batch_new: mesh %d has a corner out of range", w0 + j);
         rc = -1;
-        break;
       }
     }
-    if (rc) break;
-    PBVH *p = new PBVH();
-    p->no_device = true;
-    p->leaf_limit = leaf_limit > 0 ? leaf_limit : 10000;
-    p->totvert = V;
-    p->totprim = T;
-    p->tris.assign(tris[m], tris[m] + 3 * (size_t)T);
-    sg_build_tree(p, co[m], p->tris.data(), T, V);
-    SgLayout h;
-    sg_build_layout(p, co[m], h);
-    std::vector<int> lbeg, lend, height, leaf_cut;
-    sg_build_cut(p, lbeg, lend, height, leaf_cut);
+    for (int j = 0; j < nw && !rc; j++) {
+    const int m = w0 + j;
+    const int V = totvert[m], T = tottri[m];
+    PBVH *p = hosts[j].p;
+    SgLayout &h = hosts[j].h;
     const long long vb = b->vbase.back(), tb = b->tbase.back(), lb = b->lbase.back();
     const int L = h.L, C = (int)p->cut.node.size();
     // vertices
@@ -705,6 +735,14 @@ SgBatch *sg_batch_new(int n_meshes, const float *const *co, const int *const *tr
     b->lbase.push_back(lb + L);
     b->cbase.push_back(b->cbase.back() + C);
     BKE_pbvh_free(p);
+    hosts[j].p = nullptr;
+    hosts[j].h = SgLayout();
+    }
+    for (int j = 0; j < nw; j++) {  // what an error left unappended
+      if (hosts[j].p) BKE_pbvh_free(hosts[j].p);
+      hosts[j].p = nullptr;
+      hosts[j].h = SgLayout();
+    }
   }
   if (!rc) {
     const int sentinel[2] = {(int)(3 * b->tbase.back()), (int)vvbase};

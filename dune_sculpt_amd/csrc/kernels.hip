@@ -917,6 +917,38 @@ __global__ __launch_bounds__(256) void k_full_normals(SgDevView d, int mode) {
   const int V = d.V;
   for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    if (mode == 0 && d.vc_off) {
+      // a mesh with polygons: the corners of v's polygons, in polygon order
+      for (int q = d.vc_off[v]; q < d.vc_off[v + 1]; q++) {
+        const int2 pc = d.vc[q];
+        const int b = d.poly_off[pc.x], n = d.poly_off[pc.x + 1] - b;
+        // Newell normal of the polygon, from its last corner
+        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+        int u = d.poly_v[b + n - 1];
+        float cx = d.px[u], cy = d.py[u], cz = d.pz[u];
+        for (int k = 0; k < n; k++) {
+          u = d.poly_v[b + k];
+          const float nx = d.px[u], ny = d.py[u], nz = d.pz[u];
+          fx += (cy - ny) * (cz + nz);
+          fy += (cz - nz) * (cx + nx);
+          fz += (cx - nx) * (cy + ny);
+          cx = nx;
+          cy = ny;
+          cz = nz;
+        }
+        if (sg_normalize(fx, fy, fz) == 0.0f) fz = 1.0f;
+        // the angle at corner pc.y between its two polygon edges
+        const int up = d.poly_v[b + (pc.y + n - 1) % n], uc = d.poly_v[b + pc.y], un = d.poly_v[b + (pc.y + 1) % n];
+        float epx = d.px[up] - d.px[uc], epy = d.py[up] - d.py[uc], epz = d.pz[up] - d.pz[uc];
+        float enx = d.px[uc] - d.px[un], eny = d.py[uc] - d.py[un], enz = d.pz[uc] - d.pz[un];
+        sg_normalize(epx, epy, epz);
+        sg_normalize(enx, eny, enz);
+        const float fac = sg_saacos(-(epx * enx + epy * eny + epz * enz));
+        ax += fx * fac;
+        ay += fy * fac;
+        az += fz * fac;
+      }
+    } else
     for (int q = d.vt_off[v]; q < d.vt_off[v + 1]; q++) {
       const int4 c = d.vt4[q];
       const int ci = (int)((unsigned)c.w >> 30);
@@ -1054,6 +1086,11 @@ SgDevView sg_view_of(SgDev &d, bool respect_hide, bool halt_live) {
   v.vt_off = d.vt_off;
   v.vv_off = d.vv_off;
   v.vv_idx = d.vv_idx;
+  v.poly_off = d.poly_off;
+  v.poly_v = d.poly_v;
+  v.poly_of_tri = d.poly_of_tri;
+  v.vc_off = d.vc_off;
+  v.vc = d.vc;
   v.leaf_vbeg = d.leaf_vbeg;
   v.leaf_sh_off = d.leaf_sh_off;
   v.leaf_sh = d.leaf_sh;

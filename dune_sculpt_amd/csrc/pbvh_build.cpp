@@ -378,11 +378,40 @@ void sg_build_layout(PBVH *p, const float *co, SgLayout &h) {
       }
     }
   }
-  // vertex -> vertex CSR: distinct edge neighbours, ascending original id
+  // polygon meshes: corners in new ids and each vertex's (polygon, corner) list
+  if (p->poly_mode) {
+    const int P = (int)p->poly_off.size() - 1;
+    h.poly_v.resize(p->poly_verts.size());
+    for (size_t k = 0; k < h.poly_v.size(); k++) h.poly_v[k] = p->perm_new_of_old[p->poly_verts[k]];
+    h.vc_off.assign(V + 1, 0);
+    for (int k : h.poly_v) h.vc_off[k + 1]++;
+    for (int v = 0; v < V; v++) h.vc_off[v + 1] += h.vc_off[v];
+    h.vc.resize((size_t)std::max(h.vc_off[V], 1));
+    std::vector<int> fill(h.vc_off.begin(), h.vc_off.end() - 1);
+    for (int q = 0; q < P; q++)
+      for (int k = p->poly_off[q]; k < p->poly_off[q + 1]; k++)
+        h.vc[fill[h.poly_v[k]]++] = make_int2(q, k - p->poly_off[q]);
+  }
+  // vertex -> vertex CSR: distinct edge neighbours, ascending original id.  The edges
+  // are the mesh's: the triangles' edges, or for polygons the polygon edges (the
+  // corners before and after the vertex in each of its polygons).
   std::vector<int> vv_cnt(V, 0);
   std::vector<std::vector<int>> tmp_tls;
   auto neighbours = [&](int v, std::vector<int> &buf) {
     buf.clear();
+    if (p->poly_mode) {
+      for (int q = h.vc_off[v]; q < h.vc_off[v + 1]; q++) {
+        const int2 pc = h.vc[q];
+        const int b = p->poly_off[pc.x], n = p->poly_off[pc.x + 1] - b;
+        const int up = p->poly_verts[b + (pc.y + n - 1) % n], un = p->poly_verts[b + (pc.y + 1) % n];
+        const int self = p->perm_old_of_new[v];
+        if (up != self) buf.push_back(up);
+        if (un != self) buf.push_back(un);
+      }
+      std::sort(buf.begin(), buf.end());
+      buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
+      return;
+    }
     for (int q = vt_off[v]; q < vt_off[v + 1]; q++) {
       const int t = vt_idx[q] & 0x3fffffff;
       const int4 c = tri[t];
@@ -706,6 +735,19 @@ void sg_build_device(PBVH *p, const float *co) {
   alloc_fill(&d.vstamp, V, 0xFF, s);  // -1: never a dab's stamp (stamps start at 1)
   alloc_fill(&d.err, 4, 0, s);
   upload(&d.vperm_old, p->perm_old_of_new.data(), V, s);
+  if (p->poly_mode) {
+    // polygons for the polygon normal rules: corners (new ids), the polygon of each
+    // tri in leaf order, each vertex's (polygon, corner) list
+    const int P = (int)p->poly_off.size() - 1;
+    std::vector<int> pt((size_t)std::max(p->totprim, 1));
+    for (int i = 0; i < p->totprim; i++) pt[i] = (int)p->looptri[p->prim_indices[i]].poly;
+    upload(&d.poly_off, p->poly_off.data(), P + 1, s);
+    upload(&d.poly_v, h.poly_v.data(), std::max<size_t>(h.poly_v.size(), 1), s);
+    upload(&d.poly_of_tri, pt.data(), pt.size(), s);
+    upload(&d.vc_off, h.vc_off.data(), V + 1, s);
+    upload(&d.vc, h.vc.data(), h.vc.size(), s);
+    hipStreamSynchronize(s);  // the host arrays are freed on return
+  }
   hipStreamSynchronize(s);
 }
 
@@ -723,7 +765,8 @@ void sg_free_device(PBVH *p) {
                   d.ray_in, d.ray_out, d.place, d.place_rays, d.rl_list, d.rl_args,
                   d.draw_dirty, d.draw_tri, d.draw_off, d.draw_leaf, d.draw_fset, d.draw_poly_off,
                   d.draw_poly_v, d.draw_vbo, d.undo, d.undo_stamp, d.undo_slot, d.place_pair,
-                  d.prox_dev, d.prox_tab, d.prox_leaves, d.prox_off};
+                  d.prox_dev, d.prox_tab, d.prox_leaves, d.prox_off, d.poly_off, d.poly_v,
+                  d.poly_of_tri, d.vc_off, d.vc};
   for (void *q : ptrs) {
     if (q) hipFree(q);
   }

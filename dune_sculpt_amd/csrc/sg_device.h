@@ -54,6 +54,61 @@ __device__ __forceinline__ void sg_tri_normal(float ax, float ay, float az, floa
   sg_normalize(nx, ny, nz);
 }
 
+// BKE_mesh_calc_poly_normal (mesh_evaluate.c:62-84) of polygon q, corners read from
+// the position arrays (x, y, z): more than 4 corners Newell's method from the last
+// corner (mesh_calc_ngon_normal :39-60, add_newell_cross_v3_v3v3
+// math_vector_inline.c:961-966; a zero result is +z), 3 normal_tri_v3, 4 normal_quad_v3
+// (math_geom.cc:48-68: the cross product of the diagonals), fewer +z.
+__device__ __forceinline__ void sg_poly_normal(const int *__restrict__ poly_off, const int *__restrict__ poly_v,
+                                               const float *__restrict__ x, const float *__restrict__ y,
+                                               const float *__restrict__ z, int q, float &nx, float &ny,
+                                               float &nz) {
+  const int b = poly_off[q], n = poly_off[q + 1] - b;
+  if (n == 3 || n == 4) {
+    int c[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) c[k] = k < n ? poly_v[b + k] : 0;
+    float P[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      if (k < n) {
+        P[k][0] = x[c[k]];
+        P[k][1] = y[c[k]];
+        P[k][2] = z[c[k]];
+      }
+    }
+    if (n == 3) {
+      sg_tri_normal(P[0][0], P[0][1], P[0][2], P[1][0], P[1][1], P[1][2], P[2][0], P[2][1], P[2][2], nx, ny, nz);
+    } else {
+      const float n10 = P[0][0] - P[2][0], n11 = P[0][1] - P[2][1], n12 = P[0][2] - P[2][2];
+      const float n20 = P[1][0] - P[3][0], n21 = P[1][1] - P[3][1], n22 = P[1][2] - P[3][2];
+      nx = n11 * n22 - n12 * n21;
+      ny = n12 * n20 - n10 * n22;
+      nz = n10 * n21 - n11 * n20;
+      sg_normalize(nx, ny, nz);
+    }
+  } else if (n > 4) {
+    nx = ny = nz = 0.0f;
+    int u = poly_v[b + n - 1];
+    float ax = x[u], ay = y[u], az = z[u];
+    for (int k = 0; k < n; k++) {
+      u = poly_v[b + k];
+      const float cx = x[u], cy = y[u], cz = z[u];
+      nx += (ay - cy) * (az + cz);
+      ny += (az - cz) * (ax + cx);
+      nz += (ax - cx) * (ay + cy);
+      ax = cx;
+      ay = cy;
+      az = cz;
+    }
+    if (sg_normalize(nx, ny, nz) == 0.0f) nz = 1.0f;
+  } else {
+    nx = 0.0f;
+    ny = 0.0f;
+    nz = 1.0f;
+  }
+}
+
 // Ordered bounding-box combine: `b` covers later elements than `a`.  Equal to
 // BB_expand_with_bb(a, b) (pbvh.c:66-72), which keeps the later value on ties.
 struct BB6 {
@@ -397,6 +452,24 @@ __device__ __forceinline__ void nbr_sum(const int *__restrict__ vv_idx, int b, i
 __device__ __forceinline__ void vertex_normal(const SgDevView &d, int v, int b, int e, int stats, int stamp,
                                               long long &ntri, long long &nfa, long long &nvfa) {
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
+  if (d.poly_of_tri) {
+    // a mesh with polygons: each incident looptri adds its polygon's normal
+    // (pbvh.c:1041-1058: BKE_mesh_calc_poly_normal once per looptri)
+    for (int q = b; q < e; q++) {
+      const int t = d.vt4[q].w & 0x3fffffff;
+      float fx, fy, fz;
+      sg_poly_normal(d.poly_off, d.poly_v, d.px, d.py, d.pz, d.poly_of_tri[t], fx, fy, fz);
+      ax = ax + fx;
+      ay = ay + fy;
+      az = az + fz;
+    }
+    ntri += e - b;
+    sg_normalize(ax, ay, az);
+    d.nx[v] = ax;
+    d.ny[v] = ay;
+    d.nz[v] = az;
+    return;
+  }
   for (int q0 = b; q0 < e; q0 += 8) {
     int4 c[8];
 #pragma unroll

@@ -91,6 +91,11 @@ struct SgDev {
   int *halt = nullptr;          // [4] set by a rejected learned dab: its serial + 1
   int *seq = nullptr;           // [1] dab counter (each dab's search adds one): the learned dab's stamp
   int *vstamp = nullptr;        // [V] stamp of the last learned dab that affected the vertex
+  // polygons of a mesh that has any non-triangle polygon (null for triangle meshes):
+  // corner offsets and corner vertices (new ids), the polygon of each tri (leaf order),
+  // and each vertex's polygon corners (polygon, corner) in ascending polygon order
+  int *poly_off = nullptr, *poly_v = nullptr, *poly_of_tri = nullptr, *vc_off = nullptr;
+  int2 *vc = nullptr;
   int *vhide = nullptr;         // [V] or null
   uint8_t *vmark = nullptr;     // [V] host-driven normal update marks
   // per-dab scratch
@@ -190,6 +195,7 @@ struct PBVH {
   std::vector<int> poly_off, poly_verts;  // polygons: corner lists (original vertex ids)
   std::vector<uint8_t> poly_smooth;       // ME_SMOOTH per polygon
   std::vector<int> poly_mat;              // mat_nr per polygon
+  bool poly_mode = false;                 // some polygon is not the triangle of its looptri: polygon normal rules
   std::vector<uint8_t> vhide;
   SgCut cut;
   MVert *host_verts = nullptr;       // caller's array (borrowed; duplicated on deform)
@@ -246,6 +252,10 @@ struct SgLayout {
   std::vector<int> vt_off, vv_off, vv_idx;        // [V+1], [V+1], neighbours
   std::vector<int> leaf_vbeg, leaf_sh_off, leaf_sh, leaf_flags;
   std::vector<float> leaf_bb, leaf_obb;           // [L*8]
+  // polygon meshes only: corner vertices (new ids), each vertex's (polygon, corner)
+  // list in ascending polygon order
+  std::vector<int> poly_v, vc_off;
+  std::vector<int2> vc;
 };
 
 // host build (pbvh_build.cpp)

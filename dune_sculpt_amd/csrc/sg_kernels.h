@@ -49,6 +49,10 @@ struct SgDevView {
   float *px, *py, *pz, *qx, *qy, *qz, *nx, *ny, *nz, *mask;
   const int4 *vt4;
   const int *vt_off, *vv_off, *vv_idx;
+  // polygons (null for triangle meshes): corner offsets, corner vertices, polygon of each
+  // tri, each vertex's (polygon, corner) list
+  const int *poly_off, *poly_v, *poly_of_tri, *vc_off;
+  const int2 *vc;
   const int *leaf_vbeg, *leaf_sh_off, *leaf_sh, *leaf_flags, *leaf_cut;
   float *leaf_bb, *leaf_obb, *node_bb, *node_obb, *cut_bb, *cut_obb;
   const int *cut_node, *cut_lbeg, *cut_lend, *cut_flags, *sub_off;

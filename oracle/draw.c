@@ -19,8 +19,10 @@
  *   overlay colour  BKE_paint_face_set_overlay_color_get (kernel/intern/paint.c:2219-2235),
  *                   hsv_to_rgb (lib/intern/math_color.c:6-22), unit_float_to_uchar_clamp
  *                   (lib/intern/math_base_inline.c:678-682); BLI_hash_int_01 is absent
- *                   (BLI_hash.h is not in the tree): restated from Blender's published
- *                   Jenkins lookup3 final mix -- parity unpinned against the reference.
+ *                   (BLI_hash.h is not in the tree): it is Bob Jenkins' lookup3 hashword
+ *                   of (k, 0) with initial value 13, restated from lookup3's published
+ *                   source and pinned by lookup3's published hash vectors
+ *                   (tests/test_oracle_draw.py).
  *
  * The reference leaves the padding bytes and, without vertex colours, the `ac` bytes
  * uninitialised; here they are written as zero, as the GPU writes them.
@@ -113,14 +115,12 @@ static inline void to_short3(short out[3], const float in[3])
   out[2] = (short)(in[2] * 32767.0f);
 }
 
-/* BLI_hash_int_2d(k, 0) / 0xFFFFFFFF: lookup3's final mix on a = b = c = 0xdeadbeef +
- * (2 << 2) + 13, a += k.  (Published algorithm; the header that holds it is absent.) */
+/* lookup3's final mix (Bob Jenkins, lookup3.c `final`), exported so the tests can run
+ * lookup3's published hash vectors through it. */
 static inline uint32_t h_rot(uint32_t x, int k) { return (x << k) | (x >> (32 - k)); }
-static float hash_int_01(uint32_t k)
+void orc_lookup3_final(uint32_t abc[3])
 {
-  uint32_t a, b, c;
-  a = b = c = 0xdeadbeefu + (2u << 2) + 13u;
-  a += k;
+  uint32_t a = abc[0], b = abc[1], c = abc[2];
   c ^= b;
   c -= h_rot(b, 14);
   a ^= c;
@@ -135,7 +135,22 @@ static float hash_int_01(uint32_t k)
   b -= h_rot(a, 14);
   c ^= b;
   c -= h_rot(b, 24);
-  return (float)c * (1.0f / (float)0xFFFFFFFFu);
+  abc[0] = a;
+  abc[1] = b;
+  abc[2] = c;
+}
+
+/* BLI_hash_int_01(k) = BLI_hash_int_2d(k, 0) / 0xFFFFFFFF, where BLI_hash_int_2d is
+ * lookup3's hashword of the two words (k, 0) with initial value 13: a = b = c =
+ * 0xdeadbeef + (2 << 2) + 13, a += k, final.  (Published algorithm; the header that
+ * holds it, BLI_hash.h, is absent from the reference.) */
+float orc_hash_int_01(uint32_t k)
+{
+  uint32_t abc[3];
+  abc[0] = abc[1] = abc[2] = 0xdeadbeefu + (2u << 2) + 13u;
+  abc[0] += k;
+  orc_lookup3_final(abc);
+  return (float)abc[2] * (1.0f / (float)0xFFFFFFFFu);
 }
 
 static inline unsigned char unit_to_uchar(float v)
@@ -148,8 +163,8 @@ void orc_face_set_color(int face_set, int seed, unsigned char out[3])
   const int fs = abs(face_set);
   float hue = 0.618033988749895f * (float)(fs + (seed % 10));
   hue = hue - floorf(hue);
-  const float sat = hash_int_01((uint32_t)(fs + seed + 1));
-  const float val = hash_int_01((uint32_t)(fs + seed + 2));
+  const float sat = orc_hash_int_01((uint32_t)(fs + seed + 1));
+  const float val = orc_hash_int_01((uint32_t)(fs + seed + 2));
   const float s = 0.6f + (sat * 0.25f), v = 1.0f - (val * 0.35f);
   float nr = fabsf(hue * 6.0f - 3.0f) - 1.0f;
   float ng = 2.0f - fabsf(hue * 6.0f - 2.0f);

@@ -441,22 +441,109 @@ static void build_csr(OrcPBVH *o)
   free(fill);
 }
 
+/* BKE_mesh_calc_poly_normal (mesh_evaluate.c:62-84): more than 4 corners ->
+ * mesh_calc_ngon_normal (:39-60, Newell's method from the last corner,
+ * add_newell_cross_v3_v3v3 math_vector_inline.c:961-966, zero -> +z); 3 ->
+ * normal_tri_v3; 4 -> normal_quad_v3 (math_geom.cc:48-68, cross of the diagonals);
+ * fewer -> +z.  `c` holds the polygon's corner positions in loop order. */
+static void poly_normal(const float *co, const int *corners, int n, float r[3])
+{
+  if (n > 4) {
+    r[0] = r[1] = r[2] = 0.0f;
+    const float *v_prev = co + 3 * corners[n - 1];
+    for (int i = 0; i < n; i++) {
+      const float *v_curr = co + 3 * corners[i];
+      r[0] += (v_prev[1] - v_curr[1]) * (v_prev[2] + v_curr[2]);
+      r[1] += (v_prev[2] - v_curr[2]) * (v_prev[0] + v_curr[0]);
+      r[2] += (v_prev[0] - v_curr[0]) * (v_prev[1] + v_curr[1]);
+      v_prev = v_curr;
+    }
+    if (o_normalize(r) == 0.0f) {
+      r[2] = 1.0f;
+    }
+  }
+  else if (n == 3) {
+    orc_tri_normal(co + 3 * corners[0], co + 3 * corners[1], co + 3 * corners[2], r);
+  }
+  else if (n == 4) {
+    const float *v1 = co + 3 * corners[0], *v2 = co + 3 * corners[1], *v3 = co + 3 * corners[2],
+                *v4 = co + 3 * corners[3];
+    float n1[3], n2[3];
+    n1[0] = v1[0] - v3[0];
+    n1[1] = v1[1] - v3[1];
+    n1[2] = v1[2] - v3[2];
+    n2[0] = v2[0] - v4[0];
+    n2[1] = v2[1] - v4[1];
+    n2[2] = v2[2] - v4[2];
+    r[0] = n1[1] * n2[2] - n1[2] * n2[1];
+    r[1] = n1[2] * n2[0] - n1[0] * n2[2];
+    r[2] = n1[0] * n2[1] - n1[1] * n2[0];
+    o_normalize(r);
+  }
+  else {
+    r[0] = 0.0f;
+    r[1] = 0.0f;
+    r[2] = 1.0f;
+  }
+}
+
+/* The same in FP64 (the replay). */
+static void poly_normal64(const double *co, const int *corners, int n, double r[3])
+{
+  if (n > 4) {
+    r[0] = r[1] = r[2] = 0.0;
+    const double *v_prev = co + 3 * corners[n - 1];
+    for (int i = 0; i < n; i++) {
+      const double *v_curr = co + 3 * corners[i];
+      r[0] += (v_prev[1] - v_curr[1]) * (v_prev[2] + v_curr[2]);
+      r[1] += (v_prev[2] - v_curr[2]) * (v_prev[0] + v_curr[0]);
+      r[2] += (v_prev[0] - v_curr[0]) * (v_prev[1] + v_curr[1]);
+      v_prev = v_curr;
+    }
+    if (o_normalize64(r) == 0.0) {
+      r[2] = 1.0;
+    }
+  }
+  else if (n == 3) {
+    tri_normal64(co + 3 * corners[0], co + 3 * corners[1], co + 3 * corners[2], r);
+  }
+  else if (n == 4) {
+    const double *v1 = co + 3 * corners[0], *v2 = co + 3 * corners[1], *v3 = co + 3 * corners[2],
+                 *v4 = co + 3 * corners[3];
+    const double n1[3] = {v1[0] - v3[0], v1[1] - v3[1], v1[2] - v3[2]};
+    const double n2[3] = {v2[0] - v4[0], v2[1] - v4[1], v2[2] - v4[2]};
+    r[0] = n1[1] * n2[2] - n1[2] * n2[1];
+    r[1] = n1[2] * n2[0] - n1[0] * n2[2];
+    r[2] = n1[0] * n2[1] - n1[1] * n2[0];
+    o_normalize64(r);
+  }
+  else {
+    r[0] = 0.0;
+    r[1] = 0.0;
+    r[2] = 1.0;
+  }
+}
+
+void orc_poly_normal(const float *co, const int *corners, int n, float r[3]) { poly_normal(co, corners, n, r); }
+
 /* saacos (absent upstream; clamped acos as its name and use say). */
 static inline float o_saacos(float f)
 {
   return (f <= -1.0f) ? (float)M_PI : ((f >= 1.0f) ? 0.0f : acosf(f));
 }
 
-/* KERNEL_mesh_calc_normals_poly_and_vertex: mesh_normals.c:225-299, with the tri
- * as a 3-corner poly.  Accumulated serially in tri order (the reference adds with
- * a spin-locked vector add in task order, mesh_normals.c:54-81). */
-void orc_calc_initial_normals(const float *co, int totvert, const int *tris, int tottri,
-                              float *out)
+/* KERNEL_mesh_calc_normals_poly_and_vertex: mesh_normals.c:225-299 (accumulation
+ * :226-285, finalize :287-299).  Polygons are given by their corner offsets and corner
+ * vertex ids (poly_off == NULL: tris, each a 3-corner polygon).  Accumulated serially
+ * in polygon order (the reference adds with a spin-locked vector add in task order,
+ * mesh_normals.c:54-81). */
+void orc_calc_initial_normals_poly(const float *co, int totvert, const int *poly_off, const int *poly_verts,
+                                   int npoly, float *out)
 {
   memset(out, 0, sizeof(float) * 3 * (size_t)totvert);
-  for (int t = 0; t < tottri; t++) {
-    const int *ml = tris + 3 * t;
-    const int i_end = 2;
+  for (int p = 0; p < npoly; p++) {
+    const int *ml = poly_verts + (poly_off ? poly_off[p] : 3 * p);
+    const int i_end = (poly_off ? poly_off[p + 1] - poly_off[p] : 3) - 1;
     float pnor[3] = {0.0f, 0.0f, 0.0f};
     {
       const float *v_curr = co + 3 * ml[i_end];
@@ -509,6 +596,94 @@ void orc_calc_initial_normals(const float *co, int totvert, const int *tris, int
       no[1] = co[3 * v + 1];
       no[2] = co[3 * v + 2];
       o_normalize(no);
+    }
+  }
+}
+
+/* The triangle-mesh form: every tri its own 3-corner polygon. */
+void orc_calc_initial_normals(const float *co, int totvert, const int *tris, int tottri, float *out)
+{
+  orc_calc_initial_normals_poly(co, totvert, NULL, tris, tottri, out);
+}
+
+/* The mesh's polygons (MPoly/MLoop): looptri -> polygon, polygon corner offsets and
+ * corner vertex ids.  Normals then follow the reference's polygon rules: the initial
+ * normals weight each polygon's Newell normal by its corner angles, and the update
+ * after a dab adds BKE_mesh_calc_poly_normal of each incident looptri's polygon
+ * (pbvh.c:1041-1058).  The initial normals are recomputed here, and Smooth's 1-ring
+ * becomes the polygon-edge neighbours. */
+void orc_set_polys(OrcPBVH *o, const int *tri_poly, const int *poly_off, const int *poly_verts, int npoly)
+{
+  free(o->tri_poly);
+  free(o->poly_off);
+  free(o->poly_verts);
+  o->npoly = npoly;
+  o->tri_poly = (int *)malloc(sizeof(int) * (size_t)(o->tottri > 0 ? o->tottri : 1));
+  memcpy(o->tri_poly, tri_poly, sizeof(int) * (size_t)o->tottri);
+  o->poly_off = (int *)malloc(sizeof(int) * (size_t)(npoly + 1));
+  memcpy(o->poly_off, poly_off, sizeof(int) * (size_t)(npoly + 1));
+  o->poly_verts = (int *)malloc(sizeof(int) * (size_t)(poly_off[npoly] > 0 ? poly_off[npoly] : 1));
+  memcpy(o->poly_verts, poly_verts, sizeof(int) * (size_t)poly_off[npoly]);
+  orc_calc_initial_normals_poly(o->co, o->totvert, o->poly_off, o->poly_verts, npoly, o->no);
+  /* vertex -> vertex from the polygon edges (BKE_mesh_vert_edge_map_create,
+   * mesh_mapping.c:297: the edges of a polygon join consecutive corners; a fan
+   * triangulation's diagonals are not edges): distinct, ascending */
+  {
+    const int V = o->totvert;
+    int *cnt = (int *)calloc((size_t)V + 1, sizeof(int));
+    for (int q = 0; q < npoly; q++) {
+      for (int k = poly_off[q]; k < poly_off[q + 1]; k++) {
+        cnt[poly_verts[k] + 1]++;
+      }
+    }
+    for (int v = 0; v < V; v++) {
+      cnt[v + 1] += cnt[v];
+    }
+    int *fill = (int *)malloc(sizeof(int) * (size_t)(V + 1));
+    memcpy(fill, cnt, sizeof(int) * (size_t)(V + 1));
+    int *nb = (int *)malloc(sizeof(int) * (size_t)(2 * cnt[V] + 1));
+    for (int q = 0; q < npoly; q++) {
+      const int b = poly_off[q], n = poly_off[q + 1] - b;
+      for (int k = 0; k < n; k++) {
+        const int v = poly_verts[b + k];
+        const int f = fill[v]++;
+        nb[2 * f] = poly_verts[b + (k + n - 1) % n];
+        nb[2 * f + 1] = poly_verts[b + (k + 1) % n];
+      }
+    }
+    free(fill);
+    int *vv_off = (int *)calloc((size_t)V + 1, sizeof(int));
+    int *vv_idx = (int *)malloc(sizeof(int) * (size_t)(2 * cnt[V] + 1));
+    for (int v = 0; v < V; v++) {
+      int *lst = vv_idx + vv_off[v];
+      int n = 0;
+      for (int f = 2 * cnt[v]; f < 2 * cnt[v + 1]; f++) {
+        const int u = nb[f];
+        int dup = u == v;
+        for (int a = 0; a < n && !dup; a++) {
+          dup = lst[a] == u;
+        }
+        if (!dup) {
+          int a = n++;
+          while (a > 0 && lst[a - 1] > u) {
+            lst[a] = lst[a - 1];
+            a--;
+          }
+          lst[a] = u;
+        }
+      }
+      vv_off[v + 1] = vv_off[v] + n;
+    }
+    free(nb);
+    free(cnt);
+    free(o->vv_off);
+    free(o->vv_idx);
+    o->vv_off = vv_off;
+    o->vv_idx = vv_idx;
+  }
+  if (o->no64) {
+    for (size_t i = 0; i < 3 * (size_t)o->totvert; i++) {
+      o->no64[i] = o->no[i];
     }
   }
 }
@@ -605,6 +780,9 @@ void orc_free(OrcPBVH *o)
   free(o->mask);
   free(o->tris);
   free(o->tri_mat);
+  free(o->tri_poly);
+  free(o->poly_off);
+  free(o->poly_verts);
   free(o->vhide);
   free(o->vert_bitmap);
   free(o->vt_off);
@@ -1110,15 +1288,24 @@ static int64_t update_normals(OrcPBVH *o, int nthreads)
       double acc64[3] = {0.0, 0.0, 0.0};
       tri_reads += o->vt_off[v + 1] - o->vt_off[v];
       for (int q = o->vt_off[v]; q < o->vt_off[v + 1]; q++) {
+        /* the face normal of the looptri's polygon (BKE_mesh_calc_poly_normal), once
+         * per incident looptri */
         const int *t = o->tris + 3 * o->vt_idx[q];
+        const int *pc = t;
+        int pn = 3;
+        if (o->tri_poly) {
+          const int pi = o->tri_poly[o->vt_idx[q]];
+          pc = o->poly_verts + o->poly_off[pi];
+          pn = o->poly_off[pi + 1] - o->poly_off[pi];
+        }
         float fn[3];
-        orc_tri_normal(o->co + 3 * t[0], o->co + 3 * t[1], o->co + 3 * t[2], fn);
+        poly_normal(o->co, pc, pn, fn);
         acc[0] = acc[0] + fn[0];
         acc[1] = acc[1] + fn[1];
         acc[2] = acc[2] + fn[2];
         if (fp64) {
           double fn64[3];
-          tri_normal64(o->co64 + 3 * t[0], o->co64 + 3 * t[1], o->co64 + 3 * t[2], fn64);
+          poly_normal64(o->co64, pc, pn, fn64);
           acc64[0] += fn64[0];
           acc64[1] += fn64[1];
           acc64[2] += fn64[2];

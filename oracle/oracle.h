@@ -131,6 +131,12 @@ float orc_falloff(const OrcDab *dab, float d2);
 void orc_tri_normal(const float v1[3], const float v2[3], const float v3[3], float out[3]);
 void orc_calc_initial_normals(const float *co, int totvert, const int *tris, int tottri,
                               float *out);
+void orc_calc_initial_normals_poly(const float *co, int totvert, const int *poly_off, const int *poly_verts,
+                                   int npoly, float *out);
+void orc_poly_normal(const float *co, const int *corners, int n, float out[3]);
+/* The mesh's polygons: looptri -> polygon, corner offsets [npoly + 1], corner vertex
+ * ids.  Normals then follow the polygon rules; the initial normals are recomputed. */
+void orc_set_polys(OrcPBVH *o, const int *tri_poly, const int *poly_off, const int *poly_verts, int npoly);
 
 /* Ray cast (ray.c).  One ray against the current positions; depth0 bounds the hit
  * (hits at depth >= depth0 are ignored).  Returns hit.  stats (may be NULL): node
@@ -190,6 +196,8 @@ int orc_draw_leaf(const OrcPBVH *o, int node, const OrcPolys *polys, const float
                   int *r_smooth, int *r_material);
 /* BKE_paint_face_set_overlay_color_get (rgb) */
 void orc_face_set_color(int face_set, int seed, unsigned char out[3]);
+void orc_lookup3_final(uint32_t abc[3]);
+float orc_hash_int_01(uint32_t k);
 
 /* Threads used by the OpenMP loops (0 = runtime default). */
 void orc_set_threads(int n);

@@ -32,6 +32,10 @@ struct OrcPBVH {
   float *co, *no, *mask;
   double *co64, *no64;
   int *tris, *tri_mat;
+  /* polygons (NULL: each tri is its own polygon): looptri -> polygon, each polygon's
+   * corner range and corner vertex ids (MPoly / MLoop, types_meshdata.h:50-74) */
+  int *tri_poly, *poly_off, *poly_verts;
+  int npoly;
   uint8_t *vhide;
   ONode *nodes;
   int totnode, node_mem;

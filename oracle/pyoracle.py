@@ -84,6 +84,9 @@ def lib():
         "orc_falloff": (C.c_float, [C.POINTER(OrcDab), C.c_float]),
         "orc_tri_normal": (None, [f32p, f32p, f32p, f32p]),
         "orc_calc_initial_normals": (None, [f32p, i32, i32p, i32, f32p]),
+        "orc_calc_initial_normals_poly": (None, [f32p, i32, i32p, i32p, i32, f32p]),
+        "orc_poly_normal": (None, [f32p, i32p, i32, f32p]),
+        "orc_set_polys": (None, [vp, i32p, i32p, i32p, i32]),
         "orc_set_threads": (None, [i32]),
         "orc_get_threads": (i32, []),
         "orc_last_stats": (None, [vp, C.POINTER(C.c_int64)]),
@@ -94,6 +97,8 @@ def lib():
         "orc_draw_leaf": (i32, [vp, i32, C.POINTER(OrcPolys), f32p, i32p, i32, i32, C.POINTER(C.c_uint8), i32,
                                 i32p, i32p]),
         "orc_face_set_color": (None, [i32, i32, C.POINTER(C.c_uint8)]),
+        "orc_lookup3_final": (None, [C.POINTER(C.c_uint32)]),
+        "orc_hash_int_01": (C.c_float, [C.c_uint32]),
         "orc_isect_ray_tri": (i32, [f32p, f32p, f32p, f32p, f32p, f32p]),
         "orc_isect_ray_box": (i32, [f32p, f32p, f32p, f32p, f32p]),
         "orc_dist_sq_ray_box": (C.c_float, [f32p, f32p, f32p, f32p, f32p]),
@@ -137,7 +142,9 @@ def to_orc(d) -> OrcDab:
 
 
 class Oracle:
-    def __init__(self, co, tris, leaf_limit=128, fp64=True, hidden=None, mat=None, respect_hide=True):
+    def __init__(self, co, tris, leaf_limit=128, fp64=True, hidden=None, mat=None, respect_hide=True, polys=None):
+        """polys (optional): (tri_poly, poly_off, poly_verts) -- the mesh's polygons, for
+        the polygon normal rules; `tris` are then its looptris."""
         L = lib()
         co = np.ascontiguousarray(co, np.float32)
         tris = np.ascontiguousarray(tris, np.int32)
@@ -152,6 +159,9 @@ class Oracle:
             mp = _ip(self._mat)
         self.h = L.orc_build(_fp(co), self.totvert, _ip(tris), self.tottri, mp, hp, leaf_limit,
                              1 if respect_hide else 0, 1 if fp64 else 0)
+        if polys is not None:
+            tp, off, pv = (np.ascontiguousarray(x, np.int32) for x in polys)
+            L.orc_set_polys(self.h, _ip(tp), _ip(off), _ip(pv), len(off) - 1)
 
     def close(self):
         if self.h:
@@ -331,6 +341,24 @@ def falloff(d, d2):
     return lib().orc_falloff(C.byref(o), d2)
 
 
+def poly_normal(co, corners):
+    """BKE_mesh_calc_poly_normal of the polygon with the given corner ids."""
+    c = np.ascontiguousarray(co, np.float32)
+    k = np.ascontiguousarray(corners, np.int32)
+    out = np.zeros(3, np.float32)
+    lib().orc_poly_normal(_fp(c), _ip(k), len(k), _fp(out))
+    return out
+
+
+def initial_normals_poly(co, poly_off, poly_verts):
+    c = np.ascontiguousarray(co, np.float32)
+    off = np.ascontiguousarray(poly_off, np.int32)
+    pv = np.ascontiguousarray(poly_verts, np.int32)
+    out = np.zeros_like(c)
+    lib().orc_calc_initial_normals_poly(_fp(c), c.shape[0], _ip(off), _ip(pv), len(off) - 1, _fp(out))
+    return out
+
+
 def tri_normal(a, b, c):
     a, b, c = (np.ascontiguousarray(x, np.float32) for x in (a, b, c))
     out = np.zeros(3, np.float32)
@@ -389,6 +417,16 @@ def kelvinlet(d, co, bstrength):
     out = np.zeros(3, np.float32)
     lib().orc_kelvinlet(C.byref(od), _fp(c), _fp(b), _fp(out))
     return out
+
+
+def lookup3_final(a, b, c):
+    abc = (C.c_uint32 * 3)(a & 0xFFFFFFFF, b & 0xFFFFFFFF, c & 0xFFFFFFFF)
+    lib().orc_lookup3_final(abc)
+    return abc[0], abc[1], abc[2]
+
+
+def hash_int_01(k):
+    return float(lib().orc_hash_int_01(k & 0xFFFFFFFF))
 
 
 def face_set_color(fs, seed):

@@ -58,9 +58,9 @@ def test_learned_baselines_and_norm():
 
 def test_bench_workload_generators(sg):
     sys.path.insert(0, str(ROOT))
-    import bench
+    from dune_sculpt_amd import workloads
 
-    meshes, specs = bench.batch_meshes(sg, 6, seed=11, vmin=5000, vmax=500000, amplitude=0.01)
+    meshes, specs = workloads.batch_meshes(sg, 6, seed=11, vmin=5000, vmax=500000, amplitude=0.01)
     for (co, tris), (kind, seed, amp) in zip(meshes, specs):
         V = co.shape[0]
         if kind == "sphere":
@@ -69,7 +69,7 @@ def test_bench_workload_generators(sg):
         else:
             assert tris.shape[0] == 2 * V  # closed torus grid
         assert tris.min() >= 0 and tris.max() < V
-    table = bench.batch_table(sg, meshes, specs, 8, seed=12, tools=(sg.TOOL_DRAW, sg.TOOL_SMOOTH),
+    table = workloads.batch_table(sg, meshes, specs, 8, seed=12, tools=(sg.TOOL_DRAW, sg.TOOL_SMOOTH),
                               rmin=0.02, rmax=0.1)
     assert len(table) == 8 and all(len(r) == len(meshes) for r in table)
     for m, (co, _) in enumerate(meshes):

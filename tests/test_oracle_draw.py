@@ -1,8 +1,12 @@
 """Known-answer tests for the oracle's draw records (oracle/draw.c): the reference's
 vertex layout, the normal and mask conversions, visibility and flat/smooth normals,
-against closed forms and brute force.  The face-set overlay colour uses a hash whose
-header is absent from the reference (BLI_hash.h): its values are parity unpinned; only
-its range, determinism and the white default are checked."""
+against closed forms and brute force.  The face-set overlay colour hashes with
+BLI_hash_int_01, whose header (BLI_hash.h) is absent from the reference: it is Bob
+Jenkins' lookup3 hashword of (k, 0) with initial value 13, so the oracle's lookup3 mix is
+pinned by lookup3's own published hash vectors, and the hash by its identity with
+lookup3's byte hash of the same eight bytes."""
+import struct
+
 import numpy as np
 
 F32 = np.float32
@@ -131,3 +135,61 @@ def test_quad_and_ngon_normals(sg, po):
                              np.ones(2, np.int32))
     nor = recs[:, 16:22].copy().view(np.int16).astype(np.float64) / 32767
     assert np.abs(np.abs(nor @ t) - 1).max() < 1e-3
+
+
+def _rot(x, k):
+    return ((x << k) | (x >> (32 - k))) & 0xFFFFFFFF
+
+
+def _mix(a, b, c):
+    """lookup3.c `mix` (the block mix; the oracle holds only `final`)."""
+    M = 0xFFFFFFFF
+    a = (a - c) & M; a ^= _rot(c, 4); c = (c + b) & M
+    b = (b - a) & M; b ^= _rot(a, 6); a = (a + c) & M
+    c = (c - b) & M; c ^= _rot(b, 8); b = (b + a) & M
+    a = (a - c) & M; a ^= _rot(c, 16); c = (c + b) & M
+    b = (b - a) & M; b ^= _rot(a, 19); a = (a + c) & M
+    c = (c - b) & M; c ^= _rot(b, 4); b = (b + a) & M
+    return a, b, c
+
+
+def _hashlittle2(po, data, pc, pb):
+    """lookup3.c hashlittle2 (little-endian byte hash) with the oracle's final mix."""
+    M = 0xFFFFFFFF
+    n = len(data)
+    a = b = c = (0xDEADBEEF + n + pc) & M
+    c = (c + pb) & M
+    while n > 12:
+        w = struct.unpack_from("<3I", data, len(data) - n)
+        a, b, c = (a + w[0]) & M, (b + w[1]) & M, (c + w[2]) & M
+        a, b, c = _mix(a, b, c)
+        n -= 12
+    if n == 0:
+        return c, b
+    tail = data[len(data) - n:] + bytes(12 - n)  # the masked tail words read zero bytes
+    w = struct.unpack("<3I", tail)
+    a, b, c = (a + w[0]) & M, (b + w[1]) & M, (c + w[2]) & M
+    a, b, c = po.lookup3_final(a, b, c)
+    return c, b
+
+
+def test_lookup3_published_vectors(po):
+    """The driver vectors printed by lookup3.c (driver5): the 30-byte string runs two
+    block mixes and the oracle's final mix; the empty-key cases pin the initial value."""
+    s = b"Four score and seven years ago"
+    assert _hashlittle2(po, b"", 0, 0) == (0xDEADBEEF, 0xDEADBEEF)
+    assert _hashlittle2(po, b"", 0, 0xDEADBEEF) == (0xBD5B7DDE, 0xDEADBEEF)
+    assert _hashlittle2(po, b"", 0xDEADBEEF, 0xDEADBEEF) == (0x9C093CCD, 0xBD5B7DDE)
+    assert _hashlittle2(po, s, 0, 0) == (0x17770551, 0xCE7226E6)
+    assert _hashlittle2(po, s, 0, 1) == (0xE3607CAE, 0xBD371DE4)
+    assert _hashlittle2(po, s, 1, 0) == (0xCD628161, 0x6CBEA4B3)
+
+
+def test_hash_int_01_is_lookup3_of_two_words(po):
+    """BLI_hash_int_2d(k, 0) = hashword({k, 0}, 2, 13) = hashlittle of the same 8 bytes
+    with initial value 13 (both start from 0xdeadbeef + 8 + 13 and add the two words);
+    BLI_hash_int_01 scales it by 1 / 0xFFFFFFFF in float32."""
+    scale = np.float32(1.0) / np.float32(0xFFFFFFFF)
+    for k in [0, 1, 2, 3, 7, 42, 1000, 65535, 123456789, 0x7FFFFFFF, 0xFFFFFFFF]:
+        c, _ = _hashlittle2(po, struct.pack("<2I", k, 0), 13, 0)
+        assert np.float32(po.hash_int_01(k)) == np.float32(c) * scale, k

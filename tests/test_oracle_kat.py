@@ -295,6 +295,60 @@ def test_smooth_fixed_point_and_fan(sg, po):
     assert abs(o.positions()[0, 0] - 0.1 * (1 - s) ** 3) < 1e-7
 
 
+OCTA = np.array([(1, 0, 0), (-0.5, 0, 0), (0, 0.5, 0), (0, -1, 0), (0, 0, 0.25), (0, 0, -0.75)], np.float32)
+OCTA_TRIS = np.array([(0, 2, 4), (2, 1, 4), (1, 3, 4), (3, 0, 4), (2, 0, 5), (1, 2, 5), (3, 1, 5), (0, 3, 5)],
+                     np.int32)
+
+
+def _smooth_variants(k, s, r):
+    """Exact rational Smooth(k) on the octahedron with LIN falloff around the origin, in
+    three readings of spec S8: Jacobi with the factor fixed from the pre-dab positions
+    (the spec), in-place (Gauss-Seidel, vertex order), and Jacobi with the factor
+    recomputed from each iterate.  Every vertex has 4 edge neighbours (all but its
+    opposite), so every quantity of the spec's reading is a short dyadic rational and FP32
+    holds it exactly (the refreshed factors are not exact: that reading is only compared
+    for being different)."""
+    from fractions import Fraction as Fr
+
+    P0 = [[Fr(float(x)) for x in v] for v in OCTA]
+    nbr = [[u for u in range(6) if u != v and u != (v ^ 1)] for v in range(6)]
+
+    def fade(P, v, exact=True):
+        d = sum(c * c for c in P[v])  # squared distance: a dyadic square at the start
+        dist = Fr(float(d) ** 0.5)
+        assert dist * dist == d or not exact
+        return Fr(float(s)) * (1 - dist / Fr(float(r)))
+
+    out = {}
+    for name in ("jacobi", "inplace", "refade"):
+        P = [list(v) for v in P0]
+        f0 = [fade(P0, v) for v in range(6)]
+        for _ in range(k):
+            src = P if name == "inplace" else [list(v) for v in P]
+            f = [fade(src, v, exact=False) for v in range(6)] if name == "refade" else f0
+            for v in range(6):
+                avg = [sum(src[u][a] for u in nbr[v]) / 4 for a in range(3)]
+                P[v] = [src[v][a] + (avg[a] - src[v][a]) * f[v] for a in range(3)]
+        out[name] = np.array([[float(c) for c in v] for v in P])
+    return out
+
+
+@pytest.mark.parametrize("k", [2, 3])
+def test_smooth_jacobi_closed_form(sg, po, k):
+    """Spec S8 on a skewed octahedron, every vertex affected (r = 2, LIN falloff, s = 0.5:
+    factors 0.25 .. 0.4375): the oracle equals exact rational Jacobi with the factor
+    fixed at the dab's start, bit for bit, and the in-place and refreshed-factor readings
+    give different values, so either mistake fails here."""
+    want = _smooth_variants(k, 0.5, 2.0)
+    assert np.abs(want["jacobi"] - want["inplace"]).max() > 1e-3
+    assert np.abs(want["jacobi"] - want["refade"]).max() > 1e-3
+    o = po.Oracle(OCTA, OCTA_TRIS, leaf_limit=8)
+    got = o.dab(sg.make_dab(tool=2, location=(0, 0, 0), radius=2.0, strength=0.5, falloff=5, smooth_iterations=k))
+    assert got.tolist() == list(range(6))
+    assert np.array_equal(o.positions(), want["jacobi"].astype(np.float32))
+    o.close()
+
+
 def test_mask_scales_factor(sg, po):
     co, tris = grid(17, spacing=1 / 16)
     mask = np.full(co.shape[0], 0.25, np.float32)
