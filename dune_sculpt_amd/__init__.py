@@ -114,7 +114,7 @@ EXPORTS = [
     "sg_dab_stroke_log", "sg_sync", "sg_last_error", "sg_dab_last_affected", "sg_dab_last_stats",
     "sg_dab_last_gathered", "sg_pbvh_set_stats", "sg_pbvh_set_grid", "sg_pbvh_read_positions",
     "sg_pbvh_read_normals", "sg_pbvh_set_mask", "sg_pbvh_tree_info", "sg_pbvh_node_index",
-    "sg_pbvh_node_at", "sg_pbvh_prim_indices", "sg_mesh_icosphere_size", "sg_mesh_icosphere",
+    "sg_pbvh_node_at", "sg_pbvh_node_flag", "sg_pbvh_prim_indices", "sg_mesh_icosphere_size", "sg_mesh_icosphere",
     "sg_mesh_torus", "sg_mesh_noise_displace", "sg_pbvh_set_profile", "sg_pbvh_profile_read",
     "sg_stroke_prepare", "sg_stroke_launch", "sg_stroke_free", "sg_pbvh_set_kernel_grids",
     "sg_pbvh_get_kernel_grids", "sg_pbvh_snapshot_save", "sg_pbvh_snapshot_restore",
@@ -197,6 +197,7 @@ def lib():
         "sg_pbvh_tree_info": (i32, [vp, i32p]),
         "sg_pbvh_node_index": (i32, [vp, vp]),
         "sg_pbvh_node_at": (vp, [vp, i32]),
+        "sg_pbvh_node_flag": (i32, [vp]),
         "sg_pbvh_prim_indices": (i32, [vp, i32p, i32]),
         "sg_mesh_icosphere_size": (i32, [i32, i32p, i32p]),
         "sg_mesh_icosphere": (i32, [i32, C.c_float, f32p, i32p]),
@@ -1009,6 +1010,62 @@ class PBVH:
         a, e, d = (np.array(x, np.float32) for x in (start, end, direction))
         lib().BKE_pbvh_raycast_project_ray_root(self.h, bool(original), _fp(a), _fp(e), _fp(d))
         return a, e
+
+    # -- the drop-in update API (what the reference's callers run after host edits)
+    def mverts(self):
+        """The MVert array BKE_pbvh_get_verts returns, as an (V, 3) float32 view of its
+        coordinates (writable: edits go through it, as the reference's callers do)."""
+        ptr = lib().BKE_pbvh_get_verts(self.h)
+        raw = np.ctypeslib.as_array((C.c_uint8 * (16 * self.totvert)).from_address(ptr))
+        return raw.view(np.float32).reshape(self.totvert, 4)[:, :3]
+
+    def vert_mark_update(self, v):
+        lib().BKE_pbvh_vert_mark_update(self.h, int(v))
+
+    def node_mark_update(self, index):
+        lib().BKE_pbvh_node_mark_update(self.node(index))
+
+    def node_flag(self, index):
+        return lib().sg_pbvh_node_flag(self.node(index))
+
+    def update_bounds(self, flag):
+        lib().BKE_pbvh_update_bounds(self.h, int(flag))
+
+    def update_normals(self):
+        lib().BKE_pbvh_update_normals(self.h, None)
+
+    def coords_apply(self, co):
+        c = np.ascontiguousarray(co, np.float32)
+        lib().BKE_pbvh_vert_coords_apply(self.h, c.ctypes.data, c.shape[0])
+
+    def coords_alloc(self):
+        """BKE_pbvh_vert_coords_alloc: a malloc'd (V, 3) copy, freed here after copying."""
+        ptr = lib().BKE_pbvh_vert_coords_alloc(self.h)
+        if not ptr:
+            raise RuntimeError("BKE_pbvh_vert_coords_alloc failed: " + last_error())
+        out = np.ctypeslib.as_array((C.c_float * (3 * self.totvert)).from_address(ptr)).reshape(-1, 3).copy()
+        C.CDLL(None).free(C.c_void_p(ptr))
+        return out
+
+    def frustum_gather(self, planes, exclude=False):
+        """BKE_pbvh_search_gather with the reference's frustum callbacks and a
+        PBVHFrustumPlanes of the given (n, 4) planes: node indices, left to right."""
+        L = lib()
+        pl = np.ascontiguousarray(planes, np.float32)
+
+        class Frustum(C.Structure):
+            _fields_ = [("planes", C.POINTER(C.c_float)), ("num_planes", C.c_int)]
+
+        fr = Frustum(_fp(pl), pl.shape[0])
+        fn = L.BKE_pbvh_node_frustum_exclude_AABB if exclude else L.BKE_pbvh_node_frustum_contain_AABB
+        cb = C.cast(fn, SearchCB)
+        arr = C.POINTER(C.c_void_p)()
+        tot = C.c_int()
+        L.BKE_pbvh_search_gather(self.h, cb, C.byref(fr), C.byref(arr), C.byref(tot))
+        res = [L.sg_pbvh_node_index(self.h, arr[i]) for i in range(tot.value)]
+        if tot.value:
+            C.CDLL(None).free(arr)
+        return res
 
     def search_gather(self, callback=None):
         """BKE_pbvh_search_gather with a Python callback(node_index) -> bool."""

@@ -84,15 +84,33 @@ int sync_bbs(PBVH *p) {
   return 0;
 }
 
+// Grow-only device staging of the host-driven entry points: allocated on first use
+// (and when a larger size is asked for), then reused, so those calls allocate nothing
+// in steady state.  Freed with the PBVH.
+void *stage(PBVH *p, size_t bytes) {
+  SgDev &d = p->d;
+  bytes = std::max<size_t>(bytes, 256);
+  if (bytes <= d.stage_cap) return d.stage;
+  if (d.stage) {
+    hipStreamSynchronize(p->stream);
+    hipFree(d.stage);
+  }
+  d.stage = nullptr;
+  d.stage_cap = 0;
+  const size_t cap = std::max(bytes, (size_t)1 << 20);
+  if (sg_check_hip(hipMalloc(&d.stage, cap), "hipMalloc")) return nullptr;
+  d.stage_cap = cap;
+  return d.stage;
+}
+
 int read_aos(PBVH *p, int which, float *out) {
   if (p->no_device) return sg_set_error("no HIP device");
   const size_t n = 3 * (size_t)p->d.V;
-  float *tmp = nullptr;
-  if (sg_check_hip(hipMalloc(&tmp, std::max<size_t>(n, 1) * sizeof(float)), "hipMalloc")) return -1;
+  float *tmp = (float *)stage(p, n * sizeof(float));
+  if (!tmp) return -1;
   int rc = sg_enqueue_gather(p, which, tmp);
   if (!rc) rc = sg_check_hip(hipMemcpyAsync(out, tmp, n * sizeof(float), hipMemcpyDeviceToHost, p->stream), "copy");
   if (!rc) rc = sg_check_hip(hipStreamSynchronize(p->stream), "sync");
-  hipFree(tmp);
   return rc;
 }
 
@@ -341,9 +359,10 @@ void BKE_pbvh_build_mesh(PBVH *p, Mesh *mesh, const MPoly *mpoly, const MLoop *m
     return;
   }
   p->build_ok = true;
-  sg_enqueue_full_normals(p, 0);
+  sg_enqueue_full_normals(p);
   hipStreamSynchronize(p->stream);
   p->host_vmark.assign(totvert, 0);
+  p->host_mark_list.clear();
   invalidate_host_views(p);
   p->host_verts_stale = false;  // the caller's array is the source
 }
@@ -1230,6 +1249,8 @@ PBVHNode *sg_pbvh_node_at(PBVH *p, int index) {
   return &p->nodes[index];
 }
 
+int sg_pbvh_node_flag(PBVHNode *node) { return node ? node->flag : 0; }
+
 int sg_pbvh_prim_indices(PBVH *p, int *out, int cap) {
   if (!p) return sg_set_error("null PBVH");
   const int n = (int)p->prim_indices.size();
@@ -1292,59 +1313,191 @@ void BKE_pbvh_node_mark_redraw(PBVHNode *node) {
   node->flag |= PBVH_UpdateDrawBuffers | PBVH_UpdateRedraw;
   if (node->owner) node->owner->draw_marks_pending = true;
 }
-void BKE_pbvh_vert_mark_update(PBVH *p, int index) {
-  if (p && index >= 0 && index < p->totvert) p->host_vmark[index] = 1;
+void BKE_pbvh_vert_mark_update(PBVH *p, int index) {  // pbvh.c:1811-1815
+  if (!p || index < 0 || index >= p->totvert || p->host_vmark.empty()) return;
+  if (!p->host_vmark[index]) {
+    p->host_vmark[index] = 1;
+    p->host_mark_list.push_back(index);
+  }
 }
 
 // ---- updates ---------------------------------------------------------------
 
+// pbvh_flush_bb (pbvh.c:1369-1399) over the host mirror: leaves report and clear
+// their bounds flags, internal nodes take the union of their children's boxes (UpdateBB)
+// and copy it into orig_vb (UpdateOriginalBB); changed internal nodes are collected.
+static int host_flush_bb(PBVH *p, int ni, int flag, std::vector<int> &changed) {
+  PBVHNode &n = p->nodes[ni];
+  int update = 0;
+  if (n.flag & PBVH_Leaf) {
+    if (flag & PBVH_UpdateBB) {
+      update |= n.flag & PBVH_UpdateBB;
+      n.flag &= ~PBVH_UpdateBB;
+    }
+    if (flag & PBVH_UpdateOriginalBB) {
+      update |= n.flag & PBVH_UpdateOriginalBB;
+      n.flag &= ~PBVH_UpdateOriginalBB;
+    }
+    return update;
+  }
+  update |= host_flush_bb(p, n.children_offset, flag, changed);
+  update |= host_flush_bb(p, n.children_offset + 1, flag, changed);
+  if (update & PBVH_UpdateBB) {
+    const PBVHNode &a = p->nodes[n.children_offset], &b = p->nodes[n.children_offset + 1];
+    // update_node_vb (pbvh.c:108-128): BB_reset, then BB_expand_with_bb by both
+    // children with min_ff / max_ff
+    auto mn = [](float x, float y) { return x < y ? x : y; };
+    auto mx = [](float x, float y) { return x > y ? x : y; };
+    for (int k = 0; k < 3; k++) {
+      n.vb[k] = mn(mn(FLT_MAX, a.vb[k]), b.vb[k]);
+      n.vb[3 + k] = mx(mx(-FLT_MAX, a.vb[3 + k]), b.vb[3 + k]);
+    }
+  }
+  if (update & PBVH_UpdateOriginalBB) memcpy(n.orig_vb, n.vb, sizeof(n.vb));
+  if (update) changed.push_back(ni);
+  return update;
+}
+
 void BKE_pbvh_update_bounds(PBVH *p, int flag) {  // pbvh.c:1401-1421
-  if (!p || p->nodes.empty() || p->no_device) return;
-  if (!(flag & (PBVH_UpdateBB | PBVH_UpdateOriginalBB))) {
-    for (PBVHNode &n : p->nodes) n.flag &= ~(flag & PBVH_UpdateRedraw);
+  if (!p || p->nodes.empty() || p->no_device || !p->build_ok) return;
+  // the leaves update_search_cb gathers (any bit of `flag`), with the bounds bits each
+  // carries
+  std::vector<int> leaves, modes;
+  for (size_t l = 0; l < p->leaf_node.size(); l++) {
+    PBVHNode &n = p->nodes[p->leaf_node[l]];
+    if (!(n.flag & flag)) continue;
+    if ((flag & PBVH_UpdateRedraw) && (n.flag & PBVH_UpdateRedraw)) n.flag &= ~PBVH_UpdateRedraw;
+    const int m = n.flag & flag & (PBVH_UpdateBB | PBVH_UpdateOriginalBB);
+    if (m) {
+      leaves.push_back((int)l);
+      modes.push_back(m);
+    }
+  }
+  if (!(flag & (PBVH_UpdateBB | PBVH_UpdateOriginalBB)) || leaves.empty()) return;
+  upload_host_leaf_verts(p, leaves);
+  const int both = PBVH_UpdateBB | PBVH_UpdateOriginalBB;
+  const bool all_both = std::all_of(modes.begin(), modes.end(), [&](int m) { return m == both; });
+  const size_t nl = leaves.size();
+  if (all_both) {
+    // the brush's case: each listed leaf's box from its vertices into vb and orig_vb, the
+    // tree above on the device
+    int *dl = (int *)stage(p, nl * sizeof(int));
+    if (!dl) return;
+    hipMemcpyAsync(dl, leaves.data(), nl * sizeof(int), hipMemcpyHostToDevice, p->stream);
+    sg_enqueue_refit_list(p, dl, (int)nl);
+    sg_enqueue_flush(p);
+    if (sg_check_hip(hipStreamSynchronize(p->stream), "update bounds")) return;
+    std::vector<int> changed;
+    host_flush_bb(p, 0, flag, changed);  // clears the leaves' flags as the reference does
+    p->host_bb_stale = true;
     return;
   }
-  std::vector<int> leaves = flagged_leaves(p, flag & (PBVH_UpdateBB | PBVH_UpdateOriginalBB));
-  if (!leaves.empty()) {
-    upload_host_leaf_verts(p, leaves);
-    int *dl = nullptr;
-    if (sg_check_hip(hipMalloc(&dl, leaves.size() * sizeof(int)), "hipMalloc")) return;
-    hipMemcpyAsync(dl, leaves.data(), leaves.size() * sizeof(int), hipMemcpyHostToDevice, p->stream);
-    sg_enqueue_refit_list(p, dl, (int)leaves.size());
-    sg_enqueue_flush(p);
-    hipStreamSynchronize(p->stream);
-    hipFree(dl);
-    p->host_bb_stale = true;
+  // partial flags: the reference's per-node rules on the host mirror, the new vertex
+  // boxes computed on the device, the changed boxes written back
+  if (sync_bbs(p)) return;
+  const size_t off_out = (nl * sizeof(int) + 255) & ~(size_t)255;
+  char *st = (char *)stage(p, off_out + nl * 32);
+  if (!st) return;
+  hipMemcpyAsync(st, leaves.data(), nl * sizeof(int), hipMemcpyHostToDevice, p->stream);
+  std::vector<float> boxes(nl * 8);
+  if (sg_enqueue_leaf_boxes(p, (const int *)st, (int)nl, (float *)(st + off_out))) return;
+  hipMemcpyAsync(boxes.data(), st + off_out, nl * 32, hipMemcpyDeviceToHost, p->stream);
+  if (sg_check_hip(hipStreamSynchronize(p->stream), "leaf boxes")) return;
+  struct Rec {
+    int kind, idx, pad0, pad1;
+    float vb[8], ob[8];
+  };
+  std::vector<Rec> recs;
+  auto rec_of = [&](int kind, int idx, const PBVHNode &n) {
+    Rec r;
+    memset(&r, 0, sizeof(r));
+    r.kind = kind;
+    r.idx = idx;
+    for (int k = 0; k < 3; k++) {
+      r.vb[k] = n.vb[k];
+      r.vb[4 + k] = n.vb[3 + k];
+      r.ob[k] = n.orig_vb[k];
+      r.ob[4 + k] = n.orig_vb[3 + k];
+    }
+    return r;
+  };
+  for (size_t i = 0; i < nl; i++) {  // pbvh_update_BB_redraw_task_cb (pbvh.c:1206-1227)
+    PBVHNode &n = p->nodes[p->leaf_node[leaves[i]]];
+    if (modes[i] & PBVH_UpdateBB)
+      for (int k = 0; k < 3; k++) {
+        n.vb[k] = boxes[8 * i + k];
+        n.vb[3 + k] = boxes[8 * i + 4 + k];
+      }
+    if (modes[i] & PBVH_UpdateOriginalBB) memcpy(n.orig_vb, n.vb, sizeof(n.vb));
+    recs.push_back(rec_of(0, leaves[i], n));
   }
-  for (int l : leaves) p->nodes[p->leaf_node[l]].flag &= ~(PBVH_UpdateBB | PBVH_UpdateOriginalBB | PBVH_UpdateRedraw);
+  std::vector<int> changed;
+  host_flush_bb(p, 0, flag, changed);
+  std::vector<char> is_changed(p->nodes.size(), 0);
+  for (int ni : changed) {
+    is_changed[ni] = 1;
+    recs.push_back(rec_of(1, ni, p->nodes[ni]));
+  }
+  for (int l : leaves) is_changed[p->leaf_node[l]] = 1;
+  for (size_t c = 0; c < p->cut.node.size(); c++) {  // the search's subtree boxes are their roots' boxes
+    const int root = p->cut.node[c];
+    if (!is_changed[root]) continue;
+    recs.push_back(rec_of(2, (int)c, p->nodes[root]));
+    if (p->nodes[root].flag & PBVH_Leaf) recs.push_back(rec_of(1, root, p->nodes[root]));
+  }
+  Rec *dr = (Rec *)stage(p, recs.size() * sizeof(Rec));
+  if (!dr) return;
+  hipMemcpyAsync(dr, recs.data(), recs.size() * sizeof(Rec), hipMemcpyHostToDevice, p->stream);
+  sg_enqueue_scatter_boxes(p, dr, (int)recs.size());
+  sg_check_hip(hipStreamSynchronize(p->stream), "update bounds");
 }
 
 void BKE_pbvh_update_normals(PBVH *p, SubdivCCG *subdiv_ccg) {  // pbvh.c:4559-4587
   (void)subdiv_ccg;
-  if (!p || p->nodes.empty() || p->no_device) return;
+  if (!p || p->nodes.empty() || p->no_device || !p->build_ok) return;
   std::vector<int> leaves = flagged_leaves(p, PBVH_UpdateNormals);
   if (leaves.empty()) return;
   upload_host_leaf_verts(p, leaves);
-  std::vector<uint8_t> marks(p->d.V, 0);
-  for (int v = 0; v < p->totvert; v++) marks[p->perm_new_of_old[v]] = p->host_vmark[v];
-  hipMemcpyAsync(p->d.vmark, marks.data(), marks.size(), hipMemcpyHostToDevice, p->stream);
-  int *dl = nullptr;
-  if (sg_check_hip(hipMalloc(&dl, leaves.size() * sizeof(int)), "hipMalloc")) return;
-  hipMemcpyAsync(dl, leaves.data(), leaves.size() * sizeof(int), hipMemcpyHostToDevice, p->stream);
-  sg_enqueue_normals_list(p, dl, (int)leaves.size());
-  hipStreamSynchronize(p->stream);
-  hipFree(dl);
+  // the vertices marked on the host join the device's marks (those of
+  // BKE_pbvh_vert_coords_apply); only the marks are uploaded, not a V-sized array
+  std::vector<int> marks(p->host_mark_list.size());
+  for (size_t i = 0; i < marks.size(); i++) marks[i] = p->perm_new_of_old[p->host_mark_list[i]];
+  const size_t nm = marks.size(), nl = leaves.size();
+  const size_t off_l = (nm * sizeof(int) + 255) & ~(size_t)255;
+  char *st = (char *)stage(p, off_l + nl * sizeof(int));
+  if (!st) return;
+  if (nm) hipMemcpyAsync(st, marks.data(), nm * sizeof(int), hipMemcpyHostToDevice, p->stream);
+  hipMemcpyAsync(st + off_l, leaves.data(), nl * sizeof(int), hipMemcpyHostToDevice, p->stream);
+  sg_enqueue_set_marks(p, (const int *)st, (int)nm);
+  sg_enqueue_normals_list(p, (const int *)(st + off_l), (int)nl);
+  if (sg_check_hip(hipStreamSynchronize(p->stream), "update normals")) return;
   // marks of uniq verts of the processed leaves are consumed (pbvh.c:1084-1087)
+  std::vector<char> done(p->leaf_node.size(), 0);
   for (int l : leaves) {
-    const PBVHNode &n = p->nodes[p->leaf_node[l]];
-    for (int k = 0; k < n.uniq_verts; k++) p->host_vmark[n.vert_indices[k]] = 0;
+    done[l] = 1;
     p->nodes[p->leaf_node[l]].flag &= ~PBVH_UpdateNormals;
   }
+  std::vector<int> keep;
+  for (size_t i = 0; i < nm; i++) {
+    const int leaf = (int)(std::upper_bound(p->leaf_vbeg.begin(), p->leaf_vbeg.end(), marks[i]) -
+                           p->leaf_vbeg.begin()) - 1;
+    if (done[leaf])
+      p->host_vmark[p->host_mark_list[i]] = 0;
+    else
+      keep.push_back(p->host_mark_list[i]);
+  }
+  p->host_mark_list.swap(keep);
   p->host_normals_stale = true;
 }
 
-void BKE_pbvh_vert_coords_apply(PBVH *p, const float (*vertCos)[3], int totvert) {  // pbvh.c:4707-4749
-  if (!p || p->nodes.empty() || p->no_device) return;
+// pbvh.c:4707-4749.  Positions that differ bitwise from the new ones are replaced and
+// their vertices marked for the next normal update; every node is marked
+// (BKE_pbvh_node_mark_update) and the bounds refit with UpdateBB | UpdateOriginalBB.
+// The reference's BKE_mesh_calc_normals_looptri call here accumulates into a
+// temporary that it frees (mesh_normals.c:455-507): the vertex normals themselves
+// change only at the next BKE_pbvh_update_normals, as here.
+void BKE_pbvh_vert_coords_apply(PBVH *p, const float (*vertCos)[3], int totvert) {
+  if (!p || p->nodes.empty() || p->no_device || !p->build_ok) return;
   if (totvert != p->totvert) {
     sg_set_error("PBVH: Given deforming vcos number does not match PBVH vertex number!");
     return;
@@ -1356,20 +1509,18 @@ void BKE_pbvh_vert_coords_apply(PBVH *p, const float (*vertCos)[3], int totvert)
     p->host_verts = dup;
     p->verts_owned = true;
   }
-  float *src = nullptr;
-  if (sg_check_hip(hipMalloc(&src, 3 * (size_t)totvert * sizeof(float)), "hipMalloc")) return;
+  float *src = (float *)stage(p, 3 * (size_t)totvert * sizeof(float));
+  if (!src) return;
   hipMemcpyAsync(src, vertCos, 3 * (size_t)totvert * sizeof(float), hipMemcpyHostToDevice, p->stream);
   sg_enqueue_coords_apply(p, src);
-  sg_enqueue_full_normals(p, 1);
-  sg_enqueue_clear_marks(p);
   sg_enqueue_flush(p);
-  hipStreamSynchronize(p->stream);
-  hipFree(src);
-  for (PBVHNode &n : p->nodes) {
-    BKE_pbvh_node_mark_update(&n);
-    n.flag &= ~(PBVH_UpdateBB | PBVH_UpdateOriginalBB | PBVH_UpdateNormals);
-  }
-  invalidate_host_views(p);
+  if (sg_check_hip(hipStreamSynchronize(p->stream), "coords apply")) return;
+  if (p->host_verts && !p->host_verts_stale)
+    for (int v = 0; v < totvert; v++) memcpy(p->host_verts[v].co, vertCos[v], 12);
+  for (PBVHNode &n : p->nodes) BKE_pbvh_node_mark_update(&n);
+  std::vector<int> changed;
+  host_flush_bb(p, 0, PBVH_UpdateBB | PBVH_UpdateOriginalBB, changed);  // the flags the refit consumed
+  p->host_bb_stale = true;
 }
 
 float (*BKE_pbvh_vert_coords_alloc(PBVH *p))[3] {

@@ -813,7 +813,7 @@ SgBatch *sg_batch_new(int n_meshes, const float *const *co, const int *const *tr
     const int st[2] = {1, 0};  // serial starts at 1: the fill value of vstamp (-1) is never a serial
     rc = sg_check_hip(hipStreamSynchronize(s), "batch build");
     if (!rc) rc = sg_check_hip(hipMemcpy(b->step, st, sizeof(st), hipMemcpyHostToDevice), "copy");
-    if (!rc) rc = sg_enqueue_full_normals_view(sg_view_of(d, true, false), s, 0);
+    if (!rc) rc = sg_enqueue_full_normals_view(sg_view_of(d, true, false), s);
   }
   if (!rc) rc = sg_check_hip(hipStreamSynchronize(s), "batch build");
   if (rc) {

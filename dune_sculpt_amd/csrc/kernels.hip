@@ -903,22 +903,22 @@ __global__ __launch_bounds__(256) void k_normals_list(SgDevView d, const int *le
 
 // ---------------------------------------------------------------------------
 // Whole-mesh angle-weighted vertex normals (KERNEL_mesh_calc_normals_poly_and_vertex,
-// mesh_normals.c:225-299) as a per-vertex CSR gather in ascending tri order: the poly
-// normal by Newell's method, weighted by the corner angle saacos(-dot(e_prev,
-// e_next)) with e_prev = normalize(c_prev - c_i), e_next = normalize(c_i - c_next);
-// zero result falls back to the normalized position.  mode 1 uses normal_tri_v3 for
-// the face normal (the looptri variant, mesh_normals.c:455-507).
+// mesh_normals.c:225-299) as a per-vertex gather in ascending polygon order: each
+// polygon's normal by Newell's method, weighted by the angle at the vertex's corner,
+// saacos(-dot(e_prev, e_next)) with e_prev = normalize(c_prev - c_i), e_next =
+// normalize(c_i - c_next); a zero sum falls back to the normalized position.  A
+// triangle mesh walks its vertex->tri incidences (each tri its own polygon); a mesh
+// with polygons its vertex->(polygon, corner) list.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float sg_saacos(float f) {
   return (f <= -1.0f) ? 3.14159265358979323846f : ((f >= 1.0f) ? 0.0f : acosf(f));
 }
 
-__global__ __launch_bounds__(256) void k_full_normals(SgDevView d, int mode) {
+__global__ __launch_bounds__(256) void k_full_normals(SgDevView d) {
   const int V = d.V;
   for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    if (mode == 0 && d.vc_off) {
-      // a mesh with polygons: the corners of v's polygons, in polygon order
+    if (d.vc_off) {
       for (int q = d.vc_off[v]; q < d.vc_off[v + 1]; q++) {
         const int2 pc = d.vc[q];
         const int b = d.poly_off[pc.x], n = d.poly_off[pc.x + 1] - b;
@@ -948,22 +948,18 @@ __global__ __launch_bounds__(256) void k_full_normals(SgDevView d, int mode) {
         ay += fy * fac;
         az += fz * fac;
       }
-    } else
-    for (int q = d.vt_off[v]; q < d.vt_off[v + 1]; q++) {
-      const int4 c = d.vt4[q];
-      const int ci = (int)((unsigned)c.w >> 30);
-      const int id[3] = {c.x, c.y, c.z};
-      float P[3][3];
-      for (int j = 0; j < 3; j++) {
-        P[j][0] = d.px[id[j]];
-        P[j][1] = d.py[id[j]];
-        P[j][2] = d.pz[id[j]];
-      }
-      float fx, fy, fz;
-      if (mode == 0) {
-        fx = 0.0f;
-        fy = 0.0f;
-        fz = 0.0f;
+    } else {
+      for (int q = d.vt_off[v]; q < d.vt_off[v + 1]; q++) {
+        const int4 c = d.vt4[q];
+        const int ci = (int)((unsigned)c.w >> 30);
+        const int id[3] = {c.x, c.y, c.z};
+        float P[3][3];
+        for (int j = 0; j < 3; j++) {
+          P[j][0] = d.px[id[j]];
+          P[j][1] = d.py[id[j]];
+          P[j][2] = d.pz[id[j]];
+        }
+        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
         // Newell over corners (v_curr starts at the last corner), add_newell_cross
         int cur = 2;
         for (int nx = 0; nx <= 2; nx++) {
@@ -974,19 +970,16 @@ __global__ __launch_bounds__(256) void k_full_normals(SgDevView d, int mode) {
           cur = nx;
         }
         if (sg_normalize(fx, fy, fz) == 0.0f) fz = 1.0f;
-      } else {
-        sg_tri_normal(P[0][0], P[0][1], P[0][2], P[1][0], P[1][1], P[1][2], P[2][0], P[2][1],
-                      P[2][2], fx, fy, fz);
+        const int ip = (ci + 2) % 3, in = (ci + 1) % 3;
+        float epx = P[ip][0] - P[ci][0], epy = P[ip][1] - P[ci][1], epz = P[ip][2] - P[ci][2];
+        float enx = P[ci][0] - P[in][0], eny = P[ci][1] - P[in][1], enz = P[ci][2] - P[in][2];
+        sg_normalize(epx, epy, epz);
+        sg_normalize(enx, eny, enz);
+        const float fac = sg_saacos(-(epx * enx + epy * eny + epz * enz));
+        ax += fx * fac;
+        ay += fy * fac;
+        az += fz * fac;
       }
-      const int ip = (ci + 2) % 3, in = (ci + 1) % 3;
-      float epx = P[ip][0] - P[ci][0], epy = P[ip][1] - P[ci][1], epz = P[ip][2] - P[ci][2];
-      float enx = P[ci][0] - P[in][0], eny = P[ci][1] - P[in][1], enz = P[ci][2] - P[in][2];
-      sg_normalize(epx, epy, epz);
-      sg_normalize(enx, eny, enz);
-      const float fac = sg_saacos(-(epx * enx + epy * eny + epz * enz));
-      ax += fx * fac;
-      ay += fy * fac;
-      az += fz * fac;
     }
     if (sg_normalize(ax, ay, az) == 0.0f) {
       ax = d.px[v];
@@ -1032,8 +1025,42 @@ __global__ __launch_bounds__(256) void k_refit_all(SgDevView d) {
     d.cut_dirty[c] = 1;
 }
 
-__global__ void k_clear_marks(SgDevView d) {
-  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < d.V; v += gridDim.x * blockDim.x) d.vmark[v] = 0;
+// Vertex boxes of listed leaves into out[8 g] (not into the tree): the host flush of
+// BKE_pbvh_update_bounds with partial flags decides which box each becomes.
+__global__ __launch_bounds__(256) void k_leaf_boxes(SgDevView d, const int *leaves, int n, float *out) {
+  const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
+  const int nw = gridDim.x * blockDim.x / SG_WAVE;
+  for (int g = gw; g < n; g += nw) {
+    const int leaf = leaves[g];
+    const BB6 b = leaf_box(d, make_int2(d.leaf_vbeg[leaf], d.leaf_vbeg[leaf + 1]),
+                           make_int2(d.leaf_sh_off[leaf], d.leaf_sh_off[leaf + 1]));
+    if (lane_id() == 0) bb_store(out + 8 * (size_t)g, b);
+  }
+}
+
+// Box records written by the host flush: .x kind (0 leaf, 1 internal node, 2 search
+// subtree), .y index; then the current and the original box (8 floats each).
+struct SgBoxRec {
+  int kind, idx, pad0, pad1;
+  float vb[8], ob[8];
+};
+
+__global__ __launch_bounds__(256) void k_scatter_boxes(SgDevView d, const SgBoxRec *recs, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 16 * n; i += gridDim.x * blockDim.x) {
+    const SgBoxRec &r = recs[i >> 4];
+    const int k = i & 15;
+    float *vb = r.kind == 0 ? d.leaf_bb : (r.kind == 1 ? d.node_bb : d.cut_bb);
+    float *ob = r.kind == 0 ? d.leaf_obb : (r.kind == 1 ? d.node_obb : d.cut_obb);
+    if (k < 8)
+      vb[8 * (size_t)r.idx + k] = r.vb[k];
+    else
+      ob[8 * (size_t)r.idx + k - 8] = r.ob[k - 8];
+  }
+}
+
+// Host marks (BKE_pbvh_vert_mark_update) join the device's marks: list of new ids.
+__global__ __launch_bounds__(256) void k_set_marks(SgDevView d, const int *verts, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) d.vmark[verts[i]] = 1;
 }
 
 // copy the counters of the last dab of a stroke into its log row
@@ -1353,16 +1380,16 @@ int sg_enqueue_flush(PBVH *p) {
   return sg_check_hip(hipGetLastError(), "flush launch");
 }
 
-int sg_enqueue_full_normals(PBVH *p, int mode) {
+int sg_enqueue_full_normals(PBVH *p) {
   const SgDevView v = sg_view(p);
   const int blocks = (p->d.V + 255) / 256;
-  hipLaunchKernelGGL(k_full_normals, dim3(blocks > 0 ? blocks : 1), dim3(256), 0, p->stream, v, mode);
+  hipLaunchKernelGGL(k_full_normals, dim3(blocks > 0 ? blocks : 1), dim3(256), 0, p->stream, v);
   return sg_check_hip(hipGetLastError(), "normals launch");
 }
 
-int sg_enqueue_full_normals_view(const SgDevView &v, hipStream_t s, int mode) {
+int sg_enqueue_full_normals_view(const SgDevView &v, hipStream_t s) {
   const int blocks = (v.V + 255) / 256;
-  hipLaunchKernelGGL(k_full_normals, dim3(blocks > 0 ? blocks : 1), dim3(256), 0, s, v, mode);
+  hipLaunchKernelGGL(k_full_normals, dim3(blocks > 0 ? blocks : 1), dim3(256), 0, s, v);
   return sg_check_hip(hipGetLastError(), "normals launch");
 }
 
@@ -1395,10 +1422,25 @@ int sg_enqueue_coords_apply(PBVH *p, const float *src_dev) {
   return sg_check_hip(hipGetLastError(), "coords launch");
 }
 
-int sg_enqueue_clear_marks(PBVH *p) {
+int sg_enqueue_leaf_boxes(PBVH *p, const int *leaves_dev, int n, float *out_dev) {
   const SgDevView v = sg_view(p);
-  hipLaunchKernelGGL(k_clear_marks, dim3(256), dim3(256), 0, p->stream, v);
-  return sg_check_hip(hipGetLastError(), "clear launch");
+  if (n > 0) hipLaunchKernelGGL(k_leaf_boxes, dim3(std::max(1, std::min(p->grid_wg, (n + 3) / 4))), dim3(256), 0,
+                                p->stream, v, leaves_dev, n, out_dev);
+  return sg_check_hip(hipGetLastError(), "leaf boxes launch");
+}
+
+int sg_enqueue_scatter_boxes(PBVH *p, const void *recs_dev, int n) {
+  const SgDevView v = sg_view(p);
+  if (n > 0) hipLaunchKernelGGL(k_scatter_boxes, dim3(std::max(1, std::min(1024, (16 * n + 255) / 256))), dim3(256), 0,
+                                p->stream, v, (const SgBoxRec *)recs_dev, n);
+  return sg_check_hip(hipGetLastError(), "scatter launch");
+}
+
+int sg_enqueue_set_marks(PBVH *p, const int *verts_dev, int n) {
+  const SgDevView v = sg_view(p);
+  if (n > 0) hipLaunchKernelGGL(k_set_marks, dim3(std::max(1, std::min(1024, (n + 255) / 256))), dim3(256), 0,
+                                p->stream, v, verts_dev, n);
+  return sg_check_hip(hipGetLastError(), "marks launch");
 }
 
 int sg_enqueue_gather(PBVH *p, int which, float *out_dev) {

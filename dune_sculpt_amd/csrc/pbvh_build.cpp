@@ -574,6 +574,7 @@ void sg_build_device(PBVH *p, const float *co) {
   upload(&d.vv_off, h.vv_off.data(), V + 1, s);
   upload(&d.vv_idx, h.vv_idx.data(), h.vv_idx.size(), s);
   upload(&d.leaf_vbeg, h.leaf_vbeg.data(), L + 1, s);
+  p->leaf_vbeg = h.leaf_vbeg;
   upload(&d.leaf_sh_off, h.leaf_sh_off.data(), L + 1, s);
   upload(&d.leaf_sh, h.leaf_sh.data(), h.leaf_sh.size(), s);
   upload(&d.leaf_flags, h.leaf_flags.data(), L, s);
@@ -766,7 +767,7 @@ void sg_free_device(PBVH *p) {
                   d.draw_dirty, d.draw_tri, d.draw_off, d.draw_leaf, d.draw_fset, d.draw_poly_off,
                   d.draw_poly_v, d.draw_vbo, d.undo, d.undo_stamp, d.undo_slot, d.place_pair,
                   d.prox_dev, d.prox_tab, d.prox_leaves, d.prox_off, d.poly_off, d.poly_v,
-                  d.poly_of_tri, d.vc_off, d.vc};
+                  d.poly_of_tri, d.vc_off, d.vc, d.stage};
   for (void *q : ptrs) {
     if (q) hipFree(q);
   }

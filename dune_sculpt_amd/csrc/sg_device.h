@@ -523,7 +523,19 @@ __device__ __forceinline__ void vertex_normal(const SgDevView &d, int v, int b, 
 // Leaf bounds over ALL its vertices, uniq then shared (update_node_vb pbvh.c:108-128,
 // BB_expand in vert_indices order).  Each lane folds a contiguous chunk, then the
 // wave combines lanes in order.  Writes vb and orig_vb (UpdateOriginalBB).
+__device__ __forceinline__ BB6 leaf_box(const SgDevView &d, int2 vr, int2 sr);
 __device__ __forceinline__ void leaf_refit(const SgDevView &d, int leaf, int2 vr, int2 sr, long long &nsh) {
+  const BB6 acc = leaf_box(d, vr, sr);
+  if (lane_id() == 0) {
+    bb_store(d.leaf_bb + 8 * (size_t)leaf, acc);
+    bb_store(d.leaf_obb + 8 * (size_t)leaf, acc);
+    nsh += sr.y - sr.x;
+  }
+}
+
+// The box over a leaf's vertices (uniq range vr, then its shared vertices sr) in
+// vert_indices order; lane 0 holds the result.
+__device__ __forceinline__ BB6 leaf_box(const SgDevView &d, int2 vr, int2 sr) {
   const int nu = vr.y - vr.x, ns = sr.y - sr.x, tot = nu + ns;
   const int per = (tot + SG_WAVE - 1) / SG_WAVE;
   const int lane = lane_id();
@@ -551,12 +563,7 @@ __device__ __forceinline__ void leaf_refit(const SgDevView &d, int leaf, int2 vr
       if (vid[j] >= 0) bb_add_point(acc, x[j], y[j], z[j]);
     }
   }
-  acc = bb_wave_reduce(acc);
-  if (lane == 0) {
-    bb_store(d.leaf_bb + 8 * (size_t)leaf, acc);
-    bb_store(d.leaf_obb + 8 * (size_t)leaf, acc);
-    nsh += ns;
-  }
+  return bb_wave_reduce(acc);
 }
 
 

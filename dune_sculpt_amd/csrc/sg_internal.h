@@ -154,6 +154,10 @@ struct SgDev {
   int prox_tab_cap = 0;
   long long *prox_off = nullptr;  // float offset of each proxy in prox_dev
   int prox_off_cap = 0;
+  // grow-only staging for the host-driven entry points (lists, records, coordinate
+  // uploads and readbacks): allocated on first use, reused after
+  void *stage = nullptr;
+  size_t stage_cap = 0;         // bytes
 };
 
 // Pinned host chunk the proxies are carved from: the caller fills them in place and
@@ -203,6 +207,8 @@ struct PBVH {
   MLoopTri *looptri = nullptr;       // owned (freed at BKE_pbvh_free)
   std::vector<float> host_normals;   // [V*3] original order view
   std::vector<uint8_t> host_vmark;   // BKE_pbvh_vert_mark_update marks
+  std::vector<int> host_mark_list;   // the marked vertices (original ids), each once
+  std::vector<int> leaf_vbeg;        // [L+1] each leaf's uniq range in the new numbering
   bool host_bb_stale = false;
   bool host_verts_stale = false;
   bool host_normals_stale = false;

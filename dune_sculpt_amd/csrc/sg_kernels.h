@@ -97,7 +97,7 @@ enum { SG_K_SEARCH = 0, SG_K_AREA = 1, SG_K_BRUSH = 2, SG_K_SMOOTH = 3, SG_K_POS
 
 SgDevView sg_view(PBVH *p);
 SgDevView sg_view_of(SgDev &d, bool respect_hide, bool halt_live);
-int sg_enqueue_full_normals_view(const SgDevView &v, hipStream_t s, int mode);
+int sg_enqueue_full_normals_view(const SgDevView &v, hipStream_t s);
 int sg_enqueue_gather_raw(hipStream_t s, const float *x, const float *y, const float *z, const int *perm_old, int V,
                           float *out_dev);
 int sg_profile_collect(PBVH *p);
@@ -107,10 +107,12 @@ int sg_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long long *
 int sg_enqueue_clear_ctr(PBVH *p);
 int sg_enqueue_flush(PBVH *p);
 int sg_enqueue_log_row(PBVH *p, long long *log, int row);
-int sg_enqueue_full_normals(PBVH *p, int mode);
+int sg_enqueue_full_normals(PBVH *p);
 int sg_enqueue_refit_list(PBVH *p, const int *leaves_dev, int n);
 int sg_enqueue_normals_list(PBVH *p, const int *leaves_dev, int n);
 int sg_enqueue_coords_apply(PBVH *p, const float *src_dev);
-int sg_enqueue_clear_marks(PBVH *p);
+int sg_enqueue_leaf_boxes(PBVH *p, const int *leaves_dev, int n, float *out_dev);
+int sg_enqueue_scatter_boxes(PBVH *p, const void *recs_dev, int n);
+int sg_enqueue_set_marks(PBVH *p, const int *verts_dev, int n);
 int sg_enqueue_gather(PBVH *p, int which, float *out_dev);
 const char *sg_last_error_str();

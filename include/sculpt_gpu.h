@@ -324,6 +324,9 @@ int sg_pbvh_tree_info(PBVH *pbvh, int out[4]);
 /* Node index of a handle (its slot in the reference's nodes array order). */
 int sg_pbvh_node_index(PBVH *pbvh, PBVHNode *node);
 PBVHNode *sg_pbvh_node_at(PBVH *pbvh, int index);
+/* A node's PBVHNodeFlags (the reference's callers read node->flag directly,
+ * pbvh_intern.h:69; the handle is opaque here). */
+int sg_pbvh_node_flag(PBVHNode *node);
 /* prim_indices array (looptri order of the leaves) */
 int sg_pbvh_prim_indices(PBVH *pbvh, int *out, int cap);
 

@@ -1195,17 +1195,16 @@ static int cmp_int(const void *pa, const void *pb)
 
 static inline float mask_of(const OrcPBVH *o, int v) { return o->mask ? o->mask[v] : 0.0f; }
 
-/* BKE_pbvh_update_bounds(UpdateBB|UpdateOriginalBB|UpdateRedraw): pbvh.c:1401-1421 ->
- * :1206-1242 -> update_node_vb :108-128 -> flush :1369-1399. */
-static void update_bounds(OrcPBVH *o, int nthreads)
+/* BKE_pbvh_update_bounds(flag): pbvh.c:1401-1421 -> pbvh_update_BB_redraw :1206-1242 ->
+ * update_node_vb :108-128 -> pbvh_flush_bb :1369-1399. */
+static void update_bounds_flag(OrcPBVH *o, int flag, int nthreads)
 {
   int *upd;
-  int flag = ORC_UpdateBB | ORC_UpdateOriginalBB | ORC_UpdateRedraw;
   const int nupd = search_gather(o, update_cb, &flag, &upd, NULL);
 #pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
   for (int li = 0; li < nupd; li++) {
     ONode *n = &o->nodes[upd[li]];
-    if (n->flag & ORC_UpdateBB) {
+    if ((flag & ORC_UpdateBB) && (n->flag & ORC_UpdateBB)) {
       OBB vb;
       bb_reset(&vb);
       const int tot = n->uniq_verts + n->face_verts;
@@ -1214,12 +1213,17 @@ static void update_bounds(OrcPBVH *o, int nthreads)
       }
       n->vb = vb;
     }
-    if (n->flag & ORC_UpdateOriginalBB) {
+    if ((flag & ORC_UpdateOriginalBB) && (n->flag & ORC_UpdateOriginalBB)) {
       n->orig_vb = n->vb;
     }
-    n->flag &= ~ORC_UpdateRedraw;
+    if ((flag & ORC_UpdateRedraw) && (n->flag & ORC_UpdateRedraw)) {
+      n->flag &= ~ORC_UpdateRedraw;
+    }
   }
   free(upd);
+  if (!(flag & (ORC_UpdateBB | ORC_UpdateOriginalBB))) {
+    return;
+  }
   /* pbvh_flush_bb: serial recursion over the whole tree (explicit stack,
    * children before parents). */
   int *st = (int *)malloc(sizeof(int) * 2 * (size_t)(o->totnode + 2));
@@ -1233,10 +1237,14 @@ static void update_bounds(OrcPBVH *o, int nthreads)
     ONode *n = &o->nodes[ni];
     if (n->flag & ORC_Leaf) {
       int u = 0;
-      u |= n->flag & ORC_UpdateBB;
-      n->flag &= ~ORC_UpdateBB;
-      u |= n->flag & ORC_UpdateOriginalBB;
-      n->flag &= ~ORC_UpdateOriginalBB;
+      if (flag & ORC_UpdateBB) {
+        u |= n->flag & ORC_UpdateBB;
+        n->flag &= ~ORC_UpdateBB;
+      }
+      if (flag & ORC_UpdateOriginalBB) {
+        u |= n->flag & ORC_UpdateOriginalBB;
+        n->flag &= ~ORC_UpdateOriginalBB;
+      }
       upd_of[ni] = u;
       continue;
     }
@@ -1264,6 +1272,12 @@ static void update_bounds(OrcPBVH *o, int nthreads)
   }
   free(st);
   free(upd_of);
+}
+
+/* The bounds update a brush step ends with (UpdateBB | UpdateOriginalBB | UpdateRedraw). */
+static void update_bounds(OrcPBVH *o, int nthreads)
+{
+  update_bounds_flag(o, ORC_UpdateBB | ORC_UpdateOriginalBB | ORC_UpdateRedraw, nthreads);
 }
 
 /* Normals of the flagged vertices of UpdateNormals leaves (pbvh_faces_update_normals
@@ -1802,4 +1816,51 @@ int64_t orc_update_normals_bounds(OrcPBVH *o)
   const int64_t t = update_normals(o, orc_get_threads());
   update_bounds(o, orc_get_threads());
   return t;
+}
+
+/* ---- the drop-in update API, restated for the host-edit tests -------------- */
+
+/* A caller's edit of one vertex through the mesh arrays (BKE_pbvh_get_verts). */
+void orc_set_vert(OrcPBVH *o, int v, const float co[3])
+{
+  memcpy(o->co + 3 * (size_t)v, co, sizeof(float) * 3);
+  if (o->co64) {
+    for (int k = 0; k < 3; k++) {
+      o->co64[3 * (size_t)v + k] = co[k];
+    }
+  }
+}
+
+/* BKE_pbvh_vert_mark_update: pbvh.c:1811-1815. */
+void orc_vert_mark_update(OrcPBVH *o, int v) { o->vert_bitmap[v] = 1; }
+
+/* BKE_pbvh_node_mark_update: pbvh.c:1723-1727. */
+void orc_node_mark_update(OrcPBVH *o, int node)
+{
+  o->nodes[node].flag |= ORC_UpdateNormals | ORC_UpdateBB | ORC_UpdateOriginalBB | ORC_UpdateDrawBuffers |
+                         ORC_UpdateRedraw;
+}
+
+void orc_update_bounds(OrcPBVH *o, int flag) { update_bounds_flag(o, flag, orc_get_threads()); }
+
+void orc_update_normals(OrcPBVH *o) { update_normals(o, orc_get_threads()); }
+
+int orc_node_flag(const OrcPBVH *o, int node) { return o->nodes[node].flag; }
+
+/* BKE_pbvh_vert_coords_apply: pbvh.c:4707-4749.  Each position that differs bitwise is
+ * replaced and its vertex marked; BKE_mesh_calc_normals_looptri computes into a
+ * temporary it frees (mesh_normals.c:455-507), so the vertex normals are unchanged;
+ * every node is marked and the bounds updated with UpdateBB | UpdateOriginalBB. */
+void orc_vert_coords_apply(OrcPBVH *o, const float *vert_cos)
+{
+  for (int v = 0; v < o->totvert; v++) {
+    if (memcmp(o->co + 3 * (size_t)v, vert_cos + 3 * (size_t)v, sizeof(float) * 3) != 0) {
+      orc_set_vert(o, v, vert_cos + 3 * (size_t)v);
+      o->vert_bitmap[v] = 1;
+    }
+  }
+  for (int i = 0; i < o->totnode; i++) {
+    orc_node_mark_update(o, i);
+  }
+  update_bounds_flag(o, ORC_UpdateBB | ORC_UpdateOriginalBB, orc_get_threads());
 }

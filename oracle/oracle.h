@@ -138,6 +138,17 @@ void orc_poly_normal(const float *co, const int *corners, int n, float out[3]);
  * ids.  Normals then follow the polygon rules; the initial normals are recomputed. */
 void orc_set_polys(OrcPBVH *o, const int *tri_poly, const int *poly_off, const int *poly_verts, int npoly);
 
+/* The drop-in update API (host edits): set a vertex, the mark helpers, bounds with a
+ * flag, normals of the marked vertices of UpdateNormals leaves, node flags, and the
+ * whole-mesh coordinate replace. */
+void orc_set_vert(OrcPBVH *o, int v, const float co[3]);
+void orc_vert_mark_update(OrcPBVH *o, int v);
+void orc_node_mark_update(OrcPBVH *o, int node);
+void orc_update_bounds(OrcPBVH *o, int flag);
+void orc_update_normals(OrcPBVH *o);
+int orc_node_flag(const OrcPBVH *o, int node);
+void orc_vert_coords_apply(OrcPBVH *o, const float *vert_cos);
+
 /* Ray cast (ray.c).  One ray against the current positions; depth0 bounds the hit
  * (hits at depth >= depth0 are ignored).  Returns hit.  stats (may be NULL): node
  * box tests, leaves passing the traversal, leaves ray-tested, tris tested. */

@@ -87,6 +87,13 @@ def lib():
         "orc_calc_initial_normals_poly": (None, [f32p, i32, i32p, i32p, i32, f32p]),
         "orc_poly_normal": (None, [f32p, i32p, i32, f32p]),
         "orc_set_polys": (None, [vp, i32p, i32p, i32p, i32]),
+        "orc_set_vert": (None, [vp, i32, f32p]),
+        "orc_vert_mark_update": (None, [vp, i32]),
+        "orc_node_mark_update": (None, [vp, i32]),
+        "orc_update_bounds": (None, [vp, i32]),
+        "orc_update_normals": (None, [vp]),
+        "orc_node_flag": (i32, [vp, i32]),
+        "orc_vert_coords_apply": (None, [vp, f32p]),
         "orc_set_threads": (None, [i32]),
         "orc_get_threads": (i32, []),
         "orc_last_stats": (None, [vp, C.POINTER(C.c_int64)]),
@@ -329,6 +336,28 @@ class Oracle:
         d = np.array(direction, np.float32)
         moved = lib().orc_project_ray_root(self.h, 1 if original else 0, _fp(a), _fp(e), _fp(d))
         return bool(moved), a, e
+
+    # -- the drop-in update API
+    def set_vert(self, v, co):
+        lib().orc_set_vert(self.h, int(v), _fp(np.ascontiguousarray(co, np.float32)))
+
+    def vert_mark_update(self, v):
+        lib().orc_vert_mark_update(self.h, int(v))
+
+    def node_mark_update(self, node):
+        lib().orc_node_mark_update(self.h, int(node))
+
+    def update_bounds(self, flag):
+        lib().orc_update_bounds(self.h, int(flag))
+
+    def update_normals(self):
+        lib().orc_update_normals(self.h)
+
+    def node_flag(self, node):
+        return lib().orc_node_flag(self.h, int(node))
+
+    def vert_coords_apply(self, co):
+        lib().orc_vert_coords_apply(self.h, _fp(np.ascontiguousarray(co, np.float32)))
 
     def last_stats(self):
         out = np.zeros(8, np.int64)

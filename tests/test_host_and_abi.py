@@ -152,3 +152,28 @@ def test_host_proxies_worklist(sg, po):
     L.BKE_pbvh_gather_proxies(p.h, sg.C.byref(arr), sg.C.byref(tot))
     assert tot.value == 0
     p.close()
+
+
+def test_frustum_search_host_tree(sg, po):
+    """BKE_pbvh_search_gather with the frustum callbacks (pbvh.c:4507-4557) on the host
+    tree: six axis-aligned planes bound lo <= x <= hi, so a box is outside when some
+    bmax < lo or bmin > hi and inside when every bmin > lo and bmax < hi; boxes nest, so
+    the search returns exactly the passing leaves, left to right."""
+    co, tris = sg.noisy_icosphere(4, seed=2, amplitude=0.02)
+    p = _host_tree(sg, co, tris, 32)
+    o = po.Oracle(co, tris, leaf_limit=32)
+    lo, hi = np.float32([-0.4, -0.1, -0.2]), np.float32([0.6, 0.7, 0.9])
+    planes = np.array([[1, 0, 0, -lo[0]], [0, 1, 0, -lo[1]], [0, 0, 1, -lo[2]],
+                       [-1, 0, 0, hi[0]], [0, -1, 0, hi[1]], [0, 0, -1, hi[2]]], np.float32)
+    for exclude in (False, True):
+        want = []
+        for n in o.leaves_dfs():
+            vb, _ = o.node_bb(n)
+            outside = (vb[3:] < lo).any() or (vb[:3] > hi).any()
+            inside = (vb[:3] > lo).all() and (vb[3:] < hi).all()
+            if (not inside) if exclude else (not outside):
+                want.append(int(n))
+        got = p.frustum_gather(planes, exclude=exclude)
+        assert got == want and 0 < len(got) < len(o.leaves_dfs())
+    p.close()
+    o.close()
