@@ -546,6 +546,53 @@ def cpu_draw(o, tris, budget_s):
             "sample": f"{n} tris of the same mesh's leaves ({dt:.1f}s, C oracle, one thread, Python per leaf)"}
 
 
+def run_c1(sg, W, K):
+    """The reference's CPU-sized case: icosphere-6 (40,962 V), 200 Draw dabs, r = 0.1,
+    s = 0.5, smooth falloff, great circle with spacing 0.1 r, leaves <= 128 tris.  GPU:
+    the stroke as a captured graph from the pristine mesh (HIP events around it); CPU:
+    the oracle on the same dabs with 1 thread and with every host thread."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle as po
+
+    co, tris = sg.icosphere(6)
+    n = 200
+    radii = np.full(n, 0.1, np.float32)
+    dabs = [sg.make_dab(tool=sg.TOOL_DRAW, location=c, radius=0.1, strength=0.5)
+            for c in sg.great_circle_centres(1, radii)]
+    P = sg.PBVH(co, tris, leaf_limit=128)
+    P.snapshot()
+    prep = P.prepare(dabs, log=True)
+    times = []
+    for s in range(W + max(K, 5)):
+        P.restore()
+        P.sync()
+        P.timer_start()
+        prep.launch(sync=False)
+        ms = P.timer_stop()
+        if s >= W:
+            times.append(ms)
+    aff = int(P.stroke_log(n)[:, 1].sum())
+    prep.close()
+    P.close()
+    gpu_ms = float(np.median(times))
+    out = {"workload": "icosphere-6 (40962 V, 81920 T), 200 Draw dabs r=0.1 s=0.5 smooth falloff, leaf<=128 tris",
+           "affected": aff, "gpu_ms_per_stroke": gpu_ms, "gpu_us_per_dab": gpu_ms * 1e3 / n,
+           "gpu_vertex_updates_per_s": aff / (gpu_ms * 1e-3)}
+    for threads in (1, os.cpu_count() or 1):
+        po.set_threads(threads)
+        o = po.Oracle(co, tris, leaf_limit=128, fp64=False)
+        used = po.lib().orc_get_threads()
+        t0 = time.perf_counter()
+        a = sum(len(o.dab(d)) for d in dabs)
+        dt = time.perf_counter() - t0
+        o.close()
+        assert a == aff, "CPU and GPU affected counts differ"
+        out[f"cpu_{used}t_ms_per_stroke"] = dt * 1e3
+        out[f"cpu_{used}t_vertex_updates_per_s"] = a / dt
+    log(f"c1: {out}")
+    return out
+
+
 def cpu_baseline(sg, co, tris, dabs, budget_s, threads, before=None):
     """The oracle's dab loop on the same mesh; `before(oracle)` runs first on the fresh
     mesh (the ray cast sample shares the build)."""
@@ -607,6 +654,7 @@ def main():
     ap.add_argument("--no-undo", action="store_true", help="skip the undo-step measurement")
     ap.add_argument("--only-undo", action="store_true", help="run only the undo-step measurement")
     ap.add_argument("--no-elastic", action="store_true", help="skip the elastic deform measurement")
+    ap.add_argument("--no-c1", action="store_true", help="skip the icosphere-6 CPU-reference case")
     ap.add_argument("--only-elastic", action="store_true", help="run only the elastic deform measurement")
     ap.add_argument("--placed-only", action="store_true",
                     help="placed stroke without the fixed-centre reference stroke (one captured graph)")
@@ -776,6 +824,7 @@ def main():
         placed = None if args.no_placed else run_placed(sg, A, dabs, W, K)
     draw = None if (args.no_ray or args.no_draw) else run_draw(sg, A, dabs, W, max(K, 3))
 
+    c1 = None if args.no_c1 else run_c1(sg, W, K)
     c3 = None
     if not args.no_learned:
         c3 = run_learned(sg, A, W, args)
@@ -840,6 +889,7 @@ def main():
         "event_s": event_s,
         "alg_bytes_per_dab_by_kernel": kernel_bytes_per_dab,
         "hbm_copy": copy,
+        "c1": c1,
         "learned_smooth": c3,
         "batch": batch,
         "raycast": ray,

@@ -414,6 +414,12 @@ int sg_pbvh_set_grid(PBVH *p, int workgroups) {
   return 0;
 }
 
+int sg_pbvh_set_fused(PBVH *p, int on) {
+  if (!p) return sg_set_error("null PBVH");
+  p->fused_enabled = on != 0;
+  return p->d.halo_ok ? 1 : 0;
+}
+
 int sg_pbvh_set_kernel_grids(PBVH *p, int brush, int post, int smooth) {
   if (!p) return sg_set_error("null PBVH");
   for (int g : {brush, post, smooth})
@@ -496,6 +502,7 @@ int sg_dab_apply(PBVH *p, const SgDabParams *dab) {
   p->halt_live = k.learned && p->mlp;
   int rc = sg_enqueue_clear_ctr(p);
   if (!rc) rc = sg_enqueue_dab(p, k, curve, nullptr, 0, stamp, 0);
+  if (!rc) rc = sg_enqueue_finish(p, 0);
   if (!rc) rc = sg_enqueue_flush(p);
   long long ctr[SG_CTR_COUNT];
   int halt = 0;
@@ -632,6 +639,7 @@ static int stroke_apply(PBVH *p, const SgDabParams *dabs, const SgRay *rays, int
           return rc;
         if (flush_each && sg_enqueue_flush(p)) return -1;
       }
+      if (sg_enqueue_finish(p, n - 1 - a0)) return -1;
       if (!flush_each && sg_enqueue_flush(p)) return -1;
       if (log && sg_enqueue_log_row(p, p->d.log, n - 1 - a0)) return -1;
       return 0;
@@ -807,6 +815,7 @@ static SgStroke *stroke_prepare(PBVH *p, const SgDabParams *dabs, const SgRay *r
       if (!erc) erc = sg_enqueue_dab(p, k, curve, log ? p->d.log : nullptr, i, 0, i, nullptr, place);
       if (!erc && flush_each) erc = sg_enqueue_flush(p);
     }
+    if (!erc) erc = sg_enqueue_finish(p, n - 1);
     if (!erc && !flush_each) erc = sg_enqueue_flush(p);
     if (!erc && log) erc = sg_enqueue_log_row(p, p->d.log, n - 1);
     const hipError_t e = hipStreamEndCapture(p->stream, &s->graph);

@@ -55,23 +55,62 @@ int sg_check_hip(hipError_t e, const char *what) {
 // it into the reference's left->right order when asked).  The first wave also logs
 // the previous dab's counters and clears the next dab's counter set.
 // ---------------------------------------------------------------------------
+// The last `copy_blocks` workgroups of the search bring in the results of the previous
+// dab when it was a fused dab (fused kernel below): it wrote its affected vertices'
+// positions into the shadow copy and their normals into (mx, my, mz), because its own
+// workgroups read those arrays' old values; the copy restores positions == shadow copy.
+__device__ __forceinline__ void fused_copy_back(const SgDevView &d, int stamp, int tid, int nthreads) {
+  const int ng = d.pend[0], st = d.pend[1];
+  if (st != stamp - 1 || ng <= 0) return;
+  const int na = ng << d.aff_shift;
+  for (int i = tid; i < na; i += nthreads) {
+    const int v = aff_slot_vertex(d, i, d.aff_cnt[i >> d.aff_shift]);
+    if (v < 0) continue;
+    const float x = d.qx[v], y = d.qy[v], z = d.qz[v];
+    const float a = d.mx[v], b = d.my[v], c = d.mz[v];
+    d.px[v] = x;
+    d.py[v] = y;
+    d.pz[v] = z;
+    d.nx[v] = a;
+    d.ny[v] = b;
+    d.nz[v] = c;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_search(SgDevView d, SgDabK k, long long *log, int log_index,
-                                                int serial, const float *place) {
+                                                int serial, const float *place, int copy_blocks) {
   if (d.halt && d.halt[0]) return;
+  const int stamp = d.dstamp ? d.dstamp[serial & 1] : 0;
+  if ((int)blockIdx.x >= (int)gridDim.x - copy_blocks) {
+    const int cb = blockIdx.x - (gridDim.x - copy_blocks);
+    fused_copy_back(d, stamp, cb * blockDim.x + threadIdx.x, copy_blocks * blockDim.x);
+    return;
+  }
   sg_place(k, place);
   const int lane = lane_id();
   const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
-  const int nw = gridDim.x * blockDim.x / SG_WAVE;
+  const int nw = (gridDim.x - copy_blocks) * blockDim.x / SG_WAVE;
   // bookkeeping in the last wave, which has the fewest or no subtrees to test, so its
   // loads stay off the searching waves' chains
   const int bw = nw - 1;
-  if (gw == bw && lane < SG_CTR_COUNT) {
-    const long long prev = d.ctr_next[lane];  // the set the previous dab used
-    if (log && log_index > 0)
+  if (gw == bw && log && log_index > 0) {
+    // a fused previous dab left its affected count as per-leaf counts: sum them
+    long long naff = -1;
+    if (d.pend && d.pend[1] == stamp - 1) {
+      const int ngp = d.pend[0];
+      int s = 0;
+      for (int g = lane; g < ngp; g += SG_WAVE) s += d.aff_cnt[g];
+      naff = wave_sum_i(s);
+    }
+    if (lane < SG_CTR_COUNT) {
+      long long prev = d.ctr_next[lane];  // the set the previous dab used
+      if (lane == SG_CTR_NAFF && naff >= 0) prev = naff;
       log[(size_t)(log_index - 1) * SG_CTR_COUNT + lane] = lane == SG_CTR_NGATHER ? (prev & 0xffffffffll) : prev;
-    d.ctr_next[lane] = 0;
+    }
   }
+  if (gw == bw && lane < SG_CTR_COUNT) d.ctr_next[lane] = 0;
   if (gw == bw && lane == 0 && d.seq) d.seq[0] += 1;  // this dab's stamp for the later kernels
+  if (gw == bw && lane == 0 && d.dstamp) d.dstamp[(serial & 1) ^ 1] = stamp + 1;  // the next dab's
   if (gw == bw && lane == 0 && undo_active(d))
     atomicAdd((unsigned long long *)&d.ctr[SG_CTR_NGATHER], SG_NGATHER_UNDO);
   const float *cut_bbs = k.use_orig ? d.cut_obb : d.cut_bb;
@@ -99,10 +138,12 @@ __global__ __launch_bounds__(256) void k_search(SgDevView d, SgDabK k, long long
     for (int q0 = 0; q0 < nl; q0 += SG_WAVE * SEARCH_UNROLL) {
       bool pass[SEARCH_UNROLL];
       int2 vr[SEARCH_UNROLL], sr[SEARCH_UNROLL];
+      int4 hh[SEARCH_UNROLL], hr[SEARCH_UNROLL];
 #pragma unroll
       for (int j = 0; j < SEARCH_UNROLL; j++) {
         const int l = lb + q0 + j * SG_WAVE + lane;
         pass[j] = false;
+        hh[j] = hr[j] = make_int4(0, 0, 0, 0);
         if (l < le) {
           const BB6 b = bb_load(leaf_bbs + 8 * (size_t)l);
           const int fl = d.leaf_flags[l];
@@ -110,6 +151,10 @@ __global__ __launch_bounds__(256) void k_search(SgDevView d, SgDabK k, long long
           // the writes after the slot reservation wait on nothing else
           vr[j] = make_int2(d.leaf_vbeg[l], d.leaf_vbeg[l + 1]);
           sr[j] = make_int2(d.leaf_sh_off[l], d.leaf_sh_off[l + 1]);
+          if (k.fused) {
+            hh[j] = d.hl_hdr[l];
+            hr[j] = d.hl_ring[l];
+          }
           if (refresh) {
             bb_add_bb(tb, b);
             bb_add_bb(ab, bb_load(leaf_alt + 8 * (size_t)l));
@@ -133,9 +178,15 @@ __global__ __launch_bounds__(256) void k_search(SgDevView d, SgDabK k, long long
       for (int j = 0; j < SEARCH_UNROLL; j++) {
         if (pass[j]) {
           const int pos = off + mask_prefix(m[j]);
-          d.gathered[pos] = lb + q0 + j * SG_WAVE + lane;
+          const int l = lb + q0 + j * SG_WAVE + lane;
+          d.gathered[pos] = l;
           d.gl_v[pos] = vr[j];
           d.gl_s[pos] = sr[j];
+          if (k.fused) {
+            d.gl_h[pos] = hh[j];
+            d.gl_r[pos] = hr[j];
+            d.leaf_gstamp[l] = stamp;
+          }
         }
         off += __popcll(m[j]);
       }
@@ -159,10 +210,12 @@ __global__ __launch_bounds__(256) void k_search(SgDevView d, SgDabK k, long long
   }
 }
 
-// zero both counter sets and the halt flag at the start of a stroke
-__global__ void k_clear_ctr(long long *ctr, int *halt) {
+// zero both counter sets and the halt flag at the start of a stroke; the dab stamps
+// only grow (the first dab's stamp is above every earlier one)
+__global__ void k_clear_ctr(long long *ctr, int *halt, int *dstamp) {
   if (threadIdx.x < 2 * SG_CTR_COUNT) ctr[threadIdx.x] = 0;
   if (threadIdx.x == 0) halt[0] = 0;
+  if (threadIdx.x == 0 && dstamp) dstamp[0] = max(dstamp[0], dstamp[1]) + 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -720,6 +773,243 @@ __global__ __launch_bounds__(256) void k_post(SgDevView d, int stats, int stamp,
 }
 
 // ---------------------------------------------------------------------------
+// Fused dab: one workgroup per gathered leaf runs the brush, Smooth's Jacobi steps, the
+// normals of the leaf's affected vertices and the leaf's box in one pass, on the leaf's
+// halo staged in LDS (its uniq vertices, ring 0 = its shared vertices and their edge
+// neighbours' ring, and for Smooth(k) k more rings; tables from the build).  Every
+// halo vertex's displacement is computed here from the same inputs and in the same
+// FP32 order as the vertex's own workgroup computes it (a vertex of ring j is exact
+// after k - j Jacobi steps), so the results equal the separate kernels' bit for bit.
+// Workgroups read other leaves' vertices as they were before the dab, so the results
+// go to the shadow copy (positions) and to (mx, my, mz) (normals); the next dab's
+// search, or k_dab_finish, copies them in.  A halo vertex counts as affected only when
+// its own leaf was gathered (the search stamps gathered leaves), as the search and the
+// per-leaf vertex test decide it for that leaf.
+// ---------------------------------------------------------------------------
+#define FUSED_THREADS 256
+
+struct FusedLds {
+  float *x[2], *y[2], *z[2], *fade;
+  unsigned char *aff;
+  unsigned short *nbl, *shl;
+};
+
+__device__ __forceinline__ FusedLds fused_lds(unsigned char *base, int cap, int nbl_cap, int sh_cap) {
+  FusedLds L;
+  float *f = (float *)base;
+  for (int b = 0; b < 2; b++) {
+    L.x[b] = f;
+    L.y[b] = f + cap;
+    L.z[b] = f + 2 * cap;
+    f += 3 * cap;
+  }
+  L.fade = f;
+  f += cap;
+  L.nbl = (unsigned short *)f;
+  L.shl = L.nbl + nbl_cap;
+  L.aff = (unsigned char *)(L.shl + sh_cap);
+  return L;
+}
+
+__global__ __launch_bounds__(FUSED_THREADS) void k_dab_fused(SgDevView d, SgDabK k, const float *curve, int serial,
+                                                            int cap, int nbl_cap, int sh_cap) {
+  if (d.halt && d.halt[0]) return;
+  extern __shared__ __align__(16) unsigned char fused_smem[];
+  __shared__ int s_cnt;
+  const FusedLds S = fused_lds(fused_smem, cap, nbl_cap, sh_cap);
+  const int tid = threadIdx.x;
+  const long long ng_ctr = d.ctr[SG_CTR_NGATHER];
+  const int ng = (int)ng_ctr;
+  const int stamp = d.dstamp[serial & 1];
+  const bool smooth = k.tool == SG_TOOL_SMOOTH;
+  const int K = smooth ? k.iters : 0;
+  if (blockIdx.x == 0 && tid == 0) {  // this dab's results wait for the copy-back
+    d.pend[0] = ng;
+    d.pend[1] = stamp;
+  }
+  // Draw offset: (normalize(area) * r) * bstrength
+  float ox = 0.0f, oy = 0.0f, oz = 0.0f;
+  if (k.tool == SG_TOOL_DRAW) {
+    float ax = (float)((double)d.ctr[SG_CTR_AREA_X] * (1.0 / 4294967296.0));
+    float ay = (float)((double)d.ctr[SG_CTR_AREA_Y] * (1.0 / 4294967296.0));
+    float az = (float)((double)d.ctr[SG_CTR_AREA_Z] * (1.0 / 4294967296.0));
+    sg_normalize(ax, ay, az);
+    ox = (ax * k.r) * k.bstr;
+    oy = (ay * k.r) * k.bstr;
+    oz = (az * k.r) * k.bstr;
+  }
+  for (int g = blockIdx.x; g < ng; g += gridDim.x) {
+    const int leaf = d.gathered[g];
+    const int2 vr = d.gl_v[g], sr = d.gl_s[g];
+    const int4 hh = d.gl_h[g], hr = d.gl_r[g];
+    const int nu = vr.y - vr.x, ns = sr.y - sr.x;
+    const int re[SG_HALO_RINGS + 1] = {hh.y, hr.x, hr.y, hr.z, hr.w};  // ext count at each ring's end
+    const int nloc = nu + re[K];
+    const int ninner = K > 0 ? nu + re[K - 1] : 0;  // locals the first Jacobi step updates
+    if (tid == 0) s_cnt = 0;
+    if ((ng_ctr & SG_NGATHER_UNDO) && tid < SG_WAVE) undo_save_leaf(d, leaf, vr, d.px, d.py, d.pz);
+    for (int i = tid; i < ns; i += FUSED_THREADS) S.shl[i] = d.hl_sh[sr.x + i];
+    if (K > 0) {
+      const int nb0 = d.hl_nboff[hh.z], nb1 = d.hl_nboff[hh.z + ninner];
+      for (int j = nb0 + tid; j < nb1; j += FUSED_THREADS) S.nbl[j - nb0] = d.hl_nbl[j];
+    }
+    for (int i = tid; i < nloc; i += FUSED_THREADS) {
+      int v;
+      bool own = true;
+      if (i < nu) {
+        v = vr.x + i;
+      } else {
+        const int2 e = d.hl_v[hh.x + i - nu];
+        v = e.x;
+        own = d.leaf_gstamp[e.y] == stamp;  // its leaf was gathered by this dab's search
+      }
+      const float x = d.px[v], y = d.py[v], z = d.pz[v];
+      const float dx = k.cx - x;
+      const float dy = k.cy - y;
+      const float dz = k.cz - z;
+      const float d2 = dx * dx + dy * dy + dz * dz;
+      bool aff = own && d2 <= k.r2;
+      if (d.vhide && d.respect_hide && d.vhide[v]) aff = false;
+      float rx = x, ry = y, rz = z, fade = 0.0f;
+      if (aff) {
+        const float f = sg_falloff(k, curve, sqrtf(d2)) * (1.0f - (d.mask ? d.mask[v] : 0.0f));
+        if (k.tool == SG_TOOL_DRAW) {
+          rx = x + ox * f;
+          ry = y + oy * f;
+          rz = z + oz * f;
+        } else if (k.tool == SG_TOOL_INFLATE) {
+          const float sc = (k.bstr * f) * k.r;
+          rx = x + d.nx[v] * sc;
+          ry = y + d.ny[v] * sc;
+          rz = z + d.nz[v] * sc;
+        } else if (k.tool == SG_TOOL_GRAB) {
+          fade = k.bstr * f;
+          rx = x + k.gx * fade;
+          ry = y + k.gy * fade;
+          rz = z + k.gz * fade;
+        } else {
+          fade = k.bstr * f;
+        }
+      }
+      S.x[0][i] = rx;
+      S.y[0][i] = ry;
+      S.z[0][i] = rz;
+      S.fade[i] = fade;
+      S.aff[i] = aff ? 1 : 0;
+    }
+    __syncthreads();
+    // Smooth: Jacobi step t updates the locals of rings <= K - t from step t - 1
+    int cur = 0;
+    if (K > 0) {
+      const int nb0 = d.hl_nboff[hh.z];
+      for (int t = 1; t <= K; t++) {
+        const int nact = nu + re[K - t];
+        for (int i = tid; i < nact; i += FUSED_THREADS) {
+          const float x = S.x[cur][i], y = S.y[cur][i], z = S.z[cur][i];
+          float rx = x, ry = y, rz = z;
+          if (S.aff[i]) {
+            const int b = d.hl_nboff[hh.z + i] - nb0, e = d.hl_nboff[hh.z + i + 1] - nb0;
+            float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+            for (int j = b; j < e; j++) {
+              const int u = S.nbl[j];
+              sx = sx + S.x[cur][u];
+              sy = sy + S.y[cur][u];
+              sz = sz + S.z[cur][u];
+            }
+            if (e > b) {
+              const float fade = S.fade[i];
+              const float inv = 1.0f / (float)(e - b);
+              const float avx = sx * inv, avy = sy * inv, avz = sz * inv;
+              rx = x + (avx - x) * fade;
+              ry = y + (avy - y) * fade;
+              rz = z + (avz - z) * fade;
+            }
+          }
+          S.x[cur ^ 1][i] = rx;
+          S.y[cur ^ 1][i] = ry;
+          S.z[cur ^ 1][i] = rz;
+        }
+        __syncthreads();
+        cur ^= 1;
+      }
+    }
+    // the affected uniq vertices: normal (spec S10, incident tris in vertex->tri order),
+    // position, list record
+    for (int i = tid; i < nu; i += FUSED_THREADS) {
+      if (!S.aff[i]) continue;
+      const int v = vr.x + i;
+      const int qb = d.vt_off[v], qe = d.vt_off[v + 1];
+      float ax = 0.0f, ay = 0.0f, az = 0.0f;
+      for (int q = qb; q < qe; q++) {
+        const uint2 c = d.hl_tri[q];
+        const int a = c.x & 0xffff, b = c.x >> 16, e = c.y;
+        float fx, fy, fz;
+        sg_tri_normal(S.x[cur][a], S.y[cur][a], S.z[cur][a], S.x[cur][b], S.y[cur][b], S.z[cur][b], S.x[cur][e],
+                      S.y[cur][e], S.z[cur][e], fx, fy, fz);
+        ax = ax + fx;
+        ay = ay + fy;
+        az = az + fz;
+      }
+      sg_normalize(ax, ay, az);
+      d.mx[v] = ax;
+      d.my[v] = ay;
+      d.mz[v] = az;
+      d.qx[v] = S.x[cur][i];
+      d.qy[v] = S.y[cur][i];
+      d.qz[v] = S.z[cur][i];
+      const int pos = atomicAdd(&s_cnt, 1);
+      d.aff_v[(g << d.aff_shift) + pos] = v;
+    }
+    // the leaf's box over its uniq then shared vertices, folded as leaf_box does
+    if (tid < SG_WAVE) {
+      const int tot = nu + ns, per = (tot + SG_WAVE - 1) / SG_WAVE;
+      BB6 acc;
+      bb_reset(acc);
+      for (int c = 0; c < per; c++) {
+        const int i = tid * per + c;
+        if (i < tot) {
+          const int u = i < nu ? i : S.shl[i - nu];
+          bb_add_point(acc, S.x[cur][u], S.y[cur][u], S.z[cur][u]);
+        }
+      }
+      acc = bb_wave_reduce(acc);
+      if (tid == 0) {
+        bb_store(d.leaf_bb + 8 * (size_t)leaf, acc);
+        bb_store(d.leaf_obb + 8 * (size_t)leaf, acc);
+      }
+    }
+    __syncthreads();
+    if (tid == 0) d.aff_cnt[g] = s_cnt;
+    __syncthreads();  // LDS is reused by the next leaf
+  }
+}
+
+// After the last dab of a stroke (or a single dab): when that dab was fused, copy its
+// results in, and give it its affected count.
+__global__ __launch_bounds__(256) void k_dab_finish(SgDevView d, int serial) {
+  const int stamp = d.dstamp[serial & 1];
+  if (d.pend[1] != stamp) return;
+  const int ng = d.pend[0];
+  const int na = ng << d.aff_shift;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
+    const int v = aff_slot_vertex(d, i, d.aff_cnt[i >> d.aff_shift]);
+    if (v < 0) continue;
+    d.px[v] = d.qx[v];
+    d.py[v] = d.qy[v];
+    d.pz[v] = d.qz[v];
+    d.nx[v] = d.mx[v];
+    d.ny[v] = d.my[v];
+    d.nz[v] = d.mz[v];
+  }
+  if (blockIdx.x == 0 && threadIdx.x < SG_WAVE) {
+    int s = 0;
+    for (int g = threadIdx.x; g < ng; g += SG_WAVE) s += d.aff_cnt[g];
+    s = wave_sum_i(s);
+    if (threadIdx.x == 0) d.ctr[SG_CTR_NAFF] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Deferred refit of internal nodes.  Sub: one workgroup per dirty cut subtree,
 // leaf boxes staged in LDS, internal nodes computed height by height
 // (parent = BB_expand_with_bb(child0, child1), pbvh.c:123-124).  Top: one workgroup
@@ -1155,6 +1445,21 @@ SgDevView sg_view_of(SgDev &d, bool respect_hide, bool halt_live) {
   v.ctr = d.ctr;
   v.ctr_next = d.ctr + SG_CTR_COUNT;
   v.cut_touch = d.cut_touch;
+  v.hl_hdr = d.hl_hdr;
+  v.hl_ring = d.hl_ring;
+  v.hl_v = d.hl_v;
+  v.hl_nboff = d.hl_nboff;
+  v.hl_nbl = d.hl_nbl;
+  v.hl_sh = d.hl_sh;
+  v.hl_tri = d.hl_tri;
+  v.gl_h = d.gl_h;
+  v.gl_r = d.gl_r;
+  v.leaf_gstamp = d.leaf_gstamp;
+  v.dstamp = d.dstamp;
+  v.pend = d.pend;
+  v.mx = d.mx;
+  v.my = d.my;
+  v.mz = d.mz;
   v.halt = halt_live ? d.halt : nullptr;  // no flag test in strokes without learned dabs
   v.seq = d.seq;
   v.vstamp = d.vstamp;
@@ -1266,7 +1571,7 @@ int sg_profile_collect(PBVH *p) {
 }
 
 int sg_enqueue_clear_ctr(PBVH *p) {
-  hipLaunchKernelGGL(k_clear_ctr, dim3(1), dim3(64), 0, p->stream, p->d.ctr, p->d.halt);
+  hipLaunchKernelGGL(k_clear_ctr, dim3(1), dim3(64), 0, p->stream, p->d.ctr, p->d.halt, p->d.dstamp);
   return sg_check_hip(hipGetLastError(), "clear launch");
 }
 
@@ -1282,10 +1587,41 @@ int sg_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long long *
   const int GB = p->grid_brush > 0 ? p->grid_brush : G;
   const int GP = p->grid_post > 0 ? p->grid_post : G;
   const int GS = p->grid_smooth > 0 ? p->grid_smooth : G;
+  // the fused dab (one kernel after the search) when the mesh has halo tables and the
+  // dab is a plain brush dab it covers
+  SgDabK kk = k;
+  kk.fused = 0;
+  const size_t fused_lds = 0;
+  int fused_cap = 0;
+  const bool fused_tool = k.tool == SG_TOOL_DRAW || k.tool == SG_TOOL_INFLATE || k.tool == SG_TOOL_GRAB ||
+                          (k.tool == SG_TOOL_SMOOTH && k.iters >= 1 && k.iters <= SG_HALO_RINGS);
+  int nbl_cap = 0, sh_cap = 0;
+  size_t lds = 0;
+  if (p->d.halo_ok && p->fused_enabled && fused_tool && !k.stats && !k.orig_co && !(k.learned && p->mlp) && !col &&
+      !place) {
+    fused_cap = k.tool == SG_TOOL_SMOOTH ? p->d.halo_maxloc : p->d.halo_maxloc0;
+    nbl_cap = (k.tool == SG_TOOL_SMOOTH ? p->d.halo_maxnbl : 0) + 2;
+    sh_cap = p->d.leaf_maxsh + 2;
+    lds = (size_t)fused_cap * 7 * sizeof(float) + (size_t)(nbl_cap + sh_cap) * 2 + (size_t)fused_cap + 16;
+    kk.fused = lds <= 65536 ? 1 : 0;  // larger halos (very large leaves) take the separate kernels
+  }
+  (void)fused_lds;
+  const int copy_blocks = p->d.halo_ok ? 64 : 0;
   {
     ProfScope ps(p, SG_K_SEARCH);
-    const int sg = (p->d.C + 3) / 4;
-    SG_LAUNCH(ps, k_search, dim3(sg > 0 ? sg : 1), dim3(256), 0, s, v, k, log, log_index, serial, place);
+    const int sg = std::max(1, (p->d.C + 3) / 4);
+    SG_LAUNCH(ps, k_search, dim3(sg + copy_blocks), dim3(256), 0, s, v, kk, log, log_index, serial, place,
+              copy_blocks);
+  }
+  if (kk.fused) {
+    if (k.tool == SG_TOOL_DRAW) {
+      ProfScope ps(p, SG_K_AREA);
+      SG_LAUNCH(ps, k_area, dim3(GB), dim3(256), 0, s, v, k, place);
+    }
+    ProfScope ps(p, SG_K_BRUSH);
+    SG_LAUNCH(ps, k_dab_fused, dim3(p->grid_fused), dim3(FUSED_THREADS), lds, s, v, kk, curve_dev, serial, fused_cap,
+              nbl_cap, sh_cap);
+    return sg_check_hip(hipGetLastError(), "dab launch");
   }
   if (k.tool == SG_TOOL_ELASTIC_DEFORM) {  // every leaf; then all normals, leaf boxes and the tree
     // whole-mesh passes: a wave per leaf, a thread per vertex (enough waves in flight
@@ -1420,6 +1756,14 @@ int sg_enqueue_coords_apply(PBVH *p, const float *src_dev) {
   hipLaunchKernelGGL(k_refit_all, dim3(std::max(1, (p->d.L + 3) / 4)), dim3(256), 0, p->stream, v);  // a wave per leaf
   hipLaunchKernelGGL(k_cut_refresh, dim3(p->grid_wg), dim3(256), 0, p->stream, v);
   return sg_check_hip(hipGetLastError(), "coords launch");
+}
+
+int sg_enqueue_finish(PBVH *p, int serial_last) {
+  if (!p->d.halo_ok) return 0;
+  SgDevView v = sg_view(p);
+  v.ctr = p->d.ctr + (size_t)(serial_last & 1) * SG_CTR_COUNT;  // the last dab's counter set
+  hipLaunchKernelGGL(k_dab_finish, dim3(64), dim3(256), 0, p->stream, v, serial_last);
+  return sg_check_hip(hipGetLastError(), "finish launch");
 }
 
 int sg_enqueue_leaf_boxes(PBVH *p, const int *leaves_dev, int n, float *out_dev) {

@@ -18,6 +18,7 @@
 #include <cfloat>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
 #include <thread>
 
 #include "sg_internal.h"
@@ -549,6 +550,204 @@ void sg_build_cut(PBVH *p, std::vector<int> &lbeg, std::vector<int> &lend, std::
   }
 }
 
+// Halo tables of the fused dab kernel.  For leaf l with uniq range U (new ids):
+//   ring 0 = (shared vertices of l) and (edge neighbours of U), minus U;
+//   ring j = edge neighbours of ring j-1 (ring 0 counting U) not yet listed, j <= R.
+// Local ids: U first (its own order), then ring 0, ring 1, ..., each ring by ascending
+// vertex id.  A dab's results for l need final positions of U and ring 0 only (the
+// normals' corners and the leaf box's shared vertices are there); Smooth(k) needs rings
+// up to k, and a vertex of ring j is exact after k - j Jacobi steps.  Every vertex of a
+// ring below R has all its edge neighbours among the locals; their local ids are listed
+// in vertex->vertex order, so the local Jacobi sums in the same order as the global one.
+// Returns false (the fused path off) when a leaf has more than 65535 locals.
+static bool build_halo(PBVH *p, const SgLayout &h, hipStream_t s) {
+  SgDev &d = p->d;
+  const int L = h.L, V = h.V, R = SG_HALO_RINGS;
+  struct Leaf {
+    std::vector<int2> ext;
+    int ring_end[SG_HALO_RINGS + 1];  // ext count at the end of ring j
+    std::vector<int> nboff;           // relative to the leaf's neighbour block
+    std::vector<unsigned short> nbl;
+    bool ok = true;
+  };
+  std::vector<Leaf> lv(L);
+  auto owner_of = [&](int v) {
+    return (int)(std::upper_bound(h.leaf_vbeg.begin(), h.leaf_vbeg.end(), v) - h.leaf_vbeg.begin()) - 1;
+  };
+  std::vector<uint2> tri(h.vt4.size());
+  std::vector<unsigned short> shl(h.leaf_sh.size());
+  std::atomic<int> bad{0};
+  const int nt = 16;
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; t++) {
+    th.emplace_back([&, t]() {
+      // open addressing vertex -> local id, cleared after each leaf
+      std::vector<long long> tab(1 << 12, -1);
+      std::vector<int> used;
+      auto find = [&](int v) -> int {
+        size_t m = tab.size() - 1, i = ((size_t)v * 2654435761u) & m;
+        while (tab[i] >= 0) {
+          if ((int)(tab[i] >> 32) == v) return (int)(tab[i] & 0xffffffff);
+          i = (i + 1) & m;
+        }
+        return -1;
+      };
+      auto insert = [&](int v, int loc) {
+        size_t m = tab.size() - 1, i = ((size_t)v * 2654435761u) & m;
+        while (tab[i] >= 0) i = (i + 1) & m;
+        tab[i] = ((long long)v << 32) | (unsigned)loc;
+        used.push_back((int)i);
+      };
+      std::vector<int> ring, next, all;
+      for (int l = t; l < L; l += nt) {
+        Leaf &o = lv[l];
+        const int vb = h.leaf_vbeg[l], ve = h.leaf_vbeg[l + 1], nu = ve - vb;
+        all.clear();
+        for (int v = vb; v < ve; v++) all.push_back(v);
+        // ring 0
+        next.clear();
+        for (int k = h.leaf_sh_off[l]; k < h.leaf_sh_off[l + 1]; k++) next.push_back(h.leaf_sh[k]);
+        for (int v = vb; v < ve; v++)
+          for (int q = h.vv_off[v]; q < h.vv_off[v + 1]; q++) next.push_back(h.vv_idx[q]);
+        std::sort(next.begin(), next.end());
+        next.erase(std::unique(next.begin(), next.end()), next.end());
+        ring.clear();
+        for (int v : next)
+          if (v < vb || v >= ve) ring.push_back(v);
+        o.ring_end[0] = (int)ring.size();
+        for (int v : ring) all.push_back(v);
+        std::vector<int> prev = ring;
+        for (int j = 1; j <= R; j++) {
+          // grow the table when it would pass half full
+          while (tab.size() < 2 * (all.size() + 8 * prev.size() + 64)) {
+            tab.assign(tab.size() * 2, -1);
+            used.clear();
+          }
+          for (size_t i = 0; i < used.size(); i++) tab[used[i]] = -1;
+          used.clear();
+          for (size_t i = 0; i < all.size(); i++) insert(all[i], (int)i);
+          next.clear();
+          const std::vector<int> &src = (j == 1) ? all : prev;  // ring 1 grows from U and ring 0
+          for (int u : src)
+            for (int q = h.vv_off[u]; q < h.vv_off[u + 1]; q++) {
+              const int w = h.vv_idx[q];
+              if (find(w) < 0) next.push_back(w);
+            }
+          std::sort(next.begin(), next.end());
+          next.erase(std::unique(next.begin(), next.end()), next.end());
+          for (int v : next) all.push_back(v);
+          o.ring_end[j] = o.ring_end[j - 1] + (int)next.size();
+          prev = next;
+        }
+        if (all.size() > 65535) {
+          o.ok = false;
+          bad++;
+          continue;
+        }
+        for (size_t i = 0; i < used.size(); i++) tab[used[i]] = -1;
+        used.clear();
+        while (tab.size() < 2 * all.size() + 64) tab.assign(tab.size() * 2, -1);
+        for (size_t i = 0; i < all.size(); i++) insert(all[i], (int)i);
+        o.ext.resize(all.size() - nu);
+        for (size_t i = nu; i < all.size(); i++) o.ext[i - nu] = make_int2(all[i], owner_of(all[i]));
+        // neighbour lists of the locals before the last ring
+        const int inner = nu + o.ring_end[R - 1];
+        o.nboff.assign(inner + 1, 0);
+        // every id below must be a local of the right rings; a miss turns the fused path off
+        const int nl0 = nu + o.ring_end[0], nlR = (int)all.size();
+        bool miss = false;
+        for (int i = 0; i < inner; i++) {
+          const int u = all[i];
+          for (int q = h.vv_off[u]; q < h.vv_off[u + 1]; q++) {
+            const int w = find(h.vv_idx[q]);
+            miss |= w < 0 || w >= nlR;
+            o.nbl.push_back((unsigned short)w);
+          }
+          o.nboff[i + 1] = (int)o.nbl.size();
+        }
+        // corners of the uniq vertices' incidences, shared vertices
+        for (int v = vb; v < ve; v++)
+          for (int q = h.vt_off[v]; q < h.vt_off[v + 1]; q++) {
+            const int4 c = h.vt4[q];
+            const int a = find(c.x), b = find(c.y), e = find(c.z);
+            miss |= a < 0 || b < 0 || e < 0 || a >= nl0 || b >= nl0 || e >= nl0;
+            tri[q] = make_uint2((unsigned)a | ((unsigned)b << 16), (unsigned)e);
+          }
+        for (int k = h.leaf_sh_off[l]; k < h.leaf_sh_off[l + 1]; k++) {
+          const int w = find(h.leaf_sh[k]);
+          miss |= w < 0 || w >= nl0;
+          shl[k] = (unsigned short)w;
+        }
+        if (miss) {
+          o.ok = false;
+          bad++;
+        }
+        for (size_t i = 0; i < used.size(); i++) tab[used[i]] = -1;
+        used.clear();
+      }
+    });
+  }
+  for (auto &x : th) x.join();
+  if (bad) return false;
+  // concatenate
+  std::vector<int4> hdr(L), rng(L);
+  size_t next_ext = 0, next_nb = 0, next_nbl = 0;
+  int maxloc = 0, maxloc0 = 0, maxnbl = 0, maxsh = 0;
+  for (int l = 0; l < L; l++) {
+    const Leaf &o = lv[l];
+    const int nu = h.leaf_vbeg[l + 1] - h.leaf_vbeg[l];
+    maxnbl = std::max(maxnbl, (int)o.nbl.size());
+    maxsh = std::max(maxsh, h.leaf_sh_off[l + 1] - h.leaf_sh_off[l]);
+    hdr[l] = make_int4((int)next_ext, o.ring_end[0], (int)next_nb, o.ring_end[R]);
+    rng[l] = make_int4(o.ring_end[1], o.ring_end[2], o.ring_end[3], o.ring_end[4]);
+    next_ext += o.ext.size();
+    next_nb += o.nboff.size();
+    next_nbl += o.nbl.size();
+    maxloc = std::max(maxloc, nu + o.ring_end[R]);
+    maxloc0 = std::max(maxloc0, nu + o.ring_end[0]);
+  }
+  if (next_ext >= (1ull << 31) || next_nbl >= (1ull << 31)) return false;
+  std::vector<int2> ext(std::max<size_t>(next_ext, 1));
+  std::vector<int> nboff(std::max<size_t>(next_nb, 1));
+  std::vector<unsigned short> nbl(std::max<size_t>(next_nbl, 1));
+  {
+    size_t e = 0, b = 0, n = 0;
+    for (int l = 0; l < L; l++) {
+      const Leaf &o = lv[l];
+      std::copy(o.ext.begin(), o.ext.end(), ext.begin() + e);
+      for (size_t i = 0; i < o.nboff.size(); i++) nboff[b + i] = (int)n + o.nboff[i];
+      std::copy(o.nbl.begin(), o.nbl.end(), nbl.begin() + n);
+      e += o.ext.size();
+      b += o.nboff.size();
+      n += o.nbl.size();
+    }
+  }
+  lv.clear();
+  lv.shrink_to_fit();
+  int rc = 0;
+  rc |= upload(&d.hl_hdr, hdr.data(), L, s);
+  rc |= upload(&d.hl_ring, rng.data(), L, s);
+  rc |= upload(&d.hl_v, ext.data(), ext.size(), s);
+  rc |= upload(&d.hl_nboff, nboff.data(), nboff.size(), s);
+  rc |= upload(&d.hl_nbl, nbl.data(), nbl.size(), s);
+  rc |= upload(&d.hl_tri, tri.data(), std::max<size_t>(tri.size(), 1), s);
+  rc |= upload(&d.hl_sh, shl.data(), std::max<size_t>(shl.size(), 1), s);
+  rc |= alloc_fill(&d.leaf_gstamp, std::max(L, 1), 0xFF, s);  // -1: no dab's stamp (stamps start at 1)
+  rc |= alloc_fill(&d.dstamp, 2, 0, s);
+  rc |= alloc_fill(&d.pend, 4, 0xFF, s);  // no fused dab pending
+  rc |= alloc_fill(&d.mx, V, 0, s);
+  rc |= alloc_fill(&d.my, V, 0, s);
+  rc |= alloc_fill(&d.mz, V, 0, s);
+  rc |= alloc_fill(&d.gl_h, std::max(L, 1), 0, s);
+  rc |= alloc_fill(&d.gl_r, std::max(L, 1), 0, s);
+  hipStreamSynchronize(s);  // the host tables are freed on return
+  d.halo_maxloc = maxloc;
+  d.halo_maxloc0 = maxloc0;
+  d.halo_maxnbl = maxnbl;
+  d.leaf_maxsh = maxsh;
+  return rc == 0;
+}
+
 void sg_build_device(PBVH *p, const float *co) {
   SgDev &d = p->d;
   hipStream_t s = p->stream;
@@ -748,6 +947,8 @@ void sg_build_device(PBVH *p, const float *co) {
     upload(&d.vc_off, h.vc_off.data(), V + 1, s);
     upload(&d.vc, h.vc.data(), h.vc.size(), s);
     hipStreamSynchronize(s);  // the host arrays are freed on return
+  } else {
+    d.halo_ok = build_halo(p, h, s);
   }
   hipStreamSynchronize(s);
 }
@@ -767,7 +968,9 @@ void sg_free_device(PBVH *p) {
                   d.draw_dirty, d.draw_tri, d.draw_off, d.draw_leaf, d.draw_fset, d.draw_poly_off,
                   d.draw_poly_v, d.draw_vbo, d.undo, d.undo_stamp, d.undo_slot, d.place_pair,
                   d.prox_dev, d.prox_tab, d.prox_leaves, d.prox_off, d.poly_off, d.poly_v,
-                  d.poly_of_tri, d.vc_off, d.vc, d.stage};
+                  d.poly_of_tri, d.vc_off, d.vc, d.stage, d.hl_hdr, d.hl_ring, d.hl_v, d.hl_nboff,
+                  d.hl_nbl, d.hl_tri, d.hl_sh, d.leaf_gstamp, d.dstamp, d.mx, d.my, d.mz, d.pend,
+                  d.gl_h, d.gl_r};
   for (void *q : ptrs) {
     if (q) hipFree(q);
   }

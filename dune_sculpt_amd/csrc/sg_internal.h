@@ -158,6 +158,24 @@ struct SgDev {
   // uploads and readbacks): allocated on first use, reused after
   void *stage = nullptr;
   size_t stage_cap = 0;         // bytes
+  // Per-leaf halos of the fused dab kernel (triangle meshes; see sg_build_halo).  A
+  // leaf's local vertices are its uniq range, then ring 0 (its shared vertices and the
+  // edge neighbours of its uniq vertices), then rings 1..SG_HALO_RINGS around those.
+  bool halo_ok = false;
+  int halo_maxloc = 0, halo_maxloc0 = 0;   // most local vertices of a leaf: all rings / ring 0
+  int4 *hl_hdr = nullptr;       // [L] ext offset, ring-0 count, neighbour-offset block start, total ext count
+  int4 *hl_ring = nullptr;      // [L] ext counts at the ends of rings 1..4
+  int2 *hl_v = nullptr;         // [ext] vertex id, owner leaf (the leaf holding it as uniq)
+  int *hl_nboff = nullptr;      // per leaf: offsets into hl_nbl of its locals before the last ring (+1)
+  unsigned short *hl_nbl = nullptr;  // local ids of those locals' edge neighbours, vertex->vertex order
+  uint2 *hl_tri = nullptr;      // [3T] parallel to vt4: the incidence's corners as local ids (c0 | c1 << 16, c2)
+  unsigned short *hl_sh = nullptr;   // parallel to leaf_sh: local id of each shared vertex
+  int *leaf_gstamp = nullptr;   // [L] dab stamp of the last search that gathered the leaf
+  int *dstamp = nullptr;        // [2] dab stamps by counter-set parity
+  float *mx = nullptr, *my = nullptr, *mz = nullptr;  // [V] normals the fused kernel wrote, copied in by the next search
+  int *pend = nullptr;          // [4] the fused dab whose results wait for that copy: gathered count, stamp
+  int4 *gl_h = nullptr, *gl_r = nullptr;  // [L] the gathered leaves' halo headers (fused dabs)
+  int halo_maxnbl = 0, leaf_maxsh = 0;    // most neighbour entries / shared vertices of a leaf
 };
 
 // Pinned host chunk the proxies are carved from: the caller fills them in place and
@@ -225,6 +243,8 @@ struct PBVH {
   int dab_count = 0;
   int grid_wg = 256;                 // workgroups of the per-dab kernels (tunable)
   int grid_brush = 0, grid_post = 0, grid_smooth = 0;  // per-kernel overrides (autotuner)
+  int grid_fused = 1024;             // workgroups of the fused dab kernel (a leaf each)
+  bool fused_enabled = true;         // plain dabs run fused when the mesh has halo tables
   // device snapshot (checkpoint) of the mutable mesh state
   // slot 0: the caller's (sg_pbvh_snapshot_save/restore); slot 1: internal (autotuner, judge)
   std::vector<void *> snap[2];
@@ -263,6 +283,8 @@ struct SgLayout {
   std::vector<int> poly_v, vc_off;
   std::vector<int2> vc;
 };
+
+#define SG_HALO_RINGS 4  // Smooth(k) runs fused for k <= this
 
 // host build (pbvh_build.cpp)
 void sg_build_tree(PBVH *p, const float *co, const int *tris, int T, int V);

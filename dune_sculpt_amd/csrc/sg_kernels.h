@@ -77,6 +77,16 @@ struct SgDevView {
   long long *ctr, *ctr_next;
   const int *vperm_old;
   int *stamp_v, *stamp_v2, *stamp_t, *stamp_a;
+  // fused dabs: per-leaf halos, the gathered leaves' halo headers, gather stamps, dab
+  // stamps, the normals the fused kernel wrote and its pending record
+  const int4 *hl_hdr, *hl_ring;
+  const int2 *hl_v;
+  const int *hl_nboff;
+  const unsigned short *hl_nbl, *hl_sh;
+  const uint2 *hl_tri;
+  int4 *gl_h, *gl_r;
+  int *leaf_gstamp, *dstamp, *pend;
+  float *mx, *my, *mz;
 };
 
 // Per-dab constants (spec S1-S9), computed on the host in the same FP32 order as the
@@ -89,6 +99,7 @@ struct SgDabK {
   int hide_skip;  // 1: the search skips fully hidden subtrees and leaves; 0: every leaf (elastic deform)
   int etype;      // elastic deform mode
   float epois, enx, eny, enz, eforce;  // Poisson ratio, sculpt normal, force (Kelvinlet parameters)
+  int fused;      // the search fills the halo headers and gather stamps for the fused kernel
 };
 
 // kernel ids for the per-kernel profile
@@ -114,5 +125,6 @@ int sg_enqueue_coords_apply(PBVH *p, const float *src_dev);
 int sg_enqueue_leaf_boxes(PBVH *p, const int *leaves_dev, int n, float *out_dev);
 int sg_enqueue_scatter_boxes(PBVH *p, const void *recs_dev, int n);
 int sg_enqueue_set_marks(PBVH *p, const int *verts_dev, int n);
+int sg_enqueue_finish(PBVH *p, int serial_last);
 int sg_enqueue_gather(PBVH *p, int which, float *out_dev);
 const char *sg_last_error_str();

@@ -291,6 +291,11 @@ int sg_pbvh_set_grid(PBVH *pbvh, int workgroups);
 /* Workgroups of the brush / post / smooth-step kernels (0 = the common grid). */
 int sg_pbvh_set_kernel_grids(PBVH *pbvh, int brush, int post, int smooth);
 int sg_pbvh_get_kernel_grids(PBVH *pbvh, int out[4]);
+/* Fused dabs: on (default) a plain Draw / Inflate / Grab / Smooth(k <= 4) dab on a
+ * triangle mesh runs as the search plus one kernel per gathered leaf over the leaf's
+ * halo; off, the separate brush / Smooth-step / normals-and-bounds kernels.  Both give
+ * the same bits.  Returns whether the mesh has the halo tables the fused path needs. */
+int sg_pbvh_set_fused(PBVH *pbvh, int on);
 /* Autotuner: times a sample of dabs from the current mesh state (restored after each
  * trial) for candidate workgroup counts of each per-dab kernel and keeps the fastest.
  * out_ms (may be NULL) receives {time before, time after} of the sample in ms. */
