@@ -655,6 +655,7 @@ def main():
     ap.add_argument("--only-undo", action="store_true", help="run only the undo-step measurement")
     ap.add_argument("--no-elastic", action="store_true", help="skip the elastic deform measurement")
     ap.add_argument("--no-c1", action="store_true", help="skip the icosphere-6 CPU-reference case")
+    ap.add_argument("--no-fused", action="store_true", help="separate brush / Smooth / post kernels (A/B)")
     ap.add_argument("--only-elastic", action="store_true", help="run only the elastic deform measurement")
     ap.add_argument("--placed-only", action="store_true",
                     help="placed stroke without the fixed-centre reference stroke (one captured graph)")
@@ -680,7 +681,10 @@ def main():
     A = sg.PBVH(co, tris, leaf_limit=128)
     build_s = time.perf_counter() - t
     info = A.tree_info()
-    log(f"PBVH build {build_s:.1f}s: {info}")
+    halo = A.halo_info()
+    if args.no_fused:
+        A.set_fused(False)
+    log(f"PBVH build {build_s:.1f}s: {info}, halo {halo}, fused {halo['built'] and not args.no_fused}")
     if args.grid:
         A.set_grid(args.grid)
 
@@ -878,7 +882,8 @@ def main():
                                f"{args.dabs}-dab stroke cycling Draw/Inflate/Smooth(4)/Grab, r~U[0.02,0.1], "
                                f"s~U[0.2,1.0], leaf<=128 tris",
                    "dabs_per_step": args.dabs, "mode": args.mode, "leaves": info["leaves"],
-                   "search_subtrees": info["cuts"]},
+                   "search_subtrees": info["cuts"], "fused": bool(halo["built"] and not args.no_fused)},
+        "halo": halo,
         "hbm_gbps": gbps,
         "hbm_frac": gbps / HBM_PEAK_GBPS,
         "us_per_dab": elapsed / K / args.dabs * 1e6,

@@ -114,7 +114,7 @@ EXPORTS = [
     "sg_dab_stroke_log", "sg_sync", "sg_last_error", "sg_dab_last_affected", "sg_dab_last_stats",
     "sg_dab_last_gathered", "sg_pbvh_set_stats", "sg_pbvh_set_grid", "sg_pbvh_read_positions",
     "sg_pbvh_read_normals", "sg_pbvh_set_mask", "sg_pbvh_tree_info", "sg_pbvh_node_index",
-    "sg_pbvh_node_at", "sg_pbvh_node_flag", "sg_pbvh_set_fused", "sg_pbvh_prim_indices", "sg_mesh_icosphere_size", "sg_mesh_icosphere",
+    "sg_pbvh_node_at", "sg_pbvh_node_flag", "sg_pbvh_set_fused", "sg_pbvh_halo_info", "sg_pbvh_prim_indices", "sg_mesh_icosphere_size", "sg_mesh_icosphere",
     "sg_mesh_torus", "sg_mesh_noise_displace", "sg_pbvh_set_profile", "sg_pbvh_profile_read",
     "sg_stroke_prepare", "sg_stroke_launch", "sg_stroke_free", "sg_pbvh_set_kernel_grids",
     "sg_pbvh_get_kernel_grids", "sg_pbvh_snapshot_save", "sg_pbvh_snapshot_restore",
@@ -199,6 +199,7 @@ def lib():
         "sg_pbvh_node_at": (vp, [vp, i32]),
         "sg_pbvh_node_flag": (i32, [vp]),
         "sg_pbvh_set_fused": (i32, [vp, i32]),
+        "sg_pbvh_halo_info": (i32, [vp, C.POINTER(C.c_longlong)]),
         "sg_pbvh_prim_indices": (i32, [vp, i32p, i32]),
         "sg_mesh_icosphere_size": (i32, [i32, i32p, i32p]),
         "sg_mesh_icosphere": (i32, [i32, C.c_float, f32p, i32p]),
@@ -823,6 +824,12 @@ class PBVH:
     def set_fused(self, on=True):
         """Fused dabs on or off; returns whether the mesh has the halo tables."""
         return bool(_check(lib().sg_pbvh_set_fused(self.h, 1 if on else 0), "set fused"))
+
+    def halo_info(self):
+        out = (C.c_longlong * 5)()
+        _check(lib().sg_pbvh_halo_info(self.h, out), "halo info")
+        return {"built": bool(out[0]), "max_locals": int(out[1]), "max_locals_ring0": int(out[2]),
+                "max_neighbour_entries": int(out[3]), "bytes": int(out[4])}
 
     def set_kernel_grids(self, brush=0, post=0, smooth=0):
         _check(lib().sg_pbvh_set_kernel_grids(self.h, brush, post, smooth), "grids")

@@ -420,6 +420,17 @@ int sg_pbvh_set_fused(PBVH *p, int on) {
   return p->d.halo_ok ? 1 : 0;
 }
 
+int sg_pbvh_halo_info(PBVH *p, long long out[5]) {
+  if (!p || !out) return sg_set_error("bad arguments");
+  const SgDev &d = p->d;
+  out[0] = d.halo_ok ? 1 : 0;
+  out[1] = d.halo_maxloc;
+  out[2] = d.halo_maxloc0;
+  out[3] = d.halo_maxnbl;
+  out[4] = d.halo_bytes;
+  return 0;
+}
+
 int sg_pbvh_set_kernel_grids(PBVH *p, int brush, int post, int smooth) {
   if (!p) return sg_set_error("null PBVH");
   for (int g : {brush, post, smooth})

@@ -745,6 +745,8 @@ static bool build_halo(PBVH *p, const SgLayout &h, hipStream_t s) {
   d.halo_maxloc0 = maxloc0;
   d.halo_maxnbl = maxnbl;
   d.leaf_maxsh = maxsh;
+  d.halo_bytes = (long long)(L * 32 + ext.size() * 8 + nboff.size() * 4 + nbl.size() * 2 + tri.size() * 8 +
+                             shl.size() * 2);
   return rc == 0;
 }
 

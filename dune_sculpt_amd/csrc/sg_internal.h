@@ -176,6 +176,7 @@ struct SgDev {
   int *pend = nullptr;          // [4] the fused dab whose results wait for that copy: gathered count, stamp
   int4 *gl_h = nullptr, *gl_r = nullptr;  // [L] the gathered leaves' halo headers (fused dabs)
   int halo_maxnbl = 0, leaf_maxsh = 0;    // most neighbour entries / shared vertices of a leaf
+  long long halo_bytes = 0;               // device bytes of the halo tables
 };
 
 // Pinned host chunk the proxies are carved from: the caller fills them in place and

@@ -296,6 +296,9 @@ int sg_pbvh_get_kernel_grids(PBVH *pbvh, int out[4]);
  * halo; off, the separate brush / Smooth-step / normals-and-bounds kernels.  Both give
  * the same bits.  Returns whether the mesh has the halo tables the fused path needs. */
 int sg_pbvh_set_fused(PBVH *pbvh, int on);
+/* Halo tables of the fused dabs: {built, most locals of a leaf (all rings), most
+ * locals in ring 0, most neighbour entries of a leaf, device bytes of the tables}. */
+int sg_pbvh_halo_info(PBVH *pbvh, long long out[5]);
 /* Autotuner: times a sample of dabs from the current mesh state (restored after each
  * trial) for candidate workgroup counts of each per-dab kernel and keeps the fastest.
  * out_ms (may be NULL) receives {time before, time after} of the sample in ms. */
