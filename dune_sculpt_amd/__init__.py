@@ -126,7 +126,7 @@ EXPORTS = [
     "sg_batch_rewind", "sg_batch_run", "sg_batch_sync", "sg_batch_totals", "sg_batch_read_positions",
     "sg_batch_read_normals", "sg_batch_snapshot_save", "sg_batch_snapshot_restore", "sg_batch_set_launch",
     "sg_batch_get_launch", "sg_batch_autotune", "sg_batch_set_learned", "sg_batch_timer_start",
-    "sg_batch_timer_stop", "sg_hbm_copy_probe", "sg_raycast", "sg_raycast_device", "sg_raycast_stats",
+    "sg_batch_timer_stop", "sg_hbm_copy_probe", "sg_raycast", "sg_raycast_device", "sg_raycast_stats", "sg_raycast_stats_noalloc",
     "sg_dab_apply_stroke_placed", "sg_stroke_prepare_placed", "sg_stroke_placements", "BKE_pbvh_raycast",
     "BKE_pbvh_find_nearest_to_ray", "BKE_pbvh_node_get_tmin", "BKE_pbvh_node_raycast",
     "BKE_pbvh_node_find_nearest_to_ray", "BKE_pbvh_raycast_project_ray_root", "sg_isect_ray_tri_precalc",
@@ -257,6 +257,7 @@ def lib():
         "sg_raycast": (i32, [vp, C.POINTER(SgRay), i32, i32, C.POINTER(SgRayHit)]),
         "sg_raycast_device": (i32, [vp, vp, i32, i32, vp]),
         "sg_raycast_stats": (i32, [vp, C.POINTER(SgRay), i32, i32, C.POINTER(C.c_longlong)]),
+        "sg_raycast_stats_noalloc": (i32, [vp, C.POINTER(SgRay), i32, i32, C.POINTER(C.c_longlong)]),
         "sg_dab_apply_stroke_placed": (i32, [vp, C.POINTER(SgDabParams), C.POINTER(SgRay), i32, i32]),
         "sg_stroke_prepare_placed": (vp, [vp, C.POINTER(SgDabParams), C.POINTER(SgRay), i32, i32]),
         "sg_stroke_placements": (i32, [vp, f32p, i32]),
@@ -953,6 +954,14 @@ class PBVH:
         rays = make_rays(starts, normals, depths)
         out = (C.c_longlong * 3)()
         _check(lib().sg_raycast_stats(self.h, rays, len(rays), 1 if original else 0, out), "sg_raycast_stats")
+        return {"boxes": int(out[0]), "leaves": int(out[1]), "tris": int(out[2])}
+
+    def raycast_stats_noalloc(self, starts, normals, depths=None, original=False):
+        """raycast_stats through the entry point that allocates nothing (static scratch)."""
+        rays = make_rays(starts, normals, depths)
+        out = (C.c_longlong * 3)()
+        _check(lib().sg_raycast_stats_noalloc(self.h, rays, len(rays), 1 if original else 0, out),
+               "sg_raycast_stats_noalloc")
         return {"boxes": int(out[0]), "leaves": int(out[1]), "tris": int(out[2])}
 
     def occluded_order(self, start, direction, mode=0, original=False):

@@ -31,6 +31,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "sg_internal.h"
@@ -1023,6 +1024,43 @@ int sg_raycast_stats(PBVH *p, const SgRay *rays, int n, int original, long long 
 }
 
 // ---------------------------------------------------------------------------
+// sg_raycast_stats without allocation.  Its scratch -- the three work counters, the
+// staged rays and the hits of at most SG_NOALLOC_RAYS rays -- lives in one file-scope
+// device array; each GPU the library runs on holds its own copy until the process
+// ends.  Host code reaches it only through the address hipGetSymbolAddress returns for
+// the current device (the array's name is not a device address on the host).
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocStats = 256;  // three counters, padded to the rays' alignment
+constexpr size_t kNoAllocBytes =
+    kNoAllocStats + (size_t)SG_NOALLOC_RAYS * sizeof(SgRay) + (size_t)SG_NOALLOC_RAYS * sizeof(SgRayHit);
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch[kMaxDevices];
+
+// The scratch array's address on the current device, looked up once per device.
+static unsigned char *scratch_address() {
+  int dev = 0;
+  if (sg_check_hip(hipGetDevice(&dev), "hipGetDevice")) return nullptr;
+  if (dev < 0 || dev >= kMaxDevices) {
+    sg_set_error("device %d beyond the %d the scratch table holds", dev, kMaxDevices);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *ptr = nullptr;
+    if (sg_check_hip(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress")) return nullptr;
+    g_scratch[dev] = (unsigned char *)ptr;
+  }
+  return g_scratch[dev];
+}
+
+__global__ void k_zero_ll(long long *p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+// ---------------------------------------------------------------------------
 // The reference's callback-driven ray API (host side of the drop-ins)
 // ---------------------------------------------------------------------------
 
@@ -1167,6 +1205,32 @@ static bool node_ray(PBVH *p, PBVHNode *node, float (*origco)[3], const float st
 }
 
 extern "C" {
+
+int sg_raycast_stats_noalloc(PBVH *p, const SgRay *rays, int n, int original, long long out[3]) {
+  if (!p || n <= 0 || !rays || !out) return sg_set_error("bad arguments");
+  if (p->no_device || !p->build_ok) return sg_set_error("no device mesh");
+  if (n > SG_NOALLOC_RAYS)
+    return sg_set_error("sg_raycast_stats_noalloc: %d rays exceed the static scratch of %d rays", n, SG_NOALLOC_RAYS);
+  if (!p->d.ltri)
+    return sg_set_error("sg_raycast_stats_noalloc: the ray layout is not built yet (one sg_raycast builds it)");
+  unsigned char *base = scratch_address();
+  if (!base) return -1;
+  long long *st = (long long *)base;
+  SgRay *rin = (SgRay *)(base + kNoAllocStats);
+  SgRayHit *rout = (SgRayHit *)(base + kNoAllocStats + (size_t)SG_NOALLOC_RAYS * sizeof(SgRay));
+  hipLaunchKernelGGL(k_zero_ll, dim3(1), dim3(64), 0, p->stream, st, 3);  // a kernel, not a memset: capturable
+  int rc = sg_check_hip(hipGetLastError(), "zero launch");
+  if (!rc)
+    rc = sg_check_hip(hipMemcpyAsync(rin, rays, (size_t)n * sizeof(SgRay), hipMemcpyHostToDevice, p->stream), "copy");
+  if (!rc && p->nodes_dirty_deferred) {
+    rc = sg_enqueue_flush(p);
+    p->nodes_dirty_deferred = false;
+  }
+  if (!rc) rc = sg_enqueue_raycast(p, rin, n, original ? 1 : 0, -1, rout, nullptr, 0, st);
+  if (!rc) rc = sg_check_hip(hipMemcpyAsync(out, st, 3 * sizeof(long long), hipMemcpyDeviceToHost, p->stream), "copy");
+  if (!rc) rc = sg_check_hip(hipStreamSynchronize(p->stream), "raycast");
+  return rc;
+}
 
 void sg_isect_ray_tri_precalc(IsectRayPrecalc *out, const float dir[3]) {
   if (!out || !dir) return;

@@ -372,6 +372,14 @@ int sg_raycast(PBVH *pbvh, const SgRay *rays, int n, int original, SgRayHit *hit
 int sg_raycast_device(PBVH *pbvh, const SgRay *rays_dev, int n, int original, SgRayHit *hits_dev);
 /* Work counters of a ray batch: [0] boxes tested [1] leaves scanned [2] tris tested. */
 int sg_raycast_stats(PBVH *pbvh, const SgRay *rays, int n, int original, long long out[3]);
+/* No counterpart in the reference.  sg_raycast_stats for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * the same results with no hipMalloc / hipFree.  Its scratch (counters, staged rays and
+ * hits) is a static device array holding at most SG_NOALLOC_RAYS rays; more is an error.
+ * The ray layout must exist already (one sg_raycast on the PBVH builds it; otherwise an
+ * error).  Calls that overlap on one GPU share that array and race. */
+enum { SG_NOALLOC_RAYS = 4096 };
+int sg_raycast_stats_noalloc(PBVH *pbvh, const SgRay *rays, int n, int original, long long out[3]);
 /* A stroke whose dab centres come from rays cast on the GPU, each on the mesh as the
  * previous dab left it (the stroke location of each OperatorStrokeElement,
  * api_brush.c:3526-3576, found by SCULPT_stroke_get_location in the absent sculpt.c):
