@@ -44,170 +44,226 @@ int sg_check_hip(hipError_t e, const char *what) {
 }
 
 // ---------------------------------------------------------------------------
-// K1: search.  One wave per cut subtree (all waves independent, no block barrier):
-//   level 1  the subtree's box, leaf range, flags and last-touch stamp
-//   level 2  if the previous dab touched the subtree, its box is rebuilt from its
-//            leaf boxes (ordered reduction, loads issued together) and written back
-//   level 3  when the subtree passes, up to 8 leaves per lane are tested at once
-//   level 4  passing leaves: their vertex ranges load while one atomic per wave
-//            reserves their slots in the gathered list
+// K1: search.  A thread per leaf tests the sphere against the leaf's own box: a subtree
+// box only prunes (it contains its leaf boxes, so a leaf it rejects fails its own
+// test, and a fully hidden subtree has only hidden leaves), so the gathered set is the
+// pruned search's, and a passing leaf waits on two levels: its box, then its record
+// loaded beside one slot reservation per wave.  Reading every leaf box (32 B each)
+// costs bandwidth the pruned search would not spend, but no dependent level.
 // The gathered list is unordered (consumers never depend on its order; the API sorts
-// it into the reference's left->right order when asked).  The first wave also logs
-// the previous dab's counters and clears the next dab's counter set.
+// it into the reference's left->right order when asked).  A wave after the subtree
+// waves logs the previous dab's counters and clears the next dab's counter set.
 // ---------------------------------------------------------------------------
 // The last `copy_blocks` workgroups of the search bring in the results of the previous
 // dab when it was a fused dab (fused kernel below): it wrote its affected vertices'
 // positions into the shadow copy and their normals into (mx, my, mz), because its own
 // workgroups read those arrays' old values; the copy restores positions == shadow copy.
-__device__ __forceinline__ void fused_copy_back(const SgDevView &d, int stamp, int tid, int nthreads) {
-  const int ng = d.pend[0], st = d.pend[1];
-  if (st != stamp - 1 || ng <= 0) return;
-  const int na = ng << d.aff_shift;
-  for (int i = tid; i < na; i += nthreads) {
+#define COUNT_UNROLL 8
+// A wave's sum of the per-leaf affected counts aff_cnt[0, ng): the first 64 *
+// COUNT_UNROLL counts load unconditionally (clamped indices), so they are issued before
+// ng (itself a load) is known and the sum waits on one round trip.
+struct CountPrefetch {
+  int c[COUNT_UNROLL];
+};
+__device__ __forceinline__ CountPrefetch aff_count_prefetch(const SgDevView &d, int lane) {
+  CountPrefetch p;
+#pragma unroll
+  for (int u = 0; u < COUNT_UNROLL; u++) p.c[u] = d.aff_cnt[min(lane + u * SG_WAVE, d.L - 1)];
+  return p;
+}
+__device__ __forceinline__ int aff_count_sum(const SgDevView &d, const CountPrefetch &p, int lane, int ng) {
+  int s = 0;
+#pragma unroll
+  for (int u = 0; u < COUNT_UNROLL; u++) s += lane + u * SG_WAVE < ng ? p.c[u] : 0;
+  for (int g = lane + COUNT_UNROLL * SG_WAVE; g < ng; g += SG_WAVE) s += d.aff_cnt[g];
+  return wave_sum_i(s);
+}
+
+#define COPY_UNROLL 4
+// Caller knows a fused dab's results are pending (its record: pend[0] = its gathered
+// count).
+__device__ __forceinline__ void fused_copy_back(const SgDevView &d, int tid, int nthreads) {
+  // The first COPY_UNROLL slots of each thread load beside the pending record, and all
+  // their positions and normals load before any store: unconditional loads from clamped
+  // indices (slots below L << aff_shift are allocated; vertex 0 stands in for an unused
+  // slot), so the copies wait on two levels.
+  const int nslot = d.L << d.aff_shift, msk = (1 << d.aff_shift) - 1;
+  int cnt[COPY_UNROLL], vs[COPY_UNROLL];
+#pragma unroll
+  for (int u = 0; u < COPY_UNROLL; u++) {
+    const int i = min(tid + u * nthreads, nslot - 1);
+    cnt[u] = d.aff_cnt[i >> d.aff_shift];
+    vs[u] = d.aff_v[i];
+  }
+  const int na = d.pend[0] << d.aff_shift;
+  bool ok[COPY_UNROLL];
+  float x[COPY_UNROLL], y[COPY_UNROLL], z[COPY_UNROLL], a[COPY_UNROLL], b[COPY_UNROLL], c[COPY_UNROLL];
+#pragma unroll
+  for (int u = 0; u < COPY_UNROLL; u++) {
+    const int i = tid + u * nthreads;
+    ok[u] = i < na && (i & msk) < cnt[u];
+    const int v = ok[u] ? vs[u] : 0;
+    x[u] = d.qx[v];
+    y[u] = d.qy[v];
+    z[u] = d.qz[v];
+    a[u] = d.mx[v];
+    b[u] = d.my[v];
+    c[u] = d.mz[v];
+  }
+#pragma unroll
+  for (int u = 0; u < COPY_UNROLL; u++) {
+    if (!ok[u]) continue;
+    const int v = vs[u];
+    d.px[v] = x[u];
+    d.py[v] = y[u];
+    d.pz[v] = z[u];
+    d.nx[v] = a[u];
+    d.ny[v] = b[u];
+    d.nz[v] = c[u];
+  }
+  for (int i = tid + COPY_UNROLL * nthreads; i < na; i += nthreads) {
     const int v = aff_slot_vertex(d, i, d.aff_cnt[i >> d.aff_shift]);
     if (v < 0) continue;
-    const float x = d.qx[v], y = d.qy[v], z = d.qz[v];
-    const float a = d.mx[v], b = d.my[v], c = d.mz[v];
-    d.px[v] = x;
-    d.py[v] = y;
-    d.pz[v] = z;
-    d.nx[v] = a;
-    d.ny[v] = b;
-    d.nz[v] = c;
+    const float px = d.qx[v], py = d.qy[v], pz = d.qz[v];
+    const float nx = d.mx[v], ny = d.my[v], nz = d.mz[v];
+    d.px[v] = px;
+    d.py[v] = py;
+    d.pz[v] = pz;
+    d.nx[v] = nx;
+    d.ny[v] = ny;
+    d.nz[v] = nz;
   }
 }
 
+// Block roles, in grid order:
+//   [0, leaf_blocks)          a thread per leaf: box test, then for passing leaves the
+//                             records load beside one slot reservation per wave
+//   next cut_blocks           a wave per subtree: refresh the box of a subtree the
+//                             previous dab changed (for ray casts and the statistics);
+//                             the wave after the last subtree does the bookkeeping
+//   last copy_blocks          copy-back of a fused previous dab (copy_pending: known
+//                             when the launch is recorded)
+// No test waits on a subtree box: a leaf's own box decides, so a passing leaf's chain
+// is its box, then the reservation.
 __global__ __launch_bounds__(256) void k_search(SgDevView d, SgDabK k, long long *log, int log_index,
-                                                int serial, const float *place, int copy_blocks) {
+                                                int serial, const float *place, int leaf_blocks, int cut_blocks,
+                                                int copy_blocks, int copy_pending) {
   if (d.halt && d.halt[0]) return;
-  const int stamp = d.dstamp ? d.dstamp[serial & 1] : 0;
-  if ((int)blockIdx.x >= (int)gridDim.x - copy_blocks) {
-    const int cb = blockIdx.x - (gridDim.x - copy_blocks);
-    fused_copy_back(d, stamp, cb * blockDim.x + threadIdx.x, copy_blocks * blockDim.x);
+  const int lane = lane_id();
+  if ((int)blockIdx.x >= leaf_blocks + cut_blocks) {
+    const int cb = blockIdx.x - (leaf_blocks + cut_blocks);
+    if (copy_pending) fused_copy_back(d, cb * blockDim.x + threadIdx.x, copy_blocks * blockDim.x);
     return;
   }
   sg_place(k, place);
-  const int lane = lane_id();
-  const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
-  const int nw = (gridDim.x - copy_blocks) * blockDim.x / SG_WAVE;
-  // bookkeeping in the last wave, which has the fewest or no subtrees to test, so its
-  // loads stay off the searching waves' chains
-  const int bw = nw - 1;
-  if (gw == bw && log && log_index > 0) {
-    // a fused previous dab left its affected count as per-leaf counts: sum them
-    long long naff = -1;
-    if (d.pend && d.pend[1] == stamp - 1) {
-      const int ngp = d.pend[0];
-      int s = 0;
-      for (int g = lane; g < ngp; g += SG_WAVE) s += d.aff_cnt[g];
-      naff = wave_sum_i(s);
-    }
-    if (lane < SG_CTR_COUNT) {
-      long long prev = d.ctr_next[lane];  // the set the previous dab used
-      if (lane == SG_CTR_NAFF && naff >= 0) prev = naff;
-      log[(size_t)(log_index - 1) * SG_CTR_COUNT + lane] = lane == SG_CTR_NGATHER ? (prev & 0xffffffffll) : prev;
-    }
-  }
-  if (gw == bw && lane < SG_CTR_COUNT) d.ctr_next[lane] = 0;
-  if (gw == bw && lane == 0 && d.seq) d.seq[0] += 1;  // this dab's stamp for the later kernels
-  if (gw == bw && lane == 0 && d.dstamp) d.dstamp[(serial & 1) ^ 1] = stamp + 1;  // the next dab's
-  if (gw == bw && lane == 0 && undo_active(d))
-    atomicAdd((unsigned long long *)&d.ctr[SG_CTR_NGATHER], SG_NGATHER_UNDO);
-  const float *cut_bbs = k.use_orig ? d.cut_obb : d.cut_bb;
   const float *leaf_bbs = k.use_orig ? d.leaf_obb : d.leaf_bb;
-  const float *leaf_alt = k.use_orig ? d.leaf_bb : d.leaf_obb;
+  if ((int)blockIdx.x < leaf_blocks) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lc = min(l, d.L - 1);
+    const BB6 b = bb_load(leaf_bbs + 8 * (size_t)lc);
+    const int fl = d.leaf_flags[lc];
+    const bool pass = l < d.L && !(fl & k.hide_skip) && sphere_box(b, k.cx, k.cy, k.cz, k.r2);
+    const unsigned long long m = __ballot(pass);
+    if (m == 0) return;
+    // the passing leaves' records and subtrees load beside the reservation (one round
+    // trip for both); every lane loads, the passing ones use them
+    int4 r0, r1;
+    if (d.hl_rec) {
+      r0 = d.hl_rec[2 * lc];
+      r1 = d.hl_rec[2 * lc + 1];
+    } else {
+      r0 = make_int4(d.leaf_vbeg[lc], d.leaf_sh_off[lc], 0, 0);
+      r1 = make_int4((d.leaf_vbeg[lc + 1] - d.leaf_vbeg[lc]) | ((d.leaf_sh_off[lc + 1] - d.leaf_sh_off[lc]) << 16), 0,
+                     0, 0);
+    }
+    const int cut = d.leaf_cut[lc];
+    const int stamp = d.dstamp ? d.dstamp[serial & 1] : 0;
+    int wbase = 0;
+    if (lane == 0) wbase = (int)atomicAdd((unsigned long long *)&d.ctr[SG_CTR_NGATHER], (unsigned long long)__popcll(m));
+    wbase = __shfl(wbase, 0);
+    if (pass) {
+      const int pos = wbase + mask_prefix(m);
+      d.gathered[pos] = l;
+      d.gl_v[pos] = make_int2(r0.x, r0.x + (r1.x & 0xffff));
+      d.gl_s[pos] = make_int2(r0.y, r0.y + (r1.x >> 16));
+      if (k.fused) {
+        d.gl_rec[2 * pos] = r0;
+        d.gl_rec[2 * pos + 1] = r1;
+        d.leaf_gstamp[l] = stamp;
+      }
+      // the subtree's flags (every passing leaf of it writes the same values)
+      d.cut_touch[cut] = serial;
+      d.cut_dirty[cut] = 1;
+      d.draw_dirty[cut] = 1;  // the subtree's leaves need new draw records (UpdateDrawBuffers)
+    }
+    return;
+  }
+  // subtree waves
+  const int c = (blockIdx.x - leaf_blocks) * (blockDim.x / SG_WAVE) + threadIdx.x / SG_WAVE;
+  if (c == d.C) {  // bookkeeping
+    const int stamp = d.dstamp ? d.dstamp[serial & 1] : 0;
+    if (log && log_index > 0) {
+      // a fused previous dab left its affected count as per-leaf counts: sum them
+      long long naff = -1;
+      if (copy_pending) {
+        const CountPrefetch cp = aff_count_prefetch(d, lane);
+        naff = aff_count_sum(d, cp, lane, d.pend[0]);
+      }
+      if (lane < SG_CTR_COUNT) {
+        long long prev = d.ctr_next[lane];  // the set the previous dab used
+        if (lane == SG_CTR_NAFF && naff >= 0) prev = naff;
+        log[(size_t)(log_index - 1) * SG_CTR_COUNT + lane] = lane == SG_CTR_NGATHER ? (prev & 0xffffffffll) : prev;
+      }
+    }
+    if (lane < SG_CTR_COUNT) d.ctr_next[lane] = 0;
+    if (lane == 0 && d.seq) d.seq[0] += 1;                                 // this dab's stamp for the later kernels
+    if (lane == 0 && d.dstamp) d.dstamp[(serial & 1) ^ 1] = stamp + 1;   // the next dab's
+    if (lane == 0 && undo_active(d)) atomicAdd((unsigned long long *)&d.ctr[SG_CTR_NGATHER], SG_NGATHER_UNDO);
+    return;
+  }
+  if (c > d.C) return;
+  // The previous dab changed this subtree (its leaves' boxes moved), so its box is
+  // rebuilt here from its leaf boxes, both the current and the original one.  With the
+  // statistics on, the wave also counts the tests a pruned search makes: the subtree
+  // box, and every leaf of a passing or refreshed subtree.
+  const int touch = d.cut_touch[c];
+  const int lb = d.cut_lbeg[c], le = d.cut_lend[c];
+  const bool refresh = touch == serial - 1;
   long long ntests = 0;
-  for (int c = gw; c < d.C; c += nw) {
-    const int touch = d.cut_touch[c];
-    const BB6 cb = bb_load(cut_bbs + 8 * (size_t)c);
-    const int lb = d.cut_lbeg[c], le = d.cut_lend[c];
-    const int cfl = d.cut_flags[c];
-    // The previous dab changed this subtree, so its box is stale: every leaf is tested
-    // and the leaf boxes are folded into a fresh subtree box on the way.  The subtree
-    // box only prunes (a leaf box lies inside it, so a leaf it rejects fails its own
-    // test too, and a fully hidden subtree has only hidden leaves): the gathered set
-    // is the same as with a refreshed box, one dependent load sooner.
-    const bool refresh = touch == serial - 1;
-    ntests++;
-    if (!refresh && ((cfl & k.hide_skip) || !sphere_box(cb, k.cx, k.cy, k.cz, k.r2))) continue;
-    const int nl = le - lb;
-    ntests += nl;
-    BB6 tb, ab;  // folds of the tested and the other leaf boxes (refresh only)
-    bb_reset(tb);
-    bb_reset(ab);
-    for (int q0 = 0; q0 < nl; q0 += SG_WAVE * SEARCH_UNROLL) {
-      bool pass[SEARCH_UNROLL];
-      int2 vr[SEARCH_UNROLL], sr[SEARCH_UNROLL];
-      int4 hh[SEARCH_UNROLL], hr[SEARCH_UNROLL];
-#pragma unroll
-      for (int j = 0; j < SEARCH_UNROLL; j++) {
-        const int l = lb + q0 + j * SG_WAVE + lane;
-        pass[j] = false;
-        hh[j] = hr[j] = make_int4(0, 0, 0, 0);
-        if (l < le) {
-          const BB6 b = bb_load(leaf_bbs + 8 * (size_t)l);
-          const int fl = d.leaf_flags[l];
-          // the leaf's vertex ranges are loaded with its box, ahead of the test, so
-          // the writes after the slot reservation wait on nothing else
-          vr[j] = make_int2(d.leaf_vbeg[l], d.leaf_vbeg[l + 1]);
-          sr[j] = make_int2(d.leaf_sh_off[l], d.leaf_sh_off[l + 1]);
-          if (k.fused) {
-            hh[j] = d.hl_hdr[l];
-            hr[j] = d.hl_ring[l];
-          }
-          if (refresh) {
-            bb_add_bb(tb, b);
-            bb_add_bb(ab, bb_load(leaf_alt + 8 * (size_t)l));
-          }
-          pass[j] = !(fl & k.hide_skip) && sphere_box(b, k.cx, k.cy, k.cz, k.r2);
-        }
-      }
-      unsigned long long m[SEARCH_UNROLL];
-      int cnt = 0;
-#pragma unroll
-      for (int j = 0; j < SEARCH_UNROLL; j++) {
-        m[j] = __ballot(pass[j]);
-        cnt += __popcll(m[j]);
-      }
-      if (cnt == 0) continue;
-      int wbase = 0;
-      if (lane == 0) wbase = (int)atomicAdd((unsigned long long *)&d.ctr[SG_CTR_NGATHER], (unsigned long long)cnt);
-      wbase = __shfl(wbase, 0);
-      int off = wbase;
-#pragma unroll
-      for (int j = 0; j < SEARCH_UNROLL; j++) {
-        if (pass[j]) {
-          const int pos = off + mask_prefix(m[j]);
-          const int l = lb + q0 + j * SG_WAVE + lane;
-          d.gathered[pos] = l;
-          d.gl_v[pos] = vr[j];
-          d.gl_s[pos] = sr[j];
-          if (k.fused) {
-            d.gl_h[pos] = hh[j];
-            d.gl_r[pos] = hr[j];
-            d.leaf_gstamp[l] = stamp;
-          }
-        }
-        off += __popcll(m[j]);
-      }
-      if (lane == 0) {
-        d.cut_touch[c] = serial;
-        d.cut_dirty[c] = 1;
-        d.draw_dirty[c] = 1;  // the subtree's leaves need new draw records (UpdateDrawBuffers)
-      }
-    }
-    if (refresh) {
-      tb = bb_wave_reduce(tb);
-      ab = bb_wave_reduce(ab);
-      if (lane == 0) {
-        bb_store(d.cut_bb + 8 * (size_t)c, k.use_orig ? ab : tb);
-        bb_store(d.cut_obb + 8 * (size_t)c, k.use_orig ? tb : ab);
-      }
-    }
-  }
   if (k.stats) {
-    if (lane == 0) atomicAdd((unsigned long long *)&d.ctr[SG_CTR_TESTS], (unsigned long long)ntests);
+    const BB6 cb = bb_load((k.use_orig ? d.cut_obb : d.cut_bb) + 8 * (size_t)c);
+    const int cfl = d.cut_flags[c];
+    const bool hit = !(cfl & k.hide_skip) && sphere_box(cb, k.cx, k.cy, k.cz, k.r2);
+    ntests = 1 + ((refresh || hit) ? le - lb : 0);
   }
+  if (refresh) {
+    BB6 tb, ob;
+    bb_reset(tb);
+    bb_reset(ob);
+    for (int q0 = lb; q0 < le; q0 += SG_WAVE * SEARCH_UNROLL) {
+      BB6 nb[SEARCH_UNROLL], xb[SEARCH_UNROLL];
+#pragma unroll
+      for (int j = 0; j < SEARCH_UNROLL; j++) {
+        const int l = min(q0 + j * SG_WAVE + lane, le - 1);
+        nb[j] = bb_load(d.leaf_bb + 8 * (size_t)l);
+        xb[j] = bb_load(d.leaf_obb + 8 * (size_t)l);
+      }
+#pragma unroll
+      for (int j = 0; j < SEARCH_UNROLL; j++) {
+        if (q0 + j * SG_WAVE + lane < le) {
+          bb_add_bb(tb, nb[j]);
+          bb_add_bb(ob, xb[j]);
+        }
+      }
+    }
+    tb = bb_wave_reduce(tb);
+    ob = bb_wave_reduce(ob);
+    if (lane == 0) {
+      bb_store(d.cut_bb + 8 * (size_t)c, tb);
+      bb_store(d.cut_obb + 8 * (size_t)c, ob);
+    }
+  }
+  if (k.stats && lane == 0) atomicAdd((unsigned long long *)&d.ctr[SG_CTR_TESTS], (unsigned long long)ntests);
 }
 
 // zero both counter sets and the halt flag at the start of a stroke; the dab stamps
@@ -730,15 +786,15 @@ __global__ __launch_bounds__(256) void k_post(SgDevView d, int stats, int stamp,
   const int leaf0 = rw < d.L ? d.gathered[rw] : 0;
   const int2 vr0 = rw < d.L ? d.gl_v[rw] : make_int2(0, 0);
   const int2 sr0 = rw < d.L ? d.gl_s[rw] : make_int2(0, 0);
+  CountPrefetch cp;
+  if (gw == nw / 2) cp = aff_count_prefetch(d, lane_id());
   const int ng = (int)d.ctr[SG_CTR_NGATHER];
   const int na = ng << d.aff_shift;  // slots of the gathered leaves
   long long ntri = 0, nfa = 0, nvfa = 0, nsh = 0;
   // the dab's affected count for the host and the logs: a wave halfway through the
   // grid (between the normals' waves and the refits') sums the leaves' counts
   if (gw == nw / 2) {
-    int s = 0;
-    for (int g = lane_id(); g < ng; g += SG_WAVE) s += d.aff_cnt[g];
-    s = wave_sum_i(s);
+    const int s = aff_count_sum(d, cp, lane_id(), ng);
     if (lane_id() == 0) d.ctr[SG_CTR_NAFF] = s;
   }
   for (int i = tid; i < na; i += nthreads) {
@@ -787,37 +843,33 @@ __global__ __launch_bounds__(256) void k_post(SgDevView d, int stats, int stamp,
 // per-leaf vertex test decide it for that leaf.
 // ---------------------------------------------------------------------------
 #define FUSED_THREADS 256
+#define FUSED_ROW 8  // neighbour / incidence entries per local held in registers
 
-struct FusedLds {
-  float *x[2], *y[2], *z[2], *fade;
-  unsigned char *aff;
-  unsigned short *nbl, *shl;
-};
-
-__device__ __forceinline__ FusedLds fused_lds(unsigned char *base, int cap, int nbl_cap, int sh_cap) {
-  FusedLds L;
-  float *f = (float *)base;
-  for (int b = 0; b < 2; b++) {
-    L.x[b] = f;
-    L.y[b] = f + cap;
-    L.z[b] = f + 2 * cap;
-    f += 3 * cap;
-  }
-  L.fade = f;
-  f += cap;
-  L.nbl = (unsigned short *)f;
-  L.shl = L.nbl + nbl_cap;
-  L.aff = (unsigned char *)(L.shl + sh_cap);
-  return L;
-}
-
+// One workgroup per gathered leaf; each thread holds LPT locals (local i = tid + j * 256).
+// Global loads come in three dependent levels, each issued for every local at once:
+//   1  the counters and the workgroup's first gathered leaf (ranges, halo header)
+//   2  halo vertex ids, neighbour-row and incidence ranges, shared-vertex local ids
+//   3  positions, normals, masks, owner stamps, neighbour rows (first 8 entries) and
+//      incidence corners (first 8), into registers
+// after which the brush, the Jacobi steps, the normals and the box run from LDS and
+// registers.  Rows longer than 8 entries read the rest from memory where they are used.
+template <int LPT>
 __global__ __launch_bounds__(FUSED_THREADS) void k_dab_fused(SgDevView d, SgDabK k, const float *curve, int serial,
-                                                            int cap, int nbl_cap, int sh_cap) {
+                                                            int cap) {
   if (d.halt && d.halt[0]) return;
   extern __shared__ __align__(16) unsigned char fused_smem[];
   __shared__ int s_cnt;
-  const FusedLds S = fused_lds(fused_smem, cap, nbl_cap, sh_cap);
+  // positions of the locals, two buffers (x, y, z runs of `cap` floats each)
+  float *const S = (float *)fused_smem;
+  unsigned short *Sshl = (unsigned short *)((float *)fused_smem + 6 * cap);  // shared vertices' local ids
   const int tid = threadIdx.x;
+  // level 1: counters, stamp and the first gathered leaf's records together.  Loads
+  // below are unconditional from clamped (always valid) indices, with the selection
+  // after them: a load under a branch makes the compiler wait for it at the join.
+  const int g0 = blockIdx.x;
+  const int gc = min(g0, d.L - 1);
+  const int leaf0 = d.gathered[gc];
+  const int4 ra0 = d.gl_rec[2 * gc], rb0 = d.gl_rec[2 * gc + 1];
   const long long ng_ctr = d.ctr[SG_CTR_NGATHER];
   const int ng = (int)ng_ctr;
   const int stamp = d.dstamp[serial & 1];
@@ -838,125 +890,194 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_dab_fused(SgDevView d, SgDabK
     oy = (ay * k.r) * k.bstr;
     oz = (az * k.r) * k.bstr;
   }
-  for (int g = blockIdx.x; g < ng; g += gridDim.x) {
-    const int leaf = d.gathered[g];
-    const int2 vr = d.gl_v[g], sr = d.gl_s[g];
-    const int4 hh = d.gl_h[g], hr = d.gl_r[g];
-    const int nu = vr.y - vr.x, ns = sr.y - sr.x;
-    const int re[SG_HALO_RINGS + 1] = {hh.y, hr.x, hr.y, hr.z, hr.w};  // ext count at each ring's end
-    const int nloc = nu + re[K];
-    const int ninner = K > 0 ? nu + re[K - 1] : 0;  // locals the first Jacobi step updates
+  const bool hide = d.vhide && d.respect_hide;
+  for (int g = g0; g < ng; g += gridDim.x) {
+    const bool first = g == g0;
+    const int leaf = first ? leaf0 : d.gathered[g];
+    const int4 ra = first ? ra0 : d.gl_rec[2 * g], rb = first ? rb0 : d.gl_rec[2 * g + 1];
+    const int nu = rb.x & 0xffff, ns = rb.x >> 16;
+    const int2 vr = make_int2(ra.x, ra.x + nu), sr = make_int2(ra.y, ra.y + ns);
+    const int ext_off = ra.z, nb_blk = ra.w;
+    // ext count at the end of ring j
+    auto ring_end = [&](int j) {
+      return j <= 0 ? rb.y & 0xffff : j == 1 ? rb.y >> 16 : j == 2 ? rb.z & 0xffff : j == 3 ? rb.z >> 16 : rb.w;
+    };
+    const int nloc = nu + ring_end(K);
+    const int ninner = K > 0 ? nu + ring_end(K - 1) : 0;  // locals the first Jacobi step updates
     if (tid == 0) s_cnt = 0;
     if ((ng_ctr & SG_NGATHER_UNDO) && tid < SG_WAVE) undo_save_leaf(d, leaf, vr, d.px, d.py, d.pz);
-    for (int i = tid; i < ns; i += FUSED_THREADS) S.shl[i] = d.hl_sh[sr.x + i];
-    if (K > 0) {
-      const int nb0 = d.hl_nboff[hh.z], nb1 = d.hl_nboff[hh.z + ninner];
-      for (int j = nb0 + tid; j < nb1; j += FUSED_THREADS) S.nbl[j - nb0] = d.hl_nbl[j];
-    }
-    for (int i = tid; i < nloc; i += FUSED_THREADS) {
-      int v;
-      bool own = true;
-      if (i < nu) {
-        v = vr.x + i;
-      } else {
-        const int2 e = d.hl_v[hh.x + i - nu];
-        v = e.x;
-        own = d.leaf_gstamp[e.y] == stamp;  // its leaf was gathered by this dab's search
+    // level 2
+    int vid[LPT], owner[LPT], nbb[LPT], nbe[LPT];
+#pragma unroll
+    for (int j = 0; j < LPT; j++) {
+      const int i = tid + j * FUSED_THREADS;
+      const bool ext = i >= nu && i < nloc, inner = i < ninner;
+      // (id, owner) as one 64-bit load, so both halves stay where they are issued
+      const long long e = *(const long long *)(d.hl_v + (ext ? ext_off + i - nu : 0));
+      const int b = d.hl_nboff[inner ? nb_blk + i : 0], en = d.hl_nboff[inner ? nb_blk + i + 1 : 0];
+      int id = (int)e, ow = (int)(e >> 32);
+      if (!ext) {
+        id = -1;
+        ow = -1;
       }
-      const float x = d.px[v], y = d.py[v], z = d.pz[v];
-      const float dx = k.cx - x;
-      const float dy = k.cy - y;
-      const float dz = k.cz - z;
+      if (i < nu) id = vr.x + i;
+      vid[j] = id;
+      owner[j] = ow;
+      nbb[j] = inner ? b : 0;
+      nbe[j] = inner ? en : 0;
+    }
+    const bool uq0 = tid < nu;
+    const int v0 = uq0 ? vr.x + tid : 0;
+    const int qb0r = d.vt_off[v0], qe0r = d.vt_off[v0 + 1];
+    const int shv0 = d.hl_sh[tid < ns ? sr.x + tid : 0];
+    const int qb0 = uq0 ? qb0r : 0, qe0 = uq0 ? qe0r : 0;
+    // level 3
+    const float *mp = d.mask ? d.mask : d.px;  // stand-ins keep the loads unconditional
+    const int *hp = hide ? d.vhide : (const int *)d.px;
+    float x[LPT], y[LPT], z[LPT], nxv[LPT], nyv[LPT], nzv[LPT], mk[LPT];
+    bool own[LPT], hid[LPT];
+    unsigned short nb[LPT][FUSED_ROW];
+#pragma unroll
+    for (int j = 0; j < LPT; j++) {
+      const int v = vid[j] < 0 ? 0 : vid[j];
+      x[j] = d.px[v];
+      y[j] = d.py[v];
+      z[j] = d.pz[v];
+      nxv[j] = d.nx[v];
+      nyv[j] = d.ny[v];
+      nzv[j] = d.nz[v];
+      const float m = mp[v];
+      const int hv = hp[v];
+      const int gs = d.leaf_gstamp[owner[j] < 0 ? 0 : owner[j]];
+      mk[j] = d.mask ? m : 0.0f;
+      hid[j] = hide && hv != 0;
+      own[j] = owner[j] < 0 || gs == stamp;  // its leaf was gathered by this search
+#pragma unroll
+      for (int q = 0; q < FUSED_ROW; q++) nb[j][q] = d.hl_nbl[nbb[j] + q < nbe[j] ? nbb[j] + q : 0];
+    }
+    uint2 tri0[FUSED_ROW];
+#pragma unroll
+    for (int q = 0; q < FUSED_ROW; q++) tri0[q] = d.hl_tri[qb0 + q < qe0 ? qb0 + q : 0];
+    // brush (spec S2-S9) on every local
+    float fade[LPT];
+    bool aff[LPT];
+#pragma unroll
+    for (int j = 0; j < LPT; j++) {
+      const int i = tid + j * FUSED_THREADS;
+      const float dx = k.cx - x[j];
+      const float dy = k.cy - y[j];
+      const float dz = k.cz - z[j];
       const float d2 = dx * dx + dy * dy + dz * dz;
-      bool aff = own && d2 <= k.r2;
-      if (d.vhide && d.respect_hide && d.vhide[v]) aff = false;
-      float rx = x, ry = y, rz = z, fade = 0.0f;
-      if (aff) {
-        const float f = sg_falloff(k, curve, sqrtf(d2)) * (1.0f - (d.mask ? d.mask[v] : 0.0f));
+      aff[j] = vid[j] >= 0 && own[j] && d2 <= k.r2 && !hid[j];
+      fade[j] = 0.0f;
+      if (aff[j]) {
+        const float f = sg_falloff(k, curve, sqrtf(d2)) * (1.0f - mk[j]);
         if (k.tool == SG_TOOL_DRAW) {
-          rx = x + ox * f;
-          ry = y + oy * f;
-          rz = z + oz * f;
+          x[j] = x[j] + ox * f;
+          y[j] = y[j] + oy * f;
+          z[j] = z[j] + oz * f;
         } else if (k.tool == SG_TOOL_INFLATE) {
           const float sc = (k.bstr * f) * k.r;
-          rx = x + d.nx[v] * sc;
-          ry = y + d.ny[v] * sc;
-          rz = z + d.nz[v] * sc;
+          x[j] = x[j] + nxv[j] * sc;
+          y[j] = y[j] + nyv[j] * sc;
+          z[j] = z[j] + nzv[j] * sc;
         } else if (k.tool == SG_TOOL_GRAB) {
-          fade = k.bstr * f;
-          rx = x + k.gx * fade;
-          ry = y + k.gy * fade;
-          rz = z + k.gz * fade;
+          fade[j] = k.bstr * f;
+          x[j] = x[j] + k.gx * fade[j];
+          y[j] = y[j] + k.gy * fade[j];
+          z[j] = z[j] + k.gz * fade[j];
         } else {
-          fade = k.bstr * f;
+          fade[j] = k.bstr * f;
         }
       }
-      S.x[0][i] = rx;
-      S.y[0][i] = ry;
-      S.z[0][i] = rz;
-      S.fade[i] = fade;
-      S.aff[i] = aff ? 1 : 0;
+      if (i < nloc) {
+        S[i] = x[j];
+        S[cap + i] = y[j];
+        S[2 * cap + i] = z[j];
+      }
     }
+    if (tid < ns) Sshl[tid] = (unsigned short)shv0;
+    for (int i = tid + FUSED_THREADS; i < ns; i += FUSED_THREADS) Sshl[i] = d.hl_sh[sr.x + i];
     __syncthreads();
     // Smooth: Jacobi step t updates the locals of rings <= K - t from step t - 1
     int cur = 0;
-    if (K > 0) {
-      const int nb0 = d.hl_nboff[hh.z];
-      for (int t = 1; t <= K; t++) {
-        const int nact = nu + re[K - t];
-        for (int i = tid; i < nact; i += FUSED_THREADS) {
-          const float x = S.x[cur][i], y = S.y[cur][i], z = S.z[cur][i];
-          float rx = x, ry = y, rz = z;
-          if (S.aff[i]) {
-            const int b = d.hl_nboff[hh.z + i] - nb0, e = d.hl_nboff[hh.z + i + 1] - nb0;
+    for (int t = 1; t <= K; t++) {
+      const int nact = nu + ring_end(K - t);
+      const float *X = S + cur * 3 * cap;
+      float *W = S + (cur ^ 1) * 3 * cap;
+#pragma unroll
+      for (int j = 0; j < LPT; j++) {
+        const int i = tid + j * FUSED_THREADS;
+        if (i < nact) {
+          if (aff[j]) {
+            const int cnt = nbe[j] - nbb[j];
             float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-            for (int j = b; j < e; j++) {
-              const int u = S.nbl[j];
-              sx = sx + S.x[cur][u];
-              sy = sy + S.y[cur][u];
-              sz = sz + S.z[cur][u];
+#pragma unroll
+            for (int q = 0; q < FUSED_ROW; q++) {
+              if (q < cnt) {
+                const int u = nb[j][q];
+                sx = sx + X[u];
+                sy = sy + X[cap + u];
+                sz = sz + X[2 * cap + u];
+              }
             }
-            if (e > b) {
-              const float fade = S.fade[i];
-              const float inv = 1.0f / (float)(e - b);
+            for (int q = FUSED_ROW; q < cnt; q++) {
+              const int u = d.hl_nbl[nbb[j] + q];
+              sx = sx + X[u];
+              sy = sy + X[cap + u];
+              sz = sz + X[2 * cap + u];
+            }
+            if (cnt > 0) {
+              const float inv = 1.0f / (float)cnt;
               const float avx = sx * inv, avy = sy * inv, avz = sz * inv;
-              rx = x + (avx - x) * fade;
-              ry = y + (avy - y) * fade;
-              rz = z + (avz - z) * fade;
+              x[j] = x[j] + (avx - x[j]) * fade[j];
+              y[j] = y[j] + (avy - y[j]) * fade[j];
+              z[j] = z[j] + (avz - z[j]) * fade[j];
             }
           }
-          S.x[cur ^ 1][i] = rx;
-          S.y[cur ^ 1][i] = ry;
-          S.z[cur ^ 1][i] = rz;
+          W[i] = x[j];
+          W[cap + i] = y[j];
+          W[2 * cap + i] = z[j];
         }
-        __syncthreads();
-        cur ^= 1;
       }
+      __syncthreads();
+      cur ^= 1;
     }
     // the affected uniq vertices: normal (spec S10, incident tris in vertex->tri order),
     // position, list record
-    for (int i = tid; i < nu; i += FUSED_THREADS) {
-      if (!S.aff[i]) continue;
+    const float *F = S + cur * 3 * cap;  // final positions of the locals
+#pragma unroll
+    for (int j = 0; j < LPT; j++) {
+      const int i = tid + j * FUSED_THREADS;
+      if (i >= nu || !aff[j]) continue;
       const int v = vr.x + i;
-      const int qb = d.vt_off[v], qe = d.vt_off[v + 1];
       float ax = 0.0f, ay = 0.0f, az = 0.0f;
-      for (int q = qb; q < qe; q++) {
-        const uint2 c = d.hl_tri[q];
+      auto add_tri = [&](uint2 c) {
         const int a = c.x & 0xffff, b = c.x >> 16, e = c.y;
         float fx, fy, fz;
-        sg_tri_normal(S.x[cur][a], S.y[cur][a], S.z[cur][a], S.x[cur][b], S.y[cur][b], S.z[cur][b], S.x[cur][e],
-                      S.y[cur][e], S.z[cur][e], fx, fy, fz);
+        sg_tri_normal(F[a], F[cap + a], F[2 * cap + a], F[b], F[cap + b], F[2 * cap + b], F[e], F[cap + e],
+                      F[2 * cap + e], fx, fy, fz);
         ax = ax + fx;
         ay = ay + fy;
         az = az + fz;
+      };
+      if (j == 0) {
+        const int cnt = qe0 - qb0;
+#pragma unroll
+        for (int q = 0; q < FUSED_ROW; q++)
+          if (q < cnt) add_tri(tri0[q]);
+        for (int q = qb0 + FUSED_ROW; q < qe0; q++) add_tri(d.hl_tri[q]);
+      } else {  // leaves of more than FUSED_THREADS uniq vertices
+        const int qb = d.vt_off[v], qe = d.vt_off[v + 1];
+        for (int q = qb; q < qe; q++) add_tri(d.hl_tri[q]);
       }
       sg_normalize(ax, ay, az);
       d.mx[v] = ax;
       d.my[v] = ay;
       d.mz[v] = az;
-      d.qx[v] = S.x[cur][i];
-      d.qy[v] = S.y[cur][i];
-      d.qz[v] = S.z[cur][i];
+      d.qx[v] = x[j];
+      d.qy[v] = y[j];
+      d.qz[v] = z[j];
       const int pos = atomicAdd(&s_cnt, 1);
       d.aff_v[(g << d.aff_shift) + pos] = v;
     }
@@ -968,8 +1089,8 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_dab_fused(SgDevView d, SgDabK
       for (int c = 0; c < per; c++) {
         const int i = tid * per + c;
         if (i < tot) {
-          const int u = i < nu ? i : S.shl[i - nu];
-          bb_add_point(acc, S.x[cur][u], S.y[cur][u], S.z[cur][u]);
+          const int u = i < nu ? i : Sshl[i - nu];
+          bb_add_point(acc, F[u], F[cap + u], F[2 * cap + u]);
         }
       }
       acc = bb_wave_reduce(acc);
@@ -989,22 +1110,10 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_dab_fused(SgDevView d, SgDabK
 __global__ __launch_bounds__(256) void k_dab_finish(SgDevView d, int serial) {
   const int stamp = d.dstamp[serial & 1];
   if (d.pend[1] != stamp) return;
-  const int ng = d.pend[0];
-  const int na = ng << d.aff_shift;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
-    const int v = aff_slot_vertex(d, i, d.aff_cnt[i >> d.aff_shift]);
-    if (v < 0) continue;
-    d.px[v] = d.qx[v];
-    d.py[v] = d.qy[v];
-    d.pz[v] = d.qz[v];
-    d.nx[v] = d.mx[v];
-    d.ny[v] = d.my[v];
-    d.nz[v] = d.mz[v];
-  }
+  fused_copy_back(d, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
   if (blockIdx.x == 0 && threadIdx.x < SG_WAVE) {
-    int s = 0;
-    for (int g = threadIdx.x; g < ng; g += SG_WAVE) s += d.aff_cnt[g];
-    s = wave_sum_i(s);
+    const CountPrefetch cp = aff_count_prefetch(d, threadIdx.x);
+    const int s = aff_count_sum(d, cp, threadIdx.x, d.pend[0]);
     if (threadIdx.x == 0) d.ctr[SG_CTR_NAFF] = s;
   }
 }
@@ -1445,15 +1554,13 @@ SgDevView sg_view_of(SgDev &d, bool respect_hide, bool halt_live) {
   v.ctr = d.ctr;
   v.ctr_next = d.ctr + SG_CTR_COUNT;
   v.cut_touch = d.cut_touch;
-  v.hl_hdr = d.hl_hdr;
-  v.hl_ring = d.hl_ring;
+  v.hl_rec = d.hl_rec;
   v.hl_v = d.hl_v;
   v.hl_nboff = d.hl_nboff;
   v.hl_nbl = d.hl_nbl;
   v.hl_sh = d.hl_sh;
   v.hl_tri = d.hl_tri;
-  v.gl_h = d.gl_h;
-  v.gl_r = d.gl_r;
+  v.gl_rec = d.gl_rec;
   v.leaf_gstamp = d.leaf_gstamp;
   v.dstamp = d.dstamp;
   v.pend = d.pend;
@@ -1591,27 +1698,26 @@ int sg_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long long *
   // dab is a plain brush dab it covers
   SgDabK kk = k;
   kk.fused = 0;
-  const size_t fused_lds = 0;
-  int fused_cap = 0;
   const bool fused_tool = k.tool == SG_TOOL_DRAW || k.tool == SG_TOOL_INFLATE || k.tool == SG_TOOL_GRAB ||
                           (k.tool == SG_TOOL_SMOOTH && k.iters >= 1 && k.iters <= SG_HALO_RINGS);
-  int nbl_cap = 0, sh_cap = 0;
+  int fused_cap = 0, lpt = 0;
   size_t lds = 0;
   if (p->d.halo_ok && p->fused_enabled && fused_tool && !k.stats && !k.orig_co && !(k.learned && p->mlp) && !col &&
       !place) {
-    fused_cap = k.tool == SG_TOOL_SMOOTH ? p->d.halo_maxloc : p->d.halo_maxloc0;
-    nbl_cap = (k.tool == SG_TOOL_SMOOTH ? p->d.halo_maxnbl : 0) + 2;
-    sh_cap = p->d.leaf_maxsh + 2;
-    lds = (size_t)fused_cap * 7 * sizeof(float) + (size_t)(nbl_cap + sh_cap) * 2 + (size_t)fused_cap + 16;
-    kk.fused = lds <= 65536 ? 1 : 0;  // larger halos (very large leaves) take the separate kernels
+    fused_cap = p->d.halo_maxlocr[k.tool == SG_TOOL_SMOOTH ? k.iters : 0];
+    lpt = (fused_cap + FUSED_THREADS - 1) / FUSED_THREADS;
+    lds = (size_t)fused_cap * 6 * sizeof(float) + (size_t)(p->d.leaf_maxsh + 2) * 2 + 16;
+    // larger halos (very large leaves) take the separate kernels
+    kk.fused = lpt >= 1 && lpt <= 4 && lds <= 65536 ? 1 : 0;
   }
-  (void)fused_lds;
   const int copy_blocks = p->d.halo_ok ? 64 : 0;
   {
     ProfScope ps(p, SG_K_SEARCH);
-    const int sg = std::max(1, (p->d.C + 3) / 4);
-    SG_LAUNCH(ps, k_search, dim3(sg + copy_blocks), dim3(256), 0, s, v, kk, log, log_index, serial, place,
-              copy_blocks);
+    const int leaf_blocks = std::max(1, (p->d.L + 255) / 256);
+    const int cut_blocks = (p->d.C + 1 + 3) / 4;  // a wave per subtree and one for the bookkeeping
+    SG_LAUNCH(ps, k_search, dim3(leaf_blocks + cut_blocks + copy_blocks), dim3(256), 0, s, v, kk, log, log_index,
+              serial, place, leaf_blocks, cut_blocks, copy_blocks, p->fused_pending ? 1 : 0);
+    p->fused_pending = kk.fused != 0;
   }
   if (kk.fused) {
     if (k.tool == SG_TOOL_DRAW) {
@@ -1619,8 +1725,15 @@ int sg_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long long *
       SG_LAUNCH(ps, k_area, dim3(GB), dim3(256), 0, s, v, k, place);
     }
     ProfScope ps(p, SG_K_BRUSH);
-    SG_LAUNCH(ps, k_dab_fused, dim3(p->grid_fused), dim3(FUSED_THREADS), lds, s, v, kk, curve_dev, serial, fused_cap,
-              nbl_cap, sh_cap);
+    const dim3 gf(p->grid_fused), bf(FUSED_THREADS);
+    if (lpt == 1)
+      SG_LAUNCH(ps, k_dab_fused<1>, gf, bf, lds, s, v, kk, curve_dev, serial, fused_cap);
+    else if (lpt == 2)
+      SG_LAUNCH(ps, k_dab_fused<2>, gf, bf, lds, s, v, kk, curve_dev, serial, fused_cap);
+    else if (lpt == 3)
+      SG_LAUNCH(ps, k_dab_fused<3>, gf, bf, lds, s, v, kk, curve_dev, serial, fused_cap);
+    else
+      SG_LAUNCH(ps, k_dab_fused<4>, gf, bf, lds, s, v, kk, curve_dev, serial, fused_cap);
     return sg_check_hip(hipGetLastError(), "dab launch");
   }
   if (k.tool == SG_TOOL_ELASTIC_DEFORM) {  // every leaf; then all normals, leaf boxes and the tree
@@ -1759,6 +1872,7 @@ int sg_enqueue_coords_apply(PBVH *p, const float *src_dev) {
 }
 
 int sg_enqueue_finish(PBVH *p, int serial_last) {
+  p->fused_pending = false;
   if (!p->d.halo_ok) return 0;
   SgDevView v = sg_view(p);
   v.ctr = p->d.ctr + (size_t)(serial_last & 1) * SG_CTR_COUNT;  // the last dab's counter set

@@ -690,7 +690,7 @@ static bool build_halo(PBVH *p, const SgLayout &h, hipStream_t s) {
   for (auto &x : th) x.join();
   if (bad) return false;
   // concatenate
-  std::vector<int4> hdr(L), rng(L);
+  std::vector<int4> rec(2 * (size_t)std::max(L, 1));
   size_t next_ext = 0, next_nb = 0, next_nbl = 0;
   int maxloc = 0, maxloc0 = 0, maxnbl = 0, maxsh = 0;
   for (int l = 0; l < L; l++) {
@@ -698,13 +698,17 @@ static bool build_halo(PBVH *p, const SgLayout &h, hipStream_t s) {
     const int nu = h.leaf_vbeg[l + 1] - h.leaf_vbeg[l];
     maxnbl = std::max(maxnbl, (int)o.nbl.size());
     maxsh = std::max(maxsh, h.leaf_sh_off[l + 1] - h.leaf_sh_off[l]);
-    hdr[l] = make_int4((int)next_ext, o.ring_end[0], (int)next_nb, o.ring_end[R]);
-    rng[l] = make_int4(o.ring_end[1], o.ring_end[2], o.ring_end[3], o.ring_end[4]);
+    const int ns = h.leaf_sh_off[l + 1] - h.leaf_sh_off[l];
+    if (nu > 65535 || ns > 65535 || o.ring_end[R] > 65535) return false;  // the record's 16-bit fields
+    rec[2 * l] = make_int4(h.leaf_vbeg[l], h.leaf_sh_off[l], (int)next_ext, (int)next_nb);
+    rec[2 * l + 1] = make_int4(nu | (ns << 16), o.ring_end[0] | (o.ring_end[1] << 16),
+                               o.ring_end[2] | (o.ring_end[3] << 16), o.ring_end[4]);
     next_ext += o.ext.size();
     next_nb += o.nboff.size();
     next_nbl += o.nbl.size();
     maxloc = std::max(maxloc, nu + o.ring_end[R]);
     maxloc0 = std::max(maxloc0, nu + o.ring_end[0]);
+    for (int j = 0; j <= R; j++) d.halo_maxlocr[j] = std::max(d.halo_maxlocr[j], nu + o.ring_end[j]);
   }
   if (next_ext >= (1ull << 31) || next_nbl >= (1ull << 31)) return false;
   std::vector<int2> ext(std::max<size_t>(next_ext, 1));
@@ -725,8 +729,7 @@ static bool build_halo(PBVH *p, const SgLayout &h, hipStream_t s) {
   lv.clear();
   lv.shrink_to_fit();
   int rc = 0;
-  rc |= upload(&d.hl_hdr, hdr.data(), L, s);
-  rc |= upload(&d.hl_ring, rng.data(), L, s);
+  rc |= upload(&d.hl_rec, rec.data(), rec.size(), s);
   rc |= upload(&d.hl_v, ext.data(), ext.size(), s);
   rc |= upload(&d.hl_nboff, nboff.data(), nboff.size(), s);
   rc |= upload(&d.hl_nbl, nbl.data(), nbl.size(), s);
@@ -738,14 +741,13 @@ static bool build_halo(PBVH *p, const SgLayout &h, hipStream_t s) {
   rc |= alloc_fill(&d.mx, V, 0, s);
   rc |= alloc_fill(&d.my, V, 0, s);
   rc |= alloc_fill(&d.mz, V, 0, s);
-  rc |= alloc_fill(&d.gl_h, std::max(L, 1), 0, s);
-  rc |= alloc_fill(&d.gl_r, std::max(L, 1), 0, s);
+  rc |= alloc_fill(&d.gl_rec, 2 * std::max(L, 1), 0, s);
   hipStreamSynchronize(s);  // the host tables are freed on return
   d.halo_maxloc = maxloc;
   d.halo_maxloc0 = maxloc0;
   d.halo_maxnbl = maxnbl;
   d.leaf_maxsh = maxsh;
-  d.halo_bytes = (long long)(L * 32 + ext.size() * 8 + nboff.size() * 4 + nbl.size() * 2 + tri.size() * 8 +
+  d.halo_bytes = (long long)(rec.size() * 16 + ext.size() * 8 + nboff.size() * 4 + nbl.size() * 2 + tri.size() * 8 +
                              shl.size() * 2);
   return rc == 0;
 }
@@ -970,9 +972,9 @@ void sg_free_device(PBVH *p) {
                   d.draw_dirty, d.draw_tri, d.draw_off, d.draw_leaf, d.draw_fset, d.draw_poly_off,
                   d.draw_poly_v, d.draw_vbo, d.undo, d.undo_stamp, d.undo_slot, d.place_pair,
                   d.prox_dev, d.prox_tab, d.prox_leaves, d.prox_off, d.poly_off, d.poly_v,
-                  d.poly_of_tri, d.vc_off, d.vc, d.stage, d.hl_hdr, d.hl_ring, d.hl_v, d.hl_nboff,
+                  d.poly_of_tri, d.vc_off, d.vc, d.stage, d.hl_rec, d.hl_v, d.hl_nboff,
                   d.hl_nbl, d.hl_tri, d.hl_sh, d.leaf_gstamp, d.dstamp, d.mx, d.my, d.mz, d.pend,
-                  d.gl_h, d.gl_r};
+                  d.gl_rec};
   for (void *q : ptrs) {
     if (q) hipFree(q);
   }

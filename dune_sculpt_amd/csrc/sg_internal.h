@@ -9,6 +9,8 @@
 
 #include "../../include/sculpt_gpu.h"
 
+#define SG_HALO_RINGS 4  // Smooth(k) runs fused for k <= this
+
 // Cut of the tree used by the two-level sphere search: maximal subtrees holding at
 // most cut_leaf_cap leaves.  Leaves are numbered in DFS (left->right) order, so every
 // cut subtree owns a contiguous leaf range.
@@ -163,8 +165,10 @@ struct SgDev {
   // edge neighbours of its uniq vertices), then rings 1..SG_HALO_RINGS around those.
   bool halo_ok = false;
   int halo_maxloc = 0, halo_maxloc0 = 0;   // most local vertices of a leaf: all rings / ring 0
-  int4 *hl_hdr = nullptr;       // [L] ext offset, ring-0 count, neighbour-offset block start, total ext count
-  int4 *hl_ring = nullptr;      // [L] ext counts at the ends of rings 1..4
+  // [2L] leaf record: {uniq begin, shared begin, ext offset, neighbour-offset block
+  // start}, {nu | ns << 16, e0 | e1 << 16, e2 | e3 << 16, e4} with e_j the ext count at
+  // the end of ring j (every count below 65536)
+  int4 *hl_rec = nullptr;
   int2 *hl_v = nullptr;         // [ext] vertex id, owner leaf (the leaf holding it as uniq)
   int *hl_nboff = nullptr;      // per leaf: offsets into hl_nbl of its locals before the last ring (+1)
   unsigned short *hl_nbl = nullptr;  // local ids of those locals' edge neighbours, vertex->vertex order
@@ -174,8 +178,9 @@ struct SgDev {
   int *dstamp = nullptr;        // [2] dab stamps by counter-set parity
   float *mx = nullptr, *my = nullptr, *mz = nullptr;  // [V] normals the fused kernel wrote, copied in by the next search
   int *pend = nullptr;          // [4] the fused dab whose results wait for that copy: gathered count, stamp
-  int4 *gl_h = nullptr, *gl_r = nullptr;  // [L] the gathered leaves' halo headers (fused dabs)
+  int4 *gl_rec = nullptr;                 // [2L] the gathered leaves' records (fused dabs)
   int halo_maxnbl = 0, leaf_maxsh = 0;    // most neighbour entries / shared vertices of a leaf
+  int halo_maxlocr[SG_HALO_RINGS + 1] = {};  // most locals of a leaf up to ring j (uniq + rings 0..j)
   long long halo_bytes = 0;               // device bytes of the halo tables
 };
 
@@ -245,7 +250,8 @@ struct PBVH {
   int grid_wg = 256;                 // workgroups of the per-dab kernels (tunable)
   int grid_brush = 0, grid_post = 0, grid_smooth = 0;  // per-kernel overrides (autotuner)
   int grid_fused = 1024;             // workgroups of the fused dab kernel (a leaf each)
-  bool fused_enabled = true;         // plain dabs run fused when the mesh has halo tables
+  bool fused_enabled = true;
+  bool fused_pending = false;  // the last dab enqueued was fused and no finish followed yet         // plain dabs run fused when the mesh has halo tables
   // device snapshot (checkpoint) of the mutable mesh state
   // slot 0: the caller's (sg_pbvh_snapshot_save/restore); slot 1: internal (autotuner, judge)
   std::vector<void *> snap[2];
@@ -285,7 +291,6 @@ struct SgLayout {
   std::vector<int2> vc;
 };
 
-#define SG_HALO_RINGS 4  // Smooth(k) runs fused for k <= this
 
 // host build (pbvh_build.cpp)
 void sg_build_tree(PBVH *p, const float *co, const int *tris, int T, int V);

@@ -79,12 +79,12 @@ struct SgDevView {
   int *stamp_v, *stamp_v2, *stamp_t, *stamp_a;
   // fused dabs: per-leaf halos, the gathered leaves' halo headers, gather stamps, dab
   // stamps, the normals the fused kernel wrote and its pending record
-  const int4 *hl_hdr, *hl_ring;
+  const int4 *hl_rec;
   const int2 *hl_v;
   const int *hl_nboff;
   const unsigned short *hl_nbl, *hl_sh;
   const uint2 *hl_tri;
-  int4 *gl_h, *gl_r;
+  int4 *gl_rec;
   int *leaf_gstamp, *dstamp, *pend;
   float *mx, *my, *mz;
 };
