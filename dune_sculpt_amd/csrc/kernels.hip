@@ -853,15 +853,20 @@ __global__ __launch_bounds__(256) void k_post(SgDevView d, int stats, int stamp,
 //      incidence corners (first 8), into registers
 // after which the brush, the Jacobi steps, the normals and the box run from LDS and
 // registers.  Rows longer than 8 entries read the rest from memory where they are used.
+#define FUSED_FPT 2              // faces per face thread whose corners load up front
+#define FUSED_FACE_THREADS 192   // the first three waves compute the face normals, the last the leaf box
+
 template <int LPT>
 __global__ __launch_bounds__(FUSED_THREADS) void k_dab_fused(SgDevView d, SgDabK k, const float *curve, int serial,
-                                                            int cap) {
+                                                            int cap, int fcap) {
   if (d.halt && d.halt[0]) return;
   extern __shared__ __align__(16) unsigned char fused_smem[];
   __shared__ int s_cnt;
-  // positions of the locals, two buffers (x, y, z runs of `cap` floats each)
-  float *const S = (float *)fused_smem;
-  unsigned short *Sshl = (unsigned short *)((float *)fused_smem + 6 * cap);  // shared vertices' local ids
+  // positions of the locals, two buffers of `cap` (x, y, z, 0), the normals of the
+  // leaf's faces (`fcap` of them), the shared vertices' local ids
+  float4 *const S = (float4 *)fused_smem;
+  float4 *const Sfn = S + 2 * cap;
+  unsigned short *Sshl = (unsigned short *)(Sfn + fcap);
   const int tid = threadIdx.x;
   // level 1: counters, stamp and the first gathered leaf's records together.  Loads
   // below are unconditional from clamped (always valid) indices, with the selection
@@ -898,9 +903,10 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_dab_fused(SgDevView d, SgDabK
     const int nu = rb.x & 0xffff, ns = rb.x >> 16;
     const int2 vr = make_int2(ra.x, ra.x + nu), sr = make_int2(ra.y, ra.y + ns);
     const int ext_off = ra.z, nb_blk = ra.w;
+    const int nlt = rb.w >> 16;  // faces incident to the uniq vertices
     // ext count at the end of ring j
     auto ring_end = [&](int j) {
-      return j <= 0 ? rb.y & 0xffff : j == 1 ? rb.y >> 16 : j == 2 ? rb.z & 0xffff : j == 3 ? rb.z >> 16 : rb.w;
+      return j <= 0 ? rb.y & 0xffff : j == 1 ? rb.y >> 16 : j == 2 ? rb.z & 0xffff : j == 3 ? rb.z >> 16 : rb.w & 0xffff;
     };
     const int nloc = nu + ring_end(K);
     const int ninner = K > 0 ? nu + ring_end(K - 1) : 0;  // locals the first Jacobi step updates
@@ -931,6 +937,15 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_dab_fused(SgDevView d, SgDabK
     const int qb0r = d.vt_off[v0], qe0r = d.vt_off[v0 + 1];
     const int shv0 = d.hl_sh[tid < ns ? sr.x + tid : 0];
     const int qb0 = uq0 ? qb0r : 0, qe0 = uq0 ? qe0r : 0;
+    // the leaf's faces: their corners follow the leaf's neighbour offsets
+    const int lt_base = nb_blk + nu + ring_end(SG_HALO_RINGS - 1) + 1;
+    unsigned int fc0[FUSED_FPT], fc1[FUSED_FPT];
+#pragma unroll
+    for (int f = 0; f < FUSED_FPT; f++) {
+      const int j = tid < FUSED_FACE_THREADS ? tid + f * FUSED_FACE_THREADS : nlt;
+      fc0[f] = (unsigned)d.hl_nboff[j < nlt ? lt_base + 2 * j : 0];
+      fc1[f] = (unsigned)d.hl_nboff[j < nlt ? lt_base + 2 * j + 1 : 0];
+    }
     // level 3
     const float *mp = d.mask ? d.mask : d.px;  // stand-ins keep the loads unconditional
     const int *hp = hide ? d.vhide : (const int *)d.px;
@@ -955,9 +970,9 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_dab_fused(SgDevView d, SgDabK
 #pragma unroll
       for (int q = 0; q < FUSED_ROW; q++) nb[j][q] = d.hl_nbl[nbb[j] + q < nbe[j] ? nbb[j] + q : 0];
     }
-    uint2 tri0[FUSED_ROW];
+    unsigned short ti0[FUSED_ROW];  // the first incidences' local faces
 #pragma unroll
-    for (int q = 0; q < FUSED_ROW; q++) tri0[q] = d.hl_tri[qb0 + q < qe0 ? qb0 + q : 0];
+    for (int q = 0; q < FUSED_ROW; q++) ti0[q] = d.hl_ti[qb0 + q < qe0 ? qb0 + q : 0];
     // brush (spec S2-S9) on every local
     float fade[LPT];
     bool aff[LPT];
@@ -990,21 +1005,20 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_dab_fused(SgDevView d, SgDabK
           fade[j] = k.bstr * f;
         }
       }
-      if (i < nloc) {
-        S[i] = x[j];
-        S[cap + i] = y[j];
-        S[2 * cap + i] = z[j];
-      }
+      if (i < nloc) S[i] = make_float4(x[j], y[j], z[j], 0.0f);
     }
     if (tid < ns) Sshl[tid] = (unsigned short)shv0;
     for (int i = tid + FUSED_THREADS; i < ns; i += FUSED_THREADS) Sshl[i] = d.hl_sh[sr.x + i];
     __syncthreads();
     // Smooth: Jacobi step t updates the locals of rings <= K - t from step t - 1
+    float inv[LPT];  // 1 / neighbour count, the same value each step
+#pragma unroll
+    for (int j = 0; j < LPT; j++) inv[j] = K > 0 && nbe[j] > nbb[j] ? 1.0f / (float)(nbe[j] - nbb[j]) : 0.0f;
     int cur = 0;
     for (int t = 1; t <= K; t++) {
       const int nact = nu + ring_end(K - t);
-      const float *X = S + cur * 3 * cap;
-      float *W = S + (cur ^ 1) * 3 * cap;
+      const float4 *X = S + cur * cap;
+      float4 *W = S + (cur ^ 1) * cap;
 #pragma unroll
       for (int j = 0; j < LPT; j++) {
         const int i = tid + j * FUSED_THREADS;
@@ -1015,29 +1029,26 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_dab_fused(SgDevView d, SgDabK
 #pragma unroll
             for (int q = 0; q < FUSED_ROW; q++) {
               if (q < cnt) {
-                const int u = nb[j][q];
-                sx = sx + X[u];
-                sy = sy + X[cap + u];
-                sz = sz + X[2 * cap + u];
+                const float4 pu = X[nb[j][q]];
+                sx = sx + pu.x;
+                sy = sy + pu.y;
+                sz = sz + pu.z;
               }
             }
             for (int q = FUSED_ROW; q < cnt; q++) {
-              const int u = d.hl_nbl[nbb[j] + q];
-              sx = sx + X[u];
-              sy = sy + X[cap + u];
-              sz = sz + X[2 * cap + u];
+              const float4 pu = X[d.hl_nbl[nbb[j] + q]];
+              sx = sx + pu.x;
+              sy = sy + pu.y;
+              sz = sz + pu.z;
             }
             if (cnt > 0) {
-              const float inv = 1.0f / (float)cnt;
-              const float avx = sx * inv, avy = sy * inv, avz = sz * inv;
+              const float avx = sx * inv[j], avy = sy * inv[j], avz = sz * inv[j];
               x[j] = x[j] + (avx - x[j]) * fade[j];
               y[j] = y[j] + (avy - y[j]) * fade[j];
               z[j] = z[j] + (avz - z[j]) * fade[j];
             }
           }
-          W[i] = x[j];
-          W[cap + i] = y[j];
-          W[2 * cap + i] = z[j];
+          W[i] = make_float4(x[j], y[j], z[j], 0.0f);
         }
       }
       __syncthreads();
@@ -1045,31 +1056,66 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_dab_fused(SgDevView d, SgDabK
     }
     // the affected uniq vertices: normal (spec S10, incident tris in vertex->tri order),
     // position, list record
-    const float *F = S + cur * 3 * cap;  // final positions of the locals
+    const float4 *F = S + cur * cap;  // final positions of the locals
+    if (tid < FUSED_FACE_THREADS) {
+      // each face normal once (spec S10's face normal; faces in the tables' order)
+      auto face_normal = [&](unsigned c0, unsigned c1, int j) {
+        const float4 a = F[c0 & 0xffff], b = F[c0 >> 16], e = F[c1];
+        float fx, fy, fz;
+        sg_tri_normal(a.x, a.y, a.z, b.x, b.y, b.z, e.x, e.y, e.z, fx, fy, fz);
+        Sfn[j] = make_float4(fx, fy, fz, 0.0f);
+      };
+#pragma unroll
+      for (int f = 0; f < FUSED_FPT; f++) {
+        const int j = tid + f * FUSED_FACE_THREADS;
+        if (j < nlt) face_normal(fc0[f], fc1[f], j);
+      }
+      for (int j = tid + FUSED_FPT * FUSED_FACE_THREADS; j < nlt; j += FUSED_FACE_THREADS)
+        face_normal((unsigned)d.hl_nboff[lt_base + 2 * j], (unsigned)d.hl_nboff[lt_base + 2 * j + 1], j);
+    } else {
+      // meanwhile the last wave folds the leaf's box over its uniq then shared vertices
+      // (contiguous runs per lane, lanes in order, as leaf_box does)
+      const int lane = tid - FUSED_FACE_THREADS;
+      const int tot = nu + ns, per = (tot + SG_WAVE - 1) / SG_WAVE;
+      BB6 acc;
+      bb_reset(acc);
+      for (int c = 0; c < per; c++) {
+        const int i = lane * per + c;
+        if (i < tot) {
+          const float4 pu = F[i < nu ? i : Sshl[i - nu]];
+          bb_add_point(acc, pu.x, pu.y, pu.z);
+        }
+      }
+      acc = bb_wave_scan(acc);
+      if (lane == SG_WAVE - 1) {
+        bb_store(d.leaf_bb + 8 * (size_t)leaf, acc);
+        bb_store(d.leaf_obb + 8 * (size_t)leaf, acc);
+      }
+    }
+    __syncthreads();
+    // the affected uniq vertices: normal (the incident faces' normals summed in
+    // vertex->tri order), position, list record
 #pragma unroll
     for (int j = 0; j < LPT; j++) {
       const int i = tid + j * FUSED_THREADS;
       if (i >= nu || !aff[j]) continue;
       const int v = vr.x + i;
       float ax = 0.0f, ay = 0.0f, az = 0.0f;
-      auto add_tri = [&](uint2 c) {
-        const int a = c.x & 0xffff, b = c.x >> 16, e = c.y;
-        float fx, fy, fz;
-        sg_tri_normal(F[a], F[cap + a], F[2 * cap + a], F[b], F[cap + b], F[2 * cap + b], F[e], F[cap + e],
-                      F[2 * cap + e], fx, fy, fz);
-        ax = ax + fx;
-        ay = ay + fy;
-        az = az + fz;
+      auto add_face = [&](int f) {
+        const float4 n = Sfn[f];
+        ax = ax + n.x;
+        ay = ay + n.y;
+        az = az + n.z;
       };
       if (j == 0) {
         const int cnt = qe0 - qb0;
 #pragma unroll
         for (int q = 0; q < FUSED_ROW; q++)
-          if (q < cnt) add_tri(tri0[q]);
-        for (int q = qb0 + FUSED_ROW; q < qe0; q++) add_tri(d.hl_tri[q]);
+          if (q < cnt) add_face(ti0[q]);
+        for (int q = qb0 + FUSED_ROW; q < qe0; q++) add_face(d.hl_ti[q]);
       } else {  // leaves of more than FUSED_THREADS uniq vertices
         const int qb = d.vt_off[v], qe = d.vt_off[v + 1];
-        for (int q = qb; q < qe; q++) add_tri(d.hl_tri[q]);
+        for (int q = qb; q < qe; q++) add_face(d.hl_ti[q]);
       }
       sg_normalize(ax, ay, az);
       d.mx[v] = ax;
@@ -1080,24 +1126,6 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_dab_fused(SgDevView d, SgDabK
       d.qz[v] = z[j];
       const int pos = atomicAdd(&s_cnt, 1);
       d.aff_v[(g << d.aff_shift) + pos] = v;
-    }
-    // the leaf's box over its uniq then shared vertices, folded as leaf_box does
-    if (tid < SG_WAVE) {
-      const int tot = nu + ns, per = (tot + SG_WAVE - 1) / SG_WAVE;
-      BB6 acc;
-      bb_reset(acc);
-      for (int c = 0; c < per; c++) {
-        const int i = tid * per + c;
-        if (i < tot) {
-          const int u = i < nu ? i : Sshl[i - nu];
-          bb_add_point(acc, F[u], F[cap + u], F[2 * cap + u]);
-        }
-      }
-      acc = bb_wave_reduce(acc);
-      if (tid == 0) {
-        bb_store(d.leaf_bb + 8 * (size_t)leaf, acc);
-        bb_store(d.leaf_obb + 8 * (size_t)leaf, acc);
-      }
     }
     __syncthreads();
     if (tid == 0) d.aff_cnt[g] = s_cnt;
@@ -1559,7 +1587,7 @@ SgDevView sg_view_of(SgDev &d, bool respect_hide, bool halt_live) {
   v.hl_nboff = d.hl_nboff;
   v.hl_nbl = d.hl_nbl;
   v.hl_sh = d.hl_sh;
-  v.hl_tri = d.hl_tri;
+  v.hl_ti = d.hl_ti;
   v.gl_rec = d.gl_rec;
   v.leaf_gstamp = d.leaf_gstamp;
   v.dstamp = d.dstamp;
@@ -1700,13 +1728,15 @@ int sg_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long long *
   kk.fused = 0;
   const bool fused_tool = k.tool == SG_TOOL_DRAW || k.tool == SG_TOOL_INFLATE || k.tool == SG_TOOL_GRAB ||
                           (k.tool == SG_TOOL_SMOOTH && k.iters >= 1 && k.iters <= SG_HALO_RINGS);
-  int fused_cap = 0, lpt = 0;
+  int fused_cap = 0, fused_fcap = 1, lpt = 0;
   size_t lds = 0;
   if (p->d.halo_ok && p->fused_enabled && fused_tool && !k.stats && !k.orig_co && !(k.learned && p->mlp) && !col &&
       !place) {
     fused_cap = p->d.halo_maxlocr[k.tool == SG_TOOL_SMOOTH ? k.iters : 0];
     lpt = (fused_cap + FUSED_THREADS - 1) / FUSED_THREADS;
-    lds = (size_t)fused_cap * 6 * sizeof(float) + (size_t)(p->d.leaf_maxsh + 2) * 2 + 16;
+    fused_fcap = std::max(1, p->d.halo_maxlt);
+    lds = (size_t)fused_cap * 2 * sizeof(float4) + (size_t)fused_fcap * sizeof(float4) +
+          (size_t)(p->d.leaf_maxsh + 2) * 2 + 16;
     // larger halos (very large leaves) take the separate kernels
     kk.fused = lpt >= 1 && lpt <= 4 && lds <= 65536 ? 1 : 0;
   }
@@ -1727,13 +1757,13 @@ int sg_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long long *
     ProfScope ps(p, SG_K_BRUSH);
     const dim3 gf(p->grid_fused), bf(FUSED_THREADS);
     if (lpt == 1)
-      SG_LAUNCH(ps, k_dab_fused<1>, gf, bf, lds, s, v, kk, curve_dev, serial, fused_cap);
+      SG_LAUNCH(ps, k_dab_fused<1>, gf, bf, lds, s, v, kk, curve_dev, serial, fused_cap, fused_fcap);
     else if (lpt == 2)
-      SG_LAUNCH(ps, k_dab_fused<2>, gf, bf, lds, s, v, kk, curve_dev, serial, fused_cap);
+      SG_LAUNCH(ps, k_dab_fused<2>, gf, bf, lds, s, v, kk, curve_dev, serial, fused_cap, fused_fcap);
     else if (lpt == 3)
-      SG_LAUNCH(ps, k_dab_fused<3>, gf, bf, lds, s, v, kk, curve_dev, serial, fused_cap);
+      SG_LAUNCH(ps, k_dab_fused<3>, gf, bf, lds, s, v, kk, curve_dev, serial, fused_cap, fused_fcap);
     else
-      SG_LAUNCH(ps, k_dab_fused<4>, gf, bf, lds, s, v, kk, curve_dev, serial, fused_cap);
+      SG_LAUNCH(ps, k_dab_fused<4>, gf, bf, lds, s, v, kk, curve_dev, serial, fused_cap, fused_fcap);
     return sg_check_hip(hipGetLastError(), "dab launch");
   }
   if (k.tool == SG_TOOL_ELASTIC_DEFORM) {  // every leaf; then all normals, leaf boxes and the tree

@@ -20,6 +20,7 @@
 #include <cstring>
 #include <atomic>
 #include <thread>
+#include <unordered_map>
 
 #include "sg_internal.h"
 #include "sg_kernels.h"
@@ -568,13 +569,14 @@ static bool build_halo(PBVH *p, const SgLayout &h, hipStream_t s) {
     int ring_end[SG_HALO_RINGS + 1];  // ext count at the end of ring j
     std::vector<int> nboff;           // relative to the leaf's neighbour block
     std::vector<unsigned short> nbl;
+    std::vector<uint2> lt;            // faces incident to the uniq vertices: corners as local ids
     bool ok = true;
   };
   std::vector<Leaf> lv(L);
   auto owner_of = [&](int v) {
     return (int)(std::upper_bound(h.leaf_vbeg.begin(), h.leaf_vbeg.end(), v) - h.leaf_vbeg.begin()) - 1;
   };
-  std::vector<uint2> tri(h.vt4.size());
+  std::vector<unsigned short> ti(h.vt4.size());  // per incidence: local face index in its vertex's leaf
   std::vector<unsigned short> shl(h.leaf_sh.size());
   std::atomic<int> bad{0};
   const int nt = 16;
@@ -599,6 +601,7 @@ static bool build_halo(PBVH *p, const SgLayout &h, hipStream_t s) {
         used.push_back((int)i);
       };
       std::vector<int> ring, next, all;
+      std::unordered_map<int, int> face;  // face id -> local face index (per leaf)
       for (int l = t; l < L; l += nt) {
         Leaf &o = lv[l];
         const int vb = h.leaf_vbeg[l], ve = h.leaf_vbeg[l + 1], nu = ve - vb;
@@ -665,14 +668,29 @@ static bool build_halo(PBVH *p, const SgLayout &h, hipStream_t s) {
           }
           o.nboff[i + 1] = (int)o.nbl.size();
         }
-        // corners of the uniq vertices' incidences, shared vertices
-        for (int v = vb; v < ve; v++)
-          for (int q = h.vt_off[v]; q < h.vt_off[v + 1]; q++) {
-            const int4 c = h.vt4[q];
-            const int a = find(c.x), b = find(c.y), e = find(c.z);
-            miss |= a < 0 || b < 0 || e < 0 || a >= nl0 || b >= nl0 || e >= nl0;
-            tri[q] = make_uint2((unsigned)a | ((unsigned)b << 16), (unsigned)e);
-          }
+        // faces incident to the uniq vertices, numbered in order of first incidence
+        // (vertices in order, each in vertex->tri order); corners as local ids
+        {
+          face.clear();
+          for (int v = vb; v < ve; v++)
+            for (int q = h.vt_off[v]; q < h.vt_off[v + 1]; q++) {
+              const int4 c = h.vt4[q];
+              const int t = c.w & 0x3fffffff;
+              auto it = face.find(t);
+              int j;
+              if (it == face.end()) {
+                j = (int)o.lt.size();
+                face.emplace(t, j);
+                const int a = find(c.x), b = find(c.y), e = find(c.z);
+                miss |= a < 0 || b < 0 || e < 0 || a >= nl0 || b >= nl0 || e >= nl0;
+                o.lt.push_back(make_uint2((unsigned)a | ((unsigned)b << 16), (unsigned)e));
+              } else {
+                j = it->second;
+              }
+              miss |= j > 65535;
+              ti[q] = (unsigned short)j;
+            }
+        }
         for (int k = h.leaf_sh_off[l]; k < h.leaf_sh_off[l + 1]; k++) {
           const int w = find(h.leaf_sh[k]);
           miss |= w < 0 || w >= nl0;
@@ -692,19 +710,21 @@ static bool build_halo(PBVH *p, const SgLayout &h, hipStream_t s) {
   // concatenate
   std::vector<int4> rec(2 * (size_t)std::max(L, 1));
   size_t next_ext = 0, next_nb = 0, next_nbl = 0;
-  int maxloc = 0, maxloc0 = 0, maxnbl = 0, maxsh = 0;
+  int maxloc = 0, maxloc0 = 0, maxnbl = 0, maxsh = 0, maxlt = 0;
   for (int l = 0; l < L; l++) {
     const Leaf &o = lv[l];
     const int nu = h.leaf_vbeg[l + 1] - h.leaf_vbeg[l];
     maxnbl = std::max(maxnbl, (int)o.nbl.size());
+    maxlt = std::max(maxlt, (int)o.lt.size());
     maxsh = std::max(maxsh, h.leaf_sh_off[l + 1] - h.leaf_sh_off[l]);
     const int ns = h.leaf_sh_off[l + 1] - h.leaf_sh_off[l];
-    if (nu > 65535 || ns > 65535 || o.ring_end[R] > 65535) return false;  // the record's 16-bit fields
+    if (nu > 65535 || ns > 65535 || o.ring_end[R] > 65535 || o.lt.size() > 65535)
+      return false;  // the record's 16-bit fields
     rec[2 * l] = make_int4(h.leaf_vbeg[l], h.leaf_sh_off[l], (int)next_ext, (int)next_nb);
     rec[2 * l + 1] = make_int4(nu | (ns << 16), o.ring_end[0] | (o.ring_end[1] << 16),
-                               o.ring_end[2] | (o.ring_end[3] << 16), o.ring_end[4]);
+                               o.ring_end[2] | (o.ring_end[3] << 16), o.ring_end[4] | ((int)o.lt.size() << 16));
     next_ext += o.ext.size();
-    next_nb += o.nboff.size();
+    next_nb += o.nboff.size() + 2 * o.lt.size();
     next_nbl += o.nbl.size();
     maxloc = std::max(maxloc, nu + o.ring_end[R]);
     maxloc0 = std::max(maxloc0, nu + o.ring_end[0]);
@@ -720,9 +740,13 @@ static bool build_halo(PBVH *p, const SgLayout &h, hipStream_t s) {
       const Leaf &o = lv[l];
       std::copy(o.ext.begin(), o.ext.end(), ext.begin() + e);
       for (size_t i = 0; i < o.nboff.size(); i++) nboff[b + i] = (int)n + o.nboff[i];
+      for (size_t i = 0; i < o.lt.size(); i++) {
+        nboff[b + o.nboff.size() + 2 * i] = (int)o.lt[i].x;
+        nboff[b + o.nboff.size() + 2 * i + 1] = (int)o.lt[i].y;
+      }
       std::copy(o.nbl.begin(), o.nbl.end(), nbl.begin() + n);
       e += o.ext.size();
-      b += o.nboff.size();
+      b += o.nboff.size() + 2 * o.lt.size();
       n += o.nbl.size();
     }
   }
@@ -733,7 +757,7 @@ static bool build_halo(PBVH *p, const SgLayout &h, hipStream_t s) {
   rc |= upload(&d.hl_v, ext.data(), ext.size(), s);
   rc |= upload(&d.hl_nboff, nboff.data(), nboff.size(), s);
   rc |= upload(&d.hl_nbl, nbl.data(), nbl.size(), s);
-  rc |= upload(&d.hl_tri, tri.data(), std::max<size_t>(tri.size(), 1), s);
+  rc |= upload(&d.hl_ti, ti.data(), std::max<size_t>(ti.size(), 1), s);
   rc |= upload(&d.hl_sh, shl.data(), std::max<size_t>(shl.size(), 1), s);
   rc |= alloc_fill(&d.leaf_gstamp, std::max(L, 1), 0xFF, s);  // -1: no dab's stamp (stamps start at 1)
   rc |= alloc_fill(&d.dstamp, 2, 0, s);
@@ -747,7 +771,8 @@ static bool build_halo(PBVH *p, const SgLayout &h, hipStream_t s) {
   d.halo_maxloc0 = maxloc0;
   d.halo_maxnbl = maxnbl;
   d.leaf_maxsh = maxsh;
-  d.halo_bytes = (long long)(rec.size() * 16 + ext.size() * 8 + nboff.size() * 4 + nbl.size() * 2 + tri.size() * 8 +
+  d.halo_maxlt = maxlt;
+  d.halo_bytes = (long long)(rec.size() * 16 + ext.size() * 8 + nboff.size() * 4 + nbl.size() * 2 + ti.size() * 2 +
                              shl.size() * 2);
   return rc == 0;
 }
@@ -973,7 +998,7 @@ void sg_free_device(PBVH *p) {
                   d.draw_poly_v, d.draw_vbo, d.undo, d.undo_stamp, d.undo_slot, d.place_pair,
                   d.prox_dev, d.prox_tab, d.prox_leaves, d.prox_off, d.poly_off, d.poly_v,
                   d.poly_of_tri, d.vc_off, d.vc, d.stage, d.hl_rec, d.hl_v, d.hl_nboff,
-                  d.hl_nbl, d.hl_tri, d.hl_sh, d.leaf_gstamp, d.dstamp, d.mx, d.my, d.mz, d.pend,
+                  d.hl_nbl, d.hl_ti, d.hl_sh, d.leaf_gstamp, d.dstamp, d.mx, d.my, d.mz, d.pend,
                   d.gl_rec};
   for (void *q : ptrs) {
     if (q) hipFree(q);

@@ -149,6 +149,36 @@ __device__ __forceinline__ void bb_store(float *p, const BB6 &b) {
   *(float4 *)(p + 4) = make_float4(b.mx[0], b.mx[1], b.mx[2], 0.0f);
 }
 // wave-ordered reduction: lane i's box covers elements before lane i+1's.
+// The same ordered fold by an inclusive scan over DPP lane moves (VALU, no LDS round
+// trips): each lane folds in the partial box of the lanes before it, row by row
+// (row_shr 1, 2, 4, 8), then across rows (row_bcast 15, 31); lane 63 holds the fold of
+// all 64 lanes in lane order.  Lanes whose source lies outside their row keep their
+// own value (DPP returns `old`), and folding a box with itself changes nothing.
+// Whole wave active.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_from(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, ROWS, 0xf, false));
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void bb_scan_step(BB6 &b) {
+  BB6 e;
+  for (int i = 0; i < 3; i++) {
+    e.mn[i] = dpp_from<CTRL, ROWS>(b.mn[i]);
+    e.mx[i] = dpp_from<CTRL, ROWS>(b.mx[i]);
+  }
+  bb_add_bb(e, b);  // earlier lanes first
+  b = e;
+}
+__device__ __forceinline__ BB6 bb_wave_scan(BB6 b) {
+  bb_scan_step<0x111, 0xf>(b);  // row_shr:1
+  bb_scan_step<0x112, 0xf>(b);  // row_shr:2
+  bb_scan_step<0x114, 0xf>(b);  // row_shr:4
+  bb_scan_step<0x118, 0xf>(b);  // row_shr:8
+  bb_scan_step<0x142, 0xa>(b);  // row_bcast:15 into rows 1 and 3
+  bb_scan_step<0x143, 0xc>(b);  // row_bcast:31 into rows 2 and 3
+  return b;                     // lane 63 holds the ordered fold
+}
+
 __device__ __forceinline__ BB6 bb_wave_reduce(BB6 b) {
   for (int off = 1; off < SG_WAVE; off <<= 1) {
     BB6 o;

@@ -172,7 +172,9 @@ struct SgDev {
   int2 *hl_v = nullptr;         // [ext] vertex id, owner leaf (the leaf holding it as uniq)
   int *hl_nboff = nullptr;      // per leaf: offsets into hl_nbl of its locals before the last ring (+1)
   unsigned short *hl_nbl = nullptr;  // local ids of those locals' edge neighbours, vertex->vertex order
-  uint2 *hl_tri = nullptr;      // [3T] parallel to vt4: the incidence's corners as local ids (c0 | c1 << 16, c2)
+  unsigned short *hl_ti = nullptr;   // [3T] parallel to vt4: the incidence's face as a local face index of
+                                     // its vertex's leaf; the leaf's faces (corners as local ids, c0 | c1 << 16,
+                                     // c2) follow its neighbour offsets in hl_nboff
   unsigned short *hl_sh = nullptr;   // parallel to leaf_sh: local id of each shared vertex
   int *leaf_gstamp = nullptr;   // [L] dab stamp of the last search that gathered the leaf
   int *dstamp = nullptr;        // [2] dab stamps by counter-set parity
@@ -180,6 +182,7 @@ struct SgDev {
   int *pend = nullptr;          // [4] the fused dab whose results wait for that copy: gathered count, stamp
   int4 *gl_rec = nullptr;                 // [2L] the gathered leaves' records (fused dabs)
   int halo_maxnbl = 0, leaf_maxsh = 0;    // most neighbour entries / shared vertices of a leaf
+  int halo_maxlt = 0;                     // most faces incident to a leaf's uniq vertices
   int halo_maxlocr[SG_HALO_RINGS + 1] = {};  // most locals of a leaf up to ring j (uniq + rings 0..j)
   long long halo_bytes = 0;               // device bytes of the halo tables
 };

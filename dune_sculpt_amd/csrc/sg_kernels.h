@@ -83,7 +83,7 @@ struct SgDevView {
   const int2 *hl_v;
   const int *hl_nboff;
   const unsigned short *hl_nbl, *hl_sh;
-  const uint2 *hl_tri;
+  const unsigned short *hl_ti;
   int4 *gl_rec;
   int *leaf_gstamp, *dstamp, *pend;
   float *mx, *my, *mz;
