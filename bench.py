@@ -42,6 +42,7 @@ HBM_PEAK_GBPS = 8000.0
 # profiles/dab_step_c2.md holds the commands and the per-kernel figures.
 TRAFFIC_BYTES_PER_DAB = 5_867_877
 METRIC = "vertex-updates/s + HBM GB/s; learned vs plain Smooth dabs/s at ≤1e-3 RMS"
+NO_PERSISTENT = False  # set from --no-persistent
 
 
 def log(msg):
@@ -163,6 +164,7 @@ def run_batch(sg, name, meshes, table, W, K, tune_steps, learned_model=None, pla
            "cuts": info["cuts"], "steps": len(table), "build_s": round(build_s, 2)}
     if learned_model is not None:
         B.set_learned(learned_model)
+    out["persistent"] = bool(B.set_persistent(not NO_PERSISTENT)) and not NO_PERSISTENT
     if tune_steps:
         t = time.perf_counter()
         before, after = B.autotune(table[:tune_steps])
@@ -640,6 +642,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the many-mesh workloads")
     ap.add_argument("--copy-mb", type=int, default=1024, help="size of the measured HBM copy (0: skip)")
     ap.add_argument("--only-batch", action="store_true", help="run only the many-mesh workloads")
+    ap.add_argument("--no-persistent", action="store_true",
+                    help="many-mesh workloads with one captured graph per step instead of the persistent step (A/B)")
     ap.add_argument("--batch-meshes", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=64)
     ap.add_argument("--no-c4", action="store_true", help="skip the 256-mesh batch")
@@ -660,6 +664,8 @@ def main():
     ap.add_argument("--placed-only", action="store_true",
                     help="placed stroke without the fixed-centre reference stroke (one captured graph)")
     args = ap.parse_args()
+    global NO_PERSISTENT
+    NO_PERSISTENT = args.no_persistent
 
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs one mesh stroke per GPU; only --gpus 1 is supported "

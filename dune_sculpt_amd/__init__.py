@@ -125,7 +125,7 @@ EXPORTS = [
     "sg_learned_judge", "sg_batch_new", "sg_batch_free", "sg_batch_info", "sg_batch_load", "sg_batch_launch",
     "sg_batch_rewind", "sg_batch_run", "sg_batch_sync", "sg_batch_totals", "sg_batch_read_positions",
     "sg_batch_read_normals", "sg_batch_snapshot_save", "sg_batch_snapshot_restore", "sg_batch_set_launch",
-    "sg_batch_get_launch", "sg_batch_autotune", "sg_batch_set_learned", "sg_batch_timer_start",
+    "sg_batch_get_launch", "sg_batch_autotune", "sg_batch_set_learned", "sg_batch_set_persistent", "sg_batch_timer_start",
     "sg_batch_timer_stop", "sg_hbm_copy_probe", "sg_raycast", "sg_raycast_device", "sg_raycast_stats", "sg_raycast_stats_noalloc",
     "sg_dab_apply_stroke_placed", "sg_stroke_prepare_placed", "sg_stroke_placements", "BKE_pbvh_raycast",
     "BKE_pbvh_find_nearest_to_ray", "BKE_pbvh_node_get_tmin", "BKE_pbvh_node_raycast",
@@ -246,6 +246,7 @@ def lib():
         "sg_batch_get_launch": (i32, [vp, i32p]),
         "sg_batch_autotune": (i32, [vp, C.POINTER(SgDabParams), i32, C.POINTER(C.c_double)]),
         "sg_batch_set_learned": (i32, [vp, vp]),
+        "sg_batch_set_persistent": (i32, [vp, i32]),
         "sg_batch_timer_start": (i32, [vp]),
         "sg_batch_timer_stop": (i32, [vp, f32p]),
         "sg_pbvh_timer_start": (i32, [vp]),
@@ -1253,6 +1254,11 @@ class Batch:
         _check(lib().sg_batch_autotune(self.h, arr, steps, out), "batch autotune")
         self._rows = 0  # the tuner loaded its own sample
         return float(out[0]), float(out[1])
+
+    def set_persistent(self, on=True):
+        """Persistent step (a workgroup per mesh, every row in one launch) on or off;
+        returns whether the batch fits it."""
+        return bool(_check(lib().sg_batch_set_persistent(self.h, 1 if on else 0), "set persistent"))
 
     def set_learned(self, mlp):
         self._mlp = mlp

@@ -35,7 +35,7 @@ COMMON = [
     "-I", str(PKG.parent / "include"),
 ]
 
-SOURCES = ["kernels.hip", "learned.hip", "batch.hip", "ray.hip", "draw.hip", "undo.hip", "proxy.hip", "api.cpp", "pbvh_build.cpp", "meshgen.cpp"]
+SOURCES = ["kernels.hip", "learned.hip", "batch.hip", "batch_mesh.hip", "ray.hip", "draw.hip", "undo.hip", "proxy.hip", "api.cpp", "pbvh_build.cpp", "meshgen.cpp"]
 
 
 def _run(cmds):

@@ -521,7 +521,8 @@ void batch_free_device(SgBatch *b) {
                   d.leaf_flags, d.leaf_bb, d.leaf_obb, d.cut_bb,    d.cut_obb, d.cut_flags, d.cut_lbeg,
                   d.cut_lend, b->pc_list, b->pc_sph, b->gl_lrn, b->done, d.gathered, d.gl_v,      d.gl_s,    d.vperm_old, b->leaf_vr,
                   b->leaf_shr, b->cut_mesh, b->gl_mesh, b->vstamp, b->vfade, b->ctr,    b->tot,
-                  b->step,    b->tab,     b->gather_buf};
+                  b->step,    b->tab,     b->gather_buf, b->mesh_cut, b->mesh_leaf, b->mesh_vert,
+                  b->pm_aff_v, b->pm_aff_f, b->pm_g_off, b->pm_err};
   for (void *q : ptrs)
     if (q) hipFree(q);
   for (void *q : b->snap)
@@ -809,6 +810,33 @@ SgBatch *sg_batch_new(int n_meshes, const float *const *co, const int *const *tr
   rc |= dev_fill(&b->ctr, SB_COUNT + SB_PM * (size_t)b->M, 0, s);
   rc |= dev_fill(&b->tot, SBT_COUNT, 0, s);
   rc |= dev_fill(&b->step, 2, 0, s);
+  // persistent step tables
+  rc |= dev_fill(&b->pm_aff_v, V, 0, s);
+  rc |= dev_fill(&b->pm_aff_f, V, 0, s);
+  rc |= dev_fill(&b->pm_g_off, L, 0, s);
+  rc |= dev_fill(&b->pm_err, 1, 0, s);
+  if (!rc) {
+    std::vector<int> mc(b->M + 1), ml(b->M + 1), mv(b->M + 1);
+    b->pm_ok = true;
+    // a workgroup per mesh: by default only when the meshes are enough to spread over
+    // the chip (the autotuner times both)
+    b->persistent = b->M >= 128;
+    for (int m = 0; m <= b->M; m++) {
+      mc[m] = (int)b->cbase[m];
+      ml[m] = (int)b->lbase[m];
+      mv[m] = (int)b->vbase[m];
+      if (m < b->M && b->cbase[m + 1] - b->cbase[m] > 1024) b->pm_ok = false;
+    }
+    rc |= dev_fill(&b->mesh_cut, b->M + 1, 0, s);
+    rc |= dev_fill(&b->mesh_leaf, b->M + 1, 0, s);
+    rc |= dev_fill(&b->mesh_vert, b->M + 1, 0, s);
+    if (!rc) {
+      hipMemcpyAsync(b->mesh_cut, mc.data(), mc.size() * 4, hipMemcpyHostToDevice, s);
+      hipMemcpyAsync(b->mesh_leaf, ml.data(), ml.size() * 4, hipMemcpyHostToDevice, s);
+      hipMemcpyAsync(b->mesh_vert, mv.data(), mv.size() * 4, hipMemcpyHostToDevice, s);
+      rc = sg_check_hip(hipStreamSynchronize(s), "batch tables");
+    }
+  }
   if (!rc) {
     const int st[2] = {1, 0};  // serial starts at 1: the fill value of vstamp (-1) is never a serial
     rc = sg_check_hip(hipStreamSynchronize(s), "batch build");
@@ -924,6 +952,13 @@ int sg_batch_launch(SgBatch *b, int steps, int flags) {
   if (b->row_next == 0) {
     static const int zero = 0;
     rc = sg_check_hip(hipMemcpyAsync(b->step + 1, &zero, 4, hipMemcpyHostToDevice, b->stream), "copy");
+  }
+  if (!rc && !(flags & 1) && b->persistent && b->pm_ok && !b->learned) {
+    // one launch for all the rows: a workgroup per mesh
+    rc = sg_batch_enqueue_persistent(b, b->row_next, steps);
+    b->row_next += steps;
+    if (!rc && !(flags & 2)) rc = sg_check_hip(hipStreamSynchronize(b->stream), "batch run");
+    return rc;
   }
   for (int s = 0; s < steps && !rc; s++) {
     if (flags & 1)
@@ -1046,6 +1081,12 @@ int sg_batch_timer_stop(SgBatch *b, float *ms) {
   return sg_check_hip(hipEventElapsedTime(ms, b->timer[0], b->timer[1]), "event elapsed");
 }
 
+int sg_batch_set_persistent(SgBatch *b, int on) {
+  if (batch_check(b)) return -1;
+  b->persistent = on != 0;
+  return b->pm_ok ? 1 : 0;
+}
+
 int sg_batch_set_learned(SgBatch *b, SgMlp *m) {
   if (batch_check(b)) return -1;
   b->mlp = m;
@@ -1080,9 +1121,14 @@ int sg_batch_autotune(SgBatch *b, const SgDabParams *dabs, int steps, double *ou
     }
     return best;
   };
-  double best = time_once();
+  const double start = time_once();
+  if (start < 0) return -1;
+  // the step graph's launch shape first (the persistent step has none), then the
+  // persistent step against the tuned graph
+  const bool pm_entry = b->persistent;
+  b->persistent = false;
+  double best = pm_entry ? time_once() : start;
   if (best < 0) return -1;
-  const double start = best;
   struct Knob {
     int *v;
     std::vector<int> cand;
@@ -1109,6 +1155,14 @@ int sg_batch_autotune(SgBatch *b, const SgDabParams *dabs, int steps, double *ou
       }
       *k.v = choice;
     }
+  }
+  if (b->pm_ok && !b->learned) {
+    b->persistent = true;
+    const double t = time_once();
+    if (t > 0 && t < best * 0.97)
+      best = t;
+    else
+      b->persistent = false;
   }
   int rc = sg_batch_snapshot_restore(b);
   // put the caller's checkpoint back and the totals as they were

@@ -256,9 +256,9 @@ __global__ __launch_bounds__(256) void k_search(SgDevView d, SgDabK k, long long
         }
       }
     }
-    tb = bb_wave_reduce(tb);
-    ob = bb_wave_reduce(ob);
-    if (lane == 0) {
+    tb = bb_wave_scan(tb);
+    ob = bb_wave_scan(ob);
+    if (lane == SG_WAVE - 1) {
       bb_store(d.cut_bb + 8 * (size_t)c, tb);
       bb_store(d.cut_obb + 8 * (size_t)c, ob);
     }
@@ -947,8 +947,12 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_dab_fused(SgDevView d, SgDabK
       fc1[f] = (unsigned)d.hl_nboff[j < nlt ? lt_base + 2 * j + 1 : 0];
     }
     // level 3
-    const float *mp = d.mask ? d.mask : d.px;  // stand-ins keep the loads unconditional
+    // stand-ins keep the loads unconditional without new traffic: a tool that needs no
+    // normals, or a mesh with no mask or hidden vertices, reads the position lines again
+    const bool inflate = k.tool == SG_TOOL_INFLATE;
+    const float *mp = d.mask ? d.mask : d.px;
     const int *hp = hide ? d.vhide : (const int *)d.px;
+    const float *nxp = inflate ? d.nx : d.px, *nyp = inflate ? d.ny : d.py, *nzp = inflate ? d.nz : d.pz;
     float x[LPT], y[LPT], z[LPT], nxv[LPT], nyv[LPT], nzv[LPT], mk[LPT];
     bool own[LPT], hid[LPT];
     unsigned short nb[LPT][FUSED_ROW];
@@ -958,9 +962,9 @@ __global__ __launch_bounds__(FUSED_THREADS) void k_dab_fused(SgDevView d, SgDabK
       x[j] = d.px[v];
       y[j] = d.py[v];
       z[j] = d.pz[v];
-      nxv[j] = d.nx[v];
-      nyv[j] = d.ny[v];
-      nzv[j] = d.nz[v];
+      nxv[j] = nxp[v];
+      nyv[j] = nyp[v];
+      nzv[j] = nzp[v];
       const float m = mp[v];
       const int hv = hp[v];
       const int gs = d.leaf_gstamp[owner[j] < 0 ? 0 : owner[j]];

@@ -102,6 +102,14 @@ struct SgBatch {
   hipEvent_t timer[2] = {nullptr, nullptr};
   float *gather_buf = nullptr;
   size_t gather_cap = 0;
+  // persistent step (one workgroup per mesh runs every row in one launch): per-mesh
+  // first cut / leaf / vertex, the running dab's affected list and gathered-leaf
+  // candidate offsets, an error flag
+  int *mesh_cut = nullptr, *mesh_leaf = nullptr, *mesh_vert = nullptr;
+  int *pm_aff_v = nullptr, *pm_g_off = nullptr, *pm_err = nullptr;
+  float *pm_aff_f = nullptr;
+  bool pm_ok = false;       // every mesh fits the persistent step (at most 1024 search cuts)
+  bool persistent = false;  // graph launches of tables without learned dabs use it (default: >= 128 meshes)
 };
 
 // A learned Smooth dab of mesh m is rejected (the plain brush redoes it in the same
@@ -118,5 +126,7 @@ SgBatchView sg_batch_view(SgBatch *b);
 // enqueue one step of the batch; maxk = largest Smooth iteration count in the run,
 // learned = the run has learned Smooth dabs (model set)
 int sg_batch_enqueue_step(SgBatch *b, int maxk, bool learned);
+// enqueue rows [row0, row0 + steps) as one persistent launch (batch_mesh.hip)
+int sg_batch_enqueue_persistent(SgBatch *b, int row0, int steps);
 int sg_batch_enqueue_learned_brush(SgBatch *b, const SgBatchView &v);
 int sg_batch_enqueue_decide(SgBatch *b, const SgBatchView &v);

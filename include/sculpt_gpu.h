@@ -522,6 +522,13 @@ int sg_batch_get_launch(SgBatch *batch, int g[7]);
 /* Autotuner over the launch shape on a sample of steps (restored afterwards). */
 int sg_batch_autotune(SgBatch *batch, const SgDabParams *dabs, int steps, double out_ms[2]);
 int sg_batch_set_learned(SgBatch *batch, SgMlp *mlp);
+/* Persistent step on or off (default: on for batches of at least 128 meshes; the
+ * autotuner times both): graph launches (sg_batch_launch flags bit 0 clear) of a table
+ * without learned Smooth dabs run every row in one kernel, a workgroup per mesh
+ * advancing its own dabs with workgroup barriers only; otherwise one captured step
+ * graph per row.  Both give the same bits.  Returns whether the batch fits the
+ * persistent step (every mesh at most 1024 search cuts). */
+int sg_batch_set_persistent(SgBatch *batch, int on);
 int sg_batch_timer_start(SgBatch *batch);
 int sg_batch_timer_stop(SgBatch *batch, float *ms);
 

@@ -132,3 +132,56 @@ def test_batch_learned_accept_and_reject(sg):
         _check(b, ref, meshes)
         b.close()
     m.close()
+
+
+@pytest.mark.parametrize("leaf_limit", [128, 16])
+def test_persistent_step_matches_single_mesh_engine(sg, leaf_limit):
+    """The persistent step (a workgroup per mesh, every row in one launch): each mesh
+    bit-identical to the single-mesh engine and to the step graph, with rows where a mesh
+    has no dab, Smooth k = 1..4, every tool, totals equal to the step graph's; small leaves
+    give many gathered leaves and search cuts per dab."""
+    meshes = _meshes(sg)
+    table = _table(sg, meshes, 16, seed=11)
+    ref = _single(sg, meshes, table)
+    b = sg.Batch(meshes, leaf_limit=leaf_limit)
+    assert b.set_persistent(True), "the batch should fit the persistent step"
+    b.snapshot()
+    b.run(table)
+    _check(b, ref, meshes)
+    tot_p = b.totals(reset=True)
+    b.restore()
+    b.set_persistent(False)
+    b.run(table)
+    _check(b, ref, meshes)
+    tot_g = b.totals()
+    print("persistent", tot_p, "graph", tot_g)
+    assert tot_p == tot_g
+    b.close()
+
+
+def test_persistent_step_split_launches_and_large_dabs(sg):
+    """Rows run in pieces (the launch takes its first row from the host); dabs of radius
+    up to 0.6 on a 40,962-vertex icosphere, so some affected lists exceed what the
+    workgroup holds in registers (2048 vertices) and take the round-by-round steps."""
+    meshes = [sg.noisy_icosphere(6, seed=31, amplitude=0.02), sg.noisy_torus(200, 60, seed=32, amplitude=0.01)]
+    rng = np.random.default_rng(12)
+    table = []
+    for s in range(9):
+        row = []
+        for m, (co, _) in enumerate(meshes):
+            row.append(sg.make_dab(tool=[1, 2, 4, 5][(s + m) % 4], location=co[rng.integers(len(co))],
+                                   radius=float(rng.uniform(0.3, 0.6)), strength=0.7,
+                                   smooth_iterations=int(1 + s % 4), grab_delta=rng.normal(size=3) * 0.02))
+        table.append(row)
+    ref = _single(sg, meshes, table)
+    b = sg.Batch(meshes, leaf_limit=128)
+    b.set_persistent(True)
+    b.load(table)
+    b.launch(2)
+    b.launch(3)
+    b.launch(4)
+    _check(b, ref, meshes)
+    tot = b.totals()
+    print(tot)
+    assert tot["steps"] == 9 and tot["affected"] > 0
+    b.close()
