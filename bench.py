@@ -36,11 +36,11 @@ sys.path.insert(0, str(ROOT))
 from dune_sculpt_amd.workloads import batch_meshes, batch_table, plain_stroke, smooth_stroke  # noqa: E402
 
 HBM_PEAK_GBPS = 8000.0
-# HBM bytes per dab of the 1000-dab stroke on the 10.5M-vertex mesh, from two
-# rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) summed over the dab kernels, FETCH_SIZE
-# scaled by 2.00 and WRITE_SIZE by 1.00 (what they read for a 1 GiB device copy);
-# profiles/dab_step_c2.md holds the commands and the per-kernel figures.
-TRAFFIC_BYTES_PER_DAB = 5_867_877
+# HBM bytes per dab of the 1000-dab stroke on the 10.5M-vertex mesh (fused dabs), from
+# two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) summed over the dab kernels,
+# FETCH_SIZE scaled by 2.00 and WRITE_SIZE by 1.00 (what they read for a 1 GiB device
+# copy); profiles/dab_step_c2.md holds the commands and the per-kernel figures.
+TRAFFIC_BYTES_PER_DAB = 12_809_687
 METRIC = "vertex-updates/s + HBM GB/s; learned vs plain Smooth dabs/s at ≤1e-3 RMS"
 NO_PERSISTENT = False  # set from --no-persistent
 
@@ -595,6 +595,18 @@ def run_c1(sg, W, K):
     return out
 
 
+def cpu_model():
+    """The host CPU's model name, for the record beside the CPU baseline."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(sg, co, tris, dabs, budget_s, threads, before=None):
     """The oracle's dab loop on the same mesh; `before(oracle)` runs first on the fresh
     mesh (the ray cast sample shares the build)."""
@@ -804,7 +816,7 @@ def main():
     gbps = timed_bytes / elapsed / 1e9
 
     # Roofline of the dab step: the stroke graph is one dependent chain of small kernels
-    # (k_search -> [k_area] -> k_brush -> [k_smooth_iter x3] -> k_post per dab), so
+    # (k_search -> [k_area] -> k_dab_fused per dab), so
     # its unit is the dab: algorithmic bytes per dab over event time per dab.
     us_per_dab_ev = event_s / (K * args.dabs) * 1e6
     bytes_per_dab = stroke_bytes / args.dabs
@@ -856,7 +868,7 @@ def main():
                     draw_cpu.update(cpu_draw(o, tris, args.cpu_budget / 4))
         r = cpu_baseline(sg, co, tris, dabs, args.cpu_budget, threads, before)
         cpu = {"value": r["affected"] / r["seconds"], "unit": "vertex-updates/s", "cores": r["threads"],
-               "kind": "port",
+               "kind": "port", "host": cpu_model(),
                "sample": f"first {r['dabs']} dabs of the same stroke on the same icosphere-10 mesh "
                          f"({r['seconds']:.1f}s, C oracle -O2 -ffp-contract=off, OpenMP over gathered leaves)"}
         r1 = cpu_baseline(sg, co, tris, dabs, args.cpu_budget / 2, 1)
