@@ -35,7 +35,8 @@ COMMON = [
     "-I", str(PKG.parent / "include"),
 ]
 
-SOURCES = ["kernels.hip", "learned.hip", "batch.hip", "batch_mesh.hip", "ray.hip", "draw.hip", "undo.hip", "proxy.hip", "api.cpp", "pbvh_build.cpp", "meshgen.cpp"]
+SOURCES = ["kernels.hip", "learned.hip", "batch.hip", "batch_mesh.hip", "ray.hip", "draw.hip", "undo.hip", "proxy.hip",
+           "grids.hip", "api.cpp", "pbvh_build.cpp", "meshgen.cpp", "subdiv_ccg.cpp", "grids_api.cpp"]
 
 
 def _run(cmds):

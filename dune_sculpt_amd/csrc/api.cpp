@@ -9,6 +9,7 @@
 #include <cstring>
 
 #include "sg_internal.h"
+#include "sg_grids.h"
 #include "sg_kernels.h"
 #include "sg_learned.h"
 
@@ -59,6 +60,7 @@ void invalidate_host_views(PBVH *p) {
 
 int sync_bbs(PBVH *p) {
   if (!p->host_bb_stale || p->nodes.empty() || p->no_device) return 0;
+  if (p->type == PBVH_GRIDS) return p->gd ? sg_grids_sync_bbs(p) : 0;
   if (p->nodes_dirty_deferred) {
     if (sg_enqueue_flush(p)) return -1;
     p->nodes_dirty_deferred = false;
@@ -214,6 +216,7 @@ void BKE_pbvh_free(PBVH *p) {
   for (SgProxChunk &c : p->prox_chunks) (void)hipHostFree(c.host);
   p->prox_chunks.clear();
   if (!p->no_device) sg_free_device(p);
+  if (!p->no_device) sg_grids_free_device(p);
   for (int sl = 0; sl < 2; sl++)
     for (void *q : p->snap[sl]) hipFree(q);
   if (p->judge_buf) hipFree(p->judge_buf);
@@ -421,6 +424,7 @@ int sg_pbvh_set_fused(PBVH *p, int on) {
 }
 
 int sg_pbvh_halo_info(PBVH *p, long long out[5]) {
+  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: halo tables");
   if (!p || !out) return sg_set_error("bad arguments");
   const SgDev &d = p->d;
   out[0] = d.halo_ok ? 1 : 0;
@@ -494,12 +498,19 @@ static int snap_restore(PBVH *p, int slot) {
   return sg_check_hip(hipStreamSynchronize(p->stream), "restore");
 }
 
-int sg_pbvh_snapshot_save(PBVH *p) { return snap_save(p, 0); }
+int sg_pbvh_snapshot_save(PBVH *p) {
+  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: snapshots of PBVH_GRIDS are not built");
+  return snap_save(p, 0);
+}
 
-int sg_pbvh_snapshot_restore(PBVH *p) { return snap_restore(p, 0); }
+int sg_pbvh_snapshot_restore(PBVH *p) {
+  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: snapshots of PBVH_GRIDS are not built");
+  return snap_restore(p, 0);
+}
 
 int sg_dab_apply(PBVH *p, const SgDabParams *dab) {
   if (int rc = check_dab(p, dab)) return rc;
+  if (p->type == PBVH_GRIDS) return sg_grids_dab_apply(p, dab);
   const float *curve = nullptr;
   if (dab->curve_preset == SG_CURVE_CUSTOM && dab->curve_table) {
     if (ensure_curves(p, 1)) return -1;
@@ -717,10 +728,16 @@ static int stroke_apply(PBVH *p, const SgDabParams *dabs, const SgRay *rays, int
 }
 
 int sg_dab_apply_stroke(PBVH *p, const SgDabParams *dabs, int n, int flags) {
+  if (p && p->type == PBVH_GRIDS) {  // dab by dab
+    for (int i = 0; i < n; i++)
+      if (int rc = sg_dab_apply(p, &dabs[i])) return rc;
+    return 0;
+  }
   return stroke_apply(p, dabs, nullptr, n, flags);
 }
 
 int sg_dab_apply_stroke_placed(PBVH *p, const SgDabParams *dabs, const SgRay *rays, int n, int flags) {
+  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: PBVH_GRIDS has no ray-placed strokes");
   if (n > 0 && !rays) return sg_set_error("bad arguments");
   return stroke_apply(p, dabs, rays, n, flags);
 }
@@ -842,10 +859,12 @@ static SgStroke *stroke_prepare(PBVH *p, const SgDabParams *dabs, const SgRay *r
 }
 
 SgStroke *sg_stroke_prepare(PBVH *p, const SgDabParams *dabs, int n, int flags) {
+  if (p && p->type == PBVH_GRIDS) { sg_set_error("faces only: PBVH_GRIDS strokes run dab by dab (sg_dab_apply_stroke)"); return nullptr; }
   return stroke_prepare(p, dabs, nullptr, n, flags);
 }
 
 SgStroke *sg_stroke_prepare_placed(PBVH *p, const SgDabParams *dabs, const SgRay *rays, int n, int flags) {
+  if (p && p->type == PBVH_GRIDS) { sg_set_error("faces only: PBVH_GRIDS has no captured strokes"); return nullptr; }
   if (!rays) {
     sg_set_error("bad arguments");
     return nullptr;
@@ -933,6 +952,7 @@ static double time_stroke(PBVH *p, const SgDabParams *dabs, int n, int reps) {
 }
 
 int sg_pbvh_autotune(PBVH *p, const SgDabParams *dabs, int n, double *out_ms) {
+  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: no autotuner for PBVH_GRIDS");
   if (!p || !dabs || n <= 0) return sg_set_error("bad arguments");
   if (int rc = snap_save(p, 1)) return rc;
   const int cand[] = {32, 64, 128, 192, 256, 384, 512};
@@ -969,6 +989,7 @@ int sg_pbvh_autotune(PBVH *p, const SgDabParams *dabs, int n, double *out_ms) {
 // learned dab and keep its positions, return to the state, run the plain dab (which
 // stays applied) and compare the two on the affected vertices.
 int sg_learned_judge(PBVH *p, const SgDabParams *dab, double out[4]) {
+  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: learned Smooth is faces only");
   if (int rc = check_dab(p, dab)) return rc;
   if (!p->mlp || dab->sculpt_tool != SG_TOOL_SMOOTH) return sg_set_error("judge needs a model and a Smooth dab");
   const size_t V = (size_t)p->d.V;
@@ -1007,6 +1028,7 @@ int sg_learned_judge(PBVH *p, const SgDabParams *dab, double out[4]) {
 }
 
 int sg_pbvh_set_learned(PBVH *p, SgMlp *m) {
+  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: learned Smooth is faces only");
   if (!p) return sg_set_error("null PBVH");
   p->mlp = m;
   return 0;
@@ -1022,6 +1044,7 @@ int sg_pbvh_learned_stats(PBVH *p, long long out[2]) {
 // One plain Smooth dab that also records, for each affected vertex, the learned
 // model's input features before the dab and the displacement target after it.
 int sg_learned_collect(PBVH *p, const SgDabParams *dab, float *X, float *T, float *ok, int cap) {
+  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: learned Smooth is faces only");
   if (int rc = check_dab(p, dab)) return rc;
   if (dab->sculpt_tool != SG_TOOL_SMOOTH) return sg_set_error("collection needs a Smooth dab");
   if (cap <= 0 || !X || !T || !ok) return sg_set_error("bad arguments");
@@ -1152,6 +1175,7 @@ int sg_sync(PBVH *p) {
 int sg_dab_last_affected(PBVH *p, int *out, int cap) {
   if (!p) return sg_set_error("null PBVH");
   if (p->no_device) return sg_set_error("no HIP device");
+  if (p->type == PBVH_GRIDS) return p->gd ? sg_grids_last_affected(p, out, cap) : sg_set_error("no grids");
   int na = (int)p->last_ctr[SG_CTR_NAFF];
   std::vector<int> tmp(std::max(na, 1));
   if (p->last_tool == SG_TOOL_ELASTIC_DEFORM) {
@@ -1216,6 +1240,7 @@ int sg_dab_last_stats(PBVH *p, int64_t out[24]) {
 int sg_dab_last_gathered(PBVH *p, int *out, int cap) {
   if (!p) return sg_set_error("null PBVH");
   if (p->no_device) return sg_set_error("no HIP device");
+  if (p->type == PBVH_GRIDS) return p->gd ? sg_grids_last_gathered(p, out, cap) : sg_set_error("no grids");
   const int ng = (int)p->last_ctr[SG_CTR_NGATHER];
   std::vector<int> tmp(std::max(ng, 1));
   if (ng > 0 && sg_check_hip(hipMemcpy(tmp.data(), p->d.gathered, ng * sizeof(int), hipMemcpyDeviceToHost), "copy"))
@@ -1226,16 +1251,19 @@ int sg_dab_last_gathered(PBVH *p, int *out, int cap) {
 }
 
 int sg_pbvh_read_positions(PBVH *p, float *out) {
+  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: read grids with sg_pbvh_grids_sync_host");
   if (!p) return sg_set_error("null PBVH");
   return read_aos(p, 0, out);
 }
 
 int sg_pbvh_read_normals(PBVH *p, float *out) {
+  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: read grids with sg_pbvh_grids_sync_host");
   if (!p) return sg_set_error("null PBVH");
   return read_aos(p, 1, out);
 }
 
 int sg_pbvh_set_mask(PBVH *p, const float *mask) {
+  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: grid masks live in the grids (sg_pbvh_grids_upload)");
   if (!p) return sg_set_error("null PBVH");
   if (p->no_device) return sg_set_error("no HIP device");
   SgDev &d = p->d;
@@ -1380,6 +1408,10 @@ static int host_flush_bb(PBVH *p, int ni, int flag, std::vector<int> &changed) {
 
 void BKE_pbvh_update_bounds(PBVH *p, int flag) {  // pbvh.c:1401-1421
   if (!p || p->nodes.empty() || p->no_device || !p->build_ok) return;
+  if (p->type == PBVH_GRIDS) {
+    sg_grids_update_bounds(p, flag);
+    return;
+  }
   // the leaves update_search_cb gathers (any bit of `flag`), with the bounds bits each
   // carries
   std::vector<int> leaves, modes;
@@ -1473,8 +1505,11 @@ void BKE_pbvh_update_bounds(PBVH *p, int flag) {  // pbvh.c:1401-1421
 }
 
 void BKE_pbvh_update_normals(PBVH *p, SubdivCCG *subdiv_ccg) {  // pbvh.c:4559-4587
-  (void)subdiv_ccg;
   if (!p || p->nodes.empty() || p->no_device || !p->build_ok) return;
+  if (p->type == PBVH_GRIDS) {
+    sg_grids_update_normals(p, subdiv_ccg);
+    return;
+  }
   std::vector<int> leaves = flagged_leaves(p, PBVH_UpdateNormals);
   if (leaves.empty()) return;
   upload_host_leaf_verts(p, leaves);
@@ -1517,6 +1552,7 @@ void BKE_pbvh_update_normals(PBVH *p, SubdivCCG *subdiv_ccg) {  // pbvh.c:4559-4
 // temporary that it frees (mesh_normals.c:455-507): the vertex normals themselves
 // change only at the next BKE_pbvh_update_normals, as here.
 void BKE_pbvh_vert_coords_apply(PBVH *p, const float (*vertCos)[3], int totvert) {
+  if (p && p->type == PBVH_GRIDS) { sg_set_error("faces only: BKE_pbvh_vert_coords_apply"); return; }
   if (!p || p->nodes.empty() || p->no_device || !p->build_ok) return;
   if (totvert != p->totvert) {
     sg_set_error("PBVH: Given deforming vcos number does not match PBVH vertex number!");
@@ -1568,7 +1604,12 @@ void BKE_pbvh_node_get_original_BB(PBVHNode *node, float bb_min[3], float bb_max
 }
 
 void BKE_pbvh_node_num_verts(PBVH *p, PBVHNode *node, int *r_uniquevert, int *r_totvert) {
-  (void)p;
+  if (p && p->type == PBVH_GRIDS) {  // pbvh.c:1836-1844: every grid element is its node's own
+    const int tot = node->totprim * p->gridkey.grid_area;
+    if (r_totvert) *r_totvert = tot;
+    if (r_uniquevert) *r_uniquevert = tot;
+    return;
+  }
   if (r_totvert) *r_totvert = node->uniq_verts + node->face_verts;
   if (r_uniquevert) *r_uniquevert = node->uniq_verts;
 }
@@ -1591,7 +1632,7 @@ MVert *BKE_pbvh_get_verts(const PBVH *cp) {
 
 const float (*DUNE_pbvh_get_vert_normals(const PBVH *cp))[3] {
   PBVH *p = const_cast<PBVH *>(cp);
-  if (!p) return nullptr;
+  if (!p || p->type == PBVH_GRIDS) return nullptr;
   sync_normals(p);
   return (const float(*)[3])p->host_normals.data();
 }
@@ -1630,6 +1671,7 @@ bool sg_proxy_in_arena(const PBVH *p, const void *ptr) {
 }
 
 PBVHProxyNode *BKE_pbvh_node_add_proxy(PBVH *p, PBVHNode *node) {
+  if (p && p->type == PBVH_GRIDS) { sg_set_error("faces only: proxies"); return nullptr; }
   const int index = node->proxy_count;
   const bool in_tree = node >= p->nodes.data() && node < p->nodes.data() + p->nodes.size();
   if (index == 0 && in_tree) p->proxy_nodes.push_back((int)(node - p->nodes.data()));  // the combine's worklist

@@ -85,6 +85,7 @@ struct Builder {
   std::vector<uint8_t> vert_bitmap;
   std::vector<int> map_local;
   std::vector<int> touched;
+  bool grids = false;  // PBVH_GRIDS: prims are grids, leaves take build_grid_leaf
 
   bool mat_match(int t1, int t2) const {
     if (p->tri_mat.empty()) return true;
@@ -135,7 +136,29 @@ struct Builder {
 
   // build_mesh_leaf_node pbvh.c:256-320 with map_insert_vert :230-254.  The GHash
   // value is the insertion counter, so vert_indices is in first-insertion order.
+  // build_leaf (pbvh.c:391-407) -> build_grid_leaf_node (:383-389): fully hidden when
+  // no quad of the leaf's grids is visible; draw buffers to be rebuilt.
+  void build_grid_leaf(int ni, int offset, int count) {
+    PBVHNode &n = p->nodes[ni];
+    n.flag |= PBVH_Leaf;
+    n.children_offset = -1;
+    n.prim_off = offset;
+    n.totprim = count;
+    update_vb(n, offset, count);
+    n.uniq_verts = n.face_verts = 0;
+    n.vert_indices = nullptr;
+    n.face_vert_indices = nullptr;
+    const int totquads =
+        BKE_pbvh_count_grid_quads(p->grid_hidden, &p->prim_indices[offset], count, p->gridkey.grid_size);
+    if (totquads == 0) n.flag |= PBVH_FullyHidden;
+    n.flag |= PBVH_RebuildDrawBuffers | PBVH_UpdateDrawBuffers | PBVH_UpdateRedraw;
+  }
+
   void build_leaf(int ni, int offset, int count) {
+    if (grids) {
+      build_grid_leaf(ni, offset, count);
+      return;
+    }
     PBVHNode &n = p->nodes[ni];
     n.flag |= PBVH_Leaf;
     n.children_offset = -1;
@@ -303,6 +326,44 @@ void sg_build_tree(PBVH *p, const float *co, const int *tris, int T, int V) {
     if (p->perm_new_of_old[v] < 0) {
       p->perm_new_of_old[v] = (int)p->perm_old_of_new.size();
       p->perm_old_of_new.push_back(v);
+    }
+  }
+}
+
+// BKE_pbvh_build_grids (pbvh.c:598-643): a grid's box over all its elements in element
+// order, its centroid, the root centroid box in grid order, then the same top-down
+// build as for triangles (leaf_limit already divided by grid_size^2 by the caller).
+// Fills prim_indices, nodes and leaf_node (DFS leaf order).
+void sg_build_tree_grids(PBVH *p) {
+  Builder b;
+  b.p = p;
+  b.co = nullptr;
+  b.tris = nullptr;
+  b.grids = true;
+  const int G = p->totgrid, ga = p->gridkey.grid_area;
+  b.bbc.resize((size_t)G);
+  parallel_for(G, [&](int i) {
+    BBC &x = b.bbc[i];
+    bb_reset(x.bmin);
+    const unsigned char *base = (const unsigned char *)p->grids[i];
+    for (int j = 0; j < ga; j++) bb_expand(x.bmin, (const float *)(base + (size_t)j * p->gridkey.elem_size));
+    for (int k = 0; k < 3; k++) x.bcentroid[k] = (x.bmin[k] + x.bmax[k]) * 0.5f;
+  });
+  if (G > 0) b.build(G);
+  p->leaf_node.clear();
+  if (!p->nodes.empty()) {
+    std::vector<int> st{0};
+    while (!st.empty()) {
+      const int ni = st.back();
+      st.pop_back();
+      PBVHNode &n = p->nodes[ni];
+      if (n.flag & PBVH_Leaf) {
+        n.leaf = (int)p->leaf_node.size();
+        p->leaf_node.push_back(ni);
+      } else {
+        st.push_back(n.children_offset + 1);
+        st.push_back(n.children_offset);
+      }
     }
   }
 }

@@ -209,8 +209,21 @@ enum {
   SG_CTR_COUNT = 24,
 };
 
+struct SgGridDev;
+struct SubdivCCG;
 struct PBVH {
   PBVHType type = PBVH_FACES;
+  // PBVH_GRIDS (pbvh_intern.h:129-134, :161): the caller's grids (borrowed), their key,
+  // faces, material flags and hidden bitmaps, the SubdivCCG they belong to, and the
+  // device state
+  CCGElem **grids = nullptr;
+  CCGKey gridkey{};
+  void **gridfaces = nullptr;
+  DMFlagMat *grid_flag_mats = nullptr;
+  BLI_bitmap **grid_hidden = nullptr;
+  int totgrid = 0;
+  SubdivCCG *subdiv_ccg = nullptr;
+  SgGridDev *gd = nullptr;
   int leaf_limit = 10000;
   bool respect_hide = true;
   bool no_device = false;            // no HIP device: host tree only
@@ -254,7 +267,7 @@ struct PBVH {
   int grid_brush = 0, grid_post = 0, grid_smooth = 0;  // per-kernel overrides (autotuner)
   int grid_fused = 1024;             // workgroups of the fused dab kernel (a leaf each)
   bool fused_enabled = true;
-  bool fused_pending = false;  // the last dab enqueued was fused and no finish followed yet         // plain dabs run fused when the mesh has halo tables
+  bool fused_pending = false;        // the last dab enqueued was fused and no finish followed yet
   // device snapshot (checkpoint) of the mutable mesh state
   // slot 0: the caller's (sg_pbvh_snapshot_save/restore); slot 1: internal (autotuner, judge)
   std::vector<void *> snap[2];

@@ -3,8 +3,9 @@
  *
  * Two groups of entry points:
  *
- *  1. Drop-in replacements for the reference's PBVH C API (PBVH_FACES only).  The
- *     reference's public header (BKE_pbvh.h / DUNE_pbvh.h / KERNEL_pbvh.h) is absent,
+ *  1. Drop-in replacements for the reference's PBVH C API (PBVH_FACES; PBVH_GRIDS in
+ *     section 3).  The reference's public header (BKE_pbvh.h / DUNE_pbvh.h /
+ *     KERNEL_pbvh.h) is absent,
  *     so each signature is taken from its definition in kernel/intern/pbvh.c and its
  *     call sites (paint.c:1676-1771, :2084-2191; editors/sculpt_paint/paint_hide.c).
  *     Each declaration cites the reference function it replaces.  Mesh data stays in
@@ -531,6 +532,128 @@ int sg_batch_set_learned(SgBatch *batch, SgMlp *mlp);
 int sg_batch_set_persistent(SgBatch *batch, int on);
 int sg_batch_timer_start(SgBatch *batch);
 int sg_batch_timer_stop(SgBatch *batch, float *ms);
+
+/* ---- 3. multires grids (PBVH_GRIDS) ------------------------------------------------
+ * The CCG types live in headers the reference does not ship (BKE_ccg.h,
+ * BKE_subdiv_ccg.h); their fields follow the reference's uses: the key
+ * (kernel/intern/subdiv_ccg.c:633-646), coordinates (:368-379), neighbour sets
+ * (:1365-1380), faces (:312-323).  An element is co float[3], then the mask float
+ * (has_mask), then the normal float[3] (has_normals) (:62-90); a grid is grid_size^2
+ * elements, row-major (x fastest).  In a face corner's grid, (0,0) is the face centre,
+ * (grid_size-1, grid_size-1) the corner's coarse vertex, the column x = grid_size-1 runs
+ * along the corner's edge and the row y = grid_size-1 along the previous corner's
+ * edge (:397-468, :951-984). */
+typedef struct CCGElem CCGElem;
+typedef struct CCGKey {
+  int level;
+  int elem_size;
+  int grid_size;
+  int grid_area;
+  int grid_bytes;
+  int normal_offset;
+  int mask_offset;
+  int has_normals;
+  int has_mask;
+} CCGKey;
+typedef struct DMFlagMat { /* grid_materials_match (pbvh.c:145-148) reads flag and mat_nr */
+  short mat_nr;
+  char flag;
+} DMFlagMat;
+typedef unsigned int BLI_bitmap; /* one bit per grid element, 32 per word */
+typedef struct CCGFace CCGFace;  /* a SubdivCCGFace, as gridfaces hands it around */
+typedef struct SubdivCCGFace {
+  int num_grids;
+  int start_grid_index;
+} SubdivCCGFace;
+typedef struct SubdivCCGCoord {
+  int grid_index;
+  short x, y;
+} SubdivCCGCoord;
+typedef struct SubdivCCGNeighbors {
+  SubdivCCGCoord *coords; /* coords_fixed, or malloc'ed when size >= 256 */
+  int size;
+  int num_duplicates;
+  SubdivCCGCoord coords_fixed[256];
+} SubdivCCGNeighbors;
+
+/* Grids of a coarse polygon mesh at subdivision `level` (grid_size = 2^(level-1) + 1,
+ * subdiv_inline.h:26-29): a grid per polygon corner (subdiv_ccg.c:104-140), the face,
+ * edge and vertex neighbourhoods (:312-530), normals and mask layers on request.  The
+ * reference evaluates elements on the OpenSubdiv limit surface (KERNEL_subdiv_to_ccg,
+ * :541-562; OpenSubdiv is a third-party dependency it does not vendor); here each
+ * grid is the bilinear patch over (face centre, the corner's edge midpoint, the
+ * corner vertex, the previous edge's midpoint), after which the grids are averaged
+ * (KERNEL_subdiv_ccg_average_grids) so every duplicated element holds the same bits,
+ * and normals computed (KERNEL_subdiv_ccg_recalc_normals).  Edges are numbered in
+ * order of first use by the polygons, each oriented as first used; a vertex's edges
+ * are listed by ascending edge number.  NULL on bad input (text in sg_last_error). */
+SubdivCCG *sg_subdiv_ccg_new(const float (*coarse_co)[3], int totvert, const int *poly_offsets,
+                             const int *poly_verts, int totpoly, int level, int need_normal, int need_mask);
+/* subdiv_ccg.c:595 */
+void KERNEL_subdiv_ccg_destroy(SubdivCCG *subdiv_ccg);
+/* subdiv_ccg.c:648 */
+void KERNEL_subdiv_ccg_key_top_level(CCGKey *key, const SubdivCCG *subdiv_ccg);
+/* subdiv_ccg.c:781 every grid's normals, then the grids averaged (host) */
+void KERNEL_subdiv_ccg_recalc_normals(SubdivCCG *subdiv_ccg);
+/* subdiv_ccg.c:1171 seams inside faces, then edges, then corners (host) */
+void KERNEL_subdiv_ccg_average_grids(SubdivCCG *subdiv_ccg);
+/* subdiv_ccg.c:847 normals of the given faces' grids and the seams around them (host;
+ * a PBVH built on these grids runs the same step on the GPU in BKE_pbvh_update_normals) */
+void KERNEL_subdiv_ccg_update_normals(SubdivCCG *subdiv_ccg, CCGFace **effected_faces, int num_effected_faces);
+/* subdiv_ccg.c:1882 */
+void KERNEL_subdiv_ccg_neighbor_coords_get(const SubdivCCG *subdiv_ccg, const SubdivCCGCoord *coord,
+                                           bool include_duplicates, SubdivCCGNeighbors *r_neighbors);
+/* frees r_neighbors->coords when it is not coords_fixed */
+void sg_subdiv_ccg_neighbors_free(SubdivCCGNeighbors *neighbors);
+/* subdiv_ccg.c:1911 */
+int KERNEL_subdiv_ccg_grid_to_face_index(const SubdivCCG *subdiv_ccg, int grid_index);
+/* The SubdivCCG fields the reference's callers read (paint.c passes them to
+ * BKE_pbvh_build_grids): grids, num_grids, grid_faces (as void **gridfaces),
+ * faces, grid_flag_mats, grid_hidden (one bitmap per grid, all clear at creation). */
+CCGElem **sg_subdiv_ccg_grids(SubdivCCG *subdiv_ccg);
+int sg_subdiv_ccg_num_grids(const SubdivCCG *subdiv_ccg);
+void **sg_subdiv_ccg_grid_faces(SubdivCCG *subdiv_ccg);
+SubdivCCGFace *sg_subdiv_ccg_faces(SubdivCCG *subdiv_ccg, int *r_num_faces);
+DMFlagMat *sg_subdiv_ccg_flag_mats(SubdivCCG *subdiv_ccg);
+BLI_bitmap **sg_subdiv_ccg_grid_hidden(SubdivCCG *subdiv_ccg);
+/* Element copies, grid after grid, row-major: co / normals 3 floats, mask 1 float
+ * (-1 when the layer is absent). */
+int sg_subdiv_ccg_get_co(const SubdivCCG *subdiv_ccg, float *out);
+int sg_subdiv_ccg_set_co(SubdivCCG *subdiv_ccg, const float *in);
+int sg_subdiv_ccg_get_normals(const SubdivCCG *subdiv_ccg, float *out);
+int sg_subdiv_ccg_get_mask(const SubdivCCG *subdiv_ccg, float *out);
+int sg_subdiv_ccg_set_mask(SubdivCCG *subdiv_ccg, const float *in);
+
+/* pbvh.c:598 BKE_pbvh_build_grids: the tree over grids exactly as the reference builds
+ * it (a grid's box over all its elements, leaf_limit = max(limit / grid_size^2, 1) with
+ * limit the handle's leaf limit, 10000 by default as pbvh.c:32, material split on
+ * DMFlagMat, leaves fully hidden when every quad has a hidden corner), then the
+ * elements uploaded to HBM leaf by leaf.  The grids stay the caller's (borrowed);
+ * sg_pbvh_grids_sync_host writes the device state back into them.  Dabs, normals and
+ * the grid-seam averaging also need the SubdivCCG the grids belong to:
+ * DUNE_pbvh_subdiv_cgg_set. */
+void BKE_pbvh_build_grids(PBVH *pbvh, CCGElem **grids, int totgrid, CCGKey *key, void **gridfaces,
+                          DMFlagMat *flagmats, BLI_bitmap **grid_hidden);
+/* pbvh.c:4973 (uploads the face, edge, vertex and neighbour tables of the grids) */
+void DUNE_pbvh_subdiv_cgg_set(PBVH *pbvh, SubdivCCG *subdiv_ccg);
+/* pbvh.c:1865 */
+void BKE_pbvh_node_get_grids(PBVH *pbvh, PBVHNode *node, int **r_grid_indices, int *r_totgrid, int *r_maxgrid,
+                             int *r_gridsize, CCGElem ***r_griddata);
+/* pbvh.c:1679 / :1685 / :1691 / :1703 / :1709 */
+BLI_bitmap **BKE_pbvh_grid_hidden(const PBVH *pbvh);
+const CCGKey *BKE_pbvh_get_grid_key(const PBVH *pbvh);
+CCGElem **BKE_pbvh_get_grids(const PBVH *pbvh);
+int BKE_pbvh_get_grid_num_vertices(const PBVH *pbvh);
+int BKE_pbvh_get_grid_num_faces(const PBVH *pbvh);
+/* pbvh.c:1605: faces (SubdivCCGFace*) of the grids of UpdateNormals nodes, each once,
+ * in order of first appearance; malloc'ed, caller frees. */
+void BKE_pbvh_get_grid_updates(PBVH *pbvh, bool clear, void ***r_gridfaces, int *r_totface);
+/* pbvh.c:331 (paint_is_grid_face_hidden, absent from the reference, taken as upstream
+ * Blender's: a quad is hidden when any of its four corners is) */
+int BKE_pbvh_count_grid_quads(BLI_bitmap **grid_hidden, const int *grid_indices, int totgrid, int gridsize);
+/* Device state -> the caller's grids (co, normals, mask), and back after host edits. */
+int sg_pbvh_grids_sync_host(PBVH *pbvh);
+int sg_pbvh_grids_upload(PBVH *pbvh);
 
 /* ---- synthetic meshes and strokes (deterministic; for tests and benchmarks) ---- */
 int sg_mesh_icosphere_size(int level, int *r_totvert, int *r_tottri);

@@ -18,6 +18,8 @@
  *   initial    kernel/intern/mesh_normals.c:225-299 (angle weighted)
  *   refit      kernel/intern/pbvh.c:108-128, :1206-1242, :1369-1421
  *   brushes    project spec S1-S11 (the reference's sculpt.c/brush.c are absent)
+ *   grids      kernel/intern/subdiv_ccg.c and the PBVH_GRIDS parts of pbvh.c
+ *              (grids.c, cited there)
  *
  * Parity status of each function is stated in its own comment in the .c files.
  * An FP64 replay runs beside the FP32 state (same affected sets, every other
@@ -209,6 +211,41 @@ int orc_draw_leaf(const OrcPBVH *o, int node, const OrcPolys *polys, const float
 void orc_face_set_color(int face_set, int seed, unsigned char out[3]);
 void orc_lookup3_final(uint32_t abc[3]);
 float orc_hash_int_01(uint32_t k);
+
+/* Multires grids (grids.c): grids of a coarse polygon mesh (grid_size^2 elements per
+ * polygon corner, element (g, x, y) at g * grid_size^2 + y * grid_size + x), the tree
+ * over them, the seam averages and normals, and the grid dab.  co [G*ga*3]; no
+ * (NULL: no normal layer), mask (NULL: none), hidden bytes per element (NULL: none),
+ * grid_mat material key per grid (NULL: one material); leaf_limit before the
+ * division by grid_size^2. */
+typedef struct OrcGrids OrcGrids;
+OrcGrids *orc_grids_new(int totvert, const int *poly_off, const int *poly_verts, int totpoly, int grid_size,
+                        const float *co, const float *no, const float *mask, const uint8_t *hidden,
+                        const int *grid_mat, int leaf_limit, int respect_hide);
+void orc_grids_free(OrcGrids *o);
+int orc_grids_totnode(const OrcGrids *o);
+/* flag, children_offset (-1 for leaves), prim offset, prim count */
+void orc_grids_node_info(const OrcGrids *o, int node, int out[4]);
+void orc_grids_node_bb(const OrcGrids *o, int node, float vb[6], float orig_vb[6]);
+void orc_grids_prim_indices(const OrcGrids *o, int *out);
+void orc_grids_get(const OrcGrids *o, float *co, float *no, float *mask);
+void orc_grids_set_co(OrcGrids *o, const float *co);
+void orc_grids_last_stats(const OrcGrids *o, int64_t out[4]);
+int orc_grids_edge_count(const OrcGrids *o);
+int orc_grids_count_quads(const OrcGrids *o, const int *grids, int n);
+/* neighbours of (g, x, y) as (grid, x, y) triples; *r_ndup of them (the last) are
+ * duplicates of the element itself; returns the count */
+int orc_grids_neighbors(const OrcGrids *o, int g, int x, int y, int include_duplicates, int *out, int cap,
+                        int *r_ndup);
+void orc_grids_average(OrcGrids *o);
+void orc_grids_recalc_normals(OrcGrids *o);
+void orc_grids_update_normals_faces(OrcGrids *o, const int *faces, int nfaces);
+int orc_grids_search(const OrcGrids *o, const float center[3], float radius, int use_orig, int *out, int cap);
+/* returns |A|; out: affected element ids, ascending */
+int orc_grids_dab(OrcGrids *o, const OrcDab *dab, int *out_affected, int cap);
+void orc_grids_mark(OrcGrids *o, const int *nodes, int n, int flag);
+void orc_grids_update_normals(OrcGrids *o);
+void orc_grids_update_bounds(OrcGrids *o);
 
 /* Threads used by the OpenMP loops (0 = runtime default). */
 void orc_set_threads(int n);

@@ -50,3 +50,7 @@ struct OrcPBVH {
 };
 
 int64_t orc_update_normals_bounds(OrcPBVH *o);
+
+/* Spec S5 bstrength (oracle.c), shared with grids.c. */
+struct OrcDab;
+float orc_bstrength(const struct OrcDab *dab);

@@ -110,6 +110,26 @@ def lib():
         "orc_isect_ray_box": (i32, [f32p, f32p, f32p, f32p, f32p]),
         "orc_dist_sq_ray_box": (C.c_float, [f32p, f32p, f32p, f32p, f32p]),
         "orc_dist_sq_ray_tri": (C.c_float, [f32p, f32p, f32p, f32p, f32p, f32p]),
+        "orc_grids_new": (vp, [i32, i32p, i32p, i32, i32, f32p, f32p, f32p, C.POINTER(C.c_uint8), i32p, i32, i32]),
+        "orc_grids_free": (None, [vp]),
+        "orc_grids_totnode": (i32, [vp]),
+        "orc_grids_node_info": (None, [vp, i32, i32p]),
+        "orc_grids_node_bb": (None, [vp, i32, f32p, f32p]),
+        "orc_grids_prim_indices": (None, [vp, i32p]),
+        "orc_grids_get": (None, [vp, f32p, f32p, f32p]),
+        "orc_grids_set_co": (None, [vp, f32p]),
+        "orc_grids_last_stats": (None, [vp, C.POINTER(C.c_int64)]),
+        "orc_grids_edge_count": (i32, [vp]),
+        "orc_grids_count_quads": (i32, [vp, i32p, i32]),
+        "orc_grids_neighbors": (i32, [vp, i32, i32, i32, i32, i32p, i32, i32p]),
+        "orc_grids_average": (None, [vp]),
+        "orc_grids_recalc_normals": (None, [vp]),
+        "orc_grids_update_normals_faces": (None, [vp, i32p, i32]),
+        "orc_grids_search": (i32, [vp, f32p, C.c_float, i32, i32p, i32]),
+        "orc_grids_dab": (i32, [vp, C.POINTER(OrcDab), i32p, i32]),
+        "orc_grids_mark": (None, [vp, i32p, i32, i32]),
+        "orc_grids_update_normals": (None, [vp]),
+        "orc_grids_update_bounds": (None, [vp]),
     }
     for k, (r, a) in sig.items():
         f = getattr(L, k)
@@ -462,3 +482,125 @@ def face_set_color(fs, seed):
     out = (C.c_uint8 * 3)()
     lib().orc_face_set_color(fs, seed, out)
     return bytes(out)
+
+
+class OracleGrids:
+    """The oracle's PBVH over multires grids (grids.c)."""
+
+    def __init__(self, totvert, poly_off, poly_verts, grid_size, co, no=None, mask=None, hidden=None,
+                 grid_mat=None, leaf_limit=10000, respect_hide=True):
+        L = lib()
+        self.poly_off = np.ascontiguousarray(poly_off, np.int32)
+        self.poly_verts = np.ascontiguousarray(poly_verts, np.int32)
+        self.gs = int(grid_size)
+        self.ga = self.gs * self.gs
+        self.G = int(self.poly_off[-1] - self.poly_off[0])
+        self.E = self.G * self.ga
+        f32 = lambda a: None if a is None else np.ascontiguousarray(a, np.float32)
+        co, no, mask = f32(co), f32(no), f32(mask)
+        hid = None if hidden is None else np.ascontiguousarray(hidden, np.uint8)
+        gm = None if grid_mat is None else np.ascontiguousarray(grid_mat, np.int32)
+        fp = lambda a: None if a is None else a.ctypes.data_as(C.POINTER(C.c_float))
+        self.has_normal = no is not None
+        self.has_mask = mask is not None
+        self.h = L.orc_grids_new(int(totvert), self.poly_off.ctypes.data_as(C.POINTER(C.c_int)),
+                                 self.poly_verts.ctypes.data_as(C.POINTER(C.c_int)), len(self.poly_off) - 1,
+                                 self.gs, fp(co), fp(no), fp(mask),
+                                 None if hid is None else hid.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                 None if gm is None else gm.ctypes.data_as(C.POINTER(C.c_int)), int(leaf_limit),
+                                 1 if respect_hide else 0)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().orc_grids_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def totnode(self):
+        return lib().orc_grids_totnode(self.h)
+
+    def node_info(self, i):
+        out = np.zeros(4, np.int32)
+        lib().orc_grids_node_info(self.h, i, out.ctypes.data_as(C.POINTER(C.c_int)))
+        return out
+
+    def node_bb(self, i):
+        vb = np.zeros(6, np.float32)
+        ob = np.zeros(6, np.float32)
+        lib().orc_grids_node_bb(self.h, i, vb.ctypes.data_as(C.POINTER(C.c_float)),
+                                ob.ctypes.data_as(C.POINTER(C.c_float)))
+        return vb, ob
+
+    def boxes(self):
+        b = [self.node_bb(i) for i in range(self.totnode())]
+        return np.stack([x[0] for x in b]), np.stack([x[1] for x in b])
+
+    def prim_indices(self):
+        out = np.empty(self.G, np.int32)
+        lib().orc_grids_prim_indices(self.h, out.ctypes.data_as(C.POINTER(C.c_int)))
+        return out
+
+    def state(self):
+        co = np.empty((self.E, 3), np.float32)
+        no = np.empty((self.E, 3), np.float32)
+        mk = np.empty(self.E, np.float32)
+        fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+        lib().orc_grids_get(self.h, fp(co), fp(no), fp(mk))
+        return co, no, (mk if self.has_mask else None)
+
+    def set_co(self, co):
+        c = np.ascontiguousarray(co, np.float32)
+        lib().orc_grids_set_co(self.h, c.ctypes.data_as(C.POINTER(C.c_float)))
+
+    def neighbors(self, g, x, y, include_duplicates=False):
+        out = np.zeros(3 * 512, np.int32)
+        nd = C.c_int()
+        n = lib().orc_grids_neighbors(self.h, g, x, y, 1 if include_duplicates else 0,
+                                      out.ctypes.data_as(C.POINTER(C.c_int)), 512, C.byref(nd))
+        return [tuple(int(v) for v in out[3 * i: 3 * i + 3]) for i in range(n)], nd.value
+
+    def average(self):
+        lib().orc_grids_average(self.h)
+
+    def recalc_normals(self):
+        lib().orc_grids_recalc_normals(self.h)
+
+    def update_normals_faces(self, faces):
+        f = np.ascontiguousarray(faces, np.int32)
+        lib().orc_grids_update_normals_faces(self.h, f.ctypes.data_as(C.POINTER(C.c_int)), f.size)
+
+    def count_quads(self, grids):
+        g = np.ascontiguousarray(grids, np.int32)
+        return lib().orc_grids_count_quads(self.h, g.ctypes.data_as(C.POINTER(C.c_int)), g.size)
+
+    def search(self, center, radius, use_orig=False):
+        c = np.ascontiguousarray(center, np.float32)
+        out = np.empty(self.totnode() + 1, np.int32)
+        n = lib().orc_grids_search(self.h, c.ctypes.data_as(C.POINTER(C.c_float)), float(radius),
+                                   1 if use_orig else 0, out.ctypes.data_as(C.POINTER(C.c_int)), out.size)
+        return out[:n].copy()
+
+    def dab(self, d):
+        out = np.empty(self.E, np.int32)
+        n = lib().orc_grids_dab(self.h, C.byref(d), out.ctypes.data_as(C.POINTER(C.c_int)), out.size)
+        return out[:n].copy()
+
+    def last_stats(self):
+        out = np.zeros(4, np.int64)
+        lib().orc_grids_last_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_int64)))
+        return out
+
+    def mark(self, nodes, flag):
+        a = np.ascontiguousarray(nodes, np.int32)
+        lib().orc_grids_mark(self.h, a.ctypes.data_as(C.POINTER(C.c_int)), a.size, int(flag))
+
+    def update_normals(self):
+        lib().orc_grids_update_normals(self.h)
+
+    def update_bounds(self):
+        lib().orc_grids_update_bounds(self.h)

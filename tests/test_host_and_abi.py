@@ -16,13 +16,16 @@ ROOT = Path(__file__).resolve().parent.parent
 def test_library_exports_every_declared_symbol(sg):
     header = (ROOT / "include" / "sculpt_gpu.h").read_text()
     body = re.sub(r"/\*.*?\*/", "", header, flags=re.S)
-    names = set(re.findall(r"\b((?:BKE|DUNE|sg)_[A-Za-z0-9_]+)\s*\(", body))
-    names |= set(re.findall(r"\(\*((?:BKE|DUNE|sg)_[A-Za-z0-9_]+)\(", body))
+    names = set(re.findall(r"\b((?:BKE|DUNE|KERNEL|sg)_[A-Za-z0-9_]+)\s*\(", body))
+    names |= set(re.findall(r"\(\*((?:BKE|DUNE|KERNEL|sg)_[A-Za-z0-9_]+)\(", body))
     assert len(names) > 40
     L = C.CDLL(str(sg.LIB_PATH))
     missing = [n for n in sorted(names) if not hasattr(L, n)]
     assert not missing, missing
     assert set(sg.EXPORTS) <= names
+    from dune_sculpt_amd import grids as gr
+
+    assert set(gr.EXPORTS) <= names
 
 
 def test_library_is_gfx950_hip(sg):
