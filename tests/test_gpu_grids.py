@@ -59,6 +59,8 @@ def assert_same_state(p, o, tag):
 
 
 def run_parity(sg, po, p, o, dabs, check_every):
+    start = o.state()[0].copy()
+    total = 0
     for i, d in enumerate(dabs):
         od = po.to_orc(d)
         gathered = o.search(d.location, d.radius, use_orig=d.sculpt_tool == sg.TOOL_GRAB)
@@ -69,8 +71,11 @@ def run_parity(sg, po, p, o, dabs, check_every):
         assert np.array_equal(p.last_gathered(), gathered), f"dab {i}: gathered leaves"
         st = p.last_stats()
         assert st[0] == len(gathered) and st[1] == len(want)
+        total += len(want)
         if (i + 1) % check_every == 0 or i + 1 == len(dabs):
             assert_same_state(p, o, f"after dab {i}")
+    assert total > 10 * len(dabs)  # the stroke touched the surface
+    assert np.abs(o.state()[0] - start).max() > 1e-3
 
 
 @pytest.mark.parametrize("leaf_limit", [10000, 2000])
