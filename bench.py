@@ -595,6 +595,71 @@ def run_c1(sg, W, K):
     return out
 
 
+def run_grids(sg, W, K, args):
+    """Multires grids (PBVH_GRIDS): a cube of 64 x 64 quads per side on the unit sphere,
+    subdivision level 4 (98,304 grids of 9 x 9 = 7,962,624 elements) displaced by the
+    same seeded noise as the mesh workload (amplitude 0.02), the same 1000-dab stroke
+    (Draw / Inflate / Smooth(4) / Grab), leaves of max(10000 / 81, 1) = 123 grids.  GPU:
+    the stroke as a captured graph from the restored state (HIP events around it);
+    CPU: the oracle (one thread) on the first dabs of the same stroke within a budget,
+    its affected counts checked against the GPU's."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle as po
+    from dune_sculpt_amd import grids as gr
+
+    t = time.perf_counter()
+    co, off, pv = gr.quad_sphere(args.grids_n)
+    g = gr.Grids(co, off, pv, level=args.grids_level)
+    c = g.co()
+    d = (c / np.linalg.norm(c, axis=1, keepdims=True)).astype(np.float32)
+    g.set_co(sg.noise_displace(d.copy(), d, 1, 0.02))
+    g.recalc_normals()
+    P = gr.GridPBVH(g)
+    build_s = time.perf_counter() - t
+    dabs, _ = plain_stroke(sg, args.dabs)
+    P.snapshot()
+    prep = P.prepare(dabs, log=True)
+    times = []
+    for s in range(W + max(K, 3)):
+        P.restore()
+        P.timer_start()
+        prep.launch(sync=False)
+        ms = P.timer_stop()
+        if s >= W:
+            times.append(ms)
+    counts = P.stroke_log(len(dabs))[:, 1]
+    aff = int(counts.sum())
+    prep.close()
+    gpu_ms = float(np.median(times))
+    out = {"workload": f"quad sphere {args.grids_n}x{args.grids_n} per side, level {args.grids_level} "
+                       f"({g.G} grids of {g.gs}x{g.gs}, {g.E} elements), noise 0.02x4oct, "
+                       f"{len(dabs)}-dab Draw/Inflate/Smooth(4)/Grab stroke, leaf limit 10000 elements",
+           "build_s": build_s, "leaves": P.leaves, "affected": aff,
+           "gpu_ms_per_stroke": gpu_ms, "gpu_us_per_dab": gpu_ms * 1e3 / len(dabs),
+           "gpu_element_updates_per_s": aff / (gpu_ms * 1e-3)}
+    if not args.no_cpu:
+        P.restore()
+        P.sync_host()
+        o = po.OracleGrids(len(co), off, pv, g.gs, g.co(), g.normals())
+        t0 = time.perf_counter()
+        n = a = 0
+        for i, dab in enumerate(dabs):
+            k = len(o.dab(po.to_orc(dab)))
+            assert k == counts[i], f"grids dab {i}: oracle {k} vs GPU {counts[i]} affected"
+            a += k
+            n += 1
+            if time.perf_counter() - t0 > args.cpu_budget / 2:
+                break
+        dt = time.perf_counter() - t0
+        o.close()
+        out["cpu_baseline"] = {"value": a / dt, "unit": "element-updates/s", "cores": 1, "kind": "port",
+                               "sample": f"first {n} dabs ({dt:.1f}s, C oracle, one thread)"}
+    P.close()
+    g.close()
+    log(f"grids: {out}")
+    return out
+
+
 def cpu_model():
     """The host CPU's model name, for the record beside the CPU baseline."""
     try:
@@ -671,6 +736,10 @@ def main():
     ap.add_argument("--only-undo", action="store_true", help="run only the undo-step measurement")
     ap.add_argument("--no-elastic", action="store_true", help="skip the elastic deform measurement")
     ap.add_argument("--no-c1", action="store_true", help="skip the icosphere-6 CPU-reference case")
+    ap.add_argument("--no-grids", action="store_true", help="skip the multires grids (PBVH_GRIDS) stroke")
+    ap.add_argument("--only-grids", action="store_true", help="run only the multires grids stroke")
+    ap.add_argument("--grids-n", type=int, default=64, help="coarse quads per cube side of the grids mesh")
+    ap.add_argument("--grids-level", type=int, default=4, help="subdivision level of the grids (grid size 2^(l-1)+1)")
     ap.add_argument("--no-fused", action="store_true", help="separate brush / Smooth / post kernels (A/B)")
     ap.add_argument("--only-elastic", action="store_true", help="run only the elastic deform measurement")
     ap.add_argument("--placed-only", action="store_true",
@@ -688,6 +757,9 @@ def main():
 
     if args.only_batch:
         print(json.dumps({"batch": run_batch_workloads(sg, args, args.warmup)}))
+        return
+    if args.only_grids:
+        print(json.dumps({"grids": run_grids(sg, args.warmup, args.steps, args)}))
         return
 
     t = time.perf_counter()
@@ -847,6 +919,7 @@ def main():
     draw = None if (args.no_ray or args.no_draw) else run_draw(sg, A, dabs, W, max(K, 3))
 
     c1 = None if args.no_c1 else run_c1(sg, W, K)
+    grids = None if args.no_grids else run_grids(sg, W, K, args)
     c3 = None
     if not args.no_learned:
         c3 = run_learned(sg, A, W, args)
@@ -913,6 +986,7 @@ def main():
         "alg_bytes_per_dab_by_kernel": kernel_bytes_per_dab,
         "hbm_copy": copy,
         "c1": c1,
+        "grids": grids,
         "learned_smooth": c3,
         "batch": batch,
         "raycast": ray,

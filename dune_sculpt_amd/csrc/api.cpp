@@ -499,12 +499,16 @@ static int snap_restore(PBVH *p, int slot) {
 }
 
 int sg_pbvh_snapshot_save(PBVH *p) {
-  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: snapshots of PBVH_GRIDS are not built");
+  if (p && p->type == PBVH_GRIDS) return p->gd ? sg_grids_snapshot(p, 0) : sg_set_error("no grids");
   return snap_save(p, 0);
 }
 
 int sg_pbvh_snapshot_restore(PBVH *p) {
-  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: snapshots of PBVH_GRIDS are not built");
+  if (p && p->type == PBVH_GRIDS) {
+    if (!p->gd) return sg_set_error("no grids");
+    p->host_bb_stale = true;
+    return sg_grids_snapshot(p, 1);
+  }
   return snap_restore(p, 0);
 }
 
@@ -728,8 +732,15 @@ static int stroke_apply(PBVH *p, const SgDabParams *dabs, const SgRay *rays, int
 }
 
 int sg_dab_apply_stroke(PBVH *p, const SgDabParams *dabs, int n, int flags) {
-  if (p && p->type == PBVH_GRIDS) {  // dab by dab
-    for (int i = 0; i < n; i++)
+  if (p && p->type == PBVH_GRIDS) {
+    if (flags & SG_STROKE_GRAPH) {  // one captured graph
+      SgStroke *s = sg_stroke_prepare(p, dabs, n, flags);
+      if (!s) return -1;
+      const int rc = sg_stroke_launch(s, 1);
+      sg_stroke_free(s);
+      return rc;
+    }
+    for (int i = 0; i < n; i++)  // dab by dab
       if (int rc = sg_dab_apply(p, &dabs[i])) return rc;
     return 0;
   }
@@ -824,10 +835,24 @@ static SgStroke *stroke_prepare(PBVH *p, const SgDabParams *dabs, const SgRay *r
     }
   }
   p->halt_live = s->learned;
+  if (p->type == PBVH_GRIDS && !p->gd->topo_of && p->subdiv_ccg && sg_grids_attach_topology(p, p->subdiv_ccg)) {
+    stroke_release(s);
+    return nullptr;
+  }
   p->capturing = true;
   int rc = sg_check_hip(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal), "begin capture");
   int erc = 0;
-  if (!rc) {
+  if (!rc && p->type == PBVH_GRIDS) {  // each grid dab is self-contained (counters, normals, refit)
+    int ci = 0;
+    for (int i = 0; i < n && !erc; i++) {
+      const float *curve = nullptr;
+      if (dabs[i].curve_preset == SG_CURVE_CUSTOM && dabs[i].curve_table)
+        curve = s->curves + (size_t)(ci++) * (SG_CM_TABLE + 1);
+      erc = sg_grids_enqueue_dab(p, sg_make_dab(&dabs[i], 0), curve, log ? p->d.log : nullptr, i);
+    }
+    const hipError_t e = hipStreamEndCapture(p->stream, &s->graph);
+    rc = erc ? erc : sg_check_hip(e, "end capture");
+  } else if (!rc) {
     int ci = 0;
     erc = sg_enqueue_clear_ctr(p);
     for (int i = 0; i < n && !erc; i++) {
@@ -859,7 +884,6 @@ static SgStroke *stroke_prepare(PBVH *p, const SgDabParams *dabs, const SgRay *r
 }
 
 SgStroke *sg_stroke_prepare(PBVH *p, const SgDabParams *dabs, int n, int flags) {
-  if (p && p->type == PBVH_GRIDS) { sg_set_error("faces only: PBVH_GRIDS strokes run dab by dab (sg_dab_apply_stroke)"); return nullptr; }
   return stroke_prepare(p, dabs, nullptr, n, flags);
 }
 
@@ -902,6 +926,13 @@ int sg_stroke_launch(SgStroke *s, int sync) {
     }
   }
   p->dab_count += s->n;
+  if (!rc && sync && p->type == PBVH_GRIDS) {
+    rc = sg_grids_fetch_last(p);
+    p->last_tool = s->dabs[s->n - 1].sculpt_tool;
+    p->last_learned = false;
+    p->host_bb_stale = true;
+    return rc;
+  }
   if (!rc && (sync || s->learned)) {  // the stream is idle: record the last dab's counters
     const SgDabParams &last = s->dabs[s->n - 1];
     rc = sg_check_hip(hipMemcpy(p->last_ctr, p->d.ctr + (size_t)((s->n - 1) & 1) * SG_CTR_COUNT,

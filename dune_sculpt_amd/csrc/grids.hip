@@ -53,6 +53,7 @@ struct GView {  // kernel view of SgGridDev (by value)
   int *face_stamp, *edge_stamp, *vert_stamp;
   int *gathered, *dfaces, *ditems;
   long long *ctr, *ctr_last;
+  int *dser;  // [0] the running operation's serial, [1] the last dab's
 };
 
 GView view_of(const SgGridDev &d) {
@@ -114,6 +115,7 @@ GView view_of(const SgGridDev &d) {
   v.ditems = d.ditems;
   v.ctr = d.ctr;
   v.ctr_last = d.ctr_last;
+  v.dser = d.dser;
   return v;
 }
 
@@ -134,7 +136,8 @@ __device__ __forceinline__ bool chunk_range(const GView &g, int it, int ng, int 
 // search: a thread per leaf (spec S1 on vb, orig_vb for Grab; fully hidden leaves
 // skipped); every gathered leaf is marked for the refit
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(GR_THREADS) void kg_search(GView g, SgDabK k, int serial) {
+__global__ __launch_bounds__(GR_THREADS) void kg_search(GView g, SgDabK k) {
+  const int serial = g.dser[0];
   const int l = blockIdx.x * blockDim.x + threadIdx.x;
   const int lc = min(l, g.L - 1);
   const BB6 b = gbb_load((k.use_orig ? g.leaf_obb : g.leaf_bb) + 8 * (size_t)lc);
@@ -189,8 +192,8 @@ __global__ __launch_bounds__(GR_THREADS) void kg_area(GView g, SgDabK k, int res
 // brush: spec S2 test, S3/S4 factor, S6 Draw / S7 Inflate / S9 Grab written to both
 // position copies; Smooth keeps the factor for the Jacobi steps
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(GR_THREADS) void kg_brush(GView g, SgDabK k, const float *curve, int serial,
-                                                       int respect_hide) {
+__global__ __launch_bounds__(GR_THREADS) void kg_brush(GView g, SgDabK k, const float *curve, int respect_hide) {
+  const int serial = g.dser[0];
   const int ng = (int)g.ctr[SGG_NGATHER];
   float ox = 0.0f, oy = 0.0f, oz = 0.0f;
   if (k.tool == SG_TOOL_DRAW) {
@@ -251,7 +254,8 @@ __global__ __launch_bounds__(GR_THREADS) void kg_brush(GView g, SgDabK k, const 
 // One Jacobi step of spec S8: src -> dst for this dab's affected elements; the others
 // hold the same values in both copies.
 __global__ __launch_bounds__(GR_THREADS) void kg_smooth(GView g, const float *sx, const float *sy, const float *sz,
-                                                        float *dx, float *dy, float *dz, int serial) {
+                                                        float *dx, float *dy, float *dz) {
+  const int serial = g.dser[0];
   const int ng = (int)g.ctr[SGG_NGATHER];
   for (int it = blockIdx.x; it < ng * g.nchunk; it += gridDim.x) {
     int b, e;
@@ -283,7 +287,8 @@ __global__ __launch_bounds__(GR_THREADS) void kg_smooth(GView g, const float *sx
 }
 
 __global__ __launch_bounds__(GR_THREADS) void kg_copy_aff(GView g, const float *sx, const float *sy, const float *sz,
-                                                          float *dx, float *dy, float *dz, int serial) {
+                                                          float *dx, float *dy, float *dz) {
+  const int serial = g.dser[0];
   const int ng = (int)g.ctr[SGG_NGATHER];
   for (int it = blockIdx.x; it < ng * g.nchunk; it += gridDim.x) {
     int b, e;
@@ -299,7 +304,8 @@ __global__ __launch_bounds__(GR_THREADS) void kg_copy_aff(GView g, const float *
 
 // Faces of the grids of the listed leaves (the gathered ones when leaves == null),
 // each once (BKE_pbvh_get_grid_updates, pbvh.c:1605-1640).
-__global__ __launch_bounds__(GR_THREADS) void kg_mark_faces(GView g, const int *leaves, int nleaves, int serial) {
+__global__ __launch_bounds__(GR_THREADS) void kg_mark_faces(GView g, const int *leaves, int nleaves) {
+  const int serial = g.dser[0];
   const int n = leaves ? nleaves : (int)g.ctr[SGG_NGATHER];
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
     const int l = leaves ? leaves[i] : g.gathered[i];
@@ -356,7 +362,8 @@ __device__ __forceinline__ void avg_pair(const GView &g, int a, int b) {  // sub
 // the seams between its grids and the shared centre (:951-984); its edges and coarse
 // vertices are queued once each for the boundary and corner averages, and the leaves
 // of its grids marked for the refit.
-__global__ __launch_bounds__(GR_THREADS) void kg_face_normals(GView g, int serial) {
+__global__ __launch_bounds__(GR_THREADS) void kg_face_normals(GView g) {
+  const int serial = g.dser[0];
   const int nf = (int)g.ctr[SGG_NFACES];
   const int gs = g.gs, g1 = gs - 1, ga = g.ga;
   for (int i = blockIdx.x; i < nf; i += gridDim.x) {
@@ -459,7 +466,8 @@ __global__ __launch_bounds__(GR_THREADS) void kg_face_normals(GView g, int seria
 // A wave per queued edge or coarse vertex: the mean over its adjacent faces written
 // back to each (subdiv_ccg.c:1010-1048 boundaries, :1081-1104 corners); the leaves of
 // the written elements are marked for the refit.
-__global__ __launch_bounds__(GR_THREADS) void kg_edges_verts(GView g, int serial) {
+__global__ __launch_bounds__(GR_THREADS) void kg_edges_verts(GView g) {
+  const int serial = g.dser[0];
   const int ni = (int)g.ctr[SGG_NITEMS];
   const int lane = lane_id();
   const int w = (blockIdx.x * blockDim.x + threadIdx.x) / SG_WAVE;
@@ -541,7 +549,8 @@ __global__ __launch_bounds__(GR_THREADS) void kg_edges_verts(GView g, int serial
 }
 
 // Leaves listed by the host (drop-in update paths): marked for the refit.
-__global__ __launch_bounds__(GR_THREADS) void kg_mark_leaves(GView g, const int *leaves, int n, int serial) {
+__global__ __launch_bounds__(GR_THREADS) void kg_mark_leaves(GView g, const int *leaves, int n) {
+  const int serial = g.dser[0];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) g.leaf_mark[leaves[i]] = serial;
 }
@@ -550,7 +559,8 @@ __global__ __launch_bounds__(GR_THREADS) void kg_mark_leaves(GView g, const int 
 // (update_node_vb, pbvh.c:108-128; min_ff / max_ff keep the later of equal values, so
 // the fold is order-preserving: each thread folds a contiguous run, then the runs are
 // combined left to right); orig_vb = vb when set_orig.
-__global__ __launch_bounds__(GR_THREADS) void kg_refit_leaves(GView g, int serial, int set_orig) {
+__global__ __launch_bounds__(GR_THREADS) void kg_refit_leaves(GView g, int set_orig) {
+  const int serial = g.dser[0];
   __shared__ float s_bb[GR_THREADS][6];
   for (int l = blockIdx.x; l < g.L; l += gridDim.x) {
     if (g.leaf_mark[l] != serial) continue;
@@ -595,7 +605,8 @@ __device__ __forceinline__ BB6 child_box(const GView &g, int c) {
 // One workgroup: internal nodes height by height, a node with a refit child becomes
 // the union of its two children (pbvh_flush_bb, pbvh.c:1369-1399); then the dab's
 // counters are kept and cleared for the next dab.
-__global__ __launch_bounds__(1024) void kg_refit_nodes(GView g, int serial, int set_orig, int keep_ctr) {
+__global__ __launch_bounds__(1024) void kg_refit_nodes(GView g, int set_orig, int keep_ctr, long long *log) {
+  const int serial = g.dser[0];
   for (int h = 0; h < g.H; h++) {
     for (int i = g.lvl_off[h] + threadIdx.x; i < g.lvl_off[h + 1]; i += blockDim.x) {
       const int n = g.lvl_node[i];
@@ -611,14 +622,32 @@ __global__ __launch_bounds__(1024) void kg_refit_nodes(GView g, int serial, int 
     }
     __syncthreads();
   }
+  long long v = 0;
+  if (keep_ctr && threadIdx.x < SGG_CTR_COUNT) v = g.ctr[threadIdx.x];
+  __syncthreads();
   if (keep_ctr && threadIdx.x < SGG_CTR_COUNT) {
-    g.ctr_last[threadIdx.x] = g.ctr[threadIdx.x];
+    g.ctr_last[threadIdx.x] = v;
     g.ctr[threadIdx.x] = 0;
+  }
+  __syncthreads();
+  if (log && threadIdx.x < SG_CTR_COUNT) {  // the stroke log row, in the faces counters' layout
+    const int t = threadIdx.x;
+    const int src = t == SG_CTR_NGATHER ? SGG_NGATHER : t == SG_CTR_NAFF ? SGG_NAFF : t == SG_CTR_AREA_X ? SGG_AREA_X
+                  : t == SG_CTR_AREA_Y ? SGG_AREA_Y : t == SG_CTR_AREA_Z ? SGG_AREA_Z : t == SG_CTR_NAREA ? SGG_NAREA
+                  : -1;
+    log[t] = src >= 0 ? g.ctr_last[src] : 0;
   }
 }
 
-__global__ void kg_clear_ctr(GView g) {
+// Start of an operation (dab or drop-in update): zero the counters and take the next
+// serial on the device, so a captured stroke replays with fresh serials.
+__global__ void kg_clear_ctr(GView g, int is_dab) {
   if (threadIdx.x < SGG_CTR_COUNT) g.ctr[threadIdx.x] = 0;
+  if (threadIdx.x == 0) {
+    const int s = g.dser[0] + 1;
+    g.dser[0] = s;
+    if (is_dab) g.dser[1] = s;
+  }
 }
 
 template <class T>
@@ -683,6 +712,8 @@ void sg_grids_free_device(PBVH *p) {
   gfree(d.ctr);
   gfree(d.ctr_last);
   gfree(d.hlist);
+  gfree(d.dser);
+  if (d.snap) hipFree(d.snap);
   delete p->gd;
   p->gd = nullptr;
 }
@@ -806,6 +837,7 @@ int sg_grids_build_device(PBVH *p) {
   rc |= gzeros(&d.ctr_last, (size_t)SGG_CTR_COUNT);
   rc |= gzeros(&d.curve, (size_t)(SG_CM_TABLE + 1));
   rc |= gzeros(&d.hlist, (size_t)d.L);
+  rc |= gzeros(&d.dser, (size_t)2);
   return rc ? -1 : 0;
 }
 
@@ -882,48 +914,46 @@ static int launch_check(const char *what) { return sg_check_hip(hipGetLastError(
 
 // normals and seams of the faces of the listed leaves (the gathered ones when null),
 // then nothing else: the refit comes after
-static int enqueue_normals(PBVH *p, const GView &g, const int *leaves, int n, int serial) {
+static int enqueue_normals(PBVH *p, const GView &g, const int *leaves, int n) {
   const SgGridDev &d = *p->gd;
   const int nl = leaves ? n : d.L;
   if (nl <= 0) return 0;
-  kg_mark_faces<<<std::min(nl, 1024), GR_THREADS, 0, p->stream>>>(g, leaves, n, serial);
-  kg_face_normals<<<std::max(1, std::min(d.F, 2048)), GR_THREADS, 0, p->stream>>>(g, serial);
-  kg_edges_verts<<<std::max(1, std::min((d.Ed + d.Vc + 3) / 4, 2048)), GR_THREADS, 0, p->stream>>>(g, serial);
+  kg_mark_faces<<<std::min(nl, 1024), GR_THREADS, 0, p->stream>>>(g, leaves, n);
+  kg_face_normals<<<std::max(1, std::min(d.F, 2048)), GR_THREADS, 0, p->stream>>>(g);
+  kg_edges_verts<<<std::max(1, std::min((d.Ed + d.Vc + 3) / 4, 2048)), GR_THREADS, 0, p->stream>>>(g);
   return launch_check("grid normals");
 }
 
-int sg_grids_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev) {
+int sg_grids_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long long *log, int log_index) {
   if (!p->gd) return sg_set_error("grids: no device layout");
   SgGridDev &d = *p->gd;
   if (!d.topo_of) return sg_set_error("grids: no SubdivCCG attached (DUNE_pbvh_subdiv_cgg_set)");
   if (k.tool != SG_TOOL_DRAW && k.tool != SG_TOOL_INFLATE && k.tool != SG_TOOL_SMOOTH && k.tool != SG_TOOL_GRAB)
     return sg_set_error("grids: tool %d is not supported on PBVH_GRIDS (Draw, Inflate, Smooth, Grab)", k.tool);
   if (k.learned || k.orig_co) return sg_set_error("grids: learned Smooth and original-coordinate Grab are faces only");
-  const int serial = ++d.serial;
-  d.dab_serial = serial;
   const GView g = view_of(d);
   const int rh = p->respect_hide ? 1 : 0;
   const int eb = elem_blocks(d);
-  kg_clear_ctr<<<1, 64, 0, p->stream>>>(g);  // the list counters bound every write below
-  kg_search<<<(d.L + GR_THREADS - 1) / GR_THREADS, GR_THREADS, 0, p->stream>>>(g, k, serial);
+  kg_clear_ctr<<<1, 64, 0, p->stream>>>(g, 1);  // the list counters bound every write below
+  kg_search<<<(d.L + GR_THREADS - 1) / GR_THREADS, GR_THREADS, 0, p->stream>>>(g, k);
   if (k.tool == SG_TOOL_DRAW) kg_area<<<eb, GR_THREADS, 0, p->stream>>>(g, k, rh);
-  kg_brush<<<eb, GR_THREADS, 0, p->stream>>>(g, k, curve_dev, serial, rh);
+  kg_brush<<<eb, GR_THREADS, 0, p->stream>>>(g, k, curve_dev, rh);
   if (k.tool == SG_TOOL_SMOOTH) {
     for (int it = 0; it < k.iters; it++) {
       if (it & 1)
-        kg_smooth<<<eb, GR_THREADS, 0, p->stream>>>(g, d.qx, d.qy, d.qz, d.px, d.py, d.pz, serial);
+        kg_smooth<<<eb, GR_THREADS, 0, p->stream>>>(g, d.qx, d.qy, d.qz, d.px, d.py, d.pz);
       else
-        kg_smooth<<<eb, GR_THREADS, 0, p->stream>>>(g, d.px, d.py, d.pz, d.qx, d.qy, d.qz, serial);
+        kg_smooth<<<eb, GR_THREADS, 0, p->stream>>>(g, d.px, d.py, d.pz, d.qx, d.qy, d.qz);
     }
     if (k.iters & 1)
-      kg_copy_aff<<<eb, GR_THREADS, 0, p->stream>>>(g, d.qx, d.qy, d.qz, d.px, d.py, d.pz, serial);
+      kg_copy_aff<<<eb, GR_THREADS, 0, p->stream>>>(g, d.qx, d.qy, d.qz, d.px, d.py, d.pz);
     else
-      kg_copy_aff<<<eb, GR_THREADS, 0, p->stream>>>(g, d.px, d.py, d.pz, d.qx, d.qy, d.qz, serial);
+      kg_copy_aff<<<eb, GR_THREADS, 0, p->stream>>>(g, d.px, d.py, d.pz, d.qx, d.qy, d.qz);
   }
   if (p->gridkey.has_normals)
-    if (int rc = enqueue_normals(p, g, nullptr, 0, serial)) return rc;
-  kg_refit_leaves<<<std::max(1, std::min(d.L, 2048)), GR_THREADS, 0, p->stream>>>(g, serial, 1);
-  kg_refit_nodes<<<1, 1024, 0, p->stream>>>(g, serial, 1, 1);
+    if (int rc = enqueue_normals(p, g, nullptr, 0)) return rc;
+  kg_refit_leaves<<<std::max(1, std::min(d.L, 2048)), GR_THREADS, 0, p->stream>>>(g, 1);
+  kg_refit_nodes<<<1, 1024, 0, p->stream>>>(g, 1, 1, log ? log + (size_t)log_index * SG_CTR_COUNT : nullptr);
   return launch_check("grid dab");
 }
 
@@ -931,11 +961,9 @@ int sg_grids_enqueue_update_normals(PBVH *p, const int *leaves_dev, int n) {
   if (!p->gd || !p->gd->topo_of) return sg_set_error("grids: no SubdivCCG attached");
   SgGridDev &d = *p->gd;
   if (!p->gridkey.has_normals || n <= 0) return 0;
-  const int serial = ++d.serial;
   const GView g = view_of(d);
-  kg_clear_ctr<<<1, 64, 0, p->stream>>>(g);
-  if (int rc = enqueue_normals(p, g, leaves_dev, n, serial)) return rc;
-  kg_clear_ctr<<<1, 64, 0, p->stream>>>(g);  // the next dab starts from zero
+  kg_clear_ctr<<<1, 64, 0, p->stream>>>(g, 0);
+  if (int rc = enqueue_normals(p, g, leaves_dev, n)) return rc;
   return launch_check("grid normals");
 }
 
@@ -943,11 +971,11 @@ int sg_grids_enqueue_refit(PBVH *p, const int *leaves_dev, int n) {
   if (!p->gd) return sg_set_error("grids: no device layout");
   SgGridDev &d = *p->gd;
   if (n <= 0) return 0;
-  const int serial = ++d.serial;
   const GView g = view_of(d);
-  kg_mark_leaves<<<(n + GR_THREADS - 1) / GR_THREADS, GR_THREADS, 0, p->stream>>>(g, leaves_dev, n, serial);
-  kg_refit_leaves<<<std::max(1, std::min(d.L, 2048)), GR_THREADS, 0, p->stream>>>(g, serial, 1);
-  kg_refit_nodes<<<1, 1024, 0, p->stream>>>(g, serial, 1, 0);
+  kg_clear_ctr<<<1, 64, 0, p->stream>>>(g, 0);
+  kg_mark_leaves<<<(n + GR_THREADS - 1) / GR_THREADS, GR_THREADS, 0, p->stream>>>(g, leaves_dev, n);
+  kg_refit_leaves<<<std::max(1, std::min(d.L, 2048)), GR_THREADS, 0, p->stream>>>(g, 1);
+  kg_refit_nodes<<<1, 1024, 0, p->stream>>>(g, 1, 0, nullptr);
   return launch_check("grid refit");
 }
 
@@ -1038,4 +1066,43 @@ int sg_grids_read_stamps(PBVH *p, int *stamp) {
   const SgGridDev &d = *p->gd;
   if (sg_check_hip(hipStreamSynchronize(p->stream), "sync")) return -1;
   return sg_check_hip(hipMemcpy(stamp, d.stamp, (size_t)d.E * 4, hipMemcpyDeviceToHost), "copy");
+}
+
+// Device checkpoint of the mutable state (elements, every box); slot per PBVH.
+int sg_grids_snapshot(PBVH *p, int restore) {
+  SgGridDev &d = *p->gd;
+  const size_t E = (size_t)d.E;
+  struct Part {
+    void *dev;
+    size_t bytes;
+  } parts[] = {{d.px, E * 4}, {d.py, E * 4}, {d.pz, E * 4}, {d.nx, E * 4}, {d.ny, E * 4}, {d.nz, E * 4},
+               {d.mask, d.mask ? E * 4 : 0}, {d.leaf_bb, (size_t)d.L * 32}, {d.leaf_obb, (size_t)d.L * 32},
+               {d.node_bb, (size_t)d.N * 32}, {d.node_obb, (size_t)d.N * 32}};
+  const int np = sizeof(parts) / sizeof(parts[0]);
+  if (!restore && !d.snap) {
+    size_t tot = 0;
+    for (const Part &q : parts) tot += q.bytes;
+    if (sg_check_hip(hipMalloc(&d.snap, tot), "hipMalloc")) return -1;
+  }
+  if (!d.snap) return sg_set_error("no grids snapshot saved");
+  size_t off = 0;
+  for (int i = 0; i < np; i++) {
+    if (!parts[i].bytes) continue;
+    unsigned char *s = (unsigned char *)d.snap + off;
+    if (restore) {
+      if (sg_check_hip(hipMemcpyAsync(parts[i].dev, s, parts[i].bytes, hipMemcpyDeviceToDevice, p->stream), "copy"))
+        return -1;
+    } else {
+      if (sg_check_hip(hipMemcpyAsync(s, parts[i].dev, parts[i].bytes, hipMemcpyDeviceToDevice, p->stream), "copy"))
+        return -1;
+    }
+    off += parts[i].bytes;
+  }
+  if (restore) {  // the shadow copy equals the positions between dabs
+    const float *src[3] = {d.px, d.py, d.pz};
+    float *dst[3] = {d.qx, d.qy, d.qz};
+    for (int i = 0; i < 3; i++)
+      if (sg_check_hip(hipMemcpyAsync(dst[i], src[i], E * 4, hipMemcpyDeviceToDevice, p->stream), "copy")) return -1;
+  }
+  return sg_check_hip(hipStreamSynchronize(p->stream), "snapshot");
 }

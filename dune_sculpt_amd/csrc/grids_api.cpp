@@ -215,6 +215,16 @@ int sg_grids_dab_apply(PBVH *p, const SgDabParams *dab) {
     curve = p->gd->curve;
   }
   if (int rc = sg_grids_enqueue_dab(p, k, curve)) return rc;
+  if (int rc = sg_grids_fetch_last(p)) return rc;
+  p->last_tool = dab->sculpt_tool;
+  p->last_learned = false;
+  p->dab_count++;
+  p->host_bb_stale = true;
+  return 0;
+}
+
+// the last dab's counters (after the stream drains), in the faces counters' layout
+int sg_grids_fetch_last(PBVH *p) {
   long long ctr[SGG_CTR_COUNT];
   if (sg_check_hip(hipMemcpyAsync(ctr, p->gd->ctr_last, sizeof(ctr), hipMemcpyDeviceToHost, p->stream), "copy"))
     return -1;
@@ -226,23 +236,21 @@ int sg_grids_dab_apply(PBVH *p, const SgDabParams *dab) {
   p->last_ctr[SG_CTR_AREA_Y] = ctr[SGG_AREA_Y];
   p->last_ctr[SG_CTR_AREA_Z] = ctr[SGG_AREA_Z];
   p->last_ctr[SG_CTR_NAREA] = ctr[SGG_NAREA];
-  p->last_tool = dab->sculpt_tool;
-  p->last_learned = false;
-  p->dab_count++;
-  p->host_bb_stale = true;
   return 0;
 }
 
 int sg_grids_last_affected(PBVH *p, int *out, int cap) {
   const SgGridDev &d = *p->gd;
   std::vector<int> stamp((size_t)d.E), grid_old(d.G);
+  int ser[2] = {0, 0};
   if (sg_grids_read_stamps(p, stamp.data())) return -1;
+  if (sg_check_hip(hipMemcpy(ser, d.dser, sizeof(ser), hipMemcpyDeviceToHost), "copy")) return -1;
   if (sg_check_hip(hipMemcpy(grid_old.data(), d.grid_old, d.G * sizeof(int), hipMemcpyDeviceToHost), "copy"))
     return -1;
   std::vector<int> ids;
   for (int pos = 0; pos < d.G; pos++)
     for (int j = 0; j < d.ga; j++)
-      if (stamp[(size_t)pos * d.ga + j] == d.dab_serial) ids.push_back(grid_old[pos] * d.ga + j);
+      if (stamp[(size_t)pos * d.ga + j] == ser[1]) ids.push_back(grid_old[pos] * d.ga + j);
   std::sort(ids.begin(), ids.end());
   for (size_t i = 0; i < ids.size() && (int)i < cap; i++) out[i] = ids[i];
   return (int)ids.size();

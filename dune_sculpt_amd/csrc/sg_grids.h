@@ -89,8 +89,8 @@ struct SgGridDev {
   long long *ctr_last = nullptr;  // the last dab's counters
   float *curve = nullptr;     // CUSTOM falloff table
   int *hlist = nullptr;       // [L] leaf lists from the host (drop-in updates)
-  int serial = 0;             // bumped by every dab and update
-  int dab_serial = 0;         // the last dab's
+  void *snap = nullptr;       // checkpoint of the mutable state (sg_pbvh_snapshot_save)
+  int *dser = nullptr;        // [2] device serials: the running operation's, the last dab's
   const SubdivCCG *topo_of = nullptr;
   int topo_version = -1;
 };
@@ -106,7 +106,9 @@ struct SgDabK;
 int sg_grids_build_device(PBVH *p);
 int sg_grids_attach_topology(PBVH *p, const SubdivCCG *s);
 void sg_grids_free_device(PBVH *p);
-int sg_grids_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev);
+int sg_grids_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long long *log = nullptr,
+                         int log_index = 0);
+int sg_grids_snapshot(PBVH *p, int restore);
 int sg_grids_enqueue_update_normals(PBVH *p, const int *leaves_dev, int n);
 int sg_grids_enqueue_refit(PBVH *p, const int *leaves_dev, int n);
 int sg_grids_download(PBVH *p, float *co, float *no, float *mask);  // element order of the grids (by grid id)
@@ -116,6 +118,7 @@ int sg_grids_read_stamps(PBVH *p, int *stamp);
 // grid branches of the entry points in api.cpp (grids_api.cpp)
 struct SgDabParams;
 int sg_grids_dab_apply(PBVH *p, const SgDabParams *dab);
+int sg_grids_fetch_last(PBVH *p);
 int sg_grids_last_affected(PBVH *p, int *out, int cap);
 int sg_grids_last_gathered(PBVH *p, int *out, int cap);
 int sg_grids_sync_bbs(PBVH *p);

@@ -11,7 +11,7 @@ import ctypes as C
 
 import numpy as np
 
-from . import SgDabParams, _check, _fp, _ip, last_error, lib
+from . import STROKE_GRAPH, STROKE_LOG, PreparedStroke, SgDabParams, _check, _fp, _ip, last_error, lib
 
 EXPORTS = [
     "sg_subdiv_ccg_new", "KERNEL_subdiv_ccg_destroy", "KERNEL_subdiv_ccg_key_top_level",
@@ -264,6 +264,8 @@ class GridPBVH:
         L.DUNE_pbvh_subdiv_cgg_set(self.h, grids.h)
 
     def close(self):
+        for st in getattr(self, "_strokes", []):
+            st.close()
         if getattr(self, "h", None):
             lib().BKE_pbvh_free(self.h)
             self.h = None
@@ -273,6 +275,40 @@ class GridPBVH:
             self.close()
         except Exception:
             pass
+
+    def stroke(self, dabs, graph=True, log=False):
+        arr = (SgDabParams * len(dabs))(*dabs)
+        flags = (STROKE_GRAPH if graph else 0) | (STROKE_LOG if log else 0)
+        _check(lib().sg_dab_apply_stroke(self.h, arr, len(dabs), flags), "sg_dab_apply_stroke")
+
+    def prepare(self, dabs, log=False):
+        """Capture the dab list into a hipGraph; returns a PreparedStroke."""
+        arr = (SgDabParams * len(dabs))(*dabs)
+        h = lib().sg_stroke_prepare(self.h, arr, len(dabs), STROKE_LOG if log else 0)
+        if not h:
+            raise RuntimeError("sg_stroke_prepare failed: " + last_error())
+        st = PreparedStroke(h, self)
+        self._strokes = getattr(self, "_strokes", []) + [st]
+        return st
+
+    def stroke_log(self, n):
+        out = np.zeros((n, 24), np.int64)
+        _check(lib().sg_dab_stroke_log(self.h, out.ctypes.data_as(C.POINTER(C.c_longlong)), n), "stroke log")
+        return out
+
+    def snapshot(self):
+        _check(lib().sg_pbvh_snapshot_save(self.h), "snapshot")
+
+    def restore(self):
+        _check(lib().sg_pbvh_snapshot_restore(self.h), "restore")
+
+    def timer_start(self):
+        _check(lib().sg_pbvh_timer_start(self.h), "timer start")
+
+    def timer_stop(self):
+        ms = C.c_float()
+        _check(lib().sg_pbvh_timer_stop(self.h, C.byref(ms)), "timer stop")
+        return ms.value
 
     def dab(self, d: SgDabParams):
         _check(lib().sg_dab_apply(self.h, C.byref(d)), "sg_dab_apply")
