@@ -618,22 +618,27 @@ def run_grids(sg, W, K, args):
     build_s = time.perf_counter() - t
     dabs, _ = plain_stroke(sg, args.dabs)
     P.snapshot()
-    prep = P.prepare(dabs, log=True)
+    prep = None if args.grids_stream else P.prepare(dabs, log=True)
     times = []
     for s in range(W + max(K, 3)):
         P.restore()
         P.timer_start()
-        prep.launch(sync=False)
+        if prep:
+            prep.launch(sync=False)
+        else:
+            P.stroke(dabs, graph=False, log=True)
         ms = P.timer_stop()
         if s >= W:
             times.append(ms)
     counts = P.stroke_log(len(dabs))[:, 1]
     aff = int(counts.sum())
-    prep.close()
+    if prep:
+        prep.close()
     gpu_ms = float(np.median(times))
     out = {"workload": f"quad sphere {args.grids_n}x{args.grids_n} per side, level {args.grids_level} "
                        f"({g.G} grids of {g.gs}x{g.gs}, {g.E} elements), noise 0.02x4oct, "
                        f"{len(dabs)}-dab Draw/Inflate/Smooth(4)/Grab stroke, leaf limit 10000 elements",
+           "mode": "stream" if args.grids_stream else "graph",
            "build_s": build_s, "leaves": P.leaves, "affected": aff,
            "gpu_ms_per_stroke": gpu_ms, "gpu_us_per_dab": gpu_ms * 1e3 / len(dabs),
            "gpu_element_updates_per_s": aff / (gpu_ms * 1e-3)}
@@ -740,6 +745,7 @@ def main():
     ap.add_argument("--only-grids", action="store_true", help="run only the multires grids stroke")
     ap.add_argument("--grids-n", type=int, default=64, help="coarse quads per cube side of the grids mesh")
     ap.add_argument("--grids-level", type=int, default=4, help="subdivision level of the grids (grid size 2^(l-1)+1)")
+    ap.add_argument("--grids-stream", action="store_true", help="grids stroke as stream launches instead of a graph")
     ap.add_argument("--no-fused", action="store_true", help="separate brush / Smooth / post kernels (A/B)")
     ap.add_argument("--only-elastic", action="store_true", help="run only the elastic deform measurement")
     ap.add_argument("--placed-only", action="store_true",

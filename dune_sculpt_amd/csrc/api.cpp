@@ -740,9 +740,36 @@ int sg_dab_apply_stroke(PBVH *p, const SgDabParams *dabs, int n, int flags) {
       sg_stroke_free(s);
       return rc;
     }
-    for (int i = 0; i < n; i++)  // dab by dab
-      if (int rc = sg_dab_apply(p, &dabs[i])) return rc;
-    return 0;
+    // stream launches: every dab enqueued, one wait at the end
+    for (int i = 0; i < n; i++)
+      if (int rc = check_dab(p, &dabs[i])) return rc;
+    if (!p->gd->topo_of && p->subdiv_ccg)
+      if (int rc = sg_grids_attach_topology(p, p->subdiv_ccg)) return rc;
+    const bool log = (flags & SG_STROKE_LOG) != 0;
+    if (log && ensure_log(p, n)) return -1;
+    int ncustom = 0;
+    for (int i = 0; i < n; i++) ncustom += (dabs[i].curve_preset == SG_CURVE_CUSTOM && dabs[i].curve_table) ? 1 : 0;
+    float *curves = nullptr;
+    if (ncustom && sg_check_hip(hipMalloc(&curves, (size_t)ncustom * (SG_CM_TABLE + 1) * sizeof(float)), "hipMalloc"))
+      return -1;
+    int rc = 0, ci = 0;
+    for (int i = 0; i < n && !rc; i++) {
+      const float *curve = nullptr;
+      if (dabs[i].curve_preset == SG_CURVE_CUSTOM && dabs[i].curve_table) {
+        float *c = curves + (size_t)(ci++) * (SG_CM_TABLE + 1);
+        rc = sg_check_hip(hipMemcpyAsync(c, dabs[i].curve_table, (SG_CM_TABLE + 1) * sizeof(float),
+                                         hipMemcpyHostToDevice, p->stream),
+                          "copy");
+        curve = c;
+      }
+      if (!rc) rc = sg_grids_enqueue_dab(p, sg_make_dab(&dabs[i], 0), curve, log ? p->d.log : nullptr, i);
+    }
+    if (!rc) rc = sg_grids_fetch_last(p);  // waits for the stream
+    if (curves) hipFree(curves);
+    p->last_tool = dabs[n - 1].sculpt_tool;
+    p->dab_count += n;
+    p->host_bb_stale = true;
+    return rc;
   }
   return stroke_apply(p, dabs, nullptr, n, flags);
 }
