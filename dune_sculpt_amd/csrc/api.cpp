@@ -764,6 +764,7 @@ int sg_dab_apply_stroke(PBVH *p, const SgDabParams *dabs, int n, int flags) {
       }
       if (!rc) rc = sg_grids_enqueue_dab(p, sg_make_dab(&dabs[i], 0), curve, log ? p->d.log : nullptr, i);
     }
+    if (!rc) rc = sg_grids_enqueue_flush(p);
     if (!rc) rc = sg_grids_fetch_last(p);  // waits for the stream
     if (curves) hipFree(curves);
     p->last_tool = dabs[n - 1].sculpt_tool;
@@ -877,6 +878,7 @@ static SgStroke *stroke_prepare(PBVH *p, const SgDabParams *dabs, const SgRay *r
         curve = s->curves + (size_t)(ci++) * (SG_CM_TABLE + 1);
       erc = sg_grids_enqueue_dab(p, sg_make_dab(&dabs[i], 0), curve, log ? p->d.log : nullptr, i);
     }
+    if (!erc) erc = sg_grids_enqueue_flush(p);
     const hipError_t e = hipStreamEndCapture(p->stream, &s->graph);
     rc = erc ? erc : sg_check_hip(e, "end capture");
   } else if (!rc) {

@@ -53,7 +53,8 @@ struct GView {  // kernel view of SgGridDev (by value)
   int *face_stamp, *edge_stamp, *vert_stamp;
   int *gathered, *dfaces, *ditems;
   long long *ctr, *ctr_last;
-  int *dser;  // [0] the running operation's serial, [1] the last dab's
+  int *dser;  // [0] the running operation's serial, [1] the last dab's, [2] finished blocks
+  int *leaf_dirty, *node_dirty;  // refit since the last internal-node flush
 };
 
 GView view_of(const SgGridDev &d) {
@@ -116,6 +117,8 @@ GView view_of(const SgGridDev &d) {
   v.ctr = d.ctr;
   v.ctr_last = d.ctr_last;
   v.dser = d.dser;
+  v.leaf_dirty = d.leaf_dirty;
+  v.node_dirty = d.node_dirty;
   return v;
 }
 
@@ -192,7 +195,8 @@ __global__ __launch_bounds__(GR_THREADS) void kg_area(GView g, SgDabK k, int res
 // brush: spec S2 test, S3/S4 factor, S6 Draw / S7 Inflate / S9 Grab written to both
 // position copies; Smooth keeps the factor for the Jacobi steps
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(GR_THREADS) void kg_brush(GView g, SgDabK k, const float *curve, int respect_hide) {
+__global__ __launch_bounds__(GR_THREADS) void kg_brush(GView g, SgDabK k, const float *curve, int respect_hide,
+                                                       int claim_faces) {
   const int serial = g.dser[0];
   const int ng = (int)g.ctr[SGG_NGATHER];
   float ox = 0.0f, oy = 0.0f, oz = 0.0f;
@@ -209,6 +213,18 @@ __global__ __launch_bounds__(GR_THREADS) void kg_brush(GView g, SgDabK k, const 
   for (int it = blockIdx.x; it < ng * g.nchunk; it += gridDim.x) {
     int b, e;
     if (!chunk_range(g, it, ng, b, e)) continue;
+    if (claim_faces && it % g.nchunk == 0) {
+      // the faces of this leaf's grids, each once (BKE_pbvh_get_grid_updates,
+      // pbvh.c:1605-1640): their normals and seams are redone after the brush
+      const int l = g.gathered[it / g.nchunk];
+      for (int p = g.leaf_gbeg[l] + threadIdx.x; p < g.leaf_gbeg[l + 1]; p += GR_THREADS) {
+        const int f = g.grid_face[g.grid_old[p]];
+        if (atomicExch(&g.face_stamp[f], serial) != serial) {
+          const int slot = (int)atomicAdd((unsigned long long *)&g.ctr[SGG_NFACES], 1ull);
+          g.dfaces[slot] = f;
+        }
+      }
+    }
     for (int v = b + threadIdx.x; v < e; v += GR_THREADS) {
       if (g.hide && respect_hide && g.hide[v]) continue;
       const float x = g.px[v], y = g.py[v], z = g.pz[v];
@@ -262,22 +278,37 @@ __global__ __launch_bounds__(GR_THREADS) void kg_smooth(GView g, const float *sx
     if (!chunk_range(g, it, ng, b, e)) continue;
     for (int v = b + threadIdx.x; v < e; v += GR_THREADS) {
       if (g.stamp[v] != serial) continue;
-      const long long nb = g.nb_off[v], ne = g.nb_off[v + 1];
-      const float x = sx[v], y = sy[v], z = sz[v];
+      const int local = v % g.ga, y = local / g.gs, x = local - y * g.gs;
+      const float px = sx[v], py = sy[v], pz = sz[v];
       float ax = 0.0f, ay = 0.0f, az = 0.0f;
-      for (long long q = nb; q < ne; q++) {
-        const int u = g.nb_idx[q];
-        ax = ax + sx[u];
-        ay = ay + sy[u];
-        az = az + sz[u];
+      int cnt;
+      if (x > 0 && y > 0 && x < g.gs - 1 && y < g.gs - 1) {
+        // inside the grid: previous row, next row, previous column, next column
+        // (neighbor_coords_inner_get, subdiv_ccg.c:1870-1880)
+        const int u[4] = {v - g.gs, v + g.gs, v - 1, v + 1};
+        for (int q = 0; q < 4; q++) {
+          ax = ax + sx[u[q]];
+          ay = ay + sy[u[q]];
+          az = az + sz[u[q]];
+        }
+        cnt = 4;
+      } else {
+        const long long nb = g.nb_off[v], ne = g.nb_off[v + 1];
+        for (long long q = nb; q < ne; q++) {
+          const int u = g.nb_idx[q];
+          ax = ax + sx[u];
+          ay = ay + sy[u];
+          az = az + sz[u];
+        }
+        cnt = (int)(ne - nb);
       }
-      float rx = x, ry = y, rz = z;
-      if (ne > nb) {
+      float rx = px, ry = py, rz = pz;
+      if (cnt > 0) {
         const float fade = g.fade[v];
-        const float inv = 1.0f / (float)(ne - nb);
-        rx = x + (ax * inv - x) * fade;
-        ry = y + (ay * inv - y) * fade;
-        rz = z + (az * inv - z) * fade;
+        const float inv = 1.0f / (float)cnt;
+        rx = px + (ax * inv - px) * fade;
+        ry = py + (ay * inv - py) * fade;
+        rz = pz + (az * inv - pz) * fade;
       }
       dx[v] = rx;
       dy[v] = ry;
@@ -337,14 +368,23 @@ __device__ __forceinline__ void quad_normal(const GView &g, int gb, int qx, int 
   sg_normalize(nx, ny, nz);
 }
 
-__device__ __forceinline__ void avg_pair(const GView &g, int a, int b) {  // subdiv_ccg.c:876-900
-  float x = g.px[a] + g.px[b], y = g.py[a] + g.py[b], z = g.pz[a] + g.pz[b];
+// the leaf of element v joins this operation's refit (its position changed)
+__device__ __forceinline__ void mark_moved(const GView &g, int v, int serial) {
+  g.leaf_mark[g.grid_leaf[v / g.ga]] = serial;
+}
+
+__device__ __forceinline__ void avg_pair(const GView &g, int a, int b, int serial) {  // subdiv_ccg.c:876-900
+  const float ax0 = g.px[a], ay0 = g.py[a], az0 = g.pz[a], bx0 = g.px[b], by0 = g.py[b], bz0 = g.pz[b];
+  float x = ax0 + bx0, y = ay0 + by0, z = az0 + bz0;
   x *= 0.5f;
   y *= 0.5f;
   z *= 0.5f;
   g.px[a] = g.px[b] = g.qx[a] = g.qx[b] = x;
   g.py[a] = g.py[b] = g.qy[a] = g.qy[b] = y;
   g.pz[a] = g.pz[b] = g.qz[a] = g.qz[b] = z;
+  const unsigned xb = __float_as_uint(x), yb = __float_as_uint(y), zb = __float_as_uint(z);
+  if (xb != __float_as_uint(ax0) || yb != __float_as_uint(ay0) || zb != __float_as_uint(az0)) mark_moved(g, a, serial);
+  if (xb != __float_as_uint(bx0) || yb != __float_as_uint(by0) || zb != __float_as_uint(bz0)) mark_moved(g, b, serial);
   float u = g.nx[a] + g.nx[b], w = g.ny[a] + g.ny[b], t = g.nz[a] + g.nz[b];
   u *= 0.5f;
   w *= 0.5f;
@@ -358,10 +398,47 @@ __device__ __forceinline__ void avg_pair(const GView &g, int a, int b) {  // sub
   }
 }
 
+// the mean of the listed elements written back to each (element_accumulator_*,
+// subdiv_ccg.c:902-949); leaves of moved elements marked
+__device__ __forceinline__ void avg_list(const GView &g, const int *els, int stride, int n, int serial) {
+  float cx = 0.0f, cy = 0.0f, cz = 0.0f, nx = 0.0f, ny = 0.0f, nz = 0.0f, m = 0.0f;
+  for (int k = 0; k < n; k++) {
+    const int v = els[(size_t)k * stride];
+    cx += g.px[v];
+    cy += g.py[v];
+    cz += g.pz[v];
+    nx += g.nx[v];
+    ny += g.ny[v];
+    nz += g.nz[v];
+    if (g.mask) m += g.mask[v];
+  }
+  const float inv = 1.0f / n;
+  cx *= inv;
+  cy *= inv;
+  cz *= inv;
+  nx *= inv;
+  ny *= inv;
+  nz *= inv;
+  m *= inv;
+  for (int k = 0; k < n; k++) {
+    const int v = els[(size_t)k * stride];
+    if (__float_as_uint(g.px[v]) != __float_as_uint(cx) || __float_as_uint(g.py[v]) != __float_as_uint(cy) ||
+        __float_as_uint(g.pz[v]) != __float_as_uint(cz))
+      mark_moved(g, v, serial);
+    g.px[v] = g.qx[v] = cx;
+    g.py[v] = g.qy[v] = cy;
+    g.pz[v] = g.qz[v] = cz;
+    g.nx[v] = nx;
+    g.ny[v] = ny;
+    g.nz[v] = nz;
+    if (g.mask) g.mask[v] = m;
+  }
+}
+
 // A workgroup per dirty face: its grids' element normals (subdiv_ccg.c:670-740), then
 // the seams between its grids and the shared centre (:951-984); its edges and coarse
-// vertices are queued once each for the boundary and corner averages, and the leaves
-// of its grids marked for the refit.
+// vertices are queued once each for the boundary and corner averages.  Leaves of
+// elements the averages move are marked for the refit.
 __global__ __launch_bounds__(GR_THREADS) void kg_face_normals(GView g) {
   const int serial = g.dser[0];
   const int nf = (int)g.ctr[SGG_NFACES];
@@ -414,9 +491,9 @@ __global__ __launch_bounds__(GR_THREADS) void kg_face_normals(GView g) {
     for (int t = threadIdx.x; t < n * g1; t += GR_THREADS) {
       const int c = t / g1, ii = 1 + (t - c * g1);
       const int prev = g0 + (c + n - 1) % n;
-      avg_pair(g, elem_id(g, prev, ii, 0), elem_id(g, g0 + c, 0, ii));
+      avg_pair(g, elem_id(g, prev, ii, 0), elem_id(g, g0 + c, 0, ii), serial);
     }
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0) {  // the shared centre
       float cx = 0.0f, cy = 0.0f, cz = 0.0f, nx = 0.0f, ny = 0.0f, nz = 0.0f, m = 0.0f;
       for (int c = 0; c < n; c++) {
         const int v = g.grid_pos[g0 + c] * ga;
@@ -438,6 +515,9 @@ __global__ __launch_bounds__(GR_THREADS) void kg_face_normals(GView g) {
       m *= inv;
       for (int c = 0; c < n; c++) {
         const int v = g.grid_pos[g0 + c] * ga;
+        if (__float_as_uint(g.px[v]) != __float_as_uint(cx) || __float_as_uint(g.py[v]) != __float_as_uint(cy) ||
+            __float_as_uint(g.pz[v]) != __float_as_uint(cz))
+          mark_moved(g, v, serial);
         g.px[v] = g.qx[v] = cx;
         g.py[v] = g.qy[v] = cy;
         g.pz[v] = g.qz[v] = cz;
@@ -457,15 +537,14 @@ __global__ __launch_bounds__(GR_THREADS) void kg_face_normals(GView g) {
         const int slot = (int)atomicAdd((unsigned long long *)&g.ctr[SGG_NITEMS], 1ull);
         g.ditems[slot] = g.Ed + vv;
       }
-      g.leaf_mark[g.grid_leaf[g.grid_pos[g0 + c]]] = serial;
     }
     __syncthreads();
   }
 }
 
 // A wave per queued edge or coarse vertex: the mean over its adjacent faces written
-// back to each (subdiv_ccg.c:1010-1048 boundaries, :1081-1104 corners); the leaves of
-// the written elements are marked for the refit.
+// back to each (subdiv_ccg.c:1010-1048 boundaries, :1081-1104 corners); leaves of the
+// elements it moves are marked for the refit.
 __global__ __launch_bounds__(GR_THREADS) void kg_edges_verts(GView g) {
   const int serial = g.dser[0];
   const int ni = (int)g.ctr[SGG_NITEMS];
@@ -478,72 +557,11 @@ __global__ __launch_bounds__(GR_THREADS) void kg_edges_verts(GView g) {
     if (item < g.Ed) {
       const int fb = g.edge_foff[item], nf = g.edge_foff[item + 1] - fb;
       if (nf == 1) continue;
-      const float inv = 1.0f / nf;
-      for (int p = 1 + lane; p < n2 - 1; p += SG_WAVE) {
-        float cx = 0.0f, cy = 0.0f, cz = 0.0f, nx = 0.0f, ny = 0.0f, nz = 0.0f, m = 0.0f;
-        for (int k = 0; k < nf; k++) {
-          const int v = g.edge_el[(size_t)(fb + k) * n2 + p];
-          cx += g.px[v];
-          cy += g.py[v];
-          cz += g.pz[v];
-          nx += g.nx[v];
-          ny += g.ny[v];
-          nz += g.nz[v];
-          if (g.mask) m += g.mask[v];
-        }
-        cx *= inv;
-        cy *= inv;
-        cz *= inv;
-        nx *= inv;
-        ny *= inv;
-        nz *= inv;
-        m *= inv;
-        for (int k = 0; k < nf; k++) {
-          const int v = g.edge_el[(size_t)(fb + k) * n2 + p];
-          g.px[v] = g.qx[v] = cx;
-          g.py[v] = g.qy[v] = cy;
-          g.pz[v] = g.qz[v] = cz;
-          g.nx[v] = nx;
-          g.ny[v] = ny;
-          g.nz[v] = nz;
-          if (g.mask) g.mask[v] = m;
-          g.leaf_mark[g.grid_leaf[v / g.ga]] = serial;
-        }
-      }
+      for (int p = 1 + lane; p < n2 - 1; p += SG_WAVE) avg_list(g, g.edge_el + (size_t)fb * n2 + p, n2, nf, serial);
     } else {
       const int vv = item - g.Ed;
       const int fb = g.vert_foff[vv], nf = g.vert_foff[vv + 1] - fb;
-      if (nf == 1 || lane != 0) continue;
-      float cx = 0.0f, cy = 0.0f, cz = 0.0f, nx = 0.0f, ny = 0.0f, nz = 0.0f, m = 0.0f;
-      for (int k = 0; k < nf; k++) {
-        const int v = g.vert_el[fb + k];
-        cx += g.px[v];
-        cy += g.py[v];
-        cz += g.pz[v];
-        nx += g.nx[v];
-        ny += g.ny[v];
-        nz += g.nz[v];
-        if (g.mask) m += g.mask[v];
-      }
-      const float inv = 1.0f / nf;
-      cx *= inv;
-      cy *= inv;
-      cz *= inv;
-      nx *= inv;
-      ny *= inv;
-      nz *= inv;
-      m *= inv;
-      for (int k = 0; k < nf; k++) {
-        const int v = g.vert_el[fb + k];
-        g.px[v] = g.qx[v] = cx;
-        g.py[v] = g.qy[v] = cy;
-        g.pz[v] = g.qz[v] = cz;
-        g.nx[v] = nx;
-        g.ny[v] = ny;
-        g.nz[v] = nz;
-        if (g.mask) g.mask[v] = m;
-        g.leaf_mark[g.grid_leaf[v / g.ga]] = serial;
-      }
+      if (nf > 1 && lane == 0) avg_list(g, g.vert_el + fb, 1, nf, serial);
     }
   }
 }
@@ -555,98 +573,131 @@ __global__ __launch_bounds__(GR_THREADS) void kg_mark_leaves(GView g, const int 
   if (i < n) g.leaf_mark[leaves[i]] = serial;
 }
 
-// A workgroup per marked leaf: vb = min/max over all its elements in iteration order
-// (update_node_vb, pbvh.c:108-128; min_ff / max_ff keep the later of equal values, so
-// the fold is order-preserving: each thread folds a contiguous run, then the runs are
-// combined left to right); orig_vb = vb when set_orig.
-__global__ __launch_bounds__(GR_THREADS) void kg_refit_leaves(GView g, int set_orig) {
+// A workgroup per marked leaf: vb = min/max over its elements in iteration order
+// (update_node_vb, pbvh.c:108-128).  min_ff / max_ff keep the later of equal values, so
+// the fold is order-preserving: each wave takes a contiguous quarter of the leaf, each
+// lane folds 4 consecutive elements, the wave combines its lanes in order (ordered DPP
+// scan) and folds that after what came before; the waves combine in order.  orig_vb =
+// vb when set_orig.  The leaf stays dirty for the next internal-node flush.  With
+// `finish`, the last workgroup to end closes the dab: its counters kept (and logged in
+// the faces counters' layout) and cleared, the next serial taken.
+__global__ __launch_bounds__(GR_THREADS) void kg_refit_leaves(GView g, int set_orig, int finish, long long *log) {
   const int serial = g.dser[0];
-  __shared__ float s_bb[GR_THREADS][6];
+  constexpr int NW = GR_THREADS / SG_WAVE;
+  __shared__ float s_bb[NW][6];
+  const int lane = lane_id(), w = threadIdx.x / SG_WAVE;
   for (int l = blockIdx.x; l < g.L; l += gridDim.x) {
     if (g.leaf_mark[l] != serial) continue;
     const int b = g.leaf_gbeg[l] * g.ga, e = g.leaf_gbeg[l + 1] * g.ga;
-    const int run = (e - b + GR_THREADS - 1) / GR_THREADS;
+    const int q = (e - b + NW - 1) / NW;
+    const int wb = b + w * q, we = min(e, wb + q);
     BB6 acc;
     bb_reset(acc);
-    const int rb = b + threadIdx.x * run, re = min(e, rb + run);
-    for (int v = rb; v < re; v++) bb_add_point(acc, g.px[v], g.py[v], g.pz[v]);
-    for (int q = 0; q < 3; q++) {
-      s_bb[threadIdx.x][q] = acc.mn[q];
-      s_bb[threadIdx.x][3 + q] = acc.mx[q];
-    }
-    __syncthreads();
-    for (int s = 1; s < GR_THREADS; s *= 2) {
-      if ((threadIdx.x & (2 * s - 1)) == 0) {
-        for (int q = 0; q < 3; q++) {
-          s_bb[threadIdx.x][q] = sg_min(s_bb[threadIdx.x][q], s_bb[threadIdx.x + s][q]);
-          s_bb[threadIdx.x][3 + q] = sg_max(s_bb[threadIdx.x][3 + q], s_bb[threadIdx.x + s][3 + q]);
-        }
+    for (int base = wb; base < we; base += SG_WAVE * 4) {
+      BB6 t;
+      bb_reset(t);
+      const int v0 = base + lane * 4;
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        if (v0 + j < we) bb_add_point(t, g.px[v0 + j], g.py[v0 + j], g.pz[v0 + j]);
+      t = bb_wave_scan(t);
+      BB6 tot;
+      for (int i = 0; i < 3; i++) {
+        tot.mn[i] = __shfl(t.mn[i], SG_WAVE - 1);
+        tot.mx[i] = __shfl(t.mx[i], SG_WAVE - 1);
       }
-      __syncthreads();
+      bb_add_bb(acc, tot);
     }
-    if (threadIdx.x < 8) {
-      const float val = threadIdx.x < 3 ? s_bb[0][threadIdx.x] : (threadIdx.x == 3 || threadIdx.x == 7 ? 0.0f : s_bb[0][threadIdx.x - 1]);
-      g.leaf_bb[8 * (size_t)l + threadIdx.x] = val;
-      if (set_orig) g.leaf_obb[8 * (size_t)l + threadIdx.x] = val;
+    if (lane == 0)
+      for (int i = 0; i < 3; i++) {
+        s_bb[w][i] = acc.mn[i];
+        s_bb[w][3 + i] = acc.mx[i];
+      }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      BB6 r;
+      bb_reset(r);
+      for (int k = 0; k < NW; k++) {
+        BB6 x;
+        for (int i = 0; i < 3; i++) {
+          x.mn[i] = s_bb[k][i];
+          x.mx[i] = s_bb[k][3 + i];
+        }
+        bb_add_bb(r, x);
+      }
+      bb_store(g.leaf_bb + 8 * (size_t)l, r);
+      if (set_orig) bb_store(g.leaf_obb + 8 * (size_t)l, r);
+      g.leaf_dirty[l] = 1;
     }
     __syncthreads();
   }
+  if (!finish) return;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0 && atomicAdd(&g.dser[2], 1) == (int)gridDim.x - 1) {
+    __threadfence();
+    g.dser[2] = 0;
+    long long c[SGG_CTR_COUNT];
+    for (int i = 0; i < SGG_CTR_COUNT; i++) {
+      c[i] = g.ctr[i];
+      g.ctr_last[i] = c[i];
+      g.ctr[i] = 0;
+    }
+    if (log) {
+      for (int t = 0; t < SG_CTR_COUNT; t++) log[t] = 0;
+      log[SG_CTR_NGATHER] = c[SGG_NGATHER];
+      log[SG_CTR_NAFF] = c[SGG_NAFF];
+      log[SG_CTR_AREA_X] = c[SGG_AREA_X];
+      log[SG_CTR_AREA_Y] = c[SGG_AREA_Y];
+      log[SG_CTR_AREA_Z] = c[SGG_AREA_Z];
+      log[SG_CTR_NAREA] = c[SGG_NAREA];
+    }
+    g.dser[1] = serial;
+    g.dser[0] = serial + 1;
+  }
 }
 
-__device__ __forceinline__ bool child_marked(const GView &g, int c, int serial) {
+__device__ __forceinline__ bool child_dirty(const GView &g, int c) {
   const int lf = g.node_leaf[c];
-  return lf >= 0 ? g.leaf_mark[lf] == serial : g.node_mark[c] == serial;
+  return lf >= 0 ? g.leaf_dirty[lf] != 0 : g.node_dirty[c] != 0;
 }
 __device__ __forceinline__ BB6 child_box(const GView &g, int c) {
   const int lf = g.node_leaf[c];
   return lf >= 0 ? gbb_load(g.leaf_bb + 8 * (size_t)lf) : gbb_load(g.node_bb + 8 * (size_t)c);
 }
 
-// One workgroup: internal nodes height by height, a node with a refit child becomes
-// the union of its two children (pbvh_flush_bb, pbvh.c:1369-1399); then the dab's
-// counters are kept and cleared for the next dab.
-__global__ __launch_bounds__(1024) void kg_refit_nodes(GView g, int set_orig, int keep_ctr, long long *log) {
-  const int serial = g.dser[0];
+// One workgroup: internal nodes height by height; a node with a refit child since the
+// last flush becomes the union of its two children (pbvh_flush_bb, pbvh.c:1369-1399),
+// orig_vb following when set_orig; then every dirty flag is cleared.  The per-dab
+// sphere search reads leaf boxes only, so this runs once per stroke or update.
+__global__ __launch_bounds__(1024) void kg_flush_nodes(GView g, int set_orig) {
   for (int h = 0; h < g.H; h++) {
     for (int i = g.lvl_off[h] + threadIdx.x; i < g.lvl_off[h + 1]; i += blockDim.x) {
       const int n = g.lvl_node[i];
       const int c0 = g.node_child[n], c1 = c0 + 1;
-      if (!child_marked(g, c0, serial) && !child_marked(g, c1, serial)) continue;
+      if (!child_dirty(g, c0) && !child_dirty(g, c1)) continue;
       BB6 b;
       bb_reset(b);
       bb_add_bb(b, child_box(g, c0));
       bb_add_bb(b, child_box(g, c1));
       bb_store(g.node_bb + 8 * (size_t)n, b);
       if (set_orig) bb_store(g.node_obb + 8 * (size_t)n, b);
-      g.node_mark[n] = serial;
+      g.node_dirty[n] = 1;
     }
     __syncthreads();
   }
-  long long v = 0;
-  if (keep_ctr && threadIdx.x < SGG_CTR_COUNT) v = g.ctr[threadIdx.x];
-  __syncthreads();
-  if (keep_ctr && threadIdx.x < SGG_CTR_COUNT) {
-    g.ctr_last[threadIdx.x] = v;
-    g.ctr[threadIdx.x] = 0;
-  }
-  __syncthreads();
-  if (log && threadIdx.x < SG_CTR_COUNT) {  // the stroke log row, in the faces counters' layout
-    const int t = threadIdx.x;
-    const int src = t == SG_CTR_NGATHER ? SGG_NGATHER : t == SG_CTR_NAFF ? SGG_NAFF : t == SG_CTR_AREA_X ? SGG_AREA_X
-                  : t == SG_CTR_AREA_Y ? SGG_AREA_Y : t == SG_CTR_AREA_Z ? SGG_AREA_Z : t == SG_CTR_NAREA ? SGG_NAREA
-                  : -1;
-    log[t] = src >= 0 ? g.ctr_last[src] : 0;
-  }
+  for (int i = threadIdx.x; i < g.L; i += blockDim.x) g.leaf_dirty[i] = 0;
+  for (int i = threadIdx.x; i < g.N; i += blockDim.x) g.node_dirty[i] = 0;
 }
 
-// Start of an operation (dab or drop-in update): zero the counters and take the next
-// serial on the device, so a captured stroke replays with fresh serials.
-__global__ void kg_clear_ctr(GView g, int is_dab) {
+// Around a drop-in update (and before a dab after a failed launch): zero the counters
+// and take the next serial on the device.  A dab ends the same way in its last kernel,
+// so a captured stroke replays with fresh serials.
+__global__ void kg_clear_ctr(GView g) {
   if (threadIdx.x < SGG_CTR_COUNT) g.ctr[threadIdx.x] = 0;
   if (threadIdx.x == 0) {
-    const int s = g.dser[0] + 1;
-    g.dser[0] = s;
-    if (is_dab) g.dser[1] = s;
+    g.dser[0] = g.dser[0] + 1;
+    g.dser[2] = 0;
   }
 }
 
@@ -713,6 +764,8 @@ void sg_grids_free_device(PBVH *p) {
   gfree(d.ctr_last);
   gfree(d.hlist);
   gfree(d.dser);
+  gfree(d.leaf_dirty);
+  gfree(d.node_dirty);
   if (d.snap) hipFree(d.snap);
   delete p->gd;
   p->gd = nullptr;
@@ -837,7 +890,13 @@ int sg_grids_build_device(PBVH *p) {
   rc |= gzeros(&d.ctr_last, (size_t)SGG_CTR_COUNT);
   rc |= gzeros(&d.curve, (size_t)(SG_CM_TABLE + 1));
   rc |= gzeros(&d.hlist, (size_t)d.L);
-  rc |= gzeros(&d.dser, (size_t)2);
+  rc |= gzeros(&d.dser, (size_t)4);
+  rc |= gzeros(&d.leaf_dirty, (size_t)d.L);
+  rc |= gzeros(&d.node_dirty, (size_t)d.N);
+  {
+    const int ser0[4] = {1, 0, 0, 0};  // the first operation's serial: above every stamp's 0
+    rc |= sg_check_hip(hipMemcpy(d.dser, ser0, sizeof(ser0), hipMemcpyHostToDevice), "copy");
+  }
   return rc ? -1 : 0;
 }
 
@@ -914,11 +973,11 @@ static int launch_check(const char *what) { return sg_check_hip(hipGetLastError(
 
 // normals and seams of the faces of the listed leaves (the gathered ones when null),
 // then nothing else: the refit comes after
+// normals and seams of the faces of the listed leaves (faces already claimed when
+// leaves is null: the brush claims those of the gathered leaves)
 static int enqueue_normals(PBVH *p, const GView &g, const int *leaves, int n) {
   const SgGridDev &d = *p->gd;
-  const int nl = leaves ? n : d.L;
-  if (nl <= 0) return 0;
-  kg_mark_faces<<<std::min(nl, 1024), GR_THREADS, 0, p->stream>>>(g, leaves, n);
+  if (leaves) kg_mark_faces<<<std::max(1, std::min(n, 1024)), GR_THREADS, 0, p->stream>>>(g, leaves, n);
   kg_face_normals<<<std::max(1, std::min(d.F, 2048)), GR_THREADS, 0, p->stream>>>(g);
   kg_edges_verts<<<std::max(1, std::min((d.Ed + d.Vc + 3) / 4, 2048)), GR_THREADS, 0, p->stream>>>(g);
   return launch_check("grid normals");
@@ -934,10 +993,12 @@ int sg_grids_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long 
   const GView g = view_of(d);
   const int rh = p->respect_hide ? 1 : 0;
   const int eb = elem_blocks(d);
-  kg_clear_ctr<<<1, 64, 0, p->stream>>>(g, 1);  // the list counters bound every write below
+  const bool normals = p->gridkey.has_normals != 0;
+  if (d.ctr_unclean) kg_clear_ctr<<<1, 64, 0, p->stream>>>(g);  // a failed launch left the counters
+  d.ctr_unclean = true;
   kg_search<<<(d.L + GR_THREADS - 1) / GR_THREADS, GR_THREADS, 0, p->stream>>>(g, k);
   if (k.tool == SG_TOOL_DRAW) kg_area<<<eb, GR_THREADS, 0, p->stream>>>(g, k, rh);
-  kg_brush<<<eb, GR_THREADS, 0, p->stream>>>(g, k, curve_dev, rh);
+  kg_brush<<<eb, GR_THREADS, 0, p->stream>>>(g, k, curve_dev, rh, normals ? 1 : 0);
   if (k.tool == SG_TOOL_SMOOTH) {
     for (int it = 0; it < k.iters; it++) {
       if (it & 1)
@@ -950,11 +1011,19 @@ int sg_grids_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long 
     else
       kg_copy_aff<<<eb, GR_THREADS, 0, p->stream>>>(g, d.px, d.py, d.pz, d.qx, d.qy, d.qz);
   }
-  if (p->gridkey.has_normals)
+  if (normals)
     if (int rc = enqueue_normals(p, g, nullptr, 0)) return rc;
-  kg_refit_leaves<<<std::max(1, std::min(d.L, 2048)), GR_THREADS, 0, p->stream>>>(g, 1);
-  kg_refit_nodes<<<1, 1024, 0, p->stream>>>(g, 1, 1, log ? log + (size_t)log_index * SG_CTR_COUNT : nullptr);
-  return launch_check("grid dab");
+  kg_refit_leaves<<<std::max(1, std::min(d.L, 2048)), GR_THREADS, 0, p->stream>>>(
+      g, 1, 1, log ? log + (size_t)log_index * SG_CTR_COUNT : nullptr);
+  if (int rc = launch_check("grid dab")) return rc;
+  d.ctr_unclean = false;
+  return 0;
+}
+
+int sg_grids_enqueue_flush(PBVH *p) {
+  if (!p->gd) return 0;
+  kg_flush_nodes<<<1, 1024, 0, p->stream>>>(view_of(*p->gd), 1);
+  return launch_check("grid flush");
 }
 
 int sg_grids_enqueue_update_normals(PBVH *p, const int *leaves_dev, int n) {
@@ -962,20 +1031,22 @@ int sg_grids_enqueue_update_normals(PBVH *p, const int *leaves_dev, int n) {
   SgGridDev &d = *p->gd;
   if (!p->gridkey.has_normals || n <= 0) return 0;
   const GView g = view_of(d);
-  kg_clear_ctr<<<1, 64, 0, p->stream>>>(g, 0);
+  kg_clear_ctr<<<1, 64, 0, p->stream>>>(g);
   if (int rc = enqueue_normals(p, g, leaves_dev, n)) return rc;
+  kg_clear_ctr<<<1, 64, 0, p->stream>>>(g);
   return launch_check("grid normals");
 }
 
-int sg_grids_enqueue_refit(PBVH *p, const int *leaves_dev, int n) {
+int sg_grids_enqueue_refit(PBVH *p, const int *leaves_dev, int n, int set_orig) {
   if (!p->gd) return sg_set_error("grids: no device layout");
   SgGridDev &d = *p->gd;
   if (n <= 0) return 0;
   const GView g = view_of(d);
-  kg_clear_ctr<<<1, 64, 0, p->stream>>>(g, 0);
+  kg_clear_ctr<<<1, 64, 0, p->stream>>>(g);
   kg_mark_leaves<<<(n + GR_THREADS - 1) / GR_THREADS, GR_THREADS, 0, p->stream>>>(g, leaves_dev, n);
-  kg_refit_leaves<<<std::max(1, std::min(d.L, 2048)), GR_THREADS, 0, p->stream>>>(g, 1);
-  kg_refit_nodes<<<1, 1024, 0, p->stream>>>(g, 1, 0, nullptr);
+  kg_refit_leaves<<<std::max(1, std::min(d.L, 2048)), GR_THREADS, 0, p->stream>>>(g, set_orig, 0, nullptr);
+  kg_flush_nodes<<<1, 1024, 0, p->stream>>>(g, set_orig);
+  kg_clear_ctr<<<1, 64, 0, p->stream>>>(g);
   return launch_check("grid refit");
 }
 

@@ -215,6 +215,7 @@ int sg_grids_dab_apply(PBVH *p, const SgDabParams *dab) {
     curve = p->gd->curve;
   }
   if (int rc = sg_grids_enqueue_dab(p, k, curve)) return rc;
+  if (int rc = sg_grids_enqueue_flush(p)) return rc;
   if (int rc = sg_grids_fetch_last(p)) return rc;
   p->last_tool = dab->sculpt_tool;
   p->last_learned = false;
@@ -286,13 +287,16 @@ int sg_grids_sync_bbs(PBVH *p) {
 
 // BKE_pbvh_update_bounds (pbvh.c:1401-1421) on grids: leaves flagged with a bit of
 // `flag` among UpdateBB / UpdateOriginalBB are refit from their elements (orig_vb
-// following vb), their ancestors rebuilt; the flags are cleared.
+// following vb when `flag` holds UpdateOriginalBB), their ancestors rebuilt; the
+// flags are cleared.
 void sg_grids_update_bounds(PBVH *p, int flag) {
   const int f = flag & (PBVH_UpdateBB | PBVH_UpdateOriginalBB);
   if (!f || !p->gd) return;
   std::vector<int> leaves = flagged_leaves(p, f);
   if (!leaves.empty()) {
-    if (upload_list(p, leaves) || sg_grids_enqueue_refit(p, p->gd->hlist, (int)leaves.size())) return;
+    if (upload_list(p, leaves) ||
+        sg_grids_enqueue_refit(p, p->gd->hlist, (int)leaves.size(), (flag & PBVH_UpdateOriginalBB) ? 1 : 0))
+      return;
     if (sg_check_hip(hipStreamSynchronize(p->stream), "grid refit")) return;
     p->host_bb_stale = true;
   }

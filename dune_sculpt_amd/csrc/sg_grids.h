@@ -90,7 +90,9 @@ struct SgGridDev {
   float *curve = nullptr;     // CUSTOM falloff table
   int *hlist = nullptr;       // [L] leaf lists from the host (drop-in updates)
   void *snap = nullptr;       // checkpoint of the mutable state (sg_pbvh_snapshot_save)
-  int *dser = nullptr;        // [2] device serials: the running operation's, the last dab's
+  int *dser = nullptr;        // [4] device serials: the running operation's, the last dab's; blocks done
+  int *leaf_dirty = nullptr, *node_dirty = nullptr;  // refit since the last internal-node flush
+  bool ctr_unclean = false;   // a dab's launches failed midway: clear the counters first
   const SubdivCCG *topo_of = nullptr;
   int topo_version = -1;
 };
@@ -110,7 +112,8 @@ int sg_grids_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long 
                          int log_index = 0);
 int sg_grids_snapshot(PBVH *p, int restore);
 int sg_grids_enqueue_update_normals(PBVH *p, const int *leaves_dev, int n);
-int sg_grids_enqueue_refit(PBVH *p, const int *leaves_dev, int n);
+int sg_grids_enqueue_refit(PBVH *p, const int *leaves_dev, int n, int set_orig);
+int sg_grids_enqueue_flush(PBVH *p);  // internal nodes above the leaves refit since the last flush
 int sg_grids_download(PBVH *p, float *co, float *no, float *mask);  // element order of the grids (by grid id)
 int sg_grids_upload(PBVH *p, const float *co, const float *no, const float *mask);
 int sg_grids_read_boxes(PBVH *p, float *leaf_bb, float *node_bb);

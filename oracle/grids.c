@@ -10,7 +10,10 @@
  *   normals update on grids                      pbvh.c:4559-4587, :1605-1640
  *   bounds refit                                 pbvh.c:108-128, :1369-1421
  *   the dab                                      the faces dab's spec (oracle.c orc_dab)
- *                                                over grid elements
+ *                                                over grid elements; the averages of the
+ *                                                normals update also move elements of
+ *                                                leaves the dab did not gather, and those
+ *                                                leaves are refit too
  *
  * Coarse topology conventions (OpenSubdiv's refiner in the reference, absent there):
  * edges numbered in order of first use by the polygons, each with the vertex order of
@@ -465,9 +468,24 @@ static void grid_normals(OrcGrids *o, int g, float *fn)
   }
 }
 
+/* During a dab's normals update: the grids whose elements an average moved (their
+ * leaves are refit with the gathered ones; a leaf none of whose elements moved keeps
+ * its box).  NULL otherwise. */
+static uint8_t *s_moved;
+
+static void note_moved(const OrcGrids *o, int v, const float old[3])
+{
+  if (s_moved && memcmp(old, o->co + 3 * v, sizeof(float) * 3) != 0) {
+    s_moved[v / o->ga] = 1;
+  }
+}
+
 static void pair_avg(OrcGrids *o, int a, int b)
 {
   float *ca = o->co + 3 * a, *cb = o->co + 3 * b;
+  float olda[3], oldb[3];
+  memcpy(olda, ca, sizeof(olda));
+  memcpy(oldb, cb, sizeof(oldb));
   for (int i = 0; i < 3; i++) {
     ca[i] = ca[i] + cb[i];
   }
@@ -490,6 +508,8 @@ static void pair_avg(OrcGrids *o, int a, int b)
     o->mask[a] = m;
     o->mask[b] = m;
   }
+  note_moved(o, a, olda);
+  note_moved(o, b, oldb);
 }
 
 /* mean over a list of elements written back to each */
@@ -514,7 +534,10 @@ static void list_avg(OrcGrids *o, const int *els, int n)
   }
   m *= f;
   for (int k = 0; k < n; k++) {
+    float old[3];
+    memcpy(old, o->co + 3 * els[k], sizeof(old));
     memcpy(o->co + 3 * els[k], c, sizeof(c));
+    note_moved(o, els[k], old);
     if (o->has_normal) {
       memcpy(o->no + 3 * els[k], nn, sizeof(nn));
     }
@@ -598,27 +621,24 @@ void orc_grids_recalc_normals(OrcGrids *o)
   orc_grids_average(o);
 }
 
-/* KERNEL_subdiv_ccg_update_normals over the given faces (each listed once); grid_touch
- * (may be NULL) gets 1 for every grid whose elements were written. */
-static void update_normals_faces(OrcGrids *o, const int *faces, int nfaces, uint8_t *grid_touch)
+/* KERNEL_subdiv_ccg_update_normals over the given faces (each listed once); moved (may
+ * be NULL) gets 1 for every grid with an element whose position an average changed. */
+static void update_normals_faces(OrcGrids *o, const int *faces, int nfaces, uint8_t *moved)
 {
   if (!o->has_normal || nfaces == 0) {
     return;
   }
+  s_moved = moved;
   float *fn = (float *)malloc(sizeof(float) * 3 * (size_t)(o->gs - 1) * (size_t)(o->gs - 1));
   for (int i = 0; i < nfaces; i++) {
     const int f = faces[i];
     for (int c = 0; c < o->face_n[f]; c++) {
       grid_normals(o, o->face_start[f] + c, fn);
-      if (grid_touch) {
-        grid_touch[o->face_start[f] + c] = 1;
-      }
     }
     inner_face_avg(o, f);
   }
   free(fn);
   uint8_t *eseen = (uint8_t *)calloc((size_t)o->Ed + 1, 1), *vseen = (uint8_t *)calloc((size_t)o->Vc + 1, 1);
-  const int n2 = 2 * o->gs;
   for (int i = 0; i < nfaces; i++) {
     const int f = faces[i];
     for (int c = 0; c < o->face_n[f]; c++) {
@@ -626,27 +646,16 @@ static void update_normals_faces(OrcGrids *o, const int *faces, int nfaces, uint
       if (!eseen[e]) {
         eseen[e] = 1;
         edge_avg(o, e);
-        if (grid_touch && o->edge_foff[e + 1] - o->edge_foff[e] > 1) {
-          for (int k = o->edge_foff[e]; k < o->edge_foff[e + 1]; k++) {
-            for (int q = 1; q < n2 - 1; q++) {
-              grid_touch[o->edge_bc[(size_t)k * n2 + q].g] = 1;
-            }
-          }
-        }
       }
       if (!vseen[v]) {
         vseen[v] = 1;
         corner_avg(o, v);
-        if (grid_touch && o->vert_foff[v + 1] - o->vert_foff[v] > 1) {
-          for (int k = o->vert_foff[v]; k < o->vert_foff[v + 1]; k++) {
-            grid_touch[o->vert_cc[k].g] = 1;
-          }
-        }
       }
     }
   }
   free(eseen);
   free(vseen);
+  s_moved = NULL;
 }
 
 void orc_grids_update_normals_faces(OrcGrids *o, const int *faces, int nfaces)
@@ -1234,7 +1243,7 @@ int orc_grids_dab(OrcGrids *o, const OrcDab *dab, int *out_affected, int cap)
     }
     uint8_t *touch = (uint8_t *)calloc((size_t)o->G, 1);
     update_normals_faces(o, faces, nf, touch);
-    /* the leaves of every written grid are refit with the gathered ones */
+    /* the leaves of grids an average moved are refit with the gathered ones */
     for (int ni = 0; ni < o->totnode; ni++) {
       ONode *n = &o->nodes[ni];
       if (!(n->flag & ORC_Leaf)) {
