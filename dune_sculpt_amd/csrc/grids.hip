@@ -53,7 +53,7 @@ struct GView {  // kernel view of SgGridDev (by value)
   int *face_stamp, *edge_stamp, *vert_stamp;
   int *gathered, *dfaces, *ditems;
   long long *ctr, *ctr_last;
-  int *dser;  // [0] the running operation's serial, [1] the last dab's, [2] finished blocks
+  int *dser;  // [0] the running operation's serial, [1] the last dab's
   int *leaf_dirty, *node_dirty;  // refit since the last internal-node flush
 };
 
@@ -355,13 +355,14 @@ __device__ __forceinline__ int elem_id(const GView &g, int grid, int x, int y) {
   return g.grid_pos[grid] * g.ga + y * g.gs + x;
 }
 
-// normal_quad_v3 (math_geom.cc:48-67) of grid quad (qx, qy) at base element id gb
-__device__ __forceinline__ void quad_normal(const GView &g, int gb, int qx, int qy, float &nx, float &ny,
+// normal_quad_v3 (math_geom.cc:48-67) of grid quad (qx, qy); P is the grid's
+// positions, xyz interleaved, row-major
+__device__ __forceinline__ void quad_normal(const float *P, int gs, int qx, int qy, float &nx, float &ny,
                                             float &nz) {
-  const int v1 = gb + (qy + 1) * g.gs + qx, v2 = gb + (qy + 1) * g.gs + qx + 1;
-  const int v3 = gb + qy * g.gs + qx + 1, v4 = gb + qy * g.gs + qx;
-  const float a0 = g.px[v1] - g.px[v3], a1 = g.py[v1] - g.py[v3], a2 = g.pz[v1] - g.pz[v3];
-  const float b0 = g.px[v2] - g.px[v4], b1 = g.py[v2] - g.py[v4], b2 = g.pz[v2] - g.pz[v4];
+  const float *v1 = P + 3 * ((qy + 1) * gs + qx), *v2 = P + 3 * ((qy + 1) * gs + qx + 1);
+  const float *v3 = P + 3 * (qy * gs + qx + 1), *v4 = P + 3 * (qy * gs + qx);
+  const float a0 = v1[0] - v3[0], a1 = v1[1] - v3[1], a2 = v1[2] - v3[2];
+  const float b0 = v2[0] - v4[0], b1 = v2[1] - v4[1], b2 = v2[2] - v4[2];
   nx = a1 * b2 - a2 * b1;
   ny = a2 * b0 - a0 * b2;
   nz = a0 * b1 - a1 * b0;
@@ -440,54 +441,64 @@ __device__ __forceinline__ void avg_list(const GView &g, const int *els, int str
 // vertices are queued once each for the boundary and corner averages.  Leaves of
 // elements the averages move are marked for the refit.
 __global__ __launch_bounds__(GR_THREADS) void kg_face_normals(GView g) {
+  extern __shared__ float s_pos[];  // one grid's positions, xyz interleaved
   const int serial = g.dser[0];
   const int nf = (int)g.ctr[SGG_NFACES];
   const int gs = g.gs, g1 = gs - 1, ga = g.ga;
   for (int i = blockIdx.x; i < nf; i += gridDim.x) {
     const int f = g.dfaces[i];
     const int g0 = g.face_gbeg[f], n = g.face_gbeg[f + 1] - g0;
-    for (int t = threadIdx.x; t < n * ga; t += GR_THREADS) {
-      const int c = t / ga, local = t - c * ga;
-      const int y = local / gs, x = local - y * gs;
+    // each grid's positions staged in LDS (one coalesced pass), then its quad normals
+    // read them there
+    for (int c = 0; c < n; c++) {
       const int gb = g.grid_pos[g0 + c] * ga;
-      float ax = 0.0f, ay = 0.0f, az = 0.0f, qx, qy, qz;
-      int counter = 0;
-      if (x < g1 && y < g1) {
-        quad_normal(g, gb, x, y, qx, qy, qz);
-        ax += qx;
-        ay += qy;
-        az += qz;
-        counter++;
+      for (int t = threadIdx.x; t < ga; t += GR_THREADS) {
+        s_pos[3 * t] = g.px[gb + t];
+        s_pos[3 * t + 1] = g.py[gb + t];
+        s_pos[3 * t + 2] = g.pz[gb + t];
       }
-      if (x >= 1) {
-        if (y < g1) {
-          quad_normal(g, gb, x - 1, y, qx, qy, qz);
+      __syncthreads();
+      for (int t = threadIdx.x; t < ga; t += GR_THREADS) {
+        const int y = t / gs, x = t - y * gs;
+        float ax = 0.0f, ay = 0.0f, az = 0.0f, qx, qy, qz;
+        int counter = 0;
+        if (x < g1 && y < g1) {
+          quad_normal(s_pos, gs, x, y, qx, qy, qz);
           ax += qx;
           ay += qy;
           az += qz;
           counter++;
         }
-        if (y >= 1) {
-          quad_normal(g, gb, x - 1, y - 1, qx, qy, qz);
+        if (x >= 1) {
+          if (y < g1) {
+            quad_normal(s_pos, gs, x - 1, y, qx, qy, qz);
+            ax += qx;
+            ay += qy;
+            az += qz;
+            counter++;
+          }
+          if (y >= 1) {
+            quad_normal(s_pos, gs, x - 1, y - 1, qx, qy, qz);
+            ax += qx;
+            ay += qy;
+            az += qz;
+            counter++;
+          }
+        }
+        if (y >= 1 && x < g1) {
+          quad_normal(s_pos, gs, x, y - 1, qx, qy, qz);
           ax += qx;
           ay += qy;
           az += qz;
           counter++;
         }
+        const float inv = 1.0f / (float)counter;
+        g.nx[gb + t] = ax * inv;
+        g.ny[gb + t] = ay * inv;
+        g.nz[gb + t] = az * inv;
       }
-      if (y >= 1 && x < g1) {
-        quad_normal(g, gb, x, y - 1, qx, qy, qz);
-        ax += qx;
-        ay += qy;
-        az += qz;
-        counter++;
-      }
-      const float inv = 1.0f / (float)counter;
-      g.nx[gb + local] = ax * inv;
-      g.ny[gb + local] = ay * inv;
-      g.nz[gb + local] = az * inv;
+      __syncthreads();
     }
-    __syncthreads();
     for (int t = threadIdx.x; t < n * g1; t += GR_THREADS) {
       const int c = t / g1, ii = 1 + (t - c * g1);
       const int prev = g0 + (c + n - 1) % n;
@@ -575,15 +586,15 @@ __global__ __launch_bounds__(GR_THREADS) void kg_mark_leaves(GView g, const int 
 
 // A workgroup per marked leaf: vb = min/max over its elements in iteration order
 // (update_node_vb, pbvh.c:108-128).  min_ff / max_ff keep the later of equal values, so
-// the fold is order-preserving: each wave takes a contiguous quarter of the leaf, each
-// lane folds 4 consecutive elements, the wave combines its lanes in order (ordered DPP
-// scan) and folds that after what came before; the waves combine in order.  orig_vb =
-// vb when set_orig.  The leaf stays dirty for the next internal-node flush.  With
-// `finish`, the last workgroup to end closes the dab: its counters kept (and logged in
-// the faces counters' layout) and cleared, the next serial taken.
-__global__ __launch_bounds__(GR_THREADS) void kg_refit_leaves(GView g, int set_orig, int finish, long long *log) {
+// the fold is order-preserving: each of the 16 waves takes a contiguous sixteenth of
+// the leaf, each lane folds 4 consecutive elements, the wave combines its lanes in
+// order (ordered DPP scan) and folds that after what came before; the waves combine in
+// order.  orig_vb =
+// vb when set_orig.  The leaf stays dirty for the next internal-node flush.
+#define GR_REFIT_THREADS 1024
+__global__ __launch_bounds__(GR_REFIT_THREADS) void kg_refit_leaves(GView g, int set_orig) {
   const int serial = g.dser[0];
-  constexpr int NW = GR_THREADS / SG_WAVE;
+  constexpr int NW = GR_REFIT_THREADS / SG_WAVE;
   __shared__ float s_bb[NW][6];
   const int lane = lane_id(), w = threadIdx.x / SG_WAVE;
   for (int l = blockIdx.x; l < g.L; l += gridDim.x) {
@@ -597,9 +608,17 @@ __global__ __launch_bounds__(GR_THREADS) void kg_refit_leaves(GView g, int set_o
       BB6 t;
       bb_reset(t);
       const int v0 = base + lane * 4;
+      float x[4], y[4], z[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {  // every load issued before any is used (clamped index)
+        const int v = min(v0 + j, we - 1);
+        x[j] = g.px[v];
+        y[j] = g.py[v];
+        z[j] = g.pz[v];
+      }
 #pragma unroll
       for (int j = 0; j < 4; j++)
-        if (v0 + j < we) bb_add_point(t, g.px[v0 + j], g.py[v0 + j], g.pz[v0 + j]);
+        if (v0 + j < we) bb_add_point(t, x[j], y[j], z[j]);
       t = bb_wave_scan(t);
       BB6 tot;
       for (int i = 0; i < 3; i++) {
@@ -631,29 +650,30 @@ __global__ __launch_bounds__(GR_THREADS) void kg_refit_leaves(GView g, int set_o
     }
     __syncthreads();
   }
-  if (!finish) return;
-  __threadfence();
+}
+
+// The end of a dab: its counters kept (and logged in the faces counters' layout) and
+// cleared, the next serial taken (a captured stroke replays with fresh serials).
+__global__ void kg_finish(GView g, long long *log) {
+  const int t = threadIdx.x;
+  long long c = 0;
+  if (t < SGG_CTR_COUNT) c = g.ctr[t];
+  __shared__ long long s_c[SGG_CTR_COUNT];
+  if (t < SGG_CTR_COUNT) {
+    s_c[t] = c;
+    g.ctr_last[t] = c;
+    g.ctr[t] = 0;
+  }
   __syncthreads();
-  if (threadIdx.x == 0 && atomicAdd(&g.dser[2], 1) == (int)gridDim.x - 1) {
-    __threadfence();
-    g.dser[2] = 0;
-    long long c[SGG_CTR_COUNT];
-    for (int i = 0; i < SGG_CTR_COUNT; i++) {
-      c[i] = g.ctr[i];
-      g.ctr_last[i] = c[i];
-      g.ctr[i] = 0;
-    }
-    if (log) {
-      for (int t = 0; t < SG_CTR_COUNT; t++) log[t] = 0;
-      log[SG_CTR_NGATHER] = c[SGG_NGATHER];
-      log[SG_CTR_NAFF] = c[SGG_NAFF];
-      log[SG_CTR_AREA_X] = c[SGG_AREA_X];
-      log[SG_CTR_AREA_Y] = c[SGG_AREA_Y];
-      log[SG_CTR_AREA_Z] = c[SGG_AREA_Z];
-      log[SG_CTR_NAREA] = c[SGG_NAREA];
-    }
-    g.dser[1] = serial;
-    g.dser[0] = serial + 1;
+  if (log && t < SG_CTR_COUNT) {
+    const int src = t == SG_CTR_NGATHER ? SGG_NGATHER : t == SG_CTR_NAFF ? SGG_NAFF : t == SG_CTR_AREA_X ? SGG_AREA_X
+                  : t == SG_CTR_AREA_Y ? SGG_AREA_Y : t == SG_CTR_AREA_Z ? SGG_AREA_Z : t == SG_CTR_NAREA ? SGG_NAREA
+                  : -1;
+    log[t] = src >= 0 ? s_c[src] : 0;
+  }
+  if (t == 0) {
+    g.dser[1] = g.dser[0];
+    g.dser[0] = g.dser[0] + 1;
   }
 }
 
@@ -691,13 +711,11 @@ __global__ __launch_bounds__(1024) void kg_flush_nodes(GView g, int set_orig) {
 }
 
 // Around a drop-in update (and before a dab after a failed launch): zero the counters
-// and take the next serial on the device.  A dab ends the same way in its last kernel,
-// so a captured stroke replays with fresh serials.
+// and take the next serial on the device (a dab ends the same way in kg_finish).
 __global__ void kg_clear_ctr(GView g) {
   if (threadIdx.x < SGG_CTR_COUNT) g.ctr[threadIdx.x] = 0;
   if (threadIdx.x == 0) {
     g.dser[0] = g.dser[0] + 1;
-    g.dser[2] = 0;
   }
 }
 
@@ -720,7 +738,7 @@ void gfree(T *&p) {
 
 int elem_blocks(const SgGridDev &d) {
   const long long items = (long long)d.L * (((long long)d.maxleafg * d.ga + GR_CHUNK - 1) / GR_CHUNK);
-  return (int)std::max(1ll, std::min(items, 4096ll));
+  return (int)std::max(1ll, std::min(items, 512ll));  // a dab gathers a few leaves: blocks stride the items
 }
 
 void free_topology(SgGridDev &d) {
@@ -944,6 +962,8 @@ int sg_grids_attach_topology(PBVH *p, const SubdivCCG *s) {
   d.F = s->num_faces;
   d.Ed = s->num_edges;
   d.Vc = s->num_coarse_verts;
+  if ((size_t)d.ga * 12 > 64 * 1024)
+    return sg_set_error("grids: a grid of %d elements exceeds the 64 KiB of LDS its normals use", d.ga);
   int rc = 0;
   rc |= gupload(&d.nb_off, off.data(), off.size());
   rc |= gupload(&d.nb_idx, idx.data(), idx.size());
@@ -978,8 +998,8 @@ static int launch_check(const char *what) { return sg_check_hip(hipGetLastError(
 static int enqueue_normals(PBVH *p, const GView &g, const int *leaves, int n) {
   const SgGridDev &d = *p->gd;
   if (leaves) kg_mark_faces<<<std::max(1, std::min(n, 1024)), GR_THREADS, 0, p->stream>>>(g, leaves, n);
-  kg_face_normals<<<std::max(1, std::min(d.F, 2048)), GR_THREADS, 0, p->stream>>>(g);
-  kg_edges_verts<<<std::max(1, std::min((d.Ed + d.Vc + 3) / 4, 2048)), GR_THREADS, 0, p->stream>>>(g);
+  kg_face_normals<<<std::max(1, std::min(d.F, 512)), GR_THREADS, (size_t)d.ga * 12, p->stream>>>(g);
+  kg_edges_verts<<<std::max(1, std::min((d.Ed + d.Vc + 3) / 4, 256)), GR_THREADS, 0, p->stream>>>(g);
   return launch_check("grid normals");
 }
 
@@ -1013,8 +1033,8 @@ int sg_grids_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long 
   }
   if (normals)
     if (int rc = enqueue_normals(p, g, nullptr, 0)) return rc;
-  kg_refit_leaves<<<std::max(1, std::min(d.L, 2048)), GR_THREADS, 0, p->stream>>>(
-      g, 1, 1, log ? log + (size_t)log_index * SG_CTR_COUNT : nullptr);
+  kg_refit_leaves<<<std::max(1, std::min(d.L, 256)), GR_REFIT_THREADS, 0, p->stream>>>(g, 1);
+  kg_finish<<<1, 64, 0, p->stream>>>(g, log ? log + (size_t)log_index * SG_CTR_COUNT : nullptr);
   if (int rc = launch_check("grid dab")) return rc;
   d.ctr_unclean = false;
   return 0;
@@ -1044,7 +1064,7 @@ int sg_grids_enqueue_refit(PBVH *p, const int *leaves_dev, int n, int set_orig) 
   const GView g = view_of(d);
   kg_clear_ctr<<<1, 64, 0, p->stream>>>(g);
   kg_mark_leaves<<<(n + GR_THREADS - 1) / GR_THREADS, GR_THREADS, 0, p->stream>>>(g, leaves_dev, n);
-  kg_refit_leaves<<<std::max(1, std::min(d.L, 2048)), GR_THREADS, 0, p->stream>>>(g, set_orig, 0, nullptr);
+  kg_refit_leaves<<<std::max(1, std::min(d.L, 256)), GR_REFIT_THREADS, 0, p->stream>>>(g, set_orig);
   kg_flush_nodes<<<1, 1024, 0, p->stream>>>(g, set_orig);
   kg_clear_ctr<<<1, 64, 0, p->stream>>>(g);
   return launch_check("grid refit");
