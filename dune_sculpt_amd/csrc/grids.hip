@@ -369,9 +369,15 @@ __device__ __forceinline__ void quad_normal(const float *P, int gs, int qx, int 
   sg_normalize(nx, ny, nz);
 }
 
-// the leaf of element v joins this operation's refit (its position changed)
+// the leaf of element v joins this operation's refit (its position changed): listed
+// once, after the gathered leaves (which the search marked)
 __device__ __forceinline__ void mark_moved(const GView &g, int v, int serial) {
-  g.leaf_mark[g.grid_leaf[v / g.ga]] = serial;
+  const int l = g.grid_leaf[v / g.ga];
+  if (g.leaf_mark[l] != serial && atomicExch(&g.leaf_mark[l], serial) != serial) {
+    const int ng = (int)g.ctr[SGG_NGATHER];
+    const int slot = (int)atomicAdd((unsigned long long *)&g.ctr[SGG_NMOVED], 1ull);
+    g.gathered[ng + slot] = l;
+  }
 }
 
 __device__ __forceinline__ void avg_pair(const GView &g, int a, int b, int serial) {  // subdiv_ccg.c:876-900
@@ -440,30 +446,34 @@ __device__ __forceinline__ void avg_list(const GView &g, const int *els, int str
 // the seams between its grids and the shared centre (:951-984); its edges and coarse
 // vertices are queued once each for the boundary and corner averages.  Leaves of
 // elements the averages move are marked for the refit.
-__global__ __launch_bounds__(GR_THREADS) void kg_face_normals(GView g) {
-  extern __shared__ float s_pos[];  // one grid's positions, xyz interleaved
+__global__ __launch_bounds__(GR_THREADS) void kg_face_normals(GView g, int batch) {
+  extern __shared__ float s_pos[];  // `batch` grids' positions, xyz interleaved
   const int serial = g.dser[0];
   const int nf = (int)g.ctr[SGG_NFACES];
   const int gs = g.gs, g1 = gs - 1, ga = g.ga;
   for (int i = blockIdx.x; i < nf; i += gridDim.x) {
     const int f = g.dfaces[i];
     const int g0 = g.face_gbeg[f], n = g.face_gbeg[f + 1] - g0;
-    // each grid's positions staged in LDS (one coalesced pass), then its quad normals
-    // read them there
-    for (int c = 0; c < n; c++) {
-      const int gb = g.grid_pos[g0 + c] * ga;
-      for (int t = threadIdx.x; t < ga; t += GR_THREADS) {
-        s_pos[3 * t] = g.px[gb + t];
-        s_pos[3 * t + 1] = g.py[gb + t];
-        s_pos[3 * t + 2] = g.pz[gb + t];
+    // the face's grids' positions staged in LDS, as many grids at a time as `batch`
+    // (one coalesced pass), then their quad normals read them there
+    for (int c0 = 0; c0 < n; c0 += batch) {
+      const int nc = min(batch, n - c0);
+      for (int t = threadIdx.x; t < nc * ga; t += GR_THREADS) {
+        const int c = t / ga, local = t - c * ga;
+        const int v = g.grid_pos[g0 + c0 + c] * ga + local;
+        s_pos[3 * t] = g.px[v];
+        s_pos[3 * t + 1] = g.py[v];
+        s_pos[3 * t + 2] = g.pz[v];
       }
       __syncthreads();
-      for (int t = threadIdx.x; t < ga; t += GR_THREADS) {
-        const int y = t / gs, x = t - y * gs;
+      for (int t = threadIdx.x; t < nc * ga; t += GR_THREADS) {
+        const int c = t / ga, local = t - c * ga;
+        const int y = local / gs, x = local - y * gs;
+        const float *P = s_pos + 3 * c * ga;
         float ax = 0.0f, ay = 0.0f, az = 0.0f, qx, qy, qz;
         int counter = 0;
         if (x < g1 && y < g1) {
-          quad_normal(s_pos, gs, x, y, qx, qy, qz);
+          quad_normal(P, gs, x, y, qx, qy, qz);
           ax += qx;
           ay += qy;
           az += qz;
@@ -471,14 +481,14 @@ __global__ __launch_bounds__(GR_THREADS) void kg_face_normals(GView g) {
         }
         if (x >= 1) {
           if (y < g1) {
-            quad_normal(s_pos, gs, x - 1, y, qx, qy, qz);
+            quad_normal(P, gs, x - 1, y, qx, qy, qz);
             ax += qx;
             ay += qy;
             az += qz;
             counter++;
           }
           if (y >= 1) {
-            quad_normal(s_pos, gs, x - 1, y - 1, qx, qy, qz);
+            quad_normal(P, gs, x - 1, y - 1, qx, qy, qz);
             ax += qx;
             ay += qy;
             az += qz;
@@ -486,16 +496,17 @@ __global__ __launch_bounds__(GR_THREADS) void kg_face_normals(GView g) {
           }
         }
         if (y >= 1 && x < g1) {
-          quad_normal(s_pos, gs, x, y - 1, qx, qy, qz);
+          quad_normal(P, gs, x, y - 1, qx, qy, qz);
           ax += qx;
           ay += qy;
           az += qz;
           counter++;
         }
         const float inv = 1.0f / (float)counter;
-        g.nx[gb + t] = ax * inv;
-        g.ny[gb + t] = ay * inv;
-        g.nz[gb + t] = az * inv;
+        const int v = g.grid_pos[g0 + c0 + c] * ga + local;
+        g.nx[v] = ax * inv;
+        g.ny[v] = ay * inv;
+        g.nz[v] = az * inv;
       }
       __syncthreads();
     }
@@ -504,39 +515,48 @@ __global__ __launch_bounds__(GR_THREADS) void kg_face_normals(GView g) {
       const int prev = g0 + (c + n - 1) % n;
       avg_pair(g, elem_id(g, prev, ii, 0), elem_id(g, g0 + c, 0, ii), serial);
     }
-    if (threadIdx.x == 0) {  // the shared centre
-      float cx = 0.0f, cy = 0.0f, cz = 0.0f, nx = 0.0f, ny = 0.0f, nz = 0.0f, m = 0.0f;
+    // the shared centre: the grids' centre values gathered in parallel, summed in grid
+    // order by one thread, written back in parallel
+    __shared__ float s_c[7][GR_THREADS];
+    __shared__ float s_mean[7];
+    const bool par = n <= GR_THREADS;
+    if (par && threadIdx.x < n) {
+      const int v = g.grid_pos[g0 + threadIdx.x] * ga;
+      s_c[0][threadIdx.x] = g.px[v];
+      s_c[1][threadIdx.x] = g.py[v];
+      s_c[2][threadIdx.x] = g.pz[v];
+      s_c[3][threadIdx.x] = g.nx[v];
+      s_c[4][threadIdx.x] = g.ny[v];
+      s_c[5][threadIdx.x] = g.nz[v];
+      s_c[6][threadIdx.x] = g.mask ? g.mask[v] : 0.0f;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float a[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
       for (int c = 0; c < n; c++) {
-        const int v = g.grid_pos[g0 + c] * ga;
-        cx += g.px[v];
-        cy += g.py[v];
-        cz += g.pz[v];
-        nx += g.nx[v];
-        ny += g.ny[v];
-        nz += g.nz[v];
-        if (g.mask) m += g.mask[v];
+        const int v = par ? 0 : g.grid_pos[g0 + c] * ga;
+        const float val[7] = {par ? s_c[0][c] : g.px[v], par ? s_c[1][c] : g.py[v], par ? s_c[2][c] : g.pz[v],
+                              par ? s_c[3][c] : g.nx[v], par ? s_c[4][c] : g.ny[v], par ? s_c[5][c] : g.nz[v],
+                              par ? s_c[6][c] : (g.mask ? g.mask[v] : 0.0f)};
+        for (int q = 0; q < 7; q++) a[q] += val[q];
       }
       const float inv = 1.0f / n;
-      cx *= inv;
-      cy *= inv;
-      cz *= inv;
-      nx *= inv;
-      ny *= inv;
-      nz *= inv;
-      m *= inv;
-      for (int c = 0; c < n; c++) {
-        const int v = g.grid_pos[g0 + c] * ga;
-        if (__float_as_uint(g.px[v]) != __float_as_uint(cx) || __float_as_uint(g.py[v]) != __float_as_uint(cy) ||
-            __float_as_uint(g.pz[v]) != __float_as_uint(cz))
-          mark_moved(g, v, serial);
-        g.px[v] = g.qx[v] = cx;
-        g.py[v] = g.qy[v] = cy;
-        g.pz[v] = g.qz[v] = cz;
-        g.nx[v] = nx;
-        g.ny[v] = ny;
-        g.nz[v] = nz;
-        if (g.mask) g.mask[v] = m;
-      }
+      for (int q = 0; q < 7; q++) s_mean[q] = a[q] * inv;
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < n; c += GR_THREADS) {
+      const int v = g.grid_pos[g0 + c] * ga;
+      const float cx = s_mean[0], cy = s_mean[1], cz = s_mean[2];
+      if (__float_as_uint(g.px[v]) != __float_as_uint(cx) || __float_as_uint(g.py[v]) != __float_as_uint(cy) ||
+          __float_as_uint(g.pz[v]) != __float_as_uint(cz))
+        mark_moved(g, v, serial);
+      g.px[v] = g.qx[v] = cx;
+      g.py[v] = g.qy[v] = cy;
+      g.pz[v] = g.qz[v] = cz;
+      g.nx[v] = s_mean[3];
+      g.ny[v] = s_mean[4];
+      g.nz[v] = s_mean[5];
+      if (g.mask) g.mask[v] = s_mean[6];
     }
     for (int c = threadIdx.x; c < n; c += GR_THREADS) {
       const int e = g.face_edge[g0 + c], vv = g.face_vert[g0 + c];
@@ -577,55 +597,53 @@ __global__ __launch_bounds__(GR_THREADS) void kg_edges_verts(GView g) {
   }
 }
 
-// Leaves listed by the host (drop-in update paths): marked for the refit.
-__global__ __launch_bounds__(GR_THREADS) void kg_mark_leaves(GView g, const int *leaves, int n) {
-  const int serial = g.dser[0];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) g.leaf_mark[leaves[i]] = serial;
-}
 
-// A workgroup per marked leaf: vb = min/max over its elements in iteration order
-// (update_node_vb, pbvh.c:108-128).  min_ff / max_ff keep the later of equal values, so
-// the fold is order-preserving: each of the 16 waves takes a contiguous sixteenth of
-// the leaf, each lane folds 4 consecutive elements, the wave combines its lanes in
-// order (ordered DPP scan) and folds that after what came before; the waves combine in
-// order.  orig_vb =
-// vb when set_orig.  The leaf stays dirty for the next internal-node flush.
-#define GR_REFIT_THREADS 1024
-__global__ __launch_bounds__(GR_REFIT_THREADS) void kg_refit_leaves(GView g, int set_orig) {
-  const int serial = g.dser[0];
-  constexpr int NW = GR_REFIT_THREADS / SG_WAVE;
+// Leaf refit (update_node_vb, pbvh.c:108-128): vb = min/max over the leaf's elements
+// in iteration order.  min_ff / max_ff keep the later of equal values, so any fold that
+// combines contiguous pieces left to right gives the sequential result.  The listed
+// leaves (the host's list, or the dab's gathered leaves and those its averages moved)
+// are cut into GR_SPLIT contiguous parts, a workgroup per part: each wave folds a
+// contiguous quarter, 64 consecutive elements per round (one coalesced load per
+// coordinate, an ordered DPP scan over the lanes), 4 rounds in flight; the waves
+// combine in order into the part's box.
+#define GR_SPLIT 8
+__global__ __launch_bounds__(GR_THREADS) void kg_refit_parts(GView g, const int *list, int list_n, float *parts) {
+  constexpr int NW = GR_THREADS / SG_WAVE;
   __shared__ float s_bb[NW][6];
   const int lane = lane_id(), w = threadIdx.x / SG_WAVE;
-  for (int l = blockIdx.x; l < g.L; l += gridDim.x) {
-    if (g.leaf_mark[l] != serial) continue;
+  const int nl = list ? list_n : (int)(g.ctr[SGG_NGATHER] + g.ctr[SGG_NMOVED]);
+  for (int item = blockIdx.x; item < nl * GR_SPLIT; item += gridDim.x) {
+    const int li = item / GR_SPLIT, part = item - li * GR_SPLIT;
+    const int l = list ? list[li] : g.gathered[li];
     const int b = g.leaf_gbeg[l] * g.ga, e = g.leaf_gbeg[l + 1] * g.ga;
-    const int q = (e - b + NW - 1) / NW;
-    const int wb = b + w * q, we = min(e, wb + q);
+    const int seg = (e - b + GR_SPLIT - 1) / GR_SPLIT;
+    const int pb = min(e, b + part * seg), pe = min(e, pb + seg);
+    const int q = (pe - pb + NW - 1) / NW;
+    const int wb = min(pe, pb + w * q), we = min(pe, wb + q);
     BB6 acc;
     bb_reset(acc);
     for (int base = wb; base < we; base += SG_WAVE * 4) {
-      BB6 t;
-      bb_reset(t);
-      const int v0 = base + lane * 4;
       float x[4], y[4], z[4];
 #pragma unroll
-      for (int j = 0; j < 4; j++) {  // every load issued before any is used (clamped index)
-        const int v = min(v0 + j, we - 1);
-        x[j] = g.px[v];
-        y[j] = g.py[v];
-        z[j] = g.pz[v];
+      for (int r = 0; r < 4; r++) {  // every load issued (clamped index) before any is folded
+        const int v = min(base + r * SG_WAVE + lane, we - 1);
+        x[r] = g.px[v];
+        y[r] = g.py[v];
+        z[r] = g.pz[v];
       }
 #pragma unroll
-      for (int j = 0; j < 4; j++)
-        if (v0 + j < we) bb_add_point(t, x[j], y[j], z[j]);
-      t = bb_wave_scan(t);
-      BB6 tot;
-      for (int i = 0; i < 3; i++) {
-        tot.mn[i] = __shfl(t.mn[i], SG_WAVE - 1);
-        tot.mx[i] = __shfl(t.mx[i], SG_WAVE - 1);
+      for (int r = 0; r < 4; r++) {
+        BB6 t;
+        bb_reset(t);
+        if (base + r * SG_WAVE + lane < we) bb_add_point(t, x[r], y[r], z[r]);
+        t = bb_wave_scan(t);
+        BB6 tot;
+        for (int i = 0; i < 3; i++) {
+          tot.mn[i] = __shfl(t.mn[i], SG_WAVE - 1);
+          tot.mx[i] = __shfl(t.mx[i], SG_WAVE - 1);
+        }
+        bb_add_bb(acc, tot);
       }
-      bb_add_bb(acc, tot);
     }
     if (lane == 0)
       for (int i = 0; i < 3; i++) {
@@ -637,34 +655,46 @@ __global__ __launch_bounds__(GR_REFIT_THREADS) void kg_refit_leaves(GView g, int
       BB6 r;
       bb_reset(r);
       for (int k = 0; k < NW; k++) {
-        BB6 x;
+        BB6 xk;
         for (int i = 0; i < 3; i++) {
-          x.mn[i] = s_bb[k][i];
-          x.mx[i] = s_bb[k][3 + i];
+          xk.mn[i] = s_bb[k][i];
+          xk.mx[i] = s_bb[k][3 + i];
         }
-        bb_add_bb(r, x);
+        bb_add_bb(r, xk);
       }
-      bb_store(g.leaf_bb + 8 * (size_t)l, r);
-      if (set_orig) bb_store(g.leaf_obb + 8 * (size_t)l, r);
-      g.leaf_dirty[l] = 1;
+      bb_store(parts + 8 * (size_t)item, r);
     }
     __syncthreads();
   }
 }
 
-// The end of a dab: its counters kept (and logged in the faces counters' layout) and
-// cleared, the next serial taken (a captured stroke replays with fresh serials).
-__global__ void kg_finish(GView g, long long *log) {
+// The listed leaves' boxes from their parts, in order; orig_vb = vb when set_orig; the
+// leaves stay dirty for the next internal-node flush.  With `finish` this also closes
+// the dab: its counters kept (and logged in the faces counters' layout) and cleared,
+// the next serial taken (a captured stroke replays with fresh serials).
+__global__ __launch_bounds__(GR_THREADS) void kg_refit_combine(GView g, const int *list, int list_n,
+                                                               const float *parts, int set_orig, int finish,
+                                                               long long *log) {
   const int t = threadIdx.x;
-  long long c = 0;
-  if (t < SGG_CTR_COUNT) c = g.ctr[t];
+  const int nl = list ? list_n : (int)(g.ctr[SGG_NGATHER] + g.ctr[SGG_NMOVED]);
+  for (int li = t; li < nl; li += blockDim.x) {
+    const int l = list ? list[li] : g.gathered[li];
+    BB6 r;
+    bb_reset(r);
+    for (int p = 0; p < GR_SPLIT; p++) bb_add_bb(r, gbb_load(parts + 8 * ((size_t)li * GR_SPLIT + p)));
+    bb_store(g.leaf_bb + 8 * (size_t)l, r);
+    if (set_orig) bb_store(g.leaf_obb + 8 * (size_t)l, r);
+    g.leaf_dirty[l] = 1;
+  }
+  if (!finish) return;
   __shared__ long long s_c[SGG_CTR_COUNT];
+  if (t < SGG_CTR_COUNT) s_c[t] = g.ctr[t];
+  __syncthreads();
   if (t < SGG_CTR_COUNT) {
-    s_c[t] = c;
+    const long long c = t == SGG_NLEAVES ? nl : s_c[t];
     g.ctr_last[t] = c;
     g.ctr[t] = 0;
   }
-  __syncthreads();
   if (log && t < SG_CTR_COUNT) {
     const int src = t == SG_CTR_NGATHER ? SGG_NGATHER : t == SG_CTR_NAFF ? SGG_NAFF : t == SG_CTR_AREA_X ? SGG_AREA_X
                   : t == SG_CTR_AREA_Y ? SGG_AREA_Y : t == SG_CTR_AREA_Z ? SGG_AREA_Z : t == SG_CTR_NAREA ? SGG_NAREA
@@ -711,7 +741,7 @@ __global__ __launch_bounds__(1024) void kg_flush_nodes(GView g, int set_orig) {
 }
 
 // Around a drop-in update (and before a dab after a failed launch): zero the counters
-// and take the next serial on the device (a dab ends the same way in kg_finish).
+// and take the next serial on the device (a dab ends the same way in kg_refit_combine).
 __global__ void kg_clear_ctr(GView g) {
   if (threadIdx.x < SGG_CTR_COUNT) g.ctr[threadIdx.x] = 0;
   if (threadIdx.x == 0) {
@@ -782,6 +812,7 @@ void sg_grids_free_device(PBVH *p) {
   gfree(d.ctr_last);
   gfree(d.hlist);
   gfree(d.dser);
+  gfree(d.parts);
   gfree(d.leaf_dirty);
   gfree(d.node_dirty);
   if (d.snap) hipFree(d.snap);
@@ -910,6 +941,7 @@ int sg_grids_build_device(PBVH *p) {
   rc |= gzeros(&d.hlist, (size_t)d.L);
   rc |= gzeros(&d.dser, (size_t)4);
   rc |= gzeros(&d.leaf_dirty, (size_t)d.L);
+  rc |= gzeros(&d.parts, (size_t)d.L * GR_SPLIT * 8);
   rc |= gzeros(&d.node_dirty, (size_t)d.N);
   {
     const int ser0[4] = {1, 0, 0, 0};  // the first operation's serial: above every stamp's 0
@@ -998,7 +1030,8 @@ static int launch_check(const char *what) { return sg_check_hip(hipGetLastError(
 static int enqueue_normals(PBVH *p, const GView &g, const int *leaves, int n) {
   const SgGridDev &d = *p->gd;
   if (leaves) kg_mark_faces<<<std::max(1, std::min(n, 1024)), GR_THREADS, 0, p->stream>>>(g, leaves, n);
-  kg_face_normals<<<std::max(1, std::min(d.F, 512)), GR_THREADS, (size_t)d.ga * 12, p->stream>>>(g);
+  const int batch = std::max(1, (32 * 1024) / (d.ga * 12));  // grids staged together: up to 32 KiB of LDS
+  kg_face_normals<<<std::max(1, std::min(d.F, 512)), GR_THREADS, (size_t)batch * d.ga * 12, p->stream>>>(g, batch);
   kg_edges_verts<<<std::max(1, std::min((d.Ed + d.Vc + 3) / 4, 256)), GR_THREADS, 0, p->stream>>>(g);
   return launch_check("grid normals");
 }
@@ -1033,8 +1066,9 @@ int sg_grids_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long 
   }
   if (normals)
     if (int rc = enqueue_normals(p, g, nullptr, 0)) return rc;
-  kg_refit_leaves<<<std::max(1, std::min(d.L, 256)), GR_REFIT_THREADS, 0, p->stream>>>(g, 1);
-  kg_finish<<<1, 64, 0, p->stream>>>(g, log ? log + (size_t)log_index * SG_CTR_COUNT : nullptr);
+  kg_refit_parts<<<std::max(1, std::min(d.L * GR_SPLIT, 512)), GR_THREADS, 0, p->stream>>>(g, nullptr, 0, d.parts);
+  kg_refit_combine<<<1, GR_THREADS, 0, p->stream>>>(g, nullptr, 0, d.parts, 1, 1,
+                                                    log ? log + (size_t)log_index * SG_CTR_COUNT : nullptr);
   if (int rc = launch_check("grid dab")) return rc;
   d.ctr_unclean = false;
   return 0;
@@ -1063,8 +1097,8 @@ int sg_grids_enqueue_refit(PBVH *p, const int *leaves_dev, int n, int set_orig) 
   if (n <= 0) return 0;
   const GView g = view_of(d);
   kg_clear_ctr<<<1, 64, 0, p->stream>>>(g);
-  kg_mark_leaves<<<(n + GR_THREADS - 1) / GR_THREADS, GR_THREADS, 0, p->stream>>>(g, leaves_dev, n);
-  kg_refit_leaves<<<std::max(1, std::min(d.L, 256)), GR_REFIT_THREADS, 0, p->stream>>>(g, set_orig);
+  kg_refit_parts<<<std::max(1, std::min(n * GR_SPLIT, 512)), GR_THREADS, 0, p->stream>>>(g, leaves_dev, n, d.parts);
+  kg_refit_combine<<<1, GR_THREADS, 0, p->stream>>>(g, leaves_dev, n, d.parts, set_orig, 0, nullptr);
   kg_flush_nodes<<<1, 1024, 0, p->stream>>>(g, set_orig);
   kg_clear_ctr<<<1, 64, 0, p->stream>>>(g);
   return launch_check("grid refit");

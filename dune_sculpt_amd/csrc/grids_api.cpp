@@ -237,6 +237,9 @@ int sg_grids_fetch_last(PBVH *p) {
   p->last_ctr[SG_CTR_AREA_Y] = ctr[SGG_AREA_Y];
   p->last_ctr[SG_CTR_AREA_Z] = ctr[SGG_AREA_Z];
   p->last_ctr[SG_CTR_NAREA] = ctr[SGG_NAREA];
+  p->last_ctr[20] = ctr[SGG_NLEAVES];
+  p->last_ctr[21] = ctr[SGG_NFACES];
+  p->last_ctr[22] = ctr[SGG_NITEMS];
   return 0;
 }
 

@@ -83,7 +83,7 @@ struct SgGridDev {
   int *vert_foff = nullptr;   // [Vc + 1]
   int *vert_el = nullptr;     // element id per (vertex, face)
   int *face_stamp = nullptr, *edge_stamp = nullptr, *vert_stamp = nullptr;
-  int *gathered = nullptr;    // [L]
+  int *gathered = nullptr;    // [L] the dab's gathered leaves, then the leaves its averages moved
   int *dfaces = nullptr, *ditems = nullptr;  // dirty faces [F], dirty edges then vertices [Ed + Vc]
   long long *ctr = nullptr;   // live counters
   long long *ctr_last = nullptr;  // the last dab's counters
@@ -92,6 +92,7 @@ struct SgGridDev {
   void *snap = nullptr;       // checkpoint of the mutable state (sg_pbvh_snapshot_save)
   int *dser = nullptr;        // [4] device serials: the running operation's, the last dab's; blocks done
   int *leaf_dirty = nullptr, *node_dirty = nullptr;  // refit since the last internal-node flush
+  float *parts = nullptr;     // [L * 8 * 8] boxes of the parts of the leaves being refit
   bool ctr_unclean = false;   // a dab's launches failed midway: clear the counters first
   const SubdivCCG *topo_of = nullptr;
   int topo_version = -1;
@@ -100,7 +101,7 @@ struct SgGridDev {
 // counters (ctr / ctr_last)
 enum {
   SGG_NGATHER = 0, SGG_NAFF = 1, SGG_AREA_X = 2, SGG_AREA_Y = 3, SGG_AREA_Z = 4, SGG_NAREA = 5,
-  SGG_NFACES = 6, SGG_NITEMS = 7, SGG_NEDGES = 8, SGG_NLEAVES = 9, SGG_CTR_COUNT = 12
+  SGG_NFACES = 6, SGG_NITEMS = 7, SGG_NMOVED = 8, SGG_NLEAVES = 9, SGG_CTR_COUNT = 12
 };
 
 struct PBVH;

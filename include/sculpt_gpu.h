@@ -281,7 +281,9 @@ int sg_dab_last_affected(PBVH *pbvh, int *out, int cap);
  * plus step 1) [10] normal tri reads [11] distinct non-affected 1-ring verts (last
  * Jacobi steps) [12] distinct tris incident to A [13] distinct verts of those tris
  * [14] shared verts read by the leaf refit; learned Smooth: [15] sum (|d|/r)^2 in 2^-40
- * units, [16] out-of-range vertices, [17] 1 when accepted. */
+ * units, [16] out-of-range vertices, [17] 1 when accepted.  PBVH_GRIDS: [0], [1]
+ * (affected elements), [2..4], [8], and [20] leaves refit, [21] faces whose normals
+ * were redone, [22] edges and coarse vertices averaged. */
 int sg_dab_last_stats(PBVH *pbvh, int64_t out[24]);
 /* Gathered leaves (node indices, left->right) of the most recent dab. */
 int sg_dab_last_gathered(PBVH *pbvh, int *out, int cap);
