@@ -97,6 +97,8 @@ void DUNE_pbvh_subdiv_cgg_set(PBVH *p, SubdivCCG *s) {
   if (s && p->type == PBVH_GRIDS && p->gd) sg_grids_attach_topology(p, s);
 }
 
+void BKE_pbvh_subdiv_cgg_set(PBVH *p, SubdivCCG *s) { DUNE_pbvh_subdiv_cgg_set(p, s); }
+
 void BKE_pbvh_node_get_grids(PBVH *p, PBVHNode *node, int **r_grid_indices, int *r_totgrid, int *r_maxgrid,
                              int *r_gridsize, CCGElem ***r_griddata) {
   const bool grids = p && node && p->type == PBVH_GRIDS;

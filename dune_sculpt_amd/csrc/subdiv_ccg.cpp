@@ -377,6 +377,8 @@ void KERNEL_subdiv_ccg_key_top_level(CCGKey *key, const SubdivCCG *s) {  // subd
   key->has_mask = s->has_mask;
 }
 
+void BKE_subdiv_ccg_key_top_level(CCGKey *key, const SubdivCCG *s) { KERNEL_subdiv_ccg_key_top_level(key, s); }
+
 void KERNEL_subdiv_ccg_average_grids(SubdivCCG *s) {  // subdiv_ccg.c:1171-1189
   if (!s) return;
   for (int f = 0; f < s->num_faces; f++) average_inner_face(s, f);

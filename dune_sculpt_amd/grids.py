@@ -20,7 +20,7 @@ EXPORTS = [
     "sg_subdiv_ccg_grids", "sg_subdiv_ccg_num_grids", "sg_subdiv_ccg_grid_faces", "sg_subdiv_ccg_faces",
     "sg_subdiv_ccg_flag_mats", "sg_subdiv_ccg_grid_hidden", "sg_subdiv_ccg_get_co", "sg_subdiv_ccg_set_co",
     "sg_subdiv_ccg_get_normals", "sg_subdiv_ccg_get_mask", "sg_subdiv_ccg_set_mask", "BKE_pbvh_build_grids",
-    "DUNE_pbvh_subdiv_cgg_set", "BKE_pbvh_node_get_grids", "BKE_pbvh_grid_hidden", "BKE_pbvh_get_grid_key",
+    "DUNE_pbvh_subdiv_cgg_set", "BKE_pbvh_subdiv_cgg_set", "BKE_subdiv_ccg_key_top_level", "BKE_pbvh_node_get_grids", "BKE_pbvh_grid_hidden", "BKE_pbvh_get_grid_key",
     "BKE_pbvh_get_grids", "BKE_pbvh_get_grid_num_vertices", "BKE_pbvh_get_grid_num_faces",
     "BKE_pbvh_get_grid_updates", "BKE_pbvh_count_grid_quads", "sg_pbvh_grids_sync_host", "sg_pbvh_grids_upload",
 ]

@@ -593,8 +593,9 @@ SubdivCCG *sg_subdiv_ccg_new(const float (*coarse_co)[3], int totvert, const int
                              const int *poly_verts, int totpoly, int level, int need_normal, int need_mask);
 /* subdiv_ccg.c:595 */
 void KERNEL_subdiv_ccg_destroy(SubdivCCG *subdiv_ccg);
-/* subdiv_ccg.c:648 */
+/* subdiv_ccg.c:648; paint.c:2126 calls it by the BKE_ name: same function */
 void KERNEL_subdiv_ccg_key_top_level(CCGKey *key, const SubdivCCG *subdiv_ccg);
+void BKE_subdiv_ccg_key_top_level(CCGKey *key, const SubdivCCG *subdiv_ccg);
 /* subdiv_ccg.c:781 every grid's normals, then the grids averaged (host) */
 void KERNEL_subdiv_ccg_recalc_normals(SubdivCCG *subdiv_ccg);
 /* subdiv_ccg.c:1171 seams inside faces, then edges, then corners (host) */
@@ -636,8 +637,10 @@ int sg_subdiv_ccg_set_mask(SubdivCCG *subdiv_ccg, const float *in);
  * DUNE_pbvh_subdiv_cgg_set. */
 void BKE_pbvh_build_grids(PBVH *pbvh, CCGElem **grids, int totgrid, CCGKey *key, void **gridfaces,
                           DMFlagMat *flagmats, BLI_bitmap **grid_hidden);
-/* pbvh.c:4973 (uploads the face, edge, vertex and neighbour tables of the grids) */
+/* pbvh.c:4973 (uploads the face, edge, vertex and neighbour tables of the grids);
+ * paint.c:1680 calls it by the BKE_ name: same function */
 void DUNE_pbvh_subdiv_cgg_set(PBVH *pbvh, SubdivCCG *subdiv_ccg);
+void BKE_pbvh_subdiv_cgg_set(PBVH *pbvh, SubdivCCG *subdiv_ccg);
 /* pbvh.c:1865 */
 void BKE_pbvh_node_get_grids(PBVH *pbvh, PBVHNode *node, int **r_grid_indices, int *r_totgrid, int *r_maxgrid,
                              int *r_gridsize, CCGElem ***r_griddata);
