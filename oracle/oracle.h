@@ -43,6 +43,7 @@ enum {
   ORC_UpdateDrawBuffers = 1 << 4,
   ORC_UpdateRedraw = 1 << 5,
   ORC_FullyHidden = 1 << 10,
+  ORC_UpdateTopology = 1 << 13,
 };
 
 /* Tools (types_brush_enums.h:410-415) and falloff presets (:176-187). */
@@ -254,6 +255,32 @@ int orc_get_threads(void);
 /* Stats of the last dab for the byte model (gathered leaves etc.). */
 void orc_last_stats(const OrcPBVH *o, int64_t out[8]);
 
+/* ------------------------------------------------------------------------- */
+/* Dynamic topology, PBVH_BMESH (dyntopo.c).                                  */
+/* ------------------------------------------------------------------------- */
+enum { ORC_Subdivide = 1, ORC_Collapse = 2 }; /* PBVHTopologyUpdateMode */
+typedef struct OrcDyn OrcDyn;
+OrcDyn *orc_dyn_new(const float *co, int totvert, const int *tris, int tottri, const float *mask,
+                    const uint8_t *vhide, const uint8_t *fhide, int vert_cap, int tri_cap, int leaf_limit,
+                    int respect_hide, char *err, int errlen);
+void orc_dyn_free(OrcDyn *o);
+void orc_dyn_detail_size_set(OrcDyn *o, float detail_size);
+void orc_dyn_topology_set(OrcDyn *o, int mode, const float view_normal[3], int use_frontface, int use_projected);
+void orc_dyn_set_normals(OrcDyn *o, const float *no);
+int orc_dyn_search(const OrcDyn *o, const float center[3], float radius, int *out, int cap);
+int orc_dyn_update_topology(OrcDyn *o, int mode, const float center[3], float radius);
+int orc_dyn_subdivide_sequential(OrcDyn *o, const float center[3], float radius);
+int orc_dyn_dab(OrcDyn *o, const OrcDab *dab, uint64_t *out_affected, int cap);
+void orc_dyn_after_stroke(OrcDyn *o);
+void orc_dyn_mark(OrcDyn *o, const int *nodes, int n, int flag);
+void orc_dyn_update_normals(OrcDyn *o);
+void orc_dyn_update_bounds(OrcDyn *o);
+void orc_dyn_counts(const OrcDyn *o, int out[6]);
+void orc_dyn_export_verts(const OrcDyn *o, uint64_t *lab, float *co, float *no, float *mask, int *node);
+void orc_dyn_export_faces(const OrcDyn *o, uint64_t *labs, int *node, float *fno, uint8_t *hide);
+void orc_dyn_node(const OrcDyn *o, int node, int out[3], float vb[6], float orig_vb[6]);
+void orc_dyn_last_stats(const OrcDyn *o, int64_t out[12]);
+int orc_dyn_check(const OrcDyn *o);
 #ifdef __cplusplus
 }
 #endif

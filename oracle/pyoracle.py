@@ -130,6 +130,26 @@ def lib():
         "orc_grids_mark": (None, [vp, i32p, i32, i32]),
         "orc_grids_update_normals": (None, [vp]),
         "orc_grids_update_bounds": (None, [vp]),
+        "orc_dyn_new": (vp, [f32p, i32, i32p, i32, f32p, C.POINTER(C.c_uint8), C.POINTER(C.c_uint8), i32, i32, i32,
+                             i32, C.c_char_p, i32]),
+        "orc_dyn_free": (None, [vp]),
+        "orc_dyn_detail_size_set": (None, [vp, C.c_float]),
+        "orc_dyn_topology_set": (None, [vp, i32, f32p, i32, i32]),
+        "orc_dyn_set_normals": (None, [vp, f32p]),
+        "orc_dyn_search": (i32, [vp, f32p, C.c_float, i32p, i32]),
+        "orc_dyn_update_topology": (i32, [vp, i32, f32p, C.c_float]),
+        "orc_dyn_subdivide_sequential": (i32, [vp, f32p, C.c_float]),
+        "orc_dyn_dab": (i32, [vp, C.POINTER(OrcDab), C.POINTER(C.c_uint64), i32]),
+        "orc_dyn_after_stroke": (None, [vp]),
+        "orc_dyn_mark": (None, [vp, i32p, i32, i32]),
+        "orc_dyn_update_normals": (None, [vp]),
+        "orc_dyn_update_bounds": (None, [vp]),
+        "orc_dyn_counts": (None, [vp, i32p]),
+        "orc_dyn_export_verts": (None, [vp, C.POINTER(C.c_uint64), f32p, f32p, f32p, i32p]),
+        "orc_dyn_export_faces": (None, [vp, C.POINTER(C.c_uint64), i32p, f32p, C.POINTER(C.c_uint8)]),
+        "orc_dyn_node": (None, [vp, i32, i32p, f32p, f32p]),
+        "orc_dyn_last_stats": (None, [vp, C.POINTER(C.c_int64)]),
+        "orc_dyn_check": (i32, [vp]),
     }
     for k, (r, a) in sig.items():
         f = getattr(L, k)
@@ -604,3 +624,145 @@ class OracleGrids:
 
     def update_bounds(self):
         lib().orc_grids_update_bounds(self.h)
+
+
+SUBDIVIDE, COLLAPSE = 1, 2
+UPDATE_TOPOLOGY = 1 << 13
+
+
+class OracleDyn:
+    """The oracle's dynamic-topology PBVH (dyntopo.c).  State is read by vertex label:
+    `state()` gives live vertices sorted by label and faces as label triples rotated
+    to start at their smallest label, sorted."""
+
+    def __init__(self, co, tris, mask=None, vhide=None, fhide=None, vert_cap=None, tri_cap=None, leaf_limit=100,
+                 respect_hide=True):
+        L = lib()
+        co = np.ascontiguousarray(co, np.float32)
+        tris = np.ascontiguousarray(tris, np.int32)
+        V, T = len(co), len(tris)
+        self.has_mask = mask is not None
+        m = None if mask is None else np.ascontiguousarray(mask, np.float32)
+        vh = None if vhide is None else np.ascontiguousarray(vhide, np.uint8)
+        fh = None if fhide is None else np.ascontiguousarray(fhide, np.uint8)
+        u8 = lambda a: None if a is None else a.ctypes.data_as(C.POINTER(C.c_uint8))
+        err = C.create_string_buffer(256)
+        self.h = L.orc_dyn_new(_fp(co), V, _ip(tris), T, None if m is None else _fp(m), u8(vh), u8(fh),
+                               int(vert_cap or 4 * V), int(tri_cap or 4 * T), int(leaf_limit),
+                               1 if respect_hide else 0, err, 256)
+        if not self.h:
+            raise RuntimeError("orc_dyn_new: " + err.value.decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().orc_dyn_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def detail_size_set(self, d):
+        lib().orc_dyn_detail_size_set(self.h, float(d))
+
+    def topology_set(self, mode=SUBDIVIDE | COLLAPSE, view_normal=(0.0, 0.0, 1.0), use_frontface=False,
+                     use_projected=False):
+        vn = np.ascontiguousarray(view_normal, np.float32)
+        lib().orc_dyn_topology_set(self.h, int(mode), _fp(vn), 1 if use_frontface else 0, 1 if use_projected else 0)
+
+    def counts(self):
+        out = np.zeros(6, np.int32)
+        lib().orc_dyn_counts(self.h, _ip(out))
+        return {"verts": int(out[0]), "tris": int(out[1]), "nodes": int(out[2]), "vert_slots": int(out[3]),
+                "tri_slots": int(out[4]), "slab": int(out[5])}
+
+    def search(self, center, radius):
+        c = np.ascontiguousarray(center, np.float32)
+        n = self.counts()["nodes"]
+        out = np.empty(n + 1, np.int32)
+        k = lib().orc_dyn_search(self.h, _fp(c), float(radius), _ip(out), out.size)
+        return out[:k].copy()
+
+    def mark(self, nodes, flag):
+        a = np.ascontiguousarray(nodes, np.int32)
+        lib().orc_dyn_mark(self.h, _ip(a), a.size, int(flag))
+
+    def update_topology(self, mode, center, radius):
+        c = np.ascontiguousarray(center, np.float32)
+        return bool(lib().orc_dyn_update_topology(self.h, int(mode), _fp(c), float(radius)))
+
+    def subdivide_sequential(self, center, radius):
+        c = np.ascontiguousarray(center, np.float32)
+        return int(lib().orc_dyn_subdivide_sequential(self.h, _fp(c), float(radius)))
+
+    def dab(self, d):
+        cap = self.counts()["vert_slots"] + 1
+        out = np.empty(cap, np.uint64)
+        n = lib().orc_dyn_dab(self.h, C.byref(d), out.ctypes.data_as(C.POINTER(C.c_uint64)), cap)
+        return out[:n].copy()
+
+    def after_stroke(self):
+        lib().orc_dyn_after_stroke(self.h)
+
+    def update_normals(self):
+        lib().orc_dyn_update_normals(self.h)
+
+    def update_bounds(self):
+        lib().orc_dyn_update_bounds(self.h)
+
+    def last_stats(self):
+        out = np.zeros(12, np.int64)
+        lib().orc_dyn_last_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_int64)))
+        return out
+
+    def check(self):
+        return int(lib().orc_dyn_check(self.h))
+
+    def raw_verts(self):
+        n = self.counts()["verts"]
+        lab = np.empty(n, np.uint64)
+        co = np.empty((n, 3), np.float32)
+        no = np.empty((n, 3), np.float32)
+        mk = np.empty(n, np.float32)
+        node = np.empty(n, np.int32)
+        lib().orc_dyn_export_verts(self.h, lab.ctypes.data_as(C.POINTER(C.c_uint64)), _fp(co), _fp(no), _fp(mk),
+                                   _ip(node))
+        return lab, co, no, mk, node
+
+    def raw_faces(self):
+        n = self.counts()["tris"]
+        labs = np.empty((n, 3), np.uint64)
+        node = np.empty(n, np.int32)
+        fno = np.empty((n, 3), np.float32)
+        hide = np.empty(n, np.uint8)
+        lib().orc_dyn_export_faces(self.h, labs.ctypes.data_as(C.POINTER(C.c_uint64)), _ip(node), _fp(fno),
+                                   hide.ctypes.data_as(C.POINTER(C.c_uint8)))
+        return labs, node, fno, hide
+
+    def state(self):
+        return canonical_state(self.raw_verts(), self.raw_faces())
+
+    def nodes(self):
+        n = self.counts()["nodes"]
+        info = np.zeros((n, 3), np.int32)
+        vb = np.zeros((n, 6), np.float32)
+        ob = np.zeros((n, 6), np.float32)
+        for i in range(n):
+            lib().orc_dyn_node(self.h, i, _ip(info[i]), _fp(vb[i]), _fp(ob[i]))
+        return info, vb, ob
+
+
+def canonical_state(verts, faces):
+    """(vertex arrays sorted by label, faces rotated to their smallest label and sorted)."""
+    lab, co, no, mk, node = verts
+    o = np.argsort(lab, kind="stable")
+    v = {"lab": lab[o], "co": co[o], "no": no[o], "mask": mk[o], "node": node[o]}
+    labs, fnode, fno, hide = faces
+    r = np.argmin(labs, axis=1)
+    idx = (r[:, None] + np.arange(3)[None, :]) % 3
+    rot = np.take_along_axis(labs, idx, axis=1)
+    fo = np.lexsort((rot[:, 2], rot[:, 1], rot[:, 0]))
+    f = {"labs": rot[fo], "node": fnode[fo], "fno": fno[fo], "hide": hide[fo]}
+    return v, f
