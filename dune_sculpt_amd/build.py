@@ -36,7 +36,8 @@ COMMON = [
 ]
 
 SOURCES = ["kernels.hip", "learned.hip", "batch.hip", "batch_mesh.hip", "ray.hip", "draw.hip", "undo.hip", "proxy.hip",
-           "grids.hip", "api.cpp", "pbvh_build.cpp", "meshgen.cpp", "subdiv_ccg.cpp", "grids_api.cpp"]
+           "grids.hip", "dyntopo.hip", "api.cpp", "pbvh_build.cpp", "meshgen.cpp", "subdiv_ccg.cpp", "grids_api.cpp",
+           "dyntopo_api.cpp"]
 
 
 def _run(cmds):

@@ -10,6 +10,7 @@
 
 #include "sg_internal.h"
 #include "sg_grids.h"
+#include "sg_dyn.h"
 #include "sg_kernels.h"
 #include "sg_learned.h"
 
@@ -61,6 +62,7 @@ void invalidate_host_views(PBVH *p) {
 int sync_bbs(PBVH *p) {
   if (!p->host_bb_stale || p->nodes.empty() || p->no_device) return 0;
   if (p->type == PBVH_GRIDS) return p->gd ? sg_grids_sync_bbs(p) : 0;
+  if (p->type == PBVH_BMESH) return sg_dyn_refresh(p);
   if (p->nodes_dirty_deferred) {
     if (sg_enqueue_flush(p)) return -1;
     p->nodes_dirty_deferred = false;
@@ -217,6 +219,7 @@ void BKE_pbvh_free(PBVH *p) {
   p->prox_chunks.clear();
   if (!p->no_device) sg_free_device(p);
   if (!p->no_device) sg_grids_free_device(p);
+  sg_dyn_release(p);
   for (int sl = 0; sl < 2; sl++)
     for (void *q : p->snap[sl]) hipFree(q);
   if (p->judge_buf) hipFree(p->judge_buf);
@@ -424,7 +427,7 @@ int sg_pbvh_set_fused(PBVH *p, int on) {
 }
 
 int sg_pbvh_halo_info(PBVH *p, long long out[5]) {
-  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: halo tables");
+  if (p && p->type != PBVH_FACES) return sg_set_error("faces only: halo tables");
   if (!p || !out) return sg_set_error("bad arguments");
   const SgDev &d = p->d;
   out[0] = d.halo_ok ? 1 : 0;
@@ -499,11 +502,13 @@ static int snap_restore(PBVH *p, int slot) {
 }
 
 int sg_pbvh_snapshot_save(PBVH *p) {
+  if (p && p->type == PBVH_BMESH) return sg_set_error("PBVH_BMESH: no snapshots (the topology changes)");
   if (p && p->type == PBVH_GRIDS) return p->gd ? sg_grids_snapshot(p, 0) : sg_set_error("no grids");
   return snap_save(p, 0);
 }
 
 int sg_pbvh_snapshot_restore(PBVH *p) {
+  if (p && p->type == PBVH_BMESH) return sg_set_error("PBVH_BMESH: no snapshots (the topology changes)");
   if (p && p->type == PBVH_GRIDS) {
     if (!p->gd) return sg_set_error("no grids");
     p->host_bb_stale = true;
@@ -515,6 +520,7 @@ int sg_pbvh_snapshot_restore(PBVH *p) {
 int sg_dab_apply(PBVH *p, const SgDabParams *dab) {
   if (int rc = check_dab(p, dab)) return rc;
   if (p->type == PBVH_GRIDS) return sg_grids_dab_apply(p, dab);
+  if (p->type == PBVH_BMESH) return sg_dyn_dab_apply(p, dab);
   const float *curve = nullptr;
   if (dab->curve_preset == SG_CURVE_CUSTOM && dab->curve_table) {
     if (ensure_curves(p, 1)) return -1;
@@ -732,6 +738,11 @@ static int stroke_apply(PBVH *p, const SgDabParams *dabs, const SgRay *rays, int
 }
 
 int sg_dab_apply_stroke(PBVH *p, const SgDabParams *dabs, int n, int flags) {
+  if (p && p->type == PBVH_BMESH) {  // dab by dab: each topology round waits for its counters
+    for (int i = 0; i < n; i++)
+      if (int rc = sg_dab_apply(p, &dabs[i])) return rc;
+    return 0;
+  }
   if (p && p->type == PBVH_GRIDS) {
     if (flags & SG_STROKE_GRAPH) {  // one captured graph
       SgStroke *s = sg_stroke_prepare(p, dabs, n, flags);
@@ -776,7 +787,7 @@ int sg_dab_apply_stroke(PBVH *p, const SgDabParams *dabs, int n, int flags) {
 }
 
 int sg_dab_apply_stroke_placed(PBVH *p, const SgDabParams *dabs, const SgRay *rays, int n, int flags) {
-  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: PBVH_GRIDS has no ray-placed strokes");
+  if (p && p->type != PBVH_FACES) return sg_set_error("faces only: PBVH_GRIDS has no ray-placed strokes");
   if (n > 0 && !rays) return sg_set_error("bad arguments");
   return stroke_apply(p, dabs, rays, n, flags);
 }
@@ -913,11 +924,15 @@ static SgStroke *stroke_prepare(PBVH *p, const SgDabParams *dabs, const SgRay *r
 }
 
 SgStroke *sg_stroke_prepare(PBVH *p, const SgDabParams *dabs, int n, int flags) {
+  if (p && p->type == PBVH_BMESH) {
+    sg_set_error("PBVH_BMESH: no captured strokes (each topology round waits for its counters)");
+    return nullptr;
+  }
   return stroke_prepare(p, dabs, nullptr, n, flags);
 }
 
 SgStroke *sg_stroke_prepare_placed(PBVH *p, const SgDabParams *dabs, const SgRay *rays, int n, int flags) {
-  if (p && p->type == PBVH_GRIDS) { sg_set_error("faces only: PBVH_GRIDS has no captured strokes"); return nullptr; }
+  if (p && p->type != PBVH_FACES) { sg_set_error("faces only: PBVH_GRIDS has no captured strokes"); return nullptr; }
   if (!rays) {
     sg_set_error("bad arguments");
     return nullptr;
@@ -1012,7 +1027,7 @@ static double time_stroke(PBVH *p, const SgDabParams *dabs, int n, int reps) {
 }
 
 int sg_pbvh_autotune(PBVH *p, const SgDabParams *dabs, int n, double *out_ms) {
-  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: no autotuner for PBVH_GRIDS");
+  if (p && p->type != PBVH_FACES) return sg_set_error("faces only: no autotuner for PBVH_GRIDS");
   if (!p || !dabs || n <= 0) return sg_set_error("bad arguments");
   if (int rc = snap_save(p, 1)) return rc;
   const int cand[] = {32, 64, 128, 192, 256, 384, 512};
@@ -1049,7 +1064,7 @@ int sg_pbvh_autotune(PBVH *p, const SgDabParams *dabs, int n, double *out_ms) {
 // learned dab and keep its positions, return to the state, run the plain dab (which
 // stays applied) and compare the two on the affected vertices.
 int sg_learned_judge(PBVH *p, const SgDabParams *dab, double out[4]) {
-  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: learned Smooth is faces only");
+  if (p && p->type != PBVH_FACES) return sg_set_error("faces only: learned Smooth is faces only");
   if (int rc = check_dab(p, dab)) return rc;
   if (!p->mlp || dab->sculpt_tool != SG_TOOL_SMOOTH) return sg_set_error("judge needs a model and a Smooth dab");
   const size_t V = (size_t)p->d.V;
@@ -1088,7 +1103,7 @@ int sg_learned_judge(PBVH *p, const SgDabParams *dab, double out[4]) {
 }
 
 int sg_pbvh_set_learned(PBVH *p, SgMlp *m) {
-  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: learned Smooth is faces only");
+  if (p && p->type != PBVH_FACES) return sg_set_error("faces only: learned Smooth is faces only");
   if (!p) return sg_set_error("null PBVH");
   p->mlp = m;
   return 0;
@@ -1104,7 +1119,7 @@ int sg_pbvh_learned_stats(PBVH *p, long long out[2]) {
 // One plain Smooth dab that also records, for each affected vertex, the learned
 // model's input features before the dab and the displacement target after it.
 int sg_learned_collect(PBVH *p, const SgDabParams *dab, float *X, float *T, float *ok, int cap) {
-  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: learned Smooth is faces only");
+  if (p && p->type != PBVH_FACES) return sg_set_error("faces only: learned Smooth is faces only");
   if (int rc = check_dab(p, dab)) return rc;
   if (dab->sculpt_tool != SG_TOOL_SMOOTH) return sg_set_error("collection needs a Smooth dab");
   if (cap <= 0 || !X || !T || !ok) return sg_set_error("bad arguments");
@@ -1236,6 +1251,7 @@ int sg_dab_last_affected(PBVH *p, int *out, int cap) {
   if (!p) return sg_set_error("null PBVH");
   if (p->no_device) return sg_set_error("no HIP device");
   if (p->type == PBVH_GRIDS) return p->gd ? sg_grids_last_affected(p, out, cap) : sg_set_error("no grids");
+  if (p->type == PBVH_BMESH) return sg_set_error("PBVH_BMESH: vertices are labels (sg_pbvh_bmesh_last_affected)");
   int na = (int)p->last_ctr[SG_CTR_NAFF];
   std::vector<int> tmp(std::max(na, 1));
   if (p->last_tool == SG_TOOL_ELASTIC_DEFORM) {
@@ -1292,6 +1308,15 @@ int sg_dab_last_affected(PBVH *p, int *out, int cap) {
 
 int sg_dab_last_stats(PBVH *p, int64_t out[24]) {
   if (!p) return sg_set_error("null PBVH");
+  if (p->type == PBVH_BMESH) {  // [0] gathered [1] affected, [12..23] the dyntopo counters
+    long long st[12];
+    if (sg_pbvh_bmesh_last_stats(p, st)) return -1;
+    for (int i = 0; i < 24; i++) out[i] = 0;
+    out[0] = st[0];
+    out[1] = st[1];
+    for (int i = 0; i < 12; i++) out[12 + i] = st[i];
+    return 0;
+  }
   for (int i = 0; i < 24; i++) out[i] = p->last_ctr[i];
   out[SG_CTR_NGATHER] &= 0xffffffffll;  // the high half flags an open undo step
   return 0;
@@ -1301,6 +1326,7 @@ int sg_dab_last_gathered(PBVH *p, int *out, int cap) {
   if (!p) return sg_set_error("null PBVH");
   if (p->no_device) return sg_set_error("no HIP device");
   if (p->type == PBVH_GRIDS) return p->gd ? sg_grids_last_gathered(p, out, cap) : sg_set_error("no grids");
+  if (p->type == PBVH_BMESH) return sg_dyn_last_gathered(p, out, cap);
   const int ng = (int)p->last_ctr[SG_CTR_NGATHER];
   std::vector<int> tmp(std::max(ng, 1));
   if (ng > 0 && sg_check_hip(hipMemcpy(tmp.data(), p->d.gathered, ng * sizeof(int), hipMemcpyDeviceToHost), "copy"))
@@ -1311,19 +1337,19 @@ int sg_dab_last_gathered(PBVH *p, int *out, int cap) {
 }
 
 int sg_pbvh_read_positions(PBVH *p, float *out) {
-  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: read grids with sg_pbvh_grids_sync_host");
+  if (p && p->type != PBVH_FACES) return sg_set_error("faces only: read grids with sg_pbvh_grids_sync_host");
   if (!p) return sg_set_error("null PBVH");
   return read_aos(p, 0, out);
 }
 
 int sg_pbvh_read_normals(PBVH *p, float *out) {
-  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: read grids with sg_pbvh_grids_sync_host");
+  if (p && p->type != PBVH_FACES) return sg_set_error("faces only: read grids with sg_pbvh_grids_sync_host");
   if (!p) return sg_set_error("null PBVH");
   return read_aos(p, 1, out);
 }
 
 int sg_pbvh_set_mask(PBVH *p, const float *mask) {
-  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: grid masks live in the grids (sg_pbvh_grids_upload)");
+  if (p && p->type != PBVH_FACES) return sg_set_error("faces only: grid masks live in the grids (sg_pbvh_grids_upload)");
   if (!p) return sg_set_error("null PBVH");
   if (p->no_device) return sg_set_error("no HIP device");
   SgDev &d = p->d;
@@ -1413,12 +1439,18 @@ void BKE_pbvh_search_callback(PBVH *p, BKE_pbvh_SearchCallback scb, void *search
 // ---- marking ---------------------------------------------------------------
 
 void BKE_pbvh_node_mark_update(PBVHNode *node) {  // pbvh.c:1723-1727
-  node->flag |= PBVH_UpdateNormals | PBVH_UpdateBB | PBVH_UpdateOriginalBB | PBVH_UpdateDrawBuffers | PBVH_UpdateRedraw;
+  const int bits = PBVH_UpdateNormals | PBVH_UpdateBB | PBVH_UpdateOriginalBB | PBVH_UpdateDrawBuffers | PBVH_UpdateRedraw;
+  node->flag |= bits;
+  sg_dyn_note_flags(node, bits);
   if (node->owner) node->owner->draw_marks_pending = true;
 }
-void BKE_pbvh_node_mark_normals_update(PBVHNode *node) { node->flag |= PBVH_UpdateNormals; }
+void BKE_pbvh_node_mark_normals_update(PBVHNode *node) {
+  node->flag |= PBVH_UpdateNormals;
+  sg_dyn_note_flags(node, PBVH_UpdateNormals);
+}
 void BKE_pbvh_node_mark_redraw(PBVHNode *node) {
   node->flag |= PBVH_UpdateDrawBuffers | PBVH_UpdateRedraw;
+  sg_dyn_note_flags(node, PBVH_UpdateDrawBuffers | PBVH_UpdateRedraw);
   if (node->owner) node->owner->draw_marks_pending = true;
 }
 void BKE_pbvh_vert_mark_update(PBVH *p, int index) {  // pbvh.c:1811-1815
@@ -1470,6 +1502,10 @@ void BKE_pbvh_update_bounds(PBVH *p, int flag) {  // pbvh.c:1401-1421
   if (!p || p->nodes.empty() || p->no_device || !p->build_ok) return;
   if (p->type == PBVH_GRIDS) {
     sg_grids_update_bounds(p, flag);
+    return;
+  }
+  if (p->type == PBVH_BMESH) {
+    if (!sg_dyn_sync_flags_down(p)) sg_dyn_update_normals_bounds(p, false, true);
     return;
   }
   // the leaves update_search_cb gathers (any bit of `flag`), with the bounds bits each
@@ -1570,6 +1606,10 @@ void BKE_pbvh_update_normals(PBVH *p, SubdivCCG *subdiv_ccg) {  // pbvh.c:4559-4
     sg_grids_update_normals(p, subdiv_ccg);
     return;
   }
+  if (p->type == PBVH_BMESH) {  // pbvh_bmesh_normals_update (pbvh_bmesh.c:1635-1656)
+    if (!sg_dyn_sync_flags_down(p)) sg_dyn_update_normals_bounds(p, true, false);
+    return;
+  }
   std::vector<int> leaves = flagged_leaves(p, PBVH_UpdateNormals);
   if (leaves.empty()) return;
   upload_host_leaf_verts(p, leaves);
@@ -1612,7 +1652,7 @@ void BKE_pbvh_update_normals(PBVH *p, SubdivCCG *subdiv_ccg) {  // pbvh.c:4559-4
 // temporary that it frees (mesh_normals.c:455-507): the vertex normals themselves
 // change only at the next BKE_pbvh_update_normals, as here.
 void BKE_pbvh_vert_coords_apply(PBVH *p, const float (*vertCos)[3], int totvert) {
-  if (p && p->type == PBVH_GRIDS) { sg_set_error("faces only: BKE_pbvh_vert_coords_apply"); return; }
+  if (p && p->type != PBVH_FACES) { sg_set_error("faces only: BKE_pbvh_vert_coords_apply"); return; }
   if (!p || p->nodes.empty() || p->no_device || !p->build_ok) return;
   if (totvert != p->totvert) {
     sg_set_error("PBVH: Given deforming vcos number does not match PBVH vertex number!");
@@ -1664,6 +1704,13 @@ void BKE_pbvh_node_get_original_BB(PBVHNode *node, float bb_min[3], float bb_max
 }
 
 void BKE_pbvh_node_num_verts(PBVH *p, PBVHNode *node, int *r_uniquevert, int *r_totvert) {
+  if (p && p->type == PBVH_BMESH) {  // pbvh.c:1846-1851: unique + other vertices
+    const int u = std::max(sg_pbvh_bmesh_node_unique_verts(node, nullptr, 0), 0);
+    const int o = std::max(sg_pbvh_bmesh_node_other_verts(node, nullptr, 0), 0);
+    if (r_totvert) *r_totvert = u + o;
+    if (r_uniquevert) *r_uniquevert = u;
+    return;
+  }
   if (p && p->type == PBVH_GRIDS) {  // pbvh.c:1836-1844: every grid element is its node's own
     const int tot = node->totprim * p->gridkey.grid_area;
     if (r_totvert) *r_totvert = tot;
@@ -1692,7 +1739,7 @@ MVert *BKE_pbvh_get_verts(const PBVH *cp) {
 
 const float (*DUNE_pbvh_get_vert_normals(const PBVH *cp))[3] {
   PBVH *p = const_cast<PBVH *>(cp);
-  if (!p || p->type == PBVH_GRIDS) return nullptr;
+  if (!p || p->type != PBVH_FACES) return nullptr;
   sync_normals(p);
   return (const float(*)[3])p->host_normals.data();
 }
@@ -1731,7 +1778,7 @@ bool sg_proxy_in_arena(const PBVH *p, const void *ptr) {
 }
 
 PBVHProxyNode *BKE_pbvh_node_add_proxy(PBVH *p, PBVHNode *node) {
-  if (p && p->type == PBVH_GRIDS) { sg_set_error("faces only: proxies"); return nullptr; }
+  if (p && p->type != PBVH_FACES) { sg_set_error("faces only: proxies"); return nullptr; }
   const int index = node->proxy_count;
   const bool in_tree = node >= p->nodes.data() && node < p->nodes.data() + p->nodes.size();
   if (index == 0 && in_tree) p->proxy_nodes.push_back((int)(node - p->nodes.data()));  // the combine's worklist

@@ -336,7 +336,7 @@ static int dev_upload(T **dst, const T *src, size_t n) {
 extern "C" {
 
 int sg_pbvh_draw_buffers_build(PBVH *p, const int *face_sets, int npoly, int seed, int fset_default) {
-  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: draw buffers of PBVH_GRIDS are not built");
+  if (p && p->type != PBVH_FACES) return sg_set_error("faces only: draw buffers of PBVH_GRIDS / PBVH_BMESH are not built");
   if (!p || npoly < 0 || (npoly > 0 && !face_sets)) return sg_set_error("bad arguments");
   if (p->no_device || !p->build_ok) return sg_set_error("no device mesh");
   if (npoly != (int)p->poly_smooth.size()) return sg_set_error("face sets: %d polygons, the mesh has %zu", npoly,
@@ -413,7 +413,7 @@ int sg_pbvh_draw_buffers_build(PBVH *p, const int *face_sets, int npoly, int see
 // update gathered and those marked on the host (PBVH_UpdateDrawBuffers /
 // PBVH_RebuildDrawBuffers, cleared here as pbvh_update_draw_buffers clears them).
 int sg_pbvh_draw_buffers_update(PBVH *p, int all) {
-  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: draw buffers of PBVH_GRIDS are not built");
+  if (p && p->type != PBVH_FACES) return sg_set_error("faces only: draw buffers of PBVH_GRIDS / PBVH_BMESH are not built");
   if (!p) return sg_set_error("bad arguments");
   if (!p->d.draw_vbo) return sg_set_error("draw buffers not built");
   std::vector<int> marked;

@@ -1,0 +1,1630 @@
+// Dynamic topology (PBVH_BMESH) on the GPU: the corner table, leaf face slabs and
+// work lists in HBM; a topology update runs in rounds (pbvh_bmesh.c:1931-2005 with the
+// queue rules of :773-1090, the split of :1093-1209 and the collapse of :1264-1429),
+// then the dab's brush over the gathered leaves' own vertices, the normals of flagged
+// leaves (pbvh_bmesh.c:1635-1656) and their bounds.
+//
+// One round, each step a kernel over a list:
+//   begin    counters of the round
+//   cand     faces of the working set (and the last round's high-valence opposite
+//            vertices): the queue's edges, each registered once (corner stamp); the
+//            registrant clears the claims of the faces the edge would own
+//   claim    validity, the edge's key (length band, then a hash of its labels),
+//            atomicMax on every owned face; equal maxima recorded per face
+//   verify   an edge that holds every owned face's unshared maximum wins
+//   apply    winners rewrite only faces they own (a split's two faces, a collapse's
+//            faces around both vertices); a split records where its outer corners moved
+//   relink / owner   splits join their outer edges; collapses fix vertex owners
+// The host reads the round's counters (one wait per round) and stops when no edge
+// qualifies, none wins, or a capacity limit is reached.  Slots are allocated with
+// atomics in any order: results depend on labels only, and match the oracle.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "sg_device.h"
+#include "sg_dyn.h"
+#include "sg_kernels.h"
+
+#define DY_THREADS 256
+#define DY_BLOCKS 512
+
+namespace {
+
+struct DV {
+  int Vcap, Tcap, N, L, has_mask, respect_hide, use_frontface, use_projected, ccap;
+  float vnx, vny, vnz;
+  float *px, *py, *pz, *nx, *ny, *nz, *mask;
+  const uint8_t *vhide;
+  unsigned long long *lab;
+  int *vnode, *vcorner, *vstamp, *astamp, *aslot, *hvst;
+  int *tv, *topp, *tnode, *tslot, *wstamp;
+  unsigned long long *tie;
+  float *fnx, *fny, *fnz;
+  uint8_t *fhide;
+  unsigned long long *claim;
+  int *estamp, *remap, *remap_ser;
+  int *node_flag, *node_child, *node_dirty, *leaf_node, *lvl_off, *lvl_node;
+  float *node_bb, *node_obb;
+  int *slab, *scount;
+  int *wlist, *cand, *win, *outer, *hv_in, *hv_out, *touched, *gathered, *vlist, *aff, *nlist;
+  unsigned long long *key;
+  uint8_t *valid;
+  float *fade, *buf0, *buf1;
+  long long *ctl;
+};
+
+const int TOUCH = PBVH_UpdateNormals | PBVH_UpdateBB | PBVH_UpdateDrawBuffers;
+
+__device__ __forceinline__ int dn(int c) { return (c % 3 == 2) ? c - 2 : c + 1; }
+__device__ __forceinline__ int dp(int c) { return (c % 3 == 0) ? c + 2 : c - 1; }
+__device__ __forceinline__ unsigned long long ull(long long x) { return (unsigned long long)x; }
+
+__device__ __forceinline__ float len2(const DV &g, int a, int b) {
+  const float x = g.px[a] - g.px[b], y = g.py[a] - g.py[b], z = g.pz[a] - g.pz[b];
+  return x * x + y * y + z * z;
+}
+__device__ __forceinline__ float edge_len2(const DV &g, int c) { return len2(g, g.tv[dn(c)], g.tv[dp(c)]); }
+
+// splitmix64's finalizer (Steele, Lea, Flood, OOPSLA 2014)
+__device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
+  x ^= x >> 30;
+  x *= 0xbf58476d1ce4e5b9ULL;
+  x ^= x >> 27;
+  x *= 0x94d049bb133111ebULL;
+  x ^= x >> 31;
+  return x;
+}
+__device__ __forceinline__ unsigned long long new_label(unsigned long long a, unsigned long long b, int serial,
+                                                        int round) {
+  const unsigned long long lo = a < b ? a : b, hi = a < b ? b : a;
+  return mix64(lo ^ mix64(hi ^ mix64(((unsigned long long)(unsigned)serial << 20) ^ (unsigned long long)(unsigned)round))) |
+         (1ULL << 63);
+}
+__device__ __forceinline__ unsigned long long edge_hash33(unsigned long long a, unsigned long long b) {
+  const unsigned long long lo = a < b ? a : b, hi = a < b ? b : a;
+  return mix64(lo * 0x9E3779B97F4A7C15ULL ^ hi) >> 31;
+}
+
+// Priority within a round: the squared length's band (exponent and two mantissa
+// bits), longest first for splits, shortest first for collapses, then the label hash.
+__device__ __forceinline__ unsigned long long edge_key(const DV &g, int c, int collapse) {
+  const unsigned band = __float_as_uint(edge_len2(g, c)) >> 21;
+  const unsigned long long h = edge_hash33(g.lab[g.tv[dn(c)]], g.lab[g.tv[dp(c)]]);
+  return ((unsigned long long)(collapse ? 0x3FFu - band : band) << 33) | h;
+}
+
+// The first corner of v's fan (rewound over its boundary), -1 for a dead vertex, -2
+// past the walk bound.
+__device__ __forceinline__ int ring_start(const DV &g, int v, int &bnd) {
+  bnd = 0;
+  const int c0 = g.vcorner[v];
+  if (c0 < 0) return -1;
+  int c = c0, steps = 0;
+  for (;;) {
+    const int x = g.topp[dn(c)];
+    if (x < 0) {
+      bnd = 1;
+      break;
+    }
+    c = dn(x);
+    if (c == c0) break;
+    if (++steps > SG_DYN_RING) return -2;
+  }
+  return c;
+}
+
+// f(corner, i) for the corners around v in fan order; the count, or -1 past the bound
+// (f may then have seen part of the ring).
+template <class F>
+__device__ __forceinline__ int ring_for(const DV &g, int v, int &bnd, F f) {
+  const int start = ring_start(g, v, bnd);
+  if (start == -1) return 0;
+  if (start == -2) return -1;
+  int c = start, n = 0;
+  for (;;) {
+    if (n >= SG_DYN_RING) return -1;
+    f(c, n);
+    n++;
+    const int x = g.topp[dp(c)];
+    if (x < 0) break;
+    c = dp(x);
+    if (c == start) break;
+  }
+  return n;
+}
+
+__device__ __forceinline__ int valence(const DV &g, int v) {
+  int bnd;
+  const int n = ring_for(g, v, bnd, [](int, int) {});
+  return n < 0 ? SG_DYN_RING + 1 : n + bnd;
+}
+
+// closest_on_tri_to_point_v3 (math_geom.cc:966-1044)
+__device__ void closest_on_tri(float r[3], const float p[3], const float v1[3], const float v2[3], const float v3[3]) {
+  float ab[3], ac[3], ap[3], bp[3], cp[3];
+  for (int i = 0; i < 3; i++) {
+    ab[i] = v2[i] - v1[i];
+    ac[i] = v3[i] - v1[i];
+    ap[i] = p[i] - v1[i];
+  }
+  const float d1 = ab[0] * ap[0] + ab[1] * ap[1] + ab[2] * ap[2];
+  const float d2 = ac[0] * ap[0] + ac[1] * ap[1] + ac[2] * ap[2];
+  if (d1 <= 0.0f && d2 <= 0.0f) {
+    for (int i = 0; i < 3; i++) r[i] = v1[i];
+    return;
+  }
+  for (int i = 0; i < 3; i++) bp[i] = p[i] - v2[i];
+  const float d3 = ab[0] * bp[0] + ab[1] * bp[1] + ab[2] * bp[2];
+  const float d4 = ac[0] * bp[0] + ac[1] * bp[1] + ac[2] * bp[2];
+  if (d3 >= 0.0f && d4 <= d3) {
+    for (int i = 0; i < 3; i++) r[i] = v2[i];
+    return;
+  }
+  const float vc = d1 * d4 - d3 * d2;
+  if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+    const float v = d1 / (d1 - d3);
+    for (int i = 0; i < 3; i++) r[i] = v1[i] + ab[i] * v;
+    return;
+  }
+  for (int i = 0; i < 3; i++) cp[i] = p[i] - v3[i];
+  const float d5 = ab[0] * cp[0] + ab[1] * cp[1] + ab[2] * cp[2];
+  const float d6 = ac[0] * cp[0] + ac[1] * cp[1] + ac[2] * cp[2];
+  if (d6 >= 0.0f && d5 <= d6) {
+    for (int i = 0; i < 3; i++) r[i] = v3[i];
+    return;
+  }
+  const float vb = d5 * d2 - d1 * d6;
+  if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+    const float w = d2 / (d2 - d6);
+    for (int i = 0; i < 3; i++) r[i] = v1[i] + ac[i] * w;
+    return;
+  }
+  const float va = d3 * d6 - d5 * d4;
+  if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+    const float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    for (int i = 0; i < 3; i++) r[i] = (v3[i] - v2[i]) * w + v2[i];
+    return;
+  }
+  const float denom = 1.0f / (va + vb + vc);
+  const float v = vb * denom;
+  const float w = vc * denom;
+  for (int i = 0; i < 3; i++) r[i] = (v1[i] + ab[i] * v) + ac[i] * w;
+}
+
+// project_plane_normalized_v3_v3v3 (math_vector.c:635-641)
+__device__ __forceinline__ void project_plane(const DV &g, float out[3], const float p[3]) {
+  const float n[3] = {g.vnx, g.vny, g.vnz};
+  const float mul = p[0] * n[0] + p[1] * n[1] + p[2] * n[2];
+  for (int i = 0; i < 3; i++) out[i] = p[i] + n[i] * -mul;
+}
+
+struct Range {
+  float c[3], cp[3], r2;
+};
+
+// edge_queue_tri_in_sphere / _in_circle (pbvh_bmesh.c:773-805)
+__device__ bool tri_in_range(const DV &g, int f, const Range &R) {
+  float v[3][3];
+  for (int k = 0; k < 3; k++) {
+    const int u = g.tv[3 * f + k];
+    v[k][0] = g.px[u];
+    v[k][1] = g.py[u];
+    v[k][2] = g.pz[u];
+  }
+  float c[3];
+  if (g.use_projected) {
+    float p[3][3];
+    for (int k = 0; k < 3; k++) project_plane(g, p[k], v[k]);
+    closest_on_tri(c, R.cp, p[0], p[1], p[2]);
+    const float x = R.cp[0] - c[0], y = R.cp[1] - c[1], z = R.cp[2] - c[2];
+    return x * x + y * y + z * z <= R.r2;
+  }
+  closest_on_tri(c, R.c, v[0], v[1], v[2]);
+  const float x = R.c[0] - c[0], y = R.c[1] - c[1], z = R.c[2] - c[2];
+  return x * x + y * y + z * z <= R.r2;
+}
+
+__device__ __forceinline__ bool front_ok(const DV &g, int f) {
+  if (!g.use_frontface) return true;
+  return !(g.fnx[f] * g.vnx + g.fny[f] * g.vny + g.fnz[f] * g.vnz < 0.0f);
+}
+
+// edge_queue_insert's filter (pbvh_bmesh.c:812-833)
+__device__ __forceinline__ bool edge_insert_ok(const DV &g, int c) {
+  const int a = g.tv[dn(c)], b = g.tv[dp(c)];
+  if (g.has_mask && !(g.mask[a] < 1.0f || g.mask[b] < 1.0f)) return false;
+  return !(g.vhide[a] || g.vhide[b]);
+}
+
+__device__ __forceinline__ int canon(const DV &g, int c) {
+  const int x = g.topp[c];
+  return (x >= 0 && x < c) ? x : c;
+}
+
+// The faces an edge's operation owns: a split its two faces; a collapse the faces
+// around both vertices (the edge's faces once).  f(face) for each; false when a walk
+// fails.
+template <class F>
+__device__ bool claim_set(const DV &g, int c, int collapse, F f) {
+  if (!collapse) {
+    f(c / 3);
+    const int cB = g.topp[c];
+    if (cB >= 0) f(cB / 3);
+    return true;
+  }
+  const int p = g.tv[dn(c)], q = g.tv[dp(c)];
+  int bnd;
+  if (ring_for(g, p, bnd, [&](int k, int) { f(k / 3); }) < 0) return false;
+  const int n = ring_for(g, q, bnd, [&](int k, int) {
+    const int t = k / 3;
+    if (g.tv[3 * t] != p && g.tv[3 * t + 1] != p && g.tv[3 * t + 2] != p) f(t);
+  });
+  return n >= 0;
+}
+
+__device__ void reg_edge(const DV &g, int c, int collapse, int stamp) {
+  if (!edge_insert_ok(g, c)) return;
+  const int k = canon(g, c);
+  if (atomicExch(&g.estamp[k], stamp) == stamp) return;
+  const unsigned long long slot = atomicAdd((unsigned long long *)&g.ctl[SGD_NCAND], 1ull);
+  if (slot < (unsigned long long)g.ccap) g.cand[slot] = k;
+  claim_set(g, k, collapse, [&](int t) {
+    g.claim[t] = 0ull;
+    g.tie[t] = 0ull;
+  });
+}
+
+// long_edge_queue_edge_add_recursive (pbvh_bmesh.c:849-903), iterative
+__device__ int reg_even(const DV &g, int c0, float l0, float lim0, int stamp) {
+  int sc[SG_DYN_STACK];
+  float sl[SG_DYN_STACK], sm[SG_DYN_STACK];
+  int top = 0;
+  sc[0] = c0;
+  sl[0] = l0;
+  sm[0] = lim0;
+  top = 1;
+  while (top) {
+    top--;
+    const int c = sc[top];
+    const float l2 = sl[top];
+    float limit = sm[top];
+    const int gc = g.topp[c];
+    const int entered = gc >= 0 ? gc / 3 : c / 3;
+    if (!front_ok(g, entered)) continue;
+    reg_edge(g, c, 0, stamp);
+    if (gc < 0) continue;
+    const float cmp = l2 * 1.2f;
+    limit = limit * 1.6f;
+    const float limit2 = limit * limit;
+    const int ks[2] = {dn(gc), dp(gc)};
+    for (int i = 0; i < 2; i++) {
+      const float lo = edge_len2(g, ks[i]);
+      if (lo > sg_max(cmp, limit2)) {
+        if (top >= SG_DYN_STACK) return -1;
+        sc[top] = ks[i];
+        sl[top] = lo;
+        sm[top] = limit;
+        top++;
+      }
+    }
+  }
+  return 0;
+}
+
+// Whether collapsing the edge opposite c keeps the surface manifold (link condition,
+// no inner edge between two boundaries, no ears, opposite vertices keep enough edges).
+__device__ bool collapse_ok(const DV &g, int c) {
+  const int cB = g.topp[c];
+  const int p = g.tv[dn(c)], q = g.tv[dp(c)], oA = g.tv[c];
+  const int oB = cB >= 0 ? g.tv[cB] : -1;
+  if (g.vnode[p] < 0 || g.vnode[q] < 0) return false;
+  int bp, bq;
+  if (ring_for(g, p, bp, [](int, int) {}) < 0 || ring_for(g, q, bq, [](int, int) {}) < 0) return false;
+  if (cB >= 0 && bp && bq) return false;
+  if (g.topp[dn(c)] < 0 && g.topp[dp(c)] < 0) return false;
+  if (cB >= 0 && g.topp[dn(cB)] < 0 && g.topp[dp(cB)] < 0) return false;
+  // common neighbours: each neighbour x of q, tested against p's neighbours
+  int common = 0;
+  bool bad = false;
+  auto nbr_of_p = [&](int x) {
+    bool hit = false;
+    int b2, first = -1;
+    ring_for(g, p, b2, [&](int k, int i) {
+      if (i == 0) first = k;
+      if (g.tv[dn(k)] == x) hit = true;
+    });
+    if (b2 && first >= 0 && g.tv[dp(first)] == x) hit = true;
+    return hit;
+  };
+  int b3, first = -1;
+  ring_for(g, q, b3, [&](int k, int i) {
+    if (i == 0) first = k;
+    const int x = g.tv[dn(k)];
+    if (nbr_of_p(x)) {
+      if (x != oA && x != oB) bad = true;
+      common++;
+    }
+  });
+  if (b3 && first >= 0) {
+    const int x = g.tv[dp(first)];
+    if (nbr_of_p(x)) {
+      if (x != oA && x != oB) bad = true;
+      common++;
+    }
+  }
+  if (bad || common != (cB >= 0 ? 2 : 1)) return false;
+  int bA;
+  const int nA = ring_for(g, oA, bA, [](int, int) {});
+  const int vA = nA < 0 ? SG_DYN_RING + 1 : nA + bA;
+  if (vA - 1 < (bA ? 2 : 3)) return false;
+  if (oB >= 0) {
+    int bB;
+    const int nB = ring_for(g, oB, bB, [](int, int) {});
+    const int vB = nB < 0 ? SG_DYN_RING + 1 : nB + bB;
+    if (vB - 1 < (bB ? 2 : 3)) return false;
+  }
+  return true;
+}
+
+// ---------------------------------------------------------------------------------
+// search: a thread per leaf (spec S1 on the leaf box, fully hidden leaves skipped);
+// gathered leaves get `flags`
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(DY_THREADS) void kd_search(DV g, float cx, float cy, float cz, float r2, int flags) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  bool pass = false;
+  int node = 0;
+  if (l < g.L) {
+    node = g.leaf_node[l];
+    const int fl = g.node_flag[node];
+    const BB6 b = bb_load(g.node_bb + 8 * (size_t)node);
+    pass = !(fl & PBVH_FullyHidden) && sphere_box(b, cx, cy, cz, r2);
+  }
+  const unsigned long long m = __ballot(pass);
+  if (m == 0) return;
+  int base = 0;
+  if (lane_id() == 0) base = (int)atomicAdd((unsigned long long *)&g.ctl[SGD_NGATHER], (unsigned long long)__popcll(m));
+  base = __shfl(base, 0);
+  if (pass) {
+    g.gathered[base + mask_prefix(m)] = node;
+    atomicOr(&g.node_flag[node], flags);
+  }
+}
+
+// working set: the live faces of leaves flagged UpdateTopology (not fully hidden)
+__global__ __launch_bounds__(DY_THREADS) void kd_wbuild(DV g, int wser) {
+  for (int l = blockIdx.x; l < g.L; l += gridDim.x) {
+    const int node = g.leaf_node[l];
+    const int fl = g.node_flag[node];
+    if (!(fl & PBVH_UpdateTopology) || (fl & PBVH_FullyHidden)) continue;
+    if (threadIdx.x == 0) atomicAdd((unsigned long long *)&g.ctl[SGD_NLIST], 1ull);
+    const int n = g.scount[node];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      const int f = g.slab[(size_t)node * SG_DYN_SLAB + i];
+      if (f < 0) continue;
+      if (atomicExch(&g.wstamp[f], wser) != wser) {
+        const unsigned long long s = atomicAdd((unsigned long long *)&g.ctl[SGD_WN], 1ull);
+        g.wlist[s] = f;
+      }
+    }
+  }
+}
+
+__global__ void kd_round_begin(DV g, int first) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    g.ctl[SGD_HV] = first ? 0 : g.ctl[SGD_HV2];
+    g.ctl[SGD_HV2] = 0;
+    g.ctl[SGD_NCAND] = 0;
+    g.ctl[SGD_NVALID] = 0;
+    g.ctl[SGD_NWIN] = 0;
+    g.ctl[SGD_NTOUCH] = 0;
+  }
+}
+
+__global__ __launch_bounds__(DY_THREADS) void kd_cand(DV g, int collapse, Range R, float limit, float limit2,
+                                                       int stamp) {
+  const int wn = (int)g.ctl[SGD_WN];
+  const int hn = collapse ? 0 : (int)g.ctl[SGD_HV];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < wn + hn; i += gridDim.x * blockDim.x) {
+    if (i < wn) {
+      const int f = g.wlist[i];
+      if (g.tnode[f] < 0) continue;
+      if (!front_ok(g, f) || !tri_in_range(g, f, R)) continue;
+      for (int k = 0; k < 3; k++) {
+        const int c = 3 * f + k;
+        const float l2 = edge_len2(g, c);
+        if (collapse) {
+          if (l2 < limit2) reg_edge(g, c, 1, stamp);
+        } else if (l2 > limit2) {
+          if (reg_even(g, c, l2, limit, stamp) < 0) atomicOr((unsigned long long *)&g.ctl[SGD_FLAGS], 2ull);
+        }
+      }
+    } else {
+      // an opposite vertex of the last round's splits with more than 8 edges: its long
+      // edges (pbvh_bmesh.c:1191-1198)
+      const int v = g.hv_in[i - wn];
+      int b0;
+      const int n0 = ring_for(g, v, b0, [](int, int) {});
+      if (n0 < 0 || n0 + b0 <= 8) continue;
+      int bnd;
+      ring_for(g, v, bnd, [&](int k, int j) {
+        const int e1 = dp(k);
+        if (edge_len2(g, e1) > limit2) reg_edge(g, e1, 0, stamp);
+        if (bnd && j == 0) {
+          const int e2 = dn(k);
+          if (edge_len2(g, e2) > limit2) reg_edge(g, e2, 0, stamp);
+        }
+      });
+    }
+  }
+}
+
+__global__ __launch_bounds__(DY_THREADS) void kd_claim(DV g, int collapse) {
+  const int n = (int)min(g.ctl[SGD_NCAND], (long long)g.ccap);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int c = g.cand[i];
+    const int p = g.tv[dn(c)], q = g.tv[dp(c)];
+    bool ok = g.vnode[p] >= 0 && g.vnode[q] >= 0 && g.tnode[c / 3] >= 0;
+    if (ok && collapse) ok = collapse_ok(g, c);
+    if (ok && !collapse) {
+      const int cB = g.topp[c];
+      ok = g.scount[g.tnode[c / 3]] <= SG_DYN_SLAB / 2 &&
+           (cB < 0 || (g.tnode[cB / 3] >= 0 && g.scount[g.tnode[cB / 3]] <= SG_DYN_SLAB / 2));
+    }
+    g.valid[i] = ok ? 1 : 0;
+    if (!ok) continue;
+    atomicAdd((unsigned long long *)&g.ctl[SGD_NVALID], 1ull);
+    const unsigned long long k = edge_key(g, c, collapse);
+    g.key[i] = k;
+    claim_set(g, c, collapse, [&](int t) {
+      const unsigned long long old = atomicMax(&g.claim[t], k);
+      if (old == k) atomicMax(&g.tie[t], k);
+    });
+  }
+}
+
+__global__ __launch_bounds__(DY_THREADS) void kd_verify(DV g, int collapse) {
+  const int n = (int)min(g.ctl[SGD_NCAND], (long long)g.ccap);
+  if (!collapse) {
+    // capacity for the round's new vertices and faces (pbvh_bmesh_vert_create /
+    // face_create have no limit; these arrays do)
+    const long long nv = g.ctl[SGD_NVALID];
+    if (g.ctl[SGD_V] + nv > g.Vcap || g.ctl[SGD_T] + 2 * nv > g.Tcap) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr((unsigned long long *)&g.ctl[SGD_FLAGS], 1ull);
+      return;
+    }
+  }
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (!g.valid[i]) continue;
+    const unsigned long long k = g.key[i];
+    bool own = true;
+    claim_set(g, g.cand[i], collapse, [&](int t) { own = own && g.claim[t] == k && g.tie[t] != k; });
+    if (own) {
+      const unsigned long long s = atomicAdd((unsigned long long *)&g.ctl[SGD_NWIN], 1ull);
+      g.win[s] = g.cand[i];
+    }
+  }
+}
+
+__device__ __forceinline__ void slab_append(const DV &g, int node, int f) {
+  const int pos = atomicAdd(&g.scount[node], 1);
+  g.slab[(size_t)node * SG_DYN_SLAB + pos] = f;
+  g.tnode[f] = node;
+  g.tslot[f] = pos;
+}
+
+__device__ __forceinline__ void w_add(const DV &g, int f, int wser) {
+  if (atomicExch(&g.wstamp[f], wser) != wser) {
+    const unsigned long long s = atomicAdd((unsigned long long *)&g.ctl[SGD_WN], 1ull);
+    g.wlist[s] = f;
+  }
+}
+
+__device__ __forceinline__ void hv_add(const DV &g, int v, int stamp) {
+  if (v < 0) return;
+  if (atomicExch(&g.hvst[v], stamp) != stamp) {
+    const unsigned long long s = atomicAdd((unsigned long long *)&g.ctl[SGD_HV2], 1ull);
+    g.hv_out[s] = v;
+  }
+}
+
+// Split the edge opposite cA at a new vertex m (pbvh_bmesh.c:1093-1209): A (p, q, oA)
+// in its winding becomes A' = (p, m, oA) in its slot and NA = (m, q, oA); B likewise.
+// Outer corners keep their old neighbour ids and record where they moved (remap).
+__device__ void split_apply(const DV &g, int wi, int cA, int serial, int round, int stamp, int wser) {
+  const int cB = g.topp[cA];
+  const int A = cA / 3, B = cB >= 0 ? cB / 3 : -1;
+  const int p = g.tv[dn(cA)], q = g.tv[dp(cA)], oA = g.tv[cA];
+  const int oB = cB >= 0 ? g.tv[cB] : -1;
+  const int yA = g.topp[dp(cA)], xA = g.topp[dn(cA)];
+  const int yB = cB >= 0 ? g.topp[dp(cB)] : -1, xB = cB >= 0 ? g.topp[dn(cB)] : -1;
+  const int oldAy = dp(cA), oldAx = dn(cA), oldBy = cB >= 0 ? dp(cB) : -1, oldBx = cB >= 0 ? dn(cB) : -1;
+  const int m = (int)atomicAdd((unsigned long long *)&g.ctl[SGD_V], 1ull);
+  const int NA = (int)atomicAdd((unsigned long long *)&g.ctl[SGD_T], B >= 0 ? 2ull : 1ull);
+  const int NB = B >= 0 ? NA + 1 : -1;
+  g.px[m] = 0.5f * (g.px[p] + g.px[q]);
+  g.py[m] = 0.5f * (g.py[p] + g.py[q]);
+  g.pz[m] = 0.5f * (g.pz[p] + g.pz[q]);
+  float nx = 0.5f * (g.nx[p] + g.nx[q]), ny = 0.5f * (g.ny[p] + g.ny[q]), nz = 0.5f * (g.nz[p] + g.nz[q]);
+  sg_normalize(nx, ny, nz);
+  g.nx[m] = nx;
+  g.ny[m] = ny;
+  g.nz[m] = nz;
+  if (g.has_mask) g.mask[m] = 0.5f * (g.mask[p] + g.mask[q]);
+  ((uint8_t *)g.vhide)[m] = 0;
+  g.lab[m] = new_label(g.lab[p], g.lab[q], serial, round);
+  const int leafA = g.tnode[A], leafB = B >= 0 ? g.tnode[B] : -1;
+  g.vnode[m] = (B >= 0 && g.lab[oB] < g.lab[oA]) ? leafB : leafA;
+  int *tv = g.tv, *tp = g.topp;
+  tv[3 * A] = p;
+  tv[3 * A + 1] = m;
+  tv[3 * A + 2] = oA;
+  tv[3 * NA] = m;
+  tv[3 * NA + 1] = q;
+  tv[3 * NA + 2] = oA;
+  tp[3 * A] = 3 * NA + 1;
+  tp[3 * NA + 1] = 3 * A;
+  tp[3 * A + 1] = yA;
+  tp[3 * NA] = xA;
+  g.remap[oldAy] = 3 * A + 1;
+  g.remap_ser[oldAy] = stamp;
+  g.remap[oldAx] = 3 * NA;
+  g.remap_ser[oldAx] = stamp;
+  int4 out = make_int4(3 * A + 1, 3 * NA, -1, -1);
+  if (B >= 0) {
+    tv[3 * B] = q;
+    tv[3 * B + 1] = m;
+    tv[3 * B + 2] = oB;
+    tv[3 * NB] = m;
+    tv[3 * NB + 1] = p;
+    tv[3 * NB + 2] = oB;
+    tp[3 * B] = 3 * NB + 1;
+    tp[3 * NB + 1] = 3 * B;
+    tp[3 * B + 1] = yB;
+    tp[3 * NB] = xB;
+    g.remap[oldBy] = 3 * B + 1;
+    g.remap_ser[oldBy] = stamp;
+    g.remap[oldBx] = 3 * NB;
+    g.remap_ser[oldBx] = stamp;
+    out.z = 3 * B + 1;
+    out.w = 3 * NB;
+    tp[3 * A + 2] = 3 * NB + 2;
+    tp[3 * NB + 2] = 3 * A + 2;
+    tp[3 * NA + 2] = 3 * B + 2;
+    tp[3 * B + 2] = 3 * NA + 2;
+  } else {
+    tp[3 * A + 2] = -1;
+    tp[3 * NA + 2] = -1;
+  }
+  ((int4 *)g.outer)[wi] = out;
+  g.fnx[NA] = g.fnx[A];
+  g.fny[NA] = g.fny[A];
+  g.fnz[NA] = g.fnz[A];
+  g.fhide[NA] = g.fhide[A];
+  slab_append(g, leafA, NA);
+  atomicOr(&g.node_flag[leafA], TOUCH);
+  atomicAnd(&g.node_flag[leafA], ~PBVH_FullyHidden);
+  if (B >= 0) {
+    g.fnx[NB] = g.fnx[B];
+    g.fny[NB] = g.fny[B];
+    g.fnz[NB] = g.fnz[B];
+    g.fhide[NB] = g.fhide[B];
+    slab_append(g, leafB, NB);
+    atomicOr(&g.node_flag[leafB], TOUCH);
+    atomicAnd(&g.node_flag[leafB], ~PBVH_FullyHidden);
+  }
+  const int vs[4] = {p, q, oA, oB};
+  for (int i = 0; i < 4; i++) {
+    const int v = vs[i];
+    if (v < 0) continue;
+    const int f = g.vcorner[v] / 3;
+    if (f == A)
+      g.vcorner[v] = (v == p) ? 3 * A : (v == q) ? 3 * NA + 1 : 3 * A + 2;
+    else if (B >= 0 && f == B)
+      g.vcorner[v] = (v == q) ? 3 * B : (v == p) ? 3 * NB + 1 : 3 * B + 2;
+  }
+  g.vcorner[m] = 3 * A + 1;
+  w_add(g, A, wser);
+  w_add(g, NA, wser);
+  if (B >= 0) {
+    w_add(g, B, wser);
+    w_add(g, NB, wser);
+  }
+  hv_add(g, oA, stamp);
+  hv_add(g, oB, stamp);
+}
+
+// Collapse the edge opposite cA (pbvh_bmesh.c:1264-1429): the lower-mask vertex goes
+// (v1 = the lower label), the other moves to the midpoint.
+__device__ void collapse_apply(const DV &g, int cA) {
+  const int cB = g.topp[cA];
+  const int A = cA / 3, B = cB >= 0 ? cB / 3 : -1;
+  const int p = g.tv[dn(cA)], q = g.tv[dp(cA)], oA = g.tv[cA];
+  const int oB = cB >= 0 ? g.tv[cB] : -1;
+  const int v1 = g.lab[p] < g.lab[q] ? p : q, v2 = v1 == p ? q : p;
+  const float m1 = g.has_mask ? g.mask[v1] : 0.0f, m2 = g.has_mask ? g.mask[v2] : 0.0f;
+  const int vdel = (m1 < m2) ? v1 : v2, vcon = vdel == v1 ? v2 : v1;
+  const int xA = g.topp[dn(cA)], yA = g.topp[dp(cA)];
+  const int xB = cB >= 0 ? g.topp[dn(cB)] : -1, yB = cB >= 0 ? g.topp[dp(cB)] : -1;
+  atomicOr(&g.node_flag[g.tnode[A]], TOUCH);
+  if (B >= 0) atomicOr(&g.node_flag[g.tnode[B]], TOUCH);
+  // re-point vdel's other corners (walked before any link changes)
+  int bnd;
+  ring_for(g, vdel, bnd, [&](int k, int) {
+    const int f = k / 3;
+    if (f != A && f != B) g.tv[k] = vcon;
+  });
+  if (xA >= 0) g.topp[xA] = yA;
+  if (yA >= 0) g.topp[yA] = xA;
+  if (B >= 0) {
+    if (xB >= 0) g.topp[xB] = yB;
+    if (yB >= 0) g.topp[yB] = xB;
+  }
+  const int fA = g.vcorner[oA] / 3;
+  if (fA == A || fA == B) g.vcorner[oA] = xA >= 0 ? dn(xA) : dp(yA);
+  if (B >= 0) {
+    const int fB = g.vcorner[oB] / 3;
+    if (fB == A || fB == B) g.vcorner[oB] = xB >= 0 ? dn(xB) : dp(yB);
+  }
+  const int fc = g.vcorner[vcon] / 3;
+  if (fc == A || fc == B) g.vcorner[vcon] = xA >= 0 ? dp(xA) : dn(yA);
+  g.px[vcon] = 0.5f * (g.px[vcon] + g.px[vdel]);
+  g.py[vcon] = 0.5f * (g.py[vcon] + g.py[vdel]);
+  g.pz[vcon] = 0.5f * (g.pz[vcon] + g.pz[vdel]);
+  float nx = g.nx[vcon] + g.nx[vdel], ny = g.ny[vcon] + g.ny[vdel], nz = g.nz[vcon] + g.nz[vdel];
+  sg_normalize(nx, ny, nz);
+  g.nx[vcon] = nx;
+  g.ny[vcon] = ny;
+  g.nz[vcon] = nz;
+  const int fs[2] = {A, B};
+  for (int i = 0; i < 2; i++) {
+    const int f = fs[i];
+    if (f < 0) continue;
+    g.slab[(size_t)g.tnode[f] * SG_DYN_SLAB + g.tslot[f]] = -1;
+    g.tnode[f] = -1;
+  }
+  g.vnode[vdel] = -1;
+  g.vcorner[vdel] = -1;
+  const int s = (int)atomicAdd((unsigned long long *)&g.ctl[SGD_NTOUCH], 1ull);
+  g.touched[3 * s] = vcon;
+  g.touched[3 * s + 1] = oA;
+  g.touched[3 * s + 2] = oB;
+}
+
+__global__ __launch_bounds__(DY_THREADS) void kd_apply(DV g, int collapse, int serial, int round, int stamp,
+                                                        int wser) {
+  const int n = (int)g.ctl[SGD_NWIN];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (collapse)
+      collapse_apply(g, g.win[i]);
+    else
+      split_apply(g, i, g.win[i], serial, round, stamp, wser);
+  }
+}
+
+// second phase of the splits: outer edges meet their neighbours
+__global__ __launch_bounds__(DY_THREADS) void kd_relink(DV g, int stamp) {
+  const int n = (int)g.ctl[SGD_NWIN];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int4 o = ((const int4 *)g.outer)[i];
+    const int ks[4] = {o.x, o.y, o.z, o.w};
+    for (int j = 0; j < 4; j++) {
+      const int k = ks[j];
+      if (k < 0) continue;
+      const int y = g.topp[k];
+      if (y < 0) continue;
+      if (g.remap_ser[y] == stamp)
+        g.topp[k] = g.remap[y];
+      else
+        g.topp[y] = k;
+    }
+  }
+}
+
+// after the collapses: a vertex whose owner leaf lost its last face with it goes to the
+// smallest-numbered leaf among its faces; around the kept vertex every leaf is marked
+__global__ __launch_bounds__(DY_THREADS) void kd_owner(DV g) {
+  const int n = 3 * (int)g.ctl[SGD_NTOUCH];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int v = g.touched[i];
+    if (v < 0 || g.vcorner[v] < 0) continue;
+    const bool mark = (i % 3) == 0;
+    const int own = g.vnode[v];
+    bool has = false;
+    int best = -1, bnd;
+    ring_for(g, v, bnd, [&](int k, int) {
+      const int leaf = g.tnode[k / 3];
+      if (leaf < 0) return;
+      if (mark) atomicOr(&g.node_flag[leaf], TOUCH);
+      has = has || leaf == own;
+      if (best < 0 || leaf < best) best = leaf;
+    });
+    if (!has) {
+      if (own >= 0) atomicOr(&g.node_flag[own], PBVH_UpdateBB | PBVH_UpdateDrawBuffers);
+      g.vnode[v] = best;
+      if (best >= 0) atomicOr(&g.node_flag[best], PBVH_UpdateBB | PBVH_UpdateDrawBuffers);
+    }
+  }
+}
+
+__global__ void kd_clear_topology_flags(DV g) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < g.N; i += gridDim.x * blockDim.x)
+    g.node_flag[i] &= ~PBVH_UpdateTopology;
+}
+
+// ---------------------------------------------------------------------------------
+// the brush
+// ---------------------------------------------------------------------------------
+// the gathered leaves' own vertices, each once
+__global__ __launch_bounds__(DY_THREADS) void kd_unique(DV g, int stamp) {
+  const int ng = (int)g.ctl[SGD_NGATHER];
+  for (int gi = blockIdx.x; gi < ng; gi += gridDim.x) {
+    const int node = g.gathered[gi];
+    const int n = g.scount[node];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      const int f = g.slab[(size_t)node * SG_DYN_SLAB + i];
+      if (f < 0) continue;
+      for (int k = 0; k < 3; k++) {
+        const int v = g.tv[3 * f + k];
+        if (g.vnode[v] == node && atomicExch(&g.vstamp[v], stamp) != stamp) {
+          const unsigned long long s = atomicAdd((unsigned long long *)&g.ctl[SGD_NVL], 1ull);
+          g.vlist[s] = v;
+        }
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(DY_THREADS) void kd_area(DV g, SgDabK k) {
+  const int n = (int)g.ctl[SGD_NVL];
+  long long sx = 0, sy = 0, sz = 0;
+  int na = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int v = g.vlist[i];
+    if (g.respect_hide && g.vhide[v]) continue;
+    const float dx = k.cx - g.px[v], dy = k.cy - g.py[v], dz = k.cz - g.pz[v];
+    if (dx * dx + dy * dy + dz * dz <= k.nr2) {
+      sx += __double2ll_rn((double)g.nx[v] * 4294967296.0);
+      sy += __double2ll_rn((double)g.ny[v] * 4294967296.0);
+      sz += __double2ll_rn((double)g.nz[v] * 4294967296.0);
+      na++;
+    }
+  }
+  sx = wave_sum_ll(sx);
+  sy = wave_sum_ll(sy);
+  sz = wave_sum_ll(sz);
+  na = wave_sum_i(na);
+  if (lane_id() == 0 && na) {
+    atomicAdd((unsigned long long *)&g.ctl[SGD_AREA_X], ull(sx));
+    atomicAdd((unsigned long long *)&g.ctl[SGD_AREA_Y], ull(sy));
+    atomicAdd((unsigned long long *)&g.ctl[SGD_AREA_Z], ull(sz));
+    atomicAdd((unsigned long long *)&g.ctl[SGD_NAREA], (unsigned long long)na);
+  }
+}
+
+// spec S2 test, S3/S4 factor, S6 Draw / S7 Inflate / S9 Grab; Smooth keeps its factor
+__global__ __launch_bounds__(DY_THREADS) void kd_brush(DV g, SgDabK k, const float *curve, int stamp) {
+  const int n = (int)g.ctl[SGD_NVL];
+  float ox = 0.0f, oy = 0.0f, oz = 0.0f;
+  if (k.tool == SG_TOOL_DRAW) {
+    float ax = (float)((double)g.ctl[SGD_AREA_X] * (1.0 / 4294967296.0));
+    float ay = (float)((double)g.ctl[SGD_AREA_Y] * (1.0 / 4294967296.0));
+    float az = (float)((double)g.ctl[SGD_AREA_Z] * (1.0 / 4294967296.0));
+    sg_normalize(ax, ay, az);
+    ox = (ax * k.r) * k.bstr;
+    oy = (ay * k.r) * k.bstr;
+    oz = (az * k.r) * k.bstr;
+  }
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int v = g.vlist[i];
+    if (g.respect_hide && g.vhide[v]) continue;
+    const float x = g.px[v], y = g.py[v], z = g.pz[v];
+    const float dx = k.cx - x, dy = k.cy - y, dz = k.cz - z;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    if (!(d2 <= k.r2)) continue;
+    const int s = (int)atomicAdd((unsigned long long *)&g.ctl[SGD_NAFF], 1ull);
+    g.aff[s] = v;
+    const float mk = g.has_mask ? g.mask[v] : 0.0f;
+    const float f = sg_falloff(k, curve, sqrtf(d2)) * (1.0f - mk);
+    if (k.tool == SG_TOOL_SMOOTH) {
+      g.fade[s] = k.bstr * f;
+      g.astamp[v] = stamp;
+      g.aslot[v] = s;
+      continue;
+    }
+    float rx, ry, rz;
+    if (k.tool == SG_TOOL_DRAW) {
+      rx = x + ox * f;
+      ry = y + oy * f;
+      rz = z + oz * f;
+    } else if (k.tool == SG_TOOL_INFLATE) {
+      const float sc = (k.bstr * f) * k.r;
+      rx = x + g.nx[v] * sc;
+      ry = y + g.ny[v] * sc;
+      rz = z + g.nz[v] * sc;
+    } else {
+      const float fade = k.bstr * f;
+      rx = x + k.gx * fade;
+      ry = y + k.gy * fade;
+      rz = z + k.gz * fade;
+    }
+    g.px[v] = rx;
+    g.py[v] = ry;
+    g.pz[v] = rz;
+  }
+}
+
+__global__ __launch_bounds__(DY_THREADS) void kd_smooth_init(DV g) {
+  const int n = (int)g.ctl[SGD_NAFF];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int v = g.aff[i];
+    g.buf0[3 * (size_t)i] = g.px[v];
+    g.buf0[3 * (size_t)i + 1] = g.py[v];
+    g.buf0[3 * (size_t)i + 2] = g.pz[v];
+  }
+}
+
+// one Jacobi step of spec S8: the neighbour mean as an exact 2^-32 fixed-point sum
+__global__ __launch_bounds__(DY_THREADS) void kd_smooth(DV g, const float *src, float *dst, int stamp) {
+  const int n = (int)g.ctl[SGD_NAFF];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int v = g.aff[i];
+    long long s0 = 0, s1 = 0, s2 = 0;
+    auto acc = [&](int u) {
+      float x, y, z;
+      if (g.astamp[u] == stamp) {
+        const int j = g.aslot[u];
+        x = src[3 * (size_t)j];
+        y = src[3 * (size_t)j + 1];
+        z = src[3 * (size_t)j + 2];
+      } else {
+        x = g.px[u];
+        y = g.py[u];
+        z = g.pz[u];
+      }
+      s0 += __double2ll_rn((double)x * 4294967296.0);
+      s1 += __double2ll_rn((double)y * 4294967296.0);
+      s2 += __double2ll_rn((double)z * 4294967296.0);
+    };
+    int bnd, first = -1;
+    int cnt = ring_for(g, v, bnd, [&](int k, int j) {
+      if (j == 0) first = k;
+      acc(g.tv[dn(k)]);
+    });
+    if (cnt > 0 && bnd) {
+      acc(g.tv[dp(first)]);
+      cnt++;
+    }
+    const float sx = src[3 * (size_t)i], sy = src[3 * (size_t)i + 1], sz = src[3 * (size_t)i + 2];
+    float rx = sx, ry = sy, rz = sz;
+    if (cnt > 0) {
+      const float fade = g.fade[i];
+      const float ax = (float)(((double)s0 * (1.0 / 4294967296.0)) / (double)cnt);
+      const float ay = (float)(((double)s1 * (1.0 / 4294967296.0)) / (double)cnt);
+      const float az = (float)(((double)s2 * (1.0 / 4294967296.0)) / (double)cnt);
+      rx = sx + (ax - sx) * fade;
+      ry = sy + (ay - sy) * fade;
+      rz = sz + (az - sz) * fade;
+    }
+    dst[3 * (size_t)i] = rx;
+    dst[3 * (size_t)i + 1] = ry;
+    dst[3 * (size_t)i + 2] = rz;
+  }
+}
+
+__global__ __launch_bounds__(DY_THREADS) void kd_smooth_store(DV g, const float *src) {
+  const int n = (int)g.ctl[SGD_NAFF];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int v = g.aff[i];
+    g.px[v] = src[3 * (size_t)i];
+    g.py[v] = src[3 * (size_t)i + 1];
+    g.pz[v] = src[3 * (size_t)i + 2];
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// normals and bounds of flagged leaves
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(DY_THREADS) void kd_flagged(DV g, int mask) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  bool pass = false;
+  int node = 0;
+  if (l < g.L) {
+    node = g.leaf_node[l];
+    pass = (g.node_flag[node] & mask) != 0;
+  }
+  const unsigned long long m = __ballot(pass);
+  if (m == 0) return;
+  int base = 0;
+  if (lane_id() == 0) base = (int)atomicAdd((unsigned long long *)&g.ctl[SGD_NLIST], (unsigned long long)__popcll(m));
+  base = __shfl(base, 0);
+  if (pass) g.nlist[base + mask_prefix(m)] = node;
+}
+
+// face normals (normal_tri_v3) of the listed leaves' faces; their vertices, each once
+__global__ __launch_bounds__(DY_THREADS) void kd_face_normals(DV g, int stamp) {
+  const int nl = (int)g.ctl[SGD_NLIST];
+  for (int li = blockIdx.x; li < nl; li += gridDim.x) {
+    const int node = g.nlist[li];
+    const int n = g.scount[node];
+    int nf = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      const int f = g.slab[(size_t)node * SG_DYN_SLAB + i];
+      if (f < 0) continue;
+      nf++;
+      const int a = g.tv[3 * f], b = g.tv[3 * f + 1], c = g.tv[3 * f + 2];
+      float x, y, z;
+      sg_tri_normal(g.px[a], g.py[a], g.pz[a], g.px[b], g.py[b], g.pz[b], g.px[c], g.py[c], g.pz[c], x, y, z);
+      g.fnx[f] = x;
+      g.fny[f] = y;
+      g.fnz[f] = z;
+      const int vs[3] = {a, b, c};
+      for (int k = 0; k < 3; k++) {
+        if (atomicExch(&g.vstamp[vs[k]], stamp) != stamp) {
+          const unsigned long long s = atomicAdd((unsigned long long *)&g.ctl[SGD_NVL], 1ull);
+          g.vlist[s] = vs[k];
+        }
+      }
+    }
+    nf = wave_sum_i(nf);
+    if (lane_id() == 0 && nf) atomicAdd((unsigned long long *)&g.ctl[SGD_NFN], (unsigned long long)nf);
+    if (threadIdx.x == 0) atomicAnd(&g.node_flag[node], ~PBVH_UpdateNormals);
+  }
+}
+
+// mesh_vert_calc_normal (mesh_polygon.c:591-634): corner angle x face normal,
+// normalized; exact fixed-point sum; saacos in double
+__global__ __launch_bounds__(DY_THREADS) void kd_vert_normals(DV g) {
+  const int n = (int)g.ctl[SGD_NVL];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int v = g.vlist[i];
+    long long s0 = 0, s1 = 0, s2 = 0;
+    const float vx = g.px[v], vy = g.py[v], vz = g.pz[v];
+    int bnd;
+    const int cnt = ring_for(g, v, bnd, [&](int c, int) {
+      const int a = g.tv[dp(c)], b = g.tv[dn(c)];
+      float e1x = vx - g.px[a], e1y = vy - g.py[a], e1z = vz - g.pz[a];
+      float e2x = g.px[b] - vx, e2y = g.py[b] - vy, e2z = g.pz[b] - vz;
+      sg_normalize(e1x, e1y, e1z);
+      sg_normalize(e2x, e2y, e2z);
+      const float dot = -(e1x * e2x + e1y * e2y + e1z * e2z);
+      const float fac = (dot <= -1.0f) ? 3.14159265358979323846f : ((dot >= 1.0f) ? 0.0f : (float)acos((double)dot));
+      const int f = c / 3;
+      s0 += __double2ll_rn((double)(g.fnx[f] * fac) * 4294967296.0);
+      s1 += __double2ll_rn((double)(g.fny[f] * fac) * 4294967296.0);
+      s2 += __double2ll_rn((double)(g.fnz[f] * fac) * 4294967296.0);
+    });
+    if (cnt <= 0) continue;
+    float x = (float)((double)s0 * (1.0 / 4294967296.0));
+    float y = (float)((double)s1 * (1.0 / 4294967296.0));
+    float z = (float)((double)s2 * (1.0 / 4294967296.0));
+    sg_normalize(x, y, z);
+    g.nx[v] = x;
+    g.ny[v] = y;
+    g.nz[v] = z;
+  }
+}
+
+// leaf boxes over the leaf faces' vertices (zeros as +0): a workgroup per listed leaf
+__global__ __launch_bounds__(DY_THREADS) void kd_leaf_bounds(DV g) {
+  __shared__ float s_mn[3][DY_THREADS / 64], s_mx[3][DY_THREADS / 64];
+  const int nl = (int)g.ctl[SGD_NLIST];
+  for (int li = blockIdx.x; li < nl; li += gridDim.x) {
+    const int node = g.nlist[li];
+    const int fl = g.node_flag[node];
+    if (fl & PBVH_UpdateBB) {
+      float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+      const int n = g.scount[node];
+      for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int f = g.slab[(size_t)node * SG_DYN_SLAB + i];
+        if (f < 0) continue;
+        for (int k = 0; k < 3; k++) {
+          const int v = g.tv[3 * f + k];
+          const float q[3] = {g.px[v] + 0.0f, g.py[v] + 0.0f, g.pz[v] + 0.0f};
+          for (int a = 0; a < 3; a++) {
+            mn[a] = sg_min(mn[a], q[a]);
+            mx[a] = sg_max(mx[a], q[a]);
+          }
+        }
+      }
+      for (int a = 0; a < 3; a++) {
+        for (int o = 32; o > 0; o >>= 1) {
+          mn[a] = sg_min(mn[a], __shfl_xor(mn[a], o));
+          mx[a] = sg_max(mx[a], __shfl_xor(mx[a], o));
+        }
+      }
+      const int w = threadIdx.x / 64;
+      if (lane_id() == 0)
+        for (int a = 0; a < 3; a++) {
+          s_mn[a][w] = mn[a];
+          s_mx[a][w] = mx[a];
+        }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        float b[8];
+        for (int a = 0; a < 3; a++) {
+          float lo = s_mn[a][0], hi = s_mx[a][0];
+          for (int j = 1; j < DY_THREADS / 64; j++) {
+            lo = sg_min(lo, s_mn[a][j]);
+            hi = sg_max(hi, s_mx[a][j]);
+          }
+          b[a] = lo;
+          b[4 + a] = hi;
+        }
+        b[3] = b[7] = 0.0f;
+        for (int j = 0; j < 8; j++) g.node_bb[8 * (size_t)node + j] = b[j];
+        atomicAdd((unsigned long long *)&g.ctl[SGD_NFN + 1], 1ull);
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      if (fl & PBVH_UpdateOriginalBB)
+        for (int j = 0; j < 8; j++) g.node_obb[8 * (size_t)node + j] = g.node_bb[8 * (size_t)node + j];
+      g.node_dirty[node] = fl & (PBVH_UpdateBB | PBVH_UpdateOriginalBB);
+      atomicAnd(&g.node_flag[node], ~(PBVH_UpdateBB | PBVH_UpdateOriginalBB | PBVH_UpdateRedraw));
+    }
+  }
+}
+
+// internal nodes with an updated child: union of the children (child 1, then 2), by
+// height; one workgroup
+__global__ __launch_bounds__(1024) void kd_flush(DV g, int H) {
+  for (int h = 0; h < H; h++) {
+    const int b = g.lvl_off[h], e = g.lvl_off[h + 1];
+    for (int i = b + threadIdx.x; i < e; i += blockDim.x) {
+      const int node = g.lvl_node[i];
+      const int c = g.node_child[node];
+      const int u = g.node_dirty[c] | g.node_dirty[c + 1];
+      if (u & PBVH_UpdateBB) {
+        float bb[8];
+        for (int a = 0; a < 3; a++) {
+          float lo = FLT_MAX, hi = -FLT_MAX;
+          lo = sg_min(lo, g.node_bb[8 * (size_t)c + a]);
+          hi = sg_max(hi, g.node_bb[8 * (size_t)c + 4 + a]);
+          lo = sg_min(lo, g.node_bb[8 * (size_t)(c + 1) + a]);
+          hi = sg_max(hi, g.node_bb[8 * (size_t)(c + 1) + 4 + a]);
+          bb[a] = lo;
+          bb[4 + a] = hi;
+        }
+        bb[3] = bb[7] = 0.0f;
+        for (int j = 0; j < 8; j++) g.node_bb[8 * (size_t)node + j] = bb[j];
+      }
+      if (u & PBVH_UpdateOriginalBB)
+        for (int j = 0; j < 8; j++) g.node_obb[8 * (size_t)node + j] = g.node_bb[8 * (size_t)node + j];
+      g.node_dirty[node] = u;
+    }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < g.N; i += blockDim.x) g.node_dirty[i] = 0;
+}
+
+__global__ void kd_ctl_zero(DV g, int b, int e) {
+  const int i = b + threadIdx.x;
+  if (i < e) g.ctl[i] = 0;
+}
+
+// ---------------------------------------------------------------------------------
+// host
+// ---------------------------------------------------------------------------------
+template <class T>
+int dalloc(T *&p, size_t n) {
+  if (p) hipFree(p);
+  p = nullptr;
+  if (sg_check_hip(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)), "hipMalloc")) return -1;
+  return sg_check_hip(hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(T)), "hipMemset");
+}
+template <class T>
+void dfree(T *&p) {
+  if (p) hipFree(p);
+  p = nullptr;
+}
+
+DV view(PBVH *p) {
+  SgDynState *s = p->dyn;
+  SgDynDev &d = s->d;
+  DV g;
+  memset(&g, 0, sizeof(g));
+  g.Vcap = d.Vcap;
+  g.Tcap = d.Tcap;
+  g.N = d.N;
+  g.L = d.L;
+  g.has_mask = d.has_mask;
+  g.respect_hide = p->respect_hide ? 1 : 0;
+  g.use_frontface = s->use_frontface;
+  g.use_projected = s->use_projected;
+  g.ccap = d.ccap;
+  g.vnx = s->view_normal[0];
+  g.vny = s->view_normal[1];
+  g.vnz = s->view_normal[2];
+  g.px = d.px;
+  g.py = d.py;
+  g.pz = d.pz;
+  g.nx = d.nx;
+  g.ny = d.ny;
+  g.nz = d.nz;
+  g.mask = d.mask;
+  g.vhide = d.vhide;
+  g.lab = d.lab;
+  g.vnode = d.vnode;
+  g.vcorner = d.vcorner;
+  g.vstamp = d.vstamp;
+  g.astamp = d.astamp;
+  g.aslot = d.aslot;
+  g.hvst = d.hvst;
+  g.tv = d.tv;
+  g.topp = d.topp;
+  g.tnode = d.tnode;
+  g.tslot = d.tslot;
+  g.wstamp = d.wstamp;
+  g.tie = d.tie;
+  g.fnx = d.fnx;
+  g.fny = d.fny;
+  g.fnz = d.fnz;
+  g.fhide = d.fhide;
+  g.claim = d.claim;
+  g.estamp = d.estamp;
+  g.remap = d.remap;
+  g.remap_ser = d.remap_ser;
+  g.node_flag = d.node_flag;
+  g.node_child = d.node_child;
+  g.node_dirty = d.node_dirty;
+  g.leaf_node = d.leaf_node;
+  g.lvl_off = d.lvl_off;
+  g.lvl_node = d.lvl_node;
+  g.node_bb = d.node_bb;
+  g.node_obb = d.node_obb;
+  g.slab = d.slab;
+  g.scount = d.scount;
+  g.wlist = d.wlist;
+  g.cand = d.cand;
+  g.win = d.win;
+  g.outer = d.outer;
+  g.hv_in = d.hv0;
+  g.hv_out = d.hv1;
+  g.touched = d.touched;
+  g.gathered = d.gathered;
+  g.vlist = d.vlist;
+  g.aff = d.aff;
+  g.nlist = d.nlist;
+  g.key = d.key;
+  g.valid = d.valid;
+  g.fade = d.fade;
+  g.buf0 = d.buf0;
+  g.buf1 = d.buf1;
+  g.ctl = d.ctl;
+  return g;
+}
+
+int read_ctl(PBVH *p) {
+  SgDynDev &d = p->dyn->d;
+  if (sg_check_hip(hipMemcpyAsync(d.ctl_host, d.ctl, SGD_COUNT * sizeof(long long), hipMemcpyDeviceToHost,
+                                  p->stream),
+                   "copy"))
+    return -1;
+  return sg_check_hip(hipStreamSynchronize(p->stream), "sync");
+}
+
+void zero_ctl(PBVH *p, const DV &g, int b, int e) { kd_ctl_zero<<<1, 64, 0, p->stream>>>(g, b, e); }
+
+int launch_check(const char *what) { return sg_check_hip(hipGetLastError(), what); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------
+// upload / download
+// ---------------------------------------------------------------------------------
+void sg_dyn_free_device(PBVH *p) {
+  if (!p || !p->dyn) return;
+  SgDynDev &d = p->dyn->d;
+  float **fl[] = {&d.px, &d.py, &d.pz, &d.nx, &d.ny, &d.nz, &d.mask, &d.fnx, &d.fny, &d.fnz, &d.node_bb,
+                  &d.node_obb, &d.fade, &d.buf0, &d.buf1, &d.curve};
+  for (float **q : fl) dfree(*q);
+  int **il[] = {&d.vnode, &d.vcorner, &d.vstamp, &d.astamp, &d.aslot, &d.hvst, &d.tv, &d.topp, &d.tnode, &d.tslot,
+                &d.wstamp, &d.estamp, &d.remap, &d.remap_ser, &d.node_flag, &d.node_child, &d.node_dirty,
+                &d.leaf_node, &d.lvl_off, &d.lvl_node, &d.slab, &d.scount, &d.wlist, &d.cand, &d.win, &d.outer,
+                &d.hv0, &d.hv1, &d.touched, &d.gathered, &d.vlist, &d.aff, &d.nlist};
+  for (int **q : il) dfree(*q);
+  dfree(d.vhide);
+  dfree(d.fhide);
+  dfree(d.valid);
+  dfree(d.lab);
+  dfree(d.claim);
+  dfree(d.tie);
+  dfree(d.key);
+  dfree(d.ctl);
+  if (d.ctl_host) hipHostFree(d.ctl_host);
+  d.ctl_host = nullptr;
+  d.Vcap = d.Tcap = d.Ncap = 0;
+}
+
+int sg_dyn_upload(PBVH *p) {
+  SgDynState *s = p->dyn;
+  SgDynMesh &m = *s->bm;
+  SgDynTree &t = s->t;
+  SgDynDev &d = s->d;
+  const int N = (int)t.child.size();
+  hipStream_t st = p->stream;
+  if (d.Vcap != m.Vcap || d.Tcap != m.Tcap) {
+    sg_dyn_free_device(p);
+    const size_t V = m.Vcap, T = m.Tcap, C = 3 * (size_t)m.Tcap;
+    d.ccap = (int)std::min<long long>(2LL * m.Tcap + 8, INT32_MAX);
+    int rc = 0;
+    rc |= dalloc(d.px, V) | dalloc(d.py, V) | dalloc(d.pz, V) | dalloc(d.nx, V) | dalloc(d.ny, V) | dalloc(d.nz, V);
+    rc |= dalloc(d.mask, m.has_mask ? V : 1) | dalloc(d.vhide, V) | dalloc(d.lab, V);
+    rc |= dalloc(d.vnode, V) | dalloc(d.vcorner, V) | dalloc(d.vstamp, V) | dalloc(d.astamp, V) | dalloc(d.aslot, V);
+    rc |= dalloc(d.hvst, V) | dalloc(d.hv0, V) | dalloc(d.hv1, V) | dalloc(d.vlist, V) | dalloc(d.aff, V);
+    rc |= dalloc(d.fade, V) | dalloc(d.buf0, 3 * V) | dalloc(d.buf1, 3 * V);
+    rc |= dalloc(d.tv, C) | dalloc(d.topp, C) | dalloc(d.estamp, C) | dalloc(d.remap, C) | dalloc(d.remap_ser, C);
+    rc |= dalloc(d.tnode, T) | dalloc(d.tslot, T) | dalloc(d.wstamp, T) | dalloc(d.fnx, T) | dalloc(d.fny, T);
+    rc |= dalloc(d.fnz, T) | dalloc(d.fhide, T) | dalloc(d.claim, T) | dalloc(d.wlist, T);
+    rc |= dalloc(d.tie, T);
+    rc |= dalloc(d.cand, (size_t)d.ccap) | dalloc(d.key, (size_t)d.ccap) | dalloc(d.valid, (size_t)d.ccap);
+    rc |= dalloc(d.win, (size_t)d.ccap) | dalloc(d.outer, 4 * (size_t)d.ccap) | dalloc(d.touched, 3 * (size_t)d.ccap);
+    rc |= dalloc(d.ctl, SGD_COUNT) | dalloc(d.curve, SG_CM_TABLE + 1);
+    if (!rc) rc = sg_check_hip(hipHostMalloc((void **)&d.ctl_host, SGD_COUNT * sizeof(long long)), "hipHostMalloc");
+    if (rc) return -1;
+    d.Vcap = m.Vcap;
+    d.Tcap = m.Tcap;
+  }
+  if (N > d.Ncap) {
+    const size_t Nc = (size_t)std::max(N + N / 2, 64);
+    int rc = 0;
+    rc |= dalloc(d.node_flag, Nc) | dalloc(d.node_child, Nc) | dalloc(d.node_dirty, Nc) | dalloc(d.leaf_node, Nc);
+    rc |= dalloc(d.lvl_off, Nc + 1) | dalloc(d.lvl_node, Nc) | dalloc(d.node_bb, 8 * Nc) | dalloc(d.node_obb, 8 * Nc);
+    rc |= dalloc(d.slab, Nc * SG_DYN_SLAB) | dalloc(d.scount, Nc) | dalloc(d.gathered, Nc) | dalloc(d.nlist, Nc);
+    if (rc) return -1;
+    d.Ncap = (int)Nc;
+  }
+  d.N = N;
+  d.has_mask = m.has_mask ? 1 : 0;
+  // mesh
+  std::vector<float> x(m.V), y(m.V), z(m.V);
+  auto put3 = [&](const std::vector<float> &src, float *dx, float *dy, float *dz, int n) -> int {
+    for (int i = 0; i < n; i++) {
+      x[i] = src[3 * (size_t)i];
+      y[i] = src[3 * (size_t)i + 1];
+      z[i] = src[3 * (size_t)i + 2];
+    }
+    int rc = sg_check_hip(hipMemcpy(dx, x.data(), n * 4, hipMemcpyHostToDevice), "copy");
+    rc |= sg_check_hip(hipMemcpy(dy, y.data(), n * 4, hipMemcpyHostToDevice), "copy");
+    rc |= sg_check_hip(hipMemcpy(dz, z.data(), n * 4, hipMemcpyHostToDevice), "copy");
+    return rc;
+  };
+  int rc = put3(m.co, d.px, d.py, d.pz, m.V) | put3(m.no, d.nx, d.ny, d.nz, m.V);
+  x.resize(std::max(m.V, m.T));
+  y.resize(x.size());
+  z.resize(x.size());
+  rc |= put3(m.fno, d.fnx, d.fny, d.fnz, m.T);
+  if (m.has_mask) rc |= sg_check_hip(hipMemcpy(d.mask, m.mask.data(), m.V * 4, hipMemcpyHostToDevice), "copy");
+  rc |= sg_check_hip(hipMemcpy(d.vhide, m.vhide.data(), m.V, hipMemcpyHostToDevice), "copy");
+  rc |= sg_check_hip(hipMemcpy(d.lab, m.lab.data(), m.V * 8, hipMemcpyHostToDevice), "copy");
+  rc |= sg_check_hip(hipMemcpy(d.vnode, m.vnode.data(), m.V * 4, hipMemcpyHostToDevice), "copy");
+  rc |= sg_check_hip(hipMemcpy(d.vcorner, m.vcorner.data(), m.V * 4, hipMemcpyHostToDevice), "copy");
+  rc |= sg_check_hip(hipMemcpy(d.tv, m.tv.data(), 3 * (size_t)m.T * 4, hipMemcpyHostToDevice), "copy");
+  rc |= sg_check_hip(hipMemcpy(d.topp, m.topp.data(), 3 * (size_t)m.T * 4, hipMemcpyHostToDevice), "copy");
+  rc |= sg_check_hip(hipMemcpy(d.tnode, m.tnode.data(), m.T * 4, hipMemcpyHostToDevice), "copy");
+  rc |= sg_check_hip(hipMemcpy(d.tslot, m.tslot.data(), m.T * 4, hipMemcpyHostToDevice), "copy");
+  rc |= sg_check_hip(hipMemcpy(d.fhide, m.fhide.data(), m.T, hipMemcpyHostToDevice), "copy");
+  // tree
+  std::vector<int> leaves, height(N, 0);
+  {
+    std::vector<int> st{0};
+    while (!st.empty()) {
+      const int i = st.back();
+      st.pop_back();
+      if (t.flag[i] & PBVH_Leaf) {
+        leaves.push_back(i);
+        continue;
+      }
+      st.push_back(t.child[i] + 1);
+      st.push_back(t.child[i]);
+    }
+    // heights, children before parents (children always have larger ids)
+    for (int i = N - 1; i >= 0; i--)
+      if (!(t.flag[i] & PBVH_Leaf)) height[i] = 1 + std::max(height[t.child[i]], height[t.child[i] + 1]);
+  }
+  int H = 0;
+  for (int i = 0; i < N; i++) H = std::max(H, height[i]);
+  std::vector<int> lvl_off(H + 1, 0), lvl_node;
+  for (int h = 1; h <= H; h++) {
+    lvl_off[h - 1] = (int)lvl_node.size();
+    for (int i = 0; i < N; i++)
+      if (height[i] == h) lvl_node.push_back(i);
+  }
+  lvl_off[H] = (int)lvl_node.size();
+  d.L = (int)leaves.size();
+  d.H = H;
+  rc |= sg_check_hip(hipMemcpy(d.node_flag, t.flag.data(), N * 4, hipMemcpyHostToDevice), "copy");
+  rc |= sg_check_hip(hipMemcpy(d.node_child, t.child.data(), N * 4, hipMemcpyHostToDevice), "copy");
+  rc |= sg_check_hip(hipMemcpy(d.node_bb, t.bb.data(), (size_t)N * 32, hipMemcpyHostToDevice), "copy");
+  rc |= sg_check_hip(hipMemcpy(d.node_obb, t.obb.data(), (size_t)N * 32, hipMemcpyHostToDevice), "copy");
+  rc |= sg_check_hip(hipMemcpy(d.scount, t.scount.data(), N * 4, hipMemcpyHostToDevice), "copy");
+  rc |= sg_check_hip(hipMemcpy(d.slab, t.slab.data(), (size_t)N * SG_DYN_SLAB * 4, hipMemcpyHostToDevice), "copy");
+  rc |= sg_check_hip(hipMemcpy(d.leaf_node, leaves.data(), leaves.size() * 4, hipMemcpyHostToDevice), "copy");
+  rc |= sg_check_hip(hipMemcpy(d.lvl_off, lvl_off.data(), lvl_off.size() * 4, hipMemcpyHostToDevice), "copy");
+  if (!lvl_node.empty())
+    rc |= sg_check_hip(hipMemcpy(d.lvl_node, lvl_node.data(), lvl_node.size() * 4, hipMemcpyHostToDevice), "copy");
+  rc |= sg_check_hip(hipMemset(d.node_dirty, 0, N * 4), "memset");
+  std::vector<long long> ctl(SGD_COUNT, 0);
+  ctl[SGD_V] = m.V;
+  ctl[SGD_T] = m.T;
+  rc |= sg_check_hip(hipMemcpy(d.ctl, ctl.data(), SGD_COUNT * 8, hipMemcpyHostToDevice), "copy");
+  (void)st;
+  s->pend.assign(N, 0);
+  return rc ? -1 : 0;
+}
+
+int sg_dyn_download(PBVH *p, bool mesh) {
+  SgDynState *s = p->dyn;
+  SgDynDev &d = s->d;
+  SgDynTree &t = s->t;
+  if (sg_check_hip(hipStreamSynchronize(p->stream), "sync")) return -1;
+  const int N = d.N;
+  int rc = 0;
+  rc |= sg_check_hip(hipMemcpy(t.flag.data(), d.node_flag, N * 4, hipMemcpyDeviceToHost), "copy");
+  rc |= sg_check_hip(hipMemcpy(t.bb.data(), d.node_bb, (size_t)N * 32, hipMemcpyDeviceToHost), "copy");
+  rc |= sg_check_hip(hipMemcpy(t.obb.data(), d.node_obb, (size_t)N * 32, hipMemcpyDeviceToHost), "copy");
+  rc |= sg_check_hip(hipMemcpy(t.scount.data(), d.scount, N * 4, hipMemcpyDeviceToHost), "copy");
+  for (int i = 0; i < N; i++) {
+    if ((size_t)i < s->pend.size()) t.flag[i] |= s->pend[i];
+  }
+  if (!mesh || !s->bm) return rc ? -1 : 0;
+  rc |= sg_check_hip(hipMemcpy(t.slab.data(), d.slab, (size_t)N * SG_DYN_SLAB * 4, hipMemcpyDeviceToHost), "copy");
+  SgDynMesh &m = *s->bm;
+  long long ctl[2];
+  rc |= sg_check_hip(hipMemcpy(ctl, d.ctl, 16, hipMemcpyDeviceToHost), "copy");
+  m.V = (int)ctl[SGD_V];
+  m.T = (int)ctl[SGD_T];
+  std::vector<float> x(std::max(m.V, m.T)), y(x.size()), z(x.size());
+  auto get3 = [&](std::vector<float> &dst, const float *dx, const float *dy, const float *dz, int n) -> int {
+    int r = sg_check_hip(hipMemcpy(x.data(), dx, n * 4, hipMemcpyDeviceToHost), "copy");
+    r |= sg_check_hip(hipMemcpy(y.data(), dy, n * 4, hipMemcpyDeviceToHost), "copy");
+    r |= sg_check_hip(hipMemcpy(z.data(), dz, n * 4, hipMemcpyDeviceToHost), "copy");
+    for (int i = 0; i < n; i++) {
+      dst[3 * (size_t)i] = x[i];
+      dst[3 * (size_t)i + 1] = y[i];
+      dst[3 * (size_t)i + 2] = z[i];
+    }
+    return r;
+  };
+  rc |= get3(m.co, d.px, d.py, d.pz, m.V) | get3(m.no, d.nx, d.ny, d.nz, m.V) | get3(m.fno, d.fnx, d.fny, d.fnz, m.T);
+  if (m.has_mask) rc |= sg_check_hip(hipMemcpy(m.mask.data(), d.mask, m.V * 4, hipMemcpyDeviceToHost), "copy");
+  rc |= sg_check_hip(hipMemcpy(m.vhide.data(), d.vhide, m.V, hipMemcpyDeviceToHost), "copy");
+  rc |= sg_check_hip(hipMemcpy(m.lab.data(), d.lab, m.V * 8, hipMemcpyDeviceToHost), "copy");
+  rc |= sg_check_hip(hipMemcpy(m.vnode.data(), d.vnode, m.V * 4, hipMemcpyDeviceToHost), "copy");
+  rc |= sg_check_hip(hipMemcpy(m.vcorner.data(), d.vcorner, m.V * 4, hipMemcpyDeviceToHost), "copy");
+  rc |= sg_check_hip(hipMemcpy(m.tv.data(), d.tv, 3 * (size_t)m.T * 4, hipMemcpyDeviceToHost), "copy");
+  rc |= sg_check_hip(hipMemcpy(m.topp.data(), d.topp, 3 * (size_t)m.T * 4, hipMemcpyDeviceToHost), "copy");
+  rc |= sg_check_hip(hipMemcpy(m.tnode.data(), d.tnode, m.T * 4, hipMemcpyDeviceToHost), "copy");
+  rc |= sg_check_hip(hipMemcpy(m.tslot.data(), d.tslot, m.T * 4, hipMemcpyDeviceToHost), "copy");
+  rc |= sg_check_hip(hipMemcpy(m.fhide.data(), d.fhide, m.T, hipMemcpyDeviceToHost), "copy");
+  return rc ? -1 : 0;
+}
+
+// Host-set node flags (pend) go down: device flags |= pend.
+int sg_dyn_sync_flags_down(PBVH *p) {
+  SgDynState *s = p->dyn;
+  SgDynDev &d = s->d;
+  bool any = false;
+  for (int v : s->pend) any |= v != 0;
+  if (!any) return 0;
+  std::vector<int> f(d.N);
+  int rc = sg_check_hip(hipMemcpy(f.data(), d.node_flag, d.N * 4, hipMemcpyDeviceToHost), "copy");
+  for (int i = 0; i < d.N && i < (int)s->pend.size(); i++) f[i] |= s->pend[i];
+  rc |= sg_check_hip(hipMemcpy(d.node_flag, f.data(), d.N * 4, hipMemcpyHostToDevice), "copy");
+  std::fill(s->pend.begin(), s->pend.end(), 0);
+  return rc ? -1 : 0;
+}
+
+// ---------------------------------------------------------------------------------
+// topology update (rounds)
+// ---------------------------------------------------------------------------------
+static int phase(PBVH *p, int collapse, const float center[3], float radius, int wser, long long &ops,
+                 long long &rounds, long long &ncand_total, int &modified) {
+  SgDynState *s = p->dyn;
+  SgDynDev &d = s->d;
+  hipStream_t st = p->stream;
+  Range R;
+  R.c[0] = center[0];
+  R.c[1] = center[1];
+  R.c[2] = center[2];
+  R.r2 = radius * radius;
+  DV g = view(p);
+  if (s->use_projected) {
+    const float n[3] = {s->view_normal[0], s->view_normal[1], s->view_normal[2]};
+    const float mul = center[0] * n[0] + center[1] * n[1] + center[2] * n[2];
+    for (int i = 0; i < 3; i++) R.cp[i] = center[i] + n[i] * -mul;
+  } else {
+    R.cp[0] = R.cp[1] = R.cp[2] = 0.0f;
+  }
+  const float limit = collapse ? s->min_len : s->max_len;
+  const float limit2 = limit * limit;
+  int *hv[2] = {d.hv0, d.hv1};
+  int cur = 0;
+  for (int round = 0; round < SG_DYN_MAX_ROUNDS; round++) {
+    const int stamp = ++s->stamp;
+    g.hv_in = hv[cur];
+    g.hv_out = hv[1 - cur];
+    kd_round_begin<<<1, 64, 0, st>>>(g, round == 0);
+    kd_cand<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, collapse, R, limit, limit2, stamp);
+    kd_claim<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, collapse);
+    kd_verify<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, collapse);
+    if (launch_check("dyntopo round")) return -1;
+    // apply only when the candidate list fit (an overflowing round stops the phase)
+    if (read_ctl(p)) return -1;
+    const long long *c = d.ctl_host;
+    ncand_total += c[SGD_NCAND];
+    if (c[SGD_NCAND] == 0) break;
+    if (c[SGD_NCAND] > d.ccap) {
+      s->stats[8] |= 1;
+      break;
+    }
+    rounds++;
+    if (c[SGD_FLAGS] & 1) {
+      s->stats[8] |= 1;
+      break;
+    }
+    kd_apply<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, collapse, s->serial, round, stamp, wser);
+    if (collapse)
+      kd_owner<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g);
+    else
+      kd_relink<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, stamp);
+    if (launch_check("dyntopo apply")) return -1;
+    const long long nwin = c[SGD_NWIN];
+    ops += nwin;
+    if (nwin) {
+      modified = 1;
+    } else {
+      break;
+    }
+    cur = 1 - cur;
+  }
+  if (read_ctl(p)) return -1;
+  if (d.ctl_host[SGD_FLAGS] & 2) s->stats[8] |= 2;
+  return 0;
+}
+
+int sg_dyn_update_topology(PBVH *p, int mode, const float center[3], float radius) {
+  SgDynState *s = p->dyn;
+  SgDynDev &d = s->d;
+  hipStream_t st = p->stream;
+  DV g = view(p);
+  const int wser = ++s->stamp;
+  zero_ctl(p, g, SGD_NCAND, SGD_COUNT);
+  kd_wbuild<<<std::max(1, std::min(d.L, 4096)), DY_THREADS, 0, st>>>(g, wser);
+  if (launch_check("dyntopo working set") || read_ctl(p)) return -1;
+  s->stats[9] = d.ctl_host[SGD_NLIST];
+  int modified = 0;
+  long long ops = 0, rounds = 0, nc = 0;
+  if (mode & PBVH_Collapse) {
+    ops = rounds = nc = 0;
+    if (phase(p, 1, center, radius, wser, ops, rounds, nc, modified)) return -1;
+    s->stats[3] += ops;
+    s->stats[5] += rounds;
+    s->stats[7] += nc;
+  }
+  if (mode & PBVH_Subdivide) {
+    ops = rounds = nc = 0;
+    if (phase(p, 0, center, radius, wser, ops, rounds, nc, modified)) return -1;
+    s->stats[2] += ops;
+    s->stats[4] += rounds;
+    s->stats[6] += nc;
+  }
+  kd_clear_topology_flags<<<(d.N + 255) / 256, 256, 0, st>>>(g);
+  if (launch_check("dyntopo flags")) return -1;
+  s->serial++;
+  s->host_stale = true;
+  return modified;
+}
+
+// ---------------------------------------------------------------------------------
+// normals and bounds of flagged leaves
+// ---------------------------------------------------------------------------------
+int sg_dyn_update_normals_bounds(PBVH *p, bool normals, bool bounds) {
+  SgDynState *s = p->dyn;
+  SgDynDev &d = s->d;
+  hipStream_t st = p->stream;
+  DV g = view(p);
+  const int lb = std::max(1, (d.L + DY_THREADS - 1) / DY_THREADS);
+  if (normals) {
+    zero_ctl(p, g, SGD_NVL, SGD_NVL + 1);
+    zero_ctl(p, g, SGD_NLIST, SGD_NFN + 1);
+    kd_flagged<<<lb, DY_THREADS, 0, st>>>(g, PBVH_UpdateNormals);
+    kd_face_normals<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, ++s->stamp);
+    kd_vert_normals<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g);
+  }
+  if (bounds) {
+    zero_ctl(p, g, SGD_NLIST, SGD_NLIST + 1);
+    zero_ctl(p, g, SGD_NFN + 1, SGD_NFN + 2);
+    kd_flagged<<<lb, DY_THREADS, 0, st>>>(g, PBVH_UpdateBB | PBVH_UpdateOriginalBB | PBVH_UpdateRedraw);
+    kd_leaf_bounds<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g);
+    kd_flush<<<1, 1024, 0, st>>>(g, d.H);
+  }
+  if (launch_check("dyntopo normals/bounds") || read_ctl(p)) return -1;
+  if (normals) s->stats[10] = d.ctl_host[SGD_NFN];
+  if (bounds) s->stats[11] = d.ctl_host[SGD_NFN + 1];
+  s->host_stale = true;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------------
+// the dab
+// ---------------------------------------------------------------------------------
+int sg_dyn_dab_apply(PBVH *p, const SgDabParams *dab) {
+  SgDynState *s = p->dyn;
+  SgDynDev &d = s->d;
+  hipStream_t st = p->stream;
+  const int tool = dab->sculpt_tool;
+  if (tool != SG_TOOL_DRAW && tool != SG_TOOL_INFLATE && tool != SG_TOOL_SMOOTH && tool != SG_TOOL_GRAB)
+    return sg_set_error("PBVH_BMESH: Draw, Inflate, Smooth and Grab only");
+  if (dab->flag & (SG_DAB_ORIGINAL | SG_SMOOTH_LEARNED))
+    return sg_set_error("PBVH_BMESH: no original data or learned Smooth");
+  if (sg_dyn_sync_flags_down(p)) return -1;
+  memset(s->stats, 0, sizeof(s->stats));
+  const SgDabK k = sg_make_dab(dab, 0);
+  const float *curve = nullptr;
+  if (k.falloff == SG_CURVE_CUSTOM && dab->curve_table) {
+    if (sg_check_hip(hipMemcpyAsync(d.curve, dab->curve_table, (SG_CM_TABLE + 1) * 4, hipMemcpyHostToDevice, st),
+                     "copy"))
+      return -1;
+    curve = d.curve;
+  }
+  DV g = view(p);
+  const int lb = std::max(1, (d.L + DY_THREADS - 1) / DY_THREADS);
+  const int update = PBVH_UpdateNormals | PBVH_UpdateBB | PBVH_UpdateOriginalBB | PBVH_UpdateDrawBuffers |
+                     PBVH_UpdateRedraw;
+  if (tool != SG_TOOL_GRAB && s->mode) {
+    const float rt = k.r * 1.25f;
+    zero_ctl(p, g, SGD_NGATHER, SGD_NGATHER + 1);
+    kd_search<<<lb, DY_THREADS, 0, st>>>(g, k.cx, k.cy, k.cz, rt * rt, update | PBVH_UpdateTopology);
+    if (launch_check("dyntopo search") || read_ctl(p)) return -1;
+    if (d.ctl_host[SGD_NGATHER]) {
+      const float c[3] = {k.cx, k.cy, k.cz};
+      if (sg_dyn_update_topology(p, s->mode, c, k.r) < 0) return -1;
+    }
+  }
+  zero_ctl(p, g, SGD_NGATHER, SGD_NFN + 2);
+  kd_search<<<lb, DY_THREADS, 0, st>>>(g, k.cx, k.cy, k.cz, k.r2, update);
+  const int stamp = ++s->stamp;
+  kd_unique<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, stamp);
+  if (tool == SG_TOOL_DRAW) kd_area<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, k);
+  kd_brush<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, k, curve, stamp);
+  if (tool == SG_TOOL_SMOOTH) {
+    kd_smooth_init<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g);
+    float *a = d.buf0, *b = d.buf1;
+    for (int it = 0; it < k.iters; it++) {
+      kd_smooth<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, a, b, stamp);
+      std::swap(a, b);
+    }
+    kd_smooth_store<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, a);
+  }
+  if (launch_check("dyntopo brush") || read_ctl(p)) return -1;
+  s->stats[0] = d.ctl_host[SGD_NGATHER];
+  s->stats[1] = d.ctl_host[SGD_NAFF];
+  const long long naff = d.ctl_host[SGD_NAFF];
+  if (sg_dyn_update_normals_bounds(p, true, true)) return -1;
+  // the affected list survives the normals (they reuse the vertex list only)
+  (void)naff;
+  p->dab_count++;
+  return 0;
+}
+
+int sg_dyn_last_affected(PBVH *p, uint64_t *out, int cap) {
+  SgDynState *s = p->dyn;
+  SgDynDev &d = s->d;
+  const int n = (int)s->stats[1];
+  long long nv = 0;
+  if (sg_check_hip(hipMemcpy(&nv, d.ctl + SGD_V, 8, hipMemcpyDeviceToHost), "copy")) return -1;
+  std::vector<int> aff(n);
+  std::vector<uint64_t> lab((size_t)nv);
+  if (n && sg_check_hip(hipMemcpy(aff.data(), d.aff, n * 4, hipMemcpyDeviceToHost), "copy")) return -1;
+  if (nv && sg_check_hip(hipMemcpy(lab.data(), d.lab, (size_t)nv * 8, hipMemcpyDeviceToHost), "copy")) return -1;
+  std::vector<uint64_t> r(n);
+  for (int i = 0; i < n; i++) r[i] = lab[aff[i]];
+  std::sort(r.begin(), r.end());
+  for (int i = 0; i < n && i < cap; i++) out[i] = r[i];
+  return n;
+}

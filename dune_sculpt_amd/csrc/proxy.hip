@@ -91,7 +91,7 @@ extern "C" {
 // the original coordinates (those of the open undo step), else from the current ones.
 // Returns the number of leaves changed.
 int sg_pbvh_combine_proxies(PBVH *p, int use_orco) {
-  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: proxies");
+  if (p && p->type != PBVH_FACES) return sg_set_error("faces only: proxies");
   if (!p || p->no_device || !p->build_ok) return sg_set_error("no device mesh");
   // nodes holding proxies, in node order (BKE_pbvh_gather_proxies)
   std::vector<PBVHNode *> nodes;

@@ -974,7 +974,7 @@ static int ensure_ray_buffers(PBVH *p, int n) {
 }
 
 int sg_raycast_device(PBVH *p, const SgRay *rays_dev, int n, int original, SgRayHit *hits_dev) {
-  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: ray casts on PBVH_GRIDS are not built");
+  if (p && p->type != PBVH_FACES) return sg_set_error("faces only: ray casts on PBVH_GRIDS / PBVH_BMESH are not built");
   if (!p || n < 0 || (n > 0 && (!rays_dev || !hits_dev))) return sg_set_error("bad arguments");
   if (((uintptr_t)rays_dev | (uintptr_t)hits_dev) & 15) return sg_set_error("ray arrays must be 16-byte aligned");
   if (p->no_device) return sg_set_error("no HIP device");
@@ -989,7 +989,7 @@ int sg_raycast_device(PBVH *p, const SgRay *rays_dev, int n, int original, SgRay
 }
 
 int sg_raycast(PBVH *p, const SgRay *rays, int n, int original, SgRayHit *hits) {
-  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: ray casts on PBVH_GRIDS are not built");
+  if (p && p->type != PBVH_FACES) return sg_set_error("faces only: ray casts on PBVH_GRIDS / PBVH_BMESH are not built");
   if (!p || n < 0 || (n > 0 && (!rays || !hits))) return sg_set_error("bad arguments");
   if (p->no_device) return sg_set_error("no HIP device");
   if (n == 0) return 0;
@@ -1005,7 +1005,7 @@ int sg_raycast(PBVH *p, const SgRay *rays, int n, int original, SgRayHit *hits) 
 }
 
 int sg_raycast_stats(PBVH *p, const SgRay *rays, int n, int original, long long out[3]) {
-  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: ray casts on PBVH_GRIDS are not built");
+  if (p && p->type != PBVH_FACES) return sg_set_error("faces only: ray casts on PBVH_GRIDS / PBVH_BMESH are not built");
   if (!p || n <= 0 || !rays || !out) return sg_set_error("bad arguments");
   if (p->no_device) return sg_set_error("no HIP device");
   if (int rc = ensure_ray_buffers(p, n)) return rc;
@@ -1210,7 +1210,7 @@ static bool node_ray(PBVH *p, PBVHNode *node, float (*origco)[3], const float st
 extern "C" {
 
 int sg_raycast_stats_noalloc(PBVH *p, const SgRay *rays, int n, int original, long long out[3]) {
-  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: ray casts on PBVH_GRIDS are not built");
+  if (p && p->type != PBVH_FACES) return sg_set_error("faces only: ray casts on PBVH_GRIDS / PBVH_BMESH are not built");
   if (!p || n <= 0 || !rays || !out) return sg_set_error("bad arguments");
   if (p->no_device || !p->build_ok) return sg_set_error("no device mesh");
   if (n > SG_NOALLOC_RAYS)
@@ -1254,13 +1254,13 @@ void sg_isect_ray_tri_precalc(IsectRayPrecalc *out, const float dir[3]) {
 
 void BKE_pbvh_raycast(PBVH *p, BKE_pbvh_HitOccludedCallback cb, void *data, const float ray_start[3],
                       const float ray_normal[3], bool original) {
-  if (p && p->type == PBVH_GRIDS) { sg_set_error("faces only: ray casts on PBVH_GRIDS are not built"); return; }
+  if (p && p->type != PBVH_FACES) { sg_set_error("faces only: ray casts on PBVH_GRIDS / PBVH_BMESH are not built"); return; }
   occluded_visit(p, ray_start, ray_normal, 0, original, cb, data);
 }
 
 void BKE_pbvh_find_nearest_to_ray(PBVH *p, BKE_pbvh_SearchNearestCallback cb, void *data, const float ray_start[3],
                                   const float ray_normal[3], bool original) {
-  if (p && p->type == PBVH_GRIDS) { sg_set_error("faces only: ray casts on PBVH_GRIDS are not built"); return; }
+  if (p && p->type != PBVH_FACES) { sg_set_error("faces only: ray casts on PBVH_GRIDS / PBVH_BMESH are not built"); return; }
   occluded_visit(p, ray_start, ray_normal, 1, original, cb, data);
 }
 
@@ -1269,7 +1269,7 @@ float BKE_pbvh_node_get_tmin(PBVHNode *node) { return node ? node->tmin : 0.0f; 
 bool BKE_pbvh_node_raycast(PBVH *p, PBVHNode *node, float (*origco)[3], bool use_origco, const float ray_start[3],
                            const float ray_normal[3], IsectRayPrecalc *isect_precalc, float *depth,
                            int *active_vertex_index, int *active_face_grid_index, float *face_normal) {
-  if (p && p->type == PBVH_GRIDS) { sg_set_error("faces only: ray casts on PBVH_GRIDS are not built"); return false; }
+  if (p && p->type != PBVH_FACES) { sg_set_error("faces only: ray casts on PBVH_GRIDS / PBVH_BMESH are not built"); return false; }
   (void)use_origco;  // dyntopo only
   if (!depth) return false;
   float res[9];
@@ -1286,7 +1286,7 @@ bool BKE_pbvh_node_raycast(PBVH *p, PBVHNode *node, float (*origco)[3], bool use
 bool BKE_pbvh_node_find_nearest_to_ray(PBVH *p, PBVHNode *node, float (*origco)[3], bool use_origco,
                                        const float ray_start[3], const float ray_normal[3], float *depth,
                                        float *dist_sq) {
-  if (p && p->type == PBVH_GRIDS) { sg_set_error("faces only: ray casts on PBVH_GRIDS are not built"); return false; }
+  if (p && p->type != PBVH_FACES) { sg_set_error("faces only: ray casts on PBVH_GRIDS / PBVH_BMESH are not built"); return false; }
   (void)use_origco;
   if (!depth || !dist_sq) return false;
   float res[9];
@@ -1300,7 +1300,7 @@ bool BKE_pbvh_node_find_nearest_to_ray(PBVH *p, PBVHNode *node, float (*origco)[
 // grown by 1e-3 (absolute) and 0.1% (relative); a few FP32 operations on one box.
 void BKE_pbvh_raycast_project_ray_root(PBVH *p, bool original, float ray_start[3], float ray_end[3],
                                        float ray_normal[3]) {
-  if (p && p->type == PBVH_GRIDS) { sg_set_error("faces only: ray casts on PBVH_GRIDS are not built"); return; }
+  if (p && p->type != PBVH_FACES) { sg_set_error("faces only: ray casts on PBVH_GRIDS / PBVH_BMESH are not built"); return; }
   if (!p || p->nodes.empty()) return;
   sg_sync_host_bbs(p);
   const float *vb = original ? p->nodes[0].orig_vb : p->nodes[0].vb;

@@ -211,6 +211,7 @@ enum {
 
 struct SgGridDev;
 struct SubdivCCG;
+struct SgDynState;
 struct PBVH {
   PBVHType type = PBVH_FACES;
   // PBVH_GRIDS (pbvh_intern.h:129-134, :161): the caller's grids (borrowed), their key,
@@ -224,6 +225,7 @@ struct PBVH {
   int totgrid = 0;
   SubdivCCG *subdiv_ccg = nullptr;
   SgGridDev *gd = nullptr;
+  SgDynState *dyn = nullptr;         // PBVH_BMESH (sg_dyn.h)
   int leaf_limit = 10000;
   bool respect_hide = true;
   bool no_device = false;            // no HIP device: host tree only

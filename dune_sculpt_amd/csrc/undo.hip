@@ -170,7 +170,7 @@ static int swap_step(PBVH *p, SgUndoStep &s, int keep = 0) {
 extern "C" {
 
 int sg_undo_begin(PBVH *p, long long max_verts) {
-  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: undo steps of PBVH_GRIDS are not built");
+  if (p && p->type != PBVH_FACES) return sg_set_error("faces only: undo steps of PBVH_GRIDS / PBVH_BMESH are not built");
   if (!p || p->no_device || !p->build_ok) return sg_set_error("no device mesh");
   SgUndoState &u = p->undo;
   if (u.recording) return sg_set_error("an undo step is already open");
@@ -199,7 +199,7 @@ int sg_undo_begin(PBVH *p, long long max_verts) {
 }
 
 int sg_undo_push_node(PBVH *p, PBVHNode *node) {
-  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: undo steps of PBVH_GRIDS are not built");
+  if (p && p->type != PBVH_FACES) return sg_set_error("faces only: undo steps of PBVH_GRIDS / PBVH_BMESH are not built");
   if (!p || !node || node < p->nodes.data() || node >= p->nodes.data() + p->nodes.size())
     return sg_set_error("not a node of this PBVH");
   if (node->leaf < 0 || node->leaf >= p->d.L) return sg_set_error("not a leaf");
@@ -286,7 +286,7 @@ int sg_undo_cancel(PBVH *p) {
 // before each dab of an anchored stroke, restoring the original mesh from its undo
 // nodes; a later dab of the step saves nothing again.
 int sg_undo_restore_original(PBVH *p) {
-  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: undo steps of PBVH_GRIDS are not built");
+  if (p && p->type != PBVH_FACES) return sg_set_error("faces only: undo steps of PBVH_GRIDS / PBVH_BMESH are not built");
   if (!p) return sg_set_error("bad arguments");
   SgUndoState &u = p->undo;
   if (!u.recording) return sg_set_error("no undo step is open");
@@ -310,7 +310,7 @@ int sg_undo_restore_original(PBVH *p) {
 // coordinates the reference reads from its undo nodes (SculptOrigVertData) for Grab,
 // anchored strokes and BKE_pbvh_node_raycast's origco.
 int sg_undo_original(PBVH *p, float *co, float *no) {
-  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: undo steps of PBVH_GRIDS are not built");
+  if (p && p->type != PBVH_FACES) return sg_set_error("faces only: undo steps of PBVH_GRIDS / PBVH_BMESH are not built");
   if (!p || p->no_device || !p->build_ok) return sg_set_error("no device mesh");
   SgDev &d = p->d;
   const size_t V = (size_t)d.V;
@@ -351,7 +351,7 @@ int sg_undo_original(PBVH *p, float *co, float *no) {
 
 // Returns 0 after undoing a step, 1 when there is nothing to undo.
 int sg_undo(PBVH *p) {
-  if (p && p->type == PBVH_GRIDS) return sg_set_error("faces only: undo steps of PBVH_GRIDS are not built");
+  if (p && p->type != PBVH_FACES) return sg_set_error("faces only: undo steps of PBVH_GRIDS / PBVH_BMESH are not built");
   if (!p) return sg_set_error("bad arguments");
   SgUndoState &u = p->undo;
   if (u.recording) return sg_set_error("an undo step is open");

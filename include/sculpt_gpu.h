@@ -4,7 +4,7 @@
  * Two groups of entry points:
  *
  *  1. Drop-in replacements for the reference's PBVH C API (PBVH_FACES; PBVH_GRIDS in
- *     section 3).  The reference's public header (BKE_pbvh.h / DUNE_pbvh.h /
+ *     section 3, PBVH_BMESH in section 4).  The reference's public header (BKE_pbvh.h / DUNE_pbvh.h /
  *     KERNEL_pbvh.h) is absent,
  *     so each signature is taken from its definition in kernel/intern/pbvh.c and its
  *     call sites (paint.c:1676-1771, :2084-2191; editors/sculpt_paint/paint_hide.c).
@@ -82,7 +82,11 @@ typedef enum {
   PBVH_FullyMasked = 1 << 10,
   PBVH_FullyUnmasked = 1 << 11,
   PBVH_UpdateColor = 1 << 12,
+  PBVH_UpdateTopology = 1 << 13,
 } PBVHNodeFlags;
+
+/* PBVHTopologyUpdateMode (pbvh_bmesh.c:1932, :1956, :1976) */
+typedef enum { PBVH_Subdivide = 1, PBVH_Collapse = 2 } PBVHTopologyUpdateMode;
 
 typedef struct PBVHProxyNode { /* used at pbvh.c:4757-4789 */
   float (*co)[3];
@@ -659,6 +663,74 @@ int BKE_pbvh_count_grid_quads(BLI_bitmap **grid_hidden, const int *grid_indices,
 /* Device state -> the caller's grids (co, normals, mask), and back after host edits. */
 int sg_pbvh_grids_sync_host(PBVH *pbvh);
 int sg_pbvh_grids_upload(PBVH *pbvh);
+
+/* ---- 4. dynamic topology (PBVH_BMESH) ----------------------------------------------
+ * The reference builds this variant over a BMesh (kernel/intern/pbvh_bmesh.c), whose
+ * library is outside the path.  SgDynMesh stands in for it: a triangle mesh, manifold
+ * with boundary, held as a corner table with room to grow (vert_cap, tri_cap).  Once
+ * a PBVH is built over it, the PBVH's HBM copy is the mesh; sg_dynmesh_read fetches
+ * it.  Vertices carry 64-bit labels (the input index, or a hash of the split edge's
+ * labels for vertices a split creates) so callers can follow them across topology
+ * changes.  Topology updates run on the GPU in rounds of non-conflicting edge splits
+ * or collapses, each edge treated by the reference's rules (pbvh_bmesh.c:773-1484).
+ * A leaf holds at most SG_DYN_SLAB faces between DUNE_pbvh_bmesh_after_stroke calls. */
+typedef struct SgDynMesh SgDynMesh;
+enum { SG_DYN_SLAB = 4096 };
+/* New in this build (the BMesh it stands for is built by the absent bmesh library):
+ * co[totvert][3], tris[tottri][3] (outward winding), optional mask[totvert],
+ * vert_hidden[totvert], face_hidden[tottri]; caps >= the counts (0: 4x).  NULL on a
+ * non-manifold or degenerate mesh (sg_last_error). */
+SgDynMesh *sg_dynmesh_new(const float *co, int totvert, const int *tris, int tottri, const float *mask,
+                          const uint8_t *vert_hidden, const uint8_t *face_hidden, int vert_cap, int tri_cap);
+void sg_dynmesh_free(SgDynMesh *bm);
+/* Live vertex and face counts, and the slots in use. */
+int sg_dynmesh_counts(SgDynMesh *bm, int *r_totvert, int *r_tottri, int *r_vert_slots, int *r_tri_slots);
+/* Live vertices in slot order (label, position, normal, mask, owner node) and live
+ * faces in slot order (corner labels, leaf node, face normal, hidden); any output may
+ * be NULL.  Returns the vertex count. */
+int sg_dynmesh_read(SgDynMesh *bm, uint64_t *vert_labels, float *co, float *no, float *mask, int *vert_node,
+                    uint64_t *face_labels, int *face_node, float *face_no, uint8_t *face_hidden);
+/* pbvh_bmesh.c:1854-1929 DUNE_pbvh_build_bmesh: leaf limit 100 faces, detail size
+ * 0.75.  The reference's BMLog and custom-data offsets are accepted and unused (node
+ * indices live in the PBVH). */
+void DUNE_pbvh_build_bmesh(PBVH *pbvh, SgDynMesh *bm, bool smooth_shading, void *log, int cd_vert_node_offset,
+                           int cd_face_node_offset);
+void BKE_pbvh_build_bmesh(PBVH *pbvh, SgDynMesh *bm, bool smooth_shading, void *log, int cd_vert_node_offset,
+                          int cd_face_node_offset);
+/* pbvh_bmesh.c:1931-2005: over the leaves flagged PBVH_UpdateTopology. */
+bool DUNE_pbvh_bmesh_update_topology(PBVH *pbvh, PBVHTopologyUpdateMode mode, const float center[3],
+                                     const float view_normal[3], float radius, bool use_frontface,
+                                     bool use_projected);
+bool BKE_pbvh_bmesh_update_topology(PBVH *pbvh, PBVHTopologyUpdateMode mode, const float center[3],
+                                    const float view_normal[3], float radius, bool use_frontface, bool use_projected);
+/* pbvh_bmesh.c:2065-2078 (leaf lists compacted, leaves above the limit split) */
+void DUNE_pbvh_bmesh_after_stroke(PBVH *pbvh);
+void BKE_pbvh_bmesh_after_stroke(PBVH *pbvh);
+/* pbvh_bmesh.c:2080-2084 */
+void DUNE_pbvh_bmesh_detail_size_set(PBVH *pbvh, float detail_size);
+void BKE_pbvh_bmesh_detail_size_set(PBVH *pbvh, float detail_size);
+/* pbvh_bmesh.c:2086-2089 */
+void DUNE_pbvh_node_mark_topology_update(PBVHNode *node);
+void BKE_pbvh_node_mark_topology_update(PBVHNode *node);
+/* pbvh_bmesh.c:2091-2104 return GSets; here the node's face slots / unique and other
+ * vertex slots are written to out (up to cap), returning the count. */
+int sg_pbvh_bmesh_node_faces(PBVHNode *node, int *out, int cap);
+int sg_pbvh_bmesh_node_unique_verts(PBVHNode *node, int *out, int cap);
+int sg_pbvh_bmesh_node_other_verts(PBVHNode *node, int *out, int cap);
+/* New: what a dab on a PBVH_BMESH does before its brush (the reference's sculpt code
+ * that calls update_topology is absent): Draw, Inflate and Smooth dabs update the
+ * topology of the leaves within 1.25 r with this mode (0: none), view normal and
+ * front-face / projected options; Grab keeps the topology. */
+int sg_pbvh_bmesh_topology_set(PBVH *pbvh, int mode, const float view_normal[3], int use_frontface, int use_projected);
+/* Counters of the last dab or topology update on a PBVH_BMESH: [0] gathered leaves
+ * [1] affected verts [2] splits [3] collapses [4] split rounds [5] collapse rounds
+ * [6] split candidates [7] collapse candidates [8] 1: capacity stop, 2: walk stack
+ * [9] leaves updated [10] faces re-normalled [11] leaves refit. */
+int sg_pbvh_bmesh_last_stats(PBVH *pbvh, long long out[12]);
+/* Affected vertex labels of the last dab, ascending. */
+int sg_pbvh_bmesh_last_affected(PBVH *pbvh, uint64_t *out, int cap);
+/* Node i: flag, children_offset, live faces; current and original box (6 each). */
+int sg_pbvh_bmesh_node_info(PBVH *pbvh, int node, int out[3], float vb[6], float orig_vb[6]);
 
 /* ---- synthetic meshes and strokes (deterministic; for tests and benchmarks) ---- */
 int sg_mesh_icosphere_size(int level, int *r_totvert, int *r_tottri);

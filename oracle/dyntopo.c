@@ -33,8 +33,8 @@
  *    (a band of its length first -- longest for splits, shortest for collapses, four
  *    bands per octave of squared length -- then a hash of its two labels), and applies every edge that holds the largest key on all faces it
  *    touches (its two faces for a split, whose neighbours are joined in a second
- *    phase; every face around both vertices for a collapse); an exact tie on a face
- *    makes both edges wait.  Rounds
+ *    phase; every face around both vertices for a collapse); when two edges share the
+ *    largest key of a face, both wait.  Rounds
  *    repeat until no edge qualifies.  The reference pops one edge at a time; its order
  *    among equal lengths follows pointer hashing (GSet iteration), so it has no
  *    reproducible order to match.  A single edge, or edges that touch no common face,
@@ -69,7 +69,7 @@
 #define DYN_SLAB 4096      /* face slots per leaf */
 #define DYN_MAX_ROUNDS 64  /* rounds per phase of one topology update */
 #define DYN_RING 128       /* faces around a vertex (larger: the vertex is skipped) */
-#define DYN_STACK 64       /* depth of the even-subdivision recursion */
+#define DYN_STACK 32       /* pending edges of the even-subdivision walk */
 
 struct OrcDyn {
   int V, Vcap, T, Tcap;
@@ -1162,12 +1162,21 @@ static void collapse_edge(OrcDyn *o, int cA, int *touched)
     }
     o->tv[cs[i]] = vcon;
   }
-  /* corners of the opposite vertices and of the kept vertex, in surviving faces */
-  o->vcorner[oA] = xA >= 0 ? c_next(xA) : c_prev(yA);
-  if (B >= 0) {
-    o->vcorner[oB] = xB >= 0 ? c_next(xB) : c_prev(yB);
+  /* corners that sat in the removed faces move to surviving faces */
+  const int fA = o->vcorner[oA] / 3;
+  if (fA == A || fA == B) {
+    o->vcorner[oA] = xA >= 0 ? c_next(xA) : c_prev(yA);
   }
-  o->vcorner[vcon] = xA >= 0 ? c_prev(xA) : c_next(yA);
+  if (B >= 0) {
+    const int fB = o->vcorner[oB] / 3;
+    if (fB == A || fB == B) {
+      o->vcorner[oB] = xB >= 0 ? c_next(xB) : c_prev(yB);
+    }
+  }
+  const int fc = o->vcorner[vcon] / 3;
+  if (fc == A || fc == B) {
+    o->vcorner[vcon] = xA >= 0 ? c_prev(xA) : c_next(yA);
+  }
   float *pc = o->co + 3 * vcon, *pd = o->co + 3 * vdel;
   for (int i = 0; i < 3; i++) {
     pc[i] = 0.5f * (pc[i] + pd[i]);
@@ -1441,7 +1450,7 @@ static int phase(OrcDyn *o, int collapse, const float center[3], float radius, W
   r.estamp = (int *)calloc(3 * (size_t)o->Tcap, sizeof(int));
   r.eser = 0;
   uint64_t *best = (uint64_t *)calloc((size_t)o->Tcap, sizeof(uint64_t));
-  uint8_t *tie = (uint8_t *)calloc((size_t)o->Tcap, 1);
+  uint64_t *tie = (uint64_t *)calloc((size_t)o->Tcap, sizeof(uint64_t)); /* largest tied key */
   uint64_t *key = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)r.cand_cap);
   uint8_t *valid = (uint8_t *)malloc((size_t)r.cand_cap);
   int *hv = (int *)malloc(sizeof(int) * (size_t)o->Vcap), nhv = 0;
@@ -1558,7 +1567,9 @@ static int phase(OrcDyn *o, int collapse, const float center[3], float radius, W
       }
       for (int j = 0; j < n; j++) {
         if (best[set[j]] == key[i]) {
-          tie[set[j]] = 1;
+          if (tie[set[j]] < key[i]) {
+            tie[set[j]] = key[i];
+          }
         }
         else if (best[set[j]] < key[i]) {
           best[set[j]] = key[i];
@@ -1576,7 +1587,7 @@ static int phase(OrcDyn *o, int collapse, const float center[3], float radius, W
       const int n = claim_set(o, r.cand[i], collapse, set, 2 * DYN_RING);
       int own = 1;
       for (int j = 0; j < n && own; j++) {
-        own = best[set[j]] == key[i] && !tie[set[j]];
+        own = best[set[j]] == key[i] && tie[set[j]] != key[i];
       }
       if (own) {
         win[nwin++] = r.cand[i];

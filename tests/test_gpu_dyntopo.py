@@ -1,0 +1,144 @@
+"""PBVH_BMESH on the GPU against the oracle (oracle/dyntopo.c), bit for bit through
+vertex labels: per dab the affected vertices, gathered leaves and counters (splits,
+collapses, rounds, candidates, leaves re-normalled and refit); at checkpoints every
+vertex (position, normal, mask, owner), every face (labels, leaf, normal) and every
+node (flags, children, face count, boxes); the drop-in update_topology / normals /
+bounds path; after_stroke."""
+import numpy as np
+import pytest
+
+from dune_sculpt_amd import dyntopo as dy
+
+pytestmark = pytest.mark.gpu
+
+
+def same_state(p, o, tag):
+    pv, pf = p.state()
+    ov, of = o.state()
+    assert np.array_equal(pv["lab"], ov["lab"]), f"{tag}: vertex labels"
+    for k in ("co", "no", "mask"):
+        assert np.array_equal(pv[k].view(np.uint32), ov[k].view(np.uint32)), f"{tag}: {k}"
+    assert np.array_equal(pv["node"], ov["node"]), f"{tag}: owners"
+    assert np.array_equal(pf["labs"], of["labs"]), f"{tag}: faces"
+    assert np.array_equal(pf["node"], of["node"]), f"{tag}: face leaves"
+    assert np.array_equal(pf["fno"].view(np.uint32), of["fno"].view(np.uint32)), f"{tag}: face normals"
+    assert np.array_equal(pf["hide"], of["hide"]), f"{tag}: face hidden"
+    info, vb, ob = p.nodes()
+    oinfo, ovb, oob = o.nodes()
+    assert np.array_equal(info[:, 0] & ~(1 << 9), oinfo[:, 0] & ~(1 << 10)), f"{tag}: node flags"
+    assert np.array_equal((info[:, 0] >> 9) & 1, (oinfo[:, 0] >> 10) & 1), f"{tag}: fully hidden"
+    assert np.array_equal(info[:, 1:], oinfo[:, 1:]), f"{tag}: children / face counts"
+    assert np.array_equal(vb.view(np.uint32), ovb.view(np.uint32)), f"{tag}: boxes"
+    assert np.array_equal(ob.view(np.uint32), oob.view(np.uint32)), f"{tag}: original boxes"
+
+
+def stroke(sg, seed, n, rmin, rmax):
+    rng = np.random.default_rng(seed)
+    radii = rng.uniform(rmin, rmax, n).astype(np.float32)
+    strengths = rng.uniform(0.2, 1.0, n)
+    centres = sg.great_circle_centres(seed, radii, spacing_frac=0.5)
+    tools = [sg.TOOL_DRAW, sg.TOOL_INFLATE, sg.TOOL_SMOOTH, sg.TOOL_GRAB]
+    out = []
+    for i in range(n):
+        nxt = centres[i + 1] if i + 1 < n else centres[i]
+        out.append(sg.make_dab(tool=tools[i % 4], location=centres[i], radius=float(radii[i]),
+                               strength=float(strengths[i]), smooth_iterations=1 + (i // 4) % 3,
+                               grab_delta=(nxt - centres[i]) * 0.5))
+    return out
+
+
+def pair(sg, po, co, tris, detail, **kw):
+    m = dy.DynMesh(co, tris, **kw)
+    p = dy.DynPBVH(m)
+    o = po.OracleDyn(co, tris, **kw)
+    p.detail_size_set(detail)
+    o.detail_size_set(detail)
+    return m, p, o
+
+
+def run(sg, po, p, o, dabs, check_every):
+    splits = collapses = 0
+    for i, d in enumerate(dabs):
+        print(f"dab {i} tool {d.sculpt_tool} r {d.radius:.3f}", flush=True)
+        p.dab(d)
+        want = o.dab(po.to_orc(d))
+        got = p.last_affected()
+        ps, os_ = p.last_stats(), o.last_stats()
+        assert np.array_equal(ps, os_), f"dab {i}: counters {ps.tolist()} vs {os_.tolist()}"
+        assert np.array_equal(got, want), f"dab {i}: affected"
+        assert sorted(p.last_gathered().tolist()) == sorted(o.search(d.location, d.radius).tolist()), f"dab {i}"
+        splits += ps[2]
+        collapses += ps[3]
+        if (i + 1) % check_every == 0 or i + 1 == len(dabs):
+            same_state(p, o, f"after dab {i}")
+    return splits, collapses
+
+
+def test_stroke_matches_oracle(sg, po):
+    co, tris = sg.noisy_icosphere(4, seed=1, amplitude=0.02)
+    m, p, o = pair(sg, po, co, tris, 0.03)
+    same_state(p, o, "built")
+    splits, collapses = run(sg, po, p, o, stroke(sg, 3, 24, 0.1, 0.2), 4)
+    assert splits > 500 and collapses > 50
+    p.after_stroke()
+    o.after_stroke()
+    same_state(p, o, "after stroke")
+    # the stroke goes on over the split leaves
+    run(sg, po, p, o, stroke(sg, 4, 8, 0.1, 0.2), 4)
+    p.close()
+    m.close()
+    o.close()
+
+
+def test_mask_hidden_frontface_projected(sg, po):
+    co, tris = sg.noisy_icosphere(4, seed=5, amplitude=0.02)
+    rng = np.random.default_rng(6)
+    c64 = co.astype(np.float64)
+    mask = (0.5 + 0.5 * np.sin(6 * c64[:, 0])).astype(np.float32)
+    mask[c64[:, 1] > 0.5] = 1.0
+    vh = (c64[:, 2] < -0.6).astype(np.uint8)
+    fh = (rng.uniform(0, 1, len(tris)) < 0.03).astype(np.uint8)
+    m, p, o = pair(sg, po, co, tris, 0.035, mask=mask, vhide=vh, fhide=fh)
+    for proj, front in ((False, True), (True, False), (True, True)):
+        p.topology_set(dy.SUBDIVIDE | dy.COLLAPSE, view_normal=(0.0, 0.6, 0.8), use_frontface=front,
+                       use_projected=proj)
+        o.topology_set(po.SUBDIVIDE | po.COLLAPSE, view_normal=(0.0, 0.6, 0.8), use_frontface=front,
+                       use_projected=proj)
+        run(sg, po, p, o, stroke(sg, 7 + proj + 2 * front, 8, 0.12, 0.25), 4)
+    p.close()
+    m.close()
+    o.close()
+
+
+def test_drop_in_update_topology_normals_bounds(sg, po):
+    co, tris = sg.noisy_icosphere(4, seed=8, amplitude=0.02)
+    m, p, o = pair(sg, po, co, tris, 0.04)
+    c = (0.0, 0.0, 1.0)
+    leaves = o.search(c, 0.45)
+    assert sorted(p.search(c, 0.45).tolist()) == sorted(leaves.tolist())
+    p.mark_topology(leaves)
+    o.mark(leaves, po.UPDATE_TOPOLOGY)
+    assert p.update_topology(dy.SUBDIVIDE | dy.COLLAPSE, c, 0.35)
+    assert o.update_topology(po.SUBDIVIDE | po.COLLAPSE, c, 0.35)
+    ps, os_ = p.last_stats(), o.last_stats()
+    assert np.array_equal(ps[2:10], os_[2:10]), (ps.tolist(), os_.tolist())
+    # the caller refreshes normals and bounds of the flagged leaves
+    p.update_normals()
+    o.update_normals()
+    p.update_bounds()
+    o.update_bounds()
+    same_state(p, o, "drop-in")
+    p.close()
+    m.close()
+    o.close()
+
+
+def test_larger_mesh_long_stroke(sg, po):
+    """icosphere 6 (41k vertices), 48 dabs with subdivision and collapse."""
+    co, tris = sg.noisy_icosphere(6, seed=9, amplitude=0.01)
+    m, p, o = pair(sg, po, co, tris, 0.012)
+    splits, collapses = run(sg, po, p, o, stroke(sg, 11, 48, 0.05, 0.12), 16)
+    assert splits > 1000
+    p.close()
+    m.close()
+    o.close()
