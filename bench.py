@@ -665,6 +665,80 @@ def run_grids(sg, W, K, args):
     return out
 
 
+def run_dyntopo(sg, W, K, args):
+    """Dynamic topology (PBVH_BMESH): a noisy icosphere (level args.dyn_level, noise
+    0.02 x 4 octaves), detail size args.dyn_detail (splits past it, collapses under 0.4 of
+    it, on the leaves within 1.25 r of each Draw / Inflate / Smooth dab), a stroke of
+    args.dyn_dabs dabs cycling Draw / Inflate / Smooth(2) / Grab, r ~ U[0.02, 0.05].  Each
+    step rebuilds the mesh and PBVH (untimed) and times the stroke (the host's clock around
+    it: each topology round waits for its counters).  CPU: the oracle, one thread, on the
+    first dabs of the same stroke within a budget, every dab's counters checked against
+    the GPU's."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle as po
+    from dune_sculpt_amd import dyntopo as dy
+
+    co, tris = sg.noisy_icosphere(args.dyn_level, seed=1, amplitude=0.02)
+    rng = np.random.default_rng(5)
+    n = args.dyn_dabs
+    radii = rng.uniform(0.02, 0.05, n).astype(np.float32)
+    strengths = rng.uniform(0.2, 1.0, n)
+    centres = sg.great_circle_centres(5, radii, spacing_frac=0.3)
+    tools = [sg.TOOL_DRAW, sg.TOOL_INFLATE, sg.TOOL_SMOOTH, sg.TOOL_GRAB]
+    dabs = []
+    for i in range(n):
+        nxt = centres[i + 1] if i + 1 < n else centres[i]
+        dabs.append(sg.make_dab(tool=tools[i % 4], location=centres[i], radius=float(radii[i]),
+                                strength=float(strengths[i]), smooth_iterations=2, grab_delta=(nxt - centres[i]) * 0.5))
+    times, stats, build = [], None, []
+    for s in range(W + K):
+        t = time.perf_counter()
+        m = dy.DynMesh(co, tris)
+        P = dy.DynPBVH(m)
+        P.detail_size_set(args.dyn_detail)
+        build.append(time.perf_counter() - t)
+        st = []
+        t = time.perf_counter()
+        for d in dabs:
+            P.dab(d)
+            st.append(P.last_stats())
+        dt = time.perf_counter() - t
+        if s >= W:
+            times.append(dt)
+        stats = np.array(st)
+        cnt = m.counts()
+        P.close()
+        m.close()
+    sec = float(np.median(times))
+    aff = int(stats[:, 1].sum())
+    out = {"workload": f"icosphere-{args.dyn_level} noisy 0.02x4oct ({len(co)} V, {len(tris)} T), detail size "
+                       f"{args.dyn_detail}, {n}-dab Draw/Inflate/Smooth(2)/Grab stroke, r~U[0.02,0.05]",
+           "build_s": float(np.median(build)), "ms_per_stroke": sec * 1e3, "us_per_dab": sec * 1e6 / n,
+           "dabs_per_s": n / sec, "vertex_updates_per_s": aff / sec,
+           "per_dab": {"affected": aff / n, "splits": float(stats[:, 2].mean()),
+                       "collapses": float(stats[:, 3].mean()), "split_rounds": float(stats[:, 4].mean()),
+                       "collapse_rounds": float(stats[:, 5].mean()), "leaves_refit": float(stats[:, 11].mean())},
+           "final_counts": cnt, "capacity_stops": int((stats[:, 8] & 1).sum())}
+    if not args.no_cpu:
+        o = po.OracleDyn(co, tris)
+        o.detail_size_set(args.dyn_detail)
+        t0 = time.perf_counter()
+        k = 0
+        for i, d in enumerate(dabs):
+            o.dab(po.to_orc(d))
+            assert np.array_equal(o.last_stats(), stats[i]), f"dyntopo dab {i}: oracle counters differ from the GPU's"
+            k += 1
+            if time.perf_counter() - t0 > args.cpu_budget / 2:
+                break
+        dt = time.perf_counter() - t0
+        o.close()
+        out["cpu_baseline"] = {"value": k / dt, "unit": "dabs/s", "cores": 1, "kind": "port",
+                               "vertex_updates_per_s": float(stats[:k, 1].sum()) / dt,
+                               "sample": f"first {k} dabs ({dt:.1f}s, C oracle, one thread)"}
+    log(f"dyntopo: {out}")
+    return out
+
+
 def cpu_model():
     """The host CPU's model name, for the record beside the CPU baseline."""
     try:
@@ -746,6 +820,11 @@ def main():
     ap.add_argument("--grids-n", type=int, default=64, help="coarse quads per cube side of the grids mesh")
     ap.add_argument("--grids-level", type=int, default=4, help="subdivision level of the grids (grid size 2^(l-1)+1)")
     ap.add_argument("--grids-stream", action="store_true", help="grids stroke as stream launches instead of a graph")
+    ap.add_argument("--no-dyn", action="store_true", help="skip the dynamic topology (PBVH_BMESH) stroke")
+    ap.add_argument("--only-dyn", action="store_true", help="run only the dynamic topology stroke")
+    ap.add_argument("--dyn-level", type=int, default=8, help="icosphere level of the dynamic topology mesh")
+    ap.add_argument("--dyn-detail", type=float, default=0.003, help="dynamic topology detail size (max edge length)")
+    ap.add_argument("--dyn-dabs", type=int, default=200, help="dabs in the dynamic topology stroke")
     ap.add_argument("--no-fused", action="store_true", help="separate brush / Smooth / post kernels (A/B)")
     ap.add_argument("--only-elastic", action="store_true", help="run only the elastic deform measurement")
     ap.add_argument("--placed-only", action="store_true",
@@ -766,6 +845,9 @@ def main():
         return
     if args.only_grids:
         print(json.dumps({"grids": run_grids(sg, args.warmup, args.steps, args)}))
+        return
+    if args.only_dyn:
+        print(json.dumps({"dyntopo": run_dyntopo(sg, args.warmup, min(args.steps, 3), args)}))
         return
 
     t = time.perf_counter()
@@ -926,6 +1008,7 @@ def main():
 
     c1 = None if args.no_c1 else run_c1(sg, W, K)
     grids = None if args.no_grids else run_grids(sg, W, K, args)
+    dyn = None if args.no_dyn else run_dyntopo(sg, 1, 2, args)
     c3 = None
     if not args.no_learned:
         c3 = run_learned(sg, A, W, args)
@@ -993,6 +1076,7 @@ def main():
         "hbm_copy": copy,
         "c1": c1,
         "grids": grids,
+        "dyntopo": dyn,
         "learned_smooth": c3,
         "batch": batch,
         "raycast": ray,

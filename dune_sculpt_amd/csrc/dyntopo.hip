@@ -29,7 +29,7 @@
 #include "sg_kernels.h"
 
 #define DY_THREADS 256
-#define DY_BLOCKS 512
+#define DY_BLOCKS 160
 
 namespace {
 
@@ -47,6 +47,7 @@ struct DV {
   unsigned long long *claim;
   int *estamp, *remap, *remap_ser;
   int *node_flag, *node_child, *node_dirty, *leaf_node, *lvl_off, *lvl_node;
+  int *node_parent, *node_height, *anc_stamp, *lvl_list, *lvl_cnt;
   float *node_bb, *node_obb;
   int *slab, *scount;
   int *wlist, *cand, *win, *outer, *hv_in, *hv_out, *touched, *gathered, *vlist, *aff, *nlist;
@@ -516,11 +517,12 @@ __device__ __forceinline__ void slab_append(const DV &g, int node, int f) {
   g.tslot[f] = pos;
 }
 
-__device__ __forceinline__ void w_add(const DV &g, int f, int wser) {
-  if (atomicExch(&g.wstamp[f], wser) != wser) {
-    const unsigned long long s = atomicAdd((unsigned long long *)&g.ctl[SGD_WN], 1ull);
-    g.wlist[s] = f;
-  }
+// a leaf touched by a split: TOUCH set and FullyHidden cleared (face create /
+// remove, pbvh_bmesh.c:495-519, :634-672); the atomics only when a bit changes
+__device__ __forceinline__ void touch_leaf(const DV &g, int leaf) {
+  const int f = g.node_flag[leaf];
+  if ((f & TOUCH) != TOUCH) atomicOr(&g.node_flag[leaf], TOUCH);
+  if (f & PBVH_FullyHidden) atomicAnd(&g.node_flag[leaf], ~PBVH_FullyHidden);
 }
 
 __device__ __forceinline__ void hv_add(const DV &g, int v, int stamp) {
@@ -605,16 +607,14 @@ __device__ void split_apply(const DV &g, int wi, int cA, int serial, int round, 
   g.fnz[NA] = g.fnz[A];
   g.fhide[NA] = g.fhide[A];
   slab_append(g, leafA, NA);
-  atomicOr(&g.node_flag[leafA], TOUCH);
-  atomicAnd(&g.node_flag[leafA], ~PBVH_FullyHidden);
+  touch_leaf(g, leafA);
   if (B >= 0) {
     g.fnx[NB] = g.fnx[B];
     g.fny[NB] = g.fny[B];
     g.fnz[NB] = g.fnz[B];
     g.fhide[NB] = g.fhide[B];
     slab_append(g, leafB, NB);
-    atomicOr(&g.node_flag[leafB], TOUCH);
-    atomicAnd(&g.node_flag[leafB], ~PBVH_FullyHidden);
+    touch_leaf(g, leafB);
   }
   const int vs[4] = {p, q, oA, oB};
   for (int i = 0; i < 4; i++) {
@@ -627,11 +627,19 @@ __device__ void split_apply(const DV &g, int wi, int cA, int serial, int round, 
       g.vcorner[v] = (v == q) ? 3 * B : (v == p) ? 3 * NB + 1 : 3 * B + 2;
   }
   g.vcorner[m] = 3 * A + 1;
-  w_add(g, A, wser);
-  w_add(g, NA, wser);
-  if (B >= 0) {
-    w_add(g, B, wser);
-    w_add(g, NB, wser);
+  // the working set gains this split's faces (owned here: plain stamps, one append)
+  int fs[4], nf = 0;
+  const int own[4] = {A, NA, B, NB};
+  for (int i = 0; i < 4; i++) {
+    const int f = own[i];
+    if (f >= 0 && g.wstamp[f] != wser) {
+      g.wstamp[f] = wser;
+      fs[nf++] = f;
+    }
+  }
+  if (nf) {
+    const int base = (int)atomicAdd((unsigned long long *)&g.ctl[SGD_WN], (unsigned long long)nf);
+    for (int i = 0; i < nf; i++) g.wlist[base + i] = fs[i];
   }
   hv_add(g, oA, stamp);
   hv_add(g, oB, stamp);
@@ -1009,7 +1017,7 @@ __global__ __launch_bounds__(DY_THREADS) void kd_vert_normals(DV g) {
 }
 
 // leaf boxes over the leaf faces' vertices (zeros as +0): a workgroup per listed leaf
-__global__ __launch_bounds__(DY_THREADS) void kd_leaf_bounds(DV g) {
+__global__ __launch_bounds__(DY_THREADS) void kd_leaf_bounds(DV g, int stamp) {
   __shared__ float s_mn[3][DY_THREADS / 64], s_mx[3][DY_THREADS / 64];
   const int nl = (int)g.ctl[SGD_NLIST];
   for (int li = blockIdx.x; li < nl; li += gridDim.x) {
@@ -1063,19 +1071,30 @@ __global__ __launch_bounds__(DY_THREADS) void kd_leaf_bounds(DV g) {
     if (threadIdx.x == 0) {
       if (fl & PBVH_UpdateOriginalBB)
         for (int j = 0; j < 8; j++) g.node_obb[8 * (size_t)node + j] = g.node_bb[8 * (size_t)node + j];
-      g.node_dirty[node] = fl & (PBVH_UpdateBB | PBVH_UpdateOriginalBB);
+      const int dirty = fl & (PBVH_UpdateBB | PBVH_UpdateOriginalBB);
+      g.node_dirty[node] = dirty;
       atomicAnd(&g.node_flag[node], ~(PBVH_UpdateBB | PBVH_UpdateOriginalBB | PBVH_UpdateRedraw));
+      // the ancestors the flush visits, each listed once by height (a walk stops where
+      // another leaf's walk has passed)
+      if (dirty)
+        for (int a = g.node_parent[node]; a >= 0; a = g.node_parent[a]) {
+          if (atomicExch(&g.anc_stamp[a], stamp) == stamp) break;
+          const int h = g.node_height[a];
+          const int s = atomicAdd(&g.lvl_cnt[h], 1);
+          g.lvl_list[g.lvl_off[h - 1] + s] = a;
+        }
     }
   }
 }
 
-// internal nodes with an updated child: union of the children (child 1, then 2), by
-// height; one workgroup
+// internal nodes with an updated child: union of the children (child 1, then 2),
+// height by height over the listed ancestors of refit leaves; one workgroup.  Nodes
+// off those paths have no updated child (pbvh_flush_bb leaves them as they are).
 __global__ __launch_bounds__(1024) void kd_flush(DV g, int H) {
-  for (int h = 0; h < H; h++) {
-    const int b = g.lvl_off[h], e = g.lvl_off[h + 1];
-    for (int i = b + threadIdx.x; i < e; i += blockDim.x) {
-      const int node = g.lvl_node[i];
+  for (int h = 1; h <= H; h++) {
+    const int b = g.lvl_off[h - 1], n = g.lvl_cnt[h];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      const int node = g.lvl_list[b + i];
       const int c = g.node_child[node];
       const int u = g.node_dirty[c] | g.node_dirty[c + 1];
       if (u & PBVH_UpdateBB) {
@@ -1098,7 +1117,15 @@ __global__ __launch_bounds__(1024) void kd_flush(DV g, int H) {
     }
     __syncthreads();
   }
-  for (int i = threadIdx.x; i < g.N; i += blockDim.x) g.node_dirty[i] = 0;
+  // clear the marks: the listed ancestors, the refit leaves, the level counts
+  for (int h = 1; h <= H; h++) {
+    const int b = g.lvl_off[h - 1], n = g.lvl_cnt[h];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) g.node_dirty[g.lvl_list[b + i]] = 0;
+  }
+  const int nl = (int)g.ctl[SGD_NLIST];
+  for (int i = threadIdx.x; i < nl; i += blockDim.x) g.node_dirty[g.nlist[i]] = 0;
+  __syncthreads();
+  for (int h = threadIdx.x; h <= H; h += blockDim.x) g.lvl_cnt[h] = 0;
 }
 
 __global__ void kd_ctl_zero(DV g, int b, int e) {
@@ -1174,6 +1201,11 @@ DV view(PBVH *p) {
   g.leaf_node = d.leaf_node;
   g.lvl_off = d.lvl_off;
   g.lvl_node = d.lvl_node;
+  g.node_parent = d.node_parent;
+  g.node_height = d.node_height;
+  g.anc_stamp = d.anc_stamp;
+  g.lvl_list = d.lvl_list;
+  g.lvl_cnt = d.lvl_cnt;
   g.node_bb = d.node_bb;
   g.node_obb = d.node_obb;
   g.slab = d.slab;
@@ -1224,7 +1256,8 @@ void sg_dyn_free_device(PBVH *p) {
   for (float **q : fl) dfree(*q);
   int **il[] = {&d.vnode, &d.vcorner, &d.vstamp, &d.astamp, &d.aslot, &d.hvst, &d.tv, &d.topp, &d.tnode, &d.tslot,
                 &d.wstamp, &d.estamp, &d.remap, &d.remap_ser, &d.node_flag, &d.node_child, &d.node_dirty,
-                &d.leaf_node, &d.lvl_off, &d.lvl_node, &d.slab, &d.scount, &d.wlist, &d.cand, &d.win, &d.outer,
+                &d.leaf_node, &d.lvl_off, &d.lvl_node, &d.node_parent, &d.node_height, &d.anc_stamp, &d.lvl_list,
+                &d.lvl_cnt, &d.slab, &d.scount, &d.wlist, &d.cand, &d.win, &d.outer,
                 &d.hv0, &d.hv1, &d.touched, &d.gathered, &d.vlist, &d.aff, &d.nlist};
   for (int **q : il) dfree(*q);
   dfree(d.vhide);
@@ -1275,6 +1308,8 @@ int sg_dyn_upload(PBVH *p) {
     rc |= dalloc(d.node_flag, Nc) | dalloc(d.node_child, Nc) | dalloc(d.node_dirty, Nc) | dalloc(d.leaf_node, Nc);
     rc |= dalloc(d.lvl_off, Nc + 1) | dalloc(d.lvl_node, Nc) | dalloc(d.node_bb, 8 * Nc) | dalloc(d.node_obb, 8 * Nc);
     rc |= dalloc(d.slab, Nc * SG_DYN_SLAB) | dalloc(d.scount, Nc) | dalloc(d.gathered, Nc) | dalloc(d.nlist, Nc);
+    rc |= dalloc(d.node_parent, Nc) | dalloc(d.node_height, Nc) | dalloc(d.anc_stamp, Nc) | dalloc(d.lvl_list, Nc);
+    rc |= dalloc(d.lvl_cnt, Nc + 1);
     if (rc) return -1;
     d.Ncap = (int)Nc;
   }
@@ -1348,6 +1383,12 @@ int sg_dyn_upload(PBVH *p) {
   if (!lvl_node.empty())
     rc |= sg_check_hip(hipMemcpy(d.lvl_node, lvl_node.data(), lvl_node.size() * 4, hipMemcpyHostToDevice), "copy");
   rc |= sg_check_hip(hipMemset(d.node_dirty, 0, N * 4), "memset");
+  std::vector<int> parent(N, -1);
+  for (int i = 0; i < N; i++)
+    if (!(t.flag[i] & PBVH_Leaf)) parent[t.child[i]] = parent[t.child[i] + 1] = i;
+  rc |= sg_check_hip(hipMemcpy(d.node_parent, parent.data(), N * 4, hipMemcpyHostToDevice), "copy");
+  rc |= sg_check_hip(hipMemcpy(d.node_height, height.data(), N * 4, hipMemcpyHostToDevice), "copy");
+  rc |= sg_check_hip(hipMemset(d.lvl_cnt, 0, (H + 1) * 4), "memset");
   std::vector<long long> ctl(SGD_COUNT, 0);
   ctl[SGD_V] = m.V;
   ctl[SGD_T] = m.T;
@@ -1540,7 +1581,7 @@ int sg_dyn_update_normals_bounds(PBVH *p, bool normals, bool bounds) {
     zero_ctl(p, g, SGD_NLIST, SGD_NLIST + 1);
     zero_ctl(p, g, SGD_NFN + 1, SGD_NFN + 2);
     kd_flagged<<<lb, DY_THREADS, 0, st>>>(g, PBVH_UpdateBB | PBVH_UpdateOriginalBB | PBVH_UpdateRedraw);
-    kd_leaf_bounds<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g);
+    kd_leaf_bounds<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, ++s->stamp);
     kd_flush<<<1, 1024, 0, st>>>(g, d.H);
   }
   if (launch_check("dyntopo normals/bounds") || read_ctl(p)) return -1;

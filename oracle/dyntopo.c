@@ -89,6 +89,13 @@ struct OrcDyn {
   int mode, use_frontface, use_projected;
   float view_normal[3];
   int64_t stats[12];
+  /* scratch kept across calls, valid by stamp (sized by the capacities) */
+  int stamp;
+  int *estamp, *remap, *remap_ser; /* per corner */
+  int *wstamp, *wlist;             /* per face */
+  uint64_t *best, *tie;            /* per face: largest key, largest tied key */
+  int *vst, *vslot;                /* per vertex */
+  int *cand, cand_mem;             /* grown on demand */
 };
 
 /* stats: [0] gathered [1] affected [2] splits [3] collapses [4] split rounds
@@ -607,6 +614,16 @@ void orc_dyn_free(OrcDyn *o)
   free(o->nodes);
   free(o->slab);
   free(o->scount);
+  free(o->estamp);
+  free(o->remap);
+  free(o->remap_ser);
+  free(o->wstamp);
+  free(o->wlist);
+  free(o->best);
+  free(o->tie);
+  free(o->vst);
+  free(o->vslot);
+  free(o->cand);
   free(o);
 }
 
@@ -686,6 +703,15 @@ OrcDyn *orc_dyn_new(const float *co, int totvert, const int *tris, int tottri, c
   if (fhide) {
     memcpy(o->fhide, fhide, (size_t)tottri);
   }
+  o->estamp = (int *)calloc(3 * (size_t)tri_cap, sizeof(int));
+  o->remap = (int *)malloc(sizeof(int) * 3 * (size_t)tri_cap);
+  o->remap_ser = (int *)calloc(3 * (size_t)tri_cap, sizeof(int));
+  o->wstamp = (int *)calloc((size_t)tri_cap, sizeof(int));
+  o->wlist = (int *)malloc(sizeof(int) * (size_t)tri_cap);
+  o->best = (uint64_t *)calloc((size_t)tri_cap, sizeof(uint64_t));
+  o->tie = (uint64_t *)calloc((size_t)tri_cap, sizeof(uint64_t));
+  o->vst = (int *)calloc((size_t)vert_cap, sizeof(int));
+  o->vslot = (int *)malloc(sizeof(int) * (size_t)vert_cap);
   memcpy(o->co, co, sizeof(float) * 3 * (size_t)totvert);
   for (int v = 0; v < vert_cap; v++) {
     o->lab[v] = (uint64_t)v;
@@ -853,7 +879,7 @@ int orc_dyn_search(const OrcDyn *o, const float c[3], float radius, int *out, in
  * leaves, then the vertices of those faces (all face normals first). */
 static void d_update_normals(OrcDyn *o)
 {
-  uint8_t *vseen = (uint8_t *)calloc((size_t)o->V + 1, 1);
+  const int seen_stamp = ++o->stamp;
   int *vl = (int *)malloc(sizeof(int) * ((size_t)o->V + 1));
   int nv = 0;
   int64_t nf = 0;
@@ -872,8 +898,8 @@ static void d_update_normals(OrcDyn *o)
       nf++;
       for (int k = 0; k < 3; k++) {
         const int v = o->tv[3 * f + k];
-        if (!vseen[v]) {
-          vseen[v] = 1;
+        if (o->vst[v] != seen_stamp) {
+          o->vst[v] = seen_stamp;
           vl[nv++] = v;
         }
       }
@@ -884,7 +910,6 @@ static void d_update_normals(OrcDyn *o)
     vert_normal(o, vl[i]);
   }
   o->stats[10] = nf;
-  free(vseen);
   free(vl);
 }
 
@@ -981,8 +1006,7 @@ static const int TOUCH = ORC_UpdateNormals | ORC_UpdateBB | ORC_UpdateDrawBuffer
  * old id of its neighbour corner and the old outer corners record where they went
  * (remap, stamped with the round).  The second phase (split_relink) joins the outer
  * edges once every split of the round has moved its corners. */
-static int split_edge(OrcDyn *o, int cA, int serial, int round, int *remap, int *remap_ser, int *new_faces,
-                      int *opps, int *outer)
+static int split_edge(OrcDyn *o, int cA, int serial, int round, int stamp, int *new_faces, int *opps, int *outer)
 {
   const int cB = o->topp[cA];
   const int A = cA / 3, B = cB >= 0 ? cB / 3 : -1;
@@ -1026,10 +1050,10 @@ static int split_edge(OrcDyn *o, int cA, int serial, int round, int *remap, int 
   tp[3 * NA + 1] = 3 * A;
   tp[3 * A + 1] = yA; /* (oA, p): old neighbour id, joined by split_relink */
   tp[3 * NA] = xA;    /* (q, oA) */
-  remap[oldA_y] = 3 * A + 1;
-  remap_ser[oldA_y] = round + 1;
-  remap[oldA_x] = 3 * NA;
-  remap_ser[oldA_x] = round + 1;
+  o->remap[oldA_y] = 3 * A + 1;
+  o->remap_ser[oldA_y] = stamp;
+  o->remap[oldA_x] = 3 * NA;
+  o->remap_ser[oldA_x] = stamp;
   outer[0] = 3 * A + 1;
   outer[1] = 3 * NA;
   outer[2] = outer[3] = -1;
@@ -1045,10 +1069,10 @@ static int split_edge(OrcDyn *o, int cA, int serial, int round, int *remap, int 
     tp[3 * NB + 1] = 3 * B;
     tp[3 * B + 1] = yB; /* (oB, q) */
     tp[3 * NB] = xB;    /* (p, oB) */
-    remap[oldB_y] = 3 * B + 1;
-    remap_ser[oldB_y] = round + 1;
-    remap[oldB_x] = 3 * NB;
-    remap_ser[oldB_x] = round + 1;
+    o->remap[oldB_y] = 3 * B + 1;
+    o->remap_ser[oldB_y] = stamp;
+    o->remap[oldB_x] = 3 * NB;
+    o->remap_ser[oldB_x] = stamp;
     outer[2] = 3 * B + 1;
     outer[3] = 3 * NB;
     /* the halves: A' (p, m) with NB (m, p); NA (m, q) with B' (q, m) */
@@ -1100,7 +1124,7 @@ static int split_edge(OrcDyn *o, int cA, int serial, int round, int *remap, int 
 /* Second phase of a split: each new outer corner k meets its neighbour, moved by its
  * own split this round (remap) or not (then the neighbour's corner is pointed back
  * at k here; no other split touches it). */
-static void split_relink(OrcDyn *o, const int *outer, const int *remap, const int *remap_ser, int round)
+static void split_relink(OrcDyn *o, const int *outer, int stamp)
 {
   for (int i = 0; i < 4; i++) {
     const int k = outer[i];
@@ -1111,8 +1135,8 @@ static void split_relink(OrcDyn *o, const int *outer, const int *remap, const in
     if (y < 0) {
       continue;
     }
-    if (remap_ser[y] == round + 1) {
-      o->topp[k] = remap[y];
+    if (o->remap_ser[y] == stamp) {
+      o->topp[k] = o->remap[y];
     }
     else {
       o->topp[y] = k;
@@ -1351,9 +1375,8 @@ static uint64_t edge_key(const OrcDyn *o, int c, int collapse)
 
 typedef struct {
   OrcDyn *o;
-  int *cand, ncand, cand_cap;
-  int *estamp; /* per corner: registration round serial */
-  int eser;
+  int ncand, cand_cap;
+  int eser; /* this round's stamp (corner registrations) */
 } Reg;
 
 static void reg_edge(Reg *r, int c)
@@ -1363,12 +1386,16 @@ static void reg_edge(Reg *r, int c)
     return;
   }
   const int k = canon(o, c);
-  if (r->estamp[k] == r->eser) {
+  if (o->estamp[k] == r->eser) {
     return;
   }
-  r->estamp[k] = r->eser;
+  o->estamp[k] = r->eser;
   if (r->ncand < r->cand_cap) {
-    r->cand[r->ncand] = k;
+    if (r->ncand >= o->cand_mem) {
+      o->cand_mem = o->cand_mem ? 2 * o->cand_mem : 4096;
+      o->cand = (int *)realloc(o->cand, sizeof(int) * (size_t)o->cand_mem);
+    }
+    o->cand[r->ncand] = k;
   }
   r->ncand++;
 }
@@ -1446,24 +1473,19 @@ static int phase(OrcDyn *o, int collapse, const float center[3], float radius, W
   Reg r;
   r.o = o;
   r.cand_cap = 2 * o->Tcap + 8;
-  r.cand = (int *)malloc(sizeof(int) * (size_t)r.cand_cap);
-  r.estamp = (int *)calloc(3 * (size_t)o->Tcap, sizeof(int));
   r.eser = 0;
-  uint64_t *best = (uint64_t *)calloc((size_t)o->Tcap, sizeof(uint64_t));
-  uint64_t *tie = (uint64_t *)calloc((size_t)o->Tcap, sizeof(uint64_t)); /* largest tied key */
-  uint64_t *key = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)r.cand_cap);
-  uint8_t *valid = (uint8_t *)malloc((size_t)r.cand_cap);
-  int *hv = (int *)malloc(sizeof(int) * (size_t)o->Vcap), nhv = 0;
-  int *hv2 = (int *)malloc(sizeof(int) * (size_t)o->Vcap), nhv2 = 0;
-  int *vst = (int *)calloc((size_t)o->Vcap, sizeof(int));
+  uint64_t *best = o->best, *tie = o->tie; /* largest key, largest tied key */
+  int *vst = o->vst;
+  uint64_t *key = NULL;
+  uint8_t *valid = NULL;
+  int *touched = NULL;
+  int buf_mem = 0;
+  int *hv = NULL, *hv2 = NULL, nhv = 0, nhv2 = 0, hv_mem = 0;
   int *set = (int *)malloc(sizeof(int) * 2 * DYN_RING);
-  int *touched = (int *)malloc(sizeof(int) * 3 * (size_t)r.cand_cap);
-  int *remap = (int *)malloc(sizeof(int) * 3 * (size_t)o->Tcap);
-  int *remap_ser = (int *)calloc(3 * (size_t)o->Tcap, sizeof(int));
   int modified = 0, rounds = 0;
   int64_t ops = 0, ncand_total = 0;
   for (int round = 0; round < DYN_MAX_ROUNDS; round++) {
-    r.eser = round + 1;
+    r.eser = ++o->stamp;
     r.ncand = 0;
     for (int i = 0; i < ws->wn; i++) {
       const int f = ws->w[i];
@@ -1521,10 +1543,21 @@ static int phase(OrcDyn *o, int collapse, const float center[3], float radius, W
       break;
     }
     rounds++;
+    if (r.ncand > buf_mem) {
+      buf_mem = 2 * r.ncand;
+      key = (uint64_t *)realloc(key, sizeof(uint64_t) * (size_t)buf_mem);
+      valid = (uint8_t *)realloc(valid, (size_t)buf_mem);
+      touched = (int *)realloc(touched, sizeof(int) * 3 * (size_t)buf_mem);
+    }
+    if (2 * r.ncand > hv_mem) {
+      hv_mem = 4 * r.ncand;
+      hv = (int *)realloc(hv, sizeof(int) * (size_t)hv_mem);
+      hv2 = (int *)realloc(hv2, sizeof(int) * (size_t)hv_mem);
+    }
     /* validity and keys */
     int nvalid = 0;
     for (int i = 0; i < r.ncand; i++) {
-      const int c = r.cand[i];
+      const int c = o->cand[i];
       const int p = o->tv[c_next(c)], q = o->tv[c_prev(c)];
       int ok = o->vnode[p] >= 0 && o->vnode[q] >= 0;
       if (ok && collapse) {
@@ -1550,7 +1583,7 @@ static int phase(OrcDyn *o, int collapse, const float center[3], float radius, W
       if (!valid[i]) {
         continue;
       }
-      const int n = claim_set(o, r.cand[i], collapse, set, 2 * DYN_RING);
+      const int n = claim_set(o, o->cand[i], collapse, set, 2 * DYN_RING);
       for (int j = 0; j < n; j++) {
         best[set[j]] = 0;
         tie[set[j]] = 0;
@@ -1560,7 +1593,7 @@ static int phase(OrcDyn *o, int collapse, const float center[3], float radius, W
       if (!valid[i]) {
         continue;
       }
-      const int n = claim_set(o, r.cand[i], collapse, set, 2 * DYN_RING);
+      const int n = claim_set(o, o->cand[i], collapse, set, 2 * DYN_RING);
       if (n < 0) {
         valid[i] = 0;
         continue;
@@ -1584,13 +1617,13 @@ static int phase(OrcDyn *o, int collapse, const float center[3], float radius, W
       if (!valid[i]) {
         continue;
       }
-      const int n = claim_set(o, r.cand[i], collapse, set, 2 * DYN_RING);
+      const int n = claim_set(o, o->cand[i], collapse, set, 2 * DYN_RING);
       int own = 1;
       for (int j = 0; j < n && own; j++) {
         own = best[set[j]] == key[i] && tie[set[j]] != key[i];
       }
       if (own) {
-        win[nwin++] = r.cand[i];
+        win[nwin++] = o->cand[i];
       }
     }
     int ntouched = 0;
@@ -1602,7 +1635,7 @@ static int phase(OrcDyn *o, int collapse, const float center[3], float radius, W
       else {
         int nf[2], op[2];
         const int cB = o->topp[win[i]];
-        split_edge(o, win[i], o->serial, round, remap, remap_ser, nf, op, outer + 4 * i);
+        split_edge(o, win[i], o->serial, round, r.eser, nf, op, outer + 4 * i);
         w_add(ws, win[i] / 3);
         if (cB >= 0) {
           w_add(ws, cB / 3);
@@ -1611,8 +1644,8 @@ static int phase(OrcDyn *o, int collapse, const float center[3], float radius, W
           if (nf[j] >= 0) {
             w_add(ws, nf[j]);
           }
-          if (op[j] >= 0 && vst[op[j]] != round + 1) {
-            vst[op[j]] = round + 1;
+          if (op[j] >= 0 && vst[op[j]] != r.eser) {
+            vst[op[j]] = r.eser;
             hv2[nhv2++] = op[j];
           }
         }
@@ -1620,7 +1653,7 @@ static int phase(OrcDyn *o, int collapse, const float center[3], float radius, W
     }
     if (!collapse) {
       for (int i = 0; i < nwin; i++) {
-        split_relink(o, outer + 4 * i, remap, remap_ser, round);
+        split_relink(o, outer + 4 * i, r.eser);
       }
     }
     for (int i = 0; i < ntouched; i++) {
@@ -1646,19 +1679,12 @@ static int phase(OrcDyn *o, int collapse, const float center[3], float radius, W
   o->stats[collapse ? 3 : 2] += ops;
   o->stats[collapse ? 5 : 4] += rounds;
   o->stats[collapse ? 7 : 6] += ncand_total;
-  free(r.cand);
-  free(r.estamp);
-  free(best);
-  free(tie);
   free(key);
   free(valid);
   free(hv);
   free(hv2);
-  free(vst);
   free(set);
   free(touched);
-  free(remap);
-  free(remap_ser);
   return modified;
 }
 
@@ -1668,9 +1694,9 @@ int orc_dyn_update_topology(OrcDyn *o, int mode, const float center[3], float ra
 {
   WSet ws;
   ws.wcap = o->Tcap;
-  ws.w = (int *)malloc(sizeof(int) * (size_t)ws.wcap);
-  ws.wstamp = (int *)calloc((size_t)o->Tcap, sizeof(int));
-  ws.wser = 1;
+  ws.w = o->wlist;
+  ws.wstamp = o->wstamp;
+  ws.wser = ++o->stamp;
   ws.wn = 0;
   int nl = 0;
   for (int ni = 0; ni < o->totnode; ni++) {
@@ -1697,8 +1723,6 @@ int orc_dyn_update_topology(OrcDyn *o, int mode, const float center[3], float ra
     o->nodes[ni].flag &= ~ORC_UpdateTopology;
   }
   o->serial++;
-  free(ws.w);
-  free(ws.wstamp);
   return modified;
 }
 
@@ -1741,7 +1765,7 @@ int orc_dyn_dab(OrcDyn *o, const OrcDab *dab, uint64_t *out_affected, int cap)
   o->stats[0] = nl;
   /* the gathered leaves' unique vertices, each once */
   int *cand = (int *)malloc(sizeof(int) * ((size_t)o->V + 1));
-  uint8_t *seen = (uint8_t *)calloc((size_t)o->V + 1, 1);
+  const int seen_stamp = ++o->stamp;
   int ncand = 0;
   for (int i = 0; i < nl; i++) {
     const int ni = leaves[i];
@@ -1753,8 +1777,8 @@ int orc_dyn_dab(OrcDyn *o, const OrcDab *dab, uint64_t *out_affected, int cap)
       }
       for (int k = 0; k < 3; k++) {
         const int v = o->tv[3 * f + k];
-        if (o->vnode[v] == ni && !seen[v]) {
-          seen[v] = 1;
+        if (o->vnode[v] == ni && o->vst[v] != seen_stamp) {
+          o->vst[v] = seen_stamp;
           cand[ncand++] = v;
         }
       }
@@ -1807,14 +1831,12 @@ int orc_dyn_dab(OrcDyn *o, const OrcDab *dab, uint64_t *out_affected, int cap)
   if (tool == ORC_TOOL_SMOOTH) {
     /* spec S8: k Jacobi steps, neighbour mean as an exact fixed-point sum */
     const int iters = dab->smooth_iters > 0 ? dab->smooth_iters : 1;
-    int *slot = (int *)malloc(sizeof(int) * ((size_t)o->V + 1));
-    for (int v = 0; v < o->V; v++) {
-      slot[v] = -1;
-    }
+    const int aff_stamp = ++o->stamp;
     float *prev = (float *)malloc(sizeof(float) * 3 * ((size_t)ncand + 1));
     for (int i = 0; i < ncand; i++) {
       if (aff[i]) {
-        slot[cand[i]] = i;
+        o->vst[cand[i]] = aff_stamp;
+        o->vslot[cand[i]] = i;
       }
       memcpy(prev + 3 * i, o->co + 3 * cand[i], sizeof(float) * 3);
     }
@@ -1828,7 +1850,7 @@ int orc_dyn_dab(OrcDyn *o, const OrcDab *dab, uint64_t *out_affected, int cap)
         int64_t s[3] = {0, 0, 0};
         for (int q = 0; q < nn; q++) {
           const int u = nb[q];
-          const float *pu = slot[u] >= 0 ? prev + 3 * slot[u] : o->co + 3 * u;
+          const float *pu = o->vst[u] == aff_stamp ? prev + 3 * o->vslot[u] : o->co + 3 * u;
           for (int a = 0; a < 3; a++) {
             s[a] += (int64_t)llrint((double)pu[a] * 4294967296.0);
           }
@@ -1853,7 +1875,6 @@ int orc_dyn_dab(OrcDyn *o, const OrcDab *dab, uint64_t *out_affected, int cap)
       }
     }
     free(prev);
-    free(slot);
   }
   else {
     for (int i = 0; i < ncand; i++) {
@@ -1905,7 +1926,6 @@ int orc_dyn_dab(OrcDyn *o, const OrcDab *dab, uint64_t *out_affected, int cap)
   }
   free(leaves);
   free(cand);
-  free(seen);
   free(aff);
   free(fac);
   free(newco);
@@ -2295,12 +2315,9 @@ int orc_dyn_subdivide_sequential(OrcDyn *o, const float center[3], float radius)
     const int A = ce / 3, B = cB >= 0 ? cB / 3 : -1;
     {
       int outer[4];
-      int *rm = (int *)malloc(sizeof(int) * 3 * (size_t)o->Tcap);
-      int *rs = (int *)calloc(3 * (size_t)o->Tcap, sizeof(int));
-      split_edge(o, ce, o->serial, 1000000 + nsplit, rm, rs, nf, op, outer);
-      split_relink(o, outer, rm, rs, 1000000 + nsplit);
-      free(rm);
-      free(rs);
+      const int stamp = ++o->stamp;
+      split_edge(o, ce, o->serial, 1000000 + nsplit, stamp, nf, op, outer);
+      split_relink(o, outer, stamp);
     }
     nsplit++;
     const int fs[4] = {A, nf[0], B, nf[1]};
