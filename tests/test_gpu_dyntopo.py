@@ -142,3 +142,62 @@ def test_larger_mesh_long_stroke(sg, po):
     p.close()
     m.close()
     o.close()
+
+
+def test_capacity_stop_and_node_lists(sg, po):
+    """Tiny capacities: the split phase stops at the same round on both sides (counter
+    [8] bit 1) and the mesh stays sound; the node accessors list each leaf's live faces,
+    its unique and other vertices disjoint."""
+    co, tris = sg.noisy_icosphere(3, seed=12, amplitude=0.02)
+    V, T = len(co), len(tris)
+    m = dy.DynMesh(co, tris, vert_cap=V + 40, tri_cap=T + 80)
+    p = dy.DynPBVH(m)
+    o = po.OracleDyn(co, tris, vert_cap=V + 40, tri_cap=T + 80)
+    p.detail_size_set(0.02)
+    o.detail_size_set(0.02)
+    stopped = 0
+    for i, d in enumerate(stroke(sg, 13, 6, 0.2, 0.3)):
+        p.dab(d)
+        o.dab(po.to_orc(d))
+        ps, os_ = p.last_stats(), o.last_stats()
+        assert np.array_equal(ps, os_), (i, ps.tolist(), os_.tolist())
+        stopped += int(ps[8] & 1)
+    assert stopped > 0
+    same_state(p, o, "capacity")
+    assert o.check() == 0
+    # node lists: the leaf's live faces, and unique / other vertices disjoint
+    info, _, _ = p.nodes()
+    for leaf in np.nonzero(info[:, 0] & 1)[0][:8]:
+        faces, uniq, other = p.node_lists(leaf)
+        assert len(faces) == info[leaf, 2]
+        assert len(np.intersect1d(uniq, other)) == 0
+    p.close()
+    m.close()
+    o.close()
+
+
+def test_bench_size_stroke_final_state(sg, po):
+    """The bench's dyntopo workload (icosphere-8, 1.31 M triangles, detail 0.003, 200
+    dabs): every dab's counters, then every vertex, face and node bit-identical."""
+    co, tris = sg.noisy_icosphere(8, seed=1, amplitude=0.02)
+    m, p, o = pair(sg, po, co, tris, 0.003)
+    rng = np.random.default_rng(5)
+    n = 200
+    radii = rng.uniform(0.02, 0.05, n).astype(np.float32)
+    strengths = rng.uniform(0.2, 1.0, n)
+    centres = sg.great_circle_centres(5, radii, spacing_frac=0.3)
+    tools = [sg.TOOL_DRAW, sg.TOOL_INFLATE, sg.TOOL_SMOOTH, sg.TOOL_GRAB]
+    for i in range(n):
+        nxt = centres[i + 1] if i + 1 < n else centres[i]
+        d = sg.make_dab(tool=tools[i % 4], location=centres[i], radius=float(radii[i]),
+                        strength=float(strengths[i]), smooth_iterations=2, grab_delta=(nxt - centres[i]) * 0.5)
+        p.dab(d)
+        o.dab(po.to_orc(d))
+        assert np.array_equal(p.last_stats(), o.last_stats()), i
+    same_state(p, o, "bench stroke")
+    p.after_stroke()
+    o.after_stroke()
+    same_state(p, o, "bench stroke, after_stroke")
+    p.close()
+    m.close()
+    o.close()
