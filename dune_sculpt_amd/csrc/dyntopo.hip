@@ -51,6 +51,8 @@ struct DV {
   float *node_bb, *node_obb;
   int *slab, *scount;
   int *wlist, *cand, *win, *outer, *hv_in, *hv_out, *touched, *gathered, *vlist, *aff, *nlist;
+  int *llist, *lstamp;
+  uint8_t *fin;
   unsigned long long *key;
   uint8_t *valid;
   float *fade, *buf0, *buf1;
@@ -62,6 +64,19 @@ const int TOUCH = PBVH_UpdateNormals | PBVH_UpdateBB | PBVH_UpdateDrawBuffers;
 __device__ __forceinline__ int dn(int c) { return (c % 3 == 2) ? c - 2 : c + 1; }
 __device__ __forceinline__ int dp(int c) { return (c % 3 == 0) ? c + 2 : c - 1; }
 __device__ __forceinline__ unsigned long long ull(long long x) { return (unsigned long long)x; }
+
+// Loads of words other threads update with atomics (performed at L2): atomic loads, so
+// the topology workgroup never reads a stale cached copy across its barriers.
+__device__ __forceinline__ long long aload(const long long *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long aload(const unsigned long long *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int aload(const int *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void azero(long long *p) { atomicExch((unsigned long long *)p, 0ull); }
 
 __device__ __forceinline__ float len2(const DV &g, int a, int b) {
   const float x = g.px[a] - g.px[b], y = g.py[a] - g.py[b], z = g.pz[a] - g.pz[b];
@@ -414,102 +429,6 @@ __global__ __launch_bounds__(DY_THREADS) void kd_wbuild(DV g, int wser) {
   }
 }
 
-__global__ void kd_round_begin(DV g, int first) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    g.ctl[SGD_HV] = first ? 0 : g.ctl[SGD_HV2];
-    g.ctl[SGD_HV2] = 0;
-    g.ctl[SGD_NCAND] = 0;
-    g.ctl[SGD_NVALID] = 0;
-    g.ctl[SGD_NWIN] = 0;
-    g.ctl[SGD_NTOUCH] = 0;
-  }
-}
-
-__global__ __launch_bounds__(DY_THREADS) void kd_cand(DV g, int collapse, Range R, float limit, float limit2,
-                                                       int stamp) {
-  const int wn = (int)g.ctl[SGD_WN];
-  const int hn = collapse ? 0 : (int)g.ctl[SGD_HV];
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < wn + hn; i += gridDim.x * blockDim.x) {
-    if (i < wn) {
-      const int f = g.wlist[i];
-      if (g.tnode[f] < 0) continue;
-      if (!front_ok(g, f) || !tri_in_range(g, f, R)) continue;
-      for (int k = 0; k < 3; k++) {
-        const int c = 3 * f + k;
-        const float l2 = edge_len2(g, c);
-        if (collapse) {
-          if (l2 < limit2) reg_edge(g, c, 1, stamp);
-        } else if (l2 > limit2) {
-          if (reg_even(g, c, l2, limit, stamp) < 0) atomicOr((unsigned long long *)&g.ctl[SGD_FLAGS], 2ull);
-        }
-      }
-    } else {
-      // an opposite vertex of the last round's splits with more than 8 edges: its long
-      // edges (pbvh_bmesh.c:1191-1198)
-      const int v = g.hv_in[i - wn];
-      int b0;
-      const int n0 = ring_for(g, v, b0, [](int, int) {});
-      if (n0 < 0 || n0 + b0 <= 8) continue;
-      int bnd;
-      ring_for(g, v, bnd, [&](int k, int j) {
-        const int e1 = dp(k);
-        if (edge_len2(g, e1) > limit2) reg_edge(g, e1, 0, stamp);
-        if (bnd && j == 0) {
-          const int e2 = dn(k);
-          if (edge_len2(g, e2) > limit2) reg_edge(g, e2, 0, stamp);
-        }
-      });
-    }
-  }
-}
-
-__global__ __launch_bounds__(DY_THREADS) void kd_claim(DV g, int collapse) {
-  const int n = (int)min(g.ctl[SGD_NCAND], (long long)g.ccap);
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int c = g.cand[i];
-    const int p = g.tv[dn(c)], q = g.tv[dp(c)];
-    bool ok = g.vnode[p] >= 0 && g.vnode[q] >= 0 && g.tnode[c / 3] >= 0;
-    if (ok && collapse) ok = collapse_ok(g, c);
-    if (ok && !collapse) {
-      const int cB = g.topp[c];
-      ok = g.scount[g.tnode[c / 3]] <= SG_DYN_SLAB / 2 &&
-           (cB < 0 || (g.tnode[cB / 3] >= 0 && g.scount[g.tnode[cB / 3]] <= SG_DYN_SLAB / 2));
-    }
-    g.valid[i] = ok ? 1 : 0;
-    if (!ok) continue;
-    atomicAdd((unsigned long long *)&g.ctl[SGD_NVALID], 1ull);
-    const unsigned long long k = edge_key(g, c, collapse);
-    g.key[i] = k;
-    claim_set(g, c, collapse, [&](int t) {
-      const unsigned long long old = atomicMax(&g.claim[t], k);
-      if (old == k) atomicMax(&g.tie[t], k);
-    });
-  }
-}
-
-__global__ __launch_bounds__(DY_THREADS) void kd_verify(DV g, int collapse) {
-  const int n = (int)min(g.ctl[SGD_NCAND], (long long)g.ccap);
-  if (!collapse) {
-    // capacity for the round's new vertices and faces (pbvh_bmesh_vert_create /
-    // face_create have no limit; these arrays do)
-    const long long nv = g.ctl[SGD_NVALID];
-    if (g.ctl[SGD_V] + nv > g.Vcap || g.ctl[SGD_T] + 2 * nv > g.Tcap) {
-      if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr((unsigned long long *)&g.ctl[SGD_FLAGS], 1ull);
-      return;
-    }
-  }
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    if (!g.valid[i]) continue;
-    const unsigned long long k = g.key[i];
-    bool own = true;
-    claim_set(g, g.cand[i], collapse, [&](int t) { own = own && g.claim[t] == k && g.tie[t] != k; });
-    if (own) {
-      const unsigned long long s = atomicAdd((unsigned long long *)&g.ctl[SGD_NWIN], 1ull);
-      g.win[s] = g.cand[i];
-    }
-  }
-}
-
 __device__ __forceinline__ void slab_append(const DV &g, int node, int f) {
   const int pos = atomicAdd(&g.scount[node], 1);
   g.slab[(size_t)node * SG_DYN_SLAB + pos] = f;
@@ -525,18 +444,37 @@ __device__ __forceinline__ void touch_leaf(const DV &g, int leaf) {
   if (f & PBVH_FullyHidden) atomicAnd(&g.node_flag[leaf], ~PBVH_FullyHidden);
 }
 
-__device__ __forceinline__ void hv_add(const DV &g, int v, int stamp) {
+__device__ __forceinline__ void hv_add(const DV &g, int v, int stamp, int *hv_out) {
   if (v < 0) return;
   if (atomicExch(&g.hvst[v], stamp) != stamp) {
     const unsigned long long s = atomicAdd((unsigned long long *)&g.ctl[SGD_HV2], 1ull);
-    g.hv_out[s] = v;
+    hv_out[s] = v;
   }
+}
+
+// a working-set face in range: alive, front facing, meeting the sphere / circle
+__device__ __forceinline__ bool face_in(const DV &g, int f, const Range &R) {
+  return g.tnode[f] >= 0 && front_ok(g, f) && tri_in_range(g, f, R);
+}
+
+// A face the round changed (rewritten, created, or moved by a collapse) joins the
+// working set when a split made it, is tested again, and joins the scan list if in range.
+__device__ __forceinline__ int scan_update(const DV &g, int f, const Range &R, int lser, int *out) {
+  const bool in = face_in(g, f, R);
+  g.fin[f] = in ? 1 : 0;
+  if (in && g.lstamp[f] != lser) {
+    g.lstamp[f] = lser;
+    *out = f;
+    return 1;
+  }
+  return 0;
 }
 
 // Split the edge opposite cA at a new vertex m (pbvh_bmesh.c:1093-1209): A (p, q, oA)
 // in its winding becomes A' = (p, m, oA) in its slot and NA = (m, q, oA); B likewise.
 // Outer corners keep their old neighbour ids and record where they moved (remap).
-__device__ void split_apply(const DV &g, int wi, int cA, int serial, int round, int stamp, int wser) {
+__device__ void split_apply(const DV &g, int wi, int cA, int serial, int round, int stamp, int wser, int lser,
+                            const Range &R, int *hv_out) {
   const int cB = g.topp[cA];
   const int A = cA / 3, B = cB >= 0 ? cB / 3 : -1;
   const int p = g.tv[dn(cA)], q = g.tv[dp(cA)], oA = g.tv[cA];
@@ -627,22 +565,21 @@ __device__ void split_apply(const DV &g, int wi, int cA, int serial, int round, 
       g.vcorner[v] = (v == q) ? 3 * B : (v == p) ? 3 * NB + 1 : 3 * B + 2;
   }
   g.vcorner[m] = 3 * A + 1;
-  // the working set gains this split's faces (owned here: plain stamps, one append)
+  // this split's faces (owned here) join the working set; in-range ones the scan list
   int fs[4], nf = 0;
   const int own[4] = {A, NA, B, NB};
   for (int i = 0; i < 4; i++) {
     const int f = own[i];
-    if (f >= 0 && g.wstamp[f] != wser) {
-      g.wstamp[f] = wser;
-      fs[nf++] = f;
-    }
+    if (f < 0) continue;
+    g.wstamp[f] = wser;
+    nf += scan_update(g, f, R, lser, &fs[nf]);
   }
   if (nf) {
-    const int base = (int)atomicAdd((unsigned long long *)&g.ctl[SGD_WN], (unsigned long long)nf);
-    for (int i = 0; i < nf; i++) g.wlist[base + i] = fs[i];
+    const int base = (int)atomicAdd((unsigned long long *)&g.ctl[SGD_LN], (unsigned long long)nf);
+    for (int i = 0; i < nf; i++) g.llist[base + i] = fs[i];
   }
-  hv_add(g, oA, stamp);
-  hv_add(g, oB, stamp);
+  hv_add(g, oA, stamp, hv_out);
+  hv_add(g, oB, stamp, hv_out);
 }
 
 // Collapse the edge opposite cA (pbvh_bmesh.c:1264-1429): the lower-mask vertex goes
@@ -702,59 +639,217 @@ __device__ void collapse_apply(const DV &g, int cA) {
   g.touched[3 * s + 2] = oB;
 }
 
-__global__ __launch_bounds__(DY_THREADS) void kd_apply(DV g, int collapse, int serial, int round, int stamp,
-                                                        int wser) {
-  const int n = (int)g.ctl[SGD_NWIN];
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    if (collapse)
-      collapse_apply(g, g.win[i]);
-    else
-      split_apply(g, i, g.win[i], serial, round, stamp, wser);
-  }
-}
-
-// second phase of the splits: outer edges meet their neighbours
-__global__ __launch_bounds__(DY_THREADS) void kd_relink(DV g, int stamp) {
-  const int n = (int)g.ctl[SGD_NWIN];
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int4 o = ((const int4 *)g.outer)[i];
-    const int ks[4] = {o.x, o.y, o.z, o.w};
-    for (int j = 0; j < 4; j++) {
-      const int k = ks[j];
-      if (k < 0) continue;
-      const int y = g.topp[k];
-      if (y < 0) continue;
-      if (g.remap_ser[y] == stamp)
-        g.topp[k] = g.remap[y];
-      else
-        g.topp[y] = k;
+// ---------------------------------------------------------------------------------
+// the topology update: the working set's in-range faces (a thread per face), then one
+// workgroup runs every round of both phases, its stages separated by barriers
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(DY_THREADS) void kd_wrange(DV g, Range R, int lser) {
+  const int wn = (int)g.ctl[SGD_WN];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < wn; i += gridDim.x * blockDim.x) {
+    const int f = g.wlist[i];
+    const bool in = face_in(g, f, R);
+    g.fin[f] = in ? 1 : 0;
+    if (in) {
+      g.lstamp[f] = lser;
+      const unsigned long long s = atomicAdd((unsigned long long *)&g.ctl[SGD_LN], 1ull);
+      g.llist[s] = f;
     }
   }
 }
 
-// after the collapses: a vertex whose owner leaf lost its last face with it goes to the
-// smallest-numbered leaf among its faces; around the kept vertex every leaf is marked
-__global__ __launch_bounds__(DY_THREADS) void kd_owner(DV g) {
-  const int n = 3 * (int)g.ctl[SGD_NTOUCH];
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int v = g.touched[i];
-    if (v < 0 || g.vcorner[v] < 0) continue;
-    const bool mark = (i % 3) == 0;
-    const int own = g.vnode[v];
-    bool has = false;
-    int best = -1, bnd;
-    ring_for(g, v, bnd, [&](int k, int) {
-      const int leaf = g.tnode[k / 3];
-      if (leaf < 0) return;
-      if (mark) atomicOr(&g.node_flag[leaf], TOUCH);
-      has = has || leaf == own;
-      if (best < 0 || leaf < best) best = leaf;
-    });
-    if (!has) {
-      if (own >= 0) atomicOr(&g.node_flag[own], PBVH_UpdateBB | PBVH_UpdateDrawBuffers);
-      g.vnode[v] = best;
-      if (best >= 0) atomicOr(&g.node_flag[best], PBVH_UpdateBB | PBVH_UpdateDrawBuffers);
+#define TW_THREADS 1024
+
+__global__ __launch_bounds__(TW_THREADS) void kd_topology(DV g, int mode, Range R, float min_len, float max_len,
+                                                           int serial, int stamp0, int wser, int lser) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  long long st_ops[2] = {0, 0}, st_rounds[2] = {0, 0}, st_cand[2] = {0, 0};
+  int stamp = stamp0;
+  for (int ph = 0; ph < 2; ph++) {
+    const int collapse = ph == 0;
+    if (!(mode & (collapse ? PBVH_Collapse : PBVH_Subdivide))) continue;
+    const float limit = collapse ? min_len : max_len;
+    const float limit2 = limit * limit;
+    int *hv_in = g.hv_in, *hv_out = g.hv_out;  // the two lists, swapped per round
+    if (tid == 0) {
+      azero(&g.ctl[SGD_HV]);
+      azero(&g.ctl[SGD_HV2]);
     }
+    __syncthreads();
+    for (int round = 0; round < SG_DYN_MAX_ROUNDS; round++) {
+      stamp++;
+      if (tid == 0) {
+        azero(&g.ctl[SGD_NCAND]);
+        azero(&g.ctl[SGD_NVALID]);
+        azero(&g.ctl[SGD_NWIN]);
+        azero(&g.ctl[SGD_NTOUCH]);
+      }
+      __syncthreads();
+      // candidates: the queue's edges of in-range working-set faces, and the long
+      // edges of the last round's high-valence opposite vertices (pbvh_bmesh.c:1191-1198)
+      const int ln = (int)aload(&g.ctl[SGD_LN]);
+      const int hn = collapse ? 0 : (int)aload(&g.ctl[SGD_HV]);
+      for (int i = tid; i < ln + hn; i += nt) {
+        if (i < ln) {
+          const int f = g.llist[i];
+          if (g.tnode[f] < 0 || !g.fin[f]) continue;
+          for (int k = 0; k < 3; k++) {
+            const int c = 3 * f + k;
+            const float l2 = edge_len2(g, c);
+            if (collapse) {
+              if (l2 < limit2) reg_edge(g, c, 1, stamp);
+            } else if (l2 > limit2) {
+              if (reg_even(g, c, l2, limit, stamp) < 0) atomicOr((unsigned long long *)&g.ctl[SGD_FLAGS], 2ull);
+            }
+          }
+        } else {
+          const int v = hv_in[i - ln];
+          int b0;
+          const int n0 = ring_for(g, v, b0, [](int, int) {});
+          if (n0 < 0 || n0 + b0 <= 8) continue;
+          int bnd;
+          ring_for(g, v, bnd, [&](int k, int j) {
+            const int e1 = dp(k);
+            if (edge_len2(g, e1) > limit2) reg_edge(g, e1, 0, stamp);
+            if (bnd && j == 0) {
+              const int e2 = dn(k);
+              if (edge_len2(g, e2) > limit2) reg_edge(g, e2, 0, stamp);
+            }
+          });
+        }
+      }
+      __syncthreads();
+      const long long nc = aload(&g.ctl[SGD_NCAND]);
+      st_cand[ph] += nc;
+      if (nc == 0) break;
+      if (nc > g.ccap) {
+        if (tid == 0) atomicOr((unsigned long long *)&g.ctl[SGD_FLAGS], 1ull);
+        break;
+      }
+      st_rounds[ph]++;
+      const int n = (int)nc;
+      // validity, keys, claims
+      for (int i = tid; i < n; i += nt) {
+        const int c = g.cand[i];
+        const int p = g.tv[dn(c)], q = g.tv[dp(c)];
+        bool ok = g.vnode[p] >= 0 && g.vnode[q] >= 0 && g.tnode[c / 3] >= 0;
+        if (ok && collapse) ok = collapse_ok(g, c);
+        if (ok && !collapse) {
+          const int cB = g.topp[c];
+          ok = aload(&g.scount[g.tnode[c / 3]]) <= SG_DYN_SLAB / 2 &&
+               (cB < 0 || (g.tnode[cB / 3] >= 0 && aload(&g.scount[g.tnode[cB / 3]]) <= SG_DYN_SLAB / 2));
+        }
+        g.valid[i] = ok ? 1 : 0;
+        if (!ok) continue;
+        atomicAdd((unsigned long long *)&g.ctl[SGD_NVALID], 1ull);
+        const unsigned long long kk = edge_key(g, c, collapse);
+        g.key[i] = kk;
+        claim_set(g, c, collapse, [&](int t) {
+          const unsigned long long old = atomicMax(&g.claim[t], kk);
+          if (old == kk) atomicMax(&g.tie[t], kk);
+        });
+      }
+      __syncthreads();
+      if (!collapse) {
+        // room for the round's new vertices and faces
+        const long long nv = aload(&g.ctl[SGD_NVALID]);
+        if (aload(&g.ctl[SGD_V]) + nv > g.Vcap || aload(&g.ctl[SGD_T]) + 2 * nv > g.Tcap) {
+          if (tid == 0) atomicOr((unsigned long long *)&g.ctl[SGD_FLAGS], 1ull);
+          break;
+        }
+      }
+      // winners: the unshared largest key on every owned face
+      for (int i = tid; i < n; i += nt) {
+        if (!g.valid[i]) continue;
+        const unsigned long long kk = g.key[i];
+        bool own = true;
+        claim_set(g, g.cand[i], collapse, [&](int t) { own = own && aload(&g.claim[t]) == kk && aload(&g.tie[t]) != kk; });
+        if (own) {
+          const unsigned long long s = atomicAdd((unsigned long long *)&g.ctl[SGD_NWIN], 1ull);
+          g.win[s] = g.cand[i];
+        }
+      }
+      __syncthreads();
+      const int nw = (int)aload(&g.ctl[SGD_NWIN]);
+      for (int i = tid; i < nw; i += nt) {
+        if (collapse)
+          collapse_apply(g, g.win[i]);
+        else
+          split_apply(g, i, g.win[i], serial, round, stamp, wser, lser, R, hv_out);
+      }
+      __syncthreads();
+      if (!collapse) {
+        // outer edges meet their neighbours
+        for (int i = tid; i < nw; i += nt) {
+          const int4 o = ((const int4 *)g.outer)[i];
+          const int ks[4] = {o.x, o.y, o.z, o.w};
+          for (int j = 0; j < 4; j++) {
+            const int k = ks[j];
+            if (k < 0) continue;
+            const int y = g.topp[k];
+            if (y < 0) continue;
+            if (g.remap_ser[y] == stamp)
+              g.topp[k] = g.remap[y];
+            else
+              g.topp[y] = k;
+          }
+        }
+      } else {
+        // owners after the collapses: a vertex whose owner leaf lost its last face with
+        // it goes to the smallest-numbered leaf among its faces; around the kept vertex
+        // every leaf is marked and every working-set face tested again (it moved)
+        const int nto = 3 * (int)aload(&g.ctl[SGD_NTOUCH]);
+        for (int i = tid; i < nto; i += nt) {
+          const int v = g.touched[i];
+          if (v < 0 || g.vcorner[v] < 0) continue;
+          const bool kept = (i % 3) == 0;
+          const int own = g.vnode[v];
+          bool has = false;
+          int best = -1, bnd;
+          ring_for(g, v, bnd, [&](int k, int) {
+            const int f = k / 3;
+            const int leaf = g.tnode[f];
+            if (leaf < 0) return;
+            if (kept) {
+              atomicOr(&g.node_flag[leaf], TOUCH);
+              if (g.wstamp[f] == wser) {
+                int out;
+                if (scan_update(g, f, R, lser, &out)) {
+                  const unsigned long long s = atomicAdd((unsigned long long *)&g.ctl[SGD_LN], 1ull);
+                  g.llist[s] = out;
+                }
+              }
+            }
+            has = has || leaf == own;
+            if (best < 0 || leaf < best) best = leaf;
+          });
+          if (!has) {
+            if (own >= 0) atomicOr(&g.node_flag[own], PBVH_UpdateBB | PBVH_UpdateDrawBuffers);
+            g.vnode[v] = best;
+            if (best >= 0) atomicOr(&g.node_flag[best], PBVH_UpdateBB | PBVH_UpdateDrawBuffers);
+          }
+        }
+      }
+      __syncthreads();
+      st_ops[ph] += nw;
+      if (nw == 0) break;
+      int *t = hv_in;
+      hv_in = hv_out;
+      hv_out = t;
+      if (tid == 0) {
+        atomicExch((unsigned long long *)&g.ctl[SGD_HV], (unsigned long long)aload(&g.ctl[SGD_HV2]));
+        azero(&g.ctl[SGD_HV2]);
+      }
+      __syncthreads();
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    g.ctl[SGD_ST + 0] = st_ops[1];
+    g.ctl[SGD_ST + 1] = st_ops[0];
+    g.ctl[SGD_ST + 2] = st_rounds[1];
+    g.ctl[SGD_ST + 3] = st_rounds[0];
+    g.ctl[SGD_ST + 4] = st_cand[1];
+    g.ctl[SGD_ST + 5] = st_cand[0];
   }
 }
 
@@ -1064,7 +1159,7 @@ __global__ __launch_bounds__(DY_THREADS) void kd_leaf_bounds(DV g, int stamp) {
         }
         b[3] = b[7] = 0.0f;
         for (int j = 0; j < 8; j++) g.node_bb[8 * (size_t)node + j] = b[j];
-        atomicAdd((unsigned long long *)&g.ctl[SGD_NFN + 1], 1ull);
+        atomicAdd((unsigned long long *)&g.ctl[SGD_NREFIT], 1ull);
       }
       __syncthreads();
     }
@@ -1221,6 +1316,9 @@ DV view(PBVH *p) {
   g.vlist = d.vlist;
   g.aff = d.aff;
   g.nlist = d.nlist;
+  g.llist = d.llist;
+  g.lstamp = d.lstamp;
+  g.fin = d.fin;
   g.key = d.key;
   g.valid = d.valid;
   g.fade = d.fade;
@@ -1258,11 +1356,12 @@ void sg_dyn_free_device(PBVH *p) {
                 &d.wstamp, &d.estamp, &d.remap, &d.remap_ser, &d.node_flag, &d.node_child, &d.node_dirty,
                 &d.leaf_node, &d.lvl_off, &d.lvl_node, &d.node_parent, &d.node_height, &d.anc_stamp, &d.lvl_list,
                 &d.lvl_cnt, &d.slab, &d.scount, &d.wlist, &d.cand, &d.win, &d.outer,
-                &d.hv0, &d.hv1, &d.touched, &d.gathered, &d.vlist, &d.aff, &d.nlist};
+                &d.hv0, &d.hv1, &d.touched, &d.gathered, &d.vlist, &d.aff, &d.nlist, &d.llist, &d.lstamp};
   for (int **q : il) dfree(*q);
   dfree(d.vhide);
   dfree(d.fhide);
   dfree(d.valid);
+  dfree(d.fin);
   dfree(d.lab);
   dfree(d.claim);
   dfree(d.tie);
@@ -1292,6 +1391,7 @@ int sg_dyn_upload(PBVH *p) {
     rc |= dalloc(d.fade, V) | dalloc(d.buf0, 3 * V) | dalloc(d.buf1, 3 * V);
     rc |= dalloc(d.tv, C) | dalloc(d.topp, C) | dalloc(d.estamp, C) | dalloc(d.remap, C) | dalloc(d.remap_ser, C);
     rc |= dalloc(d.tnode, T) | dalloc(d.tslot, T) | dalloc(d.wstamp, T) | dalloc(d.fnx, T) | dalloc(d.fny, T);
+    rc |= dalloc(d.llist, T) | dalloc(d.lstamp, T) | dalloc(d.fin, T);
     rc |= dalloc(d.fnz, T) | dalloc(d.fhide, T) | dalloc(d.claim, T) | dalloc(d.wlist, T);
     rc |= dalloc(d.tie, T);
     rc |= dalloc(d.cand, (size_t)d.ccap) | dalloc(d.key, (size_t)d.ccap) | dalloc(d.valid, (size_t)d.ccap);
@@ -1463,102 +1563,42 @@ int sg_dyn_sync_flags_down(PBVH *p) {
 // ---------------------------------------------------------------------------------
 // topology update (rounds)
 // ---------------------------------------------------------------------------------
-static int phase(PBVH *p, int collapse, const float center[3], float radius, int wser, long long &ops,
-                 long long &rounds, long long &ncand_total, int &modified) {
-  SgDynState *s = p->dyn;
-  SgDynDev &d = s->d;
-  hipStream_t st = p->stream;
-  Range R;
-  R.c[0] = center[0];
-  R.c[1] = center[1];
-  R.c[2] = center[2];
-  R.r2 = radius * radius;
-  DV g = view(p);
-  if (s->use_projected) {
-    const float n[3] = {s->view_normal[0], s->view_normal[1], s->view_normal[2]};
-    const float mul = center[0] * n[0] + center[1] * n[1] + center[2] * n[2];
-    for (int i = 0; i < 3; i++) R.cp[i] = center[i] + n[i] * -mul;
-  } else {
-    R.cp[0] = R.cp[1] = R.cp[2] = 0.0f;
-  }
-  const float limit = collapse ? s->min_len : s->max_len;
-  const float limit2 = limit * limit;
-  int *hv[2] = {d.hv0, d.hv1};
-  int cur = 0;
-  for (int round = 0; round < SG_DYN_MAX_ROUNDS; round++) {
-    const int stamp = ++s->stamp;
-    g.hv_in = hv[cur];
-    g.hv_out = hv[1 - cur];
-    kd_round_begin<<<1, 64, 0, st>>>(g, round == 0);
-    kd_cand<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, collapse, R, limit, limit2, stamp);
-    kd_claim<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, collapse);
-    kd_verify<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, collapse);
-    if (launch_check("dyntopo round")) return -1;
-    // apply only when the candidate list fit (an overflowing round stops the phase)
-    if (read_ctl(p)) return -1;
-    const long long *c = d.ctl_host;
-    ncand_total += c[SGD_NCAND];
-    if (c[SGD_NCAND] == 0) break;
-    if (c[SGD_NCAND] > d.ccap) {
-      s->stats[8] |= 1;
-      break;
-    }
-    rounds++;
-    if (c[SGD_FLAGS] & 1) {
-      s->stats[8] |= 1;
-      break;
-    }
-    kd_apply<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, collapse, s->serial, round, stamp, wser);
-    if (collapse)
-      kd_owner<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g);
-    else
-      kd_relink<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, stamp);
-    if (launch_check("dyntopo apply")) return -1;
-    const long long nwin = c[SGD_NWIN];
-    ops += nwin;
-    if (nwin) {
-      modified = 1;
-    } else {
-      break;
-    }
-    cur = 1 - cur;
-  }
-  if (read_ctl(p)) return -1;
-  if (d.ctl_host[SGD_FLAGS] & 2) s->stats[8] |= 2;
-  return 0;
-}
-
 int sg_dyn_update_topology(PBVH *p, int mode, const float center[3], float radius) {
   SgDynState *s = p->dyn;
   SgDynDev &d = s->d;
   hipStream_t st = p->stream;
   DV g = view(p);
-  const int wser = ++s->stamp;
+  Range R;
+  for (int i = 0; i < 3; i++) R.c[i] = center[i];
+  R.r2 = radius * radius;
+  if (s->use_projected) {
+    const float *n = s->view_normal;
+    const float mul = center[0] * n[0] + center[1] * n[1] + center[2] * n[2];
+    for (int i = 0; i < 3; i++) R.cp[i] = center[i] + n[i] * -mul;
+  } else {
+    R.cp[0] = R.cp[1] = R.cp[2] = 0.0f;
+  }
+  const int wser = ++s->stamp, lser = ++s->stamp;
+  const int stamp0 = s->stamp;
+  s->stamp += 2 * SG_DYN_MAX_ROUNDS + 2;  // the workgroup's round stamps
   zero_ctl(p, g, SGD_NCAND, SGD_COUNT);
   kd_wbuild<<<std::max(1, std::min(d.L, 4096)), DY_THREADS, 0, st>>>(g, wser);
-  if (launch_check("dyntopo working set") || read_ctl(p)) return -1;
-  s->stats[9] = d.ctl_host[SGD_NLIST];
-  int modified = 0;
-  long long ops = 0, rounds = 0, nc = 0;
-  if (mode & PBVH_Collapse) {
-    ops = rounds = nc = 0;
-    if (phase(p, 1, center, radius, wser, ops, rounds, nc, modified)) return -1;
-    s->stats[3] += ops;
-    s->stats[5] += rounds;
-    s->stats[7] += nc;
-  }
-  if (mode & PBVH_Subdivide) {
-    ops = rounds = nc = 0;
-    if (phase(p, 0, center, radius, wser, ops, rounds, nc, modified)) return -1;
-    s->stats[2] += ops;
-    s->stats[4] += rounds;
-    s->stats[6] += nc;
-  }
+  kd_wrange<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, R, lser);
+  kd_topology<<<1, TW_THREADS, 0, st>>>(g, mode, R, s->min_len, s->max_len, s->serial, stamp0, wser, lser);
   kd_clear_topology_flags<<<(d.N + 255) / 256, 256, 0, st>>>(g);
-  if (launch_check("dyntopo flags")) return -1;
+  if (launch_check("dyntopo topology") || read_ctl(p)) return -1;
+  const long long *c = d.ctl_host;
+  s->stats[9] = c[SGD_NLIST];
+  s->stats[2] += c[SGD_ST + 0];
+  s->stats[3] += c[SGD_ST + 1];
+  s->stats[4] += c[SGD_ST + 2];
+  s->stats[5] += c[SGD_ST + 3];
+  s->stats[6] += c[SGD_ST + 4];
+  s->stats[7] += c[SGD_ST + 5];
+  s->stats[8] |= c[SGD_FLAGS] & 3;
   s->serial++;
   s->host_stale = true;
-  return modified;
+  return (c[SGD_ST + 0] + c[SGD_ST + 1]) > 0 ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------------
@@ -1579,14 +1619,14 @@ int sg_dyn_update_normals_bounds(PBVH *p, bool normals, bool bounds) {
   }
   if (bounds) {
     zero_ctl(p, g, SGD_NLIST, SGD_NLIST + 1);
-    zero_ctl(p, g, SGD_NFN + 1, SGD_NFN + 2);
+    zero_ctl(p, g, SGD_NREFIT, SGD_NREFIT + 1);
     kd_flagged<<<lb, DY_THREADS, 0, st>>>(g, PBVH_UpdateBB | PBVH_UpdateOriginalBB | PBVH_UpdateRedraw);
     kd_leaf_bounds<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, ++s->stamp);
     kd_flush<<<1, 1024, 0, st>>>(g, d.H);
   }
   if (launch_check("dyntopo normals/bounds") || read_ctl(p)) return -1;
   if (normals) s->stats[10] = d.ctl_host[SGD_NFN];
-  if (bounds) s->stats[11] = d.ctl_host[SGD_NFN + 1];
+  if (bounds) s->stats[11] = d.ctl_host[SGD_NREFIT];
   s->host_stale = true;
   return 0;
 }
@@ -1627,7 +1667,7 @@ int sg_dyn_dab_apply(PBVH *p, const SgDabParams *dab) {
       if (sg_dyn_update_topology(p, s->mode, c, k.r) < 0) return -1;
     }
   }
-  zero_ctl(p, g, SGD_NGATHER, SGD_NFN + 2);
+  zero_ctl(p, g, SGD_NGATHER, SGD_NREFIT + 1);
   kd_search<<<lb, DY_THREADS, 0, st>>>(g, k.cx, k.cy, k.cz, k.r2, update);
   const int stamp = ++s->stamp;
   kd_unique<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, stamp);

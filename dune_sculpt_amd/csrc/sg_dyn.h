@@ -48,7 +48,11 @@ enum {
   SGD_NTOUCH = 16,  // vertices a collapse round touched
   SGD_NLIST = 17,   // flagged leaves
   SGD_NFN = 18,     // faces re-normalled
-  SGD_COUNT = 24,
+  SGD_NREFIT = 19,  // leaves refit
+  SGD_LN = 20,      // faces of the working set in range (the round's scan list)
+  SGD_ST = 24,      // topology counters: splits, collapses, split rounds, collapse rounds,
+                    // split candidates, collapse candidates
+  SGD_COUNT = 32,
 };
 
 struct SgDynDev {
@@ -76,6 +80,8 @@ struct SgDynDev {
   int *slab = nullptr, *scount = nullptr;
   // work lists
   int *wlist = nullptr, *cand = nullptr, *win = nullptr, *outer = nullptr, *hv0 = nullptr, *hv1 = nullptr;
+  int *llist = nullptr, *lstamp = nullptr;  // in-range faces of the working set, membership stamp
+  uint8_t *fin = nullptr;                    // per face: in range (kept exact through the rounds)
   int *touched = nullptr, *gathered = nullptr, *vlist = nullptr, *aff = nullptr, *nlist = nullptr;
   unsigned long long *key = nullptr;
   uint8_t *valid = nullptr;
