@@ -51,7 +51,7 @@ struct DV {
   float *node_bb, *node_obb;
   int *slab, *scount;
   int *wlist, *cand, *win, *outer, *hv_in, *hv_out, *touched, *gathered, *vlist, *aff, *nlist;
-  int *llist, *lstamp;
+  int *llist, *lstamp, *seed0, *seed1, *sstamp;
   uint8_t *fin;
   unsigned long long *key;
   uint8_t *valid;
@@ -133,7 +133,8 @@ __device__ __forceinline__ int ring_start(const DV &g, int v, int &bnd) {
 }
 
 // f(corner, i) for the corners around v in fan order; the count, or -1 past the bound
-// (f may then have seen part of the ring).
+// (f may then have seen part of the ring).  Two laps: rewind, then walk (the second
+// over lines the first brought into the cache).
 template <class F>
 __device__ __forceinline__ int ring_for(const DV &g, int v, int &bnd, F f) {
   const int start = ring_start(g, v, bnd);
@@ -150,6 +151,36 @@ __device__ __forceinline__ int ring_for(const DV &g, int v, int &bnd, F f) {
     if (c == start) break;
   }
   return n;
+}
+
+#define RB 16  // corners of a closed fan held in registers
+
+// One lap from vcorner[v]: true when v's fan closes within RB corners, which are then
+// b[0..n) in fan order (a closed fan starts at vcorner[v], as ring_start finds); false
+// for an open, longer or dead fan.  Unused entries hold vcorner[v].  Callers index b
+// only with unrolled (constant) indices, so it stays in registers.
+__device__ __forceinline__ bool ring_closed(const DV &g, int v, int (&b)[RB], int &n) {
+  const int c0 = g.vcorner[v];
+  n = 0;
+#pragma unroll
+  for (int j = 0; j < RB; j++) b[j] = c0;
+  if (c0 < 0) return false;
+  int c = c0, state = 0;
+#pragma unroll
+  for (int j = 0; j < RB; j++) {
+    if (state == 0) {
+      b[j] = c;
+      n = j + 1;
+      const int x = g.topp[dp(c)];
+      if (x < 0) {
+        state = 2;
+      } else {
+        c = dp(x);
+        if (c == c0) state = 1;
+      }
+    }
+  }
+  return state == 1;
 }
 
 __device__ __forceinline__ int valence(const DV &g, int v) {
@@ -332,7 +363,7 @@ __device__ int reg_even(const DV &g, int c0, float l0, float lim0, int stamp) {
 
 // Whether collapsing the edge opposite c keeps the surface manifold (link condition,
 // no inner edge between two boundaries, no ears, opposite vertices keep enough edges).
-__device__ bool collapse_ok(const DV &g, int c) {
+__device__ bool collapse_ok_walk(const DV &g, int c) {
   const int cB = g.topp[c];
   const int p = g.tv[dn(c)], q = g.tv[dp(c)], oA = g.tv[c];
   const int oB = cB >= 0 ? g.tv[cB] : -1;
@@ -367,6 +398,52 @@ __device__ bool collapse_ok(const DV &g, int c) {
   if (b3 && first >= 0) {
     const int x = g.tv[dp(first)];
     if (nbr_of_p(x)) {
+      if (x != oA && x != oB) bad = true;
+      common++;
+    }
+  }
+  if (bad || common != (cB >= 0 ? 2 : 1)) return false;
+  int bA;
+  const int nA = ring_for(g, oA, bA, [](int, int) {});
+  const int vA = nA < 0 ? SG_DYN_RING + 1 : nA + bA;
+  if (vA - 1 < (bA ? 2 : 3)) return false;
+  if (oB >= 0) {
+    int bB;
+    const int nB = ring_for(g, oB, bB, [](int, int) {});
+    const int vB = nB < 0 ? SG_DYN_RING + 1 : nB + bB;
+    if (vB - 1 < (bB ? 2 : 3)) return false;
+  }
+  return true;
+}
+
+// collapse_ok_walk's result; when both fans close within RB corners, p's neighbours are
+// held in registers and each fan is walked once.
+__device__ bool collapse_ok(const DV &g, int c) {
+  const int p = g.tv[dn(c)], q = g.tv[dp(c)];
+  if (g.vnode[p] < 0 || g.vnode[q] < 0) return false;
+  int bP[RB], bQ[RB], nP, nQ;
+  if (!ring_closed(g, p, bP, nP) || !ring_closed(g, q, bQ, nQ)) return collapse_ok_walk(g, c);
+  const int cB = g.topp[c];
+  const int oA = g.tv[c], oB = cB >= 0 ? g.tv[cB] : -1;
+  if (g.topp[dn(c)] < 0 && g.topp[dp(c)] < 0) return false;
+  if (cB >= 0 && g.topp[dn(cB)] < 0 && g.topp[dp(cB)] < 0) return false;
+  int xP[RB], xQ[RB];
+#pragma unroll
+  for (int j = 0; j < RB; j++) {
+    xP[j] = g.tv[dn(bP[j])];
+    xQ[j] = g.tv[dn(bQ[j])];
+  }
+  // common neighbours (both fans closed: no boundary neighbour)
+  int common = 0;
+  bool bad = false;
+#pragma unroll
+  for (int i = 0; i < RB; i++) {
+    if (i >= nQ) break;
+    const int x = xQ[i];
+    bool hit = false;
+#pragma unroll
+    for (int j = 0; j < RB; j++) hit = hit || (j < nP && xP[j] == x);
+    if (hit) {
       if (x != oA && x != oB) bad = true;
       common++;
     }
@@ -470,11 +547,30 @@ __device__ __forceinline__ int scan_update(const DV &g, int f, const Range &R, i
   return 0;
 }
 
+// A seed face of a phase: an edge shorter than the limit (collapse) or longer (split).
+// Positions move only where faces are rewritten or a collapse moves a vertex, so a face
+// seeds the next round iff it seeded this one untouched, or a change left it seeding.
+__device__ __forceinline__ bool face_seeds(const DV &g, int f, float limit2, int collapse) {
+  bool any = false;
+  for (int k = 0; k < 3; k++) {
+    const float l2 = edge_len2(g, 3 * f + k);
+    any = any || (collapse ? l2 < limit2 : l2 > limit2);
+  }
+  return any;
+}
+
+__device__ __forceinline__ void seed_push(const DV &g, int f, int stamp, int *next) {
+  if (atomicExch(&g.sstamp[f], stamp) != stamp) {
+    const unsigned long long s = atomicAdd((unsigned long long *)&g.ctl[SGD_NS2], 1ull);
+    next[s] = f;
+  }
+}
+
 // Split the edge opposite cA at a new vertex m (pbvh_bmesh.c:1093-1209): A (p, q, oA)
 // in its winding becomes A' = (p, m, oA) in its slot and NA = (m, q, oA); B likewise.
 // Outer corners keep their old neighbour ids and record where they moved (remap).
 __device__ void split_apply(const DV &g, int wi, int cA, int serial, int round, int stamp, int wser, int lser,
-                            const Range &R, int *hv_out) {
+                            const Range &R, int *hv_out, float limit2, int *seed_next) {
   const int cB = g.topp[cA];
   const int A = cA / 3, B = cB >= 0 ? cB / 3 : -1;
   const int p = g.tv[dn(cA)], q = g.tv[dp(cA)], oA = g.tv[cA];
@@ -573,6 +669,7 @@ __device__ void split_apply(const DV &g, int wi, int cA, int serial, int round, 
     if (f < 0) continue;
     g.wstamp[f] = wser;
     nf += scan_update(g, f, R, lser, &fs[nf]);
+    if (g.fin[f] && face_seeds(g, f, limit2, 0)) seed_push(g, f, stamp, seed_next);
   }
   if (nf) {
     const int base = (int)atomicAdd((unsigned long long *)&g.ctl[SGD_LN], (unsigned long long)nf);
@@ -659,9 +756,16 @@ __global__ __launch_bounds__(DY_THREADS) void kd_wrange(DV g, Range R, int lser)
 
 #define TW_THREADS 1024
 
-__global__ __launch_bounds__(TW_THREADS) void kd_topology(DV g, int mode, Range R, float min_len, float max_len,
+__global__ __launch_bounds__(TW_THREADS) void kd_topology(DV g0, int mode, Range R, float min_len, float max_len,
                                                            int serial, int stamp0, int wser, int lser) {
   const int tid = threadIdx.x, nt = blockDim.x;
+  // the counters live in LDS while the workgroup runs (no other wave touches them):
+  // its many same-word atomics stay out of the L2
+  __shared__ long long sctl[SGD_COUNT];
+  for (int i = tid; i < SGD_COUNT; i += nt) sctl[i] = g0.ctl[i];
+  __syncthreads();
+  DV g = g0;
+  g.ctl = sctl;
   long long st_ops[2] = {0, 0}, st_rounds[2] = {0, 0}, st_cand[2] = {0, 0};
   int stamp = stamp0;
   for (int ph = 0; ph < 2; ph++) {
@@ -670,9 +774,12 @@ __global__ __launch_bounds__(TW_THREADS) void kd_topology(DV g, int mode, Range 
     const float limit = collapse ? min_len : max_len;
     const float limit2 = limit * limit;
     int *hv_in = g.hv_in, *hv_out = g.hv_out;  // the two lists, swapped per round
+    int *seed_cur = g.seed0, *seed_next = g.seed1;
     if (tid == 0) {
       azero(&g.ctl[SGD_HV]);
       azero(&g.ctl[SGD_HV2]);
+      azero(&g.ctl[SGD_NS]);
+      azero(&g.ctl[SGD_NS2]);
     }
     __syncthreads();
     for (int round = 0; round < SG_DYN_MAX_ROUNDS; round++) {
@@ -685,22 +792,31 @@ __global__ __launch_bounds__(TW_THREADS) void kd_topology(DV g, int mode, Range 
       }
       __syncthreads();
       // candidates: the queue's edges of in-range working-set faces, and the long
-      // edges of the last round's high-valence opposite vertices (pbvh_bmesh.c:1191-1198)
-      const int ln = (int)aload(&g.ctl[SGD_LN]);
+      // edges of the last round's high-valence opposite vertices (pbvh_bmesh.c:1191-1198).
+      // The first round of a phase scans every in-range face; later rounds only the
+      // seed faces (with an edge past the limit) the previous round kept or changed.
+      const int ln = round == 0 ? (int)aload(&g.ctl[SGD_LN]) : (int)aload(&g.ctl[SGD_NS]);
+      const int *scan = round == 0 ? g.llist : seed_cur;
       const int hn = collapse ? 0 : (int)aload(&g.ctl[SGD_HV]);
       for (int i = tid; i < ln + hn; i += nt) {
         if (i < ln) {
-          const int f = g.llist[i];
+          const int f = scan[i];
           if (g.tnode[f] < 0 || !g.fin[f]) continue;
+          bool seeds = false;
           for (int k = 0; k < 3; k++) {
             const int c = 3 * f + k;
             const float l2 = edge_len2(g, c);
             if (collapse) {
-              if (l2 < limit2) reg_edge(g, c, 1, stamp);
+              if (l2 < limit2) {
+                seeds = true;
+                reg_edge(g, c, 1, stamp);
+              }
             } else if (l2 > limit2) {
+              seeds = true;
               if (reg_even(g, c, l2, limit, stamp) < 0) atomicOr((unsigned long long *)&g.ctl[SGD_FLAGS], 2ull);
             }
           }
+          if (seeds) seed_push(g, f, stamp, seed_next);
         } else {
           const int v = hv_in[i - ln];
           int b0;
@@ -774,7 +890,7 @@ __global__ __launch_bounds__(TW_THREADS) void kd_topology(DV g, int mode, Range 
         if (collapse)
           collapse_apply(g, g.win[i]);
         else
-          split_apply(g, i, g.win[i], serial, round, stamp, wser, lser, R, hv_out);
+          split_apply(g, i, g.win[i], serial, round, stamp, wser, lser, R, hv_out, limit2, seed_next);
       }
       __syncthreads();
       if (!collapse) {
@@ -817,6 +933,7 @@ __global__ __launch_bounds__(TW_THREADS) void kd_topology(DV g, int mode, Range 
                   const unsigned long long s = atomicAdd((unsigned long long *)&g.ctl[SGD_LN], 1ull);
                   g.llist[s] = out;
                 }
+                if (g.fin[f] && face_seeds(g, f, limit2, 1)) seed_push(g, f, stamp, seed_next);
               }
             }
             has = has || leaf == own;
@@ -835,22 +952,29 @@ __global__ __launch_bounds__(TW_THREADS) void kd_topology(DV g, int mode, Range 
       int *t = hv_in;
       hv_in = hv_out;
       hv_out = t;
+      t = seed_cur;
+      seed_cur = seed_next;
+      seed_next = t;
       if (tid == 0) {
         atomicExch((unsigned long long *)&g.ctl[SGD_HV], (unsigned long long)aload(&g.ctl[SGD_HV2]));
         azero(&g.ctl[SGD_HV2]);
+        atomicExch((unsigned long long *)&g.ctl[SGD_NS], (unsigned long long)aload(&g.ctl[SGD_NS2]));
+        azero(&g.ctl[SGD_NS2]);
       }
       __syncthreads();
     }
     __syncthreads();
   }
   if (tid == 0) {
-    g.ctl[SGD_ST + 0] = st_ops[1];
-    g.ctl[SGD_ST + 1] = st_ops[0];
-    g.ctl[SGD_ST + 2] = st_rounds[1];
-    g.ctl[SGD_ST + 3] = st_rounds[0];
-    g.ctl[SGD_ST + 4] = st_cand[1];
-    g.ctl[SGD_ST + 5] = st_cand[0];
+    sctl[SGD_ST + 0] = st_ops[1];
+    sctl[SGD_ST + 1] = st_ops[0];
+    sctl[SGD_ST + 2] = st_rounds[1];
+    sctl[SGD_ST + 3] = st_rounds[0];
+    sctl[SGD_ST + 4] = st_cand[1];
+    sctl[SGD_ST + 5] = st_cand[0];
   }
+  __syncthreads();
+  for (int i = tid; i < SGD_COUNT; i += nt) g0.ctl[i] = sctl[i];
 }
 
 __global__ void kd_clear_topology_flags(DV g) {
@@ -1318,6 +1442,9 @@ DV view(PBVH *p) {
   g.nlist = d.nlist;
   g.llist = d.llist;
   g.lstamp = d.lstamp;
+  g.seed0 = d.seed0;
+  g.seed1 = d.seed1;
+  g.sstamp = d.sstamp;
   g.fin = d.fin;
   g.key = d.key;
   g.valid = d.valid;
@@ -1356,7 +1483,8 @@ void sg_dyn_free_device(PBVH *p) {
                 &d.wstamp, &d.estamp, &d.remap, &d.remap_ser, &d.node_flag, &d.node_child, &d.node_dirty,
                 &d.leaf_node, &d.lvl_off, &d.lvl_node, &d.node_parent, &d.node_height, &d.anc_stamp, &d.lvl_list,
                 &d.lvl_cnt, &d.slab, &d.scount, &d.wlist, &d.cand, &d.win, &d.outer,
-                &d.hv0, &d.hv1, &d.touched, &d.gathered, &d.vlist, &d.aff, &d.nlist, &d.llist, &d.lstamp};
+                &d.hv0, &d.hv1, &d.touched, &d.gathered, &d.vlist, &d.aff, &d.nlist, &d.llist, &d.lstamp,
+                &d.seed0, &d.seed1, &d.sstamp};
   for (int **q : il) dfree(*q);
   dfree(d.vhide);
   dfree(d.fhide);
@@ -1392,6 +1520,7 @@ int sg_dyn_upload(PBVH *p) {
     rc |= dalloc(d.tv, C) | dalloc(d.topp, C) | dalloc(d.estamp, C) | dalloc(d.remap, C) | dalloc(d.remap_ser, C);
     rc |= dalloc(d.tnode, T) | dalloc(d.tslot, T) | dalloc(d.wstamp, T) | dalloc(d.fnx, T) | dalloc(d.fny, T);
     rc |= dalloc(d.llist, T) | dalloc(d.lstamp, T) | dalloc(d.fin, T);
+    rc |= dalloc(d.seed0, T) | dalloc(d.seed1, T) | dalloc(d.sstamp, T);
     rc |= dalloc(d.fnz, T) | dalloc(d.fhide, T) | dalloc(d.claim, T) | dalloc(d.wlist, T);
     rc |= dalloc(d.tie, T);
     rc |= dalloc(d.cand, (size_t)d.ccap) | dalloc(d.key, (size_t)d.ccap) | dalloc(d.valid, (size_t)d.ccap);

@@ -50,6 +50,8 @@ enum {
   SGD_NFN = 18,     // faces re-normalled
   SGD_NREFIT = 19,  // leaves refit
   SGD_LN = 20,      // faces of the working set in range (the round's scan list)
+  SGD_NS = 21,      // seed faces this round (in range, with an edge past the phase's limit)
+  SGD_NS2 = 22,     // ... for the next round
   SGD_ST = 24,      // topology counters: splits, collapses, split rounds, collapse rounds,
                     // split candidates, collapse candidates
   SGD_COUNT = 32,
@@ -81,6 +83,7 @@ struct SgDynDev {
   // work lists
   int *wlist = nullptr, *cand = nullptr, *win = nullptr, *outer = nullptr, *hv0 = nullptr, *hv1 = nullptr;
   int *llist = nullptr, *lstamp = nullptr;  // in-range faces of the working set, membership stamp
+  int *seed0 = nullptr, *seed1 = nullptr, *sstamp = nullptr;  // seed faces of a round, the next round's, stamp
   uint8_t *fin = nullptr;                    // per face: in range (kept exact through the rounds)
   int *touched = nullptr, *gathered = nullptr, *vlist = nullptr, *aff = nullptr, *nlist = nullptr;
   unsigned long long *key = nullptr;
