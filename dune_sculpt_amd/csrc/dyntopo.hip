@@ -252,7 +252,25 @@ struct Range {
   float c[3], cp[3], r2;
 };
 
-// edge_queue_tri_in_sphere / _in_circle (pbvh_bmesh.c:773-805)
+// edge_queue_tri_in_sphere / _in_circle (pbvh_bmesh.c:773-805), the corners' positions
+// in face order
+__device__ bool tri_in_range_v(const DV &g, const float v0[3], const float v1[3], const float v2[3],
+                               const Range &R) {
+  float c[3];
+  if (g.use_projected) {
+    float p[3][3];
+    project_plane(g, p[0], v0);
+    project_plane(g, p[1], v1);
+    project_plane(g, p[2], v2);
+    closest_on_tri(c, R.cp, p[0], p[1], p[2]);
+    const float x = R.cp[0] - c[0], y = R.cp[1] - c[1], z = R.cp[2] - c[2];
+    return x * x + y * y + z * z <= R.r2;
+  }
+  closest_on_tri(c, R.c, v0, v1, v2);
+  const float x = R.c[0] - c[0], y = R.c[1] - c[1], z = R.c[2] - c[2];
+  return x * x + y * y + z * z <= R.r2;
+}
+
 __device__ bool tri_in_range(const DV &g, int f, const Range &R) {
   float v[3][3];
   for (int k = 0; k < 3; k++) {
@@ -261,22 +279,19 @@ __device__ bool tri_in_range(const DV &g, int f, const Range &R) {
     v[k][1] = g.py[u];
     v[k][2] = g.pz[u];
   }
-  float c[3];
-  if (g.use_projected) {
-    float p[3][3];
-    for (int k = 0; k < 3; k++) project_plane(g, p[k], v[k]);
-    closest_on_tri(c, R.cp, p[0], p[1], p[2]);
-    const float x = R.cp[0] - c[0], y = R.cp[1] - c[1], z = R.cp[2] - c[2];
-    return x * x + y * y + z * z <= R.r2;
-  }
-  closest_on_tri(c, R.c, v[0], v[1], v[2]);
-  const float x = R.c[0] - c[0], y = R.c[1] - c[1], z = R.c[2] - c[2];
-  return x * x + y * y + z * z <= R.r2;
+  return tri_in_range_v(g, v[0], v[1], v[2], R);
 }
 
-__device__ __forceinline__ bool front_ok(const DV &g, int f) {
+__device__ __forceinline__ bool front_ok_v(const DV &g, float x, float y, float z) {
   if (!g.use_frontface) return true;
-  return !(g.fnx[f] * g.vnx + g.fny[f] * g.vny + g.fnz[f] * g.vnz < 0.0f);
+  return !(x * g.vnx + y * g.vny + z * g.vnz < 0.0f);
+}
+
+__device__ __forceinline__ bool front_ok(const DV &g, int f) { return front_ok_v(g, g.fnx[f], g.fny[f], g.fnz[f]); }
+
+__device__ __forceinline__ float len2_v(const float a[3], const float b[3]) {
+  const float x = a[0] - b[0], y = a[1] - b[1], z = a[2] - b[2];
+  return x * x + y * y + z * z;
 }
 
 // edge_queue_insert's filter (pbvh_bmesh.c:812-833)
@@ -312,10 +327,29 @@ __device__ bool claim_set(const DV &g, int c, int collapse, F f) {
   return n >= 0;
 }
 
-__device__ void reg_edge(const DV &g, int c, int collapse, int stamp) {
+// The round's registered edges: an LDS set of canonical corners (open addressing, a
+// bounded probe); a key whose probe finds every slot taken by other keys uses the global
+// stamp instead, and does so on every attempt (slots only fill during a round), so each
+// edge registers exactly once.
+#define ESET_BITS 13
+#define ESET_N (1 << ESET_BITS)
+#define ESET_PROBES 8
+#define ESET_EMPTY 0xFFFFFFFFu
+
+__device__ __forceinline__ bool edge_first(const DV &g, unsigned *eset, int k, int stamp) {
+  const unsigned h = ((unsigned)k * 0x9E3779B1u) >> (32 - ESET_BITS);
+  for (int i = 0; i < ESET_PROBES; i++) {
+    const unsigned old = atomicCAS(&eset[(h + i) & (ESET_N - 1)], ESET_EMPTY, (unsigned)k);
+    if (old == ESET_EMPTY) return true;
+    if (old == (unsigned)k) return false;
+  }
+  return atomicExch(&g.estamp[k], stamp) != stamp;
+}
+
+__device__ void reg_edge(const DV &g, unsigned *eset, int c, int collapse, int stamp) {
   if (!edge_insert_ok(g, c)) return;
   const int k = canon(g, c);
-  if (atomicExch(&g.estamp[k], stamp) == stamp) return;
+  if (!edge_first(g, eset, k, stamp)) return;
   const unsigned long long slot = atomicAdd((unsigned long long *)&g.ctl[SGD_NCAND], 1ull);
   if (slot < (unsigned long long)g.ccap) g.cand[slot] = k;
   claim_set(g, k, collapse, [&](int t) {
@@ -325,7 +359,7 @@ __device__ void reg_edge(const DV &g, int c, int collapse, int stamp) {
 }
 
 // long_edge_queue_edge_add_recursive (pbvh_bmesh.c:849-903), iterative
-__device__ int reg_even(const DV &g, int c0, float l0, float lim0, int stamp) {
+__device__ int reg_even(const DV &g, unsigned *eset, int c0, float l0, float lim0, int stamp) {
   int sc[SG_DYN_STACK];
   float sl[SG_DYN_STACK], sm[SG_DYN_STACK];
   int top = 0;
@@ -341,18 +375,19 @@ __device__ int reg_even(const DV &g, int c0, float l0, float lim0, int stamp) {
     const int gc = g.topp[c];
     const int entered = gc >= 0 ? gc / 3 : c / 3;
     if (!front_ok(g, entered)) continue;
-    reg_edge(g, c, 0, stamp);
+    // the entered face's other edges, read before the registration's stores
+    const int ks[2] = {gc >= 0 ? dn(gc) : 0, gc >= 0 ? dp(gc) : 0};
+    const float lo[2] = {gc >= 0 ? edge_len2(g, ks[0]) : 0.0f, gc >= 0 ? edge_len2(g, ks[1]) : 0.0f};
+    reg_edge(g, eset, c, 0, stamp);
     if (gc < 0) continue;
     const float cmp = l2 * 1.2f;
     limit = limit * 1.6f;
     const float limit2 = limit * limit;
-    const int ks[2] = {dn(gc), dp(gc)};
     for (int i = 0; i < 2; i++) {
-      const float lo = edge_len2(g, ks[i]);
-      if (lo > sg_max(cmp, limit2)) {
+      if (lo[i] > sg_max(cmp, limit2)) {
         if (top >= SG_DYN_STACK) return -1;
         sc[top] = ks[i];
-        sl[top] = lo;
+        sl[top] = lo[i];
         sm[top] = limit;
         top++;
       }
@@ -569,6 +604,8 @@ __device__ __forceinline__ void seed_push(const DV &g, int f, int stamp, int *ne
 // Split the edge opposite cA at a new vertex m (pbvh_bmesh.c:1093-1209): A (p, q, oA)
 // in its winding becomes A' = (p, m, oA) in its slot and NA = (m, q, oA); B likewise.
 // Outer corners keep their old neighbour ids and record where they moved (remap).
+// Everything is read before the first store (a store may alias any array), and the four
+// faces' range and seed tests run on the positions held in registers.
 __device__ void split_apply(const DV &g, int wi, int cA, int serial, int round, int stamp, int wser, int lser,
                             const Range &R, int *hv_out, float limit2, int *seed_next) {
   const int cB = g.topp[cA];
@@ -578,22 +615,59 @@ __device__ void split_apply(const DV &g, int wi, int cA, int serial, int round, 
   const int yA = g.topp[dp(cA)], xA = g.topp[dn(cA)];
   const int yB = cB >= 0 ? g.topp[dp(cB)] : -1, xB = cB >= 0 ? g.topp[dn(cB)] : -1;
   const int oldAy = dp(cA), oldAx = dn(cA), oldBy = cB >= 0 ? dp(cB) : -1, oldBx = cB >= 0 ? dn(cB) : -1;
+  const float P[3] = {g.px[p], g.py[p], g.pz[p]}, Q[3] = {g.px[q], g.py[q], g.pz[q]};
+  const float OA[3] = {g.px[oA], g.py[oA], g.pz[oA]};
+  float OB[3] = {0.0f, 0.0f, 0.0f};
+  if (oB >= 0) {
+    OB[0] = g.px[oB];
+    OB[1] = g.py[oB];
+    OB[2] = g.pz[oB];
+  }
+  const float npx = g.nx[p], npy = g.ny[p], npz = g.nz[p], nqx = g.nx[q], nqy = g.ny[q], nqz = g.nz[q];
+  const float mp = g.has_mask ? g.mask[p] : 0.0f, mq = g.has_mask ? g.mask[q] : 0.0f;
+  const unsigned long long lp = g.lab[p], lq = g.lab[q], loA = g.lab[oA], loB = oB >= 0 ? g.lab[oB] : 0ull;
+  const int leafA = g.tnode[A], leafB = B >= 0 ? g.tnode[B] : -1;
+  const float fAx = g.fnx[A], fAy = g.fny[A], fAz = g.fnz[A];
+  const float fBx = B >= 0 ? g.fnx[B] : 0.0f, fBy = B >= 0 ? g.fny[B] : 0.0f, fBz = B >= 0 ? g.fnz[B] : 0.0f;
+  const uint8_t hA = g.fhide[A], hB = B >= 0 ? g.fhide[B] : 0;
+  const int vs[4] = {p, q, oA, oB};
+  int vc[4];
+  for (int i = 0; i < 4; i++) vc[i] = vs[i] >= 0 ? g.vcorner[vs[i]] : -1;
   const int m = (int)atomicAdd((unsigned long long *)&g.ctl[SGD_V], 1ull);
   const int NA = (int)atomicAdd((unsigned long long *)&g.ctl[SGD_T], B >= 0 ? 2ull : 1ull);
   const int NB = B >= 0 ? NA + 1 : -1;
-  g.px[m] = 0.5f * (g.px[p] + g.px[q]);
-  g.py[m] = 0.5f * (g.py[p] + g.py[q]);
-  g.pz[m] = 0.5f * (g.pz[p] + g.pz[q]);
-  float nx = 0.5f * (g.nx[p] + g.nx[q]), ny = 0.5f * (g.ny[p] + g.ny[q]), nz = 0.5f * (g.nz[p] + g.nz[q]);
+  const int own[4] = {A, NA, B, NB};
+  int ls[4];
+  for (int i = 0; i < 4; i++) ls[i] = own[i] >= 0 ? g.lstamp[own[i]] : lser;
+  // the new vertex
+  const float M[3] = {0.5f * (P[0] + Q[0]), 0.5f * (P[1] + Q[1]), 0.5f * (P[2] + Q[2])};
+  float nx = 0.5f * (npx + nqx), ny = 0.5f * (npy + nqy), nz = 0.5f * (npz + nqz);
   sg_normalize(nx, ny, nz);
+  // the four faces: in range (alive, front facing, meeting the sphere / circle) and seeds
+  bool in[4], sd[4];
+  {
+    const bool frA = front_ok_v(g, fAx, fAy, fAz), frB = front_ok_v(g, fBx, fBy, fBz);
+    in[0] = frA && tri_in_range_v(g, P, M, OA, R);
+    in[1] = frA && tri_in_range_v(g, M, Q, OA, R);
+    in[2] = B >= 0 && frB && tri_in_range_v(g, Q, M, OB, R);
+    in[3] = B >= 0 && frB && tri_in_range_v(g, M, P, OB, R);
+    const float lpm = len2_v(P, M), lmq = len2_v(M, Q), lmA = len2_v(M, OA);
+    sd[0] = lpm > limit2 || lmA > limit2 || len2_v(OA, P) > limit2;
+    sd[1] = lmq > limit2 || len2_v(Q, OA) > limit2 || lmA > limit2;
+    const float lmB = len2_v(M, OB);
+    sd[2] = lmq > limit2 || lmB > limit2 || len2_v(OB, Q) > limit2;
+    sd[3] = lpm > limit2 || len2_v(P, OB) > limit2 || lmB > limit2;
+  }
+  g.px[m] = M[0];
+  g.py[m] = M[1];
+  g.pz[m] = M[2];
   g.nx[m] = nx;
   g.ny[m] = ny;
   g.nz[m] = nz;
-  if (g.has_mask) g.mask[m] = 0.5f * (g.mask[p] + g.mask[q]);
+  if (g.has_mask) g.mask[m] = 0.5f * (mp + mq);
   ((uint8_t *)g.vhide)[m] = 0;
-  g.lab[m] = new_label(g.lab[p], g.lab[q], serial, round);
-  const int leafA = g.tnode[A], leafB = B >= 0 ? g.tnode[B] : -1;
-  g.vnode[m] = (B >= 0 && g.lab[oB] < g.lab[oA]) ? leafB : leafA;
+  g.lab[m] = new_label(lp, lq, serial, round);
+  g.vnode[m] = (B >= 0 && loB < loA) ? leafB : leafA;
   int *tv = g.tv, *tp = g.topp;
   tv[3 * A] = p;
   tv[3 * A + 1] = m;
@@ -636,40 +710,43 @@ __device__ void split_apply(const DV &g, int wi, int cA, int serial, int round, 
     tp[3 * NA + 2] = -1;
   }
   ((int4 *)g.outer)[wi] = out;
-  g.fnx[NA] = g.fnx[A];
-  g.fny[NA] = g.fny[A];
-  g.fnz[NA] = g.fnz[A];
-  g.fhide[NA] = g.fhide[A];
+  g.fnx[NA] = fAx;
+  g.fny[NA] = fAy;
+  g.fnz[NA] = fAz;
+  g.fhide[NA] = hA;
   slab_append(g, leafA, NA);
   touch_leaf(g, leafA);
   if (B >= 0) {
-    g.fnx[NB] = g.fnx[B];
-    g.fny[NB] = g.fny[B];
-    g.fnz[NB] = g.fnz[B];
-    g.fhide[NB] = g.fhide[B];
+    g.fnx[NB] = fBx;
+    g.fny[NB] = fBy;
+    g.fnz[NB] = fBz;
+    g.fhide[NB] = hB;
     slab_append(g, leafB, NB);
     touch_leaf(g, leafB);
   }
-  const int vs[4] = {p, q, oA, oB};
   for (int i = 0; i < 4; i++) {
     const int v = vs[i];
     if (v < 0) continue;
-    const int f = g.vcorner[v] / 3;
+    const int f = vc[i] / 3;
     if (f == A)
       g.vcorner[v] = (v == p) ? 3 * A : (v == q) ? 3 * NA + 1 : 3 * A + 2;
     else if (B >= 0 && f == B)
       g.vcorner[v] = (v == q) ? 3 * B : (v == p) ? 3 * NB + 1 : 3 * B + 2;
   }
   g.vcorner[m] = 3 * A + 1;
-  // this split's faces (owned here) join the working set; in-range ones the scan list
+  // this split's faces (owned here) join the working set; in-range ones the scan list,
+  // seeding ones the next round's seeds
   int fs[4], nf = 0;
-  const int own[4] = {A, NA, B, NB};
   for (int i = 0; i < 4; i++) {
     const int f = own[i];
     if (f < 0) continue;
     g.wstamp[f] = wser;
-    nf += scan_update(g, f, R, lser, &fs[nf]);
-    if (g.fin[f] && face_seeds(g, f, limit2, 0)) seed_push(g, f, stamp, seed_next);
+    g.fin[f] = in[i] ? 1 : 0;
+    if (in[i] && ls[i] != lser) {
+      g.lstamp[f] = lser;
+      fs[nf++] = f;
+    }
+    if (in[i] && sd[i]) seed_push(g, f, stamp, seed_next);
   }
   if (nf) {
     const int base = (int)atomicAdd((unsigned long long *)&g.ctl[SGD_LN], (unsigned long long)nf);
@@ -762,6 +839,7 @@ __global__ __launch_bounds__(TW_THREADS) void kd_topology(DV g0, int mode, Range
   // the counters live in LDS while the workgroup runs (no other wave touches them):
   // its many same-word atomics stay out of the L2
   __shared__ long long sctl[SGD_COUNT];
+  __shared__ unsigned eset[ESET_N];
   for (int i = tid; i < SGD_COUNT; i += nt) sctl[i] = g0.ctl[i];
   __syncthreads();
   DV g = g0;
@@ -784,6 +862,7 @@ __global__ __launch_bounds__(TW_THREADS) void kd_topology(DV g0, int mode, Range
     __syncthreads();
     for (int round = 0; round < SG_DYN_MAX_ROUNDS; round++) {
       stamp++;
+      for (int i = tid; i < ESET_N; i += nt) eset[i] = ESET_EMPTY;
       if (tid == 0) {
         azero(&g.ctl[SGD_NCAND]);
         azero(&g.ctl[SGD_NVALID]);
@@ -803,17 +882,19 @@ __global__ __launch_bounds__(TW_THREADS) void kd_topology(DV g0, int mode, Range
           const int f = scan[i];
           if (g.tnode[f] < 0 || !g.fin[f]) continue;
           bool seeds = false;
+          float l2s[3];
+          for (int k = 0; k < 3; k++) l2s[k] = edge_len2(g, 3 * f + k);  // before any registration's stores
           for (int k = 0; k < 3; k++) {
             const int c = 3 * f + k;
-            const float l2 = edge_len2(g, c);
+            const float l2 = l2s[k];
             if (collapse) {
               if (l2 < limit2) {
                 seeds = true;
-                reg_edge(g, c, 1, stamp);
+                reg_edge(g, eset, c, 1, stamp);
               }
             } else if (l2 > limit2) {
               seeds = true;
-              if (reg_even(g, c, l2, limit, stamp) < 0) atomicOr((unsigned long long *)&g.ctl[SGD_FLAGS], 2ull);
+              if (reg_even(g, eset, c, l2, limit, stamp) < 0) atomicOr((unsigned long long *)&g.ctl[SGD_FLAGS], 2ull);
             }
           }
           if (seeds) seed_push(g, f, stamp, seed_next);
@@ -825,10 +906,10 @@ __global__ __launch_bounds__(TW_THREADS) void kd_topology(DV g0, int mode, Range
           int bnd;
           ring_for(g, v, bnd, [&](int k, int j) {
             const int e1 = dp(k);
-            if (edge_len2(g, e1) > limit2) reg_edge(g, e1, 0, stamp);
+            if (edge_len2(g, e1) > limit2) reg_edge(g, eset, e1, 0, stamp);
             if (bnd && j == 0) {
               const int e2 = dn(k);
-              if (edge_len2(g, e2) > limit2) reg_edge(g, e2, 0, stamp);
+              if (edge_len2(g, e2) > limit2) reg_edge(g, eset, e2, 0, stamp);
             }
           });
         }
