@@ -523,7 +523,8 @@ __global__ __launch_bounds__(DY_THREADS) void kd_search(DV g, float cx, float cy
 }
 
 // working set: the live faces of leaves flagged UpdateTopology (not fully hidden)
-__global__ __launch_bounds__(DY_THREADS) void kd_wbuild(DV g, int wser) {
+__global__ __launch_bounds__(DY_THREADS) void kd_wbuild(DV g, int wser, int gate) {
+  if (gate && g.ctl[SGD_NGATHER] == 0) return;
   for (int l = blockIdx.x; l < g.L; l += gridDim.x) {
     const int node = g.leaf_node[l];
     const int fl = g.node_flag[node];
@@ -817,7 +818,8 @@ __device__ void collapse_apply(const DV &g, int cA) {
 // the topology update: the working set's in-range faces (a thread per face), then one
 // workgroup runs every round of both phases, its stages separated by barriers
 // ---------------------------------------------------------------------------------
-__global__ __launch_bounds__(DY_THREADS) void kd_wrange(DV g, Range R, int lser) {
+__global__ __launch_bounds__(DY_THREADS) void kd_wrange(DV g, Range R, int lser, int gate) {
+  if (gate && g.ctl[SGD_NGATHER] == 0) return;
   const int wn = (int)g.ctl[SGD_WN];
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < wn; i += gridDim.x * blockDim.x) {
     const int f = g.wlist[i];
@@ -834,8 +836,10 @@ __global__ __launch_bounds__(DY_THREADS) void kd_wrange(DV g, Range R, int lser)
 #define TW_THREADS 1024
 
 __global__ __launch_bounds__(TW_THREADS) void kd_topology(DV g0, int mode, Range R, float min_len, float max_len,
-                                                           int serial, int stamp0, int wser, int lser) {
+                                                           int gate, int stamp0, int wser, int lser) {
   const int tid = threadIdx.x, nt = blockDim.x;
+  // a dab's update runs only when its range search gathered a leaf
+  if (gate && g0.ctl[SGD_NGATHER] == 0) return;
   // the counters live in LDS while the workgroup runs (no other wave touches them):
   // its many same-word atomics stay out of the L2
   __shared__ long long sctl[SGD_COUNT];
@@ -844,6 +848,7 @@ __global__ __launch_bounds__(TW_THREADS) void kd_topology(DV g0, int mode, Range
   __syncthreads();
   DV g = g0;
   g.ctl = sctl;
+  const int serial = (int)sctl[SGD_SERIAL];
   long long st_ops[2] = {0, 0}, st_rounds[2] = {0, 0}, st_cand[2] = {0, 0};
   int stamp = stamp0;
   for (int ph = 0; ph < 2; ph++) {
@@ -1053,12 +1058,15 @@ __global__ __launch_bounds__(TW_THREADS) void kd_topology(DV g0, int mode, Range
     sctl[SGD_ST + 3] = st_rounds[0];
     sctl[SGD_ST + 4] = st_cand[1];
     sctl[SGD_ST + 5] = st_cand[0];
+    sctl[SGD_ST + 6] = sctl[SGD_NLIST];
+    sctl[SGD_SERIAL] = serial + 1;
   }
   __syncthreads();
   for (int i = tid; i < SGD_COUNT; i += nt) g0.ctl[i] = sctl[i];
 }
 
-__global__ void kd_clear_topology_flags(DV g) {
+__global__ void kd_clear_topology_flags(DV g, int gate) {
+  if (gate && g.ctl[SGD_NGATHER] == 0) return;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < g.N; i += gridDim.x * blockDim.x)
     g.node_flag[i] &= ~PBVH_UpdateTopology;
 }
@@ -1428,9 +1436,9 @@ __global__ __launch_bounds__(1024) void kd_flush(DV g, int H) {
   for (int h = threadIdx.x; h <= H; h += blockDim.x) g.lvl_cnt[h] = 0;
 }
 
-__global__ void kd_ctl_zero(DV g, int b, int e) {
-  const int i = b + threadIdx.x;
-  if (i < e) g.ctl[i] = 0;
+__global__ void kd_ctl_zero(DV g, unsigned mask) {
+  const int i = threadIdx.x;
+  if (i < SGD_COUNT && ((mask >> i) & 1u)) g.ctl[i] = 0;
 }
 
 // ---------------------------------------------------------------------------------
@@ -1545,7 +1553,13 @@ int read_ctl(PBVH *p) {
   return sg_check_hip(hipStreamSynchronize(p->stream), "sync");
 }
 
-void zero_ctl(PBVH *p, const DV &g, int b, int e) { kd_ctl_zero<<<1, 64, 0, p->stream>>>(g, b, e); }
+// slots [b, e) as a mask for zero_ctl
+unsigned slots(int b, int e) {
+  unsigned m = 0;
+  for (int i = b; i < e; i++) m |= 1u << i;
+  return m;
+}
+void zero_ctl(PBVH *p, const DV &g, unsigned mask) { kd_ctl_zero<<<1, 64, 0, p->stream>>>(g, mask); }
 
 int launch_check(const char *what) { return sg_check_hip(hipGetLastError(), what); }
 
@@ -1702,6 +1716,7 @@ int sg_dyn_upload(PBVH *p) {
   std::vector<long long> ctl(SGD_COUNT, 0);
   ctl[SGD_V] = m.V;
   ctl[SGD_T] = m.T;
+  ctl[SGD_SERIAL] = s->serial;
   rc |= sg_check_hip(hipMemcpy(d.ctl, ctl.data(), SGD_COUNT * 8, hipMemcpyHostToDevice), "copy");
   (void)st;
   s->pend.assign(N, 0);
@@ -1715,6 +1730,9 @@ int sg_dyn_download(PBVH *p, bool mesh) {
   if (sg_check_hip(hipStreamSynchronize(p->stream), "sync")) return -1;
   const int N = d.N;
   int rc = 0;
+  long long ser = 0;
+  rc |= sg_check_hip(hipMemcpy(&ser, d.ctl + SGD_SERIAL, 8, hipMemcpyDeviceToHost), "copy");
+  s->serial = (int)ser;
   rc |= sg_check_hip(hipMemcpy(t.flag.data(), d.node_flag, N * 4, hipMemcpyDeviceToHost), "copy");
   rc |= sg_check_hip(hipMemcpy(t.bb.data(), d.node_bb, (size_t)N * 32, hipMemcpyDeviceToHost), "copy");
   rc |= sg_check_hip(hipMemcpy(t.obb.data(), d.node_obb, (size_t)N * 32, hipMemcpyDeviceToHost), "copy");
@@ -1773,7 +1791,9 @@ int sg_dyn_sync_flags_down(PBVH *p) {
 // ---------------------------------------------------------------------------------
 // topology update (rounds)
 // ---------------------------------------------------------------------------------
-int sg_dyn_update_topology(PBVH *p, int mode, const float center[3], float radius) {
+// The topology update's launches (no wait).  gate: run only if the range search that
+// precedes it on the stream gathered a leaf (a dab); the counters stay in HBM.
+int topology_launch(PBVH *p, int mode, const float center[3], float radius, int gate) {
   SgDynState *s = p->dyn;
   SgDynDev &d = s->d;
   hipStream_t st = p->stream;
@@ -1791,14 +1811,16 @@ int sg_dyn_update_topology(PBVH *p, int mode, const float center[3], float radiu
   const int wser = ++s->stamp, lser = ++s->stamp;
   const int stamp0 = s->stamp;
   s->stamp += 2 * SG_DYN_MAX_ROUNDS + 2;  // the workgroup's round stamps
-  zero_ctl(p, g, SGD_NCAND, SGD_COUNT);
-  kd_wbuild<<<std::max(1, std::min(d.L, 4096)), DY_THREADS, 0, st>>>(g, wser);
-  kd_wrange<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, R, lser);
-  kd_topology<<<1, TW_THREADS, 0, st>>>(g, mode, R, s->min_len, s->max_len, s->serial, stamp0, wser, lser);
-  kd_clear_topology_flags<<<(d.N + 255) / 256, 256, 0, st>>>(g);
-  if (launch_check("dyntopo topology") || read_ctl(p)) return -1;
-  const long long *c = d.ctl_host;
-  s->stats[9] = c[SGD_NLIST];
+  zero_ctl(p, g, slots(SGD_NCAND, SGD_ST + 7) & ~(1u << SGD_NGATHER));
+  kd_wbuild<<<std::max(1, std::min(d.L, 4096)), DY_THREADS, 0, st>>>(g, wser, gate);
+  kd_wrange<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, R, lser, gate);
+  kd_topology<<<1, TW_THREADS, 0, st>>>(g, mode, R, s->min_len, s->max_len, gate, stamp0, wser, lser);
+  kd_clear_topology_flags<<<(d.N + 255) / 256, 256, 0, st>>>(g, gate);
+  return launch_check("dyntopo topology");
+}
+
+// the topology counters of the last update, from the host copy of the control words
+void topology_stats(SgDynState *s, const long long *c) {
   s->stats[2] += c[SGD_ST + 0];
   s->stats[3] += c[SGD_ST + 1];
   s->stats[4] += c[SGD_ST + 2];
@@ -1806,7 +1828,15 @@ int sg_dyn_update_topology(PBVH *p, int mode, const float center[3], float radiu
   s->stats[6] += c[SGD_ST + 4];
   s->stats[7] += c[SGD_ST + 5];
   s->stats[8] |= c[SGD_FLAGS] & 3;
-  s->serial++;
+  s->stats[9] = c[SGD_ST + 6];
+  s->serial = (int)c[SGD_SERIAL];
+}
+
+int sg_dyn_update_topology(PBVH *p, int mode, const float center[3], float radius) {
+  SgDynState *s = p->dyn;
+  if (topology_launch(p, mode, center, radius, 0) || read_ctl(p)) return -1;
+  const long long *c = s->d.ctl_host;
+  topology_stats(s, c);
   s->host_stale = true;
   return (c[SGD_ST + 0] + c[SGD_ST + 1]) > 0 ? 1 : 0;
 }
@@ -1814,27 +1844,31 @@ int sg_dyn_update_topology(PBVH *p, int mode, const float center[3], float radiu
 // ---------------------------------------------------------------------------------
 // normals and bounds of flagged leaves
 // ---------------------------------------------------------------------------------
-int sg_dyn_update_normals_bounds(PBVH *p, bool normals, bool bounds) {
+int normals_bounds_launch(PBVH *p, bool normals, bool bounds) {
   SgDynState *s = p->dyn;
   SgDynDev &d = s->d;
   hipStream_t st = p->stream;
   DV g = view(p);
   const int lb = std::max(1, (d.L + DY_THREADS - 1) / DY_THREADS);
   if (normals) {
-    zero_ctl(p, g, SGD_NVL, SGD_NVL + 1);
-    zero_ctl(p, g, SGD_NLIST, SGD_NFN + 1);
+    zero_ctl(p, g, (1u << SGD_NVL) | slots(SGD_NLIST, SGD_NFN + 1));
     kd_flagged<<<lb, DY_THREADS, 0, st>>>(g, PBVH_UpdateNormals);
     kd_face_normals<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, ++s->stamp);
     kd_vert_normals<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g);
   }
   if (bounds) {
-    zero_ctl(p, g, SGD_NLIST, SGD_NLIST + 1);
-    zero_ctl(p, g, SGD_NREFIT, SGD_NREFIT + 1);
+    zero_ctl(p, g, (1u << SGD_NLIST) | (1u << SGD_NREFIT));
     kd_flagged<<<lb, DY_THREADS, 0, st>>>(g, PBVH_UpdateBB | PBVH_UpdateOriginalBB | PBVH_UpdateRedraw);
     kd_leaf_bounds<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, ++s->stamp);
     kd_flush<<<1, 1024, 0, st>>>(g, d.H);
   }
-  if (launch_check("dyntopo normals/bounds") || read_ctl(p)) return -1;
+  return launch_check("dyntopo normals/bounds");
+}
+
+int sg_dyn_update_normals_bounds(PBVH *p, bool normals, bool bounds) {
+  SgDynState *s = p->dyn;
+  SgDynDev &d = s->d;
+  if (normals_bounds_launch(p, normals, bounds) || read_ctl(p)) return -1;
   if (normals) s->stats[10] = d.ctl_host[SGD_NFN];
   if (bounds) s->stats[11] = d.ctl_host[SGD_NREFIT];
   s->host_stale = true;
@@ -1867,17 +1901,16 @@ int sg_dyn_dab_apply(PBVH *p, const SgDabParams *dab) {
   const int lb = std::max(1, (d.L + DY_THREADS - 1) / DY_THREADS);
   const int update = PBVH_UpdateNormals | PBVH_UpdateBB | PBVH_UpdateOriginalBB | PBVH_UpdateDrawBuffers |
                      PBVH_UpdateRedraw;
-  if (tool != SG_TOOL_GRAB && s->mode) {
+  // one wait per dab: every step is queued, the counters read once at the end
+  const bool topo = tool != SG_TOOL_GRAB && s->mode;
+  if (topo) {
     const float rt = k.r * 1.25f;
-    zero_ctl(p, g, SGD_NGATHER, SGD_NGATHER + 1);
+    zero_ctl(p, g, 1u << SGD_NGATHER);
     kd_search<<<lb, DY_THREADS, 0, st>>>(g, k.cx, k.cy, k.cz, rt * rt, update | PBVH_UpdateTopology);
-    if (launch_check("dyntopo search") || read_ctl(p)) return -1;
-    if (d.ctl_host[SGD_NGATHER]) {
-      const float c[3] = {k.cx, k.cy, k.cz};
-      if (sg_dyn_update_topology(p, s->mode, c, k.r) < 0) return -1;
-    }
+    const float c[3] = {k.cx, k.cy, k.cz};
+    if (launch_check("dyntopo search") || topology_launch(p, s->mode, c, k.r, 1)) return -1;
   }
-  zero_ctl(p, g, SGD_NGATHER, SGD_NREFIT + 1);
+  zero_ctl(p, g, slots(SGD_NGATHER, SGD_NREFIT + 1));
   kd_search<<<lb, DY_THREADS, 0, st>>>(g, k.cx, k.cy, k.cz, k.r2, update);
   const int stamp = ++s->stamp;
   kd_unique<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, stamp);
@@ -1892,13 +1925,15 @@ int sg_dyn_dab_apply(PBVH *p, const SgDabParams *dab) {
     }
     kd_smooth_store<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, a);
   }
-  if (launch_check("dyntopo brush") || read_ctl(p)) return -1;
-  s->stats[0] = d.ctl_host[SGD_NGATHER];
-  s->stats[1] = d.ctl_host[SGD_NAFF];
-  const long long naff = d.ctl_host[SGD_NAFF];
-  if (sg_dyn_update_normals_bounds(p, true, true)) return -1;
-  // the affected list survives the normals (they reuse the vertex list only)
-  (void)naff;
+  if (launch_check("dyntopo brush") || normals_bounds_launch(p, true, true) || read_ctl(p)) return -1;
+  // the gathered and affected counts survive the normals and bounds (other words)
+  const long long *c = d.ctl_host;
+  s->stats[0] = c[SGD_NGATHER];
+  s->stats[1] = c[SGD_NAFF];
+  s->stats[10] = c[SGD_NFN];
+  s->stats[11] = c[SGD_NREFIT];
+  if (topo) topology_stats(s, c);
+  s->host_stale = true;
   p->dab_count++;
   return 0;
 }

@@ -53,7 +53,8 @@ enum {
   SGD_NS = 21,      // seed faces this round (in range, with an edge past the phase's limit)
   SGD_NS2 = 22,     // ... for the next round
   SGD_ST = 24,      // topology counters: splits, collapses, split rounds, collapse rounds,
-                    // split candidates, collapse candidates
+                    // split candidates, collapse candidates, leaves in the working set
+  SGD_SERIAL = 31,  // topology updates so far (labels of new vertices hash it)
   SGD_COUNT = 32,
 };
 
@@ -107,7 +108,7 @@ struct SgDynState {
   float max_len = 0.75f, min_len = 0.3f;
   int mode = PBVH_Subdivide | PBVH_Collapse, use_frontface = 0, use_projected = 0;
   float view_normal[3] = {0.0f, 0.0f, 1.0f};
-  int serial = 0;        // topology updates so far
+  int serial = 0;        // topology updates so far (HBM holds the live count while uploaded)
   int stamp = 0;         // round / dedupe stamps
   long long stats[12] = {0};
   bool host_stale = false;  // the host tree mirror trails the device
@@ -129,7 +130,7 @@ int sg_dyn_last_gathered(PBVH *p, int *out, int cap);
 int sg_dyn_upload(PBVH *p);                 // mesh + tree -> HBM (allocates on first use)
 int sg_dyn_download(PBVH *p, bool mesh);    // HBM -> host mesh (mesh) and tree
 void sg_dyn_free_device(PBVH *p);
-int sg_dyn_update_topology(PBVH *p, int mode, const float center[3], float radius);
+int sg_dyn_update_topology(PBVH *p, int mode, const float center[3], float radius);  // waits; 1 if modified
 int sg_dyn_dab_apply(PBVH *p, const SgDabParams *dab);
 int sg_dyn_update_normals_bounds(PBVH *p, bool normals, bool bounds);
 int sg_dyn_sync_flags_down(PBVH *p);
