@@ -47,7 +47,7 @@ struct DV {
   unsigned long long *claim;
   int *estamp, *remap, *remap_ser;
   int *node_flag, *node_child, *node_dirty, *leaf_node, *lvl_off, *lvl_node;
-  int *node_parent, *node_height, *anc_stamp, *lvl_list, *lvl_cnt;
+  int *node_parent, *node_height, *anc_stamp, *anc_pos, *lvl_list, *lvl_cnt;
   float *node_bb, *node_obb;
   int *slab, *scount;
   int *wlist, *cand, *win, *outer, *hv_in, *hv_out, *touched, *gathered, *vlist, *aff, *nlist;
@@ -181,6 +181,25 @@ __device__ __forceinline__ bool ring_closed(const DV &g, int v, int (&b)[RB], in
     }
   }
   return state == 1;
+}
+
+
+// ring_for's count and boundary flag: one lap for a closed fan of at most RB corners
+__device__ __forceinline__ int ring_count(const DV &g, int v, int &bnd) {
+  const int c0 = g.vcorner[v];
+  if (c0 >= 0) {
+    int c = c0;
+    for (int j = 1; j <= RB; j++) {
+      const int x = g.topp[dp(c)];
+      if (x < 0) break;
+      c = dp(x);
+      if (c == c0) {
+        bnd = 0;
+        return j;
+      }
+    }
+  }
+  return ring_for(g, v, bnd, [](int, int) {});
 }
 
 __device__ __forceinline__ int valence(const DV &g, int v) {
@@ -906,7 +925,7 @@ __global__ __launch_bounds__(TW_THREADS) void kd_topology(DV g0, int mode, Range
         } else {
           const int v = hv_in[i - ln];
           int b0;
-          const int n0 = ring_for(g, v, b0, [](int, int) {});
+          const int n0 = ring_count(g, v, b0);
           if (n0 < 0 || n0 + b0 <= 8) continue;
           int bnd;
           ring_for(g, v, bnd, [&](int k, int j) {
@@ -1390,6 +1409,7 @@ __global__ __launch_bounds__(DY_THREADS) void kd_leaf_bounds(DV g, int stamp) {
           const int h = g.node_height[a];
           const int s = atomicAdd(&g.lvl_cnt[h], 1);
           g.lvl_list[g.lvl_off[h - 1] + s] = a;
+          g.anc_pos[a] = s;
         }
     }
   }
@@ -1398,42 +1418,131 @@ __global__ __launch_bounds__(DY_THREADS) void kd_leaf_bounds(DV g, int stamp) {
 // internal nodes with an updated child: union of the children (child 1, then 2),
 // height by height over the listed ancestors of refit leaves; one workgroup.  Nodes
 // off those paths have no updated child (pbvh_flush_bb leaves them as they are).
-__global__ __launch_bounds__(1024) void kd_flush(DV g, int H) {
-  for (int h = 1; h <= H; h++) {
-    const int b = g.lvl_off[h - 1], n = g.lvl_cnt[h];
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-      const int node = g.lvl_list[b + i];
-      const int c = g.node_child[node];
-      const int u = g.node_dirty[c] | g.node_dirty[c + 1];
-      if (u & PBVH_UpdateBB) {
-        float bb[8];
-        for (int a = 0; a < 3; a++) {
-          float lo = FLT_MAX, hi = -FLT_MAX;
-          lo = sg_min(lo, g.node_bb[8 * (size_t)c + a]);
-          hi = sg_max(hi, g.node_bb[8 * (size_t)c + 4 + a]);
-          lo = sg_min(lo, g.node_bb[8 * (size_t)(c + 1) + a]);
-          hi = sg_max(hi, g.node_bb[8 * (size_t)(c + 1) + 4 + a]);
-          bb[a] = lo;
-          bb[4 + a] = hi;
-        }
-        bb[3] = bb[7] = 0.0f;
-        for (int j = 0; j < 8; j++) g.node_bb[8 * (size_t)node + j] = bb[j];
-      }
-      if (u & PBVH_UpdateOriginalBB)
-        for (int j = 0; j < 8; j++) g.node_obb[8 * (size_t)node + j] = g.node_bb[8 * (size_t)node + j];
-      g.node_dirty[node] = u;
+// Up to FL_CAP listed ancestors run from LDS: every input (their boxes, their children's
+// slots or boxes and flags) is read in one parallel pass, the heights then take one
+// barrier each, and the results go back in one pass.  More ancestors run over HBM.
+#define FL_CAP 512
+#define FL_MAXH 64
+
+__global__ __launch_bounds__(1024) void kd_flush(DV g, int H, int stamp) {
+  __shared__ int s_base[FL_MAXH + 2];  // first slot per height
+  __shared__ int s_node[FL_CAP], s_k[2][FL_CAP], s_d[FL_CAP], s_cd[2][FL_CAP];
+  __shared__ float s_bb[FL_CAP][8], s_cb[2][FL_CAP][6];
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const bool lds = H + 1 < FL_MAXH;
+  if (lds) {
+    for (int h = tid; h <= H; h += nt) s_base[h + 1] = h >= 1 ? g.lvl_cnt[h] : 0;
+    __syncthreads();
+    if (tid == 0) {
+      s_base[0] = 0;
+      s_base[1] = 0;
+      for (int h = 1; h <= H; h++) s_base[h + 1] += s_base[h];
     }
     __syncthreads();
   }
-  // clear the marks: the listed ancestors, the refit leaves, the level counts
-  for (int h = 1; h <= H; h++) {
-    const int b = g.lvl_off[h - 1], n = g.lvl_cnt[h];
-    for (int i = threadIdx.x; i < n; i += blockDim.x) g.node_dirty[g.lvl_list[b + i]] = 0;
+  const int total = lds ? s_base[H + 1] : 0;
+  if (lds && total <= FL_CAP) {
+    for (int i = tid; i < total; i += nt) {
+      int h = 1;
+      while (s_base[h + 1] <= i) h++;
+      const int node = g.lvl_list[g.lvl_off[h - 1] + (i - s_base[h])];
+      const int c = g.node_child[node];
+      s_node[i] = node;
+      for (int j = 0; j < 8; j++) s_bb[i][j] = g.node_bb[8 * (size_t)node + j];
+      for (int e = 0; e < 2; e++) {
+        const int cc = c + e;
+        int k = -1;
+        if (g.anc_stamp[cc] == stamp) {
+          const int hc = g.node_height[cc];
+          k = s_base[hc] + g.anc_pos[cc];
+        } else {
+          for (int a = 0; a < 3; a++) {
+            s_cb[e][i][a] = g.node_bb[8 * (size_t)cc + a];
+            s_cb[e][i][3 + a] = g.node_bb[8 * (size_t)cc + 4 + a];
+          }
+          s_cd[e][i] = g.node_dirty[cc];
+        }
+        s_k[e][i] = k;
+      }
+    }
+    __syncthreads();
+    for (int h = 1; h <= H; h++) {
+      for (int i = s_base[h] + tid; i < s_base[h + 1]; i += nt) {
+        const float *cb[2];
+        int u = 0;
+        for (int e = 0; e < 2; e++) {
+          const int k = s_k[e][i];
+          if (k >= 0) {
+            u |= s_d[k];
+            cb[e] = nullptr;
+          } else {
+            u |= s_cd[e][i];
+            cb[e] = s_cb[e][i];
+          }
+        }
+        if (u & PBVH_UpdateBB) {
+          for (int a = 0; a < 3; a++) {
+            float lo = FLT_MAX, hi = -FLT_MAX;
+            for (int e = 0; e < 2; e++) {
+              const int k = s_k[e][i];
+              lo = sg_min(lo, k >= 0 ? s_bb[k][a] : cb[e][a]);
+              hi = sg_max(hi, k >= 0 ? s_bb[k][4 + a] : cb[e][3 + a]);
+            }
+            s_bb[i][a] = lo;
+            s_bb[i][4 + a] = hi;
+          }
+          s_bb[i][3] = s_bb[i][7] = 0.0f;
+        }
+        s_d[i] = u;
+      }
+      __syncthreads();
+    }
+    for (int i = tid; i < total; i += nt) {
+      const int node = s_node[i], u = s_d[i];
+      if (u & PBVH_UpdateBB)
+        for (int j = 0; j < 8; j++) g.node_bb[8 * (size_t)node + j] = s_bb[i][j];
+      if (u & PBVH_UpdateOriginalBB)
+        for (int j = 0; j < 8; j++) g.node_obb[8 * (size_t)node + j] = s_bb[i][j];
+      g.node_dirty[node] = 0;
+    }
+  } else {
+    for (int h = 1; h <= H; h++) {
+      const int b = g.lvl_off[h - 1], n = g.lvl_cnt[h];
+      for (int i = tid; i < n; i += nt) {
+        const int node = g.lvl_list[b + i];
+        const int c = g.node_child[node];
+        const int u = g.node_dirty[c] | g.node_dirty[c + 1];
+        if (u & PBVH_UpdateBB) {
+          float bb[8];
+          for (int a = 0; a < 3; a++) {
+            float lo = FLT_MAX, hi = -FLT_MAX;
+            lo = sg_min(lo, g.node_bb[8 * (size_t)c + a]);
+            hi = sg_max(hi, g.node_bb[8 * (size_t)c + 4 + a]);
+            lo = sg_min(lo, g.node_bb[8 * (size_t)(c + 1) + a]);
+            hi = sg_max(hi, g.node_bb[8 * (size_t)(c + 1) + 4 + a]);
+            bb[a] = lo;
+            bb[4 + a] = hi;
+          }
+          bb[3] = bb[7] = 0.0f;
+          for (int j = 0; j < 8; j++) g.node_bb[8 * (size_t)node + j] = bb[j];
+        }
+        if (u & PBVH_UpdateOriginalBB)
+          for (int j = 0; j < 8; j++) g.node_obb[8 * (size_t)node + j] = g.node_bb[8 * (size_t)node + j];
+        g.node_dirty[node] = u;
+      }
+      __syncthreads();
+    }
+    // clear the marks of the listed ancestors
+    for (int h = 1; h <= H; h++) {
+      const int b = g.lvl_off[h - 1], n = g.lvl_cnt[h];
+      for (int i = tid; i < n; i += nt) g.node_dirty[g.lvl_list[b + i]] = 0;
+    }
   }
+  // clear the marks of the refit leaves and the level counts
   const int nl = (int)g.ctl[SGD_NLIST];
-  for (int i = threadIdx.x; i < nl; i += blockDim.x) g.node_dirty[g.nlist[i]] = 0;
+  for (int i = tid; i < nl; i += nt) g.node_dirty[g.nlist[i]] = 0;
   __syncthreads();
-  for (int h = threadIdx.x; h <= H; h += blockDim.x) g.lvl_cnt[h] = 0;
+  for (int h = tid; h <= H; h += nt) g.lvl_cnt[h] = 0;
 }
 
 __global__ void kd_ctl_zero(DV g, unsigned mask) {
@@ -1512,6 +1621,7 @@ DV view(PBVH *p) {
   g.node_parent = d.node_parent;
   g.node_height = d.node_height;
   g.anc_stamp = d.anc_stamp;
+  g.anc_pos = d.anc_pos;
   g.lvl_list = d.lvl_list;
   g.lvl_cnt = d.lvl_cnt;
   g.node_bb = d.node_bb;
@@ -1576,7 +1686,7 @@ void sg_dyn_free_device(PBVH *p) {
   for (float **q : fl) dfree(*q);
   int **il[] = {&d.vnode, &d.vcorner, &d.vstamp, &d.astamp, &d.aslot, &d.hvst, &d.tv, &d.topp, &d.tnode, &d.tslot,
                 &d.wstamp, &d.estamp, &d.remap, &d.remap_ser, &d.node_flag, &d.node_child, &d.node_dirty,
-                &d.leaf_node, &d.lvl_off, &d.lvl_node, &d.node_parent, &d.node_height, &d.anc_stamp, &d.lvl_list,
+                &d.leaf_node, &d.lvl_off, &d.lvl_node, &d.node_parent, &d.node_height, &d.anc_stamp, &d.anc_pos, &d.lvl_list,
                 &d.lvl_cnt, &d.slab, &d.scount, &d.wlist, &d.cand, &d.win, &d.outer,
                 &d.hv0, &d.hv1, &d.touched, &d.gathered, &d.vlist, &d.aff, &d.nlist, &d.llist, &d.lstamp,
                 &d.seed0, &d.seed1, &d.sstamp};
@@ -1633,6 +1743,7 @@ int sg_dyn_upload(PBVH *p) {
     rc |= dalloc(d.lvl_off, Nc + 1) | dalloc(d.lvl_node, Nc) | dalloc(d.node_bb, 8 * Nc) | dalloc(d.node_obb, 8 * Nc);
     rc |= dalloc(d.slab, Nc * SG_DYN_SLAB) | dalloc(d.scount, Nc) | dalloc(d.gathered, Nc) | dalloc(d.nlist, Nc);
     rc |= dalloc(d.node_parent, Nc) | dalloc(d.node_height, Nc) | dalloc(d.anc_stamp, Nc) | dalloc(d.lvl_list, Nc);
+    rc |= dalloc(d.anc_pos, Nc);
     rc |= dalloc(d.lvl_cnt, Nc + 1);
     if (rc) return -1;
     d.Ncap = (int)Nc;
@@ -1859,8 +1970,9 @@ int normals_bounds_launch(PBVH *p, bool normals, bool bounds) {
   if (bounds) {
     zero_ctl(p, g, (1u << SGD_NLIST) | (1u << SGD_NREFIT));
     kd_flagged<<<lb, DY_THREADS, 0, st>>>(g, PBVH_UpdateBB | PBVH_UpdateOriginalBB | PBVH_UpdateRedraw);
-    kd_leaf_bounds<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, ++s->stamp);
-    kd_flush<<<1, 1024, 0, st>>>(g, d.H);
+    const int bstamp = ++s->stamp;
+    kd_leaf_bounds<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, bstamp);
+    kd_flush<<<1, 1024, 0, st>>>(g, d.H, bstamp);
   }
   return launch_check("dyntopo normals/bounds");
 }

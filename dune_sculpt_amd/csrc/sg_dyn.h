@@ -77,7 +77,7 @@ struct SgDynDev {
   // nodes
   int *node_flag = nullptr, *node_child = nullptr, *node_dirty = nullptr, *leaf_node = nullptr;
   int *lvl_off = nullptr, *lvl_node = nullptr;  // internal nodes grouped by height (children first)
-  int *node_parent = nullptr, *node_height = nullptr, *anc_stamp = nullptr;
+  int *node_parent = nullptr, *node_height = nullptr, *anc_stamp = nullptr, *anc_pos = nullptr;
   int *lvl_list = nullptr, *lvl_cnt = nullptr;  // ancestors of refit leaves, by height (lvl_off layout)
   float *node_bb = nullptr, *node_obb = nullptr;  // [8] per node: min xyz, pad, max xyz, pad
   int *slab = nullptr, *scount = nullptr;
