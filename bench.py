@@ -671,7 +671,7 @@ def run_dyntopo(sg, W, K, args):
     it, on the leaves within 1.25 r of each Draw / Inflate / Smooth dab), a stroke of
     args.dyn_dabs dabs cycling Draw / Inflate / Smooth(2) / Grab, r ~ U[0.02, 0.05].  Each
     step rebuilds the mesh and PBVH (untimed) and times the stroke (the host's clock around
-    it: each topology round waits for its counters).  CPU: the oracle, one thread, on the
+    it: each dab waits once, at its end, for its counters).  CPU: the oracle, one thread, on the
     first dabs of the same stroke within a budget, every dab's counters checked against
     the GPU's."""
     sys.path.insert(0, str(ROOT / "oracle"))
