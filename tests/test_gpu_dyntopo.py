@@ -201,3 +201,53 @@ def test_bench_size_stroke_final_state(sg, po):
     p.close()
     m.close()
     o.close()
+
+
+def polar_disk(rings, seg, seed, amplitude=0.01):
+    """An open disk: a centre vertex of valence `seg` (past the fans the kernels hold in
+    registers), `rings` rings of `seg` vertices, the outer ring a boundary."""
+    co = [(0.0, 0.0, 0.0)]
+    for r in range(1, rings + 1):
+        rad = r / rings
+        for k in range(seg):
+            a = 2.0 * np.pi * k / seg
+            co.append((rad * np.cos(a), rad * np.sin(a), 0.0))
+    co = np.array(co, np.float32)
+    co[:, 2] = np.random.default_rng(seed).uniform(-amplitude, amplitude, len(co)).astype(np.float32)
+    tris = [(0, 1 + k, 1 + (k + 1) % seg) for k in range(seg)]
+    for r in range(1, rings):
+        b0, b1 = 1 + (r - 1) * seg, 1 + r * seg
+        for k in range(seg):
+            a, b = b0 + k, b0 + (k + 1) % seg
+            c, d = b1 + k, b1 + (k + 1) % seg
+            tris += [(a, c, d), (a, d, b)]
+    return co, np.array(tris, np.int32)
+
+
+def test_open_disk_boundary_and_high_valence(sg, po):
+    """Boundary fans (open rings, boundary edges, boundary collapse rules) and a centre of
+    valence 32 (walks past the register-held fans) through a stroke that crosses the rim
+    and the centre."""
+    co, tris = polar_disk(16, 32, seed=21)
+    m, p, o = pair(sg, po, co, tris, 0.12)
+    rng = np.random.default_rng(22)
+    tools = [sg.TOOL_DRAW, sg.TOOL_INFLATE, sg.TOOL_SMOOTH, sg.TOOL_GRAB]
+    dabs = []
+    n = 24
+    for i in range(n):
+        t = i / (n - 1)
+        rad = 1.1 * (1.0 - t)  # from past the rim to the centre
+        a = 3.0 * t
+        c = np.array([rad * np.cos(a), rad * np.sin(a), 0.0], np.float32)
+        dabs.append(sg.make_dab(tool=tools[i % 4], location=c, radius=float(rng.uniform(0.2, 0.35)),
+                                strength=float(rng.uniform(0.3, 0.9)), smooth_iterations=2,
+                                grab_delta=np.array([0.01, -0.02, 0.005], np.float32)))
+    splits, collapses = run(sg, po, p, o, dabs, 4)
+    assert splits > 100 and collapses > 10, (splits, collapses)
+    assert o.check() == 0
+    p.after_stroke()
+    o.after_stroke()
+    same_state(p, o, "disk after stroke")
+    p.close()
+    m.close()
+    o.close()
