@@ -371,10 +371,6 @@ __device__ void reg_edge(const DV &g, unsigned *eset, int c, int collapse, int s
   if (!edge_first(g, eset, k, stamp)) return;
   const unsigned long long slot = atomicAdd((unsigned long long *)&g.ctl[SGD_NCAND], 1ull);
   if (slot < (unsigned long long)g.ccap) g.cand[slot] = k;
-  claim_set(g, k, collapse, [&](int t) {
-    g.claim[t] = 0ull;
-    g.tie[t] = 0ull;
-  });
 }
 
 // long_edge_queue_edge_add_recursive (pbvh_bmesh.c:849-903), iterative
@@ -852,10 +848,10 @@ __global__ __launch_bounds__(DY_THREADS) void kd_wrange(DV g, Range R, int lser,
   }
 }
 
-#define TW_THREADS 1024
+#define TW_THREADS 768
 
 __global__ __launch_bounds__(TW_THREADS) void kd_topology(DV g0, int mode, Range R, float min_len, float max_len,
-                                                           int gate, int stamp0, int wser, int lser) {
+                                                           int gate, int stamp0, int wser, int lser, int tag_base) {
   const int tid = threadIdx.x, nt = blockDim.x;
   // a dab's update runs only when its range search gathered a leaf
   if (gate && g0.ctl[SGD_NGATHER] == 0) return;
@@ -962,7 +958,9 @@ __global__ __launch_bounds__(TW_THREADS) void kd_topology(DV g0, int mode, Range
         g.valid[i] = ok ? 1 : 0;
         if (!ok) continue;
         atomicAdd((unsigned long long *)&g.ctl[SGD_NVALID], 1ull);
-        const unsigned long long kk = edge_key(g, c, collapse);
+        // the round's tag above the key: claims of earlier rounds are smaller, so no
+        // face's claim needs clearing
+        const unsigned long long kk = ((unsigned long long)(stamp - tag_base) << 43) | edge_key(g, c, collapse);
         g.key[i] = kk;
         claim_set(g, c, collapse, [&](int t) {
           const unsigned long long old = atomicMax(&g.claim[t], kk);
@@ -1925,7 +1923,14 @@ int topology_launch(PBVH *p, int mode, const float center[3], float radius, int 
   zero_ctl(p, g, slots(SGD_NCAND, SGD_ST + 7) & ~(1u << SGD_NGATHER));
   kd_wbuild<<<std::max(1, std::min(d.L, 4096)), DY_THREADS, 0, st>>>(g, wser, gate);
   kd_wrange<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, R, lser, gate);
-  kd_topology<<<1, TW_THREADS, 0, st>>>(g, mode, R, s->min_len, s->max_len, gate, stamp0, wser, lser);
+  // round tags stay below 2^21 (keys take bits 0..42): past that, claims restart from zero
+  if (stamp0 + 2 * SG_DYN_MAX_ROUNDS + 2 - s->tag_base >= (1 << 21)) {
+    if (sg_check_hip(hipMemsetAsync(d.claim, 0, (size_t)d.Tcap * 8, st), "memset") ||
+        sg_check_hip(hipMemsetAsync(d.tie, 0, (size_t)d.Tcap * 8, st), "memset"))
+      return -1;
+    s->tag_base = stamp0;
+  }
+  kd_topology<<<1, TW_THREADS, 0, st>>>(g, mode, R, s->min_len, s->max_len, gate, stamp0, wser, lser, s->tag_base);
   kd_clear_topology_flags<<<(d.N + 255) / 256, 256, 0, st>>>(g, gate);
   return launch_check("dyntopo topology");
 }

@@ -110,6 +110,7 @@ struct SgDynState {
   float view_normal[3] = {0.0f, 0.0f, 1.0f};
   int serial = 0;        // topology updates so far (HBM holds the live count while uploaded)
   int stamp = 0;         // round / dedupe stamps
+  int tag_base = 0;      // claims carry (round stamp - tag_base) above their key
   long long stats[12] = {0};
   bool host_stale = false;  // the host tree mirror trails the device
   std::vector<int> pend;    // node flag bits set by host calls, not yet on the device
