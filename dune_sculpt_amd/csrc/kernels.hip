@@ -161,7 +161,7 @@ __global__ __launch_bounds__(256) void k_search(SgDevView d, SgDabK k, long long
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     const int lc = min(l, d.L - 1);
     const BB6 b = bb_load(leaf_bbs + 8 * (size_t)lc);
-    const int fl = d.leaf_flags[lc];
+    const int fl = k.hide_skip ? d.leaf_flags[lc] : 0;
     const bool pass = l < d.L && !(fl & k.hide_skip) && sphere_box(b, k.cx, k.cy, k.cz, k.r2);
     const unsigned long long m = __ballot(pass);
     if (m == 0) return;
@@ -1729,6 +1729,9 @@ int sg_enqueue_dab(PBVH *p, const SgDabK &k, const float *curve_dev, long long *
   // the fused dab (one kernel after the search) when the mesh has halo tables and the
   // dab is a plain brush dab it covers
   SgDabK kk = k;
+  // no leaf is fully hidden: the hidden test passes every leaf, and the search skips
+  // loading the flags
+  if (p->d.nhidden_leaves == 0) kk.hide_skip = 0;
   kk.fused = 0;
   const bool fused_tool = k.tool == SG_TOOL_DRAW || k.tool == SG_TOOL_INFLATE || k.tool == SG_TOOL_GRAB ||
                           (k.tool == SG_TOOL_SMOOTH && k.iters >= 1 && k.iters <= SG_HALO_RINGS);

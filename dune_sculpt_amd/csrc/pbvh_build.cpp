@@ -995,6 +995,8 @@ void sg_build_device(PBVH *p, const float *co) {
   alloc_fill(&d.prev_cuts, C, 0, s);
 
   d.nuniq = d.nuniq_visible = h.leaf_vbeg[L];
+  d.nhidden_leaves = 0;
+  for (int l = 0; l < L; l++) d.nhidden_leaves += h.leaf_flags[l] ? 1 : 0;
   if (!p->vhide.empty()) {
     std::vector<int> vh(V);
     for (int i = 0; i < V; i++) vh[i] = p->vhide[p->perm_old_of_new[i]];

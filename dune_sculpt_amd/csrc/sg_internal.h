@@ -62,6 +62,7 @@ struct SgUndoState {
 struct SgDev {
   int V = 0, T = 0, L = 0, N = 0, C = 0;
   long long nuniq = 0, nuniq_visible = 0;  // uniq vertices of the leaves, and the visible ones
+  int nhidden_leaves = 0;                   // fully hidden leaves (fixed at build)
   float *px = nullptr, *py = nullptr, *pz = nullptr; // positions [V]
   float *qx = nullptr, *qy = nullptr, *qz = nullptr; // shadow copy (== positions between dabs)
   float *nx = nullptr, *ny = nullptr, *nz = nullptr; // normals [V]
