@@ -202,12 +202,6 @@ __device__ __forceinline__ int ring_count(const DV &g, int v, int &bnd) {
   return ring_for(g, v, bnd, [](int, int) {});
 }
 
-__device__ __forceinline__ int valence(const DV &g, int v) {
-  int bnd;
-  const int n = ring_for(g, v, bnd, [](int, int) {});
-  return n < 0 ? SG_DYN_RING + 1 : n + bnd;
-}
-
 // closest_on_tri_to_point_v3 (math_geom.cc:966-1044)
 __device__ void closest_on_tri(float r[3], const float p[3], const float v1[3], const float v2[3], const float v3[3]) {
   float ab[3], ac[3], ap[3], bp[3], cp[3];
