@@ -150,15 +150,22 @@ __global__ __launch_bounds__(256) void k_search(SgDevView d, SgDabK k, long long
                                                 int copy_blocks, int copy_pending) {
   if (d.halt && d.halt[0]) return;
   const int lane = lane_id();
-  if ((int)blockIdx.x >= leaf_blocks + cut_blocks) {
-    const int cb = blockIdx.x - (leaf_blocks + cut_blocks);
+  // Block roles in launch order: subtree waves, copy blocks, then leaves.  The
+  // dispatcher starts blocks in index order, and the subtree refreshes and the copy are
+  // the longest blocks, so they start first.  vb: the block's number in the order
+  // leaves, subtrees, copy.
+  const int bx = (int)blockIdx.x;
+  const int vb = bx < cut_blocks ? leaf_blocks + bx
+                                 : (bx < cut_blocks + copy_blocks ? leaf_blocks + bx : bx - cut_blocks - copy_blocks);
+  if (vb >= leaf_blocks + cut_blocks) {
+    const int cb = vb - (leaf_blocks + cut_blocks);
     if (copy_pending) fused_copy_back(d, cb * blockDim.x + threadIdx.x, copy_blocks * blockDim.x);
     return;
   }
   sg_place(k, place);
   const float *leaf_bbs = k.use_orig ? d.leaf_obb : d.leaf_bb;
-  if ((int)blockIdx.x < leaf_blocks) {
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (vb < leaf_blocks) {
+    const int l = vb * blockDim.x + threadIdx.x;
     const int lc = min(l, d.L - 1);
     const BB6 b = bb_load(leaf_bbs + 8 * (size_t)lc);
     const int fl = k.hide_skip ? d.leaf_flags[lc] : 0;
@@ -199,7 +206,7 @@ __global__ __launch_bounds__(256) void k_search(SgDevView d, SgDabK k, long long
     return;
   }
   // subtree waves
-  const int c = (blockIdx.x - leaf_blocks) * (blockDim.x / SG_WAVE) + threadIdx.x / SG_WAVE;
+  const int c = (vb - leaf_blocks) * (blockDim.x / SG_WAVE) + threadIdx.x / SG_WAVE;
   if (c == d.C) {  // bookkeeping
     const int stamp = d.dstamp ? d.dstamp[serial & 1] : 0;
     if (log && log_index > 0) {
