@@ -29,7 +29,7 @@
 #include "sg_grids.h"
 
 #define GR_THREADS 256
-#define GR_CHUNK 1024  // elements per work item (4 per thread)
+#define GR_CHUNK 256  // elements per work item (1 per thread)
 
 namespace {
 
