@@ -446,7 +446,9 @@ __device__ __forceinline__ void avg_list(const GView &g, const int *els, int str
 // the seams between its grids and the shared centre (:951-984); its edges and coarse
 // vertices are queued once each for the boundary and corner averages.  Leaves of
 // elements the averages move are marked for the refit.
-__global__ __launch_bounds__(GR_THREADS) void kg_face_normals(GView g, int batch) {
+#define FN_THREADS 512  // a face's grids (4 x 81 elements) in one pass
+
+__global__ __launch_bounds__(FN_THREADS) void kg_face_normals(GView g, int batch) {
   extern __shared__ float s_pos[];  // `batch` grids' positions, xyz interleaved
   const int serial = g.dser[0];
   const int nf = (int)g.ctr[SGG_NFACES];
@@ -458,7 +460,7 @@ __global__ __launch_bounds__(GR_THREADS) void kg_face_normals(GView g, int batch
     // (one coalesced pass), then their quad normals read them there
     for (int c0 = 0; c0 < n; c0 += batch) {
       const int nc = min(batch, n - c0);
-      for (int t = threadIdx.x; t < nc * ga; t += GR_THREADS) {
+      for (int t = threadIdx.x; t < nc * ga; t += FN_THREADS) {
         const int c = t / ga, local = t - c * ga;
         const int v = g.grid_pos[g0 + c0 + c] * ga + local;
         s_pos[3 * t] = g.px[v];
@@ -466,7 +468,7 @@ __global__ __launch_bounds__(GR_THREADS) void kg_face_normals(GView g, int batch
         s_pos[3 * t + 2] = g.pz[v];
       }
       __syncthreads();
-      for (int t = threadIdx.x; t < nc * ga; t += GR_THREADS) {
+      for (int t = threadIdx.x; t < nc * ga; t += FN_THREADS) {
         const int c = t / ga, local = t - c * ga;
         const int y = local / gs, x = local - y * gs;
         const float *P = s_pos + 3 * c * ga;
@@ -510,16 +512,16 @@ __global__ __launch_bounds__(GR_THREADS) void kg_face_normals(GView g, int batch
       }
       __syncthreads();
     }
-    for (int t = threadIdx.x; t < n * g1; t += GR_THREADS) {
+    for (int t = threadIdx.x; t < n * g1; t += FN_THREADS) {
       const int c = t / g1, ii = 1 + (t - c * g1);
       const int prev = g0 + (c + n - 1) % n;
       avg_pair(g, elem_id(g, prev, ii, 0), elem_id(g, g0 + c, 0, ii), serial);
     }
     // the shared centre: the grids' centre values gathered in parallel, summed in grid
     // order by one thread, written back in parallel
-    __shared__ float s_c[7][GR_THREADS];
+    __shared__ float s_c[7][FN_THREADS];
     __shared__ float s_mean[7];
-    const bool par = n <= GR_THREADS;
+    const bool par = n <= FN_THREADS;
     if (par && threadIdx.x < n) {
       const int v = g.grid_pos[g0 + threadIdx.x] * ga;
       s_c[0][threadIdx.x] = g.px[v];
@@ -544,7 +546,7 @@ __global__ __launch_bounds__(GR_THREADS) void kg_face_normals(GView g, int batch
       for (int q = 0; q < 7; q++) s_mean[q] = a[q] * inv;
     }
     __syncthreads();
-    for (int c = threadIdx.x; c < n; c += GR_THREADS) {
+    for (int c = threadIdx.x; c < n; c += FN_THREADS) {
       const int v = g.grid_pos[g0 + c] * ga;
       const float cx = s_mean[0], cy = s_mean[1], cz = s_mean[2];
       if (__float_as_uint(g.px[v]) != __float_as_uint(cx) || __float_as_uint(g.py[v]) != __float_as_uint(cy) ||
@@ -558,7 +560,7 @@ __global__ __launch_bounds__(GR_THREADS) void kg_face_normals(GView g, int batch
       g.nz[v] = s_mean[5];
       if (g.mask) g.mask[v] = s_mean[6];
     }
-    for (int c = threadIdx.x; c < n; c += GR_THREADS) {
+    for (int c = threadIdx.x; c < n; c += FN_THREADS) {
       const int e = g.face_edge[g0 + c], vv = g.face_vert[g0 + c];
       if (atomicExch(&g.edge_stamp[e], serial) != serial) {
         const int slot = (int)atomicAdd((unsigned long long *)&g.ctr[SGG_NITEMS], 1ull);
@@ -1031,7 +1033,7 @@ static int enqueue_normals(PBVH *p, const GView &g, const int *leaves, int n) {
   const SgGridDev &d = *p->gd;
   if (leaves) kg_mark_faces<<<std::max(1, std::min(n, 1024)), GR_THREADS, 0, p->stream>>>(g, leaves, n);
   const int batch = std::max(1, (32 * 1024) / (d.ga * 12));  // grids staged together: up to 32 KiB of LDS
-  kg_face_normals<<<std::max(1, std::min(d.F, 512)), GR_THREADS, (size_t)batch * d.ga * 12, p->stream>>>(g, batch);
+  kg_face_normals<<<std::max(1, std::min(d.F, 512)), FN_THREADS, (size_t)batch * d.ga * 12, p->stream>>>(g, batch);
   kg_edges_verts<<<std::max(1, std::min((d.Ed + d.Vc + 3) / 4, 256)), GR_THREADS, 0, p->stream>>>(g);
   return launch_check("grid normals");
 }
