@@ -40,7 +40,7 @@ HBM_PEAK_GBPS = 8000.0
 # two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) summed over the dab kernels,
 # FETCH_SIZE scaled by 2.00 and WRITE_SIZE by 1.00 (what they read for a 1 GiB device
 # copy); profiles/dab_step_c2.md holds the commands and the per-kernel figures.
-TRAFFIC_BYTES_PER_DAB = 12_809_687
+TRAFFIC_BYTES_PER_DAB = 11_878_502
 METRIC = "vertex-updates/s + HBM GB/s; learned vs plain Smooth dabs/s at ≤1e-3 RMS"
 NO_PERSISTENT = False  # set from --no-persistent
 
