@@ -251,3 +251,23 @@ def test_open_disk_boundary_and_high_valence(sg, po):
     p.close()
     m.close()
     o.close()
+
+
+def test_whole_mesh_dabs_refit_every_ancestor(sg, po):
+    """Dabs covering the whole mesh with topology off: every leaf refits, so the bounds
+    flush lists every internal node (more than it stages in LDS) and runs over HBM;
+    every node box bit-identical to the oracle."""
+    co, tris = sg.noisy_icosphere(6, seed=31, amplitude=0.02)
+    m, p, o = pair(sg, po, co, tris, 0.02)
+    p.topology_set(0)
+    o.topology_set(0)
+    info, _, _ = p.nodes()
+    assert (info[:, 0] & 1).sum() > 600  # leaves; their ancestors number about as many
+    tools = [sg.TOOL_DRAW, sg.TOOL_INFLATE, sg.TOOL_SMOOTH, sg.TOOL_GRAB]
+    dabs = [sg.make_dab(tool=tools[i], location=np.array([0.1, 0.2, 0.3], np.float32), radius=1.6,
+                        strength=0.3, smooth_iterations=1, grab_delta=np.array([0.002, 0.0, -0.001], np.float32))
+            for i in range(4)]
+    run(sg, po, p, o, dabs, 2)
+    p.close()
+    m.close()
+    o.close()
