@@ -50,7 +50,7 @@ struct DV {
   int *node_parent, *node_height, *anc_stamp, *anc_pos, *lvl_list, *lvl_cnt;
   float *node_bb, *node_obb;
   int *slab, *scount;
-  int *wlist, *cand, *win, *outer, *hv_in, *hv_out, *touched, *gathered, *vlist, *aff, *nlist;
+  int *wlist, *cand, *win, *outer, *hv_in, *hv_out, *touched, *gathered, *vlist, *aff, *nlist, *nlist2;
   int *llist, *lstamp, *seed0, *seed1, *sstamp;
   uint8_t *fin;
   unsigned long long *key;
@@ -1255,20 +1255,27 @@ __global__ __launch_bounds__(DY_THREADS) void kd_smooth_store(DV g, const float 
 // ---------------------------------------------------------------------------------
 // normals and bounds of flagged leaves
 // ---------------------------------------------------------------------------------
-__global__ __launch_bounds__(DY_THREADS) void kd_flagged(DV g, int mask) {
+// both lists in one pass: leaves with a bit of mask_n (normals: nlist) and of mask_b
+// (bounds: nlist2); the normals step clears only UpdateNormals, so the bounds list is
+// the one a pass after it would make
+__global__ __launch_bounds__(DY_THREADS) void kd_flagged(DV g, int mask_n, int mask_b) {
   const int l = blockIdx.x * blockDim.x + threadIdx.x;
-  bool pass = false;
-  int node = 0;
+  int fl = 0, node = 0;
   if (l < g.L) {
     node = g.leaf_node[l];
-    pass = (g.node_flag[node] & mask) != 0;
+    fl = g.node_flag[node];
   }
-  const unsigned long long m = __ballot(pass);
-  if (m == 0) return;
-  int base = 0;
-  if (lane_id() == 0) base = (int)atomicAdd((unsigned long long *)&g.ctl[SGD_NLIST], (unsigned long long)__popcll(m));
-  base = __shfl(base, 0);
-  if (pass) g.nlist[base + mask_prefix(m)] = node;
+  const bool pn = l < g.L && (fl & mask_n) != 0, pb = l < g.L && (fl & mask_b) != 0;
+  const unsigned long long mn = __ballot(pn), mb = __ballot(pb);
+  int bn = 0, bb = 0;
+  if (lane_id() == 0) {
+    if (mn) bn = (int)atomicAdd((unsigned long long *)&g.ctl[SGD_NLIST], (unsigned long long)__popcll(mn));
+    if (mb) bb = (int)atomicAdd((unsigned long long *)&g.ctl[SGD_NLIST2], (unsigned long long)__popcll(mb));
+  }
+  bn = __shfl(bn, 0);
+  bb = __shfl(bb, 0);
+  if (pn) g.nlist[bn + mask_prefix(mn)] = node;
+  if (pb) g.nlist2[bb + mask_prefix(mb)] = node;
 }
 
 // face normals (normal_tri_v3) of the listed leaves' faces; their vertices, each once
@@ -1338,9 +1345,9 @@ __global__ __launch_bounds__(DY_THREADS) void kd_vert_normals(DV g) {
 // leaf boxes over the leaf faces' vertices (zeros as +0): a workgroup per listed leaf
 __global__ __launch_bounds__(DY_THREADS) void kd_leaf_bounds(DV g, int stamp) {
   __shared__ float s_mn[3][DY_THREADS / 64], s_mx[3][DY_THREADS / 64];
-  const int nl = (int)g.ctl[SGD_NLIST];
+  const int nl = (int)g.ctl[SGD_NLIST2];
   for (int li = blockIdx.x; li < nl; li += gridDim.x) {
-    const int node = g.nlist[li];
+    const int node = g.nlist2[li];
     const int fl = g.node_flag[node];
     if (fl & PBVH_UpdateBB) {
       float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
@@ -1531,8 +1538,8 @@ __global__ __launch_bounds__(1024) void kd_flush(DV g, int H, int stamp) {
     }
   }
   // clear the marks of the refit leaves and the level counts
-  const int nl = (int)g.ctl[SGD_NLIST];
-  for (int i = tid; i < nl; i += nt) g.node_dirty[g.nlist[i]] = 0;
+  const int nl = (int)g.ctl[SGD_NLIST2];
+  for (int i = tid; i < nl; i += nt) g.node_dirty[g.nlist2[i]] = 0;
   __syncthreads();
   for (int h = tid; h <= H; h += nt) g.lvl_cnt[h] = 0;
 }
@@ -1631,6 +1638,7 @@ DV view(PBVH *p) {
   g.vlist = d.vlist;
   g.aff = d.aff;
   g.nlist = d.nlist;
+  g.nlist2 = d.nlist2;
   g.llist = d.llist;
   g.lstamp = d.lstamp;
   g.seed0 = d.seed0;
@@ -1680,7 +1688,7 @@ void sg_dyn_free_device(PBVH *p) {
                 &d.wstamp, &d.estamp, &d.remap, &d.remap_ser, &d.node_flag, &d.node_child, &d.node_dirty,
                 &d.leaf_node, &d.lvl_off, &d.lvl_node, &d.node_parent, &d.node_height, &d.anc_stamp, &d.anc_pos, &d.lvl_list,
                 &d.lvl_cnt, &d.slab, &d.scount, &d.wlist, &d.cand, &d.win, &d.outer,
-                &d.hv0, &d.hv1, &d.touched, &d.gathered, &d.vlist, &d.aff, &d.nlist, &d.llist, &d.lstamp,
+                &d.hv0, &d.hv1, &d.touched, &d.gathered, &d.vlist, &d.aff, &d.nlist, &d.nlist2, &d.llist, &d.lstamp,
                 &d.seed0, &d.seed1, &d.sstamp};
   for (int **q : il) dfree(*q);
   dfree(d.vhide);
@@ -1736,7 +1744,7 @@ int sg_dyn_upload(PBVH *p) {
     rc |= dalloc(d.slab, Nc * SG_DYN_SLAB) | dalloc(d.scount, Nc) | dalloc(d.gathered, Nc) | dalloc(d.nlist, Nc);
     rc |= dalloc(d.node_parent, Nc) | dalloc(d.node_height, Nc) | dalloc(d.anc_stamp, Nc) | dalloc(d.lvl_list, Nc);
     rc |= dalloc(d.anc_pos, Nc);
-    rc |= dalloc(d.lvl_cnt, Nc + 1);
+    rc |= dalloc(d.lvl_cnt, Nc + 1) | dalloc(d.nlist2, Nc);
     if (rc) return -1;
     d.Ncap = (int)Nc;
   }
@@ -1960,15 +1968,14 @@ int normals_bounds_launch(PBVH *p, bool normals, bool bounds) {
   hipStream_t st = p->stream;
   DV g = view(p);
   const int lb = std::max(1, (d.L + DY_THREADS - 1) / DY_THREADS);
+  zero_ctl(p, g, (1u << SGD_NVL) | slots(SGD_NLIST, SGD_NFN + 1) | (1u << SGD_NLIST2) | (1u << SGD_NREFIT));
+  kd_flagged<<<lb, DY_THREADS, 0, st>>>(g, normals ? PBVH_UpdateNormals : 0,
+                                        bounds ? (PBVH_UpdateBB | PBVH_UpdateOriginalBB | PBVH_UpdateRedraw) : 0);
   if (normals) {
-    zero_ctl(p, g, (1u << SGD_NVL) | slots(SGD_NLIST, SGD_NFN + 1));
-    kd_flagged<<<lb, DY_THREADS, 0, st>>>(g, PBVH_UpdateNormals);
     kd_face_normals<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, ++s->stamp);
     kd_vert_normals<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g);
   }
   if (bounds) {
-    zero_ctl(p, g, (1u << SGD_NLIST) | (1u << SGD_NREFIT));
-    kd_flagged<<<lb, DY_THREADS, 0, st>>>(g, PBVH_UpdateBB | PBVH_UpdateOriginalBB | PBVH_UpdateRedraw);
     const int bstamp = ++s->stamp;
     kd_leaf_bounds<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, bstamp);
     kd_flush<<<1, 1024, 0, st>>>(g, d.H, bstamp);

@@ -52,6 +52,7 @@ enum {
   SGD_LN = 20,      // faces of the working set in range (the round's scan list)
   SGD_NS = 21,      // seed faces this round (in range, with an edge past the phase's limit)
   SGD_NS2 = 22,     // ... for the next round
+  SGD_NLIST2 = 23,  // leaves flagged for the bounds (the normals' list is SGD_NLIST)
   SGD_ST = 24,      // topology counters: splits, collapses, split rounds, collapse rounds,
                     // split candidates, collapse candidates, leaves in the working set
   SGD_SERIAL = 31,  // topology updates so far (labels of new vertices hash it)
@@ -86,7 +87,7 @@ struct SgDynDev {
   int *llist = nullptr, *lstamp = nullptr;  // in-range faces of the working set, membership stamp
   int *seed0 = nullptr, *seed1 = nullptr, *sstamp = nullptr;  // seed faces of a round, the next round's, stamp
   uint8_t *fin = nullptr;                    // per face: in range (kept exact through the rounds)
-  int *touched = nullptr, *gathered = nullptr, *vlist = nullptr, *aff = nullptr, *nlist = nullptr;
+  int *touched = nullptr, *gathered = nullptr, *vlist = nullptr, *aff = nullptr, *nlist = nullptr, *nlist2 = nullptr;
   unsigned long long *key = nullptr;
   uint8_t *valid = nullptr;
   float *fade = nullptr, *buf0 = nullptr, *buf1 = nullptr;
