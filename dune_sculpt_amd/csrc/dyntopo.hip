@@ -30,6 +30,7 @@
 
 #define DY_THREADS 256
 #define DY_BLOCKS 160
+#define LB_THREADS 1024  // a leaf's face slots (hundreds between after_stroke calls) in one pass
 
 namespace {
 
@@ -1279,7 +1280,7 @@ __global__ __launch_bounds__(DY_THREADS) void kd_flagged(DV g, int mask_n, int m
 }
 
 // face normals (normal_tri_v3) of the listed leaves' faces; their vertices, each once
-__global__ __launch_bounds__(DY_THREADS) void kd_face_normals(DV g, int stamp) {
+__global__ __launch_bounds__(LB_THREADS) void kd_face_normals(DV g, int stamp) {
   const int nl = (int)g.ctl[SGD_NLIST];
   for (int li = blockIdx.x; li < nl; li += gridDim.x) {
     const int node = g.nlist[li];
@@ -1343,8 +1344,8 @@ __global__ __launch_bounds__(DY_THREADS) void kd_vert_normals(DV g) {
 }
 
 // leaf boxes over the leaf faces' vertices (zeros as +0): a workgroup per listed leaf
-__global__ __launch_bounds__(DY_THREADS) void kd_leaf_bounds(DV g, int stamp) {
-  __shared__ float s_mn[3][DY_THREADS / 64], s_mx[3][DY_THREADS / 64];
+__global__ __launch_bounds__(LB_THREADS) void kd_leaf_bounds(DV g, int stamp) {
+  __shared__ float s_mn[3][LB_THREADS / 64], s_mx[3][LB_THREADS / 64];
   const int nl = (int)g.ctl[SGD_NLIST2];
   for (int li = blockIdx.x; li < nl; li += gridDim.x) {
     const int node = g.nlist2[li];
@@ -1381,7 +1382,7 @@ __global__ __launch_bounds__(DY_THREADS) void kd_leaf_bounds(DV g, int stamp) {
         float b[8];
         for (int a = 0; a < 3; a++) {
           float lo = s_mn[a][0], hi = s_mx[a][0];
-          for (int j = 1; j < DY_THREADS / 64; j++) {
+          for (int j = 1; j < LB_THREADS / 64; j++) {
             lo = sg_min(lo, s_mn[a][j]);
             hi = sg_max(hi, s_mx[a][j]);
           }
@@ -1972,12 +1973,12 @@ int normals_bounds_launch(PBVH *p, bool normals, bool bounds) {
   kd_flagged<<<lb, DY_THREADS, 0, st>>>(g, normals ? PBVH_UpdateNormals : 0,
                                         bounds ? (PBVH_UpdateBB | PBVH_UpdateOriginalBB | PBVH_UpdateRedraw) : 0);
   if (normals) {
-    kd_face_normals<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, ++s->stamp);
+    kd_face_normals<<<DY_BLOCKS, LB_THREADS, 0, st>>>(g, ++s->stamp);
     kd_vert_normals<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g);
   }
   if (bounds) {
     const int bstamp = ++s->stamp;
-    kd_leaf_bounds<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, bstamp);
+    kd_leaf_bounds<<<DY_BLOCKS, LB_THREADS, 0, st>>>(g, bstamp);
     kd_flush<<<1, 1024, 0, st>>>(g, d.H, bstamp);
   }
   return launch_check("dyntopo normals/bounds");
