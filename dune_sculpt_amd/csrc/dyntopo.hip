@@ -48,7 +48,7 @@ struct DV {
   unsigned long long *claim;
   int *estamp, *remap, *remap_ser;
   int *node_flag, *node_child, *node_dirty, *leaf_node, *lvl_off, *lvl_node;
-  int *node_parent, *node_height, *anc_stamp, *anc_pos, *lvl_list, *lvl_cnt;
+  int *node_parent, *node_height, *anc_stamp, *anc_pos, *anc, *lvl_list, *lvl_cnt;
   float *node_bb, *node_obb;
   int *slab, *scount;
   int *wlist, *cand, *win, *outer, *hv_in, *hv_out, *touched, *gathered, *vlist, *aff, *nlist, *nlist2;
@@ -1395,22 +1395,36 @@ __global__ __launch_bounds__(LB_THREADS) void kd_leaf_bounds(DV g, int stamp) {
       }
       __syncthreads();
     }
+    const int dirty = fl & (PBVH_UpdateBB | PBVH_UpdateOriginalBB);
     if (threadIdx.x == 0) {
       if (fl & PBVH_UpdateOriginalBB)
         for (int j = 0; j < 8; j++) g.node_obb[8 * (size_t)node + j] = g.node_bb[8 * (size_t)node + j];
-      const int dirty = fl & (PBVH_UpdateBB | PBVH_UpdateOriginalBB);
       g.node_dirty[node] = dirty;
       atomicAnd(&g.node_flag[node], ~(PBVH_UpdateBB | PBVH_UpdateOriginalBB | PBVH_UpdateRedraw));
-      // the ancestors the flush visits, each listed once by height (a walk stops where
-      // another leaf's walk has passed)
-      if (dirty)
-        for (int a = g.node_parent[node]; a >= 0; a = g.node_parent[a]) {
-          if (atomicExch(&g.anc_stamp[a], stamp) == stamp) break;
-          const int h = g.node_height[a];
-          const int s = atomicAdd(&g.lvl_cnt[h], 1);
-          g.lvl_list[g.lvl_off[h - 1] + s] = a;
-          g.anc_pos[a] = s;
-        }
+    }
+    // the ancestors the flush visits, each listed once by height: lane i of the first
+    // wave takes the i-th ancestor from the table (the stamp lists each once, whichever
+    // leaf's lane gets there first); a deeper chain goes on from the table's last entry
+    if (dirty && threadIdx.x < SG_WAVE) {
+      const int lane = threadIdx.x;
+      auto list = [&](int a) {
+        if (atomicExch(&g.anc_stamp[a], stamp) == stamp) return false;
+        const int h = g.node_height[a];
+        const int s = atomicAdd(&g.lvl_cnt[h], 1);
+        g.lvl_list[g.lvl_off[h - 1] + s] = a;
+        g.anc_pos[a] = s;
+        return true;
+      };
+      for (int j = lane; j < SG_DYN_ANC; j += SG_WAVE) {
+        const int a = g.anc[(size_t)node * SG_DYN_ANC + j];
+        if (a >= 0) list(a);
+      }
+      if (lane == 0) {
+        const int last = g.anc[(size_t)node * SG_DYN_ANC + SG_DYN_ANC - 1];
+        if (last >= 0)
+          for (int a = g.node_parent[last]; a >= 0; a = g.node_parent[a])
+            if (!list(a)) break;
+      }
     }
   }
 }
@@ -1622,6 +1636,7 @@ DV view(PBVH *p) {
   g.node_height = d.node_height;
   g.anc_stamp = d.anc_stamp;
   g.anc_pos = d.anc_pos;
+  g.anc = d.anc;
   g.lvl_list = d.lvl_list;
   g.lvl_cnt = d.lvl_cnt;
   g.node_bb = d.node_bb;
@@ -1687,7 +1702,7 @@ void sg_dyn_free_device(PBVH *p) {
   for (float **q : fl) dfree(*q);
   int **il[] = {&d.vnode, &d.vcorner, &d.vstamp, &d.astamp, &d.aslot, &d.hvst, &d.tv, &d.topp, &d.tnode, &d.tslot,
                 &d.wstamp, &d.estamp, &d.remap, &d.remap_ser, &d.node_flag, &d.node_child, &d.node_dirty,
-                &d.leaf_node, &d.lvl_off, &d.lvl_node, &d.node_parent, &d.node_height, &d.anc_stamp, &d.anc_pos, &d.lvl_list,
+                &d.leaf_node, &d.lvl_off, &d.lvl_node, &d.node_parent, &d.node_height, &d.anc_stamp, &d.anc_pos, &d.anc, &d.lvl_list,
                 &d.lvl_cnt, &d.slab, &d.scount, &d.wlist, &d.cand, &d.win, &d.outer,
                 &d.hv0, &d.hv1, &d.touched, &d.gathered, &d.vlist, &d.aff, &d.nlist, &d.nlist2, &d.llist, &d.lstamp,
                 &d.seed0, &d.seed1, &d.sstamp};
@@ -1744,7 +1759,7 @@ int sg_dyn_upload(PBVH *p) {
     rc |= dalloc(d.lvl_off, Nc + 1) | dalloc(d.lvl_node, Nc) | dalloc(d.node_bb, 8 * Nc) | dalloc(d.node_obb, 8 * Nc);
     rc |= dalloc(d.slab, Nc * SG_DYN_SLAB) | dalloc(d.scount, Nc) | dalloc(d.gathered, Nc) | dalloc(d.nlist, Nc);
     rc |= dalloc(d.node_parent, Nc) | dalloc(d.node_height, Nc) | dalloc(d.anc_stamp, Nc) | dalloc(d.lvl_list, Nc);
-    rc |= dalloc(d.anc_pos, Nc);
+    rc |= dalloc(d.anc_pos, Nc) | dalloc(d.anc, Nc * SG_DYN_ANC);
     rc |= dalloc(d.lvl_cnt, Nc + 1) | dalloc(d.nlist2, Nc);
     if (rc) return -1;
     d.Ncap = (int)Nc;
@@ -1823,6 +1838,14 @@ int sg_dyn_upload(PBVH *p) {
   for (int i = 0; i < N; i++)
     if (!(t.flag[i] & PBVH_Leaf)) parent[t.child[i]] = parent[t.child[i] + 1] = i;
   rc |= sg_check_hip(hipMemcpy(d.node_parent, parent.data(), N * 4, hipMemcpyHostToDevice), "copy");
+  {
+    std::vector<int> anc((size_t)N * SG_DYN_ANC, -1);
+    for (int i = 0; i < N; i++) {
+      int a = parent[i];
+      for (int j = 0; j < SG_DYN_ANC && a >= 0; j++, a = parent[a]) anc[(size_t)i * SG_DYN_ANC + j] = a;
+    }
+    rc |= sg_check_hip(hipMemcpy(d.anc, anc.data(), anc.size() * 4, hipMemcpyHostToDevice), "copy");
+  }
   rc |= sg_check_hip(hipMemcpy(d.node_height, height.data(), N * 4, hipMemcpyHostToDevice), "copy");
   rc |= sg_check_hip(hipMemset(d.lvl_cnt, 0, (H + 1) * 4), "memset");
   std::vector<long long> ctl(SGD_COUNT, 0);

@@ -15,6 +15,7 @@
 #define SG_DYN_MAX_ROUNDS 64  // rounds per phase (collapse, subdivide) of one topology update
 #define SG_DYN_RING 128       // faces around a vertex that a walk follows (more: the vertex is skipped)
 #define SG_DYN_STACK 32       // pending edges of the even-subdivision walk
+#define SG_DYN_ANC 64         // ancestors per node in the ancestor table (a wave's lanes)
 
 // The caller's mesh (stand-in for BMesh).  Host arrays until a PBVH is built over it;
 // then the PBVH's HBM copy is the mesh (owner set) and reads go to the device.
@@ -79,6 +80,7 @@ struct SgDynDev {
   int *node_flag = nullptr, *node_child = nullptr, *node_dirty = nullptr, *leaf_node = nullptr;
   int *lvl_off = nullptr, *lvl_node = nullptr;  // internal nodes grouped by height (children first)
   int *node_parent = nullptr, *node_height = nullptr, *anc_stamp = nullptr, *anc_pos = nullptr;
+  int *anc = nullptr;  // [N * SG_DYN_ANC]: a node's ancestors, parent first (-1 past the root)
   int *lvl_list = nullptr, *lvl_cnt = nullptr;  // ancestors of refit leaves, by height (lvl_off layout)
   float *node_bb = nullptr, *node_obb = nullptr;  // [8] per node: min xyz, pad, max xyz, pad
   int *slab = nullptr, *scount = nullptr;
