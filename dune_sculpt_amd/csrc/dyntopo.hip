@@ -1686,6 +1686,8 @@ unsigned slots(int b, int e) {
   return m;
 }
 void zero_ctl(PBVH *p, const DV &g, unsigned mask) { kd_ctl_zero<<<1, 64, 0, p->stream>>>(g, mask); }
+// the control words a topology update starts from zero (the range search's count aside)
+unsigned topology_slots() { return slots(SGD_NCAND, SGD_ST + 7) & ~(1u << SGD_NGATHER); }
 
 int launch_check(const char *what) { return sg_check_hip(hipGetLastError(), what); }
 
@@ -1928,7 +1930,8 @@ int sg_dyn_sync_flags_down(PBVH *p) {
 // ---------------------------------------------------------------------------------
 // The topology update's launches (no wait).  gate: run only if the range search that
 // precedes it on the stream gathered a leaf (a dab); the counters stay in HBM.
-int topology_launch(PBVH *p, int mode, const float center[3], float radius, int gate) {
+// clear_ctl: false when the caller already cleared the topology's control words
+int topology_launch(PBVH *p, int mode, const float center[3], float radius, int gate, bool clear_ctl = true) {
   SgDynState *s = p->dyn;
   SgDynDev &d = s->d;
   hipStream_t st = p->stream;
@@ -1946,7 +1949,7 @@ int topology_launch(PBVH *p, int mode, const float center[3], float radius, int 
   const int wser = ++s->stamp, lser = ++s->stamp;
   const int stamp0 = s->stamp;
   s->stamp += 2 * SG_DYN_MAX_ROUNDS + 2;  // the workgroup's round stamps
-  zero_ctl(p, g, slots(SGD_NCAND, SGD_ST + 7) & ~(1u << SGD_NGATHER));
+  if (clear_ctl) zero_ctl(p, g, topology_slots());
   kd_wbuild<<<std::max(1, std::min(d.L, 4096)), DY_THREADS, 0, st>>>(g, wser, gate);
   kd_wrange<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, R, lser, gate);
   // round tags stay below 2^21 (keys take bits 0..42): past that, claims restart from zero
@@ -2047,10 +2050,12 @@ int sg_dyn_dab_apply(PBVH *p, const SgDabParams *dab) {
   const bool topo = tool != SG_TOOL_GRAB && s->mode;
   if (topo) {
     const float rt = k.r * 1.25f;
-    zero_ctl(p, g, 1u << SGD_NGATHER);
+    // the range search's count and the topology's words in one clear (the search
+    // touches only its count)
+    zero_ctl(p, g, (1u << SGD_NGATHER) | topology_slots());
     kd_search<<<lb, DY_THREADS, 0, st>>>(g, k.cx, k.cy, k.cz, rt * rt, update | PBVH_UpdateTopology);
     const float c[3] = {k.cx, k.cy, k.cz};
-    if (launch_check("dyntopo search") || topology_launch(p, s->mode, c, k.r, 1)) return -1;
+    if (launch_check("dyntopo search") || topology_launch(p, s->mode, c, k.r, 1, false)) return -1;
   }
   zero_ctl(p, g, slots(SGD_NGATHER, SGD_NREFIT + 1));
   kd_search<<<lb, DY_THREADS, 0, st>>>(g, k.cx, k.cy, k.cz, k.r2, update);
