@@ -1087,7 +1087,7 @@ __global__ void kd_clear_topology_flags(DV g, int gate) {
 // the brush
 // ---------------------------------------------------------------------------------
 // the gathered leaves' own vertices, each once
-__global__ __launch_bounds__(DY_THREADS) void kd_unique(DV g, int stamp) {
+__global__ __launch_bounds__(LB_THREADS) void kd_unique(DV g, int stamp) {
   const int ng = (int)g.ctl[SGD_NGATHER];
   for (int gi = blockIdx.x; gi < ng; gi += gridDim.x) {
     const int node = g.gathered[gi];
@@ -1097,10 +1097,15 @@ __global__ __launch_bounds__(DY_THREADS) void kd_unique(DV g, int stamp) {
       if (f < 0) continue;
       for (int k = 0; k < 3; k++) {
         const int v = g.tv[3 * f + k];
-        if (g.vnode[v] == node && atomicExch(&g.vstamp[v], stamp) != stamp) {
-          const unsigned long long s = atomicAdd((unsigned long long *)&g.ctl[SGD_NVL], 1ull);
-          g.vlist[s] = v;
-        }
+        const bool add = g.vnode[v] == node && atomicExch(&g.vstamp[v], stamp) != stamp;
+        // one list reservation per wave (its first lane that adds)
+        const unsigned long long m = __ballot(add);
+        if (m == 0) continue;
+        const int leader = __ffsll((long long)m) - 1;
+        int base = 0;
+        if (lane_id() == leader) base = (int)atomicAdd((unsigned long long *)&g.ctl[SGD_NVL], (unsigned long long)__popcll(m));
+        base = __shfl(base, leader);
+        if (add) g.vlist[base + mask_prefix(m)] = v;
       }
     }
   }
@@ -2060,7 +2065,7 @@ int sg_dyn_dab_apply(PBVH *p, const SgDabParams *dab) {
   zero_ctl(p, g, slots(SGD_NGATHER, SGD_NREFIT + 1));
   kd_search<<<lb, DY_THREADS, 0, st>>>(g, k.cx, k.cy, k.cz, k.r2, update);
   const int stamp = ++s->stamp;
-  kd_unique<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, stamp);
+  kd_unique<<<DY_BLOCKS, LB_THREADS, 0, st>>>(g, stamp);
   if (tool == SG_TOOL_DRAW) kd_area<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, k);
   kd_brush<<<DY_BLOCKS, DY_THREADS, 0, st>>>(g, k, curve, stamp);
   if (tool == SG_TOOL_SMOOTH) {
